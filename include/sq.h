@@ -1,0 +1,185 @@
+/*
+ * sq.h -- C ABI of the MI355X (gfx950) batched BVH query engine.
+ *
+ * This is the drop-in boundary for the batched query path of
+ * grimandgreedy/spatial-query.  The reference binds a device backend at two
+ * places, and every entry point below replaces one of them:
+ *
+ *   - the device seam `GpuRaycaster` (crates/spatial-query/src/gpu/mod.rs:252-463):
+ *     new / headless / refit / raycast_nearest_batch, fed by Bvh::nodes() and
+ *     Bvh::prim_indices() (crates/spatial-query/src/accel/bvh.rs:465-475);
+ *   - the plugin trait `QueryBackend<D, G>` (crates/spatial-query/src/dispatch.rs:217-237),
+ *     whose raycast_nearest_batch answers one Option<Hit> per ray in order.
+ *
+ * Conventions
+ *   - Every function returns SQ_OK (0) or an SQ_ERR_* code; sq_last_error()
+ *     gives the message of the last failure on the calling thread.
+ *   - Rays are `float[n][2*D]`: origin then direction (Ray<D>, ray.rs:9-13).
+ *     Directions are used as given (Ray::new_unnormalized); callers that want
+ *     world-unit toi pass unit directions, as the reference does.
+ *   - Query input and output pointers are DEVICE pointers (hipMalloc'd or
+ *     torch tensors).  Queries are enqueued on the context's stream and return
+ *     without waiting; sq_ctx_synchronize() waits.  Calls on one context are
+ *     serialised by the caller; separate contexts are independent.
+ *   - A miss writes leaf -1 and toi +inf (the reference's `None`).
+ *   - Leaf ids are the provider's leaf indices 0..n-1 (QueryGeometry::id is
+ *     the leaf index for the built-in kinds).
+ *   - All arithmetic is FP32 with the reference's operation order; results are
+ *     bit-identical to the reference CPU path for finite inputs.
+ */
+#ifndef SQ_H
+#define SQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SQ_OK 0
+#define SQ_ERR_INVALID 1      /* bad argument                                   */
+#define SQ_ERR_NO_DEVICE 2    /* no usable GPU (the reference's `headless` None) */
+#define SQ_ERR_HIP 3          /* a HIP runtime call failed                      */
+#define SQ_ERR_UNSUPPORTED 4  /* dimension / kind combination not instantiated  */
+#define SQ_ERR_CAPACITY 5     /* variable-length output larger than capacity    */
+
+/* Built-in providers (the QueryGeometry narrow phases, geometry.rs:19-98).
+ * Primitive layouts (host or device, leaf-indexed):
+ *   SQ_PRIM_AABB     float[n][2*D]  min then max     toi = slab entry (aabb.rs:118-148)
+ *   SQ_PRIM_SPHERE   float[n][D+1]  centre, radius   differential.rs:61-82
+ *   SQ_PRIM_TRIANGLE float[n][9]    v0, v1, v2 (D=3) Moeller-Trumbore, viewport-lib-query/src/lib.rs:279-306
+ * SPHERE and TRIANGLE hits are kept only when their toi lies inside the slab
+ * interval of the primitive's own AABB, which makes every result independent
+ * of traversal order (and equal to the naive scan). */
+#define SQ_PRIM_AABB 0
+#define SQ_PRIM_SPHERE 1
+#define SQ_PRIM_TRIANGLE 2
+
+typedef struct sq_ctx sq_ctx;
+typedef struct sq_bvh sq_bvh;
+typedef struct sq_tlas sq_tlas;
+
+/* QueryFilter (crates/spatial-query/src/query/filter.rs:12-29).  The built-in
+ * kinds accept a leaf when (layers[leaf] & layer_mask) != 0 and leaf != exclude.
+ * exclude = -1 is `None`.  The default filter is {~0ull, -1}. */
+typedef struct sq_filter {
+  uint64_t layer_mask;
+  int64_t exclude;
+} sq_filter;
+
+/* TreeStats (crates/spatial-query/src/accel/stats.rs:15-32). */
+typedef struct sq_tree_stats {
+  uint64_t node_count, leaf_count, prim_count, max_depth, bytes;
+  float quality;
+} sq_tree_stats;
+
+const char* sq_last_error(void);
+int sq_version(void);
+
+/* ---- context: GpuRaycaster::headless / new (gpu/mod.rs:252-324) ---------- */
+int sq_ctx_create(int device, sq_ctx** out);   /* SQ_ERR_NO_DEVICE when absent */
+int sq_ctx_destroy(sq_ctx* ctx);
+int sq_ctx_synchronize(sq_ctx* ctx);
+/* The hipStream_t all work of this context is enqueued on. */
+void* sq_ctx_stream(sq_ctx* ctx);
+
+/* ---- host tree: Bvh (accel/bvh.rs), canonical, no device ------------------ */
+/* Bvh::build(&G) (bvh.rs:31-37, tree.rs:39-163) for a built-in provider: the
+ * reference median-split build on the host, same topology node for node.
+ * prims is a HOST pointer in the kind's layout.  Needs no GPU. */
+typedef struct sq_tree sq_tree;
+int sq_tree_build(int dim, int kind, const float* prims, uint64_t n_prims, sq_tree** out);
+/* A tree the caller already built in the reference layout (Bvh::nodes():
+ * bounds float[nn][2D], child_a, child_b, prim_count; Bvh::prim_indices()),
+ * validated and copied.  Host pointers. */
+int sq_tree_from_nodes(int dim, const float* node_bounds, const uint32_t* child_a, const uint32_t* child_b,
+                       const uint32_t* prim_count, uint64_t n_nodes, const uint32_t* prim_indices,
+                       uint64_t n_prims, sq_tree** out);
+/* Bvh::refit(&G) on the host (tree.rs:54-75): new primitives, same count. */
+int sq_tree_refit(sq_tree* tree, int kind, const float* prims);
+int sq_tree_get_stats(const sq_tree* tree, sq_tree_stats* out);   /* Bvh::stats, bvh.rs:478-489 */
+/* Bvh::nodes() / prim_indices() (bvh.rs:465-475).  Any pointer may be NULL. */
+int sq_tree_export(const sq_tree* tree, float* node_bounds, uint32_t* child_a, uint32_t* child_b,
+                   uint32_t* prim_count, uint32_t* prim_indices);
+int sq_tree_destroy(sq_tree* tree);
+
+/* ---- device tree: GpuRaycaster (gpu/mod.rs:252-341) ----------------------- */
+/* GpuRaycaster::new(device, queue, &Bvh, &G): re-pack the host tree into the
+ * device layout and upload it with the primitives (HOST pointer, kind layout).
+ * The host tree is copied; the caller keeps ownership of `tree`. */
+int sq_bvh_upload(sq_ctx* ctx, const sq_tree* tree, int kind, const float* prims, sq_bvh** out);
+/* sq_tree_build + sq_bvh_upload in one call. */
+int sq_bvh_build(sq_ctx* ctx, int dim, int kind, const float* prims, uint64_t n_prims, sq_bvh** out);
+int sq_bvh_destroy(sq_bvh* bvh);
+/* Per-leaf layer bits (host pointer, n_prims entries, or NULL for all-ones). */
+int sq_bvh_set_layers(sq_bvh* bvh, const uint64_t* layers);
+int sq_bvh_get_stats(const sq_bvh* bvh, sq_tree_stats* out);   /* Bvh::stats of the uploaded tree */
+/* Node bounds as currently held ON THE DEVICE, mapped back to reference node
+ * order (float[nn][2D], host pointer; synchronises).  Shows a device refit. */
+int sq_bvh_download_bounds(sq_bvh* bvh, float* node_bounds);
+/* GpuRaycaster::refit / Bvh::refit(&G) (gpu/mod.rs:326-341, tree.rs:54-75):
+ * new primitives in the kind's layout, DEVICE pointer, same leaf count;
+ * topology kept; bottom-up refit on the device, bit-identical to the host
+ * refit.  Asynchronous. */
+int sq_bvh_refit(sq_bvh* bvh, const float* prims_device);
+
+/* ---- ray queries (bvh.rs:51-328) ------------------------------------------ */
+/* max_toi_per_ray (device, n floats) overrides max_toi when not NULL.
+ * filter may be NULL (default filter). */
+
+/* Bvh::raycast_nearest over a batch (bvh.rs:51-124, :134-193).  out_leaf and
+ * out_toi are required; out_point, out_normal (float[n][D]) and out_sub
+ * (sub-object: the leaf for SPHERE/TRIANGLE, -1 for AABB) may be NULL. */
+int sq_raycast_nearest(sq_bvh* bvh, const float* rays, uint64_t n, float max_toi, const float* max_toi_per_ray,
+                       const sq_filter* filter, int32_t* out_leaf, float* out_toi, float* out_point,
+                       float* out_normal, int32_t* out_sub);
+/* Occlusion: out_hit[i] = raycast_nearest(..).is_some() (the reference's any-hit
+ * idiom, examples-viewport/examples/many_rays.rs:193-201), ending each ray at
+ * its first accepted hit. */
+int sq_raycast_any(sq_bvh* bvh, const float* rays, uint64_t n, float max_toi, const float* max_toi_per_ray,
+                   const sq_filter* filter, uint8_t* out_hit);
+/* Bvh::raycast_all (bvh.rs:196-259), or raycast_crossings (bvh.rs:264-328)
+ * when crossings != 0.  Results in CSR form: offsets (device, n+1, written),
+ * then out_leaf/out_toi (device, capacity entries), each ray's hits sorted by
+ * (toi total order, leaf) as hit_ordering (query/hit.rs:77-84).  *total (host)
+ * receives the hit count; if it exceeds capacity nothing is filled and
+ * SQ_ERR_CAPACITY is returned.  Synchronises once (to read the total). */
+int sq_raycast_all(sq_bvh* bvh, const float* rays, uint64_t n, float max_toi, const sq_filter* filter,
+                   int crossings, uint64_t* offsets, int32_t* out_leaf, float* out_toi, uint64_t capacity,
+                   uint64_t* total);
+/* Bvh::overlap (bvh.rs:418-456): regions float[n][2D] (min then max); each
+ * query's leaves in ascending order, CSR as above. */
+int sq_overlap(sq_bvh* bvh, const float* regions, uint64_t n, const sq_filter* filter, uint64_t* offsets,
+               int32_t* out_leaf, uint64_t capacity, uint64_t* total);
+/* ShapeCast queries (bvh.rs:331-414, query/shapecast.rs:26-73): casts are
+ * float[n][4D] = origin, dir, probe box min, probe box max (offsets from the
+ * reference point).  AABB leaves answer with the slab of the grown box;
+ * SPHERE leaves treat the probe as a ball of radius probe_max[0]; TRIANGLE
+ * leaves keep the reference default (no contact). */
+int sq_shapecast_nearest(sq_bvh* bvh, const float* casts, uint64_t n, float max_toi, const float* max_toi_per_cast,
+                         const sq_filter* filter, int32_t* out_leaf, float* out_toi, float* out_point);
+int sq_shapecast_all(sq_bvh* bvh, const float* casts, uint64_t n, float max_toi, const float* max_toi_per_cast,
+                     const sq_filter* filter, uint64_t* offsets, int32_t* out_leaf, float* out_toi,
+                     uint64_t capacity, uint64_t* total);
+
+/* Order-independent minimal work of each ray at limit t_limit[i] (the final
+ * nearest toi, or max_toi on a miss): out_nodes = node boxes that must be
+ * tested, out_prims = primitives that must be tested.  Feeds the algorithmic
+ * bytes of the roofline.  The reference's QueryStats (accel/stats.rs:39-50)
+ * counts the same events in its own traversal order. */
+int sq_required_work(sq_bvh* bvh, const float* rays, uint64_t n, const float* t_limit, uint32_t* out_nodes,
+                     uint32_t* out_prims);
+
+/* Kernel tuning knobs (block size in threads, a multiple of 64; 0 = default).
+ * The autotuner sets these; results never depend on them. */
+int sq_ctx_set_block(sq_ctx* ctx, int block_threads);
+
+/* Time (ms) of the last query kernel on this context, from HIP events recorded
+ * on the context stream around it (synchronises). */
+int sq_ctx_last_kernel_ms(sq_ctx* ctx, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SQ_H */

@@ -1,0 +1,367 @@
+"""ctypes binding of the CPU oracle (sq_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by `tests/`, by the smoke check and by the
+`cpu_baseline` leg of `bench.py`, always as the checker or the timed CPU
+baseline, never as a code path of the product.
+
+The oracle is a C++ restatement of the CPU path of grimandgreedy/spatial-query
+(see the header of sq_oracle.cpp for the file:line map).  Build it with
+`make -C oracle` (done by `__graft_entry__.build()`).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
+
+# Provider kinds (sq_oracle.cpp `enum Kind`).
+BOX, SPHERE, TRI = 0, 1, 2
+SPHERE_PLAIN, SPHERE_PLAIN_NOEXIT, TRI_PLAIN = 10, 11, 12
+INST_BOX, INST_MESH = 0, 1
+ALL_LAYERS = (1 << 64) - 1
+
+_lib = None
+
+
+def build(force: bool = False) -> None:
+    """Compile the oracle with its Makefile (g++ -O3 -ffp-contract=off)."""
+    if force and os.path.exists(_LIB_PATH):
+        os.remove(_LIB_PATH)
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+P = C.c_void_p
+U64, I64, F32, I32, U32 = C.c_uint64, C.c_int64, C.c_float, C.c_int, C.c_uint32
+
+
+def _declare(L):
+    sig = {
+        "orc_rng_range": (None, [U64, F32, F32, U64, P]),
+        "orc_rng_u64": (None, [U64, U64, P]),
+        "orc_normalize": (None, [I32, P, U64]),
+        "orc_geom_new": (P, [I32, I32, P, U32]),
+        "orc_geom_set_layers": (None, [P, P]),
+        "orc_geom_set_prims": (None, [P, P]),
+        "orc_geom_world_aabbs": (None, [P, P]),
+        "orc_geom_free": (None, [P]),
+        "orc_bvh_build": (P, [P]),
+        "orc_bvh_refit": (None, [P, P]),
+        "orc_bvh_node_count": (U64, [P]),
+        "orc_bvh_export": (None, [P, P, P, P, P, P]),
+        "orc_bvh_stats": (None, [P, P, P]),
+        "orc_bvh_free": (None, [P]),
+        "orc_raycast_nearest_batch": (None, [P, P, P, U64, F32, U64, I64, I32, I32, P, P, P, P, P, P]),
+        "orc_raycast_all_batch": (U64, [P, P, P, U64, F32, U64, I64, I32, I32, P, P, P, U64]),
+        "orc_overlap_batch": (U64, [P, P, P, U64, U64, I64, I32, P, P, U64]),
+        "orc_shapecast_nearest_batch": (None, [P, P, P, U64, P, U64, I64, I32, P, P, P, P]),
+        "orc_shapecast_all_batch": (U64, [P, P, P, U64, P, U64, I64, I32, P, P, P, U64]),
+        "orc_required_work": (None, [P, P, U64, P, P, P, I32]),
+        "orc_inst_new": (P, [I32, I32, U64, P, P, P, U32, P, P, P, P]),
+        "orc_inst_set_transforms": (None, [P, P, P, P]),
+        "orc_tlas_refit": (None, [P]),
+        "orc_tlas_rebuild": (None, [P]),
+        "orc_tlas_needs_rebuild": (I32, [P, F32]),
+        "orc_tlas_node_count": (U64, [P]),
+        "orc_tlas_export": (None, [P, P, P, P, P, P, P]),
+        "orc_tlas_raycast_nearest_batch": (None, [P, P, U64, F32, I32, I32, P, P, P, P, P]),
+        "orc_tlas_raycast_all_batch": (U64, [P, P, U64, F32, P, P, P, P, U64]),
+        "orc_tlas_overlap_batch": (U64, [P, P, U64, P, P, U64]),
+        "orc_inst_free": (None, [P]),
+        "orc_iso_apply": (None, [I32, P, P, P, I32, P]),
+        "orc_iso_transform_aabb": (None, [I32, P, P, P, P]),
+        "orc_ray_intersect": (I32, [I32, P, P, F32, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a, shape_last=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a
+
+
+# --------------------------------------------------------------------------
+# SplitMix64 as the reference tests draw it (sq/tests/conformance.rs:9-25)
+# --------------------------------------------------------------------------
+
+def rng_range(seed: int, lo: float, hi: float, n: int) -> np.ndarray:
+    out = np.empty(n, np.float32)
+    lib().orc_rng_range(seed, lo, hi, n, _p(out))
+    return out
+
+
+def rng_u64(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, np.uint64)
+    lib().orc_rng_u64(seed, n, _p(out))
+    return out
+
+
+def normalize(dirs: np.ndarray) -> np.ndarray:
+    """Ray::new normalisation (ray.rs:18-23) of [n, D] directions."""
+    d = np.array(dirs, dtype=np.float32, copy=True, order="C")
+    lib().orc_normalize(d.shape[-1], _p(d), d.shape[0])
+    return d
+
+
+# --------------------------------------------------------------------------
+# Single-level scene
+# --------------------------------------------------------------------------
+
+class Scene:
+    """A built-in provider plus its reference-built Bvh (bvh.rs:31-37)."""
+
+    def __init__(self, dim: int, kind: int, prims: np.ndarray, layers=None, build_tree: bool = True):
+        self.dim, self.kind = dim, kind
+        self.prims = np.ascontiguousarray(prims, dtype=np.float32)
+        self.n = self.prims.shape[0]
+        self._g = lib().orc_geom_new(dim, kind, _p(self.prims), self.n)
+        self.layers = None
+        if layers is not None:
+            self.set_layers(layers)
+        self._b = lib().orc_bvh_build(self._g) if build_tree else None
+
+    def set_layers(self, layers):
+        self.layers = None if layers is None else np.ascontiguousarray(layers, dtype=np.uint64)
+        lib().orc_geom_set_layers(self._g, _p(self.layers))
+
+    def set_prims(self, prims):
+        self.prims = np.ascontiguousarray(prims, dtype=np.float32)
+        lib().orc_geom_set_prims(self._g, _p(self.prims))
+
+    def refit(self):
+        lib().orc_bvh_refit(self._b, self._g)
+
+    def rebuild(self):
+        if self._b:
+            lib().orc_bvh_free(self._b)
+        self._b = lib().orc_bvh_build(self._g)
+
+    def world_aabbs(self):
+        out = np.empty((self.n, 2 * self.dim), np.float32)
+        lib().orc_geom_world_aabbs(self._g, _p(out))
+        return out
+
+    def export(self):
+        """(bounds[nn,2D], child_a, child_b, prim_count, prim_indices) as bvh.rs stores them."""
+        nn = lib().orc_bvh_node_count(self._b)
+        bounds = np.empty((nn, 2 * self.dim), np.float32)
+        ca = np.empty(nn, np.uint32); cb = np.empty(nn, np.uint32); pc = np.empty(nn, np.uint32)
+        idx = np.empty(self.n, np.uint32)
+        lib().orc_bvh_export(self._b, _p(bounds), _p(ca), _p(cb), _p(pc), _p(idx))
+        return bounds, ca, cb, pc, idx
+
+    def tree_stats(self):
+        out = np.zeros(5, np.uint64)
+        q = np.zeros(1, np.float32)
+        lib().orc_bvh_stats(self._b, _p(out), _p(q))
+        return dict(node_count=int(out[0]), leaf_count=int(out[1]), prim_count=int(out[2]),
+                    max_depth=int(out[3]), bytes=int(out[4]), quality=float(q[0]))
+
+    def raycast_nearest(self, rays, max_toi, mask=ALL_LAYERS, exclude=-1, threads=1, naive=False, stats=False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 2 * self.dim)
+        n = rays.shape[0]
+        leaf = np.empty(n, np.int64); t = np.empty(n, np.float32)
+        point = np.empty((n, self.dim), np.float32); normal = np.empty((n, self.dim), np.float32)
+        sub = np.empty(n, np.int64)
+        st = np.zeros((n, 4), np.uint64) if stats else None
+        lib().orc_raycast_nearest_batch(self._b, self._g, _p(rays), n, max_toi, mask, exclude, threads,
+                                        1 if naive else 0, _p(leaf), _p(t), _p(point), _p(normal), _p(sub), _p(st))
+        out = dict(leaf=leaf, t=t, point=point, normal=normal, sub=sub)
+        if stats:
+            out["stats"] = st
+        return out
+
+    def raycast_all(self, rays, max_toi, mask=ALL_LAYERS, exclude=-1, naive=False, crossings=False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 2 * self.dim)
+        n = rays.shape[0]
+        off = np.empty(n + 1, np.uint64)
+        L = lib()
+        total = L.orc_raycast_all_batch(self._b, self._g, _p(rays), n, max_toi, mask, exclude, int(naive), int(crossings),
+                                        _p(off), None, None, 0)
+        leaf = np.empty(total, np.int64); t = np.empty(total, np.float32)
+        L.orc_raycast_all_batch(self._b, self._g, _p(rays), n, max_toi, mask, exclude, int(naive), int(crossings),
+                                _p(off), _p(leaf), _p(t), total)
+        return off, leaf, t
+
+    def overlap(self, regions, mask=ALL_LAYERS, exclude=-1, naive=False):
+        regions = np.ascontiguousarray(regions, dtype=np.float32).reshape(-1, 2 * self.dim)
+        n = regions.shape[0]
+        off = np.empty(n + 1, np.uint64)
+        L = lib()
+        total = L.orc_overlap_batch(self._b, self._g, _p(regions), n, mask, exclude, int(naive), _p(off), None, 0)
+        leaf = np.empty(total, np.int64)
+        L.orc_overlap_batch(self._b, self._g, _p(regions), n, mask, exclude, int(naive), _p(off), _p(leaf), total)
+        return off, leaf
+
+    def shapecast_nearest(self, casts, max_toi, mask=ALL_LAYERS, exclude=-1, naive=False):
+        casts = np.ascontiguousarray(casts, dtype=np.float32).reshape(-1, 4 * self.dim)
+        n = casts.shape[0]
+        mt = np.ascontiguousarray(np.broadcast_to(np.float32(max_toi) if np.isscalar(max_toi) else max_toi, (n,)), dtype=np.float32)
+        leaf = np.empty(n, np.int64); t = np.empty(n, np.float32)
+        point = np.empty((n, self.dim), np.float32); normal = np.empty((n, self.dim), np.float32)
+        lib().orc_shapecast_nearest_batch(self._b, self._g, _p(casts), n, _p(mt), mask, exclude, int(naive),
+                                          _p(leaf), _p(t), _p(point), _p(normal))
+        return dict(leaf=leaf, t=t, point=point, normal=normal)
+
+    def shapecast_all(self, casts, max_toi, mask=ALL_LAYERS, exclude=-1, naive=False):
+        casts = np.ascontiguousarray(casts, dtype=np.float32).reshape(-1, 4 * self.dim)
+        n = casts.shape[0]
+        mt = np.ascontiguousarray(np.broadcast_to(np.float32(max_toi) if np.isscalar(max_toi) else max_toi, (n,)), dtype=np.float32)
+        off = np.empty(n + 1, np.uint64)
+        L = lib()
+        total = L.orc_shapecast_all_batch(self._b, self._g, _p(casts), n, _p(mt), mask, exclude, int(naive), _p(off), None, None, 0)
+        leaf = np.empty(total, np.int64); t = np.empty(total, np.float32)
+        L.orc_shapecast_all_batch(self._b, self._g, _p(casts), n, _p(mt), mask, exclude, int(naive), _p(off), _p(leaf), _p(t), total)
+        return off, leaf, t
+
+    def required_work(self, rays, t_limit, threads=1):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 2 * self.dim)
+        n = rays.shape[0]
+        tl = np.ascontiguousarray(t_limit, dtype=np.float32)
+        nodes = np.empty(n, np.uint64); prims = np.empty(n, np.uint64)
+        lib().orc_required_work(self._b, _p(rays), n, _p(tl), _p(nodes), _p(prims), threads)
+        return nodes, prims
+
+    def __del__(self):
+        try:
+            L = lib()
+            if getattr(self, "_b", None):
+                L.orc_bvh_free(self._b)
+            if getattr(self, "_g", None):
+                L.orc_geom_free(self._g)
+        except Exception:
+            pass
+
+
+# --------------------------------------------------------------------------
+# Two-level scene (tlas.rs)
+# --------------------------------------------------------------------------
+
+class InstanceScene:
+    def __init__(self, dim, kind, rot, trans, halves=None, mesh_tris=None, mesh_of=None, scale=None):
+        self.dim, self.kind = dim, kind
+        self.rot = np.ascontiguousarray(rot, dtype=np.float32).reshape(-1, dim, dim)
+        self.trans = np.ascontiguousarray(trans, dtype=np.float32).reshape(-1, dim)
+        self.n = self.rot.shape[0]
+        self.halves = None if halves is None else np.ascontiguousarray(halves, dtype=np.float32).reshape(-1, dim)
+        if kind == INST_MESH:
+            off = np.zeros(len(mesh_tris) + 1, np.uint64)
+            for i, m in enumerate(mesh_tris):
+                off[i + 1] = off[i] + len(m)
+            self.mesh_off = off
+            self.tris = np.ascontiguousarray(np.concatenate([np.asarray(m, np.float32).reshape(-1, 9) for m in mesh_tris]), dtype=np.float32)
+            self.mesh_of = np.ascontiguousarray(mesh_of, dtype=np.uint32)
+            self.scale = np.ascontiguousarray(scale, dtype=np.float32)
+            nm = len(mesh_tris)
+        else:
+            self.mesh_off = self.tris = self.mesh_of = self.scale = None
+            nm = 0
+        self._h = lib().orc_inst_new(dim, kind, self.n, _p(self.rot), _p(self.trans), _p(self.halves), nm,
+                                     _p(self.mesh_off), _p(self.tris), _p(self.mesh_of), _p(self.scale))
+
+    def set_transforms(self, rot, trans, halves=None):
+        self.rot = np.ascontiguousarray(rot, dtype=np.float32).reshape(-1, self.dim, self.dim)
+        self.trans = np.ascontiguousarray(trans, dtype=np.float32).reshape(-1, self.dim)
+        if halves is not None:
+            self.halves = np.ascontiguousarray(halves, dtype=np.float32).reshape(-1, self.dim)
+        lib().orc_inst_set_transforms(self._h, _p(self.rot), _p(self.trans), _p(self.halves) if halves is not None else None)
+
+    def refit(self):
+        lib().orc_tlas_refit(self._h)
+
+    def rebuild(self):
+        lib().orc_tlas_rebuild(self._h)
+
+    def needs_rebuild(self, factor):
+        return bool(lib().orc_tlas_needs_rebuild(self._h, factor))
+
+    def export(self):
+        nn = lib().orc_tlas_node_count(self._h)
+        bounds = np.empty((nn, 2 * self.dim), np.float32)
+        ca = np.empty(nn, np.uint32); cb = np.empty(nn, np.uint32); pc = np.empty(nn, np.uint32)
+        idx = np.empty(self.n, np.uint32)
+        world = np.empty((self.n, 2 * self.dim), np.float32)
+        lib().orc_tlas_export(self._h, _p(bounds), _p(ca), _p(cb), _p(pc), _p(idx), _p(world))
+        return bounds, ca, cb, pc, idx, world
+
+    def raycast_nearest(self, rays, max_toi, naive=False, threads=1):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 2 * self.dim)
+        n = rays.shape[0]
+        leaf = np.empty(n, np.int64); t = np.empty(n, np.float32)
+        point = np.empty((n, self.dim), np.float32); normal = np.empty((n, self.dim), np.float32)
+        sub = np.empty(n, np.int64)
+        lib().orc_tlas_raycast_nearest_batch(self._h, _p(rays), n, max_toi, int(naive), threads,
+                                             _p(leaf), _p(t), _p(point), _p(normal), _p(sub))
+        return dict(leaf=leaf, t=t, point=point, normal=normal, sub=sub)
+
+    def raycast_all(self, rays, max_toi):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 2 * self.dim)
+        n = rays.shape[0]
+        off = np.empty(n + 1, np.uint64)
+        L = lib()
+        total = L.orc_tlas_raycast_all_batch(self._h, _p(rays), n, max_toi, _p(off), None, None, None, 0)
+        leaf = np.empty(total, np.int64); t = np.empty(total, np.float32); sub = np.empty(total, np.int64)
+        L.orc_tlas_raycast_all_batch(self._h, _p(rays), n, max_toi, _p(off), _p(leaf), _p(t), _p(sub), total)
+        return off, leaf, t, sub
+
+    def overlap(self, regions):
+        regions = np.ascontiguousarray(regions, dtype=np.float32).reshape(-1, 2 * self.dim)
+        n = regions.shape[0]
+        off = np.empty(n + 1, np.uint64)
+        L = lib()
+        total = L.orc_tlas_overlap_batch(self._h, _p(regions), n, _p(off), None, 0)
+        leaf = np.empty(total, np.int64)
+        L.orc_tlas_overlap_batch(self._h, _p(regions), n, _p(off), _p(leaf), total)
+        return off, leaf
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().orc_inst_free(self._h)
+        except Exception:
+            pass
+
+
+def iso_apply(rot, trans, p, op):
+    """op: 0 transform_point, 1 transform_vector, 2 inverse_transform_point, 3 inverse_transform_vector."""
+    rot = np.ascontiguousarray(rot, np.float32); trans = np.ascontiguousarray(trans, np.float32)
+    p = np.ascontiguousarray(p, np.float32)
+    out = np.empty_like(p)
+    lib().orc_iso_apply(p.shape[0], _p(rot), _p(trans), _p(p), op, _p(out))
+    return out
+
+
+def iso_transform_aabb(rot, trans, box):
+    rot = np.ascontiguousarray(rot, np.float32); trans = np.ascontiguousarray(trans, np.float32)
+    box = np.ascontiguousarray(box, np.float32)
+    out = np.empty_like(box)
+    lib().orc_iso_transform_aabb(box.shape[0] // 2, _p(rot), _p(trans), _p(box), _p(out))
+    return out
+
+
+def ray_intersect(box, ray, max_toi):
+    box = np.ascontiguousarray(box, np.float32); ray = np.ascontiguousarray(ray, np.float32)
+    a = np.zeros(1, np.float32); b = np.zeros(1, np.float32)
+    hit = lib().orc_ray_intersect(box.shape[0] // 2, _p(box), _p(ray), max_toi, _p(a), _p(b))
+    return (float(a[0]), float(b[0])) if hit else None

@@ -1,0 +1,981 @@
+// engine.hip -- HIP kernels (gfx950) and the C ABI of include/sq.h.
+//
+// One lane per query, 64-lane wavefronts, a per-lane traversal stack in LDS
+// laid out entry-major (entry k of lane l at k * blockDim + l) so a wave's
+// pushes and pops hit 64 consecutive banks.  The per-ray logic lives in
+// traverse.h; this file owns device memory, launches and the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sq.h"
+#include "host_tree.h"
+#include "traverse.h"
+
+using namespace sqd;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return fail(SQ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// objects
+// ---------------------------------------------------------------------------
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  int ensure(size_t b) {
+    if (b <= bytes) return SQ_OK;
+    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    if (b == 0) return SQ_OK;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) return fail(SQ_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    bytes = b;
+    return SQ_OK;
+  }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+struct sq_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int block = 256;
+  bool timed = false;
+  DevBuf counts, scratch, keys, keys2;   // reusable temporaries of the CSR queries
+};
+
+struct sq_bvh {
+  sq_ctx* ctx = nullptr;
+  int dim = 0, kind = 0;
+  uint64_t n_prims = 0;
+  sqh::RefTree tree;
+  sqh::Packed packed;
+  DevBuf nodes, prims, ids, root, layers, levels;
+  sq_tree_stats stats{};
+};
+
+static const char* kind_name(int k) { return k == 0 ? "AABB" : k == 1 ? "SPHERE" : k == 2 ? "TRIANGLE" : "?"; }
+
+// ---------------------------------------------------------------------------
+// dispatch over (D, KIND)
+// ---------------------------------------------------------------------------
+
+template <class F> static int dispatch_dk(int dim, int kind, F&& f) {
+  if (kind == KIND_TRI) {
+    if (dim != 3) return fail(SQ_ERR_UNSUPPORTED, "TRIANGLE primitives need dim 3");
+    return f.template operator()<3, KIND_TRI>();
+  }
+  if (kind != KIND_AABB && kind != KIND_SPHERE) return fail(SQ_ERR_INVALID, "unknown primitive kind");
+#define SQ_CASE(DD)                                                       \
+  case DD:                                                                \
+    return kind == KIND_AABB ? f.template operator()<DD, KIND_AABB>()    \
+                             : f.template operator()<DD, KIND_SPHERE>();
+  switch (dim) {
+    SQ_CASE(2) SQ_CASE(3) SQ_CASE(4) SQ_CASE(5) SQ_CASE(6)
+    default: return fail(SQ_ERR_UNSUPPORTED, "dimension " + std::to_string(dim) + " not instantiated (2..6)");
+  }
+#undef SQ_CASE
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+struct NearestOut {
+  int32_t* leaf;
+  float* toi;
+  float* point;
+  float* normal;
+  int32_t* sub;
+  uint8_t* any;
+};
+
+// Nearest / any-hit rays (RayPolicy) and nearest shape casts (CastPolicy).
+// Query i reads QS floats at q + i*QS.
+template <int D, int KIND, bool ANY, bool FILT, bool CAST>
+__global__ __launch_bounds__(1024) void k_nearest(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
+                                                  const float* __restrict__ tmax, Filter f, NearestOut out, uint32_t stack_n) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  const uint32_t i = blockIdx.x * B + tid;
+  if (i >= n) return;
+  using Pol = typename std::conditional<CAST, CastPolicy<D, KIND>, RayPolicy<D, KIND>>::type;
+  constexpr int QS = CAST ? 4 * D : 2 * D;
+  constexpr int PS = Prim<D, KIND>::STRIDE;
+  Pol pol;
+  pol.setup(q + (size_t)i * QS);
+  const float lim = tmax ? tmax[i] : max_toi;
+  Stack st{lds + tid, reinterpret_cast<float*>(lds + stack_n * B) + tid, B};
+  NearestResult h = walk_nearest<D, PS, ANY, FILT>(s, pol, lim, f, st);
+  if (ANY) { out.any[i] = h.leaf != NO_LEAF ? 1 : 0; return; }
+  const bool hit = h.leaf != NO_LEAF;
+  out.leaf[i] = hit ? (int32_t)h.leaf : -1;
+  out.toi[i] = h.t;
+  if (out.point) {
+    // Ray::at / ShapeCast::at: origin + dir * t
+#pragma unroll
+    for (int k = 0; k < D; ++k) out.point[(size_t)i * D + k] = hit ? pol.r.o[k] + pol.r.d[k] * h.t : 0.0f;
+  }
+  if (!CAST && out.normal) {
+    float nv[D];
+    if (hit) hit_normal<D, KIND>(pol.r, s.prims + (size_t)h.pos * PS, h.t, lim, nv);
+    else for (int k = 0; k < D; ++k) nv[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) out.normal[(size_t)i * D + k] = nv[k];
+  }
+  if (out.sub) out.sub[i] = (hit && KIND != KIND_AABB) ? (int32_t)h.leaf : -1;
+}
+
+template <int D>
+__global__ __launch_bounds__(1024) void k_required(SceneView s, const float* __restrict__ rays, uint32_t n, const float* __restrict__ tlim,
+                                                   uint32_t* out_nodes, uint32_t* out_prims, uint32_t stack_n) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  const uint32_t i = blockIdx.x * B + tid;
+  if (i >= n) return;
+  RayD<D> r;
+  ray_setup<D>(r, rays + (size_t)i * 2 * D);
+  Stack st{lds + tid, nullptr, B};
+  uint32_t nn, np;
+  walk_required<D>(s, r, tlim[i], st, nn, np);
+  out_nodes[i] = nn;
+  out_prims[i] = np;
+}
+
+// --- variable-length queries: count pass, then fill pass -------------------
+// MODE 0: raycast_all, 1: raycast_crossings, 2: overlap, 3: shapecast_all.
+// FILL = false counts hits per query into counts[i]; FILL = true writes sort
+// keys at keys[offsets[i] + j] (hit_key for rays and casts, the leaf for overlap).
+template <int D, int KIND, int MODE, bool FILL, bool FILT>
+__global__ __launch_bounds__(1024) void k_collect(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
+                                                  const float* __restrict__ tmax, Filter f, uint32_t* counts,
+                                                  const uint64_t* __restrict__ offsets, uint64_t* keys) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  const uint32_t i = blockIdx.x * B + tid;
+  if (i >= n) return;
+  constexpr int PS = Prim<D, KIND>::STRIDE;
+  Stack st{lds + tid, nullptr, B};
+  uint32_t cnt = 0;
+  uint64_t* dst = FILL ? keys + offsets[i] : nullptr;
+  auto emit = [&](uint64_t key) {
+    if (FILL) dst[cnt] = key;
+    ++cnt;
+  };
+  const uint32_t root_ref = fbits(s.root[2 * D]);
+  if constexpr (MODE == 2) {
+    const float* rg = q + (size_t)i * 2 * D;
+    float rmn[D], rmx[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) { rmn[k] = rg[k]; rmx[k] = rg[D + k]; }
+    if (box_overlap<D>(s.root, s.root + D, rmn, rmx)) {
+      walk_visit<D>(s, root_ref,
+        [&](const float* nd, bool& ha, bool& hb) {
+          float amn[D], amx[D], bmn[D], bmx[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) { amn[k] = nd[4 * k]; amx[k] = nd[4 * k + 1]; bmn[k] = nd[4 * k + 2]; bmx[k] = nd[4 * k + 3]; }
+          ha = box_overlap<D>(amn, amx, rmn, rmx);
+          hb = box_overlap<D>(bmn, bmx, rmn, rmx);
+        },
+        [&](uint32_t first, uint32_t c) {
+          for (uint32_t j = 0; j < c; ++j) {
+            const uint32_t pos = first + j;
+            if (!overlap_narrow<D, KIND>(s.prims + (size_t)pos * PS, rmn, rmx)) continue;
+            const uint32_t leaf = s.ids[pos];
+            if (FILT && !accepts(f, leaf)) continue;
+            emit(leaf);
+          }
+        }, st);
+    }
+  } else {
+    using Pol = typename std::conditional<MODE == 3, CastPolicy<D, KIND>, RayPolicy<D, KIND>>::type;
+    constexpr int QS = MODE == 3 ? 4 * D : 2 * D;
+    Pol pol;
+    pol.setup(q + (size_t)i * QS);
+    const float lim = tmax ? tmax[i] : max_toi;
+    if (pol.root(s.root, lim)) {
+      walk_visit<D>(s, root_ref,
+        [&](const float* nd, bool& ha, bool& hb) { float ta, tb; pol.pair(nd, lim, ha, hb, ta, tb); },
+        [&](uint32_t first, uint32_t c) {
+          for (uint32_t j = 0; j < c; ++j) {
+            const uint32_t pos = first + j;
+            const float* p = s.prims + (size_t)pos * PS;
+            float t;
+            if constexpr (MODE == 1 && KIND == KIND_AABB) {
+              // Box crossings: entry, and the real exit when it lies within max_toi.
+              float te, tx; int ax;
+              if (!slab_exact<D>(pol.r, p, p + D, lim, te, tx, ax)) continue;
+              const uint32_t leaf = s.ids[pos];
+              if (FILT && !accepts(f, leaf)) continue;
+              emit(hit_key(te, leaf));
+              if (tx > te) {
+                float te2, tx2;
+                slab_exact<D>(pol.r, p, p + D, SQ_INF, te2, tx2, ax);
+                if (tx2 <= lim) emit(hit_key(tx2, leaf));
+              }
+            } else {
+              if (!pol.narrow(p, lim, t)) continue;
+              const uint32_t leaf = s.ids[pos];
+              if (FILT && !accepts(f, leaf)) continue;
+              emit(hit_key(t, leaf));
+            }
+          }
+        }, st);
+    }
+  }
+  if (!FILL) counts[i] = cnt;
+}
+
+// Exclusive scan of n values (u32 or u64) into out[0..n] (out[n] = total).
+// Tiles of 4096 = 256 threads x 16; tile sums scanned recursively by the host.
+template <class T>
+__global__ __launch_bounds__(256) void k_scan_tile(const T* __restrict__ in, uint64_t n, uint64_t* out, uint64_t* tile_sums) {
+  __shared__ uint64_t sh[256];
+  const uint64_t base = (uint64_t)blockIdx.x * 4096 + threadIdx.x * 16;
+  uint64_t v[16], run = 0;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) { v[j] = (base + j < n) ? (uint64_t)in[base + j] : 0; run += v[j]; }
+  sh[threadIdx.x] = run;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    uint64_t add = threadIdx.x >= (unsigned)off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += add;
+    __syncthreads();
+  }
+  uint64_t acc = sh[threadIdx.x] - run;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    if (base + j <= n) out[base + j] = acc;
+    acc += v[j];
+  }
+  if (threadIdx.x == 255) tile_sums[blockIdx.x] = sh[255];
+}
+
+__global__ void k_scan_add(uint64_t* out, uint64_t n1, const uint64_t* __restrict__ tile_off) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n1) out[i] += tile_off[i / 4096];
+}
+
+// Segment sort, small segments: one lane per segment, insertion sort in place.
+// Segments longer than SMALL_SEG are appended to `large` for the block sort.
+constexpr uint32_t SMALL_SEG = 32;
+__global__ void k_sort_small(uint64_t* keys, const uint64_t* __restrict__ offsets, uint32_t n, uint32_t* large, uint32_t* n_large) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t a = offsets[i], b = offsets[i + 1];
+  const uint64_t len = b - a;
+  if (len <= 1) return;
+  if (len > SMALL_SEG) { large[atomicAdd(n_large, 1u)] = i; return; }
+  uint64_t* k = keys + a;
+  for (uint32_t x = 1; x < len; ++x) {
+    uint64_t v = k[x];
+    int y = (int)x - 1;
+    while (y >= 0 && k[y] > v) { k[y + 1] = k[y]; --y; }
+    k[y + 1] = v;
+  }
+}
+
+// Segment sort, large segments: one block per segment.  Chunks of 2048 keys
+// are bitonic-sorted in LDS, then runs are merged pairwise (each element's
+// output position = its rank in its own run + its rank in the other run),
+// ping-ponging between the segment and a scratch area of the same size.
+constexpr uint32_t CHUNK = 2048;
+__global__ __launch_bounds__(256) void k_sort_large(uint64_t* keys, uint64_t* scratch, const uint64_t* __restrict__ offsets,
+                                                    const uint32_t* __restrict__ large) {
+  __shared__ uint64_t sh[CHUNK];
+  const uint32_t seg = large[blockIdx.x];
+  const uint64_t a = offsets[seg], len = offsets[seg + 1] - a;
+  uint64_t* src = keys + a;
+  uint64_t* dst = scratch + a;
+  for (uint64_t c0 = 0; c0 < len; c0 += CHUNK) {
+    const uint32_t m = (uint32_t)min((uint64_t)CHUNK, len - c0);
+    for (uint32_t j = threadIdx.x; j < CHUNK; j += blockDim.x) sh[j] = j < m ? src[c0 + j] : ~0ull;
+    __syncthreads();
+    for (uint32_t size = 2; size <= CHUNK; size <<= 1) {
+      for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        for (uint32_t j = threadIdx.x; j < CHUNK / 2; j += blockDim.x) {
+          const uint32_t lo = 2 * j - (j & (stride - 1));
+          const uint32_t hi = lo + stride;
+          const bool up = ((lo & size) == 0);
+          uint64_t x = sh[lo], y = sh[hi];
+          if ((x > y) == up) { sh[lo] = y; sh[hi] = x; }
+        }
+        __syncthreads();
+      }
+    }
+    for (uint32_t j = threadIdx.x; j < m; j += blockDim.x) src[c0 + j] = sh[j];
+    __syncthreads();
+  }
+  // merge passes
+  for (uint64_t run = CHUNK; run < len; run <<= 1) {
+    for (uint64_t j = threadIdx.x; j < len; j += blockDim.x) {
+      const uint64_t r0 = (j / (2 * run)) * (2 * run);
+      const bool left = (j - r0) < run;
+      const uint64_t mine0 = left ? r0 : r0 + run;
+      const uint64_t oth0 = left ? r0 + run : r0;
+      const uint64_t oth_len = left ? (min(len, r0 + 2 * run) > r0 + run ? min(len, r0 + 2 * run) - (r0 + run) : 0) : run;
+      const uint64_t v = src[j];
+      // rank in the other run: left elements go before equal right elements
+      uint64_t lo = 0, hi = oth_len;
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        const uint64_t w = src[oth0 + mid];
+        if (left ? (w < v) : (w <= v)) lo = mid + 1; else hi = mid;
+      }
+      dst[r0 + (j - mine0) + lo] = v;
+    }
+    __syncthreads();
+    uint64_t* t = src; src = dst; dst = t;
+    __syncthreads();
+  }
+  if (src != keys + a) {
+    for (uint64_t j = threadIdx.x; j < len; j += blockDim.x) keys[a + j] = src[j];
+  }
+}
+
+// Sorted keys -> caller outputs.
+__global__ void k_unpack(const uint64_t* __restrict__ keys, uint64_t total, int32_t* leaf, float* toi, int has_t) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const uint64_t k = keys[i];
+  leaf[i] = (int32_t)(uint32_t)k;
+  if (has_t) toi[i] = key_toi(k);
+}
+
+__global__ void k_fill_miss(uint32_t n, int dim, NearestOut out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (out.any) { out.any[i] = 0; return; }
+  out.leaf[i] = -1;
+  out.toi[i] = SQ_INF;
+  for (int k = 0; k < dim; ++k) {
+    if (out.point) out.point[(size_t)i * dim + k] = 0.0f;
+    if (out.normal) out.normal[(size_t)i * dim + k] = 0.0f;
+  }
+  if (out.sub) out.sub[i] = -1;
+}
+
+// --- refit (tree.rs:54-75 on the device) -----------------------------------
+
+// Caller-layout primitives (leaf-indexed) -> leaf-ordered device records.
+template <int D, int KIND>
+__global__ void k_scatter_prims(const float* __restrict__ src, const uint32_t* __restrict__ ids, float* __restrict__ dst, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t id = ids[k];
+  if constexpr (KIND == KIND_TRI) {
+    const float* p = src + (size_t)id * 9;
+    float* q = dst + (size_t)k * 12;
+    q[0] = p[0]; q[1] = p[1]; q[2] = p[2]; q[3] = __uint_as_float(id);
+    q[4] = p[3]; q[5] = p[4]; q[6] = p[5]; q[7] = 0.0f;
+    q[8] = p[6]; q[9] = p[7]; q[10] = p[8]; q[11] = 0.0f;
+  } else {
+    constexpr int S = Prim<D, KIND>::STRIDE;
+    const float* p = src + (size_t)id * S;
+    float* q = dst + (size_t)k * S;
+#pragma unroll
+    for (int j = 0; j < S; ++j) q[j] = p[j];
+  }
+}
+
+// Box of a child ref: a leaf folds its primitives' boxes from EMPTY in
+// prim_indices order; a pair node is the union of its two stored boxes.
+template <int D, int KIND>
+__device__ __forceinline__ void child_box(const float* nodes, const float* prims, uint32_t ref, float* mn, float* mx) {
+  constexpr int NF = (D + 1) * 4;
+#pragma unroll
+  for (int k = 0; k < D; ++k) { mn[k] = SQ_INF; mx[k] = -SQ_INF; }
+  if (ref & LEAF_BIT) {
+    const uint32_t first = ref & FIRST_MASK, cnt = ((ref >> 27) & 15u) + 1u;
+    for (uint32_t j = 0; j < cnt; ++j) {
+      float a[D], b[D];
+      prim_box<D, KIND>(prims + (size_t)(first + j) * Prim<D, KIND>::STRIDE, a, b);
+#pragma unroll
+      for (int k = 0; k < D; ++k) { mn[k] = fminf(mn[k], a[k]); mx[k] = fmaxf(mx[k], b[k]); }
+    }
+  } else {
+    const float* nd = nodes + (size_t)ref * NF;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      mn[k] = fminf(nd[4 * k + 0], nd[4 * k + 2]);
+      mx[k] = fmaxf(nd[4 * k + 1], nd[4 * k + 3]);
+    }
+  }
+}
+
+template <int D, int KIND>
+__global__ void k_refit_level(float* nodes, const float* __restrict__ prims, const uint32_t* __restrict__ level, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int NF = (D + 1) * 4;
+  float* nd = nodes + (size_t)level[i] * NF;
+  const uint32_t ra = __float_as_uint(nd[4 * D + 0]), rb = __float_as_uint(nd[4 * D + 1]);
+  float amn[D], amx[D], bmn[D], bmx[D];
+  child_box<D, KIND>(nodes, prims, ra, amn, amx);
+  child_box<D, KIND>(nodes, prims, rb, bmn, bmx);
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    nd[4 * k + 0] = amn[k]; nd[4 * k + 1] = amx[k];
+    nd[4 * k + 2] = bmn[k]; nd[4 * k + 3] = bmx[k];
+  }
+}
+
+template <int D, int KIND>
+__global__ void k_refit_root(const float* nodes, const float* __restrict__ prims, float* root) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t ref = __float_as_uint(root[2 * D]);
+  float mn[D], mx[D];
+  child_box<D, KIND>(nodes, prims, ref, mn, mx);
+#pragma unroll
+  for (int k = 0; k < D; ++k) { root[k] = mn[k]; root[D + k] = mx[k]; }
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+static inline uint32_t grid_for(uint64_t n, int block) { return (uint32_t)((n + block - 1) / block); }
+
+static SceneView view_of(const sq_bvh* b) {
+  SceneView s;
+  s.nodes = b->nodes.as<float>();
+  s.prims = b->prims.as<float>();
+  s.ids = b->ids.as<uint32_t>();
+  s.root = b->root.as<float>();
+  return s;
+}
+
+static Filter filter_of(const sq_bvh* b, const sq_filter* f, bool* filt) {
+  Filter out;
+  out.layers = b->layers.as<uint64_t>();
+  out.mask = f ? f->layer_mask : ~0ull;
+  out.exclude = f ? f->exclude : -1;
+  *filt = out.layers != nullptr || out.mask != ~0ull || out.exclude >= 0;
+  return out;
+}
+
+static int begin_timing(sq_ctx* c) {
+  if (!c->ev0) {
+    HIP_TRY(hipEventCreate(&c->ev0));
+    HIP_TRY(hipEventCreate(&c->ev1));
+  }
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  return SQ_OK;
+}
+static int end_timing(sq_ctx* c) {
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  return SQ_OK;
+}
+
+static int upload(DevBuf& d, const void* h, size_t bytes, hipStream_t s) {
+  int rc = d.ensure(bytes);
+  if (rc) return rc;
+  if (bytes) HIP_TRY(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
+  return SQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+// Exclusive scan of n counts into offsets[0..n] on the context stream.
+template <class T>
+static int scan_into(sq_ctx* c, const T* in, uint64_t n, uint64_t* out, std::vector<std::unique_ptr<DevBuf>>& temps) {
+  const uint64_t tiles = (n + 1 + 4095) / 4096;
+  temps.emplace_back(new DevBuf());
+  DevBuf& sums = *temps.back();
+  int rc = sums.ensure(tiles * 8);
+  if (rc) return rc;
+  k_scan_tile<T><<<(uint32_t)tiles, 256, 0, c->stream>>>(in, n, out, sums.as<uint64_t>());
+  HIP_TRY(hipGetLastError());
+  if (tiles > 1) {
+    temps.emplace_back(new DevBuf());
+    DevBuf& toff = *temps.back();
+    if ((rc = toff.ensure((tiles + 1) * 8))) return rc;
+    if ((rc = scan_into<uint64_t>(c, sums.as<uint64_t>(), tiles, toff.as<uint64_t>(), temps))) return rc;
+    k_scan_add<<<grid_for(n + 1, 256), 256, 0, c->stream>>>(out, n + 1, toff.as<uint64_t>());
+    HIP_TRY(hipGetLastError());
+  }
+  return SQ_OK;
+}
+
+extern "C" {
+
+const char* sq_last_error(void) { return g_err.c_str(); }
+int sq_version(void) { return 100; }
+
+int sq_ctx_create(int device, sq_ctx** out) {
+  if (!out) return fail(SQ_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(SQ_ERR_NO_DEVICE, "no HIP device");
+  if (device < 0 || device >= count) return fail(SQ_ERR_NO_DEVICE, "device index out of range");
+  HIP_TRY(hipSetDevice(device));
+  auto c = std::make_unique<sq_ctx>();
+  c->device = device;
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c.release();
+  return SQ_OK;
+}
+
+int sq_ctx_destroy(sq_ctx* c) {
+  if (!c) return SQ_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SQ_OK;
+}
+
+int sq_ctx_synchronize(sq_ctx* c) {
+  if (!c) return fail(SQ_ERR_INVALID, "ctx is NULL");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return SQ_OK;
+}
+
+void* sq_ctx_stream(sq_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int sq_ctx_set_block(sq_ctx* c, int block) {
+  if (!c) return fail(SQ_ERR_INVALID, "ctx is NULL");
+  if (block == 0) block = 256;
+  if (block < 64 || block > 1024 || block % 64) return fail(SQ_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
+  c->block = block;
+  return SQ_OK;
+}
+
+int sq_ctx_last_kernel_ms(sq_ctx* c, float* ms) {
+  if (!c || !ms) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (!c->timed) return fail(SQ_ERR_INVALID, "no timed kernel yet");
+  HIP_TRY(hipEventSynchronize(c->ev1));
+  HIP_TRY(hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return SQ_OK;
+}
+
+struct sq_tree {
+  sqh::RefTree t;
+};
+
+static void tree_stats_of(const sqh::RefTree& t, sq_tree_stats& st) {
+  st.node_count = t.node_count();
+  sqh::tree_structural(t, &st.leaf_count, &st.prim_count, &st.max_depth);
+  // size_of::<Node<D>>() = 8D + 12 bytes, plus u32 prim indices
+  st.bytes = st.node_count * (uint64_t)(8 * t.dim + 12) + t.prim_indices.size() * 4;
+  st.quality = sqh::tree_interior_surface_sum(t);
+}
+
+static int check_dim_kind(int dim, int kind) {
+  return dispatch_dk(dim, kind, [&]<int D, int K>() { return SQ_OK; });
+}
+
+int sq_tree_build(int dim, int kind, const float* prims, uint64_t n, sq_tree** out) {
+  if (!out || (n && !prims)) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  int rc = check_dim_kind(dim, kind);
+  if (rc) return rc;
+  if (n > sqh::LEAF_FIRST_MASK) return fail(SQ_ERR_INVALID, "at most 2^27 primitives per tree");
+  auto t = std::make_unique<sq_tree>();
+  std::vector<float> boxes;
+  sqh::prim_boxes(dim, kind, prims, n, boxes);
+  sqh::build_tree(dim, boxes.data(), (uint32_t)n, t->t);
+  *out = t.release();
+  return SQ_OK;
+}
+
+int sq_tree_from_nodes(int dim, const float* node_bounds, const uint32_t* child_a, const uint32_t* child_b,
+                       const uint32_t* prim_count, uint64_t n_nodes, const uint32_t* prim_indices, uint64_t n,
+                       sq_tree** out) {
+  if (!out) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (dim < 1 || dim > sqh::MAX_DIM) return fail(SQ_ERR_UNSUPPORTED, "dimension out of range");
+  if (n > sqh::LEAF_FIRST_MASK) return fail(SQ_ERR_INVALID, "at most 2^27 primitives per tree");
+  if ((n_nodes == 0) != (n == 0)) return fail(SQ_ERR_INVALID, "an empty tree has no nodes and no primitives");
+  if (n_nodes && (!node_bounds || !child_a || !child_b || !prim_count || !prim_indices)) return fail(SQ_ERR_INVALID, "NULL argument");
+  auto tp = std::make_unique<sq_tree>();
+  sqh::RefTree& t = tp->t;
+  t.dim = dim;
+  t.bounds.assign(node_bounds, node_bounds + n_nodes * 2 * dim);
+  t.child_a.assign(child_a, child_a + n_nodes);
+  t.child_b.assign(child_b, child_b + n_nodes);
+  t.prim_count.assign(prim_count, prim_count + n_nodes);
+  t.prim_indices.assign(prim_indices, prim_indices + n);
+  // The device walks this topology: every link must point forward (the
+  // reference's pre-order guarantee, tree.rs:35-38) and every leaf range must
+  // lie inside prim_indices.
+  for (uint64_t i = 0; i < n_nodes; ++i) {
+    if (t.prim_count[i] == 0) {
+      if (t.child_a[i] <= i || t.child_b[i] <= i || t.child_a[i] >= n_nodes || t.child_b[i] >= n_nodes)
+        return fail(SQ_ERR_INVALID, "child links must point forward inside the node array");
+    } else if ((uint64_t)t.child_a[i] + t.prim_count[i] > n) {
+      return fail(SQ_ERR_INVALID, "leaf range outside prim_indices");
+    }
+  }
+  for (uint64_t i = 0; i < n; ++i) if (t.prim_indices[i] >= n) return fail(SQ_ERR_INVALID, "prim index out of range");
+  *out = tp.release();
+  return SQ_OK;
+}
+
+int sq_tree_refit(sq_tree* tp, int kind, const float* prims) {
+  if (!tp || (tp->t.prim_indices.size() && !prims)) return fail(SQ_ERR_INVALID, "NULL argument");
+  sqh::RefTree& t = tp->t;
+  int rc = check_dim_kind(t.dim, kind);
+  if (rc) return rc;
+  const int D = t.dim;
+  std::vector<float> boxes;
+  sqh::prim_boxes(D, kind, prims, t.prim_indices.size(), boxes);
+  // tree.rs:54-75: one reverse pass, leaves fold their prims from EMPTY.
+  for (size_t ni = t.node_count(); ni-- > 0;) {
+    float* b = t.bounds.data() + ni * 2 * D;
+    if (t.prim_count[ni] > 0) {
+      for (int k = 0; k < D; ++k) { b[k] = INFINITY; b[D + k] = -INFINITY; }
+      for (uint32_t j = t.child_a[ni]; j < t.child_a[ni] + t.prim_count[ni]; ++j) {
+        const float* pb = boxes.data() + (size_t)t.prim_indices[j] * 2 * D;
+        for (int k = 0; k < D; ++k) { b[k] = fminf(b[k], pb[k]); b[D + k] = fmaxf(b[D + k], pb[D + k]); }
+      }
+    } else {
+      const float* ba = t.bounds.data() + (size_t)t.child_a[ni] * 2 * D;
+      const float* bb = t.bounds.data() + (size_t)t.child_b[ni] * 2 * D;
+      for (int k = 0; k < D; ++k) { b[k] = fminf(ba[k], bb[k]); b[D + k] = fmaxf(ba[D + k], bb[D + k]); }
+    }
+  }
+  return SQ_OK;
+}
+
+int sq_tree_get_stats(const sq_tree* tp, sq_tree_stats* out) {
+  if (!tp || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  tree_stats_of(tp->t, *out);
+  return SQ_OK;
+}
+
+int sq_tree_export(const sq_tree* tp, float* bounds, uint32_t* ca, uint32_t* cb, uint32_t* pc, uint32_t* idx) {
+  if (!tp) return fail(SQ_ERR_INVALID, "tree is NULL");
+  const sqh::RefTree& t = tp->t;
+  if (bounds) std::memcpy(bounds, t.bounds.data(), t.bounds.size() * 4);
+  if (ca) std::memcpy(ca, t.child_a.data(), t.child_a.size() * 4);
+  if (cb) std::memcpy(cb, t.child_b.data(), t.child_b.size() * 4);
+  if (pc) std::memcpy(pc, t.prim_count.data(), t.prim_count.size() * 4);
+  if (idx) std::memcpy(idx, t.prim_indices.data(), t.prim_indices.size() * 4);
+  return SQ_OK;
+}
+
+int sq_tree_destroy(sq_tree* tp) {
+  delete tp;
+  return SQ_OK;
+}
+
+int sq_bvh_upload(sq_ctx* ctx, const sq_tree* tp, int kind, const float* prims, sq_bvh** out) {
+  if (!ctx || !tp || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  const sqh::RefTree& t = tp->t;
+  const int dim = t.dim;
+  int rc = check_dim_kind(dim, kind);
+  if (rc) return rc;
+  const uint64_t n = t.prim_indices.size();
+  if (n && !prims) return fail(SQ_ERR_INVALID, "prims is NULL");
+  auto b = std::make_unique<sq_bvh>();
+  b->ctx = ctx; b->dim = dim; b->kind = kind; b->n_prims = n;
+  b->tree = t;
+  try {
+    sqh::pack_tree(b->tree, b->packed);
+  } catch (const std::exception& e) {
+    return fail(SQ_ERR_INVALID, e.what());
+  }
+  std::vector<float> dp;
+  sqh::pack_prims(dim, kind, prims, b->tree, dp);
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = upload(b->nodes, b->packed.nodes.data(), b->packed.nodes.size() * 4, s))) return rc;
+  if ((rc = upload(b->prims, dp.data(), dp.size() * 4, s))) return rc;
+  if ((rc = upload(b->ids, b->tree.prim_indices.data(), b->tree.prim_indices.size() * 4, s))) return rc;
+  if ((rc = upload(b->root, b->packed.root.data(), b->packed.root.size() * 4, s))) return rc;
+  if ((rc = upload(b->levels, b->packed.levels.data(), b->packed.levels.size() * 4, s))) return rc;
+  HIP_TRY(hipStreamSynchronize(s));
+  tree_stats_of(b->tree, b->stats);
+  *out = b.release();
+  return SQ_OK;
+}
+
+int sq_bvh_build(sq_ctx* ctx, int dim, int kind, const float* prims, uint64_t n, sq_bvh** out) {
+  if (!ctx || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  sq_tree* t = nullptr;
+  int rc = sq_tree_build(dim, kind, prims, n, &t);
+  if (rc) return rc;
+  rc = sq_bvh_upload(ctx, t, kind, prims, out);
+  sq_tree_destroy(t);
+  return rc;
+}
+
+int sq_bvh_destroy(sq_bvh* b) {
+  if (!b) return SQ_OK;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  delete b;
+  return SQ_OK;
+}
+
+int sq_bvh_set_layers(sq_bvh* b, const uint64_t* layers) {
+  if (!b) return fail(SQ_ERR_INVALID, "bvh is NULL");
+  if (!layers) {
+    b->layers.~DevBuf();
+    new (&b->layers) DevBuf();
+    return SQ_OK;
+  }
+  int rc = upload(b->layers, layers, b->n_prims * 8, b->ctx->stream);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  return SQ_OK;
+}
+
+int sq_bvh_get_stats(const sq_bvh* b, sq_tree_stats* out) {
+  if (!b || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = b->stats;
+  return SQ_OK;
+}
+
+int sq_bvh_download_bounds(sq_bvh* b, float* bounds) {
+  if (!b || !bounds) return fail(SQ_ERR_INVALID, "NULL argument");
+  const int D = b->dim, NF = (D + 1) * 4;
+  const size_t nn = b->tree.node_count();
+  if (!nn) return SQ_OK;
+  std::vector<float> nodes(b->packed.nodes.size()), root(16);
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  if (!nodes.empty()) HIP_TRY(hipMemcpy(nodes.data(), b->nodes.p, nodes.size() * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(root.data(), b->root.p, 16 * 4, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < nn; ++i) {
+    float* o = bounds + i * 2 * D;
+    const uint32_t slot = b->packed.slot_of_ref[i];
+    if (slot == 0xFFFFFFFFu) {
+      std::memcpy(o, root.data(), 2 * D * 4);
+    } else {
+      const float* nd = nodes.data() + (size_t)(slot >> 1) * NF;
+      const int s = slot & 1;
+      for (int k = 0; k < D; ++k) { o[k] = nd[4 * k + 2 * s]; o[D + k] = nd[4 * k + 2 * s + 1]; }
+    }
+  }
+  return SQ_OK;
+}
+
+int sq_bvh_refit(sq_bvh* b, const float* prims_device) {
+  if (!b || (b->n_prims && !prims_device)) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (!b->n_prims) return SQ_OK;
+  sq_ctx* c = b->ctx;
+  return dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+    const uint32_t n = (uint32_t)b->n_prims;
+    k_scatter_prims<D, K><<<grid_for(n, 256), 256, 0, c->stream>>>(prims_device, b->ids.as<uint32_t>(), b->prims.as<float>(), n);
+    HIP_TRY(hipGetLastError());
+    const auto& lo = b->packed.level_off;
+    for (size_t l = 0; l + 1 < lo.size(); ++l) {
+      const uint32_t cnt = lo[l + 1] - lo[l];
+      if (!cnt) continue;
+      k_refit_level<D, K><<<grid_for(cnt, 256), 256, 0, c->stream>>>(b->nodes.as<float>(), b->prims.as<float>(),
+                                                                    b->levels.as<uint32_t>() + lo[l], cnt);
+      HIP_TRY(hipGetLastError());
+    }
+    k_refit_root<D, K><<<1, 64, 0, c->stream>>>(b->nodes.as<float>(), b->prims.as<float>(), b->root.as<float>());
+    HIP_TRY(hipGetLastError());
+    return SQ_OK;
+  });
+}
+
+static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, const float* tmax, const sq_filter* f,
+                        NearestOut out, bool any, bool cast) {
+  if (!b || (n && !q)) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "at most 2^32-1 queries per call");
+  if (!n) return SQ_OK;
+  sq_ctx* c = b->ctx;
+  const int B = c->block;
+  if (!b->n_prims || (f && f->layer_mask == 0) || (cast && b->kind == KIND_TRI)) {
+    k_fill_miss<<<grid_for(n, 256), 256, 0, c->stream>>>((uint32_t)n, b->dim, out);
+    HIP_TRY(hipGetLastError());
+    return SQ_OK;
+  }
+  bool filt;
+  Filter fd = filter_of(b, f, &filt);
+  const uint32_t stack_n = b->packed.max_stack;
+  const size_t lds = (size_t)stack_n * B * 8;
+  if (lds > 160 * 1024) return fail(SQ_ERR_INVALID, "traversal stack does not fit in LDS; lower the block size");
+  SceneView s = view_of(b);
+  int rc = begin_timing(c);
+  if (rc) return rc;
+  rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+    dim3 grid(grid_for(n, B)), block(B);
+    const uint32_t nn = (uint32_t)n;
+#define SQ_LAUNCH(ANY_, FILT_, CAST_) \
+    k_nearest<D, K, ANY_, FILT_, CAST_><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, out, stack_n)
+    if (cast) { if (filt) SQ_LAUNCH(false, true, true); else SQ_LAUNCH(false, false, true); }
+    else if (any) { if (filt) SQ_LAUNCH(true, true, false); else SQ_LAUNCH(true, false, false); }
+    else { if (filt) SQ_LAUNCH(false, true, false); else SQ_LAUNCH(false, false, false); }
+#undef SQ_LAUNCH
+    HIP_TRY(hipGetLastError());
+    return SQ_OK;
+  });
+  if (rc) return rc;
+  return end_timing(c);
+}
+
+int sq_raycast_nearest(sq_bvh* b, const float* rays, uint64_t n, float max_toi, const float* tmax, const sq_filter* f,
+                       int32_t* out_leaf, float* out_toi, float* out_point, float* out_normal, int32_t* out_sub) {
+  if (n && (!out_leaf || !out_toi)) return fail(SQ_ERR_INVALID, "out_leaf and out_toi are required");
+  NearestOut o{out_leaf, out_toi, out_point, out_normal, out_sub, nullptr};
+  return nearest_impl(b, rays, n, max_toi, tmax, f, o, false, false);
+}
+
+int sq_raycast_any(sq_bvh* b, const float* rays, uint64_t n, float max_toi, const float* tmax, const sq_filter* f, uint8_t* out_hit) {
+  if (n && !out_hit) return fail(SQ_ERR_INVALID, "out_hit is required");
+  NearestOut o{nullptr, nullptr, nullptr, nullptr, nullptr, out_hit};
+  return nearest_impl(b, rays, n, max_toi, tmax, f, o, true, false);
+}
+
+int sq_shapecast_nearest(sq_bvh* b, const float* casts, uint64_t n, float max_toi, const float* tmax, const sq_filter* f,
+                         int32_t* out_leaf, float* out_toi, float* out_point) {
+  if (n && (!out_leaf || !out_toi)) return fail(SQ_ERR_INVALID, "out_leaf and out_toi are required");
+  NearestOut o{out_leaf, out_toi, out_point, nullptr, nullptr, nullptr};
+  return nearest_impl(b, casts, n, max_toi, tmax, f, o, false, true);
+}
+
+// count -> scan -> (check capacity) -> fill -> segment sort -> unpack
+static int collect_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, const float* tmax, const sq_filter* f, int mode,
+                        uint64_t* offsets, int32_t* out_leaf, float* out_toi, uint64_t capacity, uint64_t* total) {
+  if (!b || !offsets || !total || (n && !q)) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "at most 2^32-1 queries per call");
+  sq_ctx* c = b->ctx;
+  *total = 0;
+  if (!n) {
+    HIP_TRY(hipMemsetAsync(offsets, 0, 8, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SQ_OK;
+  }
+  const bool empty = !b->n_prims || (f && f->layer_mask == 0) || (mode == 3 && b->kind == KIND_TRI);
+  int rc;
+  if ((rc = c->counts.ensure(n * 4))) return rc;
+  uint32_t* counts = c->counts.as<uint32_t>();
+  if (empty) {
+    HIP_TRY(hipMemsetAsync(offsets, 0, (n + 1) * 8, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SQ_OK;
+  }
+  bool filt;
+  Filter fd = filter_of(b, f, &filt);
+  const int B = c->block;
+  const uint32_t stack_n = b->packed.max_stack;
+  const size_t lds = (size_t)stack_n * B * 4;
+  SceneView s = view_of(b);
+  const uint32_t nn = (uint32_t)n;
+  auto launch = [&](bool fill, uint64_t* keys) -> int {
+    return dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+      dim3 grid(grid_for(n, B)), block(B);
+#define SQ_COLLECT(MODE_)                                                                                              \
+      if (fill) { if (filt) k_collect<D, K, MODE_, true, true><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, counts, offsets, keys); \
+                  else k_collect<D, K, MODE_, true, false><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, counts, offsets, keys); } \
+      else { if (filt) k_collect<D, K, MODE_, false, true><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, counts, offsets, keys); \
+             else k_collect<D, K, MODE_, false, false><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, counts, offsets, keys); }
+      switch (mode) {
+        case 0: SQ_COLLECT(0) break;
+        case 1: SQ_COLLECT(1) break;
+        case 2: SQ_COLLECT(2) break;
+        default: SQ_COLLECT(3) break;
+      }
+#undef SQ_COLLECT
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    });
+  };
+  if ((rc = launch(false, nullptr))) return rc;
+  std::vector<std::unique_ptr<DevBuf>> temps;
+  if ((rc = scan_into<uint32_t>(c, counts, n, offsets, temps))) return rc;
+  HIP_TRY(hipMemcpyAsync(total, offsets + n, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const uint64_t tot = *total;
+  if (tot > capacity) return fail(SQ_ERR_CAPACITY, "result needs " + std::to_string(tot) + " entries, capacity is " + std::to_string(capacity));
+  if (!tot) return SQ_OK;
+  if (!out_leaf || (mode != 2 && !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
+  if ((rc = c->keys.ensure(tot * 8))) return rc;
+  if ((rc = c->keys2.ensure(tot * 8))) return rc;
+  if ((rc = c->scratch.ensure((n + 1) * 4 + 64))) return rc;
+  uint64_t* keys = c->keys.as<uint64_t>();
+  if ((rc = launch(true, keys))) return rc;
+  uint32_t* n_large = c->scratch.as<uint32_t>();
+  uint32_t* large = n_large + 16;
+  HIP_TRY(hipMemsetAsync(n_large, 0, 4, c->stream));
+  k_sort_small<<<grid_for(n, 256), 256, 0, c->stream>>>(keys, offsets, nn, large, n_large);
+  HIP_TRY(hipGetLastError());
+  uint32_t h_large = 0;
+  HIP_TRY(hipMemcpyAsync(&h_large, n_large, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (h_large) {
+    k_sort_large<<<h_large, 256, 0, c->stream>>>(keys, c->keys2.as<uint64_t>(), offsets, large);
+    HIP_TRY(hipGetLastError());
+  }
+  k_unpack<<<grid_for(tot, 256), 256, 0, c->stream>>>(keys, tot, out_leaf, out_toi, mode != 2 ? 1 : 0);
+  HIP_TRY(hipGetLastError());
+  return SQ_OK;
+}
+
+int sq_raycast_all(sq_bvh* b, const float* rays, uint64_t n, float max_toi, const sq_filter* f, int crossings,
+                   uint64_t* offsets, int32_t* out_leaf, float* out_toi, uint64_t capacity, uint64_t* total) {
+  return collect_impl(b, rays, n, max_toi, nullptr, f, crossings ? 1 : 0, offsets, out_leaf, out_toi, capacity, total);
+}
+
+int sq_overlap(sq_bvh* b, const float* regions, uint64_t n, const sq_filter* f, uint64_t* offsets, int32_t* out_leaf,
+               uint64_t capacity, uint64_t* total) {
+  return collect_impl(b, regions, n, 0.0f, nullptr, f, 2, offsets, out_leaf, nullptr, capacity, total);
+}
+
+int sq_shapecast_all(sq_bvh* b, const float* casts, uint64_t n, float max_toi, const float* tmax, const sq_filter* f,
+                     uint64_t* offsets, int32_t* out_leaf, float* out_toi, uint64_t capacity, uint64_t* total) {
+  return collect_impl(b, casts, n, max_toi, tmax, f, 3, offsets, out_leaf, out_toi, capacity, total);
+}
+
+int sq_required_work(sq_bvh* b, const float* rays, uint64_t n, const float* tlim, uint32_t* out_nodes, uint32_t* out_prims) {
+  if (!b || (n && (!rays || !tlim || !out_nodes || !out_prims))) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (!n) return SQ_OK;
+  sq_ctx* c = b->ctx;
+  if (!b->n_prims) {
+    HIP_TRY(hipMemsetAsync(out_nodes, 0, n * 4, c->stream));
+    HIP_TRY(hipMemsetAsync(out_prims, 0, n * 4, c->stream));
+    return SQ_OK;
+  }
+  const int B = c->block;
+  const uint32_t stack_n = b->packed.max_stack;
+  const size_t lds = (size_t)stack_n * B * 4;
+  SceneView s = view_of(b);
+  return dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+    if constexpr (K != KIND_SPHERE) {
+      k_required<D><<<grid_for(n, B), B, lds, c->stream>>>(s, rays, (uint32_t)n, tlim, out_nodes, out_prims, stack_n);
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    } else {
+      // same kernel: the walk does not look at primitives
+      k_required<D><<<grid_for(n, B), B, lds, c->stream>>>(s, rays, (uint32_t)n, tlim, out_nodes, out_prims, stack_n);
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,561 @@
+// traverse.h -- per-ray query logic of the engine, shared by the HIP kernels
+// (engine.hip) and a CPU build of the same source used by the tests.
+//
+// FP32 arithmetic follows the reference operation for operation (no
+// contraction: every TU including this header is built with
+// -ffp-contract=off; divisions and square roots are correctly rounded):
+//   slab test            crates/spatial-query/src/maths/aabb.rs:118-148
+//   ball narrow phase    crates/spatial-query/tests/differential.rs:61-82
+//   Moeller-Trumbore     crates/viewport-lib-query/src/lib.rs:279-306
+//   nearest update       crates/spatial-query/src/accel/bvh.rs:106-115 with
+//                        hit_ordering, query/hit.rs:77-84
+//
+// Traversal order is near-first over the pair-node layout (host_tree.h).  The
+// reference pushes child a then b and pops LIFO; its nearest result is still
+// reproduced exactly because every accepted hit of a built-in kind lies inside
+// the slab interval of its own AABB (AABB hits are that interval's entry;
+// SPHERE and TRIANGLE hits are guarded to it), hence inside every ancestor
+// box, so pruning by "entry <= current best toi" can never drop a hit that
+// beats the current best under (toi, leaf).  The result is the minimum under
+// (toi, leaf) over all accepted hits, whatever the visiting order.
+#pragma once
+
+#include <cstdint>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define SQ_HD __host__ __device__ __forceinline__
+#define SQ_DIV_RN(a, b) __fdiv_rn((a), (b))
+#define SQ_SQRT_RN(a) __fsqrt_rn(a)
+#else
+#include <cmath>
+#include <cstring>
+#define SQ_HD inline
+#define SQ_DIV_RN(a, b) ((a) / (b))
+#define SQ_SQRT_RN(a) sqrtf(a)
+#endif
+
+namespace sqd {
+
+constexpr float F32_EPSILON = 1.1920929e-7f;   // f32::EPSILON
+constexpr float SQ_INF = __builtin_huge_valf();
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+constexpr uint32_t FIRST_MASK = (1u << 27) - 1;
+constexpr uint32_t NO_LEAF = 0xFFFFFFFFu;
+
+enum { KIND_AABB = 0, KIND_SPHERE = 1, KIND_TRI = 2 };
+
+SQ_HD uint32_t fbits(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __float_as_uint(f);
+#else
+  uint32_t u; std::memcpy(&u, &f, 4); return u;
+#endif
+}
+SQ_HD float bitsf(uint32_t u) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __uint_as_float(u);
+#else
+  float f; std::memcpy(&f, &u, 4); return f;
+#endif
+}
+// f32::total_cmp as a signed key.
+SQ_HD int32_t total_key(float f) {
+  int32_t i = (int32_t)fbits(f);
+  return i ^ (int32_t)(((uint32_t)(i >> 31)) >> 1);
+}
+// (toi, leaf) strictly before (bt, bl) under hit_ordering.
+SQ_HD bool hit_before(float t, uint32_t leaf, float bt, uint32_t bl) {
+  int32_t a = total_key(t), b = total_key(bt);
+  return a < b || (a == b && leaf < bl);
+}
+
+struct Filter {
+  const uint64_t* layers;   // by original leaf id, or null (all ones)
+  uint64_t mask;
+  int64_t exclude;          // -1 = None
+};
+
+SQ_HD bool accepts(const Filter& f, uint32_t leaf) {
+  if (f.exclude >= 0 && (int64_t)leaf == f.exclude) return false;
+  uint64_t lay = f.layers ? f.layers[leaf] : ~0ull;
+  return (lay & f.mask) != 0;
+}
+
+template <int D> struct RayD {
+  float o[D], d[D], inv[D];
+  uint32_t par;             // bit k: |d[k]| < EPSILON (the slab's parallel branch)
+};
+
+template <int D> SQ_HD void ray_setup(RayD<D>& r, const float* src) {
+  r.par = 0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    r.o[k] = src[k];
+    r.d[k] = src[D + k];
+    // The reference computes 1.0 / d per node; hoisting it gives the same bits.
+    r.inv[k] = SQ_DIV_RN(1.0f, r.d[k]);
+    if (fabsf(r.d[k]) < F32_EPSILON) r.par |= 1u << k;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Slab tests
+// ---------------------------------------------------------------------------
+
+// Exact reference slab (aabb.rs:118-148) on one box given as min[D], max[D]
+// with stride `s` between the min and max of an axis.  Returns hit; writes the
+// entry (t_min, exactly as the reference: +0 unless an axis raised it), the
+// exit and the entry axis (-1 when no axis raised t_min).
+template <int D> SQ_HD bool slab_exact(const RayD<D>& r, const float* mn, const float* mx, float max_toi,
+                                       float& t_min_out, float& t_max_out, int& axis_out) {
+  float t_min = 0.0f, t_max = max_toi;
+  int axis = -1;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    if (r.par & (1u << k)) {
+      if (r.o[k] < mn[k] || r.o[k] > mx[k]) return false;
+    } else {
+      float t1 = (mn[k] - r.o[k]) * r.inv[k];
+      float t2 = (mx[k] - r.o[k]) * r.inv[k];
+      if (t1 > t2) { float tmp = t1; t1 = t2; t2 = tmp; }
+      if (t1 > t_min) { t_min = t1; axis = k; }
+      if (t2 < t_max) t_max = t2;
+      if (t_min > t_max) return false;
+    }
+  }
+  t_min_out = t_min; t_max_out = t_max; axis_out = axis;
+  return true;
+}
+
+// Pruning slab of both children of a pair node.  Pass/fail equals the exact
+// slab's for finite inputs (min/max replace the strict compare-and-assign; the
+// sign of a zero entry cannot change a pass/fail).  ta/tb are the entries.
+template <int D> SQ_HD void slab_pair(const RayD<D>& r, const float* nd, float limit, bool& ha, bool& hb, float& ta, float& tb) {
+  float amin = 0.0f, amax = limit, bmin = 0.0f, bmax = limit;
+  if (r.par == 0) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const float o = r.o[k], iv = r.inv[k];
+      float a1 = (nd[4 * k + 0] - o) * iv, a2 = (nd[4 * k + 1] - o) * iv;
+      float b1 = (nd[4 * k + 2] - o) * iv, b2 = (nd[4 * k + 3] - o) * iv;
+      amin = fmaxf(amin, fminf(a1, a2)); amax = fminf(amax, fmaxf(a1, a2));
+      bmin = fmaxf(bmin, fminf(b1, b2)); bmax = fminf(bmax, fmaxf(b1, b2));
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const float o = r.o[k];
+      if (r.par & (1u << k)) {
+        if (o < nd[4 * k + 0] || o > nd[4 * k + 1]) amin = SQ_INF;
+        if (o < nd[4 * k + 2] || o > nd[4 * k + 3]) bmin = SQ_INF;
+      } else {
+        const float iv = r.inv[k];
+        float a1 = (nd[4 * k + 0] - o) * iv, a2 = (nd[4 * k + 1] - o) * iv;
+        float b1 = (nd[4 * k + 2] - o) * iv, b2 = (nd[4 * k + 3] - o) * iv;
+        amin = fmaxf(amin, fminf(a1, a2)); amax = fminf(amax, fmaxf(a1, a2));
+        bmin = fmaxf(bmin, fminf(b1, b2)); bmax = fminf(bmax, fmaxf(b1, b2));
+      }
+    }
+  }
+  ha = amin <= amax; hb = bmin <= bmax; ta = amin; tb = bmin;
+}
+
+// ---------------------------------------------------------------------------
+// Narrow phases.  `p` points at the leaf-ordered device record (host_tree.h).
+// Each returns true with toi when the primitive is hit within `limit`.
+// ---------------------------------------------------------------------------
+
+template <int D> SQ_HD bool narrow_aabb(const RayD<D>& r, const float* p, float limit, float& t, int& axis) {
+  float te, tx;
+  if (!slab_exact<D>(r, p, p + D, limit, te, tx, axis)) return false;
+  t = te;
+  return true;
+}
+
+template <int D> SQ_HD float dotD(const float* a, const float* b) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int k = 0; k < D; ++k) acc = acc + a[k] * b[k];
+  return acc;
+}
+
+// Ball (differential.rs:61-82) with the box guard.
+template <int D> SQ_HD bool narrow_sphere(const RayD<D>& r, const float* p, float limit, float& t_out) {
+  float oc[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) oc[k] = r.o[k] - p[k];
+  const float rad = p[D];
+  float b = dotD<D>(oc, r.d);
+  float cc = dotD<D>(oc, oc) - rad * rad;
+  float disc = b * b - cc;
+  if (disc < 0.0f) return false;
+  float sq = SQ_SQRT_RN(disc);
+  float t = -b - sq;
+  if (t < 0.0f) t = -b + sq;
+  if (t < 0.0f || t > limit) return false;
+  // guard: toi inside the slab interval of the ball's AABB (c - r, c + r)
+  float mn[D], mx[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) { mn[k] = p[k] - rad; mx[k] = p[k] + rad; }
+  float te, tx; int ax;
+  if (!slab_exact<D>(r, mn, mx, limit, te, tx, ax)) return false;
+  if (!(te <= t && t <= tx)) return false;
+  t_out = t;
+  return true;
+}
+
+// Moeller-Trumbore (viewport-lib-query/src/lib.rs:281-306) with the reference
+// core's `t > max_toi` reject and the box guard.  p = v0.xyz,id, v1.xyz,_, v2.xyz,_
+SQ_HD bool narrow_tri(const RayD<3>& r, const float* p, float limit, float& t_out) {
+  const float v0x = p[0], v0y = p[1], v0z = p[2];
+  const float v1x = p[4], v1y = p[5], v1z = p[6];
+  const float v2x = p[8], v2y = p[9], v2z = p[10];
+  const float e1x = v1x - v0x, e1y = v1y - v0y, e1z = v1z - v0z;
+  const float e2x = v2x - v0x, e2y = v2y - v0y, e2z = v2z - v0z;
+  const float dx = r.d[0], dy = r.d[1], dz = r.d[2];
+  // p = d x e2
+  const float px = dy * e2z - dz * e2y;
+  const float py = dz * e2x - dx * e2z;
+  const float pz = dx * e2y - dy * e2x;
+  const float det = ((0.0f + e1x * px) + e1y * py) + e1z * pz;
+  if (fabsf(det) < 1e-8f) return false;
+  const float inv = SQ_DIV_RN(1.0f, det);
+  const float tx_ = r.o[0] - v0x, ty_ = r.o[1] - v0y, tz_ = r.o[2] - v0z;
+  const float u = (((0.0f + tx_ * px) + ty_ * py) + tz_ * pz) * inv;
+  if (!(u >= 0.0f && u <= 1.0f)) return false;
+  // q = tvec x e1
+  const float qx = ty_ * e1z - tz_ * e1y;
+  const float qy = tz_ * e1x - tx_ * e1z;
+  const float qz = tx_ * e1y - ty_ * e1x;
+  const float v = (((0.0f + dx * qx) + dy * qy) + dz * qz) * inv;
+  if (v < 0.0f || u + v > 1.0f) return false;
+  const float t = (((0.0f + e2x * qx) + e2y * qy) + e2z * qz) * inv;
+  if (!(t > 1e-6f)) return false;
+  if (t > limit) return false;
+  // guard: toi inside the slab interval of the triangle's AABB
+  float mn[3], mx[3];
+  mn[0] = fminf(fminf(v0x, v1x), v2x); mx[0] = fmaxf(fmaxf(v0x, v1x), v2x);
+  mn[1] = fminf(fminf(v0y, v1y), v2y); mx[1] = fmaxf(fmaxf(v0y, v1y), v2y);
+  mn[2] = fminf(fminf(v0z, v1z), v2z); mx[2] = fmaxf(fmaxf(v0z, v1z), v2z);
+  float te, tx; int ax;
+  if (!slab_exact<3>(r, mn, mx, limit, te, tx, ax)) return false;
+  if (!(te <= t && t <= tx)) return false;
+  t_out = t;
+  return true;
+}
+
+template <int D, int KIND> struct Prim;
+template <int D> struct Prim<D, KIND_AABB> { static constexpr int STRIDE = 2 * D; };
+template <int D> struct Prim<D, KIND_SPHERE> { static constexpr int STRIDE = D + 1; };
+template <> struct Prim<3, KIND_TRI> { static constexpr int STRIDE = 12; };
+
+template <int D, int KIND> SQ_HD bool narrow(const RayD<D>& r, const float* p, float limit, float& t) {
+  if constexpr (KIND == KIND_AABB) { int ax; return narrow_aabb<D>(r, p, limit, t, ax); }
+  else if constexpr (KIND == KIND_SPHERE) return narrow_sphere<D>(r, p, limit, t);
+  else return narrow_tri(r, p, limit, t);
+}
+
+// Normal and sub-object of the final hit (computed once per ray).
+// AABB: entry-face normal (-sign(d) on the entry axis, zero when the origin is
+// inside); SPHERE: normalize_or_zero(at(t) - c) (point.rs:82-89); TRIANGLE:
+// normalize_or_zero(e1 x e2) turned against the ray.
+template <int D, int KIND> SQ_HD void hit_normal(const RayD<D>& r, const float* p, float t, float limit, float* n) {
+#pragma unroll
+  for (int k = 0; k < D; ++k) n[k] = 0.0f;
+  if constexpr (KIND == KIND_AABB) {
+    float tt; int ax = -1;
+    narrow_aabb<D>(r, p, limit, tt, ax);
+    if (ax >= 0) n[ax] = r.d[ax] > 0.0f ? -1.0f : 1.0f;
+  } else if constexpr (KIND == KIND_SPHERE) {
+    float v[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) v[k] = (r.o[k] + r.d[k] * t) - p[k];
+    float len = SQ_SQRT_RN(dotD<D>(v, v));
+    if (len > F32_EPSILON) {
+      float s = SQ_DIV_RN(1.0f, len);
+#pragma unroll
+      for (int k = 0; k < D; ++k) n[k] = v[k] * s;
+    }
+  } else {
+    const float e1x = p[4] - p[0], e1y = p[5] - p[1], e1z = p[6] - p[2];
+    const float e2x = p[8] - p[0], e2y = p[9] - p[1], e2z = p[10] - p[2];
+    float c[3] = {e1y * e2z - e1z * e2y, e1z * e2x - e1x * e2z, e1x * e2y - e1y * e2x};
+    float len = SQ_SQRT_RN(dotD<3>(c, c));
+    if (len > F32_EPSILON) {
+      float s = SQ_DIV_RN(1.0f, len);
+      for (int k = 0; k < 3; ++k) n[k] = c[k] * s;
+      if (dotD<3>(n, r.d) > 0.0f) for (int k = 0; k < 3; ++k) n[k] = n[k] * -1.0f;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Query policies: what a node test and a leaf test mean for each query kind.
+// ---------------------------------------------------------------------------
+
+// The AABB of one leaf-ordered primitive record (QueryGeometry::world_aabb).
+template <int D, int KIND> SQ_HD void prim_box(const float* p, float* mn, float* mx) {
+  if constexpr (KIND == KIND_AABB) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) { mn[k] = p[k]; mx[k] = p[D + k]; }
+  } else if constexpr (KIND == KIND_SPHERE) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) { mn[k] = p[k] - p[D]; mx[k] = p[k] + p[D]; }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = fminf(fminf(p[k], p[4 + k]), p[8 + k]);
+      mx[k] = fmaxf(fmaxf(p[k], p[4 + k]), p[8 + k]);
+    }
+  }
+}
+
+// Rays (bvh.rs:51-328): slab pruning, the kind's ray narrow phase.
+template <int D, int KIND> struct RayPolicy {
+  RayD<D> r;
+  SQ_HD void setup(const float* src) { ray_setup<D>(r, src); }
+  SQ_HD bool root(const float* root, float limit) const {
+    float te, tx; int ax;
+    return slab_exact<D>(r, root, root + D, limit, te, tx, ax);
+  }
+  SQ_HD void pair(const float* nd, float limit, bool& ha, bool& hb, float& ta, float& tb) const {
+    slab_pair<D>(r, nd, limit, ha, hb, ta, tb);
+  }
+  SQ_HD bool narrow(const float* p, float limit, float& t) const { return sqd::narrow<D, KIND>(r, p, limit, t); }
+};
+
+// Shape casts (bvh.rs:331-414): the slab of the cast ray against each box
+// grown by the probe box, [min - probe.max, max - probe.min]
+// (shapecast.rs:70-73).  Narrow: AABB leaves answer with the same slab against
+// the grown leaf box (its entry); SPHERE leaves with the ball of radius
+// r + probe.max[0] (shape_overlap.rs:104-118) guarded by the grown box;
+// TRIANGLE leaves keep the reference default (no contact).
+template <int D, int KIND> struct CastPolicy {
+  RayD<D> r;
+  float pmin[D], pmax[D];
+  SQ_HD void setup(const float* src) {
+    ray_setup<D>(r, src);
+#pragma unroll
+    for (int k = 0; k < D; ++k) { pmin[k] = src[2 * D + k]; pmax[k] = src[3 * D + k]; }
+  }
+  SQ_HD bool grown_slab(const float* mn, const float* mx, float limit, float& te, float& tx) const {
+    float gmn[D], gmx[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) { gmn[k] = mn[k] - pmax[k]; gmx[k] = mx[k] - pmin[k]; }
+    int ax;
+    return slab_exact<D>(r, gmn, gmx, limit, te, tx, ax);
+  }
+  SQ_HD bool root(const float* root, float limit) const {
+    float te, tx;
+    return grown_slab(root, root + D, limit, te, tx);
+  }
+  SQ_HD void pair(const float* nd, float limit, bool& ha, bool& hb, float& ta, float& tb) const {
+    float g[4 * D + 4];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      g[4 * k + 0] = nd[4 * k + 0] - pmax[k]; g[4 * k + 1] = nd[4 * k + 1] - pmin[k];
+      g[4 * k + 2] = nd[4 * k + 2] - pmax[k]; g[4 * k + 3] = nd[4 * k + 3] - pmin[k];
+    }
+    slab_pair<D>(r, g, limit, ha, hb, ta, tb);
+  }
+  SQ_HD bool narrow(const float* p, float limit, float& t) const {
+    if constexpr (KIND == KIND_TRI) {
+      return false;
+    } else {
+      float mn[D], mx[D];
+      prim_box<D, KIND>(p, mn, mx);
+      float te, tx;
+      if constexpr (KIND == KIND_AABB) {
+        if (!grown_slab(mn, mx, limit, te, tx)) return false;
+        t = te;
+        return true;
+      } else {
+        float oc[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) oc[k] = r.o[k] - p[k];
+        const float rad = p[D] + pmax[0];
+        float b = dotD<D>(oc, r.d);
+        float cc = dotD<D>(oc, oc) - rad * rad;
+        float disc = b * b - cc;
+        if (disc < 0.0f) return false;
+        float sq = SQ_SQRT_RN(disc);
+        float tt = -b - sq;
+        if (tt < 0.0f) tt = -b + sq;
+        if (tt < 0.0f || tt > limit) return false;
+        if (!grown_slab(mn, mx, limit, te, tx)) return false;
+        if (!(te <= tt && tt <= tx)) return false;
+        t = tt;
+        return true;
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// The walks
+// ---------------------------------------------------------------------------
+
+struct SceneView {
+  const float* nodes;       // pair nodes, (D+1)*4 floats each
+  const float* prims;       // leaf-ordered records
+  const uint32_t* ids;      // leaf-ordered original leaf ids
+  const float* root;        // [2D] bounds, then the root ref's bits
+};
+
+// Stack accessor: entry k of this lane lives at ref[k * stride] / t[k * stride].
+struct Stack {
+  uint32_t* ref;
+  float* t;
+  uint32_t stride;
+};
+
+struct NearestResult {
+  float t;
+  uint32_t leaf;   // original id, NO_LEAF on a miss
+  uint32_t pos;    // leaf-ordered position of the hit primitive
+};
+
+// Nearest (ANY = false) or first accepted hit (ANY = true) of one query.
+template <int D, int PS, bool ANY, bool FILT, class Q>
+SQ_HD NearestResult walk_nearest(const SceneView& s, const Q& q, float max_toi, const Filter& f, Stack st) {
+  NearestResult res{SQ_INF, NO_LEAF, NO_LEAF};
+  float limit = max_toi;
+  if (!q.root(s.root, limit)) return res;
+  uint32_t cur = fbits(s.root[2 * D]);
+  uint32_t sp = 0;
+  constexpr int NF = (D + 1) * 4;
+  while (true) {
+    if (!(cur & LEAF_BIT)) {
+      const float* nd = s.nodes + (size_t)cur * NF;
+      bool ha, hb; float ta, tb;
+      q.pair(nd, limit, ha, hb, ta, tb);
+      const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
+      if (ha && hb) {
+        const bool a_first = ta <= tb;
+        st.ref[sp * st.stride] = a_first ? rb : ra;
+        st.t[sp * st.stride] = a_first ? tb : ta;
+        ++sp;
+        cur = a_first ? ra : rb;
+        continue;
+      }
+      if (ha) { cur = ra; continue; }
+      if (hb) { cur = rb; continue; }
+    } else {
+      const uint32_t first = cur & FIRST_MASK;
+      const uint32_t cnt = ((cur >> 27) & 15u) + 1u;
+      for (uint32_t j = 0; j < cnt; ++j) {
+        const uint32_t pos = first + j;
+        float t;
+        if (!q.narrow(s.prims + (size_t)pos * PS, limit, t)) continue;
+        const uint32_t leaf = s.ids[pos];
+        if (FILT && !accepts(f, leaf)) continue;
+        if (ANY) { res.t = t; res.leaf = leaf; res.pos = pos; return res; }
+        if (res.leaf == NO_LEAF || hit_before(t, leaf, res.t, res.leaf)) {
+          res.t = t; res.leaf = leaf; res.pos = pos; limit = t;
+        }
+      }
+    }
+    // pop, skipping entries whose entry lies beyond the current best
+    bool got = false;
+    while (sp > 0) {
+      --sp;
+      if (st.t[sp * st.stride] <= limit) { cur = st.ref[sp * st.stride]; got = true; break; }
+    }
+    if (!got) break;
+  }
+  return res;
+}
+
+// Visit every leaf range whose node chain passes `pair_pred` (fixed
+// predicate, no pruning by results): all-hits, crossings, overlap, shape-cast
+// all.  leaf_fn(first, count) handles one leaf range.
+template <int D, class PairPred, class LeafFn>
+SQ_HD void walk_visit(const SceneView& s, uint32_t root_ref, PairPred&& pair_pred, LeafFn&& leaf_fn, Stack st) {
+  uint32_t cur = root_ref;
+  uint32_t sp = 0;
+  constexpr int NF = (D + 1) * 4;
+  while (true) {
+    if (!(cur & LEAF_BIT)) {
+      const float* nd = s.nodes + (size_t)cur * NF;
+      bool ha, hb;
+      pair_pred(nd, ha, hb);
+      const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
+      if (ha && hb) { st.ref[sp * st.stride] = rb; ++sp; cur = ra; continue; }
+      if (ha) { cur = ra; continue; }
+      if (hb) { cur = rb; continue; }
+    } else {
+      leaf_fn(cur & FIRST_MASK, ((cur >> 27) & 15u) + 1u);
+    }
+    if (sp == 0) break;
+    --sp;
+    cur = st.ref[sp * st.stride];
+  }
+}
+
+// Box overlap (aabb.rs:101-108): touching counts.
+template <int D> SQ_HD bool box_overlap(const float* amn, const float* amx, const float* bmn, const float* bmx) {
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < D; ++k) ok = ok && !(amn[k] > bmx[k] || amx[k] < bmn[k]);
+  return ok;
+}
+
+// Exact overlap narrow phase of the built-in kinds: AABB and TRIANGLE by
+// their AABB; SPHERE by the nearest point of the region within the radius.
+template <int D, int KIND> SQ_HD bool overlap_narrow(const float* p, const float* rmn, const float* rmx) {
+  if constexpr (KIND == KIND_SPHERE) {
+    float d2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const float c = p[k];
+      float v;
+      if (c < rmn[k]) v = rmn[k] - c;
+      else if (c > rmx[k]) v = c - rmx[k];
+      else v = 0.0f;
+      d2 = d2 + v * v;
+    }
+    return SQ_SQRT_RN(d2) <= p[D];
+  } else {
+    float mn[D], mx[D];
+    prim_box<D, KIND>(p, mn, mx);
+    return box_overlap<D>(mn, mx, rmn, rmx);
+  }
+}
+
+// Order-independent minimal work at a fixed limit: every node box that must
+// be tested (root + both children of every passing pair) and every primitive
+// of every passing leaf.  Counted in the reference layout: a reference leaf
+// split into a synthetic chain counts as the one node it is there.
+template <int D>
+SQ_HD void walk_required(const SceneView& s, const RayD<D>& r, float t_limit, Stack st, uint32_t& n_nodes, uint32_t& n_prims) {
+  n_nodes = 1; n_prims = 0;
+  float te, tx; int ax;
+  if (!slab_exact<D>(r, s.root, s.root + D, t_limit, te, tx, ax)) return;
+  walk_visit<D>(s, fbits(s.root[2 * D]),
+    [&](const float* nd, bool& ha, bool& hb) {
+      const bool synth = fbits(nd[4 * D + 3]) != 0;
+      if (synth) { ha = hb = true; return; }
+      n_nodes += 2;
+      float mn[D], mx[D], e, x; int a;
+#pragma unroll
+      for (int k = 0; k < D; ++k) { mn[k] = nd[4 * k + 0]; mx[k] = nd[4 * k + 1]; }
+      ha = slab_exact<D>(r, mn, mx, t_limit, e, x, a);
+#pragma unroll
+      for (int k = 0; k < D; ++k) { mn[k] = nd[4 * k + 2]; mx[k] = nd[4 * k + 3]; }
+      hb = slab_exact<D>(r, mn, mx, t_limit, e, x, a);
+    },
+    [&](uint32_t, uint32_t cnt) { n_prims += cnt; }, st);
+}
+
+// Sort keys of the CSR queries: (toi in total order, leaf) for hits.
+SQ_HD uint64_t hit_key(float t, uint32_t leaf) {
+  return ((uint64_t)((uint32_t)total_key(t) ^ 0x80000000u) << 32) | leaf;
+}
+SQ_HD float key_toi(uint64_t k) {
+  int32_t i = (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
+  i ^= (int32_t)(((uint32_t)(i >> 31)) >> 1);
+  return bitsf((uint32_t)i);
+}
+
+}  // namespace sqd

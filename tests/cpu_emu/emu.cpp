@@ -1,0 +1,203 @@
+// emu.cpp -- TEST-ONLY CPU build of the engine's per-query logic.
+//
+// Compiles spatial-query_amd/csrc/traverse.h and host_tree.cpp with g++ and
+// runs the exact walks the HIP kernels run (same packed tree, same leaf-ordered
+// records), one query per loop iteration with a private stack.  It lets the
+// test suite check the kernel logic against the oracle on the CPU; it is
+// never part of the product and never a fallback for it.
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../spatial-query_amd/csrc/host_tree.h"
+#include "../../spatial-query_amd/csrc/traverse.h"
+
+using namespace sqd;
+
+namespace {
+
+struct Built {
+  sqh::RefTree tree;
+  sqh::Packed packed;
+  std::vector<float> prims;
+  SceneView view() const {
+    SceneView s;
+    s.nodes = packed.nodes.data();
+    s.prims = prims.data();
+    s.ids = tree.prim_indices.data();
+    s.root = packed.root.data();
+    return s;
+  }
+};
+
+void build(int dim, int kind, const float* prims, uint32_t n, Built& b) {
+  std::vector<float> boxes;
+  sqh::prim_boxes(dim, kind, prims, n, boxes);
+  sqh::build_tree(dim, boxes.data(), n, b.tree);
+  sqh::pack_tree(b.tree, b.packed);
+  sqh::pack_prims(dim, kind, prims, b.tree, b.prims);
+}
+
+template <class F> void par_for(uint64_t n, int threads, F&& f) {
+  if (threads <= 1 || n < 1024) { for (uint64_t i = 0; i < n; ++i) f(i); return; }
+  std::vector<std::thread> th;
+  uint64_t chunk = (n + threads - 1) / threads;
+  for (uint64_t lo = 0; lo < n; lo += chunk)
+    th.emplace_back([&, lo] { for (uint64_t i = lo; i < std::min(n, lo + chunk); ++i) f(i); });
+  for (auto& t : th) t.join();
+}
+
+template <int D, int K>
+void nearest_dk(const Built& b, const float* q, uint64_t nq, float max_toi, int any, int cast, int threads,
+                int32_t* leaf, float* toi) {
+  constexpr int PS = Prim<D, K>::STRIDE;
+  const SceneView s = b.view();
+  const uint32_t S = b.packed.max_stack;
+  Filter f{nullptr, ~0ull, -1};
+  par_for(nq, threads, [&](uint64_t i) {
+    std::vector<uint32_t> sr(S);
+    std::vector<float> stt(S);
+    Stack st{sr.data(), stt.data(), 1};
+    NearestResult h;
+    if (cast) {
+      CastPolicy<D, K> pol; pol.setup(q + i * 4 * D);
+      h = walk_nearest<D, PS, false, false>(s, pol, max_toi, f, st);
+    } else {
+      RayPolicy<D, K> pol; pol.setup(q + i * 2 * D);
+      if (any) h = walk_nearest<D, PS, true, false>(s, pol, max_toi, f, st);
+      else h = walk_nearest<D, PS, false, false>(s, pol, max_toi, f, st);
+    }
+    leaf[i] = h.leaf == NO_LEAF ? -1 : (int32_t)h.leaf;
+    toi[i] = h.t;
+  });
+}
+
+template <int D>
+void required_d(const Built& b, const float* rays, uint64_t nq, const float* tl, uint32_t* on, uint32_t* op) {
+  const SceneView s = b.view();
+  const uint32_t S = b.packed.max_stack;
+  for (uint64_t i = 0; i < nq; ++i) {
+    std::vector<uint32_t> sr(S);
+    Stack st{sr.data(), nullptr, 1};
+    RayD<D> r; ray_setup<D>(r, rays + i * 2 * D);
+    walk_required<D>(s, r, tl[i], st, on[i], op[i]);
+  }
+}
+
+template <class F> int dk(int dim, int kind, F&& f) {
+  if (kind == 2) { f.template operator()<3, 2>(); return 0; }
+  switch (dim) {
+    case 2: kind ? f.template operator()<2, 1>() : f.template operator()<2, 0>(); return 0;
+    case 3: kind ? f.template operator()<3, 1>() : f.template operator()<3, 0>(); return 0;
+    case 4: kind ? f.template operator()<4, 1>() : f.template operator()<4, 0>(); return 0;
+    case 5: kind ? f.template operator()<5, 1>() : f.template operator()<5, 0>(); return 0;
+    case 6: kind ? f.template operator()<6, 1>() : f.template operator()<6, 0>(); return 0;
+  }
+  return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* emu_build(int dim, int kind, const float* prims, uint32_t n) {
+  Built* b = new Built();
+  build(dim, kind, prims, n, *b);
+  return b;
+}
+void emu_free(void* h) { delete (Built*)h; }
+uint32_t emu_max_stack(void* h) { return ((Built*)h)->packed.max_stack; }
+uint32_t emu_n_pairs(void* h) { return ((Built*)h)->packed.n_pairs; }
+
+int emu_nearest(void* h, int dim, int kind, const float* q, uint64_t nq, float max_toi, int any, int cast, int threads,
+                int32_t* leaf, float* toi) {
+  const Built& b = *(Built*)h;
+  if (b.tree.prim_indices.empty()) { for (uint64_t i = 0; i < nq; ++i) { leaf[i] = -1; toi[i] = SQ_INF; } return 0; }
+  return dk(dim, kind, [&]<int D, int K>() { nearest_dk<D, K>(b, q, nq, max_toi, any, cast, threads, leaf, toi); });
+}
+
+// mode 0 all, 1 crossings, 2 overlap, 3 shapecast-all.  Returns the total;
+// writes counts[nq] and, when keys != NULL, the unsorted keys per query at
+// their CSR offsets (offsets computed here).
+uint64_t emu_collect(void* h, int dim, int kind, const float* q, uint64_t nq, float max_toi, int mode,
+                     uint64_t* offsets, uint64_t* keys) {
+  const Built& b = *(Built*)h;
+  uint64_t total = 0;
+  offsets[0] = 0;
+  if (b.tree.prim_indices.empty()) { for (uint64_t i = 0; i < nq; ++i) offsets[i + 1] = 0; return 0; }
+  const SceneView s = b.view();
+  const uint32_t S = b.packed.max_stack;
+  Filter f{nullptr, ~0ull, -1};
+  dk(dim, kind, [&]<int D, int K>() {
+    constexpr int PS = Prim<D, K>::STRIDE;
+    for (uint64_t i = 0; i < nq; ++i) {
+      std::vector<uint32_t> sr(S);
+      Stack st{sr.data(), nullptr, 1};
+      std::vector<uint64_t> out;
+      const uint32_t root_ref = fbits(s.root[2 * D]);
+      if (mode == 2) {
+        const float* rg = q + i * 2 * D;
+        if (box_overlap<D>(s.root, s.root + D, rg, rg + D))
+          walk_visit<D>(s, root_ref,
+            [&](const float* nd, bool& ha, bool& hb) {
+              float amn[D], amx[D], bmn[D], bmx[D];
+              for (int k = 0; k < D; ++k) { amn[k] = nd[4 * k]; amx[k] = nd[4 * k + 1]; bmn[k] = nd[4 * k + 2]; bmx[k] = nd[4 * k + 3]; }
+              ha = box_overlap<D>(amn, amx, rg, rg + D); hb = box_overlap<D>(bmn, bmx, rg, rg + D);
+            },
+            [&](uint32_t first, uint32_t c) {
+              for (uint32_t j = 0; j < c; ++j)
+                if (overlap_narrow<D, K>(s.prims + (size_t)(first + j) * PS, rg, rg + D)) out.push_back(s.ids[first + j]);
+            }, st);
+      } else {
+        auto run = [&](auto& pol) {
+          if (!pol.root(s.root, max_toi)) return;
+          walk_visit<D>(s, root_ref,
+            [&](const float* nd, bool& ha, bool& hb) { float ta, tb; pol.pair(nd, max_toi, ha, hb, ta, tb); },
+            [&](uint32_t first, uint32_t c) {
+              for (uint32_t j = 0; j < c; ++j) {
+                const float* p = s.prims + (size_t)(first + j) * PS;
+                float t;
+                if constexpr (K == KIND_AABB) {
+                  if (mode == 1) {
+                    float te, tx; int ax;
+                    if (!slab_exact<D>(pol.r, p, p + D, max_toi, te, tx, ax)) continue;
+                    out.push_back(hit_key(te, s.ids[first + j]));
+                    if (tx > te) {
+                      float te2, tx2;
+                      slab_exact<D>(pol.r, p, p + D, SQ_INF, te2, tx2, ax);
+                      if (tx2 <= max_toi) out.push_back(hit_key(tx2, s.ids[first + j]));
+                    }
+                    continue;
+                  }
+                }
+                if (pol.narrow(p, max_toi, t)) out.push_back(hit_key(t, s.ids[first + j]));
+              }
+            }, st);
+        };
+        if (mode == 3) { CastPolicy<D, K> pol; pol.setup(q + i * 4 * D); run(pol); }
+        else { RayPolicy<D, K> pol; pol.setup(q + i * 2 * D); run(pol); }
+      }
+      offsets[i + 1] = offsets[i] + out.size();
+      if (keys) std::memcpy(keys + offsets[i], out.data(), out.size() * 8);
+    }
+    total = offsets[nq];
+  });
+  return total;
+}
+
+int emu_required(void* h, int dim, const float* rays, uint64_t nq, const float* tl, uint32_t* on, uint32_t* op) {
+  const Built& b = *(Built*)h;
+  switch (dim) {
+    case 2: required_d<2>(b, rays, nq, tl, on, op); return 0;
+    case 3: required_d<3>(b, rays, nq, tl, on, op); return 0;
+    case 4: required_d<4>(b, rays, nq, tl, on, op); return 0;
+    case 5: required_d<5>(b, rays, nq, tl, on, op); return 0;
+    case 6: required_d<6>(b, rays, nq, tl, on, op); return 0;
+  }
+  return 1;
+}
+
+float emu_key_toi(uint64_t k) { return key_toi(k); }
+
+}  // extern "C"
