@@ -260,6 +260,7 @@ enum Kind {
   KIND_SPHERE_PLAIN = 10,    // differential.rs:61-82 exactly (exit root when inside)
   KIND_SPHERE_PLAIN_NOEXIT = 11,  // conformance.rs:277-291 / lib.rs doc example
   KIND_TRI_PLAIN = 12,       // viewport-lib-query MT + max_toi reject, unguarded
+  KIND_SHELL = 13,           // crossings.rs:37-111: both sphere roots are crossings
 };
 
 template <int D> struct Geometry {
@@ -268,6 +269,7 @@ template <int D> struct Geometry {
   std::vector<float> prims;      // BOX: min D, max D; SPHERE: center D, radius; TRI: v0 v1 v2
   std::vector<uint64_t> layers;  // empty = every leaf on every layer
   int stride() const { return kind == KIND_BOX ? 2 * D : (kind == KIND_TRI || kind == KIND_TRI_PLAIN) ? 9 : D + 1; }
+  bool is_ball() const { return kind == KIND_SPHERE || kind == KIND_SPHERE_PLAIN || kind == KIND_SPHERE_PLAIN_NOEXIT || kind == KIND_SHELL; }
   const float* p(size_t leaf) const { return prims.data() + leaf * stride(); }
 
   size_t leaf_count() const { return n; }
@@ -320,8 +322,28 @@ template <int D> struct Geometry {
     return true;
   }
 
+  // crossings.rs:49-61: the two parameters where the unbounded ray meets the sphere.
+  bool roots(const Ray<D>& ray, Point<D> c, float r, float* t0, float* t1) const {
+    Point<D> oc = psub(ray.origin, c);
+    float b = pdot(oc, ray.dir);
+    float disc = b * b - (pdot(oc, oc) - r * r);
+    if (disc < 0.0f) return false;
+    float sq = sqrtf(disc);
+    *t0 = -b - sq; *t1 = -b + sq;
+    return true;
+  }
+
   bool test_ray(size_t leaf, const Ray<D>& ray, float max_toi, LeafHit<D>* out) const {
     const float* q = p(leaf);
+    if (kind == KIND_SHELL) {   // crossings.rs:76-86
+      Point<D> c; for (int i = 0; i < D; ++i) c.v[i] = q[i];
+      float t0, t1;
+      if (!roots(ray, c, q[D], &t0, &t1)) return false;
+      float t = t0 >= 0.0f ? t0 : t1;
+      if (t < 0.0f || t > max_toi) return false;
+      out->toi = t; out->normal = normalize_or_zero(psub(ray.at(t), c)); out->sub = -1;
+      return true;
+    }
     if (kind == KIND_BOX) {
       float te, tx; int axis;
       if (!world_aabb(leaf).ray_intersect(ray, max_toi, &te, &tx, &axis)) return false;
@@ -381,6 +403,20 @@ template <int D> struct Geometry {
   // Crossings (geometry.rs:45-55 default = the single test_ray hit).  Boxes and
   // balls are convex: report entry and exit.
   template <class F> void test_ray_crossings(size_t leaf, const Ray<D>& ray, float max_toi, F&& out) const {
+    if (kind == KIND_SHELL) {   // crossings.rs:87-110
+      const float* q = p(leaf);
+      Point<D> c; for (int i = 0; i < D; ++i) c.v[i] = q[i];
+      float t0, t1;
+      if (!roots(ray, c, q[D], &t0, &t1)) return;
+      float ts[2] = {t0, t1};
+      for (float t : ts) {
+        if (t >= 0.0f && t <= max_toi) {
+          LeafHit<D> h; h.toi = t; h.normal = normalize_or_zero(psub(ray.at(t), c)); h.sub = -1;
+          out(h);
+        }
+      }
+      return;
+    }
     if (kind == KIND_BOX) {
       // entry and exit of the slab interval, each kept when inside [0, max_toi]
       float te, tx; int axis;
@@ -411,7 +447,7 @@ template <int D> struct Geometry {
       out->toi = te; out->normal = Point<D>::zero(); out->sub = -1;
       return true;
     }
-    if (kind == KIND_SPHERE || kind == KIND_SPHERE_PLAIN || kind == KIND_SPHERE_PLAIN_NOEXIT) {
+    if (is_ball()) {
       const float* q = p(leaf);
       Point<D> c; for (int i = 0; i < D; ++i) c.v[i] = q[i];
       float sum = q[D] + cast.aabb.max.v[0];

@@ -1,0 +1,286 @@
+"""Host-side mirror of the reference API over the C ABI.
+
+    Tree.build(dim, kind, prims)        Bvh::build(&G)            bvh.rs:31-37
+    Tree.refit(prims)                   Bvh::refit(&G)            bvh.rs:45-48
+    Tree.stats() / export()             Bvh::stats / nodes()      bvh.rs:465-489
+    DeviceBvh(ctx, tree, prims)         GpuRaycaster::new         gpu/mod.rs:252-291
+    DeviceBvh.raycast_nearest(...)      raycast_nearest_batch     bvh.rs:134-193, gpu/mod.rs:348-463
+    DeviceBvh.raycast_any(...)          nearest(..).is_some()
+    DeviceBvh.raycast_all(...)          raycast_all / _crossings  bvh.rs:196-328
+    DeviceBvh.overlap(...)              overlap                   bvh.rs:418-456
+    DeviceBvh.shapecast_nearest/all     shapecast_*               bvh.rs:331-414
+    DeviceBvh.refit(prims)              GpuRaycaster::refit       gpu/mod.rs:326-341
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from .capi import Filter, SqError, TreeStats, check, lib, SQ_ERR_CAPACITY
+
+ALL_LAYERS = (1 << 64) - 1
+
+
+def prim_stride(dim: int, kind: int) -> int:
+    return {0: 2 * dim, 1: dim + 1, 2: 9}[kind]
+
+
+def _host(a, dtype=np.float32):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _hp(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _dp(t):
+    """Device pointer of a contiguous CUDA tensor (or None)."""
+    if t is None:
+        return None
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError("query buffers must be contiguous GPU tensors")
+    return C.c_void_p(t.data_ptr())
+
+
+@dataclass
+class QueryFilter:
+    """QueryFilter (query/filter.rs:12-29): layer_mask, exclude (None = no exclusion)."""
+    layer_mask: int = ALL_LAYERS
+    exclude: int | None = None
+
+    def c(self):
+        return Filter(self.layer_mask & ALL_LAYERS, -1 if self.exclude is None else int(self.exclude))
+
+
+class Context:
+    """A device and its stream (GpuRaycaster::headless: raises SqError(NO_DEVICE) when absent)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().sq_ctx_create(device, C.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self):
+        check(lib().sq_ctx_synchronize(self._h))
+
+    def stream_ptr(self) -> int:
+        return lib().sq_ctx_stream(self._h) or 0
+
+    def set_block(self, threads: int):
+        check(lib().sq_ctx_set_block(self._h, threads))
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float()
+        check(lib().sq_ctx_last_kernel_ms(self._h, C.byref(ms)))
+        return float(ms.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _stats(s: TreeStats) -> dict:
+    return dict(node_count=s.node_count, leaf_count=s.leaf_count, prim_count=s.prim_count,
+                max_depth=s.max_depth, bytes=s.bytes, quality=float(s.quality))
+
+
+class Tree:
+    """The host tree in the reference layout (Bvh<D>): built by the reference
+    median split, same topology node for node.  Needs no GPU."""
+
+    def __init__(self, handle, dim: int, n_prims: int):
+        self._h, self.dim, self.n_prims = handle, dim, n_prims
+
+    @classmethod
+    def build(cls, dim: int, kind: int, prims) -> "Tree":
+        p = _host(prims).reshape(-1, prim_stride(dim, kind))
+        h = C.c_void_p()
+        check(lib().sq_tree_build(dim, kind, _hp(p), p.shape[0], C.byref(h)))
+        return cls(h, dim, p.shape[0])
+
+    @classmethod
+    def from_nodes(cls, dim, bounds, child_a, child_b, prim_count, prim_indices) -> "Tree":
+        b = _host(bounds); ca = _host(child_a, np.uint32); cb = _host(child_b, np.uint32)
+        pc = _host(prim_count, np.uint32); idx = _host(prim_indices, np.uint32)
+        h = C.c_void_p()
+        check(lib().sq_tree_from_nodes(dim, _hp(b), _hp(ca), _hp(cb), _hp(pc), len(ca), _hp(idx), len(idx), C.byref(h)))
+        return cls(h, dim, len(idx))
+
+    def refit(self, kind: int, prims):
+        p = _host(prims)
+        check(lib().sq_tree_refit(self._h, kind, _hp(p)))
+
+    def stats(self) -> dict:
+        s = TreeStats()
+        check(lib().sq_tree_get_stats(self._h, C.byref(s)))
+        return _stats(s)
+
+    def export(self):
+        nn = self.stats()["node_count"]
+        bounds = np.empty((nn, 2 * self.dim), np.float32)
+        ca = np.empty(nn, np.uint32); cb = np.empty(nn, np.uint32); pc = np.empty(nn, np.uint32)
+        idx = np.empty(self.n_prims, np.uint32)
+        check(lib().sq_tree_export(self._h, _hp(bounds), _hp(ca), _hp(cb), _hp(pc), _hp(idx)))
+        return bounds, ca, cb, pc, idx
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_tree_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBvh:
+    """A tree resident on the GPU with its leaf-ordered primitives."""
+
+    def __init__(self, ctx: Context, tree: Tree, kind: int, prims):
+        p = _host(prims).reshape(-1, prim_stride(tree.dim, kind))
+        h = C.c_void_p()
+        check(lib().sq_bvh_upload(ctx.handle, tree._h, kind, _hp(p), C.byref(h)))
+        self._h, self.ctx, self.dim, self.kind, self.n_prims = h, ctx, tree.dim, kind, p.shape[0]
+
+    @classmethod
+    def build(cls, ctx: Context, dim: int, kind: int, prims) -> "DeviceBvh":
+        t = Tree.build(dim, kind, prims)
+        try:
+            return cls(ctx, t, kind, prims)
+        finally:
+            t.close()
+
+    # -- management ---------------------------------------------------------
+    def set_layers(self, layers):
+        l = None if layers is None else _host(layers, np.uint64)
+        check(lib().sq_bvh_set_layers(self._h, _hp(l)))
+
+    def stats(self) -> dict:
+        s = TreeStats()
+        check(lib().sq_bvh_get_stats(self._h, C.byref(s)))
+        return _stats(s)
+
+    def download_bounds(self):
+        nn = self.stats()["node_count"]
+        out = np.empty((nn, 2 * self.dim), np.float32)
+        check(lib().sq_bvh_download_bounds(self._h, _hp(out)))
+        return out
+
+    def refit(self, prims_dev):
+        check(lib().sq_bvh_refit(self._h, _dp(prims_dev)))
+
+    # -- queries --------------------------------------------------------------
+    def _f(self, filt):
+        return None if filt is None else C.byref(filt.c())
+
+    def raycast_nearest(self, rays, max_toi, max_toi_per_ray=None, filt=None, point=False, normal=False, sub=False):
+        import torch
+        n = rays.shape[0]
+        dev = rays.device
+        leaf = torch.empty(n, dtype=torch.int32, device=dev)
+        toi = torch.empty(n, dtype=torch.float32, device=dev)
+        pt = torch.empty((n, self.dim), dtype=torch.float32, device=dev) if point else None
+        nm = torch.empty((n, self.dim), dtype=torch.float32, device=dev) if normal else None
+        sb = torch.empty(n, dtype=torch.int32, device=dev) if sub else None
+        self.raycast_nearest_into(rays, max_toi, leaf, toi, max_toi_per_ray, filt, pt, nm, sb)
+        out = dict(leaf=leaf, toi=toi)
+        if point: out["point"] = pt
+        if normal: out["normal"] = nm
+        if sub: out["sub"] = sb
+        return out
+
+    def raycast_nearest_into(self, rays, max_toi, leaf, toi, max_toi_per_ray=None, filt=None, point=None, normal=None, sub=None):
+        check(lib().sq_raycast_nearest(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(max_toi_per_ray),
+                                       self._f(filt), _dp(leaf), _dp(toi), _dp(point), _dp(normal), _dp(sub)))
+
+    def raycast_any(self, rays, max_toi, max_toi_per_ray=None, filt=None, out=None):
+        import torch
+        if out is None:
+            out = torch.empty(rays.shape[0], dtype=torch.uint8, device=rays.device)
+        check(lib().sq_raycast_any(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(max_toi_per_ray),
+                                   self._f(filt), _dp(out)))
+        return out
+
+    def _csr(self, fn, n, dev, has_t, capacity):
+        import torch
+        offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        total = C.c_uint64()
+        cap = capacity
+        while True:
+            leaf = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+            toi = torch.empty(max(cap, 1), dtype=torch.float32, device=dev) if has_t else None
+            rc = fn(offsets, leaf, toi, cap, total)
+            if rc == SQ_ERR_CAPACITY:
+                cap = int(total.value)
+                continue
+            check(rc)
+            break
+        t = int(total.value)
+        return offsets, leaf[:t], (toi[:t] if has_t else None)
+
+    def raycast_all(self, rays, max_toi, filt=None, crossings=False, capacity=0):
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_raycast_all(self._h, _dp(rays), rays.shape[0], float(max_toi), self._f(filt), int(crossings),
+                                        _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+        return self._csr(fn, rays.shape[0], rays.device, True, capacity)
+
+    def overlap(self, regions, filt=None, capacity=0):
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_overlap(self._h, _dp(regions), regions.shape[0], self._f(filt), _dp(off), _dp(leaf),
+                                    cap, C.byref(total))
+        off, leaf, _ = self._csr(fn, regions.shape[0], regions.device, False, capacity)
+        return off, leaf
+
+    def shapecast_nearest(self, casts, max_toi, max_toi_per_cast=None, filt=None, point=False):
+        import torch
+        n = casts.shape[0]
+        dev = casts.device
+        leaf = torch.empty(n, dtype=torch.int32, device=dev)
+        toi = torch.empty(n, dtype=torch.float32, device=dev)
+        pt = torch.empty((n, self.dim), dtype=torch.float32, device=dev) if point else None
+        check(lib().sq_shapecast_nearest(self._h, _dp(casts), n, float(max_toi), _dp(max_toi_per_cast), self._f(filt),
+                                         _dp(leaf), _dp(toi), _dp(pt)))
+        out = dict(leaf=leaf, toi=toi)
+        if point: out["point"] = pt
+        return out
+
+    def shapecast_all(self, casts, max_toi, max_toi_per_cast=None, filt=None, capacity=0):
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_shapecast_all(self._h, _dp(casts), casts.shape[0], float(max_toi), _dp(max_toi_per_cast),
+                                          self._f(filt), _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+        return self._csr(fn, casts.shape[0], casts.device, True, capacity)
+
+    def required_work(self, rays, t_limit):
+        import torch
+        n = rays.shape[0]
+        nodes = torch.empty(n, dtype=torch.int32, device=rays.device)
+        prims = torch.empty(n, dtype=torch.int32, device=rays.device)
+        check(lib().sq_required_work(self._h, _dp(rays), n, _dp(t_limit), _dp(nodes), _dp(prims)))
+        return nodes, prims
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_bvh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

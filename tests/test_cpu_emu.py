@@ -1,0 +1,134 @@
+"""The engine's per-query logic (spatial-query_amd/csrc/traverse.h) compiled
+for the CPU and run on the engine's own packed tree, checked against the
+oracle.  Catches traversal and packing mistakes before any GPU run; the GPU
+tests check the same logic on the device."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from sq_amd import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "cpu_emu", "_build", "libsqemu.so")
+P = C.c_void_p
+
+
+@pytest.fixture(scope="module")
+def emu():
+    if not os.path.exists(LIB):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", os.path.join(HERE, "cpu_emu")], check=True)
+    L = C.CDLL(LIB)
+    L.emu_build.restype = P; L.emu_build.argtypes = [C.c_int, C.c_int, P, C.c_uint32]
+    L.emu_free.argtypes = [P]
+    L.emu_nearest.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_int, C.c_int, P, P]
+    L.emu_collect.restype = C.c_uint64
+    L.emu_collect.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, P, P]
+    L.emu_required.argtypes = [P, C.c_int, P, C.c_uint64, P, P, P]
+    return L
+
+
+def p(a):
+    return a.ctypes.data_as(P)
+
+
+class Emu:
+    def __init__(self, L, dim, kind, prims):
+        self.L, self.dim, self.kind = L, dim, kind
+        self.prims = np.ascontiguousarray(prims, np.float32)
+        self.h = L.emu_build(dim, kind, p(self.prims), self.prims.shape[0])
+
+    def nearest(self, q, max_toi, any_=0, cast=0):
+        q = np.ascontiguousarray(q, np.float32)
+        leaf = np.empty(len(q), np.int32); t = np.empty(len(q), np.float32)
+        self.L.emu_nearest(self.h, self.dim, self.kind, p(q), len(q), max_toi, any_, cast, 8, p(leaf), p(t))
+        return leaf.astype(np.int64), t
+
+    def collect(self, q, max_toi, mode):
+        q = np.ascontiguousarray(q, np.float32)
+        off = np.empty(len(q) + 1, np.uint64)
+        tot = self.L.emu_collect(self.h, self.dim, self.kind, p(q), len(q), max_toi, mode, p(off), None)
+        keys = np.empty(tot, np.uint64)
+        self.L.emu_collect(self.h, self.dim, self.kind, p(q), len(q), max_toi, mode, p(off), p(keys))
+        for i in range(len(q)):
+            keys[off[i]:off[i + 1]] = np.sort(keys[off[i]:off[i + 1]])
+        return off, (keys & np.uint64(0xFFFFFFFF)).astype(np.int64)
+
+    def __del__(self):
+        self.L.emu_free(self.h)
+
+
+def eq(a_leaf, a_t, b_leaf, b_t):
+    return np.array_equal(a_leaf, b_leaf) and np.array_equal(a_t.view(np.uint32), b_t.view(np.uint32))
+
+
+def test_c0_nearest_any_required(emu):
+    sc, rays = W.c0()
+    o = oracle.Scene(3, oracle.BOX, sc.prims)
+    ref = o.raycast_nearest(rays, 400.0, threads=8)
+    e = Emu(emu, 3, 0, sc.prims)
+    assert eq(*e.nearest(rays, 400.0), ref["leaf"], ref["t"])
+    assert np.array_equal(e.nearest(rays, 400.0, any_=1)[0] >= 0, ref["leaf"] >= 0)
+    tl = np.where(ref["leaf"] >= 0, ref["t"], np.float32(400.0)).astype(np.float32)
+    on = np.empty(len(rays), np.uint32); op = np.empty(len(rays), np.uint32)
+    emu.emu_required(e.h, 3, p(rays), len(rays), p(tl), p(on), p(op))
+    rn, rp = o.required_work(rays, tl, threads=8)
+    assert np.array_equal(on, rn) and np.array_equal(op, rp)
+
+
+@pytest.mark.parametrize("dim", [2, 3, 4, 6])
+def test_balls_nearest(emu, dim):
+    s = W.Stream(dim)
+    prims = np.concatenate([s.range(-50.0, 50.0, 3000 * dim).reshape(-1, dim), s.range(0.5, 4.0, 3000)[:, None]], 1)
+    rays = W.random_rays(20000, 77 + dim, -60.0, 60.0, dim=dim)
+    ref = oracle.Scene(dim, oracle.SPHERE, prims).raycast_nearest(rays, 500.0, threads=8)
+    assert eq(*Emu(emu, dim, 1, prims).nearest(rays, 500.0), ref["leaf"], ref["t"])
+
+
+def test_heightfield_and_coincident_leaves(emu):
+    sc = W.heightfield_scene(grid=257, extent=128.0)
+    rays = np.concatenate([W.camera_rays(sc, res=256, seed=0x11), W.incoherent_rays(sc, 65536)])
+    ref = oracle.Scene(3, oracle.TRI, sc.prims).raycast_nearest(rays, 2000.0, threads=8)
+    assert (ref["leaf"] >= 0).sum() > 30000
+    assert eq(*Emu(emu, 3, 2, sc.prims).nearest(rays, 2000.0), ref["leaf"], ref["t"])
+    # 100 coincident balls -> one reference leaf, split into synthetic chains
+    prims = np.tile(np.array([[5, 0, 0, 1]], np.float32), (100, 1))
+    prims[::7, 3] = 1.5
+    ray = np.array([[0, 0, 0, 1, 0, 0], [0, 0.5, 0, 1, 0, 0]], np.float32)
+    o = oracle.Scene(3, oracle.SPHERE, prims)
+    ref = o.raycast_nearest(ray, 100.0)
+    e = Emu(emu, 3, 1, prims)
+    assert eq(*e.nearest(ray, 100.0), ref["leaf"], ref["t"])
+    off, leaf = e.collect(ray, 100.0, 0)
+    roff, rleaf, _ = o.raycast_all(ray, 100.0)
+    assert np.array_equal(off, roff)
+    for i in range(2):   # same (toi, leaf) sets; the emu sorts keys
+        assert np.array_equal(np.sort(leaf[off[i]:off[i + 1]]), np.sort(rleaf[roff[i]:roff[i + 1]]))
+
+
+@pytest.mark.parametrize("dim", [3, 6])
+def test_collect_and_casts(emu, dim):
+    sc = W.boxes_scene(n=4000, dim=dim, lo=-30.0, hi=30.0, hmin=0.5, hmax=3.0)
+    q = W.random_rays(2000, 5, -40.0, 40.0, dim=dim)
+    o = oracle.Scene(dim, oracle.BOX, sc.prims)
+    e = Emu(emu, dim, 0, sc.prims)
+    for mode, cross in ((0, False), (1, True)):
+        off, leaf = e.collect(q, 500.0, mode)
+        roff, rleaf, rt = o.raycast_all(q, 500.0, crossings=cross)
+        assert np.array_equal(off, roff)
+        # keys sort by (toi, leaf): the leaf order then matches hit_ordering
+        assert np.array_equal(leaf, rleaf)
+    regions = np.concatenate([q[:, :dim] - 3, q[:, :dim] + 3], 1).astype(np.float32)
+    off, leaf = e.collect(regions, 0.0, 2)
+    roff, rleaf = o.overlap(regions)
+    assert np.array_equal(off, roff) and np.array_equal(leaf, rleaf)
+    half = np.float32(0.5)
+    casts = np.concatenate([q, np.full((len(q), dim), -half), np.full((len(q), dim), half)], 1).astype(np.float32)
+    ref = o.shapecast_nearest(casts, 50.0)
+    assert eq(*e.nearest(casts, 50.0, cast=1), ref["leaf"], ref["t"])
+    off, leaf = e.collect(casts, 50.0, 3)
+    roff, rleaf, _ = o.shapecast_all(casts, 50.0)
+    assert np.array_equal(off, roff) and np.array_equal(leaf, rleaf)

@@ -1,0 +1,296 @@
+"""HIP engine vs the CPU oracle, through the C ABI (bit-exact ids and toi)."""
+import numpy as np
+import pytest
+
+import oracle
+import sq_amd
+from sq_amd import workloads as W
+
+from helpers import assert_nearest_equal, csr_equal, same_bits
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def run_nearest(ctx, dim, kind, prims, rays, max_toi, **kw):
+    b = sq_amd.DeviceBvh.build(ctx, dim, kind, prims)
+    r = b.raycast_nearest(dev(rays), max_toi, point=True, normal=True, sub=True, **kw)
+    ctx.synchronize()
+    return b, {k: host(v) for k, v in r.items()}
+
+
+def ball_scene(dim, n, seed):
+    s = W.Stream(seed)
+    c = s.range(-50.0, 50.0, n * dim).reshape(n, dim)
+    r = s.range(0.5, 4.0, n).reshape(n, 1)
+    return np.concatenate([c, r], 1).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+
+def test_c0_boxes_nearest_any_point_normal(ctx):
+    print("case: C0 10k boxes x 65,536 rays", flush=True)
+    sc, rays = W.c0()
+    o = oracle.Scene(3, oracle.BOX, sc.prims)
+    ref = o.raycast_nearest(rays, 400.0, threads=8)
+    b, got = run_nearest(ctx, 3, sq_amd.AABB, sc.prims, rays, 400.0)
+    assert_nearest_equal(got["leaf"], got["toi"], ref["leaf"], ref["t"], "C0 nearest")
+    hit = ref["leaf"] >= 0
+    assert hit.sum() > 1000 and (~hit).sum() > 1000
+    assert same_bits(got["point"][hit], ref["point"][hit]).all(), "hit points differ"
+    assert same_bits(got["normal"][hit], ref["normal"][hit]).all(), "entry normals differ"
+    assert (got["sub"] == -1).all()
+    anyh = host(b.raycast_any(dev(rays), 400.0))
+    assert np.array_equal(anyh.astype(bool), hit)
+
+
+@pytest.mark.parametrize("dim", [2, 3, 4, 5, 6])
+def test_balls_nearest_each_dimension(ctx, dim):
+    print(f"case: balls D={dim}", flush=True)
+    prims = ball_scene(dim, 4000, 0xBA11 + dim)
+    rays = W.random_rays(30000, 0x5EED + dim, -60.0, 60.0, dim=dim)
+    o = oracle.Scene(dim, oracle.SPHERE, prims)
+    ref = o.raycast_nearest(rays, 500.0, threads=8)
+    b, got = run_nearest(ctx, dim, sq_amd.SPHERE, prims, rays, 500.0)
+    assert_nearest_equal(got["leaf"], got["toi"], ref["leaf"], ref["t"], f"balls D{dim}")
+    hit = ref["leaf"] >= 0
+    assert same_bits(got["normal"][hit], ref["normal"][hit]).all()
+    assert np.array_equal(got["sub"][hit], ref["sub"][hit])
+    anyh = host(b.raycast_any(dev(rays), 500.0))
+    assert np.array_equal(anyh.astype(bool), hit)
+
+
+def test_heightfield_small_nearest_any(ctx):
+    print("case: heightfield 257^2 grid, camera + incoherent rays", flush=True)
+    sc = W.heightfield_scene(grid=257, extent=128.0)
+    rays = np.concatenate([W.camera_rays(sc, res=256, seed=0x11), W.incoherent_rays(sc, 65536)])
+    o = oracle.Scene(3, oracle.TRI, sc.prims)
+    ref = o.raycast_nearest(rays, 2000.0, threads=8)
+    b, got = run_nearest(ctx, 3, sq_amd.TRIANGLE, sc.prims, rays, 2000.0)
+    assert_nearest_equal(got["leaf"], got["toi"], ref["leaf"], ref["t"], "heightfield nearest")
+    hit = ref["leaf"] >= 0
+    assert hit.sum() > 10000
+    assert same_bits(got["normal"][hit], ref["normal"][hit]).all()
+    anyh = host(b.raycast_any(dev(rays), 2000.0))
+    assert np.array_equal(anyh.astype(bool), hit)
+
+
+def test_heightfield_full_scale_c1(ctx):
+    """The 2,097,152-triangle scene with all 8M rays: 100% id agreement."""
+    print("case: C1 full heightfield, 4M coherent + 4M incoherent rays", flush=True)
+    sc, coh, inc = W.c1()
+    rays = np.concatenate([coh, inc])
+    b = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.TRIANGLE, sc.prims)
+    r = b.raycast_nearest(dev(rays), 2000.0)
+    anyh = b.raycast_any(dev(rays), 2000.0)
+    ctx.synchronize()
+    o = oracle.Scene(3, oracle.TRI, sc.prims)
+    ref = o.raycast_nearest(rays, 2000.0, threads=16)
+    assert_nearest_equal(host(r["leaf"]), host(r["toi"]), ref["leaf"], ref["t"], "C1 nearest")
+    assert np.array_equal(host(anyh).astype(bool), ref["leaf"] >= 0)
+
+
+def test_per_ray_max_toi_and_filter(ctx):
+    print("case: per-ray max_toi, layer mask, exclude", flush=True)
+    sc, rays = W.c0(n_rays=20000)
+    tmax = W.Stream(0x7A).range(1.0, 400.0, len(rays))
+    layers = (np.uint64(1) << (np.arange(sc.prims.shape[0], dtype=np.uint64) % np.uint64(3)))
+    b = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.AABB, sc.prims)
+    b.set_layers(layers)
+    o = oracle.Scene(3, oracle.BOX, sc.prims, layers=layers)
+    for mask, excl in [(0b001, None), (0b110, None), ((1 << 64) - 1, 17), (0, None)]:
+        f = sq_amd.QueryFilter(mask, excl)
+        r = b.raycast_nearest(dev(rays), 400.0, filt=f)
+        ref = o.raycast_nearest(rays, 400.0, mask=mask, exclude=-1 if excl is None else excl)
+        assert_nearest_equal(host(r["leaf"]), host(r["toi"]), ref["leaf"], ref["t"], f"filter {mask:#x} {excl}")
+    # per-ray limits: the oracle runs each ray at its own max_toi
+    r = b.raycast_nearest(dev(rays), 400.0, max_toi_per_ray=dev(tmax))
+    b.set_layers(None)
+    o2 = oracle.Scene(3, oracle.BOX, sc.prims)
+    want_leaf = np.empty(2000, np.int64); want_t = np.empty(2000, np.float32)
+    for i in range(2000):
+        rr = o2.raycast_nearest(rays[i:i + 1], float(tmax[i]))
+        want_leaf[i] = rr["leaf"][0]; want_t[i] = rr["t"][0]
+    assert_nearest_equal(host(r["leaf"])[:2000], host(r["toi"])[:2000], want_leaf[:2000], want_t[:2000], "per-ray max_toi")
+
+
+@pytest.mark.parametrize("kind,dim", [(oracle.BOX, 3), (oracle.BOX, 6), (oracle.SPHERE, 3), (oracle.SPHERE, 5)])
+def test_all_hits_crossings(ctx, kind, dim):
+    print(f"case: all-hits / crossings kind={kind} D={dim}", flush=True)
+    if kind == oracle.BOX:
+        prims = W.boxes_scene(n=5000, dim=dim, lo=-30.0, hi=30.0, hmin=0.5, hmax=3.0).prims
+    else:
+        prims = ball_scene(dim, 5000, 0xA11)
+    rays = W.random_rays(8000, 0xA11 + dim, -40.0, 40.0, dim=dim)
+    o = oracle.Scene(dim, kind, prims)
+    b = sq_amd.DeviceBvh.build(ctx, dim, kind, prims)
+    for cross in (False, True):
+        off, leaf, toi = b.raycast_all(dev(rays), 500.0, crossings=cross)
+        roff, rleaf, rt = o.raycast_all(rays, 500.0, crossings=cross)
+        assert len(rleaf) > 100
+        csr_equal(host(off), host(leaf), host(toi), roff, rleaf, rt, f"all cross={cross}")
+
+
+@pytest.mark.parametrize("kind,dim", [(oracle.BOX, 3), (oracle.BOX, 6), (oracle.SPHERE, 4), (oracle.TRI, 3)])
+def test_overlap_sorted_ids(ctx, kind, dim):
+    print(f"case: overlap kind={kind} D={dim}", flush=True)
+    if kind == oracle.BOX:
+        prims = W.boxes_scene(n=20000, dim=dim, lo=-30.0, hi=30.0, hmin=0.2, hmax=2.0).prims
+    elif kind == oracle.SPHERE:
+        prims = ball_scene(dim, 20000, 0x0E)
+    else:
+        prims = W.heightfield_scene(grid=129, extent=64.0).prims
+    s = W.Stream(0x0E + dim)
+    c = s.range(-30.0, 30.0, 3000 * dim).reshape(-1, dim)
+    h = s.range(1.0, 8.0, 3000 * dim).reshape(-1, dim)
+    if kind == oracle.TRI:
+        c[:, 2] = 0.0
+    regions = np.concatenate([c - h, c + h], 1).astype(np.float32)
+    o = oracle.Scene(dim, kind, prims)
+    b = sq_amd.DeviceBvh.build(ctx, dim, kind, prims)
+    off, leaf = b.overlap(dev(regions))
+    roff, rleaf = o.overlap(regions)
+    assert len(rleaf) > 1000
+    csr_equal(host(off), host(leaf), None, roff, rleaf, None, "overlap")
+    # long lists exercise the block sort: one region covering everything
+    big = np.concatenate([np.full((1, dim), -1e4, np.float32), np.full((1, dim), 1e4, np.float32)], 1)
+    off, leaf = b.overlap(dev(big))
+    assert np.array_equal(host(leaf), np.arange(prims.shape[0]))
+
+
+@pytest.mark.parametrize("kind,dim", [(oracle.BOX, 3), (oracle.BOX, 6), (oracle.SPHERE, 3)])
+def test_shapecast_nearest_all(ctx, kind, dim):
+    print(f"case: shapecast kind={kind} D={dim}", flush=True)
+    if kind == oracle.BOX:
+        prims = W.boxes_scene(n=8000, dim=dim, lo=-30.0, hi=30.0, hmin=0.3, hmax=2.0).prims
+    else:
+        prims = ball_scene(dim, 8000, 0x5C)
+    rays = W.random_rays(6000, 0x5C + dim, -40.0, 40.0, dim=dim)
+    half = np.float32(0.5)
+    casts = np.concatenate([rays, np.full((len(rays), dim), -half), np.full((len(rays), dim), half)], 1).astype(np.float32)
+    tmax = W.Stream(0x5D).range(0.0, 50.0, len(rays))
+    tmax = np.maximum(tmax, np.float32(1e-3)).astype(np.float32)
+    o = oracle.Scene(dim, kind, prims)
+    b = sq_amd.DeviceBvh.build(ctx, dim, kind, prims)
+    r = b.shapecast_nearest(dev(casts), 50.0, max_toi_per_cast=dev(tmax), point=True)
+    ref = o.shapecast_nearest(casts, tmax)
+    assert (ref["leaf"] >= 0).sum() > 100
+    assert_nearest_equal(host(r["leaf"]), host(r["toi"]), ref["leaf"], ref["t"], "shapecast nearest")
+    hit = ref["leaf"] >= 0
+    assert same_bits(host(r["point"])[hit], ref["point"][hit]).all()
+    off, leaf, toi = b.shapecast_all(dev(casts), 50.0)
+    roff, rleaf, rt = o.shapecast_all(casts, 50.0)
+    csr_equal(host(off), host(leaf), host(toi), roff, rleaf, rt, "shapecast all")
+
+
+@pytest.mark.parametrize("kind,dim", [(oracle.BOX, 3), (oracle.SPHERE, 6), (oracle.TRI, 3)])
+def test_device_refit_matches_host_refit(ctx, kind, dim):
+    """Bvh::refit on the device == the reference reverse pass, bit for bit; queries after it match."""
+    print(f"case: refit kind={kind} D={dim}", flush=True)
+    import torch
+    if kind == oracle.BOX:
+        prims = W.boxes_scene(n=30000, dim=dim).prims
+    elif kind == oracle.SPHERE:
+        prims = ball_scene(dim, 30000, 0xF1)
+    else:
+        prims = W.heightfield_scene(grid=129, extent=64.0).prims
+    o = oracle.Scene(dim, kind, prims)
+    b = sq_amd.DeviceBvh.build(ctx, dim, kind, prims)
+    s = W.Stream(0xF17)
+    for frame in range(3):
+        if kind == oracle.SPHERE:
+            moved = prims.copy(); moved[:, :dim] += s.range(-5.0, 5.0, prims.shape[0] * dim).reshape(-1, dim)
+        else:
+            step = s.range(-5.0, 5.0, prims.shape[0] * dim).reshape(-1, dim)
+            moved = prims.copy()
+            reps = moved.shape[1] // dim
+            for k in range(reps):
+                moved[:, k * dim:(k + 1) * dim] += step
+        prims = moved.astype(np.float32)
+        b.refit(torch.from_numpy(prims).cuda())
+        o.set_prims(prims)
+        o.refit()
+        got = b.download_bounds()
+        want = o.export()[0]
+        assert np.array_equal(got, want), f"frame {frame}: refit bounds differ"
+        rays = W.random_rays(20000, 0xF0 + frame, -150.0, 150.0, dim=dim)
+        r = b.raycast_nearest(dev(rays), 1000.0)
+        ref = o.raycast_nearest(rays, 1000.0, threads=8)
+        assert_nearest_equal(host(r["leaf"]), host(r["toi"]), ref["leaf"], ref["t"], f"after refit {frame}")
+
+
+def test_required_work_matches_oracle(ctx):
+    print("case: required work counter", flush=True)
+    sc = W.heightfield_scene(grid=257, extent=128.0)
+    rays = W.incoherent_rays(sc, 30000)
+    o = oracle.Scene(3, oracle.TRI, sc.prims)
+    ref = o.raycast_nearest(rays, 2000.0, threads=8)
+    tl = np.where(ref["leaf"] >= 0, ref["t"], np.float32(2000.0)).astype(np.float32)
+    b = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.TRIANGLE, sc.prims)
+    nodes, prims = b.required_work(dev(rays), dev(tl))
+    rn, rp = o.required_work(rays, tl)
+    assert np.array_equal(host(nodes).astype(np.uint64), rn)
+    assert np.array_equal(host(prims).astype(np.uint64), rp)
+
+
+def test_edge_cases(ctx):
+    print("case: empty scene, empty batch, single leaf, coincident centroids, axis-parallel rays", flush=True)
+    import torch
+    # empty scene
+    b = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.SPHERE, np.zeros((0, 4), np.float32))
+    r = b.raycast_nearest(dev(np.array([[0, 0, 0, 1, 0, 0]], np.float32)), 100.0)
+    assert host(r["leaf"])[0] == -1 and np.isinf(host(r["toi"])[0])
+    # empty batch
+    b = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.SPHERE, np.array([[5, 0, 0, 1]], np.float32))
+    r = b.raycast_nearest(dev(np.zeros((0, 6), np.float32)), 100.0)
+    assert r["leaf"].numel() == 0
+    # single leaf: the reference doc example (lib.rs:49-54): toi 4, leaf 0; the away ray misses; max_toi clips
+    rays = np.array([[0, 0, 0, 1, 0, 0], [0, 0, 0, -1, 0, 0]], np.float32)
+    r = b.raycast_nearest(dev(rays), 100.0)
+    assert host(r["leaf"]).tolist() == [0, -1] and host(r["toi"])[0] == np.float32(4.0)
+    assert host(b.raycast_nearest(dev(rays[:1]), 3.0)["leaf"])[0] == -1
+    # coincident centroids: a 40-primitive leaf (split into chains on the device); exact tie -> lowest leaf
+    prims = np.tile(np.array([[5, 0, 0, 1]], np.float32), (40, 1))
+    b = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.SPHERE, prims)
+    assert b.stats()["node_count"] == 1
+    r = b.raycast_nearest(dev(rays[:1]), 100.0)
+    assert host(r["leaf"])[0] == 0
+    off, leaf, toi = b.raycast_all(dev(rays[:1]), 100.0)
+    assert host(leaf).tolist() == list(range(40))
+    # axis-parallel rays through a box grid take the slab's parallel branch
+    sc = W.boxes_scene(n=2000, lo=-20.0, hi=20.0)
+    s = W.Stream(0xAA)
+    o_ = s.range(-25.0, 25.0, 3000 * 3).reshape(-1, 3)
+    axis = (s.u64(3000) % np.uint64(3)).astype(np.int64)
+    sign = np.where(s.u64(3000) % np.uint64(2) == 0, 1.0, -1.0).astype(np.float32)
+    d = np.zeros((3000, 3), np.float32); d[np.arange(3000), axis] = sign
+    rays = np.concatenate([o_, d], 1).astype(np.float32)
+    o = oracle.Scene(3, oracle.BOX, sc.prims)
+    ref = o.raycast_nearest(rays, 400.0)
+    b = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.AABB, sc.prims)
+    r = b.raycast_nearest(dev(rays), 400.0)
+    assert_nearest_equal(host(r["leaf"]), host(r["toi"]), ref["leaf"], ref["t"], "axis-parallel")
+
+
+@pytest.mark.parametrize("block", [64, 128, 512])
+def test_block_size_does_not_change_results(ctx, block):
+    print(f"case: block size {block}", flush=True)
+    sc = W.heightfield_scene(grid=129, extent=64.0)
+    rays = W.incoherent_rays(sc, 20000)
+    b = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.TRIANGLE, sc.prims)
+    base = b.raycast_nearest(dev(rays), 2000.0)
+    ctx.set_block(block)
+    try:
+        r = b.raycast_nearest(dev(rays), 2000.0)
+    finally:
+        ctx.set_block(256)
+    assert np.array_equal(host(r["leaf"]), host(base["leaf"]))
+    assert same_bits(host(r["toi"]), host(base["toi"])).all()

@@ -1,0 +1,113 @@
+"""Host side of the product, no GPU: the C ABI library loads and exports
+every entry point of include/sq.h; Tree.build reproduces the reference
+topology node for node; host refit equals the reference refit; workloads are
+the reference RNG."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+import sq_amd
+from sq_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "sq.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(sq_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = sq_amd.lib()
+    names = declared_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) <= set(sq_amd.capi.SIGNATURES), "binding lacks declarations"
+    assert L.sq_version() >= 100
+
+
+def test_no_device_context_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(sq_amd.SqError) as e:
+        sq_amd.Context(0)
+    assert e.value.code == 2   # SQ_ERR_NO_DEVICE, the reference's `headless` None
+
+
+def _scene(kind, dim, n, seed, coincident=False):
+    s = W.Stream(seed)
+    if kind == 0:
+        c = s.range(-50.0, 50.0, n * dim).reshape(n, dim)
+        if coincident:
+            c[: n // 2] = c[0]
+        h = s.range(0.1, 2.0, n * dim).reshape(n, dim)
+        return np.concatenate([c - h, c + h], 1).astype(np.float32)
+    if kind == 1:
+        c = s.range(-50.0, 50.0, n * dim).reshape(n, dim)
+        if coincident:
+            c[: n // 2] = c[0]
+        return np.concatenate([c, s.range(0.5, 3.0, n)[:, None]], 1).astype(np.float32)
+    return W.heightfield_scene(grid=int(np.sqrt(n / 2)) + 1, extent=64.0).prims
+
+
+@pytest.mark.parametrize("kind,dim,n,coinc", [(0, 3, 10000, False), (0, 6, 30000, False), (1, 2, 5000, True),
+                                              (1, 5, 70000, False), (2, 3, 2 * 128 * 128, False), (0, 3, 3, False),
+                                              (0, 4, 0, False), (1, 3, 100000, True)])
+def test_tree_build_matches_reference_topology(kind, dim, n, coinc):
+    prims = _scene(kind, dim, n, 0xB0 + n, coinc)
+    t = sq_amd.Tree.build(dim, kind, prims)
+    o = oracle.Scene(dim, kind, prims)
+    got, want = t.export(), o.export()
+    names = ["bounds", "child_a", "child_b", "prim_count", "prim_indices"]
+    for nm, a, b in zip(names, got, want):
+        assert np.array_equal(a, b), nm
+    assert t.stats() == pytest.approx(o.tree_stats())
+
+
+@pytest.mark.parametrize("kind,dim", [(0, 3), (1, 4), (2, 3)])
+def test_host_refit_matches_reference(kind, dim):
+    prims = _scene(kind, dim, 20000, 0xCF)
+    t = sq_amd.Tree.build(dim, kind, prims)
+    o = oracle.Scene(dim, kind, prims)
+    moved = prims + W.Stream(0xCE).range(-3.0, 3.0, prims.size).reshape(prims.shape)
+    if kind == 1:
+        moved[:, dim] = prims[:, dim]
+    t.refit(kind, moved)
+    o.set_prims(moved)
+    o.refit()
+    assert np.array_equal(t.export()[0], o.export()[0])
+
+
+def test_tree_from_nodes_validates():
+    prims = _scene(0, 3, 100, 5)
+    o = oracle.Scene(3, 0, prims)
+    b, ca, cb, pc, idx = o.export()
+    t = sq_amd.Tree.from_nodes(3, b, ca, cb, pc, idx)
+    assert t.stats()["node_count"] == len(ca)
+    bad = ca.copy(); bad[0] = 0        # a link that does not point forward
+    with pytest.raises(sq_amd.SqError):
+        sq_amd.Tree.from_nodes(3, b, bad, cb, pc, idx)
+    with pytest.raises(sq_amd.SqError):
+        sq_amd.Tree.build(7, 0, np.zeros((1, 14), np.float32))   # dimension not instantiated
+    with pytest.raises(sq_amd.SqError):
+        sq_amd.Tree.build(2, 2, np.zeros((1, 9), np.float32))    # triangles need D = 3
+
+
+def test_workload_streams_are_the_reference_rng():
+    for seed in [0, 0xABCD_1234, 0xC0FFEE123456789A, (1 << 64) - 1]:
+        assert np.array_equal(W.splitmix(seed, 5000), oracle.rng_u64(seed, 5000))
+        assert np.array_equal(W.frange(W.splitmix(seed, 5000), -50, 50).view(np.uint32),
+                              oracle.rng_range(seed, -50, 50, 5000).view(np.uint32))
+    x = np.random.default_rng(1).standard_normal((20000, 5)).astype(np.float32)
+    assert np.array_equal(W.normalize_or_zero(x).view(np.uint32), oracle.normalize(x).view(np.uint32))
+    d = W.unit_dirs(W.Stream(9), 10000, 3)
+    n2 = (d.astype(np.float64) ** 2).sum(1)
+    assert np.all(np.abs(n2 - 1) < 1e-6)
+    # uniform on the sphere: mean ~ 0, second moments ~ 1/3
+    assert np.all(np.abs(d.mean(0)) < 0.03) and np.all(np.abs((d ** 2).mean(0) - 1 / 3) < 0.02)

@@ -1,0 +1,545 @@
+"""Pins the CPU oracle: the reference's own tests restated (same SplitMix64
+draws, same providers, same assertions), closed forms, and float64 textbook
+reductions that a dropped term or a wrong sign would fail.
+
+Reference tests restated here (crates/spatial-query/tests/):
+  differential.rs   BVH == naive bit-exact at D = 2, 3, 4; edge cases; ties
+  conformance.rs    repeatability; parallel == sequential for every thread count
+  two_level.rs      isometry / transform_aabb closed forms; Tlas == naive;
+                    refit == rebuild; needs_rebuild; Bvh refit == rebuild
+  shape_overlap.rs  swept ball toi 8.25; overlap [0] and [0, 1]; BVH == naive
+  crossings.rs      shell entry/exit 8 and 12; BVH == naive
+  result_identity.rs filter complement
+  stats.rs          tree stats of one-leaf / empty trees; exact counters
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from sq_amd.workloads import Stream, normalize_or_zero
+
+F32 = np.float32
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def ray_new(o, d):
+    return np.concatenate([np.asarray(o, F32), normalize_or_zero(np.asarray(d, F32)[None])[0]])[None, :]
+
+
+# ---------------------------------------------------------------------------
+# differential.rs
+# ---------------------------------------------------------------------------
+
+def _parity(prims, dim, ray, max_toi, kind=oracle.SPHERE_PLAIN):
+    s = oracle.Scene(dim, kind, prims)
+    a = s.raycast_nearest(ray, max_toi)
+    b = s.raycast_nearest(ray, max_toi, naive=True)
+    assert a["leaf"][0] == b["leaf"][0]
+    assert bits(a["t"])[0] == bits(b["t"])[0]
+    oa, la, ta = s.raycast_all(ray, max_toi)
+    ob, lb, tb = s.raycast_all(ray, max_toi, naive=True)
+    assert np.array_equal(la, lb) and np.array_equal(bits(ta), bits(tb))
+    return a, la
+
+
+@pytest.mark.parametrize("dim,seed,n,span", [(3, 0xABCD_1234, 64, 60.0), (2, 0x2222_2222, 48, 60.0), (4, 0x4444_4444, 48, 40.0)])
+def test_bvh_matches_naive(dim, seed, n, span):
+    """differential.rs:120-162: 200 random ball scenes per dimension."""
+    s = Stream(seed)
+    hits = 0
+    for _ in range(200):
+        c = s.range(-50.0, 50.0, n * dim).reshape(n, dim)
+        r = s.range(0.5, 4.0, n)
+        o = s.range(-span, span, dim)
+        d = s.range(-1.0, 1.0, dim)
+        ray = ray_new(o, d)
+        if not ray[0, dim:].any():
+            continue
+        prims = np.concatenate([c, r[:, None]], 1)
+        a, _ = _parity(prims, dim, ray, 500.0)
+        hits += a["leaf"][0] >= 0
+    assert hits > 5 or dim == 4   # the reference D=4 draws are sparse: every ray misses
+
+
+def test_edge_cases():
+    """differential.rs:164-194."""
+    ray = ray_new([0, 0, 0], [1, 0, 0])
+    empty = oracle.Scene(3, oracle.SPHERE_PLAIN, np.zeros((0, 4), F32))
+    assert empty.raycast_nearest(ray, 100.0)["leaf"][0] == -1
+    assert len(empty.raycast_all(ray, 100.0)[1]) == 0
+    one = oracle.Scene(3, oracle.SPHERE_PLAIN, np.array([[5, 0, 0, 1]], F32))
+    h = one.raycast_nearest(ray, 100.0)
+    assert h["leaf"][0] == 0 and abs(h["t"][0] - 4.0) < 1e-4
+    assert one.raycast_nearest(ray_new([0, 0, 0], [-1, 0, 0]), 100.0)["leaf"][0] == -1
+    assert one.raycast_nearest(ray, 3.0)["leaf"][0] == -1
+
+
+def test_ties_sorted_by_leaf():
+    """differential.rs:196-211: equal toi -> leaf order."""
+    s = oracle.Scene(3, oracle.SPHERE_PLAIN, np.array([[5, 0, 0, 1], [5, 0, 0, 1]], F32))
+    off, leaf, t = s.raycast_all(ray_new([0, 0, 0], [1, 0, 0]), 100.0)
+    assert leaf.tolist() == [0, 1] and t[0] <= t[1]
+    assert s.raycast_nearest(ray_new([0, 0, 0], [1, 0, 0]), 100.0)["leaf"][0] == 0
+
+
+def test_doc_example():
+    """lib.rs:21-55: balls at 5 and 10, unit radius; hit id 0 at toi 4."""
+    s = oracle.Scene(3, oracle.SPHERE_PLAIN_NOEXIT, np.array([[5, 0, 0, 1], [10, 0, 0, 1]], F32))
+    h = s.raycast_nearest(ray_new([0, 0, 0], [1, 0, 0]), 100.0)
+    assert h["leaf"][0] == 0 and abs(h["t"][0] - 4.0) < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# conformance.rs
+# ---------------------------------------------------------------------------
+
+def _scatter(n, seed):
+    s = Stream(seed)
+    out = np.empty((n, 4), F32)
+    for i in range(n):
+        out[i, :3] = s.range(-20.0, 20.0, 3)
+        out[i, 3] = s.range(0.3, 1.4, 1)[0]
+    return out
+
+
+def _fan(n, seed):
+    s = Stream(seed)
+    rays = []
+    for _ in range(n):
+        o = s.range(-30.0, 30.0, 3)
+        tgt = s.range(-8.0, 8.0, 3)
+        rays.append(ray_new(o, tgt - o)[0])
+    return np.array(rays, F32)
+
+
+def test_repeatable_and_parallel_equals_sequential():
+    """conformance.rs:106-181."""
+    balls = oracle.Scene(3, oracle.SPHERE_PLAIN_NOEXIT, _scatter(800, 0x1234_5678_9ABC_DEF0))
+    rays = _fan(500, 0xFACE_B00C_1357_9BDF)
+    seq = balls.raycast_nearest(rays, 200.0)
+    assert (seq["leaf"] >= 0).any() and (seq["leaf"] < 0).any()
+    for _ in range(20):
+        again = balls.raycast_nearest(rays, 200.0)
+        assert np.array_equal(again["leaf"], seq["leaf"]) and np.array_equal(bits(again["t"]), bits(seq["t"]))
+    for threads in [1, 2, 3, 4, 7, 8, 16, 999]:
+        par = balls.raycast_nearest(rays, 200.0, threads=threads)
+        for k in ("leaf", "sub"):
+            assert np.array_equal(par[k], seq[k])
+        for k in ("t", "point", "normal"):
+            assert np.array_equal(bits(par[k]), bits(seq[k]))
+    # single rays equal the batch
+    for i in range(0, 500, 37):
+        one = balls.raycast_nearest(rays[i:i + 1], 200.0)
+        assert one["leaf"][0] == seq["leaf"][i] and bits(one["t"])[0] == bits(seq["t"])[i]
+    empty = balls.raycast_nearest(np.zeros((0, 6), F32), 200.0, threads=8)
+    assert len(empty["leaf"]) == 0
+
+
+# ---------------------------------------------------------------------------
+# two_level.rs
+# ---------------------------------------------------------------------------
+
+def plane_rotation(dim, a, b, angle):
+    """isometry.rs:129-140 (sin/cos in f32 of the f32 angle)."""
+    r = np.eye(dim, dtype=F32)
+    sn, cs = F32(math.sin(float(angle))), F32(math.cos(float(angle)))
+    r[a, a] = cs; r[a, b] = -sn; r[b, a] = sn; r[b, b] = cs
+    return r
+
+
+def test_isometry_closed_forms():
+    """two_level.rs:180-210."""
+    rot = plane_rotation(3, 0, 1, math.pi / 2)
+    t0 = np.zeros(3, F32)
+    r = oracle.iso_apply(rot, t0, np.array([1, 0, 0], F32), 1)
+    assert abs(r[0]) < 1e-6 and abs(r[1] - 1) < 1e-6 and abs(r[2]) < 1e-6
+    back = oracle.iso_apply(rot, t0, r, 3)
+    assert abs(back[0] - 1) < 1e-6 and abs(back[1]) < 1e-6
+    rot = plane_rotation(3, 0, 1, math.pi / 4)
+    w = oracle.iso_transform_aabb(rot, np.array([10, 0, 0], F32), np.array([-.5, -.5, -.5, .5, .5, .5], F32))
+    ext = w[3:] - w[:3]
+    diag = 2.0 * (1 / math.sqrt(2))
+    assert abs(ext[0] - diag) < 1e-5 and abs(ext[1] - diag) < 1e-5 and abs(ext[2] - 1) < 1e-5
+    assert abs((w[0] + w[3]) * 0.5 - 10) < 1e-5
+    # inverse_transform_point(transform_point(p)) == p (orthonormal R) within rounding
+    p = np.array([1.5, -2.0, 3.25], F32)
+    tr = np.array([4, 5, 6], F32)
+    q = oracle.iso_apply(rot, tr, oracle.iso_apply(rot, tr, p, 0), 2)
+    assert np.allclose(q, p, atol=1e-5)
+
+
+def _random_boxes(s, dim, n):
+    """two_level.rs:115-131."""
+    rot = np.empty((n, dim, dim), F32); tr = np.empty((n, dim), F32); half = np.empty((n, dim), F32)
+    for i in range(n):
+        tr[i] = s.range(-40.0, 40.0, dim)
+        a = int(s.u64(1)[0] % np.uint64(dim))
+        b = int(s.u64(1)[0] % np.uint64(dim))
+        if b == a:
+            b = (b + 1) % dim
+        rot[i] = plane_rotation(dim, a, b, s.range(0.0, 2 * math.pi, 1)[0])
+        half[i] = np.maximum(s.range(0.5, 2.5, dim), F32(0.5))
+    return rot, tr, half
+
+
+def test_end_to_end_box_instance():
+    """two_level.rs:212-224: unit box at x = 5 -> toi 4, point x 4, normal -x."""
+    sc = oracle.InstanceScene(3, oracle.INST_BOX, np.eye(3, dtype=F32)[None], np.array([[5, 0, 0]], F32),
+                              halves=np.ones((1, 3), F32))
+    h = sc.raycast_nearest(ray_new([0, 0, 0], [1, 0, 0]), 100.0)
+    assert h["leaf"][0] == 0 and abs(h["t"][0] - 4) < 1e-4
+    assert abs(h["point"][0, 0] - 4) < 1e-4 and abs(h["normal"][0, 0] + 1) < 1e-4
+
+
+@pytest.mark.parametrize("dim,seed,n", [(3, 0x7ABC_1234, 64), (2, 0x2020_2020, 48)])
+def test_tlas_matches_naive(dim, seed, n):
+    """two_level.rs:263-292."""
+    s = Stream(seed)
+    hits = 0
+    for _ in range(200):
+        rot, tr, half = _random_boxes(s, dim, n)
+        ray = ray_new(s.range(-60.0, 60.0, dim), s.range(-1.0, 1.0, dim))
+        sc = oracle.InstanceScene(dim, oracle.INST_BOX, rot, tr, halves=half)
+        a = sc.raycast_nearest(ray, 500.0)
+        b = sc.raycast_nearest(ray, 500.0, naive=True)
+        assert a["leaf"][0] == b["leaf"][0] and bits(a["t"])[0] == bits(b["t"])[0]
+        off, leaf, t, _ = sc.raycast_all(ray, 500.0)
+        order = np.lexsort((leaf, t))   # naive_all sorts by (toi, leaf)
+        assert np.array_equal(order, np.arange(len(leaf)))
+        hits += a["leaf"][0] >= 0
+    assert hits > 0
+
+
+def test_tlas_refit_matches_rebuild_and_naive():
+    """two_level.rs:294-342."""
+    s = Stream(0xF00D_5EED)
+    rot, tr, half = _random_boxes(s, 3, 200)
+    sc = oracle.InstanceScene(3, oracle.INST_BOX, rot, tr, halves=half)
+    for frame in range(10):
+        for i in range(200):
+            tr[i] = tr[i] + s.range(-1.5, 1.5, 3)
+            if i % 7 == 0:
+                half[i] = half[i] * F32(1.05)
+        sc.set_transforms(rot, tr, half)
+        sc.refit()
+        fresh = oracle.InstanceScene(3, oracle.INST_BOX, rot, tr, halves=half)
+        rays = []
+        for _ in range(40):
+            rays.append(ray_new(s.range(-60.0, 60.0, 3), s.range(-1.0, 1.0, 3))[0])
+        rays = np.array(rays, F32)
+        a = sc.raycast_nearest(rays, 500.0)
+        b = fresh.raycast_nearest(rays, 500.0)
+        c = sc.raycast_nearest(rays, 500.0, naive=True)
+        for x in (b, c):
+            assert np.array_equal(a["leaf"], x["leaf"]) and np.array_equal(bits(a["t"]), bits(x["t"]))
+
+
+def test_needs_rebuild_after_spreading():
+    """two_level.rs:344-372."""
+    s = Stream(0xCAFE_1111)
+    n = 128
+    tr = s.range(-2.0, 2.0, n * 3).reshape(n, 3)
+    rot = np.tile(np.eye(3, dtype=F32), (n, 1, 1))
+    sc = oracle.InstanceScene(3, oracle.INST_BOX, rot, tr, halves=np.full((n, 3), 0.5, F32))
+    assert not sc.needs_rebuild(2.0)
+    sc.set_transforms(rot, s.range(-500.0, 500.0, n * 3).reshape(n, 3))
+    sc.refit()
+    assert sc.needs_rebuild(2.0)
+
+
+def test_bvh_refit_matches_rebuild():
+    """two_level.rs:405-432 (unit balls, 150 centres, 8 frames)."""
+    s = Stream(0x1357_9BDF)
+    c = s.range(-30.0, 30.0, 150 * 3).reshape(150, 3)
+    prims = np.concatenate([c, np.ones((150, 1), F32)], 1)
+    sc = oracle.Scene(3, oracle.SPHERE_PLAIN_NOEXIT, prims)
+    for _ in range(8):
+        for i in range(150):
+            prims[i, :3] = prims[i, :3] + s.range(-2.0, 2.0, 3)
+        sc.set_prims(prims)
+        sc.refit()
+        fresh = oracle.Scene(3, oracle.SPHERE_PLAIN_NOEXIT, prims)
+        rays = np.array([ray_new(s.range(-40.0, 40.0, 3), s.range(-1.0, 1.0, 3))[0] for _ in range(40)], F32)
+        a = sc.raycast_nearest(rays, 300.0)
+        b = fresh.raycast_nearest(rays, 300.0)
+        assert np.array_equal(a["leaf"], b["leaf"]) and np.array_equal(bits(a["t"]), bits(b["t"]))
+        # and the refit bounds equal the rebuilt tree's where the topology agrees: the root
+        assert np.array_equal(sc.export()[0][0], fresh.export()[0][0])
+
+
+# ---------------------------------------------------------------------------
+# shape_overlap.rs, crossings.rs, result_identity.rs, stats.rs
+# ---------------------------------------------------------------------------
+
+def ball_cast(o, d, probe=0.75):
+    d = normalize_or_zero(np.asarray(d, F32)[None])[0]
+    return np.concatenate([np.asarray(o, F32), d, np.full(3, -probe, F32), np.full(3, probe, F32)])[None]
+
+
+def test_swept_ball_toi():
+    """shape_overlap.rs:184-230: probe 0.75 vs ball r 1 at x 10 -> toi 8.25, normal -x."""
+    sc = oracle.Scene(3, oracle.SPHERE_PLAIN, np.array([[10, 0, 0, 1]], F32))
+    h = sc.shapecast_nearest(ball_cast([0, 0, 0], [1, 0, 0]), 100.0)
+    assert h["leaf"][0] == 0 and abs(h["t"][0] - 8.25) < 1e-4 and abs(h["normal"][0, 0] + 1) < 1e-4
+    # fine-step march (the reference cross-check)
+    t = 0.0
+    while math.dist((t, 0, 0), (10, 0, 0)) > 1.75:
+        t += 0.001
+    assert abs(h["t"][0] - t) < 0.002
+
+
+def test_overlap_known_answers():
+    """shape_overlap.rs:236-262."""
+    sc = oracle.Scene(3, oracle.SPHERE_PLAIN, np.array([[0, 0, 0, 1], [5, 0, 0, 1], [100, 0, 0, 1]], F32))
+    off, leaf = sc.overlap(np.array([[-1.5, -1.5, -1.5, 1.5, 1.5, 1.5]], F32))
+    assert leaf.tolist() == [0]
+    off, leaf = sc.overlap(np.array([[-1.5, -1.5, -1.5, 4.0, 1.5, 1.5]], F32))
+    assert leaf.tolist() == [0, 1]
+
+
+def test_shapecast_and_overlap_match_naive():
+    """shape_overlap.rs:153-182, 264-285."""
+    s = Stream(0x5EED_C0DE)
+    for _ in range(100):
+        c = s.range(-30.0, 30.0, 64 * 3).reshape(64, 3)
+        r = s.range(0.5, 3.0, 64)
+        prims = np.concatenate([c, r[:, None]], 1)
+        sc = oracle.Scene(3, oracle.SPHERE_PLAIN, prims)
+        cast = ball_cast(s.range(-50.0, 50.0, 3), s.range(-1.0, 1.0, 3))
+        a = sc.shapecast_nearest(cast, 500.0); b = sc.shapecast_nearest(cast, 500.0, naive=True)
+        assert a["leaf"][0] == b["leaf"][0] and bits(a["t"])[0] == bits(b["t"])[0]
+        oa, la, ta = sc.shapecast_all(cast, 500.0); ob, lb, tb = sc.shapecast_all(cast, 500.0, naive=True)
+        assert np.array_equal(la, lb) and np.array_equal(bits(ta), bits(tb))
+        ctr = s.range(-30.0, 30.0, 3); h = s.range(1.0, 6.0, 3)
+        reg = np.concatenate([ctr - h, ctr + h])[None]
+        assert np.array_equal(sc.overlap(reg)[1], sc.overlap(reg, naive=True)[1])
+
+
+def test_crossings_entry_exit():
+    """crossings.rs:113-135: a shell of radius 2 at x 10 is crossed at 8 and 12."""
+    sc = oracle.Scene(3, oracle.SHELL, np.array([[10, 0, 0, 2]], F32))
+    off, leaf, t = sc.raycast_all(ray_new([0, 0, 0], [1, 0, 0]), 100.0, crossings=True)
+    assert leaf.tolist() == [0, 0] and abs(t[0] - 8) < 1e-4 and abs(t[1] - 12) < 1e-4
+    # default crossings (test_ray_crossings not overridden) equal all-hits
+    b = oracle.Scene(3, oracle.SPHERE_PLAIN, np.array([[10, 0, 0, 2]], F32))
+    assert np.array_equal(b.raycast_all(ray_new([0, 0, 0], [1, 0, 0]), 100.0, crossings=True)[1],
+                          b.raycast_all(ray_new([0, 0, 0], [1, 0, 0]), 100.0)[1])
+
+
+@pytest.mark.parametrize("dim,seed", [(3, 0xC055_1234), (2, 0x2C05_5222)])
+def test_crossings_match_naive(dim, seed):
+    """crossings.rs:137-200."""
+    s = Stream(seed)
+    for _ in range(100):
+        c = s.range(-30.0, 30.0, 48 * dim).reshape(48, dim)
+        r = s.range(1.0, 4.0, 48)
+        sc = oracle.Scene(dim, oracle.SHELL, np.concatenate([c, r[:, None]], 1))
+        ray = ray_new(s.range(-40.0, 40.0, dim), s.range(-1.0, 1.0, dim))
+        oa, la, ta = sc.raycast_all(ray, 500.0, crossings=True)
+        ob, lb, tb = sc.raycast_all(ray, 500.0, crossings=True, naive=True)
+        assert np.array_equal(la, lb) and np.array_equal(bits(ta), bits(tb))
+
+
+def test_filter_selects_complement():
+    """result_identity.rs:123-163: even leaves on layer bit 0, odd on bit 1."""
+    n = 12
+    prims = np.array([[2.0 * (i + 1), 0, 0, 0.6] for i in range(n)], F32)
+    layers = np.array([1 << (i % 2) for i in range(n)], np.uint64)
+    sc = oracle.Scene(3, oracle.SPHERE_PLAIN, prims, layers=layers)
+    ray = ray_new([0, 0, 0], [1, 0, 0])
+    even = sc.raycast_all(ray, 100.0, mask=0b01)[1].tolist()
+    odd = sc.raycast_all(ray, 100.0, mask=0b10)[1].tolist()
+    both = sc.raycast_all(ray, 100.0)[1].tolist()
+    assert even == list(range(0, n, 2)) and odd == list(range(1, n, 2)) and both == list(range(n))
+    assert sc.raycast_nearest(ray, 100.0, mask=0b10)["leaf"][0] == 1
+    assert sc.raycast_nearest(ray, 100.0, exclude=0)["leaf"][0] == 1
+    reg = np.array([[-100, -1, -1, 100, 1, 1]], F32)
+    assert sc.overlap(reg, mask=0b01)[1].tolist() == list(range(0, n, 2))
+
+
+def test_tree_stats_and_counters():
+    """stats.rs:62-185 (balls of radius 0.5 in a line at x = 2(i+1))."""
+    line = lambda n: np.array([[2.0 * (i + 1), 0, 0, 0.5] for i in range(n)], F32)
+    s = oracle.Scene(3, oracle.SPHERE_PLAIN_NOEXIT, line(3)).tree_stats()
+    assert (s["node_count"], s["leaf_count"], s["prim_count"], s["max_depth"]) == (1, 1, 3, 1) and s["bytes"] > 0
+    s = oracle.Scene(3, oracle.SPHERE_PLAIN_NOEXIT, line(50)).tree_stats()
+    assert s["prim_count"] == 50 and 1 <= s["leaf_count"] < s["node_count"] and 1 <= s["max_depth"] <= s["node_count"]
+    s = oracle.Scene(3, oracle.SPHERE_PLAIN_NOEXIT, line(0)).tree_stats()
+    assert (s["node_count"], s["leaf_count"], s["prim_count"], s["max_depth"]) == (0, 0, 0, 0)
+    sc = oracle.Scene(3, oracle.SPHERE_PLAIN_NOEXIT, line(3))
+    st = sc.raycast_nearest(ray_new([0, 0, 0], [1, 0, 0]), 100.0, stats=True)["stats"][0]
+    # one node, one aabb test, three narrow tests, three hits (each closer... no: nearest keeps testing all)
+    assert st.tolist()[:3] == [1, 1, 3]
+    miss = sc.raycast_nearest(ray_new([0, 5, 0], [1, 0, 0]), 100.0, stats=True)["stats"][0]
+    assert miss.tolist() == [1, 1, 0, 0]
+
+
+# ---------------------------------------------------------------------------
+# narrow phases and the slab against closed forms / float64 reductions
+# ---------------------------------------------------------------------------
+
+def test_slab_closed_forms():
+    box = np.array([1, 1, 1, 3, 3, 3], F32)
+    assert oracle.ray_intersect(box, np.array([0, 2, 2, 1, 0, 0], F32), 100.0) == (1.0, 3.0)
+    assert oracle.ray_intersect(box, np.array([0, 2, 2, 1, 0, 0], F32), 0.5) is None      # max_toi clips
+    assert oracle.ray_intersect(box, np.array([2, 2, 2, 1, 0, 0], F32), 100.0) == (0.0, 1.0)   # inside: t_min 0
+    assert oracle.ray_intersect(box, np.array([0, 5, 2, 1, 0, 0], F32), 100.0) is None      # parallel, outside
+    assert oracle.ray_intersect(box, np.array([0, 3, 2, 1, 0, 0], F32), 100.0) == (1.0, 3.0)  # parallel, on face
+
+
+def test_slab_matches_float64_intervals():
+    rng = np.random.default_rng(3)
+    for _ in range(2000):
+        c = rng.uniform(-10, 10, 3); h = rng.uniform(0.1, 3, 3)
+        box = np.concatenate([c - h, c + h]).astype(F32)
+        o = rng.uniform(-20, 20, 3).astype(F32)
+        d = normalize_or_zero(rng.standard_normal((1, 3)).astype(F32))[0]
+        got = oracle.ray_intersect(box, np.concatenate([o, d]), 1000.0)
+        b64, o64, d64 = box.astype(np.float64), o.astype(np.float64), d.astype(np.float64)
+        t1 = (b64[:3] - o64) / d64; t2 = (b64[3:] - o64) / d64
+        lo = max(0.0, np.max(np.minimum(t1, t2))); hi = min(1000.0, np.min(np.maximum(t1, t2)))
+        if hi - lo > 1e-3:
+            assert got is not None and abs(got[0] - lo) < 1e-4 * max(1, lo) and abs(got[1] - hi) < 1e-4 * max(1, hi)
+        elif lo - hi > 1e-3:
+            assert got is None
+
+
+def test_moeller_trumbore_against_float64_plane_hit():
+    """A textbook reduction: plane intersection + barycentric inside test in float64."""
+    rng = np.random.default_rng(11)
+    tris = rng.uniform(-5, 5, (400, 9)).astype(F32)
+    sc = oracle.Scene(3, oracle.TRI_PLAIN, tris, build_tree=False)
+    checked = 0
+    for i in range(400):
+        v0, v1, v2 = tris[i, :3].astype(np.float64), tris[i, 3:6].astype(np.float64), tris[i, 6:].astype(np.float64)
+        ctr = (v0 + v1 + v2) / 3
+        for _ in range(5):
+            o = rng.uniform(-15, 15, 3).astype(F32)
+            tgt = ctr + rng.uniform(-3, 3, 3)
+            d = normalize_or_zero((tgt - o).astype(F32)[None])[0]
+            n = np.cross(v1 - v0, v2 - v0)
+            den = n @ d
+            if abs(den) < 1e-3:
+                continue
+            t = n @ (v0 - o) / den
+            p = o + t * d
+            # barycentric in float64
+            m = np.stack([v1 - v0, v2 - v0], 1)
+            uv, *_ = np.linalg.lstsq(m, p - v0, rcond=None)
+            inside = uv[0] >= 0 and uv[1] >= 0 and uv.sum() <= 1 and t > 1e-6
+            margin = min(uv[0], uv[1], 1 - uv.sum())
+            single = oracle.Scene(3, oracle.TRI_PLAIN, tris[i:i + 1])
+            h = single.raycast_nearest(np.concatenate([o, d])[None], 1000.0, naive=True)
+            if abs(margin) < 1e-4:
+                continue
+            checked += 1
+            if inside:
+                assert h["leaf"][0] == 0 and abs(h["t"][0] - t) < 1e-4 * max(1, abs(t))
+            else:
+                assert h["leaf"][0] == -1
+    assert checked > 1000
+
+
+def test_guard_only_drops_hits_outside_their_box():
+    """Guarded and plain kinds agree except where the plain hit lies outside
+    the primitive's own slab interval (so the guard cannot hide a real hit)."""
+    from sq_amd import workloads as W
+    sc = W.heightfield_scene(grid=257, extent=128.0)
+    rays = W.incoherent_rays(sc, 100000)
+    g = oracle.Scene(3, oracle.TRI, sc.prims)
+    p = oracle.Scene(3, oracle.TRI_PLAIN, sc.prims)
+    a = g.raycast_nearest(rays, 2000.0, threads=8)
+    b = p.raycast_nearest(rays, 2000.0, threads=8)
+    diff = (a["leaf"] != b["leaf"]) | (bits(a["t"]) != bits(b["t"]))
+    assert diff.mean() < 1e-3
+    # guarded BVH == guarded naive (the order-independence the device relies on)
+    sub = rays[:3000]
+    n = g.raycast_nearest(sub, 2000.0, naive=True)
+    assert np.array_equal(n["leaf"], a["leaf"][:3000]) and np.array_equal(bits(n["t"]), bits(a["t"][:3000]))
+
+
+# ---------------------------------------------------------------------------
+# tree build and the required-work counter
+# ---------------------------------------------------------------------------
+
+def _total_key(f):
+    i = np.asarray(f, F32).view(np.int32).astype(np.int64)
+    return i ^ ((i >> 31) & 0x7FFFFFFF)
+
+
+def _py_build(boxes):
+    """tree.rs:112-163 from its prose, in Python: returns the node list."""
+    dim = boxes.shape[1] // 2
+    cent = ((boxes[:, :dim] + boxes[:, dim:]) * F32(0.5)).astype(F32)
+    idx = list(range(len(boxes)))
+    nodes = []
+
+    def rec(start, end):
+        sel = np.array(idx[start:end])
+        b = np.concatenate([boxes[sel, :dim].min(0), boxes[sel, dim:].max(0)])
+        me = len(nodes)
+        nodes.append([b, start, 0, end - start])
+        if end - start <= 4:
+            return me
+        cmin, cmax = cent[sel].min(0), cent[sel].max(0)
+        ext = (cmax - cmin).astype(F32)
+        axis = int(np.argmax(ext))          # first maximum
+        if ext[axis] <= F32(1.1920929e-7):
+            return me
+        idx[start:end] = sorted(idx[start:end], key=lambda i: _total_key(cent[i, axis]))
+        mid = start + (end - start) // 2
+        a = rec(start, mid)
+        c = rec(mid, end)
+        nodes[me][1:] = [a, c, 0]
+        return me
+
+    if len(boxes):
+        rec(0, len(boxes))
+    return nodes, idx
+
+
+@pytest.mark.parametrize("n,dim,seed", [(1, 3, 1), (5, 3, 2), (37, 2, 3), (200, 3, 4), (333, 6, 5)])
+def test_build_matches_prose_restatement(n, dim, seed):
+    s = Stream(seed)
+    c = s.range(-10.0, 10.0, n * dim).reshape(n, dim)
+    if n > 100:
+        c[: n // 3] = c[0]                  # coincident centroids -> an over-full leaf
+    h = s.range(0.1, 1.0, n * dim).reshape(n, dim)
+    boxes = np.concatenate([c - h, c + h], 1).astype(F32)
+    sc = oracle.Scene(dim, oracle.BOX, boxes)
+    bounds, ca, cb, pc, idx = sc.export()
+    nodes, pidx = _py_build(boxes)
+    assert len(nodes) == len(ca)
+    assert idx.tolist() == pidx
+    for i, (b, a, c_, p) in enumerate(nodes):
+        assert np.array_equal(bounds[i], b) and (ca[i], cb[i], pc[i]) == (a, c_, p)
+
+
+def test_required_work_brute_force():
+    """N_node = 1 + 2 (#interior nodes whose path passes), N_prim = prims of passing leaves."""
+    s = Stream(77)
+    boxes = np.concatenate([c := s.range(-20.0, 20.0, 300 * 3).reshape(300, 3), c + 1.0], 1).astype(F32)
+    sc = oracle.Scene(3, oracle.BOX, boxes)
+    bounds, ca, cb, pc, idx = sc.export()
+    rays = np.concatenate([s.range(-30.0, 30.0, 200 * 3).reshape(200, 3),
+                           normalize_or_zero(s.range(-1.0, 1.0, 200 * 3).reshape(200, 3))], 1).astype(F32)
+    tl = s.range(1.0, 80.0, 200)
+    nn, npr = sc.required_work(rays, tl)
+    for r in range(200):
+        want_n, want_p = 1, 0
+        stack = [0]
+        while stack:
+            i = stack.pop()
+            if oracle.ray_intersect(bounds[i], rays[r], float(tl[r])) is None:
+                continue
+            if pc[i]:
+                want_p += int(pc[i])
+            else:
+                want_n += 2
+                stack += [int(ca[i]), int(cb[i])]
+        assert (nn[r], npr[r]) == (want_n, want_p)
