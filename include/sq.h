@@ -79,7 +79,14 @@ int sq_version(void);
 
 /* ---- context: GpuRaycaster::headless / new (gpu/mod.rs:252-324) ---------- */
 int sq_ctx_create(int device, sq_ctx** out);   /* SQ_ERR_NO_DEVICE when absent */
+/* Releases the caller's handle.  Trees built on the context keep it alive
+ * until they are destroyed too, in any order. */
 int sq_ctx_destroy(sq_ctx* ctx);
+/* Enqueue all further work on `stream` (a hipStream_t of the same device, e.g.
+ * the caller's framework stream; NULL is HIP's legacy default stream) instead
+ * of the context's own stream.  sq_ctx_reset_stream goes back to its own. */
+int sq_ctx_set_stream(sq_ctx* ctx, void* stream);
+int sq_ctx_reset_stream(sq_ctx* ctx);
 int sq_ctx_synchronize(sq_ctx* ctx);
 /* The hipStream_t all work of this context is enqueued on. */
 void* sq_ctx_stream(sq_ctx* ctx);
@@ -170,6 +177,46 @@ int sq_shapecast_all(sq_bvh* bvh, const float* casts, uint64_t n, float max_toi,
  * counts the same events in its own traversal order. */
 int sq_required_work(sq_bvh* bvh, const float* rays, uint64_t n, const float* t_limit, uint32_t* out_nodes,
                      uint32_t* out_prims);
+
+/* ---- two-level tree: Tlas (accel/tlas.rs) -------------------------------- */
+/* Built-in instance kinds (InstancedGeometry, query/instance.rs:19-106):
+ *   SQ_INST_BOX   local box [-half, half] (any D), the reference's
+ *                 ray_local_box provider (tests/two_level.rs:70-113);
+ *   SQ_INST_MESH  a uniformly scaled triangle mesh (D = 3): its local geometry
+ *                 is the mesh scaled by `scale`; the local ray is taken into the
+ *                 unscaled mesh frame by o/scale, d/scale (toi unchanged) and
+ *                 answered by the mesh's own device BVH (a TRIANGLE sq_bvh).
+ * An instance hit is kept only when its toi lies in the slab interval of the
+ * instance's world AABB (order independence, as for the primitive kinds). */
+#define SQ_INST_BOX 0
+#define SQ_INST_MESH 1
+/* Tlas::build (tlas.rs:35-45): rot float[n][D][D] row-major (orthonormal),
+ * trans float[n][D], halves float[n][D] (BOX), meshes[n_meshes] (MESH, borrowed:
+ * keep them alive), mesh_of[n], scale[n].  Host pointers. */
+int sq_tlas_build(sq_ctx* ctx, int dim, int inst_kind, uint64_t n_inst, const float* rot, const float* trans,
+                  const float* halves, sq_bvh* const* meshes, uint32_t n_meshes, const uint32_t* mesh_of,
+                  const float* scale, sq_tlas** out);
+int sq_tlas_destroy(sq_tlas* tlas);
+/* Tlas::refit (tlas.rs:49-54): new transforms (and halves for BOX, or NULL),
+ * host pointers; world AABBs recomputed, tree refit on the device. */
+int sq_tlas_refit(sq_tlas* tlas, const float* rot, const float* trans, const float* halves);
+/* Tlas::needs_rebuild (tlas.rs:58-60).  Synchronises. */
+int sq_tlas_needs_rebuild(sq_tlas* tlas, float factor, int* out);
+int sq_tlas_get_stats(const sq_tlas* tlas, sq_tree_stats* out);   /* Tlas::stats, tlas.rs:68-79 */
+int sq_tlas_download_bounds(sq_tlas* tlas, float* node_bounds);
+/* Tlas::raycast_nearest (tlas.rs:82-154): out_inst = instance index, out_sub =
+ * triangle index within the mesh (MESH) or -1 (BOX); normals lifted by R. */
+int sq_tlas_raycast_nearest(sq_tlas* tlas, const float* rays, uint64_t n, float max_toi, const float* max_toi_per_ray,
+                            int32_t* out_inst, float* out_toi, float* out_point, float* out_normal, int32_t* out_sub);
+int sq_tlas_raycast_any(sq_tlas* tlas, const float* rays, uint64_t n, float max_toi, const float* max_toi_per_ray,
+                        uint8_t* out_hit);
+/* Tlas::raycast_all (tlas.rs:158-215), CSR as sq_raycast_all. */
+int sq_tlas_raycast_all(sq_tlas* tlas, const float* rays, uint64_t n, float max_toi, uint64_t* offsets,
+                        int32_t* out_inst, float* out_toi, uint64_t capacity, uint64_t* total);
+/* Tlas::overlap with the default instance test (every instance of a touched
+ * leaf node), CSR, instances in ascending order. */
+int sq_tlas_overlap(sq_tlas* tlas, const float* regions, uint64_t n, uint64_t* offsets, int32_t* out_inst,
+                    uint64_t capacity, uint64_t* total);
 
 /* Kernel tuning knobs (block size in threads, a multiple of 64; 0 = default).
  * The autotuner sets these; results never depend on them. */

@@ -22,7 +22,7 @@ _LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
 # Provider kinds (sq_oracle.cpp `enum Kind`).
 BOX, SPHERE, TRI = 0, 1, 2
 SPHERE_PLAIN, SPHERE_PLAIN_NOEXIT, TRI_PLAIN, SHELL = 10, 11, 12, 13
-INST_BOX, INST_MESH = 0, 1
+INST_BOX, INST_MESH, INST_BOX_GUARDED = 0, 1, 2
 ALL_LAYERS = (1 << 64) - 1
 
 _lib = None

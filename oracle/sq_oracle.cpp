@@ -850,8 +850,9 @@ static void naive_overlap(const Geometry<D>& g, const Aabb<D>& region, const Fil
 // ---------------------------------------------------------------------------
 
 enum InstKind {
-  INST_BOX = 0,    // local box [-half, half], two_level.rs:70-113 (ray_local_box)
-  INST_MESH = 1,   // uniformly scaled triangle mesh with its own BVH (D = 3)
+  INST_BOX = 0,          // local box [-half, half], two_level.rs:70-113 (ray_local_box)
+  INST_MESH = 1,         // uniformly scaled triangle mesh with its own BVH (D = 3), guarded
+  INST_BOX_GUARDED = 2,  // INST_BOX plus the instance guard (the engine's box kind)
 };
 
 template <int D> struct Instances {
@@ -868,8 +869,9 @@ template <int D> struct Instances {
   int64_t id(size_t i) const { return (int64_t)i; }
 
   // The AABB of instance i in its own (scaled) local space.
+  bool is_box() const { return kind == INST_BOX || kind == INST_BOX_GUARDED; }
   Aabb<D> local_aabb(size_t i) const {
-    if (kind == INST_BOX) { Aabb<D> b; b.min = pmul(halves[i], -1.0f); b.max = halves[i]; return b; }
+    if (is_box()) { Aabb<D> b; b.min = pmul(halves[i], -1.0f); b.max = halves[i]; return b; }
     const Aabb<D>& m = mesh_box[mesh_of[i]];
     Aabb<D> b; b.min = pmul(m.min, scale[i]); b.max = pmul(m.max, scale[i]);
     return b;
@@ -879,7 +881,7 @@ template <int D> struct Instances {
   // unscaled mesh frame by o/s, d/s (t is unchanged) and answered by the
   // mesh's own BVH with the box-guarded triangle kind.
   bool test_ray_local(size_t i, const Ray<D>& ray, float max_toi, LeafHit<D>* out) const {
-    if (kind == INST_BOX) {
+    if (is_box()) {
       const Point<D>& half = halves[i];
       float t_min = 0.0f, t_max = max_toi; int axis = -1;
       for (int k = 0; k < D; ++k) {
@@ -942,7 +944,7 @@ template <int D> struct Tlas {
     Ray<D> local; local.origin = iso.inverse_transform_point(ray.origin); local.dir = iso.inverse_transform_vector(ray.dir);
     LeafHit<D> lh;
     if (!s.test_ray_local(i, local, max_toi, &lh)) return false;
-    if (s.kind == INST_MESH) {
+    if (s.kind != INST_BOX) {
       float te, tx;
       if (!world[i].ray_intersect(ray, max_toi, &te, &tx)) return false;
       if (!(te <= lh.toi && lh.toi <= tx)) return false;
@@ -1359,7 +1361,7 @@ void* orc_inst_new(int D, int kind, uint64_t n, const float* rot, const float* t
       for (int a = 0; a < DD; ++a) for (int c = 0; c < DD; ++c) s->xf[i].rot[a][c] = rot[i * DD * DD + a * DD + c];
       for (int a = 0; a < DD; ++a) s->xf[i].trans.v[a] = trans[i * DD + a];
     }
-    if (kind == INST_BOX) {
+    if (s->is_box()) {
       s->halves.resize(n);
       for (uint64_t i = 0; i < n; ++i) for (int a = 0; a < DD; ++a) s->halves[i].v[a] = halves[i * DD + a];
     } else {
@@ -1387,7 +1389,7 @@ void orc_inst_set_transforms(void* h, const float* rot, const float* trans, cons
     for (size_t i = 0; i < s->xf.size(); ++i) {
       for (int a = 0; a < DD; ++a) for (int c = 0; c < DD; ++c) s->xf[i].rot[a][c] = rot[i * DD * DD + a * DD + c];
       for (int a = 0; a < DD; ++a) s->xf[i].trans.v[a] = trans[i * DD + a];
-      if (halves && s->kind == INST_BOX) for (int a = 0; a < DD; ++a) s->halves[i].v[a] = halves[i * DD + a];
+      if (halves && s->is_box()) for (int a = 0; a < DD; ++a) s->halves[i].v[a] = halves[i * DD + a];
     }
   });
 }

@@ -15,103 +15,13 @@
 #include <string>
 #include <vector>
 
-#include "../../include/sq.h"
-#include "host_tree.h"
-#include "traverse.h"
+#include "engine_internal.h"
 
-using namespace sqd;
-
-// ---------------------------------------------------------------------------
-// errors
-// ---------------------------------------------------------------------------
-
-static thread_local std::string g_err;
-
-static int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIP_TRY(expr)                                                                      \
-  do {                                                                                     \
-    hipError_t e_ = (expr);                                                                \
-    if (e_ != hipSuccess) return fail(SQ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-// ---------------------------------------------------------------------------
-// objects
-// ---------------------------------------------------------------------------
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  ~DevBuf() { if (p) (void)hipFree(p); }
-  int ensure(size_t b) {
-    if (b <= bytes) return SQ_OK;
-    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
-    if (b == 0) return SQ_OK;
-    hipError_t e = hipMalloc(&p, b);
-    if (e != hipSuccess) return fail(SQ_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-    bytes = b;
-    return SQ_OK;
-  }
-  template <class T> T* as() const { return (T*)p; }
-};
-
-struct sq_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int block = 256;
-  bool timed = false;
-  DevBuf counts, scratch, keys, keys2;   // reusable temporaries of the CSR queries
-};
-
-struct sq_bvh {
-  sq_ctx* ctx = nullptr;
-  int dim = 0, kind = 0;
-  uint64_t n_prims = 0;
-  sqh::RefTree tree;
-  sqh::Packed packed;
-  DevBuf nodes, prims, ids, root, layers, levels;
-  sq_tree_stats stats{};
-};
-
-static const char* kind_name(int k) { return k == 0 ? "AABB" : k == 1 ? "SPHERE" : k == 2 ? "TRIANGLE" : "?"; }
-
-// ---------------------------------------------------------------------------
-// dispatch over (D, KIND)
-// ---------------------------------------------------------------------------
-
-template <class F> static int dispatch_dk(int dim, int kind, F&& f) {
-  if (kind == KIND_TRI) {
-    if (dim != 3) return fail(SQ_ERR_UNSUPPORTED, "TRIANGLE primitives need dim 3");
-    return f.template operator()<3, KIND_TRI>();
-  }
-  if (kind != KIND_AABB && kind != KIND_SPHERE) return fail(SQ_ERR_INVALID, "unknown primitive kind");
-#define SQ_CASE(DD)                                                       \
-  case DD:                                                                \
-    return kind == KIND_AABB ? f.template operator()<DD, KIND_AABB>()    \
-                             : f.template operator()<DD, KIND_SPHERE>();
-  switch (dim) {
-    SQ_CASE(2) SQ_CASE(3) SQ_CASE(4) SQ_CASE(5) SQ_CASE(6)
-    default: return fail(SQ_ERR_UNSUPPORTED, "dimension " + std::to_string(dim) + " not instantiated (2..6)");
-  }
-#undef SQ_CASE
-}
+thread_local std::string g_err;
 
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-
-struct NearestOut {
-  int32_t* leaf;
-  float* toi;
-  float* point;
-  float* normal;
-  int32_t* sub;
-  uint8_t* any;
-};
 
 // Nearest / any-hit rays (RayPolicy) and nearest shape casts (CastPolicy).
 // Query i reads QS floats at q + i*QS.
@@ -458,52 +368,6 @@ __global__ void k_refit_root(const float* nodes, const float* __restrict__ prims
 // helpers
 // ---------------------------------------------------------------------------
 
-static inline uint32_t grid_for(uint64_t n, int block) { return (uint32_t)((n + block - 1) / block); }
-
-static SceneView view_of(const sq_bvh* b) {
-  SceneView s;
-  s.nodes = b->nodes.as<float>();
-  s.prims = b->prims.as<float>();
-  s.ids = b->ids.as<uint32_t>();
-  s.root = b->root.as<float>();
-  return s;
-}
-
-static Filter filter_of(const sq_bvh* b, const sq_filter* f, bool* filt) {
-  Filter out;
-  out.layers = b->layers.as<uint64_t>();
-  out.mask = f ? f->layer_mask : ~0ull;
-  out.exclude = f ? f->exclude : -1;
-  *filt = out.layers != nullptr || out.mask != ~0ull || out.exclude >= 0;
-  return out;
-}
-
-static int begin_timing(sq_ctx* c) {
-  if (!c->ev0) {
-    HIP_TRY(hipEventCreate(&c->ev0));
-    HIP_TRY(hipEventCreate(&c->ev1));
-  }
-  HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  return SQ_OK;
-}
-static int end_timing(sq_ctx* c) {
-  HIP_TRY(hipEventRecord(c->ev1, c->stream));
-  c->timed = true;
-  return SQ_OK;
-}
-
-static int upload(DevBuf& d, const void* h, size_t bytes, hipStream_t s) {
-  int rc = d.ensure(bytes);
-  if (rc) return rc;
-  if (bytes) HIP_TRY(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
-  return SQ_OK;
-}
-
-// ---------------------------------------------------------------------------
-// C ABI
-// ---------------------------------------------------------------------------
-
-// Exclusive scan of n counts into offsets[0..n] on the context stream.
 template <class T>
 static int scan_into(sq_ctx* c, const T* in, uint64_t n, uint64_t* out, std::vector<std::unique_ptr<DevBuf>>& temps) {
   const uint64_t tiles = (n + 1 + 4095) / 4096;
@@ -524,6 +388,56 @@ static int scan_into(sq_ctx* c, const T* in, uint64_t n, uint64_t* out, std::vec
   return SQ_OK;
 }
 
+void ctx_release(sq_ctx* c) {
+  if (--c->refs > 0) return;
+  (void)hipSetDevice(c->device);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+}
+
+int fill_miss(sq_ctx* c, uint64_t n, int dim, NearestOut out) {
+  k_fill_miss<<<grid_for(n, 256), 256, 0, c->stream>>>((uint32_t)n, dim, out);
+  HIP_TRY(hipGetLastError());
+  return SQ_OK;
+}
+
+// count -> scan -> capacity check -> fill -> per-query sort -> unpack
+int csr_run(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* out_toi, bool has_t,
+            uint64_t capacity, uint64_t* total, const std::function<int(bool, uint64_t*)>& launch) {
+  int rc;
+  if ((rc = launch(false, nullptr))) return rc;
+  std::vector<std::unique_ptr<DevBuf>> temps;
+  if ((rc = scan_into<uint32_t>(c, c->counts.as<uint32_t>(), n, offsets, temps))) return rc;
+  HIP_TRY(hipMemcpyAsync(total, offsets + n, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const uint64_t tot = *total;
+  if (tot > capacity) return fail(SQ_ERR_CAPACITY, "result needs " + std::to_string(tot) + " entries, capacity is " + std::to_string(capacity));
+  if (!tot) return SQ_OK;
+  if (!out_leaf || (has_t && !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
+  if ((rc = c->keys.ensure(tot * 8))) return rc;
+  if ((rc = c->keys2.ensure(tot * 8))) return rc;
+  if ((rc = c->scratch.ensure((n + 1) * 4 + 64))) return rc;
+  uint64_t* keys = c->keys.as<uint64_t>();
+  if ((rc = launch(true, keys))) return rc;
+  uint32_t* n_large = c->scratch.as<uint32_t>();
+  uint32_t* large = n_large + 16;
+  HIP_TRY(hipMemsetAsync(n_large, 0, 4, c->stream));
+  k_sort_small<<<grid_for(n, 256), 256, 0, c->stream>>>(keys, offsets, (uint32_t)n, large, n_large);
+  HIP_TRY(hipGetLastError());
+  uint32_t h_large = 0;
+  HIP_TRY(hipMemcpyAsync(&h_large, n_large, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (h_large) {
+    k_sort_large<<<h_large, 256, 0, c->stream>>>(keys, c->keys2.as<uint64_t>(), offsets, large);
+    HIP_TRY(hipGetLastError());
+  }
+  k_unpack<<<grid_for(tot, 256), 256, 0, c->stream>>>(keys, tot, out_leaf, out_toi, has_t ? 1 : 0);
+  HIP_TRY(hipGetLastError());
+  return SQ_OK;
+}
+
 extern "C" {
 
 const char* sq_last_error(void) { return g_err.c_str(); }
@@ -538,19 +452,28 @@ int sq_ctx_create(int device, sq_ctx** out) {
   HIP_TRY(hipSetDevice(device));
   auto c = std::make_unique<sq_ctx>();
   c->device = device;
-  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+  c->stream = c->own;
   *out = c.release();
   return SQ_OK;
 }
 
 int sq_ctx_destroy(sq_ctx* c) {
   if (!c) return SQ_OK;
-  (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  if (c->ev0) (void)hipEventDestroy(c->ev0);
-  if (c->ev1) (void)hipEventDestroy(c->ev1);
-  (void)hipStreamDestroy(c->stream);
-  delete c;
+  ctx_release(c);
+  return SQ_OK;
+}
+
+int sq_ctx_set_stream(sq_ctx* c, void* stream) {
+  if (!c) return fail(SQ_ERR_INVALID, "ctx is NULL");
+  c->stream = (hipStream_t)stream;
+  return SQ_OK;
+}
+
+int sq_ctx_reset_stream(sq_ctx* c) {
+  if (!c) return fail(SQ_ERR_INVALID, "ctx is NULL");
+  c->stream = c->own;
   return SQ_OK;
 }
 
@@ -699,7 +622,7 @@ int sq_bvh_upload(sq_ctx* ctx, const sq_tree* tp, int kind, const float* prims, 
   const uint64_t n = t.prim_indices.size();
   if (n && !prims) return fail(SQ_ERR_INVALID, "prims is NULL");
   auto b = std::make_unique<sq_bvh>();
-  b->ctx = ctx; b->dim = dim; b->kind = kind; b->n_prims = n;
+  b->dim = dim; b->kind = kind; b->n_prims = n;
   b->tree = t;
   try {
     sqh::pack_tree(b->tree, b->packed);
@@ -717,6 +640,8 @@ int sq_bvh_upload(sq_ctx* ctx, const sq_tree* tp, int kind, const float* prims, 
   if ((rc = upload(b->levels, b->packed.levels.data(), b->packed.levels.size() * 4, s))) return rc;
   HIP_TRY(hipStreamSynchronize(s));
   tree_stats_of(b->tree, b->stats);
+  b->ctx = ctx;
+  ctx_retain(ctx);
   *out = b.release();
   return SQ_OK;
 }
@@ -733,8 +658,10 @@ int sq_bvh_build(sq_ctx* ctx, int dim, int kind, const float* prims, uint64_t n,
 
 int sq_bvh_destroy(sq_bvh* b) {
   if (!b) return SQ_OK;
-  (void)hipStreamSynchronize(b->ctx->stream);
+  sq_ctx* c = b->ctx;
+  if (c) (void)hipStreamSynchronize(c->stream);
   delete b;
+  if (c) ctx_release(c);
   return SQ_OK;
 }
 
@@ -905,35 +832,7 @@ static int collect_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
       return SQ_OK;
     });
   };
-  if ((rc = launch(false, nullptr))) return rc;
-  std::vector<std::unique_ptr<DevBuf>> temps;
-  if ((rc = scan_into<uint32_t>(c, counts, n, offsets, temps))) return rc;
-  HIP_TRY(hipMemcpyAsync(total, offsets + n, 8, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  const uint64_t tot = *total;
-  if (tot > capacity) return fail(SQ_ERR_CAPACITY, "result needs " + std::to_string(tot) + " entries, capacity is " + std::to_string(capacity));
-  if (!tot) return SQ_OK;
-  if (!out_leaf || (mode != 2 && !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
-  if ((rc = c->keys.ensure(tot * 8))) return rc;
-  if ((rc = c->keys2.ensure(tot * 8))) return rc;
-  if ((rc = c->scratch.ensure((n + 1) * 4 + 64))) return rc;
-  uint64_t* keys = c->keys.as<uint64_t>();
-  if ((rc = launch(true, keys))) return rc;
-  uint32_t* n_large = c->scratch.as<uint32_t>();
-  uint32_t* large = n_large + 16;
-  HIP_TRY(hipMemsetAsync(n_large, 0, 4, c->stream));
-  k_sort_small<<<grid_for(n, 256), 256, 0, c->stream>>>(keys, offsets, nn, large, n_large);
-  HIP_TRY(hipGetLastError());
-  uint32_t h_large = 0;
-  HIP_TRY(hipMemcpyAsync(&h_large, n_large, 4, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  if (h_large) {
-    k_sort_large<<<h_large, 256, 0, c->stream>>>(keys, c->keys2.as<uint64_t>(), offsets, large);
-    HIP_TRY(hipGetLastError());
-  }
-  k_unpack<<<grid_for(tot, 256), 256, 0, c->stream>>>(keys, tot, out_leaf, out_toi, mode != 2 ? 1 : 0);
-  HIP_TRY(hipGetLastError());
-  return SQ_OK;
+  return csr_run(c, n, offsets, out_leaf, out_toi, mode != 2, capacity, total, launch);
 }
 
 int sq_raycast_all(sq_bvh* b, const float* rays, uint64_t n, float max_toi, const sq_filter* f, int crossings,

@@ -1,0 +1,163 @@
+// engine_internal.h -- objects and helpers shared by the engine's translation
+// units (engine.hip: single-level trees; tlas.hip: two-level trees).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/sq.h"
+#include "host_tree.h"
+#include "traverse.h"
+
+using namespace sqd;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+extern thread_local std::string g_err;
+
+inline int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return fail(SQ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// objects
+// ---------------------------------------------------------------------------
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  int ensure(size_t b) {
+    if (b <= bytes) return SQ_OK;
+    if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+    if (b == 0) return SQ_OK;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) return fail(SQ_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    bytes = b;
+    return SQ_OK;
+  }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+struct sq_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;       // the stream the context created
+  hipStream_t stream = nullptr;    // where work is enqueued (own, or the caller's)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int block = 256;
+  bool timed = false;
+  int refs = 1;                    // the caller's handle + one per live tree
+  DevBuf counts, scratch, keys, keys2;   // reusable temporaries of the CSR queries
+};
+
+// Drops one reference; the last one frees the stream, events and buffers.
+void ctx_release(sq_ctx* c);
+inline void ctx_retain(sq_ctx* c) { ++c->refs; }
+
+struct sq_bvh {
+  sq_ctx* ctx = nullptr;
+  int dim = 0, kind = 0;
+  uint64_t n_prims = 0;
+  sqh::RefTree tree;
+  sqh::Packed packed;
+  DevBuf nodes, prims, ids, root, layers, levels;
+  sq_tree_stats stats{};
+};
+
+// ---------------------------------------------------------------------------
+// dispatch over (D, KIND)
+// ---------------------------------------------------------------------------
+
+template <class F> inline int dispatch_dk(int dim, int kind, F&& f) {
+  if (kind == KIND_TRI) {
+    if (dim != 3) return fail(SQ_ERR_UNSUPPORTED, "TRIANGLE primitives need dim 3");
+    return f.template operator()<3, KIND_TRI>();
+  }
+  if (kind != KIND_AABB && kind != KIND_SPHERE) return fail(SQ_ERR_INVALID, "unknown primitive kind");
+#define SQ_CASE(DD)                                                       \
+  case DD:                                                                \
+    return kind == KIND_AABB ? f.template operator()<DD, KIND_AABB>()    \
+                             : f.template operator()<DD, KIND_SPHERE>();
+  switch (dim) {
+    SQ_CASE(2) SQ_CASE(3) SQ_CASE(4) SQ_CASE(5) SQ_CASE(6)
+    default: return fail(SQ_ERR_UNSUPPORTED, "dimension " + std::to_string(dim) + " not instantiated (2..6)");
+  }
+#undef SQ_CASE
+}
+
+struct NearestOut {
+  int32_t* leaf;
+  float* toi;
+  float* point;
+  float* normal;
+  int32_t* sub;
+  uint8_t* any;
+};
+
+inline uint32_t grid_for(uint64_t n, int block) { return (uint32_t)((n + block - 1) / block); }
+
+inline SceneView view_of(const sq_bvh* b) {
+  SceneView s;
+  s.nodes = b->nodes.as<float>();
+  s.prims = b->prims.as<float>();
+  s.ids = b->ids.as<uint32_t>();
+  s.root = b->root.as<float>();
+  return s;
+}
+
+inline Filter filter_of(const sq_bvh* b, const sq_filter* f, bool* filt) {
+  Filter out;
+  out.layers = b->layers.as<uint64_t>();
+  out.mask = f ? f->layer_mask : ~0ull;
+  out.exclude = f ? f->exclude : -1;
+  *filt = out.layers != nullptr || out.mask != ~0ull || out.exclude >= 0;
+  return out;
+}
+
+inline int begin_timing(sq_ctx* c) {
+  if (!c->ev0) {
+    HIP_TRY(hipEventCreate(&c->ev0));
+    HIP_TRY(hipEventCreate(&c->ev1));
+  }
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  return SQ_OK;
+}
+inline int end_timing(sq_ctx* c) {
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  return SQ_OK;
+}
+
+inline int upload(DevBuf& d, const void* h, size_t bytes, hipStream_t s) {
+  int rc = d.ensure(bytes);
+  if (rc) return rc;
+  if (bytes) HIP_TRY(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
+  return SQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+// Exclusive scan of n counts into offsets[0..n] on the context stream.
+// Writes misses (leaf -1, toi +inf, zero point/normal, sub -1, any 0).
+int fill_miss(sq_ctx* c, uint64_t n, int dim, NearestOut out);
+
+// Variable-length (CSR) results: count pass, scan, capacity check, fill pass,
+// per-query sort by key, unpack.  launch(fill, keys) enqueues the count pass
+// (fill = false, writing ctx->counts) or the fill pass (fill = true).
+int csr_run(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* out_toi, bool has_t,
+            uint64_t capacity, uint64_t* total, const std::function<int(bool, uint64_t*)>& launch);

@@ -1,0 +1,414 @@
+// tlas.hip -- the two-level tree (crates/spatial-query/src/accel/tlas.rs) on
+// the device: a top-level pair-node tree over instance world AABBs, instance
+// records (rigid transform + kind data) and, for mesh instances, one device
+// BVH per mesh walked in the instance's frame.
+#include <cmath>
+#include <cstring>
+
+#include "engine_internal.h"
+
+namespace {
+
+struct Inst {
+  int dim, kind;
+  uint64_t n;
+  std::vector<float> rot, trans, halves, scale;   // by instance id
+  std::vector<uint32_t> mesh_of;
+  std::vector<sq_bvh*> meshes;                    // borrowed
+};
+
+uint32_t record_stride(int dim, int kind) {
+  uint32_t f = kind == INST_BOX ? dim * dim + 2 * dim : dim * dim + dim + 2;
+  return (f + 3) & ~3u;
+}
+
+// Isometry::transform_aabb(local_aabb) (isometry.rs:84-97) on the host, with
+// the reference's operation order.
+void world_boxes(const Inst& in, std::vector<float>& out) {
+  const int D = in.dim;
+  out.resize(in.n * 2 * D);
+  for (uint64_t i = 0; i < in.n; ++i) {
+    float lmin[sqh::MAX_DIM], lmax[sqh::MAX_DIM];
+    if (in.kind == INST_BOX) {
+      for (int k = 0; k < D; ++k) { lmax[k] = in.halves[i * D + k]; lmin[k] = lmax[k] * -1.0f; }
+    } else {
+      // local_aabb of a scaled mesh: the mesh root box times the scale
+      const sq_bvh* m = in.meshes[in.mesh_of[i]];
+      const float s = in.scale[i];
+      for (int k = 0; k < D; ++k) { lmin[k] = m->tree.bounds[k] * s; lmax[k] = m->tree.bounds[D + k] * s; }
+    }
+    const float* R = in.rot.data() + i * D * D;
+    const float* t = in.trans.data() + i * D;
+    float c[sqh::MAX_DIM], h[sqh::MAX_DIM], rc[sqh::MAX_DIM];
+    for (int k = 0; k < D; ++k) { c[k] = (lmin[k] + lmax[k]) * 0.5f; h[k] = (lmax[k] - lmin[k]) * 0.5f; }
+    for (int a = 0; a < D; ++a) {
+      float acc = 0.0f;
+      for (int b = 0; b < D; ++b) acc = acc + R[a * D + b] * c[b];
+      rc[a] = acc + t[a];
+    }
+    float* o = out.data() + i * 2 * D;
+    for (int a = 0; a < D; ++a) {
+      float acc = 0.0f;
+      for (int b = 0; b < D; ++b) acc = acc + fabsf(R[a * D + b]) * h[b];
+      o[a] = rc[a] - acc;
+      o[D + a] = rc[a] + acc;
+    }
+  }
+}
+
+void records(const Inst& in, std::vector<float>& out) {
+  const int D = in.dim;
+  const uint32_t xs = record_stride(D, in.kind);
+  out.assign(in.n * xs, 0.0f);
+  for (uint64_t i = 0; i < in.n; ++i) {
+    float* x = out.data() + i * xs;
+    std::memcpy(x, in.rot.data() + i * D * D, D * D * 4);
+    std::memcpy(x + D * D, in.trans.data() + i * D, D * 4);
+    if (in.kind == INST_BOX) {
+      std::memcpy(x + D * D + D, in.halves.data() + i * D, D * 4);
+    } else {
+      x[D * D + D] = 1.0f / in.scale[i];
+      std::memcpy(&x[D * D + D + 1], &in.mesh_of[i], 4);
+    }
+  }
+}
+
+}  // namespace
+
+struct sq_tlas {
+  sq_ctx* ctx = nullptr;
+  Inst in;
+  sq_bvh* top = nullptr;                  // AABB-kind tree over world boxes
+  DevBuf xf, world, blas;
+  float baseline = 0.0f;
+  uint32_t blas_stack = 0;
+  ~sq_tlas() {
+    if (top) sq_bvh_destroy(top);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+template <int D, int IK, bool ANY>
+__global__ __launch_bounds__(1024) void k_tlas_nearest(TlasView tv, const float* __restrict__ q, uint32_t n, float max_toi,
+                                                       const float* __restrict__ tmax, NearestOut out, uint32_t s_top,
+                                                       uint32_t s_bot) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  const uint32_t i = blockIdx.x * B + tid;
+  if (i >= n) return;
+  RayD<D> r;
+  ray_setup<D>(r, q + (size_t)i * 2 * D);
+  const float lim = tmax ? tmax[i] : max_toi;
+  const uint32_t S = s_top + s_bot;
+  uint32_t* refs = lds + tid;
+  float* ts = reinterpret_cast<float*>(lds + S * B) + tid;
+  Stack st{refs, ts, B};
+  Stack sb{refs + s_top * B, ts + s_top * B, B};
+  TlasHit h = walk_tlas<D, IK, ANY>(tv, r, lim, st, sb);
+  const bool hit = h.inst != NO_LEAF;
+  if (ANY) { out.any[i] = hit ? 1 : 0; return; }
+  out.leaf[i] = hit ? (int32_t)h.inst : -1;
+  out.toi[i] = h.t;
+  if (out.point) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) out.point[(size_t)i * D + k] = hit ? r.o[k] + r.d[k] * h.t : 0.0f;
+  }
+  if (out.normal) {
+    float nv[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) nv[k] = 0.0f;
+    if (hit) tlas_normal<D, IK>(tv, r, h, lim, nv);
+#pragma unroll
+    for (int k = 0; k < D; ++k) out.normal[(size_t)i * D + k] = nv[k];
+  }
+  if (out.sub) out.sub[i] = (hit && h.sub != NO_LEAF) ? (int32_t)h.sub : -1;
+}
+
+// MODE 0: raycast_all (one hit per instance, tlas.rs:158-215); MODE 2: overlap
+// with the default instance test_overlap (every instance of a touched leaf
+// node, tlas.rs overlap + instance.rs:91-93).
+template <int D, int IK, int MODE, bool FILL>
+__global__ __launch_bounds__(1024) void k_tlas_collect(TlasView tv, const float* __restrict__ q, uint32_t n, float max_toi,
+                                                       uint32_t* counts, const uint64_t* __restrict__ offsets, uint64_t* keys,
+                                                       uint32_t s_top, uint32_t s_bot) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  const uint32_t i = blockIdx.x * B + tid;
+  if (i >= n) return;
+  const SceneView& s = tv.top;
+  const uint32_t S = s_top + s_bot;
+  uint32_t* refs = lds + tid;
+  float* ts = reinterpret_cast<float*>(lds + S * B) + tid;
+  Stack st{refs, ts, B};
+  Stack sb{refs + s_top * B, ts + s_top * B, B};
+  uint32_t cnt = 0;
+  uint64_t* dst = FILL ? keys + offsets[i] : nullptr;
+  const uint32_t root_ref = fbits(s.root[2 * D]);
+  if constexpr (MODE == 2) {
+    const float* rg = q + (size_t)i * 2 * D;
+    if (box_overlap<D>(s.root, s.root + D, rg, rg + D)) {
+      walk_visit<D>(s, root_ref,
+        [&](const float* nd, bool& ha, bool& hb) {
+          if (fbits(nd[4 * D + 3])) { ha = hb = true; return; }   // parts of one reference leaf
+          float amn[D], amx[D], bmn[D], bmx[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) { amn[k] = nd[4 * k]; amx[k] = nd[4 * k + 1]; bmn[k] = nd[4 * k + 2]; bmx[k] = nd[4 * k + 3]; }
+          ha = box_overlap<D>(amn, amx, rg, rg + D);
+          hb = box_overlap<D>(bmn, bmx, rg, rg + D);
+        },
+        [&](uint32_t first, uint32_t c) {
+          for (uint32_t j = 0; j < c; ++j) { if (FILL) dst[cnt] = s.ids[first + j]; ++cnt; }
+        }, st);
+    }
+  } else {
+    RayD<D> r;
+    ray_setup<D>(r, q + (size_t)i * 2 * D);
+    float te, tx; int ax;
+    if (slab_exact<D>(r, s.root, s.root + D, max_toi, te, tx, ax)) {
+      walk_visit<D>(s, root_ref,
+        [&](const float* nd, bool& ha, bool& hb) { float ta, tb; slab_pair<D>(r, nd, max_toi, ha, hb, ta, tb); },
+        [&](uint32_t first, uint32_t c) {
+          for (uint32_t j = 0; j < c; ++j) {
+            TlasHit h;
+            if (!instance_test<D, IK>(tv, r, s.ids[first + j], max_toi, sb, h)) continue;
+            if (FILL) dst[cnt] = hit_key(h.t, h.inst);
+            ++cnt;
+          }
+        }, st);
+    }
+  }
+  if (!FILL) counts[i] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+
+template <class F> static int dispatch_inst(int dim, int kind, F&& f) {
+  if (kind == INST_MESH) {
+    if (dim != 3) return fail(SQ_ERR_UNSUPPORTED, "mesh instances need dim 3");
+    return f.template operator()<3, INST_MESH>();
+  }
+  if (kind != INST_BOX) return fail(SQ_ERR_INVALID, "unknown instance kind");
+  switch (dim) {
+    case 2: return f.template operator()<2, INST_BOX>();
+    case 3: return f.template operator()<3, INST_BOX>();
+    case 4: return f.template operator()<4, INST_BOX>();
+    case 5: return f.template operator()<5, INST_BOX>();
+    case 6: return f.template operator()<6, INST_BOX>();
+    default: return fail(SQ_ERR_UNSUPPORTED, "dimension not instantiated (2..6)");
+  }
+}
+
+static TlasView tview(const sq_tlas* t) {
+  TlasView v;
+  v.top = view_of(t->top);
+  v.xf = t->xf.as<float>();
+  v.world = t->world.as<float>();
+  v.blas = t->blas.as<SceneView>();
+  v.xs = record_stride(t->in.dim, t->in.kind);
+  return v;
+}
+
+static int tlas_upload_world(sq_tlas* t, std::vector<float>& world) {
+  world_boxes(t->in, world);
+  return upload(t->world, world.data(), world.size() * 4, t->ctx->stream);
+}
+
+static int tlas_upload_records(sq_tlas* t) {
+  std::vector<float> rec;
+  records(t->in, rec);
+  return upload(t->xf, rec.data(), rec.size() * 4, t->ctx->stream);
+}
+
+extern "C" {
+
+int sq_tlas_build(sq_ctx* ctx, int dim, int kind, uint64_t n, const float* rot, const float* trans, const float* halves,
+                  sq_bvh* const* meshes, uint32_t n_meshes, const uint32_t* mesh_of, const float* scale, sq_tlas** out) {
+  if (!ctx || !out || (n && (!rot || !trans))) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  int rc = dispatch_inst(dim, kind, [&]<int D, int K>() { return SQ_OK; });
+  if (rc) return rc;
+  if (n > sqh::LEAF_FIRST_MASK) return fail(SQ_ERR_INVALID, "at most 2^27 instances");
+  auto t = std::make_unique<sq_tlas>();
+  t->ctx = ctx;
+  Inst& in = t->in;
+  in.dim = dim; in.kind = kind; in.n = n;
+  in.rot.assign(rot, rot + n * dim * dim);
+  in.trans.assign(trans, trans + n * dim);
+  if (kind == INST_BOX) {
+    if (n && !halves) return fail(SQ_ERR_INVALID, "box instances need halves");
+    in.halves.assign(halves, halves + n * dim);
+  } else {
+    if (n && (!meshes || !mesh_of || !scale)) return fail(SQ_ERR_INVALID, "mesh instances need meshes, mesh_of and scale");
+    in.meshes.assign(meshes, meshes + n_meshes);
+    for (uint32_t m = 0; m < n_meshes; ++m) {
+      if (!meshes[m] || meshes[m]->kind != KIND_TRI || meshes[m]->n_prims == 0 || meshes[m]->ctx != ctx)
+        return fail(SQ_ERR_INVALID, "each mesh must be a non-empty TRIANGLE bvh on this context");
+    }
+    in.mesh_of.assign(mesh_of, mesh_of + n);
+    in.scale.assign(scale, scale + n);
+    for (uint64_t i = 0; i < n; ++i) {
+      if (in.mesh_of[i] >= n_meshes) return fail(SQ_ERR_INVALID, "mesh index out of range");
+      if (!(in.scale[i] > 0.0f)) return fail(SQ_ERR_INVALID, "scale must be positive");
+    }
+    std::vector<SceneView> views(n_meshes);
+    for (uint32_t m = 0; m < n_meshes; ++m) {
+      views[m] = view_of(meshes[m]);
+      t->blas_stack = std::max(t->blas_stack, meshes[m]->packed.max_stack);
+    }
+    if ((rc = upload(t->blas, views.data(), views.size() * sizeof(SceneView), ctx->stream))) return rc;
+  }
+  std::vector<float> world;
+  if ((rc = tlas_upload_world(t.get(), world))) return rc;
+  if ((rc = tlas_upload_records(t.get()))) return rc;
+  sq_tree* tree = nullptr;
+  if ((rc = sq_tree_build(dim, KIND_AABB, world.data(), n, &tree))) return rc;
+  rc = sq_bvh_upload(ctx, tree, KIND_AABB, world.data(), &t->top);
+  sq_tree_destroy(tree);
+  if (rc) return rc;
+  t->baseline = t->top->stats.quality;   // tlas.rs:38: interior surface sum at build
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx_retain(ctx);
+  *out = t.release();
+  return SQ_OK;
+}
+
+int sq_tlas_destroy(sq_tlas* t) {
+  if (!t) return SQ_OK;
+  sq_ctx* c = t->ctx;
+  (void)hipStreamSynchronize(c->stream);
+  delete t;
+  ctx_release(c);
+  return SQ_OK;
+}
+
+int sq_tlas_refit(sq_tlas* t, const float* rot, const float* trans, const float* halves) {
+  if (!t || (t->in.n && (!rot || !trans))) return fail(SQ_ERR_INVALID, "NULL argument");
+  Inst& in = t->in;
+  in.rot.assign(rot, rot + in.n * in.dim * in.dim);
+  in.trans.assign(trans, trans + in.n * in.dim);
+  if (halves && in.kind == INST_BOX) in.halves.assign(halves, halves + in.n * in.dim);
+  if (!in.n) return SQ_OK;
+  std::vector<float> world;
+  int rc;
+  if ((rc = tlas_upload_world(t, world))) return rc;
+  if ((rc = tlas_upload_records(t))) return rc;
+  // tree::refit over the new world boxes, on the device
+  return sq_bvh_refit(t->top, t->world.as<float>());
+}
+
+int sq_tlas_download_bounds(sq_tlas* t, float* bounds) {
+  if (!t) return fail(SQ_ERR_INVALID, "NULL argument");
+  return sq_bvh_download_bounds(t->top, bounds);
+}
+
+int sq_tlas_get_stats(const sq_tlas* t, sq_tree_stats* out) {
+  if (!t || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  return sq_bvh_get_stats(t->top, out);
+}
+
+int sq_tlas_needs_rebuild(sq_tlas* t, float factor, int* out) {
+  if (!t || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = 0;
+  const size_t nn = t->top->tree.node_count();
+  if (!nn) return SQ_OK;
+  // tlas.rs:58-60 over the bounds the device currently holds, summed on the
+  // host in node order (the sum is order dependent).
+  sqh::RefTree cur = t->top->tree;
+  int rc = sq_bvh_download_bounds(t->top, cur.bounds.data());
+  if (rc) return rc;
+  const float q = sqh::tree_interior_surface_sum(cur);
+  *out = (t->baseline > 0.0f && q > factor * t->baseline) ? 1 : 0;
+  return SQ_OK;
+}
+
+static int tlas_nearest_impl(sq_tlas* t, const float* rays, uint64_t n, float max_toi, const float* tmax, NearestOut out, bool any) {
+  if (!t || (n && !rays)) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "at most 2^32-1 rays per call");
+  if (!n) return SQ_OK;
+  sq_ctx* c = t->ctx;
+  if (!t->in.n) {
+    return fill_miss(c, n, t->in.dim, out);
+  }
+  const int B = c->block;
+  const uint32_t s_top = t->top->packed.max_stack, s_bot = t->blas_stack;
+  const size_t lds = (size_t)(s_top + s_bot) * B * 8;
+  if (lds > 160 * 1024) return fail(SQ_ERR_INVALID, "traversal stacks do not fit in LDS; lower the block size");
+  TlasView tv = tview(t);
+  int rc = begin_timing(c);
+  if (rc) return rc;
+  rc = dispatch_inst(t->in.dim, t->in.kind, [&]<int D, int K>() -> int {
+    if (any) k_tlas_nearest<D, K, true><<<grid_for(n, B), B, lds, c->stream>>>(tv, rays, (uint32_t)n, max_toi, tmax, out, s_top, s_bot);
+    else k_tlas_nearest<D, K, false><<<grid_for(n, B), B, lds, c->stream>>>(tv, rays, (uint32_t)n, max_toi, tmax, out, s_top, s_bot);
+    HIP_TRY(hipGetLastError());
+    return SQ_OK;
+  });
+  if (rc) return rc;
+  return end_timing(c);
+}
+
+int sq_tlas_raycast_nearest(sq_tlas* t, const float* rays, uint64_t n, float max_toi, const float* tmax, int32_t* out_inst,
+                            float* out_toi, float* out_point, float* out_normal, int32_t* out_sub) {
+  if (n && (!out_inst || !out_toi)) return fail(SQ_ERR_INVALID, "out_inst and out_toi are required");
+  NearestOut o{out_inst, out_toi, out_point, out_normal, out_sub, nullptr};
+  return tlas_nearest_impl(t, rays, n, max_toi, tmax, o, false);
+}
+
+int sq_tlas_raycast_any(sq_tlas* t, const float* rays, uint64_t n, float max_toi, const float* tmax, uint8_t* out_hit) {
+  if (n && !out_hit) return fail(SQ_ERR_INVALID, "out_hit is required");
+  NearestOut o{nullptr, nullptr, nullptr, nullptr, nullptr, out_hit};
+  return tlas_nearest_impl(t, rays, n, max_toi, tmax, o, true);
+}
+
+static int tlas_collect_impl(sq_tlas* t, const float* q, uint64_t n, float max_toi, int mode, uint64_t* offsets,
+                             int32_t* out_inst, float* out_toi, uint64_t capacity, uint64_t* total) {
+  if (!t || !offsets || !total || (n && !q)) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "at most 2^32-1 queries per call");
+  sq_ctx* c = t->ctx;
+  *total = 0;
+  int rc;
+  if (!n || !t->in.n) {
+    HIP_TRY(hipMemsetAsync(offsets, 0, (n + 1) * 8, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SQ_OK;
+  }
+  if ((rc = c->counts.ensure(n * 4))) return rc;
+  const int B = c->block;
+  const uint32_t s_top = t->top->packed.max_stack, s_bot = t->blas_stack;
+  const size_t lds = (size_t)(s_top + s_bot) * B * 8;
+  if (lds > 160 * 1024) return fail(SQ_ERR_INVALID, "traversal stacks do not fit in LDS; lower the block size");
+  TlasView tv = tview(t);
+  uint32_t* counts = c->counts.as<uint32_t>();
+  auto launch = [&](bool fill, uint64_t* keys) -> int {
+    return dispatch_inst(t->in.dim, t->in.kind, [&]<int D, int K>() -> int {
+      dim3 g(grid_for(n, B)), b(B);
+      const uint32_t nn = (uint32_t)n;
+      if (mode == 2) {
+        if (fill) k_tlas_collect<D, K, 2, true><<<g, b, lds, c->stream>>>(tv, q, nn, max_toi, counts, offsets, keys, s_top, s_bot);
+        else k_tlas_collect<D, K, 2, false><<<g, b, lds, c->stream>>>(tv, q, nn, max_toi, counts, offsets, keys, s_top, s_bot);
+      } else {
+        if (fill) k_tlas_collect<D, K, 0, true><<<g, b, lds, c->stream>>>(tv, q, nn, max_toi, counts, offsets, keys, s_top, s_bot);
+        else k_tlas_collect<D, K, 0, false><<<g, b, lds, c->stream>>>(tv, q, nn, max_toi, counts, offsets, keys, s_top, s_bot);
+      }
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    });
+  };
+  return csr_run(c, n, offsets, out_inst, out_toi, mode != 2, capacity, total, launch);
+}
+
+int sq_tlas_raycast_all(sq_tlas* t, const float* rays, uint64_t n, float max_toi, uint64_t* offsets, int32_t* out_inst,
+                        float* out_toi, uint64_t capacity, uint64_t* total) {
+  return tlas_collect_impl(t, rays, n, max_toi, 0, offsets, out_inst, out_toi, capacity, total);
+}
+
+int sq_tlas_overlap(sq_tlas* t, const float* regions, uint64_t n, uint64_t* offsets, int32_t* out_inst, uint64_t capacity,
+                    uint64_t* total) {
+  return tlas_collect_impl(t, regions, n, 0.0f, 2, offsets, out_inst, nullptr, capacity, total);
+}
+
+}  // extern "C"

@@ -558,4 +558,186 @@ SQ_HD float key_toi(uint64_t k) {
   return bitsf((uint32_t)i);
 }
 
+// ---------------------------------------------------------------------------
+// Two-level walk (crates/spatial-query/src/accel/tlas.rs) over built-in
+// instance kinds.  Instance records by id, XS floats each:
+//   INST_BOX  (any D): R[D*D] row-major, t[D], half[D]          two_level.rs:70-113
+//   INST_MESH (D = 3): R[9], t[3], 1/scale, mesh id (bits)        a uniformly
+//                      scaled triangle mesh answered by its own device BVH.
+// Both kinds keep an instance hit only when its toi lies in the slab interval
+// of the instance's world AABB, the guard that makes the top-level result
+// independent of traversal order.
+// ---------------------------------------------------------------------------
+
+enum { INST_BOX = 0, INST_MESH = 1 };
+
+// Isometry::inverse_transform_point / _vector (isometry.rs:70-78, 113-123):
+// rotate_transpose(o - t) and rotate_transpose(d), each accumulating from 0.
+template <int D> SQ_HD void iso_inverse(const float* R, const float* t, const RayD<D>& r, float* o_l, float* d_l) {
+  float v[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) v[j] = r.o[j] - t[j];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    float a = 0.0f, b = 0.0f;
+#pragma unroll
+    for (int j = 0; j < D; ++j) { a = a + R[j * D + i] * v[j]; b = b + R[j * D + i] * r.d[j]; }
+    o_l[i] = a; d_l[i] = b;
+  }
+}
+
+// Isometry::transform_vector (rotate, isometry.rs:99-110).
+template <int D> SQ_HD void iso_rotate(const float* R, const float* n, float* out) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    float a = 0.0f;
+#pragma unroll
+    for (int j = 0; j < D; ++j) a = a + R[i * D + j] * n[j];
+    out[i] = a;
+  }
+}
+
+// ray_local_box (two_level.rs:81-113): a local box [-half, half]; parallel
+// threshold 1e-9; a hit needs an entry axis, t_min > 1e-6 and t_min <= max_toi.
+template <int D> SQ_HD bool local_box(const float* o, const float* d, const float* half, float max_toi, float& t, int& axis_out) {
+  float t_min = 0.0f, t_max = max_toi;
+  int axis = -1;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    if (fabsf(d[k]) < 1.0e-9f) {
+      if (o[k] < -half[k] || o[k] > half[k]) return false;
+    } else {
+      const float inv = SQ_DIV_RN(1.0f, d[k]);
+      float t1 = (-half[k] - o[k]) * inv;
+      float t2 = (half[k] - o[k]) * inv;
+      if (t1 > t2) { float tmp = t1; t1 = t2; t2 = tmp; }
+      if (t1 > t_min) { t_min = t1; axis = k; }
+      if (t2 < t_max) t_max = t2;
+      if (t_min > t_max) return false;
+    }
+  }
+  if (!(axis >= 0 && t_min > 1.0e-6f && t_min <= max_toi)) return false;
+  t = t_min; axis_out = axis;
+  return true;
+}
+
+struct TlasView {
+  SceneView top;            // pair nodes over instance world boxes; top.ids are instance ids
+  const float* xf;          // instance records by id
+  const float* world;       // instance world boxes by id, [n][2D]
+  const SceneView* blas;    // per mesh (INST_MESH)
+  uint32_t xs;              // floats per instance record
+};
+
+struct TlasHit {
+  float t;
+  uint32_t inst;   // NO_LEAF on a miss
+  uint32_t pos;    // BLAS position of the triangle (mesh) or the entry axis (box)
+  uint32_t sub;    // triangle id within the mesh, NO_LEAF for boxes
+};
+
+// test_instance (tlas.rs:459-481) with the instance guard.
+template <int D, int IK>
+SQ_HD bool instance_test(const TlasView& tv, const RayD<D>& r, uint32_t id, float limit, Stack st_bot, TlasHit& h) {
+  const float* x = tv.xf + (size_t)id * tv.xs;
+  float ol[D], dl[D];
+  iso_inverse<D>(x, x + D * D, r, ol, dl);
+  float t;
+  if constexpr (IK == INST_BOX) {
+    int ax;
+    if (!local_box<D>(ol, dl, x + D * D + D, limit, t, ax)) return false;
+    h.pos = (uint32_t)ax; h.sub = NO_LEAF;
+  } else {
+    const float inv_s = x[D * D + D];
+    const uint32_t mesh = fbits(x[D * D + D + 1]);
+    float src[2 * D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) { src[k] = ol[k] * inv_s; src[D + k] = dl[k] * inv_s; }
+    RayPolicy<D, KIND_TRI> pol;
+    pol.setup(src);
+    Filter nof{nullptr, ~0ull, -1};
+    NearestResult b = walk_nearest<D, Prim<D, KIND_TRI>::STRIDE, false, false>(tv.blas[mesh], pol, limit, nof, st_bot);
+    if (b.leaf == NO_LEAF) return false;
+    t = b.t; h.pos = b.pos; h.sub = b.leaf;
+  }
+  const float* w = tv.world + (size_t)id * 2 * D;
+  float te, tx; int a;
+  if (!slab_exact<D>(r, w, w + D, limit, te, tx, a)) return false;
+  if (!(te <= t && t <= tx)) return false;
+  h.t = t; h.inst = id;
+  return true;
+}
+
+// Tlas::raycast_nearest (tlas.rs:104-154), near-first; ANY stops at the first hit.
+template <int D, int IK, bool ANY>
+SQ_HD TlasHit walk_tlas(const TlasView& tv, const RayD<D>& r, float max_toi, Stack st, Stack st_bot) {
+  TlasHit res{SQ_INF, NO_LEAF, NO_LEAF, NO_LEAF};
+  float limit = max_toi;
+  const SceneView& s = tv.top;
+  {
+    float te, tx; int ax;
+    if (!slab_exact<D>(r, s.root, s.root + D, limit, te, tx, ax)) return res;
+  }
+  uint32_t cur = fbits(s.root[2 * D]);
+  uint32_t sp = 0;
+  constexpr int NF = (D + 1) * 4;
+  while (true) {
+    if (!(cur & LEAF_BIT)) {
+      const float* nd = s.nodes + (size_t)cur * NF;
+      bool ha, hb; float ta, tb;
+      slab_pair<D>(r, nd, limit, ha, hb, ta, tb);
+      const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
+      if (ha && hb) {
+        const bool a_first = ta <= tb;
+        st.ref[sp * st.stride] = a_first ? rb : ra;
+        st.t[sp * st.stride] = a_first ? tb : ta;
+        ++sp;
+        cur = a_first ? ra : rb;
+        continue;
+      }
+      if (ha) { cur = ra; continue; }
+      if (hb) { cur = rb; continue; }
+    } else {
+      const uint32_t first = cur & FIRST_MASK;
+      const uint32_t cnt = ((cur >> 27) & 15u) + 1u;
+      for (uint32_t j = 0; j < cnt; ++j) {
+        TlasHit h;
+        if (!instance_test<D, IK>(tv, r, s.ids[first + j], limit, st_bot, h)) continue;
+        if (ANY) return h;
+        if (res.inst == NO_LEAF || hit_before(h.t, h.inst, res.t, res.inst)) { res = h; limit = h.t; }
+      }
+    }
+    bool got = false;
+    while (sp > 0) {
+      --sp;
+      if (st.t[sp * st.stride] <= limit) { cur = st.ref[sp * st.stride]; got = true; break; }
+    }
+    if (!got) break;
+  }
+  return res;
+}
+
+// World normal of a final TLAS hit: the local normal lifted by R (tlas.rs:478).
+template <int D, int IK>
+SQ_HD void tlas_normal(const TlasView& tv, const RayD<D>& r, const TlasHit& h, float limit, float* n) {
+  const float* x = tv.xf + (size_t)h.inst * tv.xs;
+  float ol[D], dl[D], nl[D];
+  iso_inverse<D>(x, x + D * D, r, ol, dl);
+#pragma unroll
+  for (int k = 0; k < D; ++k) nl[k] = 0.0f;
+  if constexpr (IK == INST_BOX) {
+    nl[h.pos] = dl[h.pos] > 0.0f ? -1.0f : 1.0f;
+  } else {
+    const float inv_s = x[D * D + D];
+    const uint32_t mesh = fbits(x[D * D + D + 1]);
+    float src[2 * D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) { src[k] = ol[k] * inv_s; src[D + k] = dl[k] * inv_s; }
+    RayD<D> mr;
+    ray_setup<D>(mr, src);
+    hit_normal<D, KIND_TRI>(mr, tv.blas[mesh].prims + (size_t)h.pos * Prim<D, KIND_TRI>::STRIDE, h.t, limit, nl);
+  }
+  iso_rotate<D>(x, nl, n);
+}
+
 }  // namespace sqd

@@ -57,11 +57,22 @@ class QueryFilter:
 class Context:
     """A device and its stream (GpuRaycaster::headless: raises SqError(NO_DEVICE) when absent)."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, use_torch_stream: bool = True):
         h = C.c_void_p()
         check(lib().sq_ctx_create(device, C.byref(h)))
         self._h = h
         self.device = device
+        if use_torch_stream:
+            # Order engine work with the framework's work on the same stream.
+            import torch
+            self.set_stream(torch.cuda.current_stream(device).cuda_stream)
+
+    def set_stream(self, stream_ptr: int | None):
+        """Enqueue on this hipStream_t (0 = the legacy default stream); None = the context's own."""
+        if stream_ptr is None:
+            check(lib().sq_ctx_reset_stream(self._h))
+        else:
+            check(lib().sq_ctx_set_stream(self._h, C.c_void_p(stream_ptr)))
 
     @property
     def handle(self):
@@ -277,6 +288,99 @@ class DeviceBvh:
     def close(self):
         if getattr(self, "_h", None):
             lib().sq_bvh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tlas:
+    """Tlas<D> over built-in instance kinds (accel/tlas.rs), resident on the GPU.
+
+    kind 0 (box): halves[n, D] local boxes; kind 1 (mesh, D = 3): meshes is a
+    list of TRIANGLE DeviceBvh, mesh_of[n], scale[n] uniform scales."""
+
+    BOX, MESH = 0, 1
+
+    def __init__(self, ctx: Context, dim: int, kind: int, rot, trans, halves=None, meshes=None, mesh_of=None, scale=None):
+        rot = _host(rot).reshape(-1, dim, dim); trans = _host(trans).reshape(-1, dim)
+        n = rot.shape[0]
+        self._keep = []
+        hv = _host(halves).reshape(-1, dim) if halves is not None else None
+        if kind == self.MESH:
+            arr = (C.c_void_p * len(meshes))(*[m._h.value for m in meshes])
+            mo = _host(mesh_of, np.uint32); sc = _host(scale)
+            self._keep = [arr, mo, sc, list(meshes)]
+            mesh_ptr, nm = C.cast(arr, C.c_void_p), len(meshes)
+        else:
+            mesh_ptr, nm, mo, sc = None, 0, None, None
+        h = C.c_void_p()
+        check(lib().sq_tlas_build(ctx.handle, dim, kind, n, _hp(rot), _hp(trans), _hp(hv), mesh_ptr, nm, _hp(mo), _hp(sc),
+                                  C.byref(h)))
+        self._h, self.ctx, self.dim, self.kind, self.n = h, ctx, dim, kind, n
+
+    def refit(self, rot, trans, halves=None):
+        rot = _host(rot); trans = _host(trans)
+        hv = _host(halves) if halves is not None else None
+        check(lib().sq_tlas_refit(self._h, _hp(rot), _hp(trans), _hp(hv)))
+
+    def needs_rebuild(self, factor: float) -> bool:
+        out = C.c_int()
+        check(lib().sq_tlas_needs_rebuild(self._h, factor, C.byref(out)))
+        return bool(out.value)
+
+    def stats(self) -> dict:
+        s = TreeStats()
+        check(lib().sq_tlas_get_stats(self._h, C.byref(s)))
+        return _stats(s)
+
+    def download_bounds(self):
+        nn = self.stats()["node_count"]
+        out = np.empty((nn, 2 * self.dim), np.float32)
+        check(lib().sq_tlas_download_bounds(self._h, _hp(out)))
+        return out
+
+    def raycast_nearest(self, rays, max_toi, max_toi_per_ray=None, point=False, normal=False, sub=False):
+        import torch
+        n = rays.shape[0]
+        dev = rays.device
+        inst = torch.empty(n, dtype=torch.int32, device=dev)
+        toi = torch.empty(n, dtype=torch.float32, device=dev)
+        pt = torch.empty((n, self.dim), dtype=torch.float32, device=dev) if point else None
+        nm = torch.empty((n, self.dim), dtype=torch.float32, device=dev) if normal else None
+        sb = torch.empty(n, dtype=torch.int32, device=dev) if sub else None
+        check(lib().sq_tlas_raycast_nearest(self._h, _dp(rays), n, float(max_toi), _dp(max_toi_per_ray), _dp(inst),
+                                            _dp(toi), _dp(pt), _dp(nm), _dp(sb)))
+        out = dict(leaf=inst, toi=toi)
+        if point: out["point"] = pt
+        if normal: out["normal"] = nm
+        if sub: out["sub"] = sb
+        return out
+
+    def raycast_any(self, rays, max_toi, max_toi_per_ray=None):
+        import torch
+        out = torch.empty(rays.shape[0], dtype=torch.uint8, device=rays.device)
+        check(lib().sq_tlas_raycast_any(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(max_toi_per_ray), _dp(out)))
+        return out
+
+    def raycast_all(self, rays, max_toi, capacity=0):
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_tlas_raycast_all(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(off), _dp(leaf),
+                                             _dp(toi), cap, C.byref(total))
+        return DeviceBvh._csr(self, fn, rays.shape[0], rays.device, True, capacity)
+
+    def overlap(self, regions, capacity=0):
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_tlas_overlap(self._h, _dp(regions), regions.shape[0], _dp(off), _dp(leaf), cap, C.byref(total))
+        off, leaf, _ = DeviceBvh._csr(self, fn, regions.shape[0], regions.device, False, capacity)
+        return off, leaf
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_tlas_destroy(self._h)
             self._h = None
 
     def __del__(self):

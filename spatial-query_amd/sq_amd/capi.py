@@ -37,6 +37,18 @@ SIGNATURES = {
     "sq_ctx_destroy": [P],
     "sq_ctx_synchronize": [P],
     "sq_ctx_stream": [P],
+    "sq_ctx_set_stream": [P, P],
+    "sq_ctx_reset_stream": [P],
+    "sq_tlas_build": [P, I32, I32, U64, P, P, P, P, U32, P, P, PP],
+    "sq_tlas_destroy": [P],
+    "sq_tlas_refit": [P, P, P, P],
+    "sq_tlas_needs_rebuild": [P, F32, C.POINTER(C.c_int)],
+    "sq_tlas_get_stats": [P, C.POINTER(TreeStats)],
+    "sq_tlas_download_bounds": [P, P],
+    "sq_tlas_raycast_nearest": [P, P, U64, F32, P, P, P, P, P, P],
+    "sq_tlas_raycast_any": [P, P, U64, F32, P, P],
+    "sq_tlas_raycast_all": [P, P, U64, F32, P, P, P, U64, C.POINTER(C.c_uint64)],
+    "sq_tlas_overlap": [P, P, U64, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_ctx_set_block": [P, I32],
     "sq_ctx_last_kernel_ms": [P, C.POINTER(C.c_float)],
     "sq_tree_build": [I32, I32, P, U64, PP],

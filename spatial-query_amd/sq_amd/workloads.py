@@ -228,3 +228,77 @@ def gmm_boxes_scene(n=1_000_000, dim=6, k=32, seed=0x6A55_1A40_0000_0006) -> Sce
     h = s.range(0.05, 1.0, n * dim).reshape(n, dim)
     prims = np.concatenate([c - h, c + h], axis=1).astype(np.float32)
     return Scene("gmm_boxes", dim, 0, prims, 1000.0, dict(seed=seed, means=means))
+
+
+# --- instanced meshes --------------------------------------------------------
+
+_ICO_V = None
+
+
+def _icosahedron():
+    global _ICO_V
+    if _ICO_V is None:
+        p = (1 + 5 ** 0.5) / 2
+        v = np.array([[-1, p, 0], [1, p, 0], [-1, -p, 0], [1, -p, 0], [0, -1, p], [0, 1, p], [0, -1, -p], [0, 1, -p],
+                      [p, 0, -1], [p, 0, 1], [-p, 0, -1], [-p, 0, 1]], np.float64)
+        f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                      [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                      [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]])
+        _ICO_V = (v / np.linalg.norm(v, axis=1, keepdims=True), f)
+    return _ICO_V
+
+
+def noisy_sphere(freq: int, seed: int, amp=0.15) -> np.ndarray:
+    """Geodesic sphere: each icosahedron face split into freq^2 triangles,
+    projected to the unit sphere and displaced radially by a seeded sum of
+    three sines of the direction.  Returns [20 freq^2, 9] f32 triangles."""
+    v, f = _icosahedron()
+    s = Stream(seed)
+    k = s.range(1.0, 6.0, 9).astype(np.float64).reshape(3, 3)
+    ph = s.range(0.0, 2 * math.pi, 3).astype(np.float64)
+    tris = []
+    n = freq
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    up = (ii + jj) < n                      # upward triangles
+    dn = (ii + jj) < n - 1                  # downward triangles
+    for a, b, c in f:
+        A, B, Cc = v[a], v[b], v[c]
+
+        def P(i, j):
+            w = np.stack([i / n, j / n], -1)
+            return A + w[..., :1] * (B - A) + w[..., 1:] * (Cc - A)
+        i0, j0 = ii[up], jj[up]
+        tris.append(np.stack([P(i0, j0), P(i0 + 1, j0), P(i0, j0 + 1)], 1))
+        i1, j1 = ii[dn], jj[dn]
+        tris.append(np.stack([P(i1 + 1, j1), P(i1 + 1, j1 + 1), P(i1, j1 + 1)], 1))
+    t = np.concatenate(tris)                # [T, 3, 3]
+    d = t / np.linalg.norm(t, axis=-1, keepdims=True)
+    r = 1.0 + amp * np.sum(np.sin(d @ k.T + ph), axis=-1, keepdims=True) / 3.0
+    return np.ascontiguousarray((d * r).reshape(-1, 9), np.float32)
+
+
+def random_rotations(stream: Stream, n: int) -> np.ndarray:
+    """Uniform random rotations (Shoemake quaternions) as row-major f32 3x3."""
+    u = stream.range(0.0, 1.0, n * 3).astype(np.float64).reshape(n, 3)
+    a, b = np.sqrt(1 - u[:, 0]), np.sqrt(u[:, 0])
+    x, y = a * np.sin(2 * math.pi * u[:, 1]), a * np.cos(2 * math.pi * u[:, 1])
+    z, w = b * np.sin(2 * math.pi * u[:, 2]), b * np.cos(2 * math.pi * u[:, 2])
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                  2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                  2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], 1)
+    return np.ascontiguousarray(R.reshape(n, 3, 3), np.float32)
+
+
+def instanced_scene(n_meshes=16, n_inst=4096, tri_lo=2000, tri_hi=50000, extent=1000.0, seed=0x2C2C_0000_0007):
+    """Meshes with triangle counts log-uniform in [tri_lo, tri_hi] (rounded to
+    20 f^2), instances with uniform rotations, translations U[-extent, extent]^3
+    and uniform scales U[0.5, 4]."""
+    s = Stream(seed)
+    counts = np.exp(s.range(math.log(tri_lo), math.log(tri_hi), n_meshes).astype(np.float64))
+    freqs = np.maximum(1, np.round(np.sqrt(counts / 20.0))).astype(int)
+    meshes = [noisy_sphere(int(fq), seed + 101 * (m + 1)) for m, fq in enumerate(freqs)]
+    mesh_of = (s.u64(n_inst) % np.uint64(n_meshes)).astype(np.uint32)
+    rot = random_rotations(s, n_inst)
+    trans = s.range(-extent, extent, n_inst * 3).reshape(n_inst, 3)
+    scale = s.range(0.5, 4.0, n_inst)
+    return dict(meshes=meshes, mesh_of=mesh_of, rot=rot, trans=trans, scale=scale)
