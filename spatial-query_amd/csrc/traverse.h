@@ -25,8 +25,12 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define SQ_HD __host__ __device__ __forceinline__
-#define SQ_DIV_RN(a, b) __fdiv_rn((a), (b))
-#define SQ_SQRT_RN(a) __fsqrt_rn(a)
+// IEEE division and square root (correctly rounded): plain `/` and
+// __builtin_sqrtf lower to the correctly rounded sequences under HIP's default
+// -fhip-fp32-correctly-rounded-divide-sqrt.  (__fsqrt_rn is the native,
+// approximate sqrt in this toolchain unless OCML_BASIC_ROUNDED_OPERATIONS.)
+#define SQ_DIV_RN(a, b) ((a) / (b))
+#define SQ_SQRT_RN(a) __builtin_sqrtf(a)
 #else
 #include <cmath>
 #include <cstring>
