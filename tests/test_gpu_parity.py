@@ -27,9 +27,15 @@ def run_nearest(ctx, dim, kind, prims, rays, max_toi, **kw):
     return b, {k: host(v) for k, v in r.items()}
 
 
-def ball_scene(dim, n, seed):
+def span(dim):
+    """Half-width of the scene cube: smaller in high dimension so a few
+    thousand primitives still meet many rays."""
+    return {2: 30.0, 3: 30.0, 4: 14.0, 5: 9.0, 6: 7.0}[dim]
+
+
+def ball_scene(dim, n, seed, half=50.0):
     s = W.Stream(seed)
-    c = s.range(-50.0, 50.0, n * dim).reshape(n, dim)
+    c = s.range(-half, half, n * dim).reshape(n, dim)
     r = s.range(0.5, 4.0, n).reshape(n, 1)
     return np.concatenate([c, r], 1).astype(np.float32)
 
@@ -125,11 +131,12 @@ def test_per_ray_max_toi_and_filter(ctx):
 @pytest.mark.parametrize("kind,dim", [(oracle.BOX, 3), (oracle.BOX, 6), (oracle.SPHERE, 3), (oracle.SPHERE, 5)])
 def test_all_hits_crossings(ctx, kind, dim):
     print(f"case: all-hits / crossings kind={kind} D={dim}", flush=True)
+    w = span(dim)
     if kind == oracle.BOX:
-        prims = W.boxes_scene(n=5000, dim=dim, lo=-30.0, hi=30.0, hmin=0.5, hmax=3.0).prims
+        prims = W.boxes_scene(n=5000, dim=dim, lo=-w, hi=w, hmin=0.5, hmax=3.0).prims
     else:
-        prims = ball_scene(dim, 5000, 0xA11)
-    rays = W.random_rays(8000, 0xA11 + dim, -40.0, 40.0, dim=dim)
+        prims = ball_scene(dim, 5000, 0xA11, half=w)
+    rays = W.random_rays(8000, 0xA11 + dim, -w - 10.0, w + 10.0, dim=dim)
     o = oracle.Scene(dim, kind, prims)
     b = sq_amd.DeviceBvh.build(ctx, dim, kind, prims)
     for cross in (False, True):
@@ -142,14 +149,15 @@ def test_all_hits_crossings(ctx, kind, dim):
 @pytest.mark.parametrize("kind,dim", [(oracle.BOX, 3), (oracle.BOX, 6), (oracle.SPHERE, 4), (oracle.TRI, 3)])
 def test_overlap_sorted_ids(ctx, kind, dim):
     print(f"case: overlap kind={kind} D={dim}", flush=True)
+    w = span(dim)
     if kind == oracle.BOX:
-        prims = W.boxes_scene(n=20000, dim=dim, lo=-30.0, hi=30.0, hmin=0.2, hmax=2.0).prims
+        prims = W.boxes_scene(n=20000, dim=dim, lo=-w, hi=w, hmin=0.2, hmax=2.0).prims
     elif kind == oracle.SPHERE:
-        prims = ball_scene(dim, 20000, 0x0E)
+        prims = ball_scene(dim, 20000, 0x0E, half=w)
     else:
         prims = W.heightfield_scene(grid=129, extent=64.0).prims
     s = W.Stream(0x0E + dim)
-    c = s.range(-30.0, 30.0, 3000 * dim).reshape(-1, dim)
+    c = s.range(-w, w, 3000 * dim).reshape(-1, dim)
     h = s.range(1.0, 8.0, 3000 * dim).reshape(-1, dim)
     if kind == oracle.TRI:
         c[:, 2] = 0.0
@@ -169,11 +177,12 @@ def test_overlap_sorted_ids(ctx, kind, dim):
 @pytest.mark.parametrize("kind,dim", [(oracle.BOX, 3), (oracle.BOX, 6), (oracle.SPHERE, 3)])
 def test_shapecast_nearest_all(ctx, kind, dim):
     print(f"case: shapecast kind={kind} D={dim}", flush=True)
+    w = span(dim)
     if kind == oracle.BOX:
-        prims = W.boxes_scene(n=8000, dim=dim, lo=-30.0, hi=30.0, hmin=0.3, hmax=2.0).prims
+        prims = W.boxes_scene(n=8000, dim=dim, lo=-w, hi=w, hmin=0.3, hmax=2.0).prims
     else:
-        prims = ball_scene(dim, 8000, 0x5C)
-    rays = W.random_rays(6000, 0x5C + dim, -40.0, 40.0, dim=dim)
+        prims = ball_scene(dim, 8000, 0x5C, half=w)
+    rays = W.random_rays(6000, 0x5C + dim, -w - 10.0, w + 10.0, dim=dim)
     half = np.float32(0.5)
     casts = np.concatenate([rays, np.full((len(rays), dim), -half), np.full((len(rays), dim), half)], 1).astype(np.float32)
     tmax = W.Stream(0x5D).range(0.0, 50.0, len(rays))
