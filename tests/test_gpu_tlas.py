@@ -29,9 +29,12 @@ def plane_rot(dim, a, b, ang):
     return r
 
 
+SPAN = {2: 40.0, 3: 40.0, 5: 9.0}
+
+
 def box_instances(dim, n, seed):
     s = W.Stream(seed)
-    tr = s.range(-40.0, 40.0, n * dim).reshape(n, dim)
+    tr = s.range(-SPAN[dim], SPAN[dim], n * dim).reshape(n, dim)
     rot = np.stack([plane_rot(dim, i % dim, (i + 1) % dim, float(a)) for i, a in enumerate(s.range(0.0, 6.28, n))])
     half = np.maximum(s.range(0.5, 2.5, n * dim).reshape(n, dim), np.float32(0.5))
     return rot, tr, half
@@ -43,13 +46,13 @@ def test_box_instances_nearest_any_all_refit(ctx, dim):
     rot, tr, half = box_instances(dim, 3000, 0xB0C5 + dim)
     o = oracle.InstanceScene(dim, oracle.INST_BOX_GUARDED, rot, tr, halves=half)
     t = sq_amd.Tlas(ctx, dim, sq_amd.Tlas.BOX, rot, tr, halves=half)
-    rays = W.random_rays(20000, 0x7A + dim, -60.0, 60.0, dim=dim)
+    rays = W.random_rays(20000, 0x7A + dim, -SPAN[dim] - 20.0, SPAN[dim] + 20.0, dim=dim)
     for frame in range(3):
         ref = o.raycast_nearest(rays, 500.0, threads=8)
         got = t.raycast_nearest(dev(rays), 500.0, point=True, normal=True, sub=True)
         assert_nearest_equal(host(got["leaf"]), host(got["toi"]), ref["leaf"], ref["t"], f"tlas box frame {frame}")
         hit = ref["leaf"] >= 0
-        assert hit.sum() > 500
+        assert hit.sum() > 100
         assert same_bits(host(got["normal"])[hit], ref["normal"][hit]).all()
         assert same_bits(host(got["point"])[hit], ref["point"][hit]).all()
         assert np.array_equal(host(t.raycast_any(dev(rays), 500.0)).astype(bool), hit)
