@@ -31,9 +31,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="rays in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 21, help="rays in the CPU baseline sample (0 = skip)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--quick", action="store_true", help="short run for profiling: no CPU baseline, no side figures")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a 1 GiB device copy (the known-size kernel that calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not produced (see DESIGN.md)",
@@ -157,6 +159,14 @@ def main():
                                "seconds": round(cpu_s, 2)}
         out["id_agreement_on_sample"] = float(agree.mean())
         log(f"cpu baseline {m / cpu_s / 1e6:.3f} Mrays/s on {cores} threads; agreement {agree.mean():.6f}")
+    if args.copy_calib:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_calib_bytes"] = {"read": a.numel() * 4, "written": b.numel() * 4}
     print(json.dumps(out), flush=True)
 
 
