@@ -96,6 +96,13 @@ def main():
     n_prim = prims.double().sum().item()
     bytes_launch = n * (24 + 8) + 32 * n_node + 40 * n_prim
     achieved = bytes_launch / (k_ms * 1e-3) / 1e9
+    # HBM traffic per launch, from the FETCH_SIZE / WRITE_SIZE profile of this kernel
+    # (profiles/k_nearest_c1_heightfield.md), scaled by the copy-kernel calibration.
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_c1_nearest.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = int(json.load(fh)["traffic_bytes"])
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -115,7 +122,7 @@ def main():
                    "rays_per_step": n, "tree_nodes": st["node_count"], "tree_depth": st["max_depth"],
                    "block": args.block or 256},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel_ms": round(k_ms, 4), "algorithmic_bytes_per_launch": int(bytes_launch),
                      "mean_nodes_per_ray": round(n_node / n, 2), "mean_prims_per_ray": round(n_prim / n, 2)},
     }
