@@ -33,6 +33,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-sample", type=int, default=1 << 21, help="rays in the CPU baseline sample (0 = skip)")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--rays-per-lane", type=int, default=-1, help="pooled kernel pool size per lane (0 = one ray per thread)")
+    ap.add_argument("--lds-stack", type=int, default=0)
     ap.add_argument("--quick", action="store_true", help="short run for profiling: no CPU baseline, no side figures")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB device copy (the known-size kernel that calibrates HBM counters)")
@@ -54,8 +56,14 @@ def main():
     log(f"generated C1: {sc.prims.shape[0]} tris, {n} rays in {time.time() - t0:.1f}s")
 
     ctx = sq_amd.Context(0)
+    tun = ctx.tuning()
     if args.block:
-        ctx.set_block(args.block)
+        tun["block"] = args.block
+    if args.rays_per_lane >= 0:
+        tun["rays_per_lane"] = args.rays_per_lane
+    if args.lds_stack:
+        tun["lds_stack"] = args.lds_stack
+    ctx.set_tuning(**tun)
     t0 = time.time()
     tree = sq_amd.Tree.build(3, sq_amd.TRIANGLE, sc.prims)
     t_build = time.time() - t0
@@ -120,7 +128,7 @@ def main():
         "config": {"workload": "C1 heightfield: 1025x1025 grid, 2,097,152 triangles; 4,194,304 coherent + "
                                "4,194,304 incoherent rays; nearest-hit, t_max 2000",
                    "rays_per_step": n, "tree_nodes": st["node_count"], "tree_depth": st["max_depth"],
-                   "block": args.block or 256},
+                   "tuning": ctx.tuning()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel_ms": round(k_ms, 4), "algorithmic_bytes_per_launch": int(bytes_launch),

@@ -218,8 +218,23 @@ int sq_tlas_raycast_all(sq_tlas* tlas, const float* rays, uint64_t n, float max_
 int sq_tlas_overlap(sq_tlas* tlas, const float* regions, uint64_t n, uint64_t* offsets, int32_t* out_inst,
                     uint64_t capacity, uint64_t* total);
 
-/* Kernel tuning knobs (block size in threads, a multiple of 64; 0 = default).
- * The autotuner sets these; results never depend on them. */
+/* Kernel tuning knobs.  Results never depend on them (the tests check this);
+ * only speed does.  The autotuner (sq_amd.autotune) picks them per scene.
+ *   block          threads per block, a multiple of 64 (pooled kernels: <= 256)
+ *   rays_per_lane  nearest / any / shape-cast: each 64-lane wave owns a pool of
+ *                  64 * rays_per_lane consecutive queries and refills finished
+ *                  lanes from it; 0 selects one query per thread with the whole
+ *                  stack in LDS
+ *   lds_stack      entries of each lane's traversal stack kept in LDS (power of
+ *                  two); deeper entries spill to device memory */
+typedef struct sq_tuning {
+  int block;
+  int rays_per_lane;
+  int lds_stack;
+} sq_tuning;
+int sq_ctx_set_tuning(sq_ctx* ctx, const sq_tuning* tuning);
+int sq_ctx_get_tuning(const sq_ctx* ctx, sq_tuning* tuning);
+/* Block size only (0 = default). */
 int sq_ctx_set_block(sq_ctx* ctx, int block_threads);
 
 /* Time (ms) of the last query kernel on this context, from HIP events recorded

@@ -59,6 +59,88 @@ __global__ __launch_bounds__(1024) void k_nearest(SceneView s, const float* __re
   if (out.sub) out.sub[i] = (hit && KIND != KIND_AABB) ? (int32_t)h.leaf : -1;
 }
 
+
+// Pooled nearest / any-hit / shape-cast kernel.  Each 64-lane wave owns a pool
+// of 64 * per_lane consecutive queries; a lane that finishes its query takes
+// the next one from the pool (ballot + popcount, no atomics), so lanes stay
+// busy while long queries run.  Each step descends to one leaf and tests it
+// (the walk of traverse.h).  Stack: an LDS ring of S entries per lane spilling
+// to a global column.
+template <int D, int KIND, bool ANY, bool FILT, bool CAST>
+__global__ __launch_bounds__(256) void k_nearest_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
+                                                      const float* __restrict__ tmax, Filter f, NearestOut out,
+                                                      uint32_t per_lane, uint32_t S, SpillEntry* spill, uint32_t spill_stride) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  const uint32_t gtid = blockIdx.x * B + tid;
+  const uint32_t lane = tid & 63u;
+  using Pol = typename std::conditional<CAST, CastPolicy<D, KIND>, RayPolicy<D, KIND>>::type;
+  constexpr int QS = CAST ? 4 * D : 2 * D;
+  constexpr int PS = Prim<D, KIND>::STRIDE;
+  const uint64_t pool = 64ull * per_lane;
+  uint64_t next = ((uint64_t)blockIdx.x * (B >> 6) + (tid >> 6)) * pool;   // wave-uniform
+  const uint64_t end = next + pool < (uint64_t)n ? next + pool : (uint64_t)n;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
+  Pol pol;
+  WalkState w;
+  float lim0 = 0.0f;
+  uint32_t rid = NO_LEAF;
+  bool exhausted = next >= end;
+  while (true) {
+    const bool need = rid == NO_LEAF && !exhausted;
+    const uint64_t m = __ballot(need);
+    if (m) {
+      if (need) {
+        const uint64_t my = next + __popcll(m & lt_mask);
+        if (my < end) {
+          rid = (uint32_t)my;
+          pol.setup(q + (size_t)rid * QS);
+          lim0 = tmax ? tmax[rid] : max_toi;
+          if (!walk_begin<D>(s, pol, lim0, w)) {
+            if (ANY) out.any[rid] = 0;
+            else {
+              out.leaf[rid] = -1; out.toi[rid] = SQ_INF;
+              if (out.point) for (int k = 0; k < D; ++k) out.point[(size_t)rid * D + k] = 0.0f;
+              if (!CAST && out.normal) for (int k = 0; k < D; ++k) out.normal[(size_t)rid * D + k] = 0.0f;
+              if (out.sub) out.sub[rid] = -1;
+            }
+            rid = NO_LEAF;
+          }
+        } else {
+          exhausted = true;
+        }
+      }
+      next += __popcll(m);
+    }
+    if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
+    if (rid == NO_LEAF) continue;
+    if (!walk_step<D, PS, ANY, FILT>(s, pol, f, st, w)) continue;
+    // finished: write the result of query rid
+    const NearestResult& h = w.res;
+    const bool hit = h.leaf != NO_LEAF;
+    if (ANY) {
+      out.any[rid] = hit ? 1 : 0;
+    } else {
+      out.leaf[rid] = hit ? (int32_t)h.leaf : -1;
+      out.toi[rid] = h.t;
+      if (out.point) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) out.point[(size_t)rid * D + k] = hit ? pol.r.o[k] + pol.r.d[k] * h.t : 0.0f;
+      }
+      if (!CAST && out.normal) {
+        float nv[D];
+        if (hit) hit_normal<D, KIND>(pol.r, s.prims + (size_t)h.pos * PS, h.t, lim0, nv);
+        else for (int k = 0; k < D; ++k) nv[k] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) out.normal[(size_t)rid * D + k] = nv[k];
+      }
+      if (out.sub) out.sub[rid] = (hit && KIND != KIND_AABB) ? (int32_t)h.leaf : -1;
+    }
+    rid = NO_LEAF;
+  }
+}
+
 template <int D>
 __global__ __launch_bounds__(1024) void k_required(SceneView s, const float* __restrict__ rays, uint32_t n, const float* __restrict__ tlim,
                                                    uint32_t* out_nodes, uint32_t* out_prims, uint32_t stack_n) {
@@ -485,10 +567,32 @@ int sq_ctx_synchronize(sq_ctx* c) {
 
 void* sq_ctx_stream(sq_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
+int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
+  if (!c || !t) return fail(SQ_ERR_INVALID, "NULL argument");
+  const int block = t->block ? t->block : 64;
+  if (block < 64 || block > 256 || block % 64) return fail(SQ_ERR_INVALID, "block must be 64, 128, 192 or 256");
+  if (t->rays_per_lane < 0 || t->rays_per_lane > 4096) return fail(SQ_ERR_INVALID, "rays_per_lane out of range");
+  const int S = t->lds_stack ? t->lds_stack : 8;
+  if (S < 2 || S > 64 || (S & (S - 1))) return fail(SQ_ERR_INVALID, "lds_stack must be a power of two in [2, 64]");
+  c->block = block;
+  c->rays_per_lane = t->rays_per_lane;
+  c->lds_stack = S;
+  return SQ_OK;
+}
+
+int sq_ctx_get_tuning(const sq_ctx* c, sq_tuning* t) {
+  if (!c || !t) return fail(SQ_ERR_INVALID, "NULL argument");
+  t->block = c->block;
+  t->rays_per_lane = c->rays_per_lane;
+  t->lds_stack = c->lds_stack;
+  return SQ_OK;
+}
+
 int sq_ctx_set_block(sq_ctx* c, int block) {
   if (!c) return fail(SQ_ERR_INVALID, "ctx is NULL");
-  if (block == 0) block = 256;
+  if (block == 0) block = 64;
   if (block < 64 || block > 1024 || block % 64) return fail(SQ_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
+  if (c->rays_per_lane && block > 256) return fail(SQ_ERR_INVALID, "the pooled kernels take blocks of at most 256");
   c->block = block;
   return SQ_OK;
 }
@@ -744,16 +848,47 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
   bool filt;
   Filter fd = filter_of(b, f, &filt);
   const uint32_t stack_n = b->packed.max_stack;
-  const size_t lds = (size_t)stack_n * B * 8;
-  if (lds > 160 * 1024) return fail(SQ_ERR_INVALID, "traversal stack does not fit in LDS; lower the block size");
   SceneView s = view_of(b);
-  int rc = begin_timing(c);
-  if (rc) return rc;
-  rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
-    dim3 grid(grid_for(n, B)), block(B);
-    const uint32_t nn = (uint32_t)n;
+  const uint32_t nn = (uint32_t)n;
+  if (c->rays_per_lane == 0) {
+    // one query per thread, whole stack in LDS
+    const size_t lds = (size_t)stack_n * B * 8;
+    if (lds > 160 * 1024) return fail(SQ_ERR_INVALID, "traversal stack does not fit in LDS; lower the block size");
+    int rc = begin_timing(c);
+    if (rc) return rc;
+    rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+      dim3 grid(grid_for(n, B)), block(B);
 #define SQ_LAUNCH(ANY_, FILT_, CAST_) \
-    k_nearest<D, K, ANY_, FILT_, CAST_><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, out, stack_n)
+      k_nearest<D, K, ANY_, FILT_, CAST_><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, out, stack_n)
+      if (cast) { if (filt) SQ_LAUNCH(false, true, true); else SQ_LAUNCH(false, false, true); }
+      else if (any) { if (filt) SQ_LAUNCH(true, true, false); else SQ_LAUNCH(true, false, false); }
+      else { if (filt) SQ_LAUNCH(false, true, false); else SQ_LAUNCH(false, false, false); }
+#undef SQ_LAUNCH
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    });
+    if (rc) return rc;
+    return end_timing(c);
+  }
+  // pooled kernel
+  const uint32_t S = (uint32_t)c->lds_stack;
+  const uint64_t pool = 64ull * (uint64_t)c->rays_per_lane;
+  const uint64_t waves = (n + pool - 1) / pool;
+  const uint64_t blocks = (waves + (B / 64) - 1) / (B / 64);
+  const uint64_t lanes = blocks * (uint64_t)B;
+  if (lanes > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "too many lanes");
+  const uint32_t spill_rows = stack_n > S ? stack_n - S : 0;
+  int rc;
+  if ((rc = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64))) return rc;
+  const size_t lds = (size_t)S * B * 8;
+  rc = begin_timing(c);
+  if (rc) return rc;
+  SpillEntry* spill = c->spill.as<SpillEntry>();
+  const uint32_t spl = (uint32_t)lanes, per = (uint32_t)c->rays_per_lane;
+  rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+    dim3 grid((uint32_t)blocks), block(B);
+#define SQ_LAUNCH(ANY_, FILT_, CAST_) \
+    k_nearest_pool<D, K, ANY_, FILT_, CAST_><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, out, per, S, spill, spl)
     if (cast) { if (filt) SQ_LAUNCH(false, true, true); else SQ_LAUNCH(false, false, true); }
     else if (any) { if (filt) SQ_LAUNCH(true, true, false); else SQ_LAUNCH(true, false, false); }
     else { if (filt) SQ_LAUNCH(false, true, false); else SQ_LAUNCH(false, false, false); }

@@ -57,10 +57,13 @@ struct sq_ctx {
   hipStream_t own = nullptr;       // the stream the context created
   hipStream_t stream = nullptr;    // where work is enqueued (own, or the caller's)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int block = 256;
+  int block = 64;
+  int rays_per_lane = 8;           // pooled nearest kernel: rays per lane (0 = one ray per thread)
+  int lds_stack = 8;               // pooled kernel: ring-stack entries kept in LDS per lane
   bool timed = false;
   int refs = 1;                    // the caller's handle + one per live tree
   DevBuf counts, scratch, keys, keys2;   // reusable temporaries of the CSR queries
+  DevBuf spill;                    // ring-stack spill columns of the pooled kernel
 };
 
 // Drops one reference; the last one frees the stream, events and buffers.

@@ -563,6 +563,111 @@ SQ_HD float key_toi(uint64_t k) {
 }
 
 // ---------------------------------------------------------------------------
+// Resumable nearest walk with a ring stack (the pooled kernel's per-lane state)
+// ---------------------------------------------------------------------------
+
+struct SpillEntry {
+  uint32_t ref;
+  uint32_t t;    // bits of the entry distance
+};
+
+// The newest S entries of a lane's stack live in LDS slots k & (S-1); when a
+// push would overwrite an entry, that older entry moves to the lane's spill
+// column (entry k - S at spill[(k - S) * spill_stride]); the matching pop
+// brings it back.  S is a power of two.
+struct RingStack {
+  uint32_t* ref;
+  float* t;
+  uint32_t stride;          // LDS slot j at ref[j * stride]
+  uint32_t S;
+  SpillEntry* spill;        // this lane's column
+  uint32_t spill_stride;
+  SQ_HD void push(uint32_t& sp, uint32_t r, float tt) const {
+    const uint32_t slot = (sp & (S - 1)) * stride;
+    if (sp >= S) spill[(size_t)(sp - S) * spill_stride] = SpillEntry{ref[slot], fbits(t[slot])};
+    ref[slot] = r;
+    t[slot] = tt;
+    ++sp;
+  }
+  SQ_HD void pop(uint32_t& sp, uint32_t& r, float& tt) const {
+    --sp;
+    const uint32_t slot = (sp & (S - 1)) * stride;
+    r = ref[slot];
+    tt = t[slot];
+    if (sp >= S) {
+      const SpillEntry e = spill[(size_t)(sp - S) * spill_stride];
+      ref[slot] = e.ref;
+      t[slot] = bitsf(e.t);
+    }
+  }
+};
+
+struct WalkState {
+  uint32_t cur, sp;
+  float limit;
+  NearestResult res;
+};
+
+// Start a query: false when it misses the root box (the result is a miss).
+template <int D, class Q>
+SQ_HD bool walk_begin(const SceneView& s, const Q& q, float max_toi, WalkState& w) {
+  w.res = NearestResult{SQ_INF, NO_LEAF, NO_LEAF};
+  w.limit = max_toi;
+  w.sp = 0;
+  w.cur = fbits(s.root[2 * D]);
+  return q.root(s.root, max_toi);
+}
+
+// One step: descend through interior nodes to the next leaf (near-first), then
+// test that leaf's primitives.  Returns true when the query is finished.
+template <int D, int PS, bool ANY, bool FILT, class Q>
+SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const RingStack& st, WalkState& w) {
+  constexpr int NF = (D + 1) * 4;
+  while (!(w.cur & LEAF_BIT)) {
+    const float* nd = s.nodes + (size_t)w.cur * NF;
+    bool ha, hb; float ta, tb;
+    q.pair(nd, w.limit, ha, hb, ta, tb);
+    const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
+    if (ha && hb) {
+      const bool a_first = ta <= tb;
+      st.push(w.sp, a_first ? rb : ra, a_first ? tb : ta);
+      w.cur = a_first ? ra : rb;
+    } else if (ha) {
+      w.cur = ra;
+    } else if (hb) {
+      w.cur = rb;
+    } else {
+      bool got = false;
+      while (w.sp > 0) {
+        uint32_t r; float tt;
+        st.pop(w.sp, r, tt);
+        if (tt <= w.limit) { w.cur = r; got = true; break; }
+      }
+      if (!got) return true;
+    }
+  }
+  const uint32_t first = w.cur & FIRST_MASK;
+  const uint32_t cnt = ((w.cur >> 27) & 15u) + 1u;
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const uint32_t pos = first + j;
+    float t;
+    if (!q.narrow(s.prims + (size_t)pos * PS, w.limit, t)) continue;
+    const uint32_t leaf = s.ids[pos];
+    if (FILT && !accepts(f, leaf)) continue;
+    if (ANY) { w.res.t = t; w.res.leaf = leaf; w.res.pos = pos; return true; }
+    if (w.res.leaf == NO_LEAF || hit_before(t, leaf, w.res.t, w.res.leaf)) {
+      w.res.t = t; w.res.leaf = leaf; w.res.pos = pos; w.limit = t;
+    }
+  }
+  while (w.sp > 0) {
+    uint32_t r; float tt;
+    st.pop(w.sp, r, tt);
+    if (tt <= w.limit) { w.cur = r; return false; }
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // Two-level walk (crates/spatial-query/src/accel/tlas.rs) over built-in
 // instance kinds.  Instance records by id, XS floats each:
 //   INST_BOX  (any D): R[D*D] row-major, t[D], half[D]          two_level.rs:70-113

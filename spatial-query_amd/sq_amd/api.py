@@ -18,7 +18,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .capi import Filter, SqError, TreeStats, check, lib, SQ_ERR_CAPACITY
+from .capi import Filter, SqError, TreeStats, Tuning, check, lib, SQ_ERR_CAPACITY
 
 ALL_LAYERS = (1 << 64) - 1
 
@@ -86,6 +86,14 @@ class Context:
 
     def set_block(self, threads: int):
         check(lib().sq_ctx_set_block(self._h, threads))
+
+    def set_tuning(self, block: int = 64, rays_per_lane: int = 8, lds_stack: int = 8):
+        check(lib().sq_ctx_set_tuning(self._h, C.byref(Tuning(block, rays_per_lane, lds_stack))))
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        check(lib().sq_ctx_get_tuning(self._h, C.byref(t)))
+        return dict(block=t.block, rays_per_lane=t.rays_per_lane, lds_stack=t.lds_stack)
 
     def last_kernel_ms(self) -> float:
         ms = C.c_float()

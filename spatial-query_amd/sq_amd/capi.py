@@ -26,6 +26,10 @@ class TreeStats(C.Structure):
                 ("max_depth", C.c_uint64), ("bytes", C.c_uint64), ("quality", C.c_float)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("block", C.c_int), ("rays_per_lane", C.c_int), ("lds_stack", C.c_int)]
+
+
 P, U64, I64, F32, I32, U32 = C.c_void_p, C.c_uint64, C.c_int64, C.c_float, C.c_int, C.c_uint32
 PP = C.POINTER(C.c_void_p)
 
@@ -50,6 +54,8 @@ SIGNATURES = {
     "sq_tlas_raycast_all": [P, P, U64, F32, P, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_tlas_overlap": [P, P, U64, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_ctx_set_block": [P, I32],
+    "sq_ctx_set_tuning": [P, C.POINTER(Tuning)],
+    "sq_ctx_get_tuning": [P, C.POINTER(Tuning)],
     "sq_ctx_last_kernel_ms": [P, C.POINTER(C.c_float)],
     "sq_tree_build": [I32, I32, P, U64, PP],
     "sq_tree_from_nodes": [I32, P, P, P, P, U64, P, U64, PP],

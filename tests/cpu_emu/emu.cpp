@@ -6,6 +6,8 @@
 // test suite check the kernel logic against the oracle on the CPU; it is
 // never part of the product and never a fallback for it.
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -48,13 +50,44 @@ template <class F> void par_for(uint64_t n, int threads, F&& f) {
   for (auto& t : th) t.join();
 }
 
+// The pooled kernel's per-lane walk: walk_begin + walk_step with a ring stack
+// of `ring` LDS slots (here a local array) spilling to a bounds-checked column.
+template <int D, int K, class Pol>
+NearestResult ring_walk(const Built& b, const Pol& pol, float max_toi, int any, uint32_t ring) {
+  constexpr int PS = Prim<D, K>::STRIDE;
+  const SceneView s = b.view();
+  Filter f{nullptr, ~0ull, -1};
+  std::vector<uint32_t> lref(ring);
+  std::vector<float> lt(ring);
+  const uint32_t rows = b.packed.max_stack > ring ? b.packed.max_stack - ring : 0;
+  std::vector<SpillEntry> col(rows + 1);
+  RingStack st{lref.data(), lt.data(), 1, ring, col.data(), 1};
+  WalkState w;
+  if (!walk_begin<D>(s, pol, max_toi, w)) return w.res;
+  for (;;) {
+    const bool done = any ? walk_step<D, PS, true, false>(s, pol, f, st, w) : walk_step<D, PS, false, false>(s, pol, f, st, w);
+    if (w.sp > b.packed.max_stack) { fprintf(stderr, "stack bound exceeded\n"); abort(); }
+    if (done) return w.res;
+  }
+}
+
 template <int D, int K>
 void nearest_dk(const Built& b, const float* q, uint64_t nq, float max_toi, int any, int cast, int threads,
-                int32_t* leaf, float* toi) {
+                int32_t* leaf, float* toi, uint32_t ring) {
   constexpr int PS = Prim<D, K>::STRIDE;
   const SceneView s = b.view();
   const uint32_t S = b.packed.max_stack;
   Filter f{nullptr, ~0ull, -1};
+  if (ring) {
+    par_for(nq, threads, [&](uint64_t i) {
+      NearestResult h;
+      if (cast) { CastPolicy<D, K> pol; pol.setup(q + i * 4 * D); h = ring_walk<D, K>(b, pol, max_toi, 0, ring); }
+      else { RayPolicy<D, K> pol; pol.setup(q + i * 2 * D); h = ring_walk<D, K>(b, pol, max_toi, any, ring); }
+      leaf[i] = h.leaf == NO_LEAF ? -1 : (int32_t)h.leaf;
+      toi[i] = h.t;
+    });
+    return;
+  }
   par_for(nq, threads, [&](uint64_t i) {
     std::vector<uint32_t> sr(S);
     std::vector<float> stt(S);
@@ -111,10 +144,10 @@ uint32_t emu_max_stack(void* h) { return ((Built*)h)->packed.max_stack; }
 uint32_t emu_n_pairs(void* h) { return ((Built*)h)->packed.n_pairs; }
 
 int emu_nearest(void* h, int dim, int kind, const float* q, uint64_t nq, float max_toi, int any, int cast, int threads,
-                int32_t* leaf, float* toi) {
+                int32_t* leaf, float* toi, uint32_t ring) {
   const Built& b = *(Built*)h;
   if (b.tree.prim_indices.empty()) { for (uint64_t i = 0; i < nq; ++i) { leaf[i] = -1; toi[i] = SQ_INF; } return 0; }
-  return dk(dim, kind, [&]<int D, int K>() { nearest_dk<D, K>(b, q, nq, max_toi, any, cast, threads, leaf, toi); });
+  return dk(dim, kind, [&]<int D, int K>() { nearest_dk<D, K>(b, q, nq, max_toi, any, cast, threads, leaf, toi, ring); });
 }
 
 // mode 0 all, 1 crossings, 2 overlap, 3 shapecast-all.  Returns the total;

@@ -24,7 +24,7 @@ def emu():
     L = C.CDLL(LIB)
     L.emu_build.restype = P; L.emu_build.argtypes = [C.c_int, C.c_int, P, C.c_uint32]
     L.emu_free.argtypes = [P]
-    L.emu_nearest.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_int, C.c_int, P, P]
+    L.emu_nearest.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_int, C.c_int, P, P, C.c_uint32]
     L.emu_collect.restype = C.c_uint64
     L.emu_collect.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, P, P]
     L.emu_required.argtypes = [P, C.c_int, P, C.c_uint64, P, P, P]
@@ -41,10 +41,10 @@ class Emu:
         self.prims = np.ascontiguousarray(prims, np.float32)
         self.h = L.emu_build(dim, kind, p(self.prims), self.prims.shape[0])
 
-    def nearest(self, q, max_toi, any_=0, cast=0):
+    def nearest(self, q, max_toi, any_=0, cast=0, ring=0):
         q = np.ascontiguousarray(q, np.float32)
         leaf = np.empty(len(q), np.int32); t = np.empty(len(q), np.float32)
-        self.L.emu_nearest(self.h, self.dim, self.kind, p(q), len(q), max_toi, any_, cast, 8, p(leaf), p(t))
+        self.L.emu_nearest(self.h, self.dim, self.kind, p(q), len(q), max_toi, any_, cast, 8, p(leaf), p(t), ring)
         return leaf.astype(np.int64), t
 
     def collect(self, q, max_toi, mode):
@@ -132,3 +132,24 @@ def test_collect_and_casts(emu, dim):
     off, leaf = e.collect(casts, 50.0, 3)
     roff, rleaf, _ = o.shapecast_all(casts, 50.0)
     assert np.array_equal(off, roff) and np.array_equal(leaf, rleaf)
+
+
+@pytest.mark.parametrize("ring", [2, 4, 8])
+def test_ring_stack_walk(emu, ring):
+    """The pooled kernel's resumable walk with an LDS ring of `ring` entries
+    (2 spills on almost every push) equals the oracle."""
+    sc = W.heightfield_scene(grid=257, extent=128.0)
+    rays = np.concatenate([W.camera_rays(sc, res=128, seed=0x11), W.incoherent_rays(sc, 30000)])
+    ref = oracle.Scene(3, oracle.TRI, sc.prims).raycast_nearest(rays, 2000.0, threads=8)
+    e = Emu(emu, 3, 2, sc.prims)
+    assert eq(*e.nearest(rays, 2000.0, ring=ring), ref["leaf"], ref["t"])
+    assert np.array_equal(e.nearest(rays, 2000.0, any_=1, ring=ring)[0] >= 0, ref["leaf"] >= 0)
+    sc6 = W.boxes_scene(n=3000, dim=6, lo=-7.0, hi=7.0, hmin=0.3, hmax=2.0)
+    q = W.random_rays(3000, 9, -17.0, 17.0, dim=6)
+    o6 = oracle.Scene(6, oracle.BOX, sc6.prims)
+    e6 = Emu(emu, 6, 0, sc6.prims)
+    r6 = o6.raycast_nearest(q, 500.0)
+    assert eq(*e6.nearest(q, 500.0, ring=ring), r6["leaf"], r6["t"])
+    casts = np.concatenate([q, np.full((len(q), 6), -0.5, np.float32), np.full((len(q), 6), 0.5, np.float32)], 1)
+    rc = o6.shapecast_nearest(casts, 50.0)
+    assert eq(*e6.nearest(casts, 50.0, cast=1, ring=ring), rc["leaf"], rc["t"])

@@ -289,17 +289,33 @@ def test_edge_cases(ctx):
     assert_nearest_equal(host(r["leaf"]), host(r["toi"]), ref["leaf"], ref["t"], "axis-parallel")
 
 
-@pytest.mark.parametrize("block", [64, 128, 512])
-def test_block_size_does_not_change_results(ctx, block):
-    print(f"case: block size {block}", flush=True)
+@pytest.mark.parametrize("tuning", [dict(block=64, rays_per_lane=0, lds_stack=8), dict(block=256, rays_per_lane=0, lds_stack=8),
+                                    dict(block=64, rays_per_lane=1, lds_stack=2), dict(block=128, rays_per_lane=8, lds_stack=4),
+                                    dict(block=256, rays_per_lane=64, lds_stack=32)])
+def test_tuning_does_not_change_results(ctx, tuning):
+    """Every kernel variant (one query per thread; pooled with LDS rings that
+    spill constantly or never) gives the same bits."""
+    print(f"case: tuning {tuning}", flush=True)
     sc = W.heightfield_scene(grid=129, extent=64.0)
-    rays = W.incoherent_rays(sc, 20000)
+    rays = np.concatenate([W.camera_rays(sc, res=128, seed=0x11), W.incoherent_rays(sc, 20000)])
+    o = oracle.Scene(3, oracle.TRI, sc.prims)
+    ref = o.raycast_nearest(rays, 2000.0, threads=8)
     b = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.TRIANGLE, sc.prims)
-    base = b.raycast_nearest(dev(rays), 2000.0)
-    ctx.set_block(block)
+    saved = ctx.tuning()
+    ctx.set_tuning(**tuning)
     try:
-        r = b.raycast_nearest(dev(rays), 2000.0)
+        r = b.raycast_nearest(dev(rays), 2000.0, normal=True)
+        anyh = b.raycast_any(dev(rays), 2000.0)
+        prims6 = W.boxes_scene(n=3000, dim=6, lo=-7.0, hi=7.0, hmin=0.3, hmax=2.0).prims
+        q6 = W.random_rays(3000, 9, -17.0, 17.0, dim=6)
+        b6 = sq_amd.DeviceBvh.build(ctx, 6, sq_amd.AABB, prims6)
+        casts = np.concatenate([q6, np.full((len(q6), 6), -0.5, np.float32), np.full((len(q6), 6), 0.5, np.float32)], 1)
+        rc = b6.shapecast_nearest(dev(casts), 50.0)
     finally:
-        ctx.set_block(256)
-    assert np.array_equal(host(r["leaf"]), host(base["leaf"]))
-    assert same_bits(host(r["toi"]), host(base["toi"])).all()
+        ctx.set_tuning(**saved)
+    assert_nearest_equal(host(r["leaf"]), host(r["toi"]), ref["leaf"], ref["t"], f"tuning {tuning}")
+    hit = ref["leaf"] >= 0
+    assert same_bits(host(r["normal"])[hit], ref["normal"][hit]).all()
+    assert np.array_equal(host(anyh).astype(bool), hit)
+    ref6 = oracle.Scene(6, oracle.BOX, prims6).shapecast_nearest(casts, 50.0)
+    assert_nearest_equal(host(rc["leaf"]), host(rc["toi"]), ref6["leaf"], ref6["t"], "cast tuning")
