@@ -67,7 +67,7 @@ __global__ __launch_bounds__(1024) void k_nearest(SceneView s, const float* __re
 // (the walk of traverse.h).  Stack: an LDS ring of S entries per lane spilling
 // to a global column.
 template <int D, int KIND, bool ANY, bool FILT, bool CAST>
-__global__ __launch_bounds__(256) void k_nearest_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
+__global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nearest_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                       const float* __restrict__ tmax, Filter f, NearestOut out,
                                                       uint32_t per_lane, uint32_t S, SpillEntry* spill, uint32_t spill_stride) {
   extern __shared__ uint32_t lds[];

@@ -86,6 +86,21 @@ SQ_HD bool accepts(const Filter& f, uint32_t leaf) {
   return (lay & f.mask) != 0;
 }
 
+// Fetch n16 16-byte chunks (pair nodes and triangle records are 16-byte
+// aligned) into registers: dwordx4 loads on the device.
+template <int N16> SQ_HD void fetch16(const float* src, float* dst) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float4* p = reinterpret_cast<const float4*>(src);
+#pragma unroll
+  for (int i = 0; i < N16; ++i) {
+    const float4 v = p[i];
+    dst[4 * i] = v.x; dst[4 * i + 1] = v.y; dst[4 * i + 2] = v.z; dst[4 * i + 3] = v.w;
+  }
+#else
+  for (int i = 0; i < 4 * N16; ++i) dst[i] = src[i];
+#endif
+}
+
 template <int D> struct RayD {
   float o[D], d[D], inv[D];
   uint32_t par;             // bit k: |d[k]| < EPSILON (the slab's parallel branch)
@@ -270,7 +285,9 @@ template <int D, int KIND> SQ_HD void hit_normal(const RayD<D>& r, const float* 
   if constexpr (KIND == KIND_AABB) {
     float tt; int ax = -1;
     narrow_aabb<D>(r, p, limit, tt, ax);
-    if (ax >= 0) n[ax] = r.d[ax] > 0.0f ? -1.0f : 1.0f;
+    // n[ax] = -sign(d[ax]) written as a select per axis (no dynamic indexing)
+#pragma unroll
+    for (int k = 0; k < D; ++k) n[k] = (k == ax) ? (r.d[k] > 0.0f ? -1.0f : 1.0f) : 0.0f;
   } else if constexpr (KIND == KIND_SPHERE) {
     float v[D];
 #pragma unroll
@@ -624,7 +641,8 @@ template <int D, int PS, bool ANY, bool FILT, class Q>
 SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const RingStack& st, WalkState& w) {
   constexpr int NF = (D + 1) * 4;
   while (!(w.cur & LEAF_BIT)) {
-    const float* nd = s.nodes + (size_t)w.cur * NF;
+    float nd[NF];
+    fetch16<D + 1>(s.nodes + (size_t)w.cur * NF, nd);
     bool ha, hb; float ta, tb;
     q.pair(nd, w.limit, ha, hb, ta, tb);
     const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
@@ -651,7 +669,15 @@ SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const Ring
   for (uint32_t j = 0; j < cnt; ++j) {
     const uint32_t pos = first + j;
     float t;
-    if (!q.narrow(s.prims + (size_t)pos * PS, w.limit, t)) continue;
+    bool hit;
+    if constexpr (PS == 12) {
+      float pr[12];
+      fetch16<3>(s.prims + (size_t)pos * 12, pr);
+      hit = q.narrow(pr, w.limit, t);
+    } else {
+      hit = q.narrow(s.prims + (size_t)pos * PS, w.limit, t);
+    }
+    if (!hit) continue;
     const uint32_t leaf = s.ids[pos];
     if (FILT && !accepts(f, leaf)) continue;
     if (ANY) { w.res.t = t; w.res.leaf = leaf; w.res.pos = pos; return true; }
@@ -835,7 +861,8 @@ SQ_HD void tlas_normal(const TlasView& tv, const RayD<D>& r, const TlasHit& h, f
 #pragma unroll
   for (int k = 0; k < D; ++k) nl[k] = 0.0f;
   if constexpr (IK == INST_BOX) {
-    nl[h.pos] = dl[h.pos] > 0.0f ? -1.0f : 1.0f;
+#pragma unroll
+    for (int k = 0; k < D; ++k) nl[k] = ((uint32_t)k == h.pos) ? (dl[k] > 0.0f ? -1.0f : 1.0f) : 0.0f;
   } else {
     const float inv_s = x[D * D + D];
     const uint32_t mesh = fbits(x[D * D + D + 1]);
