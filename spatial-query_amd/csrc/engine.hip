@@ -569,7 +569,7 @@ void* sq_ctx_stream(sq_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   if (!c || !t) return fail(SQ_ERR_INVALID, "NULL argument");
-  const int block = t->block ? t->block : 64;
+  const int block = t->block ? t->block : 128;
   if (block < 64 || block > 256 || block % 64) return fail(SQ_ERR_INVALID, "block must be 64, 128, 192 or 256");
   if (t->rays_per_lane < 0 || t->rays_per_lane > 4096) return fail(SQ_ERR_INVALID, "rays_per_lane out of range");
   const int S = t->lds_stack ? t->lds_stack : 8;
@@ -590,7 +590,7 @@ int sq_ctx_get_tuning(const sq_ctx* c, sq_tuning* t) {
 
 int sq_ctx_set_block(sq_ctx* c, int block) {
   if (!c) return fail(SQ_ERR_INVALID, "ctx is NULL");
-  if (block == 0) block = 64;
+  if (block == 0) block = 128;
   if (block < 64 || block > 1024 || block % 64) return fail(SQ_ERR_INVALID, "block must be a multiple of 64 in [64, 1024]");
   if (c->rays_per_lane && block > 256) return fail(SQ_ERR_INVALID, "the pooled kernels take blocks of at most 256");
   c->block = block;

@@ -57,7 +57,7 @@ struct sq_ctx {
   hipStream_t own = nullptr;       // the stream the context created
   hipStream_t stream = nullptr;    // where work is enqueued (own, or the caller's)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int block = 64;
+  int block = 128;
   int rays_per_lane = 8;           // pooled nearest kernel: rays per lane (0 = one ray per thread)
   int lds_stack = 8;               // pooled kernel: ring-stack entries kept in LDS per lane
   bool timed = false;

@@ -87,7 +87,7 @@ class Context:
     def set_block(self, threads: int):
         check(lib().sq_ctx_set_block(self._h, threads))
 
-    def set_tuning(self, block: int = 64, rays_per_lane: int = 8, lds_stack: int = 8):
+    def set_tuning(self, block: int = 128, rays_per_lane: int = 8, lds_stack: int = 8):
         check(lib().sq_ctx_set_tuning(self._h, C.byref(Tuning(block, rays_per_lane, lds_stack))))
 
     def tuning(self) -> dict:
