@@ -241,6 +241,55 @@ int sq_ctx_set_block(sq_ctx* ctx, int block_threads);
  * on the context stream around it (synchronises). */
 int sq_ctx_last_kernel_ms(sq_ctx* ctx, float* ms);
 
+/* ------------------------------------------------------------------------
+ * Learned subtree proposal (no counterpart in the reference, which has only the
+ * plain walk).  A small MLP maps a ray to the subtrees of the BVH cut at a
+ * fixed depth that most likely hold its nearest hit; the proposed subtrees are
+ * walked first to tighten t_max and the full exact walk then confirms, so
+ * sq_raycast_nearest_proposed returns exactly what sq_raycast_nearest returns
+ * (bvh.rs:51-124 semantics) whatever the network predicts.
+ *
+ * Network: x = (origin mapped from the scene box to [-1,1]^3, direction);
+ * 64 random Fourier features sin/cos(2 pi x.B), B ~ N(0, rff_scale^2) [6][64];
+ * 3 x 256 ReLU layers; logits over the subtrees.  BF16 weights (FP32 masters
+ * for Adam), FP32 accumulation on the matrix cores.
+ * ---------------------------------------------------------------------- */
+typedef struct sq_proposal sq_proposal;
+/* TRIANGLE scene, cut depth 1..12 (<= 4096 subtrees: the nodes at that depth
+ * and the leaves above it), Fourier scale rff_scale > 0, weights and Fourier
+ * matrix drawn from seed.  The scene must outlive the proposal. */
+int sq_proposal_create(sq_ctx* ctx, sq_bvh* scene, int cut_depth, float rff_scale, uint64_t seed, sq_proposal** out);
+int sq_proposal_destroy(sq_proposal* p);
+int sq_proposal_num_subtrees(const sq_proposal* p, uint32_t* out);
+int sq_proposal_num_params(const sq_proposal* p, uint64_t* out);
+/* Subtree of each nearest-hit leaf (device int32[n]; -1 for a miss): labels. */
+int sq_proposal_labels(sq_proposal* p, const int32_t* leaf, uint64_t n, int32_t* label);
+/* One Adam step (lr, 0.9, 0.999, 1e-8) on a batch: rays float[n][6], labels
+ * int32[n] (device; label < 0 ignored), n <= 2^24.  loss_out (host, may be
+ * NULL): mean softmax cross-entropy over labelled rays (synchronises).
+ * grads_out (host, may be NULL): stop after the backward pass, no update, and
+ * copy the gradients [dW1, db1, ..., dW4, db4], fp32 [out][in] and [out]. */
+int sq_proposal_train_step(sq_proposal* p, const float* rays, const int32_t* labels, uint64_t n, float lr, float* loss_out,
+                           float* grads_out);
+/* FP32 master weights [W1 [256][128], b1, W2 [256][256], b2, W3, b3,
+ * W4 [subtrees][256], b4] (host): copy out (get) and/or replace (set). */
+int sq_proposal_weights(sq_proposal* p, float* get, const float* set);
+/* The Fourier matrix B float[6][64] and the scene box float[6] (host). */
+int sq_proposal_features_spec(const sq_proposal* p, float* rff, float* box);
+/* Top-4 subtrees per ray, best first (device uint16[n][4]; 0xFFFF = none). */
+int sq_proposal_infer(sq_proposal* p, const float* rays, uint64_t n, uint16_t* top);
+/* Exact nearest hit, walking the proposed subtrees (top from
+ * sq_proposal_infer; NULL = none, the plain walk) first.  Same results as
+ * sq_raycast_nearest without per-ray limits or filter.  stats (device
+ * uint64[3], may be NULL; NULL runs the uncounted kernel) accumulates: rays
+ * whose proposed subtrees held a hit, pair nodes fetched, primitives tested. */
+int sq_raycast_nearest_proposed(sq_proposal* p, const float* rays, uint64_t n, float max_toi, const uint16_t* top,
+                                int32_t* out_leaf, float* out_toi, uint64_t* stats);
+/* C[M][N] = bf16(A)[M][K] * bf16(B)[N][K]^T in FP32 on the matrix cores, the
+ * GEMM the training step runs (device fp32 inputs, rounded to bf16 first).
+ * M, N multiples of 128, K of 32; splits > 1 uses the split-K reduction. */
+int sq_debug_gemm(sq_ctx* ctx, const float* A, const float* B, int M, int N, int K, int splits, float* C);
+
 #ifdef __cplusplus
 }
 #endif

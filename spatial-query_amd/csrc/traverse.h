@@ -635,17 +635,31 @@ SQ_HD bool walk_begin(const SceneView& s, const Q& q, float max_toi, WalkState& 
   return q.root(s.root, max_toi);
 }
 
+// Work done by a walk: pair nodes fetched and primitives tested.
+struct WorkCount {
+  uint32_t nodes = 0, prims = 0;
+};
+
 // One step: descend through interior nodes to the next leaf (near-first), then
 // test that leaf's primitives.  Returns true when the query is finished.
-template <int D, int PS, bool ANY, bool FILT, class Q>
-SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const RingStack& st, WalkState& w) {
+// COUNT adds the step's work to *wc.  SKIP treats the child refs skip[0..3]
+// as missed: subtrees this query has already walked completely (with a limit
+// at least the current one, so they hold nothing it has not seen).
+template <int D, int PS, bool ANY, bool FILT, bool COUNT = false, bool SKIP = false, class Q>
+SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const RingStack& st, WalkState& w, WorkCount* wc = nullptr,
+                     const uint32_t* skip = nullptr) {
   constexpr int NF = (D + 1) * 4;
   while (!(w.cur & LEAF_BIT)) {
     float nd[NF];
     fetch16<D + 1>(s.nodes + (size_t)w.cur * NF, nd);
+    if constexpr (COUNT) ++wc->nodes;
     bool ha, hb; float ta, tb;
     q.pair(nd, w.limit, ha, hb, ta, tb);
     const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
+    if constexpr (SKIP) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) { ha = ha && ra != skip[k]; hb = hb && rb != skip[k]; }
+    }
     if (ha && hb) {
       const bool a_first = ta <= tb;
       st.push(w.sp, a_first ? rb : ra, a_first ? tb : ta);
@@ -666,6 +680,7 @@ SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const Ring
   }
   const uint32_t first = w.cur & FIRST_MASK;
   const uint32_t cnt = ((w.cur >> 27) & 15u) + 1u;
+  if constexpr (COUNT) wc->prims += cnt;
   for (uint32_t j = 0; j < cnt; ++j) {
     const uint32_t pos = first + j;
     float t;

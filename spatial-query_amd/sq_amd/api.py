@@ -396,3 +396,144 @@ class Tlas:
             self.close()
         except Exception:
             pass
+
+
+class Proposal:
+    """Learned subtree proposal for a TRIANGLE DeviceBvh (no reference counterpart).
+
+    An MLP predicts, per ray, the subtrees (the BVH cut at `cut_depth`) most
+    likely to hold the nearest hit.  `raycast_nearest` walks the top-4 first to
+    tighten t_max, then runs the full exact walk: results equal
+    DeviceBvh.raycast_nearest bit for bit whatever the network predicts.
+    Trained on the GPU (`train_step`, `fit`) from labels the exact walk makes.
+    """
+
+    def __init__(self, bvh: DeviceBvh, cut_depth: int = 12, rff_scale: float = 1.0, seed: int = 0x5EED):
+        h = C.c_void_p()
+        check(lib().sq_proposal_create(bvh.ctx.handle, bvh._h, int(cut_depth), float(rff_scale),
+                                       int(seed) & ((1 << 64) - 1), C.byref(h)))
+        self._h, self.bvh, self.ctx, self.cut_depth = h, bvh, bvh.ctx, cut_depth
+        v = C.c_uint32()
+        check(lib().sq_proposal_num_subtrees(self._h, C.byref(v)))
+        self.n_subtrees = v.value
+        q = C.c_uint64()
+        check(lib().sq_proposal_num_params(self._h, C.byref(q)))
+        self.n_params = q.value
+
+    def shapes(self):
+        """[(out, in)] of the four layers."""
+        return [(256, 128), (256, 256), (256, 256), (self.n_subtrees, 256)]
+
+    def _split(self, flat):
+        out, o = [], 0
+        for (m, k) in self.shapes():
+            w = flat[o:o + m * k].reshape(m, k); o += m * k
+            b = flat[o:o + m]; o += m
+            out.append((w, b))
+        return out
+
+    def labels(self, leaf):
+        import torch
+        lab = torch.empty(leaf.shape[0], dtype=torch.int32, device=leaf.device)
+        check(lib().sq_proposal_labels(self._h, _dp(leaf), leaf.shape[0], _dp(lab)))
+        return lab
+
+    def train_step(self, rays, labels, lr: float = 1e-3, want_loss: bool = False):
+        loss = C.c_float(0.0)
+        check(lib().sq_proposal_train_step(self._h, _dp(rays), _dp(labels), rays.shape[0], float(lr),
+                                           C.byref(loss) if want_loss else None, None))
+        return loss.value if want_loss else None
+
+    def gradients(self, rays, labels):
+        """Gradients of the mean cross-entropy, [(dW, db)] per layer (fp32, no update)."""
+        flat = np.empty(self.n_params, np.float32)
+        check(lib().sq_proposal_train_step(self._h, _dp(rays), _dp(labels), rays.shape[0], 0.0, None, _hp(flat)))
+        return self._split(flat)
+
+    def weights(self):
+        flat = np.empty(self.n_params, np.float32)
+        check(lib().sq_proposal_weights(self._h, _hp(flat), None))
+        return self._split(flat)
+
+    def set_weights(self, layers):
+        flat = _host(np.concatenate([np.concatenate([w.ravel(), b.ravel()]) for (w, b) in layers]))
+        if flat.size != self.n_params:
+            raise ValueError("weight count mismatch")
+        check(lib().sq_proposal_weights(self._h, None, _hp(flat)))
+
+    def features_spec(self):
+        """(B float32[6][64], scene box float32[6])."""
+        B = np.empty((6, 64), np.float32)
+        box = np.empty(6, np.float32)
+        check(lib().sq_proposal_features_spec(self._h, _hp(B), _hp(box)))
+        return B, box
+
+    def infer(self, rays, out=None):
+        """Top-4 subtrees per ray, best first: int16 tensor [n][4] holding uint16 ids (0xFFFF = none)."""
+        import torch
+        if out is None:
+            out = torch.empty((rays.shape[0], 4), dtype=torch.int16, device=rays.device)
+        check(lib().sq_proposal_infer(self._h, _dp(rays), rays.shape[0], _dp(out)))
+        return out
+
+    def raycast_nearest_into(self, rays, max_toi, top, leaf, toi, stats=None):
+        check(lib().sq_raycast_nearest_proposed(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(top),
+                                                _dp(leaf), _dp(toi), _dp(stats)))
+
+    def raycast_nearest(self, rays, max_toi, top=None, count=False):
+        """Exact nearest hit walking `top` (None = the plain walk) first: dict(leaf, toi[, seeded, nodes, prims])."""
+        import torch
+        n = rays.shape[0]
+        leaf = torch.empty(n, dtype=torch.int32, device=rays.device)
+        toi = torch.empty(n, dtype=torch.float32, device=rays.device)
+        st = torch.zeros(3, dtype=torch.int64, device=rays.device) if count else None
+        self.raycast_nearest_into(rays, max_toi, top, leaf, toi, st)
+        out = dict(leaf=leaf, toi=toi)
+        if count:
+            s = st.cpu().tolist()
+            out.update(seeded=s[0], nodes=s[1], prims=s[2])
+        return out
+
+    def fit(self, rays, labels, epochs: int = 1, batch: int = 65536, lr: float = 1e-3, seed: int = 1, log=None):
+        """Adam over shuffled mini-batches of (rays, labels) (GPU tensors); returns the mean loss per epoch."""
+        import torch
+        g = torch.Generator(device=rays.device)
+        g.manual_seed(seed)
+        n = rays.shape[0]
+        losses = []
+        for e in range(epochs):
+            perm = torch.randperm(n, device=rays.device, generator=g)
+            tot, cnt = 0.0, 0
+            for s0 in range(0, n - batch + 1, batch):
+                idx = perm[s0:s0 + batch]
+                r = rays.index_select(0, idx).contiguous()
+                l = labels.index_select(0, idx).contiguous()
+                last = s0 + 2 * batch > n
+                v = self.train_step(r, l, lr, want_loss=(s0 // batch) % 16 == 0 or last)
+                if v is not None:
+                    tot += v; cnt += 1
+            losses.append(tot / max(cnt, 1))
+            if log:
+                log(f"epoch {e}: loss {losses[-1]:.4f}")
+        return losses
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_proposal_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mfma_gemm(ctx: Context, A, B, splits: int = 1):
+    """C = bf16(A) @ bf16(B)^T in fp32 on the matrix cores (the training GEMM); A [M][K], B [N][K] GPU fp32."""
+    import torch
+    M, K = A.shape
+    N = B.shape[0]
+    Cm = torch.empty((M, N), dtype=torch.float32, device=A.device)
+    check(lib().sq_debug_gemm(ctx.handle, _dp(A), _dp(B), M, N, K, splits, _dp(Cm)))
+    return Cm

@@ -54,6 +54,17 @@ SIGNATURES = {
     "sq_tlas_raycast_all": [P, P, U64, F32, P, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_tlas_overlap": [P, P, U64, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_ctx_set_block": [P, I32],
+    "sq_proposal_create": [P, P, I32, F32, U64, PP],
+    "sq_proposal_destroy": [P],
+    "sq_proposal_num_subtrees": [P, C.POINTER(C.c_uint32)],
+    "sq_proposal_num_params": [P, C.POINTER(C.c_uint64)],
+    "sq_proposal_labels": [P, P, U64, P],
+    "sq_proposal_train_step": [P, P, P, U64, F32, C.POINTER(C.c_float), P],
+    "sq_proposal_weights": [P, P, P],
+    "sq_proposal_features_spec": [P, P, P],
+    "sq_proposal_infer": [P, P, U64, P],
+    "sq_raycast_nearest_proposed": [P, P, U64, F32, P, P, P, P],
+    "sq_debug_gemm": [P, P, P, I32, I32, I32, I32, P],
     "sq_ctx_set_tuning": [P, C.POINTER(Tuning)],
     "sq_ctx_get_tuning": [P, C.POINTER(Tuning)],
     "sq_ctx_last_kernel_ms": [P, C.POINTER(C.c_float)],

@@ -175,20 +175,27 @@ def heightfield_scene(grid=1025, extent=512.0, seed=0x4E16_47F1_E1D0_0003) -> Sc
                  dict(seed=seed, grid=grid, zmin=float(Z.min()), zmax=float(Z.max()), extent=extent, params=params))
 
 
-def camera_rays(sc: Scene, res=2048, seed=0xCA3E_7A00_0000_0004, fov_deg=60.0) -> np.ndarray:
-    """Pinhole camera at a seeded pose 200 units above the surface, looking
-    down at a seeded pitch in [-50, -20] degrees; one ray per pixel centre."""
+def camera_rays(sc: Scene, res=2048, seed=0xCA3E_7A00_0000_0004, fov_deg=60.0, jitter_seed=None) -> np.ndarray:
+    """Pinhole camera at a seeded pose 200 units above the surface (scaled by
+    extent / 512 for smaller heightfields), looking
+    down at a seeded pitch in [-50, -20] degrees; one ray per pixel centre, or,
+    with jitter_seed, per uniformly drawn point of each pixel."""
     s = Stream(seed)
-    cx, cy = (float(v) for v in s.range(-256.0, 256.0, 2))
+    e = 0.5 * sc.meta["extent"]
+    cx, cy = (float(v) for v in s.range(-e, e, 2))
     yaw = float(s.range(0.0, 2 * math.pi, 1)[0])
     pitch = math.radians(float(s.range(-50.0, -20.0, 1)[0]))
-    cz = float(height(cx, cy, sc.meta["params"])) + 200.0
+    cz = float(height(cx, cy, sc.meta["params"])) + 200.0 * sc.meta["extent"] / 512.0
     fwd = np.array([math.cos(pitch) * math.cos(yaw), math.cos(pitch) * math.sin(yaw), math.sin(pitch)])
     right = np.array([math.sin(yaw), -math.cos(yaw), 0.0])
     up = np.cross(right, fwd)
     half = math.tan(math.radians(fov_deg) / 2)
     px = (np.arange(res, dtype=np.float64) + 0.5) / res * 2 - 1
     U, Vv = np.meshgrid(px * half, -px * half, indexing="xy")
+    if jitter_seed is not None:
+        j = Stream(jitter_seed).range(-0.5, 0.5, 2 * res * res).reshape(2, res, res).astype(np.float64)
+        U = U + j[0] * (2.0 / res) * half
+        Vv = Vv - j[1] * (2.0 / res) * half
     d = (fwd[None, None, :] + U[..., None] * right[None, None, :] + Vv[..., None] * up[None, None, :]).reshape(-1, 3)
     d = normalize_or_zero(d.astype(np.float32))
     o = np.broadcast_to(np.array([cx, cy, cz], np.float32), d.shape)
@@ -211,6 +218,25 @@ def c1(res=2048, n_incoherent=None, grid=1025, extent=512.0):
     coh = camera_rays(sc, res=res)
     inc = incoherent_rays(sc, res * res if n_incoherent is None else n_incoherent)
     return sc, coh, inc
+
+
+def proposal_training_rays(sc: Scene, n: int, seed=0x7EA1_0000_0000_0008) -> np.ndarray:
+    """Training rays for the learned proposal: fresh draws from the evaluation
+    distributions.  n/2 camera rays from the same seeded camera as camera_rays
+    at points drawn uniformly inside the pixels (the evaluation uses the pixel
+    centres), over as many jittered frames as needed, and n/2 incoherent rays
+    from another seed."""
+    s = Stream(seed)
+    half = n // 2
+    res = max(1, int(math.sqrt(half)))
+    cams, got = [], 0
+    while got < half:
+        r = camera_rays(sc, res=res, jitter_seed=int(s.u64(1)[0]))
+        cams.append(r)
+        got += r.shape[0]
+    coh = np.concatenate(cams)[:half]
+    inc = incoherent_rays(sc, n - half, seed=int(s.u64(1)[0]))
+    return np.ascontiguousarray(np.concatenate([coh, inc]), np.float32)
 
 
 # --- D-dimensional boxes from a Gaussian mixture ----------------------------
