@@ -2,9 +2,12 @@
 
 One step = one batched nearest-hit pass of all 8,388,608 rays (4M coherent
 camera rays + 4M incoherent rays, t_max 2000) through the HIP engine, rays and
-scene resident in HBM.  Prints one JSON line (see the metric contract).
+scene resident in HBM.  `value` is the plain pooled walk; the learned-proposal
+leg (train the MLP on 16M fresh rays, then fused inference + seeded exact walk
+on the same 8M rays) and the CPU baseline are reported beside it.  Prints one
+JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-sample N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-sample N] [--no-proposal]
 """
 from __future__ import annotations
 
@@ -20,6 +23,7 @@ sys.path.insert(0, os.path.join(ROOT, "spatial-query_amd"))
 
 METRIC = "Mrays/s nearest-hit, 2M-tri scene: learned-proposal vs plain kernel, exact hits"
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, published
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix peak, published
 
 
 def log(msg):
@@ -31,13 +35,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 21, help="rays in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=-1, help="rays in the CPU baseline sample (-1 = all, 0 = skip)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--rays-per-lane", type=int, default=-1, help="pooled kernel pool size per lane (0 = one ray per thread)")
     ap.add_argument("--lds-stack", type=int, default=0)
     ap.add_argument("--quick", action="store_true", help="short run for profiling: no CPU baseline, no side figures")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB device copy (the known-size kernel that calibrates HBM counters)")
+    ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--proposal-epochs", type=int, default=4)
+    ap.add_argument("--proposal-train-rays", type=int, default=1 << 24)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not produced (see DESIGN.md)",
@@ -124,7 +131,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 heightfield and rays)",
-        "method": "plain near-first traversal kernel (learned proposal not yet in the timed path)",
+        "method": "plain pooled near-first traversal",
         "config": {"workload": "C1 heightfield: 1025x1025 grid, 2,097,152 triangles; 4,194,304 coherent + "
                                "4,194,304 incoherent rays; nearest-hit, t_max 2000",
                    "rays_per_step": n, "tree_nodes": st["node_count"], "tree_depth": st["max_depth"],
@@ -155,9 +162,16 @@ def main():
         out["anyhit_mrays"] = round(n / (min(ts) * 1e-3) / 1e6, 1)
         out["hit_fraction"] = round(float((leaf >= 0).float().mean().item()), 4)
 
+    if not args.no_proposal and not args.quick:
+        out["learned_proposal"] = proposal_leg(args, ctx, bvh, sc, rays, leaf, toi, k_ms, n)
+        lp = out["learned_proposal"]
+        out["method"] = (f"plain pooled near-first traversal; the learned-proposal path (fused BF16 MFMA inference + "
+                         f"seeded exact walk) measures {lp['mrays']} Mrays/s = {lp['speedup_vs_plain']}x plain, and a "
+                         f"perfect proposal (t_max = exact toi per ray) bounds any proposal at {lp['perfect_limit_bound']}x")
+
     if args.cpu_sample and not args.quick:
         import oracle
-        m = min(args.cpu_sample, n)
+        m = n if args.cpu_sample < 0 else min(args.cpu_sample, n)
         idx = np.linspace(0, n - 1, m).astype(np.int64)       # evenly spread over both halves
         sample = np.ascontiguousarray(rays_h[idx])
         cores = min(16, os.cpu_count() or 1)
@@ -169,7 +183,7 @@ def main():
         got_t = toi.cpu().numpy()[idx]
         agree = (got_leaf == ref["leaf"]) & (got_t.view(np.uint32) == ref["t"].view(np.uint32))
         out["cpu_baseline"] = {"value": round(m / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
-                               "sample": f"{m} rays evenly spaced over the 8M (both halves), C++ restatement of "
+                               "sample": f"{m} of the {n} rays, evenly spaced over both halves; C++ restatement of "
                                          f"raycast_nearest_batch_parallel (contiguous chunks, {cores} threads)",
                                "seconds": round(cpu_s, 2)}
         out["id_agreement_on_sample"] = float(agree.mean())
@@ -183,6 +197,89 @@ def main():
         torch.cuda.synchronize()
         out["copy_calib_bytes"] = {"read": a.numel() * 4, "written": b.numel() * 4}
     print(json.dumps(out), flush=True)
+
+
+def proposal_leg(args, ctx, bvh, sc, rays, leaf, toi, plain_ms, n):
+    """Learned subtree proposal on the same rays: train on fresh draws from the
+    C1 distributions (jittered camera rays of the same camera, incoherent rays
+    of another seed), then time fused inference + the seeded exact walk."""
+    import numpy as np
+    import torch
+    import sq_amd
+    from sq_amd import workloads as W
+
+    res = {}
+    # bound: the plain walk with t_max = the exact toi of each ray (what a
+    # perfect, free proposal could give)
+    tl = torch.where(leaf >= 0, toi, torch.full_like(toi, 2000.0)).contiguous()
+    l2 = torch.empty_like(leaf); t2 = torch.empty_like(toi)
+    bvh.raycast_nearest_into(rays, 2000.0, l2, t2, max_toi_per_ray=tl)
+    ts = []
+    for _ in range(5):
+        bvh.raycast_nearest_into(rays, 2000.0, l2, t2, max_toi_per_ray=tl)
+        ts.append(ctx.last_kernel_ms())
+    res["perfect_limit_ms"] = round(min(ts), 4)
+    res["perfect_limit_bound"] = round(plain_ms / min(ts), 3)
+
+    p = sq_amd.Proposal(bvh, cut_depth=12, rff_scale=1.0, seed=0x5EED)
+    t0 = time.time()
+    train = W.proposal_training_rays(sc, args.proposal_train_rays)
+    rt = torch.from_numpy(train).cuda()
+    lab = p.labels(bvh.raycast_nearest(rt, 2000.0)["leaf"])
+    torch.cuda.synchronize()
+    t_data = time.time() - t0
+    # training throughput: one timed epoch-sized pass after the first
+    torch.cuda.synchronize(); t0 = time.time()
+    losses = p.fit(rt, lab, epochs=args.proposal_epochs, batch=65536, lr=1e-3, seed=1)
+    torch.cuda.synchronize()
+    t_train = time.time() - t0
+    steps = args.proposal_epochs * (rt.shape[0] // 65536)
+    flop_ray = 2.0 * (6 * 64 + 128 * 256 + 2 * 256 * 256 + 256 * p.n_subtrees)
+    res.update(subtrees=p.n_subtrees, train_rays=int(rt.shape[0]), epochs=args.proposal_epochs,
+               data_s=round(t_data, 2), train_s=round(t_train, 2), loss_first=round(losses[0], 4),
+               loss_last=round(losses[-1], 4),
+               train_tflops=round(3 * flop_ray * 65536 * steps / t_train / 1e12, 1))
+    del rt, lab, train
+
+    top = p.infer(rays)
+    pl = torch.empty_like(leaf); pt = torch.empty_like(toi)
+    p.raycast_nearest_into(rays, 2000.0, top, pl, pt)
+    torch.cuda.synchronize()
+    exact = bool(torch.equal(pl, leaf)) and bool(torch.equal(pt.view(torch.int32), toi.view(torch.int32)))
+    ti, tw = [], []
+    for _ in range(5):
+        p.infer(rays, out=top); ti.append(ctx.last_kernel_ms())
+        p.raycast_nearest_into(rays, 2000.0, top, pl, pt); tw.append(ctx.last_kernel_ms())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        p.infer(rays, out=top)
+        p.raycast_nearest_into(rays, 2000.0, top, pl, pt)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    lab_e = p.labels(leaf).cpu().numpy()
+    t = top.cpu().numpy().view(np.uint16).astype(np.int64)
+    half = n // 2
+    def rate(sl):
+        ok = lab_e[sl] >= 0
+        return round(float((t[sl][ok] == lab_e[sl][ok, None]).any(axis=1).mean()), 4)
+    cnt = p.raycast_nearest(rays, 2000.0, top, count=True)
+    plain_cnt = p.raycast_nearest(rays, 2000.0, None, count=True)
+    infer_ms = float(np.median(ti))
+    res.update(mrays=round(n / (ms * 1e-3) / 1e6, 2), ms_per_step=round(ms, 4),
+               speedup_vs_plain=round(plain_ms / ms, 3), exact_vs_plain=exact,
+               infer_ms=round(infer_ms, 4), seeded_walk_ms=round(float(np.median(tw)), 4),
+               infer_tflops=round(flop_ray * n / (infer_ms * 1e-3) / 1e12, 1),
+               mfma_utilisation=round(flop_ray * n / (infer_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
+               top4_hit_rate_coherent=rate(slice(0, half)), top4_hit_rate_incoherent=rate(slice(half, n)),
+               seeded_fraction=round(cnt["seeded"] / n, 4),
+               nodes_per_ray_plain=round(plain_cnt["nodes"] / n, 2), nodes_per_ray_seeded=round(cnt["nodes"] / n, 2),
+               prims_per_ray_plain=round(plain_cnt["prims"] / n, 2), prims_per_ray_seeded=round(cnt["prims"] / n, 2),
+               flop_per_ray=flop_ray)
+    log(f"proposal: {res['mrays']} Mrays/s ({res['speedup_vs_plain']}x plain), infer {res['infer_ms']} ms at "
+        f"{res['infer_tflops']} TFLOP/s, walk {res['seeded_walk_ms']} ms, exact {exact}, bound {res['perfect_limit_bound']}x")
+    p.close()
+    return res
 
 
 if __name__ == "__main__":
