@@ -242,6 +242,35 @@ int sq_ctx_set_block(sq_ctx* ctx, int block_threads);
 int sq_ctx_last_kernel_ms(sq_ctx* ctx, float* ms);
 
 /* ------------------------------------------------------------------------
+ * Batches: many nearest-hit batches over many scenes in one persistent,
+ * work-stealing launch.  The reference answers one batch per call
+ * (raycast_nearest_batch, bvh.rs:134-193, or Tlas::raycast_nearest per ray,
+ * tlas.rs:82-154); a batch set turns all batches of a frame into one table of
+ * 64 * rays_per_lane-ray chunks that a grid sized to the chip pulls from an
+ * atomic counter.  Results equal the single-scene calls (default filter, the
+ * given max_toi).  Every scene of a set has the same class, dimension and kind.
+ * sq_batch_submit only enqueues kernels (the counter is cleared by a kernel):
+ * it may be recorded into a hipGraph on a capturing context stream.  Tuning
+ * (block, rays_per_lane, lds_stack) is read at creation.
+ * ---------------------------------------------------------------------- */
+#define SQ_BATCH_BVH 0    /* scenes are sq_bvh*; out_leaf = primitive id */
+#define SQ_BATCH_TLAS 1   /* scenes are sq_tlas*; out_leaf = instance id */
+typedef struct sq_batch sq_batch;
+typedef struct sq_batch_query {
+  const void* scene;      /* sq_bvh* or sq_tlas*, alive while the set is used */
+  const float* rays;      /* device float[n][2*dim] */
+  uint64_t n;
+  float max_toi;
+  int32_t* out_leaf;      /* device int32[n], -1 = miss */
+  float* out_toi;         /* device float[n] */
+} sq_batch_query;
+int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* queries, uint32_t n_queries, sq_batch** out);
+int sq_batch_submit(sq_batch* batch);
+/* Chunks in the set, and the persistent grid (blocks x threads). */
+int sq_batch_get_info(const sq_batch* batch, uint32_t* n_chunks, uint32_t* grid, uint32_t* block);
+int sq_batch_destroy(sq_batch* batch);
+
+/* ------------------------------------------------------------------------
  * Learned subtree proposal (no counterpart in the reference, which has only the
  * plain walk).  A small MLP maps a ray to the subtrees of the BVH cut at a
  * fixed depth that most likely hold its nearest hit; the proposed subtrees are

@@ -1,0 +1,276 @@
+// batch.hip -- many nearest-hit batches over many scenes in one persistent,
+// work-stealing launch (sq_batch_*).  The reference answers one batch per call
+// (raycast_nearest_batch, crates/spatial-query/src/accel/bvh.rs:134-193); here
+// the batches of a frame become one table of 64 * rays_per_lane-ray chunks, and
+// a grid sized to what the chip holds resident pulls chunks from an atomic
+// counter until none are left, so uneven batch sizes leave no CU idle.
+// Enqueueing is capture-safe (kernels only: a fill kernel clears the counter),
+// so a frame of submissions can be recorded into a hipGraph.
+#include <algorithm>
+
+#include "engine_internal.h"
+#include "pool.h"
+
+namespace {
+
+struct BvhItem {
+  SceneView s;
+  const float* rays;
+  int32_t* leaf;
+  float* toi;
+  uint32_t chunk0, n;     // first chunk of this batch, rays
+  float max_toi;
+  uint32_t pad;
+};
+
+struct TlasItem {
+  TlasView tv;
+  const float* rays;
+  int32_t* inst;
+  float* toi;
+  uint32_t chunk0, n;
+  float max_toi;
+  uint32_t pad;
+};
+
+// Batch of chunk c: the last item whose first chunk is <= c (items hold >= 1
+// chunk each).  c and the items are wave-uniform, so this runs on scalar loads.
+template <class Item>
+__device__ __forceinline__ uint32_t find_item(const Item* __restrict__ items, uint32_t n_items, uint32_t c) {
+  uint32_t lo = 0, hi = n_items - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (items[mid].chunk0 <= c) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Next chunk for the whole wave (called with every lane active).
+__device__ __forceinline__ uint32_t next_chunk(uint32_t* counter) {
+  uint32_t c = 0;
+  if ((threadIdx.x & 63u) == 0) c = atomicAdd(counter, 1u);
+  return __builtin_amdgcn_readfirstlane(c);
+}
+
+template <int D, int KIND>
+__global__ __launch_bounds__(256, (D <= 3 ? 8 : 4)) void k_batch_bvh(const BvhItem* __restrict__ items, uint32_t n_items,
+                                                                     uint32_t n_chunks, uint32_t per_lane, uint32_t S,
+                                                                     SpillEntry* spill, uint32_t spill_stride, uint32_t* counter) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + blockIdx.x * B + tid, spill_stride};
+  const Filter nof{nullptr, ~0ull, -1};
+  const uint64_t pool = 64ull * per_lane;
+  for (;;) {
+    const uint32_t c = next_chunk(counter);
+    if (c >= n_chunks) break;
+    const BvhItem& it = items[find_item(items, n_items, c)];
+    const uint64_t first = (uint64_t)(c - it.chunk0) * pool;
+    const uint64_t end = first + pool < (uint64_t)it.n ? first + pool : (uint64_t)it.n;
+    const NearestOut out{it.leaf, it.toi, nullptr, nullptr, nullptr, nullptr};
+    pool_walk<D, KIND, false, false, false>(it.s, it.rays, first, end, it.max_toi, nullptr, nof, out, st, nullptr);
+  }
+}
+
+// Two-level scenes: each lane walks rays first + 64 k + lane, k < per_lane, of
+// its wave's chunk with the two-level walk (whole stacks in LDS).
+template <int D, int IK>
+__global__ __launch_bounds__(256) void k_batch_tlas(const TlasItem* __restrict__ items, uint32_t n_items, uint32_t n_chunks,
+                                                    uint32_t per_lane, uint32_t s_top, uint32_t s_bot, uint32_t* counter) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x, lane = tid & 63u;
+  const uint32_t S = s_top + s_bot;
+  uint32_t* refs = lds + tid;
+  float* ts = reinterpret_cast<float*>(lds + S * B) + tid;
+  Stack st{refs, ts, B};
+  Stack sb{refs + s_top * B, ts + s_top * B, B};
+  const uint64_t pool = 64ull * per_lane;
+  for (;;) {
+    const uint32_t c = next_chunk(counter);
+    if (c >= n_chunks) break;
+    const TlasItem& it = items[find_item(items, n_items, c)];
+    const uint64_t first = (uint64_t)(c - it.chunk0) * pool;
+    for (uint32_t k = 0; k < per_lane; ++k) {
+      const uint64_t i = first + (uint64_t)k * 64 + lane;
+      if (i >= it.n) break;
+      RayD<D> r;
+      ray_setup<D>(r, it.rays + i * 2 * D);
+      const TlasHit h = walk_tlas<D, IK, false>(it.tv, r, it.max_toi, st, sb);
+      it.inst[i] = h.inst != NO_LEAF ? (int32_t)h.inst : -1;
+      it.toi[i] = h.t;
+    }
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* p) {
+  if (threadIdx.x == 0) *p = 0u;
+}
+
+}  // namespace
+
+struct sq_batch {
+  sq_ctx* ctx = nullptr;
+  int cls = 0, dim = 0, kind = 0;
+  uint32_t n_items = 0, n_chunks = 0, per_lane = 1, S = 8, s_top = 0, s_bot = 0, grid = 0, block = 128;
+  size_t lds = 0;
+  DevBuf items, counter, spill;
+  std::vector<std::pair<uint64_t, NearestOut>> empty;   // batches over scenes without primitives: all misses
+};
+
+
+extern "C" {
+
+int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* qs, uint32_t n_queries, sq_batch** out) {
+  if (!ctx || !out || (n_queries && !qs)) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (scene_class != SQ_BATCH_BVH && scene_class != SQ_BATCH_TLAS) return fail(SQ_ERR_INVALID, "unknown scene class");
+  auto b = std::make_unique<sq_batch>();
+  b->ctx = ctx;
+  b->cls = scene_class;
+  b->per_lane = (uint32_t)std::max(1, ctx->rays_per_lane);
+  b->S = (uint32_t)ctx->lds_stack;
+  b->block = (uint32_t)std::min(ctx->block, 256);
+  const uint64_t pool = 64ull * b->per_lane;
+  std::vector<BvhItem> bi;
+  std::vector<TlasItem> ti;
+  uint64_t chunks = 0;
+  uint32_t max_stack = 0;
+  for (uint32_t i = 0; i < n_queries; ++i) {
+    const sq_batch_query& q = qs[i];
+    if (!q.scene) return fail(SQ_ERR_INVALID, "batch " + std::to_string(i) + ": NULL scene");
+    if (q.n && (!q.rays || !q.out_leaf || !q.out_toi)) return fail(SQ_ERR_INVALID, "batch " + std::to_string(i) + ": NULL buffer");
+    if (q.n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "batch " + std::to_string(i) + ": at most 2^32-1 rays");
+    int dim, kind;
+    bool empty;
+    if (scene_class == SQ_BATCH_BVH) {
+      const sq_bvh* s = static_cast<const sq_bvh*>(q.scene);
+      if (s->ctx != ctx) return fail(SQ_ERR_INVALID, "batch " + std::to_string(i) + ": scene of another context");
+      dim = s->dim; kind = s->kind; empty = s->n_prims == 0;
+      if (!empty) max_stack = std::max(max_stack, s->packed.max_stack);
+    } else {
+      const sq_tlas* t = static_cast<const sq_tlas*>(q.scene);
+      TlasInfo inf;
+      tlas_info(t, &inf);
+      if (inf.ctx != ctx) return fail(SQ_ERR_INVALID, "batch " + std::to_string(i) + ": scene of another context");
+      dim = inf.dim; kind = inf.kind; empty = inf.n_inst == 0;
+      if (!empty) { b->s_top = std::max(b->s_top, inf.s_top); b->s_bot = std::max(b->s_bot, inf.s_bot); }
+    }
+    if (i == 0) { b->dim = dim; b->kind = kind; }
+    if (dim != b->dim || kind != b->kind)
+      return fail(SQ_ERR_INVALID, "batch " + std::to_string(i) + ": every scene of a batch set needs the same dimension and kind");
+    if (!q.n) continue;
+    if (empty) {
+      b->empty.push_back({q.n, NearestOut{q.out_leaf, q.out_toi, nullptr, nullptr, nullptr, nullptr}});
+      continue;
+    }
+    const uint32_t c0 = (uint32_t)chunks;
+    chunks += (q.n + pool - 1) / pool;
+    if (chunks > 0x7FFFFFFFull) return fail(SQ_ERR_INVALID, "too many chunks in one batch set");
+    if (scene_class == SQ_BATCH_BVH) {
+      BvhItem it{};
+      it.s = view_of(static_cast<const sq_bvh*>(q.scene));
+      it.rays = q.rays; it.leaf = q.out_leaf; it.toi = q.out_toi;
+      it.chunk0 = c0; it.n = (uint32_t)q.n; it.max_toi = q.max_toi;
+      bi.push_back(it);
+    } else {
+      TlasInfo inf;
+      tlas_info(static_cast<const sq_tlas*>(q.scene), &inf);
+      TlasItem it{};
+      it.tv = inf.view;
+      it.rays = q.rays; it.inst = q.out_leaf; it.toi = q.out_toi;
+      it.chunk0 = c0; it.n = (uint32_t)q.n; it.max_toi = q.max_toi;
+      ti.push_back(it);
+    }
+  }
+  b->n_items = (uint32_t)(scene_class == SQ_BATCH_BVH ? bi.size() : ti.size());
+  b->n_chunks = (uint32_t)chunks;
+  int rc;
+  if ((rc = b->counter.ensure(256))) return rc;
+  if (b->n_items) {
+    // grid: every wave the chip holds resident, capped by the chunk count
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    int per_cu = 0;
+    if (scene_class == SQ_BATCH_BVH) {
+      b->lds = (size_t)b->S * b->block * 8;
+      rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_batch_bvh<D, K>), (int)b->block,
+                                                             b->lds));
+        return SQ_OK;
+      });
+    } else {
+      b->lds = (size_t)(b->s_top + b->s_bot) * b->block * 8;
+      if (b->lds > 160 * 1024) return fail(SQ_ERR_INVALID, "traversal stacks do not fit in LDS; lower the block size");
+      rc = dispatch_inst(b->dim, b->kind, [&]<int D, int K>() -> int {
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_batch_tlas<D, K>), (int)b->block,
+                                                             b->lds));
+        return SQ_OK;
+      });
+    }
+    if (rc) return rc;
+    const uint64_t waves_per_block = b->block / 64;
+    const uint64_t need = (chunks + waves_per_block - 1) / waves_per_block;
+    b->grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * std::max(1, per_cu), need));
+    if (scene_class == SQ_BATCH_BVH) {
+      const uint32_t rows = max_stack > b->S ? max_stack - b->S : 0;
+      if ((rc = b->spill.ensure((size_t)rows * b->grid * b->block * sizeof(SpillEntry) + 64))) return rc;
+      if ((rc = upload(b->items, bi.data(), bi.size() * sizeof(BvhItem), ctx->stream))) return rc;
+    } else {
+      if ((rc = upload(b->items, ti.data(), ti.size() * sizeof(TlasItem), ctx->stream))) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));   // the host tables end with this call
+  }
+  ctx_retain(ctx);
+  *out = b.release();
+  return SQ_OK;
+}
+
+int sq_batch_submit(sq_batch* b) {
+  if (!b) return fail(SQ_ERR_INVALID, "NULL argument");
+  sq_ctx* c = b->ctx;
+  int rc = begin_timing(c);
+  if (rc) return rc;
+  for (const auto& e : b->empty)
+    if ((rc = fill_miss(c, e.first, b->dim, e.second))) return rc;
+  if (b->n_items) {
+    k_zero_u32<<<1, 64, 0, c->stream>>>(b->counter.as<uint32_t>());
+    HIP_TRY(hipGetLastError());
+    const uint32_t spill_stride = b->grid * b->block;
+    if (b->cls == SQ_BATCH_BVH)
+      rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+        k_batch_bvh<D, K><<<b->grid, b->block, b->lds, c->stream>>>(b->items.as<BvhItem>(), b->n_items, b->n_chunks, b->per_lane, b->S,
+                                                                   b->spill.as<SpillEntry>(), spill_stride, b->counter.as<uint32_t>());
+        HIP_TRY(hipGetLastError());
+        return SQ_OK;
+      });
+    else
+      rc = dispatch_inst(b->dim, b->kind, [&]<int D, int K>() -> int {
+        k_batch_tlas<D, K><<<b->grid, b->block, b->lds, c->stream>>>(b->items.as<TlasItem>(), b->n_items, b->n_chunks, b->per_lane,
+                                                                    b->s_top, b->s_bot, b->counter.as<uint32_t>());
+        HIP_TRY(hipGetLastError());
+        return SQ_OK;
+      });
+    if (rc) return rc;
+  }
+  return end_timing(c);
+}
+
+int sq_batch_get_info(const sq_batch* b, uint32_t* n_chunks, uint32_t* grid, uint32_t* block) {
+  if (!b) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (n_chunks) *n_chunks = b->n_chunks;
+  if (grid) *grid = b->grid;
+  if (block) *block = b->block;
+  return SQ_OK;
+}
+
+int sq_batch_destroy(sq_batch* b) {
+  if (!b) return SQ_OK;
+  sq_ctx* c = b->ctx;
+  (void)hipStreamSynchronize(c->stream);
+  delete b;
+  ctx_release(c);
+  return SQ_OK;
+}
+
+}  // extern "C"

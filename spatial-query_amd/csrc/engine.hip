@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "engine_internal.h"
+#include "pool.h"
 
 thread_local std::string g_err;
 
@@ -60,12 +61,8 @@ __global__ __launch_bounds__(1024) void k_nearest(SceneView s, const float* __re
 }
 
 
-// Pooled nearest / any-hit / shape-cast kernel.  Each 64-lane wave owns a pool
-// of 64 * per_lane consecutive queries; a lane that finishes its query takes
-// the next one from the pool (ballot + popcount, no atomics), so lanes stay
-// busy while long queries run.  Each step descends to one leaf and tests it
-// (the walk of traverse.h).  Stack: an LDS ring of S entries per lane spilling
-// to a global column.
+// Pooled nearest / any-hit / shape-cast kernel: wave w of the grid walks the
+// queries [w * 64 * per_lane, (w + 1) * 64 * per_lane) with pool_walk.
 template <int D, int KIND, bool ANY, bool FILT, bool CAST>
 __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nearest_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                       const float* __restrict__ tmax, Filter f, NearestOut out,
@@ -73,72 +70,11 @@ __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nea
   extern __shared__ uint32_t lds[];
   const uint32_t B = blockDim.x, tid = threadIdx.x;
   const uint32_t gtid = blockIdx.x * B + tid;
-  const uint32_t lane = tid & 63u;
-  using Pol = typename std::conditional<CAST, CastPolicy<D, KIND>, RayPolicy<D, KIND>>::type;
-  constexpr int QS = CAST ? 4 * D : 2 * D;
-  constexpr int PS = Prim<D, KIND>::STRIDE;
   const uint64_t pool = 64ull * per_lane;
-  uint64_t next = ((uint64_t)blockIdx.x * (B >> 6) + (tid >> 6)) * pool;   // wave-uniform
+  const uint64_t next = ((uint64_t)blockIdx.x * (B >> 6) + (tid >> 6)) * pool;   // wave-uniform
   const uint64_t end = next + pool < (uint64_t)n ? next + pool : (uint64_t)n;
-  const uint64_t lt_mask = (1ull << lane) - 1ull;
   RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
-  Pol pol;
-  WalkState w;
-  float lim0 = 0.0f;
-  uint32_t rid = NO_LEAF;
-  bool exhausted = next >= end;
-  while (true) {
-    const bool need = rid == NO_LEAF && !exhausted;
-    const uint64_t m = __ballot(need);
-    if (m) {
-      if (need) {
-        const uint64_t my = next + __popcll(m & lt_mask);
-        if (my < end) {
-          rid = (uint32_t)my;
-          pol.setup(q + (size_t)rid * QS);
-          lim0 = tmax ? tmax[rid] : max_toi;
-          if (!walk_begin<D>(s, pol, lim0, w)) {
-            if (ANY) out.any[rid] = 0;
-            else {
-              out.leaf[rid] = -1; out.toi[rid] = SQ_INF;
-              if (out.point) for (int k = 0; k < D; ++k) out.point[(size_t)rid * D + k] = 0.0f;
-              if (!CAST && out.normal) for (int k = 0; k < D; ++k) out.normal[(size_t)rid * D + k] = 0.0f;
-              if (out.sub) out.sub[rid] = -1;
-            }
-            rid = NO_LEAF;
-          }
-        } else {
-          exhausted = true;
-        }
-      }
-      next += __popcll(m);
-    }
-    if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
-    if (rid == NO_LEAF) continue;
-    if (!walk_step<D, PS, ANY, FILT>(s, pol, f, st, w)) continue;
-    // finished: write the result of query rid
-    const NearestResult& h = w.res;
-    const bool hit = h.leaf != NO_LEAF;
-    if (ANY) {
-      out.any[rid] = hit ? 1 : 0;
-    } else {
-      out.leaf[rid] = hit ? (int32_t)h.leaf : -1;
-      out.toi[rid] = h.t;
-      if (out.point) {
-#pragma unroll
-        for (int k = 0; k < D; ++k) out.point[(size_t)rid * D + k] = hit ? pol.r.o[k] + pol.r.d[k] * h.t : 0.0f;
-      }
-      if (!CAST && out.normal) {
-        float nv[D];
-        if (hit) hit_normal<D, KIND>(pol.r, s.prims + (size_t)h.pos * PS, h.t, lim0, nv);
-        else for (int k = 0; k < D; ++k) nv[k] = 0.0f;
-#pragma unroll
-        for (int k = 0; k < D; ++k) out.normal[(size_t)rid * D + k] = nv[k];
-      }
-      if (out.sub) out.sub[rid] = (hit && KIND != KIND_AABB) ? (int32_t)h.leaf : -1;
-    }
-    rid = NO_LEAF;
-  }
+  pool_walk<D, KIND, ANY, FILT, CAST>(s, q, next, end, max_toi, tmax, f, out, st, nullptr);
 }
 
 template <int D>

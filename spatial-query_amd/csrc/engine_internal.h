@@ -101,6 +101,33 @@ template <class F> inline int dispatch_dk(int dim, int kind, F&& f) {
 #undef SQ_CASE
 }
 
+// Instance kinds of a two-level tree: BOX in D = 2..6, MESH in D = 3.
+template <class F> inline int dispatch_inst(int dim, int kind, F&& f) {
+  if (kind == INST_MESH) {
+    if (dim != 3) return fail(SQ_ERR_UNSUPPORTED, "mesh instances need dim 3");
+    return f.template operator()<3, INST_MESH>();
+  }
+  if (kind != INST_BOX) return fail(SQ_ERR_INVALID, "unknown instance kind");
+  switch (dim) {
+    case 2: return f.template operator()<2, INST_BOX>();
+    case 3: return f.template operator()<3, INST_BOX>();
+    case 4: return f.template operator()<4, INST_BOX>();
+    case 5: return f.template operator()<5, INST_BOX>();
+    case 6: return f.template operator()<6, INST_BOX>();
+    default: return fail(SQ_ERR_UNSUPPORTED, "dimension not instantiated (2..6)");
+  }
+}
+
+// A two-level tree's device view and sizes (tlas.hip).
+struct TlasInfo {
+  sq_ctx* ctx;
+  int dim, kind;
+  uint64_t n_inst;
+  uint32_t s_top, s_bot;   // stack depths of the top tree and the deepest mesh tree
+  TlasView view;
+};
+void tlas_info(const sq_tlas* t, TlasInfo* out);
+
 struct NearestOut {
   int32_t* leaf;
   float* toi;
@@ -130,7 +157,18 @@ inline Filter filter_of(const sq_bvh* b, const sq_filter* f, bool* filt) {
   return out;
 }
 
+// True while the context stream records a graph: timing events are then left
+// out (the enqueued work stays the same).
+inline bool capturing(const sq_ctx* c) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(c->stream, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
 inline int begin_timing(sq_ctx* c) {
+  if (capturing(c)) {
+    c->timed = false;
+    return SQ_OK;
+  }
   if (!c->ev0) {
     HIP_TRY(hipEventCreate(&c->ev0));
     HIP_TRY(hipEventCreate(&c->ev1));
@@ -139,6 +177,7 @@ inline int begin_timing(sq_ctx* c) {
   return SQ_OK;
 }
 inline int end_timing(sq_ctx* c) {
+  if (capturing(c)) return SQ_OK;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   c->timed = true;
   return SQ_OK;

@@ -187,21 +187,6 @@ __global__ __launch_bounds__(1024) void k_tlas_collect(TlasView tv, const float*
 // host
 // ---------------------------------------------------------------------------
 
-template <class F> static int dispatch_inst(int dim, int kind, F&& f) {
-  if (kind == INST_MESH) {
-    if (dim != 3) return fail(SQ_ERR_UNSUPPORTED, "mesh instances need dim 3");
-    return f.template operator()<3, INST_MESH>();
-  }
-  if (kind != INST_BOX) return fail(SQ_ERR_INVALID, "unknown instance kind");
-  switch (dim) {
-    case 2: return f.template operator()<2, INST_BOX>();
-    case 3: return f.template operator()<3, INST_BOX>();
-    case 4: return f.template operator()<4, INST_BOX>();
-    case 5: return f.template operator()<5, INST_BOX>();
-    case 6: return f.template operator()<6, INST_BOX>();
-    default: return fail(SQ_ERR_UNSUPPORTED, "dimension not instantiated (2..6)");
-  }
-}
 
 static TlasView tview(const sq_tlas* t) {
   TlasView v;
@@ -211,6 +196,16 @@ static TlasView tview(const sq_tlas* t) {
   v.blas = t->blas.as<SceneView>();
   v.xs = record_stride(t->in.dim, t->in.kind);
   return v;
+}
+
+void tlas_info(const sq_tlas* t, TlasInfo* out) {
+  out->ctx = t->ctx;
+  out->dim = t->in.dim;
+  out->kind = t->in.kind;
+  out->n_inst = t->in.n;
+  out->s_top = t->top ? t->top->packed.max_stack : 0;
+  out->s_bot = t->blas_stack;
+  out->view = tview(t);
 }
 
 static int tlas_upload_world(sq_tlas* t, std::vector<float>& world) {

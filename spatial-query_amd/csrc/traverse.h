@@ -21,6 +21,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstring>
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>

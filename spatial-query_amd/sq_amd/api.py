@@ -18,7 +18,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .capi import Filter, SqError, TreeStats, Tuning, check, lib, SQ_ERR_CAPACITY
+from .capi import BatchQuery, Filter, SqError, TreeStats, Tuning, check, lib, SQ_BATCH_BVH, SQ_BATCH_TLAS, SQ_ERR_CAPACITY
 
 ALL_LAYERS = (1 << 64) - 1
 
@@ -389,6 +389,56 @@ class Tlas:
     def close(self):
         if getattr(self, "_h", None):
             lib().sq_tlas_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchSet:
+    """Many nearest-hit batches over many scenes (all DeviceBvh, or all Tlas, of one
+    dimension and kind) answered by one persistent work-stealing launch per submit().
+
+    queries: [(scene, rays [n][2D] GPU tensor, max_toi)]; results land in
+    self.leaf[i] / self.toi[i] (instance ids for Tlas scenes).  submit() only
+    enqueues kernels, so it can be recorded into a CUDA/HIP graph.
+    """
+
+    def __init__(self, ctx: Context, queries):
+        import torch
+        if not queries:
+            raise ValueError("no queries")
+        tlas = isinstance(queries[0][0], Tlas)
+        arr = (BatchQuery * len(queries))()
+        self.leaf, self.toi, self._keep = [], [], []
+        for i, (scene, rays, max_toi) in enumerate(queries):
+            if isinstance(scene, Tlas) != tlas:
+                raise ValueError("a batch set holds scenes of one class")
+            n = rays.shape[0]
+            l = torch.empty(n, dtype=torch.int32, device=rays.device)
+            t = torch.empty(n, dtype=torch.float32, device=rays.device)
+            self.leaf.append(l); self.toi.append(t); self._keep.append((scene, rays))
+            arr[i] = BatchQuery(scene._h, _dp(rays).value if n else None, n, float(max_toi),
+                                _dp(l).value if n else None, _dp(t).value if n else None)
+        h = C.c_void_p()
+        check(lib().sq_batch_create(ctx.handle, SQ_BATCH_TLAS if tlas else SQ_BATCH_BVH, C.cast(arr, C.c_void_p),
+                                    len(queries), C.byref(h)))
+        self._h, self.ctx = h, ctx
+
+    def submit(self):
+        check(lib().sq_batch_submit(self._h))
+
+    def info(self) -> dict:
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().sq_batch_get_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(chunks=a.value, grid=b.value, block=c.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -9,6 +9,7 @@ _LIB = os.path.join(os.path.dirname(_HERE), "lib", "libsq_amd.so")
 
 SQ_OK, SQ_ERR_INVALID, SQ_ERR_NO_DEVICE, SQ_ERR_HIP, SQ_ERR_UNSUPPORTED, SQ_ERR_CAPACITY = range(6)
 SQ_PRIM_AABB, SQ_PRIM_SPHERE, SQ_PRIM_TRIANGLE = 0, 1, 2
+SQ_BATCH_BVH, SQ_BATCH_TLAS = 0, 1
 
 
 class SqError(RuntimeError):
@@ -24,6 +25,11 @@ class Filter(C.Structure):
 class TreeStats(C.Structure):
     _fields_ = [("node_count", C.c_uint64), ("leaf_count", C.c_uint64), ("prim_count", C.c_uint64),
                 ("max_depth", C.c_uint64), ("bytes", C.c_uint64), ("quality", C.c_float)]
+
+
+class BatchQuery(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("rays", C.c_void_p), ("n", C.c_uint64), ("max_toi", C.c_float),
+                ("out_leaf", C.c_void_p), ("out_toi", C.c_void_p)]
 
 
 class Tuning(C.Structure):
@@ -54,6 +60,10 @@ SIGNATURES = {
     "sq_tlas_raycast_all": [P, P, U64, F32, P, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_tlas_overlap": [P, P, U64, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_ctx_set_block": [P, I32],
+    "sq_batch_create": [P, I32, P, U32, PP],
+    "sq_batch_submit": [P],
+    "sq_batch_get_info": [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
+    "sq_batch_destroy": [P],
     "sq_proposal_create": [P, P, I32, F32, U64, PP],
     "sq_proposal_destroy": [P],
     "sq_proposal_num_subtrees": [P, C.POINTER(C.c_uint32)],
