@@ -226,11 +226,17 @@ int sq_tlas_overlap(sq_tlas* tlas, const float* regions, uint64_t n, uint64_t* o
  *                  lanes from it; 0 selects one query per thread with the whole
  *                  stack in LDS
  *   lds_stack      entries of each lane's traversal stack kept in LDS (power of
- *                  two); deeper entries spill to device memory */
+ *                  two); deeper entries spill to device memory
+ *   ray_order      1: pooled nearest / any-hit calls of >= 4096 rays walk the
+ *                  rays in Morton order (a 30-bit key interleaving the origin,
+ *                  quantized in the scene box, with the direction; sorted on
+ *                  the device); results still land at each ray's own index.
+ *                  Not for graph capture (the sort is a library call). */
 typedef struct sq_tuning {
   int block;
   int rays_per_lane;
   int lds_stack;
+  int ray_order;
 } sq_tuning;
 int sq_ctx_set_tuning(sq_ctx* ctx, const sq_tuning* tuning);
 int sq_ctx_get_tuning(const sq_ctx* ctx, sq_tuning* tuning);

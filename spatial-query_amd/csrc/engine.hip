@@ -62,11 +62,13 @@ __global__ __launch_bounds__(1024) void k_nearest(SceneView s, const float* __re
 
 
 // Pooled nearest / any-hit / shape-cast kernel: wave w of the grid walks the
-// queries [w * 64 * per_lane, (w + 1) * 64 * per_lane) with pool_walk.
+// queries [w * 64 * per_lane, (w + 1) * 64 * per_lane) with pool_walk (of the
+// Morton order when `order` is given).
 template <int D, int KIND, bool ANY, bool FILT, bool CAST>
 __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nearest_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                       const float* __restrict__ tmax, Filter f, NearestOut out,
-                                                      uint32_t per_lane, uint32_t S, SpillEntry* spill, uint32_t spill_stride) {
+                                                      uint32_t per_lane, uint32_t S, SpillEntry* spill, uint32_t spill_stride,
+                                                      const uint32_t* __restrict__ order) {
   extern __shared__ uint32_t lds[];
   const uint32_t B = blockDim.x, tid = threadIdx.x;
   const uint32_t gtid = blockIdx.x * B + tid;
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nea
   const uint64_t next = ((uint64_t)blockIdx.x * (B >> 6) + (tid >> 6)) * pool;   // wave-uniform
   const uint64_t end = next + pool < (uint64_t)n ? next + pool : (uint64_t)n;
   RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
-  pool_walk<D, KIND, ANY, FILT, CAST>(s, q, next, end, max_toi, tmax, f, out, st, nullptr);
+  pool_walk<D, KIND, ANY, FILT, CAST>(s, q, next, end, max_toi, tmax, f, out, st, order);
 }
 
 template <int D>
@@ -470,6 +472,7 @@ int sq_ctx_create(int device, sq_ctx** out) {
   HIP_TRY(hipSetDevice(device));
   auto c = std::make_unique<sq_ctx>();
   c->device = device;
+  HIP_TRY(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device));
   HIP_TRY(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
   c->stream = c->own;
   *out = c.release();
@@ -510,9 +513,11 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   if (t->rays_per_lane < 0 || t->rays_per_lane > 4096) return fail(SQ_ERR_INVALID, "rays_per_lane out of range");
   const int S = t->lds_stack ? t->lds_stack : 8;
   if (S < 2 || S > 64 || (S & (S - 1))) return fail(SQ_ERR_INVALID, "lds_stack must be a power of two in [2, 64]");
+  if (t->ray_order != 0 && t->ray_order != 1) return fail(SQ_ERR_INVALID, "ray_order must be 0 or 1");
   c->block = block;
   c->rays_per_lane = t->rays_per_lane;
   c->lds_stack = S;
+  c->ray_order = t->ray_order;
   return SQ_OK;
 }
 
@@ -521,6 +526,7 @@ int sq_ctx_get_tuning(const sq_ctx* c, sq_tuning* t) {
   t->block = c->block;
   t->rays_per_lane = c->rays_per_lane;
   t->lds_stack = c->lds_stack;
+  t->ray_order = c->ray_order;
   return SQ_OK;
 }
 
@@ -806,9 +812,11 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
     if (rc) return rc;
     return end_timing(c);
   }
-  // pooled kernel
+  // pooled kernel; small calls halve the pool until the grid has pooled_waves()
+  // waves, so that a call of a few thousand queries still spreads over the chip
   const uint32_t S = (uint32_t)c->lds_stack;
-  const uint64_t pool = 64ull * (uint64_t)c->rays_per_lane;
+  const uint32_t per = pooled_per_lane(c, n);
+  const uint64_t pool = 64ull * per;
   const uint64_t waves = (n + pool - 1) / pool;
   const uint64_t blocks = (waves + (B / 64) - 1) / (B / 64);
   const uint64_t lanes = blocks * (uint64_t)B;
@@ -819,12 +827,15 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
   const size_t lds = (size_t)S * B * 8;
   rc = begin_timing(c);
   if (rc) return rc;
+  const uint32_t* order = nullptr;
+  if (c->ray_order && !cast && n >= 4096)
+    if ((rc = ray_order(c, b->dim, q, nn, 2 * b->dim, b->root.as<float>(), &order))) return rc;
   SpillEntry* spill = c->spill.as<SpillEntry>();
-  const uint32_t spl = (uint32_t)lanes, per = (uint32_t)c->rays_per_lane;
+  const uint32_t spl = (uint32_t)lanes;
   rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
     dim3 grid((uint32_t)blocks), block(B);
 #define SQ_LAUNCH(ANY_, FILT_, CAST_) \
-    k_nearest_pool<D, K, ANY_, FILT_, CAST_><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, out, per, S, spill, spl)
+    k_nearest_pool<D, K, ANY_, FILT_, CAST_><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, out, per, S, spill, spl, order)
     if (cast) { if (filt) SQ_LAUNCH(false, true, true); else SQ_LAUNCH(false, false, true); }
     else if (any) { if (filt) SQ_LAUNCH(true, true, false); else SQ_LAUNCH(true, false, false); }
     else { if (filt) SQ_LAUNCH(false, true, false); else SQ_LAUNCH(false, false, false); }

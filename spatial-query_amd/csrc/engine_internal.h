@@ -54,16 +54,19 @@ struct DevBuf {
 
 struct sq_ctx {
   int device = 0;
+  int cus = 256;                   // compute units of the device
   hipStream_t own = nullptr;       // the stream the context created
   hipStream_t stream = nullptr;    // where work is enqueued (own, or the caller's)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int block = 128;
   int rays_per_lane = 8;           // pooled nearest kernel: rays per lane (0 = one ray per thread)
   int lds_stack = 8;               // pooled kernel: ring-stack entries kept in LDS per lane
+  int ray_order = 0;               // 1: walk rays in Morton order (order.hip)
   bool timed = false;
   int refs = 1;                    // the caller's handle + one per live tree
   DevBuf counts, scratch, keys, keys2;   // reusable temporaries of the CSR queries
   DevBuf spill;                    // ring-stack spill columns of the pooled kernel
+  DevBuf okeys, okeys2, oidx, oorder, otemp;   // Morton ray order
 };
 
 // Drops one reference; the last one frees the stream, events and buffers.
@@ -139,6 +142,15 @@ struct NearestOut {
 
 inline uint32_t grid_for(uint64_t n, int block) { return (uint32_t)((n + block - 1) / block); }
 
+// Rays per lane of a pooled launch over n queries: the tuned pool size, halved
+// while the grid would hold fewer than 8 waves per compute unit.
+inline uint32_t pooled_per_lane(const sq_ctx* c, uint64_t n) {
+  uint32_t per = (uint32_t)(c->rays_per_lane > 0 ? c->rays_per_lane : 1);
+  const uint64_t want = 8ull * (uint64_t)c->cus;
+  while (per > 1 && (n + 64ull * per - 1) / (64ull * per) < want) per >>= 1;
+  return per;
+}
+
 inline SceneView view_of(const sq_bvh* b) {
   SceneView s;
   s.nodes = b->nodes.as<float>();
@@ -193,6 +205,11 @@ inline int upload(DevBuf& d, const void* h, size_t bytes, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+
+// Morton order of n queries (origin and direction in the first 2 dim floats
+// of each qs-float record; origins quantized in box = [min dim][max dim],
+// device): *order (context buffer) lists query indices in key order.
+int ray_order(sq_ctx* c, int dim, const float* q, uint32_t n, uint32_t qs, const float* box, const uint32_t** order);
 
 // Exclusive scan of n counts into offsets[0..n] on the context stream.
 // Writes misses (leaf -1, toi +inf, zero point/normal, sub -1, any 0).

@@ -890,7 +890,7 @@ int sq_raycast_nearest_proposed(sq_proposal* p, const float* rays, uint64_t n, f
   sq_bvh* b = p->scene;
   const int B = std::min(c->block, 256);
   const uint32_t S = (uint32_t)c->lds_stack;
-  const uint32_t per = (uint32_t)std::max(1, c->rays_per_lane);
+  const uint32_t per = pooled_per_lane(c, n);
   const uint64_t pool = 64ull * per;
   const uint64_t waves = (n + pool - 1) / pool;
   const uint64_t blocks = (waves + (B / 64) - 1) / (B / 64);

@@ -87,13 +87,13 @@ class Context:
     def set_block(self, threads: int):
         check(lib().sq_ctx_set_block(self._h, threads))
 
-    def set_tuning(self, block: int = 128, rays_per_lane: int = 8, lds_stack: int = 8):
-        check(lib().sq_ctx_set_tuning(self._h, C.byref(Tuning(block, rays_per_lane, lds_stack))))
+    def set_tuning(self, block: int = 128, rays_per_lane: int = 8, lds_stack: int = 8, ray_order: int = 0):
+        check(lib().sq_ctx_set_tuning(self._h, C.byref(Tuning(block, rays_per_lane, lds_stack, ray_order))))
 
     def tuning(self) -> dict:
         t = Tuning()
         check(lib().sq_ctx_get_tuning(self._h, C.byref(t)))
-        return dict(block=t.block, rays_per_lane=t.rays_per_lane, lds_stack=t.lds_stack)
+        return dict(block=t.block, rays_per_lane=t.rays_per_lane, lds_stack=t.lds_stack, ray_order=t.ray_order)
 
     def last_kernel_ms(self) -> float:
         ms = C.c_float()

@@ -1,0 +1,257 @@
+"""Workload drivers for the batched/instanced scene and the N-dimensional scene.
+
+    python -m sq_amd.benches c2 [--frames 60] [--batches 256] [--no-tune]
+    python -m sq_amd.benches c4
+
+c2: 16 noise-displaced icospheres (2k-50k triangles, log-uniform) placed as
+4,096 instances (uniform rotations, translations U[-1000,1000]^3, scales
+U[0.5,4]) plus 100,000 boxes moving with constant velocities U[-5,5]^3.  Each
+frame refits the boxes on the device and answers 256 nearest-hit batches
+(sizes log-uniform in 1k-1M rays, origins U[-1500,1500]^3, uniform directions,
+t_max 4000; even batches on the instances, odd ones on the boxes) with two
+persistent batch launches, the whole frame recorded once into a graph and
+replayed.  The tuner picks block size, pool size and stack depth first.
+
+c4: 1,000,000 boxes in D = 6 from a 32-component Gaussian mixture; 1M
+nearest-hit rays, 262,144 overlap queries (sorted id lists) and 262,144 box
+casts (half-extent 0.5, sweep U(0,50]).
+
+Each prints one JSON line with timings and the consistency checks it ran
+(every batch against its single-scene call, in full).  The CPU-oracle checks of
+these workloads live in tests/test_gpu_workloads.py.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+
+def log(msg):
+    print(f"[benches] {msg}", file=sys.stderr, flush=True)
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def _same(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def _same_bits(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
+def c2(frames=60, n_batches=256, tune=True, check=True, seed=0xC2C2_0000_000A):
+    import torch
+    import sq_amd
+    from sq_amd import workloads as W
+    from sq_amd.autotune import autotune
+
+    out = dict(workload="c2: 16 meshes x 4096 instances + 100k moving boxes; "
+                        f"{frames} frames x {n_batches} batches (log-U 1k-1M rays)")
+    t0 = time.time()
+    sc = W.instanced_scene()
+    s = W.Stream(seed)
+    boxes = W.boxes_scene(n=100_000, seed=int(s.u64(1)[0]), lo=-1000.0, hi=1000.0, hmin=0.5, hmax=5.0).prims
+    vel = s.range(-5.0, 5.0, 100_000 * 3).reshape(100_000, 3).astype(np.float32)
+    sizes = np.round(np.exp(s.range(math.log(1e3), math.log(1e6), n_batches).astype(np.float64))).astype(np.int64)
+    ray_seeds = [int(v) for v in s.u64(n_batches)]
+    rays_h = [W.random_rays(int(n), rs, -1500.0, 1500.0) for n, rs in zip(sizes, ray_seeds)]
+    tmax = 4000.0
+    log(f"generated in {time.time() - t0:.1f}s: {int(sizes.sum())} rays per frame")
+    out.update(rays_per_frame=int(sizes.sum()), batches=n_batches, frames=frames,
+               mesh_tris=[int(m.shape[0]) for m in sc["meshes"]], instances=int(sc["rot"].shape[0]), boxes=100_000)
+
+    ctx = sq_amd.Context(0)
+    stream = torch.cuda.Stream()
+    ctx.set_stream(stream.cuda_stream)
+    t0 = time.time()
+    meshes = [sq_amd.DeviceBvh.build(ctx, 3, sq_amd.TRIANGLE, m) for m in sc["meshes"]]
+    tlas = sq_amd.Tlas(ctx, 3, sq_amd.Tlas.MESH, sc["rot"], sc["trans"], meshes=meshes, mesh_of=sc["mesh_of"],
+                       scale=sc["scale"])
+    bvh = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.AABB, boxes)
+    prims_dev = _dev(boxes)
+    rays = [_dev(r) for r in rays_h]
+    torch.cuda.synchronize()
+    log(f"scenes built in {time.time() - t0:.1f}s")
+    qt = [(tlas, rays[i], tmax) for i in range(0, n_batches, 2)]
+    qb = [(bvh, rays[i], tmax) for i in range(1, n_batches, 2)]
+
+    def frame_sets():
+        return sq_amd.BatchSet(ctx, qt), sq_amd.BatchSet(ctx, qb)
+
+    if tune:
+        def prepare():
+            bt, bb = frame_sets()
+
+            def go():
+                bvh.refit(prims_dev)
+                bt.submit()
+                bb.submit()
+                return tuple(bt.leaf) + tuple(bb.leaf)
+            prepare.keep = (bt, bb)
+            return go
+        t0 = time.time()
+        res = autotune(ctx, prepare, space=dict(block=(64, 128, 256), rays_per_lane=(2, 4, 8, 16), lds_stack=(4, 8, 16)),
+                       reps=2)
+        prepare.keep = None
+        out["tuning"] = res["best"]
+        out["tuning_ms"] = round(res["ms"], 3)
+        out["tuning_default_ms"] = round(res["baseline"], 3)
+        log(f"tuned in {time.time() - t0:.1f}s: {res['best']} {res['ms']:.2f} ms (default {res['baseline']:.2f} ms)")
+    bt, bb = frame_sets()
+    out["chunks"] = bt.info()["chunks"] + bb.info()["chunks"]
+    # warm-up (sizes buffers), then record the frame once
+    bvh.refit(prims_dev)
+    bt.submit()
+    bb.submit()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=stream):
+        bvh.refit(prims_dev)
+        bt.submit()
+        bb.submit()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def replay_ms():
+        with torch.cuda.stream(stream):
+            e0.record()
+            g.replay()
+            e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1)
+
+    # the frame-0 state (freshly built trees): graph replay against the same
+    # frame issued as one call per batch
+    prims_dev.copy_(torch.from_numpy(boxes))
+    torch.cuda.synchronize()
+    f0 = [replay_ms() for _ in range(5)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        bvh.refit(prims_dev)
+        for (scene, r, tm) in qt + qb:
+            scene.raycast_nearest(r, tm)
+    ctx.synchronize()
+    single = (time.perf_counter() - t0) * 1e3 / 3
+    f0m = float(np.median(f0))
+    out.update(frame0_ms=round(f0m, 3), frame0_mrays_per_s=round(float(sizes.sum()) / (f0m * 1e-3) / 1e6, 1),
+               single_calls_frame0_ms=round(single, 3), batching_gain=round(single / f0m, 3))
+    log(f"frame 0: graph {f0m:.2f} ms, single calls {single:.2f} ms")
+    fms, parity = [], []
+    for f in range(frames):
+        moved = boxes.copy()
+        d = vel * np.float32(f)
+        moved[:, :3] += d
+        moved[:, 3:] += d
+        prims_dev.copy_(torch.from_numpy(moved))
+        torch.cuda.synchronize()
+        fms.append(replay_ms())
+        if check and f in (0, frames - 1):
+            # every batch equals its single-scene call
+            ok = True
+            for i, (scene, r, tm) in enumerate(qt):
+                w = scene.raycast_nearest(r, tm)
+                ctx.synchronize()
+                ok &= _same(bt.leaf[i].cpu(), w["leaf"].cpu()) and _same_bits(bt.toi[i].cpu(), w["toi"].cpu())
+            for i, (scene, r, tm) in enumerate(qb):
+                w = scene.raycast_nearest(r, tm)
+                ctx.synchronize()
+                ok &= _same(bb.leaf[i].cpu(), w["leaf"].cpu()) and _same_bits(bb.toi[i].cpu(), w["toi"].cpu())
+            parity.append(dict(frame=f, batches_equal_single_calls=bool(ok)))
+            log(f"frame {f}: batches == single calls {ok}")
+    out["parity"] = parity
+    fm = float(np.median(fms))
+    out.update(frame_ms_median=round(fm, 3), frame_ms_min=round(float(np.min(fms)), 3),
+               mrays_per_s=round(float(sizes.sum()) / (fm * 1e-3) / 1e6, 1),
+               frame_ms=[round(v, 2) for v in fms])
+    # the refit alone
+    with torch.cuda.stream(stream):
+        e0.record()
+        for _ in range(10):
+            bvh.refit(prims_dev)
+        e1.record()
+    e1.synchronize()
+    out["refit_ms"] = round(e0.elapsed_time(e1) / 10, 4)
+    log(f"60 frames: median {fm:.2f} ms ({out['mrays_per_s']} Mrays/s; refit-only trees loosen as the boxes "
+        f"drift), refit {out['refit_ms']} ms")
+    return out
+
+
+def c4():
+    import torch
+    import sq_amd
+    from sq_amd import workloads as W
+
+    t0 = time.time()
+    sc = W.gmm_boxes_scene()
+    rays, regions, casts, lim = W.c4_queries(sc)
+    log(f"generated in {time.time() - t0:.1f}s")
+    out = dict(workload="c4: D=6, 1,000,000 GMM boxes; 1M rays, 262,144 overlaps, 262,144 box casts")
+    ctx = sq_amd.Context(0)
+    t0 = time.time()
+    bvh = sq_amd.DeviceBvh.build(ctx, 6, sq_amd.AABB, sc.prims)
+    torch.cuda.synchronize()
+    out["build_s"] = round(time.time() - t0, 2)
+    st = bvh.stats()
+    out.update(tree_nodes=st["node_count"], tree_depth=st["max_depth"])
+    r, q, cst, lm = _dev(rays), _dev(regions), _dev(casts), _dev(lim)
+
+    def best_ms(fn, reps=5):
+        fn()
+        ctx.synchronize()
+        ts = []
+        for _ in range(reps):
+            fn()
+            ts.append(ctx.last_kernel_ms())
+        return min(ts)
+
+    res = {}
+    ms = best_ms(lambda: res.__setitem__("ray", bvh.raycast_nearest(r, sc.max_toi)))
+    out["nearest_ms"] = round(ms, 4)
+    out["nearest_mrays"] = round(rays.shape[0] / (ms * 1e-3) / 1e6, 1)
+    # overlap: count + fill + sort is several launches; time the whole call
+    bvh.overlap(q)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        off, leaf = bvh.overlap(q)
+    ctx.synchronize()
+    oms = (time.perf_counter() - t0) * 1e3 / 3
+    out.update(overlap_ms=round(oms, 3), overlap_mqueries=round(regions.shape[0] / (oms * 1e-3) / 1e6, 2),
+               overlap_pairs=int(off[-1].item()))
+    ms = best_ms(lambda: res.__setitem__("cast", bvh.shapecast_nearest(cst, 50.0, max_toi_per_cast=lm)))
+    out["cast_ms"] = round(ms, 4)
+    out["cast_mqueries"] = round(casts.shape[0] / (ms * 1e-3) / 1e6, 1)
+    hr = res["ray"]["leaf"].cpu().numpy()
+    hc = res["cast"]["leaf"].cpu().numpy()
+    out.update(ray_hit_fraction=round(float((hr >= 0).mean()), 4), cast_hit_fraction=round(float((hc >= 0).mean()), 4))
+    log(f"nearest {out['nearest_mrays']} Mrays/s, overlap {out['overlap_mqueries']} Mq/s ({out['overlap_pairs']} pairs), "
+        f"casts {out['cast_mqueries']} Mq/s")
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("which", choices=["c2", "c4"])
+    ap.add_argument("--frames", type=int, default=60)
+    ap.add_argument("--batches", type=int, default=256)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    a = ap.parse_args()
+    if a.which == "c2":
+        res = c2(frames=a.frames, n_batches=a.batches, tune=not a.no_tune, check=not a.no_check)
+    else:
+        res = c4()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

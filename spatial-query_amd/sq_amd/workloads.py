@@ -256,6 +256,38 @@ def gmm_boxes_scene(n=1_000_000, dim=6, k=32, seed=0x6A55_1A40_0000_0006) -> Sce
     return Scene("gmm_boxes", dim, 0, prims, 1000.0, dict(seed=seed, means=means))
 
 
+def gmm_points(sc: Scene, n: int, seed: int, sigma=10.0) -> np.ndarray:
+    """Points near the mixture's components (a random mean + N(0, sigma^2)
+    per axis): query origins and centres that land where the boxes are."""
+    means = sc.meta["means"]
+    dim = means.shape[1]
+    s = Stream(seed)
+    comp = (s.u64(n) % np.uint64(means.shape[0])).astype(np.int64)
+    u1 = s.range(0.0, 1.0, n * dim).astype(np.float64).reshape(n, dim)
+    u2 = s.range(0.0, 1.0, n * dim).astype(np.float64).reshape(n, dim)
+    z = np.sqrt(-2.0 * np.log(np.maximum(u1, 1e-12))) * np.cos(2 * math.pi * u2)
+    return (means[comp] + sigma * z).astype(np.float32)
+
+
+def c4_queries(sc: Scene, n_rays=1_000_000, n_overlap=262_144, n_cast=262_144, seed=0xC4C4_0000_0009):
+    """C4 queries on the D-dimensional mixture scene: rays from points near the
+    components with uniform directions; overlap boxes centred near the
+    components with half-extents U[1, 8]; box casts (half-extent 0.5) from
+    points near the components, uniform directions, sweep length U(0, 50]."""
+    dim = sc.dim
+    o = gmm_points(sc, n_rays, seed)
+    rays = rays_array(o, unit_dirs(Stream(seed + 1), n_rays, dim))
+    c = gmm_points(sc, n_overlap, seed + 2)
+    h = Stream(seed + 3).range(1.0, 8.0, n_overlap * dim).reshape(n_overlap, dim)
+    regions = np.concatenate([c - h, c + h], axis=1).astype(np.float32)
+    co = gmm_points(sc, n_cast, seed + 4)
+    cd = unit_dirs(Stream(seed + 5), n_cast, dim)
+    half = np.full((n_cast, dim), 0.5, np.float32)
+    casts = np.concatenate([co, cd, -half, half], axis=1).astype(np.float32)
+    lim = (50.0 - Stream(seed + 6).range(0.0, 50.0, n_cast)).astype(np.float32)   # (0, 50]
+    return rays, regions, casts, lim
+
+
 # --- instanced meshes --------------------------------------------------------
 
 _ICO_V = None
