@@ -87,6 +87,32 @@ def test_gradients_match_oracle(ctx, small, cut):
     p.close()
 
 
+def test_adam_step_matches_oracle(ctx, small):
+    """One training step from fresh moments equals Adam (lr, 0.9, 0.999, 1e-8)
+    applied in float64 to the oracle's gradients: every weight moves by
+    lr * sign(g) (m/c1 = g, v/c2 = g^2), within the fp32 update rounding."""
+    print("case: one Adam step", flush=True)
+    sc, bvh = small
+    p = sq_amd.Proposal(bvh, cut_depth=6, seed=0xADA)
+    rays = W.proposal_training_rays(sc, 4096, seed=0xADA)
+    r, lab, _ = _labelled(bvh, p, rays)
+    w0 = p.weights()
+    B, box = p.features_spec()
+    _, g = mlp.loss_and_grads(mlp.features(rays, B, box), w0, lab.cpu().numpy())
+    lr = 1e-3
+    p.train_step(r, lab, lr)
+    w1 = p.weights()
+    for l in range(4):
+        for k in range(2):
+            gg = np.asarray(g[l][k], np.float64)
+            m, v = 0.1 * gg, 0.001 * gg * gg
+            want = w0[l][k] - lr * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+            # where |g| is within a bf16 rounding of 0 the sign can differ: compare the rest
+            big = np.abs(gg) > 1e-2 * np.abs(gg).max()
+            assert np.abs(w1[l][k] - want)[big].max() <= 1e-6, (l, k)
+    p.close()
+
+
 @pytest.mark.parametrize("cut", [1, 9])
 def test_infer_top4_matches_oracle(ctx, small, cut):
     print(f"case: fused top-4 inference, cut depth {cut}", flush=True)
