@@ -40,6 +40,7 @@ def main():
     ap.add_argument("--rays-per-lane", type=int, default=-1, help="pooled kernel pool size per lane (0 = one ray per thread)")
     ap.add_argument("--lds-stack", type=int, default=0)
     ap.add_argument("--ray-order", type=int, default=-1, help="1: Morton ray order, 0: as given, -1: default")
+    ap.add_argument("--descend-min", type=int, default=-1, help="pause descents below this many lanes (-1: default)")
     ap.add_argument("--quick", action="store_true", help="short run for profiling: no CPU baseline, no side figures")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB device copy (the known-size kernel that calibrates HBM counters)")
@@ -73,6 +74,8 @@ def main():
         tun["lds_stack"] = args.lds_stack
     if args.ray_order >= 0:
         tun["ray_order"] = args.ray_order
+    if args.descend_min >= 0:
+        tun["descend_min"] = args.descend_min
     ctx.set_tuning(**tun)
     t0 = time.time()
     tree = sq_amd.Tree.build(3, sq_amd.TRIANGLE, sc.prims)

@@ -221,22 +221,30 @@ int sq_tlas_overlap(sq_tlas* tlas, const float* regions, uint64_t n, uint64_t* o
 /* Kernel tuning knobs.  Results never depend on them (the tests check this);
  * only speed does.  The autotuner (sq_amd.autotune) picks them per scene.
  *   block          threads per block, a multiple of 64 (pooled kernels: <= 256)
- *   rays_per_lane  nearest / any / shape-cast: each 64-lane wave owns a pool of
- *                  64 * rays_per_lane consecutive queries and refills finished
- *                  lanes from it; 0 selects one query per thread with the whole
- *                  stack in LDS
+ *   rays_per_lane  nearest / any / shape-cast: queries are handed out in chunks
+ *                  of 64 * rays_per_lane (default 16); a grid of the waves the
+ *                  chip holds resident walks them, each wave its own chunk
+ *                  first, then chunks from a shared counter, refilling finished
+ *                  lanes as it goes; small calls halve the chunk until every
+ *                  resident wave has one; 0 selects one query per thread with
+ *                  the whole stack in LDS
  *   lds_stack      entries of each lane's traversal stack kept in LDS (power of
  *                  two); deeper entries spill to device memory
  *   ray_order      1: pooled nearest / any-hit calls of >= 4096 rays walk the
  *                  rays in Morton order (a 30-bit key interleaving the origin,
  *                  quantized in the scene box, with the direction; sorted on
  *                  the device); results still land at each ray's own index.
- *                  Not for graph capture (the sort is a library call). */
+ *                  Not for graph capture (the sort is a library call).
+ *   descend_min    pooled walks: a wave stops waiting on descents once fewer
+ *                  than this many lanes (0..64) still descend while others
+ *                  have leaf tests or new queries to do; the paused descents
+ *                  resume in the next step (0 = never pause; default 24). */
 typedef struct sq_tuning {
   int block;
   int rays_per_lane;
   int lds_stack;
   int ray_order;
+  int descend_min;
 } sq_tuning;
 int sq_ctx_set_tuning(sq_ctx* ctx, const sq_tuning* tuning);
 int sq_ctx_get_tuning(const sq_ctx* ctx, sq_tuning* tuning);

@@ -46,30 +46,25 @@ __device__ __forceinline__ uint32_t find_item(const Item* __restrict__ items, ui
   return lo;
 }
 
-// Next chunk for the whole wave (called with every lane active).
-__device__ __forceinline__ uint32_t next_chunk(uint32_t* counter) {
-  uint32_t c = 0;
-  if ((threadIdx.x & 63u) == 0) c = atomicAdd(counter, 1u);
-  return __builtin_amdgcn_readfirstlane(c);
-}
-
 template <int D, int KIND>
 __global__ __launch_bounds__(256, (D <= 3 ? 8 : 4)) void k_batch_bvh(const BvhItem* __restrict__ items, uint32_t n_items,
                                                                      uint32_t n_chunks, uint32_t per_lane, uint32_t S,
-                                                                     SpillEntry* spill, uint32_t spill_stride, uint32_t* counter) {
+                                                                     SpillEntry* spill, uint32_t spill_stride, uint32_t* counter,
+                                                                     uint32_t pause_below) {
   extern __shared__ uint32_t lds[];
   const uint32_t B = blockDim.x, tid = threadIdx.x;
   RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + blockIdx.x * B + tid, spill_stride};
   const Filter nof{nullptr, ~0ull, -1};
   const uint64_t pool = 64ull * per_lane;
   for (;;) {
-    const uint32_t c = next_chunk(counter);
+    const uint32_t c = wave_fetch_add(counter);
     if (c >= n_chunks) break;
     const BvhItem& it = items[find_item(items, n_items, c)];
     const uint64_t first = (uint64_t)(c - it.chunk0) * pool;
     const uint64_t end = first + pool < (uint64_t)it.n ? first + pool : (uint64_t)it.n;
     const NearestOut out{it.leaf, it.toi, nullptr, nullptr, nullptr, nullptr};
-    pool_walk<D, KIND, false, false, false>(it.s, it.rays, first, end, it.max_toi, nullptr, nof, out, st, nullptr);
+    pool_walk<D, KIND, false, false, false>(it.s, it.rays, OnePool{first, end}, it.max_toi, nullptr, nof, out, st, nullptr,
+                                            pause_below);
   }
 }
 
@@ -87,7 +82,7 @@ __global__ __launch_bounds__(256) void k_batch_tlas(const TlasItem* __restrict__
   Stack sb{refs + s_top * B, ts + s_top * B, B};
   const uint64_t pool = 64ull * per_lane;
   for (;;) {
-    const uint32_t c = next_chunk(counter);
+    const uint32_t c = wave_fetch_add(counter);
     if (c >= n_chunks) break;
     const TlasItem& it = items[find_item(items, n_items, c)];
     const uint64_t first = (uint64_t)(c - it.chunk0) * pool;
@@ -112,7 +107,7 @@ __global__ void k_zero_u32(uint32_t* p) {
 struct sq_batch {
   sq_ctx* ctx = nullptr;
   int cls = 0, dim = 0, kind = 0;
-  uint32_t n_items = 0, n_chunks = 0, per_lane = 1, S = 8, s_top = 0, s_bot = 0, grid = 0, block = 128;
+  uint32_t n_items = 0, n_chunks = 0, per_lane = 1, S = 8, s_top = 0, s_bot = 0, grid = 0, block = 128, descend_min = 0;
   size_t lds = 0;
   DevBuf items, counter, spill;
   std::vector<std::pair<uint64_t, NearestOut>> empty;   // batches over scenes without primitives: all misses
@@ -131,6 +126,7 @@ int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* qs, uint
   b->per_lane = (uint32_t)std::max(1, ctx->rays_per_lane);
   b->S = (uint32_t)ctx->lds_stack;
   b->block = (uint32_t)std::min(ctx->block, 256);
+  b->descend_min = (uint32_t)ctx->descend_min;
   const uint64_t pool = 64ull * b->per_lane;
   std::vector<BvhItem> bi;
   std::vector<TlasItem> ti;
@@ -240,7 +236,8 @@ int sq_batch_submit(sq_batch* b) {
     if (b->cls == SQ_BATCH_BVH)
       rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
         k_batch_bvh<D, K><<<b->grid, b->block, b->lds, c->stream>>>(b->items.as<BvhItem>(), b->n_items, b->n_chunks, b->per_lane, b->S,
-                                                                   b->spill.as<SpillEntry>(), spill_stride, b->counter.as<uint32_t>());
+                                                                   b->spill.as<SpillEntry>(), spill_stride, b->counter.as<uint32_t>(),
+                                                                   b->descend_min);
         HIP_TRY(hipGetLastError());
         return SQ_OK;
       });

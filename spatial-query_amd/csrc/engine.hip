@@ -61,22 +61,28 @@ __global__ __launch_bounds__(1024) void k_nearest(SceneView s, const float* __re
 }
 
 
-// Pooled nearest / any-hit / shape-cast kernel: wave w of the grid walks the
-// queries [w * 64 * per_lane, (w + 1) * 64 * per_lane) with pool_walk (of the
-// Morton order when `order` is given).
+// Pooled nearest / any-hit / shape-cast kernel, persistent: the grid holds as
+// many waves as the chip keeps resident, and every wave takes chunks of `chunk`
+// queries from *counter (cleared by the caller) until all n are handed out,
+// refilling its lanes across chunk boundaries (pool_walk).  With `order`, pool
+// position i is query order[i] (the Morton order).
 template <int D, int KIND, bool ANY, bool FILT, bool CAST>
 __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nearest_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                       const float* __restrict__ tmax, Filter f, NearestOut out,
-                                                      uint32_t per_lane, uint32_t S, SpillEntry* spill, uint32_t spill_stride,
-                                                      const uint32_t* __restrict__ order) {
+                                                      uint32_t chunk, uint32_t* counter, uint32_t S, SpillEntry* spill,
+                                                      uint32_t spill_stride, const uint32_t* __restrict__ order,
+                                                      uint32_t pause_below) {
   extern __shared__ uint32_t lds[];
   const uint32_t B = blockDim.x, tid = threadIdx.x;
   const uint32_t gtid = blockIdx.x * B + tid;
-  const uint64_t pool = 64ull * per_lane;
-  const uint64_t next = ((uint64_t)blockIdx.x * (B >> 6) + (tid >> 6)) * pool;   // wave-uniform
-  const uint64_t end = next + pool < (uint64_t)n ? next + pool : (uint64_t)n;
   RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
-  pool_walk<D, KIND, ANY, FILT, CAST>(s, q, next, end, max_toi, tmax, f, out, st, order);
+  const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
+  const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
+  pool_walk<D, KIND, ANY, FILT, CAST>(s, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, f, out, st, order, pause_below);
+}
+
+__global__ void k_zero_counter(uint32_t* p) {
+  if (threadIdx.x == 0) *p = 0u;
 }
 
 template <int D>
@@ -514,10 +520,12 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   const int S = t->lds_stack ? t->lds_stack : 8;
   if (S < 2 || S > 64 || (S & (S - 1))) return fail(SQ_ERR_INVALID, "lds_stack must be a power of two in [2, 64]");
   if (t->ray_order != 0 && t->ray_order != 1) return fail(SQ_ERR_INVALID, "ray_order must be 0 or 1");
+  if (t->descend_min < 0 || t->descend_min > 64) return fail(SQ_ERR_INVALID, "descend_min must be in [0, 64]");
   c->block = block;
   c->rays_per_lane = t->rays_per_lane;
   c->lds_stack = S;
   c->ray_order = t->ray_order;
+  c->descend_min = t->descend_min;
   return SQ_OK;
 }
 
@@ -527,6 +535,7 @@ int sq_ctx_get_tuning(const sq_ctx* c, sq_tuning* t) {
   t->rays_per_lane = c->rays_per_lane;
   t->lds_stack = c->lds_stack;
   t->ray_order = c->ray_order;
+  t->descend_min = c->descend_min;
   return SQ_OK;
 }
 
@@ -812,36 +821,43 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
     if (rc) return rc;
     return end_timing(c);
   }
-  // pooled kernel; small calls halve the pool until the grid has pooled_waves()
-  // waves, so that a call of a few thousand queries still spreads over the chip
+  // pooled kernel, persistent: chunks of 64 * per queries (small calls halve
+  // per until every resident wave has a chunk), a grid of the resident waves
   const uint32_t S = (uint32_t)c->lds_stack;
   const uint32_t per = pooled_per_lane(c, n);
-  const uint64_t pool = 64ull * per;
-  const uint64_t waves = (n + pool - 1) / pool;
-  const uint64_t blocks = (waves + (B / 64) - 1) / (B / 64);
-  const uint64_t lanes = blocks * (uint64_t)B;
-  if (lanes > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "too many lanes");
+  const uint64_t chunk = 64ull * per;
+  const uint64_t chunks = (n + chunk - 1) / chunk;
   const uint32_t spill_rows = stack_n > S ? stack_n - S : 0;
-  int rc;
-  if ((rc = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64))) return rc;
   const size_t lds = (size_t)S * B * 8;
+  int rc;
+  if ((rc = c->counter.ensure(256))) return rc;
   rc = begin_timing(c);
   if (rc) return rc;
   const uint32_t* order = nullptr;
   if (c->ray_order && !cast && n >= 4096)
     if ((rc = ray_order(c, b->dim, q, nn, 2 * b->dim, b->root.as<float>(), &order))) return rc;
-  SpillEntry* spill = c->spill.as<SpillEntry>();
-  const uint32_t spl = (uint32_t)lanes;
+  uint32_t* counter = c->counter.as<uint32_t>();
+  k_zero_counter<<<1, 64, 0, c->stream>>>(counter);
+  HIP_TRY(hipGetLastError());
   rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
-    dim3 grid((uint32_t)blocks), block(B);
-#define SQ_LAUNCH(ANY_, FILT_, CAST_) \
-    k_nearest_pool<D, K, ANY_, FILT_, CAST_><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, out, per, S, spill, spl, order)
-    if (cast) { if (filt) SQ_LAUNCH(false, true, true); else SQ_LAUNCH(false, false, true); }
-    else if (any) { if (filt) SQ_LAUNCH(true, true, false); else SQ_LAUNCH(true, false, false); }
-    else { if (filt) SQ_LAUNCH(false, true, false); else SQ_LAUNCH(false, false, false); }
-#undef SQ_LAUNCH
-    HIP_TRY(hipGetLastError());
-    return SQ_OK;
+    auto launch = [&]<bool ANY_, bool FILT_, bool CAST_>() -> int {
+      const void* kern = reinterpret_cast<const void*>(&k_nearest_pool<D, K, ANY_, FILT_, CAST_>);
+      int per_cu = 0;
+      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, B, lds));
+      const uint64_t wpb = (uint64_t)B / 64;
+      const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
+      const uint64_t lanes = grid * (uint64_t)B;
+      int r2 = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64);
+      if (r2) return r2;
+      k_nearest_pool<D, K, ANY_, FILT_, CAST_><<<(uint32_t)grid, B, lds, c->stream>>>(
+          s, q, nn, max_toi, tmax, fd, out, (uint32_t)chunk, counter, S, c->spill.as<SpillEntry>(), (uint32_t)lanes, order,
+          (uint32_t)c->descend_min);
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    };
+    if (cast) return filt ? launch.template operator()<false, true, true>() : launch.template operator()<false, false, true>();
+    if (any) return filt ? launch.template operator()<true, true, false>() : launch.template operator()<true, false, false>();
+    return filt ? launch.template operator()<false, true, false>() : launch.template operator()<false, false, false>();
   });
   if (rc) return rc;
   return end_timing(c);

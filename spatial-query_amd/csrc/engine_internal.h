@@ -59,14 +59,16 @@ struct sq_ctx {
   hipStream_t stream = nullptr;    // where work is enqueued (own, or the caller's)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int block = 128;
-  int rays_per_lane = 8;           // pooled nearest kernel: rays per lane (0 = one ray per thread)
+  int rays_per_lane = 16;          // pooled kernels: chunk of 64 * rays_per_lane queries (0 = one ray per thread)
   int lds_stack = 8;               // pooled kernel: ring-stack entries kept in LDS per lane
   int ray_order = 0;               // 1: walk rays in Morton order (order.hip)
+  int descend_min = 24;            // pooled walks: pause descents below this many lanes (pool.h)
   bool timed = false;
   int refs = 1;                    // the caller's handle + one per live tree
   DevBuf counts, scratch, keys, keys2;   // reusable temporaries of the CSR queries
   DevBuf spill;                    // ring-stack spill columns of the pooled kernel
   DevBuf okeys, okeys2, oidx, oorder, otemp;   // Morton ray order
+  DevBuf counter;                  // chunk counter of the persistent pooled kernel
 };
 
 // Drops one reference; the last one frees the stream, events and buffers.
@@ -143,10 +145,11 @@ struct NearestOut {
 inline uint32_t grid_for(uint64_t n, int block) { return (uint32_t)((n + block - 1) / block); }
 
 // Rays per lane of a pooled launch over n queries: the tuned pool size, halved
-// while the grid would hold fewer than 8 waves per compute unit.
+// while there would be fewer chunks than waves the chip holds resident (32 per
+// compute unit: 8 per SIMD), so every resident wave gets a chunk of its own.
 inline uint32_t pooled_per_lane(const sq_ctx* c, uint64_t n) {
   uint32_t per = (uint32_t)(c->rays_per_lane > 0 ? c->rays_per_lane : 1);
-  const uint64_t want = 8ull * (uint64_t)c->cus;
+  const uint64_t want = 32ull * (uint64_t)c->cus;
   while (per > 1 && (n + 64ull * per - 1) / (64ull * per) < want) per >>= 1;
   return per;
 }

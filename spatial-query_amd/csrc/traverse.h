@@ -641,14 +641,25 @@ struct WorkCount {
   uint32_t nodes = 0, prims = 0;
 };
 
-// One step: descend through interior nodes to the next leaf (near-first), then
-// test that leaf's primitives.  Returns true when the query is finished.
-// COUNT adds the step's work to *wc.  SKIP treats the child refs skip[0..3]
-// as missed: subtrees this query has already walked completely (with a limit
-// at least the current one, so they hold nothing it has not seen).
-template <int D, int PS, bool ANY, bool FILT, bool COUNT = false, bool SKIP = false, class Q>
+// Never pause (one lane, or no wave-level policy).
+struct NoPause {
+  SQ_HD bool operator()() const { return false; }
+};
+
+// One step: descend near-first through pair nodes to the next leaf (near child
+// next, far child pushed with its entry distance; when both children miss,
+// pop until an entry is no farther than the current best), then test that
+// leaf's primitives and pop to the next entry worth visiting.  Returns true
+// when the query is finished (nothing left to pop).  pause() is asked after
+// every node: when it says so the step returns early with the walk's state
+// intact (the wave stops waiting on a few long descents; they resume in the
+// next step), so per query the order of work is the same whatever it says.
+// COUNT adds the step's work to *wc.  SKIP treats the child refs skip[0..3] as
+// missed: subtrees this query has already walked completely (with a limit at
+// least the current one, so they hold nothing it has not seen).
+template <int D, int PS, bool ANY, bool FILT, bool COUNT = false, bool SKIP = false, class Q, class P = NoPause>
 SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const RingStack& st, WalkState& w, WorkCount* wc = nullptr,
-                     const uint32_t* skip = nullptr) {
+                     const uint32_t* skip = nullptr, const P& pause = P()) {
   constexpr int NF = (D + 1) * 4;
   while (!(w.cur & LEAF_BIT)) {
     float nd[NF];
@@ -678,6 +689,7 @@ SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const Ring
       }
       if (!got) return true;
     }
+    if (pause()) return false;
   }
   const uint32_t first = w.cur & FIRST_MASK;
   const uint32_t cnt = ((w.cur >> 27) & 15u) + 1u;

@@ -62,7 +62,8 @@ def test_bvh_batch_matches_single_calls(ctx, meshes):
     qs = _queries(meshes, 0xBA7C)
     bs = sq_amd.BatchSet(ctx, qs)
     info = bs.info()
-    assert info["chunks"] == sum((r.shape[0] + 511) // 512 for _, r, _ in qs)
+    chunk = 64 * ctx.tuning()["rays_per_lane"]
+    assert info["chunks"] == sum((r.shape[0] + chunk - 1) // chunk for _, r, _ in qs)
     for rep in range(2):
         bs.submit()
         ctx.synchronize()
