@@ -145,6 +145,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel_ms": round(k_ms, 4), "algorithmic_bytes_per_launch": int(bytes_launch),
+                     "hbm_traffic_gbs": round(traffic / (k_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "note": "achieved counts every node and triangle the minimal walk tests as a read; the 135 MB "
+                             "scene stays in L2 and the Infinity Cache, so it can exceed the HBM peak; "
+                             "hbm_traffic_gbs is the measured fabric traffic over the same time",
                      "mean_nodes_per_ray": round(n_node / n, 2), "mean_prims_per_ray": round(n_prim / n, 2)},
     }
 

@@ -16,6 +16,7 @@
 #include <cstring>
 
 #include "engine_internal.h"
+#include "pool.h"
 
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -261,11 +262,18 @@ __device__ __forceinline__ void dense_relu(bf16 (*act)[ALD], const bf16* __restr
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  // the W fragments of the next k slice load while this slice's MFMAs run
+  bf16x8 b_next[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b_next[j] = *reinterpret_cast<const bf16x8*>(W + (size_t)(c0 + 16 * j + li) * KIN + kb);
 #pragma unroll 1
   for (int k = 0; k < KIN; k += 32) {
     bf16x8 a[4], b[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(W + (size_t)(c0 + 16 * j + li) * KIN + k + kb);
+    for (int j = 0; j < 4; ++j) b[j] = b_next[j];
+    if (k + 32 < KIN)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b_next[j] = *reinterpret_cast<const bf16x8*>(W + (size_t)(c0 + 16 * j + li) * KIN + k + 32 + kb);
 #pragma unroll
     for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&act[r0 + 16 * i + li][k + kb]);
 #pragma unroll
@@ -328,37 +336,60 @@ __global__ __launch_bounds__(IT, 4) void k_infer_top4(const float* __restrict__ 
     for (int k = 0; k < 4; ++k) { tv[j][k] = -INFINITY; ti[j][k] = 0x7fffffff; }
   const bf16* W4 = mw.w[3];
   const float* b4 = mw.b[3];
-  for (int m0 = 0; m0 < n_pad; m0 += 128) {
-    f32x4 acc[2][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int cb = m0 + cr + 16 * i + 4 * (lane >> 4);
-      const f32x4 bi = f32x4{b4[cb], b4[cb + 1], b4[cb + 2], b4[cb + 3]};
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = bi;
-    }
+  // One step = one 32-wide k slice of one 128-class tile; the W4 fragments of
+  // the next step are loaded before this step's MFMAs are issued, so their L2
+  // latency hides behind the matrix work.
+  const int ksteps = HID / 32;
+  const int nsteps = (n_pad / 128) * ksteps;
+  auto w4 = [&](int step, int i) {
+    const int m0 = (step / ksteps) * 128, k = (step % ksteps) * 32;
+    return *reinterpret_cast<const bf16x8*>(W4 + (size_t)(m0 + cr + 16 * i + li) * HID + k + kb);
+  };
+  bf16x8 a_next[2] = {w4(0, 0), w4(0, 1)};
+  f32x4 acc[2][4];
 #pragma unroll 1
-    for (int k = 0; k < HID; k += 32) {
-      bf16x8 a[2], b[4];
+  for (int step = 0; step < nsteps; ++step) {
+    const int kk = step % ksteps;
+    const int m0 = (step / ksteps) * 128;
+    if (kk == 0) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8*>(W4 + (size_t)(m0 + cr + 16 * i + li) * HID + k + kb);
+      for (int i = 0; i < 2; ++i) {
+        const int cb = m0 + cr + 16 * i + 4 * (lane >> 4);
+        const f32x4 bi = f32x4{b4[cb], b4[cb + 1], b4[cb + 2], b4[cb + 3]};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&act[rr + 16 * j + li][k + kb]);
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) acc[i][j] = bi;
+      }
     }
+    const bf16x8 a[2] = {a_next[0], a_next[1]};
+    if (step + 1 < nsteps) { a_next[0] = w4(step + 1, 0); a_next[1] = w4(step + 1, 1); }
+    bf16x8 b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&act[rr + 16 * j + li][kk * 32 + kb]);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int cls = m0 + cr + 16 * i + 4 * (lane >> 4) + r;
-        if (cls >= n_cls) continue;
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (kk == ksteps - 1) {
+      // Per ray, the best of its 8 candidates against the current fourth:
+      // below it nothing can enter (a tie needs >=), so the exact ordered
+      // insertion runs only for rays with a possible entry.  Classes past
+      // n_cls exist only in the last tile.
+      const bool tail = m0 + 128 > n_cls;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (beats(acc[i][j][r], cls, tv[j][3], ti[j][3])) top4_insert(tv[j], ti[j], acc[i][j][r], cls);
+      for (int j = 0; j < 4; ++j) {
+        float best = fmaxf(fmaxf(fmaxf(acc[0][j][0], acc[0][j][1]), fmaxf(acc[0][j][2], acc[0][j][3])),
+                           fmaxf(fmaxf(acc[1][j][0], acc[1][j][1]), fmaxf(acc[1][j][2], acc[1][j][3])));
+        if (tail || best >= tv[j][3]) {
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int cls = m0 + cr + 16 * i + 4 * (lane >> 4) + r;
+              if (cls < n_cls && beats(acc[i][j][r], cls, tv[j][3], ti[j][3])) top4_insert(tv[j], ti[j], acc[i][j][r], cls);
+            }
+        }
       }
+    }
   }
   __syncthreads();   // act is free: merge buffer
   constexpr int NL = 16 * 4;                                  // candidates per ray
@@ -399,15 +430,15 @@ __global__ void k_labels(const int32_t* __restrict__ leaf, uint64_t n, const int
 // Proposal-seeded exact nearest walk (triangles, D = 3): the proposed subtrees
 // are walked first, each from its own ref, with the shrinking limit; then the
 // full walk runs from the root with that limit and best hit.  top == NULL
-// walks from the root at once (the plain walk).  Same pooled structure as the
-// plain nearest kernel.  COUNT accumulates into stats[0..2]: rays whose
+// walks from the root at once (the plain walk).  Same pooled structure and
+// pausing of descents as the plain nearest kernel.  COUNT accumulates into stats[0..2]: rays whose
 // proposals held a hit, pair nodes fetched, primitives tested.
 template <bool COUNT>
 __global__ __launch_bounds__(256, COUNT ? 4 : 8) void k_nearest_proposed(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                              const uint16_t* __restrict__ top, const uint32_t* __restrict__ sub_ref,
                                                              uint32_t n_sub, int32_t* out_leaf, float* out_toi, uint32_t per_lane,
                                                              uint32_t S, SpillEntry* spill, uint32_t spill_stride,
-                                                             unsigned long long* stats) {
+                                                             unsigned long long* stats, uint32_t pause_below) {
   extern __shared__ uint32_t lds[];
   constexpr int D = 3, PS = 12;
   const uint32_t B = blockDim.x, tid = threadIdx.x;
@@ -474,8 +505,9 @@ __global__ __launch_bounds__(256, COUNT ? 4 : 8) void k_nearest_proposed(SceneVi
       next += __popcll(m);
     }
     if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
+    const uint32_t active = (uint32_t)__popcll(__ballot(rid != NO_LEAF));
     if (rid == NO_LEAF) continue;
-    if (!walk_step<D, PS, false, false, COUNT, true>(s, pol, nof, st, w, &wc, done)) continue;
+    if (!walk_step<D, PS, false, false, COUNT, true>(s, pol, nof, st, w, &wc, done, WavePause{pause_below, active})) continue;
     if (phase < 4) { start(phase + 1); continue; }   // this subtree is exhausted
     out_leaf[rid] = w.res.leaf != NO_LEAF ? (int32_t)w.res.leaf : -1;
     out_toi[rid] = w.res.t;
@@ -903,10 +935,12 @@ int sq_raycast_nearest_proposed(sq_proposal* p, const float* rays, uint64_t n, f
   auto* st = reinterpret_cast<unsigned long long*>(stats);
   if (stats)
     k_nearest_proposed<true><<<(uint32_t)blocks, B, (size_t)S * B * 8, c->stream>>>(view_of(b), rays, (uint32_t)n, max_toi, top,
-        p->sub_ref.as<uint32_t>(), p->n_sub, out_leaf, out_toi, per, S, c->spill.as<SpillEntry>(), (uint32_t)lanes, st);
+        p->sub_ref.as<uint32_t>(), p->n_sub, out_leaf, out_toi, per, S, c->spill.as<SpillEntry>(), (uint32_t)lanes, st,
+        (uint32_t)c->descend_min);
   else
     k_nearest_proposed<false><<<(uint32_t)blocks, B, (size_t)S * B * 8, c->stream>>>(view_of(b), rays, (uint32_t)n, max_toi, top,
-        p->sub_ref.as<uint32_t>(), p->n_sub, out_leaf, out_toi, per, S, c->spill.as<SpillEntry>(), (uint32_t)lanes, st);
+        p->sub_ref.as<uint32_t>(), p->n_sub, out_leaf, out_toi, per, S, c->spill.as<SpillEntry>(), (uint32_t)lanes, st,
+        (uint32_t)c->descend_min);
   HIP_TRY(hipGetLastError());
   return end_timing(c);
 }
