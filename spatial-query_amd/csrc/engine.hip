@@ -185,6 +185,45 @@ __global__ __launch_bounds__(1024) void k_collect(SceneView s, const float* __re
   if (!FILL) counts[i] = cnt;
 }
 
+// Pooled, persistent collect (pool_collect): waves take chunks of `chunk`
+// queries from counter[0]; entries go to the staging blocks of `o`
+// (counter[1] = blocks claimed, counter[2] = overflow flag).  LDS: the S-entry
+// ring stacks, then two words per wave for its open staging block.
+template <int D, int KIND, int MODE, bool FILT>
+__global__ __launch_bounds__(256) void k_collect_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
+                                                      const float* __restrict__ tmax, Filter f, uint32_t chunk,
+                                                      uint32_t* counter, uint32_t S, SpillEntry* spill,
+                                                      uint32_t spill_stride, uint32_t pause_below, StageOut o) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  const uint32_t gtid = blockIdx.x * B + tid;
+  RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
+  volatile uint32_t* slot = lds + 2 * S * B + 2 * (tid >> 6);
+  if ((tid & 63u) == 0) {
+    slot[0] = NO_BLOCK;
+    slot[1] = 0;
+  }
+  const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
+  const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
+  pool_collect<D, KIND, MODE, FILT>(s, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, f, st,
+                                    StageAppend{o, slot}, pause_below);
+}
+
+__global__ void k_zero_words(uint32_t* p, uint32_t n) {
+  if (threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// Staged entries to their CSR slots: keys[offsets[q] + idx] = key, for the
+// first used[b] entries of each block b < n_blocks.
+__global__ __launch_bounds__(256) void k_stage_scatter(const StageEntry* __restrict__ e, const uint32_t* __restrict__ used,
+                                                       uint64_t n_blocks, const uint64_t* __restrict__ offsets, uint64_t* keys) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_blocks * STAGE_BLK) return;
+  if ((uint32_t)(i % STAGE_BLK) >= used[i / STAGE_BLK]) return;
+  const StageEntry x = e[i];
+  keys[offsets[x.q] + x.idx] = x.key;
+}
+
 // Exclusive scan of n values (u32 or u64) into out[0..n] (out[n] = total).
 // Tiles of 4096 = 256 threads x 16; tile sums scanned recursively by the host.
 template <class T>
@@ -429,24 +468,14 @@ int fill_miss(sq_ctx* c, uint64_t n, int dim, NearestOut out) {
   return SQ_OK;
 }
 
-// count -> scan -> capacity check -> fill -> per-query sort -> unpack
-int csr_run(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* out_toi, bool has_t,
-            uint64_t capacity, uint64_t* total, const std::function<int(bool, uint64_t*)>& launch) {
+// Per-query sort of keys[offsets[i] .. offsets[i+1]) (lists over 64 entries
+// in one block each), then unpack into out_leaf / out_toi.
+static int csr_sort_unpack(sq_ctx* c, uint64_t n, const uint64_t* offsets, uint64_t tot, int32_t* out_leaf, float* out_toi,
+                           bool has_t) {
   int rc;
-  if ((rc = launch(false, nullptr))) return rc;
-  std::vector<std::unique_ptr<DevBuf>> temps;
-  if ((rc = scan_into<uint32_t>(c, c->counts.as<uint32_t>(), n, offsets, temps))) return rc;
-  HIP_TRY(hipMemcpyAsync(total, offsets + n, 8, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  const uint64_t tot = *total;
-  if (tot > capacity) return fail(SQ_ERR_CAPACITY, "result needs " + std::to_string(tot) + " entries, capacity is " + std::to_string(capacity));
-  if (!tot) return SQ_OK;
-  if (!out_leaf || (has_t && !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
-  if ((rc = c->keys.ensure(tot * 8))) return rc;
   if ((rc = c->keys2.ensure(tot * 8))) return rc;
   if ((rc = c->scratch.ensure((n + 1) * 4 + 64))) return rc;
   uint64_t* keys = c->keys.as<uint64_t>();
-  if ((rc = launch(true, keys))) return rc;
   uint32_t* n_large = c->scratch.as<uint32_t>();
   uint32_t* large = n_large + 16;
   HIP_TRY(hipMemsetAsync(n_large, 0, 4, c->stream));
@@ -462,6 +491,65 @@ int csr_run(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* 
   k_unpack<<<grid_for(tot, 256), 256, 0, c->stream>>>(keys, tot, out_leaf, out_toi, has_t ? 1 : 0);
   HIP_TRY(hipGetLastError());
   return SQ_OK;
+}
+
+// count -> scan -> capacity check -> fill -> per-query sort -> unpack
+int csr_run(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* out_toi, bool has_t,
+            uint64_t capacity, uint64_t* total, const std::function<int(bool, uint64_t*)>& launch) {
+  int rc;
+  if ((rc = launch(false, nullptr))) return rc;
+  std::vector<std::unique_ptr<DevBuf>> temps;
+  if ((rc = scan_into<uint32_t>(c, c->counts.as<uint32_t>(), n, offsets, temps))) return rc;
+  HIP_TRY(hipMemcpyAsync(total, offsets + n, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const uint64_t tot = *total;
+  if (tot > capacity) return fail(SQ_ERR_CAPACITY, "result needs " + std::to_string(tot) + " entries, capacity is " + std::to_string(capacity));
+  if (!tot) return SQ_OK;
+  if (!out_leaf || (has_t && !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
+  if ((rc = c->keys.ensure(tot * 8))) return rc;
+  if ((rc = launch(true, c->keys.as<uint64_t>()))) return rc;
+  return csr_sort_unpack(c, n, offsets, tot, out_leaf, out_toi, has_t);
+}
+
+// Entries the first staged walk of a call makes room for (beyond what the
+// staging area already holds): 16M entries, 256 MiB of blocks.
+static constexpr uint64_t STAGE_FIRST = 1ull << 24;
+
+// Pooled collect: launch(want) walks every query once, appending its entries
+// to staging blocks sized for at least `want` entries, and writes per-query
+// counts; then scan -> capacity check -> scatter -> per-query sort -> unpack.
+// Only when the staging area overflowed while the total fits the capacity
+// does the walk run again, with room for the exact total (the area keeps that
+// size for later calls).
+int csr_run_staged(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* out_toi, bool has_t,
+                   uint64_t capacity, uint64_t* total, const std::function<int(uint64_t)>& launch) {
+  int rc;
+  if ((rc = launch(std::min<uint64_t>(capacity, STAGE_FIRST)))) return rc;
+  std::vector<std::unique_ptr<DevBuf>> temps;
+  if ((rc = scan_into<uint32_t>(c, c->counts.as<uint32_t>(), n, offsets, temps))) return rc;
+  uint32_t words[4];
+  HIP_TRY(hipMemcpyAsync(total, offsets + n, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(words, c->counter.as<uint32_t>(), 16, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const uint64_t tot = *total;
+  if (tot > capacity) return fail(SQ_ERR_CAPACITY, "result needs " + std::to_string(tot) + " entries, capacity is " + std::to_string(capacity));
+  if (!tot) return SQ_OK;
+  if (!out_leaf || (has_t && !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
+  if (words[2]) {
+    // the same walk again with room for every entry; counts and offsets come out the same
+    if ((rc = launch(tot))) return rc;
+    HIP_TRY(hipMemcpyAsync(words, c->counter.as<uint32_t>(), 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (words[2]) return fail(SQ_ERR_HIP, "collect staging overflowed twice");
+  }
+  if ((rc = c->keys.ensure(tot * 8))) return rc;
+  const uint64_t n_blocks = words[1];
+  if (n_blocks > c->stage.bytes / (STAGE_BLK * sizeof(StageEntry)) || n_blocks > c->stage_used.bytes / 4)
+    return fail(SQ_ERR_HIP, "collect staging block count out of range");
+  k_stage_scatter<<<grid_for(n_blocks * STAGE_BLK, 256), 256, 0, c->stream>>>(
+      c->stage.as<StageEntry>(), c->stage_used.as<uint32_t>(), n_blocks, offsets, c->keys.as<uint64_t>());
+  HIP_TRY(hipGetLastError());
+  return csr_sort_unpack(c, n, offsets, tot, out_leaf, out_toi, has_t);
 }
 
 extern "C" {
@@ -911,6 +999,55 @@ static int collect_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
   const size_t lds = (size_t)stack_n * B * 4;
   SceneView s = view_of(b);
   const uint32_t nn = (uint32_t)n;
+  if (c->rays_per_lane > 0) {
+    // pooled: one persistent walk appends to staging blocks (csr_run_staged)
+    const uint32_t S = (uint32_t)c->lds_stack;
+    const uint32_t per = pooled_per_lane(c, n);
+    const uint64_t chunk = 64ull * per;
+    const uint64_t chunks = (n + chunk - 1) / chunk;
+    const uint32_t spill_rows = stack_n > S ? stack_n - S : 0;
+    if ((rc = c->counter.ensure(256))) return rc;
+    auto staged = [&](uint64_t want) -> int {
+      return dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+        auto go = [&]<int MODE_, bool FILT_>() -> int {
+          const void* kern = reinterpret_cast<const void*>(&k_collect_pool<D, K, MODE_, FILT_>);
+          const size_t lds_p = (size_t)S * B * 8 + (size_t)(B / 64) * 8;
+          int per_cu = 0;
+          HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, B, lds_p));
+          const uint64_t wpb = (uint64_t)B / 64;
+          const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
+          const uint64_t lanes = grid * (uint64_t)B;
+          int r2 = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64);
+          if (r2) return r2;
+          // retired blocks hold over STAGE_BLK - 64 entries; each wave has one more open
+          const uint64_t blocks = std::min<uint64_t>(want / (STAGE_BLK - 64) + 1 + grid * wpb, 0xFFFFFFF0ull);
+          if ((r2 = c->stage.ensure(blocks * STAGE_BLK * sizeof(StageEntry)))) return r2;
+          if ((r2 = c->stage_used.ensure(blocks * 4))) return r2;
+          const uint64_t max_blocks = std::min<uint64_t>({c->stage.bytes / (STAGE_BLK * sizeof(StageEntry)),
+                                                          c->stage_used.bytes / 4, 0xFFFFFFF0ull});
+          uint32_t* counter = c->counter.as<uint32_t>();
+          k_zero_words<<<1, 64, 0, c->stream>>>(counter, 4);
+          HIP_TRY(hipGetLastError());
+          StageOut o{c->stage.as<StageEntry>(), c->stage_used.as<uint32_t>(), counter + 1, counter + 2,
+                     (uint32_t)max_blocks, counts};
+          k_collect_pool<D, K, MODE_, FILT_><<<(uint32_t)grid, B, lds_p, c->stream>>>(
+              s, q, nn, max_toi, tmax, fd, (uint32_t)chunk, counter, S, c->spill.as<SpillEntry>(), (uint32_t)lanes,
+              (uint32_t)c->descend_min, o);
+          HIP_TRY(hipGetLastError());
+          return SQ_OK;
+        };
+#define SQ_STAGED(MODE_) return filt ? go.template operator()<MODE_, true>() : go.template operator()<MODE_, false>();
+        switch (mode) {
+          case 0: SQ_STAGED(0)
+          case 1: SQ_STAGED(1)
+          case 2: SQ_STAGED(2)
+          default: SQ_STAGED(3)
+        }
+#undef SQ_STAGED
+      });
+    };
+    return csr_run_staged(c, n, offsets, out_leaf, out_toi, mode != 2, capacity, total, staged);
+  }
   auto launch = [&](bool fill, uint64_t* keys) -> int {
     return dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
       dim3 grid(grid_for(n, B)), block(B);

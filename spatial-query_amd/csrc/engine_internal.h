@@ -68,7 +68,8 @@ struct sq_ctx {
   DevBuf counts, scratch, keys, keys2;   // reusable temporaries of the CSR queries
   DevBuf spill;                    // ring-stack spill columns of the pooled kernel
   DevBuf okeys, okeys2, oidx, oorder, otemp;   // Morton ray order
-  DevBuf counter;                  // chunk counter of the persistent pooled kernel
+  DevBuf counter;                  // chunk counter of the persistent pooled kernels (+ staging words)
+  DevBuf stage, stage_used;        // staging blocks of the pooled collect; grow to the largest total seen
 };
 
 // Drops one reference; the last one frees the stream, events and buffers.

@@ -142,3 +142,196 @@ __device__ __forceinline__ void pool_walk(const SceneView& s, const float* __res
     rid = NO_LEAF;
   }
 }
+
+// ---------------------------------------------------------------------------
+// Pooled collect (all hits, crossings, overlap, cast-all): one walk per query
+// appends the query's entries to a staging area; the host side then scans the
+// per-query counts, scatters the entries to their CSR slots and sorts each
+// query's list, so the order in which entries were appended does not matter.
+// ---------------------------------------------------------------------------
+
+// One appended entry: query, position within the query's list, sort key.
+struct __align__(16) StageEntry {
+  uint32_t q, idx;
+  uint64_t key;
+};
+
+// Waves claim STAGE_BLK-entry blocks from *blocks and fill them; used[b] is
+// the number of entries block b holds.  A block is retired when the next
+// append does not fit, so every retired block holds more than STAGE_BLK - 64
+// entries.  Blocks at or beyond max_blocks are not written and set *overflow.
+constexpr uint32_t STAGE_BLK = 512;
+constexpr uint32_t NO_BLOCK = 0xFFFFFFFFu;
+
+struct StageOut {
+  StageEntry* e;
+  uint32_t* used;
+  uint32_t* blocks;
+  uint32_t* overflow;
+  uint32_t max_blocks;
+  uint32_t* counts;        // entries per query
+};
+
+// Appends one entry for each calling lane (any subset of the wave).  The
+// wave's open block and its fill live in two LDS words that only the lowest
+// calling lane reads and writes (volatile: another lane may be that lane
+// next time).
+struct StageAppend {
+  StageOut o;
+  volatile uint32_t* slot;     // [0] open block or NO_BLOCK, [1] entries used
+  __device__ __forceinline__ void operator()(uint32_t qi, uint32_t idx, uint64_t key) const {
+    const uint64_t m = __ballot(1);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+    const uint32_t k = (uint32_t)__popcll(m);
+    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    uint32_t blk = 0, used = 0;
+    if (lane == leader) {
+      blk = slot[0];
+      used = slot[1];
+      if (blk == NO_BLOCK || used + k > STAGE_BLK) {
+        if (blk != NO_BLOCK && blk < o.max_blocks) o.used[blk] = used;
+        blk = atomicAdd(o.blocks, 1u);
+        used = 0;
+        slot[0] = blk;
+      }
+      slot[1] = used + k;
+    }
+    blk = (uint32_t)__builtin_amdgcn_readlane((int)blk, (int)leader);
+    used = (uint32_t)__builtin_amdgcn_readlane((int)used, (int)leader);
+    if (blk < o.max_blocks) o.e[(size_t)blk * STAGE_BLK + used + rank] = StageEntry{qi, idx, key};
+    else if (lane == leader) *o.overflow = 1u;
+  }
+  // Retires the wave's open block (one lane, at the end).
+  __device__ __forceinline__ void close() const {
+    const uint32_t blk = slot[0];
+    if (blk != NO_BLOCK && blk < o.max_blocks) o.used[blk] = slot[1];
+  }
+};
+
+// Per-query state of a collect walk.  MODE 0: all ray hits, 1: box
+// crossings (entry, and exit within max_toi), 2: overlap (touching counts),
+// 3: all shape-cast hits.  Keys are hit_key(t, leaf), or the leaf for overlap.
+template <int D, int KIND, int MODE, bool FILT>
+struct CollectQuery {
+  using Pol = typename std::conditional<MODE == 3, CastPolicy<D, KIND>, RayPolicy<D, KIND>>::type;
+  static constexpr int QS = MODE == 2 ? 2 * D : (MODE == 3 ? 4 * D : 2 * D);
+  static constexpr int PS = Prim<D, KIND>::STRIDE;
+  float mn[MODE == 2 ? D : 1], mx[MODE == 2 ? D : 1];
+  Pol pol;
+  float lim;
+  __device__ __forceinline__ bool begin(const SceneView& s, const float* q, uint32_t i, float max_toi, const float* tmax) {
+    const float* p = q + (size_t)i * QS;
+    if constexpr (MODE == 2) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) { mn[k] = p[k]; mx[k] = p[D + k]; }
+      return box_overlap<D>(s.root, s.root + D, mn, mx);
+    } else {
+      pol.setup(p);
+      lim = tmax ? tmax[i] : max_toi;
+      return pol.root(s.root, lim);
+    }
+  }
+  __device__ __forceinline__ void pair(const float* nd, bool& ha, bool& hb) const {
+    if constexpr (MODE == 2) {
+      float amn[D], amx[D], bmn[D], bmx[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) { amn[k] = nd[4 * k]; amx[k] = nd[4 * k + 1]; bmn[k] = nd[4 * k + 2]; bmx[k] = nd[4 * k + 3]; }
+      ha = box_overlap<D>(amn, amx, mn, mx);
+      hb = box_overlap<D>(bmn, bmx, mn, mx);
+    } else {
+      float ta, tb;
+      pol.pair(nd, lim, ha, hb, ta, tb);
+    }
+  }
+  template <class Emit>
+  __device__ __forceinline__ void leaf(const SceneView& s, const Filter& f, uint32_t first, uint32_t c, Emit&& emit) const {
+    for (uint32_t j = 0; j < c; ++j) {
+      const uint32_t pos = first + j;
+      const float* p = s.prims + (size_t)pos * PS;
+      if constexpr (MODE == 2) {
+        if (!overlap_narrow<D, KIND>(p, mn, mx)) continue;
+        const uint32_t leaf = s.ids[pos];
+        if (FILT && !accepts(f, leaf)) continue;
+        emit((uint64_t)leaf);
+      } else if constexpr (MODE == 1 && KIND == KIND_AABB) {
+        float te, tx;
+        int ax;
+        if (!slab_exact<D>(pol.r, p, p + D, lim, te, tx, ax)) continue;
+        const uint32_t leaf = s.ids[pos];
+        if (FILT && !accepts(f, leaf)) continue;
+        emit(hit_key(te, leaf));
+        if (tx > te) {
+          float te2, tx2;
+          slab_exact<D>(pol.r, p, p + D, SQ_INF, te2, tx2, ax);
+          if (tx2 <= lim) emit(hit_key(tx2, leaf));
+        }
+      } else {
+        float t;
+        if (!pol.narrow(p, lim, t)) continue;
+        const uint32_t leaf = s.ids[pos];
+        if (FILT && !accepts(f, leaf)) continue;
+        emit(hit_key(t, leaf));
+      }
+    }
+  }
+};
+
+// One wave walks the query ranges src hands it, lanes refilled as queries
+// finish (as pool_walk), each step visiting one leaf range (visit_step) or
+// pausing below pause_below descending lanes.  Finished queries write their
+// entry count to o.counts.
+template <int D, int KIND, int MODE, bool FILT, class Src>
+__device__ __forceinline__ void pool_collect(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
+                                             const float* __restrict__ tmax, const Filter& f, const RingStack& st,
+                                             const StageAppend& app, uint32_t pause_below) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  const uint32_t root_ref = fbits(s.root[2 * D]);
+  CollectQuery<D, KIND, MODE, FILT> cq;
+  VisitState v{root_ref, 0};
+  uint32_t rid = NO_LEAF, cnt = 0;
+  uint64_t next = 0, end = 0;
+  bool exhausted = !src.get(next, end);
+  while (true) {
+    bool need = rid == NO_LEAF && !exhausted;
+    uint64_t m = __ballot(need);
+    while (m) {
+      if (next >= end) {
+        if (!src.get(next, end)) { exhausted = true; break; }
+        continue;
+      }
+      const uint64_t avail = end - next;
+      const uint64_t rank = (uint64_t)__popcll(m & lt_mask);
+      if (need && rank < avail) {
+        need = false;
+        rid = (uint32_t)(next + rank);
+        cnt = 0;
+        v = VisitState{root_ref, 0};
+        if (!cq.begin(s, q, rid, max_toi, tmax)) {
+          app.o.counts[rid] = 0;
+          rid = NO_LEAF;
+        }
+      }
+      const uint64_t taken = (uint64_t)__popcll(m);
+      next += taken < avail ? taken : avail;
+      m = __ballot(need);
+    }
+    if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
+    const uint32_t active = (uint32_t)__popcll(__ballot(rid != NO_LEAF));
+    if (rid == NO_LEAF) continue;
+    const bool done = visit_step<D>(
+        s, [&](const float* nd, bool& ha, bool& hb) { cq.pair(nd, ha, hb); },
+        [&](uint32_t first, uint32_t c) {
+          cq.leaf(s, f, first, c, [&](uint64_t key) {
+            app(rid, cnt, key);
+            ++cnt;
+          });
+        },
+        st, v, WavePause{pause_below, active});
+    if (!done) continue;
+    app.o.counts[rid] = cnt;
+    rid = NO_LEAF;
+  }
+  if (lane == 0) app.close();
+}

@@ -626,6 +626,43 @@ struct WalkState {
   NearestResult res;
 };
 
+// Resumable form of walk_visit (the same set of leaf ranges): one call
+// descends to a leaf, or stops early when pause() says so, hands the leaf
+// range to leaf_fn and pops the next entry.  True when the query has no
+// entries left.  Start with cur = root ref, sp = 0.
+struct VisitState {
+  uint32_t cur, sp;
+};
+
+template <int D, class PairPred, class LeafFn, class P>
+SQ_HD bool visit_step(const SceneView& s, PairPred&& pair_pred, LeafFn&& leaf_fn, const RingStack& st, VisitState& v,
+                      const P& pause) {
+  constexpr int NF = (D + 1) * 4;
+  float t;
+  while (!(v.cur & LEAF_BIT)) {
+    const float* nd = s.nodes + (size_t)v.cur * NF;
+    bool ha, hb;
+    pair_pred(nd, ha, hb);
+    const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
+    if (ha && hb) {
+      st.push(v.sp, rb, 0.0f);
+      v.cur = ra;
+    } else if (ha) {
+      v.cur = ra;
+    } else if (hb) {
+      v.cur = rb;
+    } else {
+      if (v.sp == 0) return true;
+      st.pop(v.sp, v.cur, t);
+    }
+    if (pause()) return false;
+  }
+  leaf_fn(v.cur & FIRST_MASK, ((v.cur >> 27) & 15u) + 1u);
+  if (v.sp == 0) return true;
+  st.pop(v.sp, v.cur, t);
+  return false;
+}
+
 // Start a query: false when it misses the root box (the result is a miss).
 template <int D, class Q>
 SQ_HD bool walk_begin(const SceneView& s, const Q& q, float max_toi, WalkState& w) {

@@ -242,7 +242,9 @@ class DeviceBvh:
         import torch
         offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
         total = C.c_uint64()
-        cap = capacity
+        # capacity 0: start from the last call's total (or 4 entries per query),
+        # so that a call whose result fits walks the queries once
+        cap = capacity or max(getattr(self, "_csr_hint", 0), 4 * n)
         while True:
             leaf = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
             toi = torch.empty(max(cap, 1), dtype=torch.float32, device=dev) if has_t else None
@@ -253,6 +255,7 @@ class DeviceBvh:
             check(rc)
             break
         t = int(total.value)
+        self._csr_hint = t
         return offsets, leaf[:t], (toi[:t] if has_t else None)
 
     def raycast_all(self, rays, max_toi, filt=None, crossings=False, capacity=0):
