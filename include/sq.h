@@ -238,13 +238,22 @@ int sq_tlas_overlap(sq_tlas* tlas, const float* regions, uint64_t n, uint64_t* o
  *   descend_min    pooled walks: a wave stops waiting on descents once fewer
  *                  than this many lanes (0..64) still descend while others
  *                  have leaf tests or new queries to do; the paused descents
- *                  resume in the next step (0 = never pause; default 24). */
+ *                  resume in the next step (0 = never pause; default 24).
+ *   split_depth    pooled all-hits / crossings / overlap / cast-all calls:
+ *                  each query is first walked this many pair-node levels down
+ *                  and split into one sub-walk per node it reaches there (or
+ *                  leaf it reaches above), so that a few long queries spread
+ *                  over the whole chip; 0 = automatic (overlap calls of fewer
+ *                  than 2M queries split at half the tree depth, others do
+ *                  not), -1 = never split, 1..24 = fixed.  Results do not
+ *                  depend on it. */
 typedef struct sq_tuning {
   int block;
   int rays_per_lane;
   int lds_stack;
   int ray_order;
   int descend_min;
+  int split_depth;
 } sq_tuning;
 int sq_ctx_set_tuning(sq_ctx* ctx, const sq_tuning* tuning);
 int sq_ctx_get_tuning(const sq_ctx* ctx, sq_tuning* tuning);

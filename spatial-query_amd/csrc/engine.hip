@@ -186,14 +186,18 @@ __global__ __launch_bounds__(1024) void k_collect(SceneView s, const float* __re
 }
 
 // Pooled, persistent collect (pool_collect): waves take chunks of `chunk`
-// queries from counter[0]; entries go to the staging blocks of `o`
-// (counter[1] = blocks claimed, counter[2] = overflow flag).  LDS: the S-entry
-// ring stacks, then two words per wave for its open staging block.
-template <int D, int KIND, int MODE, bool FILT>
+// positions from counter[0]; entries go to the staging blocks of `o`
+// (counter[1] = blocks claimed, counter[2] = overflow flag).  Positions are
+// the n queries, or with ITEMS the sub-walks k_collect_expand wrote to items
+// (*n_items of them, at most items_cap).  LDS: the S-entry ring stacks, then
+// two words per wave for its open staging block.
+template <int D, int KIND, int MODE, bool FILT, bool ITEMS>
 __global__ __launch_bounds__(256) void k_collect_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                       const float* __restrict__ tmax, Filter f, uint32_t chunk,
                                                       uint32_t* counter, uint32_t S, SpillEntry* spill,
-                                                      uint32_t spill_stride, uint32_t pause_below, StageOut o) {
+                                                      uint32_t spill_stride, uint32_t pause_below, StageOut o,
+                                                      const WorkItem* __restrict__ items, uint32_t items_cap,
+                                                      const unsigned long long* n_items) {
   extern __shared__ uint32_t lds[];
   const uint32_t B = blockDim.x, tid = threadIdx.x;
   const uint32_t gtid = blockIdx.x * B + tid;
@@ -205,23 +209,121 @@ __global__ __launch_bounds__(256) void k_collect_pool(SceneView s, const float* 
   }
   const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
   const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
-  pool_collect<D, KIND, MODE, FILT>(s, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, f, st,
-                                    StageAppend{o, slot}, pause_below);
+  const uint64_t positions = ITEMS ? std::min<uint64_t>(*n_items, items_cap) : (uint64_t)n;
+  pool_collect<D, KIND, MODE, FILT, ITEMS>(s, q, Chunks{counter, positions, chunk, home, waves}, max_toi, tmax, f, st,
+                                           StageAppend{o, slot}, pause_below, items);
+}
+
+// Splits queries into sub-walks: a depth-first walk that stops `levels` pair
+// nodes down and appends (query, node) for every node it reaches there, and
+// (query, leaf ref) for every leaf range it reaches above that, to items.
+// Without FROM_ITEMS the walks start at the root of each of the n queries
+// (which also zeroes counts[i], the item walks add to it); with FROM_ITEMS
+// they start at each of the sub-walks in_items[0 .. min(*in_count, in_cap)),
+// a grid-stride loop over them.  *out_count counts every item (64 bits); those
+// at or beyond items_cap are not written and set *overflow.  Every node and
+// leaf range the query's full walk would visit lies below exactly one item,
+// so the items' walks together visit the same leaf ranges.  At most 2^levels
+// items per input.  LDS: `levels` stack entries (ref, depth) per thread.
+template <int D, int KIND, int MODE, bool FILT, bool FROM_ITEMS>
+__global__ __launch_bounds__(256) void k_collect_expand(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
+                                                        const float* __restrict__ tmax, uint32_t levels,
+                                                        const WorkItem* __restrict__ in_items, uint32_t in_cap,
+                                                        const unsigned long long* in_count, WorkItem* items, uint32_t items_cap,
+                                                        unsigned long long* out_count, uint32_t* overflow, uint32_t* counts) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  uint32_t* sref = lds + tid;
+  uint32_t* sdep = lds + levels * B + tid;
+  constexpr int NF = (D + 1) * 4;
+  const uint64_t n_in = FROM_ITEMS ? std::min<uint64_t>(*in_count, in_cap) : (uint64_t)n;
+  for (uint64_t first = (uint64_t)blockIdx.x * B; first < n_in; first += (uint64_t)gridDim.x * B) {
+    const uint64_t i = first + tid;
+    CollectQuery<D, KIND, MODE, FILT> cq;
+    bool live = i < n_in;
+    uint32_t qi = 0, cur = fbits(s.root[2 * D]);
+    if (live) {
+      if constexpr (FROM_ITEMS) {
+        const WorkItem it = in_items[i];
+        qi = it.q;
+        cur = it.ref;
+        (void)cq.begin(s, q, qi, max_toi, tmax);        // the first expand ran the root test
+      } else {
+        qi = (uint32_t)i;
+        counts[qi] = 0;
+        live = cq.begin(s, q, qi, max_toi, tmax);
+      }
+    }
+    uint32_t depth = 0, sp = 0;
+    while (__ballot(live)) {
+      const bool at = live && ((cur & LEAF_BIT) || depth == levels);
+      const uint64_t m = __ballot(at);
+      if (m) {
+        // every index below min(count, items_cap) is written: claims are contiguous from 0
+        const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+        const uint32_t k = (uint32_t)__popcll(m);
+        uint64_t base = 0;
+        if (lane == leader) {
+          base = atomicAdd(out_count, (unsigned long long)k);
+          if (base + k > items_cap) *overflow = 1u;
+        }
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)base, (int)leader);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(base >> 32), (int)leader);
+        const uint64_t at_i = ((uint64_t)hi << 32 | lo) + (uint32_t)__popcll(m & lt_mask);
+        if (at && at_i < items_cap) items[at_i] = WorkItem{qi, cur};
+      }
+      bool pop = at;
+      if (live && !at) {
+        const float* nd = s.nodes + (size_t)cur * NF;
+        bool ha, hb;
+        cq.pair(nd, ha, hb);
+        const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
+        if (ha && hb) {
+          sref[sp * B] = rb;
+          sdep[sp * B] = depth + 1;
+          ++sp;
+          cur = ra;
+          ++depth;
+        } else if (ha || hb) {
+          cur = ha ? ra : rb;
+          ++depth;
+        } else {
+          pop = true;
+        }
+      }
+      if (pop) {
+        if (sp == 0) {
+          live = false;
+        } else {
+          --sp;
+          cur = sref[sp * B];
+          depth = sdep[sp * B];
+        }
+      }
+    }
+  }
 }
 
 __global__ void k_zero_words(uint32_t* p, uint32_t n) {
   if (threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
-// Staged entries to their CSR slots: keys[offsets[q] + idx] = key, for the
-// first used[b] entries of each block b < n_blocks.
+// Staged entries to their CSR slots, for the first used[b] entries of each
+// block b < n_blocks: keys[offsets[q] + idx] = key, or for idx = NO_IDX a free
+// slot of query q taken by counting counts[q] down (each of the query's
+// counts[q] entries takes one of its slots; the per-query sort that follows
+// fixes the order).
 __global__ __launch_bounds__(256) void k_stage_scatter(const StageEntry* __restrict__ e, const uint32_t* __restrict__ used,
-                                                       uint64_t n_blocks, const uint64_t* __restrict__ offsets, uint64_t* keys) {
+                                                       uint64_t n_blocks, const uint64_t* __restrict__ offsets, uint32_t* counts,
+                                                       uint64_t* keys) {
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n_blocks * STAGE_BLK) return;
   if ((uint32_t)(i % STAGE_BLK) >= used[i / STAGE_BLK]) return;
   const StageEntry x = e[i];
-  keys[offsets[x.q] + x.idx] = x.key;
+  const uint32_t idx = x.idx != NO_IDX ? x.idx : atomicSub(counts + x.q, 1u) - 1u;
+  keys[offsets[x.q] + idx] = x.key;
 }
 
 // Exclusive scan of n values (u32 or u64) into out[0..n] (out[n] = total).
@@ -274,9 +376,10 @@ __global__ void k_sort_small(uint64_t* keys, const uint64_t* __restrict__ offset
   }
 }
 
-// Segment sort, large segments: one block per segment.  Chunks of 2048 keys
-// are bitonic-sorted in LDS, then runs are merged pairwise (each element's
-// output position = its rank in its own run + its rank in the other run),
+// Segment sort, large segments: one block per segment.  Chunks of up to 2048
+// keys are bitonic-sorted in LDS (a network of the chunk's size rounded up to
+// a power of two), then runs are merged pairwise (each element's output
+// position = its rank in its own run + its rank in the other run),
 // ping-ponging between the segment and a scratch area of the same size.
 constexpr uint32_t CHUNK = 2048;
 __global__ __launch_bounds__(256) void k_sort_large(uint64_t* keys, uint64_t* scratch, const uint64_t* __restrict__ offsets,
@@ -288,11 +391,13 @@ __global__ __launch_bounds__(256) void k_sort_large(uint64_t* keys, uint64_t* sc
   uint64_t* dst = scratch + a;
   for (uint64_t c0 = 0; c0 < len; c0 += CHUNK) {
     const uint32_t m = (uint32_t)min((uint64_t)CHUNK, len - c0);
-    for (uint32_t j = threadIdx.x; j < CHUNK; j += blockDim.x) sh[j] = j < m ? src[c0 + j] : ~0ull;
+    uint32_t P = 64;
+    while (P < m) P <<= 1;
+    for (uint32_t j = threadIdx.x; j < P; j += blockDim.x) sh[j] = j < m ? src[c0 + j] : ~0ull;
     __syncthreads();
-    for (uint32_t size = 2; size <= CHUNK; size <<= 1) {
+    for (uint32_t size = 2; size <= P; size <<= 1) {
       for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-        for (uint32_t j = threadIdx.x; j < CHUNK / 2; j += blockDim.x) {
+        for (uint32_t j = threadIdx.x; j < P / 2; j += blockDim.x) {
           const uint32_t lo = 2 * j - (j & (stride - 1));
           const uint32_t hi = lo + stride;
           const bool up = ((lo & size) == 0);
@@ -515,39 +620,52 @@ int csr_run(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* 
 // staging area already holds): 16M entries, 256 MiB of blocks.
 static constexpr uint64_t STAGE_FIRST = 1ull << 24;
 
-// Pooled collect: launch(want) walks every query once, appending its entries
-// to staging blocks sized for at least `want` entries, and writes per-query
-// counts; then scan -> capacity check -> scatter -> per-query sort -> unpack.
-// Only when the staging area overflowed while the total fits the capacity
-// does the walk run again, with room for the exact total (the area keeps that
-// size for later calls).
+// Pooled collect: launch(want_entries, want_items) walks every query once,
+// appending its entries to staging blocks sized for at least want_entries and,
+// when it splits queries, writing sub-walks to an items buffer sized for at
+// least want_items (0: its default); it leaves per-query counts and the words
+// counter[1] blocks claimed, [2] staging overflow, [4] items overflow, [6..7]
+// and [8..9] sub-walks of the two expand stages (64 bits each).  Then scan -> capacity check -> scatter -> per-query sort ->
+// unpack.  A walk whose sub-walks did not fit (its counts are partial) runs
+// again with room for all of them; one whose entries did not fit runs again
+// with room for the total, once the total is known to fit the capacity.  The
+// buffers keep their size for later calls.
 int csr_run_staged(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* out_toi, bool has_t,
-                   uint64_t capacity, uint64_t* total, const std::function<int(uint64_t)>& launch) {
+                   uint64_t capacity, uint64_t* total, const std::function<int(uint64_t, uint64_t)>& launch) {
   int rc;
-  if ((rc = launch(std::min<uint64_t>(capacity, STAGE_FIRST)))) return rc;
+  uint64_t want_e = std::min<uint64_t>(capacity, STAGE_FIRST), want_i = 0;
+  uint32_t words[16];
   std::vector<std::unique_ptr<DevBuf>> temps;
-  if ((rc = scan_into<uint32_t>(c, c->counts.as<uint32_t>(), n, offsets, temps))) return rc;
-  uint32_t words[4];
-  HIP_TRY(hipMemcpyAsync(total, offsets + n, 8, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(words, c->counter.as<uint32_t>(), 16, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  const uint64_t tot = *total;
-  if (tot > capacity) return fail(SQ_ERR_CAPACITY, "result needs " + std::to_string(tot) + " entries, capacity is " + std::to_string(capacity));
-  if (!tot) return SQ_OK;
-  if (!out_leaf || (has_t && !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
-  if (words[2]) {
-    // the same walk again with room for every entry; counts and offsets come out the same
-    if ((rc = launch(tot))) return rc;
-    HIP_TRY(hipMemcpyAsync(words, c->counter.as<uint32_t>(), 16, hipMemcpyDeviceToHost, c->stream));
+  for (int pass = 0;; ++pass) {
+    if (pass == 3) return fail(SQ_ERR_HIP, "collect staging overflowed");
+    if ((rc = launch(want_e, want_i))) return rc;
+    if ((rc = scan_into<uint32_t>(c, c->counts.as<uint32_t>(), n, offsets, temps))) return rc;
+    HIP_TRY(hipMemcpyAsync(total, offsets + n, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(words, c->counter.as<uint32_t>(), sizeof(words), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (words[2]) return fail(SQ_ERR_HIP, "collect staging overflowed twice");
+    if (words[4]) {
+      want_i = std::max((uint64_t)words[6] | ((uint64_t)words[7] << 32), (uint64_t)words[8] | ((uint64_t)words[9] << 32));
+      if (want_i > 0xFFFFFFF0ull) return fail(SQ_ERR_INVALID, "too many sub-walks for one call; lower split_depth");
+      continue;
+    }
+    const uint64_t tot = *total;
+    if (tot > capacity) return fail(SQ_ERR_CAPACITY, "result needs " + std::to_string(tot) + " entries, capacity is " + std::to_string(capacity));
+    if (!tot) return SQ_OK;
+    if (!out_leaf || (has_t && !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
+    if (words[2]) {
+      want_e = tot;
+      continue;
+    }
+    break;
   }
+  const uint64_t tot = *total;
   if ((rc = c->keys.ensure(tot * 8))) return rc;
   const uint64_t n_blocks = words[1];
   if (n_blocks > c->stage.bytes / (STAGE_BLK * sizeof(StageEntry)) || n_blocks > c->stage_used.bytes / 4)
     return fail(SQ_ERR_HIP, "collect staging block count out of range");
   k_stage_scatter<<<grid_for(n_blocks * STAGE_BLK, 256), 256, 0, c->stream>>>(
-      c->stage.as<StageEntry>(), c->stage_used.as<uint32_t>(), n_blocks, offsets, c->keys.as<uint64_t>());
+      c->stage.as<StageEntry>(), c->stage_used.as<uint32_t>(), n_blocks, offsets, c->counts.as<uint32_t>(),
+      c->keys.as<uint64_t>());
   HIP_TRY(hipGetLastError());
   return csr_sort_unpack(c, n, offsets, tot, out_leaf, out_toi, has_t);
 }
@@ -609,11 +727,13 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   if (S < 2 || S > 64 || (S & (S - 1))) return fail(SQ_ERR_INVALID, "lds_stack must be a power of two in [2, 64]");
   if (t->ray_order != 0 && t->ray_order != 1) return fail(SQ_ERR_INVALID, "ray_order must be 0 or 1");
   if (t->descend_min < 0 || t->descend_min > 64) return fail(SQ_ERR_INVALID, "descend_min must be in [0, 64]");
+  if (t->split_depth < -1 || t->split_depth > 24) return fail(SQ_ERR_INVALID, "split_depth must be in [-1, 24]");
   c->block = block;
   c->rays_per_lane = t->rays_per_lane;
   c->lds_stack = S;
   c->ray_order = t->ray_order;
   c->descend_min = t->descend_min;
+  c->split_depth = t->split_depth;
   return SQ_OK;
 }
 
@@ -624,6 +744,7 @@ int sq_ctx_get_tuning(const sq_ctx* c, sq_tuning* t) {
   t->lds_stack = c->lds_stack;
   t->ray_order = c->ray_order;
   t->descend_min = c->descend_min;
+  t->split_depth = c->split_depth;
   return SQ_OK;
 }
 
@@ -1007,15 +1128,31 @@ static int collect_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
     const uint64_t chunks = (n + chunk - 1) / chunk;
     const uint32_t spill_rows = stack_n > S ? stack_n - S : 0;
     if ((rc = c->counter.ensure(256))) return rc;
-    auto staged = [&](uint64_t want) -> int {
+    const uint32_t levels = split_levels(c, n, b->stats.max_depth, mode == 2);
+    // deep splits expand in two stages: queries one per thread down l1 levels,
+    // then those sub-walks, many and short, one per thread down the rest
+    const uint32_t l1 = levels >= 8 ? levels / 2 : levels, l2 = levels - l1;
+    auto staged = [&](uint64_t want, uint64_t want_items) -> int {
+      if (levels) {
+        // room for every possible sub-walk up to 8M of them, more when a walk needed it
+        const uint64_t def1 = std::min<uint64_t>(n << std::min<uint32_t>(l1, 30), 1ull << 23);
+        const uint64_t def2 = std::min<uint64_t>(n << std::min<uint32_t>(levels, 30), 1ull << 23);
+        int r0 = c->items.ensure(std::max(def1, want_items) * sizeof(WorkItem));
+        if (!r0 && l2) r0 = c->items2.ensure(std::max(def2, want_items) * sizeof(WorkItem));
+        if (r0) return r0;
+      }
+      const uint32_t cap1 = (uint32_t)std::min<uint64_t>(c->items.bytes / sizeof(WorkItem), 0xFFFFFFF0ull);
+      const uint32_t cap2 = (uint32_t)std::min<uint64_t>(c->items2.bytes / sizeof(WorkItem), 0xFFFFFFF0ull);
       return dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
-        auto go = [&]<int MODE_, bool FILT_>() -> int {
-          const void* kern = reinterpret_cast<const void*>(&k_collect_pool<D, K, MODE_, FILT_>);
+        auto go = [&]<int MODE_, bool FILT_, bool ITEMS_>() -> int {
+          const void* kern = reinterpret_cast<const void*>(&k_collect_pool<D, K, MODE_, FILT_, ITEMS_>);
           const size_t lds_p = (size_t)S * B * 8 + (size_t)(B / 64) * 8;
           int per_cu = 0;
           HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, B, lds_p));
           const uint64_t wpb = (uint64_t)B / 64;
-          const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
+          const uint64_t resident = (uint64_t)c->cus * std::max(1, per_cu);
+          // item walks: the item count is known on the device only, so the grid is every resident wave
+          const uint64_t grid = ITEMS_ ? resident : std::max<uint64_t>(1, std::min<uint64_t>(resident, (chunks + wpb - 1) / wpb));
           const uint64_t lanes = grid * (uint64_t)B;
           int r2 = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64);
           if (r2) return r2;
@@ -1026,17 +1163,38 @@ static int collect_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
           const uint64_t max_blocks = std::min<uint64_t>({c->stage.bytes / (STAGE_BLK * sizeof(StageEntry)),
                                                           c->stage_used.bytes / 4, 0xFFFFFFF0ull});
           uint32_t* counter = c->counter.as<uint32_t>();
-          k_zero_words<<<1, 64, 0, c->stream>>>(counter, 4);
+          auto* count1 = reinterpret_cast<unsigned long long*>(counter + 6);
+          auto* count2 = reinterpret_cast<unsigned long long*>(counter + 8);
+          k_zero_words<<<1, 64, 0, c->stream>>>(counter, 16);
           HIP_TRY(hipGetLastError());
+          const WorkItem* walk_items = c->items.as<WorkItem>();
+          uint32_t walk_cap = cap1;
+          const unsigned long long* walk_count = count1;
+          if (ITEMS_) {
+            k_collect_expand<D, K, MODE_, FILT_, false><<<grid_for(n, 256), 256, (size_t)l1 * 256 * 8, c->stream>>>(
+                s, q, nn, max_toi, tmax, l1, nullptr, 0, nullptr, c->items.as<WorkItem>(), cap1, count1, counter + 4, counts);
+            HIP_TRY(hipGetLastError());
+            if (l2) {
+              k_collect_expand<D, K, MODE_, FILT_, true><<<(uint32_t)(c->cus * 8), 256, (size_t)l2 * 256 * 8, c->stream>>>(
+                  s, q, nn, max_toi, tmax, l2, c->items.as<WorkItem>(), cap1, count1, c->items2.as<WorkItem>(), cap2, count2,
+                  counter + 4, counts);
+              HIP_TRY(hipGetLastError());
+              walk_items = c->items2.as<WorkItem>();
+              walk_cap = cap2;
+              walk_count = count2;
+            }
+          }
           StageOut o{c->stage.as<StageEntry>(), c->stage_used.as<uint32_t>(), counter + 1, counter + 2,
                      (uint32_t)max_blocks, counts};
-          k_collect_pool<D, K, MODE_, FILT_><<<(uint32_t)grid, B, lds_p, c->stream>>>(
-              s, q, nn, max_toi, tmax, fd, (uint32_t)chunk, counter, S, c->spill.as<SpillEntry>(), (uint32_t)lanes,
-              (uint32_t)c->descend_min, o);
+          k_collect_pool<D, K, MODE_, FILT_, ITEMS_><<<(uint32_t)grid, B, lds_p, c->stream>>>(
+              s, q, nn, max_toi, tmax, fd, (uint32_t)(ITEMS_ ? 64u * c->split_per_lane : chunk), counter, S,
+              c->spill.as<SpillEntry>(), (uint32_t)lanes, (uint32_t)c->descend_min, o, walk_items, walk_cap, walk_count);
           HIP_TRY(hipGetLastError());
           return SQ_OK;
         };
-#define SQ_STAGED(MODE_) return filt ? go.template operator()<MODE_, true>() : go.template operator()<MODE_, false>();
+#define SQ_STAGED(MODE_)                                                                            \
+        if (levels) return filt ? go.template operator()<MODE_, true, true>() : go.template operator()<MODE_, false, true>(); \
+        return filt ? go.template operator()<MODE_, true, false>() : go.template operator()<MODE_, false, false>();
         switch (mode) {
           case 0: SQ_STAGED(0)
           case 1: SQ_STAGED(1)

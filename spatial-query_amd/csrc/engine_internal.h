@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <functional>
 #include <memory>
 #include <string>
@@ -63,6 +64,8 @@ struct sq_ctx {
   int lds_stack = 8;               // pooled kernel: ring-stack entries kept in LDS per lane
   int ray_order = 0;               // 1: walk rays in Morton order (order.hip)
   int descend_min = 24;            // pooled walks: pause descents below this many lanes (pool.h)
+  int split_depth = 0;             // pooled collect: split queries this many levels down (0 auto, -1 never)
+  int split_per_lane = 2;          // pooled collect over sub-walks: chunks of 64 * this many
   bool timed = false;
   int refs = 1;                    // the caller's handle + one per live tree
   DevBuf counts, scratch, keys, keys2;   // reusable temporaries of the CSR queries
@@ -70,6 +73,7 @@ struct sq_ctx {
   DevBuf okeys, okeys2, oidx, oorder, otemp;   // Morton ray order
   DevBuf counter;                  // chunk counter of the persistent pooled kernels (+ staging words)
   DevBuf stage, stage_used;        // staging blocks of the pooled collect; grow to the largest total seen
+  DevBuf items, items2;            // sub-walks of the pooled collect (first and second expand stage)
 };
 
 // Drops one reference; the last one frees the stream, events and buffers.
@@ -153,6 +157,20 @@ inline uint32_t pooled_per_lane(const sq_ctx* c, uint64_t n) {
   const uint64_t want = 32ull * (uint64_t)c->cus;
   while (per > 1 && (n + 64ull * per - 1) / (64ull * per) < want) per >>= 1;
   return per;
+}
+
+// Levels at which a pooled collect splits its queries into sub-walks.  By
+// default only overlap queries split (ray and cast walks are short and even:
+// splitting them costs more than it balances), and only calls of fewer than
+// 2M queries (more keep the pool's lanes busy by themselves): at half the tree
+// depth, rounded up, which divides the work between a short expand and
+// balanced sub-walks of similar size.
+inline uint32_t split_levels(const sq_ctx* c, uint64_t n, uint64_t tree_depth, bool overlap) {
+  const int L = c->split_depth;
+  if (L < 0 || n == 0) return 0;
+  if (L > 0) return (uint32_t)L;
+  if (!overlap || n >= (1ull << 21)) return 0;
+  return (uint32_t)std::min<uint64_t>((tree_depth + 1) / 2, 16);
 }
 
 inline SceneView view_of(const sq_bvh* b) {

@@ -150,11 +150,13 @@ __device__ __forceinline__ void pool_walk(const SceneView& s, const float* __res
 // query's list, so the order in which entries were appended does not matter.
 // ---------------------------------------------------------------------------
 
-// One appended entry: query, position within the query's list, sort key.
+// One appended entry: query, position within the query's list (NO_IDX: the
+// scatter takes a free position of the query), sort key.
 struct __align__(16) StageEntry {
   uint32_t q, idx;
   uint64_t key;
 };
+constexpr uint32_t NO_IDX = 0xFFFFFFFFu;
 
 // Waves claim STAGE_BLK-entry blocks from *blocks and fill them; used[b] is
 // the number of entries block b holds.  A block is retired when the next
@@ -277,14 +279,24 @@ struct CollectQuery {
   }
 };
 
-// One wave walks the query ranges src hands it, lanes refilled as queries
+// A query's sub-walk: the walk of query q below node ref (k_collect_expand).
+struct WorkItem {
+  uint32_t q, ref;
+};
+
+// One wave walks the ranges src hands it, lanes refilled as their walks
 // finish (as pool_walk), each step visiting one leaf range (visit_step) or
-// pausing below pause_below descending lanes.  Finished queries write their
-// entry count to o.counts.
-template <int D, int KIND, int MODE, bool FILT, class Src>
+// pausing below pause_below descending lanes.  Without ITEMS, position i is
+// query i walked from the root, its entries carry their index in the query's
+// list and a finished query writes its entry count to o.counts[i].  With
+// ITEMS, position i is the sub-walk items[i] (i < min(*n_items, items_cap));
+// its entries carry NO_IDX and a finished sub-walk adds its entry count to
+// o.counts[q] (zeroed by the expand kernel).
+template <int D, int KIND, int MODE, bool FILT, bool ITEMS, class Src>
 __device__ __forceinline__ void pool_collect(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
                                              const float* __restrict__ tmax, const Filter& f, const RingStack& st,
-                                             const StageAppend& app, uint32_t pause_below) {
+                                             const StageAppend& app, uint32_t pause_below,
+                                             const WorkItem* __restrict__ items) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   const uint32_t root_ref = fbits(s.root[2 * D]);
@@ -305,12 +317,19 @@ __device__ __forceinline__ void pool_collect(const SceneView& s, const float* __
       const uint64_t rank = (uint64_t)__popcll(m & lt_mask);
       if (need && rank < avail) {
         need = false;
-        rid = (uint32_t)(next + rank);
         cnt = 0;
-        v = VisitState{root_ref, 0};
-        if (!cq.begin(s, q, rid, max_toi, tmax)) {
-          app.o.counts[rid] = 0;
-          rid = NO_LEAF;
+        if constexpr (ITEMS) {
+          const WorkItem it = items[next + rank];
+          rid = it.q;
+          v = VisitState{it.ref, 0};
+          (void)cq.begin(s, q, rid, max_toi, tmax);     // the expand kernel ran the root test
+        } else {
+          rid = (uint32_t)(next + rank);
+          v = VisitState{root_ref, 0};
+          if (!cq.begin(s, q, rid, max_toi, tmax)) {
+            app.o.counts[rid] = 0;
+            rid = NO_LEAF;
+          }
         }
       }
       const uint64_t taken = (uint64_t)__popcll(m);
@@ -324,13 +343,17 @@ __device__ __forceinline__ void pool_collect(const SceneView& s, const float* __
         s, [&](const float* nd, bool& ha, bool& hb) { cq.pair(nd, ha, hb); },
         [&](uint32_t first, uint32_t c) {
           cq.leaf(s, f, first, c, [&](uint64_t key) {
-            app(rid, cnt, key);
+            app(rid, ITEMS ? NO_IDX : cnt, key);
             ++cnt;
           });
         },
         st, v, WavePause{pause_below, active});
     if (!done) continue;
-    app.o.counts[rid] = cnt;
+    if constexpr (ITEMS) {
+      if (cnt) atomicAdd(app.o.counts + rid, cnt);
+    } else {
+      app.o.counts[rid] = cnt;
+    }
     rid = NO_LEAF;
   }
   if (lane == 0) app.close();

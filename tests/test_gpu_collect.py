@@ -1,6 +1,7 @@
 """Variable-length queries (all hits, crossings, overlap, cast-all) through the
-pooled collect: one persistent walk per call appending to staging blocks,
-then scan, scatter and a per-query sort.
+pooled collect: queries split into sub-walks a few levels down the tree (or
+not), one persistent walk per call appending to staging blocks, then scan,
+scatter and a per-query sort.
 
 Its CSR output must equal the one-query-per-thread count/fill kernels
 (rays_per_lane = 0, pinned to the CPU oracle by test_gpu_parity.py) bit for
@@ -67,7 +68,8 @@ def _all_results(b, rays, regions, casts, kind):
 
 TUNINGS = [dict(block=128, rays_per_lane=16, lds_stack=8), dict(block=64, rays_per_lane=1, lds_stack=2),
            dict(block=256, rays_per_lane=4, lds_stack=4, descend_min=0), dict(block=128, rays_per_lane=64, lds_stack=32),
-           dict(block=64, rays_per_lane=8, lds_stack=2, descend_min=64)]
+           dict(block=64, rays_per_lane=8, lds_stack=2, descend_min=64), dict(block=128, rays_per_lane=16, split_depth=-1),
+           dict(block=128, rays_per_lane=16, split_depth=1), dict(block=256, rays_per_lane=2, lds_stack=2, split_depth=8)]
 
 
 def test_pooled_collect_equals_per_thread_kernels(ctx):
@@ -145,6 +147,36 @@ def test_staging_overflow_walks_again():
         assert (counts[~full] == len(small)).all()
         assert int(off[-1]) > (1 << 24)
         for i in (0, 1, 7, 8, n - 1):
+            seg = leaf[off[i]:off[i + 1]]
+            assert np.array_equal(seg, np.arange(2000) if full[i] else small), f"query {i}"
+        torch.cuda.synchronize()
+    finally:
+        c.close()
+
+
+def test_subwalk_and_staging_overflow_walk_again():
+    """split_depth 10 on a 512-leaf tree: each of 17,143 all-covering regions
+    splits into 512 sub-walks, 8.8M in all, over the 8M the items buffer first
+    holds, so the walk runs again with room for them; that walk's 34M entries
+    then overflow the 16M staging area and it runs a third time."""
+    import torch
+    print("case: sub-walk and staging overflow (8.8M sub-walks, 34M entries, fresh context)", flush=True)
+    c = sq_amd.Context(0)
+    try:
+        c.set_tuning(split_depth=10)
+        prims = W.boxes_scene(n=2000, dim=3, lo=-10.0, hi=10.0).prims
+        b = sq_amd.DeviceBvh.build(c, 3, sq_amd.AABB, prims)
+        n = 20000
+        big = np.tile(np.array([[-1e3, -1e3, -1e3, 1e3, 1e3, 1e3]], np.float32), (n, 1))
+        big[::7, :] = np.array([5.0, 5.0, 5.0, 6.0, 6.0, 6.0], np.float32)
+        off, leaf = b.overlap(dev(big), capacity=n * 2000)
+        off, leaf = host(off), host(leaf)
+        small = oracle.Scene(3, oracle.BOX, prims).overlap(np.array([[5, 5, 5, 6, 6, 6]], np.float32))[1]
+        full = np.ones(n, bool)
+        full[::7] = False
+        counts = np.diff(off)
+        assert (counts[full] == 2000).all() and (counts[~full] == len(small)).all()
+        for i in (0, 1, 7, 13, 14, n - 2, n - 1):
             seg = leaf[off[i]:off[i + 1]]
             assert np.array_equal(seg, np.arange(2000) if full[i] else small), f"query {i}"
         torch.cuda.synchronize()
