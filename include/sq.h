@@ -244,9 +244,9 @@ int sq_tlas_overlap(sq_tlas* tlas, const float* regions, uint64_t n, uint64_t* o
  *                  and split into one sub-walk per node it reaches there (or
  *                  leaf it reaches above), so that a few long queries spread
  *                  over the whole chip; 0 = automatic (overlap calls of fewer
- *                  than 2M queries split at half the tree depth, others do
- *                  not), -1 = never split, 1..24 = fixed.  Results do not
- *                  depend on it. */
+ *                  than 2M queries split three quarters of the way down the
+ *                  tree, others do not), -1 = never split, 1..24 = fixed.
+ *                  Results do not depend on it. */
 typedef struct sq_tuning {
   int block;
   int rays_per_lane;

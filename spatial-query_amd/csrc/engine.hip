@@ -186,18 +186,18 @@ __global__ __launch_bounds__(1024) void k_collect(SceneView s, const float* __re
 }
 
 // Pooled, persistent collect (pool_collect): waves take chunks of `chunk`
-// positions from counter[0]; entries go to the staging blocks of `o`
-// (counter[1] = blocks claimed, counter[2] = overflow flag).  Positions are
-// the n queries, or with ITEMS the sub-walks k_collect_expand wrote to items
-// (*n_items of them, at most items_cap).  LDS: the S-entry ring stacks, then
-// two words per wave for its open staging block.
+// queries from counter[0], or with ITEMS the filled parts of the first
+// min(*item_blocks, max_item_blocks) item blocks of k_collect_expand (each
+// sub-walk's entry count to item_cnt); entries go to the staging blocks of
+// `o` (counter[1] = blocks claimed, counter[2] = overflow flag).  LDS: the S-entry ring stacks, then two words per wave for
+// its open staging block.
 template <int D, int KIND, int MODE, bool FILT, bool ITEMS>
 __global__ __launch_bounds__(256) void k_collect_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                       const float* __restrict__ tmax, Filter f, uint32_t chunk,
                                                       uint32_t* counter, uint32_t S, SpillEntry* spill,
                                                       uint32_t spill_stride, uint32_t pause_below, StageOut o,
-                                                      const WorkItem* __restrict__ items, uint32_t items_cap,
-                                                      const unsigned long long* n_items) {
+                                                      const WorkItem* __restrict__ items, const uint32_t* __restrict__ item_used,
+                                                      const uint32_t* item_blocks, uint32_t max_item_blocks, uint32_t* item_cnt) {
   extern __shared__ uint32_t lds[];
   const uint32_t B = blockDim.x, tid = threadIdx.x;
   const uint32_t gtid = blockIdx.x * B + tid;
@@ -209,28 +209,36 @@ __global__ __launch_bounds__(256) void k_collect_pool(SceneView s, const float* 
   }
   const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
   const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
-  const uint64_t positions = ITEMS ? std::min<uint64_t>(*n_items, items_cap) : (uint64_t)n;
-  pool_collect<D, KIND, MODE, FILT, ITEMS>(s, q, Chunks{counter, positions, chunk, home, waves}, max_toi, tmax, f, st,
-                                           StageAppend{o, slot}, pause_below, items);
+  if constexpr (ITEMS) {
+    const uint64_t nb = std::min<uint32_t>(*item_blocks, max_item_blocks);
+    pool_collect<D, KIND, MODE, FILT, true>(s, q, BlockChunks{counter, nb, item_used, home, waves}, max_toi, tmax, f, st,
+                                            StageAppend{o, slot}, pause_below, items, item_cnt);
+  } else {
+    pool_collect<D, KIND, MODE, FILT, false>(s, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, f, st,
+                                             StageAppend{o, slot}, pause_below, items, item_cnt);
+  }
 }
 
 // Splits queries into sub-walks: a depth-first walk that stops `levels` pair
 // nodes down and appends (query, node) for every node it reaches there, and
-// (query, leaf ref) for every leaf range it reaches above that, to items.
-// Without FROM_ITEMS the walks start at the root of each of the n queries
-// (which also zeroes counts[i], the item walks add to it); with FROM_ITEMS
-// they start at each of the sub-walks in_items[0 .. min(*in_count, in_cap)),
-// a grid-stride loop over them.  *out_count counts every item (64 bits); those
-// at or beyond items_cap are not written and set *overflow.  Every node and
-// leaf range the query's full walk would visit lies below exactly one item,
-// so the items' walks together visit the same leaf ranges.  At most 2^levels
-// items per input.  LDS: `levels` stack entries (ref, depth) per thread.
+// (query, leaf ref) for every leaf range it reaches above that, to item
+// blocks.  Without FROM_ITEMS the walks start at the root of each of the n
+// queries (which also zeroes counts[i], the item walks add to it); with
+// FROM_ITEMS they start at the sub-walks of the first min(*in_blocks,
+// in_max_blocks) blocks of in_items (a grid-stride loop over their slots).
+// Each wave fills one block of ITEM_BLK at a time, claimed from *out_blocks;
+// out_used[b] = the items block b holds.  Blocks at or beyond out_max_blocks
+// are not written and set *overflow.  Every node and leaf range the query's
+// full walk would visit lies below exactly one item, so the items' walks
+// together visit the same leaf ranges.  At most 2^levels items per input.
+// LDS: `levels` stack entries (ref, depth) per thread.
 template <int D, int KIND, int MODE, bool FILT, bool FROM_ITEMS>
 __global__ __launch_bounds__(256) void k_collect_expand(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                         const float* __restrict__ tmax, uint32_t levels,
-                                                        const WorkItem* __restrict__ in_items, uint32_t in_cap,
-                                                        const unsigned long long* in_count, WorkItem* items, uint32_t items_cap,
-                                                        unsigned long long* out_count, uint32_t* overflow, uint32_t* counts) {
+                                                        const WorkItem* __restrict__ in_items, const uint32_t* __restrict__ in_used,
+                                                        const uint32_t* in_blocks, uint32_t in_max_blocks,
+                                                        WorkItem* items, uint32_t* out_used, uint32_t* out_blocks,
+                                                        uint32_t out_max_blocks, uint32_t* overflow, uint32_t* counts) {
   extern __shared__ uint32_t lds[];
   const uint32_t B = blockDim.x, tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
@@ -238,19 +246,25 @@ __global__ __launch_bounds__(256) void k_collect_expand(SceneView s, const float
   uint32_t* sref = lds + tid;
   uint32_t* sdep = lds + levels * B + tid;
   constexpr int NF = (D + 1) * 4;
-  const uint64_t n_in = FROM_ITEMS ? std::min<uint64_t>(*in_count, in_cap) : (uint64_t)n;
+  const uint64_t n_in = FROM_ITEMS ? (uint64_t)std::min<uint32_t>(*in_blocks, in_max_blocks) * ITEM_BLK : (uint64_t)n;
+  // the wave's open item block and its fill (the same in every lane: only
+  // changed where the whole wave runs)
+  uint32_t blk = NO_BLOCK, fill = 0;
   for (uint64_t first = (uint64_t)blockIdx.x * B; first < n_in; first += (uint64_t)gridDim.x * B) {
     const uint64_t i = first + tid;
     CollectQuery<D, KIND, MODE, FILT> cq;
     bool live = i < n_in;
     uint32_t qi = 0, cur = fbits(s.root[2 * D]);
-    if (live) {
-      if constexpr (FROM_ITEMS) {
+    if constexpr (FROM_ITEMS) {
+      if (live) live = (uint32_t)(i % ITEM_BLK) < in_used[i / ITEM_BLK];
+      if (live) {
         const WorkItem it = in_items[i];
         qi = it.q;
         cur = it.ref;
         (void)cq.begin(s, q, qi, max_toi, tmax);        // the first expand ran the root test
-      } else {
+      }
+    } else {
+      if (live) {
         qi = (uint32_t)i;
         counts[qi] = 0;
         live = cq.begin(s, q, qi, max_toi, tmax);
@@ -261,18 +275,17 @@ __global__ __launch_bounds__(256) void k_collect_expand(SceneView s, const float
       const bool at = live && ((cur & LEAF_BIT) || depth == levels);
       const uint64_t m = __ballot(at);
       if (m) {
-        // every index below min(count, items_cap) is written: claims are contiguous from 0
-        const uint32_t leader = (uint32_t)__builtin_ctzll(m);
         const uint32_t k = (uint32_t)__popcll(m);
-        uint64_t base = 0;
-        if (lane == leader) {
-          base = atomicAdd(out_count, (unsigned long long)k);
-          if (base + k > items_cap) *overflow = 1u;
+        if (blk == NO_BLOCK || fill + k > ITEM_BLK) {
+          if (blk != NO_BLOCK && blk < out_max_blocks && lane == 0) out_used[blk] = fill;
+          uint32_t nb = 0;
+          if (lane == 0) nb = atomicAdd(out_blocks, 1u);
+          blk = (uint32_t)__builtin_amdgcn_readfirstlane((int)nb);
+          fill = 0;
+          if (blk >= out_max_blocks && lane == 0) *overflow = 1u;
         }
-        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)base, (int)leader);
-        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(base >> 32), (int)leader);
-        const uint64_t at_i = ((uint64_t)hi << 32 | lo) + (uint32_t)__popcll(m & lt_mask);
-        if (at && at_i < items_cap) items[at_i] = WorkItem{qi, cur};
+        if (at && blk < out_max_blocks) items[(size_t)blk * ITEM_BLK + fill + (uint32_t)__popcll(m & lt_mask)] = WorkItem{qi, cur};
+        fill += k;
       }
       bool pop = at;
       if (live && !at) {
@@ -304,26 +317,41 @@ __global__ __launch_bounds__(256) void k_collect_expand(SceneView s, const float
       }
     }
   }
+  if (blk != NO_BLOCK && blk < out_max_blocks && lane == 0) out_used[blk] = fill;
 }
 
 __global__ void k_zero_words(uint32_t* p, uint32_t n) {
   if (threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
+// Each sub-walk's place in its query's list: item_cnt[p] (its entry count) is
+// replaced by the count of the query's entries placed before it, taken from
+// counts[q] (zeroed by the expand), which ends as the query's total.  Runs
+// over the filled slots of the first min(*blocks, max_blocks) item blocks.
+__global__ __launch_bounds__(256) void k_item_base(const WorkItem* __restrict__ items, const uint32_t* __restrict__ used,
+                                                   const uint32_t* blocks, uint32_t max_blocks, uint32_t* item_cnt,
+                                                   uint32_t* counts) {
+  const uint64_t slots = (uint64_t)std::min<uint32_t>(*blocks, max_blocks) * ITEM_BLK;
+  for (uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x; p < slots; p += (uint64_t)gridDim.x * 256) {
+    if ((uint32_t)(p % ITEM_BLK) >= used[p / ITEM_BLK]) continue;
+    const uint32_t c = item_cnt[p];
+    item_cnt[p] = c ? atomicAdd(counts + items[p].q, c) : 0u;
+  }
+}
+
 // Staged entries to their CSR slots, for the first used[b] entries of each
-// block b < n_blocks: keys[offsets[q] + idx] = key, or for idx = NO_IDX a free
-// slot of query q taken by counting counts[q] down (each of the query's
-// counts[q] entries takes one of its slots; the per-query sort that follows
-// fixes the order).
+// block b < n_blocks: keys[offsets[q] + idx] = key; with items, an entry's q
+// is its sub-walk p and its slot is offsets[items[p].q] + item_base[p] + idx.
 __global__ __launch_bounds__(256) void k_stage_scatter(const StageEntry* __restrict__ e, const uint32_t* __restrict__ used,
-                                                       uint64_t n_blocks, const uint64_t* __restrict__ offsets, uint32_t* counts,
+                                                       uint64_t n_blocks, const uint64_t* __restrict__ offsets,
+                                                       const WorkItem* __restrict__ items, const uint32_t* __restrict__ item_base,
                                                        uint64_t* keys) {
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n_blocks * STAGE_BLK) return;
   if ((uint32_t)(i % STAGE_BLK) >= used[i / STAGE_BLK]) return;
   const StageEntry x = e[i];
-  const uint32_t idx = x.idx != NO_IDX ? x.idx : atomicSub(counts + x.q, 1u) - 1u;
-  keys[offsets[x.q] + idx] = x.key;
+  if (items) keys[offsets[items[x.q].q] + item_base[x.q] + x.idx] = x.key;
+  else keys[offsets[x.q] + x.idx] = x.key;
 }
 
 // Exclusive scan of n values (u32 or u64) into out[0..n] (out[n] = total).
@@ -620,32 +648,42 @@ int csr_run(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* 
 // staging area already holds): 16M entries, 256 MiB of blocks.
 static constexpr uint64_t STAGE_FIRST = 1ull << 24;
 
-// Pooled collect: launch(want_entries, want_items) walks every query once,
+// Where staged entries of sub-walks find their query and place (null: the
+// entries carry their query and index).
+struct StagedWalk {
+  const WorkItem* items;
+  const uint32_t* item_base;
+};
+
+// Pooled collect: launch(want_entries, want_blocks, walk) walks every query once,
 // appending its entries to staging blocks sized for at least want_entries and,
-// when it splits queries, writing sub-walks to an items buffer sized for at
-// least want_items (0: its default); it leaves per-query counts and the words
-// counter[1] blocks claimed, [2] staging overflow, [4] items overflow, [6..7]
-// and [8..9] sub-walks of the two expand stages (64 bits each).  Then scan -> capacity check -> scatter -> per-query sort ->
+// when it splits queries, writing sub-walks to item blocks (at least
+// want_blocks per expand stage; 0: the default); it leaves per-query counts and
+// the words counter[1] staging blocks claimed, [2] staging overflow, [4] item
+// overflow, [6] and [8] item blocks claimed by the two expand stages.  Then scan -> capacity check -> scatter -> per-query sort ->
 // unpack.  A walk whose sub-walks did not fit (its counts are partial) runs
 // again with room for all of them; one whose entries did not fit runs again
 // with room for the total, once the total is known to fit the capacity.  The
 // buffers keep their size for later calls.
 int csr_run_staged(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* out_toi, bool has_t,
-                   uint64_t capacity, uint64_t* total, const std::function<int(uint64_t, uint64_t)>& launch) {
+                   uint64_t capacity, uint64_t* total, const std::function<int(uint64_t, uint64_t, StagedWalk&)>& launch) {
+  StagedWalk walk{};
   int rc;
   uint64_t want_e = std::min<uint64_t>(capacity, STAGE_FIRST), want_i = 0;
   uint32_t words[16];
   std::vector<std::unique_ptr<DevBuf>> temps;
   for (int pass = 0;; ++pass) {
-    if (pass == 3) return fail(SQ_ERR_HIP, "collect staging overflowed");
-    if ((rc = launch(want_e, want_i))) return rc;
+    if (pass == 4) return fail(SQ_ERR_HIP, "collect staging overflowed");
+    if ((rc = launch(want_e, want_i, walk))) return rc;
     if ((rc = scan_into<uint32_t>(c, c->counts.as<uint32_t>(), n, offsets, temps))) return rc;
     HIP_TRY(hipMemcpyAsync(total, offsets + n, 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(words, c->counter.as<uint32_t>(), sizeof(words), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (words[4]) {
-      want_i = std::max((uint64_t)words[6] | ((uint64_t)words[7] << 32), (uint64_t)words[8] | ((uint64_t)words[9] << 32));
-      if (want_i > 0xFFFFFFF0ull) return fail(SQ_ERR_INVALID, "too many sub-walks for one call; lower split_depth");
+      // blocks claimed vary with the waves' interleaving: leave a quarter more
+      want_i = std::max(words[6], words[8]);
+      want_i += want_i / 4 + 1024;
+      if (want_i >= 0xFFFFFFF0ull / ITEM_BLK) return fail(SQ_ERR_INVALID, "too many sub-walks for one call; lower split_depth");
       continue;
     }
     const uint64_t tot = *total;
@@ -664,7 +702,7 @@ int csr_run_staged(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, 
   if (n_blocks > c->stage.bytes / (STAGE_BLK * sizeof(StageEntry)) || n_blocks > c->stage_used.bytes / 4)
     return fail(SQ_ERR_HIP, "collect staging block count out of range");
   k_stage_scatter<<<grid_for(n_blocks * STAGE_BLK, 256), 256, 0, c->stream>>>(
-      c->stage.as<StageEntry>(), c->stage_used.as<uint32_t>(), n_blocks, offsets, c->counts.as<uint32_t>(),
+      c->stage.as<StageEntry>(), c->stage_used.as<uint32_t>(), n_blocks, offsets, walk.items, walk.item_base,
       c->keys.as<uint64_t>());
   HIP_TRY(hipGetLastError());
   return csr_sort_unpack(c, n, offsets, tot, out_leaf, out_toi, has_t);
@@ -1132,17 +1170,32 @@ static int collect_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
     // deep splits expand in two stages: queries one per thread down l1 levels,
     // then those sub-walks, many and short, one per thread down the rest
     const uint32_t l1 = levels >= 8 ? levels / 2 : levels, l2 = levels - l1;
-    auto staged = [&](uint64_t want, uint64_t want_items) -> int {
+    auto staged = [&](uint64_t want, uint64_t want_blocks, StagedWalk& walk) -> int {
+      // item blocks: room for every possible sub-walk (up to 8M of them, more
+      // when a walk needed it) at the worst fill of a retired block (over
+      // ITEM_BLK - 64), plus one open block per expanding wave
+      auto item_blocks = [&](uint32_t lv, uint64_t waves) -> uint64_t {
+        const uint64_t possible = std::min<uint64_t>(n << std::min<uint32_t>(lv, 30), 1ull << 23);
+        return std::max<uint64_t>(possible / (ITEM_BLK - 64) + 1 + waves, want_blocks);
+      };
       if (levels) {
-        // room for every possible sub-walk up to 8M of them, more when a walk needed it
-        const uint64_t def1 = std::min<uint64_t>(n << std::min<uint32_t>(l1, 30), 1ull << 23);
-        const uint64_t def2 = std::min<uint64_t>(n << std::min<uint32_t>(levels, 30), 1ull << 23);
-        int r0 = c->items.ensure(std::max(def1, want_items) * sizeof(WorkItem));
-        if (!r0 && l2) r0 = c->items2.ensure(std::max(def2, want_items) * sizeof(WorkItem));
+        const uint64_t nb1 = item_blocks(l1, (uint64_t)grid_for(n, 256) * 4);
+        int r0 = c->items.ensure(nb1 * ITEM_BLK * sizeof(WorkItem));
+        if (!r0) r0 = c->items_used.ensure(nb1 * 4);
+        if (!r0 && l2) {
+          const uint64_t nb2 = item_blocks(levels, (uint64_t)c->cus * 8 * 4);
+          r0 = c->items2.ensure(nb2 * ITEM_BLK * sizeof(WorkItem));
+          if (!r0) r0 = c->items2_used.ensure(nb2 * 4);
+        }
+        // per sub-walk entry counts, then bases, for the stage the walk takes
+        if (!r0) r0 = c->items_cnt.ensure((l2 ? c->items2.bytes : c->items.bytes) / sizeof(WorkItem) * 4);
         if (r0) return r0;
       }
-      const uint32_t cap1 = (uint32_t)std::min<uint64_t>(c->items.bytes / sizeof(WorkItem), 0xFFFFFFF0ull);
-      const uint32_t cap2 = (uint32_t)std::min<uint64_t>(c->items2.bytes / sizeof(WorkItem), 0xFFFFFFF0ull);
+      auto max_blocks_of = [](const DevBuf& it, const DevBuf& used) -> uint32_t {
+        // sub-walk positions are 32-bit
+        return (uint32_t)std::min<uint64_t>({it.bytes / (ITEM_BLK * sizeof(WorkItem)), used.bytes / 4, 0xFFFFFFFFull / ITEM_BLK});
+      };
+      const uint32_t mb1 = max_blocks_of(c->items, c->items_used), mb2 = max_blocks_of(c->items2, c->items2_used);
       return dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
         auto go = [&]<int MODE_, bool FILT_, bool ITEMS_>() -> int {
           const void* kern = reinterpret_cast<const void*>(&k_collect_pool<D, K, MODE_, FILT_, ITEMS_>);
@@ -1163,33 +1216,42 @@ static int collect_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
           const uint64_t max_blocks = std::min<uint64_t>({c->stage.bytes / (STAGE_BLK * sizeof(StageEntry)),
                                                           c->stage_used.bytes / 4, 0xFFFFFFF0ull});
           uint32_t* counter = c->counter.as<uint32_t>();
-          auto* count1 = reinterpret_cast<unsigned long long*>(counter + 6);
-          auto* count2 = reinterpret_cast<unsigned long long*>(counter + 8);
           k_zero_words<<<1, 64, 0, c->stream>>>(counter, 16);
           HIP_TRY(hipGetLastError());
           const WorkItem* walk_items = c->items.as<WorkItem>();
-          uint32_t walk_cap = cap1;
-          const unsigned long long* walk_count = count1;
+          const uint32_t* walk_used = c->items_used.as<uint32_t>();
+          const uint32_t* walk_blocks = counter + 6;
+          uint32_t walk_max = mb1;
           if (ITEMS_) {
             k_collect_expand<D, K, MODE_, FILT_, false><<<grid_for(n, 256), 256, (size_t)l1 * 256 * 8, c->stream>>>(
-                s, q, nn, max_toi, tmax, l1, nullptr, 0, nullptr, c->items.as<WorkItem>(), cap1, count1, counter + 4, counts);
+                s, q, nn, max_toi, tmax, l1, nullptr, nullptr, nullptr, 0, c->items.as<WorkItem>(),
+                c->items_used.as<uint32_t>(), counter + 6, mb1, counter + 4, counts);
             HIP_TRY(hipGetLastError());
             if (l2) {
               k_collect_expand<D, K, MODE_, FILT_, true><<<(uint32_t)(c->cus * 8), 256, (size_t)l2 * 256 * 8, c->stream>>>(
-                  s, q, nn, max_toi, tmax, l2, c->items.as<WorkItem>(), cap1, count1, c->items2.as<WorkItem>(), cap2, count2,
-                  counter + 4, counts);
+                  s, q, nn, max_toi, tmax, l2, c->items.as<WorkItem>(), c->items_used.as<uint32_t>(), counter + 6, mb1,
+                  c->items2.as<WorkItem>(), c->items2_used.as<uint32_t>(), counter + 8, mb2, counter + 4, counts);
               HIP_TRY(hipGetLastError());
               walk_items = c->items2.as<WorkItem>();
-              walk_cap = cap2;
-              walk_count = count2;
+              walk_used = c->items2_used.as<uint32_t>();
+              walk_blocks = counter + 8;
+              walk_max = mb2;
             }
           }
           StageOut o{c->stage.as<StageEntry>(), c->stage_used.as<uint32_t>(), counter + 1, counter + 2,
                      (uint32_t)max_blocks, counts};
           k_collect_pool<D, K, MODE_, FILT_, ITEMS_><<<(uint32_t)grid, B, lds_p, c->stream>>>(
-              s, q, nn, max_toi, tmax, fd, (uint32_t)(ITEMS_ ? 64u * c->split_per_lane : chunk), counter, S,
-              c->spill.as<SpillEntry>(), (uint32_t)lanes, (uint32_t)c->descend_min, o, walk_items, walk_cap, walk_count);
+              s, q, nn, max_toi, tmax, fd, (uint32_t)chunk, counter, S,
+              c->spill.as<SpillEntry>(), (uint32_t)lanes, (uint32_t)c->descend_min, o, walk_items, walk_used, walk_blocks,
+              walk_max, c->items_cnt.as<uint32_t>());
           HIP_TRY(hipGetLastError());
+          walk = StagedWalk{nullptr, nullptr};
+          if (ITEMS_) {
+            k_item_base<<<(uint32_t)(c->cus * 4), 256, 0, c->stream>>>(walk_items, walk_used, walk_blocks, walk_max,
+                                                                      c->items_cnt.as<uint32_t>(), counts);
+            HIP_TRY(hipGetLastError());
+            walk = StagedWalk{walk_items, c->items_cnt.as<uint32_t>()};
+          }
           return SQ_OK;
         };
 #define SQ_STAGED(MODE_)                                                                            \

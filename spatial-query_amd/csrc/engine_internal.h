@@ -65,7 +65,6 @@ struct sq_ctx {
   int ray_order = 0;               // 1: walk rays in Morton order (order.hip)
   int descend_min = 24;            // pooled walks: pause descents below this many lanes (pool.h)
   int split_depth = 0;             // pooled collect: split queries this many levels down (0 auto, -1 never)
-  int split_per_lane = 2;          // pooled collect over sub-walks: chunks of 64 * this many
   bool timed = false;
   int refs = 1;                    // the caller's handle + one per live tree
   DevBuf counts, scratch, keys, keys2;   // reusable temporaries of the CSR queries
@@ -73,7 +72,9 @@ struct sq_ctx {
   DevBuf okeys, okeys2, oidx, oorder, otemp;   // Morton ray order
   DevBuf counter;                  // chunk counter of the persistent pooled kernels (+ staging words)
   DevBuf stage, stage_used;        // staging blocks of the pooled collect; grow to the largest total seen
-  DevBuf items, items2;            // sub-walks of the pooled collect (first and second expand stage)
+  DevBuf items, items_used;        // sub-walk blocks of the pooled collect's first expand stage, their fill
+  DevBuf items2, items2_used;      // and of its second stage
+  DevBuf items_cnt;                // entry count, then base, of each sub-walk the collect walks
 };
 
 // Drops one reference; the last one frees the stream, events and buffers.
@@ -162,15 +163,16 @@ inline uint32_t pooled_per_lane(const sq_ctx* c, uint64_t n) {
 // Levels at which a pooled collect splits its queries into sub-walks.  By
 // default only overlap queries split (ray and cast walks are short and even:
 // splitting them costs more than it balances), and only calls of fewer than
-// 2M queries (more keep the pool's lanes busy by themselves): at half the tree
-// depth, rounded up, which divides the work between a short expand and
-// balanced sub-walks of similar size.
+// 2M queries (more keep the pool's lanes busy by themselves): three quarters
+// of the way down the tree, where the expand stages still run balanced and
+// the sub-walks left are short (measured best on 8K-262K queries of a depth-19
+// tree).
 inline uint32_t split_levels(const sq_ctx* c, uint64_t n, uint64_t tree_depth, bool overlap) {
   const int L = c->split_depth;
   if (L < 0 || n == 0) return 0;
   if (L > 0) return (uint32_t)L;
   if (!overlap || n >= (1ull << 21)) return 0;
-  return (uint32_t)std::min<uint64_t>((tree_depth + 1) / 2, 16);
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((3 * tree_depth + 2) / 4, 24));
 }
 
 inline SceneView view_of(const sq_bvh* b) {

@@ -150,13 +150,12 @@ __device__ __forceinline__ void pool_walk(const SceneView& s, const float* __res
 // query's list, so the order in which entries were appended does not matter.
 // ---------------------------------------------------------------------------
 
-// One appended entry: query, position within the query's list (NO_IDX: the
-// scatter takes a free position of the query), sort key.
+// One appended entry: query (or, from a sub-walk, the sub-walk's position in
+// the item blocks), position within that query's (sub-walk's) list, sort key.
 struct __align__(16) StageEntry {
   uint32_t q, idx;
   uint64_t key;
 };
-constexpr uint32_t NO_IDX = 0xFFFFFFFFu;
 
 // Waves claim STAGE_BLK-entry blocks from *blocks and fill them; used[b] is
 // the number of entries block b holds.  A block is retired when the next
@@ -280,29 +279,55 @@ struct CollectQuery {
 };
 
 // A query's sub-walk: the walk of query q below node ref (k_collect_expand).
+// Sub-walks are written in blocks of ITEM_BLK, claimed one at a time by a
+// wave from a block counter; used[b] says how many block b holds.
 struct WorkItem {
   uint32_t q, ref;
+};
+constexpr uint32_t ITEM_BLK = 256;
+
+// Query ranges that are the filled parts of item blocks: a wave takes its own
+// block first, then blocks from *counter, skipping empty ones.
+struct BlockChunks {
+  uint32_t* counter;
+  uint64_t n;                  // blocks
+  const uint32_t* used;
+  uint64_t home, waves;
+  bool started = false;
+  __device__ __forceinline__ bool get(uint64_t& next, uint64_t& e) {
+    while (true) {
+      uint64_t c;
+      if (!started) { started = true; c = home; }
+      else c = waves + (uint64_t)wave_fetch_add(counter);
+      if (c >= n) return false;
+      const uint32_t u = used[c];
+      if (!u) continue;
+      next = c * ITEM_BLK;
+      e = next + u;
+      return true;
+    }
+  }
 };
 
 // One wave walks the ranges src hands it, lanes refilled as their walks
 // finish (as pool_walk), each step visiting one leaf range (visit_step) or
 // pausing below pause_below descending lanes.  Without ITEMS, position i is
-// query i walked from the root, its entries carry their index in the query's
-// list and a finished query writes its entry count to o.counts[i].  With
-// ITEMS, position i is the sub-walk items[i] (i < min(*n_items, items_cap));
-// its entries carry NO_IDX and a finished sub-walk adds its entry count to
-// o.counts[q] (zeroed by the expand kernel).
+// query i walked from the root, its entries carry (i, index in the query's
+// list) and a finished query writes its entry count to o.counts[i].  With
+// ITEMS, position i is the sub-walk items[i] (src hands out filled parts of
+// item blocks), its entries carry (i, index in the sub-walk's list) and a
+// finished sub-walk writes its entry count to item_cnt[i].
 template <int D, int KIND, int MODE, bool FILT, bool ITEMS, class Src>
 __device__ __forceinline__ void pool_collect(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
                                              const float* __restrict__ tmax, const Filter& f, const RingStack& st,
                                              const StageAppend& app, uint32_t pause_below,
-                                             const WorkItem* __restrict__ items) {
+                                             const WorkItem* __restrict__ items, uint32_t* item_cnt) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   const uint32_t root_ref = fbits(s.root[2 * D]);
   CollectQuery<D, KIND, MODE, FILT> cq;
   VisitState v{root_ref, 0};
-  uint32_t rid = NO_LEAF, cnt = 0;
+  uint32_t rid = NO_LEAF, cnt = 0, pos = 0;
   uint64_t next = 0, end = 0;
   bool exhausted = !src.get(next, end);
   while (true) {
@@ -319,7 +344,8 @@ __device__ __forceinline__ void pool_collect(const SceneView& s, const float* __
         need = false;
         cnt = 0;
         if constexpr (ITEMS) {
-          const WorkItem it = items[next + rank];
+          pos = (uint32_t)(next + rank);
+          const WorkItem it = items[pos];
           rid = it.q;
           v = VisitState{it.ref, 0};
           (void)cq.begin(s, q, rid, max_toi, tmax);     // the expand kernel ran the root test
@@ -343,14 +369,14 @@ __device__ __forceinline__ void pool_collect(const SceneView& s, const float* __
         s, [&](const float* nd, bool& ha, bool& hb) { cq.pair(nd, ha, hb); },
         [&](uint32_t first, uint32_t c) {
           cq.leaf(s, f, first, c, [&](uint64_t key) {
-            app(rid, ITEMS ? NO_IDX : cnt, key);
+            app(ITEMS ? pos : rid, cnt, key);
             ++cnt;
           });
         },
         st, v, WavePause{pause_below, active});
     if (!done) continue;
     if constexpr (ITEMS) {
-      if (cnt) atomicAdd(app.o.counts + rid, cnt);
+      item_cnt[pos] = cnt;
     } else {
       app.o.counts[rid] = cnt;
     }

@@ -155,18 +155,18 @@ def test_staging_overflow_walks_again():
 
 
 def test_subwalk_and_staging_overflow_walk_again():
-    """split_depth 10 on a 512-leaf tree: each of 17,143 all-covering regions
-    splits into 512 sub-walks, 8.8M in all, over the 8M the items buffer first
-    holds, so the walk runs again with room for them; that walk's 34M entries
-    then overflow the 16M staging area and it runs a third time."""
+    """split_depth 10 on a 512-leaf tree: each of 34,285 all-covering regions
+    splits into 512 sub-walks, 17.6M in all, over the 13.3M the item blocks
+    first hold, so the walk runs again with room for them; that walk's 68.6M
+    entries then overflow the 16M staging area and it runs a third time."""
     import torch
-    print("case: sub-walk and staging overflow (8.8M sub-walks, 34M entries, fresh context)", flush=True)
+    print("case: sub-walk and staging overflow (17.6M sub-walks, 68.6M entries, fresh context)", flush=True)
     c = sq_amd.Context(0)
     try:
         c.set_tuning(split_depth=10)
         prims = W.boxes_scene(n=2000, dim=3, lo=-10.0, hi=10.0).prims
         b = sq_amd.DeviceBvh.build(c, 3, sq_amd.AABB, prims)
-        n = 20000
+        n = 40000
         big = np.tile(np.array([[-1e3, -1e3, -1e3, 1e3, 1e3, 1e3]], np.float32), (n, 1))
         big[::7, :] = np.array([5.0, 5.0, 5.0, 6.0, 6.0, 6.0], np.float32)
         off, leaf = b.overlap(dev(big), capacity=n * 2000)
