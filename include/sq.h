@@ -125,11 +125,31 @@ int sq_bvh_get_stats(const sq_bvh* bvh, sq_tree_stats* out);   /* Bvh::stats of 
 /* Node bounds as currently held ON THE DEVICE, mapped back to reference node
  * order (float[nn][2D], host pointer; synchronises).  Shows a device refit. */
 int sq_bvh_download_bounds(sq_bvh* bvh, float* node_bounds);
+/* Bvh::nodes() / prim_indices() (bvh.rs:465-475) of a device tree: node
+ * bounds as sq_bvh_download_bounds (the current device boxes), child_a,
+ * child_b, prim_count (node_count entries each) and prim_indices (n_prims).
+ * Any pointer may be NULL. */
+int sq_bvh_export(sq_bvh* bvh, float* node_bounds, uint32_t* child_a, uint32_t* child_b, uint32_t* prim_count,
+                  uint32_t* prim_indices);
 /* GpuRaycaster::refit / Bvh::refit(&G) (gpu/mod.rs:326-341, tree.rs:54-75):
  * new primitives in the kind's layout, DEVICE pointer, same leaf count;
  * topology kept; bottom-up refit on the device, bit-identical to the host
  * refit.  Asynchronous. */
 int sq_bvh_refit(sq_bvh* bvh, const float* prims_device);
+/* Bvh::build (bvh.rs:31-37, tree.rs:39-49,112-163) from primitives already in
+ * device memory, built on the device: the same tree node for node as
+ * sq_tree_build + sq_bvh_upload (same topology, prim_indices, bounds and
+ * stats), level by level with one radix sort per level.  Synchronises.  Not
+ * for graph capture. */
+int sq_bvh_build_device(sq_ctx* ctx, int dim, int kind, const float* prims_device, uint64_t n_prims, sq_bvh** out);
+/* Rebuild in place from moved primitives (device memory, the same count and
+ * kind): Bvh::build again on the new geometry, where sq_bvh_refit keeps the
+ * topology (tree.rs:54-75) and lets the tree loosen as primitives move.  The
+ * device arrays keep their addresses (a device-built tree reserves room for
+ * any tree of its primitive count), so batch sets and captured graphs over
+ * the tree stay valid; a batch set sized its traversal stack when it was
+ * made, so make it again if the rebuilt tree is deeper (stats max_depth). */
+int sq_bvh_rebuild(sq_bvh* bvh, const float* prims_device);
 
 /* ---- ray queries (bvh.rs:51-328) ------------------------------------------ */
 /* max_toi_per_ray (device, n floats) overrides max_toi when not NULL.

@@ -111,6 +111,10 @@ struct sq_batch {
   size_t lds = 0;
   DevBuf items, counter, spill;
   std::vector<std::pair<uint64_t, NearestOut>> empty;   // batches over scenes without primitives: all misses
+  // BVH scenes as the table holds them: a rebuild that moved a scene's arrays
+  // or deepened it past the spill rows sized here is refused at submit
+  std::vector<std::pair<const sq_bvh*, SceneView>> scenes;
+  uint32_t stack_cap = 0;
 };
 
 
@@ -166,6 +170,7 @@ int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* qs, uint
     if (scene_class == SQ_BATCH_BVH) {
       BvhItem it{};
       it.s = view_of(static_cast<const sq_bvh*>(q.scene));
+      b->scenes.emplace_back(static_cast<const sq_bvh*>(q.scene), it.s);
       it.rays = q.rays; it.leaf = q.out_leaf; it.toi = q.out_toi;
       it.chunk0 = c0; it.n = (uint32_t)q.n; it.max_toi = q.max_toi;
       bi.push_back(it);
@@ -209,6 +214,7 @@ int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* qs, uint
     const uint64_t need = (chunks + waves_per_block - 1) / waves_per_block;
     b->grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus * std::max(1, per_cu), need));
     if (scene_class == SQ_BATCH_BVH) {
+      b->stack_cap = max_stack;
       const uint32_t rows = max_stack > b->S ? max_stack - b->S : 0;
       if ((rc = b->spill.ensure((size_t)rows * b->grid * b->block * sizeof(SpillEntry) + 64))) return rc;
       if ((rc = upload(b->items, bi.data(), bi.size() * sizeof(BvhItem), ctx->stream))) return rc;
@@ -225,6 +231,12 @@ int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* qs, uint
 int sq_batch_submit(sq_batch* b) {
   if (!b) return fail(SQ_ERR_INVALID, "NULL argument");
   sq_ctx* c = b->ctx;
+  for (const auto& sv : b->scenes) {
+    const SceneView now = view_of(sv.first);
+    if (now.nodes != sv.second.nodes || now.prims != sv.second.prims || now.ids != sv.second.ids ||
+        now.root != sv.second.root || sv.first->packed.max_stack > b->stack_cap)
+      return fail(SQ_ERR_INVALID, "a scene of this batch set was rebuilt into new arrays or deeper; make the batch set again");
+  }
   int rc = begin_timing(c);
   if (rc) return rc;
   for (const auto& e : b->empty)

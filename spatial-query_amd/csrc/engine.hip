@@ -958,6 +958,19 @@ int sq_bvh_build(sq_ctx* ctx, int dim, int kind, const float* prims, uint64_t n,
   return rc;
 }
 
+int sq_bvh_export(sq_bvh* b, float* bounds, uint32_t* ca, uint32_t* cb, uint32_t* pc, uint32_t* idx) {
+  if (!b) return fail(SQ_ERR_INVALID, "bvh is NULL");
+  int rc = ensure_host_tree(b);
+  if (rc) return rc;
+  const sqh::RefTree& t = b->tree;
+  if (bounds && (rc = sq_bvh_download_bounds(b, bounds))) return rc;
+  if (ca) std::memcpy(ca, t.child_a.data(), t.child_a.size() * 4);
+  if (cb) std::memcpy(cb, t.child_b.data(), t.child_b.size() * 4);
+  if (pc) std::memcpy(pc, t.prim_count.data(), t.prim_count.size() * 4);
+  if (idx) std::memcpy(idx, t.prim_indices.data(), t.prim_indices.size() * 4);
+  return SQ_OK;
+}
+
 int sq_bvh_destroy(sq_bvh* b) {
   if (!b) return SQ_OK;
   sq_ctx* c = b->ctx;
@@ -982,12 +995,16 @@ int sq_bvh_set_layers(sq_bvh* b, const uint64_t* layers) {
 
 int sq_bvh_get_stats(const sq_bvh* b, sq_tree_stats* out) {
   if (!b || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  int rc = ensure_host_tree(const_cast<sq_bvh*>(b));     // quality is a host sum in node order
+  if (rc) return rc;
   *out = b->stats;
   return SQ_OK;
 }
 
 int sq_bvh_download_bounds(sq_bvh* b, float* bounds) {
   if (!b || !bounds) return fail(SQ_ERR_INVALID, "NULL argument");
+  int rc0 = ensure_host_tree(b);
+  if (rc0) return rc0;
   const int D = b->dim, NF = (D + 1) * 4;
   const size_t nn = b->tree.node_count();
   if (!nn) return SQ_OK;

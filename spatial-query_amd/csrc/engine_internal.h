@@ -53,6 +53,8 @@ struct DevBuf {
   template <class T> T* as() const { return (T*)p; }
 };
 
+constexpr int SQ_BUILD_BUFS = 32;
+
 struct sq_ctx {
   int device = 0;
   int cus = 256;                   // compute units of the device
@@ -75,6 +77,7 @@ struct sq_ctx {
   DevBuf items, items_used;        // sub-walk blocks of the pooled collect's first expand stage, their fill
   DevBuf items2, items2_used;      // and of its second stage
   DevBuf items_cnt;                // entry count, then base, of each sub-walk the collect walks
+  DevBuf build[SQ_BUILD_BUFS];     // scratch of device builds (build.hip)
 };
 
 // Drops one reference; the last one frees the stream, events and buffers.
@@ -85,11 +88,20 @@ struct sq_bvh {
   sq_ctx* ctx = nullptr;
   int dim = 0, kind = 0;
   uint64_t n_prims = 0;
-  sqh::RefTree tree;
+  sqh::RefTree tree;               // host copy of the reference tree (device builds: see host_tree_ok)
   sqh::Packed packed;
   DevBuf nodes, prims, ids, root, layers, levels;
   sq_tree_stats stats{};
+  // device builds (build.hip): the reference layout stays on the device, and
+  // tree, packed.slot_of_ref / nodes and stats.quality are filled on first use
+  bool host_tree_ok = true;
+  uint64_t ref_nodes = 0;
+  DevBuf ref_ca, ref_cb, ref_pc;
 };
+
+// Downloads the reference tree of a device-built tree into b->tree (and the
+// host arrays that depend on it); no-op for trees built on the host.
+int ensure_host_tree(sq_bvh* b);
 
 // ---------------------------------------------------------------------------
 // dispatch over (D, KIND)

@@ -624,6 +624,8 @@ int sq_proposal_create(sq_ctx* ctx, sq_bvh* scene, int cut_depth, float rff_scal
   if (scene->n_prims == 0) return fail(SQ_ERR_INVALID, "empty scene");
   if (cut_depth < 1 || cut_depth > 12) return fail(SQ_ERR_INVALID, "cut depth must be in [1, 12] (<= 4096 subtrees)");
   if (!(rff_scale > 0.0f && rff_scale < 1e3f)) return fail(SQ_ERR_INVALID, "rff_scale must be in (0, 1000)");
+  int rh = ensure_host_tree(scene);
+  if (rh) return rh;
   auto p = std::make_unique<sq_proposal>();
   p->ctx = ctx; p->scene = scene; p->cut = cut_depth;
   // Subtrees of the reference tree cut at `cut_depth`: nodes at that depth,

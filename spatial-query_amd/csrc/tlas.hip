@@ -243,6 +243,8 @@ int sq_tlas_build(sq_ctx* ctx, int dim, int kind, uint64_t n, const float* rot, 
     for (uint32_t m = 0; m < n_meshes; ++m) {
       if (!meshes[m] || meshes[m]->kind != KIND_TRI || meshes[m]->n_prims == 0 || meshes[m]->ctx != ctx)
         return fail(SQ_ERR_INVALID, "each mesh must be a non-empty TRIANGLE bvh on this context");
+      int rm = ensure_host_tree(meshes[m]);             // the mesh root box (local_aabb)
+      if (rm) return rm;
     }
     in.mesh_of.assign(mesh_of, mesh_of + n);
     in.scale.assign(scale, scale + n);

@@ -188,6 +188,32 @@ class DeviceBvh:
         finally:
             t.close()
 
+    @classmethod
+    def build_device(cls, ctx: Context, dim: int, kind: int, prims_dev) -> "DeviceBvh":
+        """Bvh::build from primitives already on the GPU (a contiguous float32
+        CUDA tensor in the kind's layout), built on the device: the same tree
+        as build() node for node."""
+        n = prims_dev.numel() // prim_stride(dim, kind)
+        obj = cls.__new__(cls)
+        h = C.c_void_p()
+        check(lib().sq_bvh_build_device(ctx.handle, dim, kind, _dp(prims_dev) if n else None, n, C.byref(h)))
+        obj._h, obj.ctx, obj.dim, obj.kind, obj.n_prims = h, ctx, dim, kind, n
+        return obj
+
+    def export(self):
+        """(bounds, child_a, child_b, prim_count, prim_indices) of the device tree
+        in the reference layout (Bvh::nodes / prim_indices)."""
+        nn = self.stats()["node_count"]
+        bounds = np.empty((nn, 2 * self.dim), np.float32)
+        ca = np.empty(nn, np.uint32); cb = np.empty(nn, np.uint32); pc = np.empty(nn, np.uint32)
+        idx = np.empty(self.n_prims, np.uint32)
+        check(lib().sq_bvh_export(self._h, _hp(bounds), _hp(ca), _hp(cb), _hp(pc), _hp(idx)))
+        return bounds, ca, cb, pc, idx
+
+    def rebuild(self, prims_dev):
+        """Build the tree again, on the device, from moved primitives (same count)."""
+        check(lib().sq_bvh_rebuild(self._h, _dp(prims_dev)))
+
     # -- management ---------------------------------------------------------
     def set_layers(self, layers):
         l = None if layers is None else _host(layers, np.uint64)

@@ -2,6 +2,7 @@
 
     python -m sq_amd.benches c2 [--frames 60] [--batches 256] [--no-tune]
     python -m sq_amd.benches c4
+    python -m sq_amd.benches build
 
 c2: 16 noise-displaced icospheres (2k-50k triangles, log-uniform) placed as
 4,096 instances (uniform rotations, translations U[-1000,1000]^3, scales
@@ -10,7 +11,9 @@ frame refits the boxes on the device and answers 256 nearest-hit batches
 (sizes log-uniform in 1k-1M rays, origins U[-1500,1500]^3, uniform directions,
 t_max 4000; even batches on the instances, odd ones on the boxes) with two
 persistent batch launches, the whole frame recorded once into a graph and
-replayed.  The tuner picks block size, pool size and stack depth first.
+replayed.  The tuner picks block size, pool size and stack depth first.  A
+second series rebuilds the boxes on the device every frame instead of
+refitting (sq_bvh_rebuild) and replays a graph of the two submits.
 
 c4: 1,000,000 boxes in D = 6 from a 32-component Gaussian mixture; 1M
 nearest-hit rays, 262,144 overlap queries (sorted id lists) and 262,144 box
@@ -182,6 +185,53 @@ def c2(frames=60, n_batches=256, tune=True, check=True, seed=0xC2C2_0000_000A):
     out["refit_ms"] = round(e0.elapsed_time(e1) / 10, 4)
     log(f"60 frames: median {fm:.2f} ms ({out['mrays_per_s']} Mrays/s; refit-only trees loosen as the boxes "
         f"drift), refit {out['refit_ms']} ms")
+
+    # the same frames with the boxes rebuilt on the device every frame (the
+    # tree keeps its arrays, so the batch set and the graph of the two
+    # submits stay valid): frame = device rebuild + graph replay
+    prims_dev.copy_(torch.from_numpy(boxes))
+    dbvh = sq_amd.DeviceBvh.build_device(ctx, 3, sq_amd.AABB, prims_dev)
+    bb2 = sq_amd.BatchSet(ctx, [(dbvh, r, tm) for (_, r, tm) in qb])
+    bt.submit()
+    bb2.submit()
+    torch.cuda.synchronize()
+    g2 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g2, stream=stream):
+        bt.submit()
+        bb2.submit()
+    rms, rb_ms, parity2 = [], [], []
+    for f in range(frames):
+        moved = boxes.copy()
+        d = vel * np.float32(f)
+        moved[:, :3] += d
+        moved[:, 3:] += d
+        prims_dev.copy_(torch.from_numpy(moved))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dbvh.rebuild(prims_dev)
+        t1 = time.perf_counter()
+        with torch.cuda.stream(stream):
+            g2.replay()
+        stream.synchronize()
+        t2 = time.perf_counter()
+        rb_ms.append((t1 - t0) * 1e3)
+        rms.append((t2 - t0) * 1e3)
+        if check and f in (0, frames - 1):
+            ok = True
+            for i, (_, r, tm) in enumerate(qb):
+                w = dbvh.raycast_nearest(r, tm)
+                ctx.synchronize()
+                ok &= _same(bb2.leaf[i].cpu(), w["leaf"].cpu()) and _same_bits(bb2.toi[i].cpu(), w["toi"].cpu())
+            parity2.append(dict(frame=f, batches_equal_single_calls=bool(ok)))
+            log(f"rebuild frame {f}: batches == single calls {ok}")
+    rm = float(np.median(rms))
+    out["rebuild_frames"] = dict(frame_ms_median=round(rm, 3), frame_ms_first=round(rms[0], 3),
+                                 frame_ms_last=round(rms[-1], 3), rebuild_ms_median=round(float(np.median(rb_ms)), 3),
+                                 mrays_per_s=round(float(sizes.sum()) / (rm * 1e-3) / 1e6, 1),
+                                 frame_ms=[round(v, 2) for v in rms], parity=parity2)
+    log(f"{frames} rebuild frames: median {rm:.2f} ms (rebuild {np.median(rb_ms):.2f} ms), first {rms[0]:.2f}, "
+        f"last {rms[-1]:.2f}")
+    bb2.close()
     return out
 
 
@@ -238,9 +288,50 @@ def c4():
     return out
 
 
+def build():
+    """Device build (sq_bvh_build_device) against the host build + upload, and
+    the device rebuild, on the scenes of the other workloads."""
+    import torch
+    import sq_amd
+    from sq_amd import workloads as W
+
+    ctx = sq_amd.Context(0)
+    out = dict(workload="builds: C1 heightfield 2,097,152 triangles; C4 1,000,000 D=6 boxes; C2 100,000 moving boxes")
+    scenes = [("c1_heightfield", 3, sq_amd.TRIANGLE, W.heightfield_scene().prims),
+              ("c4_gmm_boxes_d6", 6, sq_amd.AABB, W.gmm_boxes_scene().prims),
+              ("c2_moving_boxes", 3, sq_amd.AABB,
+               W.boxes_scene(n=100_000, seed=7, lo=-1000.0, hi=1000.0, hmin=0.5, hmax=5.0).prims)]
+    for name, dim, kind, prims in scenes:
+        pd = _dev(prims)
+        torch.cuda.synchronize()
+        b = sq_amd.DeviceBvh.build_device(ctx, dim, kind, pd)      # warm-up (allocations, code load)
+        ts = []
+        for _ in range(5):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            b.rebuild(pd)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        t0 = time.perf_counter()
+        fresh = sq_amd.DeviceBvh.build_device(ctx, dim, kind, pd)
+        t_fresh = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        hb = sq_amd.DeviceBvh.build(ctx, dim, kind, prims)
+        t_host = (time.perf_counter() - t0) * 1e3
+        st = fresh.stats()
+        same = all(np.array_equal(x.view(np.uint32), y.view(np.uint32)) for x, y in zip(hb.export(), fresh.export()))
+        out[name] = dict(prims=int(prims.shape[0]), nodes=st["node_count"], depth=st["max_depth"],
+                         rebuild_ms=round(float(np.median(ts)), 2), device_build_ms=round(t_fresh, 2),
+                         host_build_upload_ms=round(t_host, 1), speedup=round(t_host / float(np.median(ts)), 1),
+                         same_tree_as_host=bool(same))
+        log(f"{name}: rebuild {np.median(ts):.2f} ms, fresh device build {t_fresh:.2f} ms, host build + upload "
+            f"{t_host:.1f} ms, same tree {same}")
+        del b, fresh, hb
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["c2", "c4"])
+    ap.add_argument("which", choices=["c2", "c4", "build"])
     ap.add_argument("--frames", type=int, default=60)
     ap.add_argument("--batches", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
@@ -248,8 +339,10 @@ def main():
     a = ap.parse_args()
     if a.which == "c2":
         res = c2(frames=a.frames, n_batches=a.batches, tune=not a.no_tune, check=not a.no_check)
-    else:
+    elif a.which == "c4":
         res = c4()
+    else:
+        res = build()
     print(json.dumps(res), flush=True)
 
 

@@ -91,7 +91,10 @@ SIGNATURES = {
     "sq_bvh_set_layers": [P, P],
     "sq_bvh_get_stats": [P, C.POINTER(TreeStats)],
     "sq_bvh_download_bounds": [P, P],
+    "sq_bvh_export": [P, P, P, P, P, P],
     "sq_bvh_refit": [P, P],
+    "sq_bvh_build_device": [P, I32, I32, P, U64, PP],
+    "sq_bvh_rebuild": [P, P],
     "sq_raycast_nearest": [P, P, U64, F32, P, C.POINTER(Filter), P, P, P, P, P],
     "sq_raycast_any": [P, P, U64, F32, P, C.POINTER(Filter), P],
     "sq_raycast_all": [P, P, U64, F32, C.POINTER(Filter), I32, P, P, P, U64, C.POINTER(C.c_uint64)],
