@@ -151,6 +151,34 @@ int sq_bvh_build_device(sq_ctx* ctx, int dim, int kind, const float* prims_devic
  * made, so make it again if the rebuilt tree is deeper (stats max_depth). */
 int sq_bvh_rebuild(sq_bvh* bvh, const float* prims_device);
 
+/* ------------------------------------------------------------------------
+ * Consumer narrow phases: GpuGeometry::narrow_wgsl (gpu/mod.rs:86-99,261),
+ * where the reference splices a consumer's WGSL narrow test into its traversal
+ * shader.  `source` is HIP C++ defining
+ *   __device__ bool sq_narrow(const float* prim, const float* origin,
+ *                             const float* dir, float max_toi, float* toi);
+ * (QueryGeometry::test_ray, geometry.rs:33-43: true with *toi in [0, max_toi]
+ * on a hit), compiled at run time (hipRTC, gfx950) together with the engine's
+ * nearest-hit walk and its flags (no contraction, f32 denormals; plain `/` and
+ * __builtin_sqrtf are correctly rounded).  No <cstring> or <cmath> there:
+ * use builtins.  SQ_ERR_INVALID with the compiler log in sq_last_error() when
+ * it does not compile.  A hit counts only when its toi lies in the slab
+ * interval of the primitive's own world box (the engine's order-independence
+ * guard: a consumer whose hits lie in its boxes loses none). */
+typedef struct sq_narrow sq_narrow;
+int sq_narrow_create(sq_ctx* ctx, int dim, const char* source, uint32_t prim_floats, sq_narrow** out);
+int sq_narrow_destroy(sq_narrow* narrow);
+/* The tree over n consumer primitives (host memory): world boxes [n][2*dim]
+ * (QueryGeometry::world_aabb) drive the reference median split; prims
+ * [n][prim_floats] are what sq_narrow sees.  The tree answers only
+ * sq_raycast_nearest_consumer with its narrow phase. */
+int sq_bvh_build_consumer(sq_narrow* narrow, const float* boxes, const float* prims, uint64_t n, sq_bvh** out);
+/* Nearest hit per ray through the consumer's narrow phase (raycast_nearest_batch,
+ * bvh.rs:134-193, with the consumer's test_ray); device buffers as
+ * sq_raycast_nearest: rays [n][2*dim], leaf (-1 on a miss), toi (+inf). */
+int sq_raycast_nearest_consumer(sq_narrow* narrow, sq_bvh* bvh, const float* rays, uint64_t n, float max_toi,
+                                int32_t* out_leaf, float* out_toi);
+
 /* ---- ray queries (bvh.rs:51-328) ------------------------------------------ */
 /* max_toi_per_ray (device, n floats) overrides max_toi when not NULL.
  * filter may be NULL (default filter). */

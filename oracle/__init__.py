@@ -22,6 +22,7 @@ _LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
 # Provider kinds (sq_oracle.cpp `enum Kind`).
 BOX, SPHERE, TRI = 0, 1, 2
 SPHERE_PLAIN, SPHERE_PLAIN_NOEXIT, TRI_PLAIN, SHELL = 10, 11, 12, 13
+KIND_CONSUMER = 20
 INST_BOX, INST_MESH, INST_BOX_GUARDED = 0, 1, 2
 ALL_LAYERS = (1 << 64) - 1
 
@@ -55,6 +56,7 @@ def _declare(L):
         "orc_rng_u64": (None, [U64, U64, P]),
         "orc_normalize": (None, [I32, P, U64]),
         "orc_geom_new": (P, [I32, I32, P, U32]),
+        "orc_geom_new_consumer": (P, [I32, P, U32, I32, U64]),
         "orc_geom_set_layers": (None, [P, P]),
         "orc_geom_set_prims": (None, [P, P]),
         "orc_geom_world_aabbs": (None, [P, P]),
@@ -140,6 +142,23 @@ class Scene:
         if layers is not None:
             self.set_layers(layers)
         self._b = lib().orc_bvh_build(self._g) if build_tree else None
+
+    @classmethod
+    def consumer(cls, dim: int, boxes: np.ndarray, prims: np.ndarray, narrow_addr: int) -> "Scene":
+        """A consumer provider: world boxes [n][2*dim] and consumer floats
+        [n][pf], tested by the C function at narrow_addr (signature
+        int(const float* prim, const float* origin, const float* dir, float
+        max_toi, float* toi)), box-guarded like the built-in SPHERE and TRI."""
+        boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 2 * dim)
+        prims = np.ascontiguousarray(prims, dtype=np.float32).reshape(boxes.shape[0], -1)
+        obj = cls.__new__(cls)
+        obj.dim, obj.kind = dim, KIND_CONSUMER
+        obj.prims = np.ascontiguousarray(np.concatenate([boxes, prims], 1), dtype=np.float32)
+        obj.n = obj.prims.shape[0]
+        obj._g = lib().orc_geom_new_consumer(dim, _p(obj.prims), obj.n, prims.shape[1], int(narrow_addr))
+        obj.layers = None
+        obj._b = lib().orc_bvh_build(obj._g)
+        return obj
 
     def set_layers(self, layers):
         self.layers = None if layers is None else np.ascontiguousarray(layers, dtype=np.uint64)

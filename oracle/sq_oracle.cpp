@@ -261,14 +261,23 @@ enum Kind {
   KIND_SPHERE_PLAIN_NOEXIT = 11,  // conformance.rs:277-291 / lib.rs doc example
   KIND_TRI_PLAIN = 12,       // viewport-lib-query MT + max_toi reject, unguarded
   KIND_SHELL = 13,           // crossings.rs:37-111: both sphere roots are crossings
+  KIND_CONSUMER = 20,        // a consumer's test_ray (geometry.rs:33-43) through a C function, box-guarded
 };
+
+// A consumer narrow phase: true with *toi on a hit of the primitive's floats.
+typedef int (*ConsumerNarrow)(const float* prim, const float* origin, const float* dir, float max_toi, float* toi);
 
 template <int D> struct Geometry {
   int kind;
   uint32_t n;
   std::vector<float> prims;      // BOX: min D, max D; SPHERE: center D, radius; TRI: v0 v1 v2
   std::vector<uint64_t> layers;  // empty = every leaf on every layer
-  int stride() const { return kind == KIND_BOX ? 2 * D : (kind == KIND_TRI || kind == KIND_TRI_PLAIN) ? 9 : D + 1; }
+  ConsumerNarrow consumer = nullptr;   // KIND_CONSUMER: records are [world box 2D][consumer_pf floats]
+  int consumer_pf = 0;
+  int stride() const {
+    if (kind == KIND_CONSUMER) return 2 * D + consumer_pf;
+    return kind == KIND_BOX ? 2 * D : (kind == KIND_TRI || kind == KIND_TRI_PLAIN) ? 9 : D + 1;
+  }
   bool is_ball() const { return kind == KIND_SPHERE || kind == KIND_SPHERE_PLAIN || kind == KIND_SPHERE_PLAIN_NOEXIT || kind == KIND_SHELL; }
   const float* p(size_t leaf) const { return prims.data() + leaf * stride(); }
 
@@ -278,7 +287,7 @@ template <int D> struct Geometry {
   Aabb<D> world_aabb(size_t leaf) const {
     const float* q = p(leaf);
     Aabb<D> b;
-    if (kind == KIND_BOX) {
+    if (kind == KIND_BOX || kind == KIND_CONSUMER) {
       for (int i = 0; i < D; ++i) { b.min.v[i] = q[i]; b.max.v[i] = q[D + i]; }
     } else if (kind == KIND_TRI || kind == KIND_TRI_PLAIN) {
       // The AABB of the three vertices, folded with union_point from EMPTY.
@@ -335,6 +344,16 @@ template <int D> struct Geometry {
 
   bool test_ray(size_t leaf, const Ray<D>& ray, float max_toi, LeafHit<D>* out) const {
     const float* q = p(leaf);
+    if (kind == KIND_CONSUMER) {
+      float t;
+      if (!consumer(q + 2 * D, ray.origin.v, ray.dir.v, max_toi, &t)) return false;
+      if (!(t >= 0.0f && t <= max_toi)) return false;   // the test_ray contract, geometry.rs:37-39
+      if (!guard(leaf, ray, max_toi, t)) return false;
+      out->toi = t;
+      out->normal = Point<D>::zero();
+      out->sub = -1;
+      return true;
+    }
     if (kind == KIND_SHELL) {   // crossings.rs:76-86
       Point<D> c; for (int i = 0; i < D; ++i) c.v[i] = q[i];
       float t0, t1;
@@ -1127,6 +1146,19 @@ void* orc_geom_new(int D, int kind, const float* prims, uint32_t n) {
     Geometry<DD>* p = new Geometry<DD>();
     p->kind = kind; p->n = n;
     p->prims.assign(prims, prims + (size_t)n * p->stride());
+    g->p = p;
+  });
+  return g;
+}
+// A consumer geometry: records [n][2D + pf] (world box, then the consumer's
+// floats) tested by `fn` (an address, e.g. from a shared object the caller built).
+void* orc_geom_new_consumer(int D, const float* records, uint32_t n, int pf, uint64_t fn) {
+  OGeom* g = new OGeom{D, nullptr};
+  ORC_DISPATCH(D, {
+    Geometry<DD>* p = new Geometry<DD>();
+    p->kind = KIND_CONSUMER; p->n = n; p->consumer_pf = pf;
+    p->consumer = reinterpret_cast<ConsumerNarrow>(fn);
+    p->prims.assign(records, records + (size_t)n * p->stride());
     g->p = p;
   });
   return g;

@@ -806,14 +806,18 @@ int sq_ctx_last_kernel_ms(sq_ctx* c, float* ms) {
 struct sq_tree {
   sqh::RefTree t;
 };
+}  // extern "C"
 
-static void tree_stats_of(const sqh::RefTree& t, sq_tree_stats& st) {
+void tree_stats_of(const sqh::RefTree& t, sq_tree_stats& st) {
   st.node_count = t.node_count();
   sqh::tree_structural(t, &st.leaf_count, &st.prim_count, &st.max_depth);
   // size_of::<Node<D>>() = 8D + 12 bytes, plus u32 prim indices
   st.bytes = st.node_count * (uint64_t)(8 * t.dim + 12) + t.prim_indices.size() * 4;
   st.quality = sqh::tree_interior_surface_sum(t);
 }
+
+extern "C" {
+
 
 static int check_dim_kind(int dim, int kind) {
   return dispatch_dk(dim, kind, [&]<int D, int K>() { return SQ_OK; });

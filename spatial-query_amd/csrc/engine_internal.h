@@ -97,7 +97,11 @@ struct sq_bvh {
   bool host_tree_ok = true;
   uint64_t ref_nodes = 0;
   DevBuf ref_ca, ref_cb, ref_pc;
+  uint32_t consumer_pf = 0;        // trees over consumer primitives (rtc.hip): floats per primitive
 };
+
+// TreeStats of a host reference tree (tree.rs:81-110, bvh.rs:477-489).
+void tree_stats_of(const sqh::RefTree& t, sq_tree_stats& st);
 
 // Downloads the reference tree of a device-built tree into b->tree (and the
 // host arrays that depend on it); no-op for trees built on the host.

@@ -21,10 +21,16 @@
 #pragma once
 
 #include <cstdint>
+#if !defined(__HIPCC_RTC__)
+// (hipRTC, which compiles this header into consumer narrow-phase kernels,
+// provides the HIP runtime itself and no <cstring>)
 #include <cstring>
+#endif
 
-#if defined(__HIPCC__)
+#if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+#if !defined(__HIPCC_RTC__)
 #include <hip/hip_runtime.h>
+#endif
 #define SQ_HD __host__ __device__ __forceinline__
 // IEEE division and square root (correctly rounded): plain `/` and
 // __builtin_sqrtf lower to the correctly rounded sequences under HIP's default

@@ -430,6 +430,76 @@ class Tlas:
             pass
 
 
+class Narrow:
+    """A consumer narrow phase (GpuGeometry::narrow_wgsl, gpu/mod.rs:86-99,261):
+    HIP source defining
+
+        __device__ bool sq_narrow(const float* prim, const float* origin,
+                                  const float* dir, float max_toi, float* toi);
+
+    compiled at run time (hipRTC) into the engine's nearest-hit walk, for
+    primitives of prim_floats floats in dim dimensions (the source may use
+    SQ_RTC_D and SQ_RTC_PF).  Raises SqError(INVALID) with the compiler log
+    when it does not compile."""
+
+    def __init__(self, ctx: Context, dim: int, source: str, prim_floats: int):
+        h = C.c_void_p()
+        check(lib().sq_narrow_create(ctx.handle, dim, source.encode(), prim_floats, C.byref(h)))
+        self._h, self.ctx, self.dim, self.prim_floats = h, ctx, dim, prim_floats
+
+    def build(self, boxes, prims) -> "ConsumerBvh":
+        """The tree over n primitives: world boxes [n][2*dim] (host) drive the
+        reference median split; prims [n][prim_floats] are what sq_narrow sees."""
+        bx = _host(boxes).reshape(-1, 2 * self.dim)
+        pr = _host(prims).reshape(bx.shape[0], self.prim_floats) if self.prim_floats else np.zeros((bx.shape[0], 0), np.float32)
+        h = C.c_void_p()
+        check(lib().sq_bvh_build_consumer(self._h, _hp(bx), _hp(pr), bx.shape[0], C.byref(h)))
+        return ConsumerBvh(self, h, bx.shape[0])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_narrow_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ConsumerBvh:
+    """A tree over consumer primitives; answers nearest-hit rays through its
+    Narrow (raycast_nearest_batch with the consumer's test_ray)."""
+
+    def __init__(self, narrow: Narrow, handle, n: int):
+        self.narrow, self._h, self.n_prims, self.dim = narrow, handle, n, narrow.dim
+
+    def stats(self) -> dict:
+        s = TreeStats()
+        check(lib().sq_bvh_get_stats(self._h, C.byref(s)))
+        return _stats(s)
+
+    def raycast_nearest(self, rays, max_toi, out=None):
+        import torch
+        n = rays.shape[0]
+        leaf = torch.empty(n, dtype=torch.int32, device=rays.device)
+        toi = torch.empty(n, dtype=torch.float32, device=rays.device)
+        check(lib().sq_raycast_nearest_consumer(self.narrow._h, self._h, _dp(rays), n, float(max_toi), _dp(leaf), _dp(toi)))
+        return {"leaf": leaf, "toi": toi}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_bvh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class BatchSet:
     """Many nearest-hit batches over many scenes (all DeviceBvh, or all Tlas, of one
     dimension and kind) answered by one persistent work-stealing launch per submit().

@@ -95,6 +95,10 @@ SIGNATURES = {
     "sq_bvh_refit": [P, P],
     "sq_bvh_build_device": [P, I32, I32, P, U64, PP],
     "sq_bvh_rebuild": [P, P],
+    "sq_narrow_create": [P, I32, C.c_char_p, U32, PP],
+    "sq_narrow_destroy": [P],
+    "sq_bvh_build_consumer": [P, P, P, U64, PP],
+    "sq_raycast_nearest_consumer": [P, P, P, U64, F32, P, P],
     "sq_raycast_nearest": [P, P, U64, F32, P, C.POINTER(Filter), P, P, P, P, P],
     "sq_raycast_any": [P, P, U64, F32, P, C.POINTER(Filter), P],
     "sq_raycast_all": [P, P, U64, F32, C.POINTER(Filter), I32, P, P, P, U64, C.POINTER(C.c_uint64)],

@@ -1,0 +1,111 @@
+"""Consumer narrow phases compiled at run time (sq_narrow_create, hipRTC) into
+the nearest-hit walk, against the CPU oracle running the same source.
+
+  * the built-in ball as a consumer (D = 3 and 5) equals the engine's built-in
+    SPHERE kind bit for bit, and the oracle;
+  * a consumer ellipsoid equals the oracle's consumer kind (reference LIFO
+    walk, same source built by g++) bit for bit;
+  * a source that does not compile reports the compiler's log; a consumer tree
+    refuses the built-in queries; an empty tree answers misses.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sq_amd
+from sq_amd import workloads as W
+
+from consumer_narrow import BALL, ELLIPSOID, compile_cpu
+from helpers import assert_nearest_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def balls(n, dim, seed, half):
+    s = W.Stream(seed)
+    c = s.range(-half, half, n * dim).reshape(n, dim)
+    r = s.range(0.3, 3.0, n).reshape(n, 1)
+    return np.concatenate([c, r], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("dim,half", [(3, 40.0), (5, 10.0)])
+def test_consumer_ball_equals_builtin_sphere(ctx, tmp_path, dim, half):
+    print(f"case: consumer ball D={dim}", flush=True)
+    b = balls(20000, dim, 0xD0 + dim, half)
+    boxes = np.concatenate([b[:, :dim] - b[:, dim:], b[:, :dim] + b[:, dim:]], 1).astype(np.float32)
+    rays = W.random_rays(20000, 0xD1 + dim, -half - 5.0, half + 5.0, dim=dim)
+    nw = sq_amd.Narrow(ctx, dim, BALL, dim + 1)
+    cb = nw.build(boxes, b)
+    got = cb.raycast_nearest(dev(rays), 300.0)
+    builtin = sq_amd.DeviceBvh.build(ctx, dim, sq_amd.SPHERE, b).raycast_nearest(dev(rays), 300.0)
+    assert np.array_equal(host(got["leaf"]), host(builtin["leaf"]))
+    assert np.array_equal(host(got["toi"]).view(np.uint32), host(builtin["toi"]).view(np.uint32))
+    lib, addr = compile_cpu(BALL, dim, dim + 1, str(tmp_path))
+    ref = oracle.Scene.consumer(dim, boxes, b, addr).raycast_nearest(rays, 300.0, threads=8)
+    assert (ref["leaf"] >= 0).sum() > 1000
+    assert_nearest_equal(host(got["leaf"]), host(got["toi"]), ref["leaf"], ref["t"], f"consumer ball D={dim}")
+    cb.close()
+    nw.close()
+
+
+def test_consumer_ellipsoid_matches_oracle(ctx, tmp_path):
+    print("case: consumer ellipsoid", flush=True)
+    s = W.Stream(0xD5)
+    c = s.range(-60.0, 60.0, 30000 * 3).reshape(-1, 3)
+    a = s.range(0.2, 3.0, 30000 * 3).reshape(-1, 3)
+    prims = np.concatenate([c, a], 1).astype(np.float32)
+    boxes = np.concatenate([c - a, c + a], 1).astype(np.float32)
+    rays = np.concatenate([W.random_rays(30000, 0xD6, -70.0, 70.0),
+                           W.random_rays(2000, 0xD7, -5.0, 5.0)])
+    nw = sq_amd.Narrow(ctx, 3, ELLIPSOID, 6)
+    cb = nw.build(boxes, prims)
+    got = cb.raycast_nearest(dev(rays), 500.0)
+    lib, addr = compile_cpu(ELLIPSOID, 3, 6, str(tmp_path))
+    ref = oracle.Scene.consumer(3, boxes, prims, addr).raycast_nearest(rays, 500.0, threads=8)
+    assert (ref["leaf"] >= 0).sum() > 3000
+    assert_nearest_equal(host(got["leaf"]), host(got["toi"]), ref["leaf"], ref["t"], "consumer ellipsoid")
+    # the tree is the reference median split over the consumer's boxes
+    st = cb.stats()
+    ts = sq_amd.Tree.build(3, sq_amd.AABB, boxes).stats()
+    for k in ("node_count", "leaf_count", "max_depth"):
+        assert st[k] == ts[k]
+    cb.close()
+    nw.close()
+
+
+def test_consumer_errors_and_empty_tree(ctx):
+    print("case: consumer compile error, refusals, empty tree", flush=True)
+    with pytest.raises(sq_amd.SqError) as e:
+        sq_amd.Narrow(ctx, 3, "__device__ bool sq_narrow(const float* p) { return undefined_name; }", 4)
+    assert "error" in str(e.value)
+    nw = sq_amd.Narrow(ctx, 3, BALL, 4)
+    empty = nw.build(np.zeros((0, 6), np.float32), np.zeros((0, 4), np.float32))
+    r = empty.raycast_nearest(dev(W.random_rays(100, 1, -5.0, 5.0)), 100.0)
+    assert (host(r["leaf"]) == -1).all() and np.isinf(host(r["toi"])).all()
+    # a consumer tree answers only through its narrow phase
+    b = balls(50, 3, 0xD8, 10.0)
+    boxes = np.concatenate([b[:, :3] - b[:, 3:], b[:, :3] + b[:, 3:]], 1).astype(np.float32)
+    cb = nw.build(boxes, b)
+    bogus = sq_amd.DeviceBvh.__new__(sq_amd.DeviceBvh)
+    bogus._h, bogus.ctx, bogus.dim, bogus.kind, bogus.n_prims = cb._h, ctx, 3, sq_amd.AABB, 50
+    with pytest.raises(sq_amd.SqError):
+        bogus.raycast_nearest(dev(W.random_rays(10, 2, -5.0, 5.0)), 100.0)
+    bogus._h = None
+    other = sq_amd.Narrow(ctx, 3, ELLIPSOID, 6)
+    wrong = sq_amd.ConsumerBvh(other, cb._h, 50)        # the tree under another narrow phase
+    with pytest.raises(sq_amd.SqError):
+        wrong.raycast_nearest(dev(W.random_rays(10, 3, -5.0, 5.0)), 100.0)
+    wrong._h = None
+    cb.close()
+    empty.close()
+    other.close()
+    nw.close()
