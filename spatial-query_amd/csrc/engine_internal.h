@@ -155,14 +155,6 @@ struct TlasInfo {
 };
 void tlas_info(const sq_tlas* t, TlasInfo* out);
 
-struct NearestOut {
-  int32_t* leaf;
-  float* toi;
-  float* point;
-  float* normal;
-  int32_t* sub;
-  uint8_t* any;
-};
 
 inline uint32_t grid_for(uint64_t n, int block) { return (uint32_t)((n + block - 1) / block); }
 

@@ -1,10 +1,14 @@
-// pool.h -- the pooled walk shared by the single-scene kernels (engine.hip)
-// and the persistent batch kernel (batch.hip).  Device code only.
+// pool.h -- the pooled walk shared by the single-scene kernels (engine.hip),
+// the persistent batch kernel (batch.hip) and consumer narrow-phase kernels
+// compiled at run time (rtc.hip).  Device code only; needs nothing beyond
+// traverse.h, so that hipRTC can compile it too.
 #pragma once
 
 #include <type_traits>
 
-#include "engine_internal.h"
+#include "traverse.h"
+
+namespace sqd {
 
 // Pause a wave's descents once fewer than `below` lanes still descend while
 // other lanes of the wave hold queries (at a leaf, or finished and waiting for a
@@ -64,14 +68,15 @@ struct Chunks {
 // when fewer than pause_below lanes are still descending (WavePause).  The stack is
 // the caller's LDS ring spilling to a global column.  With `order`, pool
 // position i is query order[i] (rays read and results written there).
-template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Src>
-__device__ __forceinline__ void pool_walk(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
-                                          const float* __restrict__ tmax, const Filter& f, const NearestOut& out,
-                                          const RingStack& st, const uint32_t* __restrict__ order, uint32_t pause_below) {
+// The walk for any query policy Pol (setup / root / pair / narrow over
+// primitive records of PS floats, queries of QS floats).  KIND picks what the
+// optional point / normal / sub outputs mean (the built-in kinds); CAST marks
+// shape casts (no normal).
+template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Pol, int QS, int PS, class Src>
+__device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
+                                               const float* __restrict__ tmax, const Filter& f, const NearestOut& out,
+                                               const RingStack& st, const uint32_t* __restrict__ order, uint32_t pause_below) {
   const uint32_t lane = threadIdx.x & 63u;
-  using Pol = typename std::conditional<CAST, CastPolicy<D, KIND>, RayPolicy<D, KIND>>::type;
-  constexpr int QS = CAST ? 4 * D : 2 * D;
-  constexpr int PS = Prim<D, KIND>::STRIDE;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   Pol pol;
   WalkState w;
@@ -141,6 +146,16 @@ __device__ __forceinline__ void pool_walk(const SceneView& s, const float* __res
     }
     rid = NO_LEAF;
   }
+}
+
+// The built-in kinds: rays (RayPolicy) or shape casts (CastPolicy).
+template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Src>
+__device__ __forceinline__ void pool_walk(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
+                                          const float* __restrict__ tmax, const Filter& f, const NearestOut& out,
+                                          const RingStack& st, const uint32_t* __restrict__ order, uint32_t pause_below) {
+  using Pol = typename std::conditional<CAST, CastPolicy<D, KIND>, RayPolicy<D, KIND>>::type;
+  pool_walk_with<D, KIND, ANY, FILT, CAST, Pol, CAST ? 4 * D : 2 * D, Prim<D, KIND>::STRIDE>(s, q, src, max_toi, tmax, f, out,
+                                                                                           st, order, pause_below);
 }
 
 // ---------------------------------------------------------------------------
@@ -384,3 +399,5 @@ __device__ __forceinline__ void pool_collect(const SceneView& s, const float* __
   }
   if (lane == 0) app.close();
 }
+
+}  // namespace sqd

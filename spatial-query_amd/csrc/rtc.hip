@@ -2,8 +2,8 @@
 // nearest-hit walk.  The device analogue of GpuGeometry::narrow_wgsl
 // (crates/spatial-query/src/gpu/mod.rs:86-99,261), which splices a consumer's
 // WGSL narrow test into the reference's traversal shader: here the consumer's
-// HIP source is compiled together with traverse.h (embedded in the library)
-// and a one-query-per-thread nearest kernel, for gfx950.
+// HIP source is compiled together with traverse.h and pool.h (embedded in
+// the library) and the persistent pooled nearest-hit kernel, for gfx950.
 //
 // The consumer defines
 //   __device__ bool sq_narrow(const float* prim, const float* origin,
@@ -25,7 +25,7 @@
 #include <vector>
 
 #include "engine_internal.h"
-#include "rtc_traverse.inc"   // kTraverseSrc: traverse.h as a string literal (built by the Makefile)
+#include "rtc_sources.inc"    // kTraverseSrc, kPoolSrc: traverse.h and pool.h as string literals (built by the Makefile)
 
 namespace {
 
@@ -56,21 +56,25 @@ template <int D> struct ConsumerPolicy {
 };
 }  // namespace sqd
 
+// Persistent pooled walk (as the built-in kinds' k_nearest_pool): waves take
+// chunks of `chunk` rays from *counter (zeroed by the host), refill lanes as
+// rays finish, pause descents below pause_below lanes.
 extern "C" __global__ __launch_bounds__(256) void sq_consumer_nearest(sqd::SceneView s, const float* rays, unsigned n,
                                                                       float max_toi, int* out_leaf, float* out_toi,
-                                                                      unsigned stack_n) {
+                                                                      unsigned chunk, unsigned* counter, unsigned S,
+                                                                      sqd::SpillEntry* spill, unsigned spill_stride,
+                                                                      unsigned pause_below) {
   extern __shared__ unsigned lds[];
-  const unsigned B = blockDim.x, tid = threadIdx.x, i = blockIdx.x * B + tid;
-  if (i >= n) return;
-  sqd::ConsumerPolicy<SQ_RTC_D> pol;
-  pol.setup(rays + (size_t)i * 2 * SQ_RTC_D);
-  sqd::Stack st{lds + tid, reinterpret_cast<float*>(lds + stack_n * B) + tid, B};
+  const unsigned B = blockDim.x, tid = threadIdx.x;
+  const unsigned gtid = blockIdx.x * B + tid;
+  sqd::RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
+  const unsigned long long waves = (unsigned long long)gridDim.x * (B >> 6);
+  const unsigned long long home = (unsigned long long)blockIdx.x * (B >> 6) + (tid >> 6);
+  const sqd::NearestOut out{out_leaf, out_toi, nullptr, nullptr, nullptr, nullptr};
   const sqd::Filter f{nullptr, ~0ull, -1};
-  const sqd::NearestResult h =
-      sqd::walk_nearest<SQ_RTC_D, 2 * SQ_RTC_D + SQ_RTC_PF, false, false>(s, pol, max_toi, f, st);
-  const bool hit = h.leaf != sqd::NO_LEAF;
-  out_leaf[i] = hit ? (int)h.leaf : -1;
-  out_toi[i] = h.t;
+  sqd::pool_walk_with<SQ_RTC_D, sqd::KIND_AABB, false, false, false, sqd::ConsumerPolicy<SQ_RTC_D>, 2 * SQ_RTC_D,
+                      2 * SQ_RTC_D + SQ_RTC_PF>(s, rays, sqd::Chunks{counter, n, chunk, home, waves}, max_toi, nullptr, f,
+                                                out, st, nullptr, pause_below);
 }
 )SQRTC";
 
@@ -92,9 +96,11 @@ int sq_narrow_create(sq_ctx* ctx, int dim, const char* source, uint32_t prim_flo
   if (dim < 2 || dim > 6) return fail(SQ_ERR_UNSUPPORTED, "consumer narrow phases take dim 2..6");
   if (prim_floats > 256) return fail(SQ_ERR_INVALID, "at most 256 floats per primitive");
   HIP_TRY(hipSetDevice(ctx->device));
-  const std::string src = std::string(kTraverseSrc) + "\n#line 1 \"consumer\"\n" + source + "\n" + kKernelSrc;
+  const std::string src = std::string("#include \"pool.h\"\n#line 1 \"consumer\"\n") + source + "\n" + kKernelSrc;
+  const char* headers[] = {kTraverseSrc, kPoolSrc};
+  const char* names[] = {"traverse.h", "pool.h"};
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "consumer_narrow.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+  if (hiprtcCreateProgram(&prog, src.c_str(), "consumer_narrow.hip", 2, headers, names) != HIPRTC_SUCCESS)
     return fail(SQ_ERR_HIP, "hiprtcCreateProgram failed");
   const std::string dd = "-DSQ_RTC_D=" + std::to_string(dim), pp = "-DSQ_RTC_PF=" + std::to_string(prim_floats);
   // the engine's own flags: no contraction, f32 denormals, no packed FP32
@@ -187,17 +193,34 @@ int sq_raycast_nearest_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uin
   if (!n) return SQ_OK;
   sq_ctx* c = b->ctx;
   if (!b->n_prims) return fill_miss(c, n, b->dim, NearestOut{out_leaf, out_toi, nullptr, nullptr, nullptr, nullptr});
-  const uint32_t B = 128;
-  uint32_t stack_n = b->packed.max_stack;
-  const size_t lds = (size_t)stack_n * B * 8;
-  if (lds > 160 * 1024) return fail(SQ_ERR_INVALID, "traversal stack does not fit in LDS");
+  // the pooled launch of the built-in kinds (nearest_impl): chunks of
+  // 64 * per rays, a grid of the resident waves, ring stacks spilling
+  const uint32_t B = (uint32_t)c->block;
+  uint32_t S = (uint32_t)c->lds_stack;
+  const uint32_t per = pooled_per_lane(c, n);
+  const uint64_t chunk64 = 64ull * per;
+  const uint64_t chunks = (n + chunk64 - 1) / chunk64;
+  const uint32_t stack_n = b->packed.max_stack;
+  const uint32_t spill_rows = stack_n > S ? stack_n - S : 0;
+  const size_t lds = (size_t)S * B * 8;
+  int per_cu = 0;
+  HIP_TRY(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, nw->fn, (int)B, lds));
+  const uint64_t wpb = B / 64;
+  const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
+  uint32_t lanes = (uint32_t)(grid * B);
+  int rc;
+  if ((rc = c->counter.ensure(256))) return rc;
+  if ((rc = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64))) return rc;
+  HIP_TRY(hipMemsetAsync(c->counter.p, 0, 4, c->stream));
   SceneView s = view_of(b);
   const float* rp = rays;
-  uint32_t nn = (uint32_t)n;
+  uint32_t nn = (uint32_t)n, chunk = (uint32_t)chunk64, pause = (uint32_t)c->descend_min;
   int32_t* lp = out_leaf;
   float* tp = out_toi;
-  void* args[] = {&s, &rp, &nn, &max_toi, &lp, &tp, &stack_n};
-  HIP_TRY(hipModuleLaunchKernel(nw->fn, grid_for(n, B), 1, 1, B, 1, 1, (unsigned)lds, c->stream, args, nullptr));
+  uint32_t* counter = c->counter.as<uint32_t>();
+  SpillEntry* spill = c->spill.as<SpillEntry>();
+  void* args[] = {&s, &rp, &nn, &max_toi, &lp, &tp, &chunk, &counter, &S, &spill, &lanes, &pause};
+  HIP_TRY(hipModuleLaunchKernel(nw->fn, (unsigned)grid, 1, 1, B, 1, 1, (unsigned)lds, c->stream, args, nullptr));
   return SQ_OK;
 }
 

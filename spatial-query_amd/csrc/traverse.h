@@ -81,6 +81,16 @@ SQ_HD bool hit_before(float t, uint32_t leaf, float bt, uint32_t bl) {
   return a < b || (a == b && leaf < bl);
 }
 
+// Outputs of nearest-hit queries (device arrays; point, normal, sub, any optional).
+struct NearestOut {
+  int32_t* leaf;
+  float* toi;
+  float* point;
+  float* normal;
+  int32_t* sub;
+  uint8_t* any;
+};
+
 struct Filter {
   const uint64_t* layers;   // by original leaf id, or null (all ones)
   uint64_t mask;
