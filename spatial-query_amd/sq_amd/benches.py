@@ -4,6 +4,7 @@
     python -m sq_amd.benches c4
     python -m sq_amd.benches build
     python -m sq_amd.benches consumer
+    python -m sq_amd.benches dispatch
 
 c2: 16 noise-displaced icospheres (2k-50k triangles, log-uniform) placed as
 4,096 instances (uniform rotations, translations U[-1000,1000]^3, scales
@@ -395,9 +396,69 @@ def consumer():
     return out
 
 
+def dispatch():
+    """The HIP backend's constants for the reference's dispatch cost model
+    (crates/spatial-query/src/dispatch.rs:38-116): the host round trip of a
+    nearest-hit batch (rays from pinned host memory, results back, synchronised)
+    on a 100,000-ball scene, as GPU_FIXED_US (a batch of one) and the per-ray
+    slope over depth^2 (depth = log2(leaves + 1)), split between traversal and
+    narrow test in the reference's proportion."""
+    import torch
+    import sq_amd
+    from sq_amd import workloads as W
+
+    ctx = sq_amd.Context(0)
+    s = W.Stream(0xD15)
+    n = 100_000
+    c = s.range(-100.0, 100.0, n * 3).reshape(n, 3)
+    r = s.range(0.1, 2.0, n).reshape(n, 1)
+    bvh = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.SPHERE, np.concatenate([c, r], 1).astype(np.float32))
+    big = 1 << 20
+    rays_h = torch.from_numpy(W.random_rays(big, 0xD16, -120.0, 120.0)).pin_memory()
+    rays_d = torch.empty_like(rays_h, device="cuda")
+    leaf_h = torch.empty(big, dtype=torch.int32).pin_memory()
+    toi_h = torch.empty(big, dtype=torch.float32).pin_memory()
+    stream = torch.cuda.current_stream()
+
+    def round_trip(b):
+        rays_d[:b].copy_(rays_h[:b], non_blocking=True)
+        out = bvh.raycast_nearest(rays_d[:b], 400.0)
+        leaf_h[:b].copy_(out["leaf"], non_blocking=True)
+        toi_h[:b].copy_(out["toi"], non_blocking=True)
+        stream.synchronize()
+
+    res = {}
+    for b in (1, 16, 256, 4096, 65536, big):
+        reps = 50 if b <= 4096 else 10
+        round_trip(b)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            round_trip(b)
+            ts.append((time.perf_counter() - t0) * 1e6)
+        res[b] = float(np.median(ts))
+    depth = math.log2(n + 1)
+    fixed = res[1]
+    per_ray = (res[big] - res[1]) / (big - 1)
+    coef = per_ray / depth ** 2
+    ref_trav, ref_narrow = 0.00004, 0.000153          # the reference's split (dispatch.rs:66-78)
+    cpu_per_ray = (0.0020 + 0.0078) * depth ** 2       # the reference's CPU model at hint = 1 (dispatch.rs:48-55)
+    crossover = fixed / max(cpu_per_ray - per_ray, 1e-9)
+    out = dict(workload="dispatch cost constants: 100,000 balls (depth log2(leaves+1) = %.2f), host round trip "
+                        "(pinned rays in, leaf + toi out, synchronised)" % depth,
+               round_trip_us={str(k): round(v, 1) for k, v in res.items()},
+               GPU_FIXED_US=round(fixed, 1), per_ray_us=round(per_ray, 6),
+               GPU_TRAVERSAL_US_PER_LEVEL2=float("%.3g" % (coef * ref_trav / (ref_trav + ref_narrow))),
+               GPU_NARROW_US_PER_LEVEL2=float("%.3g" % (coef * ref_narrow / (ref_trav + ref_narrow))),
+               reference_GPU_FIXED_US=1350.0, reference_per_ray_us=0.053,
+               model_crossover_batch_vs_reference_cpu=round(crossover, 1))
+    log(f"fixed {fixed:.1f} us, per ray {per_ray * 1e3:.3f} ns, crossover vs the reference CPU model {crossover:.1f} rays")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["c2", "c4", "build", "consumer"])
+    ap.add_argument("which", choices=["c2", "c4", "build", "consumer", "dispatch"])
     ap.add_argument("--frames", type=int, default=60)
     ap.add_argument("--batches", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
@@ -409,8 +470,10 @@ def main():
         res = c4()
     elif a.which == "build":
         res = build()
-    else:
+    elif a.which == "consumer":
         res = consumer()
+    else:
+        res = dispatch()
     print(json.dumps(res), flush=True)
 
 
