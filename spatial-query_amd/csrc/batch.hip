@@ -68,33 +68,25 @@ __global__ __launch_bounds__(256, (D <= 3 ? 8 : 4)) void k_batch_bvh(const BvhIt
   }
 }
 
-// Two-level scenes: each lane walks rays first + 64 k + lane, k < per_lane, of
-// its wave's chunk with the two-level walk (whole stacks in LDS).
+// Two-level scenes: the resumable two-level walk (pool_walk_tlas) over each
+// chunk, both stacks S-entry LDS rings spilling to their own columns.
 template <int D, int IK>
 __global__ __launch_bounds__(256) void k_batch_tlas(const TlasItem* __restrict__ items, uint32_t n_items, uint32_t n_chunks,
-                                                    uint32_t per_lane, uint32_t s_top, uint32_t s_bot, uint32_t* counter) {
+                                                    uint32_t per_lane, uint32_t S, SpillEntry* spill_top, SpillEntry* spill_bot,
+                                                    uint32_t spill_stride, uint32_t* counter, uint32_t pause_below) {
   extern __shared__ uint32_t lds[];
-  const uint32_t B = blockDim.x, tid = threadIdx.x, lane = tid & 63u;
-  const uint32_t S = s_top + s_bot;
-  uint32_t* refs = lds + tid;
-  float* ts = reinterpret_cast<float*>(lds + S * B) + tid;
-  Stack st{refs, ts, B};
-  Stack sb{refs + s_top * B, ts + s_top * B, B};
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  const uint32_t gt = blockIdx.x * B + tid;
+  RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill_top + gt, spill_stride};
+  RingStack sb{lds + 2 * S * B + tid, reinterpret_cast<float*>(lds + 3 * S * B) + tid, B, S, spill_bot + gt, spill_stride};
   const uint64_t pool = 64ull * per_lane;
   for (;;) {
     const uint32_t c = wave_fetch_add(counter);
     if (c >= n_chunks) break;
     const TlasItem& it = items[find_item(items, n_items, c)];
     const uint64_t first = (uint64_t)(c - it.chunk0) * pool;
-    for (uint32_t k = 0; k < per_lane; ++k) {
-      const uint64_t i = first + (uint64_t)k * 64 + lane;
-      if (i >= it.n) break;
-      RayD<D> r;
-      ray_setup<D>(r, it.rays + i * 2 * D);
-      const TlasHit h = walk_tlas<D, IK, false>(it.tv, r, it.max_toi, st, sb);
-      it.inst[i] = h.inst != NO_LEAF ? (int32_t)h.inst : -1;
-      it.toi[i] = h.t;
-    }
+    const uint64_t end = first + pool < (uint64_t)it.n ? first + pool : (uint64_t)it.n;
+    pool_walk_tlas<D, IK>(it.tv, it.rays, OnePool{first, end}, it.max_toi, it.inst, it.toi, st, sb, pause_below);
   }
 }
 
@@ -108,6 +100,7 @@ struct sq_batch {
   sq_ctx* ctx = nullptr;
   int cls = 0, dim = 0, kind = 0;
   uint32_t n_items = 0, n_chunks = 0, per_lane = 1, S = 8, s_top = 0, s_bot = 0, grid = 0, block = 128, descend_min = 0;
+  uint32_t rows_top = 0;           // two-level sets: spill rows of the top stack (the BLAS stack's follow)
   size_t lds = 0;
   DevBuf items, counter, spill;
   std::vector<std::pair<uint64_t, NearestOut>> empty;   // batches over scenes without primitives: all misses
@@ -201,8 +194,7 @@ int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* qs, uint
         return SQ_OK;
       });
     } else {
-      b->lds = (size_t)(b->s_top + b->s_bot) * b->block * 8;
-      if (b->lds > 160 * 1024) return fail(SQ_ERR_INVALID, "traversal stacks do not fit in LDS; lower the block size");
+      b->lds = (size_t)2 * b->S * b->block * 8;
       rc = dispatch_inst(b->dim, b->kind, [&]<int D, int K>() -> int {
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_batch_tlas<D, K>), (int)b->block,
                                                              b->lds));
@@ -219,6 +211,11 @@ int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* qs, uint
       if ((rc = b->spill.ensure((size_t)rows * b->grid * b->block * sizeof(SpillEntry) + 64))) return rc;
       if ((rc = upload(b->items, bi.data(), bi.size() * sizeof(BvhItem), ctx->stream))) return rc;
     } else {
+      // spill columns: the top stack's rows, then the BLAS stack's
+      const uint64_t lanes = (uint64_t)b->grid * b->block;
+      b->rows_top = b->s_top > b->S ? b->s_top - b->S : 0;
+      const uint32_t rows_bot = b->s_bot > b->S ? b->s_bot - b->S : 0;
+      if ((rc = b->spill.ensure((size_t)(b->rows_top + rows_bot) * lanes * sizeof(SpillEntry) + 64))) return rc;
       if ((rc = upload(b->items, ti.data(), ti.size() * sizeof(TlasItem), ctx->stream))) return rc;
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));   // the host tables end with this call
@@ -255,8 +252,10 @@ int sq_batch_submit(sq_batch* b) {
       });
     else
       rc = dispatch_inst(b->dim, b->kind, [&]<int D, int K>() -> int {
+        SpillEntry* sp = b->spill.as<SpillEntry>();
         k_batch_tlas<D, K><<<b->grid, b->block, b->lds, c->stream>>>(b->items.as<TlasItem>(), b->n_items, b->n_chunks, b->per_lane,
-                                                                    b->s_top, b->s_bot, b->counter.as<uint32_t>());
+                                                                    b->S, sp, sp + (size_t)b->rows_top * spill_stride, spill_stride,
+                                                                    b->counter.as<uint32_t>(), b->descend_min);
         HIP_TRY(hipGetLastError());
         return SQ_OK;
       });

@@ -159,6 +159,189 @@ __device__ __forceinline__ void pool_walk(const SceneView& s, const float* __res
 }
 
 // ---------------------------------------------------------------------------
+// Two-level walk, resumable (Tlas::raycast_nearest, tlas.rs:104-154, the
+// order of walk_tlas in traverse.h made stepwise): a step descends the top
+// tree to a leaf, or tests the next instances of that leaf (box instances
+// directly; a mesh instance starts a walk of its BLAS with the local ray), or
+// takes one walk_step in the current BLAS.  A finished BLAS walk's hit goes
+// through instance_test's guard and the (toi, instance) order; the limit the
+// BLAS walk prunes with is the top-level best when it started, as in
+// instance_test.  Both stacks are LDS rings spilling to device memory.
+// ---------------------------------------------------------------------------
+
+template <int D> struct TlasWalk {
+  uint32_t cur, sp;            // top level: next ref, stack depth
+  float limit;
+  TlasHit res;
+  uint32_t first, cnt, j;      // the top leaf range under test, j its next instance
+  uint32_t phase;              // 0 descend the top, 1 test the range, 2 walk a BLAS, 3 pop the top
+  uint32_t inst, mesh;         // the mesh instance being walked
+  RayPolicy<D, KIND_TRI> pol;  // its local ray
+  WalkState wb;
+};
+
+template <int D>
+__device__ __forceinline__ bool tlas_begin(const TlasView& tv, const RayD<D>& r, float max_toi, TlasWalk<D>& w) {
+  w.res = TlasHit{SQ_INF, NO_LEAF, NO_LEAF, NO_LEAF};
+  w.limit = max_toi;
+  w.sp = 0;
+  w.phase = 0;
+  w.cur = fbits(tv.top.root[2 * D]);
+  float te, tx;
+  int ax;
+  return slab_exact<D>(r, tv.top.root, tv.top.root + D, max_toi, te, tx, ax);
+}
+
+template <int D>
+__device__ __forceinline__ void tlas_take(TlasWalk<D>& w, float t, uint32_t inst, uint32_t pos, uint32_t sub) {
+  if (w.res.inst == NO_LEAF || hit_before(t, inst, w.res.t, w.res.inst)) {
+    w.res = TlasHit{t, inst, pos, sub};
+    w.limit = t;
+  }
+}
+
+// One step of ray r; true when the ray is finished (result in w.res).
+template <int D, int IK, class P>
+__device__ __forceinline__ bool tlas_step(const TlasView& tv, const RayD<D>& r, const RingStack& st, const RingStack& sb,
+                                          TlasWalk<D>& w, const P& pause) {
+  constexpr int NF = (D + 1) * 4;
+  const SceneView& s = tv.top;
+  if constexpr (IK == INST_MESH) {
+    if (w.phase == 2) {
+    const Filter nof{nullptr, ~0ull, -1};
+    if (!walk_step<D, Prim<3, KIND_TRI>::STRIDE, false, false>(tv.blas[w.mesh], w.pol, nof, sb, w.wb, nullptr, nullptr, pause))
+      return false;
+    if (w.wb.res.leaf != NO_LEAF) {
+      // instance_test's guard: toi inside the instance's world slab at the limit the BLAS walk used
+      const float t = w.wb.res.t;
+      const float* wx = tv.world + (size_t)w.inst * 2 * D;
+      float te, tx;
+      int a;
+      if (slab_exact<D>(r, wx, wx + D, w.limit, te, tx, a) && te <= t && t <= tx)
+        tlas_take<D>(w, t, w.inst, w.wb.res.pos, w.wb.res.leaf);
+    }
+    ++w.j;
+    w.phase = 1;
+    }
+  }
+  if (w.phase == 1) {
+    while (w.j < w.cnt) {
+      const uint32_t id = s.ids[w.first + w.j];
+      if constexpr (IK == INST_BOX) {
+        TlasHit h;
+        if (instance_test<D, IK>(tv, r, id, w.limit, Stack{nullptr, nullptr, 0}, h)) tlas_take<D>(w, h.t, h.inst, h.pos, h.sub);
+        ++w.j;
+      } else {
+        const float* x = tv.xf + (size_t)id * tv.xs;
+        float ol[D], dl[D], src[2 * D];
+        iso_inverse<D>(x, x + D * D, r, ol, dl);
+        const float inv_s = x[D * D + D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) { src[k] = ol[k] * inv_s; src[D + k] = dl[k] * inv_s; }
+        w.pol.setup(src);
+        w.mesh = fbits(x[D * D + D + 1]);
+        if (!walk_begin<D>(tv.blas[w.mesh], w.pol, w.limit, w.wb)) { ++w.j; continue; }
+        w.inst = id;
+        w.phase = 2;
+        return false;                 // the BLAS walk starts with the next step
+      }
+    }
+    w.phase = 3;
+  }
+  if (w.phase == 3) {
+    bool got = false;
+    while (w.sp > 0) {
+      uint32_t ref;
+      float tt;
+      st.pop(w.sp, ref, tt);
+      if (tt <= w.limit) { w.cur = ref; got = true; break; }
+    }
+    if (!got) return true;
+    w.phase = 0;
+  }
+  // phase 0: descend the top tree to a leaf
+  while (!(w.cur & LEAF_BIT)) {
+    float nd[NF];
+    fetch16<D + 1>(s.nodes + (size_t)w.cur * NF, nd);
+    bool ha, hb;
+    float ta, tb;
+    slab_pair<D>(r, nd, w.limit, ha, hb, ta, tb);
+    const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
+    if (ha && hb) {
+      const bool a_first = ta <= tb;
+      st.push(w.sp, a_first ? rb : ra, a_first ? tb : ta);
+      w.cur = a_first ? ra : rb;
+    } else if (ha) {
+      w.cur = ra;
+    } else if (hb) {
+      w.cur = rb;
+    } else {
+      bool got = false;
+      while (w.sp > 0) {
+        uint32_t ref;
+        float tt;
+        st.pop(w.sp, ref, tt);
+        if (tt <= w.limit) { w.cur = ref; got = true; break; }
+      }
+      if (!got) return true;
+    }
+    if (pause()) return false;
+  }
+  w.first = w.cur & FIRST_MASK;
+  w.cnt = ((w.cur >> 27) & 15u) + 1u;
+  w.j = 0;
+  w.phase = 1;
+  return false;
+}
+
+// One wave walks the ray ranges src hands it through the two-level scene
+// (lanes refilled as rays finish, as pool_walk); results to out_inst / out_toi.
+template <int D, int IK, class Src>
+__device__ __forceinline__ void pool_walk_tlas(const TlasView& tv, const float* __restrict__ q, Src src, float max_toi,
+                                               int32_t* out_inst, float* out_toi, const RingStack& st, const RingStack& sb,
+                                               uint32_t pause_below) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  RayD<D> r;
+  TlasWalk<D> w;
+  uint32_t rid = NO_LEAF;
+  uint64_t next = 0, end = 0;
+  bool exhausted = !src.get(next, end);
+  while (true) {
+    bool need = rid == NO_LEAF && !exhausted;
+    uint64_t m = __ballot(need);
+    while (m) {
+      if (next >= end) {
+        if (!src.get(next, end)) { exhausted = true; break; }
+        continue;
+      }
+      const uint64_t avail = end - next;
+      const uint64_t rank = (uint64_t)__popcll(m & lt_mask);
+      if (need && rank < avail) {
+        need = false;
+        rid = (uint32_t)(next + rank);
+        ray_setup<D>(r, q + (size_t)rid * 2 * D);
+        if (!tlas_begin<D>(tv, r, max_toi, w)) {
+          out_inst[rid] = -1;
+          out_toi[rid] = SQ_INF;
+          rid = NO_LEAF;
+        }
+      }
+      const uint64_t taken = (uint64_t)__popcll(m);
+      next += taken < avail ? taken : avail;
+      m = __ballot(need);
+    }
+    if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
+    const uint32_t active = (uint32_t)__popcll(__ballot(rid != NO_LEAF));
+    if (rid == NO_LEAF) continue;
+    if (!tlas_step<D, IK>(tv, r, st, sb, w, WavePause{pause_below, active})) continue;
+    out_inst[rid] = w.res.inst != NO_LEAF ? (int32_t)w.res.inst : -1;
+    out_toi[rid] = w.res.t;
+    rid = NO_LEAF;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Pooled collect (all hits, crossings, overlap, cast-all): one walk per query
 // appends the query's entries to a staging area; the host side then scans the
 // per-query counts, scatters the entries to their CSR slots and sorts each
