@@ -84,7 +84,10 @@ int sq_ctx_create(int device, sq_ctx** out);   /* SQ_ERR_NO_DEVICE when absent *
 int sq_ctx_destroy(sq_ctx* ctx);
 /* Enqueue all further work on `stream` (a hipStream_t of the same device, e.g.
  * the caller's framework stream; NULL is HIP's legacy default stream) instead
- * of the context's own stream.  sq_ctx_reset_stream goes back to its own. */
+ * of the context's own stream.  sq_ctx_reset_stream goes back to its own.
+ * A context's calls share its scratch buffers and pool counters: work it
+ * enqueued on one stream must be finished, or ordered before, when work is
+ * enqueued on another (the calls of one stream are ordered by the stream). */
 int sq_ctx_set_stream(sq_ctx* ctx, void* stream);
 int sq_ctx_reset_stream(sq_ctx* ctx);
 int sq_ctx_synchronize(sq_ctx* ctx);

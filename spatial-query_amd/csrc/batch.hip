@@ -4,8 +4,9 @@
 // the batches of a frame become one table of 64 * rays_per_lane-ray chunks, and
 // a grid sized to what the chip holds resident pulls chunks from an atomic
 // counter until none are left, so uneven batch sizes leave no CU idle.
-// Enqueueing is capture-safe (kernels only: a fill kernel clears the counter),
-// so a frame of submissions can be recorded into a hipGraph.
+// Enqueueing is capture-safe (kernels only: the launch leaves its counters
+// cleared for the next one), so a frame of submissions can be recorded into a
+// hipGraph.
 #include <algorithm>
 
 #include "engine_internal.h"
@@ -56,16 +57,17 @@ __global__ __launch_bounds__(256, (D <= 3 ? 8 : 4)) void k_batch_bvh(const BvhIt
   RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + blockIdx.x * B + tid, spill_stride};
   const Filter nof{nullptr, ~0ull, -1};
   const uint64_t pool = 64ull * per_lane;
-  for (;;) {
-    const uint32_t c = wave_fetch_add(counter);
-    if (c >= n_chunks) break;
+  // a wave walks the chunks of one batch as one pool (lanes refilled across
+  // its chunks) until a fetch lands in a later batch, then goes on with that
+  uint32_t c = wave_fetch_add(counter);
+  while (c < n_chunks) {
     const BvhItem& it = items[find_item(items, n_items, c)];
-    const uint64_t first = (uint64_t)(c - it.chunk0) * pool;
-    const uint64_t end = first + pool < (uint64_t)it.n ? first + pool : (uint64_t)it.n;
+    const uint32_t end = it.chunk0 + (uint32_t)(((uint64_t)it.n + pool - 1) / pool);
     const NearestOut out{it.leaf, it.toi, nullptr, nullptr, nullptr, nullptr};
-    pool_walk<D, KIND, false, false, false>(it.s, it.rays, OnePool{first, end}, it.max_toi, nullptr, nof, out, st, nullptr,
-                                            pause_below);
+    pool_walk<D, KIND, false, false, false>(it.s, it.rays, ItemChunks{counter, it.chunk0, end, it.n, pool, &c}, it.max_toi,
+                                            nullptr, nof, out, st, nullptr, pause_below);
   }
+  pool_retire(counter);
 }
 
 // Two-level scenes: the resumable two-level walk (pool_walk_tlas) over each
@@ -80,18 +82,15 @@ __global__ __launch_bounds__(256) void k_batch_tlas(const TlasItem* __restrict__
   RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill_top + gt, spill_stride};
   RingStack sb{lds + 2 * S * B + tid, reinterpret_cast<float*>(lds + 3 * S * B) + tid, B, S, spill_bot + gt, spill_stride};
   const uint64_t pool = 64ull * per_lane;
-  for (;;) {
-    const uint32_t c = wave_fetch_add(counter);
-    if (c >= n_chunks) break;
+  uint32_t c = wave_fetch_add(counter);
+  while (c < n_chunks) {
     const TlasItem& it = items[find_item(items, n_items, c)];
-    const uint64_t first = (uint64_t)(c - it.chunk0) * pool;
-    const uint64_t end = first + pool < (uint64_t)it.n ? first + pool : (uint64_t)it.n;
-    pool_walk_tlas<D, IK>(it.tv, it.rays, OnePool{first, end}, it.max_toi, it.inst, it.toi, st, sb, pause_below);
+    const uint32_t end = it.chunk0 + (uint32_t)(((uint64_t)it.n + pool - 1) / pool);
+    const NearestOut out{it.inst, it.toi, nullptr, nullptr, nullptr, nullptr};
+    pool_walk_tlas<D, IK, false>(it.tv, it.rays, ItemChunks{counter, it.chunk0, end, it.n, pool, &c}, it.max_toi, nullptr, out,
+                                 st, sb, pause_below);
   }
-}
-
-__global__ void k_zero_u32(uint32_t* p) {
-  if (threadIdx.x == 0) *p = 0u;
+  pool_retire(counter);
 }
 
 }  // namespace
@@ -218,6 +217,8 @@ int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* qs, uint
       if ((rc = b->spill.ensure((size_t)(b->rows_top + rows_bot) * lanes * sizeof(SpillEntry) + 64))) return rc;
       if ((rc = upload(b->items, ti.data(), ti.size() * sizeof(TlasItem), ctx->stream))) return rc;
     }
+    // the pool counters start zero; every launch leaves them zero (pool_retire)
+    HIP_TRY(hipMemsetAsync(b->counter.p, 0, 256, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));   // the host tables end with this call
   }
   ctx_retain(ctx);
@@ -239,8 +240,6 @@ int sq_batch_submit(sq_batch* b) {
   for (const auto& e : b->empty)
     if ((rc = fill_miss(c, e.first, b->dim, e.second))) return rc;
   if (b->n_items) {
-    k_zero_u32<<<1, 64, 0, c->stream>>>(b->counter.as<uint32_t>());
-    HIP_TRY(hipGetLastError());
     const uint32_t spill_stride = b->grid * b->block;
     if (b->cls == SQ_BATCH_BVH)
       rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {

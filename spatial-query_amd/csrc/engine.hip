@@ -63,8 +63,9 @@ __global__ __launch_bounds__(1024) void k_nearest(SceneView s, const float* __re
 
 // Pooled nearest / any-hit / shape-cast kernel, persistent: the grid holds as
 // many waves as the chip keeps resident, and every wave takes chunks of `chunk`
-// queries from *counter (cleared by the caller) until all n are handed out,
-// refilling its lanes across chunk boundaries (pool_walk).  With `order`, pool
+// queries from counter[0] until all n are handed out, refilling its lanes
+// across chunk boundaries (pool_walk); the counters start zero and the launch
+// leaves them zero (pool_retire).  With `order`, pool
 // position i is query order[i] (the Morton order).
 template <int D, int KIND, bool ANY, bool FILT, bool CAST>
 __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nearest_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
@@ -79,10 +80,7 @@ __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nea
   const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
   const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
   pool_walk<D, KIND, ANY, FILT, CAST>(s, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, f, out, st, order, pause_below);
-}
-
-__global__ void k_zero_counter(uint32_t* p) {
-  if (threadIdx.x == 0) *p = 0u;
+  pool_retire(counter);
 }
 
 template <int D>
@@ -725,6 +723,9 @@ int sq_ctx_create(int device, sq_ctx** out) {
   HIP_TRY(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device));
   HIP_TRY(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
   c->stream = c->own;
+  if (int rc = c->pool_ctr.ensure(256)) return rc;
+  HIP_TRY(hipMemsetAsync(c->pool_ctr.p, 0, 256, c->own));
+  HIP_TRY(hipStreamSynchronize(c->own));
   *out = c.release();
   return SQ_OK;
 }
@@ -1104,14 +1105,12 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
   const uint32_t* order = nullptr;
   if (c->ray_order && !cast && n >= 4096)
     if ((rc = ray_order(c, b->dim, q, nn, 2 * b->dim, b->root.as<float>(), &order))) return rc;
-  uint32_t* counter = c->counter.as<uint32_t>();
-  k_zero_counter<<<1, 64, 0, c->stream>>>(counter);
-  HIP_TRY(hipGetLastError());
+  uint32_t* counter = c->pool_ctr.as<uint32_t>();
   rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
     auto launch = [&]<bool ANY_, bool FILT_, bool CAST_>() -> int {
       const void* kern = reinterpret_cast<const void*>(&k_nearest_pool<D, K, ANY_, FILT_, CAST_>);
       int per_cu = 0;
-      HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, B, lds));
+      if (int r0 = blocks_per_cu(kern, c->device, B, lds, &per_cu)) return r0;
       const uint64_t wpb = (uint64_t)B / 64;
       const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
       const uint64_t lanes = grid * (uint64_t)B;
@@ -1222,7 +1221,7 @@ static int collect_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
           const void* kern = reinterpret_cast<const void*>(&k_collect_pool<D, K, MODE_, FILT_, ITEMS_>);
           const size_t lds_p = (size_t)S * B * 8 + (size_t)(B / 64) * 8;
           int per_cu = 0;
-          HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, B, lds_p));
+          if (int r0 = blocks_per_cu(kern, c->device, B, lds_p, &per_cu)) return r0;
           const uint64_t wpb = (uint64_t)B / 64;
           const uint64_t resident = (uint64_t)c->cus * std::max(1, per_cu);
           // item walks: the item count is known on the device only, so the grid is every resident wave

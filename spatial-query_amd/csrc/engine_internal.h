@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <functional>
+#include <map>
+#include <mutex>
 #include <memory>
 #include <string>
 #include <vector>
@@ -68,6 +70,9 @@ struct sq_ctx {
   int descend_min = 24;            // pooled walks: pause descents below this many lanes (pool.h)
   int split_depth = 0;             // pooled collect: split queries this many levels down (0 auto, -1 never)
   bool timed = false;
+  // {next chunk, retired waves} of the pooled single-call launches: zero at
+  // creation, and each launch leaves them zero again (pool_retire)
+  DevBuf pool_ctr;
   int refs = 1;                    // the caller's handle + one per live tree
   DevBuf counts, scratch, keys, keys2;   // reusable temporaries of the CSR queries
   DevBuf spill;                    // ring-stack spill columns of the pooled kernel
@@ -181,6 +186,35 @@ inline uint32_t split_levels(const sq_ctx* c, uint64_t n, uint64_t tree_depth, b
   if (L > 0) return (uint32_t)L;
   if (!overlap || n >= (1ull << 21)) return 0;
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((3 * tree_depth + 2) / 4, 24));
+}
+
+// Resident blocks per CU of a kernel at a block size and LDS size, queried
+// once per (kernel, device, block, LDS) and then cached: the query is host
+// work every pooled launch would otherwise repeat.
+inline int blocks_per_cu(const void* kern, int device, int block, size_t lds, int* out) {
+  struct Key {
+    const void* k; int dev, block; size_t lds;
+    bool operator<(const Key& o) const {
+      if (k != o.k) return k < o.k;
+      if (dev != o.dev) return dev < o.dev;
+      if (block != o.block) return block < o.block;
+      return lds < o.lds;
+    }
+  };
+  static std::mutex mu;
+  static std::map<Key, int> cache;
+  const Key key{kern, device, block, lds};
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) { *out = it->second; return SQ_OK; }
+  }
+  int n = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, block, lds));
+  std::lock_guard<std::mutex> g(mu);
+  cache[key] = n;
+  *out = n;
+  return SQ_OK;
 }
 
 inline SceneView view_of(const sq_bvh* b) {

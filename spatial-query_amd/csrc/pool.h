@@ -29,9 +29,28 @@ __device__ __forceinline__ uint32_t wave_fetch_add(uint32_t* counter) {
   return __builtin_amdgcn_readfirstlane(c);
 }
 
+// A persistent launch's pool counters {next chunk, retired waves} start at
+// zero.  Every wave of the grid calls pool_retire once, after its last fetch
+// (the one that found the pool empty, whose value it has already used); the
+// wave that retires last finds every fetch of the launch done and sets both
+// words back to zero.  The next launch in stream order therefore finds them
+// cleared without a clearing launch of its own.
+__device__ __forceinline__ void pool_retire(uint32_t* counter) {
+  if ((threadIdx.x & 63u) == 0) {
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    if (atomicAdd(counter + 1, 1u) == waves - 1u) {
+      atomicExch(counter, 0u);
+      atomicExch(counter + 1, 0u);
+    }
+  }
+}
+
 // Query ranges a wave walks.  OnePool: one fixed range.  Chunks: ranges of
 // `chunk` queries taken from an atomic counter until n is covered (work
-// stealing across the whole grid).
+// stealing across the whole grid).  ItemChunks: the chunks of one batch of a
+// batch set (numbered chunk0 .. chunk_end - 1 in the set's shared counter),
+// starting from a chunk the wave already holds; a fetched chunk of a later
+// batch ends this batch's walk and is left in *carry for the next one.
 struct OnePool {
   uint64_t first, end;
   bool given = false;
@@ -41,6 +60,26 @@ struct OnePool {
     next = first;
     e = end;
     return first < end;
+  }
+};
+struct ItemChunks {
+  uint32_t* counter;
+  uint32_t chunk0, chunk_end;
+  uint64_t n, chunk;       // the batch's queries, queries per chunk
+  uint32_t* carry;         // in: the chunk held; out: the first chunk fetched past this batch
+  bool started = false;
+  __device__ __forceinline__ bool get(uint64_t& next, uint64_t& e) {
+    uint32_t c = *carry;
+    if (started) {
+      c = wave_fetch_add(counter);
+      *carry = c;
+    }
+    started = true;
+    if (c < chunk0 || c >= chunk_end) return false;
+    const uint64_t f = (uint64_t)(c - chunk0) * chunk;
+    next = f;
+    e = f + chunk < n ? f + chunk : n;
+    return true;
   }
 };
 struct Chunks {
@@ -65,7 +104,9 @@ struct Chunks {
 // queries from the current range in order; a lane whose query finishes takes
 // the next one (ballot + popcount), and the wave takes a new range when this
 // one runs out, so lanes stay busy while long queries run.  Each step walks to one leaf and tests it (walk_step), or stops early
-// when fewer than pause_below lanes are still descending (WavePause).  The stack is
+// when fewer than pause_below lanes are still descending (WavePause) so that
+// finished lanes can take new queries; once the pool is exhausted there is
+// nothing to take and the walks run without pausing.  The stack is
 // the caller's LDS ring spilling to a global column.  With `order`, pool
 // position i is query order[i] (rays read and results written there).
 // The walk for any query policy Pol (setup / root / pair / narrow over
@@ -122,7 +163,7 @@ __device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* 
     if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
     const uint32_t active = (uint32_t)__popcll(__ballot(rid != NO_LEAF));
     if (rid == NO_LEAF) continue;
-    if (!walk_step<D, PS, ANY, FILT>(s, pol, f, st, w, nullptr, nullptr, WavePause{pause_below, active})) continue;
+    if (!walk_step<D, PS, ANY, FILT>(s, pol, f, st, w, nullptr, nullptr, WavePause{exhausted ? 0u : pause_below, active})) continue;
     // finished: write the result of query rid
     const NearestResult& h = w.res;
     const bool hit = h.leaf != NO_LEAF;
@@ -200,8 +241,9 @@ __device__ __forceinline__ void tlas_take(TlasWalk<D>& w, float t, uint32_t inst
   }
 }
 
-// One step of ray r; true when the ray is finished (result in w.res).
-template <int D, int IK, class P>
+// One step of ray r; true when the ray is finished (result in w.res).  ANY
+// finishes at the first instance hit that passes the guard (walk_tlas's ANY).
+template <int D, int IK, bool ANY, class P>
 __device__ __forceinline__ bool tlas_step(const TlasView& tv, const RayD<D>& r, const RingStack& st, const RingStack& sb,
                                           TlasWalk<D>& w, const P& pause) {
   constexpr int NF = (D + 1) * 4;
@@ -217,8 +259,10 @@ __device__ __forceinline__ bool tlas_step(const TlasView& tv, const RayD<D>& r, 
       const float* wx = tv.world + (size_t)w.inst * 2 * D;
       float te, tx;
       int a;
-      if (slab_exact<D>(r, wx, wx + D, w.limit, te, tx, a) && te <= t && t <= tx)
+      if (slab_exact<D>(r, wx, wx + D, w.limit, te, tx, a) && te <= t && t <= tx) {
         tlas_take<D>(w, t, w.inst, w.wb.res.pos, w.wb.res.leaf);
+        if (ANY) return true;
+      }
     }
     ++w.j;
     w.phase = 1;
@@ -229,7 +273,10 @@ __device__ __forceinline__ bool tlas_step(const TlasView& tv, const RayD<D>& r, 
       const uint32_t id = s.ids[w.first + w.j];
       if constexpr (IK == INST_BOX) {
         TlasHit h;
-        if (instance_test<D, IK>(tv, r, id, w.limit, Stack{nullptr, nullptr, 0}, h)) tlas_take<D>(w, h.t, h.inst, h.pos, h.sub);
+        if (instance_test<D, IK>(tv, r, id, w.limit, Stack{nullptr, nullptr, 0}, h)) {
+          tlas_take<D>(w, h.t, h.inst, h.pos, h.sub);
+          if (ANY) return true;
+        }
         ++w.j;
       } else {
         const float* x = tv.xf + (size_t)id * tv.xs;
@@ -295,18 +342,42 @@ __device__ __forceinline__ bool tlas_step(const TlasView& tv, const RayD<D>& r, 
 }
 
 // One wave walks the ray ranges src hands it through the two-level scene
-// (lanes refilled as rays finish, as pool_walk); results to out_inst / out_toi.
-template <int D, int IK, class Src>
+// (lanes refilled as rays finish, as pool_walk).  Outputs as k_tlas_nearest:
+// any, or leaf (instance) and toi with optional point, normal, sub.
+template <int D, int IK, bool ANY, class Src>
 __device__ __forceinline__ void pool_walk_tlas(const TlasView& tv, const float* __restrict__ q, Src src, float max_toi,
-                                               int32_t* out_inst, float* out_toi, const RingStack& st, const RingStack& sb,
-                                               uint32_t pause_below) {
+                                               const float* __restrict__ tmax, const NearestOut& out, const RingStack& st,
+                                               const RingStack& sb, uint32_t pause_below) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   RayD<D> r;
   TlasWalk<D> w;
+  float lim0 = 0.0f;
   uint32_t rid = NO_LEAF;
   uint64_t next = 0, end = 0;
   bool exhausted = !src.get(next, end);
+  auto write = [&](uint32_t i) {
+    const bool hit = w.res.inst != NO_LEAF;
+    if (ANY) {
+      out.any[i] = hit ? 1 : 0;
+      return;
+    }
+    out.leaf[i] = hit ? (int32_t)w.res.inst : -1;
+    out.toi[i] = w.res.t;
+    if (out.point) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) out.point[(size_t)i * D + k] = hit ? r.o[k] + r.d[k] * w.res.t : 0.0f;
+    }
+    if (out.normal) {
+      float nv[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) nv[k] = 0.0f;
+      if (hit) tlas_normal<D, IK>(tv, r, w.res, lim0, nv);
+#pragma unroll
+      for (int k = 0; k < D; ++k) out.normal[(size_t)i * D + k] = nv[k];
+    }
+    if (out.sub) out.sub[i] = (hit && w.res.sub != NO_LEAF) ? (int32_t)w.res.sub : -1;
+  };
   while (true) {
     bool need = rid == NO_LEAF && !exhausted;
     uint64_t m = __ballot(need);
@@ -321,9 +392,9 @@ __device__ __forceinline__ void pool_walk_tlas(const TlasView& tv, const float* 
         need = false;
         rid = (uint32_t)(next + rank);
         ray_setup<D>(r, q + (size_t)rid * 2 * D);
-        if (!tlas_begin<D>(tv, r, max_toi, w)) {
-          out_inst[rid] = -1;
-          out_toi[rid] = SQ_INF;
+        lim0 = tmax ? tmax[rid] : max_toi;
+        if (!tlas_begin<D>(tv, r, lim0, w)) {
+          write(rid);
           rid = NO_LEAF;
         }
       }
@@ -334,9 +405,8 @@ __device__ __forceinline__ void pool_walk_tlas(const TlasView& tv, const float* 
     if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
     const uint32_t active = (uint32_t)__popcll(__ballot(rid != NO_LEAF));
     if (rid == NO_LEAF) continue;
-    if (!tlas_step<D, IK>(tv, r, st, sb, w, WavePause{pause_below, active})) continue;
-    out_inst[rid] = w.res.inst != NO_LEAF ? (int32_t)w.res.inst : -1;
-    out_toi[rid] = w.res.t;
+    if (!tlas_step<D, IK, ANY>(tv, r, st, sb, w, WavePause{exhausted ? 0u : pause_below, active})) continue;
+    write(rid);
     rid = NO_LEAF;
   }
 }
@@ -571,7 +641,7 @@ __device__ __forceinline__ void pool_collect(const SceneView& s, const float* __
             ++cnt;
           });
         },
-        st, v, WavePause{pause_below, active});
+        st, v, WavePause{exhausted ? 0u : pause_below, active});
     if (!done) continue;
     if constexpr (ITEMS) {
       item_cnt[pos] = cnt;

@@ -57,8 +57,9 @@ template <int D> struct ConsumerPolicy {
 }  // namespace sqd
 
 // Persistent pooled walk (as the built-in kinds' k_nearest_pool): waves take
-// chunks of `chunk` rays from *counter (zeroed by the host), refill lanes as
-// rays finish, pause descents below pause_below lanes.
+// chunks of `chunk` rays from counter[0] (zero at the start, left zero by
+// pool_retire), refill lanes as rays finish, pause descents below pause_below
+// lanes while the pool has rays left.
 extern "C" __global__ __launch_bounds__(256) void sq_consumer_nearest(sqd::SceneView s, const float* rays, unsigned n,
                                                                       float max_toi, int* out_leaf, float* out_toi,
                                                                       unsigned chunk, unsigned* counter, unsigned S,
@@ -75,6 +76,7 @@ extern "C" __global__ __launch_bounds__(256) void sq_consumer_nearest(sqd::Scene
   sqd::pool_walk_with<SQ_RTC_D, sqd::KIND_AABB, false, false, false, sqd::ConsumerPolicy<SQ_RTC_D>, 2 * SQ_RTC_D,
                       2 * SQ_RTC_D + SQ_RTC_PF>(s, rays, sqd::Chunks{counter, n, chunk, home, waves}, max_toi, nullptr, f,
                                                 out, st, nullptr, pause_below);
+  sqd::pool_retire(counter);
 }
 )SQRTC";
 
@@ -86,6 +88,10 @@ struct sq_narrow {
   uint32_t pf = 0;
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
+  // resident blocks per CU, cached for the last (block, LDS) pair asked
+  uint32_t occ_block = 0;
+  size_t occ_lds = 0;
+  int occ_per_cu = -1;
 };
 
 extern "C" {
@@ -204,20 +210,25 @@ int sq_raycast_nearest_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uin
   const uint32_t spill_rows = stack_n > S ? stack_n - S : 0;
   const size_t lds = (size_t)S * B * 8;
   int per_cu = 0;
-  HIP_TRY(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, nw->fn, (int)B, lds));
+  if (nw->occ_per_cu >= 0 && nw->occ_block == B && nw->occ_lds == lds) {
+    per_cu = nw->occ_per_cu;
+  } else {
+    HIP_TRY(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, nw->fn, (int)B, lds));
+    nw->occ_block = B;
+    nw->occ_lds = lds;
+    nw->occ_per_cu = per_cu;
+  }
   const uint64_t wpb = B / 64;
   const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
   uint32_t lanes = (uint32_t)(grid * B);
   int rc;
-  if ((rc = c->counter.ensure(256))) return rc;
   if ((rc = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64))) return rc;
-  HIP_TRY(hipMemsetAsync(c->counter.p, 0, 4, c->stream));
   SceneView s = view_of(b);
   const float* rp = rays;
   uint32_t nn = (uint32_t)n, chunk = (uint32_t)chunk64, pause = (uint32_t)c->descend_min;
   int32_t* lp = out_leaf;
   float* tp = out_toi;
-  uint32_t* counter = c->counter.as<uint32_t>();
+  uint32_t* counter = c->pool_ctr.as<uint32_t>();
   SpillEntry* spill = c->spill.as<SpillEntry>();
   void* args[] = {&s, &rp, &nn, &max_toi, &lp, &tp, &chunk, &counter, &S, &spill, &lanes, &pause};
   HIP_TRY(hipModuleLaunchKernel(nw->fn, (unsigned)grid, 1, 1, B, 1, 1, (unsigned)lds, c->stream, args, nullptr));

@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "engine_internal.h"
+#include "pool.h"
 
 namespace {
 
@@ -125,6 +126,25 @@ __global__ __launch_bounds__(1024) void k_tlas_nearest(TlasView tv, const float*
     for (int k = 0; k < D; ++k) out.normal[(size_t)i * D + k] = nv[k];
   }
   if (out.sub) out.sub[i] = (hit && h.sub != NO_LEAF) ? (int32_t)h.sub : -1;
+}
+
+// Pooled, persistent form of k_tlas_nearest (pool_walk_tlas): waves take
+// chunks of `chunk` rays from counter[0] (zero at the start, left zero by
+// pool_retire), both stacks S-entry LDS rings spilling to their own columns.
+template <int D, int IK, bool ANY>
+__global__ __launch_bounds__(256) void k_tlas_nearest_pool(TlasView tv, const float* __restrict__ q, uint32_t n, float max_toi,
+                                                           const float* __restrict__ tmax, NearestOut out, uint32_t chunk,
+                                                           uint32_t* counter, uint32_t S, SpillEntry* spill_top,
+                                                           SpillEntry* spill_bot, uint32_t spill_stride, uint32_t pause_below) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  const uint32_t gt = blockIdx.x * B + tid;
+  RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill_top + gt, spill_stride};
+  RingStack sb{lds + 2 * S * B + tid, reinterpret_cast<float*>(lds + 3 * S * B) + tid, B, S, spill_bot + gt, spill_stride};
+  const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
+  const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
+  pool_walk_tlas<D, IK, ANY>(tv, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, out, st, sb, pause_below);
+  pool_retire(counter);
 }
 
 // MODE 0: raycast_all (one hit per instance, tlas.rs:158-215); MODE 2: overlap
@@ -333,16 +353,48 @@ static int tlas_nearest_impl(sq_tlas* t, const float* rays, uint64_t n, float ma
   }
   const int B = c->block;
   const uint32_t s_top = t->top->packed.max_stack, s_bot = t->blas_stack;
-  const size_t lds = (size_t)(s_top + s_bot) * B * 8;
-  if (lds > 160 * 1024) return fail(SQ_ERR_INVALID, "traversal stacks do not fit in LDS; lower the block size");
   TlasView tv = tview(t);
   int rc = begin_timing(c);
   if (rc) return rc;
+  if (c->rays_per_lane == 0) {
+    // one ray per thread, whole stacks in LDS
+    const size_t lds = (size_t)(s_top + s_bot) * B * 8;
+    if (lds > 160 * 1024) return fail(SQ_ERR_INVALID, "traversal stacks do not fit in LDS; lower the block size");
+    rc = dispatch_inst(t->in.dim, t->in.kind, [&]<int D, int K>() -> int {
+      if (any) k_tlas_nearest<D, K, true><<<grid_for(n, B), B, lds, c->stream>>>(tv, rays, (uint32_t)n, max_toi, tmax, out, s_top, s_bot);
+      else k_tlas_nearest<D, K, false><<<grid_for(n, B), B, lds, c->stream>>>(tv, rays, (uint32_t)n, max_toi, tmax, out, s_top, s_bot);
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    });
+    if (rc) return rc;
+    return end_timing(c);
+  }
+  // pooled, persistent (as nearest_impl): chunks of 64 * per rays, a grid of
+  // the resident waves, S-entry rings spilling to a top and a BLAS column
+  const uint32_t S = (uint32_t)c->lds_stack;
+  const uint32_t per = pooled_per_lane(c, n);
+  const uint64_t chunk = 64ull * per;
+  const uint64_t chunks = (n + chunk - 1) / chunk;
+  const uint32_t rows_top = s_top > S ? s_top - S : 0, rows_bot = s_bot > S ? s_bot - S : 0;
+  const size_t lds = (size_t)2 * S * B * 8;
   rc = dispatch_inst(t->in.dim, t->in.kind, [&]<int D, int K>() -> int {
-    if (any) k_tlas_nearest<D, K, true><<<grid_for(n, B), B, lds, c->stream>>>(tv, rays, (uint32_t)n, max_toi, tmax, out, s_top, s_bot);
-    else k_tlas_nearest<D, K, false><<<grid_for(n, B), B, lds, c->stream>>>(tv, rays, (uint32_t)n, max_toi, tmax, out, s_top, s_bot);
-    HIP_TRY(hipGetLastError());
-    return SQ_OK;
+    auto launch = [&]<bool ANY_>() -> int {
+      int per_cu = 0;
+      if (int r0 = blocks_per_cu(reinterpret_cast<const void*>(&k_tlas_nearest_pool<D, K, ANY_>), c->device, B, lds, &per_cu))
+        return r0;
+      const uint64_t wpb = (uint64_t)B / 64;
+      const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
+      const uint64_t lanes = grid * (uint64_t)B;
+      int r2 = c->spill.ensure((size_t)(rows_top + rows_bot) * lanes * sizeof(SpillEntry) + 64);
+      if (r2) return r2;
+      SpillEntry* sp = c->spill.as<SpillEntry>();
+      k_tlas_nearest_pool<D, K, ANY_><<<(uint32_t)grid, B, lds, c->stream>>>(
+          tv, rays, (uint32_t)n, max_toi, tmax, out, (uint32_t)chunk, c->pool_ctr.as<uint32_t>(), S, sp,
+          sp + (size_t)rows_top * lanes, (uint32_t)lanes, (uint32_t)c->descend_min);
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    };
+    return any ? launch.template operator()<true>() : launch.template operator()<false>();
   });
   if (rc) return rc;
   return end_timing(c);
