@@ -167,20 +167,34 @@ int sq_bvh_rebuild(sq_bvh* bvh, const float* prims_device);
  * use builtins.  SQ_ERR_INVALID with the compiler log in sq_last_error() when
  * it does not compile.  A hit counts only when its toi lies in the slab
  * interval of the primitive's own world box (the engine's order-independence
- * guard: a consumer whose hits lie in its boxes loses none). */
+ * guard: a consumer whose hits lie in its boxes loses none).
+ * The source may also define every crossing of a ray with a primitive
+ * (QueryGeometry::test_ray_crossings, geometry.rs:45-63):
+ *   #define SQ_NARROW_CROSSINGS 1
+ *   template <class Out> __device__ void sq_narrow_crossings(const float* prim,
+ *       const float* origin, const float* dir, float max_toi, Out& out);
+ * calling out(toi) once per crossing in any order (crossings outside the box
+ * interval are dropped by the same guard); without it a crossing query gets
+ * the sq_narrow hit, the reference's default. */
 typedef struct sq_narrow sq_narrow;
 int sq_narrow_create(sq_ctx* ctx, int dim, const char* source, uint32_t prim_floats, sq_narrow** out);
 int sq_narrow_destroy(sq_narrow* narrow);
 /* The tree over n consumer primitives (host memory): world boxes [n][2*dim]
  * (QueryGeometry::world_aabb) drive the reference median split; prims
  * [n][prim_floats] are what sq_narrow sees.  The tree answers only
- * sq_raycast_nearest_consumer with its narrow phase. */
+ * sq_raycast_nearest_consumer and sq_raycast_all_consumer with its narrow
+ * phase. */
 int sq_bvh_build_consumer(sq_narrow* narrow, const float* boxes, const float* prims, uint64_t n, sq_bvh** out);
 /* Nearest hit per ray through the consumer's narrow phase (raycast_nearest_batch,
  * bvh.rs:134-193, with the consumer's test_ray); device buffers as
  * sq_raycast_nearest: rays [n][2*dim], leaf (-1 on a miss), toi (+inf). */
 int sq_raycast_nearest_consumer(sq_narrow* narrow, sq_bvh* bvh, const float* rays, uint64_t n, float max_toi,
                                 int32_t* out_leaf, float* out_toi);
+/* Bvh::raycast_all (bvh.rs:196-259) with the consumer's test_ray, or
+ * raycast_crossings (bvh.rs:264-328) with its test_ray_crossings when
+ * crossings != 0: CSR results, capacity and total as sq_raycast_all. */
+int sq_raycast_all_consumer(sq_narrow* narrow, sq_bvh* bvh, const float* rays, uint64_t n, float max_toi, int crossings,
+                            uint64_t* offsets, int32_t* out_leaf, float* out_toi, uint64_t capacity, uint64_t* total);
 
 /* ---- ray queries (bvh.rs:51-328) ------------------------------------------ */
 /* max_toi_per_ray (device, n floats) overrides max_toi when not NULL.

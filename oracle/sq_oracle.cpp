@@ -266,6 +266,10 @@ enum Kind {
 
 // A consumer narrow phase: true with *toi on a hit of the primitive's floats.
 typedef int (*ConsumerNarrow)(const float* prim, const float* origin, const float* dir, float max_toi, float* toi);
+// A consumer's crossings (test_ray_crossings, geometry.rs:45-63): calls
+// emit(ctx, toi) once per crossing of the ray with the primitive.
+typedef void (*ConsumerCrossings)(const float* prim, const float* origin, const float* dir, float max_toi,
+                                  void (*emit)(void*, float), void* ctx);
 
 template <int D> struct Geometry {
   int kind;
@@ -273,6 +277,7 @@ template <int D> struct Geometry {
   std::vector<float> prims;      // BOX: min D, max D; SPHERE: center D, radius; TRI: v0 v1 v2
   std::vector<uint64_t> layers;  // empty = every leaf on every layer
   ConsumerNarrow consumer = nullptr;   // KIND_CONSUMER: records are [world box 2D][consumer_pf floats]
+  ConsumerCrossings consumer_x = nullptr;   // its crossings, when it reports more than the nearest hit
   int consumer_pf = 0;
   int stride() const {
     if (kind == KIND_CONSUMER) return 2 * D + consumer_pf;
@@ -422,6 +427,19 @@ template <int D> struct Geometry {
   // Crossings (geometry.rs:45-55 default = the single test_ray hit).  Boxes and
   // balls are convex: report entry and exit.
   template <class F> void test_ray_crossings(size_t leaf, const Ray<D>& ray, float max_toi, F&& out) const {
+    if (kind == KIND_CONSUMER && consumer_x) {
+      // each crossing kept under the test_ray contract and the box guard, as
+      // the consumer's test_ray hits; emitted in the consumer's order
+      std::vector<float> ts;
+      consumer_x(p(leaf) + 2 * D, ray.origin.v, ray.dir.v, max_toi,
+                 [](void* c, float t) { static_cast<std::vector<float>*>(c)->push_back(t); }, &ts);
+      for (float t : ts) {
+        if (!(t >= 0.0f && t <= max_toi) || !guard(leaf, ray, max_toi, t)) continue;
+        LeafHit<D> h; h.toi = t; h.normal = Point<D>::zero(); h.sub = -1;
+        out(h);
+      }
+      return;
+    }
     if (kind == KIND_SHELL) {   // crossings.rs:87-110
       const float* q = p(leaf);
       Point<D> c; for (int i = 0; i < D; ++i) c.v[i] = q[i];
@@ -1162,6 +1180,11 @@ void* orc_geom_new_consumer(int D, const float* records, uint32_t n, int pf, uin
     g->p = p;
   });
   return g;
+}
+// Adds the consumer's crossings function (an address; 0 = the nearest hit only).
+void orc_geom_set_consumer_crossings(void* h, uint64_t fn) {
+  OGeom* g = (OGeom*)h;
+  ORC_DISPATCH(g->D, { ((Geometry<DD>*)g->p)->consumer_x = reinterpret_cast<ConsumerCrossings>(fn); });
 }
 void orc_geom_set_layers(void* h, const uint64_t* layers) {
   OGeom* g = (OGeom*)h;

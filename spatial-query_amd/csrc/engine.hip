@@ -646,13 +646,6 @@ int csr_run(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* 
 // staging area already holds): 16M entries, 256 MiB of blocks.
 static constexpr uint64_t STAGE_FIRST = 1ull << 24;
 
-// Where staged entries of sub-walks find their query and place (null: the
-// entries carry their query and index).
-struct StagedWalk {
-  const WorkItem* items;
-  const uint32_t* item_base;
-};
-
 // Pooled collect: launch(want_entries, want_blocks, walk) walks every query once,
 // appending its entries to staging blocks sized for at least want_entries and,
 // when it splits queries, writing sub-walks to item blocks (at least

@@ -277,7 +277,6 @@ inline int upload(DevBuf& d, const void* h, size_t bytes, hipStream_t s) {
 // device): *order (context buffer) lists query indices in key order.
 int ray_order(sq_ctx* c, int dim, const float* q, uint32_t n, uint32_t qs, const float* box, const uint32_t** order);
 
-// Exclusive scan of n counts into offsets[0..n] on the context stream.
 // Writes misses (leaf -1, toi +inf, zero point/normal, sub -1, any 0).
 int fill_miss(sq_ctx* c, uint64_t n, int dim, NearestOut out);
 
@@ -286,3 +285,21 @@ int fill_miss(sq_ctx* c, uint64_t n, int dim, NearestOut out);
 // (fill = false, writing ctx->counts) or the fill pass (fill = true).
 int csr_run(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* out_toi, bool has_t,
             uint64_t capacity, uint64_t* total, const std::function<int(bool, uint64_t*)>& launch);
+
+// Where staged entries of sub-walks find their query and place (null: the
+// entries carry their query and index).
+namespace sqd { struct WorkItem; }
+struct StagedWalk {
+  const sqd::WorkItem* items;
+  const uint32_t* item_base;
+};
+
+// Variable-length results from one pooled walk per pass (pool_collect):
+// launch(want_entries, want_blocks, walk) enqueues a walk appending to the
+// context's staging blocks (ctx->stage, ctx->stage_used; room for at least
+// want_entries), per-query counts to ctx->counts and the words of ctx->counter
+// ([1] staging blocks claimed, [2] staging overflow, [4] item overflow, [6] and
+// [8] item blocks claimed); then scan, capacity check, scatter, per-query sort,
+// unpack, with the walk repeated when its staging or items did not fit.
+int csr_run_staged(sq_ctx* c, uint64_t n, uint64_t* offsets, int32_t* out_leaf, float* out_toi, bool has_t,
+                   uint64_t capacity, uint64_t* total, const std::function<int(uint64_t, uint64_t, StagedWalk&)>& launch);

@@ -585,15 +585,16 @@ struct BlockChunks {
 // ITEMS, position i is the sub-walk items[i] (src hands out filled parts of
 // item blocks), its entries carry (i, index in the sub-walk's list) and a
 // finished sub-walk writes its entry count to item_cnt[i].
-template <int D, int KIND, int MODE, bool FILT, bool ITEMS, class Src>
-__device__ __forceinline__ void pool_collect(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
-                                             const float* __restrict__ tmax, const Filter& f, const RingStack& st,
-                                             const StageAppend& app, uint32_t pause_below,
-                                             const WorkItem* __restrict__ items, uint32_t* item_cnt) {
+// CQ is the per-query state (begin / pair / leaf as CollectQuery).
+template <int D, class CQ, bool ITEMS, class Src>
+__device__ __forceinline__ void pool_collect_with(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
+                                                  const float* __restrict__ tmax, const Filter& f, const RingStack& st,
+                                                  const StageAppend& app, uint32_t pause_below,
+                                                  const WorkItem* __restrict__ items, uint32_t* item_cnt) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   const uint32_t root_ref = fbits(s.root[2 * D]);
-  CollectQuery<D, KIND, MODE, FILT> cq;
+  CQ cq;
   VisitState v{root_ref, 0};
   uint32_t rid = NO_LEAF, cnt = 0, pos = 0;
   uint64_t next = 0, end = 0;
@@ -651,6 +652,16 @@ __device__ __forceinline__ void pool_collect(const SceneView& s, const float* __
     rid = NO_LEAF;
   }
   if (lane == 0) app.close();
+}
+
+// The built-in kinds and modes.
+template <int D, int KIND, int MODE, bool FILT, bool ITEMS, class Src>
+__device__ __forceinline__ void pool_collect(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
+                                             const float* __restrict__ tmax, const Filter& f, const RingStack& st,
+                                             const StageAppend& app, uint32_t pause_below,
+                                             const WorkItem* __restrict__ items, uint32_t* item_cnt) {
+  pool_collect_with<D, CollectQuery<D, KIND, MODE, FILT>, ITEMS>(s, q, src, max_toi, tmax, f, st, app, pause_below, items,
+                                                                 item_cnt);
 }
 
 }  // namespace sqd

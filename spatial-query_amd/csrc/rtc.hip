@@ -9,7 +9,14 @@
 //   __device__ bool sq_narrow(const float* prim, const float* origin,
 //                             const float* dir, float max_toi, float* toi);
 // (QueryGeometry::test_ray, geometry.rs:33-43: a hit at toi in [0, max_toi]
-// or none).  The tree is built from the consumer's world boxes
+// or none) and, optionally, every crossing of the ray with the primitive
+// (QueryGeometry::test_ray_crossings, geometry.rs:45-63), announced by
+//   #define SQ_NARROW_CROSSINGS 1
+//   template <class Out> __device__ void sq_narrow_crossings(const float* prim,
+//       const float* origin, const float* dir, float max_toi, Out& out);
+// calling out(toi) once per crossing, in any order.  Without it, crossings
+// are the single sq_narrow hit (the reference's default).  The same module
+// holds the all-hits and crossings walks (pool_collect_with).  The tree is built from the consumer's world boxes
 // (QueryGeometry::world_aabb); each leaf-ordered primitive record is its box
 // followed by its prim_floats floats, of which sq_narrow sees the latter.  As
 // for the built-in SPHERE and TRIANGLE kinds, a hit is kept only when its toi
@@ -24,7 +31,11 @@
 #include <string>
 #include <vector>
 
+#include <map>
+#include <tuple>
+
 #include "engine_internal.h"
+#include "pool.h"
 #include "rtc_sources.inc"    // kTraverseSrc, kPoolSrc: traverse.h and pool.h as string literals (built by the Makefile)
 
 namespace {
@@ -78,6 +89,88 @@ extern "C" __global__ __launch_bounds__(256) void sq_consumer_nearest(sqd::Scene
                                                 out, st, nullptr, pause_below);
   sqd::pool_retire(counter);
 }
+
+namespace sqd {
+// Per-query state of an all-hits (CROSS false) or crossings (CROSS true) walk
+// over consumer primitives: every primitive of every leaf range the ray
+// reaches within max_toi is tested (no best-toi pruning), and each hit or
+// crossing kept inside the slab interval of the primitive's own box is
+// appended with its (toi, leaf) key.
+template <int D, bool CROSS> struct ConsumerCollect {
+  ConsumerPolicy<D> pol;
+  float lim;
+  SQ_HD bool begin(const SceneView& s, const float* q, unsigned i, float max_toi, const float*) {
+    pol.setup(q + (size_t)i * 2 * D);
+    lim = max_toi;
+    return pol.root(s.root, lim);
+  }
+  SQ_HD void pair(const float* nd, bool& ha, bool& hb) const {
+    float ta, tb;
+    pol.pair(nd, lim, ha, hb, ta, tb);
+  }
+  template <class Emit>
+  SQ_HD void leaf(const SceneView& s, const Filter&, unsigned first, unsigned c, Emit&& emit) const {
+    for (unsigned j = 0; j < c; ++j) {
+      const unsigned pos = first + j;
+      const float* p = s.prims + (size_t)pos * (2 * D + SQ_RTC_PF);
+      const unsigned leaf = s.ids[pos];
+#if defined(SQ_NARROW_CROSSINGS)
+      if constexpr (CROSS) {
+        float te, tx; int ax;
+        if (!slab_exact<D>(pol.r, p, p + D, lim, te, tx, ax)) continue;
+        auto out = [&](float t) {
+          if (te <= t && t <= tx) emit(hit_key(t, leaf));
+        };
+        sq_narrow_crossings(p + 2 * D, pol.r.o, pol.r.d, lim, out);
+        continue;
+      }
+#endif
+      float t;
+      if (pol.narrow(p, lim, t)) emit(hit_key(t, leaf));
+    }
+  }
+};
+
+template <bool CROSS>
+__device__ __forceinline__ void consumer_collect(const SceneView& s, const float* rays, unsigned n, float max_toi,
+                                                 unsigned chunk, unsigned* counter, unsigned S, SpillEntry* spill,
+                                                 unsigned spill_stride, unsigned pause_below, const StageOut& o) {
+  extern __shared__ unsigned lds[];
+  const unsigned B = blockDim.x, tid = threadIdx.x;
+  const unsigned gtid = blockIdx.x * B + tid;
+  RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
+  volatile unsigned* slot = lds + 2 * S * B + 2 * (tid >> 6);
+  if ((tid & 63u) == 0) {
+    slot[0] = NO_BLOCK;
+    slot[1] = 0;
+  }
+  const unsigned long long waves = (unsigned long long)gridDim.x * (B >> 6);
+  const unsigned long long home = (unsigned long long)blockIdx.x * (B >> 6) + (tid >> 6);
+  const Filter f{nullptr, ~0ull, -1};
+  pool_collect_with<SQ_RTC_D, ConsumerCollect<SQ_RTC_D, CROSS>, false>(s, rays, Chunks{counter, n, chunk, home, waves},
+                                                                        max_toi, nullptr, f, st, StageAppend{o, slot},
+                                                                        pause_below, nullptr, nullptr);
+}
+}  // namespace sqd
+
+// All hits / all crossings of each ray (pooled collect): waves take chunks of
+// `chunk` rays from counter[0] (cleared by the host with the staging words:
+// counter[1] blocks claimed, counter[2] overflow); entries go to the staging
+// blocks of o and per-ray counts to o.counts.  LDS: the ring stacks, then two
+// words per wave.
+extern "C" __global__ __launch_bounds__(256) void sq_consumer_all(sqd::SceneView s, const float* rays, unsigned n,
+                                                                  float max_toi, unsigned chunk, unsigned* counter,
+                                                                  unsigned S, sqd::SpillEntry* spill, unsigned spill_stride,
+                                                                  unsigned pause_below, sqd::StageOut o) {
+  sqd::consumer_collect<false>(s, rays, n, max_toi, chunk, counter, S, spill, spill_stride, pause_below, o);
+}
+extern "C" __global__ __launch_bounds__(256) void sq_consumer_crossings(sqd::SceneView s, const float* rays, unsigned n,
+                                                                        float max_toi, unsigned chunk, unsigned* counter,
+                                                                        unsigned S, sqd::SpillEntry* spill,
+                                                                        unsigned spill_stride, unsigned pause_below,
+                                                                        sqd::StageOut o) {
+  sqd::consumer_collect<true>(s, rays, n, max_toi, chunk, counter, S, spill, spill_stride, pause_below, o);
+}
 )SQRTC";
 
 }  // namespace
@@ -87,12 +180,24 @@ struct sq_narrow {
   int dim = 0;
   uint32_t pf = 0;
   hipModule_t mod = nullptr;
-  hipFunction_t fn = nullptr;
-  // resident blocks per CU, cached for the last (block, LDS) pair asked
-  uint32_t occ_block = 0;
-  size_t occ_lds = 0;
-  int occ_per_cu = -1;
+  hipFunction_t fn = nullptr;          // nearest hit
+  hipFunction_t fn_all = nullptr;      // all hits
+  hipFunction_t fn_cross = nullptr;    // all crossings
+  // resident blocks per CU by (kernel, block, LDS), queried once each
+  std::map<std::tuple<hipFunction_t, uint32_t, size_t>, int> occ;
 };
+
+// Resident blocks per CU of one of the module's kernels.
+static int module_blocks_per_cu(sq_narrow* nw, hipFunction_t fn, uint32_t block, size_t lds, int* out) {
+  const auto key = std::make_tuple(fn, block, lds);
+  auto it = nw->occ.find(key);
+  if (it != nw->occ.end()) { *out = it->second; return SQ_OK; }
+  int per_cu = 0;
+  HIP_TRY(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)block, lds));
+  nw->occ[key] = per_cu;
+  *out = per_cu;
+  return SQ_OK;
+}
 
 extern "C" {
 
@@ -131,7 +236,9 @@ int sq_narrow_create(sq_ctx* ctx, int dim, const char* source, uint32_t prim_flo
   nw->dim = dim;
   nw->pf = prim_floats;
   HIP_TRY(hipModuleLoadData(&nw->mod, code.data()));
-  if (hipModuleGetFunction(&nw->fn, nw->mod, "sq_consumer_nearest") != hipSuccess) {
+  if (hipModuleGetFunction(&nw->fn, nw->mod, "sq_consumer_nearest") != hipSuccess ||
+      hipModuleGetFunction(&nw->fn_all, nw->mod, "sq_consumer_all") != hipSuccess ||
+      hipModuleGetFunction(&nw->fn_cross, nw->mod, "sq_consumer_crossings") != hipSuccess) {
     (void)hipModuleUnload(nw->mod);
     return fail(SQ_ERR_HIP, "consumer kernel not found in the compiled module");
   }
@@ -210,18 +317,11 @@ int sq_raycast_nearest_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uin
   const uint32_t spill_rows = stack_n > S ? stack_n - S : 0;
   const size_t lds = (size_t)S * B * 8;
   int per_cu = 0;
-  if (nw->occ_per_cu >= 0 && nw->occ_block == B && nw->occ_lds == lds) {
-    per_cu = nw->occ_per_cu;
-  } else {
-    HIP_TRY(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, nw->fn, (int)B, lds));
-    nw->occ_block = B;
-    nw->occ_lds = lds;
-    nw->occ_per_cu = per_cu;
-  }
+  int rc;
+  if ((rc = module_blocks_per_cu(nw, nw->fn, B, lds, &per_cu))) return rc;
   const uint64_t wpb = B / 64;
   const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
   uint32_t lanes = (uint32_t)(grid * B);
-  int rc;
   if ((rc = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64))) return rc;
   SceneView s = view_of(b);
   const float* rp = rays;
@@ -233,6 +333,62 @@ int sq_raycast_nearest_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uin
   void* args[] = {&s, &rp, &nn, &max_toi, &lp, &tp, &chunk, &counter, &S, &spill, &lanes, &pause};
   HIP_TRY(hipModuleLaunchKernel(nw->fn, (unsigned)grid, 1, 1, B, 1, 1, (unsigned)lds, c->stream, args, nullptr));
   return SQ_OK;
+}
+
+int sq_raycast_all_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uint64_t n, float max_toi, int crossings,
+                            uint64_t* offsets, int32_t* out_leaf, float* out_toi, uint64_t capacity, uint64_t* total) {
+  if (!nw || !b || !offsets || !total || (n && !rays)) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (b->kind != KIND_CONSUMER || b->dim != nw->dim || b->consumer_pf != nw->pf || b->ctx != nw->ctx)
+    return fail(SQ_ERR_INVALID, "the tree was not built for this narrow phase");
+  if (n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "at most 2^32-1 queries per call");
+  sq_ctx* c = b->ctx;
+  *total = 0;
+  if (!n || !b->n_prims) {
+    HIP_TRY(hipMemsetAsync(offsets, 0, (n + 1) * 8, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SQ_OK;
+  }
+  int rc;
+  if ((rc = c->counts.ensure(n * 4))) return rc;
+  if ((rc = c->counter.ensure(256))) return rc;
+  // the pooled collect of the built-in kinds (collect_impl, no sub-walks)
+  const uint32_t B = (uint32_t)c->block;
+  uint32_t S = (uint32_t)c->lds_stack;
+  const uint32_t per = pooled_per_lane(c, n);
+  const uint64_t chunk64 = 64ull * per;
+  const uint64_t chunks = (n + chunk64 - 1) / chunk64;
+  const uint32_t stack_n = b->packed.max_stack;
+  const uint32_t spill_rows = stack_n > S ? stack_n - S : 0;
+  const size_t lds = (size_t)S * B * 8 + (size_t)(B / 64) * 8;
+  hipFunction_t fn = crossings ? nw->fn_cross : nw->fn_all;
+  int per_cu = 0;
+  if ((rc = module_blocks_per_cu(nw, fn, B, lds, &per_cu))) return rc;
+  const uint64_t wpb = B / 64;
+  const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
+  uint32_t lanes = (uint32_t)(grid * B);
+  auto launch = [&](uint64_t want, uint64_t, StagedWalk& walk) -> int {
+    int r;
+    if ((r = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64))) return r;
+    // retired blocks hold over STAGE_BLK - 64 entries; each wave has one more open
+    const uint64_t blocks = std::min<uint64_t>(want / (STAGE_BLK - 64) + 1 + grid * wpb, 0xFFFFFFF0ull);
+    if ((r = c->stage.ensure(blocks * STAGE_BLK * sizeof(StageEntry)))) return r;
+    if ((r = c->stage_used.ensure(blocks * 4))) return r;
+    const uint64_t max_blocks =
+        std::min<uint64_t>({c->stage.bytes / (STAGE_BLK * sizeof(StageEntry)), c->stage_used.bytes / 4, 0xFFFFFFF0ull});
+    uint32_t* counter = c->counter.as<uint32_t>();
+    HIP_TRY(hipMemsetAsync(counter, 0, 64, c->stream));
+    StageOut o{c->stage.as<StageEntry>(), c->stage_used.as<uint32_t>(), counter + 1, counter + 2, (uint32_t)max_blocks,
+               c->counts.as<uint32_t>()};
+    SceneView s = view_of(b);
+    const float* rp = rays;
+    uint32_t nn = (uint32_t)n, chunk = (uint32_t)chunk64, pause = (uint32_t)c->descend_min;
+    SpillEntry* spill = c->spill.as<SpillEntry>();
+    void* args[] = {&s, &rp, &nn, &max_toi, &chunk, &counter, &S, &spill, &lanes, &pause, &o};
+    HIP_TRY(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, B, 1, 1, (unsigned)lds, c->stream, args, nullptr));
+    walk = StagedWalk{nullptr, nullptr};
+    return SQ_OK;
+  };
+  return csr_run_staged(c, n, offsets, out_leaf, out_toi, true, capacity, total, launch);
 }
 
 }  // extern "C"

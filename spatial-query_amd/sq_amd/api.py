@@ -469,8 +469,9 @@ class Narrow:
 
 
 class ConsumerBvh:
-    """A tree over consumer primitives; answers nearest-hit rays through its
-    Narrow (raycast_nearest_batch with the consumer's test_ray)."""
+    """A tree over consumer primitives; answers rays through its Narrow:
+    nearest hits (raycast_nearest_batch with the consumer's test_ray), all
+    hits and all crossings (raycast_all / raycast_crossings)."""
 
     def __init__(self, narrow: Narrow, handle, n: int):
         self.narrow, self._h, self.n_prims, self.dim = narrow, handle, n, narrow.dim
@@ -487,6 +488,15 @@ class ConsumerBvh:
         toi = torch.empty(n, dtype=torch.float32, device=rays.device)
         check(lib().sq_raycast_nearest_consumer(self.narrow._h, self._h, _dp(rays), n, float(max_toi), _dp(leaf), _dp(toi)))
         return {"leaf": leaf, "toi": toi}
+
+    _csr = DeviceBvh._csr
+
+    def raycast_all(self, rays, max_toi, crossings=False, capacity=0):
+        """CSR (offsets, leaf, toi), each ray's hits sorted by (toi, leaf)."""
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_raycast_all_consumer(self.narrow._h, self._h, _dp(rays), rays.shape[0], float(max_toi),
+                                                 int(crossings), _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+        return self._csr(fn, rays.shape[0], rays.device, True, capacity)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -49,12 +49,122 @@ __device__ bool sq_narrow(const float* p, const float* o, const float* d, float 
 """
 
 
-def compile_cpu(source: str, dim: int, pf: int, workdir: str):
+# The spheres of crossings.rs:38-111 (Shells): both roots of the unbounded ray
+# are crossings; the nearest hit is the first root at or after the origin.
+# Records: centre [D], radius.
+SHELL = r"""
+#define SQ_NARROW_CROSSINGS 1
+__device__ bool shell_roots(const float* p, const float* o, const float* d, float* t0, float* t1) {
+  float oc[SQ_RTC_D];
+  for (int k = 0; k < SQ_RTC_D; ++k) oc[k] = o[k] - p[k];
+  float b = 0.0f, l = 0.0f;
+  for (int k = 0; k < SQ_RTC_D; ++k) b = b + oc[k] * d[k];
+  for (int k = 0; k < SQ_RTC_D; ++k) l = l + oc[k] * oc[k];
+  const float disc = b * b - (l - p[SQ_RTC_D] * p[SQ_RTC_D]);
+  if (disc < 0.0f) return false;
+  const float sq = __builtin_sqrtf(disc);
+  *t0 = -b - sq;
+  *t1 = -b + sq;
+  return true;
+}
+__device__ bool sq_narrow(const float* p, const float* o, const float* d, float max_toi, float* toi) {
+  float t0, t1;
+  if (!shell_roots(p, o, d, &t0, &t1)) return false;
+  const float t = t0 >= 0.0f ? t0 : t1;
+  if (t < 0.0f || t > max_toi) return false;
+  *toi = t;
+  return true;
+}
+template <class Out>
+__device__ void sq_narrow_crossings(const float* p, const float* o, const float* d, float max_toi, Out& out) {
+  float t0, t1;
+  if (!shell_roots(p, o, d, &t0, &t1)) return;
+  if (t0 >= 0.0f && t0 <= max_toi) out(t0);
+  if (t1 >= 0.0f && t1 <= max_toi) out(t1);
+}
+"""
+
+# The implicit blobs of field.rs:40-150 (D = 3): the field |x - c|^2 - r0^2 +
+# amp (sin(f x0) + sin(f x1) + sin(f x2)), marched from the origin in steps
+# of 0.05 to max_toi, every sign change bisected 40 times to a crossing; the
+# nearest hit is the first crossing.  Records: centre [3], r0, amp, freq.
+# The sine is a polynomial of plain operations (reduction to [-pi, pi], odd
+# Taylor terms to x^17), so the device and g++ builds compute the same bits.
+BLOB = r"""
+#define SQ_NARROW_CROSSINGS 1
+__device__ float blob_sin(float x) {
+  const float k = __builtin_floorf(x * 0.15915494f + 0.5f);
+  const float r = (x - k * 6.28125f) - k * 0.0019353072f;
+  const float r2 = r * r;
+  float s = 1.0f / 355687428096000.0f;
+  s = s * r2 - 1.0f / 1307674368000.0f;
+  s = s * r2 + 1.0f / 6227020800.0f;
+  s = s * r2 - 1.0f / 39916800.0f;
+  s = s * r2 + 1.0f / 362880.0f;
+  s = s * r2 - 1.0f / 5040.0f;
+  s = s * r2 + 1.0f / 120.0f;
+  s = s * r2 - 1.0f / 6.0f;
+  s = s * r2 + 1.0f;
+  return s * r;
+}
+__device__ float blob_field(const float* p, const float* o, const float* d, float t) {
+  float x[3];
+  for (int k = 0; k < 3; ++k) x[k] = (o[k] + d[k] * t) - p[k];
+  const float base = (((0.0f + x[0] * x[0]) + x[1] * x[1]) + x[2] * x[2]) - p[3] * p[3];
+  const float bump = p[4] * ((blob_sin(p[5] * x[0]) + blob_sin(p[5] * x[1])) + blob_sin(p[5] * x[2]));
+  return base + bump;
+}
+template <class Out>
+__device__ void sq_narrow_crossings(const float* p, const float* o, const float* d, float max_toi, Out& out) {
+  const float step = 0.05f;
+  float t = 0.0f;
+  float fp = blob_field(p, o, d, t);
+  while (t < max_toi) {
+    float tn = t + step;
+    if (tn > max_toi) tn = max_toi;
+    const float fn = blob_field(p, o, d, tn);
+    if (fp != 0.0f && ((fp < 0.0f) != (fn < 0.0f))) {
+      float a = t, c = tn, fa = fp;
+      for (int i = 0; i < 40; ++i) {
+        const float m = 0.5f * (a + c);
+        const float fm = blob_field(p, o, d, m);
+        if ((fa < 0.0f) != (fm < 0.0f)) c = m;
+        else { a = m; fa = fm; }
+      }
+      out(0.5f * (a + c));
+    }
+    if (tn >= max_toi) break;
+    t = tn;
+    fp = fn;
+  }
+}
+struct BlobFirst {
+  float t;
+  bool hit;
+  __device__ void operator()(float x) { if (!hit) { t = x; hit = true; } }
+};
+__device__ bool sq_narrow(const float* p, const float* o, const float* d, float max_toi, float* toi) {
+  BlobFirst f{0.0f, false};
+  sq_narrow_crossings(p, o, d, max_toi, f);
+  if (!f.hit) return false;
+  *toi = f.t;
+  return true;
+}
+"""
+
+
+def cpu_entry_points(source: str, dim: int, pf: int, workdir: str):
     """The consumer source built for the CPU (g++ -O2 -ffp-contract=off) as a
-    shared object; returns (library, address of its C entry point)."""
+    shared object; returns (library, address of its narrow entry point,
+    address of its crossings entry point or 0 when the source has none)."""
     code = (f"#define __device__\n#define SQ_RTC_D {dim}\n#define SQ_RTC_PF {pf}\n" + source +
             '\nextern "C" int sq_narrow_c(const float* p, const float* o, const float* d, float m, float* t) '
-            "{ return sq_narrow(p, o, d, m, t) ? 1 : 0; }\n")
+            "{ return sq_narrow(p, o, d, m, t) ? 1 : 0; }\n"
+            "#ifdef SQ_NARROW_CROSSINGS\n"
+            "struct SqEmitC { void (*f)(void*, float); void* c; void operator()(float t) const { f(c, t); } };\n"
+            'extern "C" void sq_crossings_c(const float* p, const float* o, const float* d, float m, '
+            "void (*f)(void*, float), void* c) { SqEmitC e{f, c}; sq_narrow_crossings(p, o, d, m, e); }\n"
+            "#endif\n")
     # one file name per source and shape: dlopen returns an already loaded
     # library for a path it has seen
     tag = hashlib.sha1(code.encode()).hexdigest()[:12]
@@ -64,4 +174,11 @@ def compile_cpu(source: str, dim: int, pf: int, workdir: str):
         f.write(code)
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-o", so, src], check=True)
     lib = ctypes.CDLL(so)
-    return lib, ctypes.cast(lib.sq_narrow_c, ctypes.c_void_p).value
+    x = ctypes.cast(lib.sq_crossings_c, ctypes.c_void_p).value if hasattr(lib, "sq_crossings_c") else 0
+    return lib, ctypes.cast(lib.sq_narrow_c, ctypes.c_void_p).value, x
+
+
+def compile_cpu(source: str, dim: int, pf: int, workdir: str):
+    """As cpu_entry_points: (library, address of its narrow entry point)."""
+    lib, addr, _ = cpu_entry_points(source, dim, pf, workdir)
+    return lib, addr

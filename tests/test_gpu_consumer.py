@@ -6,7 +6,13 @@ the nearest-hit walk, against the CPU oracle running the same source.
   * a consumer ellipsoid equals the oracle's consumer kind (reference LIFO
     walk, same source built by g++) bit for bit;
   * a source that does not compile reports the compiler's log; a consumer tree
-    refuses the built-in queries; an empty tree answers misses.
+    refuses the built-in queries; an empty tree answers misses;
+  * all hits and crossings (pooled collect through the consumer): the ball
+    consumer's all-hits equal the built-in SPHERE's bit for bit (and its
+    crossings, with no crossings function, equal its all-hits); the shells of
+    crossings.rs and the marched implicit blobs of field.rs give the oracle's
+    consumer crossings (same source built by g++) bit for bit, CSR offsets,
+    leaf order and toi, and the blobs' nearest hits equal the oracle's.
 """
 import numpy as np
 import pytest
@@ -15,7 +21,7 @@ import oracle
 import sq_amd
 from sq_amd import workloads as W
 
-from consumer_narrow import BALL, ELLIPSOID, compile_cpu
+from consumer_narrow import BALL, BLOB, ELLIPSOID, SHELL, compile_cpu, cpu_entry_points
 from helpers import assert_nearest_equal
 
 pytestmark = pytest.mark.gpu
@@ -108,4 +114,91 @@ def test_consumer_errors_and_empty_tree(ctx):
     cb.close()
     empty.close()
     other.close()
+    nw.close()
+
+
+def csr_host(off, leaf, toi):
+    return host(off).astype(np.uint64), host(leaf).astype(np.int64), host(toi)
+
+
+def assert_csr_equal(got, ref, what):
+    go, gl, gt = got
+    ro, rl, rt = ref
+    assert np.array_equal(go, ro.astype(np.uint64)), f"{what}: offsets differ"
+    assert np.array_equal(gl, rl), f"{what}: leaf ids differ"
+    assert np.array_equal(gt.view(np.uint32), rt.view(np.uint32)), f"{what}: toi differs"
+
+
+def test_consumer_all_hits_ball_equals_builtin(ctx):
+    print("case: consumer ball all-hits vs built-in SPHERE", flush=True)
+    b = balls(20000, 3, 0xDA, 40.0)
+    boxes = np.concatenate([b[:, :3] - b[:, 3:], b[:, :3] + b[:, 3:]], 1).astype(np.float32)
+    rays = dev(W.random_rays(20000, 0xDB, -45.0, 45.0))
+    nw = sq_amd.Narrow(ctx, 3, BALL, 4)
+    cb = nw.build(boxes, b)
+    got = csr_host(*cb.raycast_all(rays, 300.0))
+    builtin = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.SPHERE, b)
+    ref = csr_host(*builtin.raycast_all(rays, 300.0))
+    assert ref[1].shape[0] > 5000
+    assert_csr_equal(got, ref, "ball all-hits")
+    # no crossings function: the crossings are the test_ray hits
+    assert_csr_equal(csr_host(*cb.raycast_all(rays, 300.0, crossings=True)), ref, "ball crossings")
+    # a capacity too small is reported and the Python side retries with the total
+    small = csr_host(*cb.raycast_all(rays, 300.0, capacity=16))
+    assert_csr_equal(small, ref, "ball all-hits from capacity 16")
+    cb.close()
+    nw.close()
+
+
+@pytest.mark.parametrize("dim", [3, 4])
+def test_consumer_shell_crossings_match_oracle(ctx, tmp_path, dim):
+    print(f"case: consumer shell crossings D={dim}", flush=True)
+    s = W.Stream(0xDC + dim)
+    n = 30000
+    c = s.range(-60.0, 60.0, n * dim).reshape(-1, dim)
+    r = s.range(1.0, 4.0, n).reshape(-1, 1)
+    sh = np.concatenate([c, r], 1).astype(np.float32)
+    boxes = np.concatenate([c - r, c + r], 1).astype(np.float32)
+    rays = W.random_rays(8000, 0xDD + dim, -65.0, 65.0, dim=dim)
+    nw = sq_amd.Narrow(ctx, dim, SHELL, dim + 1)
+    cb = nw.build(boxes, sh)
+    got = csr_host(*cb.raycast_all(dev(rays), 500.0, crossings=True))
+    lib, addr, xaddr = cpu_entry_points(SHELL, dim, dim + 1, str(tmp_path))
+    sc = oracle.Scene.consumer(dim, boxes, sh, addr, xaddr)
+    ref = sc.raycast_all(rays, 500.0, crossings=True)
+    assert ref[1].shape[0] > 3000
+    assert_csr_equal(got, ref, f"shell crossings D={dim}")
+    # nearest: the first root at or after the origin
+    near = cb.raycast_nearest(dev(rays), 500.0)
+    rn = sc.raycast_nearest(rays, 500.0, threads=8)
+    assert_nearest_equal(host(near["leaf"]), host(near["toi"]), rn["leaf"], rn["t"], f"shell nearest D={dim}")
+    cb.close()
+    nw.close()
+
+
+def test_consumer_blob_crossings_match_oracle(ctx, tmp_path):
+    print("case: consumer implicit blobs (marched field)", flush=True)
+    s = W.Stream(0xDE)
+    n = 400
+    c = s.range(-40.0, 40.0, n * 3).reshape(n, 3)
+    r0 = s.range(2.0, 5.0, n).reshape(n, 1)
+    amp = s.range(0.0, 2.0, n).reshape(n, 1)
+    freq = s.range(0.5, 1.5, n).reshape(n, 1)
+    prims = np.concatenate([c, r0, amp, freq], 1).astype(np.float32)
+    h = (r0 + amp + 1.0).astype(np.float32)
+    boxes = np.concatenate([c - h, c + h], 1).astype(np.float32)
+    rays = W.random_rays(3000, 0xDF, -45.0, 45.0)
+    nw = sq_amd.Narrow(ctx, 3, BLOB, 6)
+    cb = nw.build(boxes, prims)
+    lib, addr, xaddr = cpu_entry_points(BLOB, 3, 6, str(tmp_path))
+    sc = oracle.Scene.consumer(3, boxes, prims, addr, xaddr)
+    got = csr_host(*cb.raycast_all(dev(rays), 120.0, crossings=True))
+    ref = sc.raycast_all(rays, 120.0, crossings=True)
+    assert ref[1].shape[0] > 1000
+    assert_csr_equal(got, ref, "blob crossings")
+    near = cb.raycast_nearest(dev(rays), 120.0)
+    rn = sc.raycast_nearest(rays, 120.0, threads=8)
+    assert (rn["leaf"] >= 0).sum() > 300
+    assert_nearest_equal(host(near["leaf"]), host(near["toi"]), rn["leaf"], rn["t"], "blob nearest")
+    cb.close()
     nw.close()
