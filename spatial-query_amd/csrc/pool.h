@@ -62,6 +62,72 @@ struct OnePool {
     return first < end;
   }
 };
+// Chunks in per-XCD pools.  Workgroups go to the chip's XCDs (each with its
+// own L2) round-robin by block index, so XCD x holds blocks x, x + X, ...;
+// here it also owns every X-th stripe of XCD_STRIPE consecutive chunks
+// (stripes x, x + X, ...), walked first (home chunk, then its own counter)
+// before it steals from the other XCDs' stripes in turn.  Neighbouring (often
+// coherent) chunks of a stripe then share one L2, while every XCD's share
+// still spans the whole query range (a contiguous share per XCD would hand
+// whole regions of unequal cost to single XCDs).  Counter of XCD y:
+// counters[XCD_STRIDE * y]; the retire count (pool_retire_xcd) follows the
+// last one.  With X = 1 this is Chunks.
+constexpr uint32_t XCD_STRIDE = 32, XCD_MAX = 8, XCD_RETIRE = XCD_STRIDE * XCD_MAX, XCD_STRIPE = 8;
+struct XcdChunks {
+  uint32_t* counters;
+  uint64_t n, chunk, chunks;
+  uint32_t xcds, x;        // XCD count, this wave's XCD
+  uint64_t local;          // this wave's index among its XCD's waves
+  uint32_t wpb;            // waves per block
+  uint32_t k = 0;          // XCDs tried after the home chunk
+  bool started = false;
+  // the i-th chunk of XCD y (past the end: >= chunks, and so are all later ones)
+  __device__ __forceinline__ uint64_t nth(uint32_t y, uint64_t i) const {
+    return ((i / XCD_STRIPE) * xcds + y) * XCD_STRIPE + i % XCD_STRIPE;
+  }
+  __device__ __forceinline__ bool take(uint64_t c, uint64_t& next, uint64_t& e) const {
+    next = c * chunk;
+    e = next + chunk < n ? next + chunk : n;
+    return true;
+  }
+  __device__ __forceinline__ bool get(uint64_t& next, uint64_t& e) {
+    if (!started) {
+      started = true;
+      const uint64_t c = nth(x, local);
+      if (c < chunks) return take(c, next, e);
+    }
+    while (k < xcds) {
+      const uint32_t y = (x + k) % xcds;
+      // waves on XCD y: blocks y, y + xcds, ... of the grid
+      const uint64_t waves_y = (uint64_t)(gridDim.x > y ? (gridDim.x - y + xcds - 1) / xcds : 0) * wpb;
+      const uint64_t c = nth(y, waves_y + (uint64_t)wave_fetch_add(counters + XCD_STRIDE * y));
+      if (c < chunks) return take(c, next, e);
+      ++k;
+    }
+    return false;
+  }
+};
+
+// pool_retire for XcdChunks counters: the last wave clears every range's
+// counter and the retire count.
+__device__ __forceinline__ void pool_retire_xcd(uint32_t* counters, uint32_t xcds) {
+  if ((threadIdx.x & 63u) == 0) {
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    if (atomicAdd(counters + XCD_RETIRE, 1u) == waves - 1u) {
+      for (uint32_t y = 0; y < xcds; ++y) atomicExch(counters + XCD_STRIDE * y, 0u);
+      atomicExch(counters + XCD_RETIRE, 0u);
+    }
+  }
+}
+
+// The XcdChunks source of this wave.
+__device__ __forceinline__ XcdChunks xcd_chunks(uint32_t* counters, uint64_t n, uint64_t chunk, uint32_t xcds) {
+  const uint32_t wpb = blockDim.x >> 6;
+  const uint32_t x = blockIdx.x % xcds;
+  const uint64_t local = (uint64_t)(blockIdx.x / xcds) * wpb + (threadIdx.x >> 6);
+  return XcdChunks{counters, n, chunk, (n + chunk - 1) / chunk, xcds, x, local, wpb};
+}
+
 struct ItemChunks {
   uint32_t* counter;
   uint32_t chunk0, chunk_end;
