@@ -311,11 +311,7 @@ int sq_tlas_overlap(sq_tlas* tlas, const float* regions, uint64_t n, uint64_t* o
  *                  over the whole chip; 0 = automatic (overlap calls of fewer
  *                  than 2M queries split three quarters of the way down the
  *                  tree, others do not), -1 = never split, 1..24 = fixed.
- *                  Results do not depend on it.
- *   xcd_split      pooled nearest / any / cast walks: 1 gives each XCD (L2
- *                  domain) every 8th stripe of 8 consecutive chunks, walked
- *                  before it steals from the others; 0 (default) = one pool
- *                  (measured even or better on the benchmark workloads). */
+ *                  Results do not depend on it. */
 typedef struct sq_tuning {
   int block;
   int rays_per_lane;
@@ -323,7 +319,6 @@ typedef struct sq_tuning {
   int ray_order;
   int descend_min;
   int split_depth;
-  int xcd_split;
 } sq_tuning;
 int sq_ctx_set_tuning(sq_ctx* ctx, const sq_tuning* tuning);
 int sq_ctx_get_tuning(const sq_ctx* ctx, sq_tuning* tuning);

@@ -63,23 +63,24 @@ __global__ __launch_bounds__(1024) void k_nearest(SceneView s, const float* __re
 
 // Pooled nearest / any-hit / shape-cast kernel, persistent: the grid holds as
 // many waves as the chip keeps resident, and every wave takes chunks of `chunk`
-// queries until all n are handed out, refilling its lanes across chunk
-// boundaries (pool_walk); chunks come from per-XCD pools (xcd_chunks, one pool
-// when xcds is 1) whose counters start zero and which the launch leaves zero
-// (pool_retire_xcd).  With `order`, pool
+// queries from counter[0] until all n are handed out, refilling its lanes
+// across chunk boundaries (pool_walk); the counters start zero and the launch
+// leaves them zero (pool_retire).  With `order`, pool
 // position i is query order[i] (the Morton order).
 template <int D, int KIND, bool ANY, bool FILT, bool CAST>
 __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nearest_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                       const float* __restrict__ tmax, Filter f, NearestOut out,
                                                       uint32_t chunk, uint32_t* counter, uint32_t S, SpillEntry* spill,
                                                       uint32_t spill_stride, const uint32_t* __restrict__ order,
-                                                      uint32_t pause_below, uint32_t xcds) {
+                                                      uint32_t pause_below) {
   extern __shared__ uint32_t lds[];
   const uint32_t B = blockDim.x, tid = threadIdx.x;
   const uint32_t gtid = blockIdx.x * B + tid;
   RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
-  pool_walk<D, KIND, ANY, FILT, CAST>(s, q, xcd_chunks(counter, n, chunk, xcds), max_toi, tmax, f, out, st, order, pause_below);
-  pool_retire_xcd(counter, xcds);
+  const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
+  const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
+  pool_walk<D, KIND, ANY, FILT, CAST>(s, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, f, out, st, order, pause_below);
+  pool_retire(counter);
 }
 
 template <int D>
@@ -715,9 +716,8 @@ int sq_ctx_create(int device, sq_ctx** out) {
   HIP_TRY(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device));
   HIP_TRY(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
   c->stream = c->own;
-  c->xcds = std::min(8, std::max(1, c->cus / 32));
-  if (int rc = c->pool_ctr.ensure(POOL_CTR_BYTES)) return rc;
-  HIP_TRY(hipMemsetAsync(c->pool_ctr.p, 0, POOL_CTR_BYTES, c->own));
+  if (int rc = c->pool_ctr.ensure(256)) return rc;
+  HIP_TRY(hipMemsetAsync(c->pool_ctr.p, 0, 256, c->own));
   HIP_TRY(hipStreamSynchronize(c->own));
   *out = c.release();
   return SQ_OK;
@@ -760,14 +760,12 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   if (t->ray_order != 0 && t->ray_order != 1) return fail(SQ_ERR_INVALID, "ray_order must be 0 or 1");
   if (t->descend_min < 0 || t->descend_min > 64) return fail(SQ_ERR_INVALID, "descend_min must be in [0, 64]");
   if (t->split_depth < -1 || t->split_depth > 24) return fail(SQ_ERR_INVALID, "split_depth must be in [-1, 24]");
-  if (t->xcd_split != 0 && t->xcd_split != 1) return fail(SQ_ERR_INVALID, "xcd_split must be 0 or 1");
   c->block = block;
   c->rays_per_lane = t->rays_per_lane;
   c->lds_stack = S;
   c->ray_order = t->ray_order;
   c->descend_min = t->descend_min;
   c->split_depth = t->split_depth;
-  c->xcd_split = t->xcd_split;
   return SQ_OK;
 }
 
@@ -779,7 +777,6 @@ int sq_ctx_get_tuning(const sq_ctx* c, sq_tuning* t) {
   t->ray_order = c->ray_order;
   t->descend_min = c->descend_min;
   t->split_depth = c->split_depth;
-  t->xcd_split = c->xcd_split;
   return SQ_OK;
 }
 
@@ -1114,7 +1111,7 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
       if (r2) return r2;
       k_nearest_pool<D, K, ANY_, FILT_, CAST_><<<(uint32_t)grid, B, lds, c->stream>>>(
           s, q, nn, max_toi, tmax, fd, out, (uint32_t)chunk, counter, S, c->spill.as<SpillEntry>(), (uint32_t)lanes, order,
-          (uint32_t)c->descend_min, pool_xcds(c));
+          (uint32_t)c->descend_min);
       HIP_TRY(hipGetLastError());
       return SQ_OK;
     };

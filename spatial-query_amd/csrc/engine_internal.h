@@ -69,12 +69,9 @@ struct sq_ctx {
   int ray_order = 0;               // 1: walk rays in Morton order (order.hip)
   int descend_min = 24;            // pooled walks: pause descents below this many lanes (pool.h)
   int split_depth = 0;             // pooled collect: split queries this many levels down (0 auto, -1 never)
-  int xcd_split = 0;               // pooled nearest walks: per-XCD chunk pools (pool.h XcdChunks)
-  int xcds = 1;                    // XCDs of the device (32 CUs each on gfx950)
   bool timed = false;
-  // per-XCD chunk counters and the retire count of the pooled single-call
-  // launches: zero at creation, and each launch leaves them zero again
-  // (pool_retire_xcd)
+  // {next chunk, retired waves} of the pooled single-call launches: zero at
+  // creation, and each launch leaves them zero again (pool_retire)
   DevBuf pool_ctr;
   int refs = 1;                    // the caller's handle + one per live tree
   DevBuf counts, scratch, keys, keys2;   // reusable temporaries of the CSR queries
@@ -169,12 +166,6 @@ inline uint32_t grid_for(uint64_t n, int block) { return (uint32_t)((n + block -
 // Rays per lane of a pooled launch over n queries: the tuned pool size, halved
 // while there would be fewer chunks than waves the chip holds resident (32 per
 // compute unit: 8 per SIMD), so every resident wave gets a chunk of its own.
-// Bytes of the pooled launches' counters (XCD_RETIRE + 1 words, rounded up).
-constexpr size_t POOL_CTR_BYTES = 2048;
-
-// XCD pools a pooled nearest launch splits its chunks into.
-inline uint32_t pool_xcds(const sq_ctx* c) { return c->xcd_split ? (uint32_t)c->xcds : 1u; }
-
 inline uint32_t pooled_per_lane(const sq_ctx* c, uint64_t n) {
   uint32_t per = (uint32_t)(c->rays_per_lane > 0 ? c->rays_per_lane : 1);
   const uint64_t want = 32ull * (uint64_t)c->cus;

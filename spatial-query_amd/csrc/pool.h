@@ -62,72 +62,6 @@ struct OnePool {
     return first < end;
   }
 };
-// Chunks in per-XCD pools.  Workgroups go to the chip's XCDs (each with its
-// own L2) round-robin by block index, so XCD x holds blocks x, x + X, ...;
-// here it also owns every X-th stripe of XCD_STRIPE consecutive chunks
-// (stripes x, x + X, ...), walked first (home chunk, then its own counter)
-// before it steals from the other XCDs' stripes in turn.  Neighbouring (often
-// coherent) chunks of a stripe then share one L2, while every XCD's share
-// still spans the whole query range (a contiguous share per XCD would hand
-// whole regions of unequal cost to single XCDs).  Counter of XCD y:
-// counters[XCD_STRIDE * y]; the retire count (pool_retire_xcd) follows the
-// last one.  With X = 1 this is Chunks.
-constexpr uint32_t XCD_STRIDE = 32, XCD_MAX = 8, XCD_RETIRE = XCD_STRIDE * XCD_MAX, XCD_STRIPE = 8;
-struct XcdChunks {
-  uint32_t* counters;
-  uint64_t n, chunk, chunks;
-  uint32_t xcds, x;        // XCD count, this wave's XCD
-  uint64_t local;          // this wave's index among its XCD's waves
-  uint32_t wpb;            // waves per block
-  uint32_t k = 0;          // XCDs tried after the home chunk
-  bool started = false;
-  // the i-th chunk of XCD y (past the end: >= chunks, and so are all later ones)
-  __device__ __forceinline__ uint64_t nth(uint32_t y, uint64_t i) const {
-    return ((i / XCD_STRIPE) * xcds + y) * XCD_STRIPE + i % XCD_STRIPE;
-  }
-  __device__ __forceinline__ bool take(uint64_t c, uint64_t& next, uint64_t& e) const {
-    next = c * chunk;
-    e = next + chunk < n ? next + chunk : n;
-    return true;
-  }
-  __device__ __forceinline__ bool get(uint64_t& next, uint64_t& e) {
-    if (!started) {
-      started = true;
-      const uint64_t c = nth(x, local);
-      if (c < chunks) return take(c, next, e);
-    }
-    while (k < xcds) {
-      const uint32_t y = (x + k) % xcds;
-      // waves on XCD y: blocks y, y + xcds, ... of the grid
-      const uint64_t waves_y = (uint64_t)(gridDim.x > y ? (gridDim.x - y + xcds - 1) / xcds : 0) * wpb;
-      const uint64_t c = nth(y, waves_y + (uint64_t)wave_fetch_add(counters + XCD_STRIDE * y));
-      if (c < chunks) return take(c, next, e);
-      ++k;
-    }
-    return false;
-  }
-};
-
-// pool_retire for XcdChunks counters: the last wave clears every range's
-// counter and the retire count.
-__device__ __forceinline__ void pool_retire_xcd(uint32_t* counters, uint32_t xcds) {
-  if ((threadIdx.x & 63u) == 0) {
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    if (atomicAdd(counters + XCD_RETIRE, 1u) == waves - 1u) {
-      for (uint32_t y = 0; y < xcds; ++y) atomicExch(counters + XCD_STRIDE * y, 0u);
-      atomicExch(counters + XCD_RETIRE, 0u);
-    }
-  }
-}
-
-// The XcdChunks source of this wave.
-__device__ __forceinline__ XcdChunks xcd_chunks(uint32_t* counters, uint64_t n, uint64_t chunk, uint32_t xcds) {
-  const uint32_t wpb = blockDim.x >> 6;
-  const uint32_t x = blockIdx.x % xcds;
-  const uint64_t local = (uint64_t)(blockIdx.x / xcds) * wpb + (threadIdx.x >> 6);
-  return XcdChunks{counters, n, chunk, (n + chunk - 1) / chunk, xcds, x, local, wpb};
-}
-
 struct ItemChunks {
   uint32_t* counter;
   uint32_t chunk0, chunk_end;
@@ -170,9 +104,9 @@ struct Chunks {
 // queries from the current range in order; a lane whose query finishes takes
 // the next one (ballot + popcount), and the wave takes a new range when this
 // one runs out, so lanes stay busy while long queries run.  Each step walks to one leaf and tests it (walk_step), or stops early
-// when fewer than pause_below lanes are still descending (WavePause) so that
-// finished lanes can take new queries; once the pool is exhausted there is
-// nothing to take and the walks run without pausing.  The stack is
+// when fewer than pause_below lanes are still descending (WavePause), so
+// that finished lanes can take new queries and lanes waiting at a leaf test
+// it instead of waiting for the longest descent.  The stack is
 // the caller's LDS ring spilling to a global column.  With `order`, pool
 // position i is query order[i] (rays read and results written there).
 // The walk for any query policy Pol (setup / root / pair / narrow over
@@ -229,7 +163,7 @@ __device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* 
     if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
     const uint32_t active = (uint32_t)__popcll(__ballot(rid != NO_LEAF));
     if (rid == NO_LEAF) continue;
-    if (!walk_step<D, PS, ANY, FILT>(s, pol, f, st, w, nullptr, nullptr, WavePause{exhausted ? 0u : pause_below, active})) continue;
+    if (!walk_step<D, PS, ANY, FILT>(s, pol, f, st, w, nullptr, nullptr, WavePause{pause_below, active})) continue;
     // finished: write the result of query rid
     const NearestResult& h = w.res;
     const bool hit = h.leaf != NO_LEAF;
@@ -471,6 +405,9 @@ __device__ __forceinline__ void pool_walk_tlas(const TlasView& tv, const float* 
     if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
     const uint32_t active = (uint32_t)__popcll(__ballot(rid != NO_LEAF));
     if (rid == NO_LEAF) continue;
+    // once the pool is exhausted the two-level walk runs without pausing
+    // (measured faster there, unlike the single-level walk, where pausing
+    // also lets lanes at a leaf test it while others descend)
     if (!tlas_step<D, IK, ANY>(tv, r, st, sb, w, WavePause{exhausted ? 0u : pause_below, active})) continue;
     write(rid);
     rid = NO_LEAF;
@@ -708,7 +645,7 @@ __device__ __forceinline__ void pool_collect_with(const SceneView& s, const floa
             ++cnt;
           });
         },
-        st, v, WavePause{exhausted ? 0u : pause_below, active});
+        st, v, WavePause{pause_below, active});
     if (!done) continue;
     if constexpr (ITEMS) {
       item_cnt[pos] = cnt;

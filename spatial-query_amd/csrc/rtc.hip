@@ -68,24 +68,26 @@ template <int D> struct ConsumerPolicy {
 }  // namespace sqd
 
 // Persistent pooled walk (as the built-in kinds' k_nearest_pool): waves take
-// chunks of `chunk` rays from per-XCD pools (xcd_chunks; counters zero at the
-// start, left zero by pool_retire_xcd), refill lanes as rays finish, pause
-// descents below pause_below lanes while the pool has rays left.
+// chunks of `chunk` rays from counter[0] (zero at the start, left zero by
+// pool_retire), refill lanes as rays finish, pause descents below pause_below
+// lanes while the pool has rays left.
 extern "C" __global__ __launch_bounds__(256) void sq_consumer_nearest(sqd::SceneView s, const float* rays, unsigned n,
                                                                       float max_toi, int* out_leaf, float* out_toi,
                                                                       unsigned chunk, unsigned* counter, unsigned S,
                                                                       sqd::SpillEntry* spill, unsigned spill_stride,
-                                                                      unsigned pause_below, unsigned xcds) {
+                                                                      unsigned pause_below) {
   extern __shared__ unsigned lds[];
   const unsigned B = blockDim.x, tid = threadIdx.x;
   const unsigned gtid = blockIdx.x * B + tid;
   sqd::RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
+  const unsigned long long waves = (unsigned long long)gridDim.x * (B >> 6);
+  const unsigned long long home = (unsigned long long)blockIdx.x * (B >> 6) + (tid >> 6);
   const sqd::NearestOut out{out_leaf, out_toi, nullptr, nullptr, nullptr, nullptr};
   const sqd::Filter f{nullptr, ~0ull, -1};
   sqd::pool_walk_with<SQ_RTC_D, sqd::KIND_AABB, false, false, false, sqd::ConsumerPolicy<SQ_RTC_D>, 2 * SQ_RTC_D,
-                      2 * SQ_RTC_D + SQ_RTC_PF>(s, rays, sqd::xcd_chunks(counter, n, chunk, xcds), max_toi, nullptr, f,
+                      2 * SQ_RTC_D + SQ_RTC_PF>(s, rays, sqd::Chunks{counter, n, chunk, home, waves}, max_toi, nullptr, f,
                                                 out, st, nullptr, pause_below);
-  sqd::pool_retire_xcd(counter, xcds);
+  sqd::pool_retire(counter);
 }
 
 namespace sqd {
@@ -328,8 +330,7 @@ int sq_raycast_nearest_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uin
   float* tp = out_toi;
   uint32_t* counter = c->pool_ctr.as<uint32_t>();
   SpillEntry* spill = c->spill.as<SpillEntry>();
-  uint32_t xcds = pool_xcds(c);
-  void* args[] = {&s, &rp, &nn, &max_toi, &lp, &tp, &chunk, &counter, &S, &spill, &lanes, &pause, &xcds};
+  void* args[] = {&s, &rp, &nn, &max_toi, &lp, &tp, &chunk, &counter, &S, &spill, &lanes, &pause};
   HIP_TRY(hipModuleLaunchKernel(nw->fn, (unsigned)grid, 1, 1, B, 1, 1, (unsigned)lds, c->stream, args, nullptr));
   return SQ_OK;
 }

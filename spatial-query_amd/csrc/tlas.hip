@@ -129,22 +129,22 @@ __global__ __launch_bounds__(1024) void k_tlas_nearest(TlasView tv, const float*
 }
 
 // Pooled, persistent form of k_tlas_nearest (pool_walk_tlas): waves take
-// chunks of `chunk` rays from per-XCD pools (xcd_chunks; counters zero at the
-// start, left zero by pool_retire_xcd), both stacks S-entry LDS rings
-// spilling to their own columns.
+// chunks of `chunk` rays from counter[0] (zero at the start, left zero by
+// pool_retire), both stacks S-entry LDS rings spilling to their own columns.
 template <int D, int IK, bool ANY>
 __global__ __launch_bounds__(256) void k_tlas_nearest_pool(TlasView tv, const float* __restrict__ q, uint32_t n, float max_toi,
                                                            const float* __restrict__ tmax, NearestOut out, uint32_t chunk,
                                                            uint32_t* counter, uint32_t S, SpillEntry* spill_top,
-                                                           SpillEntry* spill_bot, uint32_t spill_stride, uint32_t pause_below,
-                                                           uint32_t xcds) {
+                                                           SpillEntry* spill_bot, uint32_t spill_stride, uint32_t pause_below) {
   extern __shared__ uint32_t lds[];
   const uint32_t B = blockDim.x, tid = threadIdx.x;
   const uint32_t gt = blockIdx.x * B + tid;
   RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill_top + gt, spill_stride};
   RingStack sb{lds + 2 * S * B + tid, reinterpret_cast<float*>(lds + 3 * S * B) + tid, B, S, spill_bot + gt, spill_stride};
-  pool_walk_tlas<D, IK, ANY>(tv, q, xcd_chunks(counter, n, chunk, xcds), max_toi, tmax, out, st, sb, pause_below);
-  pool_retire_xcd(counter, xcds);
+  const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
+  const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
+  pool_walk_tlas<D, IK, ANY>(tv, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, out, st, sb, pause_below);
+  pool_retire(counter);
 }
 
 // MODE 0: raycast_all (one hit per instance, tlas.rs:158-215); MODE 2: overlap
@@ -390,7 +390,7 @@ static int tlas_nearest_impl(sq_tlas* t, const float* rays, uint64_t n, float ma
       SpillEntry* sp = c->spill.as<SpillEntry>();
       k_tlas_nearest_pool<D, K, ANY_><<<(uint32_t)grid, B, lds, c->stream>>>(
           tv, rays, (uint32_t)n, max_toi, tmax, out, (uint32_t)chunk, c->pool_ctr.as<uint32_t>(), S, sp,
-          sp + (size_t)rows_top * lanes, (uint32_t)lanes, (uint32_t)c->descend_min, pool_xcds(c));
+          sp + (size_t)rows_top * lanes, (uint32_t)lanes, (uint32_t)c->descend_min);
       HIP_TRY(hipGetLastError());
       return SQ_OK;
     };

@@ -6,9 +6,8 @@ every launch leaves zero again (the last wave to retire clears them); batch
 sets do the same with their own counters.  A launch that left them non-zero
 would make the next one skip chunks, so here calls of many sizes (grids of 1
 to every resident wave, chunk sizes 64 to 1024) and kinds follow each other
-on one context, batch submits interleaved, with one pool and with per-XCD
-pools, and every answer must equal the one-ray-per-thread kernel
-(rays_per_lane 0), which uses no pool counter.
+on one context, batch submits interleaved, and every answer must equal the
+one-ray-per-thread kernel (rays_per_lane 0), which uses no pool counter.
 """
 import numpy as np
 import pytest
@@ -44,7 +43,7 @@ def test_calls_of_every_size_leave_the_counters_clear(ctx):
         want = [(bvh.raycast_nearest(r, 900.0), tlas.raycast_nearest(r, 900.0), bvh.raycast_any(r, 900.0)) for r in rays]
         bs = sq_amd.BatchSet(ctx, [(bvh, r, 900.0) for r in rays[:4]])
         for tune in (dict(block=64, rays_per_lane=1, lds_stack=8), dict(block=128, rays_per_lane=16, lds_stack=8),
-                     dict(block=256, rays_per_lane=4, lds_stack=4), dict(block=128, rays_per_lane=8, xcd_split=1)):
+                     dict(block=256, rays_per_lane=4, lds_stack=4)):
             ctx.set_tuning(**tune)
             for rep in range(2):
                 for i, r in enumerate(rays):
