@@ -23,7 +23,7 @@ _LIB_PATH = os.path.join(_HERE, "_build", "liborc.so")
 BOX, SPHERE, TRI = 0, 1, 2
 SPHERE_PLAIN, SPHERE_PLAIN_NOEXIT, TRI_PLAIN, SHELL = 10, 11, 12, 13
 KIND_CONSUMER = 20
-INST_BOX, INST_MESH, INST_BOX_GUARDED = 0, 1, 2
+INST_BOX, INST_MESH, INST_BOX_GUARDED, INST_BALL = 0, 1, 2, 3
 ALL_LAYERS = (1 << 64) - 1
 
 _lib = None
@@ -84,6 +84,7 @@ def _declare(L):
         "orc_tlas_raycast_nearest_batch": (None, [P, P, U64, F32, I32, I32, P, P, P, P, P]),
         "orc_tlas_raycast_all_batch": (U64, [P, P, U64, F32, P, P, P, P, U64]),
         "orc_tlas_overlap_batch": (U64, [P, P, U64, P, P, U64]),
+        "orc_tlas_shapecast_batch": (U64, [P, P, U64, F32, I32, P, P, P, U64]),
         "orc_inst_free": (None, [P]),
         "orc_iso_apply": (None, [I32, P, P, P, I32, P]),
         "orc_iso_transform_aabb": (None, [I32, P, P, P, P]),
@@ -289,7 +290,9 @@ class InstanceScene:
         self.rot = np.ascontiguousarray(rot, dtype=np.float32).reshape(-1, dim, dim)
         self.trans = np.ascontiguousarray(trans, dtype=np.float32).reshape(-1, dim)
         self.n = self.rot.shape[0]
-        self.halves = None if halves is None else np.ascontiguousarray(halves, dtype=np.float32).reshape(-1, dim)
+        # INST_BALL: halves holds one radius per instance
+        hw = 1 if kind == INST_BALL else dim
+        self.halves = None if halves is None else np.ascontiguousarray(halves, dtype=np.float32).reshape(-1, hw)
         if kind == INST_MESH:
             off = np.zeros(len(mesh_tris) + 1, np.uint64)
             for i, m in enumerate(mesh_tris):
@@ -309,7 +312,7 @@ class InstanceScene:
         self.rot = np.ascontiguousarray(rot, dtype=np.float32).reshape(-1, self.dim, self.dim)
         self.trans = np.ascontiguousarray(trans, dtype=np.float32).reshape(-1, self.dim)
         if halves is not None:
-            self.halves = np.ascontiguousarray(halves, dtype=np.float32).reshape(-1, self.dim)
+            self.halves = np.ascontiguousarray(halves, dtype=np.float32).reshape(-1, 1 if self.kind == INST_BALL else self.dim)
         lib().orc_inst_set_transforms(self._h, _p(self.rot), _p(self.trans), _p(self.halves) if halves is not None else None)
 
     def refit(self):
@@ -349,6 +352,25 @@ class InstanceScene:
         leaf = np.empty(total, np.int64); t = np.empty(total, np.float32); sub = np.empty(total, np.int64)
         L.orc_tlas_raycast_all_batch(self._h, _p(rays), n, max_toi, _p(off), _p(leaf), _p(t), _p(sub), total)
         return off, leaf, t, sub
+
+    def shapecast_nearest(self, casts, max_toi):
+        """Tlas::shapecast_nearest (tlas.rs:306-351) per cast [n][4D]: dict(leaf, t)."""
+        casts = np.ascontiguousarray(casts, dtype=np.float32).reshape(-1, 4 * self.dim)
+        n = casts.shape[0]
+        leaf = np.empty(n, np.int64); t = np.empty(n, np.float32)
+        lib().orc_tlas_shapecast_batch(self._h, _p(casts), n, max_toi, 0, None, _p(leaf), _p(t), n)
+        return dict(leaf=leaf, t=t)
+
+    def shapecast_all(self, casts, max_toi):
+        """Tlas::shapecast_all (tlas.rs:353-391): CSR (offsets, leaf, t)."""
+        casts = np.ascontiguousarray(casts, dtype=np.float32).reshape(-1, 4 * self.dim)
+        n = casts.shape[0]
+        off = np.empty(n + 1, np.uint64)
+        L = lib()
+        total = L.orc_tlas_shapecast_batch(self._h, _p(casts), n, max_toi, 1, _p(off), None, None, 0)
+        leaf = np.empty(total, np.int64); t = np.empty(total, np.float32)
+        L.orc_tlas_shapecast_batch(self._h, _p(casts), n, max_toi, 1, _p(off), _p(leaf), _p(t), total)
+        return off, leaf, t
 
     def overlap(self, regions):
         regions = np.ascontiguousarray(regions, dtype=np.float32).reshape(-1, 2 * self.dim)

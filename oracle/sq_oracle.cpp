@@ -890,6 +890,7 @@ enum InstKind {
   INST_BOX = 0,          // local box [-half, half], two_level.rs:70-113 (ray_local_box)
   INST_MESH = 1,         // uniformly scaled triangle mesh with its own BVH (D = 3), guarded
   INST_BOX_GUARDED = 2,  // INST_BOX plus the instance guard (the engine's box kind)
+  INST_BALL = 3,         // ball of radius r at the local origin (shape_overlap.rs:288-345 BallInstances), guarded
 };
 
 template <int D> struct Instances {
@@ -898,6 +899,7 @@ template <int D> struct Instances {
   std::vector<Point<D>> halves;        // INST_BOX
   std::vector<uint32_t> mesh_of;       // INST_MESH
   std::vector<float> scale;            // INST_MESH: uniform scale of the mesh
+  std::vector<float> radius;           // INST_BALL
   std::vector<std::unique_ptr<Geometry<D>>> meshes;
   std::vector<std::unique_ptr<Bvh<D>>> mesh_bvh;
   std::vector<Aabb<D>> mesh_box;       // AABB of each mesh's vertices
@@ -909,6 +911,11 @@ template <int D> struct Instances {
   bool is_box() const { return kind == INST_BOX || kind == INST_BOX_GUARDED; }
   Aabb<D> local_aabb(size_t i) const {
     if (is_box()) { Aabb<D> b; b.min = pmul(halves[i], -1.0f); b.max = halves[i]; return b; }
+    if (kind == INST_BALL) {                                          // shape_overlap.rs:320-322
+      Aabb<D> b;
+      for (int k = 0; k < D; ++k) { b.min.v[k] = -radius[i]; b.max.v[k] = radius[i]; }
+      return b;
+    }
     const Aabb<D>& m = mesh_box[mesh_of[i]];
     Aabb<D> b; b.min = pmul(m.min, scale[i]); b.max = pmul(m.max, scale[i]);
     return b;
@@ -942,6 +949,12 @@ template <int D> struct Instances {
       out->sub = -1;
       return true;
     }
+    if (kind == INST_BALL) {                                          // shape_overlap.rs:120-138 (ray_ball)
+      float t;
+      if (!ray_ball(ray, Point<D>::zero(), radius[i], max_toi, &t)) return false;
+      out->toi = t; out->normal = normalize_or_zero(ray.at(t)); out->sub = -1;
+      return true;
+    }
     float inv_s = 1.0f / scale[i];
     Ray<D> mr; mr.origin = pmul(ray.origin, inv_s); mr.dir = pmul(ray.dir, inv_s);
     Hit<D> h;
@@ -949,6 +962,49 @@ template <int D> struct Instances {
     if (!mesh_bvh[mesh_of[i]]->raycast_nearest(*meshes[mesh_of[i]], mr, max_toi, all, &h, nullptr)) return false;
     out->toi = h.toi; out->normal = h.normal; out->sub = h.leaf;
     return true;
+  }
+
+  // ray_ball (shape_overlap.rs:120-138): the entry root, else the exit root.
+  static bool ray_ball(const Ray<D>& ray, const Point<D>& c, float r, float max_toi, float* t_out) {
+    Point<D> oc = psub(ray.origin, c);
+    float b = pdot(oc, ray.dir);
+    float cc = pdot(oc, oc) - r * r;
+    float disc = b * b - cc;
+    if (disc < 0.0f) return false;
+    float sq = sqrtf(disc);
+    float t = -b - sq;
+    if (t < 0.0f) t = -b + sq;
+    if (t < 0.0f || t > max_toi) return false;
+    *t_out = t;
+    return true;
+  }
+
+  // InstancedGeometry::test_shape_cast (instance.rs:72-87): the swept ball
+  // probe (radius = the cast box's max x) against ball instances
+  // (shape_overlap.rs:326-337: the ray against the radius sum); None for the
+  // other kinds, the trait's default.
+  bool test_shape_cast(size_t i, const ShapeCast<D>& cast, float max_toi, LeafHit<D>* out) const {
+    if (kind != INST_BALL) return false;
+    const Point<D> c = xf[i].trans;
+    Ray<D> r; r.origin = cast.origin; r.dir = cast.dir;
+    float t;
+    if (!ray_ball(r, c, radius[i] + cast.aabb.max.v[0], max_toi, &t)) return false;
+    out->toi = t; out->normal = normalize_or_zero(psub(cast.at(t), c)); out->sub = -1;
+    return true;
+  }
+
+  // InstancedGeometry::test_overlap (instance.rs:89-98): ball instances test
+  // the distance from the centre to the region (shape_overlap.rs:59-72,
+  // 339-341); the other kinds keep the default (true).
+  bool test_overlap(size_t i, const Aabb<D>& region) const {
+    if (kind != INST_BALL) return true;
+    const Point<D> p = xf[i].trans;
+    float d2 = 0.0f;
+    for (int k = 0; k < D; ++k) {
+      float v = p.v[k] < region.min.v[k] ? region.min.v[k] - p.v[k] : (p.v[k] > region.max.v[k] ? p.v[k] - region.max.v[k] : 0.0f);
+      d2 += v * v;
+    }
+    return sqrtf(d2) <= radius[i];
   }
 
   bool accepts(size_t, const Filter& f) const { return f.layer_mask != 0; }
@@ -1016,6 +1072,67 @@ template <int D> struct Tlas {
     return have;
   }
 
+  // shapecast_instance (tlas.rs:437-452) with the engine's guard: the contact
+  // must lie in the swept probe's slab interval against the instance's world
+  // AABB (broadphase), which makes the result order independent.
+  bool shapecast_instance(const Instances<D>& s, size_t i, const ShapeCast<D>& cast, float max_toi, Hit<D>* out) const {
+    LeafHit<D> lh;
+    if (!s.test_shape_cast(i, cast, max_toi, &lh)) return false;
+    float te, tx;
+    if (!cast.broadphase(world[i], max_toi, &te, &tx)) return false;
+    if (!(te <= lh.toi && lh.toi <= tx)) return false;
+    out->id = s.id(i); out->leaf = (int64_t)i; out->toi = lh.toi; out->point = cast.at(lh.toi);
+    out->normal = lh.normal; out->sub = lh.sub;
+    return true;
+  }
+
+  // tlas.rs:306-351
+  bool shapecast_nearest(const Instances<D>& s, const ShapeCast<D>& cast, float max_toi, const Filter& f, Hit<D>* best_out) const {
+    if (nodes.empty()) return false;
+    bool have = false; Hit<D> best; float limit = max_toi;
+    std::vector<uint32_t> stack(1, 0u);
+    while (!stack.empty()) {
+      uint32_t ni = stack.back(); stack.pop_back();
+      const Node<D> node = nodes[ni];
+      float te, tx;
+      if (!cast.broadphase(node.bounds, limit, &te, &tx) || !(te <= limit)) continue;
+      if (node.is_leaf()) {
+        for (uint32_t k = node.child_a; k < node.child_a + node.prim_count; ++k) {
+          size_t i = idx[k];
+          if (!s.accepts(i, f)) continue;
+          Hit<D> hit;
+          if (shapecast_instance(s, i, cast, limit, &hit)) {
+            if (have && hit_ordering(best, hit) <= 0) {} else { limit = hit.toi; best = hit; have = true; }
+          }
+        }
+      } else { stack.push_back(node.child_a); stack.push_back(node.child_b); }
+    }
+    if (have) *best_out = best;
+    return have;
+  }
+
+  // tlas.rs:353-391
+  void shapecast_all(const Instances<D>& s, const ShapeCast<D>& cast, float max_toi, const Filter& f, std::vector<Hit<D>>& hits) const {
+    hits.clear();
+    if (nodes.empty()) return;
+    std::vector<uint32_t> stack(1, 0u);
+    while (!stack.empty()) {
+      uint32_t ni = stack.back(); stack.pop_back();
+      const Node<D> node = nodes[ni];
+      float te, tx;
+      if (!cast.broadphase(node.bounds, max_toi, &te, &tx)) continue;
+      if (node.is_leaf()) {
+        for (uint32_t k = node.child_a; k < node.child_a + node.prim_count; ++k) {
+          size_t i = idx[k];
+          if (!s.accepts(i, f)) continue;
+          Hit<D> hit;
+          if (shapecast_instance(s, i, cast, max_toi, &hit)) hits.push_back(hit);
+        }
+      } else { stack.push_back(node.child_a); stack.push_back(node.child_b); }
+    }
+    std::stable_sort(hits.begin(), hits.end(), [](const Hit<D>& a, const Hit<D>& b) { return hit_ordering(a, b) < 0; });
+  }
+
   // tlas.rs:158-215 (raycast_all)
   void raycast_all(const Instances<D>& s, const Ray<D>& ray, float max_toi, const Filter& f, std::vector<Hit<D>>& hits) const {
     hits.clear();
@@ -1038,8 +1155,8 @@ template <int D> struct Tlas {
     std::stable_sort(hits.begin(), hits.end(), [](const Hit<D>& a, const Hit<D>& b) { return hit_ordering(a, b) < 0; });
   }
 
-  // Overlap with the default instance test_overlap (true once the world AABB
-  // meets the region), instance.rs:91-93.
+  // tlas.rs:393-432: the instance's test_overlap once its world AABB meets the
+  // region (the default, true, for boxes and meshes).
   void overlap(const Instances<D>& s, const Aabb<D>& region, const Filter& f, std::vector<int64_t>& out) const {
     out.clear();
     if (nodes.empty()) return;
@@ -1052,6 +1169,7 @@ template <int D> struct Tlas {
         for (uint32_t k = node.child_a; k < node.child_a + node.prim_count; ++k) {
           size_t i = idx[k];
           if (!s.accepts(i, f)) continue;
+          if (!s.test_overlap(i, region)) continue;
           out.push_back((int64_t)i);
         }
       } else { stack.push_back(node.child_a); stack.push_back(node.child_b); }
@@ -1403,7 +1521,7 @@ void orc_required_work(void* bh, const float* rays, uint64_t n, const float* t_l
 
 // --- instances and Tlas -----------------------------------------------------
 // rot: [n][D][D] row-major; trans: [n][D].
-// INST_BOX: halves [n][D].  INST_MESH (D = 3): mesh_tri_off [n_meshes + 1]
+// INST_BOX: halves [n][D].  INST_BALL: radii [n] in halves.  INST_MESH (D = 3): mesh_tri_off [n_meshes + 1]
 // into tris [T][9]; mesh_of [n]; scale [n].
 void* orc_inst_new(int D, int kind, uint64_t n, const float* rot, const float* trans, const float* halves,
                    uint32_t n_meshes, const uint64_t* mesh_tri_off, const float* tris, const uint32_t* mesh_of, const float* scale) {
@@ -1419,6 +1537,8 @@ void* orc_inst_new(int D, int kind, uint64_t n, const float* rot, const float* t
     if (s->is_box()) {
       s->halves.resize(n);
       for (uint64_t i = 0; i < n; ++i) for (int a = 0; a < DD; ++a) s->halves[i].v[a] = halves[i * DD + a];
+    } else if (kind == INST_BALL) {
+      s->radius.assign(halves, halves + n);
     } else {
       for (uint32_t m = 0; m < n_meshes; ++m) {
         Geometry<DD>* g = new Geometry<DD>();
@@ -1445,6 +1565,7 @@ void orc_inst_set_transforms(void* h, const float* rot, const float* trans, cons
       for (int a = 0; a < DD; ++a) for (int c = 0; c < DD; ++c) s->xf[i].rot[a][c] = rot[i * DD * DD + a * DD + c];
       for (int a = 0; a < DD; ++a) s->xf[i].trans.v[a] = trans[i * DD + a];
       if (halves && s->is_box()) for (int a = 0; a < DD; ++a) s->halves[i].v[a] = halves[i * DD + a];
+      if (halves && s->kind == INST_BALL) s->radius[i] = halves[i];
     }
   });
 }
@@ -1539,6 +1660,44 @@ uint64_t orc_tlas_overlap_batch(void* h, const float* regions, uint64_t n, uint6
     for (uint64_t i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + all[i].size();
     total = offsets[n];
     if (cap >= total) for (uint64_t i = 0; i < n; ++i) std::memcpy(out_leaf + offsets[i], all[i].data(), all[i].size() * 8);
+  });
+  return total;
+}
+// Tlas::shapecast_nearest / shapecast_all over casts [n][4D] (origin, dir,
+// probe box min, max); all = 0: nearest (leaf -1 on a miss), else CSR.
+uint64_t orc_tlas_shapecast_batch(void* h, const float* casts, uint64_t n, float max_toi, int all, uint64_t* offsets,
+                                  int64_t* out_leaf, float* out_t, uint64_t cap) {
+  OInst* o = (OInst*)h;
+  Filter f{~0ull, -1};
+  uint64_t total = 0;
+  ORC_DISPATCH(o->D, {
+    const Tlas<DD>* t = (const Tlas<DD>*)o->tlas;
+    const Instances<DD>* s = (const Instances<DD>*)o->p;
+    auto cast_of = [&](uint64_t i) {
+      ShapeCast<DD> c;
+      const float* q = casts + i * 4 * DD;
+      for (int k = 0; k < DD; ++k) {
+        c.origin.v[k] = q[k]; c.dir.v[k] = q[DD + k]; c.aabb.min.v[k] = q[2 * DD + k]; c.aabb.max.v[k] = q[3 * DD + k];
+      }
+      return c;
+    };
+    if (!all) {
+      for (uint64_t i = 0; i < n; ++i) {
+        Hit<DD> hh;
+        if (t->shapecast_nearest(*s, cast_of(i), max_toi, f, &hh)) { out_leaf[i] = hh.leaf; out_t[i] = hh.toi; }
+        else { out_leaf[i] = -1; out_t[i] = INFINITY; }
+      }
+      total = n;
+    } else {
+      std::vector<std::vector<Hit<DD>>> hits(n);
+      for (uint64_t i = 0; i < n; ++i) t->shapecast_all(*s, cast_of(i), max_toi, f, hits[i]);
+      offsets[0] = 0;
+      for (uint64_t i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + hits[i].size();
+      total = offsets[n];
+      if (cap >= total)
+        for (uint64_t i = 0; i < n; ++i)
+          for (size_t k = 0; k < hits[i].size(); ++k) { out_leaf[offsets[i] + k] = hits[i][k].leaf; out_t[offsets[i] + k] = hits[i][k].toi; }
+    }
   });
   return total;
 }

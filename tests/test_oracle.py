@@ -320,6 +320,67 @@ def test_shapecast_and_overlap_match_naive():
         assert np.array_equal(sc.overlap(reg)[1], sc.overlap(reg, naive=True)[1])
 
 
+def ball_instances(s, n, dim=3, lo=-30.0, hi=30.0, rlo=0.5, rhi=2.0):
+    """BallInstances (shape_overlap.rs:288-305): translations only, radii U[0.5, 2]."""
+    rot = np.tile(np.eye(dim, dtype=F32), (n, 1, 1))
+    trans = s.range(lo, hi, n * dim).reshape(n, dim).astype(F32)
+    radii = s.range(rlo, rhi, n).astype(F32)
+    return rot, trans, radii
+
+
+def test_tlas_shapecast_matches_naive_via_ball_bvh():
+    """shape_overlap.rs:347-382: a TLAS of ball instances answers shape casts
+    like the naive scan over the same balls as a single-level provider."""
+    s = Stream(0x1234_ABCD)
+    for _ in range(150):
+        rot, trans, radii = ball_instances(s, 48)
+        inst = oracle.InstanceScene(3, oracle.INST_BALL, rot, trans, halves=radii)
+        balls = oracle.Scene(3, oracle.SPHERE, np.concatenate([trans, radii[:, None]], 1))
+        cast = ball_cast(s.range(-50.0, 50.0, 3), s.range(-1.0, 1.0, 3))
+        want = balls.shapecast_nearest(cast, 500.0, naive=True)
+        got = inst.shapecast_nearest(cast, 500.0)
+        assert got["leaf"][0] == want["leaf"][0] and bits(got["t"])[0] == bits(want["t"])[0]
+        _, lw, tw = balls.shapecast_all(cast, 500.0, naive=True)
+        _, lg, tg = inst.shapecast_all(cast, 500.0)
+        assert np.array_equal(lg, lw) and np.array_equal(bits(tg), bits(tw))
+
+
+def test_tlas_overlap_matches_naive_via_ball_bvh():
+    """shape_overlap.rs:384-411: the instances' exact test_overlap."""
+    s = Stream(0x9999_5555)
+    hits = 0
+    for _ in range(150):
+        rot, trans, radii = ball_instances(s, 64)
+        inst = oracle.InstanceScene(3, oracle.INST_BALL, rot, trans, halves=radii)
+        balls = oracle.Scene(3, oracle.SPHERE, np.concatenate([trans, radii[:, None]], 1))
+        c = s.range(-30.0, 30.0, 3); h = s.range(1.0, 6.0, 3)
+        reg = np.concatenate([c - h, c + h])[None].astype(F32)
+        want = balls.overlap(reg, naive=True)[1]
+        got = inst.overlap(reg)[1]
+        assert np.array_equal(got, want)
+        hits += len(got)
+    assert hits > 20
+
+
+def test_tlas_ball_rays_match_single_level_balls():
+    """Rays through ball instances (ray_ball on the local ray, shape_overlap.rs:
+    323-325) give the single-level balls' hits, and a box instance kind
+    answers no shape cast (the trait default, instance.rs:72-87)."""
+    s = Stream(0xBA11)
+    rot, trans, radii = ball_instances(s, 500, lo=-60.0, hi=60.0)
+    inst = oracle.InstanceScene(3, oracle.INST_BALL, rot, trans, halves=radii)
+    balls = oracle.Scene(3, oracle.SPHERE, np.concatenate([trans, radii[:, None]], 1))
+    d = normalize_or_zero(s.range(-1.0, 1.0, 2000 * 3).reshape(-1, 3).astype(F32))
+    rays = np.concatenate([s.range(-70.0, 70.0, 2000 * 3).reshape(-1, 3).astype(F32), d], 1)
+    a, b = inst.raycast_nearest(rays, 300.0), balls.raycast_nearest(rays, 300.0)
+    assert (a["leaf"] >= 0).sum() > 100
+    assert np.array_equal(a["leaf"], b["leaf"]) and np.array_equal(bits(a["t"]), bits(b["t"]))
+    boxes = oracle.InstanceScene(3, oracle.INST_BOX_GUARDED, rot, trans, halves=np.tile(radii[:, None], (1, 3)))
+    casts = np.concatenate([rays[:50, :6], np.full((50, 3), -0.5, F32), np.full((50, 3), 0.5, F32)], 1)
+    assert (boxes.shapecast_nearest(casts, 300.0)["leaf"] == -1).all()
+    assert boxes.shapecast_all(casts, 300.0)[0][-1] == 0
+
+
 def test_crossings_entry_exit():
     """crossings.rs:113-135: a shell of radius 2 at x 10 is crossed at 8 and 12."""
     sc = oracle.Scene(3, oracle.SHELL, np.array([[10, 0, 0, 2]], F32))
