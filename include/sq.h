@@ -250,13 +250,24 @@ int sq_required_work(sq_bvh* bvh, const float* rays, uint64_t n, const float* t_
  *   SQ_INST_MESH  a uniformly scaled triangle mesh (D = 3): its local geometry
  *                 is the mesh scaled by `scale`; the local ray is taken into the
  *                 unscaled mesh frame by o/scale, d/scale (toi unchanged) and
- *                 answered by the mesh's own device BVH (a TRIANGLE sq_bvh).
+ *                 answered by the mesh's own device BVH (a TRIANGLE sq_bvh);
+ *   SQ_INST_BALL  a ball of radius r at the local origin (D = 3), the
+ *                 reference's BallInstances (tests/shape_overlap.rs:288-345):
+ *                 rays by ray_ball, shape casts by the ray against the radius
+ *                 sum (probe radius = the cast box's max x), overlap exact
+ *                 (distance from the centre to the region <= r).
+ * Boxes and meshes keep InstancedGeometry's defaults for the rest: no shape
+ * cast (test_shape_cast = None), AABB-level overlap (test_overlap = true),
+ * and every kind reports one crossing per instance (test_ray_crossings_local's
+ * default), so Tlas::raycast_crossings is sq_tlas_raycast_all.
  * An instance hit is kept only when its toi lies in the slab interval of the
- * instance's world AABB (order independence, as for the primitive kinds). */
+ * instance's world AABB (order independence, as for the primitive kinds); a
+ * cast contact only inside the probe-grown world AABB's slab interval. */
 #define SQ_INST_BOX 0
 #define SQ_INST_MESH 1
+#define SQ_INST_BALL 2
 /* Tlas::build (tlas.rs:35-45): rot float[n][D][D] row-major (orthonormal),
- * trans float[n][D], halves float[n][D] (BOX), meshes[n_meshes] (MESH, borrowed:
+ * trans float[n][D], halves float[n][D] (BOX) or radii float[n] (BALL), meshes[n_meshes] (MESH, borrowed:
  * keep them alive), mesh_of[n], scale[n].  Host pointers. */
 int sq_tlas_build(sq_ctx* ctx, int dim, int inst_kind, uint64_t n_inst, const float* rot, const float* trans,
                   const float* halves, sq_bvh* const* meshes, uint32_t n_meshes, const uint32_t* mesh_of,
@@ -278,10 +289,20 @@ int sq_tlas_raycast_any(sq_tlas* tlas, const float* rays, uint64_t n, float max_
 /* Tlas::raycast_all (tlas.rs:158-215), CSR as sq_raycast_all. */
 int sq_tlas_raycast_all(sq_tlas* tlas, const float* rays, uint64_t n, float max_toi, uint64_t* offsets,
                         int32_t* out_inst, float* out_toi, uint64_t capacity, uint64_t* total);
-/* Tlas::overlap with the default instance test (every instance of a touched
- * leaf node), CSR, instances in ascending order. */
+/* Tlas::overlap (tlas.rs:393-432): every instance of a touched leaf node that
+ * passes its test_overlap (exact for BALL, true otherwise), CSR, instances in
+ * ascending order. */
 int sq_tlas_overlap(sq_tlas* tlas, const float* regions, uint64_t n, uint64_t* offsets, int32_t* out_inst,
                     uint64_t capacity, uint64_t* total);
+/* Tlas::shapecast_nearest (tlas.rs:306-351): casts float[n][4*D] as
+ * sq_shapecast_nearest (origin, dir, probe box min, max); out_inst -1 and toi
+ * +inf on a miss (every cast misses box and mesh instances); out_point
+ * (optional) = origin + dir * toi. */
+int sq_tlas_shapecast_nearest(sq_tlas* tlas, const float* casts, uint64_t n, float max_toi, int32_t* out_inst,
+                              float* out_toi, float* out_point);
+/* Tlas::shapecast_all (tlas.rs:353-391), CSR sorted by (toi, instance). */
+int sq_tlas_shapecast_all(sq_tlas* tlas, const float* casts, uint64_t n, float max_toi, uint64_t* offsets,
+                          int32_t* out_inst, float* out_toi, uint64_t capacity, uint64_t* total);
 
 /* Kernel tuning knobs.  Results never depend on them (the tests check this);
  * only speed does.  The autotuner (sq_amd.autotune) picks them per scene.

@@ -133,11 +133,15 @@ template <class F> inline int dispatch_dk(int dim, int kind, F&& f) {
 #undef SQ_CASE
 }
 
-// Instance kinds of a two-level tree: BOX in D = 2..6, MESH in D = 3.
+// Instance kinds of a two-level tree: BOX in D = 2..6, MESH and BALL in D = 3.
 template <class F> inline int dispatch_inst(int dim, int kind, F&& f) {
   if (kind == INST_MESH) {
     if (dim != 3) return fail(SQ_ERR_UNSUPPORTED, "mesh instances need dim 3");
     return f.template operator()<3, INST_MESH>();
+  }
+  if (kind == INST_BALL) {
+    if (dim != 3) return fail(SQ_ERR_UNSUPPORTED, "ball instances need dim 3");
+    return f.template operator()<3, INST_BALL>();
   }
   if (kind != INST_BOX) return fail(SQ_ERR_INVALID, "unknown instance kind");
   switch (dim) {

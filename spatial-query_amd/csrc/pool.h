@@ -271,7 +271,7 @@ __device__ __forceinline__ bool tlas_step(const TlasView& tv, const RayD<D>& r, 
   if (w.phase == 1) {
     while (w.j < w.cnt) {
       const uint32_t id = s.ids[w.first + w.j];
-      if constexpr (IK == INST_BOX) {
+      if constexpr (IK != INST_MESH) {
         TlasHit h;
         if (instance_test<D, IK>(tv, r, id, w.limit, Stack{nullptr, nullptr, 0}, h)) {
           tlas_take<D>(w, h.t, h.inst, h.pos, h.sub);

@@ -13,13 +13,13 @@ namespace {
 struct Inst {
   int dim, kind;
   uint64_t n;
-  std::vector<float> rot, trans, halves, scale;   // by instance id
+  std::vector<float> rot, trans, halves, scale;   // by instance id (halves: radii for BALL)
   std::vector<uint32_t> mesh_of;
   std::vector<sq_bvh*> meshes;                    // borrowed
 };
 
 uint32_t record_stride(int dim, int kind) {
-  uint32_t f = kind == INST_BOX ? dim * dim + 2 * dim : dim * dim + dim + 2;
+  uint32_t f = kind == INST_BOX ? dim * dim + 2 * dim : (kind == INST_BALL ? dim * dim + dim + 1 : dim * dim + dim + 2);
   return (f + 3) & ~3u;
 }
 
@@ -32,6 +32,9 @@ void world_boxes(const Inst& in, std::vector<float>& out) {
     float lmin[sqh::MAX_DIM], lmax[sqh::MAX_DIM];
     if (in.kind == INST_BOX) {
       for (int k = 0; k < D; ++k) { lmax[k] = in.halves[i * D + k]; lmin[k] = lmax[k] * -1.0f; }
+    } else if (in.kind == INST_BALL) {
+      // local_aabb of a ball (shape_overlap.rs:320-322)
+      for (int k = 0; k < D; ++k) { lmax[k] = in.halves[i]; lmin[k] = -in.halves[i]; }
     } else {
       // local_aabb of a scaled mesh: the mesh root box times the scale
       const sq_bvh* m = in.meshes[in.mesh_of[i]];
@@ -67,6 +70,8 @@ void records(const Inst& in, std::vector<float>& out) {
     std::memcpy(x + D * D, in.trans.data() + i * D, D * 4);
     if (in.kind == INST_BOX) {
       std::memcpy(x + D * D + D, in.halves.data() + i * D, D * 4);
+    } else if (in.kind == INST_BALL) {
+      x[D * D + D] = in.halves[i];
     } else {
       x[D * D + D] = 1.0f / in.scale[i];
       std::memcpy(&x[D * D + D + 1], &in.mesh_of[i], 4);
@@ -148,8 +153,9 @@ __global__ __launch_bounds__(256) void k_tlas_nearest_pool(TlasView tv, const fl
 }
 
 // MODE 0: raycast_all (one hit per instance, tlas.rs:158-215); MODE 2: overlap
-// with the default instance test_overlap (every instance of a touched leaf
-// node, tlas.rs overlap + instance.rs:91-93).
+// (every instance of a touched leaf node passing its test_overlap: exact for
+// balls, the default true for the other kinds, tlas.rs:393-432); MODE 3:
+// shapecast_all (tlas.rs:353-391, casts [n][4D]).
 template <int D, int IK, int MODE, bool FILL>
 __global__ __launch_bounds__(1024) void k_tlas_collect(TlasView tv, const float* __restrict__ q, uint32_t n, float max_toi,
                                                        uint32_t* counts, const uint64_t* __restrict__ offsets, uint64_t* keys,
@@ -180,7 +186,28 @@ __global__ __launch_bounds__(1024) void k_tlas_collect(TlasView tv, const float*
           hb = box_overlap<D>(bmn, bmx, rg, rg + D);
         },
         [&](uint32_t first, uint32_t c) {
-          for (uint32_t j = 0; j < c; ++j) { if (FILL) dst[cnt] = s.ids[first + j]; ++cnt; }
+          for (uint32_t j = 0; j < c; ++j) {
+            const uint32_t id = s.ids[first + j];
+            if (!instance_overlap<D, IK>(tv, id, rg, rg + D)) continue;
+            if (FILL) dst[cnt] = id;
+            ++cnt;
+          }
+        }, st);
+    }
+  } else if constexpr (MODE == 3) {
+    CastPolicy<D, KIND_AABB> pol;
+    pol.setup(q + (size_t)i * 4 * D);
+    if (IK == INST_BALL && pol.root(s.root, max_toi)) {
+      walk_visit<D>(s, root_ref,
+        [&](const float* nd, bool& ha, bool& hb) { float ta, tb; pol.pair(nd, max_toi, ha, hb, ta, tb); },
+        [&](uint32_t first, uint32_t c) {
+          for (uint32_t j = 0; j < c; ++j) {
+            const uint32_t id = s.ids[first + j];
+            float tt;
+            if (!instance_cast<D, IK>(tv, pol, id, max_toi, tt)) continue;
+            if (FILL) dst[cnt] = hit_key(tt, id);
+            ++cnt;
+          }
         }, st);
     }
   } else {
@@ -201,6 +228,70 @@ __global__ __launch_bounds__(1024) void k_tlas_collect(TlasView tv, const float*
     }
   }
   if (!FILL) counts[i] = cnt;
+}
+
+// Tlas::shapecast_nearest (tlas.rs:306-351), near-first over the probe-grown
+// top-level boxes with the shrinking limit; one cast [4D] per thread, the
+// stack in LDS.  Kinds other than BALL answer no cast (instance_cast).
+template <int D, int IK>
+__global__ __launch_bounds__(1024) void k_tlas_cast_nearest(TlasView tv, const float* __restrict__ q, uint32_t n, float max_toi,
+                                                            int32_t* out_inst, float* out_toi, float* out_point,
+                                                            uint32_t s_top) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  const uint32_t i = blockIdx.x * B + tid;
+  if (i >= n) return;
+  const SceneView& s = tv.top;
+  Stack st{lds + tid, reinterpret_cast<float*>(lds + s_top * B) + tid, B};
+  CastPolicy<D, KIND_AABB> pol;
+  pol.setup(q + (size_t)i * 4 * D);
+  float limit = max_toi, best_t = SQ_INF;
+  uint32_t best = NO_LEAF;
+  if (IK == INST_BALL && pol.root(s.root, limit)) {
+    constexpr int NF = (D + 1) * 4;
+    uint32_t cur = fbits(s.root[2 * D]), sp = 0;
+    while (true) {
+      if (!(cur & LEAF_BIT)) {
+        const float* nd = s.nodes + (size_t)cur * NF;
+        bool ha, hb;
+        float ta, tb;
+        pol.pair(nd, limit, ha, hb, ta, tb);
+        const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
+        if (ha && hb) {
+          const bool a_first = ta <= tb;
+          st.ref[sp * st.stride] = a_first ? rb : ra;
+          st.t[sp * st.stride] = a_first ? tb : ta;
+          ++sp;
+          cur = a_first ? ra : rb;
+          continue;
+        }
+        if (ha) { cur = ra; continue; }
+        if (hb) { cur = rb; continue; }
+      } else {
+        const uint32_t first = cur & FIRST_MASK;
+        const uint32_t cnt = ((cur >> 27) & 15u) + 1u;
+        for (uint32_t j = 0; j < cnt; ++j) {
+          const uint32_t id = s.ids[first + j];
+          float t;
+          if (!instance_cast<D, IK>(tv, pol, id, limit, t)) continue;
+          if (best == NO_LEAF || hit_before(t, id, best_t, best)) { best = id; best_t = t; limit = t; }
+        }
+      }
+      bool got = false;
+      while (sp > 0) {
+        --sp;
+        if (st.t[sp * st.stride] <= limit) { cur = st.ref[sp * st.stride]; got = true; break; }
+      }
+      if (!got) break;
+    }
+  }
+  const bool hit = best != NO_LEAF;
+  out_inst[i] = hit ? (int32_t)best : -1;
+  out_toi[i] = best_t;
+  if (out_point) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) out_point[(size_t)i * D + k] = hit ? pol.r.o[k] + pol.r.d[k] * best_t : 0.0f;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -257,6 +348,11 @@ int sq_tlas_build(sq_ctx* ctx, int dim, int kind, uint64_t n, const float* rot, 
   if (kind == INST_BOX) {
     if (n && !halves) return fail(SQ_ERR_INVALID, "box instances need halves");
     in.halves.assign(halves, halves + n * dim);
+  } else if (kind == INST_BALL) {
+    if (n && !halves) return fail(SQ_ERR_INVALID, "ball instances need radii (in halves)");
+    in.halves.assign(halves, halves + n);
+    for (uint64_t i = 0; i < n; ++i)
+      if (!(in.halves[i] >= 0.0f)) return fail(SQ_ERR_INVALID, "ball radii must be >= 0");
   } else {
     if (n && (!meshes || !mesh_of || !scale)) return fail(SQ_ERR_INVALID, "mesh instances need meshes, mesh_of and scale");
     in.meshes.assign(meshes, meshes + n_meshes);
@@ -309,6 +405,7 @@ int sq_tlas_refit(sq_tlas* t, const float* rot, const float* trans, const float*
   in.rot.assign(rot, rot + in.n * in.dim * in.dim);
   in.trans.assign(trans, trans + in.n * in.dim);
   if (halves && in.kind == INST_BOX) in.halves.assign(halves, halves + in.n * in.dim);
+  if (halves && in.kind == INST_BALL) in.halves.assign(halves, halves + in.n);
   if (!in.n) return SQ_OK;
   std::vector<float> world;
   int rc;
@@ -439,6 +536,9 @@ static int tlas_collect_impl(sq_tlas* t, const float* q, uint64_t n, float max_t
       if (mode == 2) {
         if (fill) k_tlas_collect<D, K, 2, true><<<g, b, lds, c->stream>>>(tv, q, nn, max_toi, counts, offsets, keys, s_top, s_bot);
         else k_tlas_collect<D, K, 2, false><<<g, b, lds, c->stream>>>(tv, q, nn, max_toi, counts, offsets, keys, s_top, s_bot);
+      } else if (mode == 3) {
+        if (fill) k_tlas_collect<D, K, 3, true><<<g, b, lds, c->stream>>>(tv, q, nn, max_toi, counts, offsets, keys, s_top, s_bot);
+        else k_tlas_collect<D, K, 3, false><<<g, b, lds, c->stream>>>(tv, q, nn, max_toi, counts, offsets, keys, s_top, s_bot);
       } else {
         if (fill) k_tlas_collect<D, K, 0, true><<<g, b, lds, c->stream>>>(tv, q, nn, max_toi, counts, offsets, keys, s_top, s_bot);
         else k_tlas_collect<D, K, 0, false><<<g, b, lds, c->stream>>>(tv, q, nn, max_toi, counts, offsets, keys, s_top, s_bot);
@@ -458,6 +558,35 @@ int sq_tlas_raycast_all(sq_tlas* t, const float* rays, uint64_t n, float max_toi
 int sq_tlas_overlap(sq_tlas* t, const float* regions, uint64_t n, uint64_t* offsets, int32_t* out_inst, uint64_t capacity,
                     uint64_t* total) {
   return tlas_collect_impl(t, regions, n, 0.0f, 2, offsets, out_inst, nullptr, capacity, total);
+}
+
+int sq_tlas_shapecast_all(sq_tlas* t, const float* casts, uint64_t n, float max_toi, uint64_t* offsets, int32_t* out_inst,
+                          float* out_toi, uint64_t capacity, uint64_t* total) {
+  return tlas_collect_impl(t, casts, n, max_toi, 3, offsets, out_inst, out_toi, capacity, total);
+}
+
+int sq_tlas_shapecast_nearest(sq_tlas* t, const float* casts, uint64_t n, float max_toi, int32_t* out_inst, float* out_toi,
+                              float* out_point) {
+  if (!t || (n && (!casts || !out_inst || !out_toi))) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "at most 2^32-1 casts per call");
+  if (!n) return SQ_OK;
+  sq_ctx* c = t->ctx;
+  if (!t->in.n) return fill_miss(c, n, t->in.dim, NearestOut{out_inst, out_toi, out_point, nullptr, nullptr, nullptr});
+  const int B = c->block;
+  const uint32_t s_top = t->top->packed.max_stack;
+  const size_t lds = (size_t)s_top * B * 8;
+  if (lds > 160 * 1024) return fail(SQ_ERR_INVALID, "traversal stack does not fit in LDS; lower the block size");
+  TlasView tv = tview(t);
+  int rc = begin_timing(c);
+  if (rc) return rc;
+  rc = dispatch_inst(t->in.dim, t->in.kind, [&]<int D, int K>() -> int {
+    k_tlas_cast_nearest<D, K><<<grid_for(n, B), B, lds, c->stream>>>(tv, casts, (uint32_t)n, max_toi, out_inst, out_toi,
+                                                                     out_point, s_top);
+    HIP_TRY(hipGetLastError());
+    return SQ_OK;
+  });
+  if (rc) return rc;
+  return end_timing(c);
 }
 
 }  // extern "C"

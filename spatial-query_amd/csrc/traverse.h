@@ -780,12 +780,16 @@ SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const Ring
 //   INST_BOX  (any D): R[D*D] row-major, t[D], half[D]          two_level.rs:70-113
 //   INST_MESH (D = 3): R[9], t[3], 1/scale, mesh id (bits)        a uniformly
 //                      scaled triangle mesh answered by its own device BVH.
-// Both kinds keep an instance hit only when its toi lies in the slab interval
+//   INST_BALL (D = 3): R[9], t[3], radius                          a ball at the
+//                      local origin (BallInstances, shape_overlap.rs:288-345):
+//                      the one kind that also answers shape casts and an exact
+//                      overlap test (the others keep the trait defaults).
+// Every kind keeps an instance hit only when its toi lies in the slab interval
 // of the instance's world AABB, the guard that makes the top-level result
 // independent of traversal order.
 // ---------------------------------------------------------------------------
 
-enum { INST_BOX = 0, INST_MESH = 1 };
+enum { INST_BOX = 0, INST_MESH = 1, INST_BALL = 2 };
 
 // Isometry::inverse_transform_point / _vector (isometry.rs:70-78, 113-123):
 // rotate_transpose(o - t) and rotate_transpose(d), each accumulating from 0.
@@ -837,6 +841,25 @@ template <int D> SQ_HD bool local_box(const float* o, const float* d, const floa
   return true;
 }
 
+// ray_ball (shape_overlap.rs:120-138) about centre c: the entry root, else the
+// exit root; rejects t < 0 or t > max_toi.  With c = NULL the centre is the
+// origin (a local ray against a ball instance).
+template <int D> SQ_HD bool ray_ball(const float* o, const float* d, const float* c, float rad, float max_toi, float& t) {
+  float oc[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) oc[k] = c ? o[k] - c[k] : o[k];
+  const float b = dotD<D>(oc, d);
+  const float cc = dotD<D>(oc, oc) - rad * rad;
+  const float disc = b * b - cc;
+  if (disc < 0.0f) return false;
+  const float sq = SQ_SQRT_RN(disc);
+  float tt = -b - sq;
+  if (tt < 0.0f) tt = -b + sq;
+  if (tt < 0.0f || tt > max_toi) return false;
+  t = tt;
+  return true;
+}
+
 struct TlasView {
   SceneView top;            // pair nodes over instance world boxes; top.ids are instance ids
   const float* xf;          // instance records by id
@@ -863,6 +886,9 @@ SQ_HD bool instance_test(const TlasView& tv, const RayD<D>& r, uint32_t id, floa
     int ax;
     if (!local_box<D>(ol, dl, x + D * D + D, limit, t, ax)) return false;
     h.pos = (uint32_t)ax; h.sub = NO_LEAF;
+  } else if constexpr (IK == INST_BALL) {
+    if (!ray_ball<D>(ol, dl, nullptr, x[D * D + D], limit, t)) return false;
+    h.pos = 0; h.sub = NO_LEAF;
   } else {
     const float inv_s = x[D * D + D];
     const uint32_t mesh = fbits(x[D * D + D + 1]);
@@ -944,6 +970,17 @@ SQ_HD void tlas_normal(const TlasView& tv, const RayD<D>& r, const TlasHit& h, f
   if constexpr (IK == INST_BOX) {
 #pragma unroll
     for (int k = 0; k < D; ++k) nl[k] = ((uint32_t)k == h.pos) ? (dl[k] > 0.0f ? -1.0f : 1.0f) : 0.0f;
+  } else if constexpr (IK == INST_BALL) {
+    // normalize_or_zero(local ray at t) (shape_overlap.rs:136)
+    float v[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) v[k] = ol[k] + dl[k] * h.t;
+    const float len = SQ_SQRT_RN(dotD<D>(v, v));
+    if (len > F32_EPSILON) {
+      const float s = SQ_DIV_RN(1.0f, len);
+#pragma unroll
+      for (int k = 0; k < D; ++k) nl[k] = v[k] * s;
+    }
   } else {
     const float inv_s = x[D * D + D];
     const uint32_t mesh = fbits(x[D * D + D + 1]);
@@ -955,6 +992,43 @@ SQ_HD void tlas_normal(const TlasView& tv, const RayD<D>& r, const TlasHit& h, f
     hit_normal<D, KIND_TRI>(mr, tv.blas[mesh].prims + (size_t)h.pos * Prim<D, KIND_TRI>::STRIDE, h.t, limit, nl);
   }
   iso_rotate<D>(x, nl, n);
+}
+
+// Tlas shape casts (tlas.rs:306-391): the swept probe (radius = the cast
+// box's max x, as the built-in SPHERE cast) against ball instances, the ray
+// against the radius sum about the instance's centre (shape_overlap.rs:
+// 326-337), kept only inside the probe's slab interval against the
+// instance's world AABB (the guard).  The other kinds answer no cast
+// (InstancedGeometry::test_shape_cast's default, instance.rs:72-87).
+template <int D, int IK>
+SQ_HD bool instance_cast(const TlasView& tv, const CastPolicy<D, KIND_AABB>& pol, uint32_t id, float limit, float& t) {
+  if constexpr (IK != INST_BALL) {
+    return false;
+  } else {
+    const float* x = tv.xf + (size_t)id * tv.xs;
+    if (!ray_ball<D>(pol.r.o, pol.r.d, x + D * D, x[D * D + D] + pol.pmax[0], limit, t)) return false;
+    const float* w = tv.world + (size_t)id * 2 * D;
+    float te, tx;
+    if (!pol.grown_slab(w, w + D, limit, te, tx)) return false;
+    return te <= t && t <= tx;
+  }
+}
+
+// InstancedGeometry::test_overlap: exact for ball instances (the distance
+// from the centre to the region, shape_overlap.rs:59-72,339-341), the
+// default (true) for the other kinds.
+template <int D, int IK>
+SQ_HD bool instance_overlap(const TlasView& tv, uint32_t id, const float* rmn, const float* rmx) {
+  if constexpr (IK != INST_BALL) {
+    return true;
+  } else {
+    const float* x = tv.xf + (size_t)id * tv.xs;
+    float p[D + 1];
+#pragma unroll
+    for (int k = 0; k < D; ++k) p[k] = x[D * D + k];
+    p[D] = x[D * D + D];
+    return overlap_narrow<D, KIND_SPHERE>(p, rmn, rmx);
+  }
 }
 
 }  // namespace sqd

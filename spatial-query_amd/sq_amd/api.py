@@ -341,15 +341,16 @@ class Tlas:
     """Tlas<D> over built-in instance kinds (accel/tlas.rs), resident on the GPU.
 
     kind 0 (box): halves[n, D] local boxes; kind 1 (mesh, D = 3): meshes is a
-    list of TRIANGLE DeviceBvh, mesh_of[n], scale[n] uniform scales."""
+    list of TRIANGLE DeviceBvh, mesh_of[n], scale[n] uniform scales; kind 2
+    (ball, D = 3): halves[n] radii of balls at the local origin."""
 
-    BOX, MESH = 0, 1
+    BOX, MESH, BALL = 0, 1, 2
 
     def __init__(self, ctx: Context, dim: int, kind: int, rot, trans, halves=None, meshes=None, mesh_of=None, scale=None):
         rot = _host(rot).reshape(-1, dim, dim); trans = _host(trans).reshape(-1, dim)
         n = rot.shape[0]
         self._keep = []
-        hv = _host(halves).reshape(-1, dim) if halves is not None else None
+        hv = _host(halves).reshape(-1, 1 if kind == self.BALL else dim) if halves is not None else None
         if kind == self.MESH:
             arr = (C.c_void_p * len(meshes))(*[m._h.value for m in meshes])
             mo = _host(mesh_of, np.uint32); sc = _host(scale)
@@ -417,6 +418,31 @@ class Tlas:
             return lib().sq_tlas_overlap(self._h, _dp(regions), regions.shape[0], _dp(off), _dp(leaf), cap, C.byref(total))
         off, leaf, _ = DeviceBvh._csr(self, fn, regions.shape[0], regions.device, False, capacity)
         return off, leaf
+
+    def raycast_crossings(self, rays, max_toi, capacity=0):
+        """Tlas::raycast_crossings (tlas.rs:227-302): the built-in instance kinds
+        report one crossing per instance (test_ray_crossings_local's default), so
+        this is raycast_all."""
+        return self.raycast_all(rays, max_toi, capacity)
+
+    def shapecast_nearest(self, casts, max_toi, point=False):
+        """Tlas::shapecast_nearest, casts [n][4D] (origin, dir, probe box min, max): dict(leaf, toi[, point])."""
+        import torch
+        n = casts.shape[0]
+        leaf = torch.empty(n, dtype=torch.int32, device=casts.device)
+        toi = torch.empty(n, dtype=torch.float32, device=casts.device)
+        pt = torch.empty((n, self.dim), dtype=torch.float32, device=casts.device) if point else None
+        check(lib().sq_tlas_shapecast_nearest(self._h, _dp(casts), n, float(max_toi), _dp(leaf), _dp(toi), _dp(pt)))
+        out = dict(leaf=leaf, toi=toi)
+        if point: out["point"] = pt
+        return out
+
+    def shapecast_all(self, casts, max_toi, capacity=0):
+        """Tlas::shapecast_all: CSR (offsets, leaf, toi) sorted by (toi, instance)."""
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_tlas_shapecast_all(self._h, _dp(casts), casts.shape[0], float(max_toi), _dp(off), _dp(leaf),
+                                               _dp(toi), cap, C.byref(total))
+        return DeviceBvh._csr(self, fn, casts.shape[0], casts.device, True, capacity)
 
     def close(self):
         if getattr(self, "_h", None):

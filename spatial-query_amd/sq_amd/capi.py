@@ -60,6 +60,8 @@ SIGNATURES = {
     "sq_tlas_raycast_any": [P, P, U64, F32, P, P],
     "sq_tlas_raycast_all": [P, P, U64, F32, P, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_tlas_overlap": [P, P, U64, P, P, U64, C.POINTER(C.c_uint64)],
+    "sq_tlas_shapecast_nearest": [P, P, U64, F32, P, P, P],
+    "sq_tlas_shapecast_all": [P, P, U64, F32, P, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_ctx_set_block": [P, I32],
     "sq_batch_create": [P, I32, P, U32, PP],
     "sq_batch_submit": [P],
