@@ -175,7 +175,20 @@ int sq_bvh_rebuild(sq_bvh* bvh, const float* prims_device);
  *       const float* origin, const float* dir, float max_toi, Out& out);
  * calling out(toi) once per crossing in any order (crossings outside the box
  * interval are dropped by the same guard); without it a crossing query gets
- * the sq_narrow hit, the reference's default. */
+ * the sq_narrow hit, the reference's default.  Likewise optional: shape casts
+ * (QueryGeometry::test_shape_cast, geometry.rs:65-77)
+ *   #define SQ_NARROW_CAST 1
+ *   __device__ bool sq_narrow_cast(const float* prim, const float* origin,
+ *       const float* dir, const float* probe_min, const float* probe_max,
+ *       float max_toi, float* toi);
+ * (a contact counts inside the probe-grown slab interval of the primitive's
+ * box; without it no cast finds a contact, the trait's default) and overlap
+ * (test_overlap, geometry.rs:79-87)
+ *   #define SQ_NARROW_OVERLAP 1
+ *   __device__ bool sq_narrow_overlap(const float* prim, const float* region_min,
+ *       const float* region_max);
+ * asked for primitives whose world box meets the region (without it, that box
+ * test alone, the trait's default). */
 typedef struct sq_narrow sq_narrow;
 int sq_narrow_create(sq_ctx* ctx, int dim, const char* source, uint32_t prim_floats, sq_narrow** out);
 int sq_narrow_destroy(sq_narrow* narrow);
@@ -195,6 +208,17 @@ int sq_raycast_nearest_consumer(sq_narrow* narrow, sq_bvh* bvh, const float* ray
  * crossings != 0: CSR results, capacity and total as sq_raycast_all. */
 int sq_raycast_all_consumer(sq_narrow* narrow, sq_bvh* bvh, const float* rays, uint64_t n, float max_toi, int crossings,
                             uint64_t* offsets, int32_t* out_leaf, float* out_toi, uint64_t capacity, uint64_t* total);
+/* Bvh::shapecast_nearest / shapecast_all (bvh.rs:331-414) through the
+ * consumer's sq_narrow_cast: casts float[n][4*dim] as sq_shapecast_nearest;
+ * nearest leaf -1 / toi +inf on a miss; all = CSR as sq_raycast_all. */
+int sq_shapecast_nearest_consumer(sq_narrow* narrow, sq_bvh* bvh, const float* casts, uint64_t n, float max_toi,
+                                  int32_t* out_leaf, float* out_toi);
+int sq_shapecast_all_consumer(sq_narrow* narrow, sq_bvh* bvh, const float* casts, uint64_t n, float max_toi,
+                              uint64_t* offsets, int32_t* out_leaf, float* out_toi, uint64_t capacity, uint64_t* total);
+/* Bvh::overlap (bvh.rs:418-456) through the consumer's sq_narrow_overlap:
+ * CSR of ascending leaf ids as sq_overlap. */
+int sq_overlap_consumer(sq_narrow* narrow, sq_bvh* bvh, const float* regions, uint64_t n, uint64_t* offsets,
+                        int32_t* out_leaf, uint64_t capacity, uint64_t* total);
 
 /* ---- ray queries (bvh.rs:51-328) ------------------------------------------ */
 /* max_toi_per_ray (device, n floats) overrides max_toi when not NULL.

@@ -58,6 +58,7 @@ def _declare(L):
         "orc_geom_new": (P, [I32, I32, P, U32]),
         "orc_geom_new_consumer": (P, [I32, P, U32, I32, U64]),
         "orc_geom_set_consumer_crossings": (None, [P, U64]),
+        "orc_geom_set_consumer_cast_overlap": (None, [P, U64, U64]),
         "orc_geom_set_layers": (None, [P, P]),
         "orc_geom_set_prims": (None, [P, P]),
         "orc_geom_world_aabbs": (None, [P, P]),
@@ -146,14 +147,20 @@ class Scene:
         self._b = lib().orc_bvh_build(self._g) if build_tree else None
 
     @classmethod
-    def consumer(cls, dim: int, boxes: np.ndarray, prims: np.ndarray, narrow_addr: int, crossings_addr: int = 0) -> "Scene":
+    def consumer(cls, dim: int, boxes: np.ndarray, prims: np.ndarray, narrow_addr: int, crossings_addr: int = 0,
+                 cast_addr: int = 0, overlap_addr: int = 0) -> "Scene":
         """A consumer provider: world boxes [n][2*dim] and consumer floats
         [n][pf], tested by the C function at narrow_addr (signature
         int(const float* prim, const float* origin, const float* dir, float
         max_toi, float* toi)), box-guarded like the built-in SPHERE and TRI;
         crossings through the C function at crossings_addr (signature
         void(prim, origin, dir, max_toi, void (*emit)(void*, float), void*)),
-        each crossing box-guarded the same way, or the nearest hit when 0."""
+        each crossing box-guarded the same way, or the nearest hit when 0; shape
+        casts through the C function at cast_addr (int(prim, origin, dir,
+        probe_min, probe_max, max_toi, float* toi), guarded by the probe-grown
+        box; none when 0) and overlap through the one at overlap_addr
+        (int(prim, region_min, region_max) after the box test; the box test
+        alone when 0)."""
         boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 2 * dim)
         prims = np.ascontiguousarray(prims, dtype=np.float32).reshape(boxes.shape[0], -1)
         obj = cls.__new__(cls)
@@ -163,6 +170,8 @@ class Scene:
         obj._g = lib().orc_geom_new_consumer(dim, _p(obj.prims), obj.n, prims.shape[1], int(narrow_addr))
         if crossings_addr:
             lib().orc_geom_set_consumer_crossings(obj._g, int(crossings_addr))
+        if cast_addr or overlap_addr:
+            lib().orc_geom_set_consumer_cast_overlap(obj._g, int(cast_addr), int(overlap_addr))
         obj.layers = None
         obj._b = lib().orc_bvh_build(obj._g)
         return obj

@@ -270,6 +270,11 @@ typedef int (*ConsumerNarrow)(const float* prim, const float* origin, const floa
 // emit(ctx, toi) once per crossing of the ray with the primitive.
 typedef void (*ConsumerCrossings)(const float* prim, const float* origin, const float* dir, float max_toi,
                                   void (*emit)(void*, float), void* ctx);
+// A consumer's swept-box contact (test_shape_cast, geometry.rs:65-77) and exact
+// overlap test (test_overlap, geometry.rs:79-87).
+typedef int (*ConsumerCast)(const float* prim, const float* origin, const float* dir, const float* probe_min,
+                            const float* probe_max, float max_toi, float* toi);
+typedef int (*ConsumerOverlap)(const float* prim, const float* region_min, const float* region_max);
 
 template <int D> struct Geometry {
   int kind;
@@ -278,6 +283,8 @@ template <int D> struct Geometry {
   std::vector<uint64_t> layers;  // empty = every leaf on every layer
   ConsumerNarrow consumer = nullptr;   // KIND_CONSUMER: records are [world box 2D][consumer_pf floats]
   ConsumerCrossings consumer_x = nullptr;   // its crossings, when it reports more than the nearest hit
+  ConsumerCast consumer_cast = nullptr;     // its shape-cast contact (none without)
+  ConsumerOverlap consumer_ov = nullptr;    // its exact overlap test (the box test alone without)
   int consumer_pf = 0;
   int stride() const {
     if (kind == KIND_CONSUMER) return 2 * D + consumer_pf;
@@ -478,6 +485,19 @@ template <int D> struct Geometry {
   // Shape cast narrow (N4 for boxes; summed-radius ball for spheres,
   // shape_overlap.rs:104-118).  Triangles keep the default None.
   bool test_shape_cast(size_t leaf, const ShapeCast<D>& cast, float max_toi, LeafHit<D>* out) const {
+    if (kind == KIND_CONSUMER) {
+      // the contact under the test_shape_cast contract and the box guard: inside
+      // the probe-grown slab interval of the primitive's world box
+      if (!consumer_cast) return false;
+      float t;
+      if (!consumer_cast(p(leaf) + 2 * D, cast.origin.v, cast.dir.v, cast.aabb.min.v, cast.aabb.max.v, max_toi, &t)) return false;
+      if (!(t >= 0.0f && t <= max_toi)) return false;
+      float te, tx;
+      if (!cast.broadphase(world_aabb(leaf), max_toi, &te, &tx)) return false;
+      if (!(te <= t && t <= tx)) return false;
+      out->toi = t; out->normal = Point<D>::zero(); out->sub = -1;
+      return true;
+    }
     if (kind == KIND_BOX) {
       float te, tx;
       if (!cast.broadphase(world_aabb(leaf), max_toi, &te, &tx)) return false;
@@ -508,6 +528,10 @@ template <int D> struct Geometry {
   // (shape_overlap.rs:60-72,121-124); triangle = its AABB.
   bool test_overlap(size_t leaf, const Aabb<D>& region) const {
     if (kind == KIND_BOX || kind == KIND_TRI || kind == KIND_TRI_PLAIN) return world_aabb(leaf).intersects(region);
+    if (kind == KIND_CONSUMER) {
+      if (!world_aabb(leaf).intersects(region)) return false;
+      return !consumer_ov || consumer_ov(p(leaf) + 2 * D, region.min.v, region.max.v);
+    }
     const float* q = p(leaf);
     float d2 = 0.0f;
     for (int i = 0; i < D; ++i) {
@@ -1298,6 +1322,14 @@ void* orc_geom_new_consumer(int D, const float* records, uint32_t n, int pf, uin
     g->p = p;
   });
   return g;
+}
+// Adds the consumer's shape-cast and overlap functions (addresses; 0 = none).
+void orc_geom_set_consumer_cast_overlap(void* h, uint64_t cast_fn, uint64_t overlap_fn) {
+  OGeom* g = (OGeom*)h;
+  ORC_DISPATCH(g->D, {
+    ((Geometry<DD>*)g->p)->consumer_cast = reinterpret_cast<ConsumerCast>(cast_fn);
+    ((Geometry<DD>*)g->p)->consumer_ov = reinterpret_cast<ConsumerOverlap>(overlap_fn);
+  });
 }
 // Adds the consumer's crossings function (an address; 0 = the nearest hit only).
 void orc_geom_set_consumer_crossings(void* h, uint64_t fn) {

@@ -15,8 +15,19 @@
 //   template <class Out> __device__ void sq_narrow_crossings(const float* prim,
 //       const float* origin, const float* dir, float max_toi, Out& out);
 // calling out(toi) once per crossing, in any order.  Without it, crossings
-// are the single sq_narrow hit (the reference's default).  The same module
-// holds the all-hits and crossings walks (pool_collect_with).  The tree is built from the consumer's world boxes
+// are the single sq_narrow hit (the reference's default).  Optionally too, a
+// swept-box contact (QueryGeometry::test_shape_cast, geometry.rs:65-77) and an
+// exact overlap test (test_overlap, geometry.rs:79-87):
+//   #define SQ_NARROW_CAST 1
+//   __device__ bool sq_narrow_cast(const float* prim, const float* origin,
+//       const float* dir, const float* probe_min, const float* probe_max,
+//       float max_toi, float* toi);
+//   #define SQ_NARROW_OVERLAP 1
+//   __device__ bool sq_narrow_overlap(const float* prim, const float* region_min,
+//       const float* region_max);
+// Without them a cast finds no contact and overlap is the box test (the
+// trait's defaults).  The same module holds the all-hits, crossings, cast and
+// overlap walks (pool_walk_with, pool_collect_with).  The tree is built from the consumer's world boxes
 // (QueryGeometry::world_aabb); each leaf-ordered primitive record is its box
 // followed by its prim_floats floats, of which sq_narrow sees the latter.  As
 // for the built-in SPHERE and TRIANGLE kinds, a hit is kept only when its toi
@@ -70,7 +81,7 @@ template <int D> struct ConsumerPolicy {
 // Persistent pooled walk (as the built-in kinds' k_nearest_pool): waves take
 // chunks of `chunk` rays from counter[0] (zero at the start, left zero by
 // pool_retire), refill lanes as rays finish, pause descents below pause_below
-// lanes while the pool has rays left.
+// lanes.
 extern "C" __global__ __launch_bounds__(256) void sq_consumer_nearest(sqd::SceneView s, const float* rays, unsigned n,
                                                                       float max_toi, int* out_leaf, float* out_toi,
                                                                       unsigned chunk, unsigned* counter, unsigned S,
@@ -131,7 +142,87 @@ template <int D, bool CROSS> struct ConsumerCollect {
   }
 };
 
-template <bool CROSS>
+// Shape casts through the consumer (QueryGeometry::test_shape_cast,
+// geometry.rs:65-77): queries [4D] = origin, dir, probe box min, max; the
+// probe-grown slab prunes (ShapeCast::broadphase, shapecast.rs:70-73) and a
+// contact counts when its toi is in [0, limit] and inside the grown slab
+// interval of the primitive's own box.  Without SQ_NARROW_CAST there is no
+// contact (the trait's default).
+template <int D> struct ConsumerCastPolicy : CastPolicy<D, KIND_AABB> {
+  __device__ bool narrow(const float* p, float limit, float& t) const {
+#if defined(SQ_NARROW_CAST)
+    float tt;
+    if (!sq_narrow_cast(p + 2 * D, this->r.o, this->r.d, this->pmin, this->pmax, limit, &tt)) return false;
+    if (!(tt >= 0.0f && tt <= limit)) return false;
+    float te, tx;
+    if (!this->grown_slab(p, p + D, limit, te, tx)) return false;
+    if (!(te <= tt && tt <= tx)) return false;
+    t = tt;
+    return true;
+#else
+    (void)p; (void)limit; (void)t;
+    return false;
+#endif
+  }
+};
+
+// All shape-cast contacts (shapecast_all, bvh.rs:381-414) per cast.
+template <int D> struct ConsumerCastAll {
+  ConsumerCastPolicy<D> pol;
+  float lim;
+  SQ_HD bool begin(const SceneView& s, const float* q, unsigned i, float max_toi, const float*) {
+    pol.setup(q + (size_t)i * 4 * D);
+    lim = max_toi;
+    return pol.root(s.root, lim);
+  }
+  SQ_HD void pair(const float* nd, bool& ha, bool& hb) const {
+    float ta, tb;
+    pol.pair(nd, lim, ha, hb, ta, tb);
+  }
+  template <class Emit>
+  SQ_HD void leaf(const SceneView& s, const Filter&, unsigned first, unsigned c, Emit&& emit) const {
+    for (unsigned j = 0; j < c; ++j) {
+      const unsigned pos = first + j;
+      float t;
+      if (pol.narrow(s.prims + (size_t)pos * (2 * D + SQ_RTC_PF), lim, t)) emit(hit_key(t, s.ids[pos]));
+    }
+  }
+};
+
+// Overlap (Bvh::overlap, bvh.rs:418-456) per region [2D]: every primitive
+// whose world box meets the region and that passes the consumer's
+// sq_narrow_overlap (QueryGeometry::test_overlap, geometry.rs:79-87; without
+// SQ_NARROW_OVERLAP the box test alone, the trait's default), keyed by leaf.
+template <int D> struct ConsumerOverlap {
+  float mn[D], mx[D];
+  SQ_HD bool begin(const SceneView& s, const float* q, unsigned i, float, const float*) {
+    const float* p = q + (size_t)i * 2 * D;
+#pragma unroll
+    for (int k = 0; k < D; ++k) { mn[k] = p[k]; mx[k] = p[D + k]; }
+    return box_overlap<D>(s.root, s.root + D, mn, mx);
+  }
+  SQ_HD void pair(const float* nd, bool& ha, bool& hb) const {
+    float amn[D], amx[D], bmn[D], bmx[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) { amn[k] = nd[4 * k]; amx[k] = nd[4 * k + 1]; bmn[k] = nd[4 * k + 2]; bmx[k] = nd[4 * k + 3]; }
+    ha = box_overlap<D>(amn, amx, mn, mx);
+    hb = box_overlap<D>(bmn, bmx, mn, mx);
+  }
+  template <class Emit>
+  SQ_HD void leaf(const SceneView& s, const Filter&, unsigned first, unsigned c, Emit&& emit) const {
+    for (unsigned j = 0; j < c; ++j) {
+      const unsigned pos = first + j;
+      const float* p = s.prims + (size_t)pos * (2 * D + SQ_RTC_PF);
+      if (!box_overlap<D>(p, p + D, mn, mx)) continue;
+#if defined(SQ_NARROW_OVERLAP)
+      if (!sq_narrow_overlap(p + 2 * D, mn, mx)) continue;
+#endif
+      emit((unsigned long long)s.ids[pos]);
+    }
+  }
+};
+
+template <class CQ>
 __device__ __forceinline__ void consumer_collect(const SceneView& s, const float* rays, unsigned n, float max_toi,
                                                  unsigned chunk, unsigned* counter, unsigned S, SpillEntry* spill,
                                                  unsigned spill_stride, unsigned pause_below, const StageOut& o) {
@@ -147,7 +238,7 @@ __device__ __forceinline__ void consumer_collect(const SceneView& s, const float
   const unsigned long long waves = (unsigned long long)gridDim.x * (B >> 6);
   const unsigned long long home = (unsigned long long)blockIdx.x * (B >> 6) + (tid >> 6);
   const Filter f{nullptr, ~0ull, -1};
-  pool_collect_with<SQ_RTC_D, ConsumerCollect<SQ_RTC_D, CROSS>, false>(s, rays, Chunks{counter, n, chunk, home, waves},
+  pool_collect_with<SQ_RTC_D, CQ, false>(s, rays, Chunks{counter, n, chunk, home, waves},
                                                                         max_toi, nullptr, f, st, StageAppend{o, slot},
                                                                         pause_below, nullptr, nullptr);
 }
@@ -162,14 +253,53 @@ extern "C" __global__ __launch_bounds__(256) void sq_consumer_all(sqd::SceneView
                                                                   float max_toi, unsigned chunk, unsigned* counter,
                                                                   unsigned S, sqd::SpillEntry* spill, unsigned spill_stride,
                                                                   unsigned pause_below, sqd::StageOut o) {
-  sqd::consumer_collect<false>(s, rays, n, max_toi, chunk, counter, S, spill, spill_stride, pause_below, o);
+  sqd::consumer_collect<sqd::ConsumerCollect<SQ_RTC_D, false>>(s, rays, n, max_toi, chunk, counter, S, spill, spill_stride,
+                                                               pause_below, o);
 }
 extern "C" __global__ __launch_bounds__(256) void sq_consumer_crossings(sqd::SceneView s, const float* rays, unsigned n,
                                                                         float max_toi, unsigned chunk, unsigned* counter,
                                                                         unsigned S, sqd::SpillEntry* spill,
                                                                         unsigned spill_stride, unsigned pause_below,
                                                                         sqd::StageOut o) {
-  sqd::consumer_collect<true>(s, rays, n, max_toi, chunk, counter, S, spill, spill_stride, pause_below, o);
+  sqd::consumer_collect<sqd::ConsumerCollect<SQ_RTC_D, true>>(s, rays, n, max_toi, chunk, counter, S, spill, spill_stride,
+                                                              pause_below, o);
+}
+extern "C" __global__ __launch_bounds__(256) void sq_consumer_cast_all(sqd::SceneView s, const float* casts, unsigned n,
+                                                                       float max_toi, unsigned chunk, unsigned* counter,
+                                                                       unsigned S, sqd::SpillEntry* spill,
+                                                                       unsigned spill_stride, unsigned pause_below,
+                                                                       sqd::StageOut o) {
+  sqd::consumer_collect<sqd::ConsumerCastAll<SQ_RTC_D>>(s, casts, n, max_toi, chunk, counter, S, spill, spill_stride,
+                                                        pause_below, o);
+}
+extern "C" __global__ __launch_bounds__(256) void sq_consumer_overlap(sqd::SceneView s, const float* regions, unsigned n,
+                                                                      float max_toi, unsigned chunk, unsigned* counter,
+                                                                      unsigned S, sqd::SpillEntry* spill,
+                                                                      unsigned spill_stride, unsigned pause_below,
+                                                                      sqd::StageOut o) {
+  sqd::consumer_collect<sqd::ConsumerOverlap<SQ_RTC_D>>(s, regions, n, max_toi, chunk, counter, S, spill, spill_stride,
+                                                        pause_below, o);
+}
+
+// Nearest shape-cast contact per cast [4D] (shapecast_nearest,
+// bvh.rs:331-378), the pooled walk with the consumer's cast policy.
+extern "C" __global__ __launch_bounds__(256) void sq_consumer_cast_nearest(sqd::SceneView s, const float* casts, unsigned n,
+                                                                           float max_toi, int* out_leaf, float* out_toi,
+                                                                           unsigned chunk, unsigned* counter, unsigned S,
+                                                                           sqd::SpillEntry* spill, unsigned spill_stride,
+                                                                           unsigned pause_below) {
+  extern __shared__ unsigned lds[];
+  const unsigned B = blockDim.x, tid = threadIdx.x;
+  const unsigned gtid = blockIdx.x * B + tid;
+  sqd::RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
+  const unsigned long long waves = (unsigned long long)gridDim.x * (B >> 6);
+  const unsigned long long home = (unsigned long long)blockIdx.x * (B >> 6) + (tid >> 6);
+  const sqd::NearestOut out{out_leaf, out_toi, nullptr, nullptr, nullptr, nullptr};
+  const sqd::Filter f{nullptr, ~0ull, -1};
+  sqd::pool_walk_with<SQ_RTC_D, sqd::KIND_AABB, false, false, true, sqd::ConsumerCastPolicy<SQ_RTC_D>, 4 * SQ_RTC_D,
+                      2 * SQ_RTC_D + SQ_RTC_PF>(s, casts, sqd::Chunks{counter, n, chunk, home, waves}, max_toi, nullptr, f,
+                                                out, st, nullptr, pause_below);
+  sqd::pool_retire(counter);
 }
 )SQRTC";
 
@@ -183,6 +313,9 @@ struct sq_narrow {
   hipFunction_t fn = nullptr;          // nearest hit
   hipFunction_t fn_all = nullptr;      // all hits
   hipFunction_t fn_cross = nullptr;    // all crossings
+  hipFunction_t fn_cast = nullptr;     // nearest shape-cast contact
+  hipFunction_t fn_cast_all = nullptr; // all shape-cast contacts
+  hipFunction_t fn_overlap = nullptr;  // overlap
   // resident blocks per CU by (kernel, block, LDS), queried once each
   std::map<std::tuple<hipFunction_t, uint32_t, size_t>, int> occ;
 };
@@ -238,7 +371,10 @@ int sq_narrow_create(sq_ctx* ctx, int dim, const char* source, uint32_t prim_flo
   HIP_TRY(hipModuleLoadData(&nw->mod, code.data()));
   if (hipModuleGetFunction(&nw->fn, nw->mod, "sq_consumer_nearest") != hipSuccess ||
       hipModuleGetFunction(&nw->fn_all, nw->mod, "sq_consumer_all") != hipSuccess ||
-      hipModuleGetFunction(&nw->fn_cross, nw->mod, "sq_consumer_crossings") != hipSuccess) {
+      hipModuleGetFunction(&nw->fn_cross, nw->mod, "sq_consumer_crossings") != hipSuccess ||
+      hipModuleGetFunction(&nw->fn_cast, nw->mod, "sq_consumer_cast_nearest") != hipSuccess ||
+      hipModuleGetFunction(&nw->fn_cast_all, nw->mod, "sq_consumer_cast_all") != hipSuccess ||
+      hipModuleGetFunction(&nw->fn_overlap, nw->mod, "sq_consumer_overlap") != hipSuccess) {
     (void)hipModuleUnload(nw->mod);
     return fail(SQ_ERR_HIP, "consumer kernel not found in the compiled module");
   }
@@ -297,17 +433,29 @@ int sq_bvh_build_consumer(sq_narrow* nw, const float* boxes, const float* prims,
   return SQ_OK;
 }
 
-int sq_raycast_nearest_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uint64_t n, float max_toi, int32_t* out_leaf,
-                                float* out_toi) {
-  if (!nw || !b || (n && (!rays || !out_leaf || !out_toi))) return fail(SQ_ERR_INVALID, "NULL argument");
+}  // extern "C"
+
+// The tree was built for this narrow phase (same dimension, record size and
+// context) and holds consumer records.
+static int check_pair(const sq_narrow* nw, const sq_bvh* b) {
   if (b->kind != KIND_CONSUMER || b->dim != nw->dim || b->consumer_pf != nw->pf || b->ctx != nw->ctx)
     return fail(SQ_ERR_INVALID, "the tree was not built for this narrow phase");
+  return SQ_OK;
+}
+
+// One pooled nearest launch of module kernel fn (sq_consumer_nearest or
+// sq_consumer_cast_nearest, queries of qs floats), as nearest_impl launches
+// the built-in kinds: chunks of 64 * per queries, a grid of the resident
+// waves, ring stacks spilling.
+static int consumer_nearest(sq_narrow* nw, sq_bvh* b, hipFunction_t fn, const float* q, uint64_t n, float max_toi,
+                            int32_t* out_leaf, float* out_toi) {
+  if (!nw || !b || (n && (!q || !out_leaf || !out_toi))) return fail(SQ_ERR_INVALID, "NULL argument");
+  int rc;
+  if ((rc = check_pair(nw, b))) return rc;
   if (n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "at most 2^32-1 queries per call");
   if (!n) return SQ_OK;
   sq_ctx* c = b->ctx;
   if (!b->n_prims) return fill_miss(c, n, b->dim, NearestOut{out_leaf, out_toi, nullptr, nullptr, nullptr, nullptr});
-  // the pooled launch of the built-in kinds (nearest_impl): chunks of
-  // 64 * per rays, a grid of the resident waves, ring stacks spilling
   const uint32_t B = (uint32_t)c->block;
   uint32_t S = (uint32_t)c->lds_stack;
   const uint32_t per = pooled_per_lane(c, n);
@@ -317,29 +465,32 @@ int sq_raycast_nearest_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uin
   const uint32_t spill_rows = stack_n > S ? stack_n - S : 0;
   const size_t lds = (size_t)S * B * 8;
   int per_cu = 0;
-  int rc;
-  if ((rc = module_blocks_per_cu(nw, nw->fn, B, lds, &per_cu))) return rc;
+  if ((rc = module_blocks_per_cu(nw, fn, B, lds, &per_cu))) return rc;
   const uint64_t wpb = B / 64;
   const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
   uint32_t lanes = (uint32_t)(grid * B);
   if ((rc = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64))) return rc;
   SceneView s = view_of(b);
-  const float* rp = rays;
+  const float* qp = q;
   uint32_t nn = (uint32_t)n, chunk = (uint32_t)chunk64, pause = (uint32_t)c->descend_min;
   int32_t* lp = out_leaf;
   float* tp = out_toi;
   uint32_t* counter = c->pool_ctr.as<uint32_t>();
   SpillEntry* spill = c->spill.as<SpillEntry>();
-  void* args[] = {&s, &rp, &nn, &max_toi, &lp, &tp, &chunk, &counter, &S, &spill, &lanes, &pause};
-  HIP_TRY(hipModuleLaunchKernel(nw->fn, (unsigned)grid, 1, 1, B, 1, 1, (unsigned)lds, c->stream, args, nullptr));
+  void* args[] = {&s, &qp, &nn, &max_toi, &lp, &tp, &chunk, &counter, &S, &spill, &lanes, &pause};
+  HIP_TRY(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, B, 1, 1, (unsigned)lds, c->stream, args, nullptr));
   return SQ_OK;
 }
 
-int sq_raycast_all_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uint64_t n, float max_toi, int crossings,
-                            uint64_t* offsets, int32_t* out_leaf, float* out_toi, uint64_t capacity, uint64_t* total) {
-  if (!nw || !b || !offsets || !total || (n && !rays)) return fail(SQ_ERR_INVALID, "NULL argument");
-  if (b->kind != KIND_CONSUMER || b->dim != nw->dim || b->consumer_pf != nw->pf || b->ctx != nw->ctx)
-    return fail(SQ_ERR_INVALID, "the tree was not built for this narrow phase");
+// Variable-length results through module kernel fn (all hits, crossings,
+// cast contacts or overlaps): the pooled collect of the built-in kinds
+// (collect_impl without sub-walks) and its staged CSR pipeline.
+static int consumer_collect(sq_narrow* nw, sq_bvh* b, hipFunction_t fn, const float* q, uint64_t n, float max_toi,
+                            bool has_t, uint64_t* offsets, int32_t* out_leaf, float* out_toi, uint64_t capacity,
+                            uint64_t* total) {
+  if (!nw || !b || !offsets || !total || (n && !q)) return fail(SQ_ERR_INVALID, "NULL argument");
+  int rc;
+  if ((rc = check_pair(nw, b))) return rc;
   if (n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "at most 2^32-1 queries per call");
   sq_ctx* c = b->ctx;
   *total = 0;
@@ -348,10 +499,8 @@ int sq_raycast_all_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uint64_
     HIP_TRY(hipStreamSynchronize(c->stream));
     return SQ_OK;
   }
-  int rc;
   if ((rc = c->counts.ensure(n * 4))) return rc;
   if ((rc = c->counter.ensure(256))) return rc;
-  // the pooled collect of the built-in kinds (collect_impl, no sub-walks)
   const uint32_t B = (uint32_t)c->block;
   uint32_t S = (uint32_t)c->lds_stack;
   const uint32_t per = pooled_per_lane(c, n);
@@ -360,7 +509,6 @@ int sq_raycast_all_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uint64_
   const uint32_t stack_n = b->packed.max_stack;
   const uint32_t spill_rows = stack_n > S ? stack_n - S : 0;
   const size_t lds = (size_t)S * B * 8 + (size_t)(B / 64) * 8;
-  hipFunction_t fn = crossings ? nw->fn_cross : nw->fn_all;
   int per_cu = 0;
   if ((rc = module_blocks_per_cu(nw, fn, B, lds, &per_cu))) return rc;
   const uint64_t wpb = B / 64;
@@ -380,15 +528,46 @@ int sq_raycast_all_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uint64_
     StageOut o{c->stage.as<StageEntry>(), c->stage_used.as<uint32_t>(), counter + 1, counter + 2, (uint32_t)max_blocks,
                c->counts.as<uint32_t>()};
     SceneView s = view_of(b);
-    const float* rp = rays;
+    const float* qp = q;
     uint32_t nn = (uint32_t)n, chunk = (uint32_t)chunk64, pause = (uint32_t)c->descend_min;
     SpillEntry* spill = c->spill.as<SpillEntry>();
-    void* args[] = {&s, &rp, &nn, &max_toi, &chunk, &counter, &S, &spill, &lanes, &pause, &o};
+    void* args[] = {&s, &qp, &nn, &max_toi, &chunk, &counter, &S, &spill, &lanes, &pause, &o};
     HIP_TRY(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, B, 1, 1, (unsigned)lds, c->stream, args, nullptr));
     walk = StagedWalk{nullptr, nullptr};
     return SQ_OK;
   };
-  return csr_run_staged(c, n, offsets, out_leaf, out_toi, true, capacity, total, launch);
+  return csr_run_staged(c, n, offsets, out_leaf, out_toi, has_t, capacity, total, launch);
+}
+
+extern "C" {
+
+int sq_raycast_nearest_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uint64_t n, float max_toi, int32_t* out_leaf,
+                                float* out_toi) {
+  return consumer_nearest(nw, b, nw ? nw->fn : nullptr, rays, n, max_toi, out_leaf, out_toi);
+}
+
+int sq_raycast_all_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uint64_t n, float max_toi, int crossings,
+                            uint64_t* offsets, int32_t* out_leaf, float* out_toi, uint64_t capacity, uint64_t* total) {
+  if (!nw) return fail(SQ_ERR_INVALID, "NULL argument");
+  return consumer_collect(nw, b, crossings ? nw->fn_cross : nw->fn_all, rays, n, max_toi, true, offsets, out_leaf, out_toi,
+                          capacity, total);
+}
+
+int sq_shapecast_nearest_consumer(sq_narrow* nw, sq_bvh* b, const float* casts, uint64_t n, float max_toi, int32_t* out_leaf,
+                                  float* out_toi) {
+  return consumer_nearest(nw, b, nw ? nw->fn_cast : nullptr, casts, n, max_toi, out_leaf, out_toi);
+}
+
+int sq_shapecast_all_consumer(sq_narrow* nw, sq_bvh* b, const float* casts, uint64_t n, float max_toi, uint64_t* offsets,
+                              int32_t* out_leaf, float* out_toi, uint64_t capacity, uint64_t* total) {
+  if (!nw) return fail(SQ_ERR_INVALID, "NULL argument");
+  return consumer_collect(nw, b, nw->fn_cast_all, casts, n, max_toi, true, offsets, out_leaf, out_toi, capacity, total);
+}
+
+int sq_overlap_consumer(sq_narrow* nw, sq_bvh* b, const float* regions, uint64_t n, uint64_t* offsets, int32_t* out_leaf,
+                        uint64_t capacity, uint64_t* total) {
+  if (!nw) return fail(SQ_ERR_INVALID, "NULL argument");
+  return consumer_collect(nw, b, nw->fn_overlap, regions, n, 0.0f, false, offsets, out_leaf, nullptr, capacity, total);
 }
 
 }  // extern "C"

@@ -524,6 +524,30 @@ class ConsumerBvh:
                                                  int(crossings), _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
         return self._csr(fn, rays.shape[0], rays.device, True, capacity)
 
+    def shapecast_nearest(self, casts, max_toi):
+        """Nearest contact per cast [n][4D] (origin, dir, probe box min, max) through sq_narrow_cast."""
+        import torch
+        n = casts.shape[0]
+        leaf = torch.empty(n, dtype=torch.int32, device=casts.device)
+        toi = torch.empty(n, dtype=torch.float32, device=casts.device)
+        check(lib().sq_shapecast_nearest_consumer(self.narrow._h, self._h, _dp(casts), n, float(max_toi), _dp(leaf),
+                                                  _dp(toi)))
+        return {"leaf": leaf, "toi": toi}
+
+    def shapecast_all(self, casts, max_toi, capacity=0):
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_shapecast_all_consumer(self.narrow._h, self._h, _dp(casts), casts.shape[0], float(max_toi),
+                                                   _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+        return self._csr(fn, casts.shape[0], casts.device, True, capacity)
+
+    def overlap(self, regions, capacity=0):
+        """CSR (offsets, leaf) of ascending leaf ids through sq_narrow_overlap."""
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_overlap_consumer(self.narrow._h, self._h, _dp(regions), regions.shape[0], _dp(off), _dp(leaf),
+                                             cap, C.byref(total))
+        off, leaf, _ = self._csr(fn, regions.shape[0], regions.device, False, capacity)
+        return off, leaf
+
     def close(self):
         if getattr(self, "_h", None):
             lib().sq_bvh_destroy(self._h)

@@ -49,6 +49,43 @@ __device__ bool sq_narrow(const float* p, const float* o, const float* d, float 
 """
 
 
+# BALL with the built-in SPHERE kind's shape cast (a ball probe of radius the
+# probe box's max x: the ray against the radius sum, shape_overlap.rs:96-110)
+# and exact overlap (distance from the centre to the region <= r,
+# shape_overlap.rs:59-72,121-124), in the same operation order.
+BALL_FULL = BALL + r"""
+#define SQ_NARROW_CAST 1
+__device__ bool sq_narrow_cast(const float* p, const float* o, const float* d, const float* pmin, const float* pmax,
+                               float max_toi, float* toi) {
+  float oc[SQ_RTC_D];
+  for (int k = 0; k < SQ_RTC_D; ++k) oc[k] = o[k] - p[k];
+  const float rad = p[SQ_RTC_D] + pmax[0];
+  float b = 0.0f, cc = 0.0f;
+  for (int k = 0; k < SQ_RTC_D; ++k) b = b + oc[k] * d[k];
+  for (int k = 0; k < SQ_RTC_D; ++k) cc = cc + oc[k] * oc[k];
+  cc = cc - rad * rad;
+  const float disc = b * b - cc;
+  if (disc < 0.0f) return false;
+  const float sq = __builtin_sqrtf(disc);
+  float t = -b - sq;
+  if (t < 0.0f) t = -b + sq;
+  if (t < 0.0f || t > max_toi) return false;
+  *toi = t;
+  return true;
+}
+#define SQ_NARROW_OVERLAP 1
+__device__ bool sq_narrow_overlap(const float* p, const float* mn, const float* mx) {
+  float d2 = 0.0f;
+  for (int k = 0; k < SQ_RTC_D; ++k) {
+    float v = 0.0f;
+    if (p[k] < mn[k]) v = mn[k] - p[k];
+    else if (p[k] > mx[k]) v = p[k] - mx[k];
+    d2 = d2 + v * v;
+  }
+  return __builtin_sqrtf(d2) <= p[SQ_RTC_D];
+}
+"""
+
 # The spheres of crossings.rs:38-111 (Shells): both roots of the unbounded ray
 # are crossings; the nearest hit is the first root at or after the origin.
 # Records: centre [D], radius.
@@ -160,6 +197,14 @@ def cpu_entry_points(source: str, dim: int, pf: int, workdir: str):
     code = (f"#define __device__\n#define SQ_RTC_D {dim}\n#define SQ_RTC_PF {pf}\n" + source +
             '\nextern "C" int sq_narrow_c(const float* p, const float* o, const float* d, float m, float* t) '
             "{ return sq_narrow(p, o, d, m, t) ? 1 : 0; }\n"
+            "#ifdef SQ_NARROW_CAST\n"
+            'extern "C" int sq_cast_c(const float* p, const float* o, const float* d, const float* a, const float* b, '
+            "float m, float* t) { return sq_narrow_cast(p, o, d, a, b, m, t) ? 1 : 0; }\n"
+            "#endif\n"
+            "#ifdef SQ_NARROW_OVERLAP\n"
+            'extern "C" int sq_overlap_c(const float* p, const float* a, const float* b) '
+            "{ return sq_narrow_overlap(p, a, b) ? 1 : 0; }\n"
+            "#endif\n"
             "#ifdef SQ_NARROW_CROSSINGS\n"
             "struct SqEmitC { void (*f)(void*, float); void* c; void operator()(float t) const { f(c, t); } };\n"
             'extern "C" void sq_crossings_c(const float* p, const float* o, const float* d, float m, '
@@ -176,6 +221,14 @@ def cpu_entry_points(source: str, dim: int, pf: int, workdir: str):
     lib = ctypes.CDLL(so)
     x = ctypes.cast(lib.sq_crossings_c, ctypes.c_void_p).value if hasattr(lib, "sq_crossings_c") else 0
     return lib, ctypes.cast(lib.sq_narrow_c, ctypes.c_void_p).value, x
+
+
+def cast_overlap_entry_points(lib):
+    """Addresses of the shape-cast and overlap entry points of a library from
+    cpu_entry_points (0 where the source defines none)."""
+    def addr(name):
+        return ctypes.cast(getattr(lib, name), ctypes.c_void_p).value if hasattr(lib, name) else 0
+    return addr("sq_cast_c"), addr("sq_overlap_c")
 
 
 def compile_cpu(source: str, dim: int, pf: int, workdir: str):

@@ -12,7 +12,11 @@ the nearest-hit walk, against the CPU oracle running the same source.
     crossings, with no crossings function, equal its all-hits); the shells of
     crossings.rs and the marched implicit blobs of field.rs give the oracle's
     consumer crossings (same source built by g++) bit for bit, CSR offsets,
-    leaf order and toi, and the blobs' nearest hits equal the oracle's.
+    leaf order and toi, and the blobs' nearest hits equal the oracle's;
+  * shape casts and overlap through the consumer (sq_narrow_cast /
+    sq_narrow_overlap): the ball consumer equals the built-in SPHERE kind and
+    the oracle bit for bit; a source without them finds no contact and
+    overlaps at the box level, as the oracle.
 """
 import numpy as np
 import pytest
@@ -21,7 +25,8 @@ import oracle
 import sq_amd
 from sq_amd import workloads as W
 
-from consumer_narrow import BALL, BLOB, ELLIPSOID, SHELL, compile_cpu, cpu_entry_points
+from consumer_narrow import (BALL, BALL_FULL, BLOB, ELLIPSOID, SHELL, cast_overlap_entry_points, compile_cpu,
+                             cpu_entry_points)
 from helpers import assert_nearest_equal
 
 pytestmark = pytest.mark.gpu
@@ -202,3 +207,53 @@ def test_consumer_blob_crossings_match_oracle(ctx, tmp_path):
     assert_nearest_equal(host(near["leaf"]), host(near["toi"]), rn["leaf"], rn["t"], "blob nearest")
     cb.close()
     nw.close()
+
+
+def test_consumer_cast_and_overlap(ctx, tmp_path):
+    print("case: consumer shape casts and overlap", flush=True)
+    b = balls(30000, 3, 0xE0, 60.0)
+    boxes = np.concatenate([b[:, :3] - b[:, 3:], b[:, :3] + b[:, 3:]], 1).astype(np.float32)
+    rays = W.random_rays(20000, 0xE1, -65.0, 65.0)
+    casts = np.concatenate([rays, np.full((20000, 3), -0.75, np.float32), np.full((20000, 3), 0.75, np.float32)], 1)
+    s = W.Stream(0xE2)
+    c = s.range(-60.0, 60.0, 20000 * 3).reshape(-1, 3)
+    h = s.range(0.5, 6.0, 20000 * 3).reshape(-1, 3)
+    regions = np.concatenate([c - h, c + h], 1).astype(np.float32)
+    nw = sq_amd.Narrow(ctx, 3, BALL_FULL, 4)
+    cb = nw.build(boxes, b)
+    builtin = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.SPHERE, b)
+    got = cb.shapecast_nearest(dev(casts), 300.0)
+    want = builtin.shapecast_nearest(dev(casts), 300.0)
+    assert (host(want["leaf"]) >= 0).sum() > 2000
+    assert np.array_equal(host(got["leaf"]), host(want["leaf"]))
+    assert np.array_equal(host(got["toi"]).view(np.uint32), host(want["toi"]).view(np.uint32))
+    ga = csr_host(*cb.shapecast_all(dev(casts), 300.0))
+    wa = csr_host(*builtin.shapecast_all(dev(casts), 300.0))
+    assert wa[1].shape[0] > 3000
+    assert_csr_equal(ga, wa, "consumer cast all vs SPHERE")
+    go = cb.overlap(dev(regions))
+    wo = builtin.overlap(dev(regions))
+    assert np.array_equal(host(go[0]), host(wo[0])) and np.array_equal(host(go[1]), host(wo[1]))
+    assert host(wo[1]).shape[0] > 3000
+    # the oracle's consumer kind running the same source
+    lib, addr, xaddr = cpu_entry_points(BALL_FULL, 3, 4, str(tmp_path))
+    caddr, oaddr = cast_overlap_entry_points(lib)
+    sc = oracle.Scene.consumer(3, boxes, b, addr, xaddr, caddr, oaddr)
+    rn = sc.shapecast_nearest(casts, 300.0)
+    assert_nearest_equal(host(got["leaf"]), host(got["toi"]), rn["leaf"], rn["t"], "consumer cast vs oracle")
+    ro = sc.overlap(regions)
+    assert np.array_equal(host(go[0]).astype(np.uint64), ro[0].astype(np.uint64))
+    assert np.array_equal(host(go[1]).astype(np.int64), ro[1])
+    cb.close()
+    nw.close()
+    # without sq_narrow_cast / sq_narrow_overlap: no contact, box-level overlap
+    nw2 = sq_amd.Narrow(ctx, 3, BALL, 4)
+    cb2 = nw2.build(boxes, b)
+    miss = cb2.shapecast_nearest(dev(casts), 300.0)
+    assert (host(miss["leaf"]) == -1).all()
+    assert int(host(cb2.shapecast_all(dev(casts), 300.0)[0])[-1]) == 0
+    bo = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.AABB, boxes).overlap(dev(regions))
+    po = cb2.overlap(dev(regions))
+    assert np.array_equal(host(po[0]), host(bo[0])) and np.array_equal(host(po[1]), host(bo[1]))
+    cb2.close()
+    nw2.close()

@@ -10,14 +10,19 @@ from the same source the engine compiles with hipRTC), pinned on the CPU:
     (crossings.rs:113-135) and give the built-in SHELL kind's crossings except
     the few the box guard drops; the implicit blobs of field.rs converge to
     the analytic sphere at amp 0 (field.rs:178-214) and their BVH crossings
-    and nearest hits equal the naive scan (field.rs:160-176).
+    and nearest hits equal the naive scan (field.rs:160-176);
+  * consumer shape casts and overlap (test_shape_cast / test_overlap through
+    C functions): the ball written as a consumer casts and overlaps like the
+    built-in SPHERE kind bit for bit; without them a cast finds nothing and
+    overlap is the box test.
 """
 import numpy as np
 
 import oracle
 from sq_amd import workloads as W
 
-from consumer_narrow import BALL, BLOB, ELLIPSOID, SHELL, compile_cpu, cpu_entry_points
+from consumer_narrow import (BALL, BALL_FULL, BLOB, ELLIPSOID, SHELL, cast_overlap_entry_points, compile_cpu,
+                             cpu_entry_points)
 
 
 def balls(n, dim, seed, half=40.0):
@@ -162,3 +167,40 @@ def test_consumer_blob_bvh_equals_naive(tmp_path):
         assert ca == cb
         assert sum(len(x[0]) for x in ca) > 40
     del lib
+
+
+def ball_casts(n, seed, half, probe=0.75):
+    rays = W.random_rays(n, seed, -half - 5.0, half + 5.0)
+    return np.concatenate([rays, np.full((n, 3), -probe, np.float32), np.full((n, 3), probe, np.float32)], 1)
+
+
+def test_consumer_cast_and_overlap_equal_builtin_sphere(tmp_path):
+    lib, addr, xaddr = cpu_entry_points(BALL_FULL, 3, 4, str(tmp_path))
+    caddr, oaddr = cast_overlap_entry_points(lib)
+    assert caddr and oaddr
+    b = balls(3000, 3, 0xC7, 40.0)
+    cons = oracle.Scene.consumer(3, ball_boxes(b, 3), b, addr, xaddr, caddr, oaddr)
+    ref = oracle.Scene(3, oracle.SPHERE, b)
+    casts = ball_casts(3000, 0xC8, 40.0)
+    a, r = cons.shapecast_nearest(casts, 300.0), ref.shapecast_nearest(casts, 300.0)
+    assert (r["leaf"] >= 0).sum() > 300
+    assert np.array_equal(a["leaf"], r["leaf"]) and np.array_equal(a["t"].view(np.uint32), r["t"].view(np.uint32))
+    ao, al, at = cons.shapecast_all(casts[:500], 300.0)
+    ro, rl, rt = ref.shapecast_all(casts[:500], 300.0)
+    assert np.array_equal(ao, ro) and np.array_equal(al, rl) and np.array_equal(at.view(np.uint32), rt.view(np.uint32))
+    s = W.Stream(0xC9)
+    c = s.range(-40.0, 40.0, 2000 * 3).reshape(-1, 3)
+    h = s.range(0.5, 6.0, 2000 * 3).reshape(-1, 3)
+    reg = np.concatenate([c - h, c + h], 1).astype(np.float32)
+    co, cl = cons.overlap(reg)
+    so, sl = ref.overlap(reg)
+    assert len(sl) > 500 and np.array_equal(co, so) and np.array_equal(cl, sl)
+    # a source without them: no contact, and overlap is the box test
+    lib2, addr2 = compile_cpu(BALL, 3, 4, str(tmp_path))
+    plain = oracle.Scene.consumer(3, ball_boxes(b, 3), b, addr2)
+    assert (plain.shapecast_nearest(casts, 300.0)["leaf"] == -1).all()
+    boxes = oracle.Scene(3, oracle.BOX, ball_boxes(b, 3))
+    po, pl = plain.overlap(reg)
+    bo, bl = boxes.overlap(reg)
+    assert np.array_equal(po, bo) and np.array_equal(pl, bl) and len(bl) > len(sl)
+    del lib, lib2
