@@ -43,6 +43,7 @@
 #include <vector>
 
 #include <map>
+#include <mutex>
 #include <tuple>
 
 #include "engine_internal.h"
@@ -317,16 +318,21 @@ struct sq_narrow {
   hipFunction_t fn_cast_all = nullptr; // all shape-cast contacts
   hipFunction_t fn_overlap = nullptr;  // overlap
   // resident blocks per CU by (kernel, block, LDS), queried once each
+  std::mutex occ_mu;
   std::map<std::tuple<hipFunction_t, uint32_t, size_t>, int> occ;
 };
 
 // Resident blocks per CU of one of the module's kernels.
 static int module_blocks_per_cu(sq_narrow* nw, hipFunction_t fn, uint32_t block, size_t lds, int* out) {
   const auto key = std::make_tuple(fn, block, lds);
-  auto it = nw->occ.find(key);
-  if (it != nw->occ.end()) { *out = it->second; return SQ_OK; }
+  {
+    std::lock_guard<std::mutex> g(nw->occ_mu);
+    auto it = nw->occ.find(key);
+    if (it != nw->occ.end()) { *out = it->second; return SQ_OK; }
+  }
   int per_cu = 0;
   HIP_TRY(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)block, lds));
+  std::lock_guard<std::mutex> g(nw->occ_mu);
   nw->occ[key] = per_cu;
   *out = per_cu;
   return SQ_OK;
