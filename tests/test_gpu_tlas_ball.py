@@ -116,3 +116,22 @@ def test_box_instances_answer_no_cast_and_ball_batches(ctx):
     got = balls.shapecast_nearest(casts, 500.0)
     want = ref.shapecast_nearest(casts_of(rays, 0.5), 500.0)
     assert_nearest_equal(host(got["leaf"]), host(got["toi"]), want["leaf"], want["t"], "ball cast after refit")
+
+
+def test_ball_instance_errors_and_empty(ctx):
+    print("case: ball instance refusals and an empty TLAS", flush=True)
+    rot2 = np.tile(np.eye(2, dtype=np.float32), (4, 1, 1))
+    with pytest.raises(sq_amd.SqError):
+        sq_amd.Tlas(ctx, 2, sq_amd.Tlas.BALL, rot2, np.zeros((4, 2), np.float32), halves=np.ones(4, np.float32))
+    rot3 = np.tile(np.eye(3, dtype=np.float32), (4, 1, 1))
+    with pytest.raises(sq_amd.SqError):
+        sq_amd.Tlas(ctx, 3, sq_amd.Tlas.BALL, rot3, np.zeros((4, 3), np.float32),
+                    halves=np.array([1, -1, 1, 1], np.float32))
+    empty = sq_amd.Tlas(ctx, 3, sq_amd.Tlas.BALL, np.zeros((0, 3, 3), np.float32), np.zeros((0, 3), np.float32),
+                        halves=np.zeros(0, np.float32))
+    casts = dev(casts_of(W.random_rays(100, 0xBA60, -10.0, 10.0), 0.5))
+    got = empty.shapecast_nearest(casts, 100.0)
+    assert (host(got["leaf"]) == -1).all() and np.isinf(host(got["toi"])).all()
+    off, _, _ = empty.shapecast_all(casts, 100.0)
+    assert (host(off) == 0).all()
+    empty.close()
