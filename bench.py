@@ -41,6 +41,7 @@ def main():
     ap.add_argument("--lds-stack", type=int, default=0)
     ap.add_argument("--ray-order", type=int, default=-1, help="1: Morton ray order, 0: as given, -1: default")
     ap.add_argument("--descend-min", type=int, default=-1, help="pause descents below this many lanes (-1: default)")
+    ap.add_argument("--wide", type=int, default=-1, help="1: walk the 4-wide records, 0: the pair nodes, -1: default")
     ap.add_argument("--quick", action="store_true", help="short run for profiling: no CPU baseline, no side figures")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB device copy (the known-size kernel that calibrates HBM counters)")
@@ -76,6 +77,8 @@ def main():
         tun["ray_order"] = args.ray_order
     if args.descend_min >= 0:
         tun["descend_min"] = args.descend_min
+    if args.wide >= 0:
+        tun["wide"] = args.wide
     ctx.set_tuning(**tun)
     t0 = time.time()
     tree = sq_amd.Tree.build(3, sq_amd.TRIANGLE, sc.prims)

@@ -356,7 +356,11 @@ int sq_tlas_shapecast_all(sq_tlas* tlas, const float* casts, uint64_t n, float m
  *                  over the whole chip; 0 = automatic (overlap calls of fewer
  *                  than 2M queries split three quarters of the way down the
  *                  tree, others do not), -1 = never split, 1..24 = fixed.
- *                  Results do not depend on it. */
+ *                  Results do not depend on it.
+ *   wide           pooled nearest / any-hit calls without a filter: 1 walks
+ *                  4-wide records (each pair node with its children's pair
+ *                  nodes, rebuilt from the tree before the walk), half the
+ *                  dependent node fetches per ray; 0 = the pair nodes. */
 typedef struct sq_tuning {
   int block;
   int rays_per_lane;
@@ -364,6 +368,7 @@ typedef struct sq_tuning {
   int ray_order;
   int descend_min;
   int split_depth;
+  int wide;
 } sq_tuning;
 int sq_ctx_set_tuning(sq_ctx* ctx, const sq_tuning* tuning);
 int sq_ctx_get_tuning(const sq_ctx* ctx, sq_tuning* tuning);

@@ -66,8 +66,9 @@ __global__ __launch_bounds__(1024) void k_nearest(SceneView s, const float* __re
 // queries from counter[0] until all n are handed out, refilling its lanes
 // across chunk boundaries (pool_walk); the counters start zero and the launch
 // leaves them zero (pool_retire).  With `order`, pool
-// position i is query order[i] (the Morton order).
-template <int D, int KIND, bool ANY, bool FILT, bool CAST>
+// position i is query order[i] (the Morton order).  QUAD walks the 4-wide
+// records s.quads (k_quads) instead of the pair nodes.
+template <int D, int KIND, bool ANY, bool FILT, bool CAST, bool QUAD = false>
 __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nearest_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                       const float* __restrict__ tmax, Filter f, NearestOut out,
                                                       uint32_t chunk, uint32_t* counter, uint32_t S, SpillEntry* spill,
@@ -79,8 +80,19 @@ __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nea
   RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
   const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
   const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
-  pool_walk<D, KIND, ANY, FILT, CAST>(s, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, f, out, st, order, pause_below);
+  pool_walk<D, KIND, ANY, FILT, CAST, Chunks, QUAD>(s, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, f, out, st, order,
+                                                  pause_below);
   pool_retire(counter);
+}
+
+// The 4-wide records of every pair node (quad_records), rebuilt from the pair
+// nodes before each QUAD walk so they follow refits and rebuilds.
+template <int D>
+__global__ void k_quads(const float* __restrict__ nodes, uint32_t n_nodes, float* __restrict__ quads) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_nodes) return;
+  constexpr int NF = (D + 1) * 4;
+  quad_records<D>(nodes, n_nodes, p, quads + (size_t)p * 2 * NF);
 }
 
 template <int D>
@@ -760,12 +772,14 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   if (t->ray_order != 0 && t->ray_order != 1) return fail(SQ_ERR_INVALID, "ray_order must be 0 or 1");
   if (t->descend_min < 0 || t->descend_min > 64) return fail(SQ_ERR_INVALID, "descend_min must be in [0, 64]");
   if (t->split_depth < -1 || t->split_depth > 24) return fail(SQ_ERR_INVALID, "split_depth must be in [-1, 24]");
+  if (t->wide != 0 && t->wide != 1) return fail(SQ_ERR_INVALID, "wide must be 0 or 1");
   c->block = block;
   c->rays_per_lane = t->rays_per_lane;
   c->lds_stack = S;
   c->ray_order = t->ray_order;
   c->descend_min = t->descend_min;
   c->split_depth = t->split_depth;
+  c->wide = t->wide;
   return SQ_OK;
 }
 
@@ -777,6 +791,7 @@ int sq_ctx_get_tuning(const sq_ctx* c, sq_tuning* t) {
   t->ray_order = c->ray_order;
   t->descend_min = c->descend_min;
   t->split_depth = c->split_depth;
+  t->wide = c->wide;
   return SQ_OK;
 }
 
@@ -1089,19 +1104,35 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
   const uint32_t per = pooled_per_lane(c, n);
   const uint64_t chunk = 64ull * per;
   const uint64_t chunks = (n + chunk - 1) / chunk;
-  const uint32_t spill_rows = stack_n > S ? stack_n - S : 0;
+  // 4-wide walk (nearest and any-hit without a filter): up to 3 pushes per
+  // record over half the levels
+  const bool quad = c->wide && !cast && !filt;
+  const uint32_t walk_stack = quad ? 3 * (stack_n / 2 + 1) + 1 : stack_n;
+  const uint32_t spill_rows = walk_stack > S ? walk_stack - S : 0;
   const size_t lds = (size_t)S * B * 8;
   int rc;
   if ((rc = c->counter.ensure(256))) return rc;
   rc = begin_timing(c);
   if (rc) return rc;
+  if (quad) {
+    const uint32_t nf = (uint32_t)(b->dim + 1) * 4;
+    const uint32_t n_nodes = (uint32_t)(b->nodes.bytes / ((size_t)nf * 4));
+    if ((rc = b->quads.ensure((size_t)std::max<uint32_t>(n_nodes, 1) * 2 * nf * 4))) return rc;
+    rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+      k_quads<D><<<grid_for(n_nodes, 256), 256, 0, c->stream>>>(b->nodes.as<float>(), n_nodes, b->quads.as<float>());
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    });
+    if (rc) return rc;
+    s.quads = b->quads.as<float>();
+  }
   const uint32_t* order = nullptr;
   if (c->ray_order && !cast && n >= 4096)
     if ((rc = ray_order(c, b->dim, q, nn, 2 * b->dim, b->root.as<float>(), &order))) return rc;
   uint32_t* counter = c->pool_ctr.as<uint32_t>();
   rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
-    auto launch = [&]<bool ANY_, bool FILT_, bool CAST_>() -> int {
-      const void* kern = reinterpret_cast<const void*>(&k_nearest_pool<D, K, ANY_, FILT_, CAST_>);
+    auto launch = [&]<bool ANY_, bool FILT_, bool CAST_, bool QUAD_ = false>() -> int {
+      const void* kern = reinterpret_cast<const void*>(&k_nearest_pool<D, K, ANY_, FILT_, CAST_, QUAD_>);
       int per_cu = 0;
       if (int r0 = blocks_per_cu(kern, c->device, B, lds, &per_cu)) return r0;
       const uint64_t wpb = (uint64_t)B / 64;
@@ -1109,12 +1140,13 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
       const uint64_t lanes = grid * (uint64_t)B;
       int r2 = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64);
       if (r2) return r2;
-      k_nearest_pool<D, K, ANY_, FILT_, CAST_><<<(uint32_t)grid, B, lds, c->stream>>>(
+      k_nearest_pool<D, K, ANY_, FILT_, CAST_, QUAD_><<<(uint32_t)grid, B, lds, c->stream>>>(
           s, q, nn, max_toi, tmax, fd, out, (uint32_t)chunk, counter, S, c->spill.as<SpillEntry>(), (uint32_t)lanes, order,
           (uint32_t)c->descend_min);
       HIP_TRY(hipGetLastError());
       return SQ_OK;
     };
+    if (quad) return any ? launch.template operator()<true, false, false, true>() : launch.template operator()<false, false, false, true>();
     if (cast) return filt ? launch.template operator()<false, true, true>() : launch.template operator()<false, false, true>();
     if (any) return filt ? launch.template operator()<true, true, false>() : launch.template operator()<true, false, false>();
     return filt ? launch.template operator()<false, true, false>() : launch.template operator()<false, false, false>();

@@ -69,6 +69,7 @@ struct sq_ctx {
   int ray_order = 0;               // 1: walk rays in Morton order (order.hip)
   int descend_min = 24;            // pooled walks: pause descents below this many lanes (pool.h)
   int split_depth = 0;             // pooled collect: split queries this many levels down (0 auto, -1 never)
+  int wide = 0;                    // pooled nearest / any: walk the 4-wide records (traverse.h quad_records)
   bool timed = false;
   // {next chunk, retired waves} of the pooled single-call launches: zero at
   // creation, and each launch leaves them zero again (pool_retire)
@@ -96,6 +97,7 @@ struct sq_bvh {
   sqh::RefTree tree;               // host copy of the reference tree (device builds: see host_tree_ok)
   sqh::Packed packed;
   DevBuf nodes, prims, ids, root, layers, levels;
+  DevBuf quads;                    // 4-wide records, rebuilt before each 4-wide walk
   sq_tree_stats stats{};
   // device builds (build.hip): the reference layout stays on the device, and
   // tree, packed.slot_of_ref / nodes and stats.quality are filled on first use

@@ -113,7 +113,7 @@ struct Chunks {
 // primitive records of PS floats, queries of QS floats).  KIND picks what the
 // optional point / normal / sub outputs mean (the built-in kinds); CAST marks
 // shape casts (no normal).
-template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Pol, int QS, int PS, class Src>
+template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Pol, int QS, int PS, class Src, bool QUAD = false>
 __device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
                                                const float* __restrict__ tmax, const Filter& f, const NearestOut& out,
                                                const RingStack& st, const uint32_t* __restrict__ order, uint32_t pause_below) {
@@ -163,7 +163,8 @@ __device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* 
     if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
     const uint32_t active = (uint32_t)__popcll(__ballot(rid != NO_LEAF));
     if (rid == NO_LEAF) continue;
-    if (!walk_step<D, PS, ANY, FILT>(s, pol, f, st, w, nullptr, nullptr, WavePause{pause_below, active})) continue;
+    if (!walk_step<D, PS, ANY, FILT, false, false, QUAD>(s, pol, f, st, w, nullptr, nullptr, WavePause{pause_below, active}))
+      continue;
     // finished: write the result of query rid
     const NearestResult& h = w.res;
     const bool hit = h.leaf != NO_LEAF;
@@ -189,14 +190,15 @@ __device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* 
   }
 }
 
-// The built-in kinds: rays (RayPolicy) or shape casts (CastPolicy).
-template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Src>
+// The built-in kinds: rays (RayPolicy) or shape casts (CastPolicy); QUAD
+// walks the 4-wide records (s.quads).
+template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Src, bool QUAD = false>
 __device__ __forceinline__ void pool_walk(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
                                           const float* __restrict__ tmax, const Filter& f, const NearestOut& out,
                                           const RingStack& st, const uint32_t* __restrict__ order, uint32_t pause_below) {
   using Pol = typename std::conditional<CAST, CastPolicy<D, KIND>, RayPolicy<D, KIND>>::type;
-  pool_walk_with<D, KIND, ANY, FILT, CAST, Pol, CAST ? 4 * D : 2 * D, Prim<D, KIND>::STRIDE>(s, q, src, max_toi, tmax, f, out,
-                                                                                           st, order, pause_below);
+  pool_walk_with<D, KIND, ANY, FILT, CAST, Pol, CAST ? 4 * D : 2 * D, Prim<D, KIND>::STRIDE, Src, QUAD>(
+      s, q, src, max_toi, tmax, f, out, st, order, pause_below);
 }
 
 // ---------------------------------------------------------------------------

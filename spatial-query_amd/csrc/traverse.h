@@ -439,7 +439,41 @@ struct SceneView {
   const float* prims;       // leaf-ordered records
   const uint32_t* ids;      // leaf-ordered original leaf ids
   const float* root;        // [2D] bounds, then the root ref's bits
+  const float* quads = nullptr;   // 4-wide records of the pair nodes (walks with QUAD), see quad_records
 };
+
+// 4-wide records for the QUAD walk.  Pair node p gets two pair-layout records
+// side by side; record s describes child s of p: the child's own pair node
+// when it is interior (so its two boxes and refs are p's grandchildren), or
+// the child's box and leaf ref with the second slot marked empty (word 4D+2 =
+// 1) when it is a leaf range.  Every box is a box of the pair layout, an exact
+// union of its primitives' boxes, so a walk over them keeps exactly the hits
+// of the pair walk (the order-independence argument above) while it fetches
+// half as many nodes one after another.  n_nodes bounds the refs read.
+template <int D>
+SQ_HD void quad_records(const float* nodes, uint32_t n_nodes, uint32_t p, float* out) {
+  constexpr int NF = (D + 1) * 4;
+  const float* nd = nodes + (size_t)p * NF;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    float* rec = out + s * NF;
+    const uint32_t r = fbits(nd[4 * D + s]);
+    if (!(r & LEAF_BIT) && r < n_nodes) {
+      const float* cn = nodes + (size_t)r * NF;
+      for (int k = 0; k < NF; ++k) rec[k] = cn[k];
+      rec[4 * D + 2] = 0.0f;
+    } else {
+      for (int k = 0; k < D; ++k) {
+        rec[4 * k + 0] = nd[4 * k + 2 * s]; rec[4 * k + 1] = nd[4 * k + 2 * s + 1];
+        rec[4 * k + 2] = nd[4 * k + 2 * s]; rec[4 * k + 3] = nd[4 * k + 2 * s + 1];
+      }
+      rec[4 * D + 0] = nd[4 * D + s];
+      rec[4 * D + 1] = nd[4 * D + s];
+      rec[4 * D + 2] = bitsf(1u);
+      rec[4 * D + 3] = 0.0f;
+    }
+  }
+}
 
 // Stack accessor: entry k of this lane lives at ref[k * stride] / t[k * stride].
 struct Stack {
@@ -710,10 +744,55 @@ struct NoPause {
 // COUNT adds the step's work to *wc.  SKIP treats the child refs skip[0..3] as
 // missed: subtrees this query has already walked completely (with a limit at
 // least the current one, so they hold nothing it has not seen).
-template <int D, int PS, bool ANY, bool FILT, bool COUNT = false, bool SKIP = false, class Q, class P = NoPause>
+template <int D, int PS, bool ANY, bool FILT, bool COUNT = false, bool SKIP = false, bool QUAD = false, class Q,
+          class P = NoPause>
 SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const RingStack& st, WalkState& w, WorkCount* wc = nullptr,
                      const uint32_t* skip = nullptr, const P& pause = P()) {
   constexpr int NF = (D + 1) * 4;
+  if constexpr (QUAD) {
+    // descend over the 4-wide records: the nearest child next, the other hit
+    // children pushed farthest first (so they pop nearest first)
+    while (!(w.cur & LEAF_BIT)) {
+      float nd[2 * NF];
+      fetch16<2 * (D + 1)>(s.quads + (size_t)w.cur * 2 * NF, nd);
+      bool h[4];
+      float t[4];
+      uint32_t r[4];
+      q.pair(nd, w.limit, h[0], h[1], t[0], t[1]);
+      q.pair(nd + NF, w.limit, h[2], h[3], t[2], t[3]);
+      r[0] = fbits(nd[4 * D + 0]); r[1] = fbits(nd[4 * D + 1]);
+      r[2] = fbits(nd[NF + 4 * D + 0]); r[3] = fbits(nd[NF + 4 * D + 1]);
+      h[1] = h[1] && fbits(nd[4 * D + 2]) == 0u;
+      h[3] = h[3] && fbits(nd[NF + 4 * D + 2]) == 0u;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) t[k] = h[k] ? t[k] : SQ_INF;
+      const uint32_t nh = (uint32_t)h[0] + (uint32_t)h[1] + (uint32_t)h[2] + (uint32_t)h[3];
+      // sort the four (t, ref) by t: misses (+inf) go last
+      auto cswap = [&](int a, int b) {
+        const bool sw = t[b] < t[a];
+        const float ta = t[a], tb = t[b];
+        const uint32_t ra = r[a], rb = r[b];
+        t[a] = sw ? tb : ta; t[b] = sw ? ta : tb;
+        r[a] = sw ? rb : ra; r[b] = sw ? ra : rb;
+      };
+      cswap(0, 1); cswap(2, 3); cswap(0, 2); cswap(1, 3); cswap(1, 2);
+      if (nh == 0) {
+        bool got = false;
+        while (w.sp > 0) {
+          uint32_t rr; float tt;
+          st.pop(w.sp, rr, tt);
+          if (tt <= w.limit) { w.cur = rr; got = true; break; }
+        }
+        if (!got) return true;
+      } else {
+        if (nh > 3) st.push(w.sp, r[3], t[3]);
+        if (nh > 2) st.push(w.sp, r[2], t[2]);
+        if (nh > 1) st.push(w.sp, r[1], t[1]);
+        w.cur = r[0];
+      }
+      if (pause()) return false;
+    }
+  }
   while (!(w.cur & LEAF_BIT)) {
     float nd[NF];
     fetch16<D + 1>(s.nodes + (size_t)w.cur * NF, nd);

@@ -23,15 +23,24 @@ struct Built {
   sqh::RefTree tree;
   sqh::Packed packed;
   std::vector<float> prims;
+  std::vector<float> quads;     // 4-wide records (quad_records), built on first use
   SceneView view() const {
     SceneView s;
     s.nodes = packed.nodes.data();
     s.prims = prims.data();
     s.ids = tree.prim_indices.data();
     s.root = packed.root.data();
+    s.quads = quads.empty() ? nullptr : quads.data();
     return s;
   }
 };
+
+template <int D> void build_quads(Built& b) {
+  constexpr int NF = (D + 1) * 4;
+  const uint32_t n = (uint32_t)(b.packed.nodes.size() / NF);
+  b.quads.assign((size_t)std::max<uint32_t>(n, 1) * 2 * NF, 0.0f);
+  for (uint32_t p = 0; p < n; ++p) quad_records<D>(b.packed.nodes.data(), n, p, b.quads.data() + (size_t)p * 2 * NF);
+}
 
 void build(int dim, int kind, const float* prims, uint32_t n, Built& b) {
   std::vector<float> boxes;
@@ -52,21 +61,24 @@ template <class F> void par_for(uint64_t n, int threads, F&& f) {
 
 // The pooled kernel's per-lane walk: walk_begin + walk_step with a ring stack
 // of `ring` LDS slots (here a local array) spilling to a bounds-checked column.
-template <int D, int K, class Pol>
+template <int D, int K, bool QUAD = false, class Pol>
 NearestResult ring_walk(const Built& b, const Pol& pol, float max_toi, int any, uint32_t ring) {
   constexpr int PS = Prim<D, K>::STRIDE;
   const SceneView s = b.view();
   Filter f{nullptr, ~0ull, -1};
   std::vector<uint32_t> lref(ring);
   std::vector<float> lt(ring);
-  const uint32_t rows = b.packed.max_stack > ring ? b.packed.max_stack - ring : 0;
+  // the engine's stack bound: pair walk max_stack; 4-wide walk 3 (max_stack / 2 + 1) + 1
+  const uint32_t bound = QUAD ? 3 * (b.packed.max_stack / 2 + 1) + 1 : b.packed.max_stack;
+  const uint32_t rows = bound > ring ? bound - ring : 0;
   std::vector<SpillEntry> col(rows + 1);
   RingStack st{lref.data(), lt.data(), 1, ring, col.data(), 1};
   WalkState w;
   if (!walk_begin<D>(s, pol, max_toi, w)) return w.res;
   for (;;) {
-    const bool done = any ? walk_step<D, PS, true, false>(s, pol, f, st, w) : walk_step<D, PS, false, false>(s, pol, f, st, w);
-    if (w.sp > b.packed.max_stack) { fprintf(stderr, "stack bound exceeded\n"); abort(); }
+    const bool done = any ? walk_step<D, PS, true, false, false, false, QUAD>(s, pol, f, st, w)
+                          : walk_step<D, PS, false, false, false, false, QUAD>(s, pol, f, st, w);
+    if (w.sp > bound) { fprintf(stderr, "stack bound exceeded\n"); abort(); }
     if (done) return w.res;
   }
 }
@@ -101,6 +113,19 @@ void nearest_dk(const Built& b, const float* q, uint64_t nq, float max_toi, int 
       if (any) h = walk_nearest<D, PS, true, false>(s, pol, max_toi, f, st);
       else h = walk_nearest<D, PS, false, false>(s, pol, max_toi, f, st);
     }
+    leaf[i] = h.leaf == NO_LEAF ? -1 : (int32_t)h.leaf;
+    toi[i] = h.t;
+  });
+}
+
+// The 4-wide walk (walk_step with QUAD) of every ray, ring stack as above.
+template <int D, int K>
+void nearest_quad_dk(Built& b, const float* q, uint64_t nq, float max_toi, int any, int threads, int32_t* leaf, float* toi,
+                     uint32_t ring) {
+  if (b.quads.empty()) build_quads<D>(b);
+  par_for(nq, threads, [&](uint64_t i) {
+    RayPolicy<D, K> pol; pol.setup(q + i * 2 * D);
+    NearestResult h = ring_walk<D, K, true>(b, pol, max_toi, any, ring);
     leaf[i] = h.leaf == NO_LEAF ? -1 : (int32_t)h.leaf;
     toi[i] = h.t;
   });
@@ -148,6 +173,14 @@ int emu_nearest(void* h, int dim, int kind, const float* q, uint64_t nq, float m
   const Built& b = *(Built*)h;
   if (b.tree.prim_indices.empty()) { for (uint64_t i = 0; i < nq; ++i) { leaf[i] = -1; toi[i] = SQ_INF; } return 0; }
   return dk(dim, kind, [&]<int D, int K>() { nearest_dk<D, K>(b, q, nq, max_toi, any, cast, threads, leaf, toi, ring); });
+}
+
+// The 4-wide walk (tuning `wide`) with a ring stack of `ring` slots.
+int emu_nearest_quad(void* h, int dim, int kind, const float* q, uint64_t nq, float max_toi, int any, int threads,
+                     int32_t* leaf, float* toi, uint32_t ring) {
+  Built& b = *(Built*)h;
+  if (b.tree.prim_indices.empty()) { for (uint64_t i = 0; i < nq; ++i) { leaf[i] = -1; toi[i] = SQ_INF; } return 0; }
+  return dk(dim, kind, [&]<int D, int K>() { nearest_quad_dk<D, K>(b, q, nq, max_toi, any, threads, leaf, toi, ring); });
 }
 
 // mode 0 all, 1 crossings, 2 overlap, 3 shapecast-all.  Returns the total;

@@ -28,6 +28,7 @@ def emu():
     L.emu_collect.restype = C.c_uint64
     L.emu_collect.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, P, P]
     L.emu_required.argtypes = [P, C.c_int, P, C.c_uint64, P, P, P]
+    L.emu_nearest_quad.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_int, P, P, C.c_uint32]
     return L
 
 
@@ -45,6 +46,12 @@ class Emu:
         q = np.ascontiguousarray(q, np.float32)
         leaf = np.empty(len(q), np.int32); t = np.empty(len(q), np.float32)
         self.L.emu_nearest(self.h, self.dim, self.kind, p(q), len(q), max_toi, any_, cast, 8, p(leaf), p(t), ring)
+        return leaf.astype(np.int64), t
+
+    def nearest_quad(self, q, max_toi, any_=0, ring=8):
+        q = np.ascontiguousarray(q, np.float32)
+        leaf = np.empty(len(q), np.int32); t = np.empty(len(q), np.float32)
+        self.L.emu_nearest_quad(self.h, self.dim, self.kind, p(q), len(q), max_toi, any_, 8, p(leaf), p(t), ring)
         return leaf.astype(np.int64), t
 
     def collect(self, q, max_toi, mode):
@@ -153,3 +160,37 @@ def test_ring_stack_walk(emu, ring):
     casts = np.concatenate([q, np.full((len(q), 6), -0.5, np.float32), np.full((len(q), 6), 0.5, np.float32)], 1)
     rc = o6.shapecast_nearest(casts, 50.0)
     assert eq(*e6.nearest(casts, 50.0, cast=1, ring=ring), rc["leaf"], rc["t"])
+
+
+@pytest.mark.parametrize("ring", [2, 8])
+def test_quad_walk(emu, ring):
+    """The 4-wide walk (tuning `wide`: each pair node with its children's pair
+    nodes) equals the oracle: heightfield triangles with coherent and
+    incoherent rays, coincident boxes (leaf chains), balls in D = 2 and 5 and
+    boxes in D = 6, nearest and any-hit, with rings that spill."""
+    sc = W.heightfield_scene(grid=257, extent=128.0)
+    rays = np.concatenate([W.camera_rays(sc, res=128, seed=0x12), W.incoherent_rays(sc, 30000)])
+    ref = oracle.Scene(3, oracle.TRI, sc.prims).raycast_nearest(rays, 2000.0, threads=8)
+    e = Emu(emu, 3, 2, sc.prims)
+    assert eq(*e.nearest_quad(rays, 2000.0, ring=ring), ref["leaf"], ref["t"])
+    assert np.array_equal(e.nearest_quad(rays, 2000.0, any_=1, ring=ring)[0] >= 0, ref["leaf"] >= 0)
+    co = np.concatenate([np.tile(np.array([[1.0, 2.0, 3.0, 1.5, 2.5, 3.5]], np.float32), (1000, 1)),
+                         W.boxes_scene(n=500, seed=5).prims])
+    q = W.random_rays(5000, 0x13, -120.0, 120.0)
+    rc = oracle.Scene(3, oracle.BOX, co).raycast_nearest(q, 500.0)
+    assert eq(*Emu(emu, 3, 0, co).nearest_quad(q, 500.0, ring=ring), rc["leaf"], rc["t"])
+    for dim, half in ((2, 40.0), (5, 10.0)):
+        s = W.Stream(0x14 + dim)
+        b = np.concatenate([s.range(-half, half, 4000 * dim).reshape(-1, dim), s.range(0.3, 3.0, 4000).reshape(-1, 1)],
+                           1).astype(np.float32)
+        qq = W.random_rays(4000, 0x15 + dim, -half - 5.0, half + 5.0, dim=dim)
+        rb = oracle.Scene(dim, oracle.SPHERE, b).raycast_nearest(qq, 300.0)
+        assert (rb["leaf"] >= 0).sum() > 100
+        assert eq(*Emu(emu, dim, 1, b).nearest_quad(qq, 300.0, ring=ring), rb["leaf"], rb["t"])
+    sc6 = W.boxes_scene(n=3000, dim=6, lo=-7.0, hi=7.0, hmin=0.3, hmax=2.0)
+    q6 = W.random_rays(3000, 9, -17.0, 17.0, dim=6)
+    r6 = oracle.Scene(6, oracle.BOX, sc6.prims).raycast_nearest(q6, 500.0)
+    assert eq(*Emu(emu, 6, 0, sc6.prims).nearest_quad(q6, 500.0, ring=ring), r6["leaf"], r6["t"])
+    one = Emu(emu, 3, 0, W.boxes_scene(n=1, seed=1).prims)
+    r1 = oracle.Scene(3, oracle.BOX, W.boxes_scene(n=1, seed=1).prims).raycast_nearest(q, 500.0)
+    assert eq(*one.nearest_quad(q, 500.0, ring=ring), r1["leaf"], r1["t"])

@@ -174,3 +174,31 @@ def test_batch_set_survives_rebuild_and_refuses_moved_scene(ctx):
     with pytest.raises(sq_amd.SqError):
         bs2.submit()
     bs2.close()
+
+
+def test_wide_walk_over_device_and_rebuilt_trees(ctx):
+    """The 4-wide walk (tuning `wide`) over a device-built tree, before and
+    after a rebuild and a refit, equals the pair walk over the host build."""
+    print("case: 4-wide walk over device-built trees", flush=True)
+    base = W.boxes_scene(n=30000, seed=0xC1).prims
+    db = sq_amd.DeviceBvh.build_device(ctx, 3, sq_amd.AABB, dev(base))
+    rays = dev(W.random_rays(20000, 0xC2, -150.0, 150.0))
+    saved = ctx.tuning()
+    try:
+        for step in range(3):
+            prims = base + np.float32(step * 2.5)
+            if step == 1:
+                db.rebuild(dev(prims))
+            elif step == 2:
+                db.refit(dev(prims))
+            ref = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.AABB, prims)
+            ctx.set_tuning(**dict(saved, wide=0))
+            want = ref.raycast_nearest(rays, 400.0)
+            ctx.set_tuning(**dict(saved, wide=1))
+            got = db.raycast_nearest(rays, 400.0)
+            anyh = db.raycast_any(rays, 400.0)
+            assert np.array_equal(host(got["leaf"]), host(want["leaf"])), f"step {step}"
+            assert np.array_equal(host(got["toi"]).view(np.uint32), host(want["toi"]).view(np.uint32)), f"step {step}"
+            assert np.array_equal(host(anyh).astype(bool), host(want["leaf"]) >= 0)
+    finally:
+        ctx.set_tuning(**saved)

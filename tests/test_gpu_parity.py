@@ -293,7 +293,9 @@ def test_edge_cases(ctx):
                                     dict(block=64, rays_per_lane=1, lds_stack=2), dict(block=128, rays_per_lane=8, lds_stack=4),
                                     dict(block=256, rays_per_lane=64, lds_stack=32),
                                     dict(block=128, rays_per_lane=8, lds_stack=8, ray_order=1),
-                                    dict(block=64, rays_per_lane=2, lds_stack=4, ray_order=1)])
+                                    dict(block=64, rays_per_lane=2, lds_stack=4, ray_order=1),
+                                    dict(block=128, rays_per_lane=16, lds_stack=8, wide=1),
+                                    dict(block=64, rays_per_lane=1, lds_stack=2, wide=1)])
 def test_tuning_does_not_change_results(ctx, tuning):
     """Every kernel variant (one query per thread; pooled with LDS rings that
     spill constantly or never; rays walked in Morton order) gives the same bits."""

@@ -43,7 +43,7 @@ def test_calls_of_every_size_leave_the_counters_clear(ctx):
         want = [(bvh.raycast_nearest(r, 900.0), tlas.raycast_nearest(r, 900.0), bvh.raycast_any(r, 900.0)) for r in rays]
         bs = sq_amd.BatchSet(ctx, [(bvh, r, 900.0) for r in rays[:4]])
         for tune in (dict(block=64, rays_per_lane=1, lds_stack=8), dict(block=128, rays_per_lane=16, lds_stack=8),
-                     dict(block=256, rays_per_lane=4, lds_stack=4)):
+                     dict(block=256, rays_per_lane=4, lds_stack=4), dict(block=128, rays_per_lane=16, wide=1)):
             ctx.set_tuning(**tune)
             for rep in range(2):
                 for i, r in enumerate(rays):
