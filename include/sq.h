@@ -24,7 +24,8 @@
  *   - A miss writes leaf -1 and toi +inf (the reference's `None`).
  *   - Leaf ids are the provider's leaf indices 0..n-1 (QueryGeometry::id is
  *     the leaf index for the built-in kinds).
- *   - All arithmetic is FP32 with the reference's operation order; results are
+ *   - All arithmetic is FP32 with the reference's operation order (FP64 for
+ *     the sq_bvh64 entries, the reference's `f64` feature); results are
  *     bit-identical to the reference CPU path for finite inputs.
  */
 #ifndef SQ_H
@@ -456,6 +457,40 @@ int sq_raycast_nearest_proposed(sq_proposal* p, const float* rays, uint64_t n, f
  * GEMM the training step runs (device fp32 inputs, rounded to bf16 first).
  * M, N multiples of 128, K of 32; splits > 1 uses the split-K reduction. */
 int sq_debug_gemm(sq_ctx* ctx, const float* A, const float* B, int M, int N, int K, int splits, float* C);
+
+/* ------------------------------------------------------------------------
+ * The `f64` feature (crates/spatial-query/src/maths/point.rs:14-18: Scalar is
+ * f64): a single-level tree over double-precision SQ_PRIM_AABB or
+ * SQ_PRIM_SPHERE primitives, D = 2..6, built with the reference median split
+ * in double (tree.rs:112-163) and walked with the same slab and leaf tests as
+ * the FP32 tree, in double.  Entries mirror their FP32 counterparts above:
+ * same argument meaning, filter, miss values and CSR form; every float
+ * argument is double here.  TRIANGLE primitives are refused (the reference's
+ * triangle test runs on glam's f32 Vec3, which the feature does not switch).
+ * ---------------------------------------------------------------------- */
+typedef struct sq_bvh64 sq_bvh64;
+/* Bvh::build (bvh.rs:31-37, tree.rs:39-49,112-163) over host double primitives: AABB [min D][max D],
+ * SPHERE [centre D][r]. */
+int sq_bvh64_create(sq_ctx* ctx, int dim, int kind, const double* prims, uint64_t n, sq_bvh64** out);
+int sq_bvh64_destroy(sq_bvh64* bvh);
+/* TreeStats (stats.rs:15-32); bytes counts the f64 node size, quality is the
+ * interior surface sum computed in double. */
+int sq_bvh64_stats(const sq_bvh64* bvh, sq_tree_stats* out);
+/* QueryFilter layers by original leaf id (host or device uint64[n]; NULL clears). */
+int sq_bvh64_set_layers(sq_bvh64* bvh, const uint64_t* layers);
+/* Bvh::raycast_nearest (bvh.rs:51-124): rays double[n][2D] (device);
+ * out_point / out_normal double[n][D] may be NULL. */
+int sq_bvh64_raycast_nearest(sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi, const sq_filter* filter,
+                             int32_t* out_leaf, double* out_toi, double* out_point, double* out_normal);
+/* raycast_nearest(..).is_some() per ray. */
+int sq_bvh64_raycast_any(sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi, const sq_filter* filter,
+                         uint8_t* out_hit);
+/* Bvh::raycast_all (bvh.rs:196-259), CSR as sq_raycast_all. */
+int sq_bvh64_raycast_all(sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi, const sq_filter* filter,
+                         uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total);
+/* Bvh::overlap (bvh.rs:418-456): regions double[n][2D], CSR as sq_overlap. */
+int sq_bvh64_overlap(sq_bvh64* bvh, const double* regions, uint64_t n, const sq_filter* filter, uint64_t* offsets,
+                     int32_t* out_leaf, uint64_t capacity, uint64_t* total);
 
 #ifdef __cplusplus
 }

@@ -75,6 +75,10 @@ def _declare(L):
         "orc_shapecast_nearest_batch": (None, [P, P, P, U64, P, U64, I64, I32, P, P, P, P]),
         "orc_shapecast_all_batch": (U64, [P, P, P, U64, P, U64, I64, I32, P, P, P, U64]),
         "orc_required_work": (None, [P, P, U64, P, P, P, I32]),
+        "orc64_raycast_nearest": (None, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, I32, P, P, P, P]),
+        "orc64_raycast_all": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, I32, P, P, P, U64]),
+        "orc64_overlap": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, I32, P, P, U64]),
+        "orc64_tree_stats": (None, [I32, I32, P, U32, P, P]),
         "orc_inst_new": (P, [I32, I32, U64, P, P, P, U32, P, P, P, P]),
         "orc_inst_set_transforms": (None, [P, P, P, P]),
         "orc_tlas_refit": (None, [P]),
@@ -292,6 +296,57 @@ class Scene:
 # --------------------------------------------------------------------------
 # Two-level scene (tlas.rs)
 # --------------------------------------------------------------------------
+
+class Scene64:
+    """The reference's `f64` feature (sq/src/maths/point.rs:14-18) over BOX or
+    SPHERE primitives in double: the naive scans of sq/src/query/naive.rs, which
+    the BVH answers exactly (sq/src/lib.rs:15-17), and the f64 tree's stats."""
+
+    def __init__(self, dim: int, kind: int, prims, layers=None):
+        self.dim, self.kind = dim, kind
+        self.prims = np.ascontiguousarray(prims, np.float64).reshape(-1, 2 * dim if kind == BOX else dim + 1)
+        self.n = self.prims.shape[0]
+        self.layers = None if layers is None else np.ascontiguousarray(layers, np.uint64)
+
+    def _q(self, q):
+        return np.ascontiguousarray(q, np.float64).reshape(-1, 2 * self.dim)
+
+    def raycast_nearest(self, rays, max_toi, mask=ALL_LAYERS, exclude=-1, threads=8):
+        r = self._q(rays)
+        n = r.shape[0]
+        leaf = np.empty(n, np.int64); t = np.empty(n); pt = np.empty((n, self.dim)); nm = np.empty((n, self.dim))
+        lib().orc64_raycast_nearest(self.dim, self.kind, _p(self.prims), self.n, _p(self.layers), mask, exclude, _p(r), n,
+                                    float(max_toi), threads, _p(leaf), _p(t), _p(pt), _p(nm))
+        return dict(leaf=leaf, t=t, point=pt, normal=nm)
+
+    def _collect(self, fn, q, has_t):
+        n = q.shape[0]
+        off = np.empty(n + 1, np.uint64)
+        tot = fn(off, None, None, 0)
+        leaf = np.empty(max(tot, 1), np.int64); t = np.empty(max(tot, 1)) if has_t else None
+        fn(off, leaf, t, tot)
+        return off, leaf[:tot], (t[:tot] if has_t else None)
+
+    def raycast_all(self, rays, max_toi, mask=ALL_LAYERS, exclude=-1, threads=8):
+        r = self._q(rays)
+        fn = lambda off, leaf, t, cap: lib().orc64_raycast_all(self.dim, self.kind, _p(self.prims), self.n, _p(self.layers),
+                                                              mask, exclude, _p(r), r.shape[0], float(max_toi), threads,
+                                                              _p(off), _p(leaf), _p(t), cap)
+        return self._collect(fn, r, True)
+
+    def overlap(self, regions, mask=ALL_LAYERS, exclude=-1, threads=8):
+        r = self._q(regions)
+        fn = lambda off, leaf, t, cap: lib().orc64_overlap(self.dim, self.kind, _p(self.prims), self.n, _p(self.layers),
+                                                          mask, exclude, _p(r), r.shape[0], threads, _p(off), _p(leaf), cap)
+        off, leaf, _ = self._collect(fn, r, False)
+        return off, leaf
+
+    def tree_stats(self):
+        out = np.zeros(5, np.uint64); q = np.zeros(1)
+        lib().orc64_tree_stats(self.dim, self.kind, _p(self.prims), self.n, _p(out), _p(q))
+        return dict(node_count=int(out[0]), leaf_count=int(out[1]), prim_count=int(out[2]), max_depth=int(out[3]),
+                    bytes=int(out[4]), quality=float(q[0]))
+
 
 class InstanceScene:
     def __init__(self, dim, kind, rot, trans, halves=None, mesh_tris=None, mesh_of=None, scale=None):

@@ -1,0 +1,382 @@
+// sq_oracle64.cpp -- CPU oracle of the reference's `f64` feature.
+//
+// TEST INFRASTRUCTURE ONLY (the same rules as sq_oracle.cpp: only `tests/` load
+// it, as the checker).  Under the feature the reference's Scalar is f64
+// (sq/src/maths/point.rs:14-18), so every operation below is the double
+// restatement of the f32 one, in the reference's order, with FP contraction
+// disabled.  It restates:
+//   * Aabb::ray_intersect          sq/src/maths/aabb.rs:118-148 (f64::EPSILON)
+//   * Aabb::intersects             sq/src/maths/aabb.rs:101-108
+//   * the naive scans              sq/src/query/naive.rs:34-62 (nearest),
+//                                  :65-109 (all hits, sorted by hit_ordering),
+//                                  :184-205 (overlap)
+//   * hit_ordering                 sq/src/query/hit.rs:77-84 (f64::total_cmp, then leaf)
+//   * tree::build and TreeStats    sq/src/accel/tree.rs:39-49,81-163
+// over the built-in BOX and SPHERE providers (box-guarded sphere, as the
+// engine's: the hit is kept only inside the slab interval of its own box).
+// BVH == naive scan is the reference's stated contract (sq/src/lib.rs:15-17),
+// so the naive scans define the expected query results.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace orc64 {
+
+static const double F64_EPSILON = 2.220446049250313e-16;  // f64::EPSILON
+static const uint32_t LEAF_SIZE = 4;                      // sq/src/accel/tree.rs:12
+
+struct Ray { const double* o; const double* d; };
+
+// aabb.rs:118-148
+template <int D>
+static bool ray_intersect(const double* mn, const double* mx, const Ray& r, double max_toi, double* tmin, double* tmax,
+                          int* axis_out) {
+  double t_min = 0.0, t_max = max_toi;
+  int axis = -1;
+  for (int i = 0; i < D; ++i) {
+    const double o = r.o[i], d = r.d[i];
+    if (std::fabs(d) < F64_EPSILON) {
+      if (o < mn[i] || o > mx[i]) return false;
+    } else {
+      const double inv = 1.0 / d;
+      double t1 = (mn[i] - o) * inv;
+      double t2 = (mx[i] - o) * inv;
+      if (t1 > t2) std::swap(t1, t2);
+      if (t1 > t_min) { t_min = t1; axis = i; }
+      if (t2 < t_max) t_max = t2;
+      if (t_min > t_max) return false;
+    }
+  }
+  *tmin = t_min; *tmax = t_max; *axis_out = axis;
+  return true;
+}
+
+// point.rs:60-66: accumulate from 0.0, left to right.
+template <int D> static double dot(const double* a, const double* b) {
+  double acc = 0.0;
+  for (int i = 0; i < D; ++i) acc = acc + a[i] * b[i];
+  return acc;
+}
+
+template <int D> struct Geom {
+  int kind;   // 0 BOX [min D][max D], 1 SPHERE [c D][r]
+  uint32_t n;
+  const double* prims;
+  const uint64_t* layers;
+  int stride() const { return kind == 0 ? 2 * D : D + 1; }
+  const double* p(size_t leaf) const { return prims + leaf * stride(); }
+  // QueryGeometry::world_aabb: the box, or Aabb::new(c - splat(r), c + splat(r))
+  void world_aabb(size_t leaf, double* mn, double* mx) const {
+    const double* q = p(leaf);
+    for (int i = 0; i < D; ++i) {
+      if (kind == 0) { mn[i] = q[i]; mx[i] = q[D + i]; }
+      else { mn[i] = q[i] - q[D]; mx[i] = q[i] + q[D]; }
+    }
+  }
+  bool accepts(size_t leaf, uint64_t mask, int64_t exclude) const {
+    if (exclude >= 0 && (int64_t)leaf == exclude) return false;
+    const uint64_t lay = layers ? layers[leaf] : ~0ull;
+    return (lay & mask) != 0;
+  }
+  // test_ray: BOX answers the slab entry with the entry face normal; SPHERE the
+  // ball (entry root, the exit root from inside), guarded by its own box.
+  bool test_ray(size_t leaf, const Ray& r, double max_toi, double* t_out, double* normal) const {
+    const double* q = p(leaf);
+    double mn[D], mx[D];
+    world_aabb(leaf, mn, mx);
+    double te, tx; int axis;
+    if (kind == 0) {
+      if (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &axis)) return false;
+      *t_out = te;
+      for (int i = 0; i < D; ++i) normal[i] = 0.0;
+      if (axis >= 0) normal[axis] = r.d[axis] > 0.0 ? -1.0 : 1.0;
+      return true;
+    }
+    double oc[D];
+    for (int i = 0; i < D; ++i) oc[i] = r.o[i] - q[i];
+    const double rad = q[D];
+    const double b = dot<D>(oc, r.d);
+    const double cc = dot<D>(oc, oc) - rad * rad;
+    const double disc = b * b - cc;
+    if (disc < 0.0) return false;
+    const double sq = std::sqrt(disc);
+    double t = -b - sq;
+    if (t < 0.0) t = -b + sq;
+    if (t < 0.0 || t > max_toi) return false;
+    if (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &axis)) return false;
+    if (!(te <= t && t <= tx)) return false;
+    *t_out = t;
+    // point.rs:82-89 normalize_or_zero(at(t) - c)
+    double v[D];
+    for (int i = 0; i < D; ++i) v[i] = (r.o[i] + r.d[i] * t) - q[i];
+    const double len = std::sqrt(dot<D>(v, v));
+    for (int i = 0; i < D; ++i) normal[i] = len > F64_EPSILON ? v[i] * (1.0 / len) : 0.0;
+    return true;
+  }
+  // test_overlap: the box test; the ball's nearest region point within r
+  bool test_overlap(size_t leaf, const double* rmn, const double* rmx) const {
+    const double* q = p(leaf);
+    if (kind == 0) {
+      for (int i = 0; i < D; ++i) if (q[i] > rmx[i] || q[D + i] < rmn[i]) return false;
+      return true;
+    }
+    double d2 = 0.0;
+    for (int i = 0; i < D; ++i) {
+      double v;
+      if (q[i] < rmn[i]) v = rmn[i] - q[i];
+      else if (q[i] > rmx[i]) v = q[i] - rmx[i];
+      else v = 0.0;
+      d2 = d2 + v * v;
+    }
+    return std::sqrt(d2) <= q[D];
+  }
+};
+
+static int64_t total_key(double f) {
+  int64_t i;
+  std::memcpy(&i, &f, 8);
+  return i ^ (int64_t)(((uint64_t)(i >> 63)) >> 1);
+}
+// hit.rs:77-84
+static bool hit_less(double ta, int64_t la, double tb, int64_t lb) {
+  const int64_t a = total_key(ta), b = total_key(tb);
+  return a < b || (a == b && la < lb);
+}
+
+// naive.rs:34-62
+template <int D>
+static bool naive_nearest(const Geom<D>& g, const Ray& r, double max_toi, uint64_t mask, int64_t exclude, int64_t* leaf_out,
+                          double* t_out, double* normal_out) {
+  bool have = false;
+  double limit = max_toi, best_t = 0.0;
+  int64_t best_leaf = -1;
+  double nrm[D];
+  for (size_t leaf = 0; leaf < g.n; ++leaf) {
+    if (!g.accepts(leaf, mask, exclude)) continue;
+    double mn[D], mx[D], te, tx; int ax;
+    g.world_aabb(leaf, mn, mx);
+    if (!ray_intersect<D>(mn, mx, r, limit, &te, &tx, &ax)) continue;   // the cheap reject
+    double t;
+    if (!g.test_ray(leaf, r, limit, &t, nrm)) continue;
+    if (have && !hit_less(t, (int64_t)leaf, best_t, best_leaf)) continue;
+    have = true; best_t = t; best_leaf = (int64_t)leaf; limit = t;
+    for (int i = 0; i < D; ++i) normal_out[i] = nrm[i];
+  }
+  *leaf_out = best_leaf; *t_out = best_t;
+  return have;
+}
+
+// naive.rs:65-109 (sorted nearest-first, then by leaf)
+template <int D>
+static void naive_all(const Geom<D>& g, const Ray& r, double max_toi, uint64_t mask, int64_t exclude,
+                      std::vector<std::pair<double, int64_t>>& out) {
+  out.clear();
+  double nrm[D];
+  for (size_t leaf = 0; leaf < g.n; ++leaf) {
+    if (!g.accepts(leaf, mask, exclude)) continue;
+    double mn[D], mx[D], te, tx; int ax;
+    g.world_aabb(leaf, mn, mx);
+    if (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &ax)) continue;
+    double t;
+    if (g.test_ray(leaf, r, max_toi, &t, nrm)) out.push_back({t, (int64_t)leaf});
+  }
+  std::stable_sort(out.begin(), out.end(),
+                   [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
+                     return hit_less(a.first, a.second, b.first, b.second);
+                   });
+}
+
+// naive.rs:184-205
+template <int D>
+static void naive_overlap(const Geom<D>& g, const double* rmn, const double* rmx, uint64_t mask, int64_t exclude,
+                          std::vector<int64_t>& out) {
+  out.clear();
+  for (size_t leaf = 0; leaf < g.n; ++leaf) {
+    if (!g.accepts(leaf, mask, exclude)) continue;
+    double mn[D], mx[D];
+    g.world_aabb(leaf, mn, mx);
+    bool inter = true;
+    for (int i = 0; i < D; ++i) if (mn[i] > rmx[i] || mx[i] < rmn[i]) inter = false;
+    if (!inter) continue;
+    if (g.test_overlap(leaf, rmn, rmx)) out.push_back((int64_t)leaf);
+  }
+}
+
+// tree.rs:112-163 in f64: union fold from EMPTY, leaf at <= 4, centroid bounds,
+// longest axis (first max wins), coincidence stop at f64::EPSILON, stable sort
+// by total_cmp of the centroid, split at count / 2.
+struct Node { std::vector<double> b; uint32_t a, c, pc; };
+
+template <int D>
+static uint32_t build(const std::vector<double>& boxes, std::vector<uint32_t>& idx, uint32_t start, uint32_t end,
+                      std::vector<Node>& nodes) {
+  Node nd;
+  nd.b.assign(2 * D, 0.0);
+  for (int i = 0; i < D; ++i) { nd.b[i] = INFINITY; nd.b[D + i] = -INFINITY; }
+  for (uint32_t k = start; k < end; ++k) {
+    const double* b = boxes.data() + (size_t)idx[k] * 2 * D;
+    for (int i = 0; i < D; ++i) { nd.b[i] = std::fmin(nd.b[i], b[i]); nd.b[D + i] = std::fmax(nd.b[D + i], b[D + i]); }
+  }
+  const uint32_t count = end - start;
+  nd.a = start; nd.c = 0; nd.pc = count;
+  const uint32_t me = (uint32_t)nodes.size();
+  nodes.push_back(nd);
+  if (count <= LEAF_SIZE) return me;
+  double cmin[D], cmax[D];
+  for (int i = 0; i < D; ++i) { cmin[i] = INFINITY; cmax[i] = -INFINITY; }
+  auto centre = [&](uint32_t prim, int i) {
+    const double* b = boxes.data() + (size_t)prim * 2 * D;
+    return (b[i] + b[D + i]) * 0.5;
+  };
+  for (uint32_t k = start; k < end; ++k)
+    for (int i = 0; i < D; ++i) { const double c = centre(idx[k], i); cmin[i] = std::fmin(cmin[i], c); cmax[i] = std::fmax(cmax[i], c); }
+  int axis = 0;
+  for (int i = 1; i < D; ++i) if (cmax[i] - cmin[i] > cmax[axis] - cmin[axis]) axis = i;
+  if (cmax[axis] - cmin[axis] <= F64_EPSILON) return me;
+  std::stable_sort(idx.begin() + start, idx.begin() + end,
+                   [&](uint32_t x, uint32_t y) { return total_key(centre(x, axis)) < total_key(centre(y, axis)); });
+  const uint32_t mid = start + count / 2;
+  const uint32_t a = build<D>(boxes, idx, start, mid, nodes);
+  const uint32_t c = build<D>(boxes, idx, mid, end, nodes);
+  nodes[me].a = a; nodes[me].c = c; nodes[me].pc = 0;
+  return me;
+}
+
+template <int D>
+static void tree_stats(const double* prims, int kind, uint32_t n, uint64_t* out5, double* quality) {
+  Geom<D> g{kind, n, prims, nullptr};
+  std::vector<double> boxes((size_t)n * 2 * D);
+  for (uint32_t i = 0; i < n; ++i) g.world_aabb(i, boxes.data() + (size_t)i * 2 * D, boxes.data() + (size_t)i * 2 * D + D);
+  std::vector<uint32_t> idx(n);
+  for (uint32_t i = 0; i < n; ++i) idx[i] = i;
+  std::vector<Node> nodes;
+  if (n) build<D>(boxes, idx, 0, n, nodes);
+  // tree.rs:81-110
+  uint64_t leaves = 0, pr = 0;
+  std::vector<uint64_t> depth(nodes.size(), 0);
+  double q = 0.0;
+  for (size_t i = nodes.size(); i-- > 0;) {
+    if (nodes[i].pc > 0) { ++leaves; pr += nodes[i].pc; depth[i] = 1; }
+    else {
+      depth[i] = 1 + std::max(depth[nodes[i].a], depth[nodes[i].c]);
+    }
+  }
+  for (size_t i = 0; i < nodes.size(); ++i) {   // interior half surfaces in node order (aabb.rs:84-97)
+    if (nodes[i].pc > 0) continue;
+    double sum = 0.0;
+    for (int a = 0; a < D; ++a) {
+      double prod = 1.0;
+      for (int k = 0; k < D; ++k) if (k != a) prod = prod * (nodes[i].b[D + k] - nodes[i].b[k]);
+      sum = sum + prod;
+    }
+    q = q + sum;
+  }
+  out5[0] = nodes.size(); out5[1] = leaves; out5[2] = pr; out5[3] = nodes.empty() ? 0 : depth[0];
+  out5[4] = nodes.size() * (uint64_t)(16 * D + 16) + (uint64_t)n * 4;
+  *quality = q;
+}
+
+template <class F> static void parallel_for(uint64_t n, int threads, F&& f) {
+  if (threads <= 1 || n < 256) { for (uint64_t i = 0; i < n; ++i) f(i); return; }
+  std::vector<std::thread> th;
+  const uint64_t chunk = (n + threads - 1) / threads;
+  for (int t = 0; t < threads; ++t) {
+    const uint64_t a = t * chunk, b = std::min<uint64_t>(n, a + chunk);
+    if (a >= b) break;
+    th.emplace_back([&, a, b] { for (uint64_t i = a; i < b; ++i) f(i); });
+  }
+  for (auto& t : th) t.join();
+}
+
+template <int D>
+static void nearest_batch(int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask, int64_t exclude,
+                          const double* rays, uint64_t nr, double max_toi, int threads, int64_t* leaf, double* t,
+                          double* point, double* normal) {
+  Geom<D> g{kind, n, prims, layers};
+  parallel_for(nr, threads, [&](uint64_t i) {
+    Ray r{rays + i * 2 * D, rays + i * 2 * D + D};
+    double nrm[D];
+    int64_t l; double tt;
+    const bool have = naive_nearest<D>(g, r, max_toi, mask, exclude, &l, &tt, nrm);
+    leaf[i] = have ? l : -1;
+    t[i] = have ? tt : INFINITY;
+    for (int k = 0; k < D; ++k) {
+      if (point) point[i * D + k] = have ? r.o[k] + r.d[k] * tt : 0.0;   // ray.rs:33-35
+      if (normal) normal[i * D + k] = have ? nrm[k] : 0.0;
+    }
+  });
+}
+
+template <int D>
+static uint64_t all_batch(int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask, int64_t exclude,
+                          const double* q, uint64_t nq, double max_toi, bool overlap, int threads, uint64_t* offsets,
+                          int64_t* leaf, double* t, uint64_t cap) {
+  Geom<D> g{kind, n, prims, layers};
+  std::vector<std::vector<std::pair<double, int64_t>>> res(nq);
+  parallel_for(nq, threads, [&](uint64_t i) {
+    if (overlap) {
+      std::vector<int64_t> ids;
+      naive_overlap<D>(g, q + i * 2 * D, q + i * 2 * D + D, mask, exclude, ids);
+      for (int64_t id : ids) res[i].push_back({0.0, id});
+    } else {
+      Ray r{q + i * 2 * D, q + i * 2 * D + D};
+      naive_all<D>(g, r, max_toi, mask, exclude, res[i]);
+    }
+  });
+  uint64_t tot = 0;
+  for (uint64_t i = 0; i < nq; ++i) { offsets[i] = tot; tot += res[i].size(); }
+  offsets[nq] = tot;
+  if (tot <= cap) {
+    uint64_t k = 0;
+    for (uint64_t i = 0; i < nq; ++i)
+      for (auto& h : res[i]) { leaf[k] = h.second; if (t) t[k] = h.first; ++k; }
+  }
+  return tot;
+}
+
+}  // namespace orc64
+
+using namespace orc64;
+
+#define ORC64_DISPATCH(D, CALL)                 \
+  switch (D) {                                 \
+    case 2: { constexpr int DD = 2; CALL; break; } \
+    case 3: { constexpr int DD = 3; CALL; break; } \
+    case 4: { constexpr int DD = 4; CALL; break; } \
+    case 5: { constexpr int DD = 5; CALL; break; } \
+    case 6: { constexpr int DD = 6; CALL; break; } \
+    default: break;                            \
+  }
+
+extern "C" {
+
+void orc64_raycast_nearest(int D, int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
+                           int64_t exclude, const double* rays, uint64_t nr, double max_toi, int threads, int64_t* leaf,
+                           double* t, double* point, double* normal) {
+  ORC64_DISPATCH(D, nearest_batch<DD>(kind, prims, n, layers, mask, exclude, rays, nr, max_toi, threads, leaf, t, point, normal));
+}
+
+uint64_t orc64_raycast_all(int D, int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
+                           int64_t exclude, const double* rays, uint64_t nr, double max_toi, int threads,
+                           uint64_t* offsets, int64_t* leaf, double* t, uint64_t cap) {
+  uint64_t tot = 0;
+  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, rays, nr, max_toi, false, threads, offsets, leaf, t, cap));
+  return tot;
+}
+
+uint64_t orc64_overlap(int D, int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
+                       int64_t exclude, const double* regions, uint64_t nq, int threads, uint64_t* offsets, int64_t* leaf,
+                       uint64_t cap) {
+  uint64_t tot = 0;
+  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, regions, nq, 0.0, true, threads, offsets, leaf, nullptr, cap));
+  return tot;
+}
+
+void orc64_tree_stats(int D, int kind, const double* prims, uint32_t n, uint64_t* out5, double* quality) {
+  ORC64_DISPATCH(D, tree_stats<DD>(prims, kind, n, out5, quality));
+}
+
+}  // extern "C"

@@ -596,6 +596,10 @@ static int scan_into(sq_ctx* c, const T* in, uint64_t n, uint64_t* out, std::vec
   return SQ_OK;
 }
 
+int scan_counts(sq_ctx* c, const uint32_t* in, uint64_t n, uint64_t* out, std::vector<std::unique_ptr<DevBuf>>& temps) {
+  return scan_into<uint32_t>(c, in, n, out, temps);
+}
+
 void ctx_release(sq_ctx* c) {
   if (--c->refs > 0) return;
   (void)hipSetDevice(c->device);

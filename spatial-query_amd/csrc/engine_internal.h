@@ -107,6 +107,10 @@ struct sq_bvh {
   uint32_t consumer_pf = 0;        // trees over consumer primitives (rtc.hip): floats per primitive
 };
 
+// Exclusive scan of n counts into out[0..n] (out[n] = total) on the context's
+// stream; temps keeps the scan's temporaries alive until the caller is done.
+int scan_counts(sq_ctx* c, const uint32_t* in, uint64_t n, uint64_t* out, std::vector<std::unique_ptr<DevBuf>>& temps);
+
 // TreeStats of a host reference tree (tree.rs:81-110, bvh.rs:477-489).
 void tree_stats_of(const sqh::RefTree& t, sq_tree_stats& st);
 

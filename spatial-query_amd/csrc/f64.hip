@@ -1,0 +1,464 @@
+// f64.hip -- the reference's `f64` feature on the device: a Bvh over
+// double-precision AABB and SPHERE leaves (D = 2..6) answering nearest-hit,
+// any-hit, all-hits and overlap queries.
+//
+// Under the feature the reference's Scalar is f64 everywhere
+// (crates/spatial-query/src/maths/point.rs:14-18): tree bounds, rays, the slab
+// (maths/aabb.rs:118-148), the leaf tests and the returned toi.  The walk
+// below runs the same slab and narrow-phase functions as the FP32 engine
+// (traverse.h), instantiated on double, over the reference median-split tree
+// built in double (host_tree.cpp).  Results are order-independent for the
+// reason the FP32 walk is (a sphere hit is kept only inside its own box's slab
+// interval), so a near-first walk answers what the reference's LIFO walk does.
+//
+// Device layout: the reference node array re-packed as pair nodes, both
+// children's boxes in one record,
+//   double box[4D] = a.min[D], a.max[D], b.min[D], b.max[D]
+//   uint4  ref     = {ref_a, ref_b, count_a, count_b}
+// where count 0 marks an interior child (ref = its pair index) and count > 0 a
+// leaf range [ref, ref + count) of the leaf-ordered primitives.  One lane per
+// query; the per-lane stack holds at most one entry per tree level.
+#include "engine_internal.h"
+
+namespace {
+
+constexpr int F64_MAX_STACK = 64;   // tree levels the per-lane stack holds (median split: about log2(n) + 1)
+
+struct View64 {
+  const double* boxes;      // [np][4D]
+  const uint4* refs;        // [np]
+  const double* prims;      // leaf-ordered, stride 2D (AABB) or D + 1 (SPHERE)
+  const uint32_t* ids;      // original leaf id of each leaf-ordered primitive
+  const uint64_t* layers;   // by original leaf id, or null
+  double root[12];          // root box: min[D], max[D]
+  uint32_t root_ref, root_count;   // root_count > 0: the root is a leaf
+  uint32_t n_prims;
+};
+
+__device__ __forceinline__ int64_t total_key64(double f) {
+  const int64_t i = __double_as_longlong(f);
+  return i ^ (int64_t)(((uint64_t)(i >> 63)) >> 1);
+}
+// (t, leaf) strictly before (bt, bl) under hit_ordering (query/hit.rs:77-84).
+__device__ __forceinline__ bool hit_before64(double t, uint32_t leaf, double bt, uint32_t bl) {
+  const int64_t a = total_key64(t), b = total_key64(bt);
+  return a < b || (a == b && leaf < bl);
+}
+
+template <int D, int KIND> constexpr int stride64() { return KIND == KIND_AABB ? 2 * D : D + 1; }
+
+// Walks every pair node whose child box passes node_pass, near child first when
+// both pass (entries from the pass test), and calls leaf(first, count) for
+// every passing leaf range.  prune(entry) skips a popped subtree whose entry
+// lower bound lies beyond the current limit.  Returns early when leaf() asks to stop.
+template <int D, class NodePass, class Leaf, class Prune>
+__device__ __forceinline__ void walk64(const View64& v, NodePass&& node_pass, Leaf&& leaf, Prune&& prune) {
+  if (v.n_prims == 0) return;
+  double te;
+  if (!node_pass(v.root, v.root + D, te)) return;
+  if (v.root_count) { leaf(v.root_ref, v.root_count); return; }
+  uint32_t stk[F64_MAX_STACK];
+  float stt[F64_MAX_STACK];
+  int sp = 0;
+  uint32_t node = v.root_ref;
+  while (true) {
+    const double* b = v.boxes + (size_t)node * 4 * D;
+    const uint4 rf = v.refs[node];
+    double ta = 0.0, tb = 0.0;
+    bool ha = node_pass(b, b + D, ta);
+    bool hb = node_pass(b + 2 * D, b + 3 * D, tb);
+    if (ha && rf.z) { if (leaf(rf.x, rf.z)) return; ha = false; }
+    if (hb && rf.w) { if (leaf(rf.y, rf.w)) return; hb = false; }
+    if (ha && hb) {
+      const bool a_first = ta <= tb;
+      // the entry stored rounded toward -inf, so the pop test never prunes a subtree the limit still reaches
+      stk[sp] = a_first ? rf.y : rf.x;
+      stt[sp] = __double2float_rd(a_first ? tb : ta);
+      ++sp;
+      node = a_first ? rf.x : rf.y;
+    } else if (ha) {
+      node = rf.x;
+    } else if (hb) {
+      node = rf.y;
+    } else {
+      bool found = false;
+      while (sp > 0) {
+        --sp;
+        if (!prune((double)stt[sp])) { node = stk[sp]; found = true; break; }
+      }
+      if (!found) return;
+    }
+  }
+}
+
+__device__ __forceinline__ bool accepts64(const View64& v, const Filter& f, uint32_t id) {
+  if (f.exclude >= 0 && (int64_t)id == f.exclude) return false;
+  const uint64_t lay = v.layers ? v.layers[id] : ~0ull;
+  return (lay & f.mask) != 0;
+}
+
+// Bvh::raycast_nearest (bvh.rs:51-124) and the any-hit idiom, one ray per lane.
+template <int D, int KIND, bool ANY>
+__global__ __launch_bounds__(256) void k64_nearest(View64 v, const double* __restrict__ rays, uint64_t n, double max_toi,
+                                                   Filter f, int32_t* out_leaf, double* out_toi, double* out_point,
+                                                   double* out_normal, uint8_t* out_any) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int PS = stride64<D, KIND>();
+  RayD<D, double> r;
+  ray_setup<D>(r, rays + i * 2 * D);
+  double best = max_toi;
+  uint32_t bl = NO_LEAF, bk = 0;
+  walk64<D>(
+      v,
+      [&](const double* mn, const double* mx, double& te) {
+        double tx; int ax;
+        return slab_exact<D>(r, mn, mx, best, te, tx, ax);
+      },
+      [&](uint32_t first, uint32_t count) {
+        for (uint32_t k = first; k < first + count; ++k) {
+          const uint32_t id = v.ids[k];
+          if (!accepts64(v, f, id)) continue;
+          double t;
+          if (!sqd::narrow<D, KIND>(r, v.prims + (size_t)k * PS, best, t)) continue;
+          if (!hit_before64(t, id, best, bl)) continue;
+          best = t; bl = id; bk = k;
+          if (ANY) return true;
+        }
+        return false;
+      },
+      [&](double entry_lb) { return entry_lb > best; });
+  if (ANY) { out_any[i] = bl != NO_LEAF; return; }
+  const bool hit = bl != NO_LEAF;
+  out_leaf[i] = hit ? (int32_t)bl : -1;
+  out_toi[i] = hit ? best : __builtin_huge_val();
+  double nrm[D];
+  if (hit && out_normal) hit_normal<D, KIND>(r, v.prims + (size_t)bk * PS, best, max_toi, nrm);
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    if (out_point) out_point[i * D + k] = hit ? r.o[k] + r.d[k] * best : 0.0;   // Ray::at (maths/ray.rs)
+    if (out_normal) out_normal[i * D + k] = hit ? nrm[k] : 0.0;
+  }
+}
+
+// Bvh::raycast_all (bvh.rs:196-259) and Bvh::overlap (bvh.rs:418-456): one
+// pass counts each query's results, the second writes them into its CSR range
+// and sorts the range (rays by (toi total order, leaf), overlap by leaf).
+template <int D, int KIND, bool OVERLAP>
+__global__ __launch_bounds__(256) void k64_collect(View64 v, const double* __restrict__ q, uint64_t n, double max_toi,
+                                                   Filter f, uint32_t* counts, const uint64_t* offsets,
+                                                   int32_t* out_leaf, double* out_toi) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int PS = stride64<D, KIND>();
+  const bool fill = offsets != nullptr;
+  uint64_t at = fill ? offsets[i] : 0;
+  uint32_t cnt = 0;
+  if constexpr (OVERLAP) {
+    const double* rmn = q + i * 2 * D;
+    const double* rmx = rmn + D;
+    walk64<D>(
+        v,
+        [&](const double* mn, const double* mx, double& te) { te = 0.0; return box_overlap<D>(mn, mx, rmn, rmx); },
+        [&](uint32_t first, uint32_t count) {
+          for (uint32_t k = first; k < first + count; ++k) {
+            const uint32_t id = v.ids[k];
+            if (!accepts64(v, f, id)) continue;
+            if (!overlap_narrow<D, KIND>(v.prims + (size_t)k * PS, rmn, rmx)) continue;
+            if (fill) out_leaf[at++] = (int32_t)id;
+            ++cnt;
+          }
+          return false;
+        },
+        [](double) { return false; });
+  } else {
+    RayD<D, double> r;
+    ray_setup<D>(r, q + i * 2 * D);
+    walk64<D>(
+        v,
+        [&](const double* mn, const double* mx, double& te) {
+          double tx; int ax;
+          return slab_exact<D>(r, mn, mx, max_toi, te, tx, ax);
+        },
+        [&](uint32_t first, uint32_t count) {
+          for (uint32_t k = first; k < first + count; ++k) {
+            const uint32_t id = v.ids[k];
+            if (!accepts64(v, f, id)) continue;
+            double t;
+            if (!sqd::narrow<D, KIND>(r, v.prims + (size_t)k * PS, max_toi, t)) continue;
+            if (fill) { out_leaf[at] = (int32_t)id; out_toi[at] = t; ++at; }
+            ++cnt;
+          }
+          return false;
+        },
+        [](double) { return false; });
+  }
+  if (!fill) { counts[i] = cnt; return; }
+  // insertion sort of this query's range (result lists are short)
+  const uint64_t b0 = offsets[i], b1 = offsets[i + 1];
+  for (uint64_t a = b0 + 1; a < b1; ++a) {
+    const int32_t l = out_leaf[a];
+    const double t = OVERLAP ? 0.0 : out_toi[a];
+    uint64_t j = a;
+    while (j > b0) {
+      const int32_t pl = out_leaf[j - 1];
+      const bool after = OVERLAP ? (pl > l) : hit_before64(t, (uint32_t)l, out_toi[j - 1], (uint32_t)pl);
+      if (!after) break;
+      out_leaf[j] = pl;
+      if (!OVERLAP) out_toi[j] = out_toi[j - 1];
+      --j;
+    }
+    out_leaf[j] = l;
+    if (!OVERLAP) out_toi[j] = t;
+  }
+}
+
+template <class F> int dispatch64(int dim, int kind, F&& f) {
+  if (kind != KIND_AABB && kind != KIND_SPHERE)
+    return fail(SQ_ERR_UNSUPPORTED, "f64 trees hold AABB or SPHERE primitives");
+#define SQ_CASE(DD)                                                       \
+  case DD:                                                                \
+    return kind == KIND_AABB ? f.template operator()<DD, KIND_AABB>()    \
+                             : f.template operator()<DD, KIND_SPHERE>();
+  switch (dim) {
+    SQ_CASE(2) SQ_CASE(3) SQ_CASE(4) SQ_CASE(5) SQ_CASE(6)
+    default: return fail(SQ_ERR_UNSUPPORTED, "dimension " + std::to_string(dim) + " not instantiated (2..6)");
+  }
+#undef SQ_CASE
+}
+
+}  // namespace
+
+struct sq_bvh64 {
+  sq_ctx* ctx = nullptr;
+  int dim = 0, kind = 0;
+  uint64_t n_prims = 0;
+  sqh::RefTreeT<double> tree;
+  DevBuf boxes, refs, prims, ids, layers;
+  View64 view{};
+  sq_tree_stats stats{};
+};
+
+namespace {
+
+Filter filter64(const sq_bvh64* b, const sq_filter* f) {
+  Filter fd;
+  fd.layers = b->layers.p ? b->layers.as<uint64_t>() : nullptr;
+  fd.mask = f ? f->layer_mask : ~0ull;
+  fd.exclude = f ? f->exclude : -1;
+  return fd;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sq_bvh64_create(sq_ctx* c, int dim, int kind, const double* prims, uint64_t n, sq_bvh64** out) {
+  if (!c || !out || (n && !prims)) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (n > sqh::LEAF_FIRST_MASK) return fail(SQ_ERR_INVALID, "too many primitives");
+  int rc = dispatch64(dim, kind, [&]<int D, int KIND>() -> int { return SQ_OK; });
+  if (rc) return rc;
+  const int ps = kind == KIND_AABB ? 2 * dim : dim + 1;
+  // QueryGeometry::world_aabb of the built-in kinds, in double
+  std::vector<double> boxes((size_t)n * 2 * dim);
+  for (uint64_t i = 0; i < n; ++i) {
+    const double* p = prims + i * ps;
+    double* bx = boxes.data() + i * 2 * dim;
+    for (int k = 0; k < dim; ++k) {
+      if (kind == KIND_AABB) { bx[k] = p[k]; bx[dim + k] = p[dim + k]; }
+      else { bx[k] = p[k] - p[dim]; bx[dim + k] = p[k] + p[dim]; }
+    }
+    if (kind == KIND_SPHERE && !(p[dim] >= 0.0)) return fail(SQ_ERR_INVALID, "sphere radius must be >= 0");
+  }
+  std::unique_ptr<sq_bvh64> b(new sq_bvh64());
+  b->ctx = c; b->dim = dim; b->kind = kind; b->n_prims = n;
+  sqh::build_tree(dim, boxes.data(), (uint32_t)n, b->tree);
+  const sqh::RefTreeT<double>& t = b->tree;
+  uint64_t leaves = 0, nprims = 0, depth = 0;
+  sqh::tree_structural(t, &leaves, &nprims, &depth);
+  if (depth > F64_MAX_STACK) return fail(SQ_ERR_UNSUPPORTED, "tree deeper than the f64 walk's stack");
+  // TreeStats (tree.rs:81-110, bvh.rs:477-489) of the f64 tree: size_of::<Node<D>>() is 16D + 12 bytes
+  // padded to the f64 alignment; the interior surface sum is summed in double and reported as float
+  b->stats.node_count = t.node_count();
+  b->stats.leaf_count = leaves;
+  b->stats.prim_count = nprims;
+  b->stats.max_depth = depth;
+  b->stats.bytes = b->stats.node_count * (uint64_t)(16 * dim + 16) + t.prim_indices.size() * 4;
+  double quality = 0.0;
+  for (size_t i = 0; i < t.node_count(); ++i) {
+    if (t.prim_count[i] > 0) continue;
+    const double* bx = t.bounds.data() + i * 2 * dim;
+    double sum = 0.0;   // Aabb::half_surface, aabb.rs:84-97
+    for (int a = 0; a < dim; ++a) {
+      double prod = 1.0;
+      for (int k = 0; k < dim; ++k) if (k != a) prod = prod * (bx[dim + k] - bx[k]);
+      sum = sum + prod;
+    }
+    quality = quality + sum;
+  }
+  b->stats.quality = (float)quality;
+  // pair nodes in reference pre-order of their interior nodes
+  const size_t nn = t.node_count();
+  std::vector<uint32_t> pair_of(nn, 0);
+  uint32_t np = 0;
+  for (size_t i = 0; i < nn; ++i) if (t.prim_count[i] == 0) pair_of[i] = np++;
+  std::vector<double> pb((size_t)np * 4 * dim);
+  std::vector<uint32_t> pr((size_t)np * 4);
+  for (size_t i = 0; i < nn; ++i) {
+    if (t.prim_count[i] != 0) continue;
+    const uint32_t p = pair_of[i];
+    const uint32_t ch[2] = {t.child_a[i], t.child_b[i]};
+    for (int s = 0; s < 2; ++s) {
+      const uint32_t cn = ch[s];
+      std::memcpy(pb.data() + ((size_t)p * 4 + 2 * s) * dim, t.bounds.data() + (size_t)cn * 2 * dim, 2 * dim * sizeof(double));
+      const bool leaf = t.prim_count[cn] != 0;
+      pr[(size_t)p * 4 + s] = leaf ? t.child_a[cn] : pair_of[cn];
+      pr[(size_t)p * 4 + 2 + s] = leaf ? t.prim_count[cn] : 0;
+    }
+  }
+  // leaf-ordered primitives and their original ids
+  std::vector<double> lp((size_t)n * ps);
+  std::vector<uint32_t> ids(n);
+  for (uint64_t k = 0; k < n; ++k) {
+    const uint32_t id = t.prim_indices[k];
+    std::memcpy(lp.data() + k * ps, prims + (size_t)id * ps, ps * sizeof(double));
+    ids[k] = id;
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  if ((rc = b->boxes.ensure(pb.size() * 8)) || (rc = b->refs.ensure(pr.size() * 4)) ||
+      (rc = b->prims.ensure(lp.size() * 8)) || (rc = b->ids.ensure(ids.size() * 4)))
+    return rc;
+  if (np) {
+    HIP_TRY(hipMemcpy(b->boxes.p, pb.data(), pb.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->refs.p, pr.data(), pr.size() * 4, hipMemcpyHostToDevice));
+  }
+  if (n) {
+    HIP_TRY(hipMemcpy(b->prims.p, lp.data(), lp.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->ids.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
+  }
+  View64& v = b->view;
+  v.boxes = b->boxes.as<double>();
+  v.refs = b->refs.as<uint4>();
+  v.prims = b->prims.as<double>();
+  v.ids = b->ids.as<uint32_t>();
+  v.layers = nullptr;
+  v.n_prims = (uint32_t)n;
+  if (nn) {
+    std::memcpy(v.root, t.bounds.data(), 2 * dim * sizeof(double));
+    v.root_count = t.prim_count[0];
+    v.root_ref = t.prim_count[0] ? t.child_a[0] : pair_of[0];
+  }
+  ctx_retain(c);
+  *out = b.release();
+  return SQ_OK;
+}
+
+int sq_bvh64_destroy(sq_bvh64* b) {
+  if (!b) return SQ_OK;
+  sq_ctx* c = b->ctx;
+  if (c) (void)hipStreamSynchronize(c->stream);
+  delete b;
+  if (c) ctx_release(c);
+  return SQ_OK;
+}
+
+int sq_bvh64_stats(const sq_bvh64* b, sq_tree_stats* out) {
+  if (!b || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = b->stats;
+  return SQ_OK;
+}
+
+int sq_bvh64_set_layers(sq_bvh64* b, const uint64_t* layers) {
+  if (!b) return fail(SQ_ERR_INVALID, "bvh is NULL");
+  if (!layers) {
+    b->layers.~DevBuf();
+    new (&b->layers) DevBuf();
+  } else {
+    int rc = b->layers.ensure(std::max<uint64_t>(b->n_prims, 1) * 8);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(b->layers.p, layers, b->n_prims * 8, hipMemcpyDefault, b->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));   // the caller may free a host array on return
+  }
+  b->view.layers = b->layers.p ? b->layers.as<uint64_t>() : nullptr;
+  return SQ_OK;
+}
+
+static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const sq_filter* f, int32_t* out_leaf,
+                     double* out_toi, double* out_point, double* out_normal, uint8_t* out_any) {
+  if (!b || (n && !rays)) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (n && !out_any && (!out_leaf || !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
+  if (!n) return SQ_OK;
+  sq_ctx* c = b->ctx;
+  const Filter fd = filter64(b, f);
+  return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
+    if (out_any)
+      k64_nearest<D, KIND, true><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, rays, n, max_toi, fd, nullptr, nullptr,
+                                                                           nullptr, nullptr, out_any);
+    else
+      k64_nearest<D, KIND, false><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, rays, n, max_toi, fd, out_leaf, out_toi,
+                                                                            out_point, out_normal, nullptr);
+    HIP_TRY(hipGetLastError());
+    return SQ_OK;
+  });
+}
+
+int sq_bvh64_raycast_nearest(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const sq_filter* f,
+                             int32_t* out_leaf, double* out_toi, double* out_point, double* out_normal) {
+  return nearest64(b, rays, n, max_toi, f, out_leaf, out_toi, out_point, out_normal, nullptr);
+}
+
+int sq_bvh64_raycast_any(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const sq_filter* f,
+                         uint8_t* out_hit) {
+  if (n && !out_hit) return fail(SQ_ERR_INVALID, "NULL argument");
+  return nearest64(b, rays, n, max_toi, f, nullptr, nullptr, nullptr, nullptr, out_hit);
+}
+
+static int collect64(sq_bvh64* b, const double* q, uint64_t n, double max_toi, const sq_filter* f, bool overlap,
+                     uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total) {
+  if (!b || !offsets || !total || (n && !q)) return fail(SQ_ERR_INVALID, "NULL argument");
+  sq_ctx* c = b->ctx;
+  *total = 0;
+  if (!n) {
+    HIP_TRY(hipMemsetAsync(offsets, 0, 8, c->stream));
+    return SQ_OK;
+  }
+  int rc;
+  if ((rc = c->counts.ensure(n * 4))) return rc;
+  const Filter fd = filter64(b, f);
+  auto launch = [&](const uint64_t* offs) -> int {
+    return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
+      if (overlap)
+        k64_collect<D, KIND, true><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, fd,
+                                                                            c->counts.as<uint32_t>(), offs, out_leaf, out_toi);
+      else
+        k64_collect<D, KIND, false><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, fd,
+                                                                             c->counts.as<uint32_t>(), offs, out_leaf, out_toi);
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    });
+  };
+  if ((rc = launch(nullptr))) return rc;
+  std::vector<std::unique_ptr<DevBuf>> temps;
+  if ((rc = scan_counts(c, c->counts.as<uint32_t>(), n, offsets, temps))) return rc;
+  HIP_TRY(hipMemcpyAsync(total, offsets + n, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  const uint64_t tot = *total;
+  if (tot > capacity)
+    return fail(SQ_ERR_CAPACITY, "result needs " + std::to_string(tot) + " entries, capacity is " + std::to_string(capacity));
+  if (!tot) return SQ_OK;
+  if (!out_leaf || (!overlap && !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
+  return launch(offsets);
+}
+
+int sq_bvh64_raycast_all(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const sq_filter* f,
+                         uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total) {
+  return collect64(b, rays, n, max_toi, f, false, offsets, out_leaf, out_toi, capacity, total);
+}
+
+int sq_bvh64_overlap(sq_bvh64* b, const double* regions, uint64_t n, const sq_filter* f, uint64_t* offsets,
+                     int32_t* out_leaf, uint64_t capacity, uint64_t* total) {
+  return collect64(b, regions, n, 0.0, f, true, offsets, out_leaf, nullptr, capacity, total);
+}
+
+}  // extern "C"

@@ -22,79 +22,99 @@ namespace sqh {
 namespace {
 
 constexpr uint32_t LEAF_SIZE = 4;                  // tree.rs:12
-constexpr float F32_EPSILON = 1.1920929e-7f;       // f32::EPSILON
+template <class R> struct Eps;
+template <> struct Eps<float> { static constexpr float v = 1.1920929e-7f; };            // f32::EPSILON
+template <> struct Eps<double> { static constexpr double v = 2.220446049250313e-16; };  // f64::EPSILON
 constexpr uint32_t PAR_MIN_COUNT = 1u << 15;       // split work across threads above this
 constexpr int PAR_MAX_DEPTH = 4;                   // at most 2^4 concurrent subtrees
 
-// f32::total_cmp order as an unsigned key.
+// f32::total_cmp / f64::total_cmp order as an unsigned key.
 inline uint32_t total_ukey(float f) {
   int32_t i;
   std::memcpy(&i, &f, 4);
   i ^= (int32_t)(((uint32_t)(i >> 31)) >> 1);
   return (uint32_t)i ^ 0x80000000u;
 }
+inline uint64_t total_ukey(double f) {
+  int64_t i;
+  std::memcpy(&i, &f, 8);
+  i ^= (int64_t)(((uint64_t)(i >> 63)) >> 1);
+  return (uint64_t)i ^ 0x8000000000000000ull;
+}
 
-struct Ctx {
+template <class R> struct Ctx {
+  using Key = decltype(total_ukey(R(0)));
   int dim;
-  const float* boxes;                 // [n][2D]
-  std::vector<float> cent;            // [n][D]
-  std::vector<uint32_t> keys;         // [D][n]
+  const R* boxes;                     // [n][2D]
+  std::vector<R> cent;                // [n][D]
+  std::vector<Key> keys;              // [D][n]
   uint32_t n;
 };
 
-struct Local {
-  std::vector<float> bounds;
+template <class R> struct Local {
+  std::vector<R> bounds;
   std::vector<uint32_t> ca, cb, pc;
-  uint32_t push(int dim, const float* b, uint32_t a, uint32_t c, uint32_t p) {
+  uint32_t push(int dim, const R* b, uint32_t a, uint32_t c, uint32_t p) {
     bounds.insert(bounds.end(), b, b + 2 * dim);
     ca.push_back(a); cb.push_back(c); pc.push_back(p);
     return (uint32_t)ca.size() - 1;
   }
 };
 
-uint32_t build_range(const Ctx& cx, uint32_t* idx, uint32_t start, uint32_t end, int depth, Local& out) {
+// Stable order of idx[start, end) by key: (key, original position) order.
+inline void sort_by_key(const uint32_t* key, uint32_t* idx, uint32_t start, uint32_t count) {
+  std::vector<uint64_t> kv(count);
+  for (uint32_t i = 0; i < count; ++i) kv[i] = ((uint64_t)key[idx[start + i]] << 32) | i;
+  std::sort(kv.begin(), kv.end());
+  std::vector<uint32_t> tmp(count);
+  for (uint32_t i = 0; i < count; ++i) tmp[i] = idx[start + (uint32_t)kv[i]];
+  std::memcpy(idx + start, tmp.data(), count * sizeof(uint32_t));
+}
+inline void sort_by_key(const uint64_t* key, uint32_t* idx, uint32_t start, uint32_t count) {
+  std::vector<std::pair<uint64_t, uint32_t>> kv(count);
+  for (uint32_t i = 0; i < count; ++i) kv[i] = {key[idx[start + i]], i};
+  std::sort(kv.begin(), kv.end());
+  std::vector<uint32_t> tmp(count);
+  for (uint32_t i = 0; i < count; ++i) tmp[i] = idx[start + kv[i].second];
+  std::memcpy(idx + start, tmp.data(), count * sizeof(uint32_t));
+}
+
+template <class R>
+uint32_t build_range(const Ctx<R>& cx, uint32_t* idx, uint32_t start, uint32_t end, int depth, Local<R>& out) {
   const int D = cx.dim;
   // tree.rs:120-123: fold union over the range, from EMPTY.
-  float nb[2 * MAX_DIM];
+  R nb[2 * MAX_DIM];
   for (int k = 0; k < D; ++k) { nb[k] = INFINITY; nb[D + k] = -INFINITY; }
   for (uint32_t i = start; i < end; ++i) {
-    const float* b = cx.boxes + (size_t)idx[i] * 2 * D;
-    for (int k = 0; k < D; ++k) { nb[k] = fminf(nb[k], b[k]); nb[D + k] = fmaxf(nb[D + k], b[D + k]); }
+    const R* b = cx.boxes + (size_t)idx[i] * 2 * D;
+    for (int k = 0; k < D; ++k) { nb[k] = std::fmin(nb[k], b[k]); nb[D + k] = std::fmax(nb[D + k], b[D + k]); }
   }
   const uint32_t count = end - start;
   const uint32_t node = out.push(D, nb, start, 0, count);   // :129-134
   if (count <= LEAF_SIZE) return node;                       // :136-138
   // :140-148: centroid bounds, longest axis (first max wins), coincidence stop.
-  float cmin[MAX_DIM], cmax[MAX_DIM];
+  R cmin[MAX_DIM], cmax[MAX_DIM];
   for (int k = 0; k < D; ++k) { cmin[k] = INFINITY; cmax[k] = -INFINITY; }
   for (uint32_t i = start; i < end; ++i) {
-    const float* c = cx.cent.data() + (size_t)idx[i] * D;
-    for (int k = 0; k < D; ++k) { cmin[k] = fminf(cmin[k], c[k]); cmax[k] = fmaxf(cmax[k], c[k]); }
+    const R* c = cx.cent.data() + (size_t)idx[i] * D;
+    for (int k = 0; k < D; ++k) { cmin[k] = std::fmin(cmin[k], c[k]); cmax[k] = std::fmax(cmax[k], c[k]); }
   }
   int axis = 0;
-  float best = cmax[0] - cmin[0];
-  for (int k = 1; k < D; ++k) { float e = cmax[k] - cmin[k]; if (e > best) { best = e; axis = k; } }
-  if (best <= F32_EPSILON) return node;
+  R best = cmax[0] - cmin[0];
+  for (int k = 1; k < D; ++k) { R e = cmax[k] - cmin[k]; if (e > best) { best = e; axis = k; } }
+  if (best <= Eps<R>::v) return node;
   // :152-153: stable sort by the centroid on `axis`.
-  const uint32_t* key = cx.keys.data() + (size_t)axis * cx.n;
-  std::vector<uint64_t> kv(count);
-  for (uint32_t i = 0; i < count; ++i) kv[i] = ((uint64_t)key[idx[start + i]] << 32) | i;
-  std::sort(kv.begin(), kv.end());  // (key, original position) order == stable order by key
-  std::vector<uint32_t> tmp(count);
-  for (uint32_t i = 0; i < count; ++i) tmp[i] = idx[start + (uint32_t)kv[i]];
-  std::memcpy(idx + start, tmp.data(), count * sizeof(uint32_t));
-  std::vector<uint64_t>().swap(kv);
-  std::vector<uint32_t>().swap(tmp);
+  sort_by_key(cx.keys.data() + (size_t)axis * cx.n, idx, start, count);
   const uint32_t mid = start + count / 2;  // :154
 
   uint32_t a, b;
   if (count >= PAR_MIN_COUNT && depth < PAR_MAX_DEPTH) {
-    Local la, lb;
+    Local<R> la, lb;
     auto fa = std::async(std::launch::async, [&] { build_range(cx, idx, start, mid, depth + 1, la); });
     build_range(cx, idx, mid, end, depth + 1, lb);
     fa.get();
     // Splice both subtrees in pre-order, shifting their interior child links.
-    auto splice = [&](Local& l) {
+    auto splice = [&](Local<R>& l) {
       const uint32_t base = (uint32_t)out.ca.size();
       out.bounds.insert(out.bounds.end(), l.bounds.begin(), l.bounds.end());
       for (size_t i = 0; i < l.ca.size(); ++i) {
@@ -159,25 +179,27 @@ void prim_boxes(int dim, int kind, const float* prims, uint64_t n, std::vector<f
   }
 }
 
-void build_tree(int dim, const float* boxes, uint32_t n, RefTree& out) {
-  out = RefTree();
+namespace {
+
+template <class R> void build_tree_t(int dim, const R* boxes, uint32_t n, RefTreeT<R>& out) {
+  out = RefTreeT<R>();
   out.dim = dim;
   out.prim_indices.resize(n);
   for (uint32_t i = 0; i < n; ++i) out.prim_indices[i] = i;
   if (n == 0) return;
-  Ctx cx;
+  Ctx<R> cx;
   cx.dim = dim; cx.boxes = boxes; cx.n = n;
   cx.cent.resize((size_t)n * dim);
   cx.keys.resize((size_t)n * dim);
   for (uint32_t i = 0; i < n; ++i) {
-    const float* b = boxes + (size_t)i * 2 * dim;
+    const R* b = boxes + (size_t)i * 2 * dim;
     for (int k = 0; k < dim; ++k) {
-      float c = (b[k] + b[dim + k]) * 0.5f;   // Aabb::center, aabb.rs:56-58
+      R c = (b[k] + b[dim + k]) * R(0.5);   // Aabb::center, aabb.rs:56-58
       cx.cent[(size_t)i * dim + k] = c;
       cx.keys[(size_t)k * n + i] = total_ukey(c);
     }
   }
-  Local l;
+  Local<R> l;
   l.ca.reserve(2 * (size_t)n / 3 + 1);
   build_range(cx, out.prim_indices.data(), 0, n, 0, l);
   out.bounds.swap(l.bounds);
@@ -186,7 +208,12 @@ void build_tree(int dim, const float* boxes, uint32_t n, RefTree& out) {
   out.prim_count.swap(l.pc);
 }
 
-void tree_structural(const RefTree& t, uint64_t* leaves, uint64_t* prims, uint64_t* depth_out) {
+}  // namespace
+
+void build_tree(int dim, const float* boxes, uint32_t n, RefTree& out) { build_tree_t(dim, boxes, n, out); }
+void build_tree(int dim, const double* boxes, uint32_t n, RefTreeT<double>& out) { build_tree_t(dim, boxes, n, out); }
+
+template <class R> void tree_structural(const RefTreeT<R>& t, uint64_t* leaves, uint64_t* prims, uint64_t* depth_out) {
   *leaves = *prims = *depth_out = 0;
   const size_t nn = t.node_count();
   if (!nn) return;
@@ -197,6 +224,8 @@ void tree_structural(const RefTree& t, uint64_t* leaves, uint64_t* prims, uint64
   }
   *depth_out = depth[0];
 }
+template void tree_structural<float>(const RefTreeT<float>&, uint64_t*, uint64_t*, uint64_t*);
+template void tree_structural<double>(const RefTreeT<double>&, uint64_t*, uint64_t*, uint64_t*);
 
 float tree_interior_surface_sum(const RefTree& t) {
   const int D = t.dim;

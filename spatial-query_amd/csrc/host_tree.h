@@ -10,14 +10,17 @@ namespace sqh {
 constexpr int MAX_DIM = 8;
 
 // A tree in the reference layout: Bvh::nodes() / Bvh::prim_indices()
-// (crates/spatial-query/src/accel/tree.rs:20-26, bvh.rs:465-475).
-struct RefTree {
+// (crates/spatial-query/src/accel/tree.rs:20-26, bvh.rs:465-475), over the
+// reference's Scalar: float, or double under its `f64` feature
+// (crates/spatial-query/src/maths/point.rs:14-18).
+template <class R> struct RefTreeT {
   int dim = 0;
-  std::vector<float> bounds;        // [nn][2*dim] min then max
+  std::vector<R> bounds;            // [nn][2*dim] min then max
   std::vector<uint32_t> child_a, child_b, prim_count;
   std::vector<uint32_t> prim_indices;
   size_t node_count() const { return child_a.size(); }
 };
+using RefTree = RefTreeT<float>;
 
 // Primitive boxes (QueryGeometry::world_aabb) of the built-in kinds, [n][2*dim].
 void prim_boxes(int dim, int kind, const float* prims, uint64_t n, std::vector<float>& boxes);
@@ -25,9 +28,10 @@ int prim_stride(int dim, int kind);  // floats per primitive in the caller's lay
 
 // tree::build (tree.rs:39-49, 112-163): same topology node for node.
 void build_tree(int dim, const float* boxes, uint32_t n, RefTree& out);
+void build_tree(int dim, const double* boxes, uint32_t n, RefTreeT<double>& out);
 
 // TreeStats pieces (tree.rs:81-110).
-void tree_structural(const RefTree& t, uint64_t* leaves, uint64_t* prims, uint64_t* depth);
+template <class R> void tree_structural(const RefTreeT<R>& t, uint64_t* leaves, uint64_t* prims, uint64_t* depth);
 float tree_interior_surface_sum(const RefTree& t);
 
 // Device layout.

@@ -70,6 +70,25 @@ SQ_HD float bitsf(uint32_t u) {
   float f; std::memcpy(&f, &u, 4); return f;
 #endif
 }
+// Scalar-generic pieces of the arithmetic below.  The FP32 walks instantiate
+// it on float; the f64 walk (f64.hip) on double, as the reference's `f64`
+// feature switches its Scalar (crates/spatial-query/src/maths/point.rs:14-18).
+template <class R> struct Eps;
+template <> struct Eps<float> { static constexpr float v = 1.1920929e-7f; };            // f32::EPSILON
+template <> struct Eps<double> { static constexpr double v = 2.220446049250313e-16; };  // f64::EPSILON
+SQ_HD float sq_abs(float x) { return fabsf(x); }
+SQ_HD double sq_abs(double x) { return fabs(x); }
+SQ_HD float sq_min(float a, float b) { return fminf(a, b); }
+SQ_HD double sq_min(double a, double b) { return fmin(a, b); }
+SQ_HD float sq_max(float a, float b) { return fmaxf(a, b); }
+SQ_HD double sq_max(double a, double b) { return fmax(a, b); }
+SQ_HD float sq_sqrt(float x) { return SQ_SQRT_RN(x); }
+#if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+SQ_HD double sq_sqrt(double x) { return __builtin_sqrt(x); }   // correctly rounded f64 sqrt
+#else
+SQ_HD double sq_sqrt(double x) { return std::sqrt(x); }
+#endif
+
 // f32::total_cmp as a signed key.
 SQ_HD int32_t total_key(float f) {
   int32_t i = (int32_t)fbits(f);
@@ -118,20 +137,20 @@ template <int N16> SQ_HD void fetch16(const float* src, float* dst) {
 #endif
 }
 
-template <int D> struct RayD {
-  float o[D], d[D], inv[D];
+template <int D, class R = float> struct RayD {
+  R o[D], d[D], inv[D];
   uint32_t par;             // bit k: |d[k]| < EPSILON (the slab's parallel branch)
 };
 
-template <int D> SQ_HD void ray_setup(RayD<D>& r, const float* src) {
+template <int D, class R = float> SQ_HD void ray_setup(RayD<D, R>& r, const R* src) {
   r.par = 0;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     r.o[k] = src[k];
     r.d[k] = src[D + k];
     // The reference computes 1.0 / d per node; hoisting it gives the same bits.
-    r.inv[k] = SQ_DIV_RN(1.0f, r.d[k]);
-    if (fabsf(r.d[k]) < F32_EPSILON) r.par |= 1u << k;
+    r.inv[k] = SQ_DIV_RN(R(1), r.d[k]);
+    if (sq_abs(r.d[k]) < Eps<R>::v) r.par |= 1u << k;
   }
 }
 
@@ -143,18 +162,18 @@ template <int D> SQ_HD void ray_setup(RayD<D>& r, const float* src) {
 // with stride `s` between the min and max of an axis.  Returns hit; writes the
 // entry (t_min, exactly as the reference: +0 unless an axis raised it), the
 // exit and the entry axis (-1 when no axis raised t_min).
-template <int D> SQ_HD bool slab_exact(const RayD<D>& r, const float* mn, const float* mx, float max_toi,
-                                       float& t_min_out, float& t_max_out, int& axis_out) {
-  float t_min = 0.0f, t_max = max_toi;
+template <int D, class R = float> SQ_HD bool slab_exact(const RayD<D, R>& r, const R* mn, const R* mx, R max_toi,
+                                                    R& t_min_out, R& t_max_out, int& axis_out) {
+  R t_min = R(0), t_max = max_toi;
   int axis = -1;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     if (r.par & (1u << k)) {
       if (r.o[k] < mn[k] || r.o[k] > mx[k]) return false;
     } else {
-      float t1 = (mn[k] - r.o[k]) * r.inv[k];
-      float t2 = (mx[k] - r.o[k]) * r.inv[k];
-      if (t1 > t2) { float tmp = t1; t1 = t2; t2 = tmp; }
+      R t1 = (mn[k] - r.o[k]) * r.inv[k];
+      R t2 = (mx[k] - r.o[k]) * r.inv[k];
+      if (t1 > t2) { R tmp = t1; t1 = t2; t2 = tmp; }
       if (t1 > t_min) { t_min = t1; axis = k; }
       if (t2 < t_max) t_max = t2;
       if (t_min > t_max) return false;
@@ -202,39 +221,39 @@ template <int D> SQ_HD void slab_pair(const RayD<D>& r, const float* nd, float l
 // Each returns true with toi when the primitive is hit within `limit`.
 // ---------------------------------------------------------------------------
 
-template <int D> SQ_HD bool narrow_aabb(const RayD<D>& r, const float* p, float limit, float& t, int& axis) {
-  float te, tx;
+template <int D, class R = float> SQ_HD bool narrow_aabb(const RayD<D, R>& r, const R* p, R limit, R& t, int& axis) {
+  R te, tx;
   if (!slab_exact<D>(r, p, p + D, limit, te, tx, axis)) return false;
   t = te;
   return true;
 }
 
-template <int D> SQ_HD float dotD(const float* a, const float* b) {
-  float acc = 0.0f;
+template <int D, class R = float> SQ_HD R dotD(const R* a, const R* b) {
+  R acc = R(0);
 #pragma unroll
   for (int k = 0; k < D; ++k) acc = acc + a[k] * b[k];
   return acc;
 }
 
 // Ball (differential.rs:61-82) with the box guard.
-template <int D> SQ_HD bool narrow_sphere(const RayD<D>& r, const float* p, float limit, float& t_out) {
-  float oc[D];
+template <int D, class R = float> SQ_HD bool narrow_sphere(const RayD<D, R>& r, const R* p, R limit, R& t_out) {
+  R oc[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) oc[k] = r.o[k] - p[k];
-  const float rad = p[D];
-  float b = dotD<D>(oc, r.d);
-  float cc = dotD<D>(oc, oc) - rad * rad;
-  float disc = b * b - cc;
-  if (disc < 0.0f) return false;
-  float sq = SQ_SQRT_RN(disc);
-  float t = -b - sq;
-  if (t < 0.0f) t = -b + sq;
-  if (t < 0.0f || t > limit) return false;
+  const R rad = p[D];
+  R b = dotD<D>(oc, r.d);
+  R cc = dotD<D>(oc, oc) - rad * rad;
+  R disc = b * b - cc;
+  if (disc < R(0)) return false;
+  R sq = sq_sqrt(disc);
+  R t = -b - sq;
+  if (t < R(0)) t = -b + sq;
+  if (t < R(0) || t > limit) return false;
   // guard: toi inside the slab interval of the ball's AABB (c - r, c + r)
-  float mn[D], mx[D];
+  R mn[D], mx[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) { mn[k] = p[k] - rad; mx[k] = p[k] + rad; }
-  float te, tx; int ax;
+  R te, tx; int ax;
   if (!slab_exact<D>(r, mn, mx, limit, te, tx, ax)) return false;
   if (!(te <= t && t <= tx)) return false;
   t_out = t;
@@ -286,7 +305,7 @@ template <int D> struct Prim<D, KIND_AABB> { static constexpr int STRIDE = 2 * D
 template <int D> struct Prim<D, KIND_SPHERE> { static constexpr int STRIDE = D + 1; };
 template <> struct Prim<3, KIND_TRI> { static constexpr int STRIDE = 12; };
 
-template <int D, int KIND> SQ_HD bool narrow(const RayD<D>& r, const float* p, float limit, float& t) {
+template <int D, int KIND, class R = float> SQ_HD bool narrow(const RayD<D, R>& r, const R* p, R limit, R& t) {
   if constexpr (KIND == KIND_AABB) { int ax; return narrow_aabb<D>(r, p, limit, t, ax); }
   else if constexpr (KIND == KIND_SPHERE) return narrow_sphere<D>(r, p, limit, t);
   else return narrow_tri(r, p, limit, t);
@@ -296,22 +315,22 @@ template <int D, int KIND> SQ_HD bool narrow(const RayD<D>& r, const float* p, f
 // AABB: entry-face normal (-sign(d) on the entry axis, zero when the origin is
 // inside); SPHERE: normalize_or_zero(at(t) - c) (point.rs:82-89); TRIANGLE:
 // normalize_or_zero(e1 x e2) turned against the ray.
-template <int D, int KIND> SQ_HD void hit_normal(const RayD<D>& r, const float* p, float t, float limit, float* n) {
+template <int D, int KIND, class R = float> SQ_HD void hit_normal(const RayD<D, R>& r, const R* p, R t, R limit, R* n) {
 #pragma unroll
-  for (int k = 0; k < D; ++k) n[k] = 0.0f;
+  for (int k = 0; k < D; ++k) n[k] = R(0);
   if constexpr (KIND == KIND_AABB) {
-    float tt; int ax = -1;
+    R tt; int ax = -1;
     narrow_aabb<D>(r, p, limit, tt, ax);
     // n[ax] = -sign(d[ax]) written as a select per axis (no dynamic indexing)
 #pragma unroll
-    for (int k = 0; k < D; ++k) n[k] = (k == ax) ? (r.d[k] > 0.0f ? -1.0f : 1.0f) : 0.0f;
+    for (int k = 0; k < D; ++k) n[k] = (k == ax) ? (r.d[k] > R(0) ? R(-1) : R(1)) : R(0);
   } else if constexpr (KIND == KIND_SPHERE) {
-    float v[D];
+    R v[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) v[k] = (r.o[k] + r.d[k] * t) - p[k];
-    float len = SQ_SQRT_RN(dotD<D>(v, v));
-    if (len > F32_EPSILON) {
-      float s = SQ_DIV_RN(1.0f, len);
+    R len = sq_sqrt(dotD<D>(v, v));
+    if (len > Eps<R>::v) {
+      R s = SQ_DIV_RN(R(1), len);
 #pragma unroll
       for (int k = 0; k < D; ++k) n[k] = v[k] * s;
     }
@@ -333,7 +352,7 @@ template <int D, int KIND> SQ_HD void hit_normal(const RayD<D>& r, const float* 
 // ---------------------------------------------------------------------------
 
 // The AABB of one leaf-ordered primitive record (QueryGeometry::world_aabb).
-template <int D, int KIND> SQ_HD void prim_box(const float* p, float* mn, float* mx) {
+template <int D, int KIND, class R = float> SQ_HD void prim_box(const R* p, R* mn, R* mx) {
   if constexpr (KIND == KIND_AABB) {
 #pragma unroll
     for (int k = 0; k < D; ++k) { mn[k] = p[k]; mx[k] = p[D + k]; }
@@ -566,7 +585,7 @@ SQ_HD void walk_visit(const SceneView& s, uint32_t root_ref, PairPred&& pair_pre
 }
 
 // Box overlap (aabb.rs:101-108): touching counts.
-template <int D> SQ_HD bool box_overlap(const float* amn, const float* amx, const float* bmn, const float* bmx) {
+template <int D, class R = float> SQ_HD bool box_overlap(const R* amn, const R* amx, const R* bmn, const R* bmx) {
   bool ok = true;
 #pragma unroll
   for (int k = 0; k < D; ++k) ok = ok && !(amn[k] > bmx[k] || amx[k] < bmn[k]);
@@ -575,21 +594,21 @@ template <int D> SQ_HD bool box_overlap(const float* amn, const float* amx, cons
 
 // Exact overlap narrow phase of the built-in kinds: AABB and TRIANGLE by
 // their AABB; SPHERE by the nearest point of the region within the radius.
-template <int D, int KIND> SQ_HD bool overlap_narrow(const float* p, const float* rmn, const float* rmx) {
+template <int D, int KIND, class R = float> SQ_HD bool overlap_narrow(const R* p, const R* rmn, const R* rmx) {
   if constexpr (KIND == KIND_SPHERE) {
-    float d2 = 0.0f;
+    R d2 = R(0);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      const float c = p[k];
-      float v;
+      const R c = p[k];
+      R v;
       if (c < rmn[k]) v = rmn[k] - c;
       else if (c > rmx[k]) v = c - rmx[k];
-      else v = 0.0f;
+      else v = R(0);
       d2 = d2 + v * v;
     }
-    return SQ_SQRT_RN(d2) <= p[D];
+    return sq_sqrt(d2) <= p[D];
   } else {
-    float mn[D], mx[D];
+    R mn[D], mx[D];
     prim_box<D, KIND>(p, mn, mx);
     return box_overlap<D>(mn, mx, rmn, rmx);
   }

@@ -337,6 +337,93 @@ class DeviceBvh:
             pass
 
 
+class DeviceBvh64:
+    """The reference's `f64` feature (maths/point.rs:14-18): a tree over
+    double-precision AABB or SPHERE primitives, built with the median split in
+    double and walked on the GPU in double.  Queries take float64 CUDA tensors
+    and answer as DeviceBvh does, with float64 toi, points and normals."""
+
+    def __init__(self, ctx: Context, dim: int, kind: int, prims):
+        p = _host(prims, np.float64).reshape(-1, prim_stride(dim, kind))
+        h = C.c_void_p()
+        check(lib().sq_bvh64_create(ctx.handle, dim, kind, _hp(p), p.shape[0], C.byref(h)))
+        self._h, self.ctx, self.dim, self.kind, self.n_prims = h, ctx, dim, kind, p.shape[0]
+
+    def set_layers(self, layers):
+        l = None if layers is None else _host(layers, np.uint64)
+        check(lib().sq_bvh64_set_layers(self._h, _hp(l)))
+
+    def stats(self) -> dict:
+        s = TreeStats()
+        check(lib().sq_bvh64_stats(self._h, C.byref(s)))
+        return _stats(s)
+
+    def _f(self, filt):
+        return None if filt is None else C.byref(filt.c())
+
+    def raycast_nearest(self, rays, max_toi, filt=None, point=False, normal=False):
+        import torch
+        n, dev = rays.shape[0], rays.device
+        leaf = torch.empty(n, dtype=torch.int32, device=dev)
+        toi = torch.empty(n, dtype=torch.float64, device=dev)
+        pt = torch.empty((n, self.dim), dtype=torch.float64, device=dev) if point else None
+        nm = torch.empty((n, self.dim), dtype=torch.float64, device=dev) if normal else None
+        check(lib().sq_bvh64_raycast_nearest(self._h, _dp(rays), n, float(max_toi), self._f(filt), _dp(leaf), _dp(toi),
+                                             _dp(pt), _dp(nm)))
+        out = dict(leaf=leaf, toi=toi)
+        if point: out["point"] = pt
+        if normal: out["normal"] = nm
+        return out
+
+    def raycast_any(self, rays, max_toi, filt=None):
+        import torch
+        out = torch.empty(rays.shape[0], dtype=torch.uint8, device=rays.device)
+        check(lib().sq_bvh64_raycast_any(self._h, _dp(rays), rays.shape[0], float(max_toi), self._f(filt), _dp(out)))
+        return out
+
+    def _csr(self, fn, n, dev, has_t, capacity):
+        import torch
+        offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        total = C.c_uint64()
+        cap = capacity or max(getattr(self, "_csr_hint", 0), 4 * n)
+        while True:
+            leaf = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+            toi = torch.empty(max(cap, 1), dtype=torch.float64, device=dev) if has_t else None
+            rc = fn(offsets, leaf, toi, cap, total)
+            if rc == SQ_ERR_CAPACITY:
+                cap = int(total.value)
+                continue
+            check(rc)
+            break
+        t = int(total.value)
+        self._csr_hint = t
+        return offsets, leaf[:t], (toi[:t] if has_t else None)
+
+    def raycast_all(self, rays, max_toi, filt=None, capacity=0):
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_bvh64_raycast_all(self._h, _dp(rays), rays.shape[0], float(max_toi), self._f(filt),
+                                              _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+        return self._csr(fn, rays.shape[0], rays.device, True, capacity)
+
+    def overlap(self, regions, filt=None, capacity=0):
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_bvh64_overlap(self._h, _dp(regions), regions.shape[0], self._f(filt), _dp(off), _dp(leaf),
+                                          cap, C.byref(total))
+        off, leaf, _ = self._csr(fn, regions.shape[0], regions.device, False, capacity)
+        return off, leaf
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_bvh64_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Tlas:
     """Tlas<D> over built-in instance kinds (accel/tlas.rs), resident on the GPU.
 

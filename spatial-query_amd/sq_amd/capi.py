@@ -38,6 +38,7 @@ class Tuning(C.Structure):
 
 
 P, U64, I64, F32, I32, U32 = C.c_void_p, C.c_uint64, C.c_int64, C.c_float, C.c_int, C.c_uint32
+F64 = C.c_double
 PP = C.POINTER(C.c_void_p)
 
 # name -> argtypes (all return int unless listed in _RESTYPE)
@@ -112,6 +113,14 @@ SIGNATURES = {
     "sq_shapecast_nearest": [P, P, U64, F32, P, C.POINTER(Filter), P, P, P],
     "sq_shapecast_all": [P, P, U64, F32, P, C.POINTER(Filter), P, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_required_work": [P, P, U64, P, P, P],
+    "sq_bvh64_create": [P, I32, I32, P, U64, PP],
+    "sq_bvh64_destroy": [P],
+    "sq_bvh64_stats": [P, C.POINTER(TreeStats)],
+    "sq_bvh64_set_layers": [P, P],
+    "sq_bvh64_raycast_nearest": [P, P, U64, F64, C.POINTER(Filter), P, P, P, P],
+    "sq_bvh64_raycast_any": [P, P, U64, F64, C.POINTER(Filter), P],
+    "sq_bvh64_raycast_all": [P, P, U64, F64, C.POINTER(Filter), P, P, P, U64, C.POINTER(C.c_uint64)],
+    "sq_bvh64_overlap": [P, P, U64, C.POINTER(Filter), P, P, U64, C.POINTER(C.c_uint64)],
 }
 _RESTYPE = {"sq_last_error": C.c_char_p, "sq_ctx_stream": C.c_void_p}
 

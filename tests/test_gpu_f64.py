@@ -1,0 +1,151 @@
+"""The `f64` feature on the device (sq_bvh64_*: sq/src/maths/point.rs:14-18,
+Scalar = f64) against the f64 oracle (oracle/sq_oracle64.cpp, the naive scans
+of sq/src/query/naive.rs in double):
+
+  * D = 2, 3, 4, 6 over BOX and SPHERE leaves: nearest (ids, toi, point and
+    normal bit for bit), any-hit, all hits and overlap (CSR bit for bit), with
+    and without a filter; the tree's stats equal the oracle's f64 tree;
+  * a 262,144-ball scene with 1M rays: the oracle on a sample, and the
+    size-independent properties on every ray (any == nearest hit, the first
+    of a ray's all-hits list is its nearest hit, lists sorted);
+  * refusals (triangles, D = 7, negative radii) and the empty tree.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sq_amd
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def scene(dim, kind, n, seed):
+    rng = np.random.default_rng(seed)
+    span = {2: 60.0, 3: 30.0, 4: 12.0, 6: 6.0}[dim]
+    c = rng.uniform(-span, span, (n, dim))
+    if kind == sq_amd.AABB:
+        prims = np.concatenate([c, c + rng.uniform(0.3, 3.0, (n, dim))], 1)
+    else:
+        prims = np.concatenate([c, rng.uniform(0.3, 2.5, (n, 1))], 1)
+    o = rng.uniform(-span, span, (3 * n // 2, dim))
+    d = rng.normal(size=o.shape)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1)
+    h = rng.uniform(0.5, 3.0, (n // 2, dim))
+    regions = np.concatenate([c[: n // 2] - h, c[: n // 2] + h], 1)
+    return prims, rays, regions, span
+
+
+def assert_csr(got, ref, what, has_t=True):
+    assert np.array_equal(host(got[0]).astype(np.uint64), ref[0].astype(np.uint64)), f"{what}: offsets"
+    assert np.array_equal(host(got[1]).astype(np.int64), ref[1]), f"{what}: ids"
+    if has_t:
+        assert np.array_equal(host(got[2]).view(np.uint64), ref[2].view(np.uint64)), f"{what}: toi"
+
+
+@pytest.mark.parametrize("dim", [2, 3, 4, 6])
+@pytest.mark.parametrize("kind", [sq_amd.AABB, sq_amd.SPHERE])
+def test_f64_matches_oracle(ctx, dim, kind):
+    print(f"case: f64 dim={dim} kind={kind}", flush=True)
+    prims, rays, regions, span = scene(dim, kind, 4000, 640 + 10 * dim + kind)
+    layers = np.random.default_rng(dim).integers(0, 4, prims.shape[0]).astype(np.uint64)
+    b = sq_amd.DeviceBvh64(ctx, dim, kind, prims)
+    ref = oracle.Scene64(dim, kind, prims)
+    max_toi = 4.0 * span
+    st, rs = b.stats(), ref.tree_stats()
+    for k in ("node_count", "leaf_count", "prim_count", "max_depth", "bytes"):
+        assert st[k] == rs[k], k
+    assert st["quality"] == float(np.float32(rs["quality"]))
+    got = b.raycast_nearest(dev(rays), max_toi, point=True, normal=True)
+    want = ref.raycast_nearest(rays, max_toi)
+    hit = want["leaf"] >= 0
+    assert hit.sum() > len(rays) // 10
+    assert np.array_equal(host(got["leaf"]).astype(np.int64), want["leaf"])
+    assert np.array_equal(host(got["toi"]).view(np.uint64), want["t"].view(np.uint64))
+    assert np.array_equal(host(got["point"]).view(np.uint64), want["point"].view(np.uint64))
+    assert np.array_equal(host(got["normal"]).view(np.uint64), want["normal"].view(np.uint64))
+    assert np.array_equal(host(b.raycast_any(dev(rays), max_toi)).astype(bool), hit)
+    r_all = ref.raycast_all(rays[:2000], max_toi)
+    assert r_all[1].shape[0] > 200
+    assert_csr(b.raycast_all(dev(rays[:2000]), max_toi), r_all, "all hits")
+    r_ov = ref.overlap(regions)
+    assert r_ov[1].shape[0] > 1000
+    assert_csr(b.overlap(dev(regions)), (r_ov[0], r_ov[1], None), "overlap", has_t=False)
+    # filters: layers by original leaf id, a mask and an excluded id
+    b.set_layers(layers)
+    reff = oracle.Scene64(dim, kind, prims, layers=layers)
+    filt = sq_amd.QueryFilter(layer_mask=0b10, exclude=int(want["leaf"][hit][0]))
+    got = b.raycast_nearest(dev(rays), max_toi, filt=filt)
+    want = reff.raycast_nearest(rays, max_toi, mask=0b10, exclude=filt.exclude)
+    assert np.array_equal(host(got["leaf"]).astype(np.int64), want["leaf"])
+    assert np.array_equal(host(got["toi"]).view(np.uint64), want["t"].view(np.uint64))
+    assert_csr(b.raycast_all(dev(rays[:2000]), max_toi, filt=filt),
+               reff.raycast_all(rays[:2000], max_toi, mask=0b10, exclude=filt.exclude), "filtered all hits")
+    r_ov = reff.overlap(regions, mask=0b10, exclude=filt.exclude)
+    assert_csr(b.overlap(dev(regions), filt=filt), (r_ov[0], r_ov[1], None), "filtered overlap", has_t=False)
+    b.close()
+
+
+def test_f64_large_ball_scene(ctx):
+    print("case: f64 262144 balls, 1M rays", flush=True)
+    rng = np.random.default_rng(0xF64)
+    n, nr = 1 << 18, 1 << 20
+    prims = np.concatenate([rng.uniform(-400, 400, (n, 3)), rng.uniform(0.5, 4.0, (n, 1))], 1)
+    o = rng.uniform(-420, 420, (nr, 3))
+    d = rng.normal(size=(nr, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1)
+    b = sq_amd.DeviceBvh64(ctx, 3, sq_amd.SPHERE, prims)
+    rd = dev(rays)
+    got = b.raycast_nearest(rd, 2000.0)
+    leaf, toi = host(got["leaf"]).astype(np.int64), host(got["toi"])
+    assert (leaf >= 0).sum() > nr // 4
+    # the oracle on an evenly spaced sample
+    sample = np.arange(0, nr, nr // 1024)
+    want = oracle.Scene64(3, sq_amd.SPHERE, prims).raycast_nearest(rays[sample], 2000.0)
+    assert np.array_equal(leaf[sample], want["leaf"])
+    assert np.array_equal(toi[sample].view(np.uint64), want["t"].view(np.uint64))
+    # size-independent properties over every ray
+    anyh = host(b.raycast_any(rd, 2000.0)).astype(bool)
+    assert np.array_equal(anyh, leaf >= 0)
+    off, ids, ts = b.raycast_all(rd, 2000.0)
+    off, ids, ts = host(off).astype(np.int64), host(ids).astype(np.int64), host(ts)
+    cnt = np.diff(off)
+    assert np.array_equal(cnt > 0, leaf >= 0)
+    first = off[:-1][cnt > 0]
+    assert np.array_equal(ids[first], leaf[cnt > 0])
+    assert np.array_equal(ts[first].view(np.uint64), toi[cnt > 0].view(np.uint64))
+    same_ray = np.repeat(np.arange(nr), cnt)
+    nxt = np.arange(len(ids) - 1)
+    inner = same_ray[nxt] == same_ray[nxt + 1]
+    a, c = nxt[inner], nxt[inner] + 1
+    assert ((ts[a] < ts[c]) | ((ts[a] == ts[c]) & (ids[a] < ids[c]))).all()
+    b.close()
+
+
+def test_f64_refusals_and_empty(ctx):
+    print("case: f64 refusals and the empty tree", flush=True)
+    with pytest.raises(sq_amd.SqError):
+        sq_amd.DeviceBvh64(ctx, 3, sq_amd.TRIANGLE, np.zeros((4, 9)))
+    with pytest.raises(sq_amd.SqError):
+        sq_amd.DeviceBvh64(ctx, 7, sq_amd.AABB, np.zeros((4, 14)))
+    with pytest.raises(sq_amd.SqError):
+        sq_amd.DeviceBvh64(ctx, 3, sq_amd.SPHERE, np.array([[0.0, 0.0, 0.0, -1.0]]))
+    empty = sq_amd.DeviceBvh64(ctx, 3, sq_amd.AABB, np.zeros((0, 6)))
+    rays = dev(np.array([[0.0, 0.0, 0.0, 1.0, 0.0, 0.0]] * 5))
+    got = empty.raycast_nearest(rays, 10.0, point=True)
+    assert (host(got["leaf"]) == -1).all() and np.isinf(host(got["toi"])).all()
+    assert (host(got["point"]) == 0).all()
+    off, _, _ = empty.raycast_all(rays, 10.0)
+    assert (host(off) == 0).all()
+    assert empty.stats()["node_count"] == 0
+    empty.close()

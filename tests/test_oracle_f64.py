@@ -1,0 +1,173 @@
+"""Pins of the f64 oracle (oracle/sq_oracle64.cpp: the reference's `f64`
+feature, sq/src/maths/point.rs:14-18), on the CPU:
+
+  * closed forms: slab entries and face normals of axis-aligned rays, the ball's
+    entry root from outside and exit root from inside, a tangent ray, a miss;
+  * dyadic scenes, where every slab and centroid operation is exact in both
+    precisions: the f64 naive scans equal the f32 oracle's BVH walk bit for bit
+    (nearest ids and toi, all hits, overlap, filters) and the f64 tree equals
+    the f32 tree (node count, leaves, depth, surface sum);
+  * random balls against an independent float128-free root solver (numpy
+    float64 with the cancellation-free quadratic form): same nearest ids where
+    the two nearest candidates are separated, toi within 1e-12 relative, and
+    every nearest toi is a root (|at(t) - c| = r);
+  * all hits: sorted by (toi, leaf), and the first hit is the nearest.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+
+def test_closed_forms():
+    box = np.array([[0.0, 0.0, 0.0, 1.0, 1.0, 1.0]])
+    s = oracle.Scene64(3, oracle.BOX, box)
+    rays = np.array([[-5.0, 0.5, 0.5, 1.0, 0.0, 0.0],
+                     [0.5, 0.5, 7.0, 0.0, 0.0, -1.0],
+                     [0.5, 0.5, 0.5, 0.0, 1.0, 0.0],    # inside: entry 0, no entry axis
+                     [-5.0, 3.0, 0.5, 1.0, 0.0, 0.0]])  # parallel slab outside: miss
+    w = s.raycast_nearest(rays, 100.0)
+    assert list(w["leaf"]) == [0, 0, 0, -1]
+    assert list(w["t"][:3]) == [5.0, 6.0, 0.0] and np.isinf(w["t"][3])
+    assert np.array_equal(w["normal"][0], [-1.0, 0.0, 0.0])
+    assert np.array_equal(w["normal"][1], [0.0, 0.0, 1.0])
+    assert np.array_equal(w["normal"][2], [0.0, 0.0, 0.0])
+    assert np.array_equal(w["point"][1], [0.5, 0.5, 1.0])
+    # max_toi cuts a hit at 5 but not at exactly 5
+    assert s.raycast_nearest(rays[:1], 4.999)["leaf"][0] == -1
+    assert s.raycast_nearest(rays[:1], 5.0)["leaf"][0] == 0
+
+    ball = np.array([[10.0, 0.0, 0.0, 2.0]])
+    b = oracle.Scene64(3, oracle.SPHERE, ball)
+    rays = np.array([[0.0, 0.0, 0.0, 1.0, 0.0, 0.0],     # entry root 8
+                     [10.0, 0.0, 0.0, 0.0, 1.0, 0.0],    # from the centre: exit root 2
+                     [0.0, 2.0, 0.0, 1.0, 0.0, 0.0],     # tangent at (10, 2, 0): t = 10
+                     [0.0, 2.5, 0.0, 1.0, 0.0, 0.0],     # miss
+                     [20.0, 0.0, 0.0, 1.0, 0.0, 0.0]])   # behind the ray: miss
+    w = b.raycast_nearest(rays, 100.0)
+    assert list(w["leaf"]) == [0, 0, 0, -1, -1]
+    assert list(w["t"][:3]) == [8.0, 2.0, 10.0]
+    assert np.array_equal(w["normal"][0], [-1.0, 0.0, 0.0])
+    assert np.array_equal(w["normal"][1], [0.0, 1.0, 0.0])
+    # overlap: the ball's nearest-point test, not its box
+    regions = np.array([[11.5, 1.5, -0.1, 12.0, 2.0, 0.1],    # box corner region outside the ball
+                        [11.0, 1.0, -0.1, 12.0, 2.0, 0.1],    # distance sqrt(2) < 2
+                        [12.0, -1.0, -1.0, 13.0, 1.0, 1.0]])  # touching at x = 12
+    off, leaf = b.overlap(regions)
+    assert list(off) == [0, 0, 1, 2] and list(leaf) == [0, 0]
+
+
+def dyadic_boxes(n, dim, seed, span):
+    rng = np.random.default_rng(seed)
+    lo = rng.integers(-span, span, (n, dim)) / 8.0
+    ext = rng.integers(1, 24, (n, dim)) / 8.0
+    return np.concatenate([lo, lo + ext], 1)
+
+
+def dyadic_rays(n, dim, seed, span, pad, zero_dirs=True):
+    rng = np.random.default_rng(seed)
+    o = rng.integers(-span - pad, span + pad, (n, dim)) / 8.0
+    d = rng.choice([-1.0, -0.5, -0.25, 0.0, 0.25, 0.5, 1.0] if zero_dirs else [-1.0, -0.5, -0.25, 0.25, 0.5, 1.0], (n, dim))
+    d[np.all(d == 0, axis=1), 0] = 1.0
+    return np.concatenate([o, d], 1)
+
+
+@pytest.mark.parametrize("dim", [2, 3, 5])
+def test_dyadic_box_scenes_equal_the_f32_oracle(dim):
+    span, pad = {2: (256, 64), 3: (256, 64), 5: (24, 16)}[dim]   # denser in 5-D, so rays still hit
+    boxes = dyadic_boxes(3000, dim, 11 + dim, span)
+    rays = dyadic_rays(3000, dim, 21 + dim, span, pad, zero_dirs=dim < 5)
+    layers = np.random.default_rng(5).integers(0, 4, 3000).astype(np.uint64)
+    s64 = oracle.Scene64(dim, oracle.BOX, boxes, layers=layers)
+    s32 = oracle.Scene(dim, oracle.BOX, boxes.astype(np.float32), layers=layers)
+    for mask, exclude in [(oracle.ALL_LAYERS, -1), (0b10, 17)]:
+        w64 = s64.raycast_nearest(rays, 50.0, mask=mask, exclude=exclude)
+        w32 = s32.raycast_nearest(rays.astype(np.float32), 50.0, mask=mask, exclude=exclude)
+        assert (w32["leaf"] >= 0).sum() > 500
+        assert np.array_equal(w64["leaf"], w32["leaf"])
+        assert np.array_equal(w64["t"], w32["t"].astype(np.float64))
+        hit = w32["leaf"] >= 0
+        assert np.array_equal(w64["normal"][hit], w32["normal"][hit].astype(np.float64))
+    a64 = s64.raycast_all(rays[:800], 50.0)
+    a32 = s32.raycast_all(rays[:800].astype(np.float32), 50.0)
+    assert np.array_equal(a64[0], a32[0].astype(np.uint64)) and np.array_equal(a64[1], a32[1])
+    assert np.array_equal(a64[2], a32[2].astype(np.float64))
+    regions = np.concatenate([boxes[:500, :dim] - 1.0, boxes[:500, :dim] + 0.5], 1)
+    o64 = s64.overlap(regions, mask=0b1)
+    o32 = s32.overlap(regions.astype(np.float32), mask=0b1)
+    assert np.array_equal(o64[0], o32[0].astype(np.uint64)) and np.array_equal(o64[1], o32[1])
+    # the f64 median split builds the f32 tree when every centroid is exact
+    t64, t32 = s64.tree_stats(), s32.tree_stats()
+    for k in ("node_count", "leaf_count", "prim_count", "max_depth"):
+        assert t64[k] == t32[k], k
+    # same boxes, so the same surface sum, up to the f32 sum's rounding
+    assert t64["quality"] == pytest.approx(float(t32["quality"]), rel=1e-5)
+    assert t64["bytes"] == t64["node_count"] * (16 * dim + 16) + 3000 * 4
+
+
+def independent_ball_roots(o, d, c, r):
+    """Both roots of |o + t d - c| = r by the cancellation-free form (not the
+    oracle's -b -+ sqrt(disc)), nan where the ray misses."""
+    oc = o - c
+    a = np.sum(d * d, -1)
+    b = np.sum(oc * d, -1)
+    cc = np.sum(oc * oc, -1) - r * r
+    disc = b * b - a * cc
+    sq = np.sqrt(np.where(disc >= 0, disc, np.nan))
+    q = -(b + np.copysign(sq, b))
+    t0, t1 = q / a, cc / q
+    return np.minimum(t0, t1), np.maximum(t0, t1)
+
+
+def test_random_balls_against_an_independent_solver():
+    rng = np.random.default_rng(7)
+    n, nr = 600, 1500
+    balls = np.concatenate([rng.uniform(-50, 50, (n, 3)), rng.uniform(0.5, 3.0, (n, 1))], 1)
+    o = rng.uniform(-60, 60, (nr, 3))
+    d = rng.normal(size=(nr, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1)
+    w = oracle.Scene64(3, oracle.SPHERE, balls).raycast_nearest(rays, 500.0)
+    lo, hi = independent_ball_roots(o[:, None, :], d[:, None, :], balls[None, :, :3], balls[None, :, 3])
+    t = np.where(lo >= 0, lo, hi)
+    t = np.where((t >= 0) & (t <= 500.0), t, np.inf)
+    best = np.argmin(t, 1)
+    tb = t[np.arange(nr), best]
+    srt = np.sort(t, 1)
+    with np.errstate(invalid="ignore"):
+        separated = (srt[:, 1] - srt[:, 0]) > 1e-9 * np.maximum(1.0, srt[:, 0])
+    hit = np.isfinite(tb)
+    assert hit.sum() > 300
+    ok = separated & hit
+    assert np.array_equal(w["leaf"][ok], best[ok])
+    assert np.allclose(w["t"][ok], tb[ok], rtol=1e-12, atol=0)
+    assert (w["leaf"][~hit] == -1).all()
+    h = w["leaf"] >= 0
+    c = balls[w["leaf"][h]]
+    p = o[h] + d[h] * w["t"][h][:, None]
+    assert np.allclose(np.linalg.norm(p - c[:, :3], axis=1), c[:, 3], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", [oracle.BOX, oracle.SPHERE])
+def test_all_hits_sorted_and_headed_by_the_nearest(kind):
+    rng = np.random.default_rng(9 + kind)
+    dim = 4
+    if kind == oracle.BOX:
+        lo = rng.uniform(-8, 8, (800, dim))
+        prims = np.concatenate([lo, lo + rng.uniform(0.5, 3.0, (800, dim))], 1)
+    else:
+        prims = np.concatenate([rng.uniform(-8, 8, (800, dim)), rng.uniform(0.5, 2.0, (800, 1))], 1)
+    rays = np.concatenate([rng.uniform(-8, 8, (700, dim)), rng.normal(size=(700, dim))], 1)
+    s = oracle.Scene64(dim, kind, prims)
+    off, leaf, t = s.raycast_all(rays, 80.0)
+    w = s.raycast_nearest(rays, 80.0)
+    off = off.astype(np.int64)
+    assert off[-1] > 300
+    for i in range(700):
+        a, b = off[i], off[i + 1]
+        if a == b:
+            assert w["leaf"][i] == -1
+            continue
+        keys = list(zip(t[a:b], leaf[a:b]))
+        assert keys == sorted(keys)
+        assert leaf[a] == w["leaf"][i] and t[a] == w["t"][i]
