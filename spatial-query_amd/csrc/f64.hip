@@ -16,13 +16,18 @@
 //   double box[4D] = a.min[D], a.max[D], b.min[D], b.max[D]
 //   uint4  ref     = {ref_a, ref_b, count_a, count_b}
 // where count 0 marks an interior child (ref = its pair index) and count > 0 a
-// leaf range [ref, ref + count) of the leaf-ordered primitives.  One lane per
-// query; the per-lane stack holds at most one entry per tree level.
+// leaf range [ref, ref + count) of the leaf-ordered primitives.  The per-lane
+// stack holds at most one entry per tree level.  Nearest and any-hit rays run
+// pooled: persistent waves whose lanes take the next ray from an atomic
+// counter as soon as their own ends, so a wave does not wait for its longest
+// walk; all-hits and overlap run one query per lane.
 #include "engine_internal.h"
+#include "pool.h"
 
 namespace {
 
 constexpr int F64_MAX_STACK = 64;   // tree levels the per-lane stack holds (median split: about log2(n) + 1)
+constexpr int F64_BLOCK = 128;      // threads per block of the pooled kernels (two waves)
 
 struct View64 {
   const double* boxes;      // [np][4D]
@@ -97,48 +102,113 @@ __device__ __forceinline__ bool accepts64(const View64& v, const Filter& f, uint
   return (lay & f.mask) != 0;
 }
 
-// Bvh::raycast_nearest (bvh.rs:51-124) and the any-hit idiom, one ray per lane.
+// Bvh::raycast_nearest (bvh.rs:51-124) and the any-hit idiom, pooled.  Each
+// loop iteration refills the wave's idle lanes from the counter (one atomic per
+// wave) and advances every walking lane by one pair node.  The wave leaves the
+// loop once the counter has passed n and none of its lanes walks; every wave
+// then retires once (pool_retire), the last one clearing the counter.
 template <int D, int KIND, bool ANY>
-__global__ __launch_bounds__(256) void k64_nearest(View64 v, const double* __restrict__ rays, uint64_t n, double max_toi,
-                                                   Filter f, int32_t* out_leaf, double* out_toi, double* out_point,
-                                                   double* out_normal, uint8_t* out_any) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const double* __restrict__ rays, uint32_t n,
+                                                        double max_toi, Filter f, uint32_t* counter, int32_t* out_leaf,
+                                                        double* out_toi, double* out_point, double* out_normal,
+                                                        uint8_t* out_any) {
   constexpr int PS = stride64<D, KIND>();
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t stk[F64_MAX_STACK];
+  float stt[F64_MAX_STACK];
   RayD<D, double> r;
-  ray_setup<D>(r, rays + i * 2 * D);
-  double best = max_toi;
-  uint32_t bl = NO_LEAF, bk = 0;
-  walk64<D>(
-      v,
-      [&](const double* mn, const double* mx, double& te) {
-        double tx; int ax;
-        return slab_exact<D>(r, mn, mx, best, te, tx, ax);
-      },
-      [&](uint32_t first, uint32_t count) {
-        for (uint32_t k = first; k < first + count; ++k) {
-          const uint32_t id = v.ids[k];
-          if (!accepts64(v, f, id)) continue;
-          double t;
-          if (!sqd::narrow<D, KIND>(r, v.prims + (size_t)k * PS, best, t)) continue;
-          if (!hit_before64(t, id, best, bl)) continue;
-          best = t; bl = id; bk = k;
-          if (ANY) return true;
-        }
-        return false;
-      },
-      [&](double entry_lb) { return entry_lb > best; });
-  if (ANY) { out_any[i] = bl != NO_LEAF; return; }
-  const bool hit = bl != NO_LEAF;
-  out_leaf[i] = hit ? (int32_t)bl : -1;
-  out_toi[i] = hit ? best : __builtin_huge_val();
-  double nrm[D];
-  if (hit && out_normal) hit_normal<D, KIND>(r, v.prims + (size_t)bk * PS, best, max_toi, nrm);
+  double best = 0.0;
+  uint32_t bl = NO_LEAF, bk = 0, node = 0, sp = 0, ray = 0;
+  bool active = false, drained = false;
+  // leaf range test: the nearest accepted hit so far; true ends an any-hit walk
+  auto test_leaf = [&](uint32_t first, uint32_t count) {
+    for (uint32_t k = first; k < first + count; ++k) {
+      const uint32_t id = v.ids[k];
+      if (!accepts64(v, f, id)) continue;
+      double t;
+      if (!sqd::narrow<D, KIND>(r, v.prims + (size_t)k * PS, best, t)) continue;
+      if (!hit_before64(t, id, best, bl)) continue;
+      best = t; bl = id; bk = k;
+      if (ANY) return true;
+    }
+    return false;
+  };
+  auto finish = [&]() {
+    active = false;
+    const bool hit = bl != NO_LEAF;
+    if (ANY) { out_any[ray] = hit; return; }
+    out_leaf[ray] = hit ? (int32_t)bl : -1;
+    out_toi[ray] = hit ? best : __builtin_huge_val();
+    double nrm[D];
+    if (hit && out_normal) hit_normal<D, KIND>(r, v.prims + (size_t)bk * PS, best, max_toi, nrm);
 #pragma unroll
-  for (int k = 0; k < D; ++k) {
-    if (out_point) out_point[i * D + k] = hit ? r.o[k] + r.d[k] * best : 0.0;   // Ray::at (maths/ray.rs)
-    if (out_normal) out_normal[i * D + k] = hit ? nrm[k] : 0.0;
+    for (int k = 0; k < D; ++k) {
+      if (out_point) out_point[(size_t)ray * D + k] = hit ? r.o[k] + r.d[k] * best : 0.0;   // Ray::at (maths/ray.rs)
+      if (out_normal) out_normal[(size_t)ray * D + k] = hit ? nrm[k] : 0.0;
+    }
+  };
+  while (true) {
+    if (!drained) {
+      const uint64_t idle = __ballot(!active);
+      if (idle) {
+        const int leader = __ffsll((unsigned long long)idle) - 1;
+        const uint32_t cnt = (uint32_t)__popcll(idle);
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(counter, cnt);
+        base = __shfl(base, leader);
+        if ((uint64_t)base + cnt >= n) drained = true;
+        if (!active) {
+          const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+          if (my < n) {
+            ray = my;
+            ray_setup<D>(r, rays + (size_t)my * 2 * D);
+            best = max_toi; bl = NO_LEAF; sp = 0;
+            double te, tx; int ax;
+            if (v.n_prims && slab_exact<D>(r, v.root, v.root + D, max_toi, te, tx, ax) && !v.root_count) {
+              node = v.root_ref;
+              active = true;
+            } else {
+              if (v.n_prims && v.root_count && slab_exact<D>(r, v.root, v.root + D, max_toi, te, tx, ax))
+                test_leaf(v.root_ref, v.root_count);
+              finish();
+            }
+          }
+        }
+      }
+    }
+    if (drained && !__ballot(active)) break;
+    if (!active) continue;
+    const double* b = v.boxes + (size_t)node * 4 * D;
+    const uint4 rf = v.refs[node];
+    double ta = 0.0, tb = 0.0, tx;
+    int ax;
+    bool ha = slab_exact<D>(r, b, b + D, best, ta, tx, ax);
+    bool hb = slab_exact<D>(r, b + 2 * D, b + 3 * D, best, tb, tx, ax);
+    bool stop = false;
+    if (ha && rf.z) { stop = test_leaf(rf.x, rf.z); ha = false; }
+    if (!stop && hb && rf.w) { stop = test_leaf(rf.y, rf.w); hb = false; }
+    if (stop) { finish(); continue; }
+    if (ha && hb) {
+      const bool a_first = ta <= tb;
+      // the entry stored rounded toward -inf, so the pop test never prunes a subtree the limit still reaches
+      stk[sp] = a_first ? rf.y : rf.x;
+      stt[sp] = __double2float_rd(a_first ? tb : ta);
+      ++sp;
+      node = a_first ? rf.x : rf.y;
+    } else if (ha) {
+      node = rf.x;
+    } else if (hb) {
+      node = rf.y;
+    } else {
+      bool found = false;
+      while (sp > 0) {
+        --sp;
+        if (!((double)stt[sp] > best)) { node = stk[sp]; found = true; break; }
+      }
+      if (!found) finish();
+    }
   }
+  pool_retire(counter);
 }
 
 // Bvh::raycast_all (bvh.rs:196-259) and Bvh::overlap (bvh.rs:418-456): one
@@ -389,17 +459,22 @@ static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi
   if (!b || (n && !rays)) return fail(SQ_ERR_INVALID, "NULL argument");
   if (n && !out_any && (!out_leaf || !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
   if (!n) return SQ_OK;
+  if (n >= (1ull << 31)) return fail(SQ_ERR_INVALID, "more than 2^31 rays in one call");
   sq_ctx* c = b->ctx;
   const Filter fd = filter64(b, f);
+  uint32_t* counter = c->pool_ctr.as<uint32_t>();
   return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
-    if (out_any)
-      k64_nearest<D, KIND, true><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, rays, n, max_toi, fd, nullptr, nullptr,
-                                                                           nullptr, nullptr, out_any);
-    else
-      k64_nearest<D, KIND, false><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, rays, n, max_toi, fd, out_leaf, out_toi,
-                                                                            out_point, out_normal, nullptr);
-    HIP_TRY(hipGetLastError());
-    return SQ_OK;
+    auto launch = [&]<bool ANY>() -> int {
+      const void* kern = reinterpret_cast<const void*>(&k64_nearest_pool<D, KIND, ANY>);
+      int per_cu = 0;
+      if (int r0 = blocks_per_cu(kern, c->device, F64_BLOCK, 0, &per_cu)) return r0;
+      const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), grid_for(n, F64_BLOCK)));
+      k64_nearest_pool<D, KIND, ANY><<<(uint32_t)grid, F64_BLOCK, 0, c->stream>>>(
+          b->view, rays, (uint32_t)n, max_toi, fd, counter, out_leaf, out_toi, out_point, out_normal, out_any);
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    };
+    return out_any ? launch.template operator()<true>() : launch.template operator()<false>();
   });
 }
 

@@ -5,6 +5,7 @@
     python -m sq_amd.benches build
     python -m sq_amd.benches consumer
     python -m sq_amd.benches dispatch
+    python -m sq_amd.benches f64
 
 c2: 16 noise-displaced icospheres (2k-50k triangles, log-uniform) placed as
 4,096 instances (uniform rotations, translations U[-1000,1000]^3, scales
@@ -456,9 +457,80 @@ def dispatch():
     return out
 
 
+def f64():
+    """The `f64` feature against FP32 on the same scenes: 1,000,000 balls in
+    D = 3 (centres U[-1000,1000]^3, radii U[0.5,4]) with 4M rays, and the C4 D = 6
+    box scene with its 1M rays; nearest and any-hit rays, plus all hits and
+    overlap on the ball scene.  Kernel times from events on the query stream."""
+    import torch
+    import sq_amd
+    from sq_amd import workloads as W
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        return best
+
+    ctx = sq_amd.Context(0)
+    out = dict(workload="f64 vs f32: 1M balls D=3 with 4M rays; C4 (1M GMM boxes D=6) with 1M rays")
+    rng = np.random.default_rng(0xF64B)
+    n, nr = 1_000_000, 1 << 22
+    balls = np.concatenate([rng.uniform(-1000, 1000, (n, 3)), rng.uniform(0.5, 4.0, (n, 1))], 1)
+    o = rng.uniform(-1000, 1000, (nr, 3))
+    d = rng.normal(size=(nr, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1)
+    scenes = [("balls", 3, sq_amd.SPHERE, balls, rays, 4000.0)]
+    sc = W.gmm_boxes_scene()
+    c4_rays, _, _, _ = W.c4_queries(sc)
+    scenes.append(("c4_boxes", 6, sq_amd.AABB, sc.prims.astype(np.float64), c4_rays.astype(np.float64), sc.max_toi))
+    for name, dim, kind, prims, q, max_toi in scenes:
+        t0 = time.time()
+        b64 = sq_amd.DeviceBvh64(ctx, dim, kind, prims)
+        build64 = time.time() - t0
+        b32 = sq_amd.DeviceBvh.build(ctx, dim, kind, prims.astype(np.float32))
+        q64 = torch.from_numpy(np.ascontiguousarray(q)).cuda()
+        q32 = q64.float().contiguous()
+        res = {}
+        ms64 = timed(lambda: res.__setitem__("f64", b64.raycast_nearest(q64, max_toi)))
+        ms32 = timed(lambda: res.__setitem__("f32", b32.raycast_nearest(q32, max_toi)))
+        any64 = timed(lambda: b64.raycast_any(q64, max_toi))
+        any32 = timed(lambda: b32.raycast_any(q32, max_toi))
+        l64, l32 = res["f64"]["leaf"].cpu().numpy(), res["f32"]["leaf"].cpu().numpy()
+        m = q.shape[0] / 1e6
+        out[name] = dict(prims=prims.shape[0], rays=q.shape[0], build64_s=round(build64, 2),
+                         nearest_f64_mrays=round(m / (ms64 * 1e-3), 1), nearest_f32_mrays=round(m / (ms32 * 1e-3), 1),
+                         any_f64_mrays=round(m / (any64 * 1e-3), 1), any_f32_mrays=round(m / (any32 * 1e-3), 1),
+                         f64_over_f32=round(ms32 / ms64, 3), hit_fraction=round(float((l64 >= 0).mean()), 4),
+                         same_leaf_as_f32=round(float((l64 == l32).mean()), 6))
+        if name == "balls":
+            sub = q64[: 1 << 20].contiguous()
+            all_ms = timed(lambda: b64.raycast_all(sub, max_toi), reps=3)
+            off, _, _ = b64.raycast_all(sub, max_toi)
+            h = rng.uniform(2.0, 20.0, (1 << 18, 3))
+            c = rng.uniform(-1000, 1000, (1 << 18, 3))
+            reg = torch.from_numpy(np.concatenate([c - h, c + h], 1)).cuda()
+            ov_ms = timed(lambda: b64.overlap(reg), reps=3)
+            off2, _ = b64.overlap(reg)
+            out[name].update(all_hits_1m_rays_ms=round(all_ms, 3), all_hits_entries=int(off[-1].item()),
+                             overlap_262k_ms=round(ov_ms, 3), overlap_pairs=int(off2[-1].item()))
+        log(f"{name}: f64 {out[name]['nearest_f64_mrays']} Mrays/s, f32 {out[name]['nearest_f32_mrays']} Mrays/s")
+        b64.close()
+        b32.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["c2", "c4", "build", "consumer", "dispatch"])
+    ap.add_argument("which", choices=["c2", "c4", "build", "consumer", "dispatch", "f64"])
     ap.add_argument("--frames", type=int, default=60)
     ap.add_argument("--batches", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
@@ -472,6 +544,8 @@ def main():
         res = build()
     elif a.which == "consumer":
         res = consumer()
+    elif a.which == "f64":
+        res = f64()
     else:
         res = dispatch()
     print(json.dumps(res), flush=True)
