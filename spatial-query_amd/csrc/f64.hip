@@ -102,6 +102,33 @@ __device__ __forceinline__ bool accepts64(const View64& v, const Filter& f, uint
   return (lay & f.mask) != 0;
 }
 
+// Pruning slab of both children of a pair node, branch-free per axis.  It
+// never rejects a box the exact slab (slab_exact, aabb.rs:118-148) accepts:
+// min/max replace the compare-and-swap, which changes no pass/fail for
+// non-parallel axes, and a parallel axis whose origin lies outside rejects
+// outright.  Only the pass/fail and the order of the two entries are used;
+// primitives are always tested exactly.
+template <int D>
+__device__ __forceinline__ void slab_pair64(const RayD<D, double>& r, const double* nd, double limit, bool& ha, bool& hb,
+                                            double& ta, double& tb) {
+  double amin = 0.0, amax = limit, bmin = 0.0, bmax = limit;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const double o = r.o[k];
+    if (r.par & (1u << k)) {
+      if (o < nd[k] || o > nd[D + k]) { amin = __builtin_huge_val(); amax = -__builtin_huge_val(); }
+      if (o < nd[2 * D + k] || o > nd[3 * D + k]) { bmin = __builtin_huge_val(); bmax = -__builtin_huge_val(); }
+    } else {
+      const double iv = r.inv[k];
+      const double a1 = (nd[k] - o) * iv, a2 = (nd[D + k] - o) * iv;
+      const double b1 = (nd[2 * D + k] - o) * iv, b2 = (nd[3 * D + k] - o) * iv;
+      amin = fmax(amin, fmin(a1, a2)); amax = fmin(amax, fmax(a1, a2));
+      bmin = fmax(bmin, fmin(b1, b2)); bmax = fmin(bmax, fmax(b1, b2));
+    }
+  }
+  ha = amin <= amax; hb = bmin <= bmax; ta = amin; tb = bmin;
+}
+
 // Bvh::raycast_nearest (bvh.rs:51-124) and the any-hit idiom, pooled.  Each
 // loop iteration refills the wave's idle lanes from the counter (one atomic per
 // wave) and advances every walking lane by one pair node.  The wave leaves the
@@ -180,10 +207,9 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
     if (!active) continue;
     const double* b = v.boxes + (size_t)node * 4 * D;
     const uint4 rf = v.refs[node];
-    double ta = 0.0, tb = 0.0, tx;
-    int ax;
-    bool ha = slab_exact<D>(r, b, b + D, best, ta, tx, ax);
-    bool hb = slab_exact<D>(r, b + 2 * D, b + 3 * D, best, tb, tx, ax);
+    double ta, tb;
+    bool ha, hb;
+    slab_pair64<D>(r, b, best, ha, hb, ta, tb);
     bool stop = false;
     if (ha && rf.z) { stop = test_leaf(rf.x, rf.z); ha = false; }
     if (!stop && hb && rf.w) { stop = test_leaf(rf.y, rf.w); hb = false; }
