@@ -28,6 +28,7 @@ namespace {
 
 constexpr int F64_MAX_STACK = 64;   // tree levels the per-lane stack holds (median split: about log2(n) + 1)
 constexpr int F64_BLOCK = 128;      // threads per block of the pooled kernels (two waves)
+constexpr uint32_t F64_RING = 8;    // stack entries per lane kept in LDS by the pooled kernels (RingStack)
 
 struct View64 {
   const double* boxes;      // [np][4D]
@@ -38,6 +39,7 @@ struct View64 {
   double root[12];          // root box: min[D], max[D]
   uint32_t root_ref, root_count;   // root_count > 0: the root is a leaf
   uint32_t n_prims;
+  uint32_t stack;           // stack entries a lane can need: the tree's depth
 };
 
 __device__ __forceinline__ int64_t total_key64(double f) {
@@ -138,11 +140,13 @@ template <int D, int KIND, bool ANY>
 __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const double* __restrict__ rays, uint32_t n,
                                                         double max_toi, Filter f, uint32_t* counter, int32_t* out_leaf,
                                                         double* out_toi, double* out_point, double* out_normal,
-                                                        uint8_t* out_any) {
+                                                        uint8_t* out_any, SpillEntry* spill, uint32_t lanes) {
   constexpr int PS = stride64<D, KIND>();
   const uint32_t lane = threadIdx.x & 63u;
-  uint32_t stk[F64_MAX_STACK];
-  float stt[F64_MAX_STACK];
+  // the newest F64_RING entries in LDS ([slot][thread] refs, then entries), older ones in this lane's spill column
+  extern __shared__ uint32_t f64_ring[];
+  const RingStack st{f64_ring + threadIdx.x, reinterpret_cast<float*>(f64_ring + F64_RING * blockDim.x) + threadIdx.x,
+                     blockDim.x, F64_RING, spill + (size_t)blockIdx.x * blockDim.x + threadIdx.x, lanes};
   RayD<D, double> r;
   double best = 0.0;
   uint32_t bl = NO_LEAF, bk = 0, node = 0, sp = 0, ray = 0;
@@ -217,9 +221,7 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
     if (ha && hb) {
       const bool a_first = ta <= tb;
       // the entry stored rounded toward -inf, so the pop test never prunes a subtree the limit still reaches
-      stk[sp] = a_first ? rf.y : rf.x;
-      stt[sp] = __double2float_rd(a_first ? tb : ta);
-      ++sp;
+      st.push(sp, a_first ? rf.y : rf.x, __double2float_rd(a_first ? tb : ta));
       node = a_first ? rf.x : rf.y;
     } else if (ha) {
       node = rf.x;
@@ -228,8 +230,10 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
     } else {
       bool found = false;
       while (sp > 0) {
-        --sp;
-        if (!((double)stt[sp] > best)) { node = stk[sp]; found = true; break; }
+        uint32_t e;
+        float lb;
+        st.pop(sp, e, lb);
+        if (!((double)lb > best)) { node = e; found = true; break; }
       }
       if (!found) finish();
     }
@@ -374,6 +378,7 @@ int sq_bvh64_create(sq_ctx* c, int dim, int kind, const double* prims, uint64_t 
   uint64_t leaves = 0, nprims = 0, depth = 0;
   sqh::tree_structural(t, &leaves, &nprims, &depth);
   if (depth > F64_MAX_STACK) return fail(SQ_ERR_UNSUPPORTED, "tree deeper than the f64 walk's stack");
+  b->view.stack = (uint32_t)depth;
   // TreeStats (tree.rs:81-110, bvh.rs:477-489) of the f64 tree: size_of::<Node<D>>() is 16D + 12 bytes
   // padded to the f64 alignment; the interior surface sum is summed in double and reported as float
   b->stats.node_count = t.node_count();
@@ -492,11 +497,17 @@ static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi
   return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
     auto launch = [&]<bool ANY>() -> int {
       const void* kern = reinterpret_cast<const void*>(&k64_nearest_pool<D, KIND, ANY>);
+      const size_t lds = (size_t)F64_RING * F64_BLOCK * 8;
       int per_cu = 0;
-      if (int r0 = blocks_per_cu(kern, c->device, F64_BLOCK, 0, &per_cu)) return r0;
+      if (int r0 = blocks_per_cu(kern, c->device, F64_BLOCK, lds, &per_cu)) return r0;
       const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), grid_for(n, F64_BLOCK)));
-      k64_nearest_pool<D, KIND, ANY><<<(uint32_t)grid, F64_BLOCK, 0, c->stream>>>(
-          b->view, rays, (uint32_t)n, max_toi, fd, counter, out_leaf, out_toi, out_point, out_normal, out_any);
+      const uint64_t lanes = grid * F64_BLOCK;
+      // spill rows: the stack holds at most depth - 1 entries, F64_RING of them in LDS
+      const uint64_t rows = b->view.stack > F64_RING ? b->view.stack - F64_RING : 0;
+      if (int r1 = c->spill.ensure((size_t)rows * lanes * sizeof(SpillEntry) + 64)) return r1;
+      k64_nearest_pool<D, KIND, ANY><<<(uint32_t)grid, F64_BLOCK, lds, c->stream>>>(
+          b->view, rays, (uint32_t)n, max_toi, fd, counter, out_leaf, out_toi, out_point, out_normal, out_any,
+          c->spill.as<SpillEntry>(), (uint32_t)lanes);
       HIP_TRY(hipGetLastError());
       return SQ_OK;
     };
