@@ -12,9 +12,9 @@
 // interval), so a near-first walk answers what the reference's LIFO walk does.
 //
 // Device layout: the reference node array re-packed as pair nodes, both
-// children's boxes in one record,
+// children's boxes and refs in one 16-byte-aligned record of 4D + 2 doubles,
 //   double box[4D] = a.min[D], a.max[D], b.min[D], b.max[D]
-//   uint4  ref     = {ref_a, ref_b, count_a, count_b}
+//   uint4  ref     = {ref_a, ref_b, count_a, count_b}   (the last 16 bytes)
 // where count 0 marks an interior child (ref = its pair index) and count > 0 a
 // leaf range [ref, ref + count) of the leaf-ordered primitives.  The per-lane
 // stack holds at most one entry per tree level.  Nearest and any-hit rays run
@@ -31,8 +31,7 @@ constexpr int F64_BLOCK = 128;      // threads per block of the pooled kernels (
 constexpr uint32_t F64_RING = 8;    // stack entries per lane kept in LDS by the pooled kernels (RingStack)
 
 struct View64 {
-  const double* boxes;      // [np][4D]
-  const uint4* refs;        // [np]
+  const double* nodes;      // [np][4D + 2]: boxes, then the refs
   const double* prims;      // leaf-ordered, stride 2D (AABB) or D + 1 (SPHERE)
   const uint32_t* ids;      // original leaf id of each leaf-ordered primitive
   const uint64_t* layers;   // by original leaf id, or null
@@ -69,8 +68,8 @@ __device__ __forceinline__ void walk64(const View64& v, NodePass&& node_pass, Le
   int sp = 0;
   uint32_t node = v.root_ref;
   while (true) {
-    const double* b = v.boxes + (size_t)node * 4 * D;
-    const uint4 rf = v.refs[node];
+    const double* b = v.nodes + (size_t)node * (4 * D + 2);
+    const uint4 rf = *reinterpret_cast<const uint4*>(b + 4 * D);
     double ta = 0.0, tb = 0.0;
     bool ha = node_pass(b, b + D, ta);
     bool hb = node_pass(b + 2 * D, b + 3 * D, tb);
@@ -209,8 +208,8 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
     }
     if (drained && !__ballot(active)) break;
     if (!active) continue;
-    const double* b = v.boxes + (size_t)node * 4 * D;
-    const uint4 rf = v.refs[node];
+    const double* b = v.nodes + (size_t)node * (4 * D + 2);
+    const uint4 rf = *reinterpret_cast<const uint4*>(b + 4 * D);
     double ta, tb;
     bool ha, hb;
     slab_pair64<D>(r, b, best, ha, hb, ta, tb);
@@ -334,7 +333,7 @@ struct sq_bvh64 {
   int dim = 0, kind = 0;
   uint64_t n_prims = 0;
   sqh::RefTreeT<double> tree;
-  DevBuf boxes, refs, prims, ids, layers;
+  DevBuf nodes, prims, ids, layers;
   View64 view{};
   sq_tree_stats stats{};
 };
@@ -356,7 +355,7 @@ extern "C" {
 int sq_bvh64_create(sq_ctx* c, int dim, int kind, const double* prims, uint64_t n, sq_bvh64** out) {
   if (!c || !out || (n && !prims)) return fail(SQ_ERR_INVALID, "NULL argument");
   *out = nullptr;
-  if (n > sqh::LEAF_FIRST_MASK) return fail(SQ_ERR_INVALID, "too many primitives");
+  if (n >= 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "too many primitives (ids are u32)");
   int rc = dispatch64(dim, kind, [&]<int D, int KIND>() -> int { return SQ_OK; });
   if (rc) return rc;
   const int ps = kind == KIND_AABB ? 2 * dim : dim + 1;
@@ -404,19 +403,21 @@ int sq_bvh64_create(sq_ctx* c, int dim, int kind, const double* prims, uint64_t 
   std::vector<uint32_t> pair_of(nn, 0);
   uint32_t np = 0;
   for (size_t i = 0; i < nn; ++i) if (t.prim_count[i] == 0) pair_of[i] = np++;
-  std::vector<double> pb((size_t)np * 4 * dim);
-  std::vector<uint32_t> pr((size_t)np * 4);
+  const size_t ns = 4 * (size_t)dim + 2;   // doubles per pair record
+  std::vector<double> pb((size_t)np * ns);
   for (size_t i = 0; i < nn; ++i) {
     if (t.prim_count[i] != 0) continue;
     const uint32_t p = pair_of[i];
     const uint32_t ch[2] = {t.child_a[i], t.child_b[i]};
+    uint32_t ref[4];
     for (int s = 0; s < 2; ++s) {
       const uint32_t cn = ch[s];
-      std::memcpy(pb.data() + ((size_t)p * 4 + 2 * s) * dim, t.bounds.data() + (size_t)cn * 2 * dim, 2 * dim * sizeof(double));
+      std::memcpy(pb.data() + (size_t)p * ns + 2 * s * dim, t.bounds.data() + (size_t)cn * 2 * dim, 2 * dim * sizeof(double));
       const bool leaf = t.prim_count[cn] != 0;
-      pr[(size_t)p * 4 + s] = leaf ? t.child_a[cn] : pair_of[cn];
-      pr[(size_t)p * 4 + 2 + s] = leaf ? t.prim_count[cn] : 0;
+      ref[s] = leaf ? t.child_a[cn] : pair_of[cn];
+      ref[2 + s] = leaf ? t.prim_count[cn] : 0;
     }
+    std::memcpy(pb.data() + (size_t)p * ns + 4 * dim, ref, sizeof(ref));
   }
   // leaf-ordered primitives and their original ids
   std::vector<double> lp((size_t)n * ps);
@@ -427,20 +428,16 @@ int sq_bvh64_create(sq_ctx* c, int dim, int kind, const double* prims, uint64_t 
     ids[k] = id;
   }
   HIP_TRY(hipSetDevice(c->device));
-  if ((rc = b->boxes.ensure(pb.size() * 8)) || (rc = b->refs.ensure(pr.size() * 4)) ||
-      (rc = b->prims.ensure(lp.size() * 8)) || (rc = b->ids.ensure(ids.size() * 4)))
+  if ((rc = b->nodes.ensure(pb.size() * 8)) || (rc = b->prims.ensure(lp.size() * 8)) ||
+      (rc = b->ids.ensure(ids.size() * 4)))
     return rc;
-  if (np) {
-    HIP_TRY(hipMemcpy(b->boxes.p, pb.data(), pb.size() * 8, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(b->refs.p, pr.data(), pr.size() * 4, hipMemcpyHostToDevice));
-  }
+  if (np) HIP_TRY(hipMemcpy(b->nodes.p, pb.data(), pb.size() * 8, hipMemcpyHostToDevice));
   if (n) {
     HIP_TRY(hipMemcpy(b->prims.p, lp.data(), lp.size() * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->ids.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice));
   }
   View64& v = b->view;
-  v.boxes = b->boxes.as<double>();
-  v.refs = b->refs.as<uint4>();
+  v.nodes = b->nodes.as<double>();
   v.prims = b->prims.as<double>();
   v.ids = b->ids.as<uint32_t>();
   v.layers = nullptr;
