@@ -488,6 +488,14 @@ int sq_bvh64_raycast_any(sq_bvh64* bvh, const double* rays, uint64_t n, double m
 /* Bvh::raycast_all (bvh.rs:196-259), CSR as sq_raycast_all. */
 int sq_bvh64_raycast_all(sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi, const sq_filter* filter,
                          uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total);
+/* Bvh::shapecast_nearest / shapecast_all (bvh.rs:331-414): casts double[n][4D]
+ * (origin, dir, probe box min, max) as sq_shapecast_nearest / _all, with one
+ * max_toi for every cast; AABB leaves answer the grown-box slab entry, SPHERE
+ * leaves the ball of radius r + probe_max[0]. */
+int sq_bvh64_shapecast_nearest(sq_bvh64* bvh, const double* casts, uint64_t n, double max_toi, const sq_filter* filter,
+                               int32_t* out_leaf, double* out_toi, double* out_point);
+int sq_bvh64_shapecast_all(sq_bvh64* bvh, const double* casts, uint64_t n, double max_toi, const sq_filter* filter,
+                           uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total);
 /* Bvh::overlap (bvh.rs:418-456): regions double[n][2D], CSR as sq_overlap. */
 int sq_bvh64_overlap(sq_bvh64* bvh, const double* regions, uint64_t n, const sq_filter* filter, uint64_t* offsets,
                      int32_t* out_leaf, uint64_t capacity, uint64_t* total);

@@ -79,6 +79,8 @@ def _declare(L):
         "orc64_raycast_all": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, I32, P, P, P, U64]),
         "orc64_overlap": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, I32, P, P, U64]),
         "orc64_tree_stats": (None, [I32, I32, P, U32, P, P]),
+        "orc64_shapecast_nearest": (None, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, I32, P, P, P]),
+        "orc64_shapecast_all": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, I32, P, P, P, U64]),
         "orc_inst_new": (P, [I32, I32, U64, P, P, P, U32, P, P, P, P]),
         "orc_inst_set_transforms": (None, [P, P, P, P]),
         "orc_tlas_refit": (None, [P]),
@@ -340,6 +342,21 @@ class Scene64:
                                                           mask, exclude, _p(r), r.shape[0], threads, _p(off), _p(leaf), cap)
         off, leaf, _ = self._collect(fn, r, False)
         return off, leaf
+
+    def shapecast_nearest(self, casts, max_toi, mask=ALL_LAYERS, exclude=-1, threads=8):
+        c = np.ascontiguousarray(casts, np.float64).reshape(-1, 4 * self.dim)
+        n = c.shape[0]
+        leaf = np.empty(n, np.int64); t = np.empty(n); pt = np.empty((n, self.dim))
+        lib().orc64_shapecast_nearest(self.dim, self.kind, _p(self.prims), self.n, _p(self.layers), mask, exclude, _p(c), n,
+                                      float(max_toi), threads, _p(leaf), _p(t), _p(pt))
+        return dict(leaf=leaf, t=t, point=pt)
+
+    def shapecast_all(self, casts, max_toi, mask=ALL_LAYERS, exclude=-1, threads=8):
+        c = np.ascontiguousarray(casts, np.float64).reshape(-1, 4 * self.dim)
+        fn = lambda off, leaf, t, cap: lib().orc64_shapecast_all(self.dim, self.kind, _p(self.prims), self.n,
+                                                                _p(self.layers), mask, exclude, _p(c), c.shape[0],
+                                                                float(max_toi), threads, _p(off), _p(leaf), _p(t), cap)
+        return self._collect(fn, c, True)
 
     def tree_stats(self):
         out = np.zeros(5, np.uint64); q = np.zeros(1)

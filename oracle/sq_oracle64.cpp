@@ -9,7 +9,9 @@
 //   * Aabb::intersects             sq/src/maths/aabb.rs:101-108
 //   * the naive scans              sq/src/query/naive.rs:34-62 (nearest),
 //                                  :65-109 (all hits, sorted by hit_ordering),
+//                                  :131-181 (shape casts, nearest and all),
 //                                  :184-205 (overlap)
+//   * ShapeCast broadphase         sq/src/query/shapecast.rs:70-73 (box grown by the probe)
 //   * hit_ordering                 sq/src/query/hit.rs:77-84 (f64::total_cmp, then leaf)
 //   * tree::build and TreeStats    sq/src/accel/tree.rs:39-49,81-163
 // over the built-in BOX and SPHERE providers (box-guarded sphere, as the
@@ -116,6 +118,41 @@ template <int D> struct Geom {
     for (int i = 0; i < D; ++i) normal[i] = len > F64_EPSILON ? v[i] * (1.0 / len) : 0.0;
     return true;
   }
+  // shapecast.rs:70-73: the slab against `mn, mx` grown by the probe box
+  bool broadphase(const double* mn, const double* mx, const Ray& r, const double* pmin, const double* pmax, double max_toi,
+                  double* te, double* tx) const {
+    double gmn[D], gmx[D];
+    for (int i = 0; i < D; ++i) { gmn[i] = mn[i] - pmax[i]; gmx[i] = mx[i] - pmin[i]; }
+    int ax;
+    return ray_intersect<D>(gmn, gmx, r, max_toi, te, tx, &ax);
+  }
+  // test_shape_cast: BOX the grown slab's entry; SPHERE the ball of radius
+  // r + probe.max[0] (shape_overlap.rs:104-118), guarded by the grown box
+  bool test_cast(size_t leaf, const Ray& r, const double* pmin, const double* pmax, double max_toi, double* t_out) const {
+    const double* q = p(leaf);
+    double mn[D], mx[D], te, tx;
+    world_aabb(leaf, mn, mx);
+    if (kind == 0) {
+      if (!broadphase(mn, mx, r, pmin, pmax, max_toi, &te, &tx)) return false;
+      *t_out = te;
+      return true;
+    }
+    double oc[D];
+    for (int i = 0; i < D; ++i) oc[i] = r.o[i] - q[i];
+    const double rad = q[D] + pmax[0];
+    const double b = dot<D>(oc, r.d);
+    const double cc = dot<D>(oc, oc) - rad * rad;
+    const double disc = b * b - cc;
+    if (disc < 0.0) return false;
+    const double sq = std::sqrt(disc);
+    double t = -b - sq;
+    if (t < 0.0) t = -b + sq;
+    if (t < 0.0 || t > max_toi) return false;
+    if (!broadphase(mn, mx, r, pmin, pmax, max_toi, &te, &tx)) return false;
+    if (!(te <= t && t <= tx)) return false;
+    *t_out = t;
+    return true;
+  }
   // test_overlap: the box test; the ball's nearest region point within r
   bool test_overlap(size_t leaf, const double* rmn, const double* rmx) const {
     const double* q = p(leaf);
@@ -182,6 +219,45 @@ static void naive_all(const Geom<D>& g, const Ray& r, double max_toi, uint64_t m
     if (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &ax)) continue;
     double t;
     if (g.test_ray(leaf, r, max_toi, &t, nrm)) out.push_back({t, (int64_t)leaf});
+  }
+  std::stable_sort(out.begin(), out.end(),
+                   [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
+                     return hit_less(a.first, a.second, b.first, b.second);
+                   });
+}
+
+// naive.rs:131-158 (nearest) and :161-181 (all, sorted by hit_ordering)
+template <int D>
+static bool naive_cast_nearest(const Geom<D>& g, const Ray& r, const double* pmin, const double* pmax, double max_toi,
+                               uint64_t mask, int64_t exclude, int64_t* leaf_out, double* t_out) {
+  bool have = false;
+  double limit = max_toi, best_t = 0.0;
+  int64_t best_leaf = -1;
+  for (size_t leaf = 0; leaf < g.n; ++leaf) {
+    if (!g.accepts(leaf, mask, exclude)) continue;
+    double mn[D], mx[D], te, tx;
+    g.world_aabb(leaf, mn, mx);
+    if (!g.broadphase(mn, mx, r, pmin, pmax, limit, &te, &tx)) continue;
+    double t;
+    if (!g.test_cast(leaf, r, pmin, pmax, limit, &t)) continue;
+    if (have && !hit_less(t, (int64_t)leaf, best_t, best_leaf)) continue;
+    have = true; best_t = t; best_leaf = (int64_t)leaf; limit = t;
+  }
+  *leaf_out = best_leaf; *t_out = best_t;
+  return have;
+}
+
+template <int D>
+static void naive_cast_all(const Geom<D>& g, const Ray& r, const double* pmin, const double* pmax, double max_toi,
+                           uint64_t mask, int64_t exclude, std::vector<std::pair<double, int64_t>>& out) {
+  out.clear();
+  for (size_t leaf = 0; leaf < g.n; ++leaf) {
+    if (!g.accepts(leaf, mask, exclude)) continue;
+    double mn[D], mx[D], te, tx;
+    g.world_aabb(leaf, mn, mx);
+    if (!g.broadphase(mn, mx, r, pmin, pmax, max_toi, &te, &tx)) continue;
+    double t;
+    if (g.test_cast(leaf, r, pmin, pmax, max_toi, &t)) out.push_back({t, (int64_t)leaf});
   }
   std::stable_sort(out.begin(), out.end(),
                    [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
@@ -311,13 +387,35 @@ static void nearest_batch(int kind, const double* prims, uint32_t n, const uint6
 }
 
 template <int D>
+static void cast_nearest_batch(int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
+                               int64_t exclude, const double* casts, uint64_t nc, double max_toi, int threads, int64_t* leaf,
+                               double* t, double* point) {
+  Geom<D> g{kind, n, prims, layers};
+  parallel_for(nc, threads, [&](uint64_t i) {
+    const double* c = casts + i * 4 * D;
+    Ray r{c, c + D};
+    int64_t l; double tt;
+    const bool have = naive_cast_nearest<D>(g, r, c + 2 * D, c + 3 * D, max_toi, mask, exclude, &l, &tt);
+    leaf[i] = have ? l : -1;
+    t[i] = have ? tt : INFINITY;
+    for (int k = 0; k < D; ++k)
+      if (point) point[i * D + k] = have ? r.o[k] + r.d[k] * tt : 0.0;   // ShapeCast::at
+  });
+}
+
+// mode 0: rays, 1: overlap regions, 2: shape casts
+template <int D>
 static uint64_t all_batch(int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask, int64_t exclude,
-                          const double* q, uint64_t nq, double max_toi, bool overlap, int threads, uint64_t* offsets,
+                          const double* q, uint64_t nq, double max_toi, int mode, int threads, uint64_t* offsets,
                           int64_t* leaf, double* t, uint64_t cap) {
   Geom<D> g{kind, n, prims, layers};
   std::vector<std::vector<std::pair<double, int64_t>>> res(nq);
   parallel_for(nq, threads, [&](uint64_t i) {
-    if (overlap) {
+    if (mode == 2) {
+      const double* c = q + i * 4 * D;
+      Ray r{c, c + D};
+      naive_cast_all<D>(g, r, c + 2 * D, c + 3 * D, max_toi, mask, exclude, res[i]);
+    } else if (mode == 1) {
       std::vector<int64_t> ids;
       naive_overlap<D>(g, q + i * 2 * D, q + i * 2 * D + D, mask, exclude, ids);
       for (int64_t id : ids) res[i].push_back({0.0, id});
@@ -363,7 +461,7 @@ uint64_t orc64_raycast_all(int D, int kind, const double* prims, uint32_t n, con
                            int64_t exclude, const double* rays, uint64_t nr, double max_toi, int threads,
                            uint64_t* offsets, int64_t* leaf, double* t, uint64_t cap) {
   uint64_t tot = 0;
-  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, rays, nr, max_toi, false, threads, offsets, leaf, t, cap));
+  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, rays, nr, max_toi, 0, threads, offsets, leaf, t, cap));
   return tot;
 }
 
@@ -371,7 +469,21 @@ uint64_t orc64_overlap(int D, int kind, const double* prims, uint32_t n, const u
                        int64_t exclude, const double* regions, uint64_t nq, int threads, uint64_t* offsets, int64_t* leaf,
                        uint64_t cap) {
   uint64_t tot = 0;
-  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, regions, nq, 0.0, true, threads, offsets, leaf, nullptr, cap));
+  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, regions, nq, 0.0, 1, threads, offsets, leaf, nullptr, cap));
+  return tot;
+}
+
+void orc64_shapecast_nearest(int D, int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
+                             int64_t exclude, const double* casts, uint64_t nc, double max_toi, int threads, int64_t* leaf,
+                             double* t, double* point) {
+  ORC64_DISPATCH(D, cast_nearest_batch<DD>(kind, prims, n, layers, mask, exclude, casts, nc, max_toi, threads, leaf, t, point));
+}
+
+uint64_t orc64_shapecast_all(int D, int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
+                             int64_t exclude, const double* casts, uint64_t nc, double max_toi, int threads,
+                             uint64_t* offsets, int64_t* leaf, double* t, uint64_t cap) {
+  uint64_t tot = 0;
+  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, casts, nc, max_toi, 2, threads, offsets, leaf, t, cap));
   return tot;
 }
 

@@ -1,6 +1,6 @@
 // f64.hip -- the reference's `f64` feature on the device: a Bvh over
 // double-precision AABB and SPHERE leaves (D = 2..6) answering nearest-hit,
-// any-hit, all-hits and overlap queries.
+// any-hit, all-hits, shape-cast (nearest and all) and overlap queries.
 //
 // Under the feature the reference's Scalar is f64 everywhere
 // (crates/spatial-query/src/maths/point.rs:14-18): tree bounds, rays, the slab
@@ -17,10 +17,10 @@
 //   uint4  ref     = {ref_a, ref_b, count_a, count_b}   (the last 16 bytes)
 // where count 0 marks an interior child (ref = its pair index) and count > 0 a
 // leaf range [ref, ref + count) of the leaf-ordered primitives.  The per-lane
-// stack holds at most one entry per tree level.  Nearest and any-hit rays run
-// pooled: persistent waves whose lanes take the next ray from an atomic
+// stack holds at most one entry per tree level.  Nearest and any-hit rays and
+// nearest shape casts run pooled: persistent waves whose lanes take the next ray from an atomic
 // counter as soon as their own ends, so a wave does not wait for its longest
-// walk; all-hits and overlap run one query per lane.
+// walk; all hits, all cast contacts and overlap run one query per lane.
 #include "engine_internal.h"
 #include "pool.h"
 
@@ -130,12 +130,49 @@ __device__ __forceinline__ void slab_pair64(const RayD<D, double>& r, const doub
   ha = amin <= amax; hb = bmin <= bmax; ta = amin; tb = bmin;
 }
 
-// Bvh::raycast_nearest (bvh.rs:51-124) and the any-hit idiom, pooled.  Each
+// What a query is for the pooled walk: a ray (RayPolicy's tests in double) or
+// a shape cast (CastPolicy<.., double>: the slab against each box grown by the
+// probe box, shapecast.rs:70-73, and the kinds' cast leaf tests).
+template <int D, int KIND, bool CAST> struct Pol64;
+template <int D, int KIND> struct Pol64<D, KIND, false> {
+  static constexpr int QS = 2 * D;   // doubles per query
+  RayD<D, double> r;
+  __device__ void setup(const double* src) { ray_setup<D>(r, src); }
+  __device__ const RayD<D, double>& ray() const { return r; }
+  __device__ bool root(const double* rb, double limit) const {
+    double te, tx; int ax;
+    return slab_exact<D>(r, rb, rb + D, limit, te, tx, ax);
+  }
+  __device__ void pair(const double* b, double limit, bool& ha, bool& hb, double& ta, double& tb) const {
+    slab_pair64<D>(r, b, limit, ha, hb, ta, tb);
+  }
+  __device__ bool narrow(const double* p, double limit, double& t) const { return sqd::narrow<D, KIND>(r, p, limit, t); }
+};
+template <int D, int KIND> struct Pol64<D, KIND, true> {
+  static constexpr int QS = 4 * D;   // origin, direction, probe box min, max
+  CastPolicy<D, KIND, double> c;
+  __device__ void setup(const double* src) { c.setup(src); }
+  __device__ const RayD<D, double>& ray() const { return c.r; }
+  __device__ bool root(const double* rb, double limit) const { return c.root(rb, limit); }
+  __device__ void pair(const double* b, double limit, bool& ha, bool& hb, double& ta, double& tb) const {
+    double g[4 * D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      g[k] = b[k] - c.pmax[k]; g[D + k] = b[D + k] - c.pmin[k];
+      g[2 * D + k] = b[2 * D + k] - c.pmax[k]; g[3 * D + k] = b[3 * D + k] - c.pmin[k];
+    }
+    slab_pair64<D>(c.r, g, limit, ha, hb, ta, tb);
+  }
+  __device__ bool narrow(const double* p, double limit, double& t) const { return c.narrow(p, limit, t); }
+};
+
+// Bvh::raycast_nearest (bvh.rs:51-124), the any-hit idiom and
+// Bvh::shapecast_nearest (bvh.rs:331-375), pooled.  Each
 // loop iteration refills the wave's idle lanes from the counter (one atomic per
 // wave) and advances every walking lane by one pair node.  The wave leaves the
 // loop once the counter has passed n and none of its lanes walks; every wave
 // then retires once (pool_retire), the last one clearing the counter.
-template <int D, int KIND, bool ANY>
+template <int D, int KIND, bool ANY, bool CAST>
 __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const double* __restrict__ rays, uint32_t n,
                                                         double max_toi, Filter f, uint32_t* counter, int32_t* out_leaf,
                                                         double* out_toi, double* out_point, double* out_normal,
@@ -146,7 +183,7 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
   extern __shared__ uint32_t f64_ring[];
   const RingStack st{f64_ring + threadIdx.x, reinterpret_cast<float*>(f64_ring + F64_RING * blockDim.x) + threadIdx.x,
                      blockDim.x, F64_RING, spill + (size_t)blockIdx.x * blockDim.x + threadIdx.x, lanes};
-  RayD<D, double> r;
+  Pol64<D, KIND, CAST> q;
   double best = 0.0;
   uint32_t bl = NO_LEAF, bk = 0, node = 0, sp = 0, ray = 0;
   bool active = false, drained = false;
@@ -156,7 +193,7 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
       const uint32_t id = v.ids[k];
       if (!accepts64(v, f, id)) continue;
       double t;
-      if (!sqd::narrow<D, KIND>(r, v.prims + (size_t)k * PS, best, t)) continue;
+      if (!q.narrow(v.prims + (size_t)k * PS, best, t)) continue;
       if (!hit_before64(t, id, best, bl)) continue;
       best = t; bl = id; bk = k;
       if (ANY) return true;
@@ -164,17 +201,23 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
     return false;
   };
   auto finish = [&]() {
+    const RayD<D, double>& r = q.ray();
     active = false;
     const bool hit = bl != NO_LEAF;
     if (ANY) { out_any[ray] = hit; return; }
     out_leaf[ray] = hit ? (int32_t)bl : -1;
     out_toi[ray] = hit ? best : __builtin_huge_val();
     double nrm[D];
-    if (hit && out_normal) hit_normal<D, KIND>(r, v.prims + (size_t)bk * PS, best, max_toi, nrm);
+    if constexpr (!CAST) {
+      if (hit && out_normal) hit_normal<D, KIND>(r, v.prims + (size_t)bk * PS, best, max_toi, nrm);
+    }
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      if (out_point) out_point[(size_t)ray * D + k] = hit ? r.o[k] + r.d[k] * best : 0.0;   // Ray::at (maths/ray.rs)
-      if (out_normal) out_normal[(size_t)ray * D + k] = hit ? nrm[k] : 0.0;
+      // Ray::at (maths/ray.rs), ShapeCast::at for casts (shapecast.rs)
+      if (out_point) out_point[(size_t)ray * D + k] = hit ? r.o[k] + r.d[k] * best : 0.0;
+      if constexpr (!CAST) {
+        if (out_normal) out_normal[(size_t)ray * D + k] = hit ? nrm[k] : 0.0;
+      }
     }
   };
   while (true) {
@@ -191,15 +234,14 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
           const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
           if (my < n) {
             ray = my;
-            ray_setup<D>(r, rays + (size_t)my * 2 * D);
+            q.setup(rays + (size_t)my * Pol64<D, KIND, CAST>::QS);
             best = max_toi; bl = NO_LEAF; sp = 0;
-            double te, tx; int ax;
-            if (v.n_prims && slab_exact<D>(r, v.root, v.root + D, max_toi, te, tx, ax) && !v.root_count) {
+            const bool in_root = v.n_prims && q.root(v.root, max_toi);
+            if (in_root && !v.root_count) {
               node = v.root_ref;
               active = true;
             } else {
-              if (v.n_prims && v.root_count && slab_exact<D>(r, v.root, v.root + D, max_toi, te, tx, ax))
-                test_leaf(v.root_ref, v.root_count);
+              if (in_root) test_leaf(v.root_ref, v.root_count);
               finish();
             }
           }
@@ -212,7 +254,7 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
     const uint4 rf = *reinterpret_cast<const uint4*>(b + 4 * D);
     double ta, tb;
     bool ha, hb;
-    slab_pair64<D>(r, b, best, ha, hb, ta, tb);
+    q.pair(b, best, ha, hb, ta, tb);
     bool stop = false;
     if (ha && rf.z) { stop = test_leaf(rf.x, rf.z); ha = false; }
     if (!stop && hb && rf.w) { stop = test_leaf(rf.y, rf.w); hb = false; }
@@ -243,7 +285,7 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
 // Bvh::raycast_all (bvh.rs:196-259) and Bvh::overlap (bvh.rs:418-456): one
 // pass counts each query's results, the second writes them into its CSR range
 // and sorts the range (rays by (toi total order, leaf), overlap by leaf).
-template <int D, int KIND, bool OVERLAP>
+template <int D, int KIND, bool OVERLAP, bool CAST = false>
 __global__ __launch_bounds__(256) void k64_collect(View64 v, const double* __restrict__ q, uint64_t n, double max_toi,
                                                    Filter f, uint32_t* counts, const uint64_t* offsets,
                                                    int32_t* out_leaf, double* out_toi) {
@@ -271,20 +313,21 @@ __global__ __launch_bounds__(256) void k64_collect(View64 v, const double* __res
         },
         [](double) { return false; });
   } else {
-    RayD<D, double> r;
-    ray_setup<D>(r, q + i * 2 * D);
+    Pol64<D, KIND, CAST> pol;
+    pol.setup(q + i * Pol64<D, KIND, CAST>::QS);
     walk64<D>(
         v,
         [&](const double* mn, const double* mx, double& te) {
           double tx; int ax;
-          return slab_exact<D>(r, mn, mx, max_toi, te, tx, ax);
+          if constexpr (CAST) return pol.c.grown_slab(mn, mx, max_toi, te, tx);
+          else return slab_exact<D>(pol.r, mn, mx, max_toi, te, tx, ax);
         },
         [&](uint32_t first, uint32_t count) {
           for (uint32_t k = first; k < first + count; ++k) {
             const uint32_t id = v.ids[k];
             if (!accepts64(v, f, id)) continue;
             double t;
-            if (!sqd::narrow<D, KIND>(r, v.prims + (size_t)k * PS, max_toi, t)) continue;
+            if (!pol.narrow(v.prims + (size_t)k * PS, max_toi, t)) continue;
             if (fill) { out_leaf[at] = (int32_t)id; out_toi[at] = t; ++at; }
             ++cnt;
           }
@@ -483,7 +526,7 @@ int sq_bvh64_set_layers(sq_bvh64* b, const uint64_t* layers) {
 }
 
 static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const sq_filter* f, int32_t* out_leaf,
-                     double* out_toi, double* out_point, double* out_normal, uint8_t* out_any) {
+                     double* out_toi, double* out_point, double* out_normal, uint8_t* out_any, bool cast) {
   if (!b || (n && !rays)) return fail(SQ_ERR_INVALID, "NULL argument");
   if (n && !out_any && (!out_leaf || !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
   if (!n) return SQ_OK;
@@ -492,8 +535,8 @@ static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi
   const Filter fd = filter64(b, f);
   uint32_t* counter = c->pool_ctr.as<uint32_t>();
   return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
-    auto launch = [&]<bool ANY>() -> int {
-      const void* kern = reinterpret_cast<const void*>(&k64_nearest_pool<D, KIND, ANY>);
+    auto launch = [&]<bool ANY, bool CAST>() -> int {
+      const void* kern = reinterpret_cast<const void*>(&k64_nearest_pool<D, KIND, ANY, CAST>);
       const size_t lds = (size_t)F64_RING * F64_BLOCK * 8;
       int per_cu = 0;
       if (int r0 = blocks_per_cu(kern, c->device, F64_BLOCK, lds, &per_cu)) return r0;
@@ -502,29 +545,37 @@ static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi
       // spill rows: the stack holds at most depth - 1 entries, F64_RING of them in LDS
       const uint64_t rows = b->view.stack > F64_RING ? b->view.stack - F64_RING : 0;
       if (int r1 = c->spill.ensure((size_t)rows * lanes * sizeof(SpillEntry) + 64)) return r1;
-      k64_nearest_pool<D, KIND, ANY><<<(uint32_t)grid, F64_BLOCK, lds, c->stream>>>(
+      k64_nearest_pool<D, KIND, ANY, CAST><<<(uint32_t)grid, F64_BLOCK, lds, c->stream>>>(
           b->view, rays, (uint32_t)n, max_toi, fd, counter, out_leaf, out_toi, out_point, out_normal, out_any,
           c->spill.as<SpillEntry>(), (uint32_t)lanes);
       HIP_TRY(hipGetLastError());
       return SQ_OK;
     };
-    return out_any ? launch.template operator()<true>() : launch.template operator()<false>();
+    if (cast) return launch.template operator()<false, true>();
+    return out_any ? launch.template operator()<true, false>() : launch.template operator()<false, false>();
   });
 }
 
 int sq_bvh64_raycast_nearest(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const sq_filter* f,
                              int32_t* out_leaf, double* out_toi, double* out_point, double* out_normal) {
-  return nearest64(b, rays, n, max_toi, f, out_leaf, out_toi, out_point, out_normal, nullptr);
+  return nearest64(b, rays, n, max_toi, f, out_leaf, out_toi, out_point, out_normal, nullptr, false);
+}
+
+int sq_bvh64_shapecast_nearest(sq_bvh64* b, const double* casts, uint64_t n, double max_toi, const sq_filter* f,
+                               int32_t* out_leaf, double* out_toi, double* out_point) {
+  return nearest64(b, casts, n, max_toi, f, out_leaf, out_toi, out_point, nullptr, nullptr, true);
 }
 
 int sq_bvh64_raycast_any(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const sq_filter* f,
                          uint8_t* out_hit) {
   if (n && !out_hit) return fail(SQ_ERR_INVALID, "NULL argument");
-  return nearest64(b, rays, n, max_toi, f, nullptr, nullptr, nullptr, nullptr, out_hit);
+  return nearest64(b, rays, n, max_toi, f, nullptr, nullptr, nullptr, nullptr, out_hit, false);
 }
 
-static int collect64(sq_bvh64* b, const double* q, uint64_t n, double max_toi, const sq_filter* f, bool overlap,
+// mode 0: rays, 1: overlap regions, 2: shape casts
+static int collect64(sq_bvh64* b, const double* q, uint64_t n, double max_toi, const sq_filter* f, int mode,
                      uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total) {
+  const bool overlap = mode == 1;
   if (!b || !offsets || !total || (n && !q)) return fail(SQ_ERR_INVALID, "NULL argument");
   sq_ctx* c = b->ctx;
   *total = 0;
@@ -537,9 +588,12 @@ static int collect64(sq_bvh64* b, const double* q, uint64_t n, double max_toi, c
   const Filter fd = filter64(b, f);
   auto launch = [&](const uint64_t* offs) -> int {
     return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
-      if (overlap)
+      if (mode == 1)
         k64_collect<D, KIND, true><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, fd,
                                                                             c->counts.as<uint32_t>(), offs, out_leaf, out_toi);
+      else if (mode == 2)
+        k64_collect<D, KIND, false, true><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, fd,
+                                                                                   c->counts.as<uint32_t>(), offs, out_leaf, out_toi);
       else
         k64_collect<D, KIND, false><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, fd,
                                                                              c->counts.as<uint32_t>(), offs, out_leaf, out_toi);
@@ -562,12 +616,17 @@ static int collect64(sq_bvh64* b, const double* q, uint64_t n, double max_toi, c
 
 int sq_bvh64_raycast_all(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const sq_filter* f,
                          uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total) {
-  return collect64(b, rays, n, max_toi, f, false, offsets, out_leaf, out_toi, capacity, total);
+  return collect64(b, rays, n, max_toi, f, 0, offsets, out_leaf, out_toi, capacity, total);
+}
+
+int sq_bvh64_shapecast_all(sq_bvh64* b, const double* casts, uint64_t n, double max_toi, const sq_filter* f,
+                           uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total) {
+  return collect64(b, casts, n, max_toi, f, 2, offsets, out_leaf, out_toi, capacity, total);
 }
 
 int sq_bvh64_overlap(sq_bvh64* b, const double* regions, uint64_t n, const sq_filter* f, uint64_t* offsets,
                      int32_t* out_leaf, uint64_t capacity, uint64_t* total) {
-  return collect64(b, regions, n, 0.0, f, true, offsets, out_leaf, nullptr, capacity, total);
+  return collect64(b, regions, n, 0.0, f, 1, offsets, out_leaf, nullptr, capacity, total);
 }
 
 }  // extern "C"

@@ -388,23 +388,23 @@ template <int D, int KIND> struct RayPolicy {
 // the grown leaf box (its entry); SPHERE leaves with the ball of radius
 // r + probe.max[0] (shape_overlap.rs:104-118) guarded by the grown box;
 // TRIANGLE leaves keep the reference default (no contact).
-template <int D, int KIND> struct CastPolicy {
-  RayD<D> r;
-  float pmin[D], pmax[D];
-  SQ_HD void setup(const float* src) {
+template <int D, int KIND, class R = float> struct CastPolicy {
+  RayD<D, R> r;
+  R pmin[D], pmax[D];
+  SQ_HD void setup(const R* src) {
     ray_setup<D>(r, src);
 #pragma unroll
     for (int k = 0; k < D; ++k) { pmin[k] = src[2 * D + k]; pmax[k] = src[3 * D + k]; }
   }
-  SQ_HD bool grown_slab(const float* mn, const float* mx, float limit, float& te, float& tx) const {
-    float gmn[D], gmx[D];
+  SQ_HD bool grown_slab(const R* mn, const R* mx, R limit, R& te, R& tx) const {
+    R gmn[D], gmx[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) { gmn[k] = mn[k] - pmax[k]; gmx[k] = mx[k] - pmin[k]; }
     int ax;
     return slab_exact<D>(r, gmn, gmx, limit, te, tx, ax);
   }
-  SQ_HD bool root(const float* root, float limit) const {
-    float te, tx;
+  SQ_HD bool root(const R* root, R limit) const {
+    R te, tx;
     return grown_slab(root, root + D, limit, te, tx);
   }
   SQ_HD void pair(const float* nd, float limit, bool& ha, bool& hb, float& ta, float& tb) const {
@@ -416,30 +416,30 @@ template <int D, int KIND> struct CastPolicy {
     }
     slab_pair<D>(r, g, limit, ha, hb, ta, tb);
   }
-  SQ_HD bool narrow(const float* p, float limit, float& t) const {
+  SQ_HD bool narrow(const R* p, R limit, R& t) const {
     if constexpr (KIND == KIND_TRI) {
       return false;
     } else {
-      float mn[D], mx[D];
+      R mn[D], mx[D];
       prim_box<D, KIND>(p, mn, mx);
-      float te, tx;
+      R te, tx;
       if constexpr (KIND == KIND_AABB) {
         if (!grown_slab(mn, mx, limit, te, tx)) return false;
         t = te;
         return true;
       } else {
-        float oc[D];
+        R oc[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) oc[k] = r.o[k] - p[k];
-        const float rad = p[D] + pmax[0];
-        float b = dotD<D>(oc, r.d);
-        float cc = dotD<D>(oc, oc) - rad * rad;
-        float disc = b * b - cc;
-        if (disc < 0.0f) return false;
-        float sq = SQ_SQRT_RN(disc);
-        float tt = -b - sq;
-        if (tt < 0.0f) tt = -b + sq;
-        if (tt < 0.0f || tt > limit) return false;
+        const R rad = p[D] + pmax[0];
+        R b = dotD<D>(oc, r.d);
+        R cc = dotD<D>(oc, oc) - rad * rad;
+        R disc = b * b - cc;
+        if (disc < R(0)) return false;
+        R sq = sq_sqrt(disc);
+        R tt = -b - sq;
+        if (tt < R(0)) tt = -b + sq;
+        if (tt < R(0) || tt > limit) return false;
         if (!grown_slab(mn, mx, limit, te, tx)) return false;
         if (!(te <= tt && tt <= tx)) return false;
         t = tt;

@@ -405,6 +405,24 @@ class DeviceBvh64:
                                               _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
         return self._csr(fn, rays.shape[0], rays.device, True, capacity)
 
+    def shapecast_nearest(self, casts, max_toi, filt=None, point=False):
+        import torch
+        n, dev = casts.shape[0], casts.device
+        leaf = torch.empty(n, dtype=torch.int32, device=dev)
+        toi = torch.empty(n, dtype=torch.float64, device=dev)
+        pt = torch.empty((n, self.dim), dtype=torch.float64, device=dev) if point else None
+        check(lib().sq_bvh64_shapecast_nearest(self._h, _dp(casts), n, float(max_toi), self._f(filt), _dp(leaf),
+                                               _dp(toi), _dp(pt)))
+        out = dict(leaf=leaf, toi=toi)
+        if point: out["point"] = pt
+        return out
+
+    def shapecast_all(self, casts, max_toi, filt=None, capacity=0):
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_bvh64_shapecast_all(self._h, _dp(casts), casts.shape[0], float(max_toi), self._f(filt),
+                                                _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+        return self._csr(fn, casts.shape[0], casts.device, True, capacity)
+
     def overlap(self, regions, filt=None, capacity=0):
         def fn(off, leaf, toi, cap, total):
             return lib().sq_bvh64_overlap(self._h, _dp(regions), regions.shape[0], self._f(filt), _dp(off), _dp(leaf),

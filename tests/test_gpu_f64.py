@@ -3,8 +3,9 @@ Scalar = f64) against the f64 oracle (oracle/sq_oracle64.cpp, the naive scans
 of sq/src/query/naive.rs in double):
 
   * D = 2, 3, 4, 6 over BOX and SPHERE leaves: nearest (ids, toi, point and
-    normal bit for bit), any-hit, all hits and overlap (CSR bit for bit), with
-    and without a filter; the tree's stats equal the oracle's f64 tree;
+    normal bit for bit), any-hit, all hits, shape casts (nearest with points,
+    all) and overlap (CSR bit for bit), with and without a filter; the tree's
+    stats equal the oracle's f64 tree;
   * a 262,144-ball scene with 1M rays: the oracle on a sample, and the
     size-independent properties on every ray (any == nearest hit, the first
     of a ray's all-hits list is its nearest hit, lists sorted);
@@ -42,7 +43,9 @@ def scene(dim, kind, n, seed):
     rays = np.concatenate([o, d], 1)
     h = rng.uniform(0.5, 3.0, (n // 2, dim))
     regions = np.concatenate([c[: n // 2] - h, c[: n // 2] + h], 1)
-    return prims, rays, regions, span
+    ph = rng.uniform(0.1, 1.0, (rays.shape[0], 1))
+    casts = np.concatenate([rays, np.repeat(-ph, dim, 1), np.repeat(ph, dim, 1)], 1)
+    return prims, rays, regions, casts, span
 
 
 def assert_csr(got, ref, what, has_t=True):
@@ -56,7 +59,7 @@ def assert_csr(got, ref, what, has_t=True):
 @pytest.mark.parametrize("kind", [sq_amd.AABB, sq_amd.SPHERE])
 def test_f64_matches_oracle(ctx, dim, kind):
     print(f"case: f64 dim={dim} kind={kind}", flush=True)
-    prims, rays, regions, span = scene(dim, kind, 4000, 640 + 10 * dim + kind)
+    prims, rays, regions, casts, span = scene(dim, kind, 4000, 640 + 10 * dim + kind)
     layers = np.random.default_rng(dim).integers(0, 4, prims.shape[0]).astype(np.uint64)
     b = sq_amd.DeviceBvh64(ctx, dim, kind, prims)
     ref = oracle.Scene64(dim, kind, prims)
@@ -80,6 +83,13 @@ def test_f64_matches_oracle(ctx, dim, kind):
     r_ov = ref.overlap(regions)
     assert r_ov[1].shape[0] > 1000
     assert_csr(b.overlap(dev(regions)), (r_ov[0], r_ov[1], None), "overlap", has_t=False)
+    cn = b.shapecast_nearest(dev(casts), max_toi, point=True)
+    wc = ref.shapecast_nearest(casts, max_toi)
+    assert (wc["leaf"] >= 0).sum() > len(casts) // 10
+    assert np.array_equal(host(cn["leaf"]).astype(np.int64), wc["leaf"])
+    assert np.array_equal(host(cn["toi"]).view(np.uint64), wc["t"].view(np.uint64))
+    assert np.array_equal(host(cn["point"]).view(np.uint64), wc["point"].view(np.uint64))
+    assert_csr(b.shapecast_all(dev(casts[:1500]), max_toi), ref.shapecast_all(casts[:1500], max_toi), "cast all")
     # filters: layers by original leaf id, a mask and an excluded id
     b.set_layers(layers)
     reff = oracle.Scene64(dim, kind, prims, layers=layers)
@@ -92,6 +102,10 @@ def test_f64_matches_oracle(ctx, dim, kind):
                reff.raycast_all(rays[:2000], max_toi, mask=0b10, exclude=filt.exclude), "filtered all hits")
     r_ov = reff.overlap(regions, mask=0b10, exclude=filt.exclude)
     assert_csr(b.overlap(dev(regions), filt=filt), (r_ov[0], r_ov[1], None), "filtered overlap", has_t=False)
+    cn = b.shapecast_nearest(dev(casts), max_toi, filt=filt)
+    wc = reff.shapecast_nearest(casts, max_toi, mask=0b10, exclude=filt.exclude)
+    assert np.array_equal(host(cn["leaf"]).astype(np.int64), wc["leaf"])
+    assert np.array_equal(host(cn["toi"]).view(np.uint64), wc["t"].view(np.uint64))
     b.close()
 
 
@@ -147,5 +161,8 @@ def test_f64_refusals_and_empty(ctx):
     assert (host(got["point"]) == 0).all()
     off, _, _ = empty.raycast_all(rays, 10.0)
     assert (host(off) == 0).all()
+    casts = dev(np.array([[0.0, 0.0, 0.0, 1.0, 0.0, 0.0, -0.5, -0.5, -0.5, 0.5, 0.5, 0.5]] * 5))
+    got = empty.shapecast_nearest(casts, 10.0)
+    assert (host(got["leaf"]) == -1).all() and np.isinf(host(got["toi"])).all()
     assert empty.stats()["node_count"] == 0
     empty.close()

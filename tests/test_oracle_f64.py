@@ -171,3 +171,49 @@ def test_all_hits_sorted_and_headed_by_the_nearest(kind):
         keys = list(zip(t[a:b], leaf[a:b]))
         assert keys == sorted(keys)
         assert leaf[a] == w["leaf"][i] and t[a] == w["t"][i]
+
+
+def test_shape_cast_closed_forms_and_identities():
+    # a box probe of half-extent 0.25 swept along +x meets the unit box at 5 - 0.25
+    s = oracle.Scene64(3, oracle.BOX, np.array([[0.0, 0.0, 0.0, 1.0, 1.0, 1.0]]))
+    cast = np.array([[-5.0, 0.5, 0.5, 1.0, 0.0, 0.0, -0.25, -0.25, -0.25, 0.25, 0.25, 0.25]])
+    w = s.shapecast_nearest(cast, 100.0)
+    assert w["leaf"][0] == 0 and w["t"][0] == 4.75 and np.array_equal(w["point"][0], [-0.25, 0.5, 0.5])
+    # a ball probe (radius probe_max[0]) against a ball: contact at the summed radius
+    b = oracle.Scene64(3, oracle.SPHERE, np.array([[10.0, 0.0, 0.0, 2.0]]))
+    cast = np.array([[0.0, 0.0, 0.0, 1.0, 0.0, 0.0, -0.5, -0.5, -0.5, 0.5, 0.5, 0.5]])
+    assert b.shapecast_nearest(cast, 100.0)["t"][0] == 7.5
+
+
+@pytest.mark.parametrize("kind", [oracle.BOX, oracle.SPHERE])
+def test_dyadic_shape_casts(kind):
+    """Dyadic scenes: BOX casts equal the f32 oracle's BVH casts bit for bit; a
+    SPHERE cast with probe half-extent h equals the ray cast against balls of
+    radius r + h (every sum, grown box and product is exact)."""
+    dim = 3
+    rng = np.random.default_rng(31 + kind)
+    if kind == oracle.BOX:
+        prims = dyadic_boxes(3000, dim, 41, 256)
+    else:
+        prims = np.concatenate([rng.integers(-256, 256, (3000, dim)) / 8.0, rng.integers(2, 24, (3000, 1)) / 8.0], 1)
+    rays = dyadic_rays(2000, dim, 43 + kind, 256, 64)
+    h = rng.integers(1, 8, (2000, 1)) / 8.0
+    casts = np.concatenate([rays, np.repeat(-h, dim, 1), np.repeat(h, dim, 1)], 1)
+    s64 = oracle.Scene64(dim, kind, prims)
+    w = s64.shapecast_nearest(casts, 60.0)
+    assert (w["leaf"] >= 0).sum() > 300
+    a64 = s64.shapecast_all(casts[:600], 60.0)
+    if kind == oracle.BOX:
+        s32 = oracle.Scene(dim, oracle.BOX, prims.astype(np.float32))
+        w32 = s32.shapecast_nearest(casts.astype(np.float32), 60.0)
+        assert np.array_equal(w["leaf"], w32["leaf"]) and np.array_equal(w["t"], w32["t"].astype(np.float64))
+        a32 = s32.shapecast_all(casts[:600].astype(np.float32), 60.0)
+        assert np.array_equal(a64[0], a32[0].astype(np.uint64)) and np.array_equal(a64[1], a32[1])
+        assert np.array_equal(a64[2], a32[2].astype(np.float64))
+    else:
+        for hv in np.unique(h):
+            sel = h[:, 0] == hv
+            grown = prims.copy()
+            grown[:, dim] += hv
+            wr = oracle.Scene64(dim, kind, grown).raycast_nearest(rays[sel], 60.0)
+            assert np.array_equal(w["leaf"][sel], wr["leaf"]) and np.array_equal(w["t"][sel], wr["t"])
