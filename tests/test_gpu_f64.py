@@ -166,3 +166,55 @@ def test_f64_refusals_and_empty(ctx):
     assert (host(got["leaf"]) == -1).all() and np.isinf(host(got["toi"])).all()
     assert empty.stats()["node_count"] == 0
     empty.close()
+
+
+@pytest.mark.parametrize("kind", [sq_amd.AABB, sq_amd.SPHERE])
+def test_f64_ties_coincident_parallel_and_unbounded(ctx, kind):
+    """Over-full leaves from coincident centroids (tree.rs:145-148), exact
+    duplicates (toi ties broken by leaf id), axis-parallel rays (the slab's
+    parallel branch) and max_toi = +inf."""
+    print(f"case: f64 ties and parallel rays kind={kind}", flush=True)
+    rng = np.random.default_rng(0x7E5 + kind)
+    dim = 3
+    c = rng.uniform(-20, 20, (2000, dim))
+    c[:300] = c[0]                      # 300 coincident centroids
+    c[300:400] = c[300:400].round(0)    # duplicates on a lattice
+    if kind == sq_amd.AABB:
+        h = rng.uniform(0.2, 1.5, (2000, dim))
+        h[:300] = rng.uniform(0.2, 1.5, (300, dim))
+        prims = np.concatenate([c - h, c + h], 1)
+        prims = np.concatenate([prims, prims[300:400]])           # exact copies
+    else:
+        r = rng.uniform(0.2, 1.5, (2000, 1))
+        prims = np.concatenate([c, r], 1)
+        prims = np.concatenate([prims, prims[300:400]])
+    o = rng.uniform(-25, 25, (3000, dim))
+    o[:200] = c[:200] + rng.uniform(-0.1, 0.1, (200, dim))       # start inside the coincident cluster
+    d = rng.normal(size=(3000, dim))
+    axis = rng.integers(0, dim, 1500)
+    d[:1500] = 0.0
+    d[np.arange(1500), axis] = rng.choice([-1.0, 1.0], 1500)    # axis-parallel rays
+    d[1500:] /= np.linalg.norm(d[1500:], axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1)
+    b = sq_amd.DeviceBvh64(ctx, dim, kind, prims)
+    ref = oracle.Scene64(dim, kind, prims)
+    st, rs = b.stats(), ref.tree_stats()
+    for k in ("node_count", "leaf_count", "prim_count", "max_depth", "bytes"):
+        assert st[k] == rs[k], k
+    for max_toi in (60.0, float("inf")):
+        got = b.raycast_nearest(dev(rays), max_toi, normal=True)
+        want = ref.raycast_nearest(rays, max_toi)
+        assert (want["leaf"] >= 0).sum() > 1000
+        assert np.array_equal(host(got["leaf"]).astype(np.int64), want["leaf"])
+        assert np.array_equal(host(got["toi"]).view(np.uint64), want["t"].view(np.uint64))
+        assert np.array_equal(host(got["normal"]).view(np.uint64), want["normal"].view(np.uint64))
+        assert_csr(b.raycast_all(dev(rays[:1000]), max_toi), ref.raycast_all(rays[:1000], max_toi), "all, ties")
+    casts = np.concatenate([rays, np.full((3000, dim), -0.25), np.full((3000, dim), 0.25)], 1)
+    cn = b.shapecast_nearest(dev(casts), float("inf"))
+    wc = ref.shapecast_nearest(casts, float("inf"))
+    assert np.array_equal(host(cn["leaf"]).astype(np.int64), wc["leaf"])
+    assert np.array_equal(host(cn["toi"]).view(np.uint64), wc["t"].view(np.uint64))
+    regions = np.concatenate([c[:500] - 0.5, c[:500] + 0.5], 1)
+    r_ov = ref.overlap(regions)
+    assert_csr(b.overlap(dev(regions)), (r_ov[0], r_ov[1], None), "overlap, ties", has_t=False)
+    b.close()
