@@ -479,23 +479,27 @@ int sq_bvh64_stats(const sq_bvh64* bvh, sq_tree_stats* out);
 /* QueryFilter layers by original leaf id (host or device uint64[n]; NULL clears). */
 int sq_bvh64_set_layers(sq_bvh64* bvh, const uint64_t* layers);
 /* Bvh::raycast_nearest (bvh.rs:51-124): rays double[n][2D] (device);
- * out_point / out_normal double[n][D] may be NULL. */
-int sq_bvh64_raycast_nearest(sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi, const sq_filter* filter,
-                             int32_t* out_leaf, double* out_toi, double* out_point, double* out_normal);
+ * max_toi_per_ray (device double[n]) may be NULL; out_point / out_normal
+ * double[n][D] may be NULL. */
+int sq_bvh64_raycast_nearest(sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi,
+                             const double* max_toi_per_ray, const sq_filter* filter, int32_t* out_leaf,
+                             double* out_toi, double* out_point, double* out_normal);
 /* raycast_nearest(..).is_some() per ray. */
-int sq_bvh64_raycast_any(sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi, const sq_filter* filter,
-                         uint8_t* out_hit);
+int sq_bvh64_raycast_any(sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi, const double* max_toi_per_ray,
+                         const sq_filter* filter, uint8_t* out_hit);
 /* Bvh::raycast_all (bvh.rs:196-259), CSR as sq_raycast_all. */
 int sq_bvh64_raycast_all(sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi, const sq_filter* filter,
                          uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total);
 /* Bvh::shapecast_nearest / shapecast_all (bvh.rs:331-414): casts double[n][4D]
- * (origin, dir, probe box min, max) as sq_shapecast_nearest / _all, with one
- * max_toi for every cast; AABB leaves answer the grown-box slab entry, SPHERE
- * leaves the ball of radius r + probe_max[0]. */
-int sq_bvh64_shapecast_nearest(sq_bvh64* bvh, const double* casts, uint64_t n, double max_toi, const sq_filter* filter,
-                               int32_t* out_leaf, double* out_toi, double* out_point);
-int sq_bvh64_shapecast_all(sq_bvh64* bvh, const double* casts, uint64_t n, double max_toi, const sq_filter* filter,
-                           uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total);
+ * (origin, dir, probe box min, max) as sq_shapecast_nearest / _all;
+ * max_toi_per_cast may be NULL.  AABB leaves answer the grown-box slab entry,
+ * SPHERE leaves the ball of radius r + probe_max[0]. */
+int sq_bvh64_shapecast_nearest(sq_bvh64* bvh, const double* casts, uint64_t n, double max_toi,
+                               const double* max_toi_per_cast, const sq_filter* filter, int32_t* out_leaf,
+                               double* out_toi, double* out_point);
+int sq_bvh64_shapecast_all(sq_bvh64* bvh, const double* casts, uint64_t n, double max_toi,
+                           const double* max_toi_per_cast, const sq_filter* filter, uint64_t* offsets,
+                           int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total);
 /* Bvh::overlap (bvh.rs:418-456): regions double[n][2D], CSR as sq_overlap. */
 int sq_bvh64_overlap(sq_bvh64* bvh, const double* regions, uint64_t n, const sq_filter* filter, uint64_t* offsets,
                      int32_t* out_leaf, uint64_t capacity, uint64_t* total);

@@ -75,12 +75,12 @@ def _declare(L):
         "orc_shapecast_nearest_batch": (None, [P, P, P, U64, P, U64, I64, I32, P, P, P, P]),
         "orc_shapecast_all_batch": (U64, [P, P, P, U64, P, U64, I64, I32, P, P, P, U64]),
         "orc_required_work": (None, [P, P, U64, P, P, P, I32]),
-        "orc64_raycast_nearest": (None, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, I32, P, P, P, P]),
+        "orc64_raycast_nearest": (None, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, P, I32, P, P, P, P]),
         "orc64_raycast_all": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, I32, P, P, P, U64]),
         "orc64_overlap": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, I32, P, P, U64]),
         "orc64_tree_stats": (None, [I32, I32, P, U32, P, P]),
-        "orc64_shapecast_nearest": (None, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, I32, P, P, P]),
-        "orc64_shapecast_all": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, I32, P, P, P, U64]),
+        "orc64_shapecast_nearest": (None, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, P, I32, P, P, P]),
+        "orc64_shapecast_all": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, P, I32, P, P, P, U64]),
         "orc_inst_new": (P, [I32, I32, U64, P, P, P, U32, P, P, P, P]),
         "orc_inst_set_transforms": (None, [P, P, P, P]),
         "orc_tlas_refit": (None, [P]),
@@ -313,12 +313,21 @@ class Scene64:
     def _q(self, q):
         return np.ascontiguousarray(q, np.float64).reshape(-1, 2 * self.dim)
 
+    @staticmethod
+    def _limits(max_toi, n):
+        """A scalar max_toi, or one per query (returned as the array)."""
+        if np.ndim(max_toi) == 0:
+            return float(max_toi), None
+        a = np.ascontiguousarray(max_toi, np.float64).reshape(n)
+        return 0.0, a
+
     def raycast_nearest(self, rays, max_toi, mask=ALL_LAYERS, exclude=-1, threads=8):
         r = self._q(rays)
         n = r.shape[0]
+        m, per = self._limits(max_toi, n)
         leaf = np.empty(n, np.int64); t = np.empty(n); pt = np.empty((n, self.dim)); nm = np.empty((n, self.dim))
         lib().orc64_raycast_nearest(self.dim, self.kind, _p(self.prims), self.n, _p(self.layers), mask, exclude, _p(r), n,
-                                    float(max_toi), threads, _p(leaf), _p(t), _p(pt), _p(nm))
+                                    m, _p(per), threads, _p(leaf), _p(t), _p(pt), _p(nm))
         return dict(leaf=leaf, t=t, point=pt, normal=nm)
 
     def _collect(self, fn, q, has_t):
@@ -346,16 +355,18 @@ class Scene64:
     def shapecast_nearest(self, casts, max_toi, mask=ALL_LAYERS, exclude=-1, threads=8):
         c = np.ascontiguousarray(casts, np.float64).reshape(-1, 4 * self.dim)
         n = c.shape[0]
+        m, per = self._limits(max_toi, n)
         leaf = np.empty(n, np.int64); t = np.empty(n); pt = np.empty((n, self.dim))
         lib().orc64_shapecast_nearest(self.dim, self.kind, _p(self.prims), self.n, _p(self.layers), mask, exclude, _p(c), n,
-                                      float(max_toi), threads, _p(leaf), _p(t), _p(pt))
+                                      m, _p(per), threads, _p(leaf), _p(t), _p(pt))
         return dict(leaf=leaf, t=t, point=pt)
 
     def shapecast_all(self, casts, max_toi, mask=ALL_LAYERS, exclude=-1, threads=8):
         c = np.ascontiguousarray(casts, np.float64).reshape(-1, 4 * self.dim)
+        m, per = self._limits(max_toi, c.shape[0])
         fn = lambda off, leaf, t, cap: lib().orc64_shapecast_all(self.dim, self.kind, _p(self.prims), self.n,
                                                                 _p(self.layers), mask, exclude, _p(c), c.shape[0],
-                                                                float(max_toi), threads, _p(off), _p(leaf), _p(t), cap)
+                                                                m, _p(per), threads, _p(off), _p(leaf), _p(t), cap)
         return self._collect(fn, c, True)
 
     def tree_stats(self):

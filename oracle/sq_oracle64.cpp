@@ -369,14 +369,14 @@ template <class F> static void parallel_for(uint64_t n, int threads, F&& f) {
 
 template <int D>
 static void nearest_batch(int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask, int64_t exclude,
-                          const double* rays, uint64_t nr, double max_toi, int threads, int64_t* leaf, double* t,
-                          double* point, double* normal) {
+                          const double* rays, uint64_t nr, double max_toi, const double* tmax, int threads, int64_t* leaf,
+                          double* t, double* point, double* normal) {
   Geom<D> g{kind, n, prims, layers};
   parallel_for(nr, threads, [&](uint64_t i) {
     Ray r{rays + i * 2 * D, rays + i * 2 * D + D};
     double nrm[D];
     int64_t l; double tt;
-    const bool have = naive_nearest<D>(g, r, max_toi, mask, exclude, &l, &tt, nrm);
+    const bool have = naive_nearest<D>(g, r, tmax ? tmax[i] : max_toi, mask, exclude, &l, &tt, nrm);
     leaf[i] = have ? l : -1;
     t[i] = have ? tt : INFINITY;
     for (int k = 0; k < D; ++k) {
@@ -388,14 +388,14 @@ static void nearest_batch(int kind, const double* prims, uint32_t n, const uint6
 
 template <int D>
 static void cast_nearest_batch(int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
-                               int64_t exclude, const double* casts, uint64_t nc, double max_toi, int threads, int64_t* leaf,
-                               double* t, double* point) {
+                               int64_t exclude, const double* casts, uint64_t nc, double max_toi, const double* tmax,
+                               int threads, int64_t* leaf, double* t, double* point) {
   Geom<D> g{kind, n, prims, layers};
   parallel_for(nc, threads, [&](uint64_t i) {
     const double* c = casts + i * 4 * D;
     Ray r{c, c + D};
     int64_t l; double tt;
-    const bool have = naive_cast_nearest<D>(g, r, c + 2 * D, c + 3 * D, max_toi, mask, exclude, &l, &tt);
+    const bool have = naive_cast_nearest<D>(g, r, c + 2 * D, c + 3 * D, tmax ? tmax[i] : max_toi, mask, exclude, &l, &tt);
     leaf[i] = have ? l : -1;
     t[i] = have ? tt : INFINITY;
     for (int k = 0; k < D; ++k)
@@ -406,15 +406,15 @@ static void cast_nearest_batch(int kind, const double* prims, uint32_t n, const 
 // mode 0: rays, 1: overlap regions, 2: shape casts
 template <int D>
 static uint64_t all_batch(int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask, int64_t exclude,
-                          const double* q, uint64_t nq, double max_toi, int mode, int threads, uint64_t* offsets,
-                          int64_t* leaf, double* t, uint64_t cap) {
+                          const double* q, uint64_t nq, double max_toi, const double* tmax, int mode, int threads,
+                          uint64_t* offsets, int64_t* leaf, double* t, uint64_t cap) {
   Geom<D> g{kind, n, prims, layers};
   std::vector<std::vector<std::pair<double, int64_t>>> res(nq);
   parallel_for(nq, threads, [&](uint64_t i) {
     if (mode == 2) {
       const double* c = q + i * 4 * D;
       Ray r{c, c + D};
-      naive_cast_all<D>(g, r, c + 2 * D, c + 3 * D, max_toi, mask, exclude, res[i]);
+      naive_cast_all<D>(g, r, c + 2 * D, c + 3 * D, tmax ? tmax[i] : max_toi, mask, exclude, res[i]);
     } else if (mode == 1) {
       std::vector<int64_t> ids;
       naive_overlap<D>(g, q + i * 2 * D, q + i * 2 * D + D, mask, exclude, ids);
@@ -452,16 +452,16 @@ using namespace orc64;
 extern "C" {
 
 void orc64_raycast_nearest(int D, int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
-                           int64_t exclude, const double* rays, uint64_t nr, double max_toi, int threads, int64_t* leaf,
-                           double* t, double* point, double* normal) {
-  ORC64_DISPATCH(D, nearest_batch<DD>(kind, prims, n, layers, mask, exclude, rays, nr, max_toi, threads, leaf, t, point, normal));
+                           int64_t exclude, const double* rays, uint64_t nr, double max_toi, const double* tmax, int threads,
+                           int64_t* leaf, double* t, double* point, double* normal) {
+  ORC64_DISPATCH(D, nearest_batch<DD>(kind, prims, n, layers, mask, exclude, rays, nr, max_toi, tmax, threads, leaf, t, point, normal));
 }
 
 uint64_t orc64_raycast_all(int D, int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
                            int64_t exclude, const double* rays, uint64_t nr, double max_toi, int threads,
                            uint64_t* offsets, int64_t* leaf, double* t, uint64_t cap) {
   uint64_t tot = 0;
-  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, rays, nr, max_toi, 0, threads, offsets, leaf, t, cap));
+  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, rays, nr, max_toi, nullptr, 0, threads, offsets, leaf, t, cap));
   return tot;
 }
 
@@ -469,21 +469,21 @@ uint64_t orc64_overlap(int D, int kind, const double* prims, uint32_t n, const u
                        int64_t exclude, const double* regions, uint64_t nq, int threads, uint64_t* offsets, int64_t* leaf,
                        uint64_t cap) {
   uint64_t tot = 0;
-  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, regions, nq, 0.0, 1, threads, offsets, leaf, nullptr, cap));
+  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, regions, nq, 0.0, nullptr, 1, threads, offsets, leaf, nullptr, cap));
   return tot;
 }
 
 void orc64_shapecast_nearest(int D, int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
-                             int64_t exclude, const double* casts, uint64_t nc, double max_toi, int threads, int64_t* leaf,
-                             double* t, double* point) {
-  ORC64_DISPATCH(D, cast_nearest_batch<DD>(kind, prims, n, layers, mask, exclude, casts, nc, max_toi, threads, leaf, t, point));
+                             int64_t exclude, const double* casts, uint64_t nc, double max_toi, const double* tmax,
+                             int threads, int64_t* leaf, double* t, double* point) {
+  ORC64_DISPATCH(D, cast_nearest_batch<DD>(kind, prims, n, layers, mask, exclude, casts, nc, max_toi, tmax, threads, leaf, t, point));
 }
 
 uint64_t orc64_shapecast_all(int D, int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
-                             int64_t exclude, const double* casts, uint64_t nc, double max_toi, int threads,
-                             uint64_t* offsets, int64_t* leaf, double* t, uint64_t cap) {
+                             int64_t exclude, const double* casts, uint64_t nc, double max_toi, const double* tmax,
+                             int threads, uint64_t* offsets, int64_t* leaf, double* t, uint64_t cap) {
   uint64_t tot = 0;
-  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, casts, nc, max_toi, 2, threads, offsets, leaf, t, cap));
+  ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, casts, nc, max_toi, tmax, 2, threads, offsets, leaf, t, cap));
   return tot;
 }
 

@@ -174,7 +174,8 @@ template <int D, int KIND> struct Pol64<D, KIND, true> {
 // then retires once (pool_retire), the last one clearing the counter.
 template <int D, int KIND, bool ANY, bool CAST>
 __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const double* __restrict__ rays, uint32_t n,
-                                                        double max_toi, Filter f, uint32_t* counter, int32_t* out_leaf,
+                                                        double max_toi, const double* __restrict__ tmax, Filter f,
+                                                        uint32_t* counter, int32_t* out_leaf,
                                                         double* out_toi, double* out_point, double* out_normal,
                                                         uint8_t* out_any, SpillEntry* spill, uint32_t lanes) {
   constexpr int PS = stride64<D, KIND>();
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
   const RingStack st{f64_ring + threadIdx.x, reinterpret_cast<float*>(f64_ring + F64_RING * blockDim.x) + threadIdx.x,
                      blockDim.x, F64_RING, spill + (size_t)blockIdx.x * blockDim.x + threadIdx.x, lanes};
   Pol64<D, KIND, CAST> q;
-  double best = 0.0;
+  double best = 0.0, lim = 0.0;   // lim: this query's max_toi
   uint32_t bl = NO_LEAF, bk = 0, node = 0, sp = 0, ray = 0;
   bool active = false, drained = false;
   // leaf range test: the nearest accepted hit so far; true ends an any-hit walk
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
     out_toi[ray] = hit ? best : __builtin_huge_val();
     double nrm[D];
     if constexpr (!CAST) {
-      if (hit && out_normal) hit_normal<D, KIND>(r, v.prims + (size_t)bk * PS, best, max_toi, nrm);
+      if (hit && out_normal) hit_normal<D, KIND>(r, v.prims + (size_t)bk * PS, best, lim, nrm);
     }
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -235,8 +236,9 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
           if (my < n) {
             ray = my;
             q.setup(rays + (size_t)my * Pol64<D, KIND, CAST>::QS);
-            best = max_toi; bl = NO_LEAF; sp = 0;
-            const bool in_root = v.n_prims && q.root(v.root, max_toi);
+            lim = tmax ? tmax[my] : max_toi;
+            best = lim; bl = NO_LEAF; sp = 0;
+            const bool in_root = v.n_prims && q.root(v.root, lim);
             if (in_root && !v.root_count) {
               node = v.root_ref;
               active = true;
@@ -287,7 +289,8 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
 // and sorts the range (rays by (toi total order, leaf), overlap by leaf).
 template <int D, int KIND, bool OVERLAP, bool CAST = false>
 __global__ __launch_bounds__(256) void k64_collect(View64 v, const double* __restrict__ q, uint64_t n, double max_toi,
-                                                   Filter f, uint32_t* counts, const uint64_t* offsets,
+                                                   const double* __restrict__ tmax, Filter f, uint32_t* counts,
+                                                   const uint64_t* offsets,
                                                    int32_t* out_leaf, double* out_toi) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -315,19 +318,20 @@ __global__ __launch_bounds__(256) void k64_collect(View64 v, const double* __res
   } else {
     Pol64<D, KIND, CAST> pol;
     pol.setup(q + i * Pol64<D, KIND, CAST>::QS);
+    const double lim = tmax ? tmax[i] : max_toi;
     walk64<D>(
         v,
         [&](const double* mn, const double* mx, double& te) {
           double tx; int ax;
-          if constexpr (CAST) return pol.c.grown_slab(mn, mx, max_toi, te, tx);
-          else return slab_exact<D>(pol.r, mn, mx, max_toi, te, tx, ax);
+          if constexpr (CAST) return pol.c.grown_slab(mn, mx, lim, te, tx);
+          else return slab_exact<D>(pol.r, mn, mx, lim, te, tx, ax);
         },
         [&](uint32_t first, uint32_t count) {
           for (uint32_t k = first; k < first + count; ++k) {
             const uint32_t id = v.ids[k];
             if (!accepts64(v, f, id)) continue;
             double t;
-            if (!pol.narrow(v.prims + (size_t)k * PS, max_toi, t)) continue;
+            if (!pol.narrow(v.prims + (size_t)k * PS, lim, t)) continue;
             if (fill) { out_leaf[at] = (int32_t)id; out_toi[at] = t; ++at; }
             ++cnt;
           }
@@ -525,8 +529,9 @@ int sq_bvh64_set_layers(sq_bvh64* b, const uint64_t* layers) {
   return SQ_OK;
 }
 
-static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const sq_filter* f, int32_t* out_leaf,
-                     double* out_toi, double* out_point, double* out_normal, uint8_t* out_any, bool cast) {
+static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const double* tmax, const sq_filter* f,
+                     int32_t* out_leaf, double* out_toi, double* out_point, double* out_normal, uint8_t* out_any,
+                     bool cast) {
   if (!b || (n && !rays)) return fail(SQ_ERR_INVALID, "NULL argument");
   if (n && !out_any && (!out_leaf || !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
   if (!n) return SQ_OK;
@@ -546,7 +551,7 @@ static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi
       const uint64_t rows = b->view.stack > F64_RING ? b->view.stack - F64_RING : 0;
       if (int r1 = c->spill.ensure((size_t)rows * lanes * sizeof(SpillEntry) + 64)) return r1;
       k64_nearest_pool<D, KIND, ANY, CAST><<<(uint32_t)grid, F64_BLOCK, lds, c->stream>>>(
-          b->view, rays, (uint32_t)n, max_toi, fd, counter, out_leaf, out_toi, out_point, out_normal, out_any,
+          b->view, rays, (uint32_t)n, max_toi, tmax, fd, counter, out_leaf, out_toi, out_point, out_normal, out_any,
           c->spill.as<SpillEntry>(), (uint32_t)lanes);
       HIP_TRY(hipGetLastError());
       return SQ_OK;
@@ -556,24 +561,27 @@ static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi
   });
 }
 
-int sq_bvh64_raycast_nearest(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const sq_filter* f,
-                             int32_t* out_leaf, double* out_toi, double* out_point, double* out_normal) {
-  return nearest64(b, rays, n, max_toi, f, out_leaf, out_toi, out_point, out_normal, nullptr, false);
+int sq_bvh64_raycast_nearest(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const double* max_toi_per_ray,
+                             const sq_filter* f, int32_t* out_leaf, double* out_toi, double* out_point,
+                             double* out_normal) {
+  return nearest64(b, rays, n, max_toi, max_toi_per_ray, f, out_leaf, out_toi, out_point, out_normal, nullptr, false);
 }
 
-int sq_bvh64_shapecast_nearest(sq_bvh64* b, const double* casts, uint64_t n, double max_toi, const sq_filter* f,
-                               int32_t* out_leaf, double* out_toi, double* out_point) {
-  return nearest64(b, casts, n, max_toi, f, out_leaf, out_toi, out_point, nullptr, nullptr, true);
+int sq_bvh64_shapecast_nearest(sq_bvh64* b, const double* casts, uint64_t n, double max_toi,
+                               const double* max_toi_per_cast, const sq_filter* f, int32_t* out_leaf, double* out_toi,
+                               double* out_point) {
+  return nearest64(b, casts, n, max_toi, max_toi_per_cast, f, out_leaf, out_toi, out_point, nullptr, nullptr, true);
 }
 
-int sq_bvh64_raycast_any(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const sq_filter* f,
-                         uint8_t* out_hit) {
+int sq_bvh64_raycast_any(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const double* max_toi_per_ray,
+                         const sq_filter* f, uint8_t* out_hit) {
   if (n && !out_hit) return fail(SQ_ERR_INVALID, "NULL argument");
-  return nearest64(b, rays, n, max_toi, f, nullptr, nullptr, nullptr, nullptr, out_hit, false);
+  return nearest64(b, rays, n, max_toi, max_toi_per_ray, f, nullptr, nullptr, nullptr, nullptr, out_hit, false);
 }
 
 // mode 0: rays, 1: overlap regions, 2: shape casts
-static int collect64(sq_bvh64* b, const double* q, uint64_t n, double max_toi, const sq_filter* f, int mode,
+static int collect64(sq_bvh64* b, const double* q, uint64_t n, double max_toi, const double* tmax, const sq_filter* f,
+                     int mode,
                      uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total) {
   const bool overlap = mode == 1;
   if (!b || !offsets || !total || (n && !q)) return fail(SQ_ERR_INVALID, "NULL argument");
@@ -589,13 +597,13 @@ static int collect64(sq_bvh64* b, const double* q, uint64_t n, double max_toi, c
   auto launch = [&](const uint64_t* offs) -> int {
     return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
       if (mode == 1)
-        k64_collect<D, KIND, true><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, fd,
+        k64_collect<D, KIND, true><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, tmax, fd,
                                                                             c->counts.as<uint32_t>(), offs, out_leaf, out_toi);
       else if (mode == 2)
-        k64_collect<D, KIND, false, true><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, fd,
+        k64_collect<D, KIND, false, true><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, tmax, fd,
                                                                                    c->counts.as<uint32_t>(), offs, out_leaf, out_toi);
       else
-        k64_collect<D, KIND, false><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, fd,
+        k64_collect<D, KIND, false><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, tmax, fd,
                                                                              c->counts.as<uint32_t>(), offs, out_leaf, out_toi);
       HIP_TRY(hipGetLastError());
       return SQ_OK;
@@ -616,17 +624,18 @@ static int collect64(sq_bvh64* b, const double* q, uint64_t n, double max_toi, c
 
 int sq_bvh64_raycast_all(sq_bvh64* b, const double* rays, uint64_t n, double max_toi, const sq_filter* f,
                          uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total) {
-  return collect64(b, rays, n, max_toi, f, 0, offsets, out_leaf, out_toi, capacity, total);
+  return collect64(b, rays, n, max_toi, nullptr, f, 0, offsets, out_leaf, out_toi, capacity, total);
 }
 
-int sq_bvh64_shapecast_all(sq_bvh64* b, const double* casts, uint64_t n, double max_toi, const sq_filter* f,
-                           uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total) {
-  return collect64(b, casts, n, max_toi, f, 2, offsets, out_leaf, out_toi, capacity, total);
+int sq_bvh64_shapecast_all(sq_bvh64* b, const double* casts, uint64_t n, double max_toi, const double* max_toi_per_cast,
+                           const sq_filter* f, uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity,
+                           uint64_t* total) {
+  return collect64(b, casts, n, max_toi, max_toi_per_cast, f, 2, offsets, out_leaf, out_toi, capacity, total);
 }
 
 int sq_bvh64_overlap(sq_bvh64* b, const double* regions, uint64_t n, const sq_filter* f, uint64_t* offsets,
                      int32_t* out_leaf, uint64_t capacity, uint64_t* total) {
-  return collect64(b, regions, n, 0.0, f, 1, offsets, out_leaf, nullptr, capacity, total);
+  return collect64(b, regions, n, 0.0, nullptr, f, 1, offsets, out_leaf, nullptr, capacity, total);
 }
 
 }  // extern "C"

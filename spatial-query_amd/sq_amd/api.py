@@ -361,24 +361,25 @@ class DeviceBvh64:
     def _f(self, filt):
         return None if filt is None else C.byref(filt.c())
 
-    def raycast_nearest(self, rays, max_toi, filt=None, point=False, normal=False):
+    def raycast_nearest(self, rays, max_toi, max_toi_per_ray=None, filt=None, point=False, normal=False):
         import torch
         n, dev = rays.shape[0], rays.device
         leaf = torch.empty(n, dtype=torch.int32, device=dev)
         toi = torch.empty(n, dtype=torch.float64, device=dev)
         pt = torch.empty((n, self.dim), dtype=torch.float64, device=dev) if point else None
         nm = torch.empty((n, self.dim), dtype=torch.float64, device=dev) if normal else None
-        check(lib().sq_bvh64_raycast_nearest(self._h, _dp(rays), n, float(max_toi), self._f(filt), _dp(leaf), _dp(toi),
-                                             _dp(pt), _dp(nm)))
+        check(lib().sq_bvh64_raycast_nearest(self._h, _dp(rays), n, float(max_toi), _dp(max_toi_per_ray), self._f(filt),
+                                             _dp(leaf), _dp(toi), _dp(pt), _dp(nm)))
         out = dict(leaf=leaf, toi=toi)
         if point: out["point"] = pt
         if normal: out["normal"] = nm
         return out
 
-    def raycast_any(self, rays, max_toi, filt=None):
+    def raycast_any(self, rays, max_toi, max_toi_per_ray=None, filt=None):
         import torch
         out = torch.empty(rays.shape[0], dtype=torch.uint8, device=rays.device)
-        check(lib().sq_bvh64_raycast_any(self._h, _dp(rays), rays.shape[0], float(max_toi), self._f(filt), _dp(out)))
+        check(lib().sq_bvh64_raycast_any(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(max_toi_per_ray),
+                                         self._f(filt), _dp(out)))
         return out
 
     def _csr(self, fn, n, dev, has_t, capacity):
@@ -405,22 +406,22 @@ class DeviceBvh64:
                                               _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
         return self._csr(fn, rays.shape[0], rays.device, True, capacity)
 
-    def shapecast_nearest(self, casts, max_toi, filt=None, point=False):
+    def shapecast_nearest(self, casts, max_toi, max_toi_per_cast=None, filt=None, point=False):
         import torch
         n, dev = casts.shape[0], casts.device
         leaf = torch.empty(n, dtype=torch.int32, device=dev)
         toi = torch.empty(n, dtype=torch.float64, device=dev)
         pt = torch.empty((n, self.dim), dtype=torch.float64, device=dev) if point else None
-        check(lib().sq_bvh64_shapecast_nearest(self._h, _dp(casts), n, float(max_toi), self._f(filt), _dp(leaf),
-                                               _dp(toi), _dp(pt)))
+        check(lib().sq_bvh64_shapecast_nearest(self._h, _dp(casts), n, float(max_toi), _dp(max_toi_per_cast),
+                                               self._f(filt), _dp(leaf), _dp(toi), _dp(pt)))
         out = dict(leaf=leaf, toi=toi)
         if point: out["point"] = pt
         return out
 
-    def shapecast_all(self, casts, max_toi, filt=None, capacity=0):
+    def shapecast_all(self, casts, max_toi, max_toi_per_cast=None, filt=None, capacity=0):
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_bvh64_shapecast_all(self._h, _dp(casts), casts.shape[0], float(max_toi), self._f(filt),
-                                                _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+            return lib().sq_bvh64_shapecast_all(self._h, _dp(casts), casts.shape[0], float(max_toi), _dp(max_toi_per_cast),
+                                                self._f(filt), _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
         return self._csr(fn, casts.shape[0], casts.device, True, capacity)
 
     def overlap(self, regions, filt=None, capacity=0):

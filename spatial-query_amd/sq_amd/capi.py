@@ -117,12 +117,12 @@ SIGNATURES = {
     "sq_bvh64_destroy": [P],
     "sq_bvh64_stats": [P, C.POINTER(TreeStats)],
     "sq_bvh64_set_layers": [P, P],
-    "sq_bvh64_raycast_nearest": [P, P, U64, F64, C.POINTER(Filter), P, P, P, P],
-    "sq_bvh64_raycast_any": [P, P, U64, F64, C.POINTER(Filter), P],
+    "sq_bvh64_raycast_nearest": [P, P, U64, F64, P, C.POINTER(Filter), P, P, P, P],
+    "sq_bvh64_raycast_any": [P, P, U64, F64, P, C.POINTER(Filter), P],
     "sq_bvh64_raycast_all": [P, P, U64, F64, C.POINTER(Filter), P, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_bvh64_overlap": [P, P, U64, C.POINTER(Filter), P, P, U64, C.POINTER(C.c_uint64)],
-    "sq_bvh64_shapecast_nearest": [P, P, U64, F64, C.POINTER(Filter), P, P, P],
-    "sq_bvh64_shapecast_all": [P, P, U64, F64, C.POINTER(Filter), P, P, P, U64, C.POINTER(C.c_uint64)],
+    "sq_bvh64_shapecast_nearest": [P, P, U64, F64, P, C.POINTER(Filter), P, P, P],
+    "sq_bvh64_shapecast_all": [P, P, U64, F64, P, C.POINTER(Filter), P, P, P, U64, C.POINTER(C.c_uint64)],
 }
 _RESTYPE = {"sq_last_error": C.c_char_p, "sq_ctx_stream": C.c_void_p}
 

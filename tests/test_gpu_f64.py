@@ -106,6 +106,20 @@ def test_f64_matches_oracle(ctx, dim, kind):
     wc = reff.shapecast_nearest(casts, max_toi, mask=0b10, exclude=filt.exclude)
     assert np.array_equal(host(cn["leaf"]).astype(np.int64), wc["leaf"])
     assert np.array_equal(host(cn["toi"]).view(np.uint64), wc["t"].view(np.uint64))
+    # per-query limits
+    lim = np.random.default_rng(dim + 100).uniform(0.1, max_toi, len(rays))
+    got = b.raycast_nearest(dev(rays), max_toi, max_toi_per_ray=dev(lim))
+    want = reff.raycast_nearest(rays, lim)
+    assert np.array_equal(host(got["leaf"]).astype(np.int64), want["leaf"])
+    assert np.array_equal(host(got["toi"]).view(np.uint64), want["t"].view(np.uint64))
+    assert np.array_equal(host(b.raycast_any(dev(rays), max_toi, max_toi_per_ray=dev(lim))).astype(bool),
+                          want["leaf"] >= 0)
+    cn = b.shapecast_nearest(dev(casts), max_toi, max_toi_per_cast=dev(lim))
+    wc = reff.shapecast_nearest(casts, lim)
+    assert np.array_equal(host(cn["leaf"]).astype(np.int64), wc["leaf"])
+    assert np.array_equal(host(cn["toi"]).view(np.uint64), wc["t"].view(np.uint64))
+    assert_csr(b.shapecast_all(dev(casts[:1500]), max_toi, max_toi_per_cast=dev(lim[:1500])),
+               reff.shapecast_all(casts[:1500], lim[:1500]), "cast all, per-cast limits")
     b.close()
 
 

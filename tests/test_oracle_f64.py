@@ -217,3 +217,26 @@ def test_dyadic_shape_casts(kind):
             grown[:, dim] += hv
             wr = oracle.Scene64(dim, kind, grown).raycast_nearest(rays[sel], 60.0)
             assert np.array_equal(w["leaf"][sel], wr["leaf"]) and np.array_equal(w["t"][sel], wr["t"])
+
+
+def test_per_query_limits_equal_scalar_calls():
+    rng = np.random.default_rng(55)
+    prims = np.concatenate([rng.uniform(-20, 20, (1500, 3)), rng.uniform(0.3, 2.0, (1500, 1))], 1)
+    rays = np.concatenate([rng.uniform(-25, 25, (2000, 3)), rng.normal(size=(2000, 3))], 1)
+    casts = np.concatenate([rays, np.full((2000, 3), -0.3), np.full((2000, 3), 0.3)], 1)
+    lim = rng.choice([3.0, 12.0, 40.0], 2000)
+    s = oracle.Scene64(3, oracle.SPHERE, prims)
+    w = s.raycast_nearest(rays, lim)
+    c = s.shapecast_nearest(casts, lim)
+    a = s.shapecast_all(casts, lim)
+    off = a[0].astype(np.int64)
+    for v in (3.0, 12.0, 40.0):
+        sel = np.flatnonzero(lim == v)
+        ws = s.raycast_nearest(rays[sel], v)
+        assert np.array_equal(w["leaf"][sel], ws["leaf"]) and np.array_equal(w["t"][sel], ws["t"])
+        cs = s.shapecast_nearest(casts[sel], v)
+        assert np.array_equal(c["leaf"][sel], cs["leaf"]) and np.array_equal(c["t"][sel], cs["t"])
+        as_ = s.shapecast_all(casts[sel], v)
+        got = np.concatenate([a[1][off[i]:off[i + 1]] for i in sel])
+        assert np.array_equal(got, as_[1])
+    assert (w["leaf"] >= 0).sum() > 300 and (w["t"][w["leaf"] >= 0] <= lim[w["leaf"] >= 0]).all()
