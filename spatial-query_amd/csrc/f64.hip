@@ -18,9 +18,10 @@
 // where count 0 marks an interior child (ref = its pair index) and count > 0 a
 // leaf range [ref, ref + count) of the leaf-ordered primitives.  The per-lane
 // stack holds at most one entry per tree level.  Nearest and any-hit rays and
-// nearest shape casts run pooled: persistent waves whose lanes take the next ray from an atomic
-// counter as soon as their own ends, so a wave does not wait for its longest
-// walk; all hits, all cast contacts and overlap run one query per lane.
+// nearest shape casts run pooled: persistent waves whose lanes take the next
+// query from an atomic counter as soon as their own ends, so a wave does not
+// wait for its longest walk; all hits, all cast contacts and overlap run one
+// query per lane.
 #include "engine_internal.h"
 #include "pool.h"
 
@@ -53,46 +54,35 @@ __device__ __forceinline__ bool hit_before64(double t, uint32_t leaf, double bt,
 
 template <int D, int KIND> constexpr int stride64() { return KIND == KIND_AABB ? 2 * D : D + 1; }
 
-// Walks every pair node whose child box passes node_pass, near child first when
-// both pass (entries from the pass test), and calls leaf(first, count) for
-// every passing leaf range.  prune(entry) skips a popped subtree whose entry
-// lower bound lies beyond the current limit.  Returns early when leaf() asks to stop.
-template <int D, class NodePass, class Leaf, class Prune>
-__device__ __forceinline__ void walk64(const View64& v, NodePass&& node_pass, Leaf&& leaf, Prune&& prune) {
+// Visits every leaf range whose boxes (and every ancestor's) pass node_pass,
+// calling leaf(first, count) for each: the walk of the variable-length
+// queries, whose limit never shrinks, so the order does not matter.
+template <int D, class NodePass, class Leaf>
+__device__ __forceinline__ void walk64(const View64& v, NodePass&& node_pass, Leaf&& leaf) {
   if (v.n_prims == 0) return;
-  double te;
-  if (!node_pass(v.root, v.root + D, te)) return;
+  if (!node_pass(v.root, v.root + D)) return;
   if (v.root_count) { leaf(v.root_ref, v.root_count); return; }
   uint32_t stk[F64_MAX_STACK];
-  float stt[F64_MAX_STACK];
   int sp = 0;
   uint32_t node = v.root_ref;
   while (true) {
     const double* b = v.nodes + (size_t)node * (4 * D + 2);
     const uint4 rf = *reinterpret_cast<const uint4*>(b + 4 * D);
-    double ta = 0.0, tb = 0.0;
-    bool ha = node_pass(b, b + D, ta);
-    bool hb = node_pass(b + 2 * D, b + 3 * D, tb);
-    if (ha && rf.z) { if (leaf(rf.x, rf.z)) return; ha = false; }
-    if (hb && rf.w) { if (leaf(rf.y, rf.w)) return; hb = false; }
+    bool ha = node_pass(b, b + D);
+    bool hb = node_pass(b + 2 * D, b + 3 * D);
+    if (ha && rf.z) { leaf(rf.x, rf.z); ha = false; }
+    if (hb && rf.w) { leaf(rf.y, rf.w); hb = false; }
     if (ha && hb) {
-      const bool a_first = ta <= tb;
-      // the entry stored rounded toward -inf, so the pop test never prunes a subtree the limit still reaches
-      stk[sp] = a_first ? rf.y : rf.x;
-      stt[sp] = __double2float_rd(a_first ? tb : ta);
-      ++sp;
-      node = a_first ? rf.x : rf.y;
+      stk[sp++] = rf.y;
+      node = rf.x;
     } else if (ha) {
       node = rf.x;
     } else if (hb) {
       node = rf.y;
+    } else if (sp > 0) {
+      node = stk[--sp];
     } else {
-      bool found = false;
-      while (sp > 0) {
-        --sp;
-        if (!prune((double)stt[sp])) { node = stk[sp]; found = true; break; }
-      }
-      if (!found) return;
+      return;
     }
   }
 }
@@ -303,7 +293,7 @@ __global__ __launch_bounds__(256) void k64_collect(View64 v, const double* __res
     const double* rmx = rmn + D;
     walk64<D>(
         v,
-        [&](const double* mn, const double* mx, double& te) { te = 0.0; return box_overlap<D>(mn, mx, rmn, rmx); },
+        [&](const double* mn, const double* mx) { return box_overlap<D>(mn, mx, rmn, rmx); },
         [&](uint32_t first, uint32_t count) {
           for (uint32_t k = first; k < first + count; ++k) {
             const uint32_t id = v.ids[k];
@@ -312,17 +302,15 @@ __global__ __launch_bounds__(256) void k64_collect(View64 v, const double* __res
             if (fill) out_leaf[at++] = (int32_t)id;
             ++cnt;
           }
-          return false;
-        },
-        [](double) { return false; });
+        });
   } else {
     Pol64<D, KIND, CAST> pol;
     pol.setup(q + i * Pol64<D, KIND, CAST>::QS);
     const double lim = tmax ? tmax[i] : max_toi;
     walk64<D>(
         v,
-        [&](const double* mn, const double* mx, double& te) {
-          double tx; int ax;
+        [&](const double* mn, const double* mx) {
+          double te, tx; int ax;
           if constexpr (CAST) return pol.c.grown_slab(mn, mx, lim, te, tx);
           else return slab_exact<D>(pol.r, mn, mx, lim, te, tx, ax);
         },
@@ -335,9 +323,7 @@ __global__ __launch_bounds__(256) void k64_collect(View64 v, const double* __res
             if (fill) { out_leaf[at] = (int32_t)id; out_toi[at] = t; ++at; }
             ++cnt;
           }
-          return false;
-        },
-        [](double) { return false; });
+        });
   }
   if (!fill) { counts[i] = cnt; return; }
   // insertion sort of this query's range (result lists are short)
