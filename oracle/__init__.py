@@ -79,6 +79,7 @@ def _declare(L):
         "orc64_raycast_all": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, I32, P, P, P, U64]),
         "orc64_overlap": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, I32, P, P, U64]),
         "orc64_tree_stats": (None, [I32, I32, P, U32, P, P]),
+        "orc64_bvh_raycast_nearest": (None, [I32, I32, P, U32, P, U64, C.c_double, I32, P, P, P]),
         "orc64_shapecast_nearest": (None, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, P, I32, P, P, P]),
         "orc64_shapecast_all": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, P, I32, P, P, P, U64]),
         "orc_inst_new": (P, [I32, I32, U64, P, P, P, U32, P, P, P, P]),
@@ -368,6 +369,17 @@ class Scene64:
                                                                 _p(self.layers), mask, exclude, _p(c), c.shape[0],
                                                                 m, _p(per), threads, _p(off), _p(leaf), _p(t), cap)
         return self._collect(fn, c, True)
+
+    def bvh_raycast_nearest(self, rays, max_toi, threads=1):
+        """Bvh::raycast_nearest in reference order over the f64 tree, as
+        raycast_nearest_batch_parallel (contiguous chunks); with the node boxes
+        and primitives tested, summed over the rays."""
+        r = self._q(rays)
+        n = r.shape[0]
+        leaf = np.empty(n, np.int64); t = np.empty(n); counts = np.zeros(2, np.uint64)
+        lib().orc64_bvh_raycast_nearest(self.dim, self.kind, _p(self.prims), self.n, _p(r), n, float(max_toi), threads,
+                                        _p(leaf), _p(t), _p(counts))
+        return dict(leaf=leaf, t=t, nodes_tested=int(counts[0]), prims_tested=int(counts[1]))
 
     def tree_stats(self):
         out = np.zeros(5, np.uint64); q = np.zeros(1)

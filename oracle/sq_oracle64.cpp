@@ -14,6 +14,10 @@
 //   * ShapeCast broadphase         sq/src/query/shapecast.rs:70-73 (box grown by the probe)
 //   * hit_ordering                 sq/src/query/hit.rs:77-84 (f64::total_cmp, then leaf)
 //   * tree::build and TreeStats    sq/src/accel/tree.rs:39-49,81-163
+//   * Bvh::raycast_nearest         sq/src/accel/bvh.rs:51-124 (push a, push b, LIFO, shrinking
+//                                  limit) with the QueryStats counters of stats.rs:39-50,
+//                                  over raycast_nearest_batch_parallel's contiguous chunks
+//                                  (bvh.rs:160-193): the timed CPU baseline
 // over the built-in BOX and SPHERE providers (box-guarded sphere, as the
 // engine's: the hit is kept only inside the slab interval of its own box).
 // BVH == naive scan is the reference's stated contract (sq/src/lib.rs:15-17),
@@ -355,6 +359,52 @@ static void tree_stats(const double* prims, int kind, uint32_t n, uint64_t* out5
   *quality = q;
 }
 
+// A built f64 tree over one geometry: tree.rs:112-163 order, plus prim_indices.
+template <int D> struct Tree64 {
+  std::vector<Node> nodes;
+  std::vector<uint32_t> idx;
+  Tree64(const Geom<D>& g) {
+    std::vector<double> boxes((size_t)g.n * 2 * D);
+    for (uint32_t i = 0; i < g.n; ++i) g.world_aabb(i, boxes.data() + (size_t)i * 2 * D, boxes.data() + (size_t)i * 2 * D + D);
+    idx.resize(g.n);
+    for (uint32_t i = 0; i < g.n; ++i) idx[i] = i;
+    if (g.n) build<D>(boxes, idx, 0, g.n, nodes);
+  }
+  // bvh.rs:51-124; counts nodes tested and primitives tested (stats.rs:39-50)
+  bool nearest(const Geom<D>& g, const Ray& r, double max_toi, int64_t* leaf_out, double* t_out, uint64_t* n_nodes,
+               uint64_t* n_prims) const {
+    if (nodes.empty()) return false;
+    bool have = false;
+    double limit = max_toi, best_t = 0.0;
+    int64_t best_leaf = -1;
+    double nrm[D];
+    std::vector<uint32_t> stack(1, 0u);
+    while (!stack.empty()) {
+      const uint32_t ni = stack.back();
+      stack.pop_back();
+      const Node& nd = nodes[ni];
+      ++*n_nodes;
+      double te, tx; int ax;
+      if (!ray_intersect<D>(nd.b.data(), nd.b.data() + D, r, limit, &te, &tx, &ax) || !(te <= limit)) continue;
+      if (nd.pc > 0) {
+        for (uint32_t k = nd.a; k < nd.a + nd.pc; ++k) {
+          const size_t leaf = idx[k];
+          ++*n_prims;
+          double t;
+          if (!g.test_ray(leaf, r, limit, &t, nrm)) continue;
+          if (have && !hit_less(t, (int64_t)leaf, best_t, best_leaf)) continue;
+          have = true; best_t = t; best_leaf = (int64_t)leaf; limit = t;
+        }
+      } else {
+        stack.push_back(nd.a);
+        stack.push_back(nd.c);
+      }
+    }
+    *leaf_out = best_leaf; *t_out = best_t;
+    return have;
+  }
+};
+
 template <class F> static void parallel_for(uint64_t n, int threads, F&& f) {
   if (threads <= 1 || n < 256) { for (uint64_t i = 0; i < n; ++i) f(i); return; }
   std::vector<std::thread> th;
@@ -435,6 +485,33 @@ static uint64_t all_batch(int kind, const double* prims, uint32_t n, const uint6
   return tot;
 }
 
+// raycast_nearest_batch_parallel (bvh.rs:160-193): contiguous chunks, one per thread
+template <int D>
+static void bvh_nearest_batch(int kind, const double* prims, uint32_t n, const double* rays, uint64_t nr, double max_toi,
+                              int threads, int64_t* leaf, double* t, uint64_t* counts) {
+  Geom<D> g{kind, n, prims, nullptr};
+  const Tree64<D> tree(g);
+  std::vector<uint64_t> nodes(std::max(threads, 1), 0), prs(std::max(threads, 1), 0);
+  const int T = std::max(threads, 1);
+  const uint64_t chunk = (nr + T - 1) / T;
+  std::vector<std::thread> th;
+  for (int w = 0; w < T; ++w) {
+    th.emplace_back([&, w] {
+      const uint64_t a = w * chunk, b = std::min<uint64_t>(nr, a + chunk);
+      for (uint64_t i = a; i < b; ++i) {
+        Ray r{rays + i * 2 * D, rays + i * 2 * D + D};
+        int64_t l; double tt;
+        const bool have = tree.nearest(g, r, max_toi, &l, &tt, &nodes[w], &prs[w]);
+        leaf[i] = have ? l : -1;
+        t[i] = have ? tt : INFINITY;
+      }
+    });
+  }
+  for (auto& x : th) x.join();
+  counts[0] = counts[1] = 0;
+  for (int w = 0; w < T; ++w) { counts[0] += nodes[w]; counts[1] += prs[w]; }
+}
+
 }  // namespace orc64
 
 using namespace orc64;
@@ -485,6 +562,12 @@ uint64_t orc64_shapecast_all(int D, int kind, const double* prims, uint32_t n, c
   uint64_t tot = 0;
   ORC64_DISPATCH(D, tot = all_batch<DD>(kind, prims, n, layers, mask, exclude, casts, nc, max_toi, tmax, 2, threads, offsets, leaf, t, cap));
   return tot;
+}
+
+// counts[2]: node boxes tested, primitives tested (summed over the rays)
+void orc64_bvh_raycast_nearest(int D, int kind, const double* prims, uint32_t n, const double* rays, uint64_t nr,
+                               double max_toi, int threads, int64_t* leaf, double* t, uint64_t* counts) {
+  ORC64_DISPATCH(D, bvh_nearest_batch<DD>(kind, prims, n, rays, nr, max_toi, threads, leaf, t, counts));
 }
 
 void orc64_tree_stats(int D, int kind, const double* prims, uint32_t n, uint64_t* out5, double* quality) {

@@ -240,3 +240,27 @@ def test_per_query_limits_equal_scalar_calls():
         got = np.concatenate([a[1][off[i]:off[i + 1]] for i in sel])
         assert np.array_equal(got, as_[1])
     assert (w["leaf"] >= 0).sum() > 300 and (w["t"][w["leaf"] >= 0] <= lim[w["leaf"] >= 0]).all()
+
+
+@pytest.mark.parametrize("dim,kind", [(2, oracle.BOX), (3, oracle.SPHERE), (5, oracle.BOX), (6, oracle.SPHERE)])
+def test_bvh_walk_equals_naive(dim, kind):
+    """The reference's contract (sq/src/lib.rs:15-17): the BVH answers what the
+    naive scan does, here in double over the f64 tree; and the parallel batch
+    equals the sequential one."""
+    rng = np.random.default_rng(70 + dim + kind)
+    span = {2: 10.0, 3: 10.0, 5: 3.0, 6: 2.5}[dim]   # denser in more dimensions, so rays still hit
+    c = rng.uniform(-span, span, (2500, dim))
+    if kind == oracle.BOX:
+        prims = np.concatenate([c, c + rng.uniform(0.2, 2.0, (2500, dim))], 1)
+    else:
+        prims = np.concatenate([c, rng.uniform(0.2, 1.5, (2500, 1))], 1)
+    rays = np.concatenate([rng.uniform(-span, span, (3000, dim)), rng.normal(size=(3000, dim))], 1)
+    s = oracle.Scene64(dim, kind, prims)
+    want = s.raycast_nearest(rays, 50.0)
+    got = s.bvh_raycast_nearest(rays, 50.0, threads=1)
+    par = s.bvh_raycast_nearest(rays, 50.0, threads=7)
+    assert (want["leaf"] >= 0).sum() > 500
+    assert np.array_equal(got["leaf"], want["leaf"]) and np.array_equal(got["t"], want["t"])
+    assert np.array_equal(par["leaf"], got["leaf"]) and np.array_equal(par["t"], got["t"])
+    assert par["nodes_tested"] == got["nodes_tested"] and par["prims_tested"] == got["prims_tested"]
+    assert got["nodes_tested"] < 3000 * s.tree_stats()["node_count"] // 4   # the tree prunes
