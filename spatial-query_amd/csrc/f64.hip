@@ -20,8 +20,8 @@
 // stack holds at most one entry per tree level.  Nearest and any-hit rays and
 // nearest shape casts run pooled: persistent waves whose lanes take the next
 // query from an atomic counter as soon as their own ends, so a wave does not
-// wait for its longest walk; all hits, all cast contacts and overlap run one
-// query per lane.
+// wait for its longest walk (all hits and all cast contacts too: counted in
+// one pass, written in a second); overlap runs one region per lane.
 #include "engine_internal.h"
 #include "pool.h"
 
@@ -156,18 +156,44 @@ template <int D, int KIND> struct Pol64<D, KIND, true> {
   __device__ bool narrow(const double* p, double limit, double& t) const { return c.narrow(p, limit, t); }
 };
 
+// Insertion sort of one query's CSR range [b0, b1): ray and cast hits by
+// (toi total order, leaf) as hit_ordering, overlap ids ascending (result lists
+// are short).
+template <bool OVERLAP>
+__device__ void sort_range64(int32_t* out_leaf, double* out_toi, uint64_t b0, uint64_t b1) {
+  for (uint64_t a = b0 + 1; a < b1; ++a) {
+    const int32_t l = out_leaf[a];
+    const double t = OVERLAP ? 0.0 : out_toi[a];
+    uint64_t j = a;
+    while (j > b0) {
+      const int32_t pl = out_leaf[j - 1];
+      const bool after = OVERLAP ? (pl > l) : hit_before64(t, (uint32_t)l, out_toi[j - 1], (uint32_t)pl);
+      if (!after) break;
+      out_leaf[j] = pl;
+      if (!OVERLAP) out_toi[j] = out_toi[j - 1];
+      --j;
+    }
+    out_leaf[j] = l;
+    if (!OVERLAP) out_toi[j] = t;
+  }
+}
+
 // Bvh::raycast_nearest (bvh.rs:51-124), the any-hit idiom and
-// Bvh::shapecast_nearest (bvh.rs:331-375), pooled.  Each
+// Bvh::shapecast_nearest (bvh.rs:331-375), pooled; with ALL, Bvh::raycast_all
+// (bvh.rs:196-259) and shapecast_all (bvh.rs:378-414): the limit stays the
+// query's max_toi and every accepted hit is counted (offsets null: counts[q]
+// = hits) or written into the query's CSR range and sorted (offsets given).  Each
 // loop iteration refills the wave's idle lanes from the counter (one atomic per
 // wave) and advances every walking lane by one pair node.  The wave leaves the
 // loop once the counter has passed n and none of its lanes walks; every wave
 // then retires once (pool_retire), the last one clearing the counter.
-template <int D, int KIND, bool ANY, bool CAST>
+template <int D, int KIND, bool ANY, bool CAST, bool ALL = false>
 __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const double* __restrict__ rays, uint32_t n,
                                                         double max_toi, const double* __restrict__ tmax, Filter f,
                                                         uint32_t* counter, int32_t* out_leaf,
                                                         double* out_toi, double* out_point, double* out_normal,
-                                                        uint8_t* out_any, SpillEntry* spill, uint32_t lanes) {
+                                                        uint8_t* out_any, SpillEntry* spill, uint32_t lanes,
+                                                        uint32_t* counts = nullptr, const uint64_t* offsets = nullptr) {
   constexpr int PS = stride64<D, KIND>();
   const uint32_t lane = threadIdx.x & 63u;
   // the newest F64_RING entries in LDS ([slot][thread] refs, then entries), older ones in this lane's spill column
@@ -176,15 +202,22 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
                      blockDim.x, F64_RING, spill + (size_t)blockIdx.x * blockDim.x + threadIdx.x, lanes};
   Pol64<D, KIND, CAST> q;
   double best = 0.0, lim = 0.0;   // lim: this query's max_toi
-  uint32_t bl = NO_LEAF, bk = 0, node = 0, sp = 0, ray = 0;
+  uint32_t bl = NO_LEAF, bk = 0, node = 0, sp = 0, ray = 0, cnt = 0;
+  uint64_t at = 0;   // ALL: the next CSR slot of this query
   bool active = false, drained = false;
-  // leaf range test: the nearest accepted hit so far; true ends an any-hit walk
+  // leaf range test: the nearest accepted hit so far (ALL: every accepted hit);
+  // true ends an any-hit walk
   auto test_leaf = [&](uint32_t first, uint32_t count) {
     for (uint32_t k = first; k < first + count; ++k) {
       const uint32_t id = v.ids[k];
       if (!accepts64(v, f, id)) continue;
       double t;
       if (!q.narrow(v.prims + (size_t)k * PS, best, t)) continue;
+      if constexpr (ALL) {
+        if (offsets) { out_leaf[at] = (int32_t)id; out_toi[at] = t; ++at; }
+        ++cnt;
+        continue;
+      }
       if (!hit_before64(t, id, best, bl)) continue;
       best = t; bl = id; bk = k;
       if (ANY) return true;
@@ -194,6 +227,11 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
   auto finish = [&]() {
     const RayD<D, double>& r = q.ray();
     active = false;
+    if constexpr (ALL) {
+      if (offsets) sort_range64<false>(out_leaf, out_toi, offsets[ray], offsets[ray + 1]);
+      else counts[ray] = cnt;
+      return;
+    }
     const bool hit = bl != NO_LEAF;
     if (ANY) { out_any[ray] = hit; return; }
     out_leaf[ray] = hit ? (int32_t)bl : -1;
@@ -216,11 +254,11 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
       const uint64_t idle = __ballot(!active);
       if (idle) {
         const int leader = __ffsll((unsigned long long)idle) - 1;
-        const uint32_t cnt = (uint32_t)__popcll(idle);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
         uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(counter, cnt);
+        if ((int)lane == leader) base = atomicAdd(counter, n_idle);
         base = __shfl(base, leader);
-        if ((uint64_t)base + cnt >= n) drained = true;
+        if ((uint64_t)base + n_idle >= n) drained = true;
         if (!active) {
           const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
           if (my < n) {
@@ -228,6 +266,7 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
             q.setup(rays + (size_t)my * Pol64<D, KIND, CAST>::QS);
             lim = tmax ? tmax[my] : max_toi;
             best = lim; bl = NO_LEAF; sp = 0;
+            if constexpr (ALL) { cnt = 0; at = offsets ? offsets[my] : 0; }
             const bool in_root = v.n_prims && q.root(v.root, lim);
             if (in_root && !v.root_count) {
               node = v.root_ref;
@@ -274,75 +313,33 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
   pool_retire(counter);
 }
 
-// Bvh::raycast_all (bvh.rs:196-259) and Bvh::overlap (bvh.rs:418-456): one
-// pass counts each query's results, the second writes them into its CSR range
-// and sorts the range (rays by (toi total order, leaf), overlap by leaf).
-template <int D, int KIND, bool OVERLAP, bool CAST = false>
-__global__ __launch_bounds__(256) void k64_collect(View64 v, const double* __restrict__ q, uint64_t n, double max_toi,
-                                                   const double* __restrict__ tmax, Filter f, uint32_t* counts,
-                                                   const uint64_t* offsets,
-                                                   int32_t* out_leaf, double* out_toi) {
+// Bvh::overlap (bvh.rs:418-456), one query per lane: one pass counts each
+// query's leaves, the second writes them into its CSR range and sorts them.
+template <int D, int KIND>
+__global__ __launch_bounds__(256) void k64_overlap(View64 v, const double* __restrict__ q, uint64_t n, Filter f,
+                                                   uint32_t* counts, const uint64_t* offsets, int32_t* out_leaf) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   constexpr int PS = stride64<D, KIND>();
   const bool fill = offsets != nullptr;
   uint64_t at = fill ? offsets[i] : 0;
   uint32_t cnt = 0;
-  if constexpr (OVERLAP) {
-    const double* rmn = q + i * 2 * D;
-    const double* rmx = rmn + D;
-    walk64<D>(
-        v,
-        [&](const double* mn, const double* mx) { return box_overlap<D>(mn, mx, rmn, rmx); },
-        [&](uint32_t first, uint32_t count) {
-          for (uint32_t k = first; k < first + count; ++k) {
-            const uint32_t id = v.ids[k];
-            if (!accepts64(v, f, id)) continue;
-            if (!overlap_narrow<D, KIND>(v.prims + (size_t)k * PS, rmn, rmx)) continue;
-            if (fill) out_leaf[at++] = (int32_t)id;
-            ++cnt;
-          }
-        });
-  } else {
-    Pol64<D, KIND, CAST> pol;
-    pol.setup(q + i * Pol64<D, KIND, CAST>::QS);
-    const double lim = tmax ? tmax[i] : max_toi;
-    walk64<D>(
-        v,
-        [&](const double* mn, const double* mx) {
-          double te, tx; int ax;
-          if constexpr (CAST) return pol.c.grown_slab(mn, mx, lim, te, tx);
-          else return slab_exact<D>(pol.r, mn, mx, lim, te, tx, ax);
-        },
-        [&](uint32_t first, uint32_t count) {
-          for (uint32_t k = first; k < first + count; ++k) {
-            const uint32_t id = v.ids[k];
-            if (!accepts64(v, f, id)) continue;
-            double t;
-            if (!pol.narrow(v.prims + (size_t)k * PS, lim, t)) continue;
-            if (fill) { out_leaf[at] = (int32_t)id; out_toi[at] = t; ++at; }
-            ++cnt;
-          }
-        });
-  }
+  const double* rmn = q + i * 2 * D;
+  const double* rmx = rmn + D;
+  walk64<D>(
+      v,
+      [&](const double* mn, const double* mx) { return box_overlap<D>(mn, mx, rmn, rmx); },
+      [&](uint32_t first, uint32_t count) {
+        for (uint32_t k = first; k < first + count; ++k) {
+          const uint32_t id = v.ids[k];
+          if (!accepts64(v, f, id)) continue;
+          if (!overlap_narrow<D, KIND>(v.prims + (size_t)k * PS, rmn, rmx)) continue;
+          if (fill) out_leaf[at++] = (int32_t)id;
+          ++cnt;
+        }
+      });
   if (!fill) { counts[i] = cnt; return; }
-  // insertion sort of this query's range (result lists are short)
-  const uint64_t b0 = offsets[i], b1 = offsets[i + 1];
-  for (uint64_t a = b0 + 1; a < b1; ++a) {
-    const int32_t l = out_leaf[a];
-    const double t = OVERLAP ? 0.0 : out_toi[a];
-    uint64_t j = a;
-    while (j > b0) {
-      const int32_t pl = out_leaf[j - 1];
-      const bool after = OVERLAP ? (pl > l) : hit_before64(t, (uint32_t)l, out_toi[j - 1], (uint32_t)pl);
-      if (!after) break;
-      out_leaf[j] = pl;
-      if (!OVERLAP) out_toi[j] = out_toi[j - 1];
-      --j;
-    }
-    out_leaf[j] = l;
-    if (!OVERLAP) out_toi[j] = t;
-  }
+  sort_range64<true>(out_leaf, nullptr, offsets[i], offsets[i + 1]);
 }
 
 template <class F> int dispatch64(int dim, int kind, F&& f) {
@@ -382,6 +379,29 @@ Filter filter64(const sq_bvh64* b, const sq_filter* f) {
 }
 
 }  // namespace
+
+// One pooled launch over n queries (the grid fills the chip; lanes refill from
+// the context's pool counter, which every launch leaves zero).
+template <int D, int KIND, bool ANY, bool CAST, bool ALL>
+static int pool_launch(sq_bvh64* b, const double* q, uint64_t n, double max_toi, const double* tmax, const Filter& fd,
+                       int32_t* out_leaf, double* out_toi, double* out_point, double* out_normal, uint8_t* out_any,
+                       uint32_t* counts, const uint64_t* offsets) {
+  sq_ctx* c = b->ctx;
+  const void* kern = reinterpret_cast<const void*>(&k64_nearest_pool<D, KIND, ANY, CAST, ALL>);
+  const size_t lds = (size_t)F64_RING * F64_BLOCK * 8;
+  int per_cu = 0;
+  if (int r0 = blocks_per_cu(kern, c->device, F64_BLOCK, lds, &per_cu)) return r0;
+  const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), grid_for(n, F64_BLOCK)));
+  const uint64_t lanes = grid * F64_BLOCK;
+  // spill rows: the stack holds at most depth - 2 entries, F64_RING of them in LDS
+  const uint64_t rows = b->view.stack > F64_RING ? b->view.stack - F64_RING : 0;
+  if (int r1 = c->spill.ensure((size_t)rows * lanes * sizeof(SpillEntry) + 64)) return r1;
+  k64_nearest_pool<D, KIND, ANY, CAST, ALL><<<(uint32_t)grid, F64_BLOCK, lds, c->stream>>>(
+      b->view, q, (uint32_t)n, max_toi, tmax, fd, c->pool_ctr.as<uint32_t>(), out_leaf, out_toi, out_point, out_normal,
+      out_any, c->spill.as<SpillEntry>(), (uint32_t)lanes, counts, offsets);
+  HIP_TRY(hipGetLastError());
+  return SQ_OK;
+}
 
 extern "C" {
 
@@ -521,29 +541,17 @@ static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi
   if (!b || (n && !rays)) return fail(SQ_ERR_INVALID, "NULL argument");
   if (n && !out_any && (!out_leaf || !out_toi)) return fail(SQ_ERR_INVALID, "output arrays are NULL");
   if (!n) return SQ_OK;
-  if (n >= (1ull << 31)) return fail(SQ_ERR_INVALID, "more than 2^31 rays in one call");
-  sq_ctx* c = b->ctx;
+  if (n >= (1ull << 31)) return fail(SQ_ERR_INVALID, "more than 2^31 queries in one call");
   const Filter fd = filter64(b, f);
-  uint32_t* counter = c->pool_ctr.as<uint32_t>();
   return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
-    auto launch = [&]<bool ANY, bool CAST>() -> int {
-      const void* kern = reinterpret_cast<const void*>(&k64_nearest_pool<D, KIND, ANY, CAST>);
-      const size_t lds = (size_t)F64_RING * F64_BLOCK * 8;
-      int per_cu = 0;
-      if (int r0 = blocks_per_cu(kern, c->device, F64_BLOCK, lds, &per_cu)) return r0;
-      const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), grid_for(n, F64_BLOCK)));
-      const uint64_t lanes = grid * F64_BLOCK;
-      // spill rows: the stack holds at most depth - 1 entries, F64_RING of them in LDS
-      const uint64_t rows = b->view.stack > F64_RING ? b->view.stack - F64_RING : 0;
-      if (int r1 = c->spill.ensure((size_t)rows * lanes * sizeof(SpillEntry) + 64)) return r1;
-      k64_nearest_pool<D, KIND, ANY, CAST><<<(uint32_t)grid, F64_BLOCK, lds, c->stream>>>(
-          b->view, rays, (uint32_t)n, max_toi, tmax, fd, counter, out_leaf, out_toi, out_point, out_normal, out_any,
-          c->spill.as<SpillEntry>(), (uint32_t)lanes);
-      HIP_TRY(hipGetLastError());
-      return SQ_OK;
-    };
-    if (cast) return launch.template operator()<false, true>();
-    return out_any ? launch.template operator()<true, false>() : launch.template operator()<false, false>();
+    if (cast)
+      return pool_launch<D, KIND, false, true, false>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi, out_point,
+                                                      nullptr, nullptr, nullptr, nullptr);
+    if (out_any)
+      return pool_launch<D, KIND, true, false, false>(b, rays, n, max_toi, tmax, fd, nullptr, nullptr, nullptr, nullptr,
+                                                      out_any, nullptr, nullptr);
+    return pool_launch<D, KIND, false, false, false>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi, out_point,
+                                                     out_normal, nullptr, nullptr, nullptr);
   });
 }
 
@@ -577,22 +585,24 @@ static int collect64(sq_bvh64* b, const double* q, uint64_t n, double max_toi, c
     HIP_TRY(hipMemsetAsync(offsets, 0, 8, c->stream));
     return SQ_OK;
   }
+  if (n >= (1ull << 31)) return fail(SQ_ERR_INVALID, "more than 2^31 queries in one call");
   int rc;
   if ((rc = c->counts.ensure(n * 4))) return rc;
   const Filter fd = filter64(b, f);
+  uint32_t* counts = c->counts.as<uint32_t>();
+  // rays and casts: the pooled walk (count pass, then fill pass); overlap: one region per lane
   auto launch = [&](const uint64_t* offs) -> int {
     return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
-      if (mode == 1)
-        k64_collect<D, KIND, true><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, tmax, fd,
-                                                                            c->counts.as<uint32_t>(), offs, out_leaf, out_toi);
-      else if (mode == 2)
-        k64_collect<D, KIND, false, true><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, tmax, fd,
-                                                                                   c->counts.as<uint32_t>(), offs, out_leaf, out_toi);
-      else
-        k64_collect<D, KIND, false><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, max_toi, tmax, fd,
-                                                                             c->counts.as<uint32_t>(), offs, out_leaf, out_toi);
-      HIP_TRY(hipGetLastError());
-      return SQ_OK;
+      if (mode == 1) {
+        k64_overlap<D, KIND><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, fd, counts, offs, out_leaf);
+        HIP_TRY(hipGetLastError());
+        return SQ_OK;
+      }
+      if (mode == 2)
+        return pool_launch<D, KIND, false, true, true>(b, q, n, max_toi, tmax, fd, out_leaf, out_toi, nullptr, nullptr,
+                                                       nullptr, counts, offs);
+      return pool_launch<D, KIND, false, false, true>(b, q, n, max_toi, tmax, fd, out_leaf, out_toi, nullptr, nullptr,
+                                                      nullptr, counts, offs);
     });
   };
   if ((rc = launch(nullptr))) return rc;
