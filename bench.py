@@ -56,7 +56,8 @@ def main():
     ap.add_argument("--f64", action="store_true", help="measure the f64 feature (1M balls, 4M rays) instead")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not produced (see DESIGN.md)",
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not produced: rays are "
+                                   "independent, so N GPUs would each take a contiguous ray range of a replicated scene",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.f64:
