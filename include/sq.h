@@ -503,6 +503,21 @@ int sq_bvh64_shapecast_all(sq_bvh64* bvh, const double* casts, uint64_t n, doubl
 /* Bvh::overlap (bvh.rs:418-456): regions double[n][2D], CSR as sq_overlap. */
 int sq_bvh64_overlap(sq_bvh64* bvh, const double* regions, uint64_t n, const sq_filter* filter, uint64_t* offsets,
                      int32_t* out_leaf, uint64_t capacity, uint64_t* total);
+/* Tlas over box instances under the `f64` feature (tlas.rs:35-56, the
+ * two_level.rs box instances): host rot double[n][D][D] (row-major rotation),
+ * trans double[n][D], halves double[n][D] (local box [-half, half]); D = 2..6.
+ * World boxes are Isometry::transform_aabb (isometry.rs:84-97) in double. */
+typedef struct sq_tlas64 sq_tlas64;
+int sq_tlas64_create(sq_ctx* ctx, int dim, uint64_t n, const double* rot, const double* trans, const double* halves,
+                     sq_tlas64** out);
+int sq_tlas64_destroy(sq_tlas64* tlas);
+int sq_tlas64_stats(const sq_tlas64* tlas, sq_tree_stats* out);
+/* Tlas::raycast_nearest (tlas.rs:104-154) as sq_tlas_raycast_nearest: the
+ * instance id and toi of the nearest hit (out_point double[n][D] may be NULL). */
+int sq_tlas64_raycast_nearest(sq_tlas64* tlas, const double* rays, uint64_t n, double max_toi,
+                              const double* max_toi_per_ray, int32_t* out_inst, double* out_toi, double* out_point);
+int sq_tlas64_raycast_any(sq_tlas64* tlas, const double* rays, uint64_t n, double max_toi,
+                          const double* max_toi_per_ray, uint8_t* out_hit);
 
 #ifdef __cplusplus
 }

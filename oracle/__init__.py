@@ -80,6 +80,7 @@ def _declare(L):
         "orc64_overlap": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, I32, P, P, U64]),
         "orc64_tree_stats": (None, [I32, I32, P, U32, P, P]),
         "orc64_bvh_raycast_nearest": (None, [I32, I32, P, U32, P, U64, C.c_double, I32, P, P, P]),
+        "orc64_tlas_box_nearest": (None, [I32, U64, P, P, P, P, U64, C.c_double, I32, P, P, P]),
         "orc64_shapecast_nearest": (None, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, P, I32, P, P, P]),
         "orc64_shapecast_all": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, P, I32, P, P, P, U64]),
         "orc_inst_new": (P, [I32, I32, U64, P, P, P, U32, P, P, P, P]),
@@ -386,6 +387,29 @@ class Scene64:
         lib().orc64_tree_stats(self.dim, self.kind, _p(self.prims), self.n, _p(out), _p(q))
         return dict(node_count=int(out[0]), leaf_count=int(out[1]), prim_count=int(out[2]), max_depth=int(out[3]),
                     bytes=int(out[4]), quality=float(q[0]))
+
+
+class BoxInstances64:
+    """Box instances under the `f64` feature (two_level.rs:70-167 in double):
+    the naive instance scan with the world-box guard, and the world boxes."""
+
+    def __init__(self, dim, rot, trans, halves):
+        self.dim = dim
+        self.rot = np.ascontiguousarray(rot, np.float64).reshape(-1, dim, dim)
+        self.trans = np.ascontiguousarray(trans, np.float64).reshape(-1, dim)
+        self.halves = np.ascontiguousarray(halves, np.float64).reshape(-1, dim)
+        self.n = self.rot.shape[0]
+
+    def raycast_nearest(self, rays, max_toi, threads=8, world=False):
+        r = np.ascontiguousarray(rays, np.float64).reshape(-1, 2 * self.dim)
+        nr = r.shape[0]
+        leaf = np.empty(nr, np.int64); t = np.empty(nr)
+        w = np.empty((self.n, 2 * self.dim)) if world else None
+        lib().orc64_tlas_box_nearest(self.dim, self.n, _p(self.rot), _p(self.trans), _p(self.halves), _p(r), nr,
+                                     float(max_toi), threads, _p(leaf), _p(t), _p(w))
+        out = dict(leaf=leaf, t=t)
+        if world: out["world"] = w
+        return out
 
 
 class InstanceScene:

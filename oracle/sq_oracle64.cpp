@@ -12,6 +12,9 @@
 //                                  :131-181 (shape casts, nearest and all),
 //                                  :184-205 (overlap)
 //   * ShapeCast broadphase         sq/src/query/shapecast.rs:70-73 (box grown by the probe)
+//   * box instances                two_level.rs:70-113 (ray_local_box), the naive instance scan
+//                                  of two_level.rs:126-167, Isometry inverse and transform_aabb
+//                                  (isometry.rs:20-124), kept inside the world box's slab interval
 //   * hit_ordering                 sq/src/query/hit.rs:77-84 (f64::total_cmp, then leaf)
 //   * tree::build and TreeStats    sq/src/accel/tree.rs:39-49,81-163
 //   * Bvh::raycast_nearest         sq/src/accel/bvh.rs:51-124 (push a, push b, LIFO, shrinking
@@ -512,6 +515,78 @@ static void bvh_nearest_batch(int kind, const double* prims, uint32_t n, const d
   for (int w = 0; w < T; ++w) { counts[0] += nodes[w]; counts[1] += prs[w]; }
 }
 
+// Box instances: R [D][D] row-major, t [D], half [D] each.
+template <int D> struct Inst64 {
+  uint64_t n;
+  const double *rot, *trans, *half;
+  // Isometry::transform_aabb of [-half, half]: centre R c + t, half extent |R| h
+  void world_box(uint64_t i, double* mn, double* mx) const {
+    const double *R = rot + i * D * D, *t = trans + i * D, *h = half + i * D;
+    double c[D], e[D];
+    for (int k = 0; k < D; ++k) { const double lo = h[k] * -1.0; c[k] = (lo + h[k]) * 0.5; e[k] = (h[k] - lo) * 0.5; }
+    for (int a = 0; a < D; ++a) {
+      double acc = 0.0;
+      for (int b = 0; b < D; ++b) acc = acc + R[a * D + b] * c[b];
+      const double rc = acc + t[a];
+      double w = 0.0;
+      for (int b = 0; b < D; ++b) w = w + std::fabs(R[a * D + b]) * e[b];
+      mn[a] = rc - w; mx[a] = rc + w;
+    }
+  }
+  // test_instance: inverse transform (rotate_transpose of o - t, and of d), the
+  // local box slab with its 1e-9 parallel threshold and t > 1e-6 entry rule,
+  // then the guard against the world box
+  bool test(uint64_t i, const Ray& r, double max_toi, double* t_out) const {
+    const double *R = rot + i * D * D, *t = trans + i * D, *h = half + i * D;
+    double v[D], o[D], d[D];
+    for (int j = 0; j < D; ++j) v[j] = r.o[j] - t[j];
+    for (int a = 0; a < D; ++a) {
+      double x = 0.0, y = 0.0;
+      for (int j = 0; j < D; ++j) { x = x + R[j * D + a] * v[j]; y = y + R[j * D + a] * r.d[j]; }
+      o[a] = x; d[a] = y;
+    }
+    double t_min = 0.0, t_max = max_toi;
+    int axis = -1;
+    for (int k = 0; k < D; ++k) {
+      if (std::fabs(d[k]) < 1.0e-9) {
+        if (o[k] < -h[k] || o[k] > h[k]) return false;
+      } else {
+        const double inv = 1.0 / d[k];
+        double t1 = (-h[k] - o[k]) * inv, t2 = (h[k] - o[k]) * inv;
+        if (t1 > t2) std::swap(t1, t2);
+        if (t1 > t_min) { t_min = t1; axis = k; }
+        if (t2 < t_max) t_max = t2;
+        if (t_min > t_max) return false;
+      }
+    }
+    if (!(axis >= 0 && t_min > 1.0e-6 && t_min <= max_toi)) return false;
+    double mn[D], mx[D], te, tx; int ax;
+    world_box(i, mn, mx);
+    if (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &ax)) return false;
+    if (!(te <= t_min && t_min <= tx)) return false;
+    *t_out = t_min;
+    return true;
+  }
+};
+
+template <int D>
+static void tlas_box_nearest(uint64_t n, const double* rot, const double* trans, const double* half, const double* rays,
+                             uint64_t nr, double max_toi, int threads, int64_t* leaf, double* t, double* world) {
+  Inst64<D> s{n, rot, trans, half};
+  if (world) for (uint64_t i = 0; i < n; ++i) s.world_box(i, world + i * 2 * D, world + i * 2 * D + D);
+  parallel_for(nr, threads, [&](uint64_t q) {
+    Ray r{rays + q * 2 * D, rays + q * 2 * D + D};
+    int64_t bl = -1; double bt = 0.0;
+    for (uint64_t i = 0; i < n; ++i) {   // two_level.rs:126-167
+      double tt;
+      if (!s.test(i, r, max_toi, &tt)) continue;
+      if (bl < 0 || hit_less(tt, (int64_t)i, bt, bl)) { bl = (int64_t)i; bt = tt; }
+    }
+    leaf[q] = bl;
+    t[q] = bl >= 0 ? bt : INFINITY;
+  });
+}
+
 }  // namespace orc64
 
 using namespace orc64;
@@ -568,6 +643,12 @@ uint64_t orc64_shapecast_all(int D, int kind, const double* prims, uint32_t n, c
 void orc64_bvh_raycast_nearest(int D, int kind, const double* prims, uint32_t n, const double* rays, uint64_t nr,
                                double max_toi, int threads, int64_t* leaf, double* t, uint64_t* counts) {
   ORC64_DISPATCH(D, bvh_nearest_batch<DD>(kind, prims, n, rays, nr, max_toi, threads, leaf, t, counts));
+}
+
+void orc64_tlas_box_nearest(int D, uint64_t n, const double* rot, const double* trans, const double* half,
+                            const double* rays, uint64_t nr, double max_toi, int threads, int64_t* leaf, double* t,
+                            double* world) {
+  ORC64_DISPATCH(D, tlas_box_nearest<DD>(n, rot, trans, half, rays, nr, max_toi, threads, leaf, t, world));
 }
 
 void orc64_tree_stats(int D, int kind, const double* prims, uint32_t n, uint64_t* out5, double* quality) {

@@ -30,6 +30,9 @@ namespace {
 constexpr int F64_MAX_STACK = 64;   // tree levels the per-lane stack holds (median split: about log2(n) + 1)
 constexpr int F64_BLOCK = 128;      // threads per block of the pooled kernels (two waves)
 constexpr uint32_t F64_RING = 8;    // stack entries per lane kept in LDS by the pooled kernels (RingStack)
+// Leaves of an f64 two-level tree: box instances (two_level.rs:70-113), one
+// record each of R[D][D] (row-major), t[D], half[D], world box min[D], max[D].
+constexpr int KIND_INST64 = 3;
 
 struct View64 {
   const double* nodes;      // [np][4D + 2]: boxes, then the refs
@@ -52,7 +55,9 @@ __device__ __forceinline__ bool hit_before64(double t, uint32_t leaf, double bt,
   return a < b || (a == b && leaf < bl);
 }
 
-template <int D, int KIND> constexpr int stride64() { return KIND == KIND_AABB ? 2 * D : D + 1; }
+template <int D, int KIND> constexpr int stride64() {
+  return KIND == KIND_AABB ? 2 * D : KIND == KIND_SPHERE ? D + 1 : D * D + 4 * D;
+}
 
 // Visits every leaf range whose boxes (and every ancestor's) pass node_pass,
 // calling leaf(first, count) for each: the walk of the variable-length
@@ -136,7 +141,21 @@ template <int D, int KIND> struct Pol64<D, KIND, false> {
   __device__ void pair(const double* b, double limit, bool& ha, bool& hb, double& ta, double& tb) const {
     slab_pair64<D>(r, b, limit, ha, hb, ta, tb);
   }
-  __device__ bool narrow(const double* p, double limit, double& t) const { return sqd::narrow<D, KIND>(r, p, limit, t); }
+  __device__ bool narrow(const double* p, double limit, double& t) const {
+    if constexpr (KIND == KIND_INST64) {
+      // test_instance (tlas.rs:459-481): the ray in the instance's frame against
+      // its local box, kept inside the slab interval of its world box
+      double ol[D], dl[D];
+      iso_inverse<D>(p, p + D * D, r, ol, dl);
+      int ax;
+      if (!local_box<D>(ol, dl, p + D * D + D, limit, t, ax)) return false;
+      double te, tx;
+      if (!slab_exact<D>(r, p + D * D + 2 * D, p + D * D + 3 * D, limit, te, tx, ax)) return false;
+      return te <= t && t <= tx;
+    } else {
+      return sqd::narrow<D, KIND>(r, p, limit, t);
+    }
+  }
 };
 template <int D, int KIND> struct Pol64<D, KIND, true> {
   static constexpr int QS = 4 * D;   // origin, direction, probe box min, max
@@ -237,14 +256,14 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
     out_leaf[ray] = hit ? (int32_t)bl : -1;
     out_toi[ray] = hit ? best : __builtin_huge_val();
     double nrm[D];
-    if constexpr (!CAST) {
+    if constexpr (!CAST && KIND != KIND_INST64) {
       if (hit && out_normal) hit_normal<D, KIND>(r, v.prims + (size_t)bk * PS, best, lim, nrm);
     }
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       // Ray::at (maths/ray.rs), ShapeCast::at for casts (shapecast.rs)
       if (out_point) out_point[(size_t)ray * D + k] = hit ? r.o[k] + r.d[k] * best : 0.0;
-      if constexpr (!CAST) {
+      if constexpr (!CAST && KIND != KIND_INST64) {
         if (out_normal) out_normal[(size_t)ray * D + k] = hit ? nrm[k] : 0.0;
       }
     }
@@ -405,24 +424,11 @@ static int pool_launch(sq_bvh64* b, const double* q, uint64_t n, double max_toi,
 
 extern "C" {
 
-int sq_bvh64_create(sq_ctx* c, int dim, int kind, const double* prims, uint64_t n, sq_bvh64** out) {
-  if (!c || !out || (n && !prims)) return fail(SQ_ERR_INVALID, "NULL argument");
-  *out = nullptr;
-  if (n >= 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "too many primitives (ids are u32)");
-  int rc = dispatch64(dim, kind, [&]<int D, int KIND>() -> int { return SQ_OK; });
-  if (rc) return rc;
-  const int ps = kind == KIND_AABB ? 2 * dim : dim + 1;
-  // QueryGeometry::world_aabb of the built-in kinds, in double
-  std::vector<double> boxes((size_t)n * 2 * dim);
-  for (uint64_t i = 0; i < n; ++i) {
-    const double* p = prims + i * ps;
-    double* bx = boxes.data() + i * 2 * dim;
-    for (int k = 0; k < dim; ++k) {
-      if (kind == KIND_AABB) { bx[k] = p[k]; bx[dim + k] = p[dim + k]; }
-      else { bx[k] = p[k] - p[dim]; bx[dim + k] = p[k] + p[dim]; }
-    }
-    if (kind == KIND_SPHERE && !(p[dim] >= 0.0)) return fail(SQ_ERR_INVALID, "sphere radius must be >= 0");
-  }
+// The f64 tree over n leaves with world boxes `boxes` [n][2D] and device
+// records `prims` [n][ps] (leaf-ordered on upload).
+static int create64(sq_ctx* c, int dim, int kind, const double* prims, int ps, const std::vector<double>& boxes, uint64_t n,
+                    sq_bvh64** out) {
+  int rc;
   std::unique_ptr<sq_bvh64> b(new sq_bvh64());
   b->ctx = c; b->dim = dim; b->kind = kind; b->n_prims = n;
   sqh::build_tree(dim, boxes.data(), (uint32_t)n, b->tree);
@@ -505,6 +511,72 @@ int sq_bvh64_create(sq_ctx* c, int dim, int kind, const double* prims, uint64_t 
   return SQ_OK;
 }
 
+int sq_bvh64_create(sq_ctx* c, int dim, int kind, const double* prims, uint64_t n, sq_bvh64** out) {
+  if (!c || !out || (n && !prims)) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (n >= 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "too many primitives (ids are u32)");
+  int rc = dispatch64(dim, kind, [&]<int D, int KIND>() -> int { return SQ_OK; });
+  if (rc) return rc;
+  const int ps = kind == KIND_AABB ? 2 * dim : dim + 1;
+  // QueryGeometry::world_aabb of the built-in kinds, in double
+  std::vector<double> boxes((size_t)n * 2 * dim);
+  for (uint64_t i = 0; i < n; ++i) {
+    const double* p = prims + i * ps;
+    double* bx = boxes.data() + i * 2 * dim;
+    for (int k = 0; k < dim; ++k) {
+      if (kind == KIND_AABB) { bx[k] = p[k]; bx[dim + k] = p[dim + k]; }
+      else { bx[k] = p[k] - p[dim]; bx[dim + k] = p[k] + p[dim]; }
+    }
+    if (kind == KIND_SPHERE && !(p[dim] >= 0.0)) return fail(SQ_ERR_INVALID, "sphere radius must be >= 0");
+  }
+  return create64(c, dim, kind, prims, ps, boxes, n, out);
+}
+
+// Tlas::build (tlas.rs:35-56) over box instances in double: each world box is
+// Isometry::transform_aabb of the local box [-half, half] (isometry.rs:84-97).
+int sq_tlas64_create(sq_ctx* c, int dim, uint64_t n, const double* rot, const double* trans, const double* halves,
+                     sq_tlas64** out) {
+  if (!c || !out || (n && (!rot || !trans || !halves))) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (dim < 2 || dim > 6) return fail(SQ_ERR_UNSUPPORTED, "dimension " + std::to_string(dim) + " not instantiated (2..6)");
+  if (n >= 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "too many instances (ids are u32)");
+  const int D = dim, ps = D * D + 4 * D;
+  std::vector<double> boxes((size_t)n * 2 * D), recs((size_t)n * ps);
+  for (uint64_t i = 0; i < n; ++i) {
+    const double* R = rot + i * D * D;
+    const double* t = trans + i * D;
+    const double* h = halves + i * D;
+    double* rec = recs.data() + i * ps;
+    for (int k = 0; k < D; ++k)
+      if (!(h[k] >= 0.0)) return fail(SQ_ERR_INVALID, "instance half extents must be >= 0");
+    std::memcpy(rec, R, D * D * sizeof(double));
+    std::memcpy(rec + D * D, t, D * sizeof(double));
+    std::memcpy(rec + D * D + D, h, D * sizeof(double));
+    // transform_aabb: centre R c + t (c = 0 for [-h, h]), half |R| h
+    double* w = boxes.data() + i * 2 * D;
+    for (int a = 0; a < D; ++a) {
+      double cc = 0.0;
+      for (int b = 0; b < D; ++b) cc = cc + R[a * D + b] * ((h[b] * -1.0 + h[b]) * 0.5);
+      const double rc = cc + t[a];
+      double acc = 0.0;
+      for (int b = 0; b < D; ++b) acc = acc + std::fabs(R[a * D + b]) * ((h[b] - h[b] * -1.0) * 0.5);
+      w[a] = rc - acc;
+      w[D + a] = rc + acc;
+    }
+    std::memcpy(rec + D * D + 2 * D, w, 2 * D * sizeof(double));
+  }
+  sq_bvh64* b = nullptr;
+  if (int rc = create64(c, dim, KIND_INST64, recs.data(), ps, boxes, n, &b)) return rc;
+  *out = reinterpret_cast<sq_tlas64*>(b);
+  return SQ_OK;
+}
+
+int sq_tlas64_destroy(sq_tlas64* t) { return sq_bvh64_destroy(reinterpret_cast<sq_bvh64*>(t)); }
+
+int sq_tlas64_stats(const sq_tlas64* t, sq_tree_stats* out) {
+  return sq_bvh64_stats(reinterpret_cast<const sq_bvh64*>(t), out);
+}
+
 int sq_bvh64_destroy(sq_bvh64* b) {
   if (!b) return SQ_OK;
   sq_ctx* c = b->ctx;
@@ -543,6 +615,24 @@ static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi
   if (!n) return SQ_OK;
   if (n >= (1ull << 31)) return fail(SQ_ERR_INVALID, "more than 2^31 queries in one call");
   const Filter fd = filter64(b, f);
+  if (b->kind == KIND_INST64) {
+    if (cast) return fail(SQ_ERR_UNSUPPORTED, "no shape casts on f64 instance trees");
+    auto go = [&]<int D>() -> int {
+      if (out_any)
+        return pool_launch<D, KIND_INST64, true, false, false>(b, rays, n, max_toi, tmax, fd, nullptr, nullptr, nullptr,
+                                                               nullptr, out_any, nullptr, nullptr);
+      return pool_launch<D, KIND_INST64, false, false, false>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi, out_point,
+                                                              nullptr, nullptr, nullptr, nullptr);
+    };
+    switch (b->dim) {
+      case 2: return go.template operator()<2>();
+      case 3: return go.template operator()<3>();
+      case 4: return go.template operator()<4>();
+      case 5: return go.template operator()<5>();
+      case 6: return go.template operator()<6>();
+      default: return fail(SQ_ERR_UNSUPPORTED, "dimension not instantiated (2..6)");
+    }
+  }
   return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
     if (cast)
       return pool_launch<D, KIND, false, true, false>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi, out_point,
@@ -571,6 +661,19 @@ int sq_bvh64_raycast_any(sq_bvh64* b, const double* rays, uint64_t n, double max
                          const sq_filter* f, uint8_t* out_hit) {
   if (n && !out_hit) return fail(SQ_ERR_INVALID, "NULL argument");
   return nearest64(b, rays, n, max_toi, max_toi_per_ray, f, nullptr, nullptr, nullptr, nullptr, out_hit, false);
+}
+
+int sq_tlas64_raycast_nearest(sq_tlas64* t, const double* rays, uint64_t n, double max_toi,
+                              const double* max_toi_per_ray, int32_t* out_inst, double* out_toi, double* out_point) {
+  return nearest64(reinterpret_cast<sq_bvh64*>(t), rays, n, max_toi, max_toi_per_ray, nullptr, out_inst, out_toi,
+                   out_point, nullptr, nullptr, false);
+}
+
+int sq_tlas64_raycast_any(sq_tlas64* t, const double* rays, uint64_t n, double max_toi, const double* max_toi_per_ray,
+                          uint8_t* out_hit) {
+  if (n && !out_hit) return fail(SQ_ERR_INVALID, "NULL argument");
+  return nearest64(reinterpret_cast<sq_bvh64*>(t), rays, n, max_toi, max_toi_per_ray, nullptr, nullptr, nullptr, nullptr,
+                   nullptr, out_hit, false);
 }
 
 // mode 0: rays, 1: overlap regions, 2: shape casts

@@ -891,13 +891,14 @@ enum { INST_BOX = 0, INST_MESH = 1, INST_BALL = 2 };
 
 // Isometry::inverse_transform_point / _vector (isometry.rs:70-78, 113-123):
 // rotate_transpose(o - t) and rotate_transpose(d), each accumulating from 0.
-template <int D> SQ_HD void iso_inverse(const float* R, const float* t, const RayD<D>& r, float* o_l, float* d_l) {
-  float v[D];
+template <int D, class S = float>
+SQ_HD void iso_inverse(const S* R, const S* t, const RayD<D, S>& r, S* o_l, S* d_l) {
+  S v[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) v[j] = r.o[j] - t[j];
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    float a = 0.0f, b = 0.0f;
+    S a = S(0), b = S(0);
 #pragma unroll
     for (int j = 0; j < D; ++j) { a = a + R[j * D + i] * v[j]; b = b + R[j * D + i] * r.d[j]; }
     o_l[i] = a; d_l[i] = b;
@@ -917,24 +918,28 @@ template <int D> SQ_HD void iso_rotate(const float* R, const float* n, float* ou
 
 // ray_local_box (two_level.rs:81-113): a local box [-half, half]; parallel
 // threshold 1e-9; a hit needs an entry axis, t_min > 1e-6 and t_min <= max_toi.
-template <int D> SQ_HD bool local_box(const float* o, const float* d, const float* half, float max_toi, float& t, int& axis_out) {
-  float t_min = 0.0f, t_max = max_toi;
+// (The literals are the Scalar's: 1e-9 and 1e-6 rounded to float, or double.)
+template <int D, class S = float>
+SQ_HD bool local_box(const S* o, const S* d, const S* half, S max_toi, S& t, int& axis_out) {
+  const S par = S(sizeof(S) == 4 ? (double)1.0e-9f : 1.0e-9);
+  const S tiny = S(sizeof(S) == 4 ? (double)1.0e-6f : 1.0e-6);
+  S t_min = S(0), t_max = max_toi;
   int axis = -1;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    if (fabsf(d[k]) < 1.0e-9f) {
+    if (sq_abs(d[k]) < par) {
       if (o[k] < -half[k] || o[k] > half[k]) return false;
     } else {
-      const float inv = SQ_DIV_RN(1.0f, d[k]);
-      float t1 = (-half[k] - o[k]) * inv;
-      float t2 = (half[k] - o[k]) * inv;
-      if (t1 > t2) { float tmp = t1; t1 = t2; t2 = tmp; }
+      const S inv = SQ_DIV_RN(S(1), d[k]);
+      S t1 = (-half[k] - o[k]) * inv;
+      S t2 = (half[k] - o[k]) * inv;
+      if (t1 > t2) { S tmp = t1; t1 = t2; t2 = tmp; }
       if (t1 > t_min) { t_min = t1; axis = k; }
       if (t2 < t_max) t_max = t2;
       if (t_min > t_max) return false;
     }
   }
-  if (!(axis >= 0 && t_min > 1.0e-6f && t_min <= max_toi)) return false;
+  if (!(axis >= 0 && t_min > tiny && t_min <= max_toi)) return false;
   t = t_min; axis_out = axis;
   return true;
 }

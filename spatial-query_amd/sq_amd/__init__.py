@@ -10,9 +10,9 @@ The package loads `lib/libsq_amd.so` (built by `make -C spatial-query_amd`) and
 raises when it is missing: there is no CPU fallback.
 """
 from .capi import (SQ_PRIM_AABB, SQ_PRIM_SPHERE, SQ_PRIM_TRIANGLE, SqError, lib, library_path)
-from .api import (BatchSet, ConsumerBvh, Context, DeviceBvh, DeviceBvh64, Narrow, Proposal, QueryFilter, Tlas, Tree, mfma_gemm, prim_stride)
+from .api import (BatchSet, ConsumerBvh, Context, DeviceBvh, DeviceBvh64, Narrow, Proposal, QueryFilter, Tlas, Tlas64, Tree, mfma_gemm, prim_stride)
 
 AABB, SPHERE, TRIANGLE = SQ_PRIM_AABB, SQ_PRIM_SPHERE, SQ_PRIM_TRIANGLE
 
-__all__ = ["BatchSet", "ConsumerBvh", "Context", "DeviceBvh", "DeviceBvh64", "Narrow", "Proposal", "mfma_gemm", "Tlas", "QueryFilter", "Tree", "SqError", "lib", "library_path", "prim_stride",
+__all__ = ["BatchSet", "ConsumerBvh", "Context", "DeviceBvh", "DeviceBvh64", "Narrow", "Proposal", "mfma_gemm", "Tlas", "Tlas64", "QueryFilter", "Tree", "SqError", "lib", "library_path", "prim_stride",
            "AABB", "SPHERE", "TRIANGLE"]

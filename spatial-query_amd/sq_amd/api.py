@@ -443,6 +443,54 @@ class DeviceBvh64:
             pass
 
 
+class Tlas64:
+    """Tlas over box instances under the `f64` feature: rotations [n][D][D],
+    translations [n][D], half extents [n][D] (host float64)."""
+
+    def __init__(self, ctx: Context, dim: int, rot, trans, halves):
+        r = _host(rot, np.float64).reshape(-1, dim, dim)
+        t = _host(trans, np.float64).reshape(-1, dim)
+        h = _host(halves, np.float64).reshape(-1, dim)
+        hd = C.c_void_p()
+        check(lib().sq_tlas64_create(ctx.handle, dim, r.shape[0], _hp(r), _hp(t), _hp(h), C.byref(hd)))
+        self._h, self.ctx, self.dim, self.n = hd, ctx, dim, r.shape[0]
+
+    def stats(self) -> dict:
+        s = TreeStats()
+        check(lib().sq_tlas64_stats(self._h, C.byref(s)))
+        return _stats(s)
+
+    def raycast_nearest(self, rays, max_toi, max_toi_per_ray=None, point=False):
+        import torch
+        n, dev = rays.shape[0], rays.device
+        leaf = torch.empty(n, dtype=torch.int32, device=dev)
+        toi = torch.empty(n, dtype=torch.float64, device=dev)
+        pt = torch.empty((n, self.dim), dtype=torch.float64, device=dev) if point else None
+        check(lib().sq_tlas64_raycast_nearest(self._h, _dp(rays), n, float(max_toi), _dp(max_toi_per_ray), _dp(leaf),
+                                              _dp(toi), _dp(pt)))
+        out = dict(leaf=leaf, toi=toi)
+        if point: out["point"] = pt
+        return out
+
+    def raycast_any(self, rays, max_toi, max_toi_per_ray=None):
+        import torch
+        out = torch.empty(rays.shape[0], dtype=torch.uint8, device=rays.device)
+        check(lib().sq_tlas64_raycast_any(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(max_toi_per_ray),
+                                          _dp(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_tlas64_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Tlas:
     """Tlas<D> over built-in instance kinds (accel/tlas.rs), resident on the GPU.
 

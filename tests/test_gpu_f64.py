@@ -232,3 +232,50 @@ def test_f64_ties_coincident_parallel_and_unbounded(ctx, kind):
     r_ov = ref.overlap(regions)
     assert_csr(b.overlap(dev(regions)), (r_ov[0], r_ov[1], None), "overlap, ties", has_t=False)
     b.close()
+
+
+def orthonormal(rng, n, dim):
+    q, r = np.linalg.qr(rng.normal(size=(n, dim, dim)))
+    return q * np.sign(np.diagonal(r, axis1=1, axis2=2))[:, None, :]
+
+
+@pytest.mark.parametrize("dim", [2, 3, 6])
+def test_f64_box_instances_match_oracle(ctx, dim):
+    """The f64 Tlas over box instances (two_level.rs:70-167 in double): nearest
+    instance, toi and point bit for bit against the oracle's instance scan,
+    any-hit, per-ray limits; the top tree is the f64 tree of the oracle's world
+    boxes (same stats, so the world boxes agree)."""
+    print(f"case: f64 box instances dim={dim}", flush=True)
+    rng = np.random.default_rng(0xB0C5 + dim)
+    n = 3000
+    span = {2: 200.0, 3: 100.0, 6: 12.0}[dim]
+    rot = orthonormal(rng, n, dim)
+    trans = rng.uniform(-span, span, (n, dim))
+    halves = rng.uniform(0.5, 4.0, (n, dim))
+    o = rng.uniform(-span, span, (5000, dim))
+    d = rng.normal(size=(5000, dim))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1)
+    max_toi = 4.0 * span
+    tl = sq_amd.Tlas64(ctx, dim, rot, trans, halves)
+    ref = oracle.BoxInstances64(dim, rot, trans, halves)
+    want = ref.raycast_nearest(rays, max_toi, world=True)
+    assert (want["leaf"] >= 0).sum() > 800
+    got = tl.raycast_nearest(dev(rays), max_toi, point=True)
+    assert np.array_equal(host(got["leaf"]).astype(np.int64), want["leaf"])
+    assert np.array_equal(host(got["toi"]).view(np.uint64), want["t"].view(np.uint64))
+    hit = want["leaf"] >= 0
+    pts = rays[:, :dim] + rays[:, dim:] * np.where(hit, want["t"], 0.0)[:, None]
+    assert np.array_equal(host(got["point"])[hit].view(np.uint64), pts[hit].view(np.uint64))
+    assert np.array_equal(host(tl.raycast_any(dev(rays), max_toi)).astype(bool), hit)
+    lim = rng.uniform(0.1, max_toi, len(rays))
+    got = tl.raycast_nearest(dev(rays), max_toi, max_toi_per_ray=dev(lim))
+    per = [ref.raycast_nearest(rays[i:i + 1], lim[i]) for i in range(0, len(rays), 25)]
+    idx = np.arange(0, len(rays), 25)
+    assert np.array_equal(host(got["leaf"]).astype(np.int64)[idx], np.array([p["leaf"][0] for p in per]))
+    assert np.array_equal(host(got["toi"])[idx].view(np.uint64), np.array([p["t"][0] for p in per]).view(np.uint64))
+    st, rs = tl.stats(), oracle.Scene64(dim, oracle.BOX, want["world"]).tree_stats()
+    for k in ("node_count", "leaf_count", "prim_count", "max_depth"):
+        assert st[k] == rs[k], k
+    assert st["quality"] == float(np.float32(rs["quality"]))
+    tl.close()

@@ -264,3 +264,31 @@ def test_bvh_walk_equals_naive(dim, kind):
     assert np.array_equal(par["leaf"], got["leaf"]) and np.array_equal(par["t"], got["t"])
     assert par["nodes_tested"] == got["nodes_tested"] and par["prims_tested"] == got["prims_tested"]
     assert got["nodes_tested"] < 3000 * s.tree_stats()["node_count"] // 4   # the tree prunes
+
+
+@pytest.mark.parametrize("dim", [2, 3, 5])
+def test_box_instances_dyadic_equal_the_f32_oracle(dim):
+    """Box instances (two_level.rs:70-167) in double: with signed-permutation
+    rotations and dyadic translations, half extents and rays every operation is
+    exact in both precisions, so the f64 scan equals the f32 oracle's guarded
+    box instances bit for bit, world boxes included."""
+    rng = np.random.default_rng(90 + dim)
+    n = 400
+    rot = np.zeros((n, dim, dim))
+    for i in range(n):
+        rot[i, np.arange(dim), rng.permutation(dim)] = rng.choice([-1.0, 1.0], dim)
+    span = {2: 160, 3: 160, 5: 24}[dim]   # denser in 5-D, so rays still hit
+    trans = rng.integers(-span, span, (n, dim)) / 8.0
+    halves = rng.integers(1, 24, (n, dim)) / 8.0
+    rays = dyadic_rays(3000, dim, 95 + dim, span, 16, zero_dirs=False)
+    s64 = oracle.BoxInstances64(dim, rot, trans, halves)
+    w = s64.raycast_nearest(rays, 80.0, world=True)
+    s32 = oracle.InstanceScene(dim, oracle.INST_BOX_GUARDED, rot.astype(np.float32), trans.astype(np.float32),
+                               halves=halves.astype(np.float32))
+    w32 = s32.raycast_nearest(rays.astype(np.float32), 80.0, naive=True)
+    assert (w["leaf"] >= 0).sum() > 300
+    assert np.array_equal(w["leaf"], w32["leaf"]) and np.array_equal(w["t"], w32["t"].astype(np.float64))
+    for i in range(0, n, 37):
+        box = np.concatenate([-halves[i], halves[i]]).astype(np.float32)
+        wb = oracle.iso_transform_aabb(rot[i].astype(np.float32), trans[i].astype(np.float32), box)
+        assert np.array_equal(w["world"][i], np.asarray(wb, np.float64).reshape(-1))
