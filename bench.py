@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--proposal-epochs", type=int, default=4)
     ap.add_argument("--proposal-train-rays", type=int, default=1 << 24)
     ap.add_argument("--f64", action="store_true", help="measure the f64 feature (1M balls, 4M rays) instead")
+    ap.add_argument("--hit-mode", choices=["reference", "guarded"], default="reference",
+                    help="hit mode of the timed walk (reference: the reference's hits, its walk order; guarded: "
+                         "box-guarded hits, near-first)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not produced: rays are "
@@ -90,6 +93,7 @@ def main():
     if args.wide >= 0:
         tun["wide"] = args.wide
     ctx.set_tuning(**tun)
+    ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED if args.hit_mode == "guarded" else sq_amd.SQ_HITS_REFERENCE)
     t0 = time.time()
     tree = sq_amd.Tree.build(3, sq_amd.TRIANGLE, sc.prims)
     t_build = time.time() - t0

@@ -94,6 +94,30 @@ int sq_ctx_reset_stream(sq_ctx* ctx);
 int sq_ctx_synchronize(sq_ctx* ctx);
 /* The hipStream_t all work of this context is enqueued on. */
 void* sq_ctx_stream(sq_ctx* ctx);
+/* Which hits the context's queries return (the reference has one behaviour,
+ * SQ_HITS_REFERENCE; the other is this engine's opt-in).
+ *   SQ_HITS_REFERENCE (default)  every query returns what the reference's CPU
+ *       path returns for the built-in provider: the narrow phases exactly as
+ *       the reference's providers (no extra test), and nearest-hit rays and
+ *       shape casts over leaves whose hits may fall outside their own AABB
+ *       (SPHERE, TRIANGLE, consumer narrow phases, every instance kind) walk
+ *       the reference's order (bvh.rs:73-124: push child a, then b, pop LIFO,
+ *       skip what lies beyond the current best).  AABB leaves walk near-first
+ *       in both modes, which gives the same hits.
+ *   SQ_HITS_GUARDED  SPHERE and TRIANGLE hits (and instance hits, against the
+ *       instance's world box) count only when their toi lies inside the slab
+ *       interval of their own AABB; then any walk order gives one answer, and
+ *       nearest queries walk near-first (fewer nodes).  It differs from the
+ *       reference only on rays whose reference answer lies outside its box
+ *       (40 of the 8,388,608 rays of the 2M-triangle heightfield workload).
+ *       The learned proposal's seeded walk (sq_raycast_nearest_proposed)
+ *       always answers in this mode.
+ * All-hits, crossings, overlap and cast-all lists hold the reference's hits in
+ * both modes (their walks never shrink a limit, so order does not matter). */
+#define SQ_HITS_REFERENCE 0
+#define SQ_HITS_GUARDED 1
+int sq_ctx_set_hit_mode(sq_ctx* ctx, int mode);
+int sq_ctx_get_hit_mode(const sq_ctx* ctx);   /* -1 for a NULL ctx */
 
 /* ---- host tree: Bvh (accel/bvh.rs), canonical, no device ------------------ */
 /* Bvh::build(&G) (bvh.rs:31-37, tree.rs:39-163) for a built-in provider: the
@@ -415,7 +439,8 @@ int sq_batch_destroy(sq_batch* batch);
  * fixed depth that most likely hold its nearest hit; the proposed subtrees are
  * walked first to tighten t_max and the full exact walk then confirms, so
  * sq_raycast_nearest_proposed returns exactly what sq_raycast_nearest returns
- * (bvh.rs:51-124 semantics) whatever the network predicts.
+ * under SQ_HITS_GUARDED (whose answer does not depend on the walk order)
+ * whatever the network predicts.
  *
  * Network: x = (origin mapped from the scene box to [-1,1]^3, direction);
  * 64 random Fourier features sin/cos(2 pi x.B), B ~ N(0, rff_scale^2) [6][64];
@@ -448,7 +473,8 @@ int sq_proposal_features_spec(const sq_proposal* p, float* rff, float* box);
 int sq_proposal_infer(sq_proposal* p, const float* rays, uint64_t n, uint16_t* top);
 /* Exact nearest hit, walking the proposed subtrees (top from
  * sq_proposal_infer; NULL = none, the plain walk) first.  Same results as
- * sq_raycast_nearest without per-ray limits or filter.  stats (device
+ * sq_raycast_nearest under SQ_HITS_GUARDED without per-ray limits or filter
+ * (whatever the context's hit mode).  stats (device
  * uint64[3], may be NULL; NULL runs the uncounted kernel) accumulates: rays
  * whose proposed subtrees held a hit, pair nodes fetched, primitives tested. */
 int sq_raycast_nearest_proposed(sq_proposal* p, const float* rays, uint64_t n, float max_toi, const uint16_t* top,

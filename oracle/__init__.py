@@ -56,6 +56,8 @@ def _declare(L):
         "orc_rng_u64": (None, [U64, U64, P]),
         "orc_normalize": (None, [I32, P, U64]),
         "orc_geom_new": (P, [I32, I32, P, U32]),
+        "orc_geom_set_guarded": (None, [P, I32]),
+        "orc_inst_set_guarded": (None, [P, I32]),
         "orc_geom_new_consumer": (P, [I32, P, U32, I32, U64]),
         "orc_geom_set_consumer_crossings": (None, [P, U64]),
         "orc_geom_set_consumer_cast_overlap": (None, [P, U64, U64]),
@@ -80,7 +82,7 @@ def _declare(L):
         "orc64_overlap": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, I32, P, P, U64]),
         "orc64_tree_stats": (None, [I32, I32, P, U32, P, P]),
         "orc64_bvh_raycast_nearest": (None, [I32, I32, P, U32, P, U64, C.c_double, I32, P, P, P]),
-        "orc64_tlas_box_nearest": (None, [I32, U64, P, P, P, P, U64, C.c_double, I32, P, P, P]),
+        "orc64_tlas_box_nearest": (None, [I32, U64, P, P, P, P, U64, C.c_double, I32, I32, P, P, P]),
         "orc64_shapecast_nearest": (None, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, P, I32, P, P, P]),
         "orc64_shapecast_all": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, P, I32, P, P, P, U64]),
         "orc_inst_new": (P, [I32, I32, U64, P, P, P, U32, P, P, P, P]),
@@ -144,11 +146,17 @@ def normalize(dirs: np.ndarray) -> np.ndarray:
 class Scene:
     """A built-in provider plus its reference-built Bvh (bvh.rs:31-37)."""
 
-    def __init__(self, dim: int, kind: int, prims: np.ndarray, layers=None, build_tree: bool = True):
-        self.dim, self.kind = dim, kind
+    def __init__(self, dim: int, kind: int, prims: np.ndarray, layers=None, build_tree: bool = True,
+                 guarded: bool = False):
+        """guarded: the engine's SQ_HITS_GUARDED restated (SPHERE and TRI hits
+        kept only inside their own box's slab interval); default the plain
+        providers, the reference's results."""
+        self.dim, self.kind, self.guarded = dim, kind, guarded
         self.prims = np.ascontiguousarray(prims, dtype=np.float32)
         self.n = self.prims.shape[0]
         self._g = lib().orc_geom_new(dim, kind, _p(self.prims), self.n)
+        if guarded:
+            lib().orc_geom_set_guarded(self._g, 1)
         self.layers = None
         if layers is not None:
             self.set_layers(layers)
@@ -156,26 +164,28 @@ class Scene:
 
     @classmethod
     def consumer(cls, dim: int, boxes: np.ndarray, prims: np.ndarray, narrow_addr: int, crossings_addr: int = 0,
-                 cast_addr: int = 0, overlap_addr: int = 0) -> "Scene":
+                 cast_addr: int = 0, overlap_addr: int = 0, guarded: bool = False) -> "Scene":
         """A consumer provider: world boxes [n][2*dim] and consumer floats
         [n][pf], tested by the C function at narrow_addr (signature
         int(const float* prim, const float* origin, const float* dir, float
-        max_toi, float* toi)), box-guarded like the built-in SPHERE and TRI;
-        crossings through the C function at crossings_addr (signature
-        void(prim, origin, dir, max_toi, void (*emit)(void*, float), void*)),
-        each crossing box-guarded the same way, or the nearest hit when 0; shape
+        max_toi, float* toi)); crossings through the C function at
+        crossings_addr (signature void(prim, origin, dir, max_toi,
+        void (*emit)(void*, float), void*)), or the nearest hit when 0; shape
         casts through the C function at cast_addr (int(prim, origin, dir,
-        probe_min, probe_max, max_toi, float* toi), guarded by the probe-grown
-        box; none when 0) and overlap through the one at overlap_addr
-        (int(prim, region_min, region_max) after the box test; the box test
-        alone when 0)."""
+        probe_min, probe_max, max_toi, float* toi); none when 0) and overlap
+        through the one at overlap_addr (int(prim, region_min, region_max)
+        after the box test; the box test alone when 0).  guarded: each hit,
+        crossing and cast contact kept only inside its primitive's box slab
+        interval (probe-grown for casts), the engine's SQ_HITS_GUARDED."""
         boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 2 * dim)
         prims = np.ascontiguousarray(prims, dtype=np.float32).reshape(boxes.shape[0], -1)
         obj = cls.__new__(cls)
-        obj.dim, obj.kind = dim, KIND_CONSUMER
+        obj.dim, obj.kind, obj.guarded = dim, KIND_CONSUMER, guarded
         obj.prims = np.ascontiguousarray(np.concatenate([boxes, prims], 1), dtype=np.float32)
         obj.n = obj.prims.shape[0]
         obj._g = lib().orc_geom_new_consumer(dim, _p(obj.prims), obj.n, prims.shape[1], int(narrow_addr))
+        if guarded:
+            lib().orc_geom_set_guarded(obj._g, 1)
         if crossings_addr:
             lib().orc_geom_set_consumer_crossings(obj._g, int(crossings_addr))
         if cast_addr or overlap_addr:
@@ -301,13 +311,20 @@ class Scene:
 # Two-level scene (tlas.rs)
 # --------------------------------------------------------------------------
 
+F64_GUARDED, F64_BVH = 0x100, 0x200   # flags of the f64 oracle's `kind` argument (sq_oracle64.cpp)
+
+
 class Scene64:
     """The reference's `f64` feature (sq/src/maths/point.rs:14-18) over BOX or
-    SPHERE primitives in double: the naive scans of sq/src/query/naive.rs, which
-    the BVH answers exactly (sq/src/lib.rs:15-17), and the f64 tree's stats."""
+    SPHERE primitives in double: the reference's Bvh walks over its f64 tree
+    (bvh.rs:51-456), or with naive=True the naive scans of
+    sq/src/query/naive.rs; guarded=True restates the engine's SQ_HITS_GUARDED
+    leaf tests (a sphere hit kept inside its own box), under which the two
+    agree exactly (sq/src/lib.rs:15-17).  Also the f64 tree's stats."""
 
-    def __init__(self, dim: int, kind: int, prims, layers=None):
+    def __init__(self, dim: int, kind: int, prims, layers=None, guarded: bool = False, naive: bool = False):
         self.dim, self.kind = dim, kind
+        self._kf = kind | (F64_GUARDED if guarded else 0) | (0 if naive else F64_BVH)
         self.prims = np.ascontiguousarray(prims, np.float64).reshape(-1, 2 * dim if kind == BOX else dim + 1)
         self.n = self.prims.shape[0]
         self.layers = None if layers is None else np.ascontiguousarray(layers, np.uint64)
@@ -328,7 +345,7 @@ class Scene64:
         n = r.shape[0]
         m, per = self._limits(max_toi, n)
         leaf = np.empty(n, np.int64); t = np.empty(n); pt = np.empty((n, self.dim)); nm = np.empty((n, self.dim))
-        lib().orc64_raycast_nearest(self.dim, self.kind, _p(self.prims), self.n, _p(self.layers), mask, exclude, _p(r), n,
+        lib().orc64_raycast_nearest(self.dim, self._kf, _p(self.prims), self.n, _p(self.layers), mask, exclude, _p(r), n,
                                     m, _p(per), threads, _p(leaf), _p(t), _p(pt), _p(nm))
         return dict(leaf=leaf, t=t, point=pt, normal=nm)
 
@@ -342,14 +359,14 @@ class Scene64:
 
     def raycast_all(self, rays, max_toi, mask=ALL_LAYERS, exclude=-1, threads=8):
         r = self._q(rays)
-        fn = lambda off, leaf, t, cap: lib().orc64_raycast_all(self.dim, self.kind, _p(self.prims), self.n, _p(self.layers),
+        fn = lambda off, leaf, t, cap: lib().orc64_raycast_all(self.dim, self._kf, _p(self.prims), self.n, _p(self.layers),
                                                               mask, exclude, _p(r), r.shape[0], float(max_toi), threads,
                                                               _p(off), _p(leaf), _p(t), cap)
         return self._collect(fn, r, True)
 
     def overlap(self, regions, mask=ALL_LAYERS, exclude=-1, threads=8):
         r = self._q(regions)
-        fn = lambda off, leaf, t, cap: lib().orc64_overlap(self.dim, self.kind, _p(self.prims), self.n, _p(self.layers),
+        fn = lambda off, leaf, t, cap: lib().orc64_overlap(self.dim, self._kf, _p(self.prims), self.n, _p(self.layers),
                                                           mask, exclude, _p(r), r.shape[0], threads, _p(off), _p(leaf), cap)
         off, leaf, _ = self._collect(fn, r, False)
         return off, leaf
@@ -359,14 +376,14 @@ class Scene64:
         n = c.shape[0]
         m, per = self._limits(max_toi, n)
         leaf = np.empty(n, np.int64); t = np.empty(n); pt = np.empty((n, self.dim))
-        lib().orc64_shapecast_nearest(self.dim, self.kind, _p(self.prims), self.n, _p(self.layers), mask, exclude, _p(c), n,
+        lib().orc64_shapecast_nearest(self.dim, self._kf, _p(self.prims), self.n, _p(self.layers), mask, exclude, _p(c), n,
                                       m, _p(per), threads, _p(leaf), _p(t), _p(pt))
         return dict(leaf=leaf, t=t, point=pt)
 
     def shapecast_all(self, casts, max_toi, mask=ALL_LAYERS, exclude=-1, threads=8):
         c = np.ascontiguousarray(casts, np.float64).reshape(-1, 4 * self.dim)
         m, per = self._limits(max_toi, c.shape[0])
-        fn = lambda off, leaf, t, cap: lib().orc64_shapecast_all(self.dim, self.kind, _p(self.prims), self.n,
+        fn = lambda off, leaf, t, cap: lib().orc64_shapecast_all(self.dim, self._kf, _p(self.prims), self.n,
                                                                 _p(self.layers), mask, exclude, _p(c), c.shape[0],
                                                                 m, _p(per), threads, _p(off), _p(leaf), _p(t), cap)
         return self._collect(fn, c, True)
@@ -378,7 +395,7 @@ class Scene64:
         r = self._q(rays)
         n = r.shape[0]
         leaf = np.empty(n, np.int64); t = np.empty(n); counts = np.zeros(2, np.uint64)
-        lib().orc64_bvh_raycast_nearest(self.dim, self.kind, _p(self.prims), self.n, _p(r), n, float(max_toi), threads,
+        lib().orc64_bvh_raycast_nearest(self.dim, self._kf, _p(self.prims), self.n, _p(r), n, float(max_toi), threads,
                                         _p(leaf), _p(t), _p(counts))
         return dict(leaf=leaf, t=t, nodes_tested=int(counts[0]), prims_tested=int(counts[1]))
 
@@ -391,10 +408,14 @@ class Scene64:
 
 class BoxInstances64:
     """Box instances under the `f64` feature (two_level.rs:70-167 in double):
-    the naive instance scan with the world-box guard, and the world boxes."""
+    the Tlas walk of tlas.rs:104-154 in the reference's order over the
+    median-split tree of the world boxes (naive=True: the naive instance scan
+    of two_level.rs:126-167), with guarded=True the engine's world-box guard;
+    and the world boxes."""
 
-    def __init__(self, dim, rot, trans, halves):
+    def __init__(self, dim, rot, trans, halves, guarded: bool = False, naive: bool = False):
         self.dim = dim
+        self._flags = (F64_GUARDED if guarded else 0) | (0 if naive else F64_BVH)
         self.rot = np.ascontiguousarray(rot, np.float64).reshape(-1, dim, dim)
         self.trans = np.ascontiguousarray(trans, np.float64).reshape(-1, dim)
         self.halves = np.ascontiguousarray(halves, np.float64).reshape(-1, dim)
@@ -406,14 +427,17 @@ class BoxInstances64:
         leaf = np.empty(nr, np.int64); t = np.empty(nr)
         w = np.empty((self.n, 2 * self.dim)) if world else None
         lib().orc64_tlas_box_nearest(self.dim, self.n, _p(self.rot), _p(self.trans), _p(self.halves), _p(r), nr,
-                                     float(max_toi), threads, _p(leaf), _p(t), _p(w))
+                                     float(max_toi), threads, self._flags, _p(leaf), _p(t), _p(w))
         out = dict(leaf=leaf, t=t)
         if world: out["world"] = w
         return out
 
 
 class InstanceScene:
-    def __init__(self, dim, kind, rot, trans, halves=None, mesh_tris=None, mesh_of=None, scale=None):
+    def __init__(self, dim, kind, rot, trans, halves=None, mesh_tris=None, mesh_of=None, scale=None, guarded=False):
+        """guarded: the engine's SQ_HITS_GUARDED restated (instance hits kept
+        inside their world box's slab interval, mesh triangles guarded);
+        default plain, the reference's Tlas results."""
         self.dim, self.kind = dim, kind
         self.rot = np.ascontiguousarray(rot, dtype=np.float32).reshape(-1, dim, dim)
         self.trans = np.ascontiguousarray(trans, dtype=np.float32).reshape(-1, dim)
@@ -435,6 +459,9 @@ class InstanceScene:
             nm = 0
         self._h = lib().orc_inst_new(dim, kind, self.n, _p(self.rot), _p(self.trans), _p(self.halves), nm,
                                      _p(self.mesh_off), _p(self.tris), _p(self.mesh_of), _p(self.scale))
+        self.guarded = guarded or kind == INST_BOX_GUARDED
+        if self.guarded:
+            lib().orc_inst_set_guarded(self._h, 1)
 
     def set_transforms(self, rot, trans, halves=None):
         self.rot = np.ascontiguousarray(rot, dtype=np.float32).reshape(-1, self.dim, self.dim)

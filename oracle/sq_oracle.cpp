@@ -21,12 +21,14 @@
 //   SPHERE (N2) sq/tests/differential.rs:61-82 (exit root when inside)
 //   TRI    (N3) Moeller-Trumbore crates/viewport-lib-query/src/lib.rs:279-306,
 //               plus the `t > max_toi` reject of sq/src/query/geometry.rs:37-39
-// The engine's SPHERE and TRI kinds are *box-guarded*: a narrow hit is kept only
-// when its toi lies inside the slab interval of the primitive's own AABB.  That
-// makes every hit lie inside every ancestor box, so the nearest result no longer
-// depends on traversal order (the BVH == naive contract of sq/src/lib.rs:15-17
-// then holds exactly).  The unguarded variants are kept for restating the
-// reference's own tests, whose providers are plain.
+// By default every provider is plain, as the reference's: the results are the
+// reference's.  A geometry or instance set marked `guarded` restates the
+// engine's SQ_HITS_GUARDED mode instead: SPHERE, TRI and consumer hits (and
+// instance hits, against the instance's world box) are kept only when their
+// toi lies inside the slab interval of their own AABB, which makes every hit
+// lie inside every ancestor box, so the nearest result no longer depends on
+// traversal order (BVH == naive then holds exactly).  The *_PLAIN kinds are
+// the plain providers under names of their own.
 //
 // Parity pins for each part live in tests/test_oracle.py (closed forms from the
 // reference tests, BVH == naive, refit == rebuild, parallel == sequential,
@@ -255,13 +257,13 @@ template <int D> struct ShapeCast {  // shapecast.rs:26-73
 
 enum Kind {
   KIND_BOX = 0,              // N1
-  KIND_SPHERE = 1,           // N2, box-guarded
-  KIND_TRI = 2,              // N3, box-guarded (D = 3)
+  KIND_SPHERE = 1,           // N2 (box-guarded when the geometry is `guarded`)
+  KIND_TRI = 2,              // N3, D = 3 (box-guarded when the geometry is `guarded`)
   KIND_SPHERE_PLAIN = 10,    // differential.rs:61-82 exactly (exit root when inside)
   KIND_SPHERE_PLAIN_NOEXIT = 11,  // conformance.rs:277-291 / lib.rs doc example
   KIND_TRI_PLAIN = 12,       // viewport-lib-query MT + max_toi reject, unguarded
   KIND_SHELL = 13,           // crossings.rs:37-111: both sphere roots are crossings
-  KIND_CONSUMER = 20,        // a consumer's test_ray (geometry.rs:33-43) through a C function, box-guarded
+  KIND_CONSUMER = 20,        // a consumer's test_ray (geometry.rs:33-43) through a C function (box-guarded when `guarded`)
 };
 
 // A consumer narrow phase: true with *toi on a hit of the primitive's floats.
@@ -286,6 +288,7 @@ template <int D> struct Geometry {
   ConsumerCast consumer_cast = nullptr;     // its shape-cast contact (none without)
   ConsumerOverlap consumer_ov = nullptr;    // its exact overlap test (the box test alone without)
   int consumer_pf = 0;
+  bool guarded = false;                     // SPHERE / TRI / CONSUMER: the engine's SQ_HITS_GUARDED restated
   int stride() const {
     if (kind == KIND_CONSUMER) return 2 * D + consumer_pf;
     return kind == KIND_BOX ? 2 * D : (kind == KIND_TRI || kind == KIND_TRI_PLAIN) ? 9 : D + 1;
@@ -322,8 +325,10 @@ template <int D> struct Geometry {
     return (lay & f.layer_mask) != 0;
   }
 
-  // The guard: toi must lie in the slab interval of the leaf's own AABB.
+  // The guard (only for a `guarded` geometry): toi must lie in the slab
+  // interval of the leaf's own AABB.
   bool guard(size_t leaf, const Ray<D>& ray, float max_toi, float t) const {
+    if (!guarded) return true;
     float te, tx;
     if (!world_aabb(leaf).ray_intersect(ray, max_toi, &te, &tx)) return false;
     return te <= t && t <= tx;
@@ -388,7 +393,7 @@ template <int D> struct Geometry {
       Point<D> c; for (int i = 0; i < D; ++i) c.v[i] = q[i];
       float t;
       if (!ball(ray, c, q[D], max_toi, kind != KIND_SPHERE_PLAIN_NOEXIT, &t)) return false;
-      if (kind == KIND_SPHERE && !guard(leaf, ray, max_toi, t)) return false;
+      if (kind == KIND_SPHERE && !guard(leaf, ray, max_toi, t)) return false;   // (a no-op unless guarded)
       out->toi = t;
       out->normal = normalize_or_zero(psub(ray.at(t), c));
       out->sub = (int64_t)leaf;
@@ -486,15 +491,15 @@ template <int D> struct Geometry {
   // shape_overlap.rs:104-118).  Triangles keep the default None.
   bool test_shape_cast(size_t leaf, const ShapeCast<D>& cast, float max_toi, LeafHit<D>* out) const {
     if (kind == KIND_CONSUMER) {
-      // the contact under the test_shape_cast contract and the box guard: inside
-      // the probe-grown slab interval of the primitive's world box
+      // the contact under the test_shape_cast contract (and, guarded, inside
+      // the probe-grown slab interval of the primitive's world box)
       if (!consumer_cast) return false;
       float t;
       if (!consumer_cast(p(leaf) + 2 * D, cast.origin.v, cast.dir.v, cast.aabb.min.v, cast.aabb.max.v, max_toi, &t)) return false;
       if (!(t >= 0.0f && t <= max_toi)) return false;
       float te, tx;
-      if (!cast.broadphase(world_aabb(leaf), max_toi, &te, &tx)) return false;
-      if (!(te <= t && t <= tx)) return false;
+      if (guarded && !cast.broadphase(world_aabb(leaf), max_toi, &te, &tx)) return false;
+      if (guarded && !(te <= t && t <= tx)) return false;
       out->toi = t; out->normal = Point<D>::zero(); out->sub = -1;
       return true;
     }
@@ -511,7 +516,7 @@ template <int D> struct Geometry {
       Ray<D> r; r.origin = cast.origin; r.dir = cast.dir;
       float t;
       if (!ball(r, c, sum, max_toi, true, &t)) return false;
-      if (kind == KIND_SPHERE) {
+      if (kind == KIND_SPHERE && guarded) {
         float te, tx;
         if (!cast.broadphase(world_aabb(leaf), max_toi, &te, &tx)) return false;
         if (!(te <= t && t <= tx)) return false;
@@ -912,9 +917,9 @@ static void naive_overlap(const Geometry<D>& g, const Aabb<D>& region, const Fil
 
 enum InstKind {
   INST_BOX = 0,          // local box [-half, half], two_level.rs:70-113 (ray_local_box)
-  INST_MESH = 1,         // uniformly scaled triangle mesh with its own BVH (D = 3), guarded
-  INST_BOX_GUARDED = 2,  // INST_BOX plus the instance guard (the engine's box kind)
-  INST_BALL = 3,         // ball of radius r at the local origin (shape_overlap.rs:288-345 BallInstances), guarded
+  INST_MESH = 1,         // uniformly scaled triangle mesh with its own BVH (D = 3)
+  INST_BOX_GUARDED = 2,  // INST_BOX in a `guarded` set (orc_inst_new's shorthand)
+  INST_BALL = 3,         // ball of radius r at the local origin (shape_overlap.rs:288-345 BallInstances)
 };
 
 template <int D> struct Instances {
@@ -927,6 +932,9 @@ template <int D> struct Instances {
   std::vector<std::unique_ptr<Geometry<D>>> meshes;
   std::vector<std::unique_ptr<Bvh<D>>> mesh_bvh;
   std::vector<Aabb<D>> mesh_box;       // AABB of each mesh's vertices
+  // SQ_HITS_GUARDED restated: every instance hit is kept only inside its world
+  // box's slab interval, and a mesh's triangles are the guarded kind
+  bool guarded = false;
 
   size_t instance_count() const { return xf.size(); }
   int64_t id(size_t i) const { return (int64_t)i; }
@@ -947,7 +955,8 @@ template <int D> struct Instances {
 
   // two_level.rs:81-113 for boxes.  Meshes: the local ray is moved into the
   // unscaled mesh frame by o/s, d/s (t is unchanged) and answered by the
-  // mesh's own BVH with the box-guarded triangle kind.
+  // mesh's own Bvh::raycast_nearest (bvh.rs:51-124) over the triangle kind
+  // (guarded in a `guarded` set).
   bool test_ray_local(size_t i, const Ray<D>& ray, float max_toi, LeafHit<D>* out) const {
     if (is_box()) {
       const Point<D>& half = halves[i];
@@ -1054,14 +1063,15 @@ template <int D> struct Tlas {
     return baseline > 0.0f && tree_interior_surface_sum(nodes) > factor * baseline;
   }
 
-  // tlas.rs:459-481.  Mesh instances add the box guard against the instance's
-  // world AABB, which keeps the two-level result traversal-order independent.
+  // tlas.rs:459-481.  A `guarded` set adds the box guard against the
+  // instance's world AABB, which makes the two-level result traversal-order
+  // independent (the engine's SQ_HITS_GUARDED).
   bool test_instance(const Instances<D>& s, size_t i, const Ray<D>& ray, float max_toi, Hit<D>* out) const {
     const Isometry<D>& iso = s.xf[i];
     Ray<D> local; local.origin = iso.inverse_transform_point(ray.origin); local.dir = iso.inverse_transform_vector(ray.dir);
     LeafHit<D> lh;
     if (!s.test_ray_local(i, local, max_toi, &lh)) return false;
-    if (s.kind != INST_BOX) {
+    if (s.guarded) {
       float te, tx;
       if (!world[i].ray_intersect(ray, max_toi, &te, &tx)) return false;
       if (!(te <= lh.toi && lh.toi <= tx)) return false;
@@ -1096,15 +1106,15 @@ template <int D> struct Tlas {
     return have;
   }
 
-  // shapecast_instance (tlas.rs:437-452) with the engine's guard: the contact
-  // must lie in the swept probe's slab interval against the instance's world
+  // shapecast_instance (tlas.rs:437-452); a `guarded` set keeps the contact
+  // only inside the swept probe's slab interval against the instance's world
   // AABB (broadphase), which makes the result order independent.
   bool shapecast_instance(const Instances<D>& s, size_t i, const ShapeCast<D>& cast, float max_toi, Hit<D>* out) const {
     LeafHit<D> lh;
     if (!s.test_shape_cast(i, cast, max_toi, &lh)) return false;
     float te, tx;
-    if (!cast.broadphase(world[i], max_toi, &te, &tx)) return false;
-    if (!(te <= lh.toi && lh.toi <= tx)) return false;
+    if (s.guarded && !cast.broadphase(world[i], max_toi, &te, &tx)) return false;
+    if (s.guarded && !(te <= lh.toi && lh.toi <= tx)) return false;
     out->id = s.id(i); out->leaf = (int64_t)i; out->toi = lh.toi; out->point = cast.at(lh.toi);
     out->normal = lh.normal; out->sub = lh.sub;
     return true;
@@ -1300,6 +1310,11 @@ void orc_normalize(int D, float* v, uint64_t n) {
 }
 
 // --- geometry -------------------------------------------------------------
+// Marks a geometry `guarded` (the engine's SQ_HITS_GUARDED) or plain (default).
+void orc_geom_set_guarded(void* h, int guarded) {
+  OGeom* g = (OGeom*)h;
+  ORC_DISPATCH(g->D, { ((Geometry<DD>*)g->p)->guarded = guarded != 0; });
+}
 void* orc_geom_new(int D, int kind, const float* prims, uint32_t n) {
   OGeom* g = new OGeom{D, nullptr};
   ORC_DISPATCH(D, {
@@ -1560,7 +1575,8 @@ void* orc_inst_new(int D, int kind, uint64_t n, const float* rot, const float* t
   OInst* o = new OInst{D, nullptr, nullptr};
   ORC_DISPATCH(D, {
     Instances<DD>* s = new Instances<DD>();
-    s->kind = kind;
+    s->kind = kind == INST_BOX_GUARDED ? INST_BOX : kind;
+    s->guarded = kind == INST_BOX_GUARDED;
     s->xf.resize(n);
     for (uint64_t i = 0; i < n; ++i) {
       for (int a = 0; a < DD; ++a) for (int c = 0; c < DD; ++c) s->xf[i].rot[a][c] = rot[i * DD * DD + a * DD + c];
@@ -1588,6 +1604,16 @@ void* orc_inst_new(int D, int kind, uint64_t n, const float* rot, const float* t
     Tlas<DD>* t = new Tlas<DD>(); t->build(*s); o->tlas = t;
   });
   return o;
+}
+// Marks an instance set `guarded` (the engine's SQ_HITS_GUARDED: instance hits
+// kept inside their world boxes, mesh triangles guarded) or plain (default).
+void orc_inst_set_guarded(void* h, int guarded) {
+  OInst* o = (OInst*)h;
+  ORC_DISPATCH(o->D, {
+    Instances<DD>* s = (Instances<DD>*)o->p;
+    s->guarded = guarded != 0;
+    for (auto& m : s->meshes) m->guarded = guarded != 0;
+  });
 }
 void orc_inst_set_transforms(void* h, const float* rot, const float* trans, const float* halves) {
   OInst* o = (OInst*)h;

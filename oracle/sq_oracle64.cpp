@@ -21,10 +21,14 @@
 //                                  limit) with the QueryStats counters of stats.rs:39-50,
 //                                  over raycast_nearest_batch_parallel's contiguous chunks
 //                                  (bvh.rs:160-193): the timed CPU baseline
-// over the built-in BOX and SPHERE providers (box-guarded sphere, as the
-// engine's: the hit is kept only inside the slab interval of its own box).
-// BVH == naive scan is the reference's stated contract (sq/src/lib.rs:15-17),
-// so the naive scans define the expected query results.
+//   * the Bvh walks of every query  bvh.rs:196-259 (all hits), :331-414 (shape casts),
+//                                  :418-456 (overlap), and the Tlas walk tlas.rs:104-154
+//                                  over box instances: the expected query results
+// over the built-in BOX and SPHERE providers.  A `kind` with FLAG_GUARDED set
+// restates the engine's SQ_HITS_GUARDED mode instead (a sphere or instance hit
+// kept only inside the slab interval of its own box, so that BVH == naive scan,
+// the reference's stated contract, sq/src/lib.rs:15-17, holds exactly); with
+// FLAG_BVH the queries run the reference's Bvh walks, else the naive scans.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -36,6 +40,9 @@ namespace orc64 {
 
 static const double F64_EPSILON = 2.220446049250313e-16;  // f64::EPSILON
 static const uint32_t LEAF_SIZE = 4;                      // sq/src/accel/tree.rs:12
+// Flags or-ed into the `kind` argument of the C entry points.
+static const int FLAG_GUARDED = 0x100;   // the engine's SQ_HITS_GUARDED leaf tests
+static const int FLAG_BVH = 0x200;       // the reference's Bvh / Tlas walks instead of the naive scans
 
 struct Ray { const double* o; const double* d; };
 
@@ -75,6 +82,7 @@ template <int D> struct Geom {
   uint32_t n;
   const double* prims;
   const uint64_t* layers;
+  bool guarded = false;   // sphere hits and contacts kept inside their box's slab interval
   int stride() const { return kind == 0 ? 2 * D : D + 1; }
   const double* p(size_t leaf) const { return prims + leaf * stride(); }
   // QueryGeometry::world_aabb: the box, or Aabb::new(c - splat(r), c + splat(r))
@@ -91,7 +99,8 @@ template <int D> struct Geom {
     return (lay & mask) != 0;
   }
   // test_ray: BOX answers the slab entry with the entry face normal; SPHERE the
-  // ball (entry root, the exit root from inside), guarded by its own box.
+  // ball (entry root, the exit root from inside), guarded by its own box when
+  // `guarded`.
   bool test_ray(size_t leaf, const Ray& r, double max_toi, double* t_out, double* normal) const {
     const double* q = p(leaf);
     double mn[D], mx[D];
@@ -115,8 +124,8 @@ template <int D> struct Geom {
     double t = -b - sq;
     if (t < 0.0) t = -b + sq;
     if (t < 0.0 || t > max_toi) return false;
-    if (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &axis)) return false;
-    if (!(te <= t && t <= tx)) return false;
+    if (guarded && !ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &axis)) return false;
+    if (guarded && !(te <= t && t <= tx)) return false;
     *t_out = t;
     // point.rs:82-89 normalize_or_zero(at(t) - c)
     double v[D];
@@ -134,7 +143,8 @@ template <int D> struct Geom {
     return ray_intersect<D>(gmn, gmx, r, max_toi, te, tx, &ax);
   }
   // test_shape_cast: BOX the grown slab's entry; SPHERE the ball of radius
-  // r + probe.max[0] (shape_overlap.rs:104-118), guarded by the grown box
+  // r + probe.max[0] (shape_overlap.rs:104-118), guarded by the grown box when
+  // `guarded`
   bool test_cast(size_t leaf, const Ray& r, const double* pmin, const double* pmax, double max_toi, double* t_out) const {
     const double* q = p(leaf);
     double mn[D], mx[D], te, tx;
@@ -155,8 +165,8 @@ template <int D> struct Geom {
     double t = -b - sq;
     if (t < 0.0) t = -b + sq;
     if (t < 0.0 || t > max_toi) return false;
-    if (!broadphase(mn, mx, r, pmin, pmax, max_toi, &te, &tx)) return false;
-    if (!(te <= t && t <= tx)) return false;
+    if (guarded && !broadphase(mn, mx, r, pmin, pmax, max_toi, &te, &tx)) return false;
+    if (guarded && !(te <= t && t <= tx)) return false;
     *t_out = t;
     return true;
   }
@@ -369,9 +379,34 @@ template <int D> struct Tree64 {
   Tree64(const Geom<D>& g) {
     std::vector<double> boxes((size_t)g.n * 2 * D);
     for (uint32_t i = 0; i < g.n; ++i) g.world_aabb(i, boxes.data() + (size_t)i * 2 * D, boxes.data() + (size_t)i * 2 * D + D);
-    idx.resize(g.n);
-    for (uint32_t i = 0; i < g.n; ++i) idx[i] = i;
-    if (g.n) build<D>(boxes, idx, 0, g.n, nodes);
+    init(boxes, g.n);
+  }
+  // over leaf boxes [n][min D, max D] (Tlas::build, tlas.rs:35-45: the world boxes)
+  Tree64(const std::vector<double>& boxes, uint32_t n) { init(boxes, n); }
+  void init(const std::vector<double>& boxes, uint32_t n) {
+    idx.resize(n);
+    for (uint32_t i = 0; i < n; ++i) idx[i] = i;
+    if (n) build<D>(boxes, idx, 0, n, nodes);
+  }
+  // The reference's walk (bvh.rs:73-124 and its siblings): pop a node, test it
+  // with pass(min, max) (which reads the caller's current limit), then either
+  // hand each primitive of a leaf to leaf(id) in prim_indices order, or push
+  // child a, then child b.
+  template <class Pass, class LeafFn> void walk(Pass&& pass, LeafFn&& leaf) const {
+    if (nodes.empty()) return;
+    std::vector<uint32_t> stack(1, 0u);
+    while (!stack.empty()) {
+      const uint32_t ni = stack.back();
+      stack.pop_back();
+      const Node& nd = nodes[ni];
+      if (!pass(nd.b.data(), nd.b.data() + D)) continue;
+      if (nd.pc > 0) {
+        for (uint32_t k = nd.a; k < nd.a + nd.pc; ++k) leaf((size_t)idx[k]);
+      } else {
+        stack.push_back(nd.a);
+        stack.push_back(nd.c);
+      }
+    }
   }
   // bvh.rs:51-124; counts nodes tested and primitives tested (stats.rs:39-50)
   bool nearest(const Geom<D>& g, const Ray& r, double max_toi, int64_t* leaf_out, double* t_out, uint64_t* n_nodes,
@@ -420,16 +455,66 @@ template <class F> static void parallel_for(uint64_t n, int threads, F&& f) {
   for (auto& t : th) t.join();
 }
 
+// The geometry of a `kind` argument with its flags.
+template <int D> static Geom<D> geom_of(int kindf, uint32_t n, const double* prims, const uint64_t* layers) {
+  Geom<D> g{kindf & 0xFF, n, prims, layers};
+  g.guarded = (kindf & FLAG_GUARDED) != 0;
+  return g;
+}
+
+// Bvh::raycast_nearest (bvh.rs:73-124) with the normal of the best hit.
 template <int D>
-static void nearest_batch(int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask, int64_t exclude,
+static bool bvh_nearest(const Tree64<D>& tree, const Geom<D>& g, const Ray& r, double max_toi, uint64_t mask, int64_t exclude,
+                        int64_t* leaf_out, double* t_out, double* normal_out) {
+  bool have = false;
+  double limit = max_toi, best_t = 0.0;
+  int64_t best_leaf = -1;
+  tree.walk([&](const double* mn, const double* mx) { double te, tx; int ax; return ray_intersect<D>(mn, mx, r, limit, &te, &tx, &ax); },
+            [&](size_t leaf) {
+              if (!g.accepts(leaf, mask, exclude)) return;
+              double t, nrm[D];
+              if (!g.test_ray(leaf, r, limit, &t, nrm)) return;
+              if (have && !hit_less(t, (int64_t)leaf, best_t, best_leaf)) return;
+              have = true; best_t = t; best_leaf = (int64_t)leaf; limit = t;
+              for (int i = 0; i < D; ++i) normal_out[i] = nrm[i];
+            });
+  *leaf_out = best_leaf; *t_out = best_t;
+  return have;
+}
+
+// Bvh::shapecast_nearest (bvh.rs:331-375): nodes tested by the probe-grown slab.
+template <int D>
+static bool bvh_cast_nearest(const Tree64<D>& tree, const Geom<D>& g, const Ray& r, const double* pmin, const double* pmax,
+                             double max_toi, uint64_t mask, int64_t exclude, int64_t* leaf_out, double* t_out) {
+  bool have = false;
+  double limit = max_toi, best_t = 0.0;
+  int64_t best_leaf = -1;
+  tree.walk([&](const double* mn, const double* mx) { double te, tx; return g.broadphase(mn, mx, r, pmin, pmax, limit, &te, &tx); },
+            [&](size_t leaf) {
+              if (!g.accepts(leaf, mask, exclude)) return;
+              double t;
+              if (!g.test_cast(leaf, r, pmin, pmax, limit, &t)) return;
+              if (have && !hit_less(t, (int64_t)leaf, best_t, best_leaf)) return;
+              have = true; best_t = t; best_leaf = (int64_t)leaf; limit = t;
+            });
+  *leaf_out = best_leaf; *t_out = best_t;
+  return have;
+}
+
+template <int D>
+static void nearest_batch(int kindf, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask, int64_t exclude,
                           const double* rays, uint64_t nr, double max_toi, const double* tmax, int threads, int64_t* leaf,
                           double* t, double* point, double* normal) {
-  Geom<D> g{kind, n, prims, layers};
+  const Geom<D> g = geom_of<D>(kindf, n, prims, layers);
+  const bool bvh = (kindf & FLAG_BVH) != 0;
+  const Tree64<D> tree(bvh ? g : Geom<D>{g.kind, 0, prims, layers});
   parallel_for(nr, threads, [&](uint64_t i) {
     Ray r{rays + i * 2 * D, rays + i * 2 * D + D};
     double nrm[D];
     int64_t l; double tt;
-    const bool have = naive_nearest<D>(g, r, tmax ? tmax[i] : max_toi, mask, exclude, &l, &tt, nrm);
+    const double lim = tmax ? tmax[i] : max_toi;
+    const bool have = bvh ? bvh_nearest<D>(tree, g, r, lim, mask, exclude, &l, &tt, nrm)
+                          : naive_nearest<D>(g, r, lim, mask, exclude, &l, &tt, nrm);
     leaf[i] = have ? l : -1;
     t[i] = have ? tt : INFINITY;
     for (int k = 0; k < D; ++k) {
@@ -440,15 +525,19 @@ static void nearest_batch(int kind, const double* prims, uint32_t n, const uint6
 }
 
 template <int D>
-static void cast_nearest_batch(int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
+static void cast_nearest_batch(int kindf, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
                                int64_t exclude, const double* casts, uint64_t nc, double max_toi, const double* tmax,
                                int threads, int64_t* leaf, double* t, double* point) {
-  Geom<D> g{kind, n, prims, layers};
+  const Geom<D> g = geom_of<D>(kindf, n, prims, layers);
+  const bool bvh = (kindf & FLAG_BVH) != 0;
+  const Tree64<D> tree(bvh ? g : Geom<D>{g.kind, 0, prims, layers});
   parallel_for(nc, threads, [&](uint64_t i) {
     const double* c = casts + i * 4 * D;
     Ray r{c, c + D};
     int64_t l; double tt;
-    const bool have = naive_cast_nearest<D>(g, r, c + 2 * D, c + 3 * D, tmax ? tmax[i] : max_toi, mask, exclude, &l, &tt);
+    const double lim = tmax ? tmax[i] : max_toi;
+    const bool have = bvh ? bvh_cast_nearest<D>(tree, g, r, c + 2 * D, c + 3 * D, lim, mask, exclude, &l, &tt)
+                          : naive_cast_nearest<D>(g, r, c + 2 * D, c + 3 * D, lim, mask, exclude, &l, &tt);
     leaf[i] = have ? l : -1;
     t[i] = have ? tt : INFINITY;
     for (int k = 0; k < D; ++k)
@@ -458,12 +547,51 @@ static void cast_nearest_batch(int kind, const double* prims, uint32_t n, const 
 
 // mode 0: rays, 1: overlap regions, 2: shape casts
 template <int D>
-static uint64_t all_batch(int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask, int64_t exclude,
+static uint64_t all_batch(int kindf, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask, int64_t exclude,
                           const double* q, uint64_t nq, double max_toi, const double* tmax, int mode, int threads,
                           uint64_t* offsets, int64_t* leaf, double* t, uint64_t cap) {
-  Geom<D> g{kind, n, prims, layers};
+  const Geom<D> g = geom_of<D>(kindf, n, prims, layers);
+  const bool bvh = (kindf & FLAG_BVH) != 0;
+  const Tree64<D> tree(bvh ? g : Geom<D>{g.kind, 0, prims, layers});
   std::vector<std::vector<std::pair<double, int64_t>>> res(nq);
+  auto by_hit = [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
+    return hit_less(a.first, a.second, b.first, b.second);
+  };
   parallel_for(nq, threads, [&](uint64_t i) {
+    if (bvh) {
+      // bvh.rs:196-259 (all hits), :378-414 (cast all), :418-456 (overlap, by leaf)
+      auto& out = res[i];
+      if (mode == 2) {
+        const double* c = q + i * 4 * D;
+        Ray r{c, c + D};
+        const double lim = tmax ? tmax[i] : max_toi;
+        tree.walk([&](const double* mn, const double* mx) { double te, tx; return g.broadphase(mn, mx, r, c + 2 * D, c + 3 * D, lim, &te, &tx); },
+                  [&](size_t lf) {
+                    double tt;
+                    if (g.accepts(lf, mask, exclude) && g.test_cast(lf, r, c + 2 * D, c + 3 * D, lim, &tt)) out.push_back({tt, (int64_t)lf});
+                  });
+        std::stable_sort(out.begin(), out.end(), by_hit);
+      } else if (mode == 1) {
+        const double *rmn = q + i * 2 * D, *rmx = rmn + D;
+        tree.walk([&](const double* mn, const double* mx) {
+                    for (int k = 0; k < D; ++k) if (mn[k] > rmx[k] || mx[k] < rmn[k]) return false;
+                    return true;
+                  },
+                  [&](size_t lf) { if (g.accepts(lf, mask, exclude) && g.test_overlap(lf, rmn, rmx)) out.push_back({0.0, (int64_t)lf}); });
+        std::stable_sort(out.begin(), out.end(), [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
+          return a.second < b.second;
+        });
+      } else {
+        Ray r{q + i * 2 * D, q + i * 2 * D + D};
+        tree.walk([&](const double* mn, const double* mx) { double te, tx; int ax; return ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &ax); },
+                  [&](size_t lf) {
+                    double tt, nrm[D];
+                    if (g.accepts(lf, mask, exclude) && g.test_ray(lf, r, max_toi, &tt, nrm)) out.push_back({tt, (int64_t)lf});
+                  });
+        std::stable_sort(out.begin(), out.end(), by_hit);
+      }
+      return;
+    }
     if (mode == 2) {
       const double* c = q + i * 4 * D;
       Ray r{c, c + D};
@@ -490,9 +618,9 @@ static uint64_t all_batch(int kind, const double* prims, uint32_t n, const uint6
 
 // raycast_nearest_batch_parallel (bvh.rs:160-193): contiguous chunks, one per thread
 template <int D>
-static void bvh_nearest_batch(int kind, const double* prims, uint32_t n, const double* rays, uint64_t nr, double max_toi,
+static void bvh_nearest_batch(int kindf, const double* prims, uint32_t n, const double* rays, uint64_t nr, double max_toi,
                               int threads, int64_t* leaf, double* t, uint64_t* counts) {
-  Geom<D> g{kind, n, prims, nullptr};
+  const Geom<D> g = geom_of<D>(kindf, n, prims, nullptr);
   const Tree64<D> tree(g);
   std::vector<uint64_t> nodes(std::max(threads, 1), 0), prs(std::max(threads, 1), 0);
   const int T = std::max(threads, 1);
@@ -519,6 +647,7 @@ static void bvh_nearest_batch(int kind, const double* prims, uint32_t n, const d
 template <int D> struct Inst64 {
   uint64_t n;
   const double *rot, *trans, *half;
+  bool guarded = false;   // the engine's SQ_HITS_GUARDED: a hit kept inside its world box's slab interval
   // Isometry::transform_aabb of [-half, half]: centre R c + t, half extent |R| h
   void world_box(uint64_t i, double* mn, double* mx) const {
     const double *R = rot + i * D * D, *t = trans + i * D, *h = half + i * D;
@@ -534,8 +663,8 @@ template <int D> struct Inst64 {
     }
   }
   // test_instance: inverse transform (rotate_transpose of o - t, and of d), the
-  // local box slab with its 1e-9 parallel threshold and t > 1e-6 entry rule,
-  // then the guard against the world box
+  // local box slab with its 1e-9 parallel threshold and t > 1e-6 entry rule
+  // (guarded: then the guard against the world box)
   bool test(uint64_t i, const Ray& r, double max_toi, double* t_out) const {
     const double *R = rot + i * D * D, *t = trans + i * D, *h = half + i * D;
     double v[D], o[D], d[D];
@@ -560,6 +689,7 @@ template <int D> struct Inst64 {
       }
     }
     if (!(axis >= 0 && t_min > 1.0e-6 && t_min <= max_toi)) return false;
+    if (!guarded) { *t_out = t_min; return true; }
     double mn[D], mx[D], te, tx; int ax;
     world_box(i, mn, mx);
     if (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &ax)) return false;
@@ -569,18 +699,37 @@ template <int D> struct Inst64 {
   }
 };
 
+// flags: FLAG_GUARDED (the engine's guard), FLAG_BVH (the Tlas walk of
+// tlas.rs:104-154 over the median-split tree of the world boxes, in the
+// reference's order; else the naive instance scan of two_level.rs:126-167).
 template <int D>
 static void tlas_box_nearest(uint64_t n, const double* rot, const double* trans, const double* half, const double* rays,
-                             uint64_t nr, double max_toi, int threads, int64_t* leaf, double* t, double* world) {
+                             uint64_t nr, double max_toi, int threads, int flags, int64_t* leaf, double* t, double* world) {
   Inst64<D> s{n, rot, trans, half};
-  if (world) for (uint64_t i = 0; i < n; ++i) s.world_box(i, world + i * 2 * D, world + i * 2 * D + D);
+  s.guarded = (flags & FLAG_GUARDED) != 0;
+  std::vector<double> wb(n * 2 * D);
+  for (uint64_t i = 0; i < n; ++i) s.world_box(i, wb.data() + i * 2 * D, wb.data() + i * 2 * D + D);
+  if (world) std::copy(wb.begin(), wb.end(), world);
+  const bool bvh = (flags & FLAG_BVH) != 0;
+  const Tree64<D> tree(wb, bvh ? (uint32_t)n : 0u);
   parallel_for(nr, threads, [&](uint64_t q) {
     Ray r{rays + q * 2 * D, rays + q * 2 * D + D};
     int64_t bl = -1; double bt = 0.0;
-    for (uint64_t i = 0; i < n; ++i) {   // two_level.rs:126-167
-      double tt;
-      if (!s.test(i, r, max_toi, &tt)) continue;
-      if (bl < 0 || hit_less(tt, (int64_t)i, bt, bl)) { bl = (int64_t)i; bt = tt; }
+    if (bvh) {
+      double limit = max_toi;
+      tree.walk([&](const double* mn, const double* mx) { double te, tx; int ax; return ray_intersect<D>(mn, mx, r, limit, &te, &tx, &ax); },
+                [&](size_t i) {
+                  double tt;
+                  if (!s.test(i, r, limit, &tt)) return;
+                  if (bl >= 0 && !hit_less(tt, (int64_t)i, bt, bl)) return;
+                  bl = (int64_t)i; bt = tt; limit = tt;
+                });
+    } else {
+      for (uint64_t i = 0; i < n; ++i) {   // two_level.rs:126-167
+        double tt;
+        if (!s.test(i, r, max_toi, &tt)) continue;
+        if (bl < 0 || hit_less(tt, (int64_t)i, bt, bl)) { bl = (int64_t)i; bt = tt; }
+      }
     }
     leaf[q] = bl;
     t[q] = bl >= 0 ? bt : INFINITY;
@@ -646,13 +795,13 @@ void orc64_bvh_raycast_nearest(int D, int kind, const double* prims, uint32_t n,
 }
 
 void orc64_tlas_box_nearest(int D, uint64_t n, const double* rot, const double* trans, const double* half,
-                            const double* rays, uint64_t nr, double max_toi, int threads, int64_t* leaf, double* t,
+                            const double* rays, uint64_t nr, double max_toi, int threads, int flags, int64_t* leaf, double* t,
                             double* world) {
-  ORC64_DISPATCH(D, tlas_box_nearest<DD>(n, rot, trans, half, rays, nr, max_toi, threads, leaf, t, world));
+  ORC64_DISPATCH(D, tlas_box_nearest<DD>(n, rot, trans, half, rays, nr, max_toi, threads, flags, leaf, t, world));
 }
 
 void orc64_tree_stats(int D, int kind, const double* prims, uint32_t n, uint64_t* out5, double* quality) {
-  ORC64_DISPATCH(D, tree_stats<DD>(prims, kind, n, out5, quality));
+  ORC64_DISPATCH(D, tree_stats<DD>(prims, kind & 0xFF, n, out5, quality));
 }
 
 }  // extern "C"

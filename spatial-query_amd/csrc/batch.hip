@@ -47,7 +47,8 @@ __device__ __forceinline__ uint32_t find_item(const Item* __restrict__ items, ui
   return lo;
 }
 
-template <int D, int KIND>
+// EXACT: the reference's hits (pool_walk); else the guarded kinds, near-first.
+template <int D, int KIND, bool EXACT>
 __global__ __launch_bounds__(256, (D <= 3 ? 8 : 4)) void k_batch_bvh(const BvhItem* __restrict__ items, uint32_t n_items,
                                                                      uint32_t n_chunks, uint32_t per_lane, uint32_t S,
                                                                      SpillEntry* spill, uint32_t spill_stride, uint32_t* counter,
@@ -64,15 +65,16 @@ __global__ __launch_bounds__(256, (D <= 3 ? 8 : 4)) void k_batch_bvh(const BvhIt
     const BvhItem& it = items[find_item(items, n_items, c)];
     const uint32_t end = it.chunk0 + (uint32_t)(((uint64_t)it.n + pool - 1) / pool);
     const NearestOut out{it.leaf, it.toi, nullptr, nullptr, nullptr, nullptr};
-    pool_walk<D, KIND, false, false, false>(it.s, it.rays, ItemChunks{counter, it.chunk0, end, it.n, pool, &c}, it.max_toi,
+    pool_walk<D, KIND, false, false, false, ItemChunks, false, EXACT>(it.s, it.rays, ItemChunks{counter, it.chunk0, end, it.n, pool, &c}, it.max_toi,
                                             nullptr, nof, out, st, nullptr, pause_below);
   }
   pool_retire(counter);
 }
 
 // Two-level scenes: the resumable two-level walk (pool_walk_tlas) over each
-// chunk, both stacks S-entry LDS rings spilling to their own columns.
-template <int D, int IK>
+// chunk, both stacks S-entry LDS rings spilling to their own columns.  EXACT as
+// pool_walk_tlas.
+template <int D, int IK, bool EXACT>
 __global__ __launch_bounds__(256) void k_batch_tlas(const TlasItem* __restrict__ items, uint32_t n_items, uint32_t n_chunks,
                                                     uint32_t per_lane, uint32_t S, SpillEntry* spill_top, SpillEntry* spill_bot,
                                                     uint32_t spill_stride, uint32_t* counter, uint32_t pause_below) {
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(256) void k_batch_tlas(const TlasItem* __restrict__
     const TlasItem& it = items[find_item(items, n_items, c)];
     const uint32_t end = it.chunk0 + (uint32_t)(((uint64_t)it.n + pool - 1) / pool);
     const NearestOut out{it.inst, it.toi, nullptr, nullptr, nullptr, nullptr};
-    pool_walk_tlas<D, IK, false>(it.tv, it.rays, ItemChunks{counter, it.chunk0, end, it.n, pool, &c}, it.max_toi, nullptr, out,
+    pool_walk_tlas<D, IK, false, ItemChunks, EXACT>(it.tv, it.rays, ItemChunks{counter, it.chunk0, end, it.n, pool, &c}, it.max_toi, nullptr, out,
                                  st, sb, pause_below);
   }
   pool_retire(counter);
@@ -107,7 +109,22 @@ struct sq_batch {
   // or deepened it past the spill rows sized here is refused at submit
   std::vector<std::pair<const sq_bvh*, SceneView>> scenes;
   uint32_t stack_cap = 0;
+  bool exact = true;               // the context's hit mode when the set was made
 };
+
+// Call fb<D, K, EXACT>() (BVH sets) or ft<D, IK, EXACT>() (two-level sets) for
+// the set's dimension, kind and hit mode (AABB leaves have one walk, exact in
+// both modes).
+template <class FB, class FT> static int dispatch_set(const sq_batch* b, FB&& fb, FT&& ft) {
+  if (b->cls == SQ_BATCH_BVH)
+    return dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+      if constexpr (K == KIND_AABB) return fb.template operator()<D, K, true>();
+      else return b->exact ? fb.template operator()<D, K, true>() : fb.template operator()<D, K, false>();
+    });
+  return dispatch_inst(b->dim, b->kind, [&]<int D, int K>() -> int {
+    return b->exact ? ft.template operator()<D, K, true>() : ft.template operator()<D, K, false>();
+  });
+}
 
 
 extern "C" {
@@ -123,6 +140,7 @@ int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* qs, uint
   b->S = (uint32_t)ctx->lds_stack;
   b->block = (uint32_t)std::min(ctx->block, 256);
   b->descend_min = (uint32_t)ctx->descend_min;
+  b->exact = ctx->hit_mode != SQ_HITS_GUARDED;
   const uint64_t pool = 64ull * b->per_lane;
   std::vector<BvhItem> bi;
   std::vector<TlasItem> ti;
@@ -185,21 +203,19 @@ int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* qs, uint
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     int per_cu = 0;
-    if (scene_class == SQ_BATCH_BVH) {
-      b->lds = (size_t)b->S * b->block * 8;
-      rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_batch_bvh<D, K>), (int)b->block,
-                                                             b->lds));
-        return SQ_OK;
-      });
-    } else {
-      b->lds = (size_t)2 * b->S * b->block * 8;
-      rc = dispatch_inst(b->dim, b->kind, [&]<int D, int K>() -> int {
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_batch_tlas<D, K>), (int)b->block,
-                                                             b->lds));
-        return SQ_OK;
-      });
-    }
+    b->lds = (size_t)(scene_class == SQ_BATCH_BVH ? 1 : 2) * b->S * b->block * 8;
+    rc = dispatch_set(
+        b.get(),
+        [&]<int D, int K, bool EXACT>() -> int {
+          HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_batch_bvh<D, K, EXACT>),
+                                                               (int)b->block, b->lds));
+          return SQ_OK;
+        },
+        [&]<int D, int K, bool EXACT>() -> int {
+          HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_batch_tlas<D, K, EXACT>),
+                                                               (int)b->block, b->lds));
+          return SQ_OK;
+        });
     if (rc) return rc;
     const uint64_t waves_per_block = b->block / 64;
     const uint64_t need = (chunks + waves_per_block - 1) / waves_per_block;
@@ -241,23 +257,24 @@ int sq_batch_submit(sq_batch* b) {
     if ((rc = fill_miss(c, e.first, b->dim, e.second))) return rc;
   if (b->n_items) {
     const uint32_t spill_stride = b->grid * b->block;
-    if (b->cls == SQ_BATCH_BVH)
-      rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
-        k_batch_bvh<D, K><<<b->grid, b->block, b->lds, c->stream>>>(b->items.as<BvhItem>(), b->n_items, b->n_chunks, b->per_lane, b->S,
-                                                                   b->spill.as<SpillEntry>(), spill_stride, b->counter.as<uint32_t>(),
-                                                                   b->descend_min);
-        HIP_TRY(hipGetLastError());
-        return SQ_OK;
-      });
-    else
-      rc = dispatch_inst(b->dim, b->kind, [&]<int D, int K>() -> int {
-        SpillEntry* sp = b->spill.as<SpillEntry>();
-        k_batch_tlas<D, K><<<b->grid, b->block, b->lds, c->stream>>>(b->items.as<TlasItem>(), b->n_items, b->n_chunks, b->per_lane,
-                                                                    b->S, sp, sp + (size_t)b->rows_top * spill_stride, spill_stride,
-                                                                    b->counter.as<uint32_t>(), b->descend_min);
-        HIP_TRY(hipGetLastError());
-        return SQ_OK;
-      });
+    SpillEntry* sp = b->spill.as<SpillEntry>();
+    rc = dispatch_set(
+        b,
+        [&]<int D, int K, bool EXACT>() -> int {
+          k_batch_bvh<D, K, EXACT><<<b->grid, b->block, b->lds, c->stream>>>(b->items.as<BvhItem>(), b->n_items, b->n_chunks,
+                                                                            b->per_lane, b->S, sp, spill_stride,
+                                                                            b->counter.as<uint32_t>(), b->descend_min);
+          HIP_TRY(hipGetLastError());
+          return SQ_OK;
+        },
+        [&]<int D, int K, bool EXACT>() -> int {
+          k_batch_tlas<D, K, EXACT><<<b->grid, b->block, b->lds, c->stream>>>(b->items.as<TlasItem>(), b->n_items, b->n_chunks,
+                                                                             b->per_lane, b->S, sp,
+                                                                             sp + (size_t)b->rows_top * spill_stride, spill_stride,
+                                                                             b->counter.as<uint32_t>(), b->descend_min);
+          HIP_TRY(hipGetLastError());
+          return SQ_OK;
+        });
     if (rc) return rc;
   }
   return end_timing(c);

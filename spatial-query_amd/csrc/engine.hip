@@ -25,22 +25,24 @@ thread_local std::string g_err;
 // ---------------------------------------------------------------------------
 
 // Nearest / any-hit rays (RayPolicy) and nearest shape casts (CastPolicy).
-// Query i reads QS floats at q + i*QS.
-template <int D, int KIND, bool ANY, bool FILT, bool CAST>
+// Query i reads QS floats at q + i*QS.  EXACT: the reference's hits (unguarded
+// narrow phases, nearest queries whose hits may leave their box in the
+// reference's order); else the guarded kinds, near-first.
+template <int D, int KIND, bool ANY, bool FILT, bool CAST, bool EXACT>
 __global__ __launch_bounds__(1024) void k_nearest(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                   const float* __restrict__ tmax, Filter f, NearestOut out, uint32_t stack_n) {
   extern __shared__ uint32_t lds[];
   const uint32_t B = blockDim.x, tid = threadIdx.x;
   const uint32_t i = blockIdx.x * B + tid;
   if (i >= n) return;
-  using Pol = typename std::conditional<CAST, CastPolicy<D, KIND>, RayPolicy<D, KIND>>::type;
+  using Pol = typename std::conditional<CAST, CastPolicy<D, KIND, float, !EXACT>, RayPolicy<D, KIND, !EXACT>>::type;
   constexpr int QS = CAST ? 4 * D : 2 * D;
   constexpr int PS = Prim<D, KIND>::STRIDE;
   Pol pol;
   pol.setup(q + (size_t)i * QS);
   const float lim = tmax ? tmax[i] : max_toi;
   Stack st{lds + tid, reinterpret_cast<float*>(lds + stack_n * B) + tid, B};
-  NearestResult h = walk_nearest<D, PS, ANY, FILT>(s, pol, lim, f, st);
+  NearestResult h = walk_nearest<D, PS, ANY, FILT, EXACT && !ANY && !Pol::INSIDE>(s, pol, lim, f, st);
   if (ANY) { out.any[i] = h.leaf != NO_LEAF ? 1 : 0; return; }
   const bool hit = h.leaf != NO_LEAF;
   out.leaf[i] = hit ? (int32_t)h.leaf : -1;
@@ -67,8 +69,8 @@ __global__ __launch_bounds__(1024) void k_nearest(SceneView s, const float* __re
 // across chunk boundaries (pool_walk); the counters start zero and the launch
 // leaves them zero (pool_retire).  With `order`, pool
 // position i is query order[i] (the Morton order).  QUAD walks the 4-wide
-// records s.quads (k_quads) instead of the pair nodes.
-template <int D, int KIND, bool ANY, bool FILT, bool CAST, bool QUAD = false>
+// records s.quads (k_quads) instead of the pair nodes.  EXACT as k_nearest.
+template <int D, int KIND, bool ANY, bool FILT, bool CAST, bool QUAD, bool EXACT>
 __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nearest_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                       const float* __restrict__ tmax, Filter f, NearestOut out,
                                                       uint32_t chunk, uint32_t* counter, uint32_t S, SpillEntry* spill,
@@ -80,8 +82,8 @@ __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nea
   RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
   const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
   const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
-  pool_walk<D, KIND, ANY, FILT, CAST, Chunks, QUAD>(s, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, f, out, st, order,
-                                                  pause_below);
+  pool_walk<D, KIND, ANY, FILT, CAST, Chunks, QUAD, EXACT>(s, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, f, out,
+                                                         st, order, pause_below);
   pool_retire(counter);
 }
 
@@ -140,6 +142,7 @@ __global__ __launch_bounds__(1024) void k_collect(SceneView s, const float* __re
     if (box_overlap<D>(s.root, s.root + D, rmn, rmx)) {
       walk_visit<D>(s, root_ref,
         [&](const float* nd, bool& ha, bool& hb) {
+          if (KIND != KIND_AABB && synthetic_pair<D>(nd)) { ha = hb = true; return; }   // parts of one reference leaf
           float amn[D], amx[D], bmn[D], bmx[D];
 #pragma unroll
           for (int k = 0; k < D; ++k) { amn[k] = nd[4 * k]; amx[k] = nd[4 * k + 1]; bmn[k] = nd[4 * k + 2]; bmx[k] = nd[4 * k + 3]; }
@@ -164,7 +167,11 @@ __global__ __launch_bounds__(1024) void k_collect(SceneView s, const float* __re
     const float lim = tmax ? tmax[i] : max_toi;
     if (pol.root(s.root, lim)) {
       walk_visit<D>(s, root_ref,
-        [&](const float* nd, bool& ha, bool& hb) { float ta, tb; pol.pair(nd, lim, ha, hb, ta, tb); },
+        [&](const float* nd, bool& ha, bool& hb) {
+          if (KIND != KIND_AABB && synthetic_pair<D>(nd)) { ha = hb = true; return; }   // parts of one reference leaf
+          float ta, tb;
+          pol.pair(nd, lim, ha, hb, ta, tb);
+        },
         [&](uint32_t first, uint32_t c) {
           for (uint32_t j = 0; j < c; ++j) {
             const uint32_t pos = first + j;
@@ -787,6 +794,15 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   return SQ_OK;
 }
 
+int sq_ctx_set_hit_mode(sq_ctx* c, int mode) {
+  if (!c) return fail(SQ_ERR_INVALID, "ctx is NULL");
+  if (mode != SQ_HITS_REFERENCE && mode != SQ_HITS_GUARDED) return fail(SQ_ERR_INVALID, "hit mode must be SQ_HITS_REFERENCE or SQ_HITS_GUARDED");
+  c->hit_mode = mode;
+  return SQ_OK;
+}
+
+int sq_ctx_get_hit_mode(const sq_ctx* c) { return c ? c->hit_mode : -1; }
+
 int sq_ctx_get_tuning(const sq_ctx* c, sq_tuning* t) {
   if (!c || !t) return fail(SQ_ERR_INVALID, "NULL argument");
   t->block = c->block;
@@ -1090,12 +1106,18 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
     if (rc) return rc;
     rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
       dim3 grid(grid_for(n, B)), block(B);
+      auto go = [&]<bool EXACT_>() {
 #define SQ_LAUNCH(ANY_, FILT_, CAST_) \
-      k_nearest<D, K, ANY_, FILT_, CAST_><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, out, stack_n)
-      if (cast) { if (filt) SQ_LAUNCH(false, true, true); else SQ_LAUNCH(false, false, true); }
-      else if (any) { if (filt) SQ_LAUNCH(true, true, false); else SQ_LAUNCH(true, false, false); }
-      else { if (filt) SQ_LAUNCH(false, true, false); else SQ_LAUNCH(false, false, false); }
+        k_nearest<D, K, ANY_, FILT_, CAST_, EXACT_><<<grid, block, lds, c->stream>>>(s, q, nn, max_toi, tmax, fd, out, stack_n)
+        if (cast) { if (filt) SQ_LAUNCH(false, true, true); else SQ_LAUNCH(false, false, true); }
+        else if (any) { if (filt) SQ_LAUNCH(true, true, false); else SQ_LAUNCH(true, false, false); }
+        else { if (filt) SQ_LAUNCH(false, true, false); else SQ_LAUNCH(false, false, false); }
 #undef SQ_LAUNCH
+      };
+      // AABB hits lie inside their boxes: the one walk is exact in both modes
+      if constexpr (K == KIND_AABB) go.template operator()<true>();
+      else if (c->hit_mode == SQ_HITS_GUARDED) go.template operator()<false>();
+      else go.template operator()<true>();
       HIP_TRY(hipGetLastError());
       return SQ_OK;
     });
@@ -1108,9 +1130,10 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
   const uint32_t per = pooled_per_lane(c, n);
   const uint64_t chunk = 64ull * per;
   const uint64_t chunks = (n + chunk - 1) / chunk;
-  // 4-wide walk (nearest and any-hit without a filter): up to 3 pushes per
-  // record over half the levels
-  const bool quad = c->wide && !cast && !filt;
+  // 4-wide walk (nearest and any-hit without a filter, over hits inside their
+  // boxes: AABB leaves, or the guarded kinds): up to 3 pushes per record over
+  // half the levels
+  const bool quad = c->wide && !cast && !filt && (b->kind == KIND_AABB || c->hit_mode == SQ_HITS_GUARDED);
   const uint32_t walk_stack = quad ? 3 * (stack_n / 2 + 1) + 1 : stack_n;
   const uint32_t spill_rows = walk_stack > S ? walk_stack - S : 0;
   const size_t lds = (size_t)S * B * 8;
@@ -1135,8 +1158,8 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
     if ((rc = ray_order(c, b->dim, q, nn, 2 * b->dim, b->root.as<float>(), &order))) return rc;
   uint32_t* counter = c->pool_ctr.as<uint32_t>();
   rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
-    auto launch = [&]<bool ANY_, bool FILT_, bool CAST_, bool QUAD_ = false>() -> int {
-      const void* kern = reinterpret_cast<const void*>(&k_nearest_pool<D, K, ANY_, FILT_, CAST_, QUAD_>);
+    auto launch = [&]<bool ANY_, bool FILT_, bool CAST_, bool QUAD_, bool EXACT_>() -> int {
+      const void* kern = reinterpret_cast<const void*>(&k_nearest_pool<D, K, ANY_, FILT_, CAST_, QUAD_, EXACT_>);
       int per_cu = 0;
       if (int r0 = blocks_per_cu(kern, c->device, B, lds, &per_cu)) return r0;
       const uint64_t wpb = (uint64_t)B / 64;
@@ -1144,16 +1167,27 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
       const uint64_t lanes = grid * (uint64_t)B;
       int r2 = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64);
       if (r2) return r2;
-      k_nearest_pool<D, K, ANY_, FILT_, CAST_, QUAD_><<<(uint32_t)grid, B, lds, c->stream>>>(
+      k_nearest_pool<D, K, ANY_, FILT_, CAST_, QUAD_, EXACT_><<<(uint32_t)grid, B, lds, c->stream>>>(
           s, q, nn, max_toi, tmax, fd, out, (uint32_t)chunk, counter, S, c->spill.as<SpillEntry>(), (uint32_t)lanes, order,
           (uint32_t)c->descend_min);
       HIP_TRY(hipGetLastError());
       return SQ_OK;
     };
-    if (quad) return any ? launch.template operator()<true, false, false, true>() : launch.template operator()<false, false, false, true>();
-    if (cast) return filt ? launch.template operator()<false, true, true>() : launch.template operator()<false, false, true>();
-    if (any) return filt ? launch.template operator()<true, true, false>() : launch.template operator()<true, false, false>();
-    return filt ? launch.template operator()<false, true, false>() : launch.template operator()<false, false, false>();
+    auto pick = [&]<bool EXACT_>() -> int {
+      if constexpr (K == KIND_AABB || !EXACT_) {   // the 4-wide walk needs hits inside their boxes
+        if (quad) return any ? launch.template operator()<true, false, false, true, EXACT_>()
+                             : launch.template operator()<false, false, false, true, EXACT_>();
+      }
+      if (cast) return filt ? launch.template operator()<false, true, true, false, EXACT_>()
+                            : launch.template operator()<false, false, true, false, EXACT_>();
+      if (any) return filt ? launch.template operator()<true, true, false, false, EXACT_>()
+                           : launch.template operator()<true, false, false, false, EXACT_>();
+      return filt ? launch.template operator()<false, true, false, false, EXACT_>()
+                  : launch.template operator()<false, false, false, false, EXACT_>();
+    };
+    // AABB hits lie inside their boxes: the one walk is exact in both modes
+    if constexpr (K == KIND_AABB) return pick.template operator()<true>();
+    else return c->hit_mode == SQ_HITS_GUARDED ? pick.template operator()<false>() : pick.template operator()<true>();
   });
   if (rc) return rc;
   return end_timing(c);

@@ -70,6 +70,7 @@ struct sq_ctx {
   int descend_min = 24;            // pooled walks: pause descents below this many lanes (pool.h)
   int split_depth = 0;             // pooled collect: split queries this many levels down (0 auto, -1 never)
   int wide = 0;                    // pooled nearest / any: walk the 4-wide records (traverse.h quad_records)
+  int hit_mode = SQ_HITS_REFERENCE;   // SQ_HITS_REFERENCE or SQ_HITS_GUARDED (sq.h)
   bool timed = false;
   // {next chunk, retired waves} of the pooled single-call launches: zero at
   // creation, and each launch leaves them zero again (pool_retire)

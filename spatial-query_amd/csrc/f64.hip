@@ -7,9 +7,12 @@
 // (maths/aabb.rs:118-148), the leaf tests and the returned toi.  The walk
 // below runs the same slab and narrow-phase functions as the FP32 engine
 // (traverse.h), instantiated on double, over the reference median-split tree
-// built in double (host_tree.cpp).  Results are order-independent for the
-// reason the FP32 walk is (a sphere hit is kept only inside its own box's slab
-// interval), so a near-first walk answers what the reference's LIFO walk does.
+// built in double (host_tree.cpp).  As in the FP32 engine, nearest queries
+// over leaves whose hits may fall outside their own box (balls, box instances)
+// walk the reference's order (push a, then b, pop LIFO) with the reference's
+// unguarded leaf tests, and so answer what the reference's walk does; under
+// SQ_HITS_GUARDED those hits are kept only inside their box's slab interval and
+// the walk is near-first.
 //
 // Device layout: the reference node array re-packed as pair nodes, both
 // children's boxes and refs in one 16-byte-aligned record of 4D + 2 doubles,
@@ -128,8 +131,10 @@ __device__ __forceinline__ void slab_pair64(const RayD<D, double>& r, const doub
 // What a query is for the pooled walk: a ray (RayPolicy's tests in double) or
 // a shape cast (CastPolicy<.., double>: the slab against each box grown by the
 // probe box, shapecast.rs:70-73, and the kinds' cast leaf tests).
-template <int D, int KIND, bool CAST> struct Pol64;
-template <int D, int KIND> struct Pol64<D, KIND, false> {
+// GUARD: balls and box instances keep a hit only inside the slab interval of
+// their world box (SQ_HITS_GUARDED).
+template <int D, int KIND, bool CAST, bool GUARD> struct Pol64;
+template <int D, int KIND, bool GUARD> struct Pol64<D, KIND, false, GUARD> {
   static constexpr int QS = 2 * D;   // doubles per query
   RayD<D, double> r;
   __device__ void setup(const double* src) { ray_setup<D>(r, src); }
@@ -144,22 +149,23 @@ template <int D, int KIND> struct Pol64<D, KIND, false> {
   __device__ bool narrow(const double* p, double limit, double& t) const {
     if constexpr (KIND == KIND_INST64) {
       // test_instance (tlas.rs:459-481): the ray in the instance's frame against
-      // its local box, kept inside the slab interval of its world box
+      // its local box (with GUARD kept inside the slab interval of its world box)
       double ol[D], dl[D];
       iso_inverse<D>(p, p + D * D, r, ol, dl);
       int ax;
       if (!local_box<D>(ol, dl, p + D * D + D, limit, t, ax)) return false;
+      if constexpr (!GUARD) return true;
       double te, tx;
       if (!slab_exact<D>(r, p + D * D + 2 * D, p + D * D + 3 * D, limit, te, tx, ax)) return false;
       return te <= t && t <= tx;
     } else {
-      return sqd::narrow<D, KIND>(r, p, limit, t);
+      return sqd::narrow<D, KIND, double, GUARD>(r, p, limit, t);
     }
   }
 };
-template <int D, int KIND> struct Pol64<D, KIND, true> {
+template <int D, int KIND, bool GUARD> struct Pol64<D, KIND, true, GUARD> {
   static constexpr int QS = 4 * D;   // origin, direction, probe box min, max
-  CastPolicy<D, KIND, double> c;
+  CastPolicy<D, KIND, double, GUARD> c;
   __device__ void setup(const double* src) { c.setup(src); }
   __device__ const RayD<D, double>& ray() const { return c.r; }
   __device__ bool root(const double* rb, double limit) const { return c.root(rb, limit); }
@@ -205,8 +211,11 @@ __device__ void sort_range64(int32_t* out_leaf, double* out_toi, uint64_t b0, ui
 // loop iteration refills the wave's idle lanes from the counter (one atomic per
 // wave) and advances every walking lane by one pair node.  The wave leaves the
 // loop once the counter has passed n and none of its lanes walks; every wave
-// then retires once (pool_retire), the last one clearing the counter.
-template <int D, int KIND, bool ANY, bool CAST, bool ALL = false>
+// then retires once (pool_retire), the last one clearing the counter.  EXACT:
+// the reference's hits (unguarded leaf tests; nearest queries over balls and
+// box instances in the reference's order, see the REFO step below); else the
+// guarded leaf tests near-first.
+template <int D, int KIND, bool ANY, bool CAST, bool ALL, bool EXACT>
 __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const double* __restrict__ rays, uint32_t n,
                                                         double max_toi, const double* __restrict__ tmax, Filter f,
                                                         uint32_t* counter, int32_t* out_leaf,
@@ -219,11 +228,16 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
   extern __shared__ uint32_t f64_ring[];
   const RingStack st{f64_ring + threadIdx.x, reinterpret_cast<float*>(f64_ring + F64_RING * blockDim.x) + threadIdx.x,
                      blockDim.x, F64_RING, spill + (size_t)blockIdx.x * blockDim.x + threadIdx.x, lanes};
-  Pol64<D, KIND, CAST> q;
+  using Q = Pol64<D, KIND, CAST, !EXACT>;
+  // the reference's order: box leaves (and box casts) hit inside their boxes,
+  // so only the other kinds' nearest queries need it
+  constexpr bool REFO = EXACT && !ANY && !ALL && KIND != KIND_AABB;
+  Q q;
   double best = 0.0, lim = 0.0;   // lim: this query's max_toi
   uint32_t bl = NO_LEAF, bk = 0, node = 0, sp = 0, ray = 0, cnt = 0;
   uint64_t at = 0;   // ALL: the next CSR slot of this query
   bool active = false, drained = false;
+  bool resume = false;   // REFO: `node` is a popped parent whose child b is done; its child a comes next
   // leaf range test: the nearest accepted hit so far (ALL: every accepted hit);
   // true ends an any-hit walk
   auto test_leaf = [&](uint32_t first, uint32_t count) {
@@ -282,9 +296,9 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
           const uint32_t my = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
           if (my < n) {
             ray = my;
-            q.setup(rays + (size_t)my * Pol64<D, KIND, CAST>::QS);
+            q.setup(rays + (size_t)my * Q::QS);
             lim = tmax ? tmax[my] : max_toi;
-            best = lim; bl = NO_LEAF; sp = 0;
+            best = lim; bl = NO_LEAF; sp = 0; resume = false;
             if constexpr (ALL) { cnt = 0; at = offsets ? offsets[my] : 0; }
             const bool in_root = v.n_prims && q.root(v.root, lim);
             if (in_root && !v.root_count) {
@@ -305,6 +319,40 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
     double ta, tb;
     bool ha, hb;
     q.pair(b, best, ha, hb, ta, tb);
+    if constexpr (REFO) {
+      // bvh.rs:73-124: child b's subtree, then child a, each tested at the
+      // best when it comes off the reference's stack.  A stack entry is the
+      // parent of a pending child a (its stored entry, rounded toward -inf,
+      // only screens: the pair is fetched again and a's slab redone at the
+      // current best, the reference's test of a when it pops).
+      if (resume) { hb = false; resume = false; }
+      if (hb && rf.w) {
+        test_leaf(rf.y, rf.w);
+        hb = false;
+        if (ha) {
+          double ta2, tb2;
+          bool hb2;
+          q.pair(b, best, ha, hb2, ta2, tb2);
+          ta = ta2;
+        }
+      }
+      if (hb) {
+        if (ha) st.push(sp, node, __double2float_rd(ta));
+        node = rf.y;
+        continue;
+      }
+      if (ha && !rf.z) { node = rf.x; continue; }
+      if (ha) test_leaf(rf.x, rf.z);
+      bool found = false;
+      while (sp > 0) {
+        uint32_t e;
+        float lb;
+        st.pop(sp, e, lb);
+        if (!((double)lb > best)) { node = e; resume = true; found = true; break; }
+      }
+      if (!found) finish();
+      continue;
+    }
     bool stop = false;
     if (ha && rf.z) { stop = test_leaf(rf.x, rf.z); ha = false; }
     if (!stop && hb && rf.w) { stop = test_leaf(rf.y, rf.w); hb = false; }
@@ -401,12 +449,12 @@ Filter filter64(const sq_bvh64* b, const sq_filter* f) {
 
 // One pooled launch over n queries (the grid fills the chip; lanes refill from
 // the context's pool counter, which every launch leaves zero).
-template <int D, int KIND, bool ANY, bool CAST, bool ALL>
+template <int D, int KIND, bool ANY, bool CAST, bool ALL, bool EXACT = true>
 static int pool_launch(sq_bvh64* b, const double* q, uint64_t n, double max_toi, const double* tmax, const Filter& fd,
                        int32_t* out_leaf, double* out_toi, double* out_point, double* out_normal, uint8_t* out_any,
                        uint32_t* counts, const uint64_t* offsets) {
   sq_ctx* c = b->ctx;
-  const void* kern = reinterpret_cast<const void*>(&k64_nearest_pool<D, KIND, ANY, CAST, ALL>);
+  const void* kern = reinterpret_cast<const void*>(&k64_nearest_pool<D, KIND, ANY, CAST, ALL, EXACT>);
   const size_t lds = (size_t)F64_RING * F64_BLOCK * 8;
   int per_cu = 0;
   if (int r0 = blocks_per_cu(kern, c->device, F64_BLOCK, lds, &per_cu)) return r0;
@@ -415,7 +463,7 @@ static int pool_launch(sq_bvh64* b, const double* q, uint64_t n, double max_toi,
   // spill rows: the stack holds at most depth - 2 entries, F64_RING of them in LDS
   const uint64_t rows = b->view.stack > F64_RING ? b->view.stack - F64_RING : 0;
   if (int r1 = c->spill.ensure((size_t)rows * lanes * sizeof(SpillEntry) + 64)) return r1;
-  k64_nearest_pool<D, KIND, ANY, CAST, ALL><<<(uint32_t)grid, F64_BLOCK, lds, c->stream>>>(
+  k64_nearest_pool<D, KIND, ANY, CAST, ALL, EXACT><<<(uint32_t)grid, F64_BLOCK, lds, c->stream>>>(
       b->view, q, (uint32_t)n, max_toi, tmax, fd, c->pool_ctr.as<uint32_t>(), out_leaf, out_toi, out_point, out_normal,
       out_any, c->spill.as<SpillEntry>(), (uint32_t)lanes, counts, offsets);
   HIP_TRY(hipGetLastError());
@@ -615,14 +663,18 @@ static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi
   if (!n) return SQ_OK;
   if (n >= (1ull << 31)) return fail(SQ_ERR_INVALID, "more than 2^31 queries in one call");
   const Filter fd = filter64(b, f);
+  const bool guarded = b->ctx->hit_mode == SQ_HITS_GUARDED;
   if (b->kind == KIND_INST64) {
     if (cast) return fail(SQ_ERR_UNSUPPORTED, "no shape casts on f64 instance trees");
     auto go = [&]<int D>() -> int {
-      if (out_any)
-        return pool_launch<D, KIND_INST64, true, false, false>(b, rays, n, max_toi, tmax, fd, nullptr, nullptr, nullptr,
-                                                               nullptr, out_any, nullptr, nullptr);
-      return pool_launch<D, KIND_INST64, false, false, false>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi, out_point,
-                                                              nullptr, nullptr, nullptr, nullptr);
+      auto run = [&]<bool EXACT>() -> int {
+        if (out_any)
+          return pool_launch<D, KIND_INST64, true, false, false, EXACT>(b, rays, n, max_toi, tmax, fd, nullptr, nullptr,
+                                                                        nullptr, nullptr, out_any, nullptr, nullptr);
+        return pool_launch<D, KIND_INST64, false, false, false, EXACT>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi,
+                                                                       out_point, nullptr, nullptr, nullptr, nullptr);
+      };
+      return guarded ? run.template operator()<false>() : run.template operator()<true>();
     };
     switch (b->dim) {
       case 2: return go.template operator()<2>();
@@ -634,14 +686,19 @@ static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi
     }
   }
   return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
-    if (cast)
-      return pool_launch<D, KIND, false, true, false>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi, out_point,
-                                                      nullptr, nullptr, nullptr, nullptr);
-    if (out_any)
-      return pool_launch<D, KIND, true, false, false>(b, rays, n, max_toi, tmax, fd, nullptr, nullptr, nullptr, nullptr,
-                                                      out_any, nullptr, nullptr);
-    return pool_launch<D, KIND, false, false, false>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi, out_point,
-                                                     out_normal, nullptr, nullptr, nullptr);
+    auto run = [&]<bool EXACT>() -> int {
+      if (cast)
+        return pool_launch<D, KIND, false, true, false, EXACT>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi, out_point,
+                                                               nullptr, nullptr, nullptr, nullptr);
+      if (out_any)
+        return pool_launch<D, KIND, true, false, false, EXACT>(b, rays, n, max_toi, tmax, fd, nullptr, nullptr, nullptr,
+                                                               nullptr, out_any, nullptr, nullptr);
+      return pool_launch<D, KIND, false, false, false, EXACT>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi, out_point,
+                                                              out_normal, nullptr, nullptr, nullptr);
+    };
+    // box leaves hit inside their boxes: the one walk is exact in both modes
+    if constexpr (KIND == KIND_AABB) return run.template operator()<true>();
+    else return guarded ? run.template operator()<false>() : run.template operator()<true>();
   });
 }
 

@@ -112,8 +112,9 @@ struct Chunks {
 // The walk for any query policy Pol (setup / root / pair / narrow over
 // primitive records of PS floats, queries of QS floats).  KIND picks what the
 // optional point / normal / sub outputs mean (the built-in kinds); CAST marks
-// shape casts (no normal).
-template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Pol, int QS, int PS, class Src, bool QUAD = false>
+// shape casts (no normal); REF walks the reference's order (walk_step).
+template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Pol, int QS, int PS, class Src, bool QUAD = false,
+          bool REF = false>
 __device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
                                                const float* __restrict__ tmax, const Filter& f, const NearestOut& out,
                                                const RingStack& st, const uint32_t* __restrict__ order, uint32_t pause_below) {
@@ -163,7 +164,7 @@ __device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* 
     if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
     const uint32_t active = (uint32_t)__popcll(__ballot(rid != NO_LEAF));
     if (rid == NO_LEAF) continue;
-    if (!walk_step<D, PS, ANY, FILT, false, false, QUAD>(s, pol, f, st, w, nullptr, nullptr, WavePause{pause_below, active}))
+    if (!walk_step<D, PS, ANY, FILT, false, false, QUAD, REF>(s, pol, f, st, w, nullptr, nullptr, WavePause{pause_below, active}))
       continue;
     // finished: write the result of query rid
     const NearestResult& h = w.res;
@@ -191,13 +192,16 @@ __device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* 
 }
 
 // The built-in kinds: rays (RayPolicy) or shape casts (CastPolicy); QUAD
-// walks the 4-wide records (s.quads).
-template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Src, bool QUAD = false>
+// walks the 4-wide records (s.quads).  EXACT: the reference's answer for every
+// query (unguarded narrow phases; nearest queries whose hits may leave their
+// box walk the reference's order); else the guarded kinds near-first.
+template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Src, bool QUAD = false, bool EXACT = true>
 __device__ __forceinline__ void pool_walk(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
                                           const float* __restrict__ tmax, const Filter& f, const NearestOut& out,
                                           const RingStack& st, const uint32_t* __restrict__ order, uint32_t pause_below) {
-  using Pol = typename std::conditional<CAST, CastPolicy<D, KIND>, RayPolicy<D, KIND>>::type;
-  pool_walk_with<D, KIND, ANY, FILT, CAST, Pol, CAST ? 4 * D : 2 * D, Prim<D, KIND>::STRIDE, Src, QUAD>(
+  using Pol = typename std::conditional<CAST, CastPolicy<D, KIND, float, !EXACT>, RayPolicy<D, KIND, !EXACT>>::type;
+  constexpr bool REF = EXACT && !ANY && !Pol::INSIDE;
+  pool_walk_with<D, KIND, ANY, FILT, CAST, Pol, CAST ? 4 * D : 2 * D, Prim<D, KIND>::STRIDE, Src, QUAD, REF>(
       s, q, src, max_toi, tmax, f, out, st, order, pause_below);
 }
 
@@ -207,24 +211,26 @@ __device__ __forceinline__ void pool_walk(const SceneView& s, const float* __res
 // tree to a leaf, or tests the next instances of that leaf (box instances
 // directly; a mesh instance starts a walk of its BLAS with the local ray), or
 // takes one walk_step in the current BLAS.  A finished BLAS walk's hit goes
-// through instance_test's guard and the (toi, instance) order; the limit the
-// BLAS walk prunes with is the top-level best when it started, as in
-// instance_test.  Both stacks are LDS rings spilling to device memory.
+// through the (toi, instance) order (with GUARD after instance_test's guard);
+// the limit the BLAS walk prunes with is the top-level best when it started,
+// as in instance_test.  EXACT walks the top level in the reference's order
+// (order_top) and each BLAS as the reference's Bvh::raycast_nearest; else both
+// guarded, near-first.  Both stacks are LDS rings spilling to device memory.
 // ---------------------------------------------------------------------------
 
-template <int D> struct TlasWalk {
+template <int D, bool EXACT = true> struct TlasWalk {
   uint32_t cur, sp;            // top level: next ref, stack depth
   float limit;
   TlasHit res;
   uint32_t first, cnt, j;      // the top leaf range under test, j its next instance
   uint32_t phase;              // 0 descend the top, 1 test the range, 2 walk a BLAS, 3 pop the top
   uint32_t inst, mesh;         // the mesh instance being walked
-  RayPolicy<D, KIND_TRI> pol;  // its local ray
+  RayPolicy<D, KIND_TRI, !EXACT> pol;  // its local ray
   WalkState wb;
 };
 
-template <int D>
-__device__ __forceinline__ bool tlas_begin(const TlasView& tv, const RayD<D>& r, float max_toi, TlasWalk<D>& w) {
+template <int D, bool EXACT>
+__device__ __forceinline__ bool tlas_begin(const TlasView& tv, const RayD<D>& r, float max_toi, TlasWalk<D, EXACT>& w) {
   w.res = TlasHit{SQ_INF, NO_LEAF, NO_LEAF, NO_LEAF};
   w.limit = max_toi;
   w.sp = 0;
@@ -235,8 +241,8 @@ __device__ __forceinline__ bool tlas_begin(const TlasView& tv, const RayD<D>& r,
   return slab_exact<D>(r, tv.top.root, tv.top.root + D, max_toi, te, tx, ax);
 }
 
-template <int D>
-__device__ __forceinline__ void tlas_take(TlasWalk<D>& w, float t, uint32_t inst, uint32_t pos, uint32_t sub) {
+template <int D, bool EXACT>
+__device__ __forceinline__ void tlas_take(TlasWalk<D, EXACT>& w, float t, uint32_t inst, uint32_t pos, uint32_t sub) {
   if (w.res.inst == NO_LEAF || hit_before(t, inst, w.res.t, w.res.inst)) {
     w.res = TlasHit{t, inst, pos, sub};
     w.limit = t;
@@ -244,24 +250,29 @@ __device__ __forceinline__ void tlas_take(TlasWalk<D>& w, float t, uint32_t inst
 }
 
 // One step of ray r; true when the ray is finished (result in w.res).  ANY
-// finishes at the first instance hit that passes the guard (walk_tlas's ANY).
-template <int D, int IK, bool ANY, class P>
+// finishes at the first accepted instance hit (walk_tlas's ANY).
+template <int D, int IK, bool ANY, bool EXACT, class P>
 __device__ __forceinline__ bool tlas_step(const TlasView& tv, const RayD<D>& r, const RingStack& st, const RingStack& sb,
-                                          TlasWalk<D>& w, const P& pause) {
+                                          TlasWalk<D, EXACT>& w, const P& pause) {
   constexpr int NF = (D + 1) * 4;
   const SceneView& s = tv.top;
   if constexpr (IK == INST_MESH) {
     if (w.phase == 2) {
     const Filter nof{nullptr, ~0ull, -1};
-    if (!walk_step<D, Prim<3, KIND_TRI>::STRIDE, false, false>(tv.blas[w.mesh], w.pol, nof, sb, w.wb, nullptr, nullptr, pause))
+    if (!walk_step<D, Prim<3, KIND_TRI>::STRIDE, false, false, false, false, false, EXACT>(tv.blas[w.mesh], w.pol, nof, sb,
+                                                                                          w.wb, nullptr, nullptr, pause))
       return false;
     if (w.wb.res.leaf != NO_LEAF) {
-      // instance_test's guard: toi inside the instance's world slab at the limit the BLAS walk used
       const float t = w.wb.res.t;
-      const float* wx = tv.world + (size_t)w.inst * 2 * D;
-      float te, tx;
-      int a;
-      if (slab_exact<D>(r, wx, wx + D, w.limit, te, tx, a) && te <= t && t <= tx) {
+      bool keep = true;
+      if constexpr (!EXACT) {
+        // instance_test's guard: toi inside the instance's world slab at the limit the BLAS walk used
+        const float* wx = tv.world + (size_t)w.inst * 2 * D;
+        float te, tx;
+        int a;
+        keep = slab_exact<D>(r, wx, wx + D, w.limit, te, tx, a) && te <= t && t <= tx;
+      }
+      if (keep) {
         tlas_take<D>(w, t, w.inst, w.wb.res.pos, w.wb.res.leaf);
         if (ANY) return true;
       }
@@ -275,7 +286,7 @@ __device__ __forceinline__ bool tlas_step(const TlasView& tv, const RayD<D>& r, 
       const uint32_t id = s.ids[w.first + w.j];
       if constexpr (IK != INST_MESH) {
         TlasHit h;
-        if (instance_test<D, IK>(tv, r, id, w.limit, Stack{nullptr, nullptr, 0}, h)) {
+        if (instance_test<D, IK, !EXACT>(tv, r, id, w.limit, Stack{nullptr, nullptr, 0}, h)) {
           tlas_take<D>(w, h.t, h.inst, h.pos, h.sub);
           if (ANY) return true;
         }
@@ -312,12 +323,11 @@ __device__ __forceinline__ bool tlas_step(const TlasView& tv, const RayD<D>& r, 
   while (!(w.cur & LEAF_BIT)) {
     float nd[NF];
     fetch16<D + 1>(s.nodes + (size_t)w.cur * NF, nd);
-    bool ha, hb;
+    bool ha, hb, a_first;
     float ta, tb;
-    slab_pair<D>(r, nd, w.limit, ha, hb, ta, tb);
+    order_top<D, EXACT, ANY>(r, nd, w.limit, ha, hb, ta, tb, a_first);
     const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
     if (ha && hb) {
-      const bool a_first = ta <= tb;
       st.push(w.sp, a_first ? rb : ra, a_first ? tb : ta);
       w.cur = a_first ? ra : rb;
     } else if (ha) {
@@ -345,15 +355,16 @@ __device__ __forceinline__ bool tlas_step(const TlasView& tv, const RayD<D>& r, 
 
 // One wave walks the ray ranges src hands it through the two-level scene
 // (lanes refilled as rays finish, as pool_walk).  Outputs as k_tlas_nearest:
-// any, or leaf (instance) and toi with optional point, normal, sub.
-template <int D, int IK, bool ANY, class Src>
+// any, or leaf (instance) and toi with optional point, normal, sub.  EXACT as
+// tlas_step.
+template <int D, int IK, bool ANY, class Src, bool EXACT = true>
 __device__ __forceinline__ void pool_walk_tlas(const TlasView& tv, const float* __restrict__ q, Src src, float max_toi,
                                                const float* __restrict__ tmax, const NearestOut& out, const RingStack& st,
                                                const RingStack& sb, uint32_t pause_below) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   RayD<D> r;
-  TlasWalk<D> w;
+  TlasWalk<D, EXACT> w;
   float lim0 = 0.0f;
   uint32_t rid = NO_LEAF;
   uint64_t next = 0, end = 0;
@@ -410,7 +421,7 @@ __device__ __forceinline__ void pool_walk_tlas(const TlasView& tv, const float* 
     // once the pool is exhausted the two-level walk runs without pausing
     // (measured faster there, unlike the single-level walk, where pausing
     // also lets lanes at a leaf test it while others descend)
-    if (!tlas_step<D, IK, ANY>(tv, r, st, sb, w, WavePause{exhausted ? 0u : pause_below, active})) continue;
+    if (!tlas_step<D, IK, ANY, EXACT>(tv, r, st, sb, w, WavePause{exhausted ? 0u : pause_below, active})) continue;
     write(rid);
     rid = NO_LEAF;
   }
@@ -507,6 +518,9 @@ struct CollectQuery {
     }
   }
   __device__ __forceinline__ void pair(const float* nd, bool& ha, bool& hb) const {
+    // the parts of one reference leaf: every primitive of the leaf is tested
+    // (a ball or triangle may answer outside its own box)
+    if (KIND != KIND_AABB && synthetic_pair<D>(nd)) { ha = hb = true; return; }
     if constexpr (MODE == 2) {
       float amn[D], amx[D], bmn[D], bmx[D];
 #pragma unroll

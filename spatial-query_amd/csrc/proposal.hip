@@ -429,7 +429,9 @@ __global__ void k_labels(const int32_t* __restrict__ leaf, uint64_t n, const int
 
 // Proposal-seeded exact nearest walk (triangles, D = 3): the proposed subtrees
 // are walked first, each from its own ref, with the shrinking limit; then the
-// full walk runs from the root with that limit and best hit.  top == NULL
+// full walk runs from the root with that limit and best hit.  The triangles
+// are the guarded kind (SQ_HITS_GUARDED): every accepted hit lies inside its
+// own box, so the answer does not depend on which subtrees go first.  top == NULL
 // walks from the root at once (the plain walk).  Same pooled structure and
 // pausing of descents as the plain nearest kernel.  COUNT accumulates into stats[0..2]: rays whose
 // proposals held a hit, pair nodes fetched, primitives tested.
@@ -451,7 +453,7 @@ __global__ __launch_bounds__(256, COUNT ? 4 : 8) void k_nearest_proposed(SceneVi
   RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
   const Filter nof{nullptr, ~0ull, -1};
   const uint32_t root_ref = fbits(s.root[2 * D]);
-  RayPolicy<D, KIND_TRI> pol;
+  RayPolicy<D, KIND_TRI, true> pol;
   WalkState w;
   WorkCount wc;
   uint32_t rid = NO_LEAF;

@@ -29,11 +29,13 @@
 // trait's defaults).  The same module holds the all-hits, crossings, cast and
 // overlap walks (pool_walk_with, pool_collect_with).  The tree is built from the consumer's world boxes
 // (QueryGeometry::world_aabb); each leaf-ordered primitive record is its box
-// followed by its prim_floats floats, of which sq_narrow sees the latter.  As
-// for the built-in SPHERE and TRIANGLE kinds, a hit is kept only when its toi
-// lies in the slab interval of the primitive's own box, which makes the
-// result the minimum under (toi, leaf) over the kept hits, whatever the order
-// of the walk (the argument of traverse.h).
+// followed by its prim_floats floats, of which sq_narrow sees the latter.  A
+// hit counts as the reference counts it (in [0, max_toi]); nearest queries walk
+// the reference's order (traverse.h ORDER_REF), so the answer is the
+// reference's for any consumer, hits outside their own box included.  Under
+// SQ_HITS_GUARDED, as for the built-in SPHERE and TRIANGLE kinds, a hit is
+// kept only when its toi lies in the slab interval of the primitive's own box
+// and the walk is near-first.
 #include <hip/hiprtc.h>
 
 #include <cstdio>
@@ -56,7 +58,9 @@ constexpr int KIND_CONSUMER = 3;
 
 const char* kKernelSrc = R"SQRTC(
 namespace sqd {
-template <int D> struct ConsumerPolicy {
+// GUARD keeps a hit only inside the slab interval of the primitive's box.
+template <int D, bool GUARD> struct ConsumerPolicy {
+  static constexpr bool INSIDE = GUARD;
   RayD<D> r;
   SQ_HD void setup(const float* src) { ray_setup<D>(r, src); }
   SQ_HD bool root(const float* root, float limit) const {
@@ -70,9 +74,11 @@ template <int D> struct ConsumerPolicy {
     float tt;
     if (!sq_narrow(p + 2 * D, r.o, r.d, limit, &tt)) return false;
     if (!(tt >= 0.0f && tt <= limit)) return false;
-    float te, tx; int ax;
-    if (!slab_exact<D>(r, p, p + D, limit, te, tx, ax)) return false;
-    if (!(te <= tt && tt <= tx)) return false;
+    if constexpr (GUARD) {
+      float te, tx; int ax;
+      if (!slab_exact<D>(r, p, p + D, limit, te, tx, ax)) return false;
+      if (!(te <= tt && tt <= tx)) return false;
+    }
     t = tt;
     return true;
   }
@@ -82,12 +88,12 @@ template <int D> struct ConsumerPolicy {
 // Persistent pooled walk (as the built-in kinds' k_nearest_pool): waves take
 // chunks of `chunk` rays from counter[0] (zero at the start, left zero by
 // pool_retire), refill lanes as rays finish, pause descents below pause_below
-// lanes.
-extern "C" __global__ __launch_bounds__(256) void sq_consumer_nearest(sqd::SceneView s, const float* rays, unsigned n,
-                                                                      float max_toi, int* out_leaf, float* out_toi,
-                                                                      unsigned chunk, unsigned* counter, unsigned S,
-                                                                      sqd::SpillEntry* spill, unsigned spill_stride,
-                                                                      unsigned pause_below) {
+// lanes.  GUARD: the guarded, near-first walk; else the reference's order.
+namespace sqd {
+template <bool GUARD>
+__device__ __forceinline__ void consumer_nearest(SceneView s, const float* rays, unsigned n, float max_toi, int* out_leaf,
+                                                 float* out_toi, unsigned chunk, unsigned* counter, unsigned S,
+                                                 SpillEntry* spill, unsigned spill_stride, unsigned pause_below) {
   extern __shared__ unsigned lds[];
   const unsigned B = blockDim.x, tid = threadIdx.x;
   const unsigned gtid = blockIdx.x * B + tid;
@@ -96,20 +102,36 @@ extern "C" __global__ __launch_bounds__(256) void sq_consumer_nearest(sqd::Scene
   const unsigned long long home = (unsigned long long)blockIdx.x * (B >> 6) + (tid >> 6);
   const sqd::NearestOut out{out_leaf, out_toi, nullptr, nullptr, nullptr, nullptr};
   const sqd::Filter f{nullptr, ~0ull, -1};
-  sqd::pool_walk_with<SQ_RTC_D, sqd::KIND_AABB, false, false, false, sqd::ConsumerPolicy<SQ_RTC_D>, 2 * SQ_RTC_D,
-                      2 * SQ_RTC_D + SQ_RTC_PF>(s, rays, sqd::Chunks{counter, n, chunk, home, waves}, max_toi, nullptr, f,
-                                                out, st, nullptr, pause_below);
+  sqd::pool_walk_with<SQ_RTC_D, sqd::KIND_AABB, false, false, false, sqd::ConsumerPolicy<SQ_RTC_D, GUARD>, 2 * SQ_RTC_D,
+                      2 * SQ_RTC_D + SQ_RTC_PF, sqd::Chunks, false, !GUARD>(
+      s, rays, sqd::Chunks{counter, n, chunk, home, waves}, max_toi, nullptr, f, out, st, nullptr, pause_below);
   sqd::pool_retire(counter);
+}
+}  // namespace sqd
+
+extern "C" __global__ __launch_bounds__(256) void sq_consumer_nearest(sqd::SceneView s, const float* rays, unsigned n,
+                                                                      float max_toi, int* out_leaf, float* out_toi,
+                                                                      unsigned chunk, unsigned* counter, unsigned S,
+                                                                      sqd::SpillEntry* spill, unsigned spill_stride,
+                                                                      unsigned pause_below) {
+  sqd::consumer_nearest<false>(s, rays, n, max_toi, out_leaf, out_toi, chunk, counter, S, spill, spill_stride, pause_below);
+}
+extern "C" __global__ __launch_bounds__(256) void sq_consumer_nearest_guarded(sqd::SceneView s, const float* rays, unsigned n,
+                                                                              float max_toi, int* out_leaf, float* out_toi,
+                                                                              unsigned chunk, unsigned* counter, unsigned S,
+                                                                              sqd::SpillEntry* spill, unsigned spill_stride,
+                                                                              unsigned pause_below) {
+  sqd::consumer_nearest<true>(s, rays, n, max_toi, out_leaf, out_toi, chunk, counter, S, spill, spill_stride, pause_below);
 }
 
 namespace sqd {
 // Per-query state of an all-hits (CROSS false) or crossings (CROSS true) walk
 // over consumer primitives: every primitive of every leaf range the ray
-// reaches within max_toi is tested (no best-toi pruning), and each hit or
-// crossing kept inside the slab interval of the primitive's own box is
-// appended with its (toi, leaf) key.
+// reaches within max_toi is tested (no best-toi pruning; the parts of a
+// reference leaf split into a synthetic chain are all tested), and each hit or
+// crossing in [0, max_toi] is appended with its (toi, leaf) key.
 template <int D, bool CROSS> struct ConsumerCollect {
-  ConsumerPolicy<D> pol;
+  ConsumerPolicy<D, false> pol;
   float lim;
   SQ_HD bool begin(const SceneView& s, const float* q, unsigned i, float max_toi, const float*) {
     pol.setup(q + (size_t)i * 2 * D);
@@ -117,6 +139,7 @@ template <int D, bool CROSS> struct ConsumerCollect {
     return pol.root(s.root, lim);
   }
   SQ_HD void pair(const float* nd, bool& ha, bool& hb) const {
+    if (synthetic_pair<D>(nd)) { ha = hb = true; return; }
     float ta, tb;
     pol.pair(nd, lim, ha, hb, ta, tb);
   }
@@ -128,10 +151,8 @@ template <int D, bool CROSS> struct ConsumerCollect {
       const unsigned leaf = s.ids[pos];
 #if defined(SQ_NARROW_CROSSINGS)
       if constexpr (CROSS) {
-        float te, tx; int ax;
-        if (!slab_exact<D>(pol.r, p, p + D, lim, te, tx, ax)) continue;
         auto out = [&](float t) {
-          if (te <= t && t <= tx) emit(hit_key(t, leaf));
+          if (t >= 0.0f && t <= lim) emit(hit_key(t, leaf));
         };
         sq_narrow_crossings(p + 2 * D, pol.r.o, pol.r.d, lim, out);
         continue;
@@ -146,18 +167,21 @@ template <int D, bool CROSS> struct ConsumerCollect {
 // Shape casts through the consumer (QueryGeometry::test_shape_cast,
 // geometry.rs:65-77): queries [4D] = origin, dir, probe box min, max; the
 // probe-grown slab prunes (ShapeCast::broadphase, shapecast.rs:70-73) and a
-// contact counts when its toi is in [0, limit] and inside the grown slab
-// interval of the primitive's own box.  Without SQ_NARROW_CAST there is no
-// contact (the trait's default).
-template <int D> struct ConsumerCastPolicy : CastPolicy<D, KIND_AABB> {
+// contact counts when its toi is in [0, limit] (with GUARD, also inside the
+// grown slab interval of the primitive's own box).  Without SQ_NARROW_CAST
+// there is no contact (the trait's default).
+template <int D, bool GUARD> struct ConsumerCastPolicy : CastPolicy<D, KIND_AABB> {
+  static constexpr bool INSIDE = GUARD;
   __device__ bool narrow(const float* p, float limit, float& t) const {
 #if defined(SQ_NARROW_CAST)
     float tt;
     if (!sq_narrow_cast(p + 2 * D, this->r.o, this->r.d, this->pmin, this->pmax, limit, &tt)) return false;
     if (!(tt >= 0.0f && tt <= limit)) return false;
-    float te, tx;
-    if (!this->grown_slab(p, p + D, limit, te, tx)) return false;
-    if (!(te <= tt && tt <= tx)) return false;
+    if constexpr (GUARD) {
+      float te, tx;
+      if (!this->grown_slab(p, p + D, limit, te, tx)) return false;
+      if (!(te <= tt && tt <= tx)) return false;
+    }
     t = tt;
     return true;
 #else
@@ -169,7 +193,7 @@ template <int D> struct ConsumerCastPolicy : CastPolicy<D, KIND_AABB> {
 
 // All shape-cast contacts (shapecast_all, bvh.rs:381-414) per cast.
 template <int D> struct ConsumerCastAll {
-  ConsumerCastPolicy<D> pol;
+  ConsumerCastPolicy<D, false> pol;
   float lim;
   SQ_HD bool begin(const SceneView& s, const float* q, unsigned i, float max_toi, const float*) {
     pol.setup(q + (size_t)i * 4 * D);
@@ -177,6 +201,7 @@ template <int D> struct ConsumerCastAll {
     return pol.root(s.root, lim);
   }
   SQ_HD void pair(const float* nd, bool& ha, bool& hb) const {
+    if (synthetic_pair<D>(nd)) { ha = hb = true; return; }
     float ta, tb;
     pol.pair(nd, lim, ha, hb, ta, tb);
   }
@@ -283,12 +308,14 @@ extern "C" __global__ __launch_bounds__(256) void sq_consumer_overlap(sqd::Scene
 }
 
 // Nearest shape-cast contact per cast [4D] (shapecast_nearest,
-// bvh.rs:331-378), the pooled walk with the consumer's cast policy.
-extern "C" __global__ __launch_bounds__(256) void sq_consumer_cast_nearest(sqd::SceneView s, const float* casts, unsigned n,
-                                                                           float max_toi, int* out_leaf, float* out_toi,
-                                                                           unsigned chunk, unsigned* counter, unsigned S,
-                                                                           sqd::SpillEntry* spill, unsigned spill_stride,
-                                                                           unsigned pause_below) {
+// bvh.rs:331-378), the pooled walk with the consumer's cast policy (GUARD as
+// consumer_nearest).
+namespace sqd {
+template <bool GUARD>
+__device__ __forceinline__ void consumer_cast_nearest(SceneView s, const float* casts, unsigned n, float max_toi,
+                                                      int* out_leaf, float* out_toi, unsigned chunk, unsigned* counter,
+                                                      unsigned S, SpillEntry* spill, unsigned spill_stride,
+                                                      unsigned pause_below) {
   extern __shared__ unsigned lds[];
   const unsigned B = blockDim.x, tid = threadIdx.x;
   const unsigned gtid = blockIdx.x * B + tid;
@@ -297,10 +324,28 @@ extern "C" __global__ __launch_bounds__(256) void sq_consumer_cast_nearest(sqd::
   const unsigned long long home = (unsigned long long)blockIdx.x * (B >> 6) + (tid >> 6);
   const sqd::NearestOut out{out_leaf, out_toi, nullptr, nullptr, nullptr, nullptr};
   const sqd::Filter f{nullptr, ~0ull, -1};
-  sqd::pool_walk_with<SQ_RTC_D, sqd::KIND_AABB, false, false, true, sqd::ConsumerCastPolicy<SQ_RTC_D>, 4 * SQ_RTC_D,
-                      2 * SQ_RTC_D + SQ_RTC_PF>(s, casts, sqd::Chunks{counter, n, chunk, home, waves}, max_toi, nullptr, f,
-                                                out, st, nullptr, pause_below);
+  sqd::pool_walk_with<SQ_RTC_D, sqd::KIND_AABB, false, false, true, sqd::ConsumerCastPolicy<SQ_RTC_D, GUARD>, 4 * SQ_RTC_D,
+                      2 * SQ_RTC_D + SQ_RTC_PF, sqd::Chunks, false, !GUARD>(
+      s, casts, sqd::Chunks{counter, n, chunk, home, waves}, max_toi, nullptr, f, out, st, nullptr, pause_below);
   sqd::pool_retire(counter);
+}
+}  // namespace sqd
+
+extern "C" __global__ __launch_bounds__(256) void sq_consumer_cast_nearest(sqd::SceneView s, const float* casts, unsigned n,
+                                                                           float max_toi, int* out_leaf, float* out_toi,
+                                                                           unsigned chunk, unsigned* counter, unsigned S,
+                                                                           sqd::SpillEntry* spill, unsigned spill_stride,
+                                                                           unsigned pause_below) {
+  sqd::consumer_cast_nearest<false>(s, casts, n, max_toi, out_leaf, out_toi, chunk, counter, S, spill, spill_stride, pause_below);
+}
+extern "C" __global__ __launch_bounds__(256) void sq_consumer_cast_nearest_guarded(sqd::SceneView s, const float* casts,
+                                                                                   unsigned n, float max_toi, int* out_leaf,
+                                                                                   float* out_toi, unsigned chunk,
+                                                                                   unsigned* counter, unsigned S,
+                                                                                   sqd::SpillEntry* spill,
+                                                                                   unsigned spill_stride,
+                                                                                   unsigned pause_below) {
+  sqd::consumer_cast_nearest<true>(s, casts, n, max_toi, out_leaf, out_toi, chunk, counter, S, spill, spill_stride, pause_below);
 }
 )SQRTC";
 
@@ -312,6 +357,8 @@ struct sq_narrow {
   uint32_t pf = 0;
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;          // nearest hit
+  hipFunction_t fn_g = nullptr;        // nearest hit, SQ_HITS_GUARDED
+  hipFunction_t fn_cast_g = nullptr;   // nearest shape-cast contact, SQ_HITS_GUARDED
   hipFunction_t fn_all = nullptr;      // all hits
   hipFunction_t fn_cross = nullptr;    // all crossings
   hipFunction_t fn_cast = nullptr;     // nearest shape-cast contact
@@ -379,6 +426,8 @@ int sq_narrow_create(sq_ctx* ctx, int dim, const char* source, uint32_t prim_flo
       hipModuleGetFunction(&nw->fn_all, nw->mod, "sq_consumer_all") != hipSuccess ||
       hipModuleGetFunction(&nw->fn_cross, nw->mod, "sq_consumer_crossings") != hipSuccess ||
       hipModuleGetFunction(&nw->fn_cast, nw->mod, "sq_consumer_cast_nearest") != hipSuccess ||
+      hipModuleGetFunction(&nw->fn_g, nw->mod, "sq_consumer_nearest_guarded") != hipSuccess ||
+      hipModuleGetFunction(&nw->fn_cast_g, nw->mod, "sq_consumer_cast_nearest_guarded") != hipSuccess ||
       hipModuleGetFunction(&nw->fn_cast_all, nw->mod, "sq_consumer_cast_all") != hipSuccess ||
       hipModuleGetFunction(&nw->fn_overlap, nw->mod, "sq_consumer_overlap") != hipSuccess) {
     (void)hipModuleUnload(nw->mod);
@@ -549,7 +598,8 @@ extern "C" {
 
 int sq_raycast_nearest_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uint64_t n, float max_toi, int32_t* out_leaf,
                                 float* out_toi) {
-  return consumer_nearest(nw, b, nw ? nw->fn : nullptr, rays, n, max_toi, out_leaf, out_toi);
+  return consumer_nearest(nw, b, nw ? (nw->ctx->hit_mode == SQ_HITS_GUARDED ? nw->fn_g : nw->fn) : nullptr, rays, n, max_toi,
+                          out_leaf, out_toi);
 }
 
 int sq_raycast_all_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uint64_t n, float max_toi, int crossings,
@@ -561,7 +611,8 @@ int sq_raycast_all_consumer(sq_narrow* nw, sq_bvh* b, const float* rays, uint64_
 
 int sq_shapecast_nearest_consumer(sq_narrow* nw, sq_bvh* b, const float* casts, uint64_t n, float max_toi, int32_t* out_leaf,
                                   float* out_toi) {
-  return consumer_nearest(nw, b, nw ? nw->fn_cast : nullptr, casts, n, max_toi, out_leaf, out_toi);
+  return consumer_nearest(nw, b, nw ? (nw->ctx->hit_mode == SQ_HITS_GUARDED ? nw->fn_cast_g : nw->fn_cast) : nullptr, casts, n,
+                          max_toi, out_leaf, out_toi);
 }
 
 int sq_shapecast_all_consumer(sq_narrow* nw, sq_bvh* b, const float* casts, uint64_t n, float max_toi, uint64_t* offsets,

@@ -97,7 +97,7 @@ struct sq_tlas {
 // kernels
 // ---------------------------------------------------------------------------
 
-template <int D, int IK, bool ANY>
+template <int D, int IK, bool ANY, bool EXACT>
 __global__ __launch_bounds__(1024) void k_tlas_nearest(TlasView tv, const float* __restrict__ q, uint32_t n, float max_toi,
                                                        const float* __restrict__ tmax, NearestOut out, uint32_t s_top,
                                                        uint32_t s_bot) {
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(1024) void k_tlas_nearest(TlasView tv, const float*
   float* ts = reinterpret_cast<float*>(lds + S * B) + tid;
   Stack st{refs, ts, B};
   Stack sb{refs + s_top * B, ts + s_top * B, B};
-  TlasHit h = walk_tlas<D, IK, ANY>(tv, r, lim, st, sb);
+  TlasHit h = walk_tlas<D, IK, ANY, EXACT>(tv, r, lim, st, sb);
   const bool hit = h.inst != NO_LEAF;
   if (ANY) { out.any[i] = hit ? 1 : 0; return; }
   out.leaf[i] = hit ? (int32_t)h.inst : -1;
@@ -136,7 +136,8 @@ __global__ __launch_bounds__(1024) void k_tlas_nearest(TlasView tv, const float*
 // Pooled, persistent form of k_tlas_nearest (pool_walk_tlas): waves take
 // chunks of `chunk` rays from counter[0] (zero at the start, left zero by
 // pool_retire), both stacks S-entry LDS rings spilling to their own columns.
-template <int D, int IK, bool ANY>
+// EXACT: the reference's hits (tlas_step); else the guarded walk.
+template <int D, int IK, bool ANY, bool EXACT>
 __global__ __launch_bounds__(256) void k_tlas_nearest_pool(TlasView tv, const float* __restrict__ q, uint32_t n, float max_toi,
                                                            const float* __restrict__ tmax, NearestOut out, uint32_t chunk,
                                                            uint32_t* counter, uint32_t S, SpillEntry* spill_top,
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(256) void k_tlas_nearest_pool(TlasView tv, const fl
   RingStack sb{lds + 2 * S * B + tid, reinterpret_cast<float*>(lds + 3 * S * B) + tid, B, S, spill_bot + gt, spill_stride};
   const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
   const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
-  pool_walk_tlas<D, IK, ANY>(tv, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, out, st, sb, pause_below);
+  pool_walk_tlas<D, IK, ANY, Chunks, EXACT>(tv, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, out, st, sb, pause_below);
   pool_retire(counter);
 }
 
@@ -199,7 +200,11 @@ __global__ __launch_bounds__(1024) void k_tlas_collect(TlasView tv, const float*
     pol.setup(q + (size_t)i * 4 * D);
     if (IK == INST_BALL && pol.root(s.root, max_toi)) {
       walk_visit<D>(s, root_ref,
-        [&](const float* nd, bool& ha, bool& hb) { float ta, tb; pol.pair(nd, max_toi, ha, hb, ta, tb); },
+        [&](const float* nd, bool& ha, bool& hb) {
+          if (synthetic_pair<D>(nd)) { ha = hb = true; return; }   // parts of one reference leaf
+          float ta, tb;
+          pol.pair(nd, max_toi, ha, hb, ta, tb);
+        },
         [&](uint32_t first, uint32_t c) {
           for (uint32_t j = 0; j < c; ++j) {
             const uint32_t id = s.ids[first + j];
@@ -216,7 +221,11 @@ __global__ __launch_bounds__(1024) void k_tlas_collect(TlasView tv, const float*
     float te, tx; int ax;
     if (slab_exact<D>(r, s.root, s.root + D, max_toi, te, tx, ax)) {
       walk_visit<D>(s, root_ref,
-        [&](const float* nd, bool& ha, bool& hb) { float ta, tb; slab_pair<D>(r, nd, max_toi, ha, hb, ta, tb); },
+        [&](const float* nd, bool& ha, bool& hb) {
+          if (synthetic_pair<D>(nd)) { ha = hb = true; return; }   // parts of one reference leaf
+          float ta, tb;
+          slab_pair<D>(r, nd, max_toi, ha, hb, ta, tb);
+        },
         [&](uint32_t first, uint32_t c) {
           for (uint32_t j = 0; j < c; ++j) {
             TlasHit h;
@@ -230,10 +239,12 @@ __global__ __launch_bounds__(1024) void k_tlas_collect(TlasView tv, const float*
   if (!FILL) counts[i] = cnt;
 }
 
-// Tlas::shapecast_nearest (tlas.rs:306-351), near-first over the probe-grown
-// top-level boxes with the shrinking limit; one cast [4D] per thread, the
-// stack in LDS.  Kinds other than BALL answer no cast (instance_cast).
-template <int D, int IK>
+// Tlas::shapecast_nearest (tlas.rs:306-351) over the probe-grown top-level
+// boxes with the shrinking limit; one cast [4D] per thread, the stack in LDS.
+// EXACT: unguarded contacts in the reference's order (synthetic pairs passed
+// through in leaf order); else guarded contacts, near-first.  Kinds other
+// than BALL answer no cast (instance_cast).
+template <int D, int IK, bool EXACT>
 __global__ __launch_bounds__(1024) void k_tlas_cast_nearest(TlasView tv, const float* __restrict__ q, uint32_t n, float max_toi,
                                                             int32_t* out_inst, float* out_toi, float* out_point,
                                                             uint32_t s_top) {
@@ -253,12 +264,16 @@ __global__ __launch_bounds__(1024) void k_tlas_cast_nearest(TlasView tv, const f
     while (true) {
       if (!(cur & LEAF_BIT)) {
         const float* nd = s.nodes + (size_t)cur * NF;
-        bool ha, hb;
+        bool ha, hb, a_first;
         float ta, tb;
-        pol.pair(nd, limit, ha, hb, ta, tb);
+        if (EXACT && synthetic_pair<D>(nd)) {
+          ha = hb = true; ta = tb = -SQ_INF; a_first = true;
+        } else {
+          pol.pair(nd, limit, ha, hb, ta, tb);
+          a_first = EXACT ? false : ta <= tb;
+        }
         const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
         if (ha && hb) {
-          const bool a_first = ta <= tb;
           st.ref[sp * st.stride] = a_first ? rb : ra;
           st.t[sp * st.stride] = a_first ? tb : ta;
           ++sp;
@@ -273,7 +288,7 @@ __global__ __launch_bounds__(1024) void k_tlas_cast_nearest(TlasView tv, const f
         for (uint32_t j = 0; j < cnt; ++j) {
           const uint32_t id = s.ids[first + j];
           float t;
-          if (!instance_cast<D, IK>(tv, pol, id, limit, t)) continue;
+          if (!instance_cast<D, IK, !EXACT>(tv, pol, id, limit, t)) continue;
           if (best == NO_LEAF || hit_before(t, id, best_t, best)) { best = id; best_t = t; limit = t; }
         }
       }
@@ -457,9 +472,14 @@ static int tlas_nearest_impl(sq_tlas* t, const float* rays, uint64_t n, float ma
     // one ray per thread, whole stacks in LDS
     const size_t lds = (size_t)(s_top + s_bot) * B * 8;
     if (lds > 160 * 1024) return fail(SQ_ERR_INVALID, "traversal stacks do not fit in LDS; lower the block size");
+    const bool exact = c->hit_mode != SQ_HITS_GUARDED;
     rc = dispatch_inst(t->in.dim, t->in.kind, [&]<int D, int K>() -> int {
-      if (any) k_tlas_nearest<D, K, true><<<grid_for(n, B), B, lds, c->stream>>>(tv, rays, (uint32_t)n, max_toi, tmax, out, s_top, s_bot);
-      else k_tlas_nearest<D, K, false><<<grid_for(n, B), B, lds, c->stream>>>(tv, rays, (uint32_t)n, max_toi, tmax, out, s_top, s_bot);
+      auto go = [&]<bool ANY_, bool EXACT_>() {
+        k_tlas_nearest<D, K, ANY_, EXACT_><<<grid_for(n, B), B, lds, c->stream>>>(tv, rays, (uint32_t)n, max_toi, tmax, out, s_top,
+                                                                                s_bot);
+      };
+      if (any) exact ? go.template operator()<true, true>() : go.template operator()<true, false>();
+      else exact ? go.template operator()<false, true>() : go.template operator()<false, false>();
       HIP_TRY(hipGetLastError());
       return SQ_OK;
     });
@@ -475,9 +495,9 @@ static int tlas_nearest_impl(sq_tlas* t, const float* rays, uint64_t n, float ma
   const uint32_t rows_top = s_top > S ? s_top - S : 0, rows_bot = s_bot > S ? s_bot - S : 0;
   const size_t lds = (size_t)2 * S * B * 8;
   rc = dispatch_inst(t->in.dim, t->in.kind, [&]<int D, int K>() -> int {
-    auto launch = [&]<bool ANY_>() -> int {
+    auto launch = [&]<bool ANY_, bool EXACT_>() -> int {
       int per_cu = 0;
-      if (int r0 = blocks_per_cu(reinterpret_cast<const void*>(&k_tlas_nearest_pool<D, K, ANY_>), c->device, B, lds, &per_cu))
+      if (int r0 = blocks_per_cu(reinterpret_cast<const void*>(&k_tlas_nearest_pool<D, K, ANY_, EXACT_>), c->device, B, lds, &per_cu))
         return r0;
       const uint64_t wpb = (uint64_t)B / 64;
       const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
@@ -485,13 +505,14 @@ static int tlas_nearest_impl(sq_tlas* t, const float* rays, uint64_t n, float ma
       int r2 = c->spill.ensure((size_t)(rows_top + rows_bot) * lanes * sizeof(SpillEntry) + 64);
       if (r2) return r2;
       SpillEntry* sp = c->spill.as<SpillEntry>();
-      k_tlas_nearest_pool<D, K, ANY_><<<(uint32_t)grid, B, lds, c->stream>>>(
+      k_tlas_nearest_pool<D, K, ANY_, EXACT_><<<(uint32_t)grid, B, lds, c->stream>>>(
           tv, rays, (uint32_t)n, max_toi, tmax, out, (uint32_t)chunk, c->pool_ctr.as<uint32_t>(), S, sp,
           sp + (size_t)rows_top * lanes, (uint32_t)lanes, (uint32_t)c->descend_min);
       HIP_TRY(hipGetLastError());
       return SQ_OK;
     };
-    return any ? launch.template operator()<true>() : launch.template operator()<false>();
+    if (c->hit_mode == SQ_HITS_GUARDED) return any ? launch.template operator()<true, false>() : launch.template operator()<false, false>();
+    return any ? launch.template operator()<true, true>() : launch.template operator()<false, true>();
   });
   if (rc) return rc;
   return end_timing(c);
@@ -579,9 +600,14 @@ int sq_tlas_shapecast_nearest(sq_tlas* t, const float* casts, uint64_t n, float 
   TlasView tv = tview(t);
   int rc = begin_timing(c);
   if (rc) return rc;
+  const bool exact = c->hit_mode != SQ_HITS_GUARDED;
   rc = dispatch_inst(t->in.dim, t->in.kind, [&]<int D, int K>() -> int {
-    k_tlas_cast_nearest<D, K><<<grid_for(n, B), B, lds, c->stream>>>(tv, casts, (uint32_t)n, max_toi, out_inst, out_toi,
-                                                                     out_point, s_top);
+    if (exact)
+      k_tlas_cast_nearest<D, K, true><<<grid_for(n, B), B, lds, c->stream>>>(tv, casts, (uint32_t)n, max_toi, out_inst, out_toi,
+                                                                           out_point, s_top);
+    else
+      k_tlas_cast_nearest<D, K, false><<<grid_for(n, B), B, lds, c->stream>>>(tv, casts, (uint32_t)n, max_toi, out_inst, out_toi,
+                                                                            out_point, s_top);
     HIP_TRY(hipGetLastError());
     return SQ_OK;
   });

@@ -10,14 +10,19 @@
 //   nearest update       crates/spatial-query/src/accel/bvh.rs:106-115 with
 //                        hit_ordering, query/hit.rs:77-84
 //
-// Traversal order is near-first over the pair-node layout (host_tree.h).  The
-// reference pushes child a then b and pops LIFO; its nearest result is still
-// reproduced exactly because every accepted hit of a built-in kind lies inside
-// the slab interval of its own AABB (AABB hits are that interval's entry;
-// SPHERE and TRIANGLE hits are guarded to it), hence inside every ancestor
-// box, so pruning by "entry <= current best toi" can never drop a hit that
-// beats the current best under (toi, leaf).  The result is the minimum under
-// (toi, leaf) over all accepted hits, whatever the visiting order.
+// Two walk orders.  ORDER_REF walks the reference's own order: push child a,
+// then b, pop LIFO (so b's subtree first), a popped entry skipped when it lies
+// beyond the current best; with the narrow phases unguarded this is
+// bvh.rs:73-124 step for step and returns the reference's answer for any
+// provider, including hits that fall outside their own AABB.  ORDER_NEAR
+// descends near-first; it returns the same answer when every accepted hit lies
+// inside its own AABB's slab interval (AABB hits are that interval's entry;
+// with GUARD, SPHERE and TRIANGLE hits are kept only inside it), since such a
+// hit lies inside every ancestor box and pruning by "entry <= current best
+// toi" can then never drop a hit that beats the current best under
+// (toi, leaf): the result is the minimum over all accepted hits, whatever the
+// visiting order.  Queries whose limit never shrinks (any-hit, all hits,
+// crossings, overlap) visit the same set in any order and run unguarded.
 #pragma once
 
 #include <cstdint>
@@ -235,8 +240,9 @@ template <int D, class R = float> SQ_HD R dotD(const R* a, const R* b) {
   return acc;
 }
 
-// Ball (differential.rs:61-82) with the box guard.
-template <int D, class R = float> SQ_HD bool narrow_sphere(const RayD<D, R>& r, const R* p, R limit, R& t_out) {
+// Ball (differential.rs:61-82); GUARD keeps a hit only inside the slab
+// interval of the ball's AABB.
+template <int D, class R = float, bool GUARD = false> SQ_HD bool narrow_sphere(const RayD<D, R>& r, const R* p, R limit, R& t_out) {
   R oc[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) oc[k] = r.o[k] - p[k];
@@ -249,20 +255,23 @@ template <int D, class R = float> SQ_HD bool narrow_sphere(const RayD<D, R>& r, 
   R t = -b - sq;
   if (t < R(0)) t = -b + sq;
   if (t < R(0) || t > limit) return false;
-  // guard: toi inside the slab interval of the ball's AABB (c - r, c + r)
-  R mn[D], mx[D];
+  if constexpr (GUARD) {
+    // toi inside the slab interval of the ball's AABB (c - r, c + r)
+    R mn[D], mx[D];
 #pragma unroll
-  for (int k = 0; k < D; ++k) { mn[k] = p[k] - rad; mx[k] = p[k] + rad; }
-  R te, tx; int ax;
-  if (!slab_exact<D>(r, mn, mx, limit, te, tx, ax)) return false;
-  if (!(te <= t && t <= tx)) return false;
+    for (int k = 0; k < D; ++k) { mn[k] = p[k] - rad; mx[k] = p[k] + rad; }
+    R te, tx; int ax;
+    if (!slab_exact<D>(r, mn, mx, limit, te, tx, ax)) return false;
+    if (!(te <= t && t <= tx)) return false;
+  }
   t_out = t;
   return true;
 }
 
 // Moeller-Trumbore (viewport-lib-query/src/lib.rs:281-306) with the reference
-// core's `t > max_toi` reject and the box guard.  p = v0.xyz,id, v1.xyz,_, v2.xyz,_
-SQ_HD bool narrow_tri(const RayD<3>& r, const float* p, float limit, float& t_out) {
+// core's `t > max_toi` reject; GUARD keeps a hit only inside the slab interval
+// of the triangle's AABB.  p = v0.xyz,id, v1.xyz,_, v2.xyz,_
+template <bool GUARD = false> SQ_HD bool narrow_tri(const RayD<3>& r, const float* p, float limit, float& t_out) {
   const float v0x = p[0], v0y = p[1], v0z = p[2];
   const float v1x = p[4], v1y = p[5], v1z = p[6];
   const float v2x = p[8], v2y = p[9], v2z = p[10];
@@ -288,14 +297,16 @@ SQ_HD bool narrow_tri(const RayD<3>& r, const float* p, float limit, float& t_ou
   const float t = (((0.0f + e2x * qx) + e2y * qy) + e2z * qz) * inv;
   if (!(t > 1e-6f)) return false;
   if (t > limit) return false;
-  // guard: toi inside the slab interval of the triangle's AABB
-  float mn[3], mx[3];
-  mn[0] = fminf(fminf(v0x, v1x), v2x); mx[0] = fmaxf(fmaxf(v0x, v1x), v2x);
-  mn[1] = fminf(fminf(v0y, v1y), v2y); mx[1] = fmaxf(fmaxf(v0y, v1y), v2y);
-  mn[2] = fminf(fminf(v0z, v1z), v2z); mx[2] = fmaxf(fmaxf(v0z, v1z), v2z);
-  float te, tx; int ax;
-  if (!slab_exact<3>(r, mn, mx, limit, te, tx, ax)) return false;
-  if (!(te <= t && t <= tx)) return false;
+  if constexpr (GUARD) {
+    // toi inside the slab interval of the triangle's AABB
+    float mn[3], mx[3];
+    mn[0] = fminf(fminf(v0x, v1x), v2x); mx[0] = fmaxf(fmaxf(v0x, v1x), v2x);
+    mn[1] = fminf(fminf(v0y, v1y), v2y); mx[1] = fmaxf(fmaxf(v0y, v1y), v2y);
+    mn[2] = fminf(fminf(v0z, v1z), v2z); mx[2] = fmaxf(fmaxf(v0z, v1z), v2z);
+    float te, tx; int ax;
+    if (!slab_exact<3>(r, mn, mx, limit, te, tx, ax)) return false;
+    if (!(te <= t && t <= tx)) return false;
+  }
   t_out = t;
   return true;
 }
@@ -305,11 +316,15 @@ template <int D> struct Prim<D, KIND_AABB> { static constexpr int STRIDE = 2 * D
 template <int D> struct Prim<D, KIND_SPHERE> { static constexpr int STRIDE = D + 1; };
 template <> struct Prim<3, KIND_TRI> { static constexpr int STRIDE = 12; };
 
-template <int D, int KIND, class R = float> SQ_HD bool narrow(const RayD<D, R>& r, const R* p, R limit, R& t) {
+template <int D, int KIND, class R = float, bool GUARD = false> SQ_HD bool narrow(const RayD<D, R>& r, const R* p, R limit, R& t) {
   if constexpr (KIND == KIND_AABB) { int ax; return narrow_aabb<D>(r, p, limit, t, ax); }
-  else if constexpr (KIND == KIND_SPHERE) return narrow_sphere<D>(r, p, limit, t);
-  else return narrow_tri(r, p, limit, t);
+  else if constexpr (KIND == KIND_SPHERE) return narrow_sphere<D, R, GUARD>(r, p, limit, t);
+  else return narrow_tri<GUARD>(r, p, limit, t);
 }
+
+// True when the leaf-ordered record is a synthetic pair node: one part of a
+// reference leaf split into a chain (host_tree.h), not a node of the reference.
+template <int D> SQ_HD bool synthetic_pair(const float* nd) { return fbits(nd[4 * D + 3]) != 0u; }
 
 // Normal and sub-object of the final hit (computed once per ray).
 // AABB: entry-face normal (-sign(d) on the entry axis, zero when the origin is
@@ -368,8 +383,12 @@ template <int D, int KIND, class R = float> SQ_HD void prim_box(const R* p, R* m
   }
 }
 
-// Rays (bvh.rs:51-328): slab pruning, the kind's ray narrow phase.
-template <int D, int KIND> struct RayPolicy {
+// Rays (bvh.rs:51-328): slab pruning, the kind's ray narrow phase.  INSIDE:
+// every accepted hit lies inside its own primitive's AABB slab interval (so a
+// near-first walk, and pruning inside a synthetic chain, keep the reference's
+// answer).
+template <int D, int KIND, bool GUARD = false> struct RayPolicy {
+  static constexpr bool INSIDE = KIND == KIND_AABB || GUARD;
   RayD<D> r;
   SQ_HD void setup(const float* src) { ray_setup<D>(r, src); }
   SQ_HD bool root(const float* root, float limit) const {
@@ -379,16 +398,18 @@ template <int D, int KIND> struct RayPolicy {
   SQ_HD void pair(const float* nd, float limit, bool& ha, bool& hb, float& ta, float& tb) const {
     slab_pair<D>(r, nd, limit, ha, hb, ta, tb);
   }
-  SQ_HD bool narrow(const float* p, float limit, float& t) const { return sqd::narrow<D, KIND>(r, p, limit, t); }
+  SQ_HD bool narrow(const float* p, float limit, float& t) const { return sqd::narrow<D, KIND, float, GUARD>(r, p, limit, t); }
 };
 
 // Shape casts (bvh.rs:331-414): the slab of the cast ray against each box
 // grown by the probe box, [min - probe.max, max - probe.min]
 // (shapecast.rs:70-73).  Narrow: AABB leaves answer with the same slab against
 // the grown leaf box (its entry); SPHERE leaves with the ball of radius
-// r + probe.max[0] (shape_overlap.rs:104-118) guarded by the grown box;
-// TRIANGLE leaves keep the reference default (no contact).
-template <int D, int KIND, class R = float> struct CastPolicy {
+// r + probe.max[0] (shape_overlap.rs:104-118), with GUARD kept only inside
+// the grown box's slab interval; TRIANGLE leaves keep the reference default
+// (no contact).
+template <int D, int KIND, class R = float, bool GUARD = false> struct CastPolicy {
+  static constexpr bool INSIDE = KIND != KIND_SPHERE || GUARD;
   RayD<D, R> r;
   R pmin[D], pmax[D];
   SQ_HD void setup(const R* src) {
@@ -440,8 +461,10 @@ template <int D, int KIND, class R = float> struct CastPolicy {
         R tt = -b - sq;
         if (tt < R(0)) tt = -b + sq;
         if (tt < R(0) || tt > limit) return false;
-        if (!grown_slab(mn, mx, limit, te, tx)) return false;
-        if (!(te <= tt && tt <= tx)) return false;
+        if constexpr (GUARD) {
+          if (!grown_slab(mn, mx, limit, te, tx)) return false;
+          if (!(te <= tt && tt <= tx)) return false;
+        }
         t = tt;
         return true;
       }
@@ -507,8 +530,25 @@ struct NearestResult {
   uint32_t pos;    // leaf-ordered position of the hit primitive
 };
 
-// Nearest (ANY = false) or first accepted hit (ANY = true) of one query.
-template <int D, int PS, bool ANY, bool FILT, class Q>
+// The two children of a pair node in walk order: whether each is to be
+// visited, which comes first, and the entries.  A synthetic pair (one part of
+// a reference leaf) is passed through in leaf order, entry -inf, unless every
+// accepted hit lies inside its primitive's box (Q::INSIDE), where its boxes
+// prune like any other.  REF: child b first (the reference pushes a, then b);
+// else the nearer entry first.
+template <int D, bool REF, class Q>
+SQ_HD void order_pair(const Q& q, const float* nd, float limit, bool& ha, bool& hb, float& ta, float& tb, bool& a_first) {
+  if (!Q::INSIDE && synthetic_pair<D>(nd)) {
+    ha = hb = true; ta = tb = -SQ_INF; a_first = true;
+    return;
+  }
+  q.pair(nd, limit, ha, hb, ta, tb);
+  a_first = REF ? false : ta <= tb;
+}
+
+// Nearest (ANY = false) or first accepted hit (ANY = true) of one query, in
+// the reference's order (REF) or near-first.
+template <int D, int PS, bool ANY, bool FILT, bool REF = false, class Q>
 SQ_HD NearestResult walk_nearest(const SceneView& s, const Q& q, float max_toi, const Filter& f, Stack st) {
   NearestResult res{SQ_INF, NO_LEAF, NO_LEAF};
   float limit = max_toi;
@@ -519,11 +559,10 @@ SQ_HD NearestResult walk_nearest(const SceneView& s, const Q& q, float max_toi, 
   while (true) {
     if (!(cur & LEAF_BIT)) {
       const float* nd = s.nodes + (size_t)cur * NF;
-      bool ha, hb; float ta, tb;
-      q.pair(nd, limit, ha, hb, ta, tb);
+      bool ha, hb, a_first; float ta, tb;
+      order_pair<D, REF>(q, nd, limit, ha, hb, ta, tb, a_first);
       const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
       if (ha && hb) {
-        const bool a_first = ta <= tb;
         st.ref[sp * st.stride] = a_first ? rb : ra;
         st.t[sp * st.stride] = a_first ? tb : ta;
         ++sp;
@@ -752,8 +791,9 @@ struct NoPause {
   SQ_HD bool operator()() const { return false; }
 };
 
-// One step: descend near-first through pair nodes to the next leaf (near child
-// next, far child pushed with its entry distance; when both children miss,
+// One step: descend through pair nodes to the next leaf (the child order_pair
+// puts first next, the other pushed with its entry distance; REF walks the
+// reference's order, else near-first; when both children miss,
 // pop until an entry is no farther than the current best), then test that
 // leaf's primitives and pop to the next entry worth visiting.  Returns true
 // when the query is finished (nothing left to pop).  pause() is asked after
@@ -762,12 +802,15 @@ struct NoPause {
 // next step), so per query the order of work is the same whatever it says.
 // COUNT adds the step's work to *wc.  SKIP treats the child refs skip[0..3] as
 // missed: subtrees this query has already walked completely (with a limit at
-// least the current one, so they hold nothing it has not seen).
-template <int D, int PS, bool ANY, bool FILT, bool COUNT = false, bool SKIP = false, bool QUAD = false, class Q,
-          class P = NoPause>
+// least the current one, so they hold nothing it has not seen).  QUAD walks
+// the 4-wide records near-first and needs Q::INSIDE (its records hold the
+// synthetic pairs' boxes like any other).
+template <int D, int PS, bool ANY, bool FILT, bool COUNT = false, bool SKIP = false, bool QUAD = false, bool REF = false,
+          class Q, class P = NoPause>
 SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const RingStack& st, WalkState& w, WorkCount* wc = nullptr,
                      const uint32_t* skip = nullptr, const P& pause = P()) {
   constexpr int NF = (D + 1) * 4;
+  static_assert(!QUAD || (Q::INSIDE && !REF), "the 4-wide walk is near-first over inside hits only");
   if constexpr (QUAD) {
     // descend over the 4-wide records: the nearest child next, the other hit
     // children pushed farthest first (so they pop nearest first)
@@ -816,15 +859,14 @@ SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const Ring
     float nd[NF];
     fetch16<D + 1>(s.nodes + (size_t)w.cur * NF, nd);
     if constexpr (COUNT) ++wc->nodes;
-    bool ha, hb; float ta, tb;
-    q.pair(nd, w.limit, ha, hb, ta, tb);
+    bool ha, hb, a_first; float ta, tb;
+    order_pair<D, REF>(q, nd, w.limit, ha, hb, ta, tb, a_first);
     const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
     if constexpr (SKIP) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) { ha = ha && ra != skip[k]; hb = hb && rb != skip[k]; }
     }
     if (ha && hb) {
-      const bool a_first = ta <= tb;
       st.push(w.sp, a_first ? rb : ra, a_first ? tb : ta);
       w.cur = a_first ? ra : rb;
     } else if (ha) {
@@ -882,9 +924,11 @@ SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const Ring
 //                      local origin (BallInstances, shape_overlap.rs:288-345):
 //                      the one kind that also answers shape casts and an exact
 //                      overlap test (the others keep the trait defaults).
-// Every kind keeps an instance hit only when its toi lies in the slab interval
-// of the instance's world AABB, the guard that makes the top-level result
-// independent of traversal order.
+// With GUARD every kind keeps an instance hit only when its toi lies in the
+// slab interval of the instance's world AABB, which makes the top-level result
+// independent of traversal order (the near-first walk); without it the hit is
+// the reference's (tlas.rs:459-481) and nearest queries walk the top level in
+// the reference's order.
 // ---------------------------------------------------------------------------
 
 enum { INST_BOX = 0, INST_MESH = 1, INST_BALL = 2 };
@@ -978,8 +1022,11 @@ struct TlasHit {
   uint32_t sub;    // triangle id within the mesh, NO_LEAF for boxes
 };
 
-// test_instance (tlas.rs:459-481) with the instance guard.
-template <int D, int IK>
+// test_instance (tlas.rs:459-481); GUARD adds the instance guard.  A mesh
+// instance answers with its mesh's nearest hit: the reference's walk of the
+// mesh BVH (unguarded triangles, the reference's order), or with GUARD the
+// guarded near-first walk.
+template <int D, int IK, bool GUARD = false>
 SQ_HD bool instance_test(const TlasView& tv, const RayD<D>& r, uint32_t id, float limit, Stack st_bot, TlasHit& h) {
   const float* x = tv.xf + (size_t)id * tv.xs;
   float ol[D], dl[D];
@@ -998,23 +1045,41 @@ SQ_HD bool instance_test(const TlasView& tv, const RayD<D>& r, uint32_t id, floa
     float src[2 * D];
 #pragma unroll
     for (int k = 0; k < D; ++k) { src[k] = ol[k] * inv_s; src[D + k] = dl[k] * inv_s; }
-    RayPolicy<D, KIND_TRI> pol;
+    RayPolicy<D, KIND_TRI, GUARD> pol;
     pol.setup(src);
     Filter nof{nullptr, ~0ull, -1};
-    NearestResult b = walk_nearest<D, Prim<D, KIND_TRI>::STRIDE, false, false>(tv.blas[mesh], pol, limit, nof, st_bot);
+    NearestResult b = walk_nearest<D, Prim<D, KIND_TRI>::STRIDE, false, false, !GUARD>(tv.blas[mesh], pol, limit, nof, st_bot);
     if (b.leaf == NO_LEAF) return false;
     t = b.t; h.pos = b.pos; h.sub = b.leaf;
   }
-  const float* w = tv.world + (size_t)id * 2 * D;
-  float te, tx; int a;
-  if (!slab_exact<D>(r, w, w + D, limit, te, tx, a)) return false;
-  if (!(te <= t && t <= tx)) return false;
+  if constexpr (GUARD) {
+    const float* w = tv.world + (size_t)id * 2 * D;
+    float te, tx; int a;
+    if (!slab_exact<D>(r, w, w + D, limit, te, tx, a)) return false;
+    if (!(te <= t && t <= tx)) return false;
+  }
   h.t = t; h.inst = id;
   return true;
 }
 
-// Tlas::raycast_nearest (tlas.rs:104-154), near-first; ANY stops at the first hit.
-template <int D, int IK, bool ANY>
+// The two top-level children of a pair node in walk order (as order_pair):
+// EXACT (unguarded instance tests) passes synthetic pairs through in leaf
+// order and walks nearest queries in the reference's order (b first); else
+// near-first over the guarded tests.
+template <int D, bool EXACT, bool ANY>
+SQ_HD void order_top(const RayD<D>& r, const float* nd, float limit, bool& ha, bool& hb, float& ta, float& tb, bool& a_first) {
+  if (EXACT && synthetic_pair<D>(nd)) {
+    ha = hb = true; ta = tb = -SQ_INF; a_first = true;
+    return;
+  }
+  slab_pair<D>(r, nd, limit, ha, hb, ta, tb);
+  a_first = (EXACT && !ANY) ? false : ta <= tb;
+}
+
+// Tlas::raycast_nearest (tlas.rs:104-154); ANY stops at the first hit.
+// EXACT: unguarded instance tests in the reference's order; else guarded,
+// near-first.
+template <int D, int IK, bool ANY, bool EXACT = true>
 SQ_HD TlasHit walk_tlas(const TlasView& tv, const RayD<D>& r, float max_toi, Stack st, Stack st_bot) {
   TlasHit res{SQ_INF, NO_LEAF, NO_LEAF, NO_LEAF};
   float limit = max_toi;
@@ -1029,11 +1094,10 @@ SQ_HD TlasHit walk_tlas(const TlasView& tv, const RayD<D>& r, float max_toi, Sta
   while (true) {
     if (!(cur & LEAF_BIT)) {
       const float* nd = s.nodes + (size_t)cur * NF;
-      bool ha, hb; float ta, tb;
-      slab_pair<D>(r, nd, limit, ha, hb, ta, tb);
+      bool ha, hb, a_first; float ta, tb;
+      order_top<D, EXACT, ANY>(r, nd, limit, ha, hb, ta, tb, a_first);
       const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
       if (ha && hb) {
-        const bool a_first = ta <= tb;
         st.ref[sp * st.stride] = a_first ? rb : ra;
         st.t[sp * st.stride] = a_first ? tb : ta;
         ++sp;
@@ -1047,7 +1111,7 @@ SQ_HD TlasHit walk_tlas(const TlasView& tv, const RayD<D>& r, float max_toi, Sta
       const uint32_t cnt = ((cur >> 27) & 15u) + 1u;
       for (uint32_t j = 0; j < cnt; ++j) {
         TlasHit h;
-        if (!instance_test<D, IK>(tv, r, s.ids[first + j], limit, st_bot, h)) continue;
+        if (!instance_test<D, IK, !EXACT>(tv, r, s.ids[first + j], limit, st_bot, h)) continue;
         if (ANY) return h;
         if (res.inst == NO_LEAF || hit_before(h.t, h.inst, res.t, res.inst)) { res = h; limit = h.t; }
       }
@@ -1100,16 +1164,17 @@ SQ_HD void tlas_normal(const TlasView& tv, const RayD<D>& r, const TlasHit& h, f
 // Tlas shape casts (tlas.rs:306-391): the swept probe (radius = the cast
 // box's max x, as the built-in SPHERE cast) against ball instances, the ray
 // against the radius sum about the instance's centre (shape_overlap.rs:
-// 326-337), kept only inside the probe's slab interval against the
-// instance's world AABB (the guard).  The other kinds answer no cast
+// 326-337), with GUARD kept only inside the probe's slab interval against the
+// instance's world AABB.  The other kinds answer no cast
 // (InstancedGeometry::test_shape_cast's default, instance.rs:72-87).
-template <int D, int IK>
+template <int D, int IK, bool GUARD = false>
 SQ_HD bool instance_cast(const TlasView& tv, const CastPolicy<D, KIND_AABB>& pol, uint32_t id, float limit, float& t) {
   if constexpr (IK != INST_BALL) {
     return false;
   } else {
     const float* x = tv.xf + (size_t)id * tv.xs;
     if (!ray_ball<D>(pol.r.o, pol.r.d, x + D * D, x[D * D + D] + pol.pmax[0], limit, t)) return false;
+    if constexpr (!GUARD) return true;
     const float* w = tv.world + (size_t)id * 2 * D;
     float te, tx;
     if (!pol.grown_slab(w, w + D, limit, te, tx)) return false;

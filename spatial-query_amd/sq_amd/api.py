@@ -13,7 +13,11 @@
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import itertools
+import sys
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -21,6 +25,52 @@ import numpy as np
 from .capi import BatchQuery, Filter, SqError, TreeStats, Tuning, check, lib, SQ_BATCH_BVH, SQ_BATCH_TLAS, SQ_ERR_CAPACITY
 
 ALL_LAYERS = (1 << 64) - 1
+SQ_HITS_REFERENCE, SQ_HITS_GUARDED = 0, 1   # sq_ctx_set_hit_mode (include/sq.h)
+
+# Teardown order.  Engine objects release device memory and synchronise
+# streams (the context's own, or torch's when the context shares it) when they
+# close.  An object still alive when the interpreter shuts down (held by a
+# module global, a saved traceback, a reference cycle) would otherwise be
+# closed by the garbage collector during finalization, after torch's exit
+# hooks have run and while the HIP runtime is being torn down; a C++
+# exception thrown by the runtime there cannot be caught and ends the process
+# with std::terminate.  So every object is tracked here; an exit hook, kept
+# the newest one registered (so it runs before torch's, which atexit runs in
+# reverse order of registration), closes the live ones newest first (trees
+# before the contexts they hold), and __del__ does nothing once the
+# interpreter is finalizing.
+_live = weakref.WeakValueDictionary()
+_serial = itertools.count()
+
+
+def _close_all():
+    for k in sorted(_live.keys(), reverse=True):
+        obj = _live.get(k)
+        if obj is None:
+            continue
+        try:
+            obj.close()
+        except Exception:
+            pass
+
+
+def _track(obj):
+    """Registers an engine object for _close_all and keeps the exit hook last."""
+    _live[next(_serial)] = obj
+    atexit.unregister(_close_all)
+    atexit.register(_close_all)
+    return obj
+
+
+def _finalize(obj):
+    """The body of every __del__: close, unless the interpreter is shutting
+    down (then the exit hook has closed it, or nothing safely can)."""
+    if sys.is_finalizing():
+        return
+    try:
+        obj.close()
+    except Exception:
+        pass
 
 
 def prim_stride(dim: int, kind: int) -> int:
@@ -35,12 +85,22 @@ def _hp(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
-def _dp(t):
-    """Device pointer of a contiguous CUDA tensor (or None)."""
+def _dp(t, dtype: str | None = None, cols: int | None = None, n: int | None = None, per: int = 1):
+    """Device pointer of a contiguous CUDA tensor (or None), after the checks
+    that keep a kernel inside the buffer: the element type (`dtype`, a torch
+    dtype name), for query arrays the row length (`cols`: a [n][cols] tensor),
+    and for per-query inputs and outputs at least n * per elements.  A mismatch
+    raises ValueError before anything reaches the device."""
     if t is None:
         return None
     if not t.is_cuda or not t.is_contiguous():
         raise ValueError("query buffers must be contiguous GPU tensors")
+    if dtype is not None and str(t.dtype) != "torch." + dtype:
+        raise ValueError(f"expected a {dtype} tensor, got {t.dtype}")
+    if cols is not None and (t.dim() != 2 or t.shape[1] != cols):
+        raise ValueError(f"expected a [n][{cols}] tensor, got shape {tuple(t.shape)}")
+    if n is not None and t.numel() < n * per:
+        raise ValueError(f"expected at least {n * per} elements, got {t.numel()}")
     return C.c_void_p(t.data_ptr())
 
 
@@ -62,6 +122,7 @@ class Context:
         check(lib().sq_ctx_create(device, C.byref(h)))
         self._h = h
         self.device = device
+        _track(self)
         if use_torch_stream:
             # Order engine work with the framework's work on the same stream.
             import torch
@@ -98,6 +159,15 @@ class Context:
         return dict(block=t.block, rays_per_lane=t.rays_per_lane, lds_stack=t.lds_stack, ray_order=t.ray_order,
                     descend_min=t.descend_min, split_depth=t.split_depth, wide=t.wide)
 
+    def set_hit_mode(self, mode: int):
+        """SQ_HITS_REFERENCE (default: the reference's hits for every query) or
+        SQ_HITS_GUARDED (SPHERE / TRIANGLE / instance hits kept only inside
+        their own box's slab interval, walked near-first); see include/sq.h."""
+        check(lib().sq_ctx_set_hit_mode(self._h, int(mode)))
+
+    def hit_mode(self) -> int:
+        return int(lib().sq_ctx_get_hit_mode(self._h))
+
     def last_kernel_ms(self) -> float:
         ms = C.c_float()
         check(lib().sq_ctx_last_kernel_ms(self._h, C.byref(ms)))
@@ -109,10 +179,7 @@ class Context:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 def _stats(s: TreeStats) -> dict:
@@ -126,6 +193,7 @@ class Tree:
 
     def __init__(self, handle, dim: int, n_prims: int):
         self._h, self.dim, self.n_prims = handle, dim, n_prims
+        _track(self)
 
     @classmethod
     def build(cls, dim: int, kind: int, prims) -> "Tree":
@@ -165,10 +233,7 @@ class Tree:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class DeviceBvh:
@@ -179,6 +244,7 @@ class DeviceBvh:
         h = C.c_void_p()
         check(lib().sq_bvh_upload(ctx.handle, tree._h, kind, _hp(p), C.byref(h)))
         self._h, self.ctx, self.dim, self.kind, self.n_prims = h, ctx, tree.dim, kind, p.shape[0]
+        _track(self)
 
     @classmethod
     def build(cls, ctx: Context, dim: int, kind: int, prims) -> "DeviceBvh":
@@ -196,9 +262,9 @@ class DeviceBvh:
         n = prims_dev.numel() // prim_stride(dim, kind)
         obj = cls.__new__(cls)
         h = C.c_void_p()
-        check(lib().sq_bvh_build_device(ctx.handle, dim, kind, _dp(prims_dev) if n else None, n, C.byref(h)))
+        check(lib().sq_bvh_build_device(ctx.handle, dim, kind, _dp(prims_dev, "float32") if n else None, n, C.byref(h)))
         obj._h, obj.ctx, obj.dim, obj.kind, obj.n_prims = h, ctx, dim, kind, n
-        return obj
+        return _track(obj)
 
     def export(self):
         """(bounds, child_a, child_b, prim_count, prim_indices) of the device tree
@@ -212,7 +278,7 @@ class DeviceBvh:
 
     def rebuild(self, prims_dev):
         """Build the tree again, on the device, from moved primitives (same count)."""
-        check(lib().sq_bvh_rebuild(self._h, _dp(prims_dev)))
+        check(lib().sq_bvh_rebuild(self._h, _dp(prims_dev, "float32", n=self.n_prims, per=self._stride())))
 
     # -- management ---------------------------------------------------------
     def set_layers(self, layers):
@@ -231,7 +297,11 @@ class DeviceBvh:
         return out
 
     def refit(self, prims_dev):
-        check(lib().sq_bvh_refit(self._h, _dp(prims_dev)))
+        check(lib().sq_bvh_refit(self._h, _dp(prims_dev, "float32", n=self.n_prims, per=self._stride())))
+
+    def _stride(self) -> int:
+        # consumer trees (kind 3) hold records the consumer laid out: not checked here
+        return prim_stride(self.dim, self.kind) if self.kind in (0, 1, 2) else 0
 
     # -- queries --------------------------------------------------------------
     def _f(self, filt):
@@ -254,15 +324,18 @@ class DeviceBvh:
         return out
 
     def raycast_nearest_into(self, rays, max_toi, leaf, toi, max_toi_per_ray=None, filt=None, point=None, normal=None, sub=None):
-        check(lib().sq_raycast_nearest(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(max_toi_per_ray),
-                                       self._f(filt), _dp(leaf), _dp(toi), _dp(point), _dp(normal), _dp(sub)))
+        n = rays.shape[0]
+        check(lib().sq_raycast_nearest(self._h, _dp(rays, "float32", 2 * self.dim), n, float(max_toi),
+                                       _dp(max_toi_per_ray, "float32", n=n), self._f(filt), _dp(leaf, "int32", n=n),
+                                       _dp(toi, "float32", n=n), _dp(point, "float32", n=n, per=self.dim),
+                                       _dp(normal, "float32", n=n, per=self.dim), _dp(sub, "int32", n=n)))
 
     def raycast_any(self, rays, max_toi, max_toi_per_ray=None, filt=None, out=None):
         import torch
         if out is None:
             out = torch.empty(rays.shape[0], dtype=torch.uint8, device=rays.device)
-        check(lib().sq_raycast_any(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(max_toi_per_ray),
-                                   self._f(filt), _dp(out)))
+        check(lib().sq_raycast_any(self._h, _dp(rays, "float32", 2 * self.dim), rays.shape[0], float(max_toi), _dp(max_toi_per_ray, "float32", n=rays.shape[0]),
+                                   self._f(filt), _dp(out, "uint8", n=rays.shape[0])))
         return out
 
     def _csr(self, fn, n, dev, has_t, capacity):
@@ -287,13 +360,13 @@ class DeviceBvh:
 
     def raycast_all(self, rays, max_toi, filt=None, crossings=False, capacity=0):
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_raycast_all(self._h, _dp(rays), rays.shape[0], float(max_toi), self._f(filt), int(crossings),
-                                        _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+            return lib().sq_raycast_all(self._h, _dp(rays, "float32", 2 * self.dim), rays.shape[0], float(max_toi), self._f(filt), int(crossings),
+                                        _dp(off, "int64"), _dp(leaf, "int32"), _dp(toi, "float32"), cap, C.byref(total))
         return self._csr(fn, rays.shape[0], rays.device, True, capacity)
 
     def overlap(self, regions, filt=None, capacity=0):
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_overlap(self._h, _dp(regions), regions.shape[0], self._f(filt), _dp(off), _dp(leaf),
+            return lib().sq_overlap(self._h, _dp(regions, "float32", 2 * self.dim), regions.shape[0], self._f(filt), _dp(off, "int64"), _dp(leaf, "int32"),
                                     cap, C.byref(total))
         off, leaf, _ = self._csr(fn, regions.shape[0], regions.device, False, capacity)
         return off, leaf
@@ -305,16 +378,16 @@ class DeviceBvh:
         leaf = torch.empty(n, dtype=torch.int32, device=dev)
         toi = torch.empty(n, dtype=torch.float32, device=dev)
         pt = torch.empty((n, self.dim), dtype=torch.float32, device=dev) if point else None
-        check(lib().sq_shapecast_nearest(self._h, _dp(casts), n, float(max_toi), _dp(max_toi_per_cast), self._f(filt),
-                                         _dp(leaf), _dp(toi), _dp(pt)))
+        check(lib().sq_shapecast_nearest(self._h, _dp(casts, "float32", 4 * self.dim), n, float(max_toi), _dp(max_toi_per_cast, "float32", n=casts.shape[0]), self._f(filt),
+                                         _dp(leaf, "int32"), _dp(toi, "float32"), _dp(pt, "float32")))
         out = dict(leaf=leaf, toi=toi)
         if point: out["point"] = pt
         return out
 
     def shapecast_all(self, casts, max_toi, max_toi_per_cast=None, filt=None, capacity=0):
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_shapecast_all(self._h, _dp(casts), casts.shape[0], float(max_toi), _dp(max_toi_per_cast),
-                                          self._f(filt), _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+            return lib().sq_shapecast_all(self._h, _dp(casts, "float32", 4 * self.dim), casts.shape[0], float(max_toi), _dp(max_toi_per_cast, "float32", n=casts.shape[0]),
+                                          self._f(filt), _dp(off, "int64"), _dp(leaf, "int32"), _dp(toi, "float32"), cap, C.byref(total))
         return self._csr(fn, casts.shape[0], casts.device, True, capacity)
 
     def required_work(self, rays, t_limit):
@@ -322,7 +395,7 @@ class DeviceBvh:
         n = rays.shape[0]
         nodes = torch.empty(n, dtype=torch.int32, device=rays.device)
         prims = torch.empty(n, dtype=torch.int32, device=rays.device)
-        check(lib().sq_required_work(self._h, _dp(rays), n, _dp(t_limit), _dp(nodes), _dp(prims)))
+        check(lib().sq_required_work(self._h, _dp(rays, "float32", 2 * self.dim), n, _dp(t_limit, "float32", n=n), _dp(nodes, "int32"), _dp(prims, "int32")))
         return nodes, prims
 
     def close(self):
@@ -331,10 +404,7 @@ class DeviceBvh:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class DeviceBvh64:
@@ -348,6 +418,7 @@ class DeviceBvh64:
         h = C.c_void_p()
         check(lib().sq_bvh64_create(ctx.handle, dim, kind, _hp(p), p.shape[0], C.byref(h)))
         self._h, self.ctx, self.dim, self.kind, self.n_prims = h, ctx, dim, kind, p.shape[0]
+        _track(self)
 
     def set_layers(self, layers):
         l = None if layers is None else _host(layers, np.uint64)
@@ -368,8 +439,8 @@ class DeviceBvh64:
         toi = torch.empty(n, dtype=torch.float64, device=dev)
         pt = torch.empty((n, self.dim), dtype=torch.float64, device=dev) if point else None
         nm = torch.empty((n, self.dim), dtype=torch.float64, device=dev) if normal else None
-        check(lib().sq_bvh64_raycast_nearest(self._h, _dp(rays), n, float(max_toi), _dp(max_toi_per_ray), self._f(filt),
-                                             _dp(leaf), _dp(toi), _dp(pt), _dp(nm)))
+        check(lib().sq_bvh64_raycast_nearest(self._h, _dp(rays, "float64", 2 * self.dim), n, float(max_toi), _dp(max_toi_per_ray, "float64", n=rays.shape[0]), self._f(filt),
+                                             _dp(leaf, "int32"), _dp(toi, "float64"), _dp(pt, "float64"), _dp(nm, "float64")))
         out = dict(leaf=leaf, toi=toi)
         if point: out["point"] = pt
         if normal: out["normal"] = nm
@@ -378,8 +449,8 @@ class DeviceBvh64:
     def raycast_any(self, rays, max_toi, max_toi_per_ray=None, filt=None):
         import torch
         out = torch.empty(rays.shape[0], dtype=torch.uint8, device=rays.device)
-        check(lib().sq_bvh64_raycast_any(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(max_toi_per_ray),
-                                         self._f(filt), _dp(out)))
+        check(lib().sq_bvh64_raycast_any(self._h, _dp(rays, "float64", 2 * self.dim), rays.shape[0], float(max_toi), _dp(max_toi_per_ray, "float64", n=rays.shape[0]),
+                                         self._f(filt), _dp(out, "uint8", n=rays.shape[0])))
         return out
 
     def _csr(self, fn, n, dev, has_t, capacity):
@@ -402,8 +473,8 @@ class DeviceBvh64:
 
     def raycast_all(self, rays, max_toi, filt=None, capacity=0):
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_bvh64_raycast_all(self._h, _dp(rays), rays.shape[0], float(max_toi), self._f(filt),
-                                              _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+            return lib().sq_bvh64_raycast_all(self._h, _dp(rays, "float64", 2 * self.dim), rays.shape[0], float(max_toi), self._f(filt),
+                                              _dp(off, "int64"), _dp(leaf, "int32"), _dp(toi, "float64"), cap, C.byref(total))
         return self._csr(fn, rays.shape[0], rays.device, True, capacity)
 
     def shapecast_nearest(self, casts, max_toi, max_toi_per_cast=None, filt=None, point=False):
@@ -412,21 +483,21 @@ class DeviceBvh64:
         leaf = torch.empty(n, dtype=torch.int32, device=dev)
         toi = torch.empty(n, dtype=torch.float64, device=dev)
         pt = torch.empty((n, self.dim), dtype=torch.float64, device=dev) if point else None
-        check(lib().sq_bvh64_shapecast_nearest(self._h, _dp(casts), n, float(max_toi), _dp(max_toi_per_cast),
-                                               self._f(filt), _dp(leaf), _dp(toi), _dp(pt)))
+        check(lib().sq_bvh64_shapecast_nearest(self._h, _dp(casts, "float64", 4 * self.dim), n, float(max_toi), _dp(max_toi_per_cast, "float64", n=casts.shape[0]),
+                                               self._f(filt), _dp(leaf, "int32"), _dp(toi, "float64"), _dp(pt, "float64")))
         out = dict(leaf=leaf, toi=toi)
         if point: out["point"] = pt
         return out
 
     def shapecast_all(self, casts, max_toi, max_toi_per_cast=None, filt=None, capacity=0):
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_bvh64_shapecast_all(self._h, _dp(casts), casts.shape[0], float(max_toi), _dp(max_toi_per_cast),
-                                                self._f(filt), _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+            return lib().sq_bvh64_shapecast_all(self._h, _dp(casts, "float64", 4 * self.dim), casts.shape[0], float(max_toi), _dp(max_toi_per_cast, "float64", n=casts.shape[0]),
+                                                self._f(filt), _dp(off, "int64"), _dp(leaf, "int32"), _dp(toi, "float64"), cap, C.byref(total))
         return self._csr(fn, casts.shape[0], casts.device, True, capacity)
 
     def overlap(self, regions, filt=None, capacity=0):
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_bvh64_overlap(self._h, _dp(regions), regions.shape[0], self._f(filt), _dp(off), _dp(leaf),
+            return lib().sq_bvh64_overlap(self._h, _dp(regions, "float64", 2 * self.dim), regions.shape[0], self._f(filt), _dp(off, "int64"), _dp(leaf, "int32"),
                                           cap, C.byref(total))
         off, leaf, _ = self._csr(fn, regions.shape[0], regions.device, False, capacity)
         return off, leaf
@@ -437,10 +508,7 @@ class DeviceBvh64:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class Tlas64:
@@ -454,6 +522,7 @@ class Tlas64:
         hd = C.c_void_p()
         check(lib().sq_tlas64_create(ctx.handle, dim, r.shape[0], _hp(r), _hp(t), _hp(h), C.byref(hd)))
         self._h, self.ctx, self.dim, self.n = hd, ctx, dim, r.shape[0]
+        _track(self)
 
     def stats(self) -> dict:
         s = TreeStats()
@@ -466,8 +535,8 @@ class Tlas64:
         leaf = torch.empty(n, dtype=torch.int32, device=dev)
         toi = torch.empty(n, dtype=torch.float64, device=dev)
         pt = torch.empty((n, self.dim), dtype=torch.float64, device=dev) if point else None
-        check(lib().sq_tlas64_raycast_nearest(self._h, _dp(rays), n, float(max_toi), _dp(max_toi_per_ray), _dp(leaf),
-                                              _dp(toi), _dp(pt)))
+        check(lib().sq_tlas64_raycast_nearest(self._h, _dp(rays, "float64", 2 * self.dim), n, float(max_toi), _dp(max_toi_per_ray, "float64", n=rays.shape[0]), _dp(leaf, "int32"),
+                                              _dp(toi, "float64"), _dp(pt, "float64")))
         out = dict(leaf=leaf, toi=toi)
         if point: out["point"] = pt
         return out
@@ -475,8 +544,8 @@ class Tlas64:
     def raycast_any(self, rays, max_toi, max_toi_per_ray=None):
         import torch
         out = torch.empty(rays.shape[0], dtype=torch.uint8, device=rays.device)
-        check(lib().sq_tlas64_raycast_any(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(max_toi_per_ray),
-                                          _dp(out)))
+        check(lib().sq_tlas64_raycast_any(self._h, _dp(rays, "float64", 2 * self.dim), rays.shape[0], float(max_toi), _dp(max_toi_per_ray, "float64", n=rays.shape[0]),
+                                          _dp(out, "uint8", n=rays.shape[0])))
         return out
 
     def close(self):
@@ -485,10 +554,7 @@ class Tlas64:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class Tlas:
@@ -516,6 +582,7 @@ class Tlas:
         check(lib().sq_tlas_build(ctx.handle, dim, kind, n, _hp(rot), _hp(trans), _hp(hv), mesh_ptr, nm, _hp(mo), _hp(sc),
                                   C.byref(h)))
         self._h, self.ctx, self.dim, self.kind, self.n = h, ctx, dim, kind, n
+        _track(self)
 
     def refit(self, rot, trans, halves=None):
         rot = _host(rot); trans = _host(trans)
@@ -547,8 +614,8 @@ class Tlas:
         pt = torch.empty((n, self.dim), dtype=torch.float32, device=dev) if point else None
         nm = torch.empty((n, self.dim), dtype=torch.float32, device=dev) if normal else None
         sb = torch.empty(n, dtype=torch.int32, device=dev) if sub else None
-        check(lib().sq_tlas_raycast_nearest(self._h, _dp(rays), n, float(max_toi), _dp(max_toi_per_ray), _dp(inst),
-                                            _dp(toi), _dp(pt), _dp(nm), _dp(sb)))
+        check(lib().sq_tlas_raycast_nearest(self._h, _dp(rays, "float32", 2 * self.dim), n, float(max_toi), _dp(max_toi_per_ray, "float32", n=rays.shape[0]), _dp(inst, "int32"),
+                                            _dp(toi, "float32"), _dp(pt, "float32"), _dp(nm, "float32"), _dp(sb, "int32")))
         out = dict(leaf=inst, toi=toi)
         if point: out["point"] = pt
         if normal: out["normal"] = nm
@@ -558,18 +625,18 @@ class Tlas:
     def raycast_any(self, rays, max_toi, max_toi_per_ray=None):
         import torch
         out = torch.empty(rays.shape[0], dtype=torch.uint8, device=rays.device)
-        check(lib().sq_tlas_raycast_any(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(max_toi_per_ray), _dp(out)))
+        check(lib().sq_tlas_raycast_any(self._h, _dp(rays, "float32", 2 * self.dim), rays.shape[0], float(max_toi), _dp(max_toi_per_ray, "float32", n=rays.shape[0]), _dp(out, "uint8", n=rays.shape[0])))
         return out
 
     def raycast_all(self, rays, max_toi, capacity=0):
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_tlas_raycast_all(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(off), _dp(leaf),
-                                             _dp(toi), cap, C.byref(total))
+            return lib().sq_tlas_raycast_all(self._h, _dp(rays, "float32", 2 * self.dim), rays.shape[0], float(max_toi), _dp(off, "int64"), _dp(leaf, "int32"),
+                                             _dp(toi, "float32"), cap, C.byref(total))
         return DeviceBvh._csr(self, fn, rays.shape[0], rays.device, True, capacity)
 
     def overlap(self, regions, capacity=0):
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_tlas_overlap(self._h, _dp(regions), regions.shape[0], _dp(off), _dp(leaf), cap, C.byref(total))
+            return lib().sq_tlas_overlap(self._h, _dp(regions, "float32", 2 * self.dim), regions.shape[0], _dp(off, "int64"), _dp(leaf, "int32"), cap, C.byref(total))
         off, leaf, _ = DeviceBvh._csr(self, fn, regions.shape[0], regions.device, False, capacity)
         return off, leaf
 
@@ -586,7 +653,7 @@ class Tlas:
         leaf = torch.empty(n, dtype=torch.int32, device=casts.device)
         toi = torch.empty(n, dtype=torch.float32, device=casts.device)
         pt = torch.empty((n, self.dim), dtype=torch.float32, device=casts.device) if point else None
-        check(lib().sq_tlas_shapecast_nearest(self._h, _dp(casts), n, float(max_toi), _dp(leaf), _dp(toi), _dp(pt)))
+        check(lib().sq_tlas_shapecast_nearest(self._h, _dp(casts, "float32", 4 * self.dim), n, float(max_toi), _dp(leaf, "int32"), _dp(toi, "float32"), _dp(pt, "float32")))
         out = dict(leaf=leaf, toi=toi)
         if point: out["point"] = pt
         return out
@@ -594,8 +661,8 @@ class Tlas:
     def shapecast_all(self, casts, max_toi, capacity=0):
         """Tlas::shapecast_all: CSR (offsets, leaf, toi) sorted by (toi, instance)."""
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_tlas_shapecast_all(self._h, _dp(casts), casts.shape[0], float(max_toi), _dp(off), _dp(leaf),
-                                               _dp(toi), cap, C.byref(total))
+            return lib().sq_tlas_shapecast_all(self._h, _dp(casts, "float32", 4 * self.dim), casts.shape[0], float(max_toi), _dp(off, "int64"), _dp(leaf, "int32"),
+                                               _dp(toi, "float32"), cap, C.byref(total))
         return DeviceBvh._csr(self, fn, casts.shape[0], casts.device, True, capacity)
 
     def close(self):
@@ -604,10 +671,7 @@ class Tlas:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class Narrow:
@@ -626,6 +690,7 @@ class Narrow:
         h = C.c_void_p()
         check(lib().sq_narrow_create(ctx.handle, dim, source.encode(), prim_floats, C.byref(h)))
         self._h, self.ctx, self.dim, self.prim_floats = h, ctx, dim, prim_floats
+        _track(self)
 
     def build(self, boxes, prims) -> "ConsumerBvh":
         """The tree over n primitives: world boxes [n][2*dim] (host) drive the
@@ -642,10 +707,7 @@ class Narrow:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class ConsumerBvh:
@@ -655,6 +717,7 @@ class ConsumerBvh:
 
     def __init__(self, narrow: Narrow, handle, n: int):
         self.narrow, self._h, self.n_prims, self.dim = narrow, handle, n, narrow.dim
+        _track(self)
 
     def stats(self) -> dict:
         s = TreeStats()
@@ -666,7 +729,7 @@ class ConsumerBvh:
         n = rays.shape[0]
         leaf = torch.empty(n, dtype=torch.int32, device=rays.device)
         toi = torch.empty(n, dtype=torch.float32, device=rays.device)
-        check(lib().sq_raycast_nearest_consumer(self.narrow._h, self._h, _dp(rays), n, float(max_toi), _dp(leaf), _dp(toi)))
+        check(lib().sq_raycast_nearest_consumer(self.narrow._h, self._h, _dp(rays, "float32", 2 * self.dim), n, float(max_toi), _dp(leaf, "int32"), _dp(toi, "float32")))
         return {"leaf": leaf, "toi": toi}
 
     _csr = DeviceBvh._csr
@@ -674,8 +737,8 @@ class ConsumerBvh:
     def raycast_all(self, rays, max_toi, crossings=False, capacity=0):
         """CSR (offsets, leaf, toi), each ray's hits sorted by (toi, leaf)."""
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_raycast_all_consumer(self.narrow._h, self._h, _dp(rays), rays.shape[0], float(max_toi),
-                                                 int(crossings), _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+            return lib().sq_raycast_all_consumer(self.narrow._h, self._h, _dp(rays, "float32", 2 * self.dim), rays.shape[0], float(max_toi),
+                                                 int(crossings), _dp(off, "int64"), _dp(leaf, "int32"), _dp(toi, "float32"), cap, C.byref(total))
         return self._csr(fn, rays.shape[0], rays.device, True, capacity)
 
     def shapecast_nearest(self, casts, max_toi):
@@ -684,20 +747,20 @@ class ConsumerBvh:
         n = casts.shape[0]
         leaf = torch.empty(n, dtype=torch.int32, device=casts.device)
         toi = torch.empty(n, dtype=torch.float32, device=casts.device)
-        check(lib().sq_shapecast_nearest_consumer(self.narrow._h, self._h, _dp(casts), n, float(max_toi), _dp(leaf),
-                                                  _dp(toi)))
+        check(lib().sq_shapecast_nearest_consumer(self.narrow._h, self._h, _dp(casts, "float32", 4 * self.dim), n, float(max_toi), _dp(leaf, "int32"),
+                                                  _dp(toi, "float32")))
         return {"leaf": leaf, "toi": toi}
 
     def shapecast_all(self, casts, max_toi, capacity=0):
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_shapecast_all_consumer(self.narrow._h, self._h, _dp(casts), casts.shape[0], float(max_toi),
-                                                   _dp(off), _dp(leaf), _dp(toi), cap, C.byref(total))
+            return lib().sq_shapecast_all_consumer(self.narrow._h, self._h, _dp(casts, "float32", 4 * self.dim), casts.shape[0], float(max_toi),
+                                                   _dp(off, "int64"), _dp(leaf, "int32"), _dp(toi, "float32"), cap, C.byref(total))
         return self._csr(fn, casts.shape[0], casts.device, True, capacity)
 
     def overlap(self, regions, capacity=0):
         """CSR (offsets, leaf) of ascending leaf ids through sq_narrow_overlap."""
         def fn(off, leaf, toi, cap, total):
-            return lib().sq_overlap_consumer(self.narrow._h, self._h, _dp(regions), regions.shape[0], _dp(off), _dp(leaf),
+            return lib().sq_overlap_consumer(self.narrow._h, self._h, _dp(regions, "float32", 2 * self.dim), regions.shape[0], _dp(off, "int64"), _dp(leaf, "int32"),
                                              cap, C.byref(total))
         off, leaf, _ = self._csr(fn, regions.shape[0], regions.device, False, capacity)
         return off, leaf
@@ -708,10 +771,7 @@ class ConsumerBvh:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class BatchSet:
@@ -737,12 +797,13 @@ class BatchSet:
             l = torch.empty(n, dtype=torch.int32, device=rays.device)
             t = torch.empty(n, dtype=torch.float32, device=rays.device)
             self.leaf.append(l); self.toi.append(t); self._keep.append((scene, rays))
-            arr[i] = BatchQuery(scene._h, _dp(rays).value if n else None, n, float(max_toi),
+            arr[i] = BatchQuery(scene._h, _dp(rays, "float32", 2 * scene.dim).value if n else None, n, float(max_toi),
                                 _dp(l).value if n else None, _dp(t).value if n else None)
         h = C.c_void_p()
         check(lib().sq_batch_create(ctx.handle, SQ_BATCH_TLAS if tlas else SQ_BATCH_BVH, C.cast(arr, C.c_void_p),
                                     len(queries), C.byref(h)))
         self._h, self.ctx = h, ctx
+        _track(self)
 
     def submit(self):
         check(lib().sq_batch_submit(self._h))
@@ -758,10 +819,7 @@ class BatchSet:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 class Proposal:
@@ -779,6 +837,7 @@ class Proposal:
         check(lib().sq_proposal_create(bvh.ctx.handle, bvh._h, int(cut_depth), float(rff_scale),
                                        int(seed) & ((1 << 64) - 1), C.byref(h)))
         self._h, self.bvh, self.ctx, self.cut_depth = h, bvh, bvh.ctx, cut_depth
+        _track(self)
         v = C.c_uint32()
         check(lib().sq_proposal_num_subtrees(self._h, C.byref(v)))
         self.n_subtrees = v.value
@@ -801,19 +860,20 @@ class Proposal:
     def labels(self, leaf):
         import torch
         lab = torch.empty(leaf.shape[0], dtype=torch.int32, device=leaf.device)
-        check(lib().sq_proposal_labels(self._h, _dp(leaf), leaf.shape[0], _dp(lab)))
+        check(lib().sq_proposal_labels(self._h, _dp(leaf, "int32"), leaf.shape[0], _dp(lab)))
         return lab
 
     def train_step(self, rays, labels, lr: float = 1e-3, want_loss: bool = False):
         loss = C.c_float(0.0)
-        check(lib().sq_proposal_train_step(self._h, _dp(rays), _dp(labels), rays.shape[0], float(lr),
+        check(lib().sq_proposal_train_step(self._h, _dp(rays, "float32", 6), _dp(labels, "int32", n=rays.shape[0]), rays.shape[0], float(lr),
                                            C.byref(loss) if want_loss else None, None))
         return loss.value if want_loss else None
 
     def gradients(self, rays, labels):
         """Gradients of the mean cross-entropy, [(dW, db)] per layer (fp32, no update)."""
         flat = np.empty(self.n_params, np.float32)
-        check(lib().sq_proposal_train_step(self._h, _dp(rays), _dp(labels), rays.shape[0], 0.0, None, _hp(flat)))
+        check(lib().sq_proposal_train_step(self._h, _dp(rays, "float32", 6), _dp(labels, "int32", n=rays.shape[0]), rays.shape[0],
+                                           0.0, None, _hp(flat)))
         return self._split(flat)
 
     def weights(self):
@@ -839,12 +899,14 @@ class Proposal:
         import torch
         if out is None:
             out = torch.empty((rays.shape[0], 4), dtype=torch.int16, device=rays.device)
-        check(lib().sq_proposal_infer(self._h, _dp(rays), rays.shape[0], _dp(out)))
+        check(lib().sq_proposal_infer(self._h, _dp(rays, "float32", 6), rays.shape[0], _dp(out, "int16", n=rays.shape[0], per=4)))
         return out
 
     def raycast_nearest_into(self, rays, max_toi, top, leaf, toi, stats=None):
-        check(lib().sq_raycast_nearest_proposed(self._h, _dp(rays), rays.shape[0], float(max_toi), _dp(top),
-                                                _dp(leaf), _dp(toi), _dp(stats)))
+        n = rays.shape[0]
+        check(lib().sq_raycast_nearest_proposed(self._h, _dp(rays, "float32", 6), n, float(max_toi),
+                                                _dp(top, "int16", n=n, per=4), _dp(leaf, "int32", n=n),
+                                                _dp(toi, "float32", n=n), _dp(stats, "int64", n=3)))
 
     def raycast_nearest(self, rays, max_toi, top=None, count=False):
         """Exact nearest hit walking `top` (None = the plain walk) first: dict(leaf, toi[, seeded, nodes, prims])."""
@@ -889,10 +951,7 @@ class Proposal:
             self._h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _finalize(self)
 
 
 def mfma_gemm(ctx: Context, A, B, splits: int = 1):

@@ -10,6 +10,7 @@ _LIB = os.path.join(os.path.dirname(_HERE), "lib", "libsq_amd.so")
 SQ_OK, SQ_ERR_INVALID, SQ_ERR_NO_DEVICE, SQ_ERR_HIP, SQ_ERR_UNSUPPORTED, SQ_ERR_CAPACITY = range(6)
 SQ_PRIM_AABB, SQ_PRIM_SPHERE, SQ_PRIM_TRIANGLE = 0, 1, 2
 SQ_BATCH_BVH, SQ_BATCH_TLAS = 0, 1
+SQ_HITS_REFERENCE, SQ_HITS_GUARDED = 0, 1
 
 
 class SqError(RuntimeError):
@@ -80,6 +81,8 @@ SIGNATURES = {
     "sq_raycast_nearest_proposed": [P, P, U64, F32, P, P, P, P],
     "sq_debug_gemm": [P, P, P, I32, I32, I32, I32, P],
     "sq_ctx_set_tuning": [P, C.POINTER(Tuning)],
+    "sq_ctx_set_hit_mode": [P, I32],
+    "sq_ctx_get_hit_mode": [P],
     "sq_ctx_get_tuning": [P, C.POINTER(Tuning)],
     "sq_ctx_last_kernel_ms": [P, C.POINTER(C.c_float)],
     "sq_tree_build": [I32, I32, P, U64, PP],

@@ -61,7 +61,7 @@ template <class F> void par_for(uint64_t n, int threads, F&& f) {
 
 // The pooled kernel's per-lane walk: walk_begin + walk_step with a ring stack
 // of `ring` LDS slots (here a local array) spilling to a bounds-checked column.
-template <int D, int K, bool QUAD = false, class Pol>
+template <int D, int K, bool QUAD = false, bool REF = false, class Pol>
 NearestResult ring_walk(const Built& b, const Pol& pol, float max_toi, int any, uint32_t ring) {
   constexpr int PS = Prim<D, K>::STRIDE;
   const SceneView s = b.view();
@@ -77,24 +77,30 @@ NearestResult ring_walk(const Built& b, const Pol& pol, float max_toi, int any, 
   if (!walk_begin<D>(s, pol, max_toi, w)) return w.res;
   for (;;) {
     const bool done = any ? walk_step<D, PS, true, false, false, false, QUAD>(s, pol, f, st, w)
-                          : walk_step<D, PS, false, false, false, false, QUAD>(s, pol, f, st, w);
+                          : walk_step<D, PS, false, false, false, false, QUAD, REF>(s, pol, f, st, w);
     if (w.sp > bound) { fprintf(stderr, "stack bound exceeded\n"); abort(); }
     if (done) return w.res;
   }
 }
 
-template <int D, int K>
+// GUARD: the engine's SQ_HITS_GUARDED walk (guarded kinds, near-first); else
+// SQ_HITS_REFERENCE (unguarded, nearest queries over leaves whose hits may
+// leave their box in the reference's order).
+template <int D, int K, bool GUARD>
 void nearest_dk(const Built& b, const float* q, uint64_t nq, float max_toi, int any, int cast, int threads,
                 int32_t* leaf, float* toi, uint32_t ring) {
   constexpr int PS = Prim<D, K>::STRIDE;
+  using RP = RayPolicy<D, K, GUARD>;
+  using CP = CastPolicy<D, K, float, GUARD>;
+  constexpr bool REF_R = !GUARD && !RP::INSIDE, REF_C = !GUARD && !CP::INSIDE;
   const SceneView s = b.view();
   const uint32_t S = b.packed.max_stack;
   Filter f{nullptr, ~0ull, -1};
   if (ring) {
     par_for(nq, threads, [&](uint64_t i) {
       NearestResult h;
-      if (cast) { CastPolicy<D, K> pol; pol.setup(q + i * 4 * D); h = ring_walk<D, K>(b, pol, max_toi, 0, ring); }
-      else { RayPolicy<D, K> pol; pol.setup(q + i * 2 * D); h = ring_walk<D, K>(b, pol, max_toi, any, ring); }
+      if (cast) { CP pol; pol.setup(q + i * 4 * D); h = ring_walk<D, K, false, REF_C>(b, pol, max_toi, 0, ring); }
+      else { RP pol; pol.setup(q + i * 2 * D); h = ring_walk<D, K, false, REF_R>(b, pol, max_toi, any, ring); }
       leaf[i] = h.leaf == NO_LEAF ? -1 : (int32_t)h.leaf;
       toi[i] = h.t;
     });
@@ -106,25 +112,26 @@ void nearest_dk(const Built& b, const float* q, uint64_t nq, float max_toi, int 
     Stack st{sr.data(), stt.data(), 1};
     NearestResult h;
     if (cast) {
-      CastPolicy<D, K> pol; pol.setup(q + i * 4 * D);
-      h = walk_nearest<D, PS, false, false>(s, pol, max_toi, f, st);
+      CP pol; pol.setup(q + i * 4 * D);
+      h = walk_nearest<D, PS, false, false, REF_C>(s, pol, max_toi, f, st);
     } else {
-      RayPolicy<D, K> pol; pol.setup(q + i * 2 * D);
+      RP pol; pol.setup(q + i * 2 * D);
       if (any) h = walk_nearest<D, PS, true, false>(s, pol, max_toi, f, st);
-      else h = walk_nearest<D, PS, false, false>(s, pol, max_toi, f, st);
+      else h = walk_nearest<D, PS, false, false, REF_R>(s, pol, max_toi, f, st);
     }
     leaf[i] = h.leaf == NO_LEAF ? -1 : (int32_t)h.leaf;
     toi[i] = h.t;
   });
 }
 
-// The 4-wide walk (walk_step with QUAD) of every ray, ring stack as above.
+// The 4-wide walk (walk_step with QUAD) of every ray over the guarded kinds,
+// ring stack as above.
 template <int D, int K>
 void nearest_quad_dk(Built& b, const float* q, uint64_t nq, float max_toi, int any, int threads, int32_t* leaf, float* toi,
                      uint32_t ring) {
   if (b.quads.empty()) build_quads<D>(b);
   par_for(nq, threads, [&](uint64_t i) {
-    RayPolicy<D, K> pol; pol.setup(q + i * 2 * D);
+    RayPolicy<D, K, true> pol; pol.setup(q + i * 2 * D);
     NearestResult h = ring_walk<D, K, true>(b, pol, max_toi, any, ring);
     leaf[i] = h.leaf == NO_LEAF ? -1 : (int32_t)h.leaf;
     toi[i] = h.t;
@@ -169,10 +176,13 @@ uint32_t emu_max_stack(void* h) { return ((Built*)h)->packed.max_stack; }
 uint32_t emu_n_pairs(void* h) { return ((Built*)h)->packed.n_pairs; }
 
 int emu_nearest(void* h, int dim, int kind, const float* q, uint64_t nq, float max_toi, int any, int cast, int threads,
-                int32_t* leaf, float* toi, uint32_t ring) {
+                int32_t* leaf, float* toi, uint32_t ring, int guarded) {
   const Built& b = *(Built*)h;
   if (b.tree.prim_indices.empty()) { for (uint64_t i = 0; i < nq; ++i) { leaf[i] = -1; toi[i] = SQ_INF; } return 0; }
-  return dk(dim, kind, [&]<int D, int K>() { nearest_dk<D, K>(b, q, nq, max_toi, any, cast, threads, leaf, toi, ring); });
+  return dk(dim, kind, [&]<int D, int K>() {
+    if (guarded) nearest_dk<D, K, true>(b, q, nq, max_toi, any, cast, threads, leaf, toi, ring);
+    else nearest_dk<D, K, false>(b, q, nq, max_toi, any, cast, threads, leaf, toi, ring);
+  });
 }
 
 // The 4-wide walk (tuning `wide`) with a ring stack of `ring` slots.
@@ -207,6 +217,7 @@ uint64_t emu_collect(void* h, int dim, int kind, const float* q, uint64_t nq, fl
         if (box_overlap<D>(s.root, s.root + D, rg, rg + D))
           walk_visit<D>(s, root_ref,
             [&](const float* nd, bool& ha, bool& hb) {
+              if (K != KIND_AABB && synthetic_pair<D>(nd)) { ha = hb = true; return; }
               float amn[D], amx[D], bmn[D], bmx[D];
               for (int k = 0; k < D; ++k) { amn[k] = nd[4 * k]; amx[k] = nd[4 * k + 1]; bmn[k] = nd[4 * k + 2]; bmx[k] = nd[4 * k + 3]; }
               ha = box_overlap<D>(amn, amx, rg, rg + D); hb = box_overlap<D>(bmn, bmx, rg, rg + D);
@@ -219,7 +230,11 @@ uint64_t emu_collect(void* h, int dim, int kind, const float* q, uint64_t nq, fl
         auto run = [&](auto& pol) {
           if (!pol.root(s.root, max_toi)) return;
           walk_visit<D>(s, root_ref,
-            [&](const float* nd, bool& ha, bool& hb) { float ta, tb; pol.pair(nd, max_toi, ha, hb, ta, tb); },
+            [&](const float* nd, bool& ha, bool& hb) {
+              if (K != KIND_AABB && synthetic_pair<D>(nd)) { ha = hb = true; return; }
+              float ta, tb;
+              pol.pair(nd, max_toi, ha, hb, ta, tb);
+            },
             [&](uint32_t first, uint32_t c) {
               for (uint32_t j = 0; j < c; ++j) {
                 const float* p = s.prims + (size_t)(first + j) * PS;

@@ -24,7 +24,8 @@ def emu():
     L = C.CDLL(LIB)
     L.emu_build.restype = P; L.emu_build.argtypes = [C.c_int, C.c_int, P, C.c_uint32]
     L.emu_free.argtypes = [P]
-    L.emu_nearest.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_int, C.c_int, P, P, C.c_uint32]
+    L.emu_nearest.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_int, C.c_int, P, P, C.c_uint32,
+                              C.c_int]
     L.emu_collect.restype = C.c_uint64
     L.emu_collect.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, P, P]
     L.emu_required.argtypes = [P, C.c_int, P, C.c_uint64, P, P, P]
@@ -42,10 +43,11 @@ class Emu:
         self.prims = np.ascontiguousarray(prims, np.float32)
         self.h = L.emu_build(dim, kind, p(self.prims), self.prims.shape[0])
 
-    def nearest(self, q, max_toi, any_=0, cast=0, ring=0):
+    def nearest(self, q, max_toi, any_=0, cast=0, ring=0, guarded=False):
         q = np.ascontiguousarray(q, np.float32)
         leaf = np.empty(len(q), np.int32); t = np.empty(len(q), np.float32)
-        self.L.emu_nearest(self.h, self.dim, self.kind, p(q), len(q), max_toi, any_, cast, 8, p(leaf), p(t), ring)
+        self.L.emu_nearest(self.h, self.dim, self.kind, p(q), len(q), max_toi, any_, cast, 8, p(leaf), p(t), ring,
+                           int(guarded))
         return leaf.astype(np.int64), t
 
     def nearest_quad(self, q, max_toi, any_=0, ring=8):
@@ -165,12 +167,13 @@ def test_ring_stack_walk(emu, ring):
 @pytest.mark.parametrize("ring", [2, 8])
 def test_quad_walk(emu, ring):
     """The 4-wide walk (tuning `wide`: each pair node with its children's pair
-    nodes) equals the oracle: heightfield triangles with coherent and
+    nodes; near-first, so for balls and triangles it runs under
+    SQ_HITS_GUARDED) equals the oracle: heightfield triangles with coherent and
     incoherent rays, coincident boxes (leaf chains), balls in D = 2 and 5 and
     boxes in D = 6, nearest and any-hit, with rings that spill."""
     sc = W.heightfield_scene(grid=257, extent=128.0)
     rays = np.concatenate([W.camera_rays(sc, res=128, seed=0x12), W.incoherent_rays(sc, 30000)])
-    ref = oracle.Scene(3, oracle.TRI, sc.prims).raycast_nearest(rays, 2000.0, threads=8)
+    ref = oracle.Scene(3, oracle.TRI, sc.prims, guarded=True).raycast_nearest(rays, 2000.0, threads=8)
     e = Emu(emu, 3, 2, sc.prims)
     assert eq(*e.nearest_quad(rays, 2000.0, ring=ring), ref["leaf"], ref["t"])
     assert np.array_equal(e.nearest_quad(rays, 2000.0, any_=1, ring=ring)[0] >= 0, ref["leaf"] >= 0)
@@ -184,7 +187,7 @@ def test_quad_walk(emu, ring):
         b = np.concatenate([s.range(-half, half, 4000 * dim).reshape(-1, dim), s.range(0.3, 3.0, 4000).reshape(-1, 1)],
                            1).astype(np.float32)
         qq = W.random_rays(4000, 0x15 + dim, -half - 5.0, half + 5.0, dim=dim)
-        rb = oracle.Scene(dim, oracle.SPHERE, b).raycast_nearest(qq, 300.0)
+        rb = oracle.Scene(dim, oracle.SPHERE, b, guarded=True).raycast_nearest(qq, 300.0)
         assert (rb["leaf"] >= 0).sum() > 100
         assert eq(*Emu(emu, dim, 1, b).nearest_quad(qq, 300.0, ring=ring), rb["leaf"], rb["t"])
     sc6 = W.boxes_scene(n=3000, dim=6, lo=-7.0, hi=7.0, hmin=0.3, hmax=2.0)
@@ -194,3 +197,29 @@ def test_quad_walk(emu, ring):
     one = Emu(emu, 3, 0, W.boxes_scene(n=1, seed=1).prims)
     r1 = oracle.Scene(3, oracle.BOX, W.boxes_scene(n=1, seed=1).prims).raycast_nearest(q, 500.0)
     assert eq(*one.nearest_quad(q, 500.0, ring=ring), r1["leaf"], r1["t"])
+
+
+@pytest.mark.parametrize("ring", [0, 2, 8])
+def test_hit_modes(emu, ring):
+    """Both hit modes of the walk against the oracle on scenes where they
+    differ: SQ_HITS_REFERENCE (plain narrow phases, the reference's order)
+    equals the oracle's plain providers, and SQ_HITS_GUARDED (guarded narrow
+    phases, near-first) the guarded ones, for balls in D = 2 (where a hit
+    outside its box's slab interval decides some rays) and the heightfield;
+    any-hit is the plain existence of a hit in the reference mode."""
+    s = W.Stream(0x16)
+    b = np.concatenate([s.range(-40.0, 40.0, 4000 * 2).reshape(-1, 2), s.range(0.3, 3.0, 4000).reshape(-1, 1)],
+                       1).astype(np.float32)
+    q = W.random_rays(4000, 0x17, -45.0, 45.0, dim=2)
+    sc = W.heightfield_scene(grid=257, extent=128.0)
+    rays = np.concatenate([W.camera_rays(sc, res=128, seed=0x12), W.incoherent_rays(sc, 30000)])
+    differ = 0
+    for dim, kind, ok, prims, qq, tmax in ((2, 1, oracle.SPHERE, b, q, 300.0), (3, 2, oracle.TRI, sc.prims, rays, 2000.0)):
+        e = Emu(emu, dim, kind, prims)
+        plain = oracle.Scene(dim, ok, prims).raycast_nearest(qq, tmax, threads=8)
+        guarded = oracle.Scene(dim, ok, prims, guarded=True).raycast_nearest(qq, tmax, threads=8)
+        assert eq(*e.nearest(qq, tmax, ring=ring), plain["leaf"], plain["t"])
+        assert eq(*e.nearest(qq, tmax, ring=ring, guarded=True), guarded["leaf"], guarded["t"])
+        assert np.array_equal(e.nearest(qq, tmax, any_=1, ring=ring)[0] >= 0, plain["leaf"] >= 0)
+        differ += int((plain["leaf"] != guarded["leaf"]).sum())
+    assert differ > 0, "the scenes should hold rays where the two modes differ"

@@ -68,6 +68,43 @@ def test_consumer_ball_equals_builtin_sphere(ctx, tmp_path, dim, half):
     nw.close()
 
 
+def test_consumer_hits_outside_their_boxes(ctx, tmp_path):
+    """A provider whose hits leave their own boxes: balls whose world boxes are
+    shrunk to 0.6 of the radius (the reference asks only that the ray meet the
+    leaf box, geometry.rs:37-45).  Here the walk order decides many answers;
+    the default mode returns the reference's LIFO walk's answer bit for bit
+    (oracle consumer kind, same source built by g++), which differs from the
+    naive scan and from the guarded answer on many rays; SQ_HITS_GUARDED
+    returns the guarded oracle's."""
+    print("case: consumer hits outside their boxes, both modes", flush=True)
+    dim = 3
+    b = balls(20000, dim, 0xD8, 40.0)
+    shrink = b[:, dim:] * np.float32(0.6)
+    boxes = np.concatenate([b[:, :dim] - shrink, b[:, :dim] + shrink], 1).astype(np.float32)
+    rays = W.random_rays(30000, 0xD9, -45.0, 45.0, dim=dim)
+    nw = sq_amd.Narrow(ctx, dim, BALL, dim + 1)
+    cb = nw.build(boxes, b)
+    lib, addr = compile_cpu(BALL, dim, dim + 1, str(tmp_path))
+    sc = oracle.Scene.consumer(dim, boxes, b, addr)
+    ref = sc.raycast_nearest(rays, 300.0, threads=8)
+    naive = sc.raycast_nearest(rays, 300.0, naive=True)
+    guarded = oracle.Scene.consumer(dim, boxes, b, addr, guarded=True).raycast_nearest(rays, 300.0, threads=8)
+    got = cb.raycast_nearest(dev(rays), 300.0)
+    assert_nearest_equal(host(got["leaf"]), host(got["toi"]), ref["leaf"], ref["t"], "consumer, reference order")
+    order = int((ref["leaf"] != naive["leaf"]).sum())
+    gdiff = int((ref["leaf"] != guarded["leaf"]).sum())
+    print(f"rays where order decides (reference walk != naive scan): {order}; guarded differs on {gdiff}", flush=True)
+    assert order > 10 and gdiff > 10
+    try:
+        ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED)
+        g = cb.raycast_nearest(dev(rays), 300.0)
+        assert_nearest_equal(host(g["leaf"]), host(g["toi"]), guarded["leaf"], guarded["t"], "consumer, guarded mode")
+    finally:
+        ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
+    cb.close()
+    nw.close()
+
+
 def test_consumer_ellipsoid_matches_oracle(ctx, tmp_path):
     print("case: consumer ellipsoid", flush=True)
     s = W.Stream(0xD5)

@@ -279,3 +279,75 @@ def test_f64_box_instances_match_oracle(ctx, dim):
         assert st[k] == rs[k], k
     assert st["quality"] == float(np.float32(rs["quality"]))
     tl.close()
+
+
+@pytest.mark.parametrize("dim,scene", [(3, "identity"), (2, "identity"), (3, "plane")])
+def test_f64_box_instances_on_their_world_faces(ctx, dim, scene):
+    """f64 box instances whose faces lie on their world boxes (identity
+    rotations) or that turn in one plane, non-dyadic centres and extents: the
+    default mode equals the reference Tlas walk (unguarded instance test, the
+    reference's order over the same f64 top tree) bit for bit; SQ_HITS_GUARDED
+    equals the guarded oracle; the guard changes the answer on some rays."""
+    print(f"case: f64 box instances D={dim}, {scene}, both hit modes", flush=True)
+    rng = np.random.default_rng(0xFACE + dim)
+    n, span = 2000, 60.0
+    trans = rng.uniform(-span, span, (n, dim))
+    halves = rng.uniform(0.3, 2.7, (n, dim))
+    rot = np.broadcast_to(np.eye(dim), (n, dim, dim)).copy()
+    if scene == "plane":
+        ang = rng.uniform(0, 2 * np.pi, n)
+        rot[:, 0, 0] = np.cos(ang); rot[:, 0, 1] = -np.sin(ang); rot[:, 1, 0] = np.sin(ang); rot[:, 1, 1] = np.cos(ang)
+    o = rng.uniform(-span - 10, span + 10, (20000, dim))
+    d = rng.normal(size=(20000, dim))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1)
+    tl = sq_amd.Tlas64(ctx, dim, rot, trans, halves)
+    plain = oracle.BoxInstances64(dim, rot, trans, halves).raycast_nearest(rays, 500.0)
+    guarded = oracle.BoxInstances64(dim, rot, trans, halves, guarded=True).raycast_nearest(rays, 500.0)
+    assert (plain["leaf"] >= 0).sum() > 2000
+    got = tl.raycast_nearest(dev(rays), 500.0)
+    assert np.array_equal(host(got["leaf"]).astype(np.int64), plain["leaf"])
+    assert np.array_equal(host(got["toi"]).view(np.uint64), plain["t"].view(np.uint64))
+    assert np.array_equal(host(tl.raycast_any(dev(rays), 500.0)).astype(bool), plain["leaf"] >= 0)
+    try:
+        ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED)
+        g = tl.raycast_nearest(dev(rays), 500.0)
+        assert np.array_equal(host(g["leaf"]).astype(np.int64), guarded["leaf"])
+        assert np.array_equal(host(g["toi"]).view(np.uint64), guarded["t"].view(np.uint64))
+    finally:
+        ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
+    differ = int((plain["leaf"] != guarded["leaf"]).sum())
+    print(f"rays whose reference answer the guard changes: {differ} of {len(rays)}", flush=True)
+    assert differ > 0
+    tl.close()
+
+
+def test_f64_balls_both_hit_modes(ctx):
+    """f64 balls in D = 2 (hits near box faces decide some rays): the default
+    mode equals the reference's Bvh walk over the f64 tree, SQ_HITS_GUARDED the
+    guarded oracle; nearest casts too."""
+    print("case: f64 balls D=2, both hit modes", flush=True)
+    rng = np.random.default_rng(0xBA2)
+    prims = np.concatenate([rng.uniform(-40, 40, (8000, 2)), rng.uniform(0.3, 3.0, (8000, 1))], 1)
+    o = rng.uniform(-45, 45, (40000, 2))
+    d = rng.normal(size=(40000, 2))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1)
+    b = sq_amd.DeviceBvh64(ctx, 2, sq_amd.SPHERE, prims)
+    casts = np.concatenate([rays[:5000], np.full((5000, 2), -0.25), np.full((5000, 2), 0.25)], 1)
+    for guarded in (False, True):
+        ref = oracle.Scene64(2, sq_amd.SPHERE, prims, guarded=guarded)
+        want = ref.raycast_nearest(rays, 300.0)
+        wc = ref.shapecast_nearest(casts, 300.0)
+        try:
+            if guarded:
+                ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED)
+            got = b.raycast_nearest(dev(rays), 300.0)
+            gc = b.shapecast_nearest(dev(casts), 300.0)
+        finally:
+            ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
+        assert np.array_equal(host(got["leaf"]).astype(np.int64), want["leaf"]), f"guarded={guarded}"
+        assert np.array_equal(host(got["toi"]).view(np.uint64), want["t"].view(np.uint64))
+        assert np.array_equal(host(gc["leaf"]).astype(np.int64), wc["leaf"]), f"casts guarded={guarded}"
+        assert np.array_equal(host(gc["toi"]).view(np.uint64), wc["t"].view(np.uint64))
+    b.close()

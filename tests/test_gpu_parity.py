@@ -90,18 +90,101 @@ def test_heightfield_small_nearest_any(ctx):
 
 
 def test_heightfield_full_scale_c1(ctx):
-    """The 2,097,152-triangle scene with all 8M rays: 100% id agreement."""
+    """The 2,097,152-triangle scene with all 8M rays: 100% id and toi agreement
+    with the reference walk (plain Moeller-Trumbore, the reference's order);
+    SQ_HITS_GUARDED agrees with the guarded oracle on every ray too, and the
+    rays where the two modes differ are counted (the guard's cost in
+    exactness on this workload)."""
     print("case: C1 full heightfield, 4M coherent + 4M incoherent rays", flush=True)
     sc, coh, inc = W.c1()
     rays = np.concatenate([coh, inc])
     b = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.TRIANGLE, sc.prims)
     r = b.raycast_nearest(dev(rays), 2000.0)
     anyh = b.raycast_any(dev(rays), 2000.0)
+    try:
+        ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED)
+        g = b.raycast_nearest(dev(rays), 2000.0)
+    finally:
+        ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
     ctx.synchronize()
-    o = oracle.Scene(3, oracle.TRI, sc.prims)
-    ref = o.raycast_nearest(rays, 2000.0, threads=16)
+    ref = oracle.Scene(3, oracle.TRI, sc.prims).raycast_nearest(rays, 2000.0, threads=16)
     assert_nearest_equal(host(r["leaf"]), host(r["toi"]), ref["leaf"], ref["t"], "C1 nearest")
     assert np.array_equal(host(anyh).astype(bool), ref["leaf"] >= 0)
+    gref = oracle.Scene(3, oracle.TRI, sc.prims, guarded=True).raycast_nearest(rays, 2000.0, threads=16)
+    assert_nearest_equal(host(g["leaf"]), host(g["toi"]), gref["leaf"], gref["t"], "C1 nearest, guarded mode")
+    differ = int(((gref["leaf"] != ref["leaf"]) | ~same_bits(gref["t"], ref["t"])).sum())
+    print(f"C1 rays whose answer the guard changes: {differ} of {len(rays)}", flush=True)
+    assert differ < 1000
+
+
+@pytest.mark.parametrize("dim,seed,n,span", [(3, 0xABCD_1234, 64, 60.0), (2, 0x2222_2222, 48, 60.0),
+                                             (4, 0x4444_4444, 48, 40.0)])
+def test_differential_scenes_match_plain_balls(ctx, dim, seed, n, span):
+    """The 200 random ball scenes per dimension of differential.rs:120-162
+    (same SplitMix64 draws, the reference's own ray of each scene plus 255
+    more per scene from another stream): the device's default mode equals the
+    oracle's plain ball provider (the differential.rs provider,
+    differential.rs:61-82) walked in the reference's order, bit for bit."""
+    print(f"case: differential.rs scenes D={dim}", flush=True)
+    s = W.Stream(seed)
+    extra = W.Stream(seed ^ 0x5EED)
+    hits = 0
+    for k in range(200):
+        c = s.range(-50.0, 50.0, n * dim).reshape(n, dim)
+        r = s.range(0.5, 4.0, n)
+        o = s.range(-span, span, dim)
+        d = s.range(-1.0, 1.0, dim)
+        d = W.normalize_or_zero(d[None])[0]
+        prims = np.concatenate([c, r[:, None]], 1).astype(np.float32)
+        more = np.concatenate([extra.range(-span, span, 255 * dim).reshape(255, dim),
+                               W.normalize_or_zero(extra.range(-1.0, 1.0, 255 * dim).reshape(255, dim))], 1)
+        rays = np.concatenate([np.concatenate([o, d])[None], more]).astype(np.float32)
+        rays = rays[np.any(rays[:, dim:] != 0, axis=1)]
+        ref = oracle.Scene(dim, oracle.SPHERE_PLAIN, prims).raycast_nearest(rays, 500.0)
+        b = sq_amd.DeviceBvh.build(ctx, dim, sq_amd.SPHERE, prims)
+        got = b.raycast_nearest(dev(rays), 500.0, normal=True)
+        assert_nearest_equal(host(got["leaf"]), host(got["toi"]), ref["leaf"], ref["t"], f"differential D{dim} scene {k}")
+        hit = ref["leaf"] >= 0
+        assert same_bits(host(got["normal"])[hit], ref["normal"][hit]).all()
+        hits += int(hit.sum())
+        b.close()
+    assert hits > (50 if dim == 4 else 1000)   # the reference's D = 4 draws are sparse
+
+
+def test_hit_modes_heightfield_and_balls(ctx):
+    """Both hit modes against the oracle on scenes where they differ (the
+    257^2 heightfield with camera and incoherent rays; balls in D = 2):
+    SQ_HITS_REFERENCE equals the plain providers walked in the reference's
+    order, SQ_HITS_GUARDED the guarded providers; nearest, any-hit, and all
+    hits (the same in both modes)."""
+    print("case: hit modes, heightfield 257^2 and D=2 balls", flush=True)
+    sc = W.heightfield_scene(grid=257, extent=128.0)
+    rays_h = np.concatenate([W.camera_rays(sc, res=256, seed=0x11), W.incoherent_rays(sc, 65536)])
+    s = W.Stream(0x16)
+    balls2 = np.concatenate([s.range(-40.0, 40.0, 8000 * 2).reshape(-1, 2), s.range(0.3, 3.0, 8000).reshape(-1, 1)],
+                            1).astype(np.float32)
+    rays_b = W.random_rays(60000, 0x17, -45.0, 45.0, dim=2)
+    differ = 0
+    for dim, kind, okind, prims, rays, tmax in ((3, sq_amd.TRIANGLE, oracle.TRI, sc.prims, rays_h, 2000.0),
+                                                (2, sq_amd.SPHERE, oracle.SPHERE, balls2, rays_b, 300.0)):
+        b = sq_amd.DeviceBvh.build(ctx, dim, kind, prims)
+        plain = oracle.Scene(dim, okind, prims).raycast_nearest(rays, tmax, threads=8)
+        guarded = oracle.Scene(dim, okind, prims, guarded=True).raycast_nearest(rays, tmax, threads=8)
+        got = b.raycast_nearest(dev(rays), tmax)
+        assert_nearest_equal(host(got["leaf"]), host(got["toi"]), plain["leaf"], plain["t"], f"D{dim} reference mode")
+        assert np.array_equal(host(b.raycast_any(dev(rays), tmax)).astype(bool), plain["leaf"] >= 0)
+        off, leaf, toi = b.raycast_all(dev(rays[:5000]), tmax)
+        roff, rleaf, rt = oracle.Scene(dim, okind, prims).raycast_all(rays[:5000], tmax)
+        csr_equal(host(off), host(leaf), host(toi), roff, rleaf, rt, f"D{dim} all hits")
+        try:
+            ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED)
+            g = b.raycast_nearest(dev(rays), tmax)
+            assert_nearest_equal(host(g["leaf"]), host(g["toi"]), guarded["leaf"], guarded["t"], f"D{dim} guarded mode")
+        finally:
+            ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
+        differ += int((plain["leaf"] != guarded["leaf"]).sum())
+        b.close()
+    assert differ > 0
 
 
 def test_per_ray_max_toi_and_filter(ctx):

@@ -6,7 +6,9 @@
     rounding points; tolerance 1e-2 relative per tensor, written below);
   * fused top-4 inference against the oracle's logits;
   * the proposal-seeded walk bit-exact against the plain walk and the CPU
-    oracle whatever the network predicts (untrained, random ids, empty slots);
+    oracle whatever the network predicts (untrained, random ids, empty slots),
+    under SQ_HITS_GUARDED, the mode whose answer does not depend on the order
+    in which subtrees are walked (this module runs its walks in it);
   * what a trained network gains over an untrained one.
 """
 import numpy as np
@@ -30,6 +32,13 @@ def dev(a, dt=np.float32):
 def rel(a, b):
     a = np.asarray(a, np.float64); b = np.asarray(b, np.float64)
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def guarded_mode(ctx):
+    ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED)
+    yield
+    ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
 
 
 @pytest.fixture(scope="module")
@@ -166,7 +175,7 @@ def test_proposed_walk_is_exact(ctx, small, mode):
     assert np.array_equal(gl, plain["leaf"].cpu().numpy())
     assert np.array_equal(gt.view(np.uint32), plain["toi"].cpu().numpy().view(np.uint32))
     if mode in ("random_ids", "short_tmax"):
-        ref = oracle.Scene(3, oracle.TRI, sc.prims).raycast_nearest(rays, tmax, threads=8)
+        ref = oracle.Scene(3, oracle.TRI, sc.prims, guarded=True).raycast_nearest(rays, tmax, threads=8)
         assert_nearest_equal(gl, gt, ref["leaf"], ref["t"], f"proposed walk ({mode})")
     if mode in ("empty", "none"):
         assert got["seeded"] == 0

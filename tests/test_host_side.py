@@ -111,3 +111,97 @@ def test_workload_streams_are_the_reference_rng():
     assert np.all(np.abs(n2 - 1) < 1e-6)
     # uniform on the sphere: mean ~ 0, second moments ~ 1/3
     assert np.all(np.abs(d.mean(0)) < 0.03) and np.all(np.abs((d ** 2).mean(0) - 1 / 3) < 0.02)
+
+
+class _FakeTensor:
+    """Stands in for a CUDA tensor in the argument checks (no GPU here)."""
+
+    def __init__(self, dtype, shape):
+        self.dtype, self.shape, self.is_cuda = dtype, tuple(shape), True
+
+    def is_contiguous(self):
+        return True
+
+    def dim(self):
+        return len(self.shape)
+
+    def numel(self):
+        return int(np.prod(self.shape))
+
+    def data_ptr(self):
+        return 0x1000
+
+
+def test_device_pointer_checks_refuse_mismatched_buffers():
+    """A float32 ray array handed to the f64 entries (or a float64 one to the
+    FP32 entries), a row length that is not 2D / 4D, or an output shorter than
+    the queries is refused before any C call, so no kernel reads or writes past
+    a buffer."""
+    import torch
+    from sq_amd.api import _dp
+    rays32 = _FakeTensor(torch.float32, (100, 6))
+    rays64 = _FakeTensor(torch.float64, (100, 6))
+    assert _dp(rays32, "float32", 6).value == 0x1000
+    assert _dp(rays64, "float64", 6).value == 0x1000
+    with pytest.raises(ValueError, match="float64"):
+        _dp(rays32, "float64", 6)
+    with pytest.raises(ValueError, match="float32"):
+        _dp(rays64, "float32", 6)
+    with pytest.raises(ValueError, match=r"\[n\]\[12\]"):
+        _dp(rays32, "float32", 12)               # a cast array needs 4D floats per row
+    with pytest.raises(ValueError, match="at least 100"):
+        _dp(_FakeTensor(torch.float32, (99,)), "float32", n=100)
+    with pytest.raises(ValueError, match="at least 300"):
+        _dp(_FakeTensor(torch.float32, (100, 2)), "float32", n=100, per=3)
+    assert _dp(None, "float32", 6) is None
+    with pytest.raises(ValueError, match="GPU"):
+        _dp(torch.zeros(3), "float32")
+
+
+def test_every_query_wrapper_checks_its_buffers():
+    """Every device buffer a query wrapper hands to the C ABI goes through the
+    dtype check (the source of sq_amd/api.py, read as text)."""
+    src = open(os.path.join(ROOT, "spatial-query_amd", "sq_amd", "api.py")).read()
+    calls = re.findall(r"_dp\(([a-z_]+)\)", src)
+    # the only unchecked ones: BatchSet's own outputs and the debug GEMM's operands
+    assert set(calls) <= {"l", "t", "lab", "A", "B", "Cm"}, sorted(set(calls))
+    for name in ("rays", "casts", "regions", "max_toi_per_ray", "max_toi_per_cast"):
+        for m in re.finditer(r"_dp\(%s, \"(float32|float64)\"" % name, src):
+            assert m.group(1) in ("float32", "float64")
+
+
+def test_teardown_closes_live_objects_newest_first(monkeypatch):
+    """The exit hook closes every live engine object, newest first (trees
+    before the contexts they hold), once each; __del__ closes normally but
+    does nothing while the interpreter is finalizing."""
+    from sq_amd import api
+    closed = []
+
+    class Obj:
+        def __init__(self, name):
+            self.name, self.open = name, True
+
+        def close(self):
+            if self.open:
+                closed.append(self.name)
+                self.open = False
+
+    objs = [api._track(Obj(n)) for n in ("ctx", "tree", "bvh")]
+    api._close_all()
+    assert closed == ["bvh", "tree", "ctx"]
+    api._close_all()
+    assert closed == ["bvh", "tree", "ctx"]
+    late = Obj("late")
+    monkeypatch.setattr(api.sys, "is_finalizing", lambda: True)
+    api._finalize(late)
+    assert late.open
+    monkeypatch.setattr(api.sys, "is_finalizing", lambda: False)
+    api._finalize(late)
+    assert not late.open and closed[-1] == "late"
+    del objs
+
+
+def test_hit_mode_constants_match_the_header():
+    src = open(os.path.join(ROOT, "include", "sq.h")).read()
+    assert re.search(r"#define SQ_HITS_REFERENCE 0\b", src) and re.search(r"#define SQ_HITS_GUARDED 1\b", src)
+    assert (sq_amd.SQ_HITS_REFERENCE, sq_amd.SQ_HITS_GUARDED) == (0, 1)

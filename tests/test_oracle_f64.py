@@ -158,19 +158,27 @@ def test_all_hits_sorted_and_headed_by_the_nearest(kind):
     else:
         prims = np.concatenate([rng.uniform(-8, 8, (800, dim)), rng.uniform(0.5, 2.0, (800, 1))], 1)
     rays = np.concatenate([rng.uniform(-8, 8, (700, dim)), rng.normal(size=(700, dim))], 1)
-    s = oracle.Scene64(dim, kind, prims)
-    off, leaf, t = s.raycast_all(rays, 80.0)
-    w = s.raycast_nearest(rays, 80.0)
-    off = off.astype(np.int64)
-    assert off[-1] > 300
-    for i in range(700):
-        a, b = off[i], off[i + 1]
-        if a == b:
-            assert w["leaf"][i] == -1
-            continue
-        keys = list(zip(t[a:b], leaf[a:b]))
-        assert keys == sorted(keys)
-        assert leaf[a] == w["leaf"][i] and t[a] == w["t"][i]
+    # guarded leaf tests: the nearest hit is the first of all hits on every ray;
+    # the reference's (plain) walk can end on a later one, never an earlier one
+    # (a hit outside its ball's box may be pruned once a hit below that box's
+    # entry is known), and both find a hit on the same rays
+    for guarded in (True, False):
+        s = oracle.Scene64(dim, kind, prims, guarded=guarded)
+        off, leaf, t = s.raycast_all(rays, 80.0)
+        w = s.raycast_nearest(rays, 80.0)
+        off = off.astype(np.int64)
+        assert off[-1] > 300
+        for i in range(700):
+            a, b = off[i], off[i + 1]
+            if a == b:
+                assert w["leaf"][i] == -1
+                continue
+            keys = list(zip(t[a:b], leaf[a:b]))
+            assert keys == sorted(keys)
+            if guarded or kind == oracle.BOX:
+                assert leaf[a] == w["leaf"][i] and t[a] == w["t"][i]
+            else:
+                assert w["leaf"][i] >= 0 and (t[a], leaf[a]) <= (w["t"][i], w["leaf"][i])
 
 
 def test_shape_cast_closed_forms_and_identities():
