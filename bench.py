@@ -2,10 +2,13 @@
 
 One step = one batched nearest-hit pass of all 8,388,608 rays (4M coherent
 camera rays + 4M incoherent rays, t_max 2000) through the HIP engine, rays and
-scene resident in HBM.  `value` is the plain pooled walk; the learned-proposal
-leg (train the MLP on 16M fresh rays, then fused inference + seeded exact walk
-on the same 8M rays) and the CPU baseline are reported beside it.  Prints one
-JSON line.
+scene resident in HBM.  `value` is the plain pooled walk in the default hit
+mode (SQ_HITS_REFERENCE: the reference's hits on every ray, its walk order).
+Reported beside it: the guarded near-first walk (SQ_HITS_GUARDED) with the
+number of rays where its answer differs from the reference's, the
+learned-proposal leg (train the MLP on 16M fresh rays, then fused inference +
+seeded walk on the same 8M rays; the seeded walk answers in the guarded mode)
+and the CPU baseline (all host threads, and one).  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-sample N] [--no-proposal]
 
@@ -134,13 +137,17 @@ def main():
     n_prim = prims.double().sum().item()
     bytes_launch = n * (24 + 8) + 32 * n_node + 40 * n_prim
     achieved = bytes_launch / (k_ms * 1e-3) / 1e9
-    # HBM traffic per launch, from the FETCH_SIZE / WRITE_SIZE profile of this kernel
-    # (profiles/k_nearest_c1_heightfield.md), scaled by the copy-kernel calibration.
-    traffic = None
+    # HBM traffic per launch and the VALU figures of the timed kernel, from its
+    # rocprofv3 counter runs (profiles/k_nearest_pool_c1_heightfield.md): fetch
+    # and write bytes scaled by the copy-kernel calibration.
+    traffic, valu = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_c1_nearest.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = int(json.load(fh)["traffic_bytes"])
+            prof = json.load(fh)
+        if prof.get("hit_mode", "reference") == args.hit_mode:
+            traffic = int(prof["traffic_bytes"])
+            valu = prof.get("valu")
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -154,18 +161,23 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 heightfield and rays)",
-        "method": "plain pooled near-first traversal",
+        "method": ("plain pooled walk, the reference's order (exact: the reference's hit on every ray)"
+                   if args.hit_mode == "reference" else "plain pooled near-first walk over guarded hits"),
         "config": {"workload": "C1 heightfield: 1025x1025 grid, 2,097,152 triangles; 4,194,304 coherent + "
                                "4,194,304 incoherent rays; nearest-hit, t_max 2000",
                    "rays_per_step": n, "tree_nodes": st["node_count"], "tree_depth": st["max_depth"],
-                   "tuning": ctx.tuning()},
+                   "hit_mode": args.hit_mode, "tuning": ctx.tuning()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel_ms": round(k_ms, 4), "algorithmic_bytes_per_launch": int(bytes_launch),
-                     "hbm_traffic_gbs": round(traffic / (k_ms * 1e-3) / 1e9, 1) if traffic else None,
-                     "note": "achieved counts every node and triangle the minimal walk tests as a read; the 135 MB "
-                             "scene stays in L2 and the Infinity Cache, so it can exceed the HBM peak; "
-                             "hbm_traffic_gbs is the measured fabric traffic over the same time",
+                     "hbm_measured_gbs": round(traffic / (k_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "hbm_measured_frac": round(traffic / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                     "valu": valu,
+                     "note": "achieved: the algorithmic bytes (every node box and triangle of the order-independent "
+                             "minimal walk read once, rays in, results out) over the kernel time; "
+                             "hbm_measured: FETCH_SIZE + WRITE_SIZE bytes of the same kernel over the same time "
+                             "(the scene is mostly served from L2 and the Infinity Cache); valu: the share of SIMD "
+                             "cycles issuing VALU and the active-lane share of those issues (the walk's limit)",
                      "mean_nodes_per_ray": round(n_node / n, 2), "mean_prims_per_ray": round(n_prim / n, 2)},
     }
 
@@ -188,33 +200,59 @@ def main():
             ts.append(ctx.last_kernel_ms())
         out["anyhit_mrays"] = round(n / (min(ts) * 1e-3) / 1e6, 1)
         out["hit_fraction"] = round(float((leaf >= 0).float().mean().item()), 4)
+        # the other hit mode on the same rays: its speed, and the rays whose
+        # answer (id or toi bits) differs from this run's
+        other = "guarded" if args.hit_mode == "reference" else "reference"
+        ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED if other == "guarded" else sq_amd.SQ_HITS_REFERENCE)
+        lo = torch.empty_like(leaf); to = torch.empty_like(toi)
+        bvh.raycast_nearest_into(rays, 2000.0, lo, to)
+        ts = []
+        for _ in range(5):
+            bvh.raycast_nearest_into(rays, 2000.0, lo, to)
+            ts.append(ctx.last_kernel_ms())
+        ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED if args.hit_mode == "guarded" else sq_amd.SQ_HITS_REFERENCE)
+        differ = int(((lo != leaf) | (to.view(torch.int32) != toi.view(torch.int32))).sum().item())
+        out[f"{other}_mode"] = {"mrays": round(n / (float(np.median(ts)) * 1e-3) / 1e6, 1),
+                                "kernel_ms": round(float(np.median(ts)), 4),
+                                "rays_whose_answer_differs": differ, "of_rays": n}
+        log(f"{other} mode: {out[other + '_mode']['mrays']} Mrays/s, {differ} of {n} rays answer differently")
 
     if not args.no_proposal and not args.quick:
         out["learned_proposal"] = proposal_leg(args, ctx, bvh, sc, rays, leaf, toi, k_ms, n)
         lp = out["learned_proposal"]
-        out["method"] = (f"plain pooled near-first traversal; the learned-proposal path (fused BF16 MFMA inference + "
-                         f"seeded exact walk) measures {lp['mrays']} Mrays/s = {lp['speedup_vs_plain']}x plain, and a "
-                         f"perfect proposal (t_max = exact toi per ray) bounds any proposal at {lp['perfect_limit_bound']}x")
+        out["method"] += (f"; the learned-proposal path (fused BF16 MFMA inference + seeded walk, guarded hits) "
+                          f"measures {lp['mrays']} Mrays/s = {lp['speedup_vs_value']}x this walk and "
+                          f"{lp['speedup_vs_guarded_plain']}x the guarded plain walk, and a perfect proposal (t_max = "
+                          f"the exact toi per ray) bounds any proposal at {lp['perfect_limit_bound']}x the guarded walk")
 
     if args.cpu_sample and not args.quick:
         import oracle
         m = n if args.cpu_sample < 0 else min(args.cpu_sample, n)
         idx = np.linspace(0, n - 1, m).astype(np.int64)       # evenly spread over both halves
         sample = np.ascontiguousarray(rays_h[idx])
-        cores = min(16, os.cpu_count() or 1)
-        o = oracle.Scene(3, oracle.TRI, sc.prims)
+        cores = os.cpu_count() or 1                            # every host thread (nproc)
+        o = oracle.Scene(3, oracle.TRI, sc.prims, guarded=args.hit_mode == "guarded")
         t0 = time.perf_counter()
         ref = o.raycast_nearest(sample, 2000.0, threads=cores)
         cpu_s = time.perf_counter() - t0
         got_leaf = leaf.cpu().numpy()[idx].astype(np.int64)
         got_t = toi.cpu().numpy()[idx]
         agree = (got_leaf == ref["leaf"]) & (got_t.view(np.uint32) == ref["t"].view(np.uint32))
+        # one thread on a bounded sample (every 32nd ray of the same sample)
+        one = np.ascontiguousarray(sample[::32])
+        t0 = time.perf_counter()
+        o.raycast_nearest(one, 2000.0, threads=1)
+        one_s = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(m / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": cores, "kind": "port",
                                "sample": f"{m} of the {n} rays, evenly spaced over both halves; C++ restatement of "
-                                         f"raycast_nearest_batch_parallel (contiguous chunks, {cores} threads)",
-                               "seconds": round(cpu_s, 2)}
+                                         f"raycast_nearest_batch_parallel (contiguous chunks, one per thread, "
+                                         f"{cores} threads = nproc), built -O3 -ffp-contract=off",
+                               "seconds": round(cpu_s, 2), "cpu_model": _cpu_model(),
+                               "one_thread": {"value": round(len(one) / one_s / 1e6, 4), "unit": "Mrays/s",
+                                              "rays": int(len(one)), "seconds": round(one_s, 2)}}
         out["id_agreement_on_sample"] = float(agree.mean())
-        log(f"cpu baseline {m / cpu_s / 1e6:.3f} Mrays/s on {cores} threads; agreement {agree.mean():.6f}")
+        log(f"cpu baseline {m / cpu_s / 1e6:.3f} Mrays/s on {cores} threads, {len(one) / one_s / 1e6:.3f} on one; "
+            f"agreement {agree.mean():.6f}")
     if args.copy_calib:
         a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
@@ -224,6 +262,17 @@ def main():
         torch.cuda.synchronize()
         out["copy_calib_bytes"] = {"read": a.numel() * 4, "written": b.numel() * 4}
     print(json.dumps(out), flush=True)
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def f64_run(args):
@@ -267,7 +316,7 @@ def f64_run(args):
         import oracle
         m = n if args.cpu_sample < 0 else min(args.cpu_sample, n)
         idx = np.linspace(0, n - 1, m).astype(np.int64)
-        cores = min(16, os.cpu_count() or 1)
+        cores = os.cpu_count() or 1
         ref = oracle.Scene64(3, oracle.SPHERE, balls)
         t0 = time.perf_counter()
         w = ref.bvh_raycast_nearest(rays_h[idx], max_toi, threads=cores)
@@ -303,17 +352,29 @@ def proposal_leg(args, ctx, bvh, sc, rays, leaf, toi, plain_ms, n):
     from sq_amd import workloads as W
 
     res = {}
-    # bound: the plain walk with t_max = the exact toi of each ray (what a
-    # perfect, free proposal could give)
-    tl = torch.where(leaf >= 0, toi, torch.full_like(toi, 2000.0)).contiguous()
+    mode0 = ctx.hit_mode()
+    # the seeded walk answers under SQ_HITS_GUARDED (order-independent hits), so
+    # its exactness and speed are measured against the guarded plain walk
+    ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED)
+    gl = torch.empty_like(leaf); gt = torch.empty_like(toi)
+    bvh.raycast_nearest_into(rays, 2000.0, gl, gt)
+    ts = []
+    for _ in range(5):
+        bvh.raycast_nearest_into(rays, 2000.0, gl, gt)
+        ts.append(ctx.last_kernel_ms())
+    guarded_ms = float(np.median(ts))
+    # bound: the guarded plain walk with t_max = the exact toi of each ray (what
+    # a perfect, free proposal could give)
+    tl = torch.where(gl >= 0, gt, torch.full_like(gt, 2000.0)).contiguous()
     l2 = torch.empty_like(leaf); t2 = torch.empty_like(toi)
     bvh.raycast_nearest_into(rays, 2000.0, l2, t2, max_toi_per_ray=tl)
     ts = []
     for _ in range(5):
         bvh.raycast_nearest_into(rays, 2000.0, l2, t2, max_toi_per_ray=tl)
         ts.append(ctx.last_kernel_ms())
+    res["guarded_plain_ms"] = round(guarded_ms, 4)
     res["perfect_limit_ms"] = round(min(ts), 4)
-    res["perfect_limit_bound"] = round(plain_ms / min(ts), 3)
+    res["perfect_limit_bound"] = round(guarded_ms / min(ts), 3)
 
     p = sq_amd.Proposal(bvh, cut_depth=12, rff_scale=1.0, seed=0x5EED)
     t0 = time.time()
@@ -339,7 +400,8 @@ def proposal_leg(args, ctx, bvh, sc, rays, leaf, toi, plain_ms, n):
     pl = torch.empty_like(leaf); pt = torch.empty_like(toi)
     p.raycast_nearest_into(rays, 2000.0, top, pl, pt)
     torch.cuda.synchronize()
-    exact = bool(torch.equal(pl, leaf)) and bool(torch.equal(pt.view(torch.int32), toi.view(torch.int32)))
+    exact = bool(torch.equal(pl, gl)) and bool(torch.equal(pt.view(torch.int32), gt.view(torch.int32)))
+    vs_value = int(((pl != leaf) | (pt.view(torch.int32) != toi.view(torch.int32))).sum().item())
     ti, tw = [], []
     for _ in range(5):
         p.infer(rays, out=top); ti.append(ctx.last_kernel_ms())
@@ -351,7 +413,7 @@ def proposal_leg(args, ctx, bvh, sc, rays, leaf, toi, plain_ms, n):
         p.raycast_nearest_into(rays, 2000.0, top, pl, pt)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    lab_e = p.labels(leaf).cpu().numpy()
+    lab_e = p.labels(gl).cpu().numpy()
     t = top.cpu().numpy().view(np.uint16).astype(np.int64)
     half = n // 2
     def rate(sl):
@@ -361,7 +423,8 @@ def proposal_leg(args, ctx, bvh, sc, rays, leaf, toi, plain_ms, n):
     plain_cnt = p.raycast_nearest(rays, 2000.0, None, count=True)
     infer_ms = float(np.median(ti))
     res.update(mrays=round(n / (ms * 1e-3) / 1e6, 2), ms_per_step=round(ms, 4),
-               speedup_vs_plain=round(plain_ms / ms, 3), exact_vs_plain=exact,
+               speedup_vs_value=round(plain_ms / ms, 3), speedup_vs_guarded_plain=round(guarded_ms / ms, 3),
+               exact_vs_guarded_plain=exact, rays_differing_from_value=vs_value,
                infer_ms=round(infer_ms, 4), seeded_walk_ms=round(float(np.median(tw)), 4),
                infer_tflops=round(flop_ray * n / (infer_ms * 1e-3) / 1e12, 1),
                mfma_utilisation=round(flop_ray * n / (infer_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
@@ -370,9 +433,11 @@ def proposal_leg(args, ctx, bvh, sc, rays, leaf, toi, plain_ms, n):
                nodes_per_ray_plain=round(plain_cnt["nodes"] / n, 2), nodes_per_ray_seeded=round(cnt["nodes"] / n, 2),
                prims_per_ray_plain=round(plain_cnt["prims"] / n, 2), prims_per_ray_seeded=round(cnt["prims"] / n, 2),
                flop_per_ray=flop_ray)
-    log(f"proposal: {res['mrays']} Mrays/s ({res['speedup_vs_plain']}x plain), infer {res['infer_ms']} ms at "
-        f"{res['infer_tflops']} TFLOP/s, walk {res['seeded_walk_ms']} ms, exact {exact}, bound {res['perfect_limit_bound']}x")
+    log(f"proposal: {res['mrays']} Mrays/s ({res['speedup_vs_guarded_plain']}x guarded plain, {res['speedup_vs_value']}x "
+        f"value), infer {res['infer_ms']} ms at {res['infer_tflops']} TFLOP/s, walk {res['seeded_walk_ms']} ms, exact vs "
+        f"guarded {exact}, bound {res['perfect_limit_bound']}x")
     p.close()
+    ctx.set_hit_mode(mode0)
     return res
 
 

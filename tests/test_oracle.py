@@ -604,3 +604,176 @@ def test_required_work_brute_force():
                 want_n += 2
                 stack += [int(ca[i]), int(cb[i])]
         assert (nn[r], npr[r]) == (want_n, want_p)
+
+
+# ---------------------------------------------------------------------------
+# Box crossings (the AABB kind's test_ray_crossings): the builder's convention
+# for a convex box, entry always (0 from inside), exit only when the box's own
+# exit lies within max_toi.  Pinned by closed forms.
+# ---------------------------------------------------------------------------
+
+def _crossings(dim, box, o, d, max_toi):
+    sc = oracle.Scene(dim, oracle.BOX, np.asarray([box], F32))
+    ray = np.concatenate([np.asarray(o, F32), np.asarray(d, F32)])[None]
+    off, leaf, t = sc.raycast_all(ray, max_toi, crossings=True)
+    return leaf.tolist(), t.tolist()
+
+
+def test_box_crossings_closed_forms():
+    # box [1,3]^3 crossed along +x from x = 0: entry 1, exit 3
+    assert _crossings(3, [1, 1, 1, 3, 3, 3], [0, 2, 2], [1, 0, 0], 100.0) == ([0, 0], [1.0, 3.0])
+    # along -x from x = 5: entry 2, exit 4
+    assert _crossings(3, [1, 1, 1, 3, 3, 3], [5, 2, 2], [-1, 0, 0], 100.0) == ([0, 0], [2.0, 4.0])
+    # origin inside: entry 0 (the slab's t_min stays 0), exit 1
+    assert _crossings(3, [1, 1, 1, 3, 3, 3], [2, 2, 2], [1, 0, 0], 100.0) == ([0, 0], [0.0, 1.0])
+    # max_toi inside the box: the entry only (the exit lies beyond max_toi)
+    assert _crossings(3, [1, 1, 1, 3, 3, 3], [0, 2, 2], [1, 0, 0], 2.0) == ([0], [1.0])
+    # max_toi exactly at the exit: both
+    assert _crossings(3, [1, 1, 1, 3, 3, 3], [0, 2, 2], [1, 0, 0], 3.0) == ([0, 0], [1.0, 3.0])
+    # max_toi before the entry, or the ray beside the box: nothing
+    assert _crossings(3, [1, 1, 1, 3, 3, 3], [0, 2, 2], [1, 0, 0], 0.5) == ([], [])
+    assert _crossings(3, [1, 1, 1, 3, 3, 3], [0, 5, 2], [1, 0, 0], 100.0) == ([], [])
+    # a diagonal ray through [0,4]^2 from (-1,-2): entry when y reaches 0 (t = 2 / |d|·...)
+    d = np.array([1.0, 1.0], F32) / np.float32(np.sqrt(2.0))
+    leaf, t = _crossings(2, [0, 0, 4, 4], [-1, -2], d, 100.0)
+    inv = np.float32(1.0) / d[0]
+    assert leaf == [0, 0]
+    assert t[0] == float((np.float32(0) - np.float32(-2)) * inv)       # entry by y
+    assert t[1] == float((np.float32(4) - np.float32(-1)) * inv)       # exit by x
+    # D = 6: a box crossed along the last axis
+    lo, hi = [0.0] * 6, [1.0] * 6
+    lo[5], hi[5] = 10.0, 14.0
+    assert _crossings(6, lo + hi, [0.5] * 5 + [0.0], [0, 0, 0, 0, 0, 1], 100.0) == ([0, 0], [10.0, 14.0])
+
+
+def test_box_crossings_are_entry_and_exit_of_every_box():
+    """On a random box scene every crossing list is the sorted union, over the
+    boxes the walk reaches, of each box's slab entry and (when it lies within
+    max_toi and after the entry) its exit, computed here independently from the
+    slab closed form."""
+    rng = np.random.default_rng(0xC2055)
+    lo = rng.uniform(-20, 20, (300, 3)).astype(F32)
+    boxes = np.concatenate([lo, lo + rng.uniform(0.5, 4.0, (300, 3)).astype(F32)], 1)
+    sc = oracle.Scene(3, oracle.BOX, boxes)
+    rays = np.concatenate([rng.uniform(-25, 25, (400, 3)), normalize_or_zero(rng.normal(size=(400, 3)).astype(F32))],
+                          1).astype(F32)
+    off, leaf, t = sc.raycast_all(rays, 30.0, crossings=True)
+    for i in range(len(rays)):
+        want = []
+        for b in range(len(boxes)):
+            hit = oracle.ray_intersect(boxes[b], rays[i], 30.0)
+            if hit is None:
+                continue
+            te, _ = hit
+            want.append((np.float32(te), b))
+            full = oracle.ray_intersect(boxes[b], rays[i], np.inf)
+            if full[1] > te and full[1] <= np.float32(30.0):
+                want.append((np.float32(full[1]), b))
+        want.sort()
+        got = list(zip(t[off[i]:off[i + 1]].tolist(), leaf[off[i]:off[i + 1]].tolist()))
+        assert got == [(float(a), b) for a, b in want], i
+
+
+# ---------------------------------------------------------------------------
+# Mesh instances (the builder's INST_MESH provider: test_ray_local is the
+# mesh's own Bvh::raycast_nearest on the ray moved into the unscaled mesh
+# frame, o / s and d / s, t kept).  Pinned by transforms under which every
+# operation is exact, where the instance answer must equal the single-level
+# tree over the explicitly transformed triangles and the naive scan.
+# ---------------------------------------------------------------------------
+
+def _dyadic_mesh(rng, n_tri):
+    """Triangles with vertices on the 1/8 grid of [-4, 4]^3 (exact arithmetic
+    for every product and sum of the walk and Moeller-Trumbore below)."""
+    v = rng.integers(-32, 33, (n_tri, 9)).astype(F32) / F32(8.0)
+    return v
+
+
+def _signed_perm(rng):
+    p = rng.permutation(3)
+    s = rng.choice([-1.0, 1.0], 3)
+    r = np.zeros((3, 3), F32)
+    r[np.arange(3), p] = s
+    return r
+
+
+def test_mesh_instance_identity_equals_the_mesh_tree():
+    """R = I, t = 0, s = 1: the instance answers exactly as the mesh's own
+    single-level tree (nearest, triangle id, toi, normal), in both modes."""
+    rng = np.random.default_rng(0x1D3)
+    from sq_amd import workloads as W
+    mesh = W.noisy_sphere(6, 0xA1)
+    o = rng.uniform(-3, 3, (3000, 3)).astype(F32)
+    tgt = rng.uniform(-0.9, 0.9, (3000, 3)).astype(F32)
+    rays = np.concatenate([o, normalize_or_zero(tgt - o)], 1).astype(F32)
+    for guarded in (False, True):
+        inst = oracle.InstanceScene(3, oracle.INST_MESH, np.eye(3, dtype=F32)[None], np.zeros((1, 3), F32),
+                                    mesh_tris=[mesh], mesh_of=np.zeros(1, np.uint32), scale=np.ones(1, F32),
+                                    guarded=guarded)
+        a = inst.raycast_nearest(rays, 100.0)
+        b = oracle.Scene(3, oracle.TRI, mesh, guarded=guarded).raycast_nearest(rays, 100.0)
+        hit = b["leaf"] >= 0
+        assert hit.sum() > 1000
+        assert np.array_equal(a["leaf"] >= 0, hit)
+        assert np.array_equal(a["sub"][hit], b["leaf"][hit])
+        assert np.array_equal(bits(a["t"]), bits(b["t"]))
+        assert np.array_equal(bits(a["normal"][hit]), bits(b["normal"][hit]))
+
+
+def test_mesh_instances_dyadic_transforms_equal_world_triangles():
+    """Signed-permutation rotations, power-of-two scales, translations and
+    vertices on a dyadic grid, and dyadic unnormalised ray directions: every
+    operation of the instance test (R^T(o - t), d / s, Moeller-Trumbore in the
+    mesh frame) is exact, so the two-level answer equals, bit for bit, the
+    single-level tree over the triangles transformed explicitly into the world
+    (R v s + t) and the naive scan over them (guarded mode, whose answer does
+    not depend on the tree).  A transposed rotation, a scale applied the wrong
+    way or a dropped translation fails it."""
+    rng = np.random.default_rng(0xD1AD)
+    meshes = [_dyadic_mesh(rng, 60), _dyadic_mesh(rng, 90)]
+    n_inst = 40
+    rot = np.stack([_signed_perm(rng) for _ in range(n_inst)])
+    scale = rng.choice([0.5, 1.0, 2.0, 4.0], n_inst).astype(F32)
+    trans = (rng.integers(-160, 161, (n_inst, 3)) / 4.0).astype(F32)
+    mesh_of = rng.integers(0, 2, n_inst).astype(np.uint32)
+    world, owner = [], []
+    for i in range(n_inst):
+        m = meshes[mesh_of[i]].reshape(-1, 3, 3)
+        w = np.einsum("ab,tvb->tva", rot[i], m) * scale[i] + trans[i]
+        world.append(w.reshape(-1, 9).astype(F32))
+        owner += [(i, k) for k in range(len(m))]
+    world = np.concatenate(world)
+    o = (rng.integers(-200, 201, (6000, 3)) / 4.0).astype(F32)
+    d = (rng.integers(-4, 5, (6000, 3)) / 4.0).astype(F32)
+    keep = np.any(d != 0, axis=1)
+    rays = np.concatenate([o[keep], d[keep]], 1)
+    inst = oracle.InstanceScene(3, oracle.INST_MESH, rot, trans, mesh_tris=meshes, mesh_of=mesh_of, scale=scale,
+                                guarded=True)
+    a = inst.raycast_nearest(rays, 1000.0)
+    flat = oracle.Scene(3, oracle.TRI, world, guarded=True)
+    b = flat.raycast_nearest(rays, 1000.0)
+    nv = flat.raycast_nearest(rays, 1000.0, naive=True)
+    hit = b["leaf"] >= 0
+    assert hit.sum() > 300
+    assert np.array_equal(b["leaf"], nv["leaf"]) and np.array_equal(bits(b["t"]), bits(nv["t"]))
+    assert np.array_equal(a["leaf"] >= 0, hit)
+    got = [(int(a["leaf"][j]), int(a["sub"][j])) for j in np.nonzero(hit)[0]]
+    want = [owner[b["leaf"][j]] for j in np.nonzero(hit)[0]]
+    # the same triangle answers; ties between instances sharing a toi are
+    # ordered by instance id on the two-level side and by global index on the
+    # flat side, which rank instances the same way (owner is instance-major)
+    assert got == want
+    assert np.array_equal(bits(a["t"]), bits(b["t"]))
+    # all hits (the same set in both modes): one per instance, its nearest
+    # triangle, equal to the per-instance minimum of the flat all-hits list
+    off, leaf, t, sub = inst.raycast_all(rays[:500], 1000.0)
+    foff, fleaf, ft = flat.raycast_all(rays[:500], 1000.0)
+    for j in range(500):
+        best = {}
+        for g, tt in zip(fleaf[foff[j]:foff[j + 1]], ft[foff[j]:foff[j + 1]]):
+            i, k = owner[g]
+            if i not in best or (tt, k) < best[i]:
+                best[i] = (tt, k)
+        want = sorted((tt, i, k) for i, (tt, k) in best.items())
+        got = sorted(zip(t[off[j]:off[j + 1]].tolist(), leaf[off[j]:off[j + 1]].tolist(), sub[off[j]:off[j + 1]].tolist()))
+        assert [(float(a_), int(b_), int(c_)) for a_, b_, c_ in want] == got, j
