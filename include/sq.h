@@ -255,6 +255,23 @@ int sq_overlap_consumer(sq_narrow* narrow, sq_bvh* bvh, const float* regions, ui
 int sq_raycast_nearest(sq_bvh* bvh, const float* rays, uint64_t n, float max_toi, const float* max_toi_per_ray,
                        const sq_filter* filter, int32_t* out_leaf, float* out_toi, float* out_point,
                        float* out_normal, int32_t* out_sub);
+/* sq_raycast_nearest without allocating (no counterpart in the reference):
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as sq_raycast_nearest; the walk's counters and stack
+ * spill live in a file-scope device array of SQ_NOALLOC_SCRATCH_BYTES on each
+ * device (held until the process ends) instead of the context's buffers, and
+ * the call creates no timing events.  Limits: the tree's stack must fit the
+ * array (the LDS part of the stack grows to 64 entries first, then the grid
+ * shrinks; SQ_ERR_UNSUPPORTED when even one block does not fit), and the pooled
+ * walk always runs (rays_per_lane 0, ray_order and the 4-wide walk are not used).
+ * Calls that overlap on one device race on the array: order them (one stream,
+ * or events).  The context must come from sq_ctx_create (which looks up the
+ * array's address on its device). */
+#define SQ_NOALLOC_SCRATCH_BYTES (16u << 20)
+int sq_raycast_nearest_noalloc(sq_bvh* bvh, const float* rays, uint64_t n, float max_toi, const float* max_toi_per_ray,
+                               const sq_filter* filter, int32_t* out_leaf, float* out_toi, float* out_point,
+                               float* out_normal, int32_t* out_sub);
 /* Occlusion: out_hit[i] = raycast_nearest(..).is_some() (the reference's any-hit
  * idiom, examples-viewport/examples/many_rays.rs:193-201), ending each ray at
  * its first accepted hit. */

@@ -20,6 +20,29 @@
 
 thread_local std::string g_err;
 
+// Device memory of sq_raycast_nearest_noalloc: the pooled walk's two counters
+// (the first 256 bytes) and its ring-stack spill columns (the rest), in place
+// of the context's buffers.  Each device holds its own copy until the process
+// ends; its address there comes from hipGetSymbolAddress (g_scratch below),
+// since host code cannot use the array's name as a device address.
+static __device__ __align__(256) unsigned char scratch[SQ_NOALLOC_SCRATCH_BYTES];
+constexpr int SQ_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch[SQ_MAX_DEVICES];   // scratch's address on each device, set once by scratch_init
+
+// Looks up scratch's address on `device`, the current device (once per device).
+static int scratch_init(int device) {
+  if (device < 0 || device >= SQ_MAX_DEVICES) return fail(SQ_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch[device]) return SQ_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(SQ_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[device] = p;
+  return SQ_OK;
+}
+
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
@@ -742,6 +765,7 @@ int sq_ctx_create(int device, sq_ctx** out) {
   if (int rc = c->pool_ctr.ensure(256)) return rc;
   HIP_TRY(hipMemsetAsync(c->pool_ctr.p, 0, 256, c->own));
   HIP_TRY(hipStreamSynchronize(c->own));
+  if (int rc = scratch_init(device)) return rc;   // sq_raycast_nearest_noalloc's memory on this device
   *out = c.release();
   return SQ_OK;
 }
@@ -1198,6 +1222,73 @@ int sq_raycast_nearest(sq_bvh* b, const float* rays, uint64_t n, float max_toi, 
   if (n && (!out_leaf || !out_toi)) return fail(SQ_ERR_INVALID, "out_leaf and out_toi are required");
   NearestOut o{out_leaf, out_toi, out_point, out_normal, out_sub, nullptr};
   return nearest_impl(b, rays, n, max_toi, tmax, f, o, false, false);
+}
+
+// sq_raycast_nearest without allocating: the pooled walk's counters and spill
+// columns live in the device's scratch array.  The ring stack kept in LDS is
+// the context's lds_stack, doubled (up to 64 entries) until the resident grid's
+// spill fits the array; past that the grid shrinks to what fits.  No timing
+// events (last_kernel_ms keeps the previous call's time).
+int sq_raycast_nearest_noalloc(sq_bvh* b, const float* rays, uint64_t n, float max_toi, const float* tmax,
+                               const sq_filter* f, int32_t* out_leaf, float* out_toi, float* out_point, float* out_normal,
+                               int32_t* out_sub) {
+  if (!b || (n && (!rays || !out_leaf || !out_toi))) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "at most 2^32-1 queries per call");
+  if (!n) return SQ_OK;
+  sq_ctx* c = b->ctx;
+  NearestOut out{out_leaf, out_toi, out_point, out_normal, out_sub, nullptr};
+  if (!b->n_prims || (f && f->layer_mask == 0)) {
+    k_fill_miss<<<grid_for(n, 256), 256, 0, c->stream>>>((uint32_t)n, b->dim, out);
+    HIP_TRY(hipGetLastError());
+    return SQ_OK;
+  }
+  void* base = (c->device >= 0 && c->device < SQ_MAX_DEVICES) ? g_scratch[c->device] : nullptr;
+  if (!base) return fail(SQ_ERR_HIP, "no scratch address for this device (context not created through sq_ctx_create)");
+  bool filt;
+  Filter fd = filter_of(b, f, &filt);
+  const uint32_t stack_n = b->packed.max_stack;
+  const SceneView s = view_of(b);
+  const uint32_t nn = (uint32_t)n;
+  const int B = c->block;
+  const uint32_t per = pooled_per_lane(c, n);
+  const uint64_t chunk = 64ull * per;
+  const uint64_t chunks = (n + chunk - 1) / chunk;
+  const uint64_t wpb = (uint64_t)B / 64;
+  const uint64_t spill_cap = SQ_NOALLOC_SCRATCH_BYTES - 256;
+  uint32_t* counter = static_cast<uint32_t*>(base);
+  SpillEntry* spill = reinterpret_cast<SpillEntry*>(static_cast<unsigned char*>(base) + 256);
+  return dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+    auto launch = [&]<bool FILT_, bool EXACT_>() -> int {
+      const void* kern = reinterpret_cast<const void*>(&k_nearest_pool<D, K, false, FILT_, false, false, EXACT_>);
+      uint32_t S = (uint32_t)c->lds_stack;
+      uint64_t grid = 0;
+      for (;;) {
+        int per_cu = 0;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, B, (size_t)S * B * 8));
+        grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
+        const uint64_t rows = stack_n > S ? stack_n - S : 0;
+        if (rows * grid * B * sizeof(SpillEntry) <= spill_cap) break;
+        if (S < 64 && (size_t)(2 * S) * B * 8 <= 64 * 1024) { S *= 2; continue; }
+        grid = spill_cap / (rows * B * sizeof(SpillEntry));
+        if (!grid) return fail(SQ_ERR_UNSUPPORTED, "tree too deep for the no-allocation scratch");
+        break;
+      }
+      const uint64_t lanes = grid * (uint64_t)B;
+      k_zero_words<<<1, 64, 0, c->stream>>>(counter, 64);   // the pool counters start at zero
+      HIP_TRY(hipGetLastError());
+      k_nearest_pool<D, K, false, FILT_, false, false, EXACT_><<<(uint32_t)grid, B, (size_t)S * B * 8, c->stream>>>(
+          s, rays, nn, max_toi, tmax, fd, out, (uint32_t)chunk, counter, S, spill, (uint32_t)lanes, nullptr,
+          (uint32_t)c->descend_min);
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    };
+    auto pick = [&]<bool EXACT_>() -> int {
+      return filt ? launch.template operator()<true, EXACT_>() : launch.template operator()<false, EXACT_>();
+    };
+    // AABB hits lie inside their boxes: the one walk is exact in both modes
+    if constexpr (K == KIND_AABB) return pick.template operator()<true>();
+    else return c->hit_mode == SQ_HITS_GUARDED ? pick.template operator()<false>() : pick.template operator()<true>();
+  });
 }
 
 int sq_raycast_any(sq_bvh* b, const float* rays, uint64_t n, float max_toi, const float* tmax, const sq_filter* f, uint8_t* out_hit) {

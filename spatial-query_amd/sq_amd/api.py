@@ -330,6 +330,18 @@ class DeviceBvh:
                                        _dp(toi, "float32", n=n), _dp(point, "float32", n=n, per=self.dim),
                                        _dp(normal, "float32", n=n, per=self.dim), _dp(sub, "int32", n=n)))
 
+    def raycast_nearest_noalloc_into(self, rays, max_toi, leaf, toi, max_toi_per_ray=None, filt=None, point=None,
+                                     normal=None, sub=None):
+        """sq_raycast_nearest_noalloc: raycast_nearest_into without any device
+        allocation (the walk's counters and stack spill in the library's fixed
+        per-device array); the caller provides every output."""
+        n = rays.shape[0]
+        check(lib().sq_raycast_nearest_noalloc(self._h, _dp(rays, "float32", 2 * self.dim), n, float(max_toi),
+                                               _dp(max_toi_per_ray, "float32", n=n), self._f(filt),
+                                               _dp(leaf, "int32", n=n), _dp(toi, "float32", n=n),
+                                               _dp(point, "float32", n=n, per=self.dim),
+                                               _dp(normal, "float32", n=n, per=self.dim), _dp(sub, "int32", n=n)))
+
     def raycast_any(self, rays, max_toi, max_toi_per_ray=None, filt=None, out=None):
         import torch
         if out is None:
