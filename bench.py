@@ -219,6 +219,7 @@ def main():
 
     if not args.no_proposal and not args.quick:
         out["learned_proposal"] = proposal_leg(args, ctx, bvh, sc, rays, leaf, toi, k_ms, n)
+        out["learned_proposal"]["instanced"] = proposal_instanced_leg(args, ctx)
         lp = out["learned_proposal"]
         out["method"] += (f"; the learned-proposal path (fused BF16 MFMA inference + seeded walk, guarded hits) "
                           f"measures {lp['mrays']} Mrays/s = {lp['speedup_vs_value']}x this walk and "
@@ -437,6 +438,89 @@ def proposal_leg(args, ctx, bvh, sc, rays, leaf, toi, plain_ms, n):
         f"value), infer {res['infer_ms']} ms at {res['infer_tflops']} TFLOP/s, walk {res['seeded_walk_ms']} ms, exact vs "
         f"guarded {exact}, bound {res['perfect_limit_bound']}x")
     p.close()
+    ctx.set_hit_mode(mode0)
+    return res
+
+
+def proposal_instanced_leg(args, ctx):
+    """The proposal's instanced half: the two-level scene of 16 noise-displaced
+    icospheres (2k-50k triangles) placed as 4,096 instances (uniform rotations,
+    translations U[-1000,1000]^3, scales U[0.5,4]); the top tree over the
+    instances cut at depth 12; trained on fresh random rays (origins
+    U[-1500,1500]^3, uniform directions, t_max 4000) labelled by the exact
+    two-level walk, evaluated on 8M rays of another seed.  All walks answer
+    under SQ_HITS_GUARDED (the seeded walk's mode)."""
+    import numpy as np
+    import torch
+    import sq_amd
+    from sq_amd import workloads as W
+
+    mode0 = ctx.hit_mode()
+    ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED)
+    tmax = 4000.0
+    sc = W.instanced_scene()
+    meshes = [sq_amd.DeviceBvh.build(ctx, 3, sq_amd.TRIANGLE, m) for m in sc["meshes"]]
+    tl = sq_amd.Tlas(ctx, 3, sq_amd.Tlas.MESH, sc["rot"], sc["trans"], meshes=meshes, mesh_of=sc["mesh_of"],
+                     scale=sc["scale"])
+    n = 1 << 23
+    rays = torch.from_numpy(W.random_rays(n, 0x1C3E_0000_0001, -1500.0, 1500.0)).cuda()
+    res = dict(scene="4,096 instances of 16 meshes (2k-50k triangles), rays U[-1500,1500]^3 x uniform directions, "
+                     "t_max 4000", eval_rays=n)
+
+    def timed(fn, reps=5):
+        fn()
+        ts = []
+        for _ in range(reps):
+            fn()
+            ts.append(ctx.last_kernel_ms())
+        return float(np.median(ts))
+
+    plain = tl.raycast_nearest(rays, tmax, sub=True)
+    plain_ms = timed(lambda: tl.raycast_nearest(rays, tmax))
+    hit = plain["leaf"] >= 0
+    lim = torch.where(hit, plain["toi"], torch.full_like(plain["toi"], tmax)).contiguous()
+    perfect_ms = timed(lambda: tl.raycast_nearest(rays, tmax, max_toi_per_ray=lim))
+    res.update(hit_fraction=round(float(hit.float().mean().item()), 4), guarded_plain_ms=round(plain_ms, 4),
+               guarded_plain_mrays=round(n / (plain_ms * 1e-3) / 1e6, 1), perfect_limit_ms=round(perfect_ms, 4),
+               perfect_limit_bound=round(plain_ms / perfect_ms, 3))
+
+    p = sq_amd.Proposal(tl, cut_depth=12, rff_scale=1.0, seed=0x5EED)
+    t0 = time.time()
+    rt = torch.from_numpy(W.random_rays(args.proposal_train_rays, 0x1C3E_0000_0002, -1500.0, 1500.0)).cuda()
+    lab = p.labels(tl.raycast_nearest(rt, tmax)["leaf"])
+    torch.cuda.synchronize()
+    t_data = time.time() - t0
+    t0 = time.time()
+    losses = p.fit(rt, lab, epochs=args.proposal_epochs, batch=65536, lr=1e-3, seed=1)
+    torch.cuda.synchronize()
+    res.update(subtrees=p.n_subtrees, train_rays=int(rt.shape[0]), labelled_train_rays=int((lab >= 0).sum().item()),
+               epochs=args.proposal_epochs, data_s=round(t_data, 2), train_s=round(time.time() - t0, 2),
+               loss_first=round(losses[0], 4), loss_last=round(losses[-1], 4))
+    del rt, lab
+
+    top = p.infer(rays)
+    got = p.raycast_nearest(rays, tmax, top, count=True)
+    torch.cuda.synchronize()
+    exact = all(bool(torch.equal(got[k], plain[k])) for k in ("leaf", "sub")) and \
+        bool(torch.equal(got["toi"].view(torch.int32), plain["toi"].view(torch.int32)))
+    infer_ms = timed(lambda: p.infer(rays, out=top))
+    pl = torch.empty_like(plain["leaf"]); pt = torch.empty_like(plain["toi"])
+    walk_ms = timed(lambda: p.raycast_nearest_into(rays, tmax, top, pl, pt))
+    lab_e = p.labels(plain["leaf"]).cpu().numpy()
+    tt = top.cpu().numpy().view(np.uint16).astype(np.int64)
+    ok = lab_e >= 0
+    learned_ms = infer_ms + walk_ms
+    res.update(exact_vs_guarded_plain=exact, infer_ms=round(infer_ms, 4), seeded_walk_ms=round(walk_ms, 4),
+               mrays=round(n / (learned_ms * 1e-3) / 1e6, 1), speedup_vs_guarded_plain=round(plain_ms / learned_ms, 3),
+               seeded_walk_speedup=round(plain_ms / walk_ms, 3),
+               top4_hit_rate=round(float((tt[ok] == lab_e[ok, None]).any(axis=1).mean()), 4),
+               seeded_fraction=round(got["seeded"] / n, 4))
+    log(f"proposal (instanced): hits {res['hit_fraction']}, bound {res['perfect_limit_bound']}x, plain {plain_ms:.3f} ms, "
+        f"infer {infer_ms:.3f} + walk {walk_ms:.3f} ms, top-4 hit rate {res['top4_hit_rate']}, exact {exact}")
+    p.close()
+    tl.close()
+    for m in meshes:
+        m.close()
     ctx.set_hit_mode(mode0)
     return res
 

@@ -110,8 +110,8 @@ void* sq_ctx_stream(sq_ctx* ctx);
  *       nearest queries walk near-first (fewer nodes).  It differs from the
  *       reference only on rays whose reference answer lies outside its box
  *       (40 of the 8,388,608 rays of the 2M-triangle heightfield workload).
- *       The learned proposal's seeded walk (sq_raycast_nearest_proposed)
- *       always answers in this mode.
+ *       The learned proposal's seeded walks (sq_raycast_nearest_proposed,
+ *       sq_tlas_raycast_nearest_proposed) always answer in this mode.
  * All-hits, crossings, overlap and cast-all lists hold the reference's hits in
  * both modes (their walks never shrink a limit, so order does not matter). */
 #define SQ_HITS_REFERENCE 0
@@ -326,9 +326,10 @@ int sq_required_work(sq_bvh* bvh, const float* rays, uint64_t n, const float* t_
  * cast (test_shape_cast = None), AABB-level overlap (test_overlap = true),
  * and every kind reports one crossing per instance (test_ray_crossings_local's
  * default), so Tlas::raycast_crossings is sq_tlas_raycast_all.
- * An instance hit is kept only when its toi lies in the slab interval of the
- * instance's world AABB (order independence, as for the primitive kinds); a
- * cast contact only inside the probe-grown world AABB's slab interval. */
+ * Under SQ_HITS_GUARDED an instance hit is kept only when its toi lies in the
+ * slab interval of the instance's world AABB (order independence, as for the
+ * primitive kinds); a cast contact only inside the probe-grown world AABB's
+ * slab interval.  SQ_HITS_REFERENCE keeps what the instance test returns. */
 #define SQ_INST_BOX 0
 #define SQ_INST_MESH 1
 #define SQ_INST_BALL 2
@@ -469,6 +470,12 @@ typedef struct sq_proposal sq_proposal;
  * and the leaves above it), Fourier scale rff_scale > 0, weights and Fourier
  * matrix drawn from seed.  The scene must outlive the proposal. */
 int sq_proposal_create(sq_ctx* ctx, sq_bvh* scene, int cut_depth, float rff_scale, uint64_t seed, sq_proposal** out);
+/* The same over a two-level scene (D = 3, any instance kind): the subtrees are
+ * those of the Tlas's top tree over the instances, cut at cut_depth, the
+ * labels the subtrees of the hit instances (sq_proposal_labels takes the
+ * instance ids of sq_tlas_raycast_nearest), the scene box the top tree's root
+ * box at creation.  The Tlas must outlive the proposal. */
+int sq_proposal_create_tlas(sq_ctx* ctx, sq_tlas* scene, int cut_depth, float rff_scale, uint64_t seed, sq_proposal** out);
 int sq_proposal_destroy(sq_proposal* p);
 int sq_proposal_num_subtrees(const sq_proposal* p, uint32_t* out);
 int sq_proposal_num_params(const sq_proposal* p, uint64_t* out);
@@ -496,6 +503,14 @@ int sq_proposal_infer(sq_proposal* p, const float* rays, uint64_t n, uint16_t* t
  * whose proposed subtrees held a hit, pair nodes fetched, primitives tested. */
 int sq_raycast_nearest_proposed(sq_proposal* p, const float* rays, uint64_t n, float max_toi, const uint16_t* top,
                                 int32_t* out_leaf, float* out_toi, uint64_t* stats);
+/* Exact nearest hit over the Tlas of a proposal from sq_proposal_create_tlas,
+ * walking the proposed top-tree subtrees first: the same instance, toi and
+ * sub-object (out_sub, may be NULL) as sq_tlas_raycast_nearest under
+ * SQ_HITS_GUARDED without per-ray limits (whatever the context's hit mode).
+ * stats (device uint64[3], may be NULL): stats[0] accumulates the rays whose
+ * proposed subtrees held a hit; stats[1], stats[2] are left unchanged. */
+int sq_tlas_raycast_nearest_proposed(sq_proposal* p, const float* rays, uint64_t n, float max_toi, const uint16_t* top,
+                                     int32_t* out_inst, float* out_toi, int32_t* out_sub, uint64_t* stats);
 /* C[M][N] = bf16(A)[M][K] * bf16(B)[N][K]^T in FP32 on the matrix cores, the
  * GEMM the training step runs (device fp32 inputs, rounded to bf16 first).
  * M, N multiples of 128, K of 32; splits > 1 uses the split-K reduction. */

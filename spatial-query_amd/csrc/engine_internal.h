@@ -291,6 +291,14 @@ int ray_order(sq_ctx* c, int dim, const float* q, uint32_t n, uint32_t qs, const
 // Writes misses (leaf -1, toi +inf, zero point/normal, sub -1, any 0).
 int fill_miss(sq_ctx* c, uint64_t n, int dim, NearestOut out);
 
+// Two-level tree access for the learned proposal (tlas.hip): the top tree over
+// the instances' world boxes, the dimension, and the seeded walk over guarded
+// hits (proposed top-tree subtrees first, then the full walk).
+sq_bvh* tlas_top_tree(sq_tlas* t);
+int tlas_dim(const sq_tlas* t);
+int tlas_nearest_proposed(sq_tlas* t, const float* rays, uint64_t n, float max_toi, const uint16_t* top, const uint32_t* sub_ref,
+                          uint32_t n_sub, int32_t* out_inst, float* out_toi, int32_t* out_sub, uint64_t* stats);
+
 // Variable-length (CSR) results: count pass, scan, capacity check, fill pass,
 // per-query sort by key, unpack.  launch(fill, keys) enqueues the count pass
 // (fill = false, writing ctx->counts) or the fill pass (fill = true).

@@ -250,10 +250,11 @@ __device__ __forceinline__ void tlas_take(TlasWalk<D, EXACT>& w, float t, uint32
 }
 
 // One step of ray r; true when the ray is finished (result in w.res).  ANY
-// finishes at the first accepted instance hit (walk_tlas's ANY).
-template <int D, int IK, bool ANY, bool EXACT, class P>
+// finishes at the first accepted instance hit (walk_tlas's ANY).  SKIP treats
+// the top-tree child refs skip[0..3] as missed (subtrees already walked).
+template <int D, int IK, bool ANY, bool EXACT, bool SKIP = false, class P>
 __device__ __forceinline__ bool tlas_step(const TlasView& tv, const RayD<D>& r, const RingStack& st, const RingStack& sb,
-                                          TlasWalk<D, EXACT>& w, const P& pause) {
+                                          TlasWalk<D, EXACT>& w, const P& pause, const uint32_t* skip = nullptr) {
   constexpr int NF = (D + 1) * 4;
   const SceneView& s = tv.top;
   if constexpr (IK == INST_MESH) {
@@ -327,6 +328,10 @@ __device__ __forceinline__ bool tlas_step(const TlasView& tv, const RayD<D>& r, 
     float ta, tb;
     order_top<D, EXACT, ANY>(r, nd, w.limit, ha, hb, ta, tb, a_first);
     const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
+    if constexpr (SKIP) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) { ha = ha && ra != skip[k]; hb = hb && rb != skip[k]; }
+    }
     if (ha && hb) {
       st.push(w.sp, a_first ? rb : ra, a_first ? tb : ta);
       w.cur = a_first ? ra : rb;

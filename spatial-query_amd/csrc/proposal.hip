@@ -539,7 +539,8 @@ struct Layer {
 
 struct sq_proposal {
   sq_ctx* ctx = nullptr;
-  sq_bvh* scene = nullptr;         // not owned: destroyed after the proposal
+  sq_bvh* scene = nullptr;         // the tree cut into subtrees (a Tlas's top tree); not owned
+  sq_tlas* tlas = nullptr;         // the two-level scene, when the proposal serves one; not owned
   uint32_t n_sub = 0;
   int cut = 0;
   Layer L[4];
@@ -619,17 +620,20 @@ MlpWeights weights_of(const sq_proposal* p) {
 
 extern "C" {
 
-int sq_proposal_create(sq_ctx* ctx, sq_bvh* scene, int cut_depth, float rff_scale, uint64_t seed, sq_proposal** out) {
-  if (!ctx || !scene || !out) return fail(SQ_ERR_INVALID, "NULL argument");
-  *out = nullptr;
-  if (scene->kind != KIND_TRI || scene->dim != 3) return fail(SQ_ERR_UNSUPPORTED, "the proposal serves TRIANGLE scenes (D = 3)");
+}  // extern "C"
+
+namespace {
+
+// A proposal over the subtrees of `scene` cut at `cut_depth` (the tree of a
+// TRIANGLE sq_bvh, or a Tlas's top tree over its instances, tlas != NULL).
+int proposal_create(sq_ctx* ctx, sq_bvh* scene, sq_tlas* tlas, int cut_depth, float rff_scale, uint64_t seed, sq_proposal** out) {
   if (scene->n_prims == 0) return fail(SQ_ERR_INVALID, "empty scene");
   if (cut_depth < 1 || cut_depth > 12) return fail(SQ_ERR_INVALID, "cut depth must be in [1, 12] (<= 4096 subtrees)");
   if (!(rff_scale > 0.0f && rff_scale < 1e3f)) return fail(SQ_ERR_INVALID, "rff_scale must be in (0, 1000)");
   int rh = ensure_host_tree(scene);
   if (rh) return rh;
   auto p = std::make_unique<sq_proposal>();
-  p->ctx = ctx; p->scene = scene; p->cut = cut_depth;
+  p->ctx = ctx; p->scene = scene; p->tlas = tlas; p->cut = cut_depth;
   // Subtrees of the reference tree cut at `cut_depth`: nodes at that depth,
   // and leaves above it.  Their device refs and the subtree of every leaf.
   const sqh::RefTree& t = scene->tree;
@@ -707,6 +711,24 @@ int sq_proposal_create(sq_ctx* ctx, sq_bvh* scene, int cut_depth, float rff_scal
   ctx_retain(ctx);
   *out = p.release();
   return SQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sq_proposal_create(sq_ctx* ctx, sq_bvh* scene, int cut_depth, float rff_scale, uint64_t seed, sq_proposal** out) {
+  if (!ctx || !scene || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (scene->kind != KIND_TRI || scene->dim != 3) return fail(SQ_ERR_UNSUPPORTED, "the proposal serves TRIANGLE scenes (D = 3)");
+  return proposal_create(ctx, scene, nullptr, cut_depth, rff_scale, seed, out);
+}
+
+int sq_proposal_create_tlas(sq_ctx* ctx, sq_tlas* scene, int cut_depth, float rff_scale, uint64_t seed, sq_proposal** out) {
+  if (!ctx || !scene || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (tlas_dim(scene) != 3) return fail(SQ_ERR_UNSUPPORTED, "the proposal serves D = 3 scenes");
+  return proposal_create(ctx, tlas_top_tree(scene), scene, cut_depth, rff_scale, seed, out);
 }
 
 int sq_proposal_destroy(sq_proposal* p) {
@@ -920,6 +942,7 @@ int sq_proposal_infer(sq_proposal* p, const float* rays, uint64_t n, uint16_t* t
 int sq_raycast_nearest_proposed(sq_proposal* p, const float* rays, uint64_t n, float max_toi, const uint16_t* top, int32_t* out_leaf,
                                 float* out_toi, uint64_t* stats) {
   if (!p || (n && (!rays || !out_leaf || !out_toi))) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (p->tlas) return fail(SQ_ERR_INVALID, "the proposal serves a Tlas: use sq_tlas_raycast_nearest_proposed");
   if (n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "at most 2^32-1 rays per call");
   if (!n) return SQ_OK;
   sq_ctx* c = p->ctx;
@@ -947,6 +970,16 @@ int sq_raycast_nearest_proposed(sq_proposal* p, const float* rays, uint64_t n, f
         (uint32_t)c->descend_min);
   HIP_TRY(hipGetLastError());
   return end_timing(c);
+}
+
+// Exact nearest hit over a Tlas seeded by its proposal (the proposed
+// top-tree subtrees first): the guarded two-level walk's answer.
+int sq_tlas_raycast_nearest_proposed(sq_proposal* p, const float* rays, uint64_t n, float max_toi, const uint16_t* top,
+                                     int32_t* out_inst, float* out_toi, int32_t* out_sub, uint64_t* stats) {
+  if (!p || (n && (!rays || !out_inst || !out_toi))) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (!p->tlas) return fail(SQ_ERR_INVALID, "the proposal serves a single-level tree: use sq_raycast_nearest_proposed");
+  return tlas_nearest_proposed(p->tlas, rays, n, max_toi, top, p->sub_ref.as<uint32_t>(), p->n_sub, out_inst, out_toi, out_sub,
+                               stats);
 }
 
 // Test hook for the GEMM: C[M][N] (fp32) = A[M][K] * B[N][K]^T, bf16 inputs

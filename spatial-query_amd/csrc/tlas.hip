@@ -153,6 +153,109 @@ __global__ __launch_bounds__(256) void k_tlas_nearest_pool(TlasView tv, const fl
   pool_retire(counter);
 }
 
+// Proposal-seeded two-level nearest walk (D = 3) over guarded hits
+// (SQ_HITS_GUARDED): the proposed top-tree subtrees top[ray][0..3] (ids >=
+// n_sub mark empty slots) are walked first, each from its own ref, with the
+// shrinking limit; then the full walk runs from the root with that limit and
+// best hit, skipping the subtrees already walked.  A guarded hit lies inside
+// its instance's world box (and, for meshes, its triangle's box), so the
+// answer does not depend on which subtrees go first: it is the guarded
+// k_tlas_nearest_pool's.  top == NULL walks from the root at once.  COUNT adds
+// to stats[0] the rays whose proposed subtrees held a hit.
+template <int IK, bool COUNT>
+__global__ __launch_bounds__(256) void k_tlas_nearest_proposed(TlasView tv, const float* __restrict__ q, uint32_t n, float max_toi,
+                                                               const uint16_t* __restrict__ top, const uint32_t* __restrict__ sub_ref,
+                                                               uint32_t n_sub, int32_t* out_inst, float* out_toi, int32_t* out_sub,
+                                                               uint32_t chunk, uint32_t* counter, uint32_t S, SpillEntry* spill_top,
+                                                               SpillEntry* spill_bot, uint32_t spill_stride, uint32_t pause_below,
+                                                               unsigned long long* stats) {
+  constexpr int D = 3;
+  extern __shared__ uint32_t lds[];
+  const uint32_t B = blockDim.x, tid = threadIdx.x;
+  const uint32_t gt = blockIdx.x * B + tid;
+  RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill_top + gt, spill_stride};
+  RingStack sb{lds + 2 * S * B + tid, reinterpret_cast<float*>(lds + 3 * S * B) + tid, B, S, spill_bot + gt, spill_stride};
+  const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
+  const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
+  Chunks src{counter, n, chunk, home, waves};
+  const uint32_t lane = tid & 63u;
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  const uint32_t root_ref = fbits(tv.top.root[2 * D]);
+  RayD<D> r;
+  TlasWalk<D, false> w;
+  uint32_t rid = NO_LEAF, seeded = 0;
+  uint32_t done[4];   // subtrees this ray has walked; the root ref (never a child) fills empty slots
+  int phase = 0;      // 0..3: a proposed subtree, 4: the full walk from the root
+  uint64_t next = 0, end = 0;
+  bool exhausted = !src.get(next, end);
+  auto start = [&](int ph) {
+    w.sp = 0;
+    w.phase = 0;
+    if (ph == 0)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) done[k] = root_ref;
+    if (top)
+      for (; ph < 4; ++ph) {
+        const uint32_t c = top[(size_t)rid * 4 + ph];
+        if (c < n_sub) {
+          phase = ph;
+          w.cur = sub_ref[c];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) done[k] = k == ph ? w.cur : done[k];
+          return;
+        }
+      }
+    phase = 4;
+    if (w.res.inst != NO_LEAF) ++seeded;
+    w.cur = root_ref;
+  };
+  auto write = [&](uint32_t i) {
+    const bool hit = w.res.inst != NO_LEAF;
+    out_inst[i] = hit ? (int32_t)w.res.inst : -1;
+    out_toi[i] = w.res.t;
+    if (out_sub) out_sub[i] = (hit && w.res.sub != NO_LEAF) ? (int32_t)w.res.sub : -1;
+  };
+  while (true) {
+    bool need = rid == NO_LEAF && !exhausted;
+    uint64_t m = __ballot(need);
+    while (m) {
+      if (next >= end) {
+        if (!src.get(next, end)) { exhausted = true; break; }
+        continue;
+      }
+      const uint64_t avail = end - next;
+      const uint64_t rank = (uint64_t)__popcll(m & lt_mask);
+      if (need && rank < avail) {
+        need = false;
+        rid = (uint32_t)(next + rank);
+        ray_setup<D>(r, q + (size_t)rid * 2 * D);
+        if (!tlas_begin<D>(tv, r, max_toi, w)) {
+          write(rid);
+          rid = NO_LEAF;
+        } else {
+          start(0);
+        }
+      }
+      const uint64_t taken = (uint64_t)__popcll(m);
+      next += taken < avail ? taken : avail;
+      m = __ballot(need);
+    }
+    if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
+    const uint32_t active = (uint32_t)__popcll(__ballot(rid != NO_LEAF));
+    if (rid == NO_LEAF) continue;
+    if (!tlas_step<D, IK, false, false, true>(tv, r, st, sb, w, WavePause{exhausted ? 0u : pause_below, active}, done)) continue;
+    if (phase < 4) { start(phase + 1); continue; }   // this subtree is exhausted
+    write(rid);
+    rid = NO_LEAF;
+  }
+  if constexpr (COUNT) {
+    unsigned long long v = seeded;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0 && v) atomicAdd(stats, v);
+  }
+  pool_retire(counter);
+}
+
 // MODE 0: raycast_all (one hit per instance, tlas.rs:158-215); MODE 2: overlap
 // (every instance of a touched leaf node passing its test_overlap: exact for
 // balls, the default true for the other kinds, tlas.rs:393-432); MODE 3:
@@ -616,3 +719,54 @@ int sq_tlas_shapecast_nearest(sq_tlas* t, const float* casts, uint64_t n, float 
 }
 
 }  // extern "C"
+
+sq_bvh* tlas_top_tree(sq_tlas* t) { return t->top; }
+int tlas_dim(const sq_tlas* t) { return t->in.dim; }
+
+int tlas_nearest_proposed(sq_tlas* t, const float* rays, uint64_t n, float max_toi, const uint16_t* top, const uint32_t* sub_ref,
+                          uint32_t n_sub, int32_t* out_inst, float* out_toi, int32_t* out_sub, uint64_t* stats) {
+  if (n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "at most 2^32-1 rays per call");
+  if (t->in.dim != 3) return fail(SQ_ERR_UNSUPPORTED, "the proposal serves D = 3 scenes");
+  if (!n) return SQ_OK;
+  sq_ctx* c = t->ctx;
+  if (!t->in.n) return fill_miss(c, n, 3, NearestOut{out_inst, out_toi, nullptr, nullptr, out_sub, nullptr});
+  const int B = std::min(c->block, 256);
+  const uint32_t s_top = t->top->packed.max_stack, s_bot = t->blas_stack;
+  const uint32_t S = (uint32_t)c->lds_stack;
+  const uint32_t per = pooled_per_lane(c, n);
+  const uint64_t chunk = 64ull * per;
+  const uint64_t chunks = (n + chunk - 1) / chunk;
+  const uint32_t rows_top = s_top > S ? s_top - S : 0, rows_bot = s_bot > S ? s_bot - S : 0;
+  const size_t lds = (size_t)2 * S * B * 8;
+  TlasView tv = tview(t);
+  int rc = begin_timing(c);
+  if (rc) return rc;
+  auto launch = [&]<int K, bool COUNT>() -> int {
+    int per_cu = 0;
+    if (int r0 = blocks_per_cu(reinterpret_cast<const void*>(&k_tlas_nearest_proposed<K, COUNT>), c->device, B, lds, &per_cu))
+      return r0;
+    const uint64_t wpb = (uint64_t)B / 64;
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->cus * std::max(1, per_cu), (chunks + wpb - 1) / wpb));
+    const uint64_t lanes = grid * (uint64_t)B;
+    int r2 = c->spill.ensure((size_t)(rows_top + rows_bot) * lanes * sizeof(SpillEntry) + 64);
+    if (r2) return r2;
+    SpillEntry* sp = c->spill.as<SpillEntry>();
+    k_tlas_nearest_proposed<K, COUNT><<<(uint32_t)grid, B, lds, c->stream>>>(
+        tv, rays, (uint32_t)n, max_toi, top, sub_ref, n_sub, out_inst, out_toi, out_sub, (uint32_t)chunk, c->pool_ctr.as<uint32_t>(),
+        S, sp, sp + (size_t)rows_top * lanes, (uint32_t)lanes, (uint32_t)c->descend_min,
+        reinterpret_cast<unsigned long long*>(stats));
+    HIP_TRY(hipGetLastError());
+    return SQ_OK;
+  };
+  auto by_count = [&]<int K>() -> int {
+    return stats ? launch.template operator()<K, true>() : launch.template operator()<K, false>();
+  };
+  switch (t->in.kind) {
+    case INST_BOX: rc = by_count.template operator()<INST_BOX>(); break;
+    case INST_MESH: rc = by_count.template operator()<INST_MESH>(); break;
+    case INST_BALL: rc = by_count.template operator()<INST_BALL>(); break;
+    default: rc = fail(SQ_ERR_UNSUPPORTED, "instance kind");
+  }
+  if (rc) return rc;
+  return end_timing(c);
+}

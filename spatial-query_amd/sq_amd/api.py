@@ -835,20 +835,24 @@ class BatchSet:
 
 
 class Proposal:
-    """Learned subtree proposal for a TRIANGLE DeviceBvh (no reference counterpart).
+    """Learned subtree proposal for a TRIANGLE DeviceBvh or a D = 3 Tlas (no
+    reference counterpart).
 
-    An MLP predicts, per ray, the subtrees (the BVH cut at `cut_depth`) most
-    likely to hold the nearest hit.  `raycast_nearest` walks the top-4 first to
-    tighten t_max, then runs the full exact walk: results equal
-    DeviceBvh.raycast_nearest bit for bit whatever the network predicts.
-    Trained on the GPU (`train_step`, `fit`) from labels the exact walk makes.
+    An MLP predicts, per ray, the subtrees (the BVH cut at `cut_depth`; for a
+    Tlas, its top tree over the instances) most likely to hold the nearest
+    hit.  `raycast_nearest` walks the top-4 first to tighten t_max, then runs
+    the full exact walk: results equal the scene's raycast_nearest under
+    SQ_HITS_GUARDED bit for bit whatever the network predicts.  Trained on the
+    GPU (`train_step`, `fit`) from labels the exact walk makes (for a Tlas, the
+    hit instance ids).
     """
 
-    def __init__(self, bvh: DeviceBvh, cut_depth: int = 12, rff_scale: float = 1.0, seed: int = 0x5EED):
+    def __init__(self, scene, cut_depth: int = 12, rff_scale: float = 1.0, seed: int = 0x5EED):
         h = C.c_void_p()
-        check(lib().sq_proposal_create(bvh.ctx.handle, bvh._h, int(cut_depth), float(rff_scale),
-                                       int(seed) & ((1 << 64) - 1), C.byref(h)))
-        self._h, self.bvh, self.ctx, self.cut_depth = h, bvh, bvh.ctx, cut_depth
+        self.tlas = scene if isinstance(scene, Tlas) else None
+        create = lib().sq_proposal_create_tlas if self.tlas is not None else lib().sq_proposal_create
+        check(create(scene.ctx.handle, scene._h, int(cut_depth), float(rff_scale), int(seed) & ((1 << 64) - 1), C.byref(h)))
+        self._h, self.bvh, self.ctx, self.cut_depth = h, (None if self.tlas is not None else scene), scene.ctx, cut_depth
         _track(self)
         v = C.c_uint32()
         check(lib().sq_proposal_num_subtrees(self._h, C.byref(v)))
@@ -914,21 +918,34 @@ class Proposal:
         check(lib().sq_proposal_infer(self._h, _dp(rays, "float32", 6), rays.shape[0], _dp(out, "int16", n=rays.shape[0], per=4)))
         return out
 
-    def raycast_nearest_into(self, rays, max_toi, top, leaf, toi, stats=None):
+    def raycast_nearest_into(self, rays, max_toi, top, leaf, toi, stats=None, sub=None):
+        """leaf: the hit primitive (for a Tlas, the instance; sub, Tlas only, the sub-object)."""
         n = rays.shape[0]
+        if self.tlas is not None:
+            check(lib().sq_tlas_raycast_nearest_proposed(self._h, _dp(rays, "float32", 6), n, float(max_toi),
+                                                         _dp(top, "int16", n=n, per=4), _dp(leaf, "int32", n=n),
+                                                         _dp(toi, "float32", n=n), _dp(sub, "int32", n=n),
+                                                         _dp(stats, "int64", n=3)))
+            return
+        if sub is not None:
+            raise ValueError("sub-objects are reported for a Tlas proposal only")
         check(lib().sq_raycast_nearest_proposed(self._h, _dp(rays, "float32", 6), n, float(max_toi),
                                                 _dp(top, "int16", n=n, per=4), _dp(leaf, "int32", n=n),
                                                 _dp(toi, "float32", n=n), _dp(stats, "int64", n=3)))
 
     def raycast_nearest(self, rays, max_toi, top=None, count=False):
-        """Exact nearest hit walking `top` (None = the plain walk) first: dict(leaf, toi[, seeded, nodes, prims])."""
+        """Exact nearest hit walking `top` (None = the plain walk) first: dict(leaf, toi[, sub][, seeded, nodes,
+        prims]); for a Tlas, leaf is the instance and sub the sub-object, and only `seeded` is counted."""
         import torch
         n = rays.shape[0]
         leaf = torch.empty(n, dtype=torch.int32, device=rays.device)
         toi = torch.empty(n, dtype=torch.float32, device=rays.device)
+        sub = torch.empty(n, dtype=torch.int32, device=rays.device) if self.tlas is not None else None
         st = torch.zeros(3, dtype=torch.int64, device=rays.device) if count else None
-        self.raycast_nearest_into(rays, max_toi, top, leaf, toi, st)
+        self.raycast_nearest_into(rays, max_toi, top, leaf, toi, st, sub)
         out = dict(leaf=leaf, toi=toi)
+        if sub is not None:
+            out["sub"] = sub
         if count:
             s = st.cpu().tolist()
             out.update(seeded=s[0], nodes=s[1], prims=s[2])

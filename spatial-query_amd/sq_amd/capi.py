@@ -70,6 +70,7 @@ SIGNATURES = {
     "sq_batch_get_info": [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
     "sq_batch_destroy": [P],
     "sq_proposal_create": [P, P, I32, F32, U64, PP],
+    "sq_proposal_create_tlas": [P, P, I32, F32, U64, PP],
     "sq_proposal_destroy": [P],
     "sq_proposal_num_subtrees": [P, C.POINTER(C.c_uint32)],
     "sq_proposal_num_params": [P, C.POINTER(C.c_uint64)],
@@ -79,6 +80,7 @@ SIGNATURES = {
     "sq_proposal_features_spec": [P, P, P],
     "sq_proposal_infer": [P, P, U64, P],
     "sq_raycast_nearest_proposed": [P, P, U64, F32, P, P, P, P],
+    "sq_tlas_raycast_nearest_proposed": [P, P, U64, F32, P, P, P, P, P],
     "sq_debug_gemm": [P, P, P, I32, I32, I32, I32, P],
     "sq_ctx_set_tuning": [P, C.POINTER(Tuning)],
     "sq_ctx_set_hit_mode": [P, I32],

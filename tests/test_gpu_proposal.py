@@ -231,3 +231,201 @@ def test_training_gains_over_untrained(ctx, small):
     assert np.array_equal(w1["leaf"].cpu().numpy(), exact["leaf"].cpu().numpy())
     assert np.array_equal(w1["toi"].cpu().numpy().view(np.uint32), exact["toi"].cpu().numpy().view(np.uint32))
     p.close()
+
+
+# --- the proposal over a two-level scene (Tlas top tree cut into subtrees) ---
+
+@pytest.fixture(scope="module")
+def instanced(ctx):
+    sc = W.instanced_scene(n_meshes=5, n_inst=400, tri_lo=200, tri_hi=3000, extent=100.0)
+    meshes = [sq_amd.DeviceBvh.build(ctx, 3, sq_amd.TRIANGLE, m) for m in sc["meshes"]]
+    t = sq_amd.Tlas(ctx, 3, sq_amd.Tlas.MESH, sc["rot"], sc["trans"], meshes=meshes, mesh_of=sc["mesh_of"], scale=sc["scale"])
+    yield sc, t
+    t.close()
+    for m in meshes:
+        m.close()
+
+
+def _top_for(p, r, mode):
+    import torch
+    if mode in ("untrained", "short_tmax"):
+        return p.infer(r)
+    if mode == "random_ids":
+        g = torch.Generator(device="cuda"); g.manual_seed(9)
+        return torch.randint(0, p.n_subtrees, (r.shape[0], 4), device="cuda", generator=g).to(torch.int16)
+    if mode == "empty":
+        return torch.full((r.shape[0], 4), -1, dtype=torch.int16, device="cuda")
+    return None
+
+
+@pytest.mark.parametrize("mode", ["untrained", "random_ids", "empty", "none", "short_tmax"])
+def test_tlas_proposed_walk_is_exact(ctx, instanced, mode):
+    """Mesh instances: the seeded two-level walk returns the guarded
+    sq_tlas_raycast_nearest's instance, toi and triangle bit for bit whatever
+    the proposals, and the guarded CPU oracle's."""
+    print(f"case: tlas proposed walk, {mode}", flush=True)
+    sc, t = instanced
+    p = sq_amd.Proposal(t, cut_depth=6, seed=0x78)
+    assert 16 <= p.n_subtrees <= 64
+    rays = W.random_rays(30000, 0xC3, -120.0, 120.0)
+    r = dev(rays)
+    tmax = 40.0 if mode == "short_tmax" else 1000.0
+    plain = t.raycast_nearest(r, tmax, sub=True)
+    got = p.raycast_nearest(r, tmax, _top_for(p, r, mode), count=True)
+    gl, gt, gs = (got[k].cpu().numpy() for k in ("leaf", "toi", "sub"))
+    assert np.array_equal(gl, plain["leaf"].cpu().numpy())
+    assert np.array_equal(gt.view(np.uint32), plain["toi"].cpu().numpy().view(np.uint32))
+    assert np.array_equal(gs, plain["sub"].cpu().numpy())
+    hit = gl >= 0
+    assert hit.sum() > 200
+    if mode in ("random_ids", "short_tmax"):
+        o = oracle.InstanceScene(3, oracle.INST_MESH, sc["rot"], sc["trans"], mesh_tris=sc["meshes"], mesh_of=sc["mesh_of"],
+                                 scale=sc["scale"], guarded=True)
+        ref = o.raycast_nearest(rays, tmax, threads=8)
+        assert_nearest_equal(gl, gt, ref["leaf"], ref["t"], f"tlas proposed walk ({mode})")
+        assert np.array_equal(gs[hit], ref["sub"][hit])
+    if mode in ("empty", "none"):
+        assert got["seeded"] == 0
+    if mode == "random_ids":
+        assert got["seeded"] > 0
+    p.close()
+
+
+@pytest.mark.parametrize("kind", ["box", "ball"])
+def test_tlas_proposed_walk_box_and_ball_instances(ctx, kind):
+    print(f"case: tlas proposed walk, {kind} instances", flush=True)
+    n = 3000
+    s = W.Stream(0xB0B + (kind == "ball"))
+    rot = W.random_rotations(s, n)
+    tr = s.range(-100.0, 100.0, n * 3).reshape(n, 3)
+    if kind == "box":
+        halves = s.range(0.3, 3.0, n * 3).reshape(n, 3)
+        t = sq_amd.Tlas(ctx, 3, sq_amd.Tlas.BOX, rot, tr, halves=halves)
+        o = oracle.InstanceScene(3, oracle.INST_BOX, rot, tr, halves=halves, guarded=True)
+    else:
+        halves = s.range(0.3, 3.0, n)
+        t = sq_amd.Tlas(ctx, 3, sq_amd.Tlas.BALL, rot, tr, halves=halves)
+        o = oracle.InstanceScene(3, oracle.INST_BALL, rot, tr, halves=halves, guarded=True)
+    p = sq_amd.Proposal(t, cut_depth=9, seed=0x79)
+    rays = W.random_rays(40000, 0xC4, -120.0, 120.0)
+    r = dev(rays)
+    plain = t.raycast_nearest(r, 1000.0)
+    got = p.raycast_nearest(r, 1000.0, _top_for(p, r, "random_ids"))
+    gl, gt = got["leaf"].cpu().numpy(), got["toi"].cpu().numpy()
+    assert np.array_equal(gl, plain["leaf"].cpu().numpy())
+    assert np.array_equal(gt.view(np.uint32), plain["toi"].cpu().numpy().view(np.uint32))
+    ref = o.raycast_nearest(rays, 1000.0, threads=8)
+    assert (ref["leaf"] >= 0).sum() > 2000
+    assert_nearest_equal(gl, gt, ref["leaf"], ref["t"], f"tlas proposed walk ({kind} instances)")
+    p.close()
+    t.close()
+
+
+def _cone_rays(n, seed, origin, half_angle_deg):
+    """Rays from one origin, directions uniform in the cone of the given half
+    angle around the direction to the scene's centre (the world origin)."""
+    s = W.Stream(seed)
+    u = s.range(0.0, 1.0, 2 * n).reshape(n, 2).astype(np.float64)
+    o = np.asarray(origin, np.float64)
+    f = -o / np.linalg.norm(o)
+    a = np.array([1.0, 0.0, 0.0]) if abs(f[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    e1 = np.cross(f, a); e1 /= np.linalg.norm(e1)
+    e2 = np.cross(f, e1)
+    ct = 1.0 - u[:, 0] * (1.0 - np.cos(np.radians(half_angle_deg)))
+    st = np.sqrt(np.maximum(0.0, 1.0 - ct * ct))
+    ph = 2.0 * np.pi * u[:, 1]
+    d = ct[:, None] * f + (st * np.cos(ph))[:, None] * e1 + (st * np.sin(ph))[:, None] * e2
+    d = W.normalize_or_zero(d.astype(np.float32))
+    return W.rays_array(np.broadcast_to(o.astype(np.float32), d.shape), d)
+
+
+def test_tlas_training_gains_over_untrained(ctx, instanced):
+    """Rays from one viewpoint into the instance cloud (training and
+    evaluation directions drawn from the same cone with different seeds);
+    labels are the top-tree subtrees of the hit instances.  The trained
+    network proposes the hit subtree far more often than the untrained one,
+    seeds more rays, and the results stay exact."""
+    print("case: tlas proposal training gain", flush=True)
+    sc, t = instanced
+    p = sq_amd.Proposal(t, cut_depth=6, rff_scale=1.0, seed=0x4)
+    eye = (60.0, -230.0, 90.0)
+    train = _cone_rays(1 << 18, 0x7A2, eye, 25.0)
+    rt = dev(train)
+    lab = p.labels(t.raycast_nearest(rt, 1000.0)["leaf"])
+    evalr = _cone_rays(1 << 16, 0xE7A2, eye, 25.0)
+    re = dev(evalr)
+    exact = t.raycast_nearest(re, 1000.0, sub=True)
+    lab_e = p.labels(exact["leaf"]).cpu().numpy()
+    ok = lab_e >= 0
+
+    def hit_rate(top):
+        tt = top.cpu().numpy().view(np.uint16).astype(np.int64)
+        return float((tt[ok] == lab_e[ok, None]).any(axis=1).mean())
+
+    top0 = p.infer(re)
+    h0 = hit_rate(top0)
+    w0 = p.raycast_nearest(re, 1000.0, top0, count=True)
+    losses = p.fit(rt, lab, epochs=8, batch=8192, lr=1e-3)
+    top1 = p.infer(re)
+    h1 = hit_rate(top1)
+    w1 = p.raycast_nearest(re, 1000.0, top1, count=True)
+    print(f"subtrees {p.n_subtrees}; labelled eval rays {int(ok.sum())}; loss {losses[0]:.3f} -> {losses[-1]:.3f}; "
+          f"top-4 hit rate {h0:.3f} -> {h1:.3f}; seeded rays {w0['seeded']} -> {w1['seeded']}", flush=True)
+    assert ok.sum() > 5000
+    assert losses[-1] < losses[0]
+    assert h1 >= h0 + 0.3
+    assert w1["seeded"] > 2 * w0["seeded"]
+    for k in ("leaf", "sub"):
+        assert np.array_equal(w1[k].cpu().numpy(), exact[k].cpu().numpy())
+    assert np.array_equal(w1["toi"].cpu().numpy().view(np.uint32), exact["toi"].cpu().numpy().view(np.uint32))
+    p.close()
+
+
+def test_c3_configuration_heightfield_cut12(ctx):
+    """The learned proposal at its own configuration on the full 2,097,152-
+    triangle heightfield: cut depth 12 (<= 4,096 subtrees), a short Adam run
+    (1M labelled training rays, lr 1e-3, batch 65,536), then 1M fresh
+    evaluation rays from the heightfield distributions (half camera rays at
+    pixel centres, half incoherent): the seeded walk equals the plain walk on
+    every ray and the guarded CPU oracle on a sample, and training raises the
+    camera rays' top-4 hit rate."""
+    print("case: C3 configuration, 2M-triangle heightfield, cut depth 12", flush=True)
+    import torch
+    sc = W.heightfield_scene()
+    bvh = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.TRIANGLE, sc.prims)
+    p = sq_amd.Proposal(bvh, cut_depth=12, rff_scale=1.0, seed=0xC3)
+    assert 2048 < p.n_subtrees <= 4096
+    train = W.proposal_training_rays(sc, 1 << 20, seed=0xC3C3)
+    rt, lab, _ = _labelled(bvh, p, train)
+    cam = W.camera_rays(sc, res=2048)[::8]                       # 524,288 pixel-centre rays
+    inc = W.incoherent_rays(sc, 1 << 19, seed=0xC3E0)
+    evalr = np.ascontiguousarray(np.concatenate([cam, inc]))
+    re, lab_e, exact = _labelled(bvh, p, evalr)
+    lab_e = lab_e.cpu().numpy()
+    sl = slice(0, cam.shape[0])
+
+    def hit_rate(top):
+        tt = top.cpu().numpy().view(np.uint16).astype(np.int64)[sl]
+        l = lab_e[sl]
+        ok = l >= 0
+        return float((tt[ok] == l[ok, None]).any(axis=1).mean())
+
+    h0 = hit_rate(p.infer(re))
+    losses = p.fit(rt, lab, epochs=1, batch=65536, lr=1e-3)
+    top = p.infer(re)
+    h1 = hit_rate(top)
+    got = p.raycast_nearest(re, 2000.0, top, count=True)
+    torch.cuda.synchronize()
+    n = re.shape[0]
+    print(f"subtrees {p.n_subtrees}; loss {losses[0]:.3f}; camera top-4 hit rate {h0:.3f} -> {h1:.3f}; "
+          f"seeded {got['seeded']} of {n}", flush=True)
+    gl, gt = got["leaf"].cpu().numpy(), got["toi"].cpu().numpy()
+    assert np.array_equal(gl, exact["leaf"].cpu().numpy())
+    assert np.array_equal(gt.view(np.uint32), exact["toi"].cpu().numpy().view(np.uint32))
+    idx = np.concatenate([np.arange(0, cam.shape[0], 256), cam.shape[0] + np.arange(0, inc.shape[0], 256)])
+    ref = oracle.Scene(3, oracle.TRI, sc.prims, guarded=True).raycast_nearest(evalr[idx], 2000.0, threads=8)
+    assert_nearest_equal(gl[idx], gt[idx], ref["leaf"], ref["t"], "C3 proposed walk vs oracle sample")
+    assert (ref["leaf"] >= 0).mean() > 0.5
+    assert h1 > h0 + 0.2
+    p.close()
+    bvh.close()
