@@ -178,6 +178,21 @@ int sq_bvh_build_device(sq_ctx* ctx, int dim, int kind, const float* prims_devic
  * the tree stay valid; a batch set sized its traversal stack when it was
  * made, so make it again if the rebuilt tree is deeper (stats max_depth). */
 int sq_bvh_rebuild(sq_bvh* bvh, const float* prims_device);
+/* Interior surface sum (tree.rs:104-110, the Tlas::needs_rebuild figure,
+ * tlas.rs:56-60) of the tree's current device bounds, computed on the device:
+ * each node's half surface in f32 as the reference computes it, summed in f64
+ * in a fixed order (deterministic; within f32 rounding of the reference's
+ * running f32 sum).  Synchronises. */
+int sq_bvh_surface_sum(sq_bvh* bvh, double* out);
+/* Refit, then rebuild when the refit tree has loosened: the policy of
+ * Tlas::needs_rebuild (tlas.rs:56-60) applied to a moving single-level tree.
+ * sq_bvh_refit on prims_device, then, when the device surface sum exceeds
+ * factor (>= 1) times the sum of the tree as last built (taken at the first
+ * call, and after every rebuild), sq_bvh_rebuild on the same primitives;
+ * *rebuilt = 1 then.  Synchronises (not for graph capture); a rebuild keeps
+ * the device arrays of a device-built tree, and sq_batch_check says whether
+ * a batch set (or a graph of its submits) over the tree is still valid. */
+int sq_bvh_refit_auto(sq_bvh* bvh, const float* prims_device, float factor, int* rebuilt);
 
 /* ------------------------------------------------------------------------
  * Consumer narrow phases: GpuGeometry::narrow_wgsl (gpu/mod.rs:86-99,261),
@@ -447,6 +462,11 @@ typedef struct sq_batch_query {
 } sq_batch_query;
 int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* queries, uint32_t n_queries, sq_batch** out);
 int sq_batch_submit(sq_batch* batch);
+/* SQ_OK while every scene of the set keeps the arrays and the stack depth the
+ * set was made for (sq_batch_submit checks the same first; a captured graph of
+ * submits does not), SQ_ERR_INVALID once one was rebuilt into new arrays or
+ * deeper: make the set (and its graph) again. */
+int sq_batch_check(sq_batch* batch);
 /* Chunks in the set, and the persistent grid (blocks x threads). */
 int sq_batch_get_info(const sq_batch* batch, uint32_t* n_chunks, uint32_t* grid, uint32_t* block);
 int sq_batch_destroy(sq_batch* batch);

@@ -242,16 +242,22 @@ int sq_batch_create(sq_ctx* ctx, int scene_class, const sq_batch_query* qs, uint
   return SQ_OK;
 }
 
-int sq_batch_submit(sq_batch* b) {
+int sq_batch_check(sq_batch* b) {
   if (!b) return fail(SQ_ERR_INVALID, "NULL argument");
-  sq_ctx* c = b->ctx;
   for (const auto& sv : b->scenes) {
     const SceneView now = view_of(sv.first);
     if (now.nodes != sv.second.nodes || now.prims != sv.second.prims || now.ids != sv.second.ids ||
         now.root != sv.second.root || sv.first->packed.max_stack > b->stack_cap)
       return fail(SQ_ERR_INVALID, "a scene of this batch set was rebuilt into new arrays or deeper; make the batch set again");
   }
-  int rc = begin_timing(c);
+  return SQ_OK;
+}
+
+int sq_batch_submit(sq_batch* b) {
+  int rc = sq_batch_check(b);
+  if (rc) return rc;
+  sq_ctx* c = b->ctx;
+  rc = begin_timing(c);
   if (rc) return rc;
   for (const auto& e : b->empty)
     if ((rc = fill_miss(c, e.first, b->dim, e.second))) return rc;

@@ -791,6 +791,7 @@ int sq_bvh_rebuild(sq_bvh* b, const float* prims_device) {
   if (!b->n_prims) return SQ_OK;
   int rc = device_build(b, prims_device);
   if (rc) return rc;
+  b->auto_baseline = 0.0;          // sq_bvh_refit_auto takes the rebuilt tree as its new baseline
   HIP_TRY(hipStreamSynchronize(b->ctx->stream));
   return SQ_OK;
 }

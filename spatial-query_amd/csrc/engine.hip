@@ -602,6 +602,67 @@ __global__ void k_refit_root(const float* nodes, const float* __restrict__ prims
   for (int k = 0; k < D; ++k) { root[k] = mn[k]; root[D + k] = mx[k]; }
 }
 
+// Interior surface sum (tree.rs:104-110) on the device: Aabb::half_surface
+// (aabb.rs:84-97) of every interior node of the reference tree, each term in
+// f32 in the reference's operation order, summed in f64 by a fixed-order
+// reduction (the same value on every call for the same bounds; the reference's
+// f32 running sum in node order can differ from it in its last bits).  The
+// interior nodes are the root when its ref is a pair node, and every child slot
+// of a pair node whose ref is a pair node that is not a synthetic chain node
+// (a synthetic chain stands for one over-full reference leaf).
+template <int D>
+__device__ __forceinline__ float half_surface_of(const float* lo, const float* hi, int stride) {
+  float sum = 0.0f;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    float prod = 1.0f;
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+      if (c != a) prod = prod * (hi[c * stride] - lo[c * stride]);
+    sum = sum + prod;
+  }
+  return sum;
+}
+
+template <int D>
+__device__ __forceinline__ bool interior_ref(const float* nodes, uint32_t r) {
+  constexpr int NF = (D + 1) * 4;
+  return !(r & LEAF_BIT) && __float_as_uint(nodes[(size_t)r * NF + 4 * D + 3]) == 0u;
+}
+
+constexpr int SURF_BLOCKS = 512;
+
+template <int D>
+__global__ __launch_bounds__(256) void k_surface_partials(const float* __restrict__ nodes, uint32_t n_pairs, double* part) {
+  constexpr int NF = (D + 1) * 4;
+  __shared__ double red[256];
+  double s = 0.0;
+  for (uint32_t p = blockIdx.x * 256u + threadIdx.x; p < n_pairs; p += gridDim.x * 256u) {
+    const float* nd = nodes + (size_t)p * NF;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      if (interior_ref<D>(nodes, __float_as_uint(nd[4 * D + c]))) s += (double)half_surface_of<D>(nd + 2 * c, nd + 2 * c + 1, 4);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// The partials in block order plus the root's term.
+template <int D>
+__global__ void k_surface_total(const double* __restrict__ part, uint32_t n_part, const float* __restrict__ nodes,
+                                const float* __restrict__ root, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (uint32_t i = 0; i < n_part; ++i) s += part[i];
+  if (interior_ref<D>(nodes, __float_as_uint(root[2 * D]))) s += (double)half_surface_of<D>(root, root + D, 1);
+  *out = s;
+}
+
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
@@ -1103,6 +1164,60 @@ int sq_bvh_refit(sq_bvh* b, const float* prims_device) {
     HIP_TRY(hipGetLastError());
     return SQ_OK;
   });
+}
+
+// The device interior surface sum of b's current bounds into *out (host);
+// synchronises the context's stream.
+static int surface_sum(sq_bvh* b, double* out) {
+  sq_ctx* c = b->ctx;
+  *out = 0.0;
+  const uint32_t n_pairs = b->packed.n_pairs;
+  int rc = c->scratch.ensure((SURF_BLOCKS + 1) * sizeof(double));
+  if (rc) return rc;
+  double* part = c->scratch.as<double>();
+  auto go = [&]<int D>() -> int {
+    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(SURF_BLOCKS, (n_pairs + 255) / 256));
+    k_surface_partials<D><<<blocks, 256, 0, c->stream>>>(b->nodes.as<float>(), n_pairs, part);
+    HIP_TRY(hipGetLastError());
+    k_surface_total<D><<<1, 64, 0, c->stream>>>(part, blocks, b->nodes.as<float>(), b->root.as<float>(), part + SURF_BLOCKS);
+    HIP_TRY(hipGetLastError());
+    return SQ_OK;
+  };
+  switch (b->dim) {
+    case 2: rc = go.template operator()<2>(); break;
+    case 3: rc = go.template operator()<3>(); break;
+    case 4: rc = go.template operator()<4>(); break;
+    case 5: rc = go.template operator()<5>(); break;
+    case 6: rc = go.template operator()<6>(); break;
+    default: return fail(SQ_ERR_UNSUPPORTED, "dimension not instantiated");
+  }
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(out, part + SURF_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return SQ_OK;
+}
+
+int sq_bvh_surface_sum(sq_bvh* b, double* out) {
+  if (!b || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = 0.0;
+  if (!b->n_prims) return SQ_OK;
+  return surface_sum(b, out);
+}
+
+int sq_bvh_refit_auto(sq_bvh* b, const float* prims_device, float factor, int* rebuilt) {
+  if (!b || !rebuilt || (b->n_prims && !prims_device)) return fail(SQ_ERR_INVALID, "NULL argument");
+  *rebuilt = 0;
+  if (!(factor >= 1.0f)) return fail(SQ_ERR_INVALID, "factor must be >= 1");
+  if (!b->n_prims) return SQ_OK;
+  int rc;
+  if (!(b->auto_baseline > 0.0) && (rc = surface_sum(b, &b->auto_baseline))) return rc;   // the tree as built
+  if ((rc = sq_bvh_refit(b, prims_device))) return rc;
+  double q = 0.0;
+  if ((rc = surface_sum(b, &q))) return rc;
+  if (!(q > (double)factor * b->auto_baseline)) return SQ_OK;
+  if ((rc = sq_bvh_rebuild(b, prims_device))) return rc;
+  *rebuilt = 1;
+  return surface_sum(b, &b->auto_baseline);
 }
 
 static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, const float* tmax, const sq_filter* f,

@@ -106,6 +106,7 @@ struct sq_bvh {
   uint64_t ref_nodes = 0;
   DevBuf ref_ca, ref_cb, ref_pc;
   uint32_t consumer_pf = 0;        // trees over consumer primitives (rtc.hip): floats per primitive
+  double auto_baseline = 0.0;      // sq_bvh_refit_auto: surface sum of the tree as last built (0 = not yet taken)
 };
 
 // Exclusive scan of n counts into out[0..n] (out[n] = total) on the context's

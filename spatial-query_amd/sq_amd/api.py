@@ -280,6 +280,20 @@ class DeviceBvh:
         """Build the tree again, on the device, from moved primitives (same count)."""
         check(lib().sq_bvh_rebuild(self._h, _dp(prims_dev, "float32", n=self.n_prims, per=self._stride())))
 
+    def surface_sum(self) -> float:
+        """Interior surface sum of the current device bounds (tree.rs:104-110), summed on the device."""
+        v = C.c_double()
+        check(lib().sq_bvh_surface_sum(self._h, C.byref(v)))
+        return v.value
+
+    def refit_auto(self, prims_dev, factor: float = 1.5) -> bool:
+        """Refit to moved primitives, and rebuild on the device once the surface sum exceeds factor x the
+        sum of the tree as last built (Tlas::needs_rebuild's test); True when it rebuilt."""
+        r = C.c_int()
+        check(lib().sq_bvh_refit_auto(self._h, _dp(prims_dev, "float32", n=self.n_prims, per=self._stride()),
+                                      float(factor), C.byref(r)))
+        return bool(r.value)
+
     # -- management ---------------------------------------------------------
     def set_layers(self, layers):
         l = None if layers is None else _host(layers, np.uint64)
@@ -819,6 +833,10 @@ class BatchSet:
 
     def submit(self):
         check(lib().sq_batch_submit(self._h))
+
+    def valid(self) -> bool:
+        """False once a scene was rebuilt into new arrays or deeper (make the set, and any graph of it, again)."""
+        return lib().sq_batch_check(self._h) == 0
 
     def info(self) -> dict:
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()

@@ -234,7 +234,75 @@ def c2(frames=60, n_batches=256, tune=True, check=True, seed=0xC2C2_0000_000A):
                                  frame_ms=[round(v, 2) for v in rms], parity=parity2)
     log(f"{frames} rebuild frames: median {rm:.2f} ms (rebuild {np.median(rb_ms):.2f} ms), first {rms[0]:.2f}, "
         f"last {rms[-1]:.2f}")
+
+    # the same frames under the refit-or-rebuild policy (sq_bvh_refit_auto:
+    # refit, and rebuild on the device once the tree's surface sum passes
+    # factor x that of the tree as last built, Tlas::needs_rebuild's test):
+    # frame = policy step + graph replay of the two submits; factors 1.1 to 2
+    # swept, the fastest reported as policy_frames
+    def record():
+        b = sq_amd.BatchSet(ctx, [(dbvh, r, tm) for (_, r, tm) in qb])
+        bt.submit()
+        b.submit()
+        torch.cuda.synchronize()
+        gg = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gg, stream=stream):
+            bt.submit()
+            b.submit()
+        return b, gg
+
     bb2.close()
+    state = {}
+
+    def policy_run(factor):
+        prims_dev.copy_(torch.from_numpy(boxes))
+        dbvh.rebuild(prims_dev)
+        if state.get("set") is None or not state["set"].valid():
+            if state.get("set") is not None:
+                state["set"].close()
+            state["set"], state["graph"] = record()
+        pms, rebuilt, recorded, par = [], [], 0, []
+        for f in range(frames):
+            moved = boxes.copy()
+            d = vel * np.float32(f)
+            moved[:, :3] += d
+            moved[:, 3:] += d
+            prims_dev.copy_(torch.from_numpy(moved))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = dbvh.refit_auto(prims_dev, factor)
+            if r and not state["set"].valid():
+                state["set"].close()
+                state["set"], state["graph"] = record()
+                recorded += 1
+            with torch.cuda.stream(stream):
+                state["graph"].replay()
+            stream.synchronize()
+            pms.append((time.perf_counter() - t0) * 1e3)
+            rebuilt.append(bool(r))
+            if check and f in (0, frames // 2, frames - 1):
+                ok = True
+                for i, (_, r_, tm) in enumerate(qb):
+                    w = dbvh.raycast_nearest(r_, tm)
+                    ctx.synchronize()
+                    ok &= _same(state["set"].leaf[i].cpu(), w["leaf"].cpu()) and _same_bits(state["set"].toi[i].cpu(),
+                                                                                            w["toi"].cpu())
+                par.append(dict(frame=f, batches_equal_single_calls=bool(ok)))
+        pm = float(np.median(pms))
+        log(f"{frames} policy frames (factor {factor}): median {pm:.2f} ms, {sum(rebuilt)} rebuilds, "
+            f"max {np.max(pms):.2f} ms, batches == single calls {all(p_['batches_equal_single_calls'] for p_ in par)}")
+        return dict(factor=factor, frame_ms_median=round(pm, 3), frame_ms_first=round(pms[0], 3),
+                    frame_ms_last=round(pms[-1], 3), frame_ms_max=round(float(np.max(pms)), 3),
+                    rebuilds=int(sum(rebuilt)), rebuild_frames=[i for i, v in enumerate(rebuilt) if v],
+                    graphs_recorded_again=recorded, mrays_per_s=round(float(sizes.sum()) / (pm * 1e-3) / 1e6, 1),
+                    frame_ms=[round(v, 2) for v in pms], parity=par)
+
+    runs = [policy_run(fc) for fc in (1.1, 1.25, 1.5, 2.0)]
+    best = min(runs, key=lambda r_: r_["frame_ms_median"])
+    out["policy_frames"] = best
+    out["policy_factor_sweep"] = [dict(factor=r_["factor"], frame_ms_median=r_["frame_ms_median"],
+                                       frame_ms_max=r_["frame_ms_max"], rebuilds=r_["rebuilds"]) for r_ in runs]
+    state["set"].close()
     return out
 
 

@@ -67,6 +67,7 @@ SIGNATURES = {
     "sq_ctx_set_block": [P, I32],
     "sq_batch_create": [P, I32, P, U32, PP],
     "sq_batch_submit": [P],
+    "sq_batch_check": [P],
     "sq_batch_get_info": [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
     "sq_batch_destroy": [P],
     "sq_proposal_create": [P, P, I32, F32, U64, PP],
@@ -103,6 +104,8 @@ SIGNATURES = {
     "sq_bvh_refit": [P, P],
     "sq_bvh_build_device": [P, I32, I32, P, U64, PP],
     "sq_bvh_rebuild": [P, P],
+    "sq_bvh_surface_sum": [P, C.POINTER(C.c_double)],
+    "sq_bvh_refit_auto": [P, P, F32, C.POINTER(C.c_int)],
     "sq_narrow_create": [P, I32, C.c_char_p, U32, PP],
     "sq_narrow_destroy": [P],
     "sq_bvh_build_consumer": [P, P, P, U64, PP],
