@@ -118,15 +118,21 @@ class Context:
     """A device and its stream (GpuRaycaster::headless: raises SqError(NO_DEVICE) when absent)."""
 
     def __init__(self, device: int = 0, use_torch_stream: bool = True):
+        ts = None
+        if use_torch_stream:
+            # torch sets up its device runtime first: a process whose HIP
+            # runtime the engine started before torch's first device call
+            # finds no devices in torch
+            import torch
+            ts = torch.cuda.current_stream(device).cuda_stream
         h = C.c_void_p()
         check(lib().sq_ctx_create(device, C.byref(h)))
         self._h = h
         self.device = device
         _track(self)
-        if use_torch_stream:
+        if ts is not None:
             # Order engine work with the framework's work on the same stream.
-            import torch
-            self.set_stream(torch.cuda.current_stream(device).cuda_stream)
+            self.set_stream(ts)
 
     def set_stream(self, stream_ptr: int | None):
         """Enqueue on this hipStream_t (0 = the legacy default stream); None = the context's own."""

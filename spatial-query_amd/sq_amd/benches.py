@@ -468,12 +468,16 @@ def consumer():
 def dispatch():
     """The HIP backend's constants for the reference's dispatch cost model
     (crates/spatial-query/src/dispatch.rs:38-116): the host round trip of a
-    nearest-hit batch (rays from pinned host memory, results back, synchronised)
-    on a 100,000-ball scene, as GPU_FIXED_US (a batch of one) and the per-ray
-    slope over depth^2 (depth = log2(leaves + 1)), split between traversal and
-    narrow test in the reference's proportion."""
-    import torch
+    nearest-hit batch through sq_amd.dispatch.HipBackend (rays through pinned
+    host memory, leaf + toi back, synchronised) on a 100,000-ball scene.  A
+    batch of up to ~100 rays takes as long as its longest ray's chain of
+    dependent node fetches, so the constants are the line through the round
+    trips of 256 and 1M rays: GPU_FIXED_US its intercept, the per-ray slope over
+    depth^2 (depth = log2(leaves + 1)) split between traversal and narrow test
+    in the reference's proportion."""
+    import torch  # noqa: F401
     import sq_amd
+    from sq_amd import dispatch as dp
     from sq_amd import workloads as W
 
     ctx = sq_amd.Context(0)
@@ -483,45 +487,37 @@ def dispatch():
     r = s.range(0.1, 2.0, n).reshape(n, 1)
     bvh = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.SPHERE, np.concatenate([c, r], 1).astype(np.float32))
     big = 1 << 20
-    rays_h = torch.from_numpy(W.random_rays(big, 0xD16, -120.0, 120.0)).pin_memory()
-    rays_d = torch.empty_like(rays_h, device="cuda")
-    leaf_h = torch.empty(big, dtype=torch.int32).pin_memory()
-    toi_h = torch.empty(big, dtype=torch.float32).pin_memory()
-    stream = torch.cuda.current_stream()
-
-    def round_trip(b):
-        rays_d[:b].copy_(rays_h[:b], non_blocking=True)
-        out = bvh.raycast_nearest(rays_d[:b], 400.0)
-        leaf_h[:b].copy_(out["leaf"], non_blocking=True)
-        toi_h[:b].copy_(out["toi"], non_blocking=True)
-        stream.synchronize()
+    rays = W.random_rays(big, 0xD16, -120.0, 120.0)
+    be = dp.HipBackend(bvh)
 
     res = {}
-    for b in (1, 16, 256, 4096, 65536, big):
+    for b in (1, 4, 16, 64, 256, 1024, 4096, 65536, big):
         reps = 50 if b <= 4096 else 10
-        round_trip(b)
+        be.raycast_nearest_batch(rays[:b], 400.0)
         ts = []
         for _ in range(reps):
             t0 = time.perf_counter()
-            round_trip(b)
+            be.raycast_nearest_batch(rays[:b], 400.0)
             ts.append((time.perf_counter() - t0) * 1e6)
         res[b] = float(np.median(ts))
-    depth = math.log2(n + 1)
-    fixed = res[1]
-    per_ray = (res[big] - res[1]) / (big - 1)
-    coef = per_ray / depth ** 2
-    ref_trav, ref_narrow = 0.00004, 0.000153          # the reference's split (dispatch.rs:66-78)
-    cpu_per_ray = (0.0020 + 0.0078) * depth ** 2       # the reference's CPU model at hint = 1 (dispatch.rs:48-55)
-    crossover = fixed / max(cpu_per_ray - per_ray, 1e-9)
+    depth = dp.depth(n)
+    per_ray = (res[big] - res[256]) / (big - 256)
+    fixed = res[256] - 256 * per_ray
+    costs = dp.fit_gpu(dp.Costs.reference(), fixed, per_ray, n)
+    d = dp.Dispatcher(dp.Costs.reference())
+    d.register(dp.HipBackend(bvh, costs))
+    crossover = d.crossover(n)
     out = dict(workload="dispatch cost constants: 100,000 balls (depth log2(leaves+1) = %.2f), host round trip "
-                        "(pinned rays in, leaf + toi out, synchronised)" % depth,
+                        "through HipBackend (pinned rays in, leaf + toi out, synchronised)" % depth,
                round_trip_us={str(k): round(v, 1) for k, v in res.items()},
                GPU_FIXED_US=round(fixed, 1), per_ray_us=round(per_ray, 6),
-               GPU_TRAVERSAL_US_PER_LEVEL2=float("%.3g" % (coef * ref_trav / (ref_trav + ref_narrow))),
-               GPU_NARROW_US_PER_LEVEL2=float("%.3g" % (coef * ref_narrow / (ref_trav + ref_narrow))),
+               GPU_TRAVERSAL_US_PER_LEVEL2=float("%.3g" % costs.gpu_traversal_us_per_level2),
+               GPU_NARROW_US_PER_LEVEL2=float("%.3g" % costs.gpu_narrow_us_per_level2),
+               fit="line through the round trips of 256 and 1,048,576 rays",
                reference_GPU_FIXED_US=1350.0, reference_per_ray_us=0.053,
-               model_crossover_batch_vs_reference_cpu=round(crossover, 1))
-    log(f"fixed {fixed:.1f} us, per ray {per_ray * 1e3:.3f} ns, crossover vs the reference CPU model {crossover:.1f} rays")
+               model_crossover_batch_vs_reference_cpu=crossover)
+    log(f"fixed {fixed:.1f} us, per ray {per_ray * 1e3:.3f} ns, model crossover vs the reference CPU constants "
+        f"{crossover} rays")
     return out
 
 
