@@ -122,16 +122,20 @@ class Context:
         if use_torch_stream:
             # torch sets up its device runtime first: a process whose HIP
             # runtime the engine started before torch's first device call
-            # finds no devices in torch
+            # finds no devices in torch.  No device: sq_ctx_create reports it.
             import torch
-            ts = torch.cuda.current_stream(device).cuda_stream
+            if torch.cuda.device_count() > device:
+                ts = torch.cuda.current_stream(device).cuda_stream
         h = C.c_void_p()
         check(lib().sq_ctx_create(device, C.byref(h)))
         self._h = h
         self.device = device
         _track(self)
-        if ts is not None:
+        if use_torch_stream:
             # Order engine work with the framework's work on the same stream.
+            if ts is None:
+                import torch
+                ts = torch.cuda.current_stream(device).cuda_stream
             self.set_stream(ts)
 
     def set_stream(self, stream_ptr: int | None):
