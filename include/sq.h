@@ -287,6 +287,20 @@ int sq_raycast_nearest(sq_bvh* bvh, const float* rays, uint64_t n, float max_toi
 int sq_raycast_nearest_noalloc(sq_bvh* bvh, const float* rays, uint64_t n, float max_toi, const float* max_toi_per_ray,
                                const sq_filter* filter, int32_t* out_leaf, float* out_toi, float* out_point,
                                float* out_normal, int32_t* out_sub);
+/* Bvh::raycast_nearest_stats / raycast_all_stats (bvh.rs:61-70, 206-215,
+ * QueryStats stats.rs:39-50): per ray, out_stats[i] = {nodes_visited,
+ * aabb_tests, narrow_tests, hits} as the reference counts them on its own walk
+ * (every popped node a visit and an AABB test, every filter-accepted primitive
+ * of a passing leaf a narrow test, every narrow test that hits a hit), from a
+ * walk in the reference's order over the reference's hits whatever the
+ * context's hit mode; device uint32[n][4].  The nearest form also writes the
+ * nearest hit (leaf, toi); the all-hits form only the counters (the hit lists
+ * come from sq_raycast_all).  Built-in kinds (AABB, SPHERE, TRIANGLE).  A
+ * diagnostic walk, one ray per thread: not the fast path. */
+int sq_raycast_nearest_stats(sq_bvh* bvh, const float* rays, uint64_t n, float max_toi, const float* max_toi_per_ray,
+                             const sq_filter* filter, int32_t* out_leaf, float* out_toi, uint32_t* out_stats);
+int sq_raycast_all_stats(sq_bvh* bvh, const float* rays, uint64_t n, float max_toi, const sq_filter* filter,
+                         uint32_t* out_stats);
 /* Occlusion: out_hit[i] = raycast_nearest(..).is_some() (the reference's any-hit
  * idiom, examples-viewport/examples/many_rays.rs:193-201), ending each ray at
  * its first accepted hit. */

@@ -73,6 +73,7 @@ def _declare(L):
         "orc_bvh_free": (None, [P]),
         "orc_raycast_nearest_batch": (None, [P, P, P, U64, F32, U64, I64, I32, I32, P, P, P, P, P, P]),
         "orc_raycast_all_batch": (U64, [P, P, P, U64, F32, U64, I64, I32, I32, P, P, P, U64]),
+        "orc_raycast_all_stats": (None, [P, P, P, U64, F32, U64, I64, P]),
         "orc_overlap_batch": (U64, [P, P, P, U64, U64, I64, I32, P, P, U64]),
         "orc_shapecast_nearest_batch": (None, [P, P, P, U64, P, U64, I64, I32, P, P, P, P]),
         "orc_shapecast_all_batch": (U64, [P, P, P, U64, P, U64, I64, I32, P, P, P, U64]),
@@ -256,6 +257,13 @@ class Scene:
         L.orc_raycast_all_batch(self._b, self._g, _p(rays), n, max_toi, mask, exclude, int(naive), int(crossings),
                                 _p(off), _p(leaf), _p(t), total)
         return off, leaf, t
+
+    def raycast_all_stats(self, rays, max_toi, mask=ALL_LAYERS, exclude=-1):
+        """Bvh::raycast_all_stats' counters per ray (bvh.rs:206-215): uint64 [n][4]."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 2 * self.dim)
+        st = np.zeros((rays.shape[0], 4), np.uint64)
+        lib().orc_raycast_all_stats(self._b, self._g, _p(rays), rays.shape[0], max_toi, mask, exclude, _p(st))
+        return st
 
     def overlap(self, regions, mask=ALL_LAYERS, exclude=-1, naive=False):
         regions = np.ascontiguousarray(regions, dtype=np.float32).reshape(-1, 2 * self.dim)

@@ -1481,6 +1481,24 @@ uint64_t orc_raycast_all_batch(void* bh, void* gh, const float* rays, uint64_t n
   return total;
 }
 
+// Bvh::raycast_all_stats' counters per ray (bvh.rs:206-215): out_stats[n][4].
+void orc_raycast_all_stats(void* bh, void* gh, const float* rays, uint64_t n, float max_toi, uint64_t mask, int64_t exclude,
+                           uint64_t* out_stats) {
+  OBvh* b = (OBvh*)bh; OGeom* g = (OGeom*)gh;
+  Filter f{mask, exclude};
+  ORC_DISPATCH(g->D, {
+    const Bvh<DD>* t = (const Bvh<DD>*)b->p;
+    const Geometry<DD>* geo = (const Geometry<DD>*)g->p;
+    for (uint64_t i = 0; i < n; ++i) {
+      std::vector<Hit<DD>> hits;
+      Stats st{0, 0, 0, 0};
+      t->raycast_all(*geo, load_ray<DD>(rays + i * 2 * DD), max_toi, f, false, hits, &st);
+      out_stats[4 * i] = st.nodes_visited; out_stats[4 * i + 1] = st.aabb_tests;
+      out_stats[4 * i + 2] = st.narrow_tests; out_stats[4 * i + 3] = st.hits;
+    }
+  });
+}
+
 // regions: [n][2D] (min then max).  Leaves per query in ascending order.
 uint64_t orc_overlap_batch(void* bh, void* gh, const float* regions, uint64_t n, uint64_t mask, int64_t exclude, int naive,
                            uint64_t* offsets, int64_t* out_leaf, uint64_t cap) {

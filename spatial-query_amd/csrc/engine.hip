@@ -86,6 +86,37 @@ __global__ __launch_bounds__(1024) void k_nearest(SceneView s, const float* __re
 }
 
 
+__global__ void k_zero_u32(uint32_t* p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+// The reference's QueryStats per ray (walk_counted): nearest (ALL = false,
+// leaf and toi written when out_leaf is given) or all hits (ALL).  One ray per
+// thread, grid-stride; each thread's stack is a column of `stack` (refs, then
+// entries), `lanes` threads wide.
+template <int D, int KIND, bool ALL, bool FILT>
+__global__ __launch_bounds__(256) void k_counted(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
+                                                 const float* __restrict__ tmax, Filter f, int32_t* out_leaf, float* out_toi,
+                                                 uint32_t* out_stats, uint32_t* stack, uint32_t stack_n) {
+  const uint32_t lanes = gridDim.x * blockDim.x;
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int PS = Prim<D, KIND>::STRIDE;
+  Stack st{stack + tid, reinterpret_cast<float*>(stack + (size_t)stack_n * lanes) + tid, lanes};
+  for (uint32_t i = tid; i < n; i += lanes) {
+    RayPolicy<D, KIND, false> pol;
+    pol.setup(q + (size_t)i * 2 * D);
+    uint32_t cnt[4];
+    const NearestResult h = walk_counted<D, PS, ALL, FILT>(s, pol, tmax ? tmax[i] : max_toi, f, st, cnt);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out_stats[(size_t)i * 4 + k] = cnt[k];
+    if (!ALL && out_leaf) {
+      out_leaf[i] = h.leaf != NO_LEAF ? (int32_t)h.leaf : -1;
+      out_toi[i] = h.t;
+    }
+  }
+}
+
 // Pooled nearest / any-hit / shape-cast kernel, persistent: the grid holds as
 // many waves as the chip keeps resident, and every wave takes chunks of `chunk`
 // queries from counter[0] until all n are handed out, refilling its lanes
@@ -1404,6 +1435,60 @@ int sq_raycast_nearest_noalloc(sq_bvh* b, const float* rays, uint64_t n, float m
     if constexpr (K == KIND_AABB) return pick.template operator()<true>();
     else return c->hit_mode == SQ_HITS_GUARDED ? pick.template operator()<false>() : pick.template operator()<true>();
   });
+}
+
+// Bvh::raycast_nearest_stats / raycast_all_stats (bvh.rs:61-70, 206-215):
+// the reference's counters per ray, from a walk in the reference's order over
+// the reference's hits (whatever the context's hit mode).
+static int counted_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, const float* tmax, const sq_filter* f, bool all,
+                        int32_t* out_leaf, float* out_toi, uint32_t* out_stats) {
+  if (!b || !out_stats || (n && !q)) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (!all && n && (!out_leaf || !out_toi)) return fail(SQ_ERR_INVALID, "out_leaf and out_toi are required");
+  if (n > 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "at most 2^32-1 queries per call");
+  if (!n) return SQ_OK;
+  sq_ctx* c = b->ctx;
+  if (b->kind != KIND_AABB && b->kind != KIND_SPHERE && b->kind != KIND_TRI)
+    return fail(SQ_ERR_UNSUPPORTED, "counters are kept for the built-in primitive kinds");
+  if (!b->n_prims) {
+    // an empty Bvh answers before its walk: no counts (bvh.rs:80-82)
+    k_zero_u32<<<grid_for(4 * n, 256), 256, 0, c->stream>>>(out_stats, 4 * n);
+    HIP_TRY(hipGetLastError());
+    if (!all) {
+      k_fill_miss<<<grid_for(n, 256), 256, 0, c->stream>>>((uint32_t)n, b->dim,
+                                                          NearestOut{out_leaf, out_toi, nullptr, nullptr, nullptr, nullptr});
+      HIP_TRY(hipGetLastError());
+    }
+    return SQ_OK;
+  }
+  bool filt;
+  Filter fd = filter_of(b, f, &filt);
+  const uint32_t stack_n = b->packed.max_stack;
+  const uint32_t lanes = (uint32_t)std::min<uint64_t>((n + 255) / 256 * 256, 1u << 18);
+  int rc = c->spill.ensure((size_t)stack_n * lanes * 8 + 64);
+  if (rc) return rc;
+  if ((rc = begin_timing(c))) return rc;
+  SceneView s = view_of(b);
+  rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+#define SQ_LAUNCH(ALL_, FILT_) \
+    k_counted<D, K, ALL_, FILT_><<<lanes / 256, 256, 0, c->stream>>>(s, q, (uint32_t)n, max_toi, tmax, fd, out_leaf, out_toi, \
+                                                                    out_stats, c->spill.as<uint32_t>(), stack_n)
+    if (all) { if (filt) SQ_LAUNCH(true, true); else SQ_LAUNCH(true, false); }
+    else { if (filt) SQ_LAUNCH(false, true); else SQ_LAUNCH(false, false); }
+#undef SQ_LAUNCH
+    HIP_TRY(hipGetLastError());
+    return SQ_OK;
+  });
+  if (rc) return rc;
+  return end_timing(c);
+}
+
+int sq_raycast_nearest_stats(sq_bvh* b, const float* rays, uint64_t n, float max_toi, const float* tmax, const sq_filter* f,
+                             int32_t* out_leaf, float* out_toi, uint32_t* out_stats) {
+  return counted_impl(b, rays, n, max_toi, tmax, f, false, out_leaf, out_toi, out_stats);
+}
+
+int sq_raycast_all_stats(sq_bvh* b, const float* rays, uint64_t n, float max_toi, const sq_filter* f, uint32_t* out_stats) {
+  return counted_impl(b, rays, n, max_toi, nullptr, f, true, nullptr, nullptr, out_stats);
 }
 
 int sq_raycast_any(sq_bvh* b, const float* rays, uint64_t n, float max_toi, const float* tmax, const sq_filter* f, uint8_t* out_hit) {

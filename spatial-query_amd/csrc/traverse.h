@@ -597,6 +597,79 @@ SQ_HD NearestResult walk_nearest(const SceneView& s, const Q& q, float max_toi, 
   return res;
 }
 
+// The reference's QueryStats (stats.rs:39-50) of one ray, counted where
+// bvh.rs:89-121 (nearest) and bvh.rs:231-257 (all hits) count them: every
+// popped node is a node visit and an AABB test, every filter-accepted
+// primitive of a passing leaf a narrow test (accepts before the test, as
+// there), every narrow test that returns a hit a hit, kept or not.  The
+// reference pushes both children of every interior node whose test passes and
+// pops every pushed node, so node visits = 1 (the root) + 2 per interior node
+// passing; the pieces of an over-full leaf (a synthetic chain) are one
+// reference leaf, walked untested in leaf order.  The walk itself is the
+// reference's: child b's subtree first, child a pushed with its entry and
+// skipped at its pop once beyond the best hit (the pop-time test, see
+// order_pair), the provider's own hits.  ALL: no pruning (raycast_all), the
+// result is then meaningless.  cnt = {nodes_visited, aabb_tests,
+// narrow_tests, hits}.
+template <int D, int PS, bool ALL, bool FILT, class Q>
+SQ_HD NearestResult walk_counted(const SceneView& s, const Q& q, float max_toi, const Filter& f, Stack st, uint32_t* cnt) {
+  NearestResult res{SQ_INF, NO_LEAF, NO_LEAF};
+  float limit = max_toi;
+  cnt[0] = 1; cnt[1] = 1; cnt[2] = 0; cnt[3] = 0;
+  if (!q.root(s.root, limit)) return res;
+  uint32_t cur = fbits(s.root[2 * D]);
+  uint32_t sp = 0;
+  constexpr int NF = (D + 1) * 4;
+  while (true) {
+    if (!(cur & LEAF_BIT)) {
+      const float* nd = s.nodes + (size_t)cur * NF;
+      const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
+      uint32_t first = ra, second = rb;
+      bool hf = true, hs = true;
+      float ts = -SQ_INF;
+      if (!synthetic_pair<D>(nd)) {
+        bool ha, hb;
+        float ta, tb;
+        q.pair(nd, limit, ha, hb, ta, tb);
+        cnt[0] += 2; cnt[1] += 2;
+        first = rb; hf = hb;
+        second = ra; hs = ha; ts = ta;
+      }
+      if (hf && hs) {
+        st.ref[sp * st.stride] = second;
+        st.t[sp * st.stride] = ts;
+        ++sp;
+        cur = first;
+        continue;
+      }
+      if (hf) { cur = first; continue; }
+      if (hs) { cur = second; continue; }
+    } else {
+      const uint32_t first = cur & FIRST_MASK;
+      const uint32_t n = ((cur >> 27) & 15u) + 1u;
+      for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t pos = first + j;
+        const uint32_t leaf = s.ids[pos];
+        if (FILT && !accepts(f, leaf)) continue;
+        ++cnt[2];
+        float t;
+        if (!q.narrow(s.prims + (size_t)pos * PS, limit, t)) continue;
+        ++cnt[3];
+        if (!ALL && (res.leaf == NO_LEAF || hit_before(t, leaf, res.t, res.leaf))) {
+          res.t = t; res.leaf = leaf; res.pos = pos; limit = t;
+        }
+      }
+    }
+    bool got = false;
+    while (sp > 0) {
+      --sp;
+      if (st.t[sp * st.stride] <= limit) { cur = st.ref[sp * st.stride]; got = true; break; }
+    }
+    if (!got) break;
+  }
+  return res;
+}
+
 // Visit every leaf range whose node chain passes `pair_pred` (fixed
 // predicate, no pruning by results): all-hits, crossings, overlap, shape-cast
 // all.  leaf_fn(first, count) handles one leaf range.

@@ -374,6 +374,28 @@ class DeviceBvh:
                                    self._f(filt), _dp(out, "uint8", n=rays.shape[0])))
         return out
 
+    def raycast_nearest_stats(self, rays, max_toi, max_toi_per_ray=None, filt=None):
+        """Bvh::raycast_nearest_stats per ray: dict(leaf, toi, stats int32 [n][4] = nodes_visited, aabb_tests,
+        narrow_tests, hits), counted on a walk in the reference's order over its hits (a diagnostic walk)."""
+        import torch
+        n, dev = rays.shape[0], rays.device
+        leaf = torch.empty(n, dtype=torch.int32, device=dev)
+        toi = torch.empty(n, dtype=torch.float32, device=dev)
+        st = torch.empty((n, 4), dtype=torch.int32, device=dev)
+        check(lib().sq_raycast_nearest_stats(self._h, _dp(rays, "float32", 2 * self.dim), n, float(max_toi),
+                                             _dp(max_toi_per_ray, "float32", n=n), self._f(filt), _dp(leaf, "int32", n=n),
+                                             _dp(toi, "float32", n=n), _dp(st, "int32", n=n, per=4)))
+        return dict(leaf=leaf, toi=toi, stats=st)
+
+    def raycast_all_stats(self, rays, max_toi, filt=None):
+        """Bvh::raycast_all_stats' counters per ray (int32 [n][4]); the hit lists come from raycast_all."""
+        import torch
+        n = rays.shape[0]
+        st = torch.empty((n, 4), dtype=torch.int32, device=rays.device)
+        check(lib().sq_raycast_all_stats(self._h, _dp(rays, "float32", 2 * self.dim), n, float(max_toi), self._f(filt),
+                                         _dp(st, "int32", n=n, per=4)))
+        return st
+
     def _csr(self, fn, n, dev, has_t, capacity):
         import torch
         offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)

@@ -117,6 +117,8 @@ SIGNATURES = {
     "sq_raycast_nearest": [P, P, U64, F32, P, C.POINTER(Filter), P, P, P, P, P],
     "sq_raycast_nearest_noalloc": [P, P, U64, F32, P, C.POINTER(Filter), P, P, P, P, P],
     "sq_raycast_any": [P, P, U64, F32, P, C.POINTER(Filter), P],
+    "sq_raycast_nearest_stats": [P, P, U64, F32, P, C.POINTER(Filter), P, P, P],
+    "sq_raycast_all_stats": [P, P, U64, F32, C.POINTER(Filter), P],
     "sq_raycast_all": [P, P, U64, F32, C.POINTER(Filter), I32, P, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_overlap": [P, P, U64, C.POINTER(Filter), P, P, U64, C.POINTER(C.c_uint64)],
     "sq_shapecast_nearest": [P, P, U64, F32, P, C.POINTER(Filter), P, P, P],

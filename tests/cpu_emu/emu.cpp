@@ -193,6 +193,32 @@ int emu_nearest_quad(void* h, int dim, int kind, const float* q, uint64_t nq, fl
   return dk(dim, kind, [&]<int D, int K>() { nearest_quad_dk<D, K>(b, q, nq, max_toi, any, threads, leaf, toi, ring); });
 }
 
+// The reference's counters of every ray (walk_counted, as k_counted runs it):
+// nearest (all = 0, leaf and toi written) or all hits; stats[nq][4].
+int emu_counted(void* h, int dim, int kind, const float* q, uint64_t nq, float max_toi, int all, int32_t* leaf, float* toi,
+                uint32_t* stats) {
+  const Built& b = *(Built*)h;
+  if (b.tree.prim_indices.empty()) {
+    for (uint64_t i = 0; i < nq; ++i) { leaf[i] = -1; toi[i] = SQ_INF; for (int k = 0; k < 4; ++k) stats[4 * i + k] = 0; }
+    return 0;
+  }
+  return dk(dim, kind, [&]<int D, int K>() {
+    const SceneView s = b.view();
+    const uint32_t S = b.packed.max_stack;
+    const Filter f{nullptr, ~0ull, -1};
+    par_for(nq, 8, [&](uint64_t i) {
+      std::vector<uint32_t> sr(S);
+      std::vector<float> stt(S);
+      RayPolicy<D, K, false> pol;
+      pol.setup(q + i * 2 * D);
+      NearestResult hres = all ? walk_counted<D, Prim<D, K>::STRIDE, true, false>(s, pol, max_toi, f, Stack{sr.data(), stt.data(), 1}, stats + 4 * i)
+                               : walk_counted<D, Prim<D, K>::STRIDE, false, false>(s, pol, max_toi, f, Stack{sr.data(), stt.data(), 1}, stats + 4 * i);
+      leaf[i] = hres.leaf == NO_LEAF ? -1 : (int32_t)hres.leaf;
+      toi[i] = hres.t;
+    });
+  });
+}
+
 // mode 0 all, 1 crossings, 2 overlap, 3 shapecast-all.  Returns the total;
 // writes counts[nq] and, when keys != NULL, the unsorted keys per query at
 // their CSR offsets (offsets computed here).

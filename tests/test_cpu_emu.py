@@ -30,6 +30,7 @@ def emu():
     L.emu_collect.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, P, P]
     L.emu_required.argtypes = [P, C.c_int, P, C.c_uint64, P, P, P]
     L.emu_nearest_quad.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_int, P, P, C.c_uint32]
+    L.emu_counted.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, P, P, P]
     return L
 
 
@@ -55,6 +56,13 @@ class Emu:
         leaf = np.empty(len(q), np.int32); t = np.empty(len(q), np.float32)
         self.L.emu_nearest_quad(self.h, self.dim, self.kind, p(q), len(q), max_toi, any_, 8, p(leaf), p(t), ring)
         return leaf.astype(np.int64), t
+
+    def counted(self, q, max_toi, all_=0):
+        q = np.ascontiguousarray(q, np.float32)
+        leaf = np.empty(len(q), np.int32); t = np.empty(len(q), np.float32)
+        st = np.empty((len(q), 4), np.uint32)
+        self.L.emu_counted(self.h, self.dim, self.kind, p(q), len(q), max_toi, all_, p(leaf), p(t), p(st))
+        return leaf.astype(np.int64), t, st
 
     def collect(self, q, max_toi, mode):
         q = np.ascontiguousarray(q, np.float32)
@@ -223,3 +231,44 @@ def test_hit_modes(emu, ring):
         assert np.array_equal(e.nearest(qq, tmax, any_=1, ring=ring)[0] >= 0, plain["leaf"] >= 0)
         differ += int((plain["leaf"] != guarded["leaf"]).sum())
     assert differ > 0, "the scenes should hold rays where the two modes differ"
+
+
+def _counted_cases():
+    """(name, dim, kind, prims, rays, max_toi): boxes, balls D = 2..6, the
+    heightfield, and 100 coincident balls (one over-full reference leaf)."""
+    out = []
+    sc = W.boxes_scene(n=3000, seed=0xC0C0, lo=-100.0, hi=100.0, hmin=0.1, hmax=2.0)
+    out.append(("boxes", 3, 0, sc.prims, W.random_rays(8000, 0xC0C1, -150.0, 150.0), 400.0))
+    for dim in (2, 4, 6):
+        s = W.Stream(40 + dim)
+        prims = np.concatenate([s.range(-50.0, 50.0, 2000 * dim).reshape(-1, dim), s.range(0.5, 4.0, 2000)[:, None]], 1)
+        out.append((f"balls{dim}", dim, 1, prims, W.random_rays(6000, 90 + dim, -60.0, 60.0, dim=dim), 500.0))
+    hf = W.heightfield_scene(grid=129, extent=64.0)
+    rays = np.concatenate([W.camera_rays(hf, res=64, seed=0x12), W.incoherent_rays(hf, 8192)])
+    out.append(("heightfield", 3, 2, hf.prims, rays, 2000.0))
+    dup = np.tile(np.array([[5, 0, 0, 1]], np.float32), (100, 1))
+    dup[::7, 3] = 1.5
+    out.append(("coincident", 3, 1, dup, np.array([[0, 0, 0, 1, 0, 0], [0, 0.5, 0, 1, 0, 0], [0, 9, 0, 1, 0, 0]],
+                                                  np.float32), 100.0))
+    return out
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_reference_counters(emu, case):
+    """The device's counted walk (walk_counted, run on the CPU) gives the
+    reference's QueryStats on every ray, nearest and all hits, and the
+    reference's nearest hit."""
+    name, dim, kind, prims, rays, tmax = _counted_cases()[case]
+    okind = {0: oracle.BOX, 1: oracle.SPHERE, 2: oracle.TRI}[kind]
+    o = oracle.Scene(dim, okind, prims)
+    e = Emu(emu, dim, kind, prims)
+    ref = o.raycast_nearest(rays, tmax, threads=8, stats=True)
+    leaf, t, st = e.counted(rays, tmax)
+    assert eq(leaf, t, ref["leaf"], ref["t"]), name
+    assert np.array_equal(st.astype(np.uint64), ref["stats"]), name
+    _, _, sa = e.counted(rays, tmax, all_=1)
+    assert np.array_equal(sa.astype(np.uint64), o.raycast_all_stats(rays, tmax)), name
+    if name == "coincident":     # one reference leaf of 100 balls: one node, every ball tested
+        assert ref["stats"][:, 0].tolist() == [1, 1, 1] and ref["stats"][:2, 2].tolist() == [100, 100]
+    else:
+        assert (ref["stats"][:, 0] > 1).any()

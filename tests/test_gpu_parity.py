@@ -406,3 +406,59 @@ def test_tuning_does_not_change_results(ctx, tuning):
     assert np.array_equal(host(anyh).astype(bool), hit)
     ref6 = oracle.Scene(6, oracle.BOX, prims6).shapecast_nearest(casts, 50.0)
     assert_nearest_equal(host(rc["leaf"]), host(rc["toi"]), ref6["leaf"], ref6["t"], "cast tuning")
+
+
+@pytest.mark.parametrize("case", ["c0_boxes", "balls3", "balls6", "heightfield", "coincident", "filter"])
+def test_reference_query_stats(ctx, case):
+    """sq_raycast_nearest_stats / sq_raycast_all_stats: the reference's
+    QueryStats per ray (nodes_visited, aabb_tests, narrow_tests, hits), equal
+    to the oracle's counters on every ray, with the reference's nearest hit; in
+    both hit modes (the counted walk is the reference's whatever the mode)."""
+    print(f"case: reference query stats, {case}", flush=True)
+    mask, exclude, layers = oracle.ALL_LAYERS, -1, None
+    if case == "c0_boxes":
+        sc = W.c0()[0]
+        dim, kind, okind, prims, tmax = 3, sq_amd.AABB, oracle.BOX, sc.prims, 400.0
+        rays = W.c0()[1][:20000]
+    elif case.startswith("balls"):
+        dim = int(case[-1])
+        prims = ball_scene(dim, 20000, 0x57A7 + dim, half=span(dim) * 3)
+        kind, okind, tmax = sq_amd.SPHERE, oracle.SPHERE, 500.0
+        rays = W.random_rays(20000, 0x57A8 + dim, -span(dim) * 3.5, span(dim) * 3.5, dim=dim)
+    elif case == "heightfield":
+        hf = W.heightfield_scene(grid=257, extent=128.0)
+        dim, kind, okind, prims, tmax = 3, sq_amd.TRIANGLE, oracle.TRI, hf.prims, 2000.0
+        rays = np.concatenate([W.camera_rays(hf, res=128, seed=0x13), W.incoherent_rays(hf, 16384)])
+    elif case == "coincident":
+        prims = np.tile(np.array([[5, 0, 0, 1]], np.float32), (100, 1))
+        prims[::7, 3] = 1.5
+        prims = np.concatenate([prims, ball_scene(3, 500, 0xC01, half=40.0)])
+        dim, kind, okind, tmax = 3, sq_amd.SPHERE, oracle.SPHERE, 100.0
+        rays = np.concatenate([np.array([[0, 0, 0, 1, 0, 0], [0, 0.5, 0, 1, 0, 0]], np.float32),
+                               W.random_rays(4000, 0xC02, -50.0, 50.0)])
+    else:
+        prims = ball_scene(3, 5000, 0xF17, half=40.0)
+        dim, kind, okind, tmax = 3, sq_amd.SPHERE, oracle.SPHERE, 500.0
+        rays = W.random_rays(8000, 0xF18, -50.0, 50.0)
+        layers = (np.arange(5000, dtype=np.uint64) % np.uint64(3)) + np.uint64(1)
+        mask, exclude = 0b10, 7
+    o = oracle.Scene(dim, okind, prims)
+    if layers is not None:
+        o.set_layers(layers)
+    ref = o.raycast_nearest(rays, tmax, mask=mask, exclude=exclude, threads=8, stats=True)
+    ref_all = o.raycast_all_stats(rays, tmax, mask=mask, exclude=exclude)
+    b = sq_amd.DeviceBvh.build(ctx, dim, kind, prims)
+    filt = None
+    if layers is not None:
+        b.set_layers(layers)
+        filt = sq_amd.QueryFilter(layer_mask=mask, exclude=exclude)
+    for mode in (sq_amd.SQ_HITS_REFERENCE, sq_amd.SQ_HITS_GUARDED):
+        ctx.set_hit_mode(mode)
+        got = b.raycast_nearest_stats(dev(rays), tmax, filt=filt)
+        sa = b.raycast_all_stats(dev(rays), tmax, filt=filt)
+        ctx.synchronize()
+        assert_nearest_equal(host(got["leaf"]), host(got["toi"]), ref["leaf"], ref["t"], f"{case} counted walk")
+        assert np.array_equal(host(got["stats"]).astype(np.uint64), ref["stats"]), f"{case}: nearest counters"
+        assert np.array_equal(host(sa).astype(np.uint64), ref_all), f"{case}: all-hits counters"
+    ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
+    b.close()

@@ -438,6 +438,35 @@ def test_tree_stats_and_counters():
     assert st.tolist()[:3] == [1, 1, 3]
     miss = sc.raycast_nearest(ray_new([0, 5, 0], [1, 0, 0]), 100.0, stats=True)["stats"][0]
     assert miss.tolist() == [1, 1, 0, 0]
+    # all_hits_counters_are_exact (stats.rs:107-122): one node, one test, three narrow tests, three hits
+    assert sc.raycast_all_stats(ray_new([0, 0, 0], [1, 0, 0]), 100.0)[0].tolist() == [1, 1, 3, 3]
+    assert sc.raycast_all_stats(ray_new([0, 5, 0], [1, 0, 0]), 100.0)[0].tolist() == [1, 1, 0, 0]
+
+
+def test_query_counters_on_a_larger_tree():
+    """stats_path_matches_plain_path_and_repeats (stats.rs:165-185) and what
+    the walk fixes: every popped node is tested (visits == AABB tests), both
+    children of every passing interior node are popped (visits odd: 1 + 2k),
+    a hit is a narrow test that hit, and the all-hits walk, which never prunes,
+    tests every node the nearest walk tests and more."""
+    rng = np.random.default_rng(11)
+    c = rng.uniform(-30, 30, (400, 3)).astype(F32)
+    balls = np.concatenate([c, np.full((400, 1), 1.0, F32)], 1)
+    sc = oracle.Scene(3, oracle.SPHERE, balls)
+    o = rng.uniform(-50, 50, (300, 3)).astype(F32)
+    rays = np.concatenate([o, -o / np.linalg.norm(o, axis=1, keepdims=True)], 1).astype(F32)
+    a = sc.raycast_nearest(rays, 200.0, stats=True)
+    b = sc.raycast_nearest(rays, 200.0, stats=True)
+    assert np.array_equal(a["stats"], b["stats"]) and np.array_equal(a["leaf"], sc.raycast_nearest(rays, 200.0)["leaf"])
+    st = a["stats"].astype(np.int64)
+    nn = sc.tree_stats()["node_count"]
+    assert (st[:, 0] == st[:, 1]).all() and (st[:, 0] % 2 == 1).all()
+    assert (st[:, 3] <= st[:, 2]).all() and (st[:, 3] >= (a["leaf"] >= 0)).all()
+    assert (st[:, 0] <= st[:, 2] + nn).all() and st[:, 0].max() > 1
+    al = sc.raycast_all_stats(rays, 200.0).astype(np.int64)
+    off, _, _ = sc.raycast_all(rays, 200.0)
+    assert np.array_equal(al[:, 3], np.diff(off.astype(np.int64)))
+    assert (al[:, 0] >= st[:, 0]).all() and (al[:, 2] >= st[:, 2]).all() and (al[:, 0] % 2 == 1).all()
 
 
 # ---------------------------------------------------------------------------
