@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--ray-order", type=int, default=-1, help="1: Morton ray order, 0: as given, -1: default")
     ap.add_argument("--descend-min", type=int, default=-1, help="pause descents below this many lanes (-1: default)")
     ap.add_argument("--wide", type=int, default=-1, help="1: walk the 4-wide records, 0: the pair nodes, -1: default")
+    ap.add_argument("--packet", type=int, default=-1, help="1: one wave-coherent walk per 64 rays, 0: one ray per lane, -1: default")
     ap.add_argument("--quick", action="store_true", help="short run for profiling: no CPU baseline, no side figures")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB device copy (the known-size kernel that calibrates HBM counters)")
@@ -95,6 +96,8 @@ def main():
         tun["descend_min"] = args.descend_min
     if args.wide >= 0:
         tun["wide"] = args.wide
+    if args.packet >= 0:
+        tun["packet"] = args.packet
     ctx.set_tuning(**tun)
     ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED if args.hit_mode == "guarded" else sq_amd.SQ_HITS_REFERENCE)
     t0 = time.time()
@@ -192,6 +195,27 @@ def main():
                 bvh.raycast_nearest_into(rr, 2000.0, l2, t2)
                 ts.append(ctx.last_kernel_ms())
             out[f"nearest_{name}_mrays"] = round(half / (min(ts) * 1e-3) / 1e6, 1)
+        # the coherent half as a coherent batch: tuning `packet` (the batch in its
+        # own Morton order, one wave-coherent walk per 64 rays); same answers
+        tun0 = ctx.tuning()
+        rr = rays[:half].contiguous()
+        pl = torch.empty(half, dtype=torch.int32, device="cuda"); pt = torch.empty(half, device="cuda")
+        ctx.set_tuning(**dict(tun0, packet=1))
+        bvh.raycast_nearest_into(rr, 2000.0, pl, pt)
+        ts = []
+        for _ in range(5):
+            bvh.raycast_nearest_into(rr, 2000.0, pl, pt)
+            ts.append(ctx.last_kernel_ms())
+        ctx.set_tuning(**tun0)
+        same = bool(torch.equal(pl, leaf[:half])) and bool(torch.equal(pt.view(torch.int32), toi[:half].view(torch.int32)))
+        out["nearest_coherent_packet"] = {
+            "mrays": round(half / (min(ts) * 1e-3) / 1e6, 1), "ms": round(min(ts), 4),
+            "speedup_vs_one_ray_per_lane": round(out["nearest_coherent_mrays"] and
+                                                 half / (min(ts) * 1e-3) / 1e6 / out["nearest_coherent_mrays"], 3),
+            "same_answers": same,
+            "note": "sort (batch-bounds Morton key, rocPRIM radix sort) + packet walk, both inside the timed events"}
+        log(f"coherent half, packet walk: {out['nearest_coherent_packet']['mrays']} Mrays/s "
+            f"({out['nearest_coherent_packet']['speedup_vs_one_ray_per_lane']}x one ray per lane), same answers {same}")
         anyh = torch.empty(n, dtype=torch.uint8, device="cuda")
         bvh.raycast_any(rays, 2000.0, out=anyh)
         ts = []

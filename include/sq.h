@@ -413,9 +413,10 @@ int sq_tlas_shapecast_all(sq_tlas* tlas, const float* casts, uint64_t n, float m
  *   lds_stack      entries of each lane's traversal stack kept in LDS (power of
  *                  two); deeper entries spill to device memory
  *   ray_order      1: pooled nearest / any-hit calls of >= 4096 rays walk the
- *                  rays in Morton order (a 30-bit key interleaving the origin,
- *                  quantized in the scene box, with the direction; sorted on
- *                  the device); results still land at each ray's own index.
+ *                  rays in Morton order (a 30-bit key interleaving those
+ *                  components of origin and direction that vary in the batch,
+ *                  each quantized within the batch's own bounds; sorted on the
+ *                  device); results still land at each ray's own index.
  *                  Not for graph capture (the sort is a library call).
  *   descend_min    pooled walks: a wave stops waiting on descents once fewer
  *                  than this many lanes (0..64) still descend while others
@@ -432,7 +433,17 @@ int sq_tlas_shapecast_all(sq_tlas* tlas, const float* casts, uint64_t n, float m
  *   wide           pooled nearest / any-hit calls without a filter: 1 walks
  *                  4-wide records (each pair node with its children's pair
  *                  nodes, rebuilt from the tree before the walk), half the
- *                  dependent node fetches per ray; 0 = the pair nodes. */
+ *                  dependent node fetches per ray; 0 = the pair nodes.
+ *   packet         nearest-hit ray calls without a filter: 1 walks each 64
+ *                  rays of the batch (in the Morton order above, for calls of
+ *                  >= 4096 rays) as one wave-coherent packet: one walk per
+ *                  wave, nodes and primitives loaded once for all lanes, each
+ *                  lane testing its own ray against its own limit, so each ray
+ *                  visits exactly the nodes of its own walk (results as
+ *                  packet 0, bit for bit).  For coherent batches (camera rays:
+ *                  1.6x on 4M camera rays of the 2M-triangle heightfield);
+ *                  incoherent batches walk many times slower this way.  Not for
+ *                  graph capture (the sort). */
 typedef struct sq_tuning {
   int block;
   int rays_per_lane;
@@ -441,6 +452,7 @@ typedef struct sq_tuning {
   int descend_min;
   int split_depth;
   int wide;
+  int packet;
 } sq_tuning;
 int sq_ctx_set_tuning(sq_ctx* ctx, const sq_tuning* tuning);
 int sq_ctx_get_tuning(const sq_ctx* ctx, sq_tuning* tuning);

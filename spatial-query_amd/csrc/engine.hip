@@ -900,6 +900,7 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   if (t->descend_min < 0 || t->descend_min > 64) return fail(SQ_ERR_INVALID, "descend_min must be in [0, 64]");
   if (t->split_depth < -1 || t->split_depth > 24) return fail(SQ_ERR_INVALID, "split_depth must be in [-1, 24]");
   if (t->wide != 0 && t->wide != 1) return fail(SQ_ERR_INVALID, "wide must be 0 or 1");
+  if (t->packet != 0 && t->packet != 1) return fail(SQ_ERR_INVALID, "packet must be 0 or 1");
   c->block = block;
   c->rays_per_lane = t->rays_per_lane;
   c->lds_stack = S;
@@ -907,6 +908,7 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   c->descend_min = t->descend_min;
   c->split_depth = t->split_depth;
   c->wide = t->wide;
+  c->packet = t->packet;
   return SQ_OK;
 }
 
@@ -928,6 +930,7 @@ int sq_ctx_get_tuning(const sq_ctx* c, sq_tuning* t) {
   t->descend_min = c->descend_min;
   t->split_depth = c->split_depth;
   t->wide = c->wide;
+  t->packet = c->packet;
   return SQ_OK;
 }
 
@@ -1268,6 +1271,16 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
   const uint32_t stack_n = b->packed.max_stack;
   SceneView s = view_of(b);
   const uint32_t nn = (uint32_t)n;
+  if (c->packet && !any && !cast && !filt) {
+    // rays in the batch-adaptive Morton order, one wave-coherent walk per 64
+    // of them (packet.hip)
+    int rc = begin_timing(c);
+    if (rc) return rc;
+    const uint32_t* order = nullptr;
+    if (n >= 4096 && (rc = ray_order(c, b->dim, q, nn, 2 * b->dim, &order))) return rc;
+    if ((rc = packet_nearest(b, q, n, max_toi, tmax, out, c->hit_mode != SQ_HITS_GUARDED, order))) return rc;
+    return end_timing(c);
+  }
   if (c->rays_per_lane == 0) {
     // one query per thread, whole stack in LDS
     const size_t lds = (size_t)stack_n * B * 8;
@@ -1325,7 +1338,7 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
   }
   const uint32_t* order = nullptr;
   if (c->ray_order && !cast && n >= 4096)
-    if ((rc = ray_order(c, b->dim, q, nn, 2 * b->dim, b->root.as<float>(), &order))) return rc;
+    if ((rc = ray_order(c, b->dim, q, nn, 2 * b->dim, &order))) return rc;
   uint32_t* counter = c->pool_ctr.as<uint32_t>();
   rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
     auto launch = [&]<bool ANY_, bool FILT_, bool CAST_, bool QUAD_, bool EXACT_>() -> int {

@@ -70,6 +70,7 @@ struct sq_ctx {
   int descend_min = 24;            // pooled walks: pause descents below this many lanes (pool.h)
   int split_depth = 0;             // pooled collect: split queries this many levels down (0 auto, -1 never)
   int wide = 0;                    // pooled nearest / any: walk the 4-wide records (traverse.h quad_records)
+  int packet = 0;                  // nearest rays: 1 = one wave-coherent walk per 64 rays (packet.hip)
   int hit_mode = SQ_HITS_REFERENCE;   // SQ_HITS_REFERENCE or SQ_HITS_GUARDED (sq.h)
   bool timed = false;
   // {next chunk, retired waves} of the pooled single-call launches: zero at
@@ -287,10 +288,15 @@ inline int upload(DevBuf& d, const void* h, size_t bytes, hipStream_t s) {
 // Morton order of n queries (origin and direction in the first 2 dim floats
 // of each qs-float record; origins quantized in box = [min dim][max dim],
 // device): *order (context buffer) lists query indices in key order.
-int ray_order(sq_ctx* c, int dim, const float* q, uint32_t n, uint32_t qs, const float* box, const uint32_t** order);
+int ray_order(sq_ctx* c, int dim, const float* q, uint32_t n, uint32_t qs, const uint32_t** order);
 
 // Writes misses (leaf -1, toi +inf, zero point/normal, sub -1, any 0).
 int fill_miss(sq_ctx* c, uint64_t n, int dim, NearestOut out);
+
+// Nearest hits by the wave-coherent packet walk (packet.hip; no filter):
+// exact = the reference's hits and order, else guarded hits.
+int packet_nearest(sq_bvh* b, const float* q, uint64_t n, float max_toi, const float* tmax, NearestOut out, bool exact,
+                   const uint32_t* order);
 
 // Two-level tree access for the learned proposal (tlas.hip): the top tree over
 // the instances' world boxes, the dimension, and the seeded walk over guarded

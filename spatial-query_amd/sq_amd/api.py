@@ -159,15 +159,15 @@ class Context:
         check(lib().sq_ctx_set_block(self._h, threads))
 
     def set_tuning(self, block: int = 128, rays_per_lane: int = 16, lds_stack: int = 8, ray_order: int = 0,
-                   descend_min: int = 24, split_depth: int = 0, wide: int = 0):
+                   descend_min: int = 24, split_depth: int = 0, wide: int = 0, packet: int = 0):
         check(lib().sq_ctx_set_tuning(self._h, C.byref(Tuning(block, rays_per_lane, lds_stack, ray_order, descend_min,
-                                                              split_depth, wide))))
+                                                              split_depth, wide, packet))))
 
     def tuning(self) -> dict:
         t = Tuning()
         check(lib().sq_ctx_get_tuning(self._h, C.byref(t)))
         return dict(block=t.block, rays_per_lane=t.rays_per_lane, lds_stack=t.lds_stack, ray_order=t.ray_order,
-                    descend_min=t.descend_min, split_depth=t.split_depth, wide=t.wide)
+                    descend_min=t.descend_min, split_depth=t.split_depth, wide=t.wide, packet=t.packet)
 
     def set_hit_mode(self, mode: int):
         """SQ_HITS_REFERENCE (default: the reference's hits for every query) or

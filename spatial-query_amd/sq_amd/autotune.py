@@ -1,5 +1,6 @@
 """Autotuner for the engine's kernel tuning: block size, rays per lane (pool
-size), LDS ring-stack depth and Morton ray order.
+size), LDS ring-stack depth, Morton ray order and the packet walk (one
+wave-coherent walk per 64 rays, for coherent batches such as camera rays).
 
 Results never depend on the tuning, only speed does; the tuner checks that too:
 every candidate's results must equal the first candidate's bit for bit.
@@ -16,7 +17,7 @@ from __future__ import annotations
 import itertools
 import time
 
-DEFAULT_SPACE = dict(block=(64, 128, 256), rays_per_lane=(4, 8, 16), lds_stack=(4, 8, 16), ray_order=(0, 1))
+DEFAULT_SPACE = dict(block=(64, 128, 256), rays_per_lane=(4, 8, 16), lds_stack=(4, 8, 16), ray_order=(0, 1), packet=(0, 1))
 
 
 def _same(a, b) -> bool:

@@ -462,3 +462,50 @@ def test_reference_query_stats(ctx, case):
         assert np.array_equal(host(sa).astype(np.uint64), ref_all), f"{case}: all-hits counters"
     ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
     b.close()
+
+
+@pytest.mark.parametrize("case", ["heightfield", "balls3", "balls5", "c0_boxes", "coincident"])
+def test_packet_walk(ctx, case):
+    """Tuning `packet` (one wave-coherent walk per 64 rays, scalar node and
+    primitive loads, per-lane limits): the reference's answer on every ray in
+    the default hit mode, the guarded oracle's in the guarded mode, for
+    coherent camera rays and incoherent rays alike."""
+    print(f"case: packet walk, {case}", flush=True)
+    if case == "heightfield":
+        hf = W.heightfield_scene(grid=257, extent=128.0)
+        dim, kind, okind, prims, tmax = 3, sq_amd.TRIANGLE, oracle.TRI, hf.prims, 2000.0
+        rays = np.concatenate([W.camera_rays(hf, res=256, seed=0x14), W.incoherent_rays(hf, 20000)])
+    elif case.startswith("balls"):
+        dim = int(case[-1])
+        prims = ball_scene(dim, 20000, 0x9AC + dim, half=span(dim) * 3)
+        kind, okind, tmax = sq_amd.SPHERE, oracle.SPHERE, 500.0
+        rays = W.random_rays(30000, 0x9AD + dim, -span(dim) * 3.5, span(dim) * 3.5, dim=dim)
+    elif case == "c0_boxes":
+        sc, rays = W.c0(30000)
+        dim, kind, okind, prims, tmax = 3, sq_amd.AABB, oracle.BOX, sc.prims, 400.0
+    else:
+        prims = np.tile(np.array([[5, 0, 0, 1]], np.float32), (100, 1))
+        prims[::7, 3] = 1.5
+        prims = np.concatenate([prims, ball_scene(3, 500, 0xC03, half=40.0)])
+        dim, kind, okind, tmax = 3, sq_amd.SPHERE, oracle.SPHERE, 100.0
+        rays = np.concatenate([np.array([[0, 0, 0, 1, 0, 0], [0, 0.5, 0, 1, 0, 0]], np.float32),
+                               W.random_rays(4000, 0xC04, -50.0, 50.0)])
+    b = sq_amd.DeviceBvh.build(ctx, dim, kind, prims)
+    entry = ctx.tuning()
+    try:
+        for guarded in (False, True):
+            ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED if guarded else sq_amd.SQ_HITS_REFERENCE)
+            ref = oracle.Scene(dim, okind, prims, guarded=guarded).raycast_nearest(rays, tmax, threads=8)
+            ctx.set_tuning(**dict(entry, packet=1))
+            got = b.raycast_nearest(dev(rays), tmax, point=True, normal=True, sub=True)
+            ctx.set_tuning(**entry)
+            plain = b.raycast_nearest(dev(rays), tmax, point=True, normal=True, sub=True)
+            ctx.synchronize()
+            assert (ref["leaf"] >= 0).sum() > 100
+            assert_nearest_equal(host(got["leaf"]), host(got["toi"]), ref["leaf"], ref["t"], f"packet {case} guarded={guarded}")
+            for k in ("point", "normal", "sub"):
+                assert same_bits(host(got[k]), host(plain[k])).all(), f"packet {case}: {k}"
+    finally:
+        ctx.set_tuning(**entry)
+        ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
+    b.close()

@@ -255,3 +255,33 @@ def test_autotuner_on_batch_sets(ctx):
         prepare.keep = None
     finally:
         ctx.set_tuning(**entry)
+
+
+def test_autotuner_picks_the_packet_walk_for_camera_rays(ctx):
+    """A batch class the tuner decides on: 1M camera rays over the 2M-triangle
+    heightfield walk faster as packets (batch-bounds Morton order, one
+    wave-coherent walk per 64 rays), 1M incoherent rays several times slower;
+    the tuner picks packet for the first and not for the second, and every
+    candidate gives the same answers (the tuner checks)."""
+    from sq_amd.autotune import autotune
+    print("case: autotuner, packet walk per batch class", flush=True)
+    sc = W.heightfield_scene()
+    bvh = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.TRIANGLE, sc.prims)
+    entry = ctx.tuning()
+    try:
+        for name, rays_h, want in (("camera", W.camera_rays(sc, res=1024), 1), ("incoherent", W.incoherent_rays(sc, 1 << 20), 0)):
+            r = dev(rays_h)
+
+            def prepare():
+                def go():
+                    o = bvh.raycast_nearest(r, 2000.0)
+                    return (o["leaf"], o["toi"])
+                return go
+
+            res = autotune(ctx, prepare, space=dict(packet=(0, 1)), reps=3)
+            print(f"{name}: {[(c['packet'], round(ms, 3)) for c, ms in res['table']]}", flush=True)
+            assert res["best"]["packet"] == want, name
+            ctx.set_tuning(**entry)
+    finally:
+        ctx.set_tuning(**entry)
+    bvh.close()
