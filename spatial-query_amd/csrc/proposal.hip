@@ -21,6 +21,7 @@
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 namespace {
 
@@ -223,6 +224,15 @@ __global__ void k_adam(float* w, float* mo, float* ve, const float* __restrict__
   }
 }
 
+// Floats as unsigned integers in the same order (LDS atomic max of a float).
+__device__ __forceinline__ uint32_t ordered_bits(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float from_ordered(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
 // (v, c) ranks before (w, d): larger logit first, then the smaller class id.
 __device__ __forceinline__ bool beats(float v, int c, float w, int d) { return v > w || (v == w && c < d); }
 
@@ -244,16 +254,33 @@ __device__ __forceinline__ void top4_insert(float (&tv)[4], int (&ti)[4], float 
 // Fused inference: features -> 3 ReLU layers -> logits -> top-4, for IR = 128
 // rays per block of 512 threads (8 waves).  Activations stay in LDS (one
 // [128][HID + 8] bf16 buffer, overwritten layer by layer); weights stream from
-// L2.  Hidden layers: wave w owns rays 64 (w >> 2) .. +64 and outputs
-// 64 (w & 3) .. +64 (4 x 4 MFMA tiles).  The last layer is computed transposed
-// (classes as MFMA rows, rays as columns): per 128-class tile, wave w owns
-// classes 32 (w >> 1) .. +32 and rays 64 (w & 1) .. +64, so each lane holds 8
-// classes of each of its 4 rays and keeps every ray's best four in registers;
-// the 16 partial lists of a ray (4 waves x 4 lane groups) are merged in LDS.
+// L2 in fragment order (k_hidden_fragments, k_w4_fragments: each wave's
+// operand fragment is 1 KB of consecutive memory, eight whole cache lines per
+// load).  Hidden layers: wave w owns rays 64 (w >> 2) .. +64 and outputs
+// 64 (w & 3) .. +64 (4 x 4 MFMA tiles of 16 x 16).  The last layer is computed
+// transposed (classes as MFMA rows, rays as columns) on 32 x 32 MFMA tiles:
+// per 128-class tile, wave w owns classes 32 (w >> 1) .. +32 and rays
+// 64 (w & 1) .. +64, so each lane holds 16 classes of each of its 2 rays and
+// keeps every ray's best four in registers, skipping candidates below the
+// ray's shared threshold; the 8 partial lists of a ray (4 waves x 2 lane
+// halves) are merged in LDS.
 constexpr int IR = 128, ALD = HID + 8, IT = 512;
 
+// A hidden layer's W [HID][KIN] (bf16) in the order dense_relu reads it: for
+// K step s (K 32 s ..), output group g (outputs 64 g ..), sub-tile j (outputs
+// + 16 j ..) and lane l, the 8 values lane l feeds to one 16 x 16 x 32 MFMA
+// (output 64 g + 16 j + l % 16, K 32 s + 8 (l / 16) ..) are contiguous: one
+// fragment is 1 KB of consecutive memory.
+__global__ void k_hidden_fragments(const bf16* __restrict__ w, uint32_t kin, bf16* out) {
+  const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= (uint32_t)HID * kin / 8) return;
+  const uint32_t l = f % 64, j = (f / 64) % 4, g = (f / 256) % 4, s = f / 1024;
+  *reinterpret_cast<bf16x8*>(out + (size_t)f * 8) =
+      *reinterpret_cast<const bf16x8*>(w + (size_t)(64 * g + 16 * j + l % 16) * kin + 32 * s + 8 * (l / 16));
+}
+
 template <int KIN>
-__device__ __forceinline__ void dense_relu(bf16 (*act)[ALD], const bf16* __restrict__ W, const float* __restrict__ bias) {
+__device__ __forceinline__ void dense_relu(bf16 (*act)[ALD], const bf16* __restrict__ Wf, const float* __restrict__ bias) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r0 = (wave >> 2) * 64, c0 = (wave & 3) * 64;
   const int li = lane & 15, kb = (lane >> 4) * 8;
@@ -262,10 +289,12 @@ __device__ __forceinline__ void dense_relu(bf16 (*act)[ALD], const bf16* __restr
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  // the W fragments of the next k slice load while this slice's MFMAs run
+  // the W fragments (fragment order: ((s * 4 + output group) * 4 + j) * 64 +
+  // lane) of the next k slice load while this slice's MFMAs run
+  const bf16* wf = Wf + ((size_t)(wave & 3) * 4 * 64 + lane) * 8;
   bf16x8 b_next[4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) b_next[j] = *reinterpret_cast<const bf16x8*>(W + (size_t)(c0 + 16 * j + li) * KIN + kb);
+  for (int j = 0; j < 4; ++j) b_next[j] = *reinterpret_cast<const bf16x8*>(wf + j * 64 * 8);
 #pragma unroll 1
   for (int k = 0; k < KIN; k += 32) {
     bf16x8 a[4], b[4];
@@ -273,7 +302,7 @@ __device__ __forceinline__ void dense_relu(bf16 (*act)[ALD], const bf16* __restr
     for (int j = 0; j < 4; ++j) b[j] = b_next[j];
     if (k + 32 < KIN)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b_next[j] = *reinterpret_cast<const bf16x8*>(W + (size_t)(c0 + 16 * j + li) * KIN + k + 32 + kb);
+      for (int j = 0; j < 4; ++j) b_next[j] = *reinterpret_cast<const bf16x8*>(wf + ((size_t)(k / 32 + 1) * 16 + j) * 64 * 8);
 #pragma unroll
     for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&act[r0 + 16 * i + li][k + kb]);
 #pragma unroll
@@ -296,109 +325,182 @@ __device__ __forceinline__ void dense_relu(bf16 (*act)[ALD], const bf16* __restr
 
 struct MlpWeights {
   const bf16* w[4];     // [out_pad][in] bf16
+  const bf16* w4f;      // the last layer's weights in fragment order (k_w4_fragments)
+  const bf16* wf[3];    // the hidden layers' weights in fragment order (k_hidden_fragments)
   const float* b[4];    // [out_pad]
   const float* rff;     // [6][N_RFF]
   const float* box;     // scene min[3], max[3]
 };
 
+// The last layer's W4 [n_pad][HID] (bf16) in the order k_infer_top4 reads it:
+// for tile T (classes 128 T ..), step s (K 32 s ..), class group c (classes
+// + 32 c ..), slice sl (K + 16 sl ..) and lane l, the 8 values lane l feeds
+// to one 32 x 32 x 16 MFMA (class 128 T + 32 c + l % 32, K 32 s + 16 sl +
+// 8 (l / 32) ..) are contiguous, so a wave reads each fragment as 1 KB of
+// consecutive memory: eight whole cache lines per load instruction instead of
+// 32-byte pieces of 32 rows.
+__global__ void k_w4_fragments(const bf16* __restrict__ w4, uint32_t n_pad, bf16* out) {
+  const uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;   // one 8-value fragment piece per thread
+  const uint64_t pieces = (uint64_t)n_pad * HID / 8;
+  if (f >= pieces) return;
+  const uint32_t l = (uint32_t)(f % 64);
+  uint64_t r = f / 64;
+  const uint32_t sl = (uint32_t)(r % 2); r /= 2;
+  const uint32_t c = (uint32_t)(r % 4); r /= 4;
+  const uint32_t s = (uint32_t)(r % (HID / 32)); r /= HID / 32;
+  const uint64_t T = r;
+  const uint64_t row = 128 * T + 32 * c + (l % 32);
+  const uint32_t k = 32 * s + 16 * sl + 8 * (l / 32);
+  *reinterpret_cast<bf16x8*>(out + f * 8) = *reinterpret_cast<const bf16x8*>(w4 + row * HID + k);
+}
+
 __global__ __launch_bounds__(IT, 4) void k_infer_top4(const float* __restrict__ rays, uint64_t n, MlpWeights mw, int n_cls, int n_pad,
                                                       uint16_t* top) {
   __shared__ __align__(16) bf16 act[IR][ALD];
+  // Per ray, the largest fourth-best logit any of its partial lists holds so
+  // far (ordered bits): the ray's final fourth-best is at least that large, so
+  // a candidate below it cannot enter the ray's top four and is skipped.
+  __shared__ uint32_t thr[IR];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t ray0 = (uint64_t)blockIdx.x * IR;
-  // features: four threads per ray, 16 frequencies each
+  if (tid < IR) thr[tid] = ordered_bits(-INFINITY);
+  // features: lane j of every wave computes frequency j for 16 rays (one ray
+  // at a time for the whole wave: the ray's inputs are wave-uniform, the
+  // Fourier column j stays in registers)
   {
-    const int rr = tid >> 2, qq = tid & 3;
-    const uint64_t m = ray0 + rr;
-    float x[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (m < n) ray_features(rays + m * 6, mw.box, x);
-    for (int j = qq * 16; j < qq * 16 + 16; ++j) {
+    const int j = lane;
+    const int w8 = __builtin_amdgcn_readfirstlane(wave);
+    float bj[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bj[k] = mw.rff[k * N_RFF + j];
+#pragma unroll 1
+    for (int i = 0; i < IR / 8; ++i) {
+      const int rr = w8 * (IR / 8) + i;
+      const uint64_t m = ray0 + rr;
+      float x[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      if (m < n) ray_features(rays + m * 6, mw.box, x);
       float z = 0.0f;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) z += x[k] * mw.rff[k * N_RFF + j];
+      for (int k = 0; k < 6; ++k) z += x[k] * bj[k];
       const float a = 6.28318530717958647692f * z;
-      act[rr][j] = f2bf(m < n ? sinf(a) : 0.0f);
-      act[rr][N_RFF + j] = f2bf(m < n ? cosf(a) : 0.0f);
+      float sv, cv;
+      sincosf(a, &sv, &cv);                      // one argument reduction for both
+      act[rr][j] = f2bf(m < n ? sv : 0.0f);
+      act[rr][N_RFF + j] = f2bf(m < n ? cv : 0.0f);
     }
   }
   __syncthreads();
-  dense_relu<F_IN>(act, mw.w[0], mw.b[0]);
-  dense_relu<HID>(act, mw.w[1], mw.b[1]);
-  dense_relu<HID>(act, mw.w[2], mw.b[2]);
-  // last layer, transposed: classes m0 + cr + 16 i + 4 (lane >> 4) + r, rays rr + 16 j + (lane & 15)
+  dense_relu<F_IN>(act, mw.wf[0], mw.b[0]);
+  dense_relu<HID>(act, mw.wf[1], mw.b[1]);
+  dense_relu<HID>(act, mw.wf[2], mw.b[2]);
+  // Last layer, transposed (classes as MFMA rows, rays as columns), on the
+  // 32x32x16 MFMA: per 128-class tile wave w owns classes cr = 32 (w >> 1) ..
+  // +32 and the two 32-ray tiles rr + 32 j, rr = 64 (w & 1); lane l holds, for
+  // ray rr + 32 j + (l & 31), the classes m0 + cr + 8 (r >> 2) + 4 (l >> 5) +
+  // (r & 3), r < 16, and keeps that ray's best four in registers (two rays per
+  // lane).  Operands: lane l supplies class (or ray) l & 31 and K values
+  // 8 (l >> 5) .. +8 of a 16-wide K slice.
   const int cr = (wave >> 1) * 32, rr = (wave & 1) * 64;
-  const int li = lane & 15, kb = (lane >> 4) * 8;
-  float tv[4][4];
-  int ti[4][4];
+  const int l32 = lane & 31, kh = (lane >> 5) * 8;
+  float tv[2][4];
+  int ti[2][4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
+  for (int j = 0; j < 2; ++j)
 #pragma unroll
     for (int k = 0; k < 4; ++k) { tv[j][k] = -INFINITY; ti[j][k] = 0x7fffffff; }
-  const bf16* W4 = mw.w[3];
   const float* b4 = mw.b[3];
-  // One step = one 32-wide k slice of one 128-class tile; the W4 fragments of
-  // the next step are loaded before this step's MFMAs are issued, so their L2
-  // latency hides behind the matrix work.
+  // One step = one 32-wide K slice of one 128-class tile (two 16-wide MFMA
+  // slices per ray tile).  The W4 stream from L2 is what bounds this loop, so
+  // its fragments run ahead in a ring of four steps' registers, loaded two
+  // steps at a time, two to three steps before use.  The ray
+  // fragments come from LDS at the start of each step; each tile's 32 biases
+  // of this wave arrive by scalar loads (the same for every lane).
   const int ksteps = HID / 32;
-  const int nsteps = (n_pad / 128) * ksteps;
-  auto w4 = [&](int step, int i) {
-    const int m0 = (step / ksteps) * 128, k = (step % ksteps) * 32;
-    return *reinterpret_cast<const bf16x8*>(W4 + (size_t)(m0 + cr + 16 * i + li) * HID + k + kb);
+  const int nsteps = (n_pad / 128) * ksteps;     // a multiple of 8
+  const bf16* W4f = mw.w4f + ((size_t)(cr / 32) * 2 * 64 + lane) * 8;
+  auto w4 = [&](int step, bf16x8 (&a)[2]) {
+    // fragment order: ((step * 4 + class group) * 2 + slice) * 64 + lane
+    const bf16* f = W4f + (size_t)step * (4 * 2 * 64 * 8);
+#pragma unroll
+    for (int sl = 0; sl < 2; ++sl) a[sl] = *reinterpret_cast<const bf16x8*>(f + sl * (64 * 8));
   };
-  bf16x8 a_next[2] = {w4(0, 0), w4(0, 1)};
-  f32x4 acc[2][4];
-#pragma unroll 1
-  for (int step = 0; step < nsteps; ++step) {
+  typedef const float __attribute__((address_space(4)))* const_fp;
+  f32x16 acc[2];
+  auto tile_step = [&](int step, const bf16x8 (&a)[2]) {
     const int kk = step % ksteps;
-    const int m0 = (step / ksteps) * 128;
+    const int tile = step / ksteps, m0 = tile * 128;
+    bf16x8 b[2][2];
+#pragma unroll
+    for (int sl = 0; sl < 2; ++sl)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b[sl][j] = *reinterpret_cast<const bf16x8*>(&act[rr + 32 * j + l32][kk * 32 + 16 * sl + kh]);
     if (kk == 0) {
+      // classes m0 + cr + 8 g + 4 h + r of lane half h = lane >> 5
+      const const_fp bs = (const_fp)(b4 + m0 + cr);
+      float bw[32];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int cb = m0 + cr + 16 * i + 4 * (lane >> 4);
-        const f32x4 bi = f32x4{b4[cb], b4[cb + 1], b4[cb + 2], b4[cb + 3]};
+      for (int k = 0; k < 32; ++k) bw[k] = bs[k];     // wave-uniform: scalar loads
+      const bool hi = (lane >> 5) != 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = bi;
-      }
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float v = hi ? bw[8 * g + 4 + r] : bw[8 * g + r];
+          acc[0][4 * g + r] = v;
+          acc[1][4 * g + r] = v;
+        }
     }
-    const bf16x8 a[2] = {a_next[0], a_next[1]};
-    if (step + 1 < nsteps) { a_next[0] = w4(step + 1, 0); a_next[1] = w4(step + 1, 1); }
-    bf16x8 b[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&act[rr + 16 * j + li][kk * 32 + kb]);
+    for (int sl = 0; sl < 2; ++sl)
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < 2; ++j) acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[sl], b[sl][j], acc[j], 0, 0, 0);
     if (kk == ksteps - 1) {
-      // Per ray, the best of its 8 candidates against the current fourth:
-      // below it nothing can enter (a tie needs >=), so the exact ordered
-      // insertion runs only for rays with a possible entry.  Classes past
-      // n_cls exist only in the last tile.
+      // Per ray, the best of its 16 candidates against th, the larger of this
+      // list's fourth and the ray's shared threshold: below th nothing can
+      // enter the ray's top four (a tie with the fourth needs >=), so the exact
+      // ordered insertion runs only for candidates at or above it.  A list
+      // whose fourth rose publishes it.  Classes past n_cls exist only in the
+      // last tile.
       const bool tail = m0 + 128 > n_cls;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        float best = fmaxf(fmaxf(fmaxf(acc[0][j][0], acc[0][j][1]), fmaxf(acc[0][j][2], acc[0][j][3])),
-                           fmaxf(fmaxf(acc[1][j][0], acc[1][j][1]), fmaxf(acc[1][j][2], acc[1][j][3])));
-        if (tail || best >= tv[j][3]) {
+      for (int j = 0; j < 2; ++j) {
+        const int ray = rr + 32 * j + l32;
+        const float th = fmaxf(tv[j][3], from_ordered(thr[ray]));
+        float best = acc[j][0];
 #pragma unroll
-          for (int i = 0; i < 2; ++i)
+        for (int r = 1; r < 16; ++r) best = fmaxf(best, acc[j][r]);
+        if (tail || best >= th) {
+          const float before = tv[j][3];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int cls = m0 + cr + 16 * i + 4 * (lane >> 4) + r;
-              if (cls < n_cls && beats(acc[i][j][r], cls, tv[j][3], ti[j][3])) top4_insert(tv[j], ti[j], acc[i][j][r], cls);
-            }
+          for (int r = 0; r < 16; ++r) {
+            const int cls = m0 + cr + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (cls < n_cls && acc[j][r] >= th && beats(acc[j][r], cls, tv[j][3], ti[j][3]))
+              top4_insert(tv[j], ti[j], acc[j][r], cls);
+          }
+          if (tv[j][3] > before) atomicMax(&thr[ray], ordered_bits(tv[j][3]));
         }
       }
     }
+  };
+  bf16x8 a0[2], a1[2], a2[2], a3[2];
+  w4(0, a0); w4(1, a1); w4(2, a2); w4(3, a3);
+#pragma unroll 1
+  for (int step = 0; step < nsteps; step += 4) {
+    tile_step(step, a0);
+    tile_step(step + 1, a1);
+    if (step + 4 < nsteps) { w4(step + 4, a0); w4(step + 5, a1); }
+    tile_step(step + 2, a2);
+    tile_step(step + 3, a3);
+    if (step + 6 < nsteps) { w4(step + 6, a2); w4(step + 7, a3); }
   }
   __syncthreads();   // act is free: merge buffer
-  constexpr int NL = 16 * 4;                                  // candidates per ray
+  constexpr int NL = 8 * 4;                                   // candidates per ray: 8 lists of 4
   float* cv = reinterpret_cast<float*>(&act[0][0]);          // [IR][NL]
   int* ci = reinterpret_cast<int*>(cv + IR * NL);             // [IR][NL]
-  const int list = (wave >> 1) * 4 + (lane >> 4);
+  const int list = (wave >> 1) * 2 + (lane >> 5);
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int ray = rr + 16 * j + li;
+  for (int j = 0; j < 2; ++j) {
+    const int ray = rr + 32 * j + l32;
 #pragma unroll
     for (int k = 0; k < 4; ++k) { cv[ray * NL + list * 4 + k] = tv[j][k]; ci[ray * NL + list * 4 + k] = ti[j][k]; }
   }
@@ -545,6 +647,8 @@ struct sq_proposal {
   int cut = 0;
   Layer L[4];
   DevBuf rff, box, sub_ref, sub_of_leaf;
+  DevBuf w4f;                      // the last layer's weights in fragment order (inference)
+  DevBuf wf[3];                    // the hidden layers' weights in fragment order (inference)
   // training temporaries (grow with the batch)
   uint64_t cap = 0;
   DevBuf F, H[3], Z, dZ, dZt, HT, dH, dHb, loss, lab, cnt, part, bpart, tmpT;
@@ -611,6 +715,8 @@ int bias_grad(sq_proposal* p, const bf16* dY, uint64_t M, int cols, float* g) {
 MlpWeights weights_of(const sq_proposal* p) {
   MlpWeights mw;
   for (int l = 0; l < 4; ++l) { mw.w[l] = p->L[l].w_nk.as<bf16>(); mw.b[l] = p->L[l].b.as<float>(); }
+  mw.w4f = p->w4f.as<bf16>();
+  for (int l = 0; l < 3; ++l) mw.wf[l] = p->wf[l].as<bf16>();
   mw.rff = p->rff.as<float>();
   mw.box = p->box.as<float>();
   return mw;
@@ -930,6 +1036,16 @@ int sq_proposal_infer(sq_proposal* p, const float* rays, uint64_t n, uint16_t* t
   if (blocks > 0x7FFFFFFFull) return fail(SQ_ERR_INVALID, "too many rays");
   int rc = begin_timing(c);
   if (rc) return rc;
+  const uint64_t pieces = (uint64_t)p->L[3].out_pad * HID / 8;
+  if ((rc = p->w4f.ensure(pieces * 16))) return rc;
+  k_w4_fragments<<<grid_for(pieces, 256), 256, 0, c->stream>>>(p->L[3].w_nk.as<bf16>(), (uint32_t)p->L[3].out_pad, p->w4f.as<bf16>());
+  HIP_TRY(hipGetLastError());
+  for (int l = 0; l < 3; ++l) {
+    const uint32_t kin = (uint32_t)p->L[l].in, hp = HID * kin / 8;
+    if ((rc = p->wf[l].ensure((size_t)hp * 16))) return rc;
+    k_hidden_fragments<<<grid_for(hp, 256), 256, 0, c->stream>>>(p->L[l].w_nk.as<bf16>(), kin, p->wf[l].as<bf16>());
+    HIP_TRY(hipGetLastError());
+  }
   k_infer_top4<<<(uint32_t)blocks, IT, 0, c->stream>>>(rays, n, weights_of(p), p->L[3].out, p->L[3].out_pad, top);
   HIP_TRY(hipGetLastError());
   return end_timing(c);
