@@ -148,7 +148,7 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as fh:
             prof = json.load(fh)
-        if prof.get("hit_mode", "reference") == args.hit_mode:
+        if prof.get("hit_mode", "reference") == args.hit_mode and prof.get("wide", 0) == ctx.tuning()["wide"]:
             traffic = int(prof["traffic_bytes"])
             valu = prof.get("valu")
     out = {

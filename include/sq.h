@@ -433,7 +433,15 @@ int sq_tlas_shapecast_all(sq_tlas* tlas, const float* casts, uint64_t n, float m
  *   wide           pooled nearest / any-hit calls without a filter: 1 walks
  *                  4-wide records (each pair node with its children's pair
  *                  nodes, rebuilt from the tree before the walk), half the
- *                  dependent node fetches per ray; 0 = the pair nodes.
+ *                  dependent node fetches per ray; 0 = the pair nodes; 2 (the
+ *                  default) = the 4-wide records for the walks in the
+ *                  reference's order (nearest rays over SPHERE / TRIANGLE
+ *                  leaves in SQ_HITS_REFERENCE; 0.92x the time on the 8.4M
+ *                  heightfield rays, 0.86x on the incoherent half), the pair
+ *                  nodes elsewhere.  Over SPHERE / TRIANGLE leaves in
+ *                  SQ_HITS_REFERENCE the 4-wide walk needs a tree without
+ *                  over-full leaves (coincident centroids); other trees take
+ *                  the pair nodes.
  *   packet         nearest-hit ray calls without a filter: 1 walks each 64
  *                  rays of the batch (in the Morton order above, for calls of
  *                  >= 4096 rays) as one wave-coherent packet: one walk per

@@ -690,6 +690,7 @@ int device_build(sq_bvh* b, const float* prims) {
   P = sqh::Packed();
   P.dim = D;
   P.n_pairs = n_pairs;
+  P.n_synth = n_synth;
   P.max_stack = n_pairs ? maxd + 2 : 1;
   P.level_off.push_back(0);
   for (size_t d = hist.size(); d-- > 0;) P.level_off.push_back(P.level_off.back() + hist[d]);
@@ -750,6 +751,7 @@ int ensure_host_tree(sq_bvh* b) {
     for (int k = 0; k < D; ++k) { bo[k] = nd[4 * k + 2 * s]; bo[D + k] = nd[4 * k + 2 * s + 1]; }
   }
   const uint32_t max_stack = b->packed.max_stack;
+  const uint32_t n_synth = b->packed.n_synth;
   const auto level_off = b->packed.level_off;
   b->tree = std::move(t);
   try {
@@ -758,6 +760,7 @@ int ensure_host_tree(sq_bvh* b) {
     return fail(SQ_ERR_HIP, e.what());
   }
   b->packed.max_stack = max_stack;
+  b->packed.n_synth = n_synth;
   b->packed.level_off = level_off;
   b->stats.quality = sqh::tree_interior_surface_sum(b->tree);
   b->host_tree_ok = true;

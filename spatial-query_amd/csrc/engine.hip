@@ -899,7 +899,7 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   if (t->ray_order != 0 && t->ray_order != 1) return fail(SQ_ERR_INVALID, "ray_order must be 0 or 1");
   if (t->descend_min < 0 || t->descend_min > 64) return fail(SQ_ERR_INVALID, "descend_min must be in [0, 64]");
   if (t->split_depth < -1 || t->split_depth > 24) return fail(SQ_ERR_INVALID, "split_depth must be in [-1, 24]");
-  if (t->wide != 0 && t->wide != 1) return fail(SQ_ERR_INVALID, "wide must be 0 or 1");
+  if (t->wide < 0 || t->wide > 2) return fail(SQ_ERR_INVALID, "wide must be 0, 1 or 2");
   if (t->packet != 0 && t->packet != 1) return fail(SQ_ERR_INVALID, "packet must be 0 or 1");
   c->block = block;
   c->rays_per_lane = t->rays_per_lane;
@@ -1313,10 +1313,15 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
   const uint32_t per = pooled_per_lane(c, n);
   const uint64_t chunk = 64ull * per;
   const uint64_t chunks = (n + chunk - 1) / chunk;
-  // 4-wide walk (nearest and any-hit without a filter, over hits inside their
-  // boxes: AABB leaves, or the guarded kinds): up to 3 pushes per record over
-  // half the levels
-  const bool quad = c->wide && !cast && !filt && (b->kind == KIND_AABB || c->hit_mode == SQ_HITS_GUARDED);
+  // 4-wide walk (nearest and any-hit without a filter; near-first over hits
+  // inside their boxes: AABB leaves, or the guarded kinds; else in the
+  // reference's order over a tree without synthetic pairs): up to 3 pushes
+  // per record over half the levels.  wide 2 takes it for the walks in the
+  // reference's order only (nearest rays over SPHERE / TRIANGLE leaves in the
+  // default hit mode), where it pays most
+  const bool ref_walk = !any && c->hit_mode != SQ_HITS_GUARDED && b->kind != KIND_AABB;
+  const bool quad = (c->wide == 1 || (c->wide == 2 && ref_walk)) && !cast && !filt &&
+                    (b->kind == KIND_AABB || c->hit_mode == SQ_HITS_GUARDED || b->packed.n_synth == 0);
   const uint32_t walk_stack = quad ? 3 * (stack_n / 2 + 1) + 1 : stack_n;
   const uint32_t spill_rows = walk_stack > S ? walk_stack - S : 0;
   const size_t lds = (size_t)S * B * 8;
@@ -1328,7 +1333,8 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
     const uint32_t nf = (uint32_t)(b->dim + 1) * 4;
     const uint32_t n_nodes = (uint32_t)(b->nodes.bytes / ((size_t)nf * 4));
     if ((rc = b->quads.ensure((size_t)std::max<uint32_t>(n_nodes, 1) * 2 * nf * 4))) return rc;
-    rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
+    // (a tree whose root is a leaf has no pair nodes and no records to build)
+    if (n_nodes) rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
       k_quads<D><<<grid_for(n_nodes, 256), 256, 0, c->stream>>>(b->nodes.as<float>(), n_nodes, b->quads.as<float>());
       HIP_TRY(hipGetLastError());
       return SQ_OK;
@@ -1357,10 +1363,8 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
       return SQ_OK;
     };
     auto pick = [&]<bool EXACT_>() -> int {
-      if constexpr (K == KIND_AABB || !EXACT_) {   // the 4-wide walk needs hits inside their boxes
-        if (quad) return any ? launch.template operator()<true, false, false, true, EXACT_>()
-                             : launch.template operator()<false, false, false, true, EXACT_>();
-      }
+      if (quad) return any ? launch.template operator()<true, false, false, true, EXACT_>()
+                           : launch.template operator()<false, false, false, true, EXACT_>();
       if (cast) return filt ? launch.template operator()<false, true, true, false, EXACT_>()
                             : launch.template operator()<false, false, true, false, EXACT_>();
       if (any) return filt ? launch.template operator()<true, true, false, false, EXACT_>()

@@ -69,7 +69,8 @@ struct sq_ctx {
   int ray_order = 0;               // 1: walk rays in Morton order (order.hip)
   int descend_min = 24;            // pooled walks: pause descents below this many lanes (pool.h)
   int split_depth = 0;             // pooled collect: split queries this many levels down (0 auto, -1 never)
-  int wide = 0;                    // pooled nearest / any: walk the 4-wide records (traverse.h quad_records)
+  int wide = 2;                    // pooled nearest / any: walk the 4-wide records (traverse.h quad_records);
+                                   // 2 = for the walks in the reference's order only
   int packet = 0;                  // nearest rays: 1 = one wave-coherent walk per 64 rays (packet.hip)
   int hit_mode = SQ_HITS_REFERENCE;   // SQ_HITS_REFERENCE or SQ_HITS_GUARDED (sq.h)
   bool timed = false;

@@ -325,6 +325,8 @@ void pack_tree(const RefTree& t, Packed& out) {
     std::memcpy(&out.nodes[(size_t)p * NF + 4 * D + 3], &synth[p], 4);
   }
   out.n_pairs = np;
+  out.n_synth = 0;
+  for (uint32_t p = 0; p < np; ++p) out.n_synth += synth[p];
 
   // Level lists, deepest first, for the level-synchronous device refit.
   uint32_t maxd = 0;

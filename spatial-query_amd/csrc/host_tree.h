@@ -60,6 +60,7 @@ struct Packed {
   // reference node except the root, the (pair, slot) holding its box.
   std::vector<uint32_t> slot_of_ref; // pair*2 + slot; root = 0xFFFFFFFF
   uint32_t max_stack = 0;            // pair-tree depth bound for the traversal stack
+  uint32_t n_synth = 0;              // synthetic pair nodes (chains of over-full leaves)
 };
 
 // Re-pack a reference tree for the device.

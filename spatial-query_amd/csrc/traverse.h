@@ -876,14 +876,26 @@ struct NoPause {
 // COUNT adds the step's work to *wc.  SKIP treats the child refs skip[0..3] as
 // missed: subtrees this query has already walked completely (with a limit at
 // least the current one, so they hold nothing it has not seen).  QUAD walks
-// the 4-wide records near-first and needs Q::INSIDE (its records hold the
-// synthetic pairs' boxes like any other).
+// the 4-wide records: near-first with Q::INSIDE (its records hold the
+// synthetic pairs' boxes like any other), or with REF in the reference's
+// order over a tree without synthetic pairs.  The REF form visits the
+// grandchildren bb, ba, ab, aa of a pair node (child b's subtree first, its
+// own child b first, as the reference pushes a then b at every node) and
+// tests each against the limit when it is popped, so it visits the leaves the
+// pair walk visits, in the same order, with the same limits: a grandchild's
+// slab interval lies inside its parent's (parent and child bounds nest and
+// each slab operation rounds monotonically), so a grandchild passing at its
+// pop means the skipped test of its parent, made at the same limit (nothing
+// is tested between the two pops), passes too.
 template <int D, int PS, bool ANY, bool FILT, bool COUNT = false, bool SKIP = false, bool QUAD = false, bool REF = false,
           class Q, class P = NoPause>
 SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const RingStack& st, WalkState& w, WorkCount* wc = nullptr,
                      const uint32_t* skip = nullptr, const P& pause = P()) {
   constexpr int NF = (D + 1) * 4;
-  static_assert(!QUAD || (Q::INSIDE && !REF), "the 4-wide walk is near-first over inside hits only");
+  // (any-hit queries never shrink their limit, so they visit the same leaves
+  // in any order; without synthetic pairs the 4-wide records prune by the
+  // pair layout's own boxes)
+  static_assert(!QUAD || Q::INSIDE || REF || ANY, "the 4-wide walk over hits that may leave their boxes walks the reference's order");
   if constexpr (QUAD) {
     // descend over the 4-wide records: the nearest child next, the other hit
     // children pushed farthest first (so they pop nearest first)
@@ -899,6 +911,34 @@ SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const Ring
       r[2] = fbits(nd[NF + 4 * D + 0]); r[3] = fbits(nd[NF + 4 * D + 1]);
       h[1] = h[1] && fbits(nd[4 * D + 2]) == 0u;
       h[3] = h[3] && fbits(nd[NF + 4 * D + 2]) == 0u;
+      if constexpr (REF) {
+        // slots in the reference's order aa, ab, ba, bb are pushed as found,
+        // the last hit one walked next: bb's subtree first, aa's last
+        bool have = false;
+        uint32_t nr = 0;
+        float ntt = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (!h[k]) continue;
+          if (have) st.push(w.sp, nr, ntt);
+          have = true;
+          nr = r[k];
+          ntt = t[k];
+        }
+        if (have) {
+          w.cur = nr;
+        } else {
+          bool got = false;
+          while (w.sp > 0) {
+            uint32_t rr; float tt;
+            st.pop(w.sp, rr, tt);
+            if (tt <= w.limit) { w.cur = rr; got = true; break; }
+          }
+          if (!got) return true;
+        }
+        if (pause()) return false;
+        continue;
+      }
 #pragma unroll
       for (int k = 0; k < 4; ++k) t[k] = h[k] ? t[k] : SQ_INF;
       const uint32_t nh = (uint32_t)h[0] + (uint32_t)h[1] + (uint32_t)h[2] + (uint32_t)h[3];

@@ -159,7 +159,7 @@ class Context:
         check(lib().sq_ctx_set_block(self._h, threads))
 
     def set_tuning(self, block: int = 128, rays_per_lane: int = 16, lds_stack: int = 8, ray_order: int = 0,
-                   descend_min: int = 24, split_depth: int = 0, wide: int = 0, packet: int = 0):
+                   descend_min: int = 24, split_depth: int = 0, wide: int = 2, packet: int = 0):
         check(lib().sq_ctx_set_tuning(self._h, C.byref(Tuning(block, rays_per_lane, lds_stack, ray_order, descend_min,
                                                               split_depth, wide, packet))))
 

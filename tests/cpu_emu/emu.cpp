@@ -62,7 +62,7 @@ template <class F> void par_for(uint64_t n, int threads, F&& f) {
 // The pooled kernel's per-lane walk: walk_begin + walk_step with a ring stack
 // of `ring` LDS slots (here a local array) spilling to a bounds-checked column.
 template <int D, int K, bool QUAD = false, bool REF = false, class Pol>
-NearestResult ring_walk(const Built& b, const Pol& pol, float max_toi, int any, uint32_t ring) {
+NearestResult ring_walk(const Built& b, const Pol& pol, float max_toi, int any, uint32_t ring, uint32_t* prims_tested = nullptr) {
   constexpr int PS = Prim<D, K>::STRIDE;
   const SceneView s = b.view();
   Filter f{nullptr, ~0ull, -1};
@@ -75,11 +75,15 @@ NearestResult ring_walk(const Built& b, const Pol& pol, float max_toi, int any, 
   RingStack st{lref.data(), lt.data(), 1, ring, col.data(), 1};
   WalkState w;
   if (!walk_begin<D>(s, pol, max_toi, w)) return w.res;
+  WorkCount wc;
   for (;;) {
-    const bool done = any ? walk_step<D, PS, true, false, false, false, QUAD>(s, pol, f, st, w)
-                          : walk_step<D, PS, false, false, false, false, QUAD, REF>(s, pol, f, st, w);
+    const bool done = any ? walk_step<D, PS, true, false, true, false, QUAD>(s, pol, f, st, w, &wc)
+                          : walk_step<D, PS, false, false, true, false, QUAD, REF>(s, pol, f, st, w, &wc);
     if (w.sp > bound) { fprintf(stderr, "stack bound exceeded\n"); abort(); }
-    if (done) return w.res;
+    if (done) {
+      if (prims_tested) *prims_tested = wc.prims;
+      return w.res;
+    }
   }
 }
 
@@ -124,15 +128,18 @@ void nearest_dk(const Built& b, const float* q, uint64_t nq, float max_toi, int 
   });
 }
 
-// The 4-wide walk (walk_step with QUAD) of every ray over the guarded kinds,
-// ring stack as above.
-template <int D, int K>
+// The 4-wide walk (walk_step with QUAD) of every ray, ring stack as above:
+// GUARD near-first over the guarded kinds; else unguarded, nearest queries
+// over leaves whose hits may leave their box in the reference's order (the
+// engine takes this form only for trees without synthetic pairs).
+template <int D, int K, bool GUARD>
 void nearest_quad_dk(Built& b, const float* q, uint64_t nq, float max_toi, int any, int threads, int32_t* leaf, float* toi,
-                     uint32_t ring) {
+                     uint32_t ring, uint32_t* prims_tested) {
   if (b.quads.empty()) build_quads<D>(b);
+  constexpr bool REF = !GUARD && !RayPolicy<D, K, GUARD>::INSIDE;
   par_for(nq, threads, [&](uint64_t i) {
-    RayPolicy<D, K, true> pol; pol.setup(q + i * 2 * D);
-    NearestResult h = ring_walk<D, K, true>(b, pol, max_toi, any, ring);
+    RayPolicy<D, K, GUARD> pol; pol.setup(q + i * 2 * D);
+    NearestResult h = ring_walk<D, K, true, REF>(b, pol, max_toi, any, ring, prims_tested ? prims_tested + i : nullptr);
     leaf[i] = h.leaf == NO_LEAF ? -1 : (int32_t)h.leaf;
     toi[i] = h.t;
   });
@@ -185,13 +192,41 @@ int emu_nearest(void* h, int dim, int kind, const float* q, uint64_t nq, float m
   });
 }
 
-// The 4-wide walk (tuning `wide`) with a ring stack of `ring` slots.
+// The 4-wide walk (tuning `wide`) with a ring stack of `ring` slots; -2 when
+// an unguarded walk is asked of a tree with synthetic pairs.
+// prims_tested (optional, [nq]): primitives each walk tested.
 int emu_nearest_quad(void* h, int dim, int kind, const float* q, uint64_t nq, float max_toi, int any, int threads,
-                     int32_t* leaf, float* toi, uint32_t ring) {
+                     int32_t* leaf, float* toi, uint32_t ring, int guarded, uint32_t* prims_tested) {
   Built& b = *(Built*)h;
   if (b.tree.prim_indices.empty()) { for (uint64_t i = 0; i < nq; ++i) { leaf[i] = -1; toi[i] = SQ_INF; } return 0; }
-  return dk(dim, kind, [&]<int D, int K>() { nearest_quad_dk<D, K>(b, q, nq, max_toi, any, threads, leaf, toi, ring); });
+  if (!guarded && kind != 0 && b.packed.n_synth != 0) return -2;
+  return dk(dim, kind, [&]<int D, int K>() {
+    if (guarded) nearest_quad_dk<D, K, true>(b, q, nq, max_toi, any, threads, leaf, toi, ring, prims_tested);
+    else nearest_quad_dk<D, K, false>(b, q, nq, max_toi, any, threads, leaf, toi, ring, prims_tested);
+  });
 }
+
+// The pair walk of emu_nearest with a ring stack (ring > 0), returning the
+// primitives each walk tested as well.
+int emu_nearest_pair_counted(void* h, int dim, int kind, const float* q, uint64_t nq, float max_toi, int guarded, uint32_t ring,
+                             int32_t* leaf, float* toi, uint32_t* prims_tested) {
+  const Built& b = *(Built*)h;
+  if (b.tree.prim_indices.empty() || !ring) return -1;
+  return dk(dim, kind, [&]<int D, int K>() {
+    auto go = [&]<bool GUARD>() {
+      constexpr bool REF = !GUARD && !RayPolicy<D, K, GUARD>::INSIDE;
+      par_for(nq, 8, [&](uint64_t i) {
+        RayPolicy<D, K, GUARD> pol; pol.setup(q + i * 2 * D);
+        NearestResult r = ring_walk<D, K, false, REF>(b, pol, max_toi, 0, ring, prims_tested + i);
+        leaf[i] = r.leaf == NO_LEAF ? -1 : (int32_t)r.leaf;
+        toi[i] = r.t;
+      });
+    };
+    if (guarded) go.template operator()<true>();
+    else go.template operator()<false>();
+  });
+}
+uint32_t emu_n_synth(void* h) { return ((Built*)h)->packed.n_synth; }
 
 // The reference's counters of every ray (walk_counted, as k_counted runs it):
 // nearest (all = 0, leaf and toi written) or all hits; stats[nq][4].

@@ -29,7 +29,11 @@ def emu():
     L.emu_collect.restype = C.c_uint64
     L.emu_collect.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, P, P]
     L.emu_required.argtypes = [P, C.c_int, P, C.c_uint64, P, P, P]
-    L.emu_nearest_quad.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_int, P, P, C.c_uint32]
+    L.emu_nearest_quad.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_int, P, P, C.c_uint32,
+                                   C.c_int, P]
+    L.emu_nearest_pair_counted.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_uint32, P, P, P]
+    L.emu_n_synth.restype = C.c_uint32
+    L.emu_n_synth.argtypes = [P]
     L.emu_counted.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, P, P, P]
     return L
 
@@ -51,11 +55,23 @@ class Emu:
                            int(guarded))
         return leaf.astype(np.int64), t
 
-    def nearest_quad(self, q, max_toi, any_=0, ring=8):
+    def nearest_quad(self, q, max_toi, any_=0, ring=8, guarded=True, counted=False):
         q = np.ascontiguousarray(q, np.float32)
         leaf = np.empty(len(q), np.int32); t = np.empty(len(q), np.float32)
-        self.L.emu_nearest_quad(self.h, self.dim, self.kind, p(q), len(q), max_toi, any_, 8, p(leaf), p(t), ring)
-        return leaf.astype(np.int64), t
+        tested = np.zeros(len(q), np.uint32)
+        rc = self.L.emu_nearest_quad(self.h, self.dim, self.kind, p(q), len(q), max_toi, any_, 8, p(leaf), p(t), ring,
+                                     int(guarded), p(tested))
+        assert rc == 0, rc
+        return (leaf.astype(np.int64), t, tested) if counted else (leaf.astype(np.int64), t)
+
+    def nearest_pair_counted(self, q, max_toi, ring=8, guarded=False):
+        q = np.ascontiguousarray(q, np.float32)
+        leaf = np.empty(len(q), np.int32); t = np.empty(len(q), np.float32)
+        tested = np.zeros(len(q), np.uint32)
+        rc = self.L.emu_nearest_pair_counted(self.h, self.dim, self.kind, p(q), len(q), max_toi, int(guarded), ring,
+                                             p(leaf), p(t), p(tested))
+        assert rc == 0, rc
+        return leaf.astype(np.int64), t, tested
 
     def counted(self, q, max_toi, all_=0):
         q = np.ascontiguousarray(q, np.float32)
@@ -205,6 +221,51 @@ def test_quad_walk(emu, ring):
     one = Emu(emu, 3, 0, W.boxes_scene(n=1, seed=1).prims)
     r1 = oracle.Scene(3, oracle.BOX, W.boxes_scene(n=1, seed=1).prims).raycast_nearest(q, 500.0)
     assert eq(*one.nearest_quad(q, 500.0, ring=ring), r1["leaf"], r1["t"])
+
+
+@pytest.mark.parametrize("ring", [2, 8])
+def test_quad_walk_reference_order(emu, ring):
+    """The 4-wide walk in the reference's order (SQ_HITS_REFERENCE, plain
+    narrow phases, trees without synthetic pairs) equals the oracle's plain
+    providers on every ray, including the rays whose answer is a hit outside
+    its own box (where any other order may differ): the heightfield with
+    coherent and incoherent rays, balls in D = 2 and 5, nearest and any-hit.
+    A walk order that takes child a's subtree first, or a grandchild tested
+    at push time only, fails on those rays."""
+    sc = W.heightfield_scene(grid=257, extent=128.0)
+    rays = np.concatenate([W.camera_rays(sc, res=128, seed=0x12), W.incoherent_rays(sc, 30000)])
+    plain = oracle.Scene(3, oracle.TRI, sc.prims).raycast_nearest(rays, 2000.0, threads=8)
+    guarded = oracle.Scene(3, oracle.TRI, sc.prims, guarded=True).raycast_nearest(rays, 2000.0, threads=8)
+    e = Emu(emu, 3, 2, sc.prims)
+    assert emu.emu_n_synth(e.h) == 0
+    ql, qt, qn = e.nearest_quad(rays, 2000.0, ring=ring, guarded=False, counted=True)
+    assert eq(ql, qt, plain["leaf"], plain["t"])
+    # the same leaves in the same order with the same limits: every ray tests
+    # exactly the primitives the pair walk in the reference's order tests
+    # (a near-first or a-first order tests different ones on most hitting rays)
+    pl, pt, pn = e.nearest_pair_counted(rays, 2000.0, ring=ring, guarded=False)
+    assert eq(pl, pt, plain["leaf"], plain["t"])
+    assert np.array_equal(qn, pn)
+    gn = e.nearest_quad(rays, 2000.0, ring=ring, guarded=True, counted=True)[2]
+    assert (gn != pn).sum() > len(rays) // 10
+    assert np.array_equal(e.nearest_quad(rays, 2000.0, any_=1, ring=ring, guarded=False)[0] >= 0, plain["leaf"] >= 0)
+    differ = int((plain["leaf"] != guarded["leaf"]).sum())
+    for dim, half in ((2, 40.0), (5, 10.0)):
+        s = W.Stream(0x16 if dim == 2 else 0x14 + dim)
+        b = np.concatenate([s.range(-half, half, 4000 * dim).reshape(-1, dim), s.range(0.3, 3.0, 4000).reshape(-1, 1)],
+                           1).astype(np.float32)
+        qq = W.random_rays(4000, 0x17 if dim == 2 else 0x15 + dim, -half - 5.0, half + 5.0, dim=dim)
+        eb = Emu(emu, dim, 1, b)
+        if emu.emu_n_synth(eb.h):
+            continue
+        rb = oracle.Scene(dim, oracle.SPHERE, b).raycast_nearest(qq, 300.0)
+        rg = oracle.Scene(dim, oracle.SPHERE, b, guarded=True).raycast_nearest(qq, 300.0)
+        differ += int((rb["leaf"] != rg["leaf"]).sum())
+        ql, qt, qn = eb.nearest_quad(qq, 300.0, ring=ring, guarded=False, counted=True)
+        assert eq(ql, qt, rb["leaf"], rb["t"])
+        assert np.array_equal(qn, eb.nearest_pair_counted(qq, 300.0, ring=ring)[2])
+        assert np.array_equal(eb.nearest_quad(qq, 300.0, any_=1, ring=ring, guarded=False)[0] >= 0, rb["leaf"] >= 0)
+    assert differ > 0, "the scenes should hold rays whose reference answer lies outside its box"
 
 
 @pytest.mark.parametrize("ring", [0, 2, 8])
