@@ -148,7 +148,9 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as fh:
             prof = json.load(fh)
-        if prof.get("hit_mode", "reference") == args.hit_mode and prof.get("wide", 0) == ctx.tuning()["wide"]:
+        tun_now = ctx.tuning()
+        if (prof.get("hit_mode", "reference") == args.hit_mode and prof.get("wide", 0) == tun_now["wide"]
+                and prof.get("lookahead", 0) == tun_now["lookahead"]):
             traffic = int(prof["traffic_bytes"])
             valu = prof.get("valu")
     out = {

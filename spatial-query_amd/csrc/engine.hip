@@ -124,8 +124,11 @@ __global__ __launch_bounds__(256) void k_counted(SceneView s, const float* __res
 // leaves them zero (pool_retire).  With `order`, pool
 // position i is query order[i] (the Morton order).  QUAD walks the 4-wide
 // records s.quads (k_quads) instead of the pair nodes.  EXACT as k_nearest.
-template <int D, int KIND, bool ANY, bool FILT, bool CAST, bool QUAD, bool EXACT>
-__global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nearest_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
+// SPEC: the reference-order walks take the look-ahead form (walk_step_spec),
+// held to 7 waves per SIMD (its pending leaf and position need registers the
+// 8-wave budget of 64 would spill: C1 5.68 against 5.81 ms; 6 waves 6.41).
+template <int D, int KIND, bool ANY, bool FILT, bool CAST, bool QUAD, bool EXACT, bool SPEC = false>
+__global__ __launch_bounds__(256, (SPEC ? 7 : (D <= 3 && !CAST && !FILT ? 8 : 4))) void k_nearest_pool(SceneView s, const float* __restrict__ q, uint32_t n, float max_toi,
                                                       const float* __restrict__ tmax, Filter f, NearestOut out,
                                                       uint32_t chunk, uint32_t* counter, uint32_t S, SpillEntry* spill,
                                                       uint32_t spill_stride, const uint32_t* __restrict__ order,
@@ -136,8 +139,8 @@ __global__ __launch_bounds__(256, (D <= 3 && !CAST && !FILT ? 8 : 4)) void k_nea
   RingStack st{lds + tid, reinterpret_cast<float*>(lds + S * B) + tid, B, S, spill + gtid, spill_stride};
   const uint64_t waves = (uint64_t)gridDim.x * (B >> 6);
   const uint64_t home = (uint64_t)blockIdx.x * (B >> 6) + (tid >> 6);
-  pool_walk<D, KIND, ANY, FILT, CAST, Chunks, QUAD, EXACT>(s, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax, f, out,
-                                                         st, order, pause_below);
+  pool_walk<D, KIND, ANY, FILT, CAST, Chunks, QUAD, EXACT, SPEC>(s, q, Chunks{counter, n, chunk, home, waves}, max_toi, tmax,
+                                                               f, out, st, order, pause_below);
   pool_retire(counter);
 }
 
@@ -901,6 +904,7 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   if (t->split_depth < -1 || t->split_depth > 24) return fail(SQ_ERR_INVALID, "split_depth must be in [-1, 24]");
   if (t->wide < 0 || t->wide > 2) return fail(SQ_ERR_INVALID, "wide must be 0, 1 or 2");
   if (t->packet != 0 && t->packet != 1) return fail(SQ_ERR_INVALID, "packet must be 0 or 1");
+  if (t->lookahead != 0 && t->lookahead != 1) return fail(SQ_ERR_INVALID, "lookahead must be 0 or 1");
   c->block = block;
   c->rays_per_lane = t->rays_per_lane;
   c->lds_stack = S;
@@ -909,6 +913,7 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   c->split_depth = t->split_depth;
   c->wide = t->wide;
   c->packet = t->packet;
+  c->lookahead = t->lookahead;
   return SQ_OK;
 }
 
@@ -931,6 +936,7 @@ int sq_ctx_get_tuning(const sq_ctx* c, sq_tuning* t) {
   t->split_depth = c->split_depth;
   t->wide = c->wide;
   t->packet = c->packet;
+  t->lookahead = c->lookahead;
   return SQ_OK;
 }
 
@@ -1347,8 +1353,8 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
     if ((rc = ray_order(c, b->dim, q, nn, 2 * b->dim, &order))) return rc;
   uint32_t* counter = c->pool_ctr.as<uint32_t>();
   rc = dispatch_dk(b->dim, b->kind, [&]<int D, int K>() -> int {
-    auto launch = [&]<bool ANY_, bool FILT_, bool CAST_, bool QUAD_, bool EXACT_>() -> int {
-      const void* kern = reinterpret_cast<const void*>(&k_nearest_pool<D, K, ANY_, FILT_, CAST_, QUAD_, EXACT_>);
+    auto launch = [&]<bool ANY_, bool FILT_, bool CAST_, bool QUAD_, bool EXACT_, bool SPEC_ = false>() -> int {
+      const void* kern = reinterpret_cast<const void*>(&k_nearest_pool<D, K, ANY_, FILT_, CAST_, QUAD_, EXACT_, SPEC_>);
       int per_cu = 0;
       if (int r0 = blocks_per_cu(kern, c->device, B, lds, &per_cu)) return r0;
       const uint64_t wpb = (uint64_t)B / 64;
@@ -1356,13 +1362,21 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
       const uint64_t lanes = grid * (uint64_t)B;
       int r2 = c->spill.ensure((size_t)spill_rows * lanes * sizeof(SpillEntry) + 64);
       if (r2) return r2;
-      k_nearest_pool<D, K, ANY_, FILT_, CAST_, QUAD_, EXACT_><<<(uint32_t)grid, B, lds, c->stream>>>(
+      k_nearest_pool<D, K, ANY_, FILT_, CAST_, QUAD_, EXACT_, SPEC_><<<(uint32_t)grid, B, lds, c->stream>>>(
           s, q, nn, max_toi, tmax, fd, out, (uint32_t)chunk, counter, S, c->spill.as<SpillEntry>(), (uint32_t)lanes, order,
           (uint32_t)c->descend_min);
       HIP_TRY(hipGetLastError());
       return SQ_OK;
     };
     auto pick = [&]<bool EXACT_>() -> int {
+      // the look-ahead form of the walks in the reference's order
+      if constexpr (EXACT_ && K != KIND_AABB) {
+        if (c->lookahead && !any && !cast) {
+          if (quad) return launch.template operator()<false, false, false, true, true, true>();
+          return filt ? launch.template operator()<false, true, false, false, true, true>()
+                      : launch.template operator()<false, false, false, false, true, true>();
+        }
+      }
       if (quad) return any ? launch.template operator()<true, false, false, true, EXACT_>()
                            : launch.template operator()<false, false, false, true, EXACT_>();
       if (cast) return filt ? launch.template operator()<false, true, true, false, EXACT_>()

@@ -114,14 +114,15 @@ struct Chunks {
 // optional point / normal / sub outputs mean (the built-in kinds); CAST marks
 // shape casts (no normal); REF walks the reference's order (walk_step).
 template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Pol, int QS, int PS, class Src, bool QUAD = false,
-          bool REF = false>
+          bool REF = false, bool SPEC = false>
 __device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
                                                const float* __restrict__ tmax, const Filter& f, const NearestOut& out,
                                                const RingStack& st, const uint32_t* __restrict__ order, uint32_t pause_below) {
+  static_assert(!SPEC || (REF && !ANY), "the look-ahead walk is the reference-order nearest walk");
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   Pol pol;
-  WalkState w;
+  typename std::conditional<SPEC, SpecState, WalkState>::type w;
   float lim0 = 0.0f;
   uint32_t rid = NO_LEAF;
   uint64_t next = 0, end = 0;
@@ -145,7 +146,10 @@ __device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* 
           rid = order ? order[my] : (uint32_t)my;
           pol.setup(q + (size_t)rid * QS);
           lim0 = tmax ? tmax[rid] : max_toi;
-          if (!walk_begin<D>(s, pol, lim0, w)) {
+          bool live;
+          if constexpr (SPEC) live = spec_begin<D>(s, pol, lim0, w);
+          else live = walk_begin<D>(s, pol, lim0, w);
+          if (!live) {
             if (ANY) out.any[rid] = 0;
             else {
               out.leaf[rid] = -1; out.toi[rid] = SQ_INF;
@@ -164,8 +168,10 @@ __device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* 
     if (__ballot(!(exhausted && rid == NO_LEAF)) == 0) break;
     const uint32_t active = (uint32_t)__popcll(__ballot(rid != NO_LEAF));
     if (rid == NO_LEAF) continue;
-    if (!walk_step<D, PS, ANY, FILT, false, false, QUAD, REF>(s, pol, f, st, w, nullptr, nullptr, WavePause{pause_below, active}))
-      continue;
+    bool done;
+    if constexpr (SPEC) done = walk_step_spec<D, PS, FILT, QUAD>(s, pol, f, st, w, WavePause{pause_below, active});
+    else done = walk_step<D, PS, ANY, FILT, false, false, QUAD, REF>(s, pol, f, st, w, nullptr, nullptr, WavePause{pause_below, active});
+    if (!done) continue;
     // finished: write the result of query rid
     const NearestResult& h = w.res;
     const bool hit = h.leaf != NO_LEAF;
@@ -194,14 +200,15 @@ __device__ __forceinline__ void pool_walk_with(const SceneView& s, const float* 
 // The built-in kinds: rays (RayPolicy) or shape casts (CastPolicy); QUAD
 // walks the 4-wide records (s.quads).  EXACT: the reference's answer for every
 // query (unguarded narrow phases; nearest queries whose hits may leave their
-// box walk the reference's order); else the guarded kinds near-first.
-template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Src, bool QUAD = false, bool EXACT = true>
+// box walk the reference's order); else the guarded kinds near-first.  SPEC:
+// the reference-order walks take the look-ahead form (walk_step_spec).
+template <int D, int KIND, bool ANY, bool FILT, bool CAST, class Src, bool QUAD = false, bool EXACT = true, bool SPEC = false>
 __device__ __forceinline__ void pool_walk(const SceneView& s, const float* __restrict__ q, Src src, float max_toi,
                                           const float* __restrict__ tmax, const Filter& f, const NearestOut& out,
                                           const RingStack& st, const uint32_t* __restrict__ order, uint32_t pause_below) {
   using Pol = typename std::conditional<CAST, CastPolicy<D, KIND, float, !EXACT>, RayPolicy<D, KIND, !EXACT>>::type;
   constexpr bool REF = EXACT && !ANY && !Pol::INSIDE;
-  pool_walk_with<D, KIND, ANY, FILT, CAST, Pol, CAST ? 4 * D : 2 * D, Prim<D, KIND>::STRIDE, Src, QUAD, REF>(
+  pool_walk_with<D, KIND, ANY, FILT, CAST, Pol, CAST ? 4 * D : 2 * D, Prim<D, KIND>::STRIDE, Src, QUAD, REF, SPEC && REF>(
       s, q, src, max_toi, tmax, f, out, st, order, pause_below);
 }
 

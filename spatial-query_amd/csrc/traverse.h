@@ -1027,6 +1027,146 @@ SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const Ring
   return true;
 }
 
+// Look-ahead form of the walk in the reference's order (REF): a lane that
+// reaches a leaf keeps it pending and goes on descending with the limit it
+// has, towards its next leaf in the same order, and stops there; the pending
+// leaf is then tested and the descent position is tested again against the
+// new limit (popped past when its entry lies beyond it).  Leaves are tested in
+// the reference's order, each with the limit the reference tests it with and
+// only when its entry passes that limit: entries only grow down the tree, so
+// a node passed on the larger limit holds no leaf whose entry passes a smaller
+// one unless the node's own entry does, and every entry pushed is tested
+// again when popped.  So the walk tests exactly the primitives walk_step
+// tests, while lanes that reach a leaf early take their next descent instead
+// of waiting for the rest of the wave.  QUAD walks the 4-wide records (trees
+// without synthetic pairs), else the pair nodes (synthetic pairs passed
+// through in leaf order, as order_pair does).
+constexpr uint32_t NO_CUR = 0xFFFFFFFFu;   // no descent position (never a leaf ref: at most 2^27 - 1 primitives)
+
+struct SpecState {
+  uint32_t cur, sp, pend;   // descent position (NO_CUR: none left), stack depth, pending leaf (0: none)
+  float cur_t, limit;       // the position's entry; the current limit
+  NearestResult res;
+};
+
+template <int D, class Q>
+SQ_HD bool spec_begin(const SceneView& s, const Q& q, float max_toi, SpecState& w) {
+  w.res = NearestResult{SQ_INF, NO_LEAF, NO_LEAF};
+  w.limit = max_toi;
+  w.sp = 0;
+  w.pend = 0;
+  w.cur = fbits(s.root[2 * D]);
+  w.cur_t = -SQ_INF;   // the root is the first position, descended before any leaf test
+  return q.root(s.root, max_toi);
+}
+
+// Next stack entry whose entry passes `limit` (NO_CUR when none is left).
+SQ_HD void pop_next(const RingStack& st, uint32_t& sp, float limit, uint32_t& cur, float& cur_t) {
+  cur = NO_CUR;
+  while (sp > 0) {
+    uint32_t r; float tt;
+    st.pop(sp, r, tt);
+    if (tt <= limit) { cur = r; cur_t = tt; return; }
+  }
+}
+
+template <int D, int PS, bool FILT, bool QUAD, class Q, class P = NoPause, bool COUNT = false>
+SQ_HD bool walk_step_spec(const SceneView& s, const Q& q, const Filter& f, const RingStack& st, SpecState& w,
+                          const P& pause = P(), WorkCount* wc = nullptr) {
+  constexpr int NF = (D + 1) * 4;
+  if (w.pend == 0 && w.cur != NO_CUR && (w.cur & LEAF_BIT)) {
+    w.pend = w.cur;
+    pop_next(st, w.sp, w.limit, w.cur, w.cur_t);
+  }
+  // descend: to the next leaf, and past a pending one to the leaf after it
+  while (w.cur != NO_CUR && !(w.cur & LEAF_BIT)) {
+    if constexpr (QUAD) {
+      float nd[2 * NF];
+      fetch16<2 * (D + 1)>(s.quads + (size_t)w.cur * 2 * NF, nd);
+      bool h[4];
+      float t[4];
+      uint32_t r[4];
+      q.pair(nd, w.limit, h[0], h[1], t[0], t[1]);
+      q.pair(nd + NF, w.limit, h[2], h[3], t[2], t[3]);
+      r[0] = fbits(nd[4 * D + 0]); r[1] = fbits(nd[4 * D + 1]);
+      r[2] = fbits(nd[NF + 4 * D + 0]); r[3] = fbits(nd[NF + 4 * D + 1]);
+      h[1] = h[1] && fbits(nd[4 * D + 2]) == 0u;
+      h[3] = h[3] && fbits(nd[NF + 4 * D + 2]) == 0u;
+      // aa, ab, ba pushed as found, the last hit one walked next (bb first)
+      bool have = false;
+      uint32_t nr = 0;
+      float ntt = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (!h[k]) continue;
+        if (have) st.push(w.sp, nr, ntt);
+        have = true;
+        nr = r[k];
+        ntt = t[k];
+      }
+      if (have) { w.cur = nr; w.cur_t = ntt; }
+      else pop_next(st, w.sp, w.limit, w.cur, w.cur_t);
+    } else {
+      float nd[NF];
+      fetch16<D + 1>(s.nodes + (size_t)w.cur * NF, nd);
+      bool ha, hb, a_first; float ta, tb;
+      order_pair<D, true>(q, nd, w.limit, ha, hb, ta, tb, a_first);
+      const uint32_t ra = fbits(nd[4 * D + 0]), rb = fbits(nd[4 * D + 1]);
+      if (ha && hb) {
+        st.push(w.sp, a_first ? rb : ra, a_first ? tb : ta);
+        w.cur = a_first ? ra : rb;
+        w.cur_t = a_first ? ta : tb;
+      } else if (ha) { w.cur = ra; w.cur_t = ta; }
+      else if (hb) { w.cur = rb; w.cur_t = tb; }
+      else pop_next(st, w.sp, w.limit, w.cur, w.cur_t);
+    }
+    if (w.pend == 0 && w.cur != NO_CUR && (w.cur & LEAF_BIT)) {
+      w.pend = w.cur;
+      pop_next(st, w.sp, w.limit, w.cur, w.cur_t);
+    }
+    if (pause()) break;
+  }
+  if (w.pend == 0) return w.cur == NO_CUR;
+  const uint32_t first = w.pend & FIRST_MASK;
+  const uint32_t cnt = ((w.pend >> 27) & 15u) + 1u;
+  w.pend = 0;
+  if constexpr (COUNT) wc->prims += cnt;
+  if constexpr (PS == 12) {
+    // triangles: the next record is fetched while this one is tested; the
+    // id rides in the record (v0.xyz, id)
+    float pr[12], nx[12];
+    fetch16<3>(s.prims + (size_t)first * 12, pr);
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t pos = first + j;
+      if (j + 1 < cnt) fetch16<3>(s.prims + (size_t)(pos + 1) * 12, nx);
+      float t;
+      if (q.narrow(pr, w.limit, t)) {
+        const uint32_t leaf = fbits(pr[3]);
+        if (!FILT || accepts(f, leaf)) {
+          if (w.res.leaf == NO_LEAF || hit_before(t, leaf, w.res.t, w.res.leaf)) {
+            w.res.t = t; w.res.leaf = leaf; w.res.pos = pos; w.limit = t;
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 12; ++k) pr[k] = nx[k];
+    }
+  } else {
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t pos = first + j;
+      float t;
+      if (!q.narrow(s.prims + (size_t)pos * PS, w.limit, t)) continue;
+      const uint32_t leaf = s.ids[pos];
+      if (FILT && !accepts(f, leaf)) continue;
+      if (w.res.leaf == NO_LEAF || hit_before(t, leaf, w.res.t, w.res.leaf)) {
+        w.res.t = t; w.res.leaf = leaf; w.res.pos = pos; w.limit = t;
+      }
+    }
+  }
+  if (w.cur != NO_CUR && !(w.cur_t <= w.limit)) pop_next(st, w.sp, w.limit, w.cur, w.cur_t);
+  return w.cur == NO_CUR;
+}
+
 // ---------------------------------------------------------------------------
 // Two-level walk (crates/spatial-query/src/accel/tlas.rs) over built-in
 // instance kinds.  Instance records by id, XS floats each:

@@ -194,6 +194,54 @@ int emu_nearest(void* h, int dim, int kind, const float* q, uint64_t nq, float m
 
 // The 4-wide walk (tuning `wide`) with a ring stack of `ring` slots; -2 when
 // an unguarded walk is asked of a tree with synthetic pairs.
+// Pauses every `every`-th node (resumable descents, as a wave pauses them).
+struct EveryPause {
+  uint32_t every;
+  mutable uint32_t n = 0;
+  bool operator()() const { return every && ++n % every == 0; }
+};
+
+// The look-ahead walk in the reference's order (walk_step_spec) of every ray
+// over the 4-wide records (quad) or the pair nodes, unguarded, with a ring
+// stack and a descent paused every `pause_every` nodes (0: never); returns
+// the primitives each walk tested.  -2 for quad over a tree with synthetic
+// pairs.
+int emu_nearest_spec(void* h, int dim, int kind, const float* q, uint64_t nq, float max_toi, int quad, uint32_t ring,
+                     uint32_t pause_every, int32_t* leaf, float* toi, uint32_t* prims_tested) {
+  Built& b = *(Built*)h;
+  if (b.tree.prim_indices.empty() || !ring || kind == 0) return -1;
+  if (quad && b.packed.n_synth != 0) return -2;
+  return dk(dim, kind, [&]<int D, int K>() {
+    if (quad && b.quads.empty()) build_quads<D>(b);
+    constexpr int PS = Prim<D, K>::STRIDE;
+    const SceneView s = b.view();
+    const uint32_t bound = quad ? 3 * (b.packed.max_stack / 2 + 1) + 1 : b.packed.max_stack;
+    par_for(nq, 8, [&](uint64_t i) {
+      RayPolicy<D, K, false> pol; pol.setup(q + i * 2 * D);
+      Filter f{nullptr, ~0ull, -1};
+      std::vector<uint32_t> lref(ring);
+      std::vector<float> lt(ring);
+      const uint32_t rows = bound > ring ? bound - ring : 0;
+      std::vector<SpillEntry> col(rows + 1);
+      RingStack st{lref.data(), lt.data(), 1, ring, col.data(), 1};
+      SpecState w;
+      WorkCount wc;
+      if (spec_begin<D>(s, pol, max_toi, w)) {
+        EveryPause pz{pause_every};
+        for (;;) {
+          const bool done = quad ? walk_step_spec<D, PS, false, true, RayPolicy<D, K, false>, EveryPause, true>(s, pol, f, st, w, pz, &wc)
+                                 : walk_step_spec<D, PS, false, false, RayPolicy<D, K, false>, EveryPause, true>(s, pol, f, st, w, pz, &wc);
+          if (w.sp > bound) { fprintf(stderr, "stack bound exceeded\n"); abort(); }
+          if (done) break;
+        }
+      }
+      leaf[i] = w.res.leaf == NO_LEAF ? -1 : (int32_t)w.res.leaf;
+      toi[i] = w.res.t;
+      prims_tested[i] = wc.prims;
+    });
+  });
+}
+
 // prims_tested (optional, [nq]): primitives each walk tested.
 int emu_nearest_quad(void* h, int dim, int kind, const float* q, uint64_t nq, float max_toi, int any, int threads,
                      int32_t* leaf, float* toi, uint32_t ring, int guarded, uint32_t* prims_tested) {

@@ -32,6 +32,7 @@ def emu():
     L.emu_nearest_quad.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_int, P, P, C.c_uint32,
                                    C.c_int, P]
     L.emu_nearest_pair_counted.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_uint32, P, P, P]
+    L.emu_nearest_spec.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, C.c_uint32, C.c_uint32, P, P, P]
     L.emu_n_synth.restype = C.c_uint32
     L.emu_n_synth.argtypes = [P]
     L.emu_counted.argtypes = [P, C.c_int, C.c_int, P, C.c_uint64, C.c_float, C.c_int, P, P, P]
@@ -70,6 +71,15 @@ class Emu:
         tested = np.zeros(len(q), np.uint32)
         rc = self.L.emu_nearest_pair_counted(self.h, self.dim, self.kind, p(q), len(q), max_toi, int(guarded), ring,
                                              p(leaf), p(t), p(tested))
+        assert rc == 0, rc
+        return leaf.astype(np.int64), t, tested
+
+    def nearest_spec(self, q, max_toi, quad, ring=8, pause_every=0):
+        q = np.ascontiguousarray(q, np.float32)
+        leaf = np.empty(len(q), np.int32); t = np.empty(len(q), np.float32)
+        tested = np.zeros(len(q), np.uint32)
+        rc = self.L.emu_nearest_spec(self.h, self.dim, self.kind, p(q), len(q), max_toi, int(quad), ring, pause_every,
+                                     p(leaf), p(t), p(tested))
         assert rc == 0, rc
         return leaf.astype(np.int64), t, tested
 
@@ -266,6 +276,45 @@ def test_quad_walk_reference_order(emu, ring):
         assert np.array_equal(qn, eb.nearest_pair_counted(qq, 300.0, ring=ring)[2])
         assert np.array_equal(eb.nearest_quad(qq, 300.0, any_=1, ring=ring, guarded=False)[0] >= 0, rb["leaf"] >= 0)
     assert differ > 0, "the scenes should hold rays whose reference answer lies outside its box"
+
+
+@pytest.mark.parametrize("ring,pause_every", [(2, 0), (8, 0), (8, 3), (4, 7)])
+def test_lookahead_walk(emu, ring, pause_every):
+    """The look-ahead form of the reference-order walk (walk_step_spec: a
+    lane keeps a reached leaf pending and descends on to its next leaf with
+    the limit it has, then tests the pending leaf and checks its position
+    again) tests exactly the primitives the pair walk in the reference's order
+    tests, ray for ray, and returns the oracle's plain answers: over the 4-wide
+    records and over the pair nodes, on the heightfield (camera and incoherent
+    rays), balls in D = 2 and 5, and coincident balls (an over-full leaf walked
+    as synthetic pairs), with descents paused and resumed and rings that
+    spill."""
+    sc = W.heightfield_scene(grid=257, extent=128.0)
+    rays = np.concatenate([W.camera_rays(sc, res=128, seed=0x12), W.incoherent_rays(sc, 30000)])
+    cases = [(3, 2, oracle.TRI, sc.prims, rays, 2000.0)]
+    for dim, half in ((2, 40.0), (5, 10.0)):
+        s = W.Stream(0x16 if dim == 2 else 0x14 + dim)
+        b = np.concatenate([s.range(-half, half, 4000 * dim).reshape(-1, dim), s.range(0.3, 3.0, 4000).reshape(-1, 1)],
+                           1).astype(np.float32)
+        cases.append((dim, 1, oracle.SPHERE, b, W.random_rays(4000, 0x17 if dim == 2 else 0x15 + dim, -half - 5.0,
+                                                              half + 5.0, dim=dim), 300.0))
+    dup = np.concatenate([np.tile(np.array([[5, 0, 0, 1]], np.float32), (100, 1)),
+                          np.tile(np.array([[-5, 3, 1, 2]], np.float32), (60, 1))])
+    dup[::7, 3] = 1.5
+    cases.append((3, 1, oracle.SPHERE, dup, W.random_rays(3000, 0x19, -8.0, 8.0), 100.0))
+    synth_seen = False
+    for dim, kind, ok, prims, q, tmax in cases:
+        e = Emu(emu, dim, kind, prims)
+        plain = oracle.Scene(dim, ok, prims).raycast_nearest(q, tmax, threads=8)
+        pl, pt, pn = e.nearest_pair_counted(q, tmax, ring=ring)
+        assert eq(pl, pt, plain["leaf"], plain["t"])
+        quads = (True, False) if emu.emu_n_synth(e.h) == 0 else (False,)
+        synth_seen |= emu.emu_n_synth(e.h) > 0
+        for quad in quads:
+            sl, stt, sn = e.nearest_spec(q, tmax, quad, ring=ring, pause_every=pause_every)
+            assert eq(sl, stt, plain["leaf"], plain["t"]), (dim, kind, quad)
+            assert np.array_equal(sn, pn), (dim, kind, quad, int((sn != pn).sum()))
+    assert synth_seen
 
 
 @pytest.mark.parametrize("ring", [0, 2, 8])

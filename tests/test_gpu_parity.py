@@ -235,6 +235,64 @@ def test_wide_walk_reference_order(ctx):
         ctx.set_tuning(**saved)
 
 
+@pytest.mark.parametrize("wide", [0, 1])
+def test_lookahead_walk(ctx, wide):
+    """The look-ahead form of the reference-order walk (tuning `lookahead`)
+    over the pair nodes and the 4-wide records equals the oracle's plain
+    providers on scenes where hits leave their boxes (the 257^2 heightfield,
+    balls in D = 2 and 5, coincident balls walked as synthetic pairs), with a
+    filter and with rings that spill; at full C1 scale it equals the walk
+    without look-ahead on all 8,388,608 rays."""
+    print(f"case: look-ahead walk, wide {wide}", flush=True)
+    sc = W.heightfield_scene(grid=257, extent=128.0)
+    rays_h = np.concatenate([W.camera_rays(sc, res=256, seed=0x11), W.incoherent_rays(sc, 65536)])
+    s = W.Stream(0x16)
+    balls2 = np.concatenate([s.range(-40.0, 40.0, 8000 * 2).reshape(-1, 2), s.range(0.3, 3.0, 8000).reshape(-1, 1)],
+                            1).astype(np.float32)
+    rays_b = W.random_rays(60000, 0x17, -45.0, 45.0, dim=2)
+    balls5 = ball_scene(5, 4000, 0x55, half=10.0)
+    rays_5 = W.random_rays(20000, 0x56, -15.0, 15.0, dim=5)
+    dup = np.concatenate([np.tile(np.array([[5, 0, 0, 1]], np.float32), (100, 1)),
+                          np.tile(np.array([[-5, 3, 1, 2]], np.float32), (60, 1))])
+    dup[::7, 3] = 1.5
+    rays_d = W.random_rays(3000, 0x19, -8.0, 8.0)
+    saved = ctx.tuning()
+    cases = ((3, sq_amd.TRIANGLE, oracle.TRI, sc.prims, rays_h, 2000.0), (2, sq_amd.SPHERE, oracle.SPHERE, balls2, rays_b, 300.0),
+             (5, sq_amd.SPHERE, oracle.SPHERE, balls5, rays_5, 300.0), (3, sq_amd.SPHERE, oracle.SPHERE, dup, rays_d, 100.0))
+    try:
+        for tuning in (dict(block=128, rays_per_lane=16, lds_stack=8), dict(block=64, rays_per_lane=1, lds_stack=2)):
+            ctx.set_tuning(**dict(saved, wide=wide, lookahead=1, **tuning))
+            for dim, kind, okind, prims, rays, tmax in cases:
+                b = sq_amd.DeviceBvh.build(ctx, dim, kind, prims)
+                plain = oracle.Scene(dim, okind, prims).raycast_nearest(rays, tmax, threads=8)
+                got = b.raycast_nearest(dev(rays), tmax, normal=True)
+                assert_nearest_equal(host(got["leaf"]), host(got["toi"]), plain["leaf"], plain["t"], f"D{dim} {tuning}")
+                hit = plain["leaf"] >= 0
+                assert same_bits(host(got["normal"])[hit], plain["normal"][hit]).all()
+                b.close()
+        # a filter (the pair walk's look-ahead form; the 4-wide walk takes no filter)
+        layers = (np.uint64(1) << (np.arange(sc.prims.shape[0], dtype=np.uint64) % np.uint64(3)))
+        b = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.TRIANGLE, sc.prims)
+        b.set_layers(layers)
+        ref = oracle.Scene(3, oracle.TRI, sc.prims, layers=layers).raycast_nearest(rays_h, 2000.0, mask=0b110, exclude=7,
+                                                                                    threads=8)
+        got = b.raycast_nearest(dev(rays_h), 2000.0, filt=sq_amd.QueryFilter(0b110, 7))
+        assert_nearest_equal(host(got["leaf"]), host(got["toi"]), ref["leaf"], ref["t"], "filtered look-ahead")
+        b.close()
+        sc1, coh, inc = W.c1()
+        rays1 = dev(np.concatenate([coh, inc]))
+        b1 = sq_amd.DeviceBvh.build(ctx, 3, sq_amd.TRIANGLE, sc1.prims)
+        ctx.set_tuning(**dict(saved, wide=wide, lookahead=1))
+        look = b1.raycast_nearest(rays1, 2000.0)
+        ctx.set_tuning(**dict(saved, wide=wide, lookahead=0))
+        plain1 = b1.raycast_nearest(rays1, 2000.0)
+        assert np.array_equal(host(look["leaf"]), host(plain1["leaf"]))
+        assert np.array_equal(host(look["toi"]).view(np.uint32), host(plain1["toi"]).view(np.uint32))
+        b1.close()
+    finally:
+        ctx.set_tuning(**saved)
+
+
 def test_per_ray_max_toi_and_filter(ctx):
     print("case: per-ray max_toi, layer mask, exclude", flush=True)
     sc, rays = W.c0(n_rays=20000)
@@ -428,7 +486,9 @@ def test_edge_cases(ctx):
                                     dict(block=128, rays_per_lane=16, lds_stack=8, wide=1),
                                     dict(block=64, rays_per_lane=1, lds_stack=2, wide=1),
                                     dict(block=128, rays_per_lane=16, lds_stack=8, wide=0),
-                                    dict(block=64, rays_per_lane=2, lds_stack=2, wide=0)])
+                                    dict(block=64, rays_per_lane=2, lds_stack=2, wide=0),
+                                    dict(block=128, rays_per_lane=16, lds_stack=8, wide=2, lookahead=1),
+                                    dict(block=64, rays_per_lane=2, lds_stack=2, wide=0, lookahead=1)])
 def test_tuning_does_not_change_results(ctx, tuning):
     """Every kernel variant (one query per thread; pooled with LDS rings that
     spill constantly or never; rays walked in Morton order) gives the same bits."""
