@@ -631,6 +631,12 @@ int sq_bvh64_overlap(sq_bvh64* bvh, const double* regions, uint64_t n, const sq_
 typedef struct sq_tlas64 sq_tlas64;
 int sq_tlas64_create(sq_ctx* ctx, int dim, uint64_t n, const double* rot, const double* trans, const double* halves,
                      sq_tlas64** out);
+/* Tlas::build over f64 instances of a built-in kind: SQ_INST_BOX (halves
+ * double[n][D], as sq_tlas64_create) or SQ_INST_BALL (D = 3, radii double[n]:
+ * the reference's BallInstances, tests/shape_overlap.rs:288-345).  Host
+ * pointers. */
+int sq_tlas64_create_kind(sq_ctx* ctx, int dim, int inst_kind, uint64_t n, const double* rot, const double* trans,
+                          const double* halves, sq_tlas64** out);
 int sq_tlas64_destroy(sq_tlas64* tlas);
 int sq_tlas64_stats(const sq_tlas64* tlas, sq_tree_stats* out);
 /* Tlas::raycast_nearest (tlas.rs:104-154) as sq_tlas_raycast_nearest: the
@@ -639,6 +645,24 @@ int sq_tlas64_raycast_nearest(sq_tlas64* tlas, const double* rays, uint64_t n, d
                               const double* max_toi_per_ray, int32_t* out_inst, double* out_toi, double* out_point);
 int sq_tlas64_raycast_any(sq_tlas64* tlas, const double* rays, uint64_t n, double max_toi,
                           const double* max_toi_per_ray, uint8_t* out_hit);
+/* Tlas::raycast_all (tlas.rs:158-215) in double, CSR as sq_bvh64_raycast_all
+ * (instances by (toi, instance)); also Tlas::raycast_crossings (tlas.rs:221-304)
+ * for these kinds, which report one crossing per instance. */
+int sq_tlas64_raycast_all(sq_tlas64* tlas, const double* rays, uint64_t n, double max_toi, uint64_t* offsets,
+                          int32_t* out_inst, double* out_toi, uint64_t capacity, uint64_t* total);
+/* Tlas::overlap (tlas.rs:393-432) in double: every instance of a leaf node
+ * touching the region that passes test_overlap (exact for balls, true for
+ * boxes), instances ascending. */
+int sq_tlas64_overlap(sq_tlas64* tlas, const double* regions, uint64_t n, uint64_t* offsets, int32_t* out_inst,
+                      uint64_t capacity, uint64_t* total);
+/* Tlas::shapecast_nearest / shapecast_all (tlas.rs:306-391) in double: casts
+ * double[n][4*D] (origin, dir, probe box min, max).  Ball instances answer the
+ * cast ray against the ball of radius r + the probe's max x about the centre;
+ * box instances keep the trait default (no contact: every cast misses). */
+int sq_tlas64_shapecast_nearest(sq_tlas64* tlas, const double* casts, uint64_t n, double max_toi, int32_t* out_inst,
+                                double* out_toi, double* out_point);
+int sq_tlas64_shapecast_all(sq_tlas64* tlas, const double* casts, uint64_t n, double max_toi, uint64_t* offsets,
+                            int32_t* out_inst, double* out_toi, uint64_t capacity, uint64_t* total);
 
 #ifdef __cplusplus
 }

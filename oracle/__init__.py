@@ -84,6 +84,7 @@ def _declare(L):
         "orc64_tree_stats": (None, [I32, I32, P, U32, P, P]),
         "orc64_bvh_raycast_nearest": (None, [I32, I32, P, U32, P, U64, C.c_double, I32, P, P, P]),
         "orc64_tlas_box_nearest": (None, [I32, U64, P, P, P, P, U64, C.c_double, I32, I32, P, P, P]),
+        "orc64_tlas_query": (U64, [I32, I32, U64, P, P, P, P, U64, C.c_double, I32, I32, I32, P, P, P, U64]),
         "orc64_shapecast_nearest": (None, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, P, I32, P, P, P]),
         "orc64_shapecast_all": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, P, I32, P, P, P, U64]),
         "orc_inst_new": (P, [I32, I32, U64, P, P, P, U32, P, P, P, P]),
@@ -439,6 +440,60 @@ class BoxInstances64:
         out = dict(leaf=leaf, t=t)
         if world: out["world"] = w
         return out
+
+
+class Instances64:
+    """The Tlas queries under the `f64` feature over box instances (kind 0,
+    two_level.rs:70-113: halves [n][D]) or ball instances (kind 1,
+    shape_overlap.rs:288-345: radii [n]): the Tlas walks of tlas.rs in the
+    reference's order over the median-split tree of the world boxes
+    (naive=True: the instance scans of two_level.rs:134-174 and naive.rs's
+    rules), guarded=True the engine's world-box guard.  raycast_all is also
+    Tlas::raycast_crossings for these kinds (one crossing per instance)."""
+
+    def __init__(self, dim, kind, rot, trans, half, guarded: bool = False, naive: bool = False):
+        self.dim, self.kind = dim, kind
+        self._flags = (F64_GUARDED if guarded else 0) | (0 if naive else F64_BVH)
+        self.rot = np.ascontiguousarray(rot, np.float64).reshape(-1, dim, dim)
+        self.trans = np.ascontiguousarray(trans, np.float64).reshape(-1, dim)
+        self.half = np.ascontiguousarray(half, np.float64).reshape(-1, dim) if kind == 0 else \
+            np.ascontiguousarray(half, np.float64).reshape(-1)
+        self.n = self.rot.shape[0]
+
+    def _run(self, q, qs, max_toi, mode, off, leaf, t, cap, threads):
+        return lib().orc64_tlas_query(self.dim, self.kind, self.n, _p(self.rot), _p(self.trans), _p(self.half), _p(q),
+                                      q.shape[0], float(max_toi), mode, threads, self._flags, _p(off), _p(leaf), _p(t),
+                                      cap)
+
+    def _nearest(self, q, qs, max_toi, mode, threads):
+        q = np.ascontiguousarray(q, np.float64).reshape(-1, qs)
+        leaf = np.empty(q.shape[0], np.int64); t = np.empty(q.shape[0])
+        self._run(q, qs, max_toi, mode, None, leaf, t, 0, threads)
+        return dict(leaf=leaf, t=t)
+
+    def _list(self, q, qs, max_toi, mode, threads):
+        q = np.ascontiguousarray(q, np.float64).reshape(-1, qs)
+        off = np.empty(q.shape[0] + 1, np.uint64)
+        tot = self._run(q, qs, max_toi, mode, off, None, None, 0, threads)
+        leaf = np.empty(max(tot, 1), np.int64); t = np.empty(max(tot, 1))
+        self._run(q, qs, max_toi, mode, off, leaf, t, tot, threads)
+        return off, leaf[:tot], t[:tot]
+
+    def raycast_nearest(self, rays, max_toi, threads=8):
+        return self._nearest(rays, 2 * self.dim, max_toi, 0, threads)
+
+    def raycast_all(self, rays, max_toi, threads=8):
+        return self._list(rays, 2 * self.dim, max_toi, 1, threads)
+
+    def overlap(self, regions, threads=8):
+        off, leaf, _ = self._list(regions, 2 * self.dim, 0.0, 2, threads)
+        return off, leaf
+
+    def shapecast_nearest(self, casts, max_toi, threads=8):
+        return self._nearest(casts, 4 * self.dim, max_toi, 3, threads)
+
+    def shapecast_all(self, casts, max_toi, threads=8):
+        return self._list(casts, 4 * self.dim, max_toi, 4, threads)
 
 
 class InstanceScene:

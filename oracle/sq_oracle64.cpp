@@ -15,6 +15,9 @@
 //   * box instances                two_level.rs:70-113 (ray_local_box), the naive instance scan
 //                                  of two_level.rs:126-167, Isometry inverse and transform_aabb
 //                                  (isometry.rs:20-124), kept inside the world box's slab interval
+//   * ball instances               shape_overlap.rs:288-345 (BallInstances: ray_ball on the local
+//                                  ray :120-138, the cast against the radius sum :326-337,
+//                                  point_to_aabb overlap :59-72,339-341)
 //   * hit_ordering                 sq/src/query/hit.rs:77-84 (f64::total_cmp, then leaf)
 //   * tree::build and TreeStats    sq/src/accel/tree.rs:39-49,81-163
 //   * Bvh::raycast_nearest         sq/src/accel/bvh.rs:51-124 (push a, push b, LIFO, shrinking
@@ -22,8 +25,11 @@
 //                                  over raycast_nearest_batch_parallel's contiguous chunks
 //                                  (bvh.rs:160-193): the timed CPU baseline
 //   * the Bvh walks of every query  bvh.rs:196-259 (all hits), :331-414 (shape casts),
-//                                  :418-456 (overlap), and the Tlas walk tlas.rs:104-154
-//                                  over box instances: the expected query results
+//                                  :418-456 (overlap), and the Tlas walks tlas.rs:104-154
+//                                  (nearest), :158-215 (all hits), :221-304 (crossings: one per
+//                                  instance by test_ray_crossings_local's default), :306-391
+//                                  (shape casts), :393-432 (overlap) over box and ball
+//                                  instances: the expected query results
 // over the built-in BOX and SPHERE providers.  A `kind` with FLAG_GUARDED set
 // restates the engine's SQ_HITS_GUARDED mode instead (a sphere or instance hit
 // kept only inside the slab interval of its own box, so that BVH == naive scan,
@@ -643,16 +649,20 @@ static void bvh_nearest_batch(int kindf, const double* prims, uint32_t n, const 
   for (int w = 0; w < T; ++w) { counts[0] += nodes[w]; counts[1] += prs[w]; }
 }
 
-// Box instances: R [D][D] row-major, t [D], half [D] each.
+// Instances: R [D][D] row-major, t [D] each, and half [D] (boxes,
+// two_level.rs:70-113: local box [-half, half]) or a radius (balls,
+// shape_overlap.rs:288-345: local box [-r, r]^D, a ball at the local origin).
 template <int D> struct Inst64 {
   uint64_t n;
-  const double *rot, *trans, *half;
+  const double *rot, *trans, *half;   // half: [n][D] (boxes) or radii [n] (balls)
   bool guarded = false;   // the engine's SQ_HITS_GUARDED: a hit kept inside its world box's slab interval
-  // Isometry::transform_aabb of [-half, half]: centre R c + t, half extent |R| h
+  int kind = 0;           // 0 boxes, 1 balls
+  double h(uint64_t i, int k) const { return kind ? half[i] : half[i * D + k]; }
+  // Isometry::transform_aabb of the local box [-h, h]: centre R c + t, half extent |R| h
   void world_box(uint64_t i, double* mn, double* mx) const {
-    const double *R = rot + i * D * D, *t = trans + i * D, *h = half + i * D;
+    const double *R = rot + i * D * D, *t = trans + i * D;
     double c[D], e[D];
-    for (int k = 0; k < D; ++k) { const double lo = h[k] * -1.0; c[k] = (lo + h[k]) * 0.5; e[k] = (h[k] - lo) * 0.5; }
+    for (int k = 0; k < D; ++k) { const double hk = h(i, k), lo = hk * -1.0; c[k] = (lo + hk) * 0.5; e[k] = (hk - lo) * 0.5; }
     for (int a = 0; a < D; ++a) {
       double acc = 0.0;
       for (int b = 0; b < D; ++b) acc = acc + R[a * D + b] * c[b];
@@ -662,11 +672,27 @@ template <int D> struct Inst64 {
       mn[a] = rc - w; mx[a] = rc + w;
     }
   }
-  // test_instance: inverse transform (rotate_transpose of o - t, and of d), the
-  // local box slab with its 1e-9 parallel threshold and t > 1e-6 entry rule
+  // ray_ball (shape_overlap.rs:120-138) about centre c (NULL: the origin)
+  static bool ray_ball(const double* o, const double* d, const double* c, double rad, double max_toi, double* t_out) {
+    double oc[D];
+    for (int k = 0; k < D; ++k) oc[k] = c ? o[k] - c[k] : o[k];
+    const double b = dot<D>(oc, d);
+    const double cc = dot<D>(oc, oc) - rad * rad;
+    const double disc = b * b - cc;
+    if (disc < 0.0) return false;
+    const double sq = std::sqrt(disc);
+    double t = -b - sq;
+    if (t < 0.0) t = -b + sq;
+    if (t < 0.0 || t > max_toi) return false;
+    *t_out = t;
+    return true;
+  }
+  // test_instance (tlas.rs:459-481): inverse transform (rotate_transpose of
+  // o - t, and of d), then boxes: the local box slab with its 1e-9 parallel
+  // threshold and t > 1e-6 entry rule; balls: ray_ball about the origin
   // (guarded: then the guard against the world box)
   bool test(uint64_t i, const Ray& r, double max_toi, double* t_out) const {
-    const double *R = rot + i * D * D, *t = trans + i * D, *h = half + i * D;
+    const double *R = rot + i * D * D, *t = trans + i * D;
     double v[D], o[D], d[D];
     for (int j = 0; j < D; ++j) v[j] = r.o[j] - t[j];
     for (int a = 0; a < D; ++a) {
@@ -674,28 +700,70 @@ template <int D> struct Inst64 {
       for (int j = 0; j < D; ++j) { x = x + R[j * D + a] * v[j]; y = y + R[j * D + a] * r.d[j]; }
       o[a] = x; d[a] = y;
     }
-    double t_min = 0.0, t_max = max_toi;
-    int axis = -1;
-    for (int k = 0; k < D; ++k) {
-      if (std::fabs(d[k]) < 1.0e-9) {
-        if (o[k] < -h[k] || o[k] > h[k]) return false;
-      } else {
-        const double inv = 1.0 / d[k];
-        double t1 = (-h[k] - o[k]) * inv, t2 = (h[k] - o[k]) * inv;
-        if (t1 > t2) std::swap(t1, t2);
-        if (t1 > t_min) { t_min = t1; axis = k; }
-        if (t2 < t_max) t_max = t2;
-        if (t_min > t_max) return false;
+    double t_hit;
+    if (kind) {
+      if (!ray_ball(o, d, nullptr, half[i], max_toi, &t_hit)) return false;
+    } else {
+      double t_min = 0.0, t_max = max_toi;
+      int axis = -1;
+      for (int k = 0; k < D; ++k) {
+        const double hk = half[i * D + k];
+        if (std::fabs(d[k]) < 1.0e-9) {
+          if (o[k] < -hk || o[k] > hk) return false;
+        } else {
+          const double inv = 1.0 / d[k];
+          double t1 = (-hk - o[k]) * inv, t2 = (hk - o[k]) * inv;
+          if (t1 > t2) std::swap(t1, t2);
+          if (t1 > t_min) { t_min = t1; axis = k; }
+          if (t2 < t_max) t_max = t2;
+          if (t_min > t_max) return false;
+        }
       }
+      if (!(axis >= 0 && t_min > 1.0e-6 && t_min <= max_toi)) return false;
+      t_hit = t_min;
     }
-    if (!(axis >= 0 && t_min > 1.0e-6 && t_min <= max_toi)) return false;
-    if (!guarded) { *t_out = t_min; return true; }
-    double mn[D], mx[D], te, tx; int ax;
-    world_box(i, mn, mx);
-    if (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &ax)) return false;
-    if (!(te <= t_min && t_min <= tx)) return false;
-    *t_out = t_min;
+    if (guarded) {
+      double mn[D], mx[D], te, tx; int ax;
+      world_box(i, mn, mx);
+      if (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &ax)) return false;
+      if (!(te <= t_hit && t_hit <= tx)) return false;
+    }
+    *t_out = t_hit;
     return true;
+  }
+  // test_shape_cast: boxes keep the trait default (no contact, instance.rs:
+  // 72-87); balls the cast ray against the ball of radius r + the probe's
+  // max x about the centre (shape_overlap.rs:326-337, the built-in SPHERE
+  // cast's probe radius), guarded: inside the probe-grown world box's slab
+  // interval
+  bool cast(uint64_t i, const Ray& r, const double* pmin, const double* pmax, double max_toi, double* t_out) const {
+    if (!kind) return false;
+    double t_hit;
+    if (!ray_ball(r.o, r.d, trans + i * D, half[i] + pmax[0], max_toi, &t_hit)) return false;
+    if (guarded) {
+      double mn[D], mx[D], te, tx; int ax;
+      world_box(i, mn, mx);
+      for (int k = 0; k < D; ++k) { mn[k] = mn[k] - pmax[k]; mx[k] = mx[k] - pmin[k]; }
+      if (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &ax)) return false;
+      if (!(te <= t_hit && t_hit <= tx)) return false;
+    }
+    *t_out = t_hit;
+    return true;
+  }
+  // test_overlap: boxes the trait default (true, instance.rs:89-98); balls
+  // point_to_aabb(centre, region) <= r (shape_overlap.rs:59-72,339-341)
+  bool overlap(uint64_t i, const double* rmn, const double* rmx) const {
+    if (!kind) return true;
+    const double* c = trans + i * D;
+    double d2 = 0.0;
+    for (int k = 0; k < D; ++k) {
+      double v;
+      if (c[k] < rmn[k]) v = rmn[k] - c[k];
+      else if (c[k] > rmx[k]) v = c[k] - rmx[k];
+      else v = 0.0;
+      d2 = d2 + v * v;
+    }
+    return std::sqrt(d2) <= half[i];
   }
 };
 
@@ -734,6 +802,112 @@ static void tlas_box_nearest(uint64_t n, const double* rot, const double* trans,
     leaf[q] = bl;
     t[q] = bl >= 0 ? bt : INFINITY;
   });
+}
+
+// Every Tlas query over box (kind 0) or ball (kind 1) instances, flags as
+// above.  mode 0: raycast_nearest (tlas.rs:104-154; naive: two_level.rs:
+// 134-154); 1: raycast_all (tlas.rs:158-215; naive_all, two_level.rs:156-174)
+// = raycast_crossings for these kinds (one crossing per instance, the
+// test_ray_crossings_local default); 2: overlap (tlas.rs:393-432: every
+// instance of a leaf node touching the region that passes test_overlap;
+// naive: every instance whose world box touches the region and passes it,
+// naive.rs:184-205's rule); 3: shapecast_nearest (tlas.rs:306-351); 4:
+// shapecast_all (tlas.rs:353-391).  Rays [nq][2D], regions [nq][2D], casts
+// [nq][4D] (origin, dir, probe min, max).  Modes 0 and 3 write leaf[nq],
+// t[nq]; the others CSR (offsets[nq + 1]; leaf, t when the total fits cap)
+// and return the total.
+template <int D>
+static uint64_t tlas_query(int kind, uint64_t n, const double* rot, const double* trans, const double* half, const double* q,
+                           uint64_t nq, double max_toi, int mode, int threads, int flags, uint64_t* offsets, int64_t* leaf,
+                           double* t, uint64_t cap) {
+  Inst64<D> s{n, rot, trans, half};
+  s.guarded = (flags & FLAG_GUARDED) != 0;
+  s.kind = kind;
+  std::vector<double> wb(n * 2 * D);
+  for (uint64_t i = 0; i < n; ++i) s.world_box(i, wb.data() + i * 2 * D, wb.data() + i * 2 * D + D);
+  const bool bvh = (flags & FLAG_BVH) != 0;
+  const Tree64<D> tree(wb, bvh ? (uint32_t)n : 0u);
+  std::vector<std::vector<std::pair<double, int64_t>>> res((mode == 0 || mode == 3) ? 0 : nq);
+  auto by_hit = [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
+    return hit_less(a.first, a.second, b.first, b.second);
+  };
+  auto grown = [&](const double* mn, const double* mx, const Ray& r, const double* pmin, const double* pmax, double lim) {
+    double gmn[D], gmx[D], te, tx; int ax;
+    for (int k = 0; k < D; ++k) { gmn[k] = mn[k] - pmax[k]; gmx[k] = mx[k] - pmin[k]; }
+    return ray_intersect<D>(gmn, gmx, r, lim, &te, &tx, &ax) && te <= lim;
+  };
+  parallel_for(nq, threads, [&](uint64_t i) {
+    if (mode == 0 || mode == 3) {
+      const int qs = mode == 0 ? 2 * D : 4 * D;
+      const double* c = q + i * qs;
+      Ray r{c, c + D};
+      int64_t bl = -1; double bt = 0.0, limit = max_toi;
+      auto take = [&](size_t k, double lim) {
+        double tt;
+        const bool h = mode == 0 ? s.test(k, r, lim, &tt) : s.cast(k, r, c + 2 * D, c + 3 * D, lim, &tt);
+        if (!h) return;
+        if (bl >= 0 && !hit_less(tt, (int64_t)k, bt, bl)) return;
+        bl = (int64_t)k; bt = tt; limit = tt;
+      };
+      if (bvh) {
+        tree.walk([&](const double* mn, const double* mx) {
+                    if (mode == 3) return grown(mn, mx, r, c + 2 * D, c + 3 * D, limit);
+                    double te, tx; int ax;
+                    return ray_intersect<D>(mn, mx, r, limit, &te, &tx, &ax) && te <= limit;
+                  },
+                  [&](size_t k) { take(k, limit); });
+      } else {
+        for (uint64_t k = 0; k < n; ++k) take(k, max_toi);
+      }
+      leaf[i] = bl;
+      t[i] = bl >= 0 ? bt : INFINITY;
+      return;
+    }
+    auto& out = res[i];
+    if (mode == 2) {
+      const double *rmn = q + i * 2 * D, *rmx = rmn + D;
+      auto touch = [&](const double* mn, const double* mx) {
+        for (int k = 0; k < D; ++k) if (mn[k] > rmx[k] || mx[k] < rmn[k]) return false;
+        return true;
+      };
+      if (bvh) tree.walk(touch, [&](size_t k) { if (s.overlap(k, rmn, rmx)) out.push_back({0.0, (int64_t)k}); });
+      else
+        for (uint64_t k = 0; k < n; ++k)
+          if (touch(wb.data() + k * 2 * D, wb.data() + k * 2 * D + D) && s.overlap(k, rmn, rmx)) out.push_back({0.0, (int64_t)k});
+      std::stable_sort(out.begin(), out.end(), [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
+        return a.second < b.second;
+      });
+    } else {
+      const int qs = mode == 1 ? 2 * D : 4 * D;
+      const double* c = q + i * qs;
+      Ray r{c, c + D};
+      auto take = [&](size_t k) {
+        double tt;
+        if (mode == 1 ? s.test(k, r, max_toi, &tt) : s.cast(k, r, c + 2 * D, c + 3 * D, max_toi, &tt))
+          out.push_back({tt, (int64_t)k});
+      };
+      if (bvh)
+        tree.walk([&](const double* mn, const double* mx) {
+                    if (mode == 4) return grown(mn, mx, r, c + 2 * D, c + 3 * D, max_toi);
+                    double te, tx; int ax;
+                    return ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &ax);
+                  },
+                  take);
+      else
+        for (uint64_t k = 0; k < n; ++k) take(k);
+      std::stable_sort(out.begin(), out.end(), by_hit);
+    }
+  });
+  if (mode == 0 || mode == 3) return 0;
+  uint64_t tot = 0;
+  for (uint64_t i = 0; i < nq; ++i) { offsets[i] = tot; tot += res[i].size(); }
+  offsets[nq] = tot;
+  if (tot <= cap) {
+    uint64_t k = 0;
+    for (uint64_t i = 0; i < nq; ++i)
+      for (auto& hh : res[i]) { leaf[k] = hh.second; if (t) t[k] = hh.first; ++k; }
+  }
+  return tot;
 }
 
 }  // namespace orc64
@@ -798,6 +972,15 @@ void orc64_tlas_box_nearest(int D, uint64_t n, const double* rot, const double* 
                             const double* rays, uint64_t nr, double max_toi, int threads, int flags, int64_t* leaf, double* t,
                             double* world) {
   ORC64_DISPATCH(D, tlas_box_nearest<DD>(n, rot, trans, half, rays, nr, max_toi, threads, flags, leaf, t, world));
+}
+
+// Tlas queries over box (kind 0) or ball (kind 1) instances: tlas_query.
+uint64_t orc64_tlas_query(int D, int kind, uint64_t n, const double* rot, const double* trans, const double* half,
+                          const double* q, uint64_t nq, double max_toi, int mode, int threads, int flags, uint64_t* offsets,
+                          int64_t* leaf, double* t, uint64_t cap) {
+  uint64_t tot = 0;
+  ORC64_DISPATCH(D, tot = tlas_query<DD>(kind, n, rot, trans, half, q, nq, max_toi, mode, threads, flags, offsets, leaf, t, cap));
+  return tot;
 }
 
 void orc64_tree_stats(int D, int kind, const double* prims, uint32_t n, uint64_t* out5, double* quality) {

@@ -33,9 +33,13 @@ namespace {
 constexpr int F64_MAX_STACK = 64;   // tree levels the per-lane stack holds (median split: about log2(n) + 1)
 constexpr int F64_BLOCK = 128;      // threads per block of the pooled kernels (two waves)
 constexpr uint32_t F64_RING = 8;    // stack entries per lane kept in LDS by the pooled kernels (RingStack)
-// Leaves of an f64 two-level tree: box instances (two_level.rs:70-113), one
-// record each of R[D][D] (row-major), t[D], half[D], world box min[D], max[D].
+// Leaves of an f64 two-level tree, one record each of R[D][D] (row-major),
+// t[D], half[D], world box min[D], max[D]: box instances (two_level.rs:70-113,
+// local box [-half, half]) or ball instances (shape_overlap.rs:288-345, a ball
+// of radius half[0] at the local origin, local box [-r, r]^D; D = 3).
 constexpr int KIND_INST64 = 3;
+constexpr int KIND_BALL64 = 4;
+template <int KIND> constexpr bool is_inst64() { return KIND == KIND_INST64 || KIND == KIND_BALL64; }
 
 struct View64 {
   const double* nodes;      // [np][4D + 2]: boxes, then the refs
@@ -60,6 +64,22 @@ __device__ __forceinline__ bool hit_before64(double t, uint32_t leaf, double bt,
 
 template <int D, int KIND> constexpr int stride64() {
   return KIND == KIND_AABB ? 2 * D : KIND == KIND_SPHERE ? D + 1 : D * D + 4 * D;
+}
+
+// InstancedGeometry::test_overlap of an instance record: exact for balls (the
+// distance from the centre to the region, shape_overlap.rs:59-72,339-341),
+// the trait default (true) for boxes (instance.rs:89-98).
+template <int D, int KIND>
+__device__ __forceinline__ bool inst_overlap64(const double* p, const double* rmn, const double* rmx) {
+  if constexpr (KIND == KIND_BALL64) {
+    double c[D + 1];
+#pragma unroll
+    for (int k = 0; k < D; ++k) c[k] = p[D * D + k];
+    c[D] = p[D * D + D];
+    return overlap_narrow<D, KIND_SPHERE>(c, rmn, rmx);
+  } else {
+    return true;
+  }
 }
 
 // Visits every leaf range whose boxes (and every ancestor's) pass node_pass,
@@ -147,13 +167,18 @@ template <int D, int KIND, bool GUARD> struct Pol64<D, KIND, false, GUARD> {
     slab_pair64<D>(r, b, limit, ha, hb, ta, tb);
   }
   __device__ bool narrow(const double* p, double limit, double& t) const {
-    if constexpr (KIND == KIND_INST64) {
+    if constexpr (is_inst64<KIND>()) {
       // test_instance (tlas.rs:459-481): the ray in the instance's frame against
-      // its local box (with GUARD kept inside the slab interval of its world box)
+      // its local box, or ray_ball about the local origin (with GUARD kept
+      // inside the slab interval of its world box)
       double ol[D], dl[D];
       iso_inverse<D>(p, p + D * D, r, ol, dl);
       int ax;
-      if (!local_box<D>(ol, dl, p + D * D + D, limit, t, ax)) return false;
+      if constexpr (KIND == KIND_INST64) {
+        if (!local_box<D>(ol, dl, p + D * D + D, limit, t, ax)) return false;
+      } else {
+        if (!ray_ball<D, double>(ol, dl, (const double*)nullptr, p[D * D + D], limit, t)) return false;
+      }
       if constexpr (!GUARD) return true;
       double te, tx;
       if (!slab_exact<D>(r, p + D * D + 2 * D, p + D * D + 3 * D, limit, te, tx, ax)) return false;
@@ -165,7 +190,8 @@ template <int D, int KIND, bool GUARD> struct Pol64<D, KIND, false, GUARD> {
 };
 template <int D, int KIND, bool GUARD> struct Pol64<D, KIND, true, GUARD> {
   static constexpr int QS = 4 * D;   // origin, direction, probe box min, max
-  CastPolicy<D, KIND, double, GUARD> c;
+  // (instance kinds: the cast's slab and probe; the instance test below)
+  CastPolicy<D, is_inst64<KIND>() ? KIND_AABB : KIND, double, GUARD> c;
   __device__ void setup(const double* src) { c.setup(src); }
   __device__ const RayD<D, double>& ray() const { return c.r; }
   __device__ bool root(const double* rb, double limit) const { return c.root(rb, limit); }
@@ -178,7 +204,22 @@ template <int D, int KIND, bool GUARD> struct Pol64<D, KIND, true, GUARD> {
     }
     slab_pair64<D>(c.r, g, limit, ha, hb, ta, tb);
   }
-  __device__ bool narrow(const double* p, double limit, double& t) const { return c.narrow(p, limit, t); }
+  __device__ bool narrow(const double* p, double limit, double& t) const {
+    if constexpr (KIND == KIND_INST64) {
+      return false;   // InstancedGeometry::test_shape_cast's default: no contact (instance.rs:72-87)
+    } else if constexpr (KIND == KIND_BALL64) {
+      // shape_overlap.rs:326-337: the cast ray against the ball of radius
+      // r + the probe's max x (the built-in SPHERE cast's probe radius) about
+      // the centre; GUARD: inside the probe-grown world box's slab interval
+      if (!ray_ball<D, double>(c.r.o, c.r.d, p + D * D, p[D * D + D] + c.pmax[0], limit, t)) return false;
+      if constexpr (!GUARD) return true;
+      double te, tx;
+      if (!c.grown_slab(p + D * D + 2 * D, p + D * D + 3 * D, limit, te, tx)) return false;
+      return te <= t && t <= tx;
+    } else {
+      return c.narrow(p, limit, t);
+    }
+  }
 };
 
 // Insertion sort of one query's CSR range [b0, b1): ray and cast hits by
@@ -270,14 +311,14 @@ __global__ __launch_bounds__(F64_BLOCK) void k64_nearest_pool(View64 v, const do
     out_leaf[ray] = hit ? (int32_t)bl : -1;
     out_toi[ray] = hit ? best : __builtin_huge_val();
     double nrm[D];
-    if constexpr (!CAST && KIND != KIND_INST64) {
+    if constexpr (!CAST && !is_inst64<KIND>()) {
       if (hit && out_normal) hit_normal<D, KIND>(r, v.prims + (size_t)bk * PS, best, lim, nrm);
     }
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       // Ray::at (maths/ray.rs), ShapeCast::at for casts (shapecast.rs)
       if (out_point) out_point[(size_t)ray * D + k] = hit ? r.o[k] + r.d[k] * best : 0.0;
-      if constexpr (!CAST && KIND != KIND_INST64) {
+      if constexpr (!CAST && !is_inst64<KIND>()) {
         if (out_normal) out_normal[(size_t)ray * D + k] = hit ? nrm[k] : 0.0;
       }
     }
@@ -400,7 +441,11 @@ __global__ __launch_bounds__(256) void k64_overlap(View64 v, const double* __res
         for (uint32_t k = first; k < first + count; ++k) {
           const uint32_t id = v.ids[k];
           if (!accepts64(v, f, id)) continue;
-          if (!overlap_narrow<D, KIND>(v.prims + (size_t)k * PS, rmn, rmx)) continue;
+          if constexpr (is_inst64<KIND>()) {
+            if (!inst_overlap64<D, KIND>(v.prims + (size_t)k * PS, rmn, rmx)) continue;
+          } else {
+            if (!overlap_narrow<D, KIND>(v.prims + (size_t)k * PS, rmn, rmx)) continue;
+          }
           if (fill) out_leaf[at++] = (int32_t)id;
           ++cnt;
         }
@@ -421,6 +466,22 @@ template <class F> int dispatch64(int dim, int kind, F&& f) {
     default: return fail(SQ_ERR_UNSUPPORTED, "dimension " + std::to_string(dim) + " not instantiated (2..6)");
   }
 #undef SQ_CASE
+}
+
+// The instance kinds of an f64 Tlas: boxes in D = 2..6, balls in D = 3.
+template <class F> int dispatch_inst64(int dim, int kind, F&& f) {
+  if (kind == KIND_BALL64) {
+    if (dim != 3) return fail(SQ_ERR_UNSUPPORTED, "ball instances need dim 3");
+    return f.template operator()<3, KIND_BALL64>();
+  }
+  switch (dim) {
+    case 2: return f.template operator()<2, KIND_INST64>();
+    case 3: return f.template operator()<3, KIND_INST64>();
+    case 4: return f.template operator()<4, KIND_INST64>();
+    case 5: return f.template operator()<5, KIND_INST64>();
+    case 6: return f.template operator()<6, KIND_INST64>();
+    default: return fail(SQ_ERR_UNSUPPORTED, "dimension " + std::to_string(dim) + " not instantiated (2..6)");
+  }
 }
 
 }  // namespace
@@ -580,12 +641,17 @@ int sq_bvh64_create(sq_ctx* c, int dim, int kind, const double* prims, uint64_t 
   return create64(c, dim, kind, prims, ps, boxes, n, out);
 }
 
-// Tlas::build (tlas.rs:35-56) over box instances in double: each world box is
-// Isometry::transform_aabb of the local box [-half, half] (isometry.rs:84-97).
-int sq_tlas64_create(sq_ctx* c, int dim, uint64_t n, const double* rot, const double* trans, const double* halves,
-                     sq_tlas64** out) {
+// Tlas::build (tlas.rs:35-56) over box or ball instances in double: each world
+// box is Isometry::transform_aabb of the local box, [-half, half] or
+// [-r, r]^D (isometry.rs:84-97).  Balls store splat(r) as their half extents.
+int sq_tlas64_create_kind(sq_ctx* c, int dim, int inst_kind, uint64_t n, const double* rot, const double* trans,
+                          const double* halves, sq_tlas64** out) {
   if (!c || !out || (n && (!rot || !trans || !halves))) return fail(SQ_ERR_INVALID, "NULL argument");
   *out = nullptr;
+  if (inst_kind != SQ_INST_BOX && inst_kind != SQ_INST_BALL)
+    return fail(SQ_ERR_UNSUPPORTED, "f64 instance trees hold SQ_INST_BOX or SQ_INST_BALL instances");
+  const bool ball = inst_kind == SQ_INST_BALL;
+  if (ball && dim != 3) return fail(SQ_ERR_UNSUPPORTED, "ball instances need dim 3");
   if (dim < 2 || dim > 6) return fail(SQ_ERR_UNSUPPORTED, "dimension " + std::to_string(dim) + " not instantiated (2..6)");
   if (n >= 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "too many instances (ids are u32)");
   const int D = dim, ps = D * D + 4 * D;
@@ -593,10 +659,11 @@ int sq_tlas64_create(sq_ctx* c, int dim, uint64_t n, const double* rot, const do
   for (uint64_t i = 0; i < n; ++i) {
     const double* R = rot + i * D * D;
     const double* t = trans + i * D;
-    const double* h = halves + i * D;
+    double h[6];
+    for (int k = 0; k < D; ++k) h[k] = ball ? halves[i] : halves[i * D + k];
     double* rec = recs.data() + i * ps;
     for (int k = 0; k < D; ++k)
-      if (!(h[k] >= 0.0)) return fail(SQ_ERR_INVALID, "instance half extents must be >= 0");
+      if (!(h[k] >= 0.0)) return fail(SQ_ERR_INVALID, ball ? "ball radii must be >= 0" : "instance half extents must be >= 0");
     std::memcpy(rec, R, D * D * sizeof(double));
     std::memcpy(rec + D * D, t, D * sizeof(double));
     std::memcpy(rec + D * D + D, h, D * sizeof(double));
@@ -614,9 +681,14 @@ int sq_tlas64_create(sq_ctx* c, int dim, uint64_t n, const double* rot, const do
     std::memcpy(rec + D * D + 2 * D, w, 2 * D * sizeof(double));
   }
   sq_bvh64* b = nullptr;
-  if (int rc = create64(c, dim, KIND_INST64, recs.data(), ps, boxes, n, &b)) return rc;
+  if (int rc = create64(c, dim, ball ? KIND_BALL64 : KIND_INST64, recs.data(), ps, boxes, n, &b)) return rc;
   *out = reinterpret_cast<sq_tlas64*>(b);
   return SQ_OK;
+}
+
+int sq_tlas64_create(sq_ctx* c, int dim, uint64_t n, const double* rot, const double* trans, const double* halves,
+                     sq_tlas64** out) {
+  return sq_tlas64_create_kind(c, dim, SQ_INST_BOX, n, rot, trans, halves, out);
 }
 
 int sq_tlas64_destroy(sq_tlas64* t) { return sq_bvh64_destroy(reinterpret_cast<sq_bvh64*>(t)); }
@@ -664,26 +736,20 @@ static int nearest64(sq_bvh64* b, const double* rays, uint64_t n, double max_toi
   if (n >= (1ull << 31)) return fail(SQ_ERR_INVALID, "more than 2^31 queries in one call");
   const Filter fd = filter64(b, f);
   const bool guarded = b->ctx->hit_mode == SQ_HITS_GUARDED;
-  if (b->kind == KIND_INST64) {
-    if (cast) return fail(SQ_ERR_UNSUPPORTED, "no shape casts on f64 instance trees");
-    auto go = [&]<int D>() -> int {
+  if (b->kind == KIND_INST64 || b->kind == KIND_BALL64) {
+    return dispatch_inst64(b->dim, b->kind, [&]<int D, int IK>() -> int {
       auto run = [&]<bool EXACT>() -> int {
+        if (cast)
+          return pool_launch<D, IK, false, true, false, EXACT>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi, out_point,
+                                                               nullptr, nullptr, nullptr, nullptr);
         if (out_any)
-          return pool_launch<D, KIND_INST64, true, false, false, EXACT>(b, rays, n, max_toi, tmax, fd, nullptr, nullptr,
-                                                                        nullptr, nullptr, out_any, nullptr, nullptr);
-        return pool_launch<D, KIND_INST64, false, false, false, EXACT>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi,
-                                                                       out_point, nullptr, nullptr, nullptr, nullptr);
+          return pool_launch<D, IK, true, false, false, EXACT>(b, rays, n, max_toi, tmax, fd, nullptr, nullptr, nullptr,
+                                                               nullptr, out_any, nullptr, nullptr);
+        return pool_launch<D, IK, false, false, false, EXACT>(b, rays, n, max_toi, tmax, fd, out_leaf, out_toi, out_point,
+                                                              nullptr, nullptr, nullptr, nullptr);
       };
       return guarded ? run.template operator()<false>() : run.template operator()<true>();
-    };
-    switch (b->dim) {
-      case 2: return go.template operator()<2>();
-      case 3: return go.template operator()<3>();
-      case 4: return go.template operator()<4>();
-      case 5: return go.template operator()<5>();
-      case 6: return go.template operator()<6>();
-      default: return fail(SQ_ERR_UNSUPPORTED, "dimension not instantiated (2..6)");
-    }
+    });
   }
   return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
     auto run = [&]<bool EXACT>() -> int {
@@ -751,19 +817,23 @@ static int collect64(sq_bvh64* b, const double* q, uint64_t n, double max_toi, c
   const Filter fd = filter64(b, f);
   uint32_t* counts = c->counts.as<uint32_t>();
   // rays and casts: the pooled walk (count pass, then fill pass); overlap: one region per lane
+  // (lists never shrink a limit: every kind keeps its unguarded hits in both modes)
+  auto body = [&]<int D, int KIND>(const uint64_t* offs) -> int {
+    if (mode == 1) {
+      k64_overlap<D, KIND><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, fd, counts, offs, out_leaf);
+      HIP_TRY(hipGetLastError());
+      return SQ_OK;
+    }
+    if (mode == 2)
+      return pool_launch<D, KIND, false, true, true>(b, q, n, max_toi, tmax, fd, out_leaf, out_toi, nullptr, nullptr,
+                                                     nullptr, counts, offs);
+    return pool_launch<D, KIND, false, false, true>(b, q, n, max_toi, tmax, fd, out_leaf, out_toi, nullptr, nullptr,
+                                                    nullptr, counts, offs);
+  };
   auto launch = [&](const uint64_t* offs) -> int {
-    return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int {
-      if (mode == 1) {
-        k64_overlap<D, KIND><<<grid_for(n, 256), 256, 0, c->stream>>>(b->view, q, n, fd, counts, offs, out_leaf);
-        HIP_TRY(hipGetLastError());
-        return SQ_OK;
-      }
-      if (mode == 2)
-        return pool_launch<D, KIND, false, true, true>(b, q, n, max_toi, tmax, fd, out_leaf, out_toi, nullptr, nullptr,
-                                                       nullptr, counts, offs);
-      return pool_launch<D, KIND, false, false, true>(b, q, n, max_toi, tmax, fd, out_leaf, out_toi, nullptr, nullptr,
-                                                      nullptr, counts, offs);
-    });
+    if (b->kind == KIND_INST64 || b->kind == KIND_BALL64)
+      return dispatch_inst64(b->dim, b->kind, [&]<int D, int KIND>() -> int { return body.template operator()<D, KIND>(offs); });
+    return dispatch64(b->dim, b->kind, [&]<int D, int KIND>() -> int { return body.template operator()<D, KIND>(offs); });
   };
   if ((rc = launch(nullptr))) return rc;
   std::vector<std::unique_ptr<DevBuf>> temps;
@@ -792,6 +862,36 @@ int sq_bvh64_shapecast_all(sq_bvh64* b, const double* casts, uint64_t n, double 
 int sq_bvh64_overlap(sq_bvh64* b, const double* regions, uint64_t n, const sq_filter* f, uint64_t* offsets,
                      int32_t* out_leaf, uint64_t capacity, uint64_t* total) {
   return collect64(b, regions, n, 0.0, nullptr, f, 1, offsets, out_leaf, nullptr, capacity, total);
+}
+
+// Tlas::raycast_all (tlas.rs:158-215), also Tlas::raycast_crossings
+// (tlas.rs:221-304) for these kinds: one crossing per instance, the
+// test_ray_crossings_local default.
+int sq_tlas64_raycast_all(sq_tlas64* t, const double* rays, uint64_t n, double max_toi, uint64_t* offsets,
+                          int32_t* out_inst, double* out_toi, uint64_t capacity, uint64_t* total) {
+  return collect64(reinterpret_cast<sq_bvh64*>(t), rays, n, max_toi, nullptr, nullptr, 0, offsets, out_inst, out_toi,
+                   capacity, total);
+}
+
+// Tlas::overlap (tlas.rs:393-432): every instance of a leaf node touching the
+// region that passes test_overlap (exact for balls, true for boxes).
+int sq_tlas64_overlap(sq_tlas64* t, const double* regions, uint64_t n, uint64_t* offsets, int32_t* out_inst,
+                      uint64_t capacity, uint64_t* total) {
+  return collect64(reinterpret_cast<sq_bvh64*>(t), regions, n, 0.0, nullptr, nullptr, 1, offsets, out_inst, nullptr,
+                   capacity, total);
+}
+
+// Tlas::shapecast_nearest (tlas.rs:306-351) and shapecast_all (:353-391).
+int sq_tlas64_shapecast_nearest(sq_tlas64* t, const double* casts, uint64_t n, double max_toi, int32_t* out_inst,
+                                double* out_toi, double* out_point) {
+  return nearest64(reinterpret_cast<sq_bvh64*>(t), casts, n, max_toi, nullptr, nullptr, out_inst, out_toi, out_point,
+                   nullptr, nullptr, true);
+}
+
+int sq_tlas64_shapecast_all(sq_tlas64* t, const double* casts, uint64_t n, double max_toi, uint64_t* offsets,
+                            int32_t* out_inst, double* out_toi, uint64_t capacity, uint64_t* total) {
+  return collect64(reinterpret_cast<sq_bvh64*>(t), casts, n, max_toi, nullptr, nullptr, 2, offsets, out_inst, out_toi,
+                   capacity, total);
 }
 
 }  // extern "C"

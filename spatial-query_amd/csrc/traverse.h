@@ -1243,19 +1243,20 @@ SQ_HD bool local_box(const S* o, const S* d, const S* half, S max_toi, S& t, int
 
 // ray_ball (shape_overlap.rs:120-138) about centre c: the entry root, else the
 // exit root; rejects t < 0 or t > max_toi.  With c = NULL the centre is the
-// origin (a local ray against a ball instance).
-template <int D> SQ_HD bool ray_ball(const float* o, const float* d, const float* c, float rad, float max_toi, float& t) {
-  float oc[D];
+// origin (a local ray against a ball instance).  Scalar-generic (the f64
+// feature instantiates it on double).
+template <int D, class S = float> SQ_HD bool ray_ball(const S* o, const S* d, const S* c, S rad, S max_toi, S& t) {
+  S oc[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) oc[k] = c ? o[k] - c[k] : o[k];
-  const float b = dotD<D>(oc, d);
-  const float cc = dotD<D>(oc, oc) - rad * rad;
-  const float disc = b * b - cc;
-  if (disc < 0.0f) return false;
-  const float sq = SQ_SQRT_RN(disc);
-  float tt = -b - sq;
-  if (tt < 0.0f) tt = -b + sq;
-  if (tt < 0.0f || tt > max_toi) return false;
+  const S b = dotD<D>(oc, d);
+  const S cc = dotD<D>(oc, oc) - rad * rad;
+  const S disc = b * b - cc;
+  if (disc < S(0)) return false;
+  const S sq = sq_sqrt(disc);
+  S tt = -b - sq;
+  if (tt < S(0)) tt = -b + sq;
+  if (tt < S(0) || tt > max_toi) return false;
   t = tt;
   return true;
 }
@@ -1290,7 +1291,7 @@ SQ_HD bool instance_test(const TlasView& tv, const RayD<D>& r, uint32_t id, floa
     if (!local_box<D>(ol, dl, x + D * D + D, limit, t, ax)) return false;
     h.pos = (uint32_t)ax; h.sub = NO_LEAF;
   } else if constexpr (IK == INST_BALL) {
-    if (!ray_ball<D>(ol, dl, nullptr, x[D * D + D], limit, t)) return false;
+    if (!ray_ball<D>(ol, dl, (const float*)nullptr, x[D * D + D], limit, t)) return false;
     h.pos = 0; h.sub = NO_LEAF;
   } else {
     const float inv_s = x[D * D + D];

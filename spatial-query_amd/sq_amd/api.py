@@ -571,17 +571,54 @@ class DeviceBvh64:
 
 
 class Tlas64:
-    """Tlas over box instances under the `f64` feature: rotations [n][D][D],
-    translations [n][D], half extents [n][D] (host float64)."""
+    """Tlas under the `f64` feature over box instances (kind 0: rotations
+    [n][D][D], translations [n][D], half extents [n][D]) or ball instances
+    (kind 2, D = 3: halves = radii [n]); host float64.  raycast_all is also
+    Tlas::raycast_crossings for these kinds (one crossing per instance)."""
 
-    def __init__(self, ctx: Context, dim: int, rot, trans, halves):
+    def __init__(self, ctx: Context, dim: int, rot, trans, halves, kind: int = 0):
         r = _host(rot, np.float64).reshape(-1, dim, dim)
         t = _host(trans, np.float64).reshape(-1, dim)
-        h = _host(halves, np.float64).reshape(-1, dim)
+        h = _host(halves, np.float64).reshape(-1) if kind == 2 else _host(halves, np.float64).reshape(-1, dim)
+        if h.shape[0] != r.shape[0] or t.shape[0] != r.shape[0]:
+            raise ValueError("rot, trans and halves must describe the same number of instances")
         hd = C.c_void_p()
-        check(lib().sq_tlas64_create(ctx.handle, dim, r.shape[0], _hp(r), _hp(t), _hp(h), C.byref(hd)))
+        check(lib().sq_tlas64_create_kind(ctx.handle, dim, int(kind), r.shape[0], _hp(r), _hp(t), _hp(h), C.byref(hd)))
         self._h, self.ctx, self.dim, self.n = hd, ctx, dim, r.shape[0]
         _track(self)
+
+    _csr = DeviceBvh64._csr
+
+    def raycast_all(self, rays, max_toi, capacity=0):
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_tlas64_raycast_all(self._h, _dp(rays, "float64", 2 * self.dim), rays.shape[0], float(max_toi),
+                                               _dp(off, "int64"), _dp(leaf, "int32"), _dp(toi, "float64"), cap, C.byref(total))
+        return self._csr(fn, rays.shape[0], rays.device, True, capacity)
+
+    def overlap(self, regions, capacity=0):
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_tlas64_overlap(self._h, _dp(regions, "float64", 2 * self.dim), regions.shape[0], _dp(off, "int64"),
+                                           _dp(leaf, "int32"), cap, C.byref(total))
+        off, leaf, _ = self._csr(fn, regions.shape[0], regions.device, False, capacity)
+        return off, leaf
+
+    def shapecast_nearest(self, casts, max_toi, point=False):
+        import torch
+        n, dev = casts.shape[0], casts.device
+        leaf = torch.empty(n, dtype=torch.int32, device=dev)
+        toi = torch.empty(n, dtype=torch.float64, device=dev)
+        pt = torch.empty((n, self.dim), dtype=torch.float64, device=dev) if point else None
+        check(lib().sq_tlas64_shapecast_nearest(self._h, _dp(casts, "float64", 4 * self.dim), n, float(max_toi),
+                                                _dp(leaf, "int32"), _dp(toi, "float64"), _dp(pt, "float64")))
+        out = dict(leaf=leaf, toi=toi)
+        if point: out["point"] = pt
+        return out
+
+    def shapecast_all(self, casts, max_toi, capacity=0):
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_tlas64_shapecast_all(self._h, _dp(casts, "float64", 4 * self.dim), casts.shape[0], float(max_toi),
+                                                 _dp(off, "int64"), _dp(leaf, "int32"), _dp(toi, "float64"), cap, C.byref(total))
+        return self._csr(fn, casts.shape[0], casts.device, True, capacity)
 
     def stats(self) -> dict:
         s = TreeStats()

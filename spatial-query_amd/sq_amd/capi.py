@@ -134,6 +134,11 @@ SIGNATURES = {
     "sq_bvh64_raycast_all": [P, P, U64, F64, C.POINTER(Filter), P, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_bvh64_overlap": [P, P, U64, C.POINTER(Filter), P, P, U64, C.POINTER(C.c_uint64)],
     "sq_tlas64_create": [P, I32, U64, P, P, P, PP],
+    "sq_tlas64_create_kind": [P, I32, I32, U64, P, P, P, PP],
+    "sq_tlas64_raycast_all": [P, P, U64, F64, P, P, P, U64, P],
+    "sq_tlas64_overlap": [P, P, U64, P, P, U64, P],
+    "sq_tlas64_shapecast_nearest": [P, P, U64, F64, P, P, P],
+    "sq_tlas64_shapecast_all": [P, P, U64, F64, P, P, P, U64, P],
     "sq_tlas64_destroy": [P],
     "sq_tlas64_stats": [P, C.POINTER(TreeStats)],
     "sq_tlas64_raycast_nearest": [P, P, U64, F64, P, P, P, P],

@@ -351,3 +351,109 @@ def test_f64_balls_both_hit_modes(ctx):
         assert np.array_equal(host(gc["leaf"]).astype(np.int64), wc["leaf"]), f"casts guarded={guarded}"
         assert np.array_equal(host(gc["toi"]).view(np.uint64), wc["t"].view(np.uint64))
     b.close()
+
+
+def _csr_equal64(got, want, with_t=True):
+    off, leaf = host(got[0]).astype(np.uint64), host(got[1]).astype(np.int64)
+    assert np.array_equal(off, want[0].astype(np.uint64)), "per-query counts differ"
+    assert np.array_equal(leaf, want[1]), "ids or order differ"
+    if with_t:
+        assert np.array_equal(host(got[2]).view(np.uint64), want[2].view(np.uint64)), "toi bits differ"
+
+
+@pytest.mark.parametrize("dim", [2, 3, 6])
+def test_f64_tlas_box_instance_queries(ctx, dim):
+    """The f64 Tlas over box instances answers every Tlas query as the f64
+    oracle's Tlas walks (oracle.Instances64): all hits (= crossings) as CSR
+    bit for bit in both hit modes, overlap (every instance of a leaf node
+    touching the region, test_overlap's default), and shape casts that never
+    touch a box instance (the trait default)."""
+    print(f"case: f64 Tlas box-instance queries dim={dim}", flush=True)
+    rng = np.random.default_rng(0x7A5 + dim)
+    n = 3000
+    span = {2: 200.0, 3: 100.0, 6: 12.0}[dim]
+    rot = orthonormal(rng, n, dim)
+    trans = rng.uniform(-span, span, (n, dim))
+    halves = rng.uniform(0.5, 4.0, (n, dim))
+    o = rng.uniform(-span, span, (4000, dim))
+    d = rng.normal(size=(4000, dim))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1)
+    max_toi = 4.0 * span
+    tl = sq_amd.Tlas64(ctx, dim, rot, trans, halves)
+    ref = oracle.Instances64(dim, 0, rot, trans, halves)
+    want = ref.raycast_all(rays, max_toi)
+    assert want[1].size > 1000
+    for mode in (sq_amd.SQ_HITS_REFERENCE, sq_amd.SQ_HITS_GUARDED):
+        ctx.set_hit_mode(mode)
+        try:
+            _csr_equal64(tl.raycast_all(dev(rays), max_toi), want)
+        finally:
+            ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
+    c = rng.uniform(-span, span, (3000, dim)); h = rng.uniform(0.05, 0.05 * span, (3000, dim))
+    regions = np.concatenate([c - h, c + h], 1)
+    wo = ref.overlap(regions)
+    assert wo[1].size > 500
+    _csr_equal64(tl.overlap(dev(regions)), wo, with_t=False)
+    casts = np.concatenate([rays[:500], np.full((500, dim), -0.5), np.full((500, dim), 0.5)], 1)
+    cn = tl.shapecast_nearest(dev(casts), max_toi, point=True)
+    assert (host(cn["leaf"]) == -1).all() and np.isinf(host(cn["toi"])).all()
+    off, leaf, _ = tl.shapecast_all(dev(casts), max_toi)
+    assert int(host(off)[-1]) == 0
+    tl.close()
+
+
+@pytest.mark.parametrize("rotated", [False, True])
+def test_f64_tlas_ball_instances(ctx, rotated):
+    """Ball instances under the `f64` feature (shape_overlap.rs:288-345 in
+    double): nearest rays, any-hit, all hits, shape casts (nearest and all)
+    and exact overlap equal the oracle's Tlas walks bit for bit in the
+    default mode, and the guarded oracle in SQ_HITS_GUARDED; with translations
+    only they equal the naive scans over the same balls as a single-level
+    provider (the reference's own tests' ground truth)."""
+    print(f"case: f64 ball instances rotated={rotated}", flush=True)
+    rng = np.random.default_rng(0xBA1 + rotated)
+    n = 4000
+    trans = rng.uniform(-60.0, 60.0, (n, 3))
+    radii = rng.uniform(0.5, 2.0, n)
+    rot = orthonormal(rng, n, 3) if rotated else np.tile(np.eye(3), (n, 1, 1))
+    o = rng.uniform(-70.0, 70.0, (6000, 3))
+    aim = trans[rng.integers(0, n, 6000)] + rng.uniform(-2.5, 2.5, (6000, 3)) - o
+    rays = np.concatenate([o, aim / np.linalg.norm(aim, axis=1, keepdims=True)], 1)
+    casts = np.concatenate([rays, np.full((6000, 3), -0.75), np.full((6000, 3), 0.75)], 1)
+    c = rng.uniform(-60.0, 60.0, (3000, 3)); h = rng.uniform(1.0, 6.0, (3000, 3))
+    regions = np.concatenate([c - h, c + h], 1)
+    tl = sq_amd.Tlas64(ctx, 3, rot, trans, radii, kind=2)
+    for guarded in (False, True):
+        ref = oracle.Instances64(3, 1, rot, trans, radii, guarded=guarded)
+        ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED if guarded else sq_amd.SQ_HITS_REFERENCE)
+        try:
+            w = ref.raycast_nearest(rays, 300.0)
+            assert (w["leaf"] >= 0).sum() > 2000
+            g = tl.raycast_nearest(dev(rays), 300.0)
+            assert np.array_equal(host(g["leaf"]).astype(np.int64), w["leaf"])
+            assert np.array_equal(host(g["toi"]).view(np.uint64), w["t"].view(np.uint64))
+            assert np.array_equal(host(tl.raycast_any(dev(rays), 300.0)).astype(bool), w["leaf"] >= 0)
+            wc = ref.shapecast_nearest(casts, 300.0)
+            gc = tl.shapecast_nearest(dev(casts), 300.0)
+            assert np.array_equal(host(gc["leaf"]).astype(np.int64), wc["leaf"])
+            assert np.array_equal(host(gc["toi"]).view(np.uint64), wc["t"].view(np.uint64))
+        finally:
+            ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
+    ref = oracle.Instances64(3, 1, rot, trans, radii)
+    _csr_equal64(tl.raycast_all(dev(rays), 300.0), ref.raycast_all(rays, 300.0))
+    wa = ref.shapecast_all(casts, 300.0)
+    assert wa[1].size > 2000
+    _csr_equal64(tl.shapecast_all(dev(casts), 300.0), wa)
+    wo = ref.overlap(regions)
+    assert wo[1].size > 1000
+    _csr_equal64(tl.overlap(dev(regions)), wo, with_t=False)
+    if not rotated:
+        balls = oracle.Scene64(3, oracle.SPHERE, np.concatenate([trans, radii[:, None]], 1), naive=True)
+        wb = balls.raycast_nearest(rays, 300.0)
+        g = tl.raycast_nearest(dev(rays), 300.0)
+        assert np.array_equal(host(g["leaf"]).astype(np.int64), wb["leaf"])
+        assert np.array_equal(host(g["toi"]).view(np.uint64), wb["t"].view(np.uint64))
+        _csr_equal64(tl.overlap(dev(regions)), balls.overlap(regions), with_t=False)
+        _csr_equal64(tl.shapecast_all(dev(casts), 300.0), balls.shapecast_all(casts, 300.0))
+    tl.close()

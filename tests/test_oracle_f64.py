@@ -300,3 +300,189 @@ def test_box_instances_dyadic_equal_the_f32_oracle(dim):
         box = np.concatenate([-halves[i], halves[i]]).astype(np.float32)
         wb = oracle.iso_transform_aabb(rot[i].astype(np.float32), trans[i].astype(np.float32), box)
         assert np.array_equal(w["world"][i], np.asarray(wb, np.float64).reshape(-1))
+
+
+# ---------------------------------------------------------------------------
+# The f64 Tlas queries over box and ball instances (oracle.Instances64)
+# ---------------------------------------------------------------------------
+
+def _rays64(rng, n, span, dim=3):
+    o = rng.uniform(-span, span, (n, dim))
+    d = rng.normal(size=(n, dim))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o, d], 1)
+
+
+def _casts64(rays, half):
+    n, dim = rays.shape[0], rays.shape[1] // 2
+    return np.concatenate([rays, np.full((n, dim), -half), np.full((n, dim), half)], 1)
+
+
+def test_ball_instance_closed_forms():
+    """A ball instance of radius 1 at (5, 0, 0): a ray along +x from the origin
+    enters at 4, one from the centre leaves at 1, one passing at height 2
+    misses; a cube probe of half 0.5 swept along +x touches at 5 - 1.5 = 3.5
+    (shape_overlap.rs:326-337); a region whose nearest point lies 0.8 from the
+    centre overlaps, one that touches the ball's world box but lies
+    sqrt(3 * 0.8^2) = 1.39 away does not (point_to_aabb, :59-72); a box
+    instance answers no cast (the trait default) and overlaps at AABB level:
+    every instance of a leaf node touching the region, even one whose own
+    world box misses it (a 3-instance Tlas is one leaf)."""
+    ball = oracle.Instances64(3, 1, np.eye(3)[None], np.array([[5.0, 0.0, 0.0]]), np.array([1.0]))
+    rays = np.array([[0, 0, 0, 1, 0, 0], [5, 0, 0, 1, 0, 0], [0, 2, 0, 1, 0, 0]], np.float64)
+    r = ball.raycast_nearest(rays, 100.0)
+    assert r["leaf"].tolist() == [0, 0, -1] and r["t"][:2].tolist() == [4.0, 1.0]
+    off, leaf, t = ball.raycast_all(rays, 100.0)
+    assert off.tolist() == [0, 1, 2, 2] and t.tolist() == [4.0, 1.0]
+    c = ball.shapecast_nearest(_casts64(rays[:1], 0.5), 100.0)
+    assert c["leaf"][0] == 0 and c["t"][0] == 3.5
+    regions = np.array([[3.5, -1, -1, 4.2, 1, 1], [5.8, 0.8, 0.8, 6.5, 1.5, 1.5]], np.float64)
+    off, leaf = ball.overlap(regions)
+    assert off.tolist() == [0, 1, 1] and leaf.tolist() == [0]
+    boxes = oracle.Instances64(3, 0, np.tile(np.eye(3), (3, 1, 1)), np.array([[5.0, 0, 0], [9.0, 0, 0], [30.0, 0, 0]]),
+                               np.ones((3, 3)))
+    assert boxes.raycast_nearest(rays[:1], 100.0)["t"][0] == 4.0
+    off, leaf, t = boxes.raycast_all(rays[:1], 100.0)
+    assert leaf.tolist() == [0, 1, 2] and t.tolist() == [4.0, 8.0, 29.0]
+    assert (boxes.shapecast_nearest(_casts64(rays, 0.5), 100.0)["leaf"] == -1).all()
+    assert boxes.shapecast_all(_casts64(rays, 0.5), 100.0)[0][-1] == 0
+    # the region touches the leaf's box (x 4 .. 31) near x 10 but no instance's own box
+    assert boxes.overlap(np.array([[10.5, -0.5, -0.5, 11.0, 0.5, 0.5]]))[1].tolist() == [0, 1, 2]
+    assert boxes.overlap(np.array([[10.5, 2.5, -0.5, 11.0, 3.0, 0.5]]))[1].tolist() == []
+    assert boxes.overlap(np.array([[9.5, -0.5, -0.5, 11.0, 0.5, 0.5]]))[1].tolist() == [0, 1, 2]
+    naive = oracle.Instances64(3, 0, np.tile(np.eye(3), (3, 1, 1)), np.array([[5.0, 0, 0], [9.0, 0, 0], [30.0, 0, 0]]),
+                               np.ones((3, 3)), naive=True)
+    assert naive.overlap(np.array([[10.5, -0.5, -0.5, 11.0, 0.5, 0.5]]))[1].tolist() == []
+    assert naive.overlap(np.array([[9.5, -0.5, -0.5, 11.0, 0.5, 0.5]]))[1].tolist() == [1]
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_tlas_ball_instances_match_the_naive_balls(seed):
+    """shape_overlap.rs:347-411 in double: a Tlas of ball instances
+    (translations) answers rays, shape casts (nearest and all) and overlap
+    exactly as the naive scan over the same balls as a single-level provider
+    (the SPHERE kind: ray_ball, the radius-sum cast, point_to_aabb); with
+    rotated instances the Tlas walk equals the naive instance scan."""
+    rng = np.random.default_rng(700 + seed)
+    hits = casts_hit = overlaps = 0
+    for trial in range(40):
+        n = 48 if trial % 2 else 64
+        trans = rng.uniform(-30.0, 30.0, (n, 3))
+        radii = rng.uniform(0.5, 2.0, n)
+        inst = oracle.Instances64(3, 1, np.tile(np.eye(3), (n, 1, 1)), trans, radii)
+        balls = oracle.Scene64(3, oracle.SPHERE, np.concatenate([trans, radii[:, None]], 1), naive=True)
+        rays = _rays64(rng, 64, 50.0)
+        # half of them aimed near a ball centre, so that many hit
+        aim = trans[rng.integers(0, n, 32)] + rng.uniform(-2.0, 2.0, (32, 3)) - rays[:32, :3]
+        rays[:32, 3:] = aim / np.linalg.norm(aim, axis=1, keepdims=True)
+        casts = _casts64(rays, 0.75)
+        a, b = inst.raycast_nearest(rays, 500.0), balls.raycast_nearest(rays, 500.0)
+        assert np.array_equal(a["leaf"], b["leaf"]) and np.array_equal(a["t"], b["t"])
+        oa, la, ta = inst.raycast_all(rays, 500.0)
+        ob, lb, tb = balls.raycast_all(rays, 500.0)
+        assert np.array_equal(oa, ob) and np.array_equal(la, lb) and np.array_equal(ta, tb)
+        a, b = inst.shapecast_nearest(casts, 500.0), balls.shapecast_nearest(casts, 500.0)
+        assert np.array_equal(a["leaf"], b["leaf"]) and np.array_equal(a["t"], b["t"])
+        oa, la, ta = inst.shapecast_all(casts, 500.0)
+        ob, lb, tb = balls.shapecast_all(casts, 500.0)
+        assert np.array_equal(oa, ob) and np.array_equal(la, lb) and np.array_equal(ta, tb)
+        c = rng.uniform(-30.0, 30.0, (32, 3)); h = rng.uniform(1.0, 6.0, (32, 3))
+        regions = np.concatenate([c - h, c + h], 1)
+        oa, la = inst.overlap(regions)
+        ob, lb = balls.overlap(regions)
+        assert np.array_equal(oa, ob) and np.array_equal(la, lb)
+        hits += int((inst.raycast_nearest(rays, 500.0)["leaf"] >= 0).sum())
+        casts_hit += int(ta.size)
+        overlaps += int(la.size)
+        # rotated instances (a ball is invariant, the local ray is not): Tlas == naive instance scan
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        rot = np.tile(q, (n, 1, 1))
+        ri = oracle.Instances64(3, 1, rot, trans, radii)
+        rn = oracle.Instances64(3, 1, rot, trans, radii, naive=True)
+        a, b = ri.raycast_nearest(rays, 500.0), rn.raycast_nearest(rays, 500.0)
+        assert np.array_equal(a["leaf"], b["leaf"]) and np.array_equal(a["t"], b["t"])
+        oa, la, ta = ri.raycast_all(rays, 500.0)
+        ob, lb, tb = rn.raycast_all(rays, 500.0)
+        assert np.array_equal(oa, ob) and np.array_equal(la, lb) and np.array_equal(ta, tb)
+    assert hits > 300 and casts_hit > 300 and overlaps > 100
+
+
+def _plane_rotations(rng, n, dim):
+    rot = np.tile(np.eye(dim), (n, 1, 1))
+    for i in range(n):
+        a, b = rng.choice(dim, 2, replace=False)
+        th = rng.uniform(0.0, 2 * np.pi)
+        rot[i, a, a] = np.cos(th); rot[i, b, b] = np.cos(th)
+        rot[i, a, b] = -np.sin(th); rot[i, b, a] = np.sin(th)
+    return rot
+
+
+@pytest.mark.parametrize("dim", [2, 3, 6])
+def test_tlas_box_instance_queries(dim):
+    """Box instances (two_level.rs:70-174) in double, plane rotations: the Tlas
+    walks equal the naive instance scans for all hits (= crossings, one per
+    instance) in both modes and for nearest hits in the guarded mode; no
+    shape cast ever touches a box instance; overlap holds every instance whose
+    world box touches the region and, beyond those, only instances of leaf
+    nodes touching it (tlas.rs:393-432 with test_overlap's default)."""
+    rng = np.random.default_rng(800 + dim)
+    n = 300
+    span = {2: 40.0, 3: 12.0, 6: 4.0}[dim]   # denser in higher dimension, so rays still hit
+    trans = rng.uniform(-span, span, (n, dim))
+    halves = rng.uniform(0.5, 2.5, (n, dim))
+    rot = _plane_rotations(rng, n, dim)
+    rays = _rays64(rng, 2000, 1.5 * span, dim)
+    total = 0
+    for guarded in (False, True):
+        t = oracle.Instances64(dim, 0, rot, trans, halves, guarded=guarded)
+        nv = oracle.Instances64(dim, 0, rot, trans, halves, guarded=guarded, naive=True)
+        oa, la, ta = t.raycast_all(rays, 500.0)
+        ob, lb, tb = nv.raycast_all(rays, 500.0)
+        assert np.array_equal(oa, ob) and np.array_equal(la, lb) and np.array_equal(ta, tb)
+        total += la.size
+        if guarded:
+            a, b = t.raycast_nearest(rays, 500.0), nv.raycast_nearest(rays, 500.0)
+            assert np.array_equal(a["leaf"], b["leaf"]) and np.array_equal(a["t"], b["t"])
+        casts = _casts64(rays, 0.5)
+        assert (t.shapecast_nearest(casts, 500.0)["leaf"] == -1).all() and t.shapecast_all(casts, 500.0)[0][-1] == 0
+    assert total > 1000
+    c = rng.uniform(-span, span, (400, dim)); h = rng.uniform(0.1, 0.1 * span, (400, dim))
+    regions = np.concatenate([c - h, c + h], 1)
+    t = oracle.Instances64(dim, 0, rot, trans, halves)
+    nv = oracle.Instances64(dim, 0, rot, trans, halves, naive=True)
+    oa, la = t.overlap(regions)
+    ob, lb = nv.overlap(regions)
+    extra = 0
+    for i in range(len(regions)):
+        got, want = set(la[oa[i]:oa[i + 1]].tolist()), set(lb[ob[i]:ob[i + 1]].tolist())
+        assert want <= got
+        extra += len(got - want)
+    assert lb.size > 200 and extra > 0   # leaf-level reports do occur
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_box_instance_queries_dyadic_equal_the_f32_oracle(dim):
+    """With signed-permutation rotations and dyadic translations, half extents
+    and rays every operation is exact in both precisions and the two trees
+    are the same, so the f64 Tlas walks equal the f32 oracle's Tlas (plain box
+    instances) bit for bit: all hits and overlap (leaf-level sets included)."""
+    rng = np.random.default_rng(900 + dim)
+    n = 300
+    rot = np.zeros((n, dim, dim))
+    for i in range(n):
+        rot[i, np.arange(dim), rng.permutation(dim)] = rng.choice([-1.0, 1.0], dim)
+    trans = rng.integers(-160, 160, (n, dim)) / 8.0
+    halves = rng.integers(1, 24, (n, dim)) / 8.0
+    rays = dyadic_rays(2000, dim, 905 + dim, 160, 16, zero_dirs=False)
+    s64 = oracle.Instances64(dim, 0, rot, trans, halves)
+    s32 = oracle.InstanceScene(dim, oracle.INST_BOX, rot.astype(np.float32), trans.astype(np.float32),
+                               halves=halves.astype(np.float32))
+    o64, l64, t64 = s64.raycast_all(rays, 80.0)
+    o32, l32, t32, _ = s32.raycast_all(rays.astype(np.float32), 80.0)
+    assert l64.size > 500
+    assert np.array_equal(o64, o32) and np.array_equal(l64, l32) and np.array_equal(t64, t32.astype(np.float64))
+    c = rng.integers(-160, 160, (300, dim)) / 8.0; h = rng.integers(1, 40, (300, dim)) / 8.0
+    regions = np.concatenate([c - h, c + h], 1)
+    o64, l64 = s64.overlap(regions)
+    o32, l32 = s32.overlap(regions.astype(np.float32))
+    assert l64.size > 200 and np.array_equal(o64, o32) and np.array_equal(l64, l32)
