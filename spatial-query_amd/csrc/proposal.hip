@@ -251,6 +251,20 @@ __device__ __forceinline__ void top4_insert(float (&tv)[4], int (&ti)[4], float 
   }
 }
 
+// Insert (v, c) into a best-first list of four whose entries all have smaller
+// class ids than c and whose fourth v beats (v > tv[3]): v goes before every
+// entry it exceeds and after every entry it ties (the smaller id ranks first),
+// the entries below it shift down one place, branch-free.
+__device__ __forceinline__ void top4_insert_after(float (&tv)[4], int (&ti)[4], float v, int c) {
+  const bool b0 = v > tv[0], b1 = v > tv[1], b2 = v > tv[2];
+  const float v3 = b2 ? tv[2] : v, v2 = b1 ? tv[1] : (b2 ? v : tv[2]);
+  const float v1 = b0 ? tv[0] : (b1 ? v : tv[1]), v0 = b0 ? v : tv[0];
+  const int c3 = b2 ? ti[2] : c, c2 = b1 ? ti[1] : (b2 ? c : ti[2]);
+  const int c1 = b0 ? ti[0] : (b1 ? c : ti[1]), c0 = b0 ? c : ti[0];
+  tv[0] = v0; tv[1] = v1; tv[2] = v2; tv[3] = v3;
+  ti[0] = c0; ti[1] = c1; ti[2] = c2; ti[3] = c3;
+}
+
 // Fused inference: features -> 3 ReLU layers -> logits -> top-4, for IR = 128
 // rays per block of 512 threads (8 waves).  Activations stay in LDS (one
 // [128][HID + 8] bf16 buffer, overwritten layer by layer); weights stream from
@@ -457,10 +471,13 @@ __global__ __launch_bounds__(IT, 4) void k_infer_top4(const float* __restrict__ 
     if (kk == ksteps - 1) {
       // Per ray, the best of its 16 candidates against th, the larger of this
       // list's fourth and the ray's shared threshold: below th nothing can
-      // enter the ray's top four (a tie with the fourth needs >=), so the exact
-      // ordered insertion runs only for candidates at or above it.  A list
-      // whose fourth rose publishes it.  Classes past n_cls exist only in the
-      // last tile.
+      // enter the ray's top four (a tie with another list's fourth needs >=),
+      // so the insertion runs only for candidates at or above it.  A lane meets
+      // its classes in increasing order (tiles, then r), so a candidate enters
+      // its own list only above the list's fourth (a tie keeps the earlier,
+      // smaller id) and ranks after the entries it ties: the branch-free
+      // insertion top4_insert_after.  A list whose fourth rose publishes it.
+      // Classes past n_cls exist only in the last tile.
       const bool tail = m0 + 128 > n_cls;
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
@@ -469,13 +486,21 @@ __global__ __launch_bounds__(IT, 4) void k_infer_top4(const float* __restrict__ 
         float best = acc[j][0];
 #pragma unroll
         for (int r = 1; r < 16; ++r) best = fmaxf(best, acc[j][r]);
-        if (tail || best >= th) {
+        if (best >= th) {
           const float before = tv[j][3];
+          if (!tail) {
 #pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const int cls = m0 + cr + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (cls < n_cls && acc[j][r] >= th && beats(acc[j][r], cls, tv[j][3], ti[j][3]))
-              top4_insert(tv[j], ti[j], acc[j][r], cls);
+            for (int r = 0; r < 16; ++r) {
+              const float v = acc[j][r];
+              if (v >= th && v > tv[j][3]) top4_insert_after(tv[j], ti[j], v, m0 + cr + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3));
+            }
+          } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+              const int cls = m0 + cr + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+              const float v = acc[j][r];
+              if (cls < n_cls && v >= th && v > tv[j][3]) top4_insert_after(tv[j], ti[j], v, cls);
+            }
           }
           if (tv[j][3] > before) atomicMax(&thr[ray], ordered_bits(tv[j][3]));
         }

@@ -151,6 +151,54 @@ def test_infer_top4_matches_oracle(ctx, small, cut):
     p.close()
 
 
+@pytest.mark.parametrize("group", [3, 5])
+def test_infer_top4_exact_ties_keep_the_smaller_ids(ctx, small, group):
+    """Exact ties in the fused top-4: the last layer's rows and biases repeat
+    in groups of 3 or 5 classes (so a group's classes straddle the lanes and
+    partial lists that hold them), making the logits of a group bit-identical
+    on the device.  The top four are then the classes of the best groups in
+    ascending id order: for every ray whose two best groups are separated
+    beyond the float64 oracle's rounding tolerance, the device's list is
+    exactly that (a tie kept in the wrong order, or a smaller id lost to a
+    later tie, fails)."""
+    print(f"case: fused top-4 with exact ties, groups of {group}", flush=True)
+    sc, bvh = small
+    p = sq_amd.Proposal(bvh, cut_depth=9, seed=0x7E + group)
+    layers = p.weights()
+    rng = np.random.default_rng(group)
+    n = p.n_subtrees
+    w4, b4 = layers[-1]
+    g = np.arange(w4.shape[0]) // group
+    w4 = w4[g * group].copy()
+    b4 = (0.2 * rng.standard_normal(b4.shape)).astype(np.float32)[g * group]
+    layers = layers[:-1] + [(w4, b4)]
+    p.set_weights(layers)
+    rays = W.proposal_training_rays(sc, 20000, seed=0x71E + group)
+    top = p.infer(dev(rays)).cpu().numpy().view(np.uint16).astype(np.int64)
+    B, box = p.features_spec()
+    _, z = mlp.forward(mlp.features(rays, B, box), layers)
+    z = np.asarray(z, np.float64)[:, :n]
+    zg = z[:, ::group]                              # one logit per group
+    order = np.argsort(-zg, axis=1, kind="stable")
+    tol = 1e-3 * np.abs(z).max()
+    sizes = np.minimum(group, n - np.arange(zg.shape[1]) * group)
+    checked = 0
+    for i in range(len(rays)):
+        want, k = [], 0
+        while len(want) < 4:
+            gi = order[i, k]
+            want += list(range(gi * group, gi * group + sizes[gi]))[: 4 - len(want)]
+            k += 1
+        # the groups that supply the four must be separated from the next ones
+        if zg[i, order[i, k - 1]] - zg[i, order[i, k]] <= tol or any(
+                zg[i, order[i, j]] - zg[i, order[i, j + 1]] <= tol for j in range(k - 1)):
+            continue
+        assert top[i].tolist() == want, (i, top[i].tolist(), want)
+        checked += 1
+    assert checked > 10000
+    p.close()
+
+
 @pytest.mark.parametrize("mode", ["untrained", "random_ids", "empty", "none", "short_tmax"])
 def test_proposed_walk_is_exact(ctx, small, mode):
     import torch
