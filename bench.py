@@ -150,7 +150,8 @@ def main():
             prof = json.load(fh)
         tun_now = ctx.tuning()
         if (prof.get("hit_mode", "reference") == args.hit_mode and prof.get("wide", 0) == tun_now["wide"]
-                and prof.get("lookahead", 0) == tun_now["lookahead"]):
+                and prof.get("lookahead", 0) == tun_now["lookahead"]
+                and prof.get("lookahead_pause", 24) == tun_now["lookahead_pause"]):
             traffic = int(prof["traffic_bytes"])
             valu = prof.get("valu")
     out = {

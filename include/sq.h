@@ -450,6 +450,11 @@ int sq_tlas_shapecast_all(sq_tlas* tlas, const float* casts, uint64_t n, float m
  *                  against the new limit (the same primitives tested with the
  *                  same limits, results as 0 bit for bit; 0.94x the time on the
  *                  8.4M heightfield rays); 0 = one leaf at a time.
+ *   lookahead_pause descend_min of the look-ahead walk (0..64, default 40):
+ *                  its lanes keep descending past a pending leaf, so it pauses
+ *                  at a higher count than the other walks (on the 8.4M
+ *                  heightfield rays 0.96x the time at 40 against 24, where
+ *                  the other walks are fastest).  Results do not depend on it.
  *   packet         nearest-hit ray calls without a filter: 1 walks each 64
  *                  rays of the batch (in the Morton order above, for calls of
  *                  >= 4096 rays) as one wave-coherent packet: one walk per
@@ -470,6 +475,7 @@ typedef struct sq_tuning {
   int wide;
   int packet;
   int lookahead;
+  int lookahead_pause;
 } sq_tuning;
 int sq_ctx_set_tuning(sq_ctx* ctx, const sq_tuning* tuning);
 int sq_ctx_get_tuning(const sq_ctx* ctx, sq_tuning* tuning);

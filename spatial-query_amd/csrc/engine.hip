@@ -905,6 +905,7 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   if (t->wide < 0 || t->wide > 2) return fail(SQ_ERR_INVALID, "wide must be 0, 1 or 2");
   if (t->packet != 0 && t->packet != 1) return fail(SQ_ERR_INVALID, "packet must be 0 or 1");
   if (t->lookahead != 0 && t->lookahead != 1) return fail(SQ_ERR_INVALID, "lookahead must be 0 or 1");
+  if (t->lookahead_pause < 0 || t->lookahead_pause > 64) return fail(SQ_ERR_INVALID, "lookahead_pause must be in [0, 64]");
   c->block = block;
   c->rays_per_lane = t->rays_per_lane;
   c->lds_stack = S;
@@ -914,6 +915,7 @@ int sq_ctx_set_tuning(sq_ctx* c, const sq_tuning* t) {
   c->wide = t->wide;
   c->packet = t->packet;
   c->lookahead = t->lookahead;
+  c->lookahead_pause = t->lookahead_pause;
   return SQ_OK;
 }
 
@@ -937,6 +939,7 @@ int sq_ctx_get_tuning(const sq_ctx* c, sq_tuning* t) {
   t->wide = c->wide;
   t->packet = c->packet;
   t->lookahead = c->lookahead;
+  t->lookahead_pause = c->lookahead_pause;
   return SQ_OK;
 }
 
@@ -1364,7 +1367,7 @@ static int nearest_impl(sq_bvh* b, const float* q, uint64_t n, float max_toi, co
       if (r2) return r2;
       k_nearest_pool<D, K, ANY_, FILT_, CAST_, QUAD_, EXACT_, SPEC_><<<(uint32_t)grid, B, lds, c->stream>>>(
           s, q, nn, max_toi, tmax, fd, out, (uint32_t)chunk, counter, S, c->spill.as<SpillEntry>(), (uint32_t)lanes, order,
-          (uint32_t)c->descend_min);
+          (uint32_t)(SPEC_ ? c->lookahead_pause : c->descend_min));
       HIP_TRY(hipGetLastError());
       return SQ_OK;
     };

@@ -73,6 +73,7 @@ struct sq_ctx {
                                    // 2 = for the walks in the reference's order only
   int packet = 0;                  // nearest rays: 1 = one wave-coherent walk per 64 rays (packet.hip)
   int lookahead = 1;               // pooled walks in the reference's order: 1 = look-ahead form (walk_step_spec)
+  int lookahead_pause = 40;        // descend_min of the look-ahead walk
   int hit_mode = SQ_HITS_REFERENCE;   // SQ_HITS_REFERENCE or SQ_HITS_GUARDED (sq.h)
   bool timed = false;
   // {next chunk, retired waves} of the pooled single-call launches: zero at

@@ -159,16 +159,17 @@ class Context:
         check(lib().sq_ctx_set_block(self._h, threads))
 
     def set_tuning(self, block: int = 128, rays_per_lane: int = 16, lds_stack: int = 8, ray_order: int = 0,
-                   descend_min: int = 24, split_depth: int = 0, wide: int = 2, packet: int = 0, lookahead: int = 1):
+                   descend_min: int = 24, split_depth: int = 0, wide: int = 2, packet: int = 0, lookahead: int = 1,
+                   lookahead_pause: int = 40):
         check(lib().sq_ctx_set_tuning(self._h, C.byref(Tuning(block, rays_per_lane, lds_stack, ray_order, descend_min,
-                                                              split_depth, wide, packet, lookahead))))
+                                                              split_depth, wide, packet, lookahead, lookahead_pause))))
 
     def tuning(self) -> dict:
         t = Tuning()
         check(lib().sq_ctx_get_tuning(self._h, C.byref(t)))
         return dict(block=t.block, rays_per_lane=t.rays_per_lane, lds_stack=t.lds_stack, ray_order=t.ray_order,
                     descend_min=t.descend_min, split_depth=t.split_depth, wide=t.wide, packet=t.packet,
-                    lookahead=t.lookahead)
+                    lookahead=t.lookahead, lookahead_pause=t.lookahead_pause)
 
     def set_hit_mode(self, mode: int):
         """SQ_HITS_REFERENCE (default: the reference's hits for every query) or

@@ -36,7 +36,7 @@ class BatchQuery(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("block", C.c_int), ("rays_per_lane", C.c_int), ("lds_stack", C.c_int), ("ray_order", C.c_int),
                 ("descend_min", C.c_int), ("split_depth", C.c_int), ("wide", C.c_int), ("packet", C.c_int),
-                ("lookahead", C.c_int)]
+                ("lookahead", C.c_int), ("lookahead_pause", C.c_int)]
 
 
 P, U64, I64, F32, I32, U32 = C.c_void_p, C.c_uint64, C.c_int64, C.c_float, C.c_int, C.c_uint32

@@ -488,7 +488,9 @@ def test_edge_cases(ctx):
                                     dict(block=128, rays_per_lane=16, lds_stack=8, wide=0),
                                     dict(block=64, rays_per_lane=2, lds_stack=2, wide=0),
                                     dict(block=128, rays_per_lane=16, lds_stack=8, wide=2, lookahead=1),
-                                    dict(block=64, rays_per_lane=2, lds_stack=2, wide=0, lookahead=1)])
+                                    dict(block=64, rays_per_lane=2, lds_stack=2, wide=0, lookahead=1),
+                                    dict(block=128, rays_per_lane=16, lds_stack=8, wide=2, lookahead=1, lookahead_pause=0),
+                                    dict(block=128, rays_per_lane=4, lds_stack=4, wide=2, lookahead=1, lookahead_pause=64)])
 def test_tuning_does_not_change_results(ctx, tuning):
     """Every kernel variant (one query per thread; pooled with LDS rings that
     spill constantly or never; rays walked in Morton order) gives the same bits."""
