@@ -670,6 +670,34 @@ int sq_tlas64_shapecast_nearest(sq_tlas64* tlas, const double* casts, uint64_t n
 int sq_tlas64_shapecast_all(sq_tlas64* tlas, const double* casts, uint64_t n, double max_toi, uint64_t* offsets,
                             int32_t* out_inst, double* out_toi, uint64_t capacity, uint64_t* total);
 
+/* Consumer narrow phases in double: any QueryGeometry provider under the f64
+ * feature (geometry.rs:33-43 test_ray with Scalar = f64), as sq_narrow_create
+ * above does it for FP32.  The source defines
+ *   __device__ bool sq_narrow64(const double* prim, const double* origin,
+ *                               const double* dir, double max_toi, double* toi);
+ * over prim_doubles doubles per primitive (SQ_RTC_D, SQ_RTC_PF as above) and is
+ * compiled at run time (hipRTC) into the pooled f64 walk.  A hit counts in
+ * [0, max_toi].  Nearest queries return the reference's answer (its LIFO order
+ * with the consumer's unguarded hits); under SQ_HITS_GUARDED a hit is kept only
+ * inside the slab interval of its world box and the walk is near-first.  All
+ * hits are every hit in both modes (CSR as sq_bvh64_raycast_all). */
+typedef struct sq_narrow64 sq_narrow64;
+int sq_narrow64_create(sq_ctx* ctx, int dim, const char* source, uint32_t prim_doubles, sq_narrow64** out);
+int sq_narrow64_destroy(sq_narrow64* narrow);
+/* The f64 tree over n consumer primitives: host world boxes double[n][2*dim]
+ * (QueryGeometry::world_aabb, the median split's input) and host
+ * double[n][prim_doubles] (what sq_narrow64 sees).  Only the consumer entries
+ * below answer it (the sq_bvh64 queries refuse it; stats and destroy as any
+ * sq_bvh64). */
+int sq_bvh64_build_consumer(sq_narrow64* narrow, const double* boxes, const double* prims, uint64_t n, sq_bvh64** out);
+/* Bvh::raycast_nearest_batch (bvh.rs:134-193) with the consumer's test_ray, in
+ * double: rays double[n][2*dim], out_leaf -1 and out_toi +inf on a miss. */
+int sq_raycast64_nearest_consumer(sq_narrow64* narrow, sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi,
+                                  int32_t* out_leaf, double* out_toi);
+/* Bvh::raycast_all (bvh.rs:196-259) with the consumer's test_ray, in double. */
+int sq_raycast64_all_consumer(sq_narrow64* narrow, sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi,
+                              uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total);
+
 #ifdef __cplusplus
 }
 #endif

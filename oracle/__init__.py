@@ -82,6 +82,7 @@ def _declare(L):
         "orc64_raycast_all": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, I32, P, P, P, U64]),
         "orc64_overlap": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, I32, P, P, U64]),
         "orc64_tree_stats": (None, [I32, I32, P, U32, P, P]),
+        "orc64_consumer_query": (U64, [I32, I32, P, U32, I32, U64, P, U64, C.c_double, I32, I32, P, P, P, U64]),
         "orc64_bvh_raycast_nearest": (None, [I32, I32, P, U32, P, U64, C.c_double, I32, P, P, P]),
         "orc64_tlas_box_nearest": (None, [I32, U64, P, P, P, P, U64, C.c_double, I32, I32, P, P, P]),
         "orc64_tlas_query": (U64, [I32, I32, U64, P, P, P, P, U64, C.c_double, I32, I32, I32, P, P, P, U64]),
@@ -413,6 +414,51 @@ class Scene64:
         lib().orc64_tree_stats(self.dim, self.kind, _p(self.prims), self.n, _p(out), _p(q))
         return dict(node_count=int(out[0]), leaf_count=int(out[1]), prim_count=int(out[2]), max_depth=int(out[3]),
                     bytes=int(out[4]), quality=float(q[0]))
+
+
+class Consumer64:
+    """A consumer provider under the `f64` feature: world boxes [n][2*dim] and
+    the consumer's doubles [n][pd], its test_ray a C function at narrow_addr
+    (int f(const double* prim, const double* origin, const double* dir,
+    double max_toi, double* toi), the consumer's source built for the CPU).
+    Nearest hits walk the reference's Bvh (bvh.rs:51-124, LIFO) or, with
+    naive=True, scan every primitive (naive.rs); all hits as raycast_all
+    (bvh.rs:196-259).  A hit counts in [0, max_toi] (geometry.rs:37-39);
+    guarded=True keeps it only inside its box's slab interval (the engine's
+    SQ_HITS_GUARDED)."""
+
+    def __init__(self, dim: int, boxes, prims, narrow_addr: int, guarded: bool = False, naive: bool = False):
+        self.dim = dim
+        bx = np.ascontiguousarray(boxes, np.float64).reshape(-1, 2 * dim)
+        pr = np.ascontiguousarray(prims, np.float64).reshape(bx.shape[0], -1)
+        self.pd = pr.shape[1]
+        self.records = np.ascontiguousarray(np.concatenate([bx, pr], 1))
+        self.n = bx.shape[0]
+        self.addr = int(narrow_addr)
+        self._flags = (F64_GUARDED if guarded else 0) | (0 if naive else F64_BVH)
+
+    def _q(self, q):
+        return np.ascontiguousarray(q, np.float64).reshape(-1, 2 * self.dim)
+
+    def raycast_nearest(self, rays, max_toi, threads=8):
+        r = self._q(rays)
+        n = r.shape[0]
+        leaf = np.empty(n, np.int64); t = np.empty(n)
+        lib().orc64_consumer_query(self.dim, self._flags, _p(self.records), self.n, self.pd, self.addr, _p(r), n,
+                                   float(max_toi), 0, threads, _p(leaf), _p(t), None, 0)
+        return dict(leaf=leaf, t=t)
+
+    def raycast_all(self, rays, max_toi, threads=8):
+        r = self._q(rays)
+        n = r.shape[0]
+        off = np.empty(n + 1, np.uint64)
+        call = lambda leaf, t, cap: lib().orc64_consumer_query(self.dim, self._flags, _p(self.records), self.n, self.pd,
+                                                               self.addr, _p(r), n, float(max_toi), 1, threads, _p(leaf),
+                                                               _p(t), _p(off), cap)
+        tot = call(None, None, 0)
+        leaf = np.empty(max(tot, 1), np.int64); t = np.empty(max(tot, 1))
+        call(leaf, t, tot)
+        return off, leaf[:tot], t[:tot]
 
 
 class BoxInstances64:

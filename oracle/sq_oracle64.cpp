@@ -30,7 +30,9 @@
 //                                  instance by test_ray_crossings_local's default), :306-391
 //                                  (shape casts), :393-432 (overlap) over box and ball
 //                                  instances: the expected query results
-// over the built-in BOX and SPHERE providers.  A `kind` with FLAG_GUARDED set
+// over the built-in BOX and SPHERE providers, and over a consumer provider
+// (QueryGeometry::test_ray of geometry.rs:33-43 in double, a C function built
+// from the consumer's source: orc64_consumer_query).  A `kind` with FLAG_GUARDED set
 // restates the engine's SQ_HITS_GUARDED mode instead (a sphere or instance hit
 // kept only inside the slab interval of its own box, so that BVH == naive scan,
 // the reference's stated contract, sq/src/lib.rs:15-17, holds exactly); with
@@ -83,19 +85,26 @@ template <int D> static double dot(const double* a, const double* b) {
   return acc;
 }
 
+// A consumer's test_ray in double (geometry.rs:33-43): nonzero with *toi on a
+// hit of the primitive's doubles.
+typedef int (*ConsumerNarrow64)(const double* prim, const double* origin, const double* dir, double max_toi, double* toi);
+
 template <int D> struct Geom {
-  int kind;   // 0 BOX [min D][max D], 1 SPHERE [c D][r]
+  int kind;   // 0 BOX [min D][max D], 1 SPHERE [c D][r], 2 CONSUMER [world box 2D][consumer_pd doubles]
   uint32_t n;
   const double* prims;
   const uint64_t* layers;
-  bool guarded = false;   // sphere hits and contacts kept inside their box's slab interval
-  int stride() const { return kind == 0 ? 2 * D : D + 1; }
+  bool guarded = false;   // sphere and consumer hits (and contacts) kept inside their box's slab interval
+  ConsumerNarrow64 consumer = nullptr;
+  int consumer_pd = 0;
+  int stride() const { return kind == 0 ? 2 * D : kind == 1 ? D + 1 : 2 * D + consumer_pd; }
   const double* p(size_t leaf) const { return prims + leaf * stride(); }
-  // QueryGeometry::world_aabb: the box, or Aabb::new(c - splat(r), c + splat(r))
+  // QueryGeometry::world_aabb: the box (BOX, and the consumer's given box), or
+  // Aabb::new(c - splat(r), c + splat(r))
   void world_aabb(size_t leaf, double* mn, double* mx) const {
     const double* q = p(leaf);
     for (int i = 0; i < D; ++i) {
-      if (kind == 0) { mn[i] = q[i]; mx[i] = q[D + i]; }
+      if (kind == 0 || kind == 2) { mn[i] = q[i]; mx[i] = q[D + i]; }
       else { mn[i] = q[i] - q[D]; mx[i] = q[i] + q[D]; }
     }
   }
@@ -112,6 +121,17 @@ template <int D> struct Geom {
     double mn[D], mx[D];
     world_aabb(leaf, mn, mx);
     double te, tx; int axis;
+    if (kind == 2) {
+      // the consumer's hit, counted in [0, max_toi] (the test_ray contract, geometry.rs:37-39)
+      double t;
+      if (!consumer(q + 2 * D, r.o, r.d, max_toi, &t)) return false;
+      if (!(t >= 0.0 && t <= max_toi)) return false;
+      if (guarded && !ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &axis)) return false;
+      if (guarded && !(te <= t && t <= tx)) return false;
+      *t_out = t;
+      for (int i = 0; i < D; ++i) normal[i] = 0.0;
+      return true;
+    }
     if (kind == 0) {
       if (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &axis)) return false;
       *t_out = te;
@@ -510,8 +530,8 @@ static bool bvh_cast_nearest(const Tree64<D>& tree, const Geom<D>& g, const Ray&
 template <int D>
 static void nearest_batch(int kindf, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask, int64_t exclude,
                           const double* rays, uint64_t nr, double max_toi, const double* tmax, int threads, int64_t* leaf,
-                          double* t, double* point, double* normal) {
-  const Geom<D> g = geom_of<D>(kindf, n, prims, layers);
+                          double* t, double* point, double* normal, const Geom<D>* given = nullptr) {
+  const Geom<D> g = given ? *given : geom_of<D>(kindf, n, prims, layers);
   const bool bvh = (kindf & FLAG_BVH) != 0;
   const Tree64<D> tree(bvh ? g : Geom<D>{g.kind, 0, prims, layers});
   parallel_for(nr, threads, [&](uint64_t i) {
@@ -555,8 +575,8 @@ static void cast_nearest_batch(int kindf, const double* prims, uint32_t n, const
 template <int D>
 static uint64_t all_batch(int kindf, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask, int64_t exclude,
                           const double* q, uint64_t nq, double max_toi, const double* tmax, int mode, int threads,
-                          uint64_t* offsets, int64_t* leaf, double* t, uint64_t cap) {
-  const Geom<D> g = geom_of<D>(kindf, n, prims, layers);
+                          uint64_t* offsets, int64_t* leaf, double* t, uint64_t cap, const Geom<D>* given = nullptr) {
+  const Geom<D> g = given ? *given : geom_of<D>(kindf, n, prims, layers);
   const bool bvh = (kindf & FLAG_BVH) != 0;
   const Tree64<D> tree(bvh ? g : Geom<D>{g.kind, 0, prims, layers});
   std::vector<std::vector<std::pair<double, int64_t>>> res(nq);
@@ -924,6 +944,26 @@ using namespace orc64;
     default: break;                            \
   }
 
+// A consumer provider's queries (records [n][2D + pd]: world box, then the
+// consumer's doubles): mode 0 nearest (FLAG_BVH: the reference's LIFO walk,
+// else the naive scan), mode 1 all hits.  FLAG_GUARDED: hits kept inside
+// their box's slab interval.
+template <int D>
+static uint64_t consumer_query(int flags, const double* records, uint32_t n, int pd, uint64_t fn, const double* rays,
+                               uint64_t nr, double max_toi, int mode, int threads, int64_t* leaf, double* t,
+                               uint64_t* offsets, uint64_t cap) {
+  Geom<D> g{2, n, records, nullptr};
+  g.guarded = (flags & FLAG_GUARDED) != 0;
+  g.consumer = reinterpret_cast<ConsumerNarrow64>(fn);
+  g.consumer_pd = pd;
+  const int kindf = 2 | (flags & FLAG_BVH);
+  if (mode == 0) {
+    nearest_batch<D>(kindf, records, n, nullptr, ~0ull, -1, rays, nr, max_toi, nullptr, threads, leaf, t, nullptr, nullptr, &g);
+    return 0;
+  }
+  return all_batch<D>(kindf, records, n, nullptr, ~0ull, -1, rays, nr, max_toi, nullptr, 0, threads, offsets, leaf, t, cap, &g);
+}
+
 extern "C" {
 
 void orc64_raycast_nearest(int D, int kind, const double* prims, uint32_t n, const uint64_t* layers, uint64_t mask,
@@ -980,6 +1020,14 @@ uint64_t orc64_tlas_query(int D, int kind, uint64_t n, const double* rot, const 
                           int64_t* leaf, double* t, uint64_t cap) {
   uint64_t tot = 0;
   ORC64_DISPATCH(D, tot = tlas_query<DD>(kind, n, rot, trans, half, q, nq, max_toi, mode, threads, flags, offsets, leaf, t, cap));
+  return tot;
+}
+
+uint64_t orc64_consumer_query(int D, int flags, const double* records, uint32_t n, int pd, uint64_t fn, const double* rays,
+                              uint64_t nr, double max_toi, int mode, int threads, int64_t* leaf, double* t, uint64_t* offsets,
+                              uint64_t cap) {
+  uint64_t tot = 0;
+  ORC64_DISPATCH(D, tot = consumer_query<DD>(flags, records, n, pd, fn, rays, nr, max_toi, mode, threads, leaf, t, offsets, cap));
   return tot;
 }
 

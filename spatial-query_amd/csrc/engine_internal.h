@@ -116,6 +116,9 @@ struct sq_bvh {
 // Exclusive scan of n counts into out[0..n] (out[n] = total) on the context's
 // stream; temps keeps the scan's temporaries alive until the caller is done.
 int scan_counts(sq_ctx* c, const uint32_t* in, uint64_t n, uint64_t* out, std::vector<std::unique_ptr<DevBuf>>& temps);
+// A consumer's source compiled at run time into a loaded module (rtc.hip).
+int rtc_load_module(sq_ctx* ctx, const char* header, const char* source, const char* kernels, int dim, uint32_t pf,
+                    hipModule_t* out);
 
 // TreeStats of a host reference tree (tree.rs:81-110, bvh.rs:477-489).
 void tree_stats_of(const sqh::RefTree& t, sq_tree_stats& st);

@@ -50,7 +50,7 @@
 
 #include "engine_internal.h"
 #include "pool.h"
-#include "rtc_sources.inc"    // kTraverseSrc, kPoolSrc: traverse.h and pool.h as string literals (built by the Makefile)
+#include "rtc_sources.inc"    // kTraverseSrc, kPoolSrc, kWalk64Src: traverse.h, pool.h, walk64.h as string literals (built by the Makefile)
 
 namespace {
 
@@ -385,21 +385,21 @@ static int module_blocks_per_cu(sq_narrow* nw, hipFunction_t fn, uint32_t block,
   return SQ_OK;
 }
 
-extern "C" {
-
-int sq_narrow_create(sq_ctx* ctx, int dim, const char* source, uint32_t prim_floats, sq_narrow** out) {
-  if (!ctx || !source || !out) return fail(SQ_ERR_INVALID, "NULL argument");
-  *out = nullptr;
-  if (dim < 2 || dim > 6) return fail(SQ_ERR_UNSUPPORTED, "consumer narrow phases take dim 2..6");
-  if (prim_floats > 256) return fail(SQ_ERR_INVALID, "at most 256 floats per primitive");
+// Compiles a consumer's source for gfx950 (hipRTC) after `#include <header>`
+// (one of the embedded traverse.h, pool.h, walk64.h) and before `kernels`,
+// with SQ_RTC_D = dim and SQ_RTC_PF = pf, and loads the module on the
+// context's device.  A source that does not compile is SQ_ERR_INVALID with the
+// compiler's log.
+int rtc_load_module(sq_ctx* ctx, const char* header, const char* source, const char* kernels, int dim, uint32_t pf,
+                    hipModule_t* out) {
   HIP_TRY(hipSetDevice(ctx->device));
-  const std::string src = std::string("#include \"pool.h\"\n#line 1 \"consumer\"\n") + source + "\n" + kKernelSrc;
-  const char* headers[] = {kTraverseSrc, kPoolSrc};
-  const char* names[] = {"traverse.h", "pool.h"};
+  const std::string src = std::string("#include \"") + header + "\"\n#line 1 \"consumer\"\n" + source + "\n" + kernels;
+  const char* headers[] = {kTraverseSrc, kPoolSrc, kWalk64Src};
+  const char* names[] = {"traverse.h", "pool.h", "walk64.h"};
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "consumer_narrow.hip", 2, headers, names) != HIPRTC_SUCCESS)
+  if (hiprtcCreateProgram(&prog, src.c_str(), "consumer_narrow.hip", 3, headers, names) != HIPRTC_SUCCESS)
     return fail(SQ_ERR_HIP, "hiprtcCreateProgram failed");
-  const std::string dd = "-DSQ_RTC_D=" + std::to_string(dim), pp = "-DSQ_RTC_PF=" + std::to_string(prim_floats);
+  const std::string dd = "-DSQ_RTC_D=" + std::to_string(dim), pp = "-DSQ_RTC_PF=" + std::to_string(pf);
   // the engine's own flags: no contraction, f32 denormals, no packed FP32
   const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++20", "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
                         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", dd.c_str(), pp.c_str()};
@@ -417,11 +417,22 @@ int sq_narrow_create(sq_ctx* ctx, int dim, const char* source, uint32_t prim_flo
   std::vector<char> code(code_n);
   hiprtcGetCode(prog, code.data());
   hiprtcDestroyProgram(&prog);
+  HIP_TRY(hipModuleLoadData(out, code.data()));
+  return SQ_OK;
+}
+
+extern "C" {
+
+int sq_narrow_create(sq_ctx* ctx, int dim, const char* source, uint32_t prim_floats, sq_narrow** out) {
+  if (!ctx || !source || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (dim < 2 || dim > 6) return fail(SQ_ERR_UNSUPPORTED, "consumer narrow phases take dim 2..6");
+  if (prim_floats > 256) return fail(SQ_ERR_INVALID, "at most 256 floats per primitive");
   auto nw = std::make_unique<sq_narrow>();
   nw->ctx = ctx;
   nw->dim = dim;
   nw->pf = prim_floats;
-  HIP_TRY(hipModuleLoadData(&nw->mod, code.data()));
+  if (int rc = rtc_load_module(ctx, "pool.h", source, kKernelSrc, dim, prim_floats, &nw->mod)) return rc;
   if (hipModuleGetFunction(&nw->fn, nw->mod, "sq_consumer_nearest") != hipSuccess ||
       hipModuleGetFunction(&nw->fn_all, nw->mod, "sq_consumer_all") != hipSuccess ||
       hipModuleGetFunction(&nw->fn_cross, nw->mod, "sq_consumer_crossings") != hipSuccess ||

@@ -12,9 +12,9 @@ raises when it is missing: there is no CPU fallback.
 """
 from .capi import (SQ_HITS_GUARDED, SQ_HITS_REFERENCE, SQ_PRIM_AABB, SQ_PRIM_SPHERE, SQ_PRIM_TRIANGLE, SqError, lib,
                    library_path)
-from .api import (BatchSet, ConsumerBvh, Context, DeviceBvh, DeviceBvh64, Narrow, Proposal, QueryFilter, Tlas, Tlas64, Tree, mfma_gemm, prim_stride)
+from .api import (BatchSet, ConsumerBvh, ConsumerBvh64, Context, DeviceBvh, DeviceBvh64, Narrow, Narrow64, Proposal, QueryFilter, Tlas, Tlas64, Tree, mfma_gemm, prim_stride)
 
 AABB, SPHERE, TRIANGLE = SQ_PRIM_AABB, SQ_PRIM_SPHERE, SQ_PRIM_TRIANGLE
 
-__all__ = ["BatchSet", "ConsumerBvh", "Context", "DeviceBvh", "DeviceBvh64", "Narrow", "Proposal", "mfma_gemm", "Tlas", "Tlas64", "QueryFilter", "Tree", "SqError", "lib", "library_path", "prim_stride",
+__all__ = ["BatchSet", "ConsumerBvh", "ConsumerBvh64", "Context", "DeviceBvh", "DeviceBvh64", "Narrow", "Narrow64", "Proposal", "mfma_gemm", "Tlas", "Tlas64", "QueryFilter", "Tree", "SqError", "lib", "library_path", "prim_stride",
            "AABB", "SPHERE", "TRIANGLE", "SQ_HITS_REFERENCE", "SQ_HITS_GUARDED"]

@@ -871,6 +871,85 @@ class ConsumerBvh:
         _finalize(self)
 
 
+class Narrow64:
+    """A consumer narrow phase in double (the f64 feature's QueryGeometry::
+    test_ray, geometry.rs:33-43): HIP source defining
+
+        __device__ bool sq_narrow64(const double* prim, const double* origin,
+                                    const double* dir, double max_toi, double* toi);
+
+    compiled at run time (hipRTC) into the pooled f64 walk, for primitives of
+    prim_doubles doubles in dim dimensions (SQ_RTC_D and SQ_RTC_PF as for
+    Narrow).  Raises SqError(INVALID) with the compiler log when it does not
+    compile."""
+
+    def __init__(self, ctx: Context, dim: int, source: str, prim_doubles: int):
+        h = C.c_void_p()
+        check(lib().sq_narrow64_create(ctx.handle, dim, source.encode(), prim_doubles, C.byref(h)))
+        self._h, self.ctx, self.dim, self.prim_doubles = h, ctx, dim, prim_doubles
+        _track(self)
+
+    def build(self, boxes, prims) -> "ConsumerBvh64":
+        """The f64 tree over n primitives: world boxes [n][2*dim] (host) drive
+        the reference median split; prims [n][prim_doubles] are what
+        sq_narrow64 sees."""
+        bx = _host(boxes, np.float64).reshape(-1, 2 * self.dim)
+        pr = (_host(prims, np.float64).reshape(bx.shape[0], self.prim_doubles) if self.prim_doubles
+              else np.zeros((bx.shape[0], 0), np.float64))
+        h = C.c_void_p()
+        check(lib().sq_bvh64_build_consumer(self._h, _hp(bx), _hp(pr), bx.shape[0], C.byref(h)))
+        return ConsumerBvh64(self, h, bx.shape[0])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_narrow64_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        _finalize(self)
+
+
+class ConsumerBvh64:
+    """An f64 tree over consumer primitives; answers float64 rays through its
+    Narrow64: nearest hits and all hits."""
+
+    def __init__(self, narrow: Narrow64, handle, n: int):
+        self.narrow, self._h, self.n_prims, self.dim = narrow, handle, n, narrow.dim
+        _track(self)
+
+    def stats(self) -> dict:
+        s = TreeStats()
+        check(lib().sq_bvh64_stats(self._h, C.byref(s)))
+        return _stats(s)
+
+    def raycast_nearest(self, rays, max_toi):
+        import torch
+        n = rays.shape[0]
+        leaf = torch.empty(n, dtype=torch.int32, device=rays.device)
+        toi = torch.empty(n, dtype=torch.float64, device=rays.device)
+        check(lib().sq_raycast64_nearest_consumer(self.narrow._h, self._h, _dp(rays, "float64", 2 * self.dim), n, float(max_toi),
+                                                  _dp(leaf, "int32"), _dp(toi, "float64")))
+        return {"leaf": leaf, "toi": toi}
+
+    _csr = DeviceBvh64._csr
+
+    def raycast_all(self, rays, max_toi, capacity=0):
+        """CSR (offsets, leaf, toi), each ray's hits sorted by (toi, leaf)."""
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_raycast64_all_consumer(self.narrow._h, self._h, _dp(rays, "float64", 2 * self.dim), rays.shape[0],
+                                                   float(max_toi), _dp(off, "int64"), _dp(leaf, "int32"), _dp(toi, "float64"), cap,
+                                                   C.byref(total))
+        return self._csr(fn, rays.shape[0], rays.device, True, capacity)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sq_bvh64_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        _finalize(self)
+
+
 class BatchSet:
     """Many nearest-hit batches over many scenes (all DeviceBvh, or all Tlas, of one
     dimension and kind) answered by one persistent work-stealing launch per submit().

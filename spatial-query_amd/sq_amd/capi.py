@@ -133,6 +133,11 @@ SIGNATURES = {
     "sq_bvh64_raycast_any": [P, P, U64, F64, P, C.POINTER(Filter), P],
     "sq_bvh64_raycast_all": [P, P, U64, F64, C.POINTER(Filter), P, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_bvh64_overlap": [P, P, U64, C.POINTER(Filter), P, P, U64, C.POINTER(C.c_uint64)],
+    "sq_narrow64_create": [P, I32, C.c_char_p, U32, PP],
+    "sq_narrow64_destroy": [P],
+    "sq_bvh64_build_consumer": [P, P, P, U64, PP],
+    "sq_raycast64_nearest_consumer": [P, P, P, U64, F64, P, P],
+    "sq_raycast64_all_consumer": [P, P, P, U64, F64, P, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_tlas64_create": [P, I32, U64, P, P, P, PP],
     "sq_tlas64_create_kind": [P, I32, I32, U64, P, P, P, PP],
     "sq_tlas64_raycast_all": [P, P, U64, F64, P, P, P, U64, P],

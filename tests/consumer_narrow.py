@@ -235,3 +235,61 @@ def compile_cpu(source: str, dim: int, pf: int, workdir: str):
     """As cpu_entry_points: (library, address of its narrow entry point)."""
     lib, addr, _ = cpu_entry_points(source, dim, pf, workdir)
     return lib, addr
+
+
+# Consumer narrow phases in double (sq_narrow64, the f64 feature).  BALL64 is
+# the built-in f64 SPHERE narrow phase in the same operation order
+# (differential.rs:61-82); ELLIPSOID64 the axis-aligned ellipsoid above in double.
+BALL64 = r"""
+__device__ bool sq_narrow64(const double* p, const double* o, const double* d, double max_toi, double* toi) {
+  double oc[SQ_RTC_D];
+  for (int k = 0; k < SQ_RTC_D; ++k) oc[k] = o[k] - p[k];
+  double b = 0.0, cc = 0.0;
+  for (int k = 0; k < SQ_RTC_D; ++k) b = b + oc[k] * d[k];
+  for (int k = 0; k < SQ_RTC_D; ++k) cc = cc + oc[k] * oc[k];
+  cc = cc - p[SQ_RTC_D] * p[SQ_RTC_D];
+  const double disc = b * b - cc;
+  if (disc < 0.0) return false;
+  const double sq = __builtin_sqrt(disc);
+  double t = -b - sq;
+  if (t < 0.0) t = -b + sq;
+  if (t < 0.0 || t > max_toi) return false;
+  *toi = t;
+  return true;
+}
+"""
+
+ELLIPSOID64 = r"""
+__device__ bool sq_narrow64(const double* p, const double* o, const double* d, double max_toi, double* toi) {
+  double u[3], v[3];
+  for (int k = 0; k < 3; ++k) { u[k] = (o[k] - p[k]) / p[3 + k]; v[k] = d[k] / p[3 + k]; }
+  const double A = ((0.0 + v[0] * v[0]) + v[1] * v[1]) + v[2] * v[2];
+  const double B = ((0.0 + u[0] * v[0]) + u[1] * v[1]) + u[2] * v[2];
+  const double C = (((0.0 + u[0] * u[0]) + u[1] * u[1]) + u[2] * u[2]) - 1.0;
+  const double disc = B * B - A * C;
+  if (disc < 0.0) return false;
+  const double s = __builtin_sqrt(disc);
+  double t = (-B - s) / A;
+  if (t < 0.0) t = (-B + s) / A;
+  if (t < 0.0 || t > max_toi) return false;
+  *toi = t;
+  return true;
+}
+"""
+
+
+def compile_cpu64(source: str, dim: int, pd: int, workdir: str):
+    """An f64 consumer source built for the CPU (g++ -O2 -ffp-contract=off);
+    returns (library, address of its int sq_narrow64_c(...) entry point) for
+    oracle.Consumer64."""
+    code = (f"#define __device__\n#define SQ_RTC_D {dim}\n#define SQ_RTC_PF {pd}\n" + source +
+            '\nextern "C" int sq_narrow64_c(const double* p, const double* o, const double* d, double m, double* t) '
+            "{ return sq_narrow64(p, o, d, m, t) ? 1 : 0; }\n")
+    tag = hashlib.sha1(code.encode()).hexdigest()[:12]
+    src = os.path.join(workdir, f"narrow64_{tag}.cpp")
+    so = os.path.join(workdir, f"narrow64_{tag}.so")
+    with open(src, "w") as f:
+        f.write(code)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-o", so, src], check=True)
+    lib = ctypes.CDLL(so)
+    return lib, ctypes.cast(lib.sq_narrow64_c, ctypes.c_void_p).value

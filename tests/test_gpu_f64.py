@@ -9,7 +9,12 @@ of sq/src/query/naive.rs in double):
   * a 262,144-ball scene with 1M rays: the oracle on a sample, and the
     size-independent properties on every ray (any == nearest hit, the first
     of a ray's all-hits list is its nearest hit, lists sorted);
-  * refusals (triangles, D = 7, negative radii) and the empty tree.
+  * refusals (triangles, D = 7, negative radii) and the empty tree;
+  * consumer narrow phases in double (sq_narrow64, compiled at run time): the
+    built-in ball's arithmetic equals the f64 SPHERE kind and the oracle's
+    consumer kind bit for bit in both hit modes; balls and ellipsoids with
+    shrunk boxes equal the oracle's reference-order walk and guarded walk;
+    compile errors, refusals and the empty tree.
 """
 import numpy as np
 import pytest
@@ -457,3 +462,129 @@ def test_f64_tlas_ball_instances(ctx, rotated):
         _csr_equal64(tl.overlap(dev(regions)), balls.overlap(regions), with_t=False)
         _csr_equal64(tl.shapecast_all(dev(casts), 300.0), balls.shapecast_all(casts, 300.0))
     tl.close()
+
+
+# --- consumer narrow phases in double (sq_narrow64) --------------------------
+
+def _consumer_scene(dim, n, seed, shrink=1.0):
+    rng = np.random.default_rng(seed)
+    span = {2: 60.0, 3: 30.0, 5: 9.0}[dim]
+    c = rng.uniform(-span, span, (n, dim))
+    r = rng.uniform(0.3, 3.0, (n, 1))
+    o = rng.uniform(-span - 5.0, span + 5.0, (2 * n, dim))
+    d = rng.normal(size=o.shape)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([c, r], 1), np.concatenate([o, d], 1), np.concatenate([c - shrink * r, c + shrink * r], 1)
+
+
+def _same_nearest64(got, leaf, t, what):
+    assert np.array_equal(host(got["leaf"]).astype(np.int64), np.asarray(leaf).astype(np.int64)), f"{what}: ids"
+    assert np.array_equal(host(got["toi"]).view(np.uint64), np.asarray(t).view(np.uint64)), f"{what}: toi bits"
+
+
+@pytest.mark.parametrize("dim", [3, 5])
+def test_f64_consumer_ball_equals_builtin_sphere(ctx, tmp_path, dim):
+    """The built-in ball's arithmetic as an f64 consumer (hipRTC into the
+    pooled f64 walk) equals the engine's f64 SPHERE kind and the oracle's f64
+    consumer kind (same source built by g++) bit for bit: nearest hits in both
+    hit modes, all hits."""
+    from consumer_narrow import BALL64, compile_cpu64
+    print(f"case: f64 consumer ball D={dim}", flush=True)
+    balls, rays, boxes = _consumer_scene(dim, 20000, 0xE0 + dim)
+    nw = sq_amd.Narrow64(ctx, dim, BALL64, dim + 1)
+    cb = nw.build(boxes, balls)
+    builtin = sq_amd.DeviceBvh64(ctx, dim, sq_amd.SPHERE, balls)
+    _, addr = compile_cpu64(BALL64, dim, dim + 1, str(tmp_path))
+    saved = ctx.hit_mode()
+    try:
+        for mode, guarded in ((sq_amd.SQ_HITS_REFERENCE, False), (sq_amd.SQ_HITS_GUARDED, True)):
+            ctx.set_hit_mode(mode)
+            got = cb.raycast_nearest(dev(rays), 200.0)
+            want = builtin.raycast_nearest(dev(rays), 200.0)
+            _same_nearest64(got, host(want["leaf"]), host(want["toi"]), f"builtin, guarded={guarded}")
+            ref = oracle.Consumer64(dim, boxes, balls, addr, guarded=guarded).raycast_nearest(rays, 200.0)
+            assert (ref["leaf"] >= 0).sum() > 5000
+            _same_nearest64(got, ref["leaf"], ref["t"], f"oracle, guarded={guarded}")
+            ga = cb.raycast_all(dev(rays), 200.0)
+            _csr_equal64(ga, tuple(host(x) for x in builtin.raycast_all(dev(rays), 200.0)))
+            _csr_equal64(ga, oracle.Consumer64(dim, boxes, balls, addr).raycast_all(rays, 200.0))
+        assert cb.stats() == builtin.stats()
+    finally:
+        ctx.set_hit_mode(saved)
+        cb.close()
+        builtin.close()
+        nw.close()
+
+
+@pytest.mark.parametrize("source", ["ball", "ellipsoid"])
+def test_f64_consumer_hits_outside_their_boxes(ctx, tmp_path, source):
+    """Providers whose hits leave their own world boxes (balls and ellipsoids
+    whose boxes are shrunk to 0.6 of their extent): the default mode returns
+    the reference's LIFO walk's answer (oracle f64 consumer kind, same source
+    by g++) bit for bit, which differs from the naive scan on many rays;
+    SQ_HITS_GUARDED returns the guarded oracle's; all hits equal the oracle's
+    walk at max_toi."""
+    from consumer_narrow import BALL64, ELLIPSOID64, compile_cpu64
+    print(f"case: f64 consumer {source} with shrunk boxes", flush=True)
+    dim = 3
+    balls, rays, boxes = _consumer_scene(dim, 20000, 0xE7, shrink=0.6)
+    if source == "ball":
+        src, prims, pd = BALL64, balls, 4
+    else:
+        rng = np.random.default_rng(0xE8)
+        semi = rng.uniform(0.3, 3.0, (balls.shape[0], 3))
+        c = balls[:, :3]
+        src, prims, pd = ELLIPSOID64, np.concatenate([c, semi], 1), 6
+        boxes = np.concatenate([c - 0.6 * semi, c + 0.6 * semi], 1)
+    nw = sq_amd.Narrow64(ctx, dim, src, pd)
+    cb = nw.build(boxes, prims)
+    _, addr = compile_cpu64(src, dim, pd, str(tmp_path))
+    saved = ctx.hit_mode()
+    try:
+        ref = oracle.Consumer64(dim, boxes, prims, addr).raycast_nearest(rays, 200.0)
+        naive = oracle.Consumer64(dim, boxes, prims, addr, naive=True).raycast_nearest(rays, 200.0)
+        assert (ref["leaf"] != naive["leaf"]).sum() > 50
+        ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
+        _same_nearest64(cb.raycast_nearest(dev(rays), 200.0), ref["leaf"], ref["t"], "reference order")
+        ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED)
+        g = oracle.Consumer64(dim, boxes, prims, addr, guarded=True).raycast_nearest(rays, 200.0)
+        _same_nearest64(cb.raycast_nearest(dev(rays), 200.0), g["leaf"], g["t"], "guarded")
+        _csr_equal64(cb.raycast_all(dev(rays), 200.0), oracle.Consumer64(dim, boxes, prims, addr).raycast_all(rays, 200.0))
+    finally:
+        ctx.set_hit_mode(saved)
+        cb.close()
+        nw.close()
+
+
+def test_f64_consumer_errors_and_empty_tree(ctx):
+    from consumer_narrow import BALL64, ELLIPSOID64
+    print("case: f64 consumer compile error, refusals, empty tree", flush=True)
+    with pytest.raises(sq_amd.SqError) as e:
+        sq_amd.Narrow64(ctx, 3, "__device__ bool sq_narrow64(const double* p) { return undefined_name; }", 4)
+    assert "error" in str(e.value)
+    nw = sq_amd.Narrow64(ctx, 3, BALL64, 4)
+    empty = nw.build(np.zeros((0, 6)), np.zeros((0, 4)))
+    rays = _consumer_scene(3, 100, 5)[1]
+    r = empty.raycast_nearest(dev(rays), 100.0)
+    assert (host(r["leaf"]) == -1).all() and np.isinf(host(r["toi"])).all()
+    off, leaf, _ = empty.raycast_all(dev(rays), 100.0)
+    assert (host(off) == 0).all() and leaf.numel() == 0
+    balls, rays, boxes = _consumer_scene(3, 50, 6)
+    with pytest.raises(sq_amd.SqError):
+        nw.build(np.concatenate([boxes[:, 3:], boxes[:, :3]], 1), balls)   # min > max
+    cb = nw.build(boxes, balls)
+    # a consumer tree answers only through its narrow phase
+    bogus = sq_amd.DeviceBvh64.__new__(sq_amd.DeviceBvh64)
+    bogus._h, bogus.ctx, bogus.dim, bogus.kind, bogus.n_prims = cb._h, ctx, 3, sq_amd.SPHERE, 50
+    with pytest.raises(sq_amd.SqError):
+        bogus.raycast_nearest(dev(rays), 100.0)
+    bogus._h = None
+    other = sq_amd.Narrow64(ctx, 3, ELLIPSOID64, 6)
+    wrong = sq_amd.ConsumerBvh64(other, cb._h, 50)        # the tree under another narrow phase
+    with pytest.raises(sq_amd.SqError):
+        wrong.raycast_nearest(dev(rays), 100.0)
+    wrong._h = None
+    cb.close()
+    empty.close()
+    other.close()
+    nw.close()

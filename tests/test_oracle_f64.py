@@ -11,7 +11,11 @@ feature, sq/src/maths/point.rs:14-18), on the CPU:
     float64 with the cancellation-free quadratic form): same nearest ids where
     the two nearest candidates are separated, toi within 1e-12 relative, and
     every nearest toi is a root (|at(t) - c| = r);
-  * all hits: sorted by (toi, leaf), and the first hit is the nearest.
+  * all hits: sorted by (toi, leaf), and the first hit is the nearest;
+  * the consumer kind (a test_ray in double through a C function): the
+    built-in ball's arithmetic as a consumer equals the SPHERE kind bit for bit
+    in every walk and mode; with shrunk boxes the reference's walk and the
+    naive scan differ, every answer is a true hit, and the guarded forms agree.
 """
 import numpy as np
 import pytest
@@ -486,3 +490,66 @@ def test_box_instance_queries_dyadic_equal_the_f32_oracle(dim):
     o64, l64 = s64.overlap(regions)
     o32, l32 = s32.overlap(regions.astype(np.float32))
     assert l64.size > 200 and np.array_equal(o64, o32) and np.array_equal(l64, l32)
+
+
+def _balls_and_rays(dim, n, seed):
+    rng = np.random.default_rng(seed)
+    span = {2: 60.0, 3: 30.0, 5: 9.0}[dim]
+    c = rng.uniform(-span, span, (n, dim))
+    r = rng.uniform(0.3, 3.0, (n, 1))
+    o = rng.uniform(-span - 5.0, span + 5.0, (2 * n, dim))
+    d = rng.normal(size=o.shape)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([c, r], 1), np.concatenate([o, d], 1)
+
+
+@pytest.mark.parametrize("dim", [2, 3, 5])
+def test_consumer64_ball_equals_the_builtin_sphere(tmp_path, dim):
+    """The oracle's f64 consumer kind running the built-in ball's arithmetic
+    (BALL64, built by g++) equals its SPHERE kind bit for bit: nearest hits in
+    the reference's walk and the naive scan, guarded and not, and all hits.  A
+    record offset, a dropped range check or a guard on the wrong box fails it."""
+    from consumer_narrow import BALL64, compile_cpu64
+    balls, rays = _balls_and_rays(dim, 3000, 40 + dim)
+    boxes = np.concatenate([balls[:, :dim] - balls[:, dim:], balls[:, :dim] + balls[:, dim:]], 1)
+    _, addr = compile_cpu64(BALL64, dim, dim + 1, str(tmp_path))
+    for guarded in (False, True):
+        for naive in (False, True):
+            cons = oracle.Consumer64(dim, boxes, balls, addr, guarded=guarded, naive=naive)
+            ref = oracle.Scene64(dim, oracle.SPHERE, balls, guarded=guarded, naive=naive)
+            a, b = cons.raycast_nearest(rays, 200.0), ref.raycast_nearest(rays, 200.0)
+            assert (b["leaf"] >= 0).sum() > 500
+            assert np.array_equal(a["leaf"], b["leaf"])
+            assert np.array_equal(a["t"].view(np.uint64), b["t"].view(np.uint64))
+            ca, cb = cons.raycast_all(rays, 200.0), ref.raycast_all(rays, 200.0)
+            for x, y in zip(ca, cb):
+                assert np.array_equal(np.asarray(x).view(np.uint64), np.asarray(y).view(np.uint64))
+
+
+def test_consumer64_hits_outside_their_boxes(tmp_path):
+    """Balls whose world boxes are shrunk to 0.6 of the radius (the reference
+    asks only that the ray meet the leaf box, geometry.rs:37-45): the boxes'
+    reject at the shrinking limit (the tree's nodes, the naive scan's cheap
+    reject, naive.rs:34-62) then depends on the order the leaves are met, so
+    the reference's walk and the naive scan answer differently on many rays,
+    each answer a true hit of its ball.  With the guard both keep only hits
+    inside their boxes and agree exactly."""
+    from consumer_narrow import BALL64, compile_cpu64
+    dim = 3
+    balls, rays = _balls_and_rays(dim, 3000, 77)
+    c, r = balls[:, :dim], balls[:, dim:]
+    boxes = np.concatenate([c - 0.6 * r, c + 0.6 * r], 1)
+    _, addr = compile_cpu64(BALL64, dim, dim + 1, str(tmp_path))
+    walk = oracle.Consumer64(dim, boxes, balls, addr).raycast_nearest(rays, 200.0)
+    naive = oracle.Consumer64(dim, boxes, balls, addr, naive=True).raycast_nearest(rays, 200.0)
+    assert (walk["leaf"] != naive["leaf"]).sum() > 50
+    for res in (walk, naive):
+        hit = res["leaf"] >= 0
+        assert hit.sum() > 1000
+        p = rays[hit, :dim] + rays[hit, dim:] * res["t"][hit, None]
+        dist = np.linalg.norm(p - c[res["leaf"][hit]], axis=1)
+        assert np.allclose(dist, r[res["leaf"][hit], 0], rtol=1e-9, atol=1e-9)
+    gw = oracle.Consumer64(dim, boxes, balls, addr, guarded=True).raycast_nearest(rays, 200.0)
+    gn = oracle.Consumer64(dim, boxes, balls, addr, guarded=True, naive=True).raycast_nearest(rays, 200.0)
+    assert np.array_equal(gw["leaf"], gn["leaf"])
+    assert np.array_equal(gw["t"].view(np.uint64), gn["t"].view(np.uint64))
