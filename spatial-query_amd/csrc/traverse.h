@@ -484,37 +484,46 @@ struct SceneView {
   const float* quads = nullptr;   // 4-wide records of the pair nodes (walks with QUAD), see quad_records
 };
 
-// 4-wide records for the QUAD walk.  Pair node p gets two pair-layout records
-// side by side; record s describes child s of p: the child's own pair node
-// when it is interior (so its two boxes and refs are p's grandchildren), or
-// the child's box and leaf ref with the second slot marked empty (word 4D+2 =
-// 1) when it is a leaf range.  Every box is a box of the pair layout, an exact
-// union of its primitives' boxes, so a walk over them keeps exactly the hits
-// of the pair walk (the order-independence argument above) while it fetches
-// half as many nodes one after another.  n_nodes bounds the refs read.
+// 4-wide records for the QUAD walk.  Pair node p gets one record of 2 (D+1)
+// 16-byte chunks whose first 2D+1 are read: the D box chunks of slot 0, one
+// chunk of the four refs {aa, ab, ba, bb}, then the D box chunks of slot 1
+// (floats 4 (D+1) ..; the last chunk is padding, so records stay 128-byte
+// aligned at D = 3).  Slot s describes child s of p: the two boxes of the
+// child's own pair node when it is interior (p's grandchildren), or the
+// child's box twice with its leaf ref in the first place and QUAD_EMPTY in the
+// second when it is a leaf range.  Every box is a box of the pair layout, an
+// exact union of its primitives' boxes, so a walk over them keeps exactly the
+// hits of the pair walk (the order-independence argument above) while it
+// fetches half as many nodes one after another, in 2D+1 loads per record
+// instead of 2 (D+1).  n_nodes bounds the refs read.
+constexpr uint32_t QUAD_EMPTY = 0xFFFFFFFFu;   // no second box in this slot (never a ref: at most 2^27 - 1 primitives)
+
 template <int D>
 SQ_HD void quad_records(const float* nodes, uint32_t n_nodes, uint32_t p, float* out) {
   constexpr int NF = (D + 1) * 4;
   const float* nd = nodes + (size_t)p * NF;
+  uint32_t refs[4];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     float* rec = out + s * NF;
     const uint32_t r = fbits(nd[4 * D + s]);
     if (!(r & LEAF_BIT) && r < n_nodes) {
       const float* cn = nodes + (size_t)r * NF;
-      for (int k = 0; k < NF; ++k) rec[k] = cn[k];
-      rec[4 * D + 2] = 0.0f;
+      for (int k = 0; k < 4 * D; ++k) rec[k] = cn[k];
+      refs[2 * s] = fbits(cn[4 * D + 0]);
+      refs[2 * s + 1] = fbits(cn[4 * D + 1]);
     } else {
       for (int k = 0; k < D; ++k) {
         rec[4 * k + 0] = nd[4 * k + 2 * s]; rec[4 * k + 1] = nd[4 * k + 2 * s + 1];
         rec[4 * k + 2] = nd[4 * k + 2 * s]; rec[4 * k + 3] = nd[4 * k + 2 * s + 1];
       }
-      rec[4 * D + 0] = nd[4 * D + s];
-      rec[4 * D + 1] = nd[4 * D + s];
-      rec[4 * D + 2] = bitsf(1u);
-      rec[4 * D + 3] = 0.0f;
+      refs[2 * s] = r;
+      refs[2 * s + 1] = QUAD_EMPTY;
     }
   }
+  // slot 0's box chunks are followed by the refs, slot 1's begin at NF
+  for (int k = 0; k < 4; ++k) out[4 * D + k] = bitsf(refs[k]);
+  for (int k = 0; k < 4; ++k) out[NF + 4 * D + k] = 0.0f;
 }
 
 // Stack accessor: entry k of this lane lives at ref[k * stride] / t[k * stride].
@@ -901,16 +910,16 @@ SQ_HD bool walk_step(const SceneView& s, const Q& q, const Filter& f, const Ring
     // children pushed farthest first (so they pop nearest first)
     while (!(w.cur & LEAF_BIT)) {
       float nd[2 * NF];
-      fetch16<2 * (D + 1)>(s.quads + (size_t)w.cur * 2 * NF, nd);
+      fetch16<2 * D + 1>(s.quads + (size_t)w.cur * 2 * NF, nd);
       bool h[4];
       float t[4];
       uint32_t r[4];
       q.pair(nd, w.limit, h[0], h[1], t[0], t[1]);
       q.pair(nd + NF, w.limit, h[2], h[3], t[2], t[3]);
-      r[0] = fbits(nd[4 * D + 0]); r[1] = fbits(nd[4 * D + 1]);
-      r[2] = fbits(nd[NF + 4 * D + 0]); r[3] = fbits(nd[NF + 4 * D + 1]);
-      h[1] = h[1] && fbits(nd[4 * D + 2]) == 0u;
-      h[3] = h[3] && fbits(nd[NF + 4 * D + 2]) == 0u;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) r[k] = fbits(nd[4 * D + k]);
+      h[1] = h[1] && r[1] != QUAD_EMPTY;
+      h[3] = h[3] && r[3] != QUAD_EMPTY;
       if constexpr (REF) {
         // slots in the reference's order aa, ab, ba, bb are pushed as found,
         // the last hit one walked next: bb's subtree first, aa's last
@@ -1082,16 +1091,16 @@ SQ_HD bool walk_step_spec(const SceneView& s, const Q& q, const Filter& f, const
   while (w.cur != NO_CUR && !(w.cur & LEAF_BIT)) {
     if constexpr (QUAD) {
       float nd[2 * NF];
-      fetch16<2 * (D + 1)>(s.quads + (size_t)w.cur * 2 * NF, nd);
+      fetch16<2 * D + 1>(s.quads + (size_t)w.cur * 2 * NF, nd);
       bool h[4];
       float t[4];
       uint32_t r[4];
       q.pair(nd, w.limit, h[0], h[1], t[0], t[1]);
       q.pair(nd + NF, w.limit, h[2], h[3], t[2], t[3]);
-      r[0] = fbits(nd[4 * D + 0]); r[1] = fbits(nd[4 * D + 1]);
-      r[2] = fbits(nd[NF + 4 * D + 0]); r[3] = fbits(nd[NF + 4 * D + 1]);
-      h[1] = h[1] && fbits(nd[4 * D + 2]) == 0u;
-      h[3] = h[3] && fbits(nd[NF + 4 * D + 2]) == 0u;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) r[k] = fbits(nd[4 * D + k]);
+      h[1] = h[1] && r[1] != QUAD_EMPTY;
+      h[3] = h[3] && r[3] != QUAD_EMPTY;
       // aa, ab, ba pushed as found, the last hit one walked next (bb first)
       bool have = false;
       uint32_t nr = 0;
