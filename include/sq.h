@@ -600,6 +600,11 @@ typedef struct sq_bvh64 sq_bvh64;
  * SPHERE [centre D][r]. */
 int sq_bvh64_create(sq_ctx* ctx, int dim, int kind, const double* prims, uint64_t n, sq_bvh64** out);
 int sq_bvh64_destroy(sq_bvh64* bvh);
+/* Bvh::refit (bvh.rs:45-48, tree.rs:54-75) under the feature: the same number
+ * of host double primitives at new positions (AABB or SPHERE trees); the
+ * topology is kept, bounds refit in double, records uploaded again (the
+ * device arrays stay where they are).  Synchronises the context stream. */
+int sq_bvh64_refit(sq_bvh64* bvh, const double* prims);
 /* TreeStats (stats.rs:15-32); bytes counts the f64 node size, quality is the
  * interior surface sum computed in double. */
 int sq_bvh64_stats(const sq_bvh64* bvh, sq_tree_stats* out);
@@ -644,6 +649,13 @@ int sq_tlas64_create(sq_ctx* ctx, int dim, uint64_t n, const double* rot, const 
 int sq_tlas64_create_kind(sq_ctx* ctx, int dim, int inst_kind, uint64_t n, const double* rot, const double* trans,
                           const double* halves, sq_tlas64** out);
 int sq_tlas64_destroy(sq_tlas64* tlas);
+/* Tlas::refit (tlas.rs:49-54) under the feature: new host transforms, and
+ * half extents (BOX) or radii (BALL), or NULL to keep them; world boxes by
+ * transform_aabb in double, the top tree refit.  Synchronises. */
+int sq_tlas64_refit(sq_tlas64* tlas, const double* rot, const double* trans, const double* halves);
+/* Tlas::needs_rebuild (tlas.rs:58-60): interior surface sum now > factor
+ * times its value at the build. */
+int sq_tlas64_needs_rebuild(sq_tlas64* tlas, double factor, int* out);
 int sq_tlas64_stats(const sq_tlas64* tlas, sq_tree_stats* out);
 /* Tlas::raycast_nearest (tlas.rs:104-154) as sq_tlas_raycast_nearest: the
  * instance id and toi of the nearest hit (out_point double[n][D] may be NULL). */

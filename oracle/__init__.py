@@ -84,6 +84,9 @@ def _declare(L):
         "orc64_tree_stats": (None, [I32, I32, P, U32, P, P]),
         "orc64_consumer_query": (U64, [I32, I32, P, U32, I32, U64, P, U64, C.c_double, I32, I32, P, P, P, U64]),
         "orc64_bvh_raycast_nearest": (None, [I32, I32, P, U32, P, U64, C.c_double, I32, P, P, P]),
+        "orc64_bvh_refit_raycast_nearest": (None, [I32, I32, P, P, U32, P, U64, C.c_double, I32, P, P, P, P]),
+        "orc64_tlas_refit_query": (U64, [I32, I32, U64, P, P, P, P, P, P, P, U64, C.c_double, I32, I32, I32, P, P, P, U64,
+                                          P]),
         "orc64_tlas_box_nearest": (None, [I32, U64, P, P, P, P, U64, C.c_double, I32, I32, P, P, P]),
         "orc64_tlas_query": (U64, [I32, I32, U64, P, P, P, P, U64, C.c_double, I32, I32, I32, P, P, P, U64]),
         "orc64_shapecast_nearest": (None, [I32, I32, P, U32, P, U64, I64, P, U64, C.c_double, P, I32, P, P, P]),
@@ -409,6 +412,19 @@ class Scene64:
                                         _p(leaf), _p(t), _p(counts))
         return dict(leaf=leaf, t=t, nodes_tested=int(counts[0]), prims_tested=int(counts[1]))
 
+    def bvh_raycast_nearest_refit(self, built_from, rays, max_toi, threads=1):
+        """bvh_raycast_nearest over the tree built on the primitives
+        `built_from` and refit to this scene's (Bvh::refit, bvh.rs:45-48:
+        the build's topology, fresh bounds); with the refit tree's interior
+        surface sum (tree.rs:104-110) as `quality`."""
+        r = self._q(rays)
+        n = r.shape[0]
+        b0 = np.ascontiguousarray(built_from, np.float64).reshape(self.prims.shape)
+        leaf = np.empty(n, np.int64); t = np.empty(n); counts = np.zeros(2, np.uint64); q = np.zeros(1)
+        lib().orc64_bvh_refit_raycast_nearest(self.dim, self._kf, _p(b0), _p(self.prims), self.n, _p(r), n,
+                                              float(max_toi), threads, _p(leaf), _p(t), _p(counts), _p(q))
+        return dict(leaf=leaf, t=t, quality=float(q[0]))
+
     def tree_stats(self):
         out = np.zeros(5, np.uint64); q = np.zeros(1)
         lib().orc64_tree_stats(self.dim, self.kind, _p(self.prims), self.n, _p(out), _p(q))
@@ -497,19 +513,40 @@ class Instances64:
     rules), guarded=True the engine's world-box guard.  raycast_all is also
     Tlas::raycast_crossings for these kinds (one crossing per instance)."""
 
-    def __init__(self, dim, kind, rot, trans, half, guarded: bool = False, naive: bool = False):
+    def __init__(self, dim, kind, rot, trans, half, guarded: bool = False, naive: bool = False, built_from=None):
+        """built_from: (rot, trans, half) of the instances the tree was built
+        over; the walks then run over that tree refit to these instances'
+        world boxes (Tlas::refit, tlas.rs:49-54)."""
         self.dim, self.kind = dim, kind
         self._flags = (F64_GUARDED if guarded else 0) | (0 if naive else F64_BVH)
-        self.rot = np.ascontiguousarray(rot, np.float64).reshape(-1, dim, dim)
-        self.trans = np.ascontiguousarray(trans, np.float64).reshape(-1, dim)
-        self.half = np.ascontiguousarray(half, np.float64).reshape(-1, dim) if kind == 0 else \
-            np.ascontiguousarray(half, np.float64).reshape(-1)
+        arrs = lambda r, t, h: (np.ascontiguousarray(r, np.float64).reshape(-1, dim, dim),
+                                np.ascontiguousarray(t, np.float64).reshape(-1, dim),
+                                np.ascontiguousarray(h, np.float64).reshape(-1, dim) if kind == 0 else
+                                np.ascontiguousarray(h, np.float64).reshape(-1))
+        self.rot, self.trans, self.half = arrs(rot, trans, half)
+        self.built_from = None if built_from is None else arrs(*built_from)
         self.n = self.rot.shape[0]
 
-    def _run(self, q, qs, max_toi, mode, off, leaf, t, cap, threads):
-        return lib().orc64_tlas_query(self.dim, self.kind, self.n, _p(self.rot), _p(self.trans), _p(self.half), _p(q),
-                                      q.shape[0], float(max_toi), mode, threads, self._flags, _p(off), _p(leaf), _p(t),
-                                      cap)
+    def _run(self, q, qs, max_toi, mode, off, leaf, t, cap, threads, quality=None):
+        if self.built_from is None and quality is None:
+            return lib().orc64_tlas_query(self.dim, self.kind, self.n, _p(self.rot), _p(self.trans), _p(self.half),
+                                          _p(q), q.shape[0], float(max_toi), mode, threads, self._flags, _p(off),
+                                          _p(leaf), _p(t), cap)
+        r0, t0, h0 = self.built_from if self.built_from is not None else (self.rot, self.trans, self.half)
+        return lib().orc64_tlas_refit_query(self.dim, self.kind, self.n, _p(r0), _p(t0), _p(h0), _p(self.rot),
+                                            _p(self.trans), _p(self.half), _p(q), q.shape[0], float(max_toi), mode,
+                                            threads, self._flags, _p(off), _p(leaf), _p(t), cap, _p(quality))
+
+    def quality(self):
+        """Interior surface sum of the (refit) tree, tree.rs:104-110."""
+        q = np.zeros(1)
+        self._run(np.zeros((0, 2 * self.dim)), 2 * self.dim, 1.0, 0, None, np.zeros(0, np.int64), np.zeros(0), 0, 1,
+                  quality=q)
+        return float(q[0])
+
+    def needs_rebuild(self, factor, baseline):
+        """Tlas::needs_rebuild (tlas.rs:58-60) against the build's quality."""
+        return baseline > 0.0 and self.quality() > factor * baseline
 
     def _nearest(self, q, qs, max_toi, mode, threads):
         q = np.ascontiguousarray(q, np.float64).reshape(-1, qs)

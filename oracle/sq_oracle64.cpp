@@ -414,6 +414,42 @@ template <int D> struct Tree64 {
     for (uint32_t i = 0; i < n; ++i) idx[i] = i;
     if (n) build<D>(boxes, idx, 0, n, nodes);
   }
+  // tree::refit (tree.rs:54-75) over fresh leaf boxes [n][min D, max D]: one
+  // reverse pass over the nodes, a leaf folding its primitives' boxes from
+  // EMPTY in prim_indices order, an interior node the union of its children
+  // (Aabb::union, aabb.rs:38-43: componentwise min and max).
+  void refit(const std::vector<double>& boxes) {
+    for (size_t ni = nodes.size(); ni-- > 0;) {
+      Node& nd = nodes[ni];
+      if (nd.pc > 0) {
+        for (int i = 0; i < D; ++i) { nd.b[i] = INFINITY; nd.b[D + i] = -INFINITY; }
+        for (uint32_t k = nd.a; k < nd.a + nd.pc; ++k) {
+          const double* b = boxes.data() + (size_t)idx[k] * 2 * D;
+          for (int i = 0; i < D; ++i) { nd.b[i] = std::fmin(nd.b[i], b[i]); nd.b[D + i] = std::fmax(nd.b[D + i], b[D + i]); }
+        }
+      } else {
+        const Node& a = nodes[nd.a];
+        const Node& c = nodes[nd.c];
+        for (int i = 0; i < D; ++i) { nd.b[i] = std::fmin(a.b[i], c.b[i]); nd.b[D + i] = std::fmax(a.b[D + i], c.b[D + i]); }
+      }
+    }
+  }
+  // tree::interior_surface_sum (tree.rs:104-110): half surfaces of the
+  // interior nodes summed in node order.
+  double quality() const {
+    double q = 0.0;
+    for (const Node& nd : nodes) {
+      if (nd.pc > 0) continue;
+      double sum = 0.0;
+      for (int a = 0; a < D; ++a) {
+        double prod = 1.0;
+        for (int k = 0; k < D; ++k) if (k != a) prod = prod * (nd.b[D + k] - nd.b[k]);
+        sum = sum + prod;
+      }
+      q = q + sum;
+    }
+    return q;
+  }
   // The reference's walk (bvh.rs:73-124 and its siblings): pop a node, test it
   // with pass(min, max) (which reads the caller's current limit), then either
   // hand each primitive of a leaf to leaf(id) in prim_indices order, or push
@@ -643,11 +679,21 @@ static uint64_t all_batch(int kindf, const double* prims, uint32_t n, const uint
 }
 
 // raycast_nearest_batch_parallel (bvh.rs:160-193): contiguous chunks, one per thread
+// prims_build (may be NULL): the tree is built over these primitives and then
+// refit to `prims` (Bvh::refit, bvh.rs:45-48), so it keeps their topology;
+// *quality (may be NULL) receives its interior surface sum.
 template <int D>
 static void bvh_nearest_batch(int kindf, const double* prims, uint32_t n, const double* rays, uint64_t nr, double max_toi,
-                              int threads, int64_t* leaf, double* t, uint64_t* counts) {
+                              int threads, int64_t* leaf, double* t, uint64_t* counts, const double* prims_build = nullptr,
+                              double* quality = nullptr) {
   const Geom<D> g = geom_of<D>(kindf, n, prims, nullptr);
-  const Tree64<D> tree(g);
+  Tree64<D> tree(prims_build ? geom_of<D>(kindf, n, prims_build, nullptr) : g);
+  if (prims_build) {
+    std::vector<double> boxes((size_t)n * 2 * D);
+    for (uint32_t i = 0; i < n; ++i) g.world_aabb(i, boxes.data() + (size_t)i * 2 * D, boxes.data() + (size_t)i * 2 * D + D);
+    tree.refit(boxes);
+  }
+  if (quality) *quality = tree.quality();
   std::vector<uint64_t> nodes(std::max(threads, 1), 0), prs(std::max(threads, 1), 0);
   const int T = std::max(threads, 1);
   const uint64_t chunk = (nr + T - 1) / T;
@@ -836,17 +882,30 @@ static void tlas_box_nearest(uint64_t n, const double* rot, const double* trans,
 // [nq][4D] (origin, dir, probe min, max).  Modes 0 and 3 write leaf[nq],
 // t[nq]; the others CSR (offsets[nq + 1]; leaf, t when the total fits cap)
 // and return the total.
+// build (may be NULL): {rot, trans, half} of the instances the tree was built
+// over; it is then refit to the world boxes of the current ones (Tlas::refit,
+// tlas.rs:49-54), keeping the build's topology; *quality (may be NULL)
+// receives the tree's interior surface sum.
 template <int D>
 static uint64_t tlas_query(int kind, uint64_t n, const double* rot, const double* trans, const double* half, const double* q,
                            uint64_t nq, double max_toi, int mode, int threads, int flags, uint64_t* offsets, int64_t* leaf,
-                           double* t, uint64_t cap) {
+                           double* t, uint64_t cap, const double* const* build = nullptr, double* quality = nullptr) {
   Inst64<D> s{n, rot, trans, half};
   s.guarded = (flags & FLAG_GUARDED) != 0;
   s.kind = kind;
   std::vector<double> wb(n * 2 * D);
   for (uint64_t i = 0; i < n; ++i) s.world_box(i, wb.data() + i * 2 * D, wb.data() + i * 2 * D + D);
   const bool bvh = (flags & FLAG_BVH) != 0;
-  const Tree64<D> tree(wb, bvh ? (uint32_t)n : 0u);
+  std::vector<double> wb0;
+  if (build) {
+    Inst64<D> s0{n, build[0], build[1], build[2]};
+    s0.kind = kind;
+    wb0.resize(n * 2 * D);
+    for (uint64_t i = 0; i < n; ++i) s0.world_box(i, wb0.data() + i * 2 * D, wb0.data() + i * 2 * D + D);
+  }
+  Tree64<D> tree(build ? wb0 : wb, bvh ? (uint32_t)n : 0u);
+  if (build) tree.refit(wb);
+  if (quality) *quality = tree.quality();
   std::vector<std::vector<std::pair<double, int64_t>>> res((mode == 0 || mode == 3) ? 0 : nq);
   auto by_hit = [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
     return hit_less(a.first, a.second, b.first, b.second);
@@ -1008,6 +1067,14 @@ void orc64_bvh_raycast_nearest(int D, int kind, const double* prims, uint32_t n,
   ORC64_DISPATCH(D, bvh_nearest_batch<DD>(kind, prims, n, rays, nr, max_toi, threads, leaf, t, counts));
 }
 
+// The same walk over the tree built on prims_build and refit to prims
+// (Bvh::refit); quality: the refit tree's interior surface sum.
+void orc64_bvh_refit_raycast_nearest(int D, int kind, const double* prims_build, const double* prims, uint32_t n,
+                                     const double* rays, uint64_t nr, double max_toi, int threads, int64_t* leaf, double* t,
+                                     uint64_t* counts, double* quality) {
+  ORC64_DISPATCH(D, bvh_nearest_batch<DD>(kind, prims, n, rays, nr, max_toi, threads, leaf, t, counts, prims_build, quality));
+}
+
 void orc64_tlas_box_nearest(int D, uint64_t n, const double* rot, const double* trans, const double* half,
                             const double* rays, uint64_t nr, double max_toi, int threads, int flags, int64_t* leaf, double* t,
                             double* world) {
@@ -1020,6 +1087,19 @@ uint64_t orc64_tlas_query(int D, int kind, uint64_t n, const double* rot, const 
                           int64_t* leaf, double* t, uint64_t cap) {
   uint64_t tot = 0;
   ORC64_DISPATCH(D, tot = tlas_query<DD>(kind, n, rot, trans, half, q, nq, max_toi, mode, threads, flags, offsets, leaf, t, cap));
+  return tot;
+}
+
+// tlas_query over the tree built on the instances {rot0, trans0, half0} and
+// refit to {rot, trans, half} (Tlas::refit); quality: its interior surface sum.
+uint64_t orc64_tlas_refit_query(int D, int kind, uint64_t n, const double* rot0, const double* trans0, const double* half0,
+                                const double* rot, const double* trans, const double* half, const double* q, uint64_t nq,
+                                double max_toi, int mode, int threads, int flags, uint64_t* offsets, int64_t* leaf, double* t,
+                                uint64_t cap, double* quality) {
+  uint64_t tot = 0;
+  const double* build[3] = {rot0, trans0, half0};
+  ORC64_DISPATCH(D, tot = tlas_query<DD>(kind, n, rot, trans, half, q, nq, max_toi, mode, threads, flags, offsets, leaf, t, cap,
+                                         build, quality));
   return tot;
 }
 

@@ -111,6 +111,9 @@ struct sq_bvh64 {
   int dim = 0, kind = 0;
   uint64_t n_prims = 0;
   uint32_t consumer_pd = 0;   // KIND_CONSUMER64: the narrow phase's doubles per primitive
+  int ps = 0;                 // doubles per leaf record
+  double baseline = 0.0;      // interior surface sum at the build (Tlas::needs_rebuild)
+  std::vector<double> inst_half;   // instance trees: the half extents (or radii) last given
   sqh::RefTreeT<double> tree;
   DevBuf nodes, prims, ids, layers;
   View64 view{};
@@ -154,31 +157,16 @@ static int pool_launch(sq_bvh64* b, const double* q, uint64_t n, double max_toi,
 
 extern "C" {
 
-// The f64 tree over n leaves with world boxes `boxes` [n][2D] and device
-// records `prims` [n][ps] (leaf-ordered on upload).
-static int create64(sq_ctx* c, int dim, int kind, const double* prims, int ps, const std::vector<double>& boxes, uint64_t n,
-                    sq_bvh64** out) {
-  int rc;
-  std::unique_ptr<sq_bvh64> b(new sq_bvh64());
-  b->ctx = c; b->dim = dim; b->kind = kind; b->n_prims = n;
-  sqh::build_tree(dim, boxes.data(), (uint32_t)n, b->tree);
-  const sqh::RefTreeT<double>& t = b->tree;
-  uint64_t leaves = 0, nprims = 0, depth = 0;
-  sqh::tree_structural(t, &leaves, &nprims, &depth);
-  if (depth > F64_MAX_STACK) return fail(SQ_ERR_UNSUPPORTED, "tree deeper than the f64 walk's stack");
-  b->view.stack = (uint32_t)depth;
-  // TreeStats (tree.rs:81-110, bvh.rs:477-489) of the f64 tree: size_of::<Node<D>>() is 16D + 12 bytes
-  // padded to the f64 alignment; the interior surface sum is summed in double and reported as float
-  b->stats.node_count = t.node_count();
-  b->stats.leaf_count = leaves;
-  b->stats.prim_count = nprims;
-  b->stats.max_depth = depth;
-  b->stats.bytes = b->stats.node_count * (uint64_t)(16 * dim + 16) + t.prim_indices.size() * 4;
+// tree::interior_surface_sum (tree.rs:104-110) of the f64 tree: the half
+// surfaces (Aabb::half_surface, aabb.rs:84-97) of the interior nodes summed in
+// node order, in double.
+static double surface_sum64(const sqh::RefTreeT<double>& t) {
+  const int dim = t.dim;
   double quality = 0.0;
   for (size_t i = 0; i < t.node_count(); ++i) {
     if (t.prim_count[i] > 0) continue;
     const double* bx = t.bounds.data() + i * 2 * dim;
-    double sum = 0.0;   // Aabb::half_surface, aabb.rs:84-97
+    double sum = 0.0;
     for (int a = 0; a < dim; ++a) {
       double prod = 1.0;
       for (int k = 0; k < dim; ++k) if (k != a) prod = prod * (bx[dim + k] - bx[k]);
@@ -186,7 +174,39 @@ static int create64(sq_ctx* c, int dim, int kind, const double* prims, int ps, c
     }
     quality = quality + sum;
   }
-  b->stats.quality = (float)quality;
+  return quality;
+}
+
+// tree::refit (tree.rs:54-75) over fresh leaf boxes [n][2D]: one reverse pass,
+// a leaf folding its primitives' boxes from EMPTY in prim_indices order, an
+// interior node the union of its children (componentwise min and max, exact).
+static void refit_bounds64(sqh::RefTreeT<double>& t, const std::vector<double>& boxes) {
+  const int D = t.dim;
+  for (size_t ni = t.node_count(); ni-- > 0;) {
+    double* b = t.bounds.data() + ni * 2 * D;
+    if (t.prim_count[ni] > 0) {
+      for (int k = 0; k < D; ++k) { b[k] = INFINITY; b[D + k] = -INFINITY; }
+      for (uint32_t j = t.child_a[ni]; j < t.child_a[ni] + t.prim_count[ni]; ++j) {
+        const double* pb = boxes.data() + (size_t)t.prim_indices[j] * 2 * D;
+        for (int k = 0; k < D; ++k) { b[k] = std::fmin(b[k], pb[k]); b[D + k] = std::fmax(b[D + k], pb[D + k]); }
+      }
+    } else {
+      const double* ba = t.bounds.data() + (size_t)t.child_a[ni] * 2 * D;
+      const double* bb = t.bounds.data() + (size_t)t.child_b[ni] * 2 * D;
+      for (int k = 0; k < D; ++k) { b[k] = std::fmin(ba[k], bb[k]); b[D + k] = std::fmax(ba[D + k], bb[D + k]); }
+    }
+  }
+}
+
+// The pair records, leaf-ordered records `prims` [n][ps] (caller order) and
+// ids of b->tree to the device, and the root bounds into the walk's view.  A
+// refit uploads the same sizes again: the arrays stay where they are.
+static int upload64(sq_bvh64* b, const double* prims) {
+  sq_ctx* c = b->ctx;
+  const int dim = b->dim, ps = b->ps;
+  const uint64_t n = b->n_prims;
+  const sqh::RefTreeT<double>& t = b->tree;
+  int rc;
   // pair nodes in reference pre-order of their interior nodes
   const size_t nn = t.node_count();
   std::vector<uint32_t> pair_of(nn, 0);
@@ -220,6 +240,8 @@ static int create64(sq_ctx* c, int dim, int kind, const double* prims, int ps, c
   if ((rc = b->nodes.ensure(pb.size() * 8)) || (rc = b->prims.ensure(lp.size() * 8)) ||
       (rc = b->ids.ensure(ids.size() * 4)))
     return rc;
+  // (walks already enqueued on the context stream read the old arrays)
+  HIP_TRY(hipStreamSynchronize(c->stream));
   if (np) HIP_TRY(hipMemcpy(b->nodes.p, pb.data(), pb.size() * 8, hipMemcpyHostToDevice));
   if (n) {
     HIP_TRY(hipMemcpy(b->prims.p, lp.data(), lp.size() * 8, hipMemcpyHostToDevice));
@@ -229,27 +251,49 @@ static int create64(sq_ctx* c, int dim, int kind, const double* prims, int ps, c
   v.nodes = b->nodes.as<double>();
   v.prims = b->prims.as<double>();
   v.ids = b->ids.as<uint32_t>();
-  v.layers = nullptr;
   v.n_prims = (uint32_t)n;
   if (nn) {
     std::memcpy(v.root, t.bounds.data(), 2 * dim * sizeof(double));
     v.root_count = t.prim_count[0];
     v.root_ref = t.prim_count[0] ? t.child_a[0] : pair_of[0];
   }
+  b->stats.quality = (float)surface_sum64(t);
+  return SQ_OK;
+}
+
+// The f64 tree over n leaves with world boxes `boxes` [n][2D] and device
+// records `prims` [n][ps] (leaf-ordered on upload).
+static int create64(sq_ctx* c, int dim, int kind, const double* prims, int ps, const std::vector<double>& boxes, uint64_t n,
+                    sq_bvh64** out) {
+  int rc;
+  std::unique_ptr<sq_bvh64> b(new sq_bvh64());
+  b->ctx = c; b->dim = dim; b->kind = kind; b->n_prims = n; b->ps = ps;
+  sqh::build_tree(dim, boxes.data(), (uint32_t)n, b->tree);
+  const sqh::RefTreeT<double>& t = b->tree;
+  uint64_t leaves = 0, nprims = 0, depth = 0;
+  sqh::tree_structural(t, &leaves, &nprims, &depth);
+  if (depth > F64_MAX_STACK) return fail(SQ_ERR_UNSUPPORTED, "tree deeper than the f64 walk's stack");
+  b->view.stack = (uint32_t)depth;
+  b->view.layers = nullptr;
+  // TreeStats (tree.rs:81-110, bvh.rs:477-489) of the f64 tree: size_of::<Node<D>>() is 16D + 12 bytes
+  // padded to the f64 alignment; the interior surface sum is summed in double and reported as float
+  b->stats.node_count = t.node_count();
+  b->stats.leaf_count = leaves;
+  b->stats.prim_count = nprims;
+  b->stats.max_depth = depth;
+  b->stats.bytes = b->stats.node_count * (uint64_t)(16 * dim + 16) + t.prim_indices.size() * 4;
+  b->baseline = surface_sum64(t);
+  if ((rc = upload64(b.get(), prims))) return rc;
   ctx_retain(c);
   *out = b.release();
   return SQ_OK;
 }
 
-int sq_bvh64_create(sq_ctx* c, int dim, int kind, const double* prims, uint64_t n, sq_bvh64** out) {
-  if (!c || !out || (n && !prims)) return fail(SQ_ERR_INVALID, "NULL argument");
-  *out = nullptr;
-  if (n >= 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "too many primitives (ids are u32)");
-  int rc = dispatch64(dim, kind, [&]<int D, int KIND>() -> int { return SQ_OK; });
-  if (rc) return rc;
+// QueryGeometry::world_aabb of the built-in kinds, in double: the box, or
+// Aabb::new(c - splat(r), c + splat(r)).
+static int prim_boxes64(int dim, int kind, const double* prims, uint64_t n, std::vector<double>& boxes) {
   const int ps = kind == KIND_AABB ? 2 * dim : dim + 1;
-  // QueryGeometry::world_aabb of the built-in kinds, in double
-  std::vector<double> boxes((size_t)n * 2 * dim);
+  boxes.resize((size_t)n * 2 * dim);
   for (uint64_t i = 0; i < n; ++i) {
     const double* p = prims + i * ps;
     double* bx = boxes.data() + i * 2 * dim;
@@ -259,12 +303,40 @@ int sq_bvh64_create(sq_ctx* c, int dim, int kind, const double* prims, uint64_t 
     }
     if (kind == KIND_SPHERE && !(p[dim] >= 0.0)) return fail(SQ_ERR_INVALID, "sphere radius must be >= 0");
   }
-  return create64(c, dim, kind, prims, ps, boxes, n, out);
+  return SQ_OK;
+}
+
+int sq_bvh64_create(sq_ctx* c, int dim, int kind, const double* prims, uint64_t n, sq_bvh64** out) {
+  if (!c || !out || (n && !prims)) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (n >= 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "too many primitives (ids are u32)");
+  int rc = dispatch64(dim, kind, [&]<int D, int KIND>() -> int { return SQ_OK; });
+  if (rc) return rc;
+  std::vector<double> boxes;
+  if ((rc = prim_boxes64(dim, kind, prims, n, boxes))) return rc;
+  return create64(c, dim, kind, prims, kind == KIND_AABB ? 2 * dim : dim + 1, boxes, n, out);
+}
+
+// Bvh::refit (bvh.rs:45-48) under the `f64` feature: the same number of
+// primitives at new positions (host, caller order); the topology is kept,
+// bounds refit in double (tree.rs:54-75), records uploaded again.
+int sq_bvh64_refit(sq_bvh64* b, const double* prims) {
+  if (!b || (b->n_prims && !prims)) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (b->kind != KIND_AABB && b->kind != KIND_SPHERE)
+    return fail(SQ_ERR_UNSUPPORTED, "sq_bvh64_refit takes trees over AABB or SPHERE primitives (instance trees: sq_tlas64_refit)");
+  std::vector<double> boxes;
+  int rc;
+  if ((rc = prim_boxes64(b->dim, b->kind, prims, b->n_prims, boxes))) return rc;
+  refit_bounds64(b->tree, boxes);
+  return upload64(b, prims);
 }
 
 // Tlas::build (tlas.rs:35-56) over box or ball instances in double: each world
 // box is Isometry::transform_aabb of the local box, [-half, half] or
 // [-r, r]^D (isometry.rs:84-97).  Balls store splat(r) as their half extents.
+static int inst_records64(int dim, bool ball, uint64_t n, const double* rot, const double* trans, const double* halves,
+                          std::vector<double>& recs, std::vector<double>& boxes);
+
 int sq_tlas64_create_kind(sq_ctx* c, int dim, int inst_kind, uint64_t n, const double* rot, const double* trans,
                           const double* halves, sq_tlas64** out) {
   if (!c || !out || (n && (!rot || !trans || !halves))) return fail(SQ_ERR_INVALID, "NULL argument");
@@ -275,8 +347,25 @@ int sq_tlas64_create_kind(sq_ctx* c, int dim, int inst_kind, uint64_t n, const d
   if (ball && dim != 3) return fail(SQ_ERR_UNSUPPORTED, "ball instances need dim 3");
   if (dim < 2 || dim > 6) return fail(SQ_ERR_UNSUPPORTED, "dimension " + std::to_string(dim) + " not instantiated (2..6)");
   if (n >= 0xFFFFFFFFull) return fail(SQ_ERR_INVALID, "too many instances (ids are u32)");
+  std::vector<double> boxes, recs;
+  int rc = inst_records64(dim, ball, n, rot, trans, halves, recs, boxes);
+  if (rc) return rc;
+  sq_bvh64* b = nullptr;
+  if ((rc = create64(c, dim, ball ? KIND_BALL64 : KIND_INST64, recs.data(), dim * dim + 4 * dim, boxes, n, &b))) return rc;
+  b->inst_half.assign(halves, halves + n * (ball ? 1 : dim));
+  *out = reinterpret_cast<sq_tlas64*>(b);
+  return SQ_OK;
+}
+
+// Instance records [n][D*D + 4D] (R, t, half extents, world box) and world
+// boxes [n][2D] of box or ball instances: each world box is
+// Isometry::transform_aabb of the local box, [-half, half] or [-r, r]^D
+// (isometry.rs:84-97).  Balls store splat(r) as their half extents.
+static int inst_records64(int dim, bool ball, uint64_t n, const double* rot, const double* trans, const double* halves,
+                          std::vector<double>& recs, std::vector<double>& boxes) {
   const int D = dim, ps = D * D + 4 * D;
-  std::vector<double> boxes((size_t)n * 2 * D), recs((size_t)n * ps);
+  boxes.resize((size_t)n * 2 * D);
+  recs.resize((size_t)n * ps);
   for (uint64_t i = 0; i < n; ++i) {
     const double* R = rot + i * D * D;
     const double* t = trans + i * D;
@@ -301,9 +390,33 @@ int sq_tlas64_create_kind(sq_ctx* c, int dim, int inst_kind, uint64_t n, const d
     }
     std::memcpy(rec + D * D + 2 * D, w, 2 * D * sizeof(double));
   }
-  sq_bvh64* b = nullptr;
-  if (int rc = create64(c, dim, ball ? KIND_BALL64 : KIND_INST64, recs.data(), ps, boxes, n, &b)) return rc;
-  *out = reinterpret_cast<sq_tlas64*>(b);
+  return SQ_OK;
+}
+
+// Tlas::refit (tlas.rs:49-54) under the `f64` feature: new transforms (and
+// half extents or radii, or NULL to keep them), host; world boxes by
+// transform_aabb in double, the top tree refit (tree.rs:54-75), records
+// uploaded again.
+int sq_tlas64_refit(sq_tlas64* tl, const double* rot, const double* trans, const double* halves) {
+  sq_bvh64* b = reinterpret_cast<sq_bvh64*>(tl);
+  if (!b || (b->n_prims && (!rot || !trans))) return fail(SQ_ERR_INVALID, "NULL argument");
+  if (b->kind != KIND_INST64 && b->kind != KIND_BALL64) return fail(SQ_ERR_INVALID, "not an instance tree");
+  const bool ball = b->kind == KIND_BALL64;
+  const uint64_t n = b->n_prims;
+  std::vector<double> boxes, recs;
+  int rc = inst_records64(b->dim, ball, n, rot, trans, halves ? halves : b->inst_half.data(), recs, boxes);
+  if (rc) return rc;   // (nothing changed)
+  if (halves) b->inst_half.assign(halves, halves + n * (ball ? 1 : b->dim));
+  refit_bounds64(b->tree, boxes);
+  return upload64(b, recs.data());
+}
+
+// Tlas::needs_rebuild (tlas.rs:58-60): the top tree's interior surface sum
+// now against factor times its value at the build.
+int sq_tlas64_needs_rebuild(sq_tlas64* tl, double factor, int* out) {
+  sq_bvh64* b = reinterpret_cast<sq_bvh64*>(tl);
+  if (!b || !out) return fail(SQ_ERR_INVALID, "NULL argument");
+  *out = (b->baseline > 0.0 && surface_sum64(b->tree) > factor * b->baseline) ? 1 : 0;
   return SQ_OK;
 }
 

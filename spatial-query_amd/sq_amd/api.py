@@ -480,6 +480,14 @@ class DeviceBvh64:
         self._h, self.ctx, self.dim, self.kind, self.n_prims = h, ctx, dim, kind, p.shape[0]
         _track(self)
 
+    def refit(self, prims):
+        """Bvh::refit in double: the same number of primitives at new
+        positions (host float64, the layout of the build)."""
+        p = _host(prims, np.float64).reshape(-1, prim_stride(self.dim, self.kind))
+        if p.shape[0] != self.n_prims:
+            raise ValueError(f"refit takes {self.n_prims} primitives, got {p.shape[0]}")
+        check(lib().sq_bvh64_refit(self._h, _hp(p)))
+
     def set_layers(self, layers):
         l = None if layers is None else _host(layers, np.uint64)
         check(lib().sq_bvh64_set_layers(self._h, _hp(l)))
@@ -585,8 +593,24 @@ class Tlas64:
             raise ValueError("rot, trans and halves must describe the same number of instances")
         hd = C.c_void_p()
         check(lib().sq_tlas64_create_kind(ctx.handle, dim, int(kind), r.shape[0], _hp(r), _hp(t), _hp(h), C.byref(hd)))
-        self._h, self.ctx, self.dim, self.n = hd, ctx, dim, r.shape[0]
+        self._h, self.ctx, self.dim, self.n, self.kind = hd, ctx, dim, r.shape[0], kind
         _track(self)
+
+    def refit(self, rot, trans, halves=None):
+        """Tlas::refit in double: new transforms (host float64, as at the
+        build), and half extents / radii (None keeps them)."""
+        r = _host(rot, np.float64).reshape(-1, self.dim, self.dim)
+        t = _host(trans, np.float64).reshape(-1, self.dim)
+        h = None if halves is None else (_host(halves, np.float64).reshape(-1) if self.kind == 2 else
+                                         _host(halves, np.float64).reshape(-1, self.dim))
+        if r.shape[0] != self.n or t.shape[0] != self.n or (h is not None and h.shape[0] != self.n):
+            raise ValueError(f"refit takes the {self.n} instances of the build")
+        check(lib().sq_tlas64_refit(self._h, _hp(r), _hp(t), _hp(h)))
+
+    def needs_rebuild(self, factor: float) -> bool:
+        out = C.c_int()
+        check(lib().sq_tlas64_needs_rebuild(self._h, float(factor), C.byref(out)))
+        return bool(out.value)
 
     _csr = DeviceBvh64._csr
 

@@ -145,6 +145,9 @@ SIGNATURES = {
     "sq_tlas64_shapecast_nearest": [P, P, U64, F64, P, P, P],
     "sq_tlas64_shapecast_all": [P, P, U64, F64, P, P, P, U64, P],
     "sq_tlas64_destroy": [P],
+    "sq_tlas64_refit": [P, P, P, P],
+    "sq_tlas64_needs_rebuild": [P, C.c_double, C.POINTER(C.c_int)],
+    "sq_bvh64_refit": [P, P],
     "sq_tlas64_stats": [P, C.POINTER(TreeStats)],
     "sq_tlas64_raycast_nearest": [P, P, U64, F64, P, P, P, P],
     "sq_tlas64_raycast_any": [P, P, U64, F64, P, P],

@@ -10,6 +10,8 @@ of sq/src/query/naive.rs in double):
     size-independent properties on every ray (any == nearest hit, the first
     of a ray's all-hits list is its nearest hit, lists sorted);
   * refusals (triangles, D = 7, negative radii) and the empty tree;
+  * refit (Bvh::refit, Tlas::refit, Tlas::needs_rebuild) against the oracle's
+    tree built on the first positions and refit to the second;
   * consumer narrow phases in double (sq_narrow64, compiled at run time): the
     built-in ball's arithmetic equals the f64 SPHERE kind and the oracle's
     consumer kind bit for bit in both hit modes; balls and ellipsoids with
@@ -588,3 +590,101 @@ def test_f64_consumer_errors_and_empty_tree(ctx):
     empty.close()
     other.close()
     nw.close()
+
+
+# --- refit under the feature (Bvh::refit, Tlas::refit, Tlas::needs_rebuild) ---
+
+@pytest.mark.parametrize("dim,kind", [(3, oracle.SPHERE), (2, oracle.BOX)])
+def test_f64_bvh_refit(ctx, dim, kind):
+    """sq_bvh64_refit (bvh.rs:45-48, tree.rs:54-75 in double): built on one
+    set of primitives and refit to moved ones, the tree answers nearest hits
+    bit for bit as the oracle's tree built on the first set and refit to the
+    second, in both hit modes, all hits as the naive scan over the moved set,
+    and reports the refit tree's surface sum; a second refit back to the
+    build's primitives gives the build's answers again."""
+    print(f"case: f64 bvh refit dim={dim} kind={kind}", flush=True)
+    rng = np.random.default_rng(0x5EF1 + dim)
+    n = 20000
+    span = {2: 60.0, 3: 30.0}[dim]
+    c = rng.uniform(-span, span, (n, dim))
+    p0 = np.concatenate([c, c + rng.uniform(0.2, 2.0, (n, dim))], 1) if kind == oracle.BOX else \
+        np.concatenate([c, rng.uniform(0.2, 1.5, (n, 1))], 1)
+    p2 = p0.copy()
+    mv = rng.uniform(-3.0, 3.0, (n, dim))
+    p2[:, :dim] += mv
+    if kind == oracle.BOX:
+        p2[:, dim:] += mv
+    o = rng.uniform(-span, span, (20000, dim)); d = rng.normal(size=(20000, dim))
+    rays = np.concatenate([o, d / np.linalg.norm(d, axis=1, keepdims=True)], 1)
+    b = sq_amd.DeviceBvh64(ctx, dim, kind, p0)
+    built = b.raycast_nearest(dev(rays), 200.0)
+    b.refit(p2)
+    for guarded in (False, True):
+        ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED if guarded else sq_amd.SQ_HITS_REFERENCE)
+        try:
+            want = oracle.Scene64(dim, kind, p2, guarded=guarded).bvh_raycast_nearest_refit(p0, rays, 200.0, threads=8)
+            if guarded:   # the guarded walk is order-independent: the naive guarded scan
+                want = oracle.Scene64(dim, kind, p2, guarded=True, naive=True).raycast_nearest(rays, 200.0)
+            got = b.raycast_nearest(dev(rays), 200.0)
+            assert (want["leaf"] >= 0).sum() > 3000
+            assert np.array_equal(host(got["leaf"]).astype(np.int64), want["leaf"])
+            assert np.array_equal(host(got["toi"]).view(np.uint64), want["t"].view(np.uint64))
+        finally:
+            ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
+    q = oracle.Scene64(dim, kind, p2).bvh_raycast_nearest_refit(p0, rays[:1], 200.0)["quality"]
+    assert b.stats()["quality"] == float(np.float32(q))
+    _csr_equal64(b.raycast_all(dev(rays), 200.0), oracle.Scene64(dim, kind, p2).raycast_all(rays, 200.0))
+    b.refit(p0)
+    back = b.raycast_nearest(dev(rays), 200.0)
+    assert np.array_equal(host(back["leaf"]), host(built["leaf"]))
+    assert np.array_equal(host(back["toi"]).view(np.uint64), host(built["toi"]).view(np.uint64))
+    with pytest.raises(ValueError):
+        b.refit(p0[:-1])
+    b.close()
+
+
+@pytest.mark.parametrize("kind", [0, 2])
+def test_f64_tlas_refit_and_needs_rebuild(ctx, kind):
+    """sq_tlas64_refit / sq_tlas64_needs_rebuild (tlas.rs:49-60 in double): a
+    Tlas over box (kind 0) or ball (kind 2) instances, refit to new rotations
+    and translations, answers nearest hits in both hit modes and all hits bit
+    for bit as the oracle's Tlas built on the first transforms and refit to
+    the second; its surface sum is the oracle's, and needs_rebuild agrees with
+    the oracle's at factors below and above the drift."""
+    print(f"case: f64 tlas refit kind={kind}", flush=True)
+    rng = np.random.default_rng(0x7EF1 + kind)
+    n, dim = 3000, 3
+    rot0 = orthonormal(rng, n, dim)
+    t0 = rng.uniform(-60.0, 60.0, (n, dim))
+    half = rng.uniform(0.5, 3.0, (n, dim)) if kind == 0 else rng.uniform(0.5, 2.0, n)
+    rot1 = orthonormal(rng, n, dim)
+    t1 = t0 + rng.uniform(-4.0, 4.0, (n, dim))
+    o = rng.uniform(-70.0, 70.0, (8000, dim)); d = rng.normal(size=(8000, dim))
+    rays = np.concatenate([o, d / np.linalg.norm(d, axis=1, keepdims=True)], 1)
+    ok = 0 if kind == 0 else 1
+    tl = sq_amd.Tlas64(ctx, dim, rot0, t0, half, kind=kind)
+    q0 = oracle.Instances64(dim, ok, rot0, t0, half).quality()
+    assert not tl.needs_rebuild(1.0)
+    tl.refit(rot1, t1)   # halves kept
+    for guarded in (False, True):
+        ref = oracle.Instances64(dim, ok, rot1, t1, half, guarded=guarded, built_from=(rot0, t0, half))
+        ctx.set_hit_mode(sq_amd.SQ_HITS_GUARDED if guarded else sq_amd.SQ_HITS_REFERENCE)
+        try:
+            w = ref.raycast_nearest(rays, 400.0)
+            g = tl.raycast_nearest(dev(rays), 400.0)
+            assert (w["leaf"] >= 0).sum() > 1000
+            assert np.array_equal(host(g["leaf"]).astype(np.int64), w["leaf"])
+            assert np.array_equal(host(g["toi"]).view(np.uint64), w["t"].view(np.uint64))
+        finally:
+            ctx.set_hit_mode(sq_amd.SQ_HITS_REFERENCE)
+    ref = oracle.Instances64(dim, ok, rot1, t1, half, built_from=(rot0, t0, half))
+    _csr_equal64(tl.raycast_all(dev(rays), 400.0), ref.raycast_all(rays, 400.0))
+    q1 = ref.quality()
+    assert tl.stats()["quality"] == float(np.float32(q1))
+    for f in (1.0, q1 / q0 * 0.999, q1 / q0 * 1.001, 2.0):
+        assert tl.needs_rebuild(f) == ref.needs_rebuild(f, q0), f
+    tl.refit(rot0, t0, half)   # back to the build
+    assert not tl.needs_rebuild(1.0)
+    with pytest.raises(ValueError):
+        tl.refit(rot0[:-1], t0[:-1])
+    tl.close()

@@ -553,3 +553,111 @@ def test_consumer64_hits_outside_their_boxes(tmp_path):
     gn = oracle.Consumer64(dim, boxes, balls, addr, guarded=True, naive=True).raycast_nearest(rays, 200.0)
     assert np.array_equal(gw["leaf"], gn["leaf"])
     assert np.array_equal(gw["t"].view(np.uint64), gn["t"].view(np.uint64))
+
+
+# Refit under the `f64` feature (Bvh::refit, bvh.rs:45-48; Tlas::refit and
+# needs_rebuild, tlas.rs:49-60; tree::refit, tree.rs:54-75).  Pins: a refit to
+# the same primitives is the build; a dyadic translation of every primitive
+# (exact in double) refits to the tree the build over the moved primitives
+# makes, so every answer equals the build's with the rays moved alike and the
+# surface sum is unchanged; under random motion the refit tree answers what
+# the naive scans over the moved primitives answer.  A refit that left any
+# level of bounds stale loses the moved hits and fails the first two.
+
+def _dyadic_prims(rng, n, dim, kind, span=2048):
+    c = rng.integers(-span, span, (n, dim)) / 64.0
+    if kind == oracle.BOX:
+        return np.concatenate([c, c + rng.integers(8, 128, (n, dim)) / 64.0], 1)
+    return np.concatenate([c, rng.integers(16, 96, (n, 1)) / 64.0], 1)
+
+
+def _moved(prims, dim, kind, v):
+    p = prims.copy()
+    p[:, :dim] += v
+    if kind == oracle.BOX:
+        p[:, dim:2 * dim] += v
+    return p
+
+
+@pytest.mark.parametrize("dim,kind", [(2, oracle.BOX), (3, oracle.SPHERE), (6, oracle.BOX), (4, oracle.SPHERE)])
+def test_bvh_refit_pins(dim, kind):
+    rng = np.random.default_rng(910 + dim + kind)
+    n = 3000
+    span = {2: 2048, 3: 2048, 4: 512, 6: 128}[dim]   # denser in more dimensions, so rays still hit
+    p0 = _dyadic_prims(rng, n, dim, kind, span)
+    d = rng.normal(size=(2500, dim))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)   # unit directions (Ray::new), as the ball test assumes
+    rays = np.concatenate([rng.integers(-span - span // 8, span + span // 8, (2500, dim)) / 64.0, d], 1)
+    s0 = oracle.Scene64(dim, kind, p0)
+    built = s0.bvh_raycast_nearest(rays, 200.0)
+    q0 = s0.tree_stats()["quality"]
+    same = s0.bvh_raycast_nearest_refit(p0, rays, 200.0)
+    assert np.array_equal(same["leaf"], built["leaf"]) and np.array_equal(same["t"], built["t"])
+    assert same["quality"] == q0
+    assert (built["leaf"] >= 0).sum() > 400
+    # dyadic translation: the refit tree is the moved build
+    v = np.array([17.25, -3.5, 6.125, -0.75, 2.0, 9.5][:dim])
+    p1 = _moved(p0, dim, kind, v)
+    r1 = rays.copy(); r1[:, :dim] += v
+    s1 = oracle.Scene64(dim, kind, p1)
+    ref = s1.bvh_raycast_nearest_refit(p0, r1, 200.0)
+    direct = s1.bvh_raycast_nearest(r1, 200.0)
+    assert np.array_equal(ref["leaf"], built["leaf"]) and np.array_equal(ref["t"], built["t"])
+    assert np.array_equal(direct["leaf"], built["leaf"]) and np.array_equal(direct["t"], built["t"])
+    assert ref["quality"] == q0 == s1.tree_stats()["quality"]
+    # random motion: the refit tree answers the naive scan over the moved primitives
+    p2 = _moved(p0, dim, kind, rng.uniform(-4.0, 4.0, (n, dim)))
+    s2 = oracle.Scene64(dim, kind, p2)
+    got = s2.bvh_raycast_nearest_refit(p0, rays, 200.0)
+    want = s2.raycast_nearest(rays, 200.0)
+    assert np.array_equal(got["leaf"], want["leaf"]) and np.array_equal(got["t"], want["t"])
+    assert got["quality"] > q0   # the moved boxes spread the build's nodes
+
+
+def _signed_permutations(rng, n, dim):
+    rot = np.zeros((n, dim, dim))
+    for i in range(n):
+        rot[i, np.arange(dim), rng.permutation(dim)] = rng.choice([-1.0, 1.0], dim)
+    return rot
+
+
+@pytest.mark.parametrize("dim,kind", [(3, 0), (2, 0), (3, 1)])
+def test_tlas_refit_pins(dim, kind):
+    rng = np.random.default_rng(930 + dim + kind)
+    n = 400
+    rot = _signed_permutations(rng, n, dim)
+    t0 = rng.integers(-1024, 1024, (n, dim)) / 32.0
+    half = rng.integers(8, 64, (n, dim)) / 32.0 if kind == 0 else rng.integers(16, 64, n) / 32.0
+    d = rng.normal(size=(2500, dim))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([rng.integers(-1200, 1200, (2500, dim)) / 32.0, d], 1)
+    built = oracle.Instances64(dim, kind, rot, t0, half)
+    q0 = built.quality()
+    same = oracle.Instances64(dim, kind, rot, t0, half, built_from=(rot, t0, half))
+    a, b = built.raycast_nearest(rays, 300.0), same.raycast_nearest(rays, 300.0)
+    assert np.array_equal(a["leaf"], b["leaf"]) and np.array_equal(a["t"], b["t"]) and same.quality() == q0
+    assert (a["leaf"] >= 0).sum() > 300
+    # dyadic translation of every instance: the refit tree is the moved build
+    v = np.array([40.5, -12.25, 3.0][:dim])
+    r1 = rays.copy(); r1[:, :dim] += v
+    moved = oracle.Instances64(dim, kind, rot, t0 + v, half, built_from=(rot, t0, half))
+    direct = oracle.Instances64(dim, kind, rot, t0 + v, half)
+    m, d = moved.raycast_nearest(r1, 300.0), direct.raycast_nearest(r1, 300.0)
+    assert np.array_equal(m["leaf"], a["leaf"]) and np.array_equal(m["t"], a["t"])
+    assert np.array_equal(d["leaf"], a["leaf"]) and np.array_equal(d["t"], a["t"])
+    assert moved.quality() == q0 == direct.quality()
+    assert not moved.needs_rebuild(1.0, q0)
+    # random motion and new rotations: all hits (one crossing per instance, in
+    # any tree order) and guarded nearest hits equal the naive scans
+    t2 = t0 + rng.uniform(-6.0, 6.0, (n, dim))
+    rot2 = _plane_rotations(rng, n, dim)
+    moved = oracle.Instances64(dim, kind, rot2, t2, half, built_from=(rot, t0, half))
+    naive = oracle.Instances64(dim, kind, rot2, t2, half, naive=True)
+    oa, la, ta = moved.raycast_all(rays, 300.0)
+    ob, lb, tb = naive.raycast_all(rays, 300.0)
+    assert np.array_equal(oa, ob) and np.array_equal(la, lb) and np.array_equal(ta, tb) and la.size > 300
+    g = oracle.Instances64(dim, kind, rot2, t2, half, guarded=True, built_from=(rot, t0, half))
+    gn = oracle.Instances64(dim, kind, rot2, t2, half, guarded=True, naive=True)
+    x, y = g.raycast_nearest(rays, 300.0), gn.raycast_nearest(rays, 300.0)
+    assert np.array_equal(x["leaf"], y["leaf"]) and np.array_equal(x["t"], y["t"])
+    assert moved.quality() > q0 and moved.needs_rebuild(1.0, q0)
