@@ -709,6 +709,17 @@ int sq_raycast64_nearest_consumer(sq_narrow64* narrow, sq_bvh64* bvh, const doub
 /* Bvh::raycast_all (bvh.rs:196-259) with the consumer's test_ray, in double. */
 int sq_raycast64_all_consumer(sq_narrow64* narrow, sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi,
                               uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total);
+/* Bvh::raycast_crossings (bvh.rs:264-328) under the feature: every crossing
+ * the source reports through an optional
+ *   #define SQ_NARROW64_CROSSINGS 1
+ *   template <class Out> __device__ void sq_narrow64_crossings(const double* prim,
+ *       const double* origin, const double* dir, double max_toi, Out& out);
+ * (out(toi) once per crossing, any order; each kept in [0, max_toi]), or
+ * without one the sq_narrow64 hit (the trait's default); CSR as
+ * sq_raycast64_all_consumer. */
+int sq_raycast64_crossings_consumer(sq_narrow64* narrow, sq_bvh64* bvh, const double* rays, uint64_t n, double max_toi,
+                                    uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity,
+                                    uint64_t* total);
 
 #ifdef __cplusplus
 }

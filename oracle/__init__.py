@@ -83,6 +83,7 @@ def _declare(L):
         "orc64_overlap": (U64, [I32, I32, P, U32, P, U64, I64, P, U64, I32, P, P, U64]),
         "orc64_tree_stats": (None, [I32, I32, P, U32, P, P]),
         "orc64_consumer_query": (U64, [I32, I32, P, U32, I32, U64, P, U64, C.c_double, I32, I32, P, P, P, U64]),
+        "orc64_consumer_crossings": (U64, [I32, I32, P, U32, I32, U64, U64, P, U64, C.c_double, I32, P, P, P, U64]),
         "orc64_bvh_raycast_nearest": (None, [I32, I32, P, U32, P, U64, C.c_double, I32, P, P, P]),
         "orc64_bvh_refit_raycast_nearest": (None, [I32, I32, P, P, U32, P, U64, C.c_double, I32, P, P, P, P]),
         "orc64_tlas_refit_query": (U64, [I32, I32, U64, P, P, P, P, P, P, P, U64, C.c_double, I32, I32, I32, P, P, P, U64,
@@ -471,6 +472,21 @@ class Consumer64:
         call = lambda leaf, t, cap: lib().orc64_consumer_query(self.dim, self._flags, _p(self.records), self.n, self.pd,
                                                                self.addr, _p(r), n, float(max_toi), 1, threads, _p(leaf),
                                                                _p(t), _p(off), cap)
+        tot = call(None, None, 0)
+        leaf = np.empty(max(tot, 1), np.int64); t = np.empty(max(tot, 1))
+        call(leaf, t, tot)
+        return off, leaf[:tot], t[:tot]
+
+    def raycast_crossings(self, rays, max_toi, crossings_addr, threads=8):
+        """Bvh::raycast_crossings (bvh.rs:264-328): every crossing the
+        consumer's crossings function (C address) reports, per ray sorted by
+        (toi, leaf)."""
+        r = self._q(rays)
+        n = r.shape[0]
+        off = np.empty(n + 1, np.uint64)
+        call = lambda leaf, t, cap: lib().orc64_consumer_crossings(self.dim, self._flags, _p(self.records), self.n, self.pd,
+                                                                   self.addr, int(crossings_addr), _p(r), n, float(max_toi),
+                                                                   threads, _p(leaf), _p(t), _p(off), cap)
         tot = call(None, None, 0)
         leaf = np.empty(max(tot, 1), np.int64); t = np.empty(max(tot, 1))
         call(leaf, t, tot)

@@ -88,6 +88,10 @@ template <int D> static double dot(const double* a, const double* b) {
 // A consumer's test_ray in double (geometry.rs:33-43): nonzero with *toi on a
 // hit of the primitive's doubles.
 typedef int (*ConsumerNarrow64)(const double* prim, const double* origin, const double* dir, double max_toi, double* toi);
+// A consumer's test_ray_crossings in double (geometry.rs:45-63): calls
+// emit(ctx, toi) once per crossing of the ray with the primitive.
+typedef void (*ConsumerCrossings64)(const double* prim, const double* origin, const double* dir, double max_toi,
+                                    void (*emit)(void*, double), void* ctx);
 
 template <int D> struct Geom {
   int kind;   // 0 BOX [min D][max D], 1 SPHERE [c D][r], 2 CONSUMER [world box 2D][consumer_pd doubles]
@@ -96,6 +100,7 @@ template <int D> struct Geom {
   const uint64_t* layers;
   bool guarded = false;   // sphere and consumer hits (and contacts) kept inside their box's slab interval
   ConsumerNarrow64 consumer = nullptr;
+  ConsumerCrossings64 consumer_x = nullptr;   // set: all-hits lists report every crossing (raycast_crossings)
   int consumer_pd = 0;
   int stride() const { return kind == 0 ? 2 * D : kind == 1 ? D + 1 : 2 * D + consumer_pd; }
   const double* p(size_t leaf) const { return prims + leaf * stride(); }
@@ -107,6 +112,27 @@ template <int D> struct Geom {
       if (kind == 0 || kind == 2) { mn[i] = q[i]; mx[i] = q[D + i]; }
       else { mn[i] = q[i] - q[D]; mx[i] = q[i] + q[D]; }
     }
+  }
+  // The leaf's hits for an all-hits list: every crossing the consumer reports
+  // (bvh.rs:264-328 calling test_ray_crossings), each kept under the test_ray
+  // contract and, when `guarded`, inside its box's slab interval; else the
+  // test_ray hit.
+  template <class F> void list_hits(size_t leaf, const Ray& r, double max_toi, F&& out) const {
+    if (kind == 2 && consumer_x) {
+      std::vector<double> ts;
+      consumer_x(p(leaf) + 2 * D, r.o, r.d, max_toi, [](void* c, double t) { static_cast<std::vector<double>*>(c)->push_back(t); },
+                 &ts);
+      double mn[D], mx[D], te, tx; int ax;
+      world_aabb(leaf, mn, mx);
+      for (double t : ts) {
+        if (!(t >= 0.0 && t <= max_toi)) continue;
+        if (guarded && (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &ax) || !(te <= t && t <= tx))) continue;
+        out(t);
+      }
+      return;
+    }
+    double t, nrm[D];
+    if (test_ray(leaf, r, max_toi, &t, nrm)) out(t);
   }
   bool accepts(size_t leaf, uint64_t mask, int64_t exclude) const {
     if (exclude >= 0 && (int64_t)leaf == exclude) return false;
@@ -260,9 +286,9 @@ static void naive_all(const Geom<D>& g, const Ray& r, double max_toi, uint64_t m
     double mn[D], mx[D], te, tx; int ax;
     g.world_aabb(leaf, mn, mx);
     if (!ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &ax)) continue;
-    double t;
-    if (g.test_ray(leaf, r, max_toi, &t, nrm)) out.push_back({t, (int64_t)leaf});
+    g.list_hits(leaf, r, max_toi, [&](double t) { out.push_back({t, (int64_t)leaf}); });
   }
+  (void)nrm;
   std::stable_sort(out.begin(), out.end(),
                    [](const std::pair<double, int64_t>& a, const std::pair<double, int64_t>& b) {
                      return hit_less(a.first, a.second, b.first, b.second);
@@ -647,8 +673,7 @@ static uint64_t all_batch(int kindf, const double* prims, uint32_t n, const uint
         Ray r{q + i * 2 * D, q + i * 2 * D + D};
         tree.walk([&](const double* mn, const double* mx) { double te, tx; int ax; return ray_intersect<D>(mn, mx, r, max_toi, &te, &tx, &ax); },
                   [&](size_t lf) {
-                    double tt, nrm[D];
-                    if (g.accepts(lf, mask, exclude) && g.test_ray(lf, r, max_toi, &tt, nrm)) out.push_back({tt, (int64_t)lf});
+                    if (g.accepts(lf, mask, exclude)) g.list_hits(lf, r, max_toi, [&](double tt) { out.push_back({tt, (int64_t)lf}); });
                   });
         std::stable_sort(out.begin(), out.end(), by_hit);
       }
@@ -1010,10 +1035,11 @@ using namespace orc64;
 template <int D>
 static uint64_t consumer_query(int flags, const double* records, uint32_t n, int pd, uint64_t fn, const double* rays,
                                uint64_t nr, double max_toi, int mode, int threads, int64_t* leaf, double* t,
-                               uint64_t* offsets, uint64_t cap) {
+                               uint64_t* offsets, uint64_t cap, uint64_t fnx = 0) {
   Geom<D> g{2, n, records, nullptr};
   g.guarded = (flags & FLAG_GUARDED) != 0;
   g.consumer = reinterpret_cast<ConsumerNarrow64>(fn);
+  g.consumer_x = reinterpret_cast<ConsumerCrossings64>(fnx);
   g.consumer_pd = pd;
   const int kindf = 2 | (flags & FLAG_BVH);
   if (mode == 0) {
@@ -1108,6 +1134,16 @@ uint64_t orc64_consumer_query(int D, int flags, const double* records, uint32_t 
                               uint64_t cap) {
   uint64_t tot = 0;
   ORC64_DISPATCH(D, tot = consumer_query<DD>(flags, records, n, pd, fn, rays, nr, max_toi, mode, threads, leaf, t, offsets, cap));
+  return tot;
+}
+
+// Bvh::raycast_crossings over a consumer whose crossings function is fnx
+// (test_ray_crossings): CSR lists as orc64_consumer_query's mode 1.
+uint64_t orc64_consumer_crossings(int D, int flags, const double* records, uint32_t n, int pd, uint64_t fn, uint64_t fnx,
+                                  const double* rays, uint64_t nr, double max_toi, int threads, int64_t* leaf, double* t,
+                                  uint64_t* offsets, uint64_t cap) {
+  uint64_t tot = 0;
+  ORC64_DISPATCH(D, tot = consumer_query<DD>(flags, records, n, pd, fn, rays, nr, max_toi, 1, threads, leaf, t, offsets, cap, fnx));
   return tot;
 }
 

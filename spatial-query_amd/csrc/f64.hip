@@ -649,8 +649,9 @@ namespace {
 
 const char* kKernelSrc64 = R"SQRTC(
 namespace sqd {
-template <int D, bool GUARD> struct ConsumerPol64 {
+template <int D, bool GUARD, bool XC = false> struct ConsumerPol64 {
   static constexpr int QS = 2 * D;
+  static constexpr bool CROSS = XC;   // all-hits lists of every crossing (test_ray_crossings)
   RayD<D, double> r;
   __device__ void setup(const double* src) { ray_setup<D>(r, src); }
   __device__ const RayD<D, double>& ray() const { return r; }
@@ -673,13 +674,33 @@ template <int D, bool GUARD> struct ConsumerPol64 {
     t = tt;
     return true;
   }
+  // test_ray_crossings (geometry.rs:45-63): the source's sq_narrow64_crossings
+  // when it defines one (SQ_NARROW64_CROSSINGS), each crossing kept under the
+  // test_ray contract ([0, limit]; with GUARD, inside its box's slab
+  // interval); else the trait's default, the one test_ray hit
+  template <class Emit> __device__ void crossings(const double* p, double limit, Emit&& emit) const {
+#if defined(SQ_NARROW64_CROSSINGS)
+    auto out = [&](double tt) {
+      if (!(tt >= 0.0 && tt <= limit)) return;
+      if constexpr (GUARD) {
+        double te, tx; int ax;
+        if (!slab_exact<D>(r, p, p + D, limit, te, tx, ax) || !(te <= tt && tt <= tx)) return;
+      }
+      emit(tt);
+    };
+    sq_narrow64_crossings(p + 2 * D, r.o, r.d, limit, out);
+#else
+    double tt;
+    if (narrow(p, limit, tt)) emit(tt);
+#endif
+  }
 };
-template <bool EXACT, bool ALL>
+template <bool EXACT, bool ALL, bool XC = false>
 __device__ __forceinline__ void consumer64(const View64& v, const double* rays, unsigned n, double max_toi, unsigned* counter,
                                            int* out_leaf, double* out_toi, SpillEntry* spill, unsigned lanes, unsigned* counts,
                                            const unsigned long long* offsets) {
   const Filter f{nullptr, ~0ull, -1};
-  pool64_walk<SQ_RTC_D, KIND_CONSUMER64, false, false, ALL, EXACT, ConsumerPol64<SQ_RTC_D, !EXACT>, 2 * SQ_RTC_D + SQ_RTC_PF>(
+  pool64_walk<SQ_RTC_D, KIND_CONSUMER64, false, false, ALL, EXACT, ConsumerPol64<SQ_RTC_D, !EXACT, XC>, 2 * SQ_RTC_D + SQ_RTC_PF>(
       v, rays, n, max_toi, nullptr, f, counter, out_leaf, out_toi, nullptr, nullptr, nullptr, spill, lanes, counts,
       reinterpret_cast<const uint64_t*>(offsets));
 }
@@ -702,6 +723,12 @@ extern "C" __global__ __launch_bounds__(sqd::F64_BLOCK) void sq_consumer64_all(
     sqd::SpillEntry* spill, unsigned lanes, unsigned* counts, const unsigned long long* offsets) {
   sqd::consumer64<true, true>(v, rays, n, max_toi, counter, out_leaf, out_toi, spill, lanes, counts, offsets);
 }
+// all crossings (Bvh::raycast_crossings): as sq_consumer64_all, every crossing of every primitive
+extern "C" __global__ __launch_bounds__(sqd::F64_BLOCK) void sq_consumer64_crossings(
+    sqd::View64 v, const double* rays, unsigned n, double max_toi, unsigned* counter, int* out_leaf, double* out_toi,
+    sqd::SpillEntry* spill, unsigned lanes, unsigned* counts, const unsigned long long* offsets) {
+  sqd::consumer64<true, true, true>(v, rays, n, max_toi, counter, out_leaf, out_toi, spill, lanes, counts, offsets);
+}
 )SQRTC";
 
 }  // namespace
@@ -714,7 +741,8 @@ struct sq_narrow64 {
   hipFunction_t fn = nullptr;       // nearest hit
   hipFunction_t fn_g = nullptr;     // nearest hit, SQ_HITS_GUARDED
   hipFunction_t fn_all = nullptr;   // all hits (count, then fill)
-  int per_cu[3] = {-1, -1, -1};     // resident blocks per CU of fn, fn_g, fn_all (queried at creation)
+  hipFunction_t fn_x = nullptr;     // all crossings (count, then fill)
+  int per_cu[4] = {-1, -1, -1, -1}; // resident blocks per CU of fn, fn_g, fn_all, fn_x (queried at creation)
 };
 
 namespace {
@@ -751,7 +779,7 @@ int consumer64_launch(sq_narrow64* nw, sq_bvh64* b, hipFunction_t fn, int k, con
   void* args_nearest[] = {&v, &rp, &nn, &max_toi, &counter, &lp, &tp, &spill, &lanes};
   void* args_all[] = {&v, &rp, &nn, &max_toi, &counter, &lp, &tp, &spill, &lanes, &cp, &op};
   HIP_TRY(hipModuleLaunchKernel(fn, (unsigned)grid, 1, 1, F64_BLOCK, 1, 1, (unsigned)lds, c->stream,
-                                k == 2 ? args_all : args_nearest, nullptr));
+                                k >= 2 ? args_all : args_nearest, nullptr));   // 2, 3: the list kernels (11 arguments)
   return SQ_OK;
 }
 
@@ -771,13 +799,14 @@ int sq_narrow64_create(sq_ctx* ctx, int dim, const char* source, uint32_t prim_d
   if (int rc = rtc_load_module(ctx, "walk64.h", source, kKernelSrc64, dim, prim_doubles, &nw->mod)) return rc;
   if (hipModuleGetFunction(&nw->fn, nw->mod, "sq_consumer64_nearest") != hipSuccess ||
       hipModuleGetFunction(&nw->fn_g, nw->mod, "sq_consumer64_nearest_guarded") != hipSuccess ||
-      hipModuleGetFunction(&nw->fn_all, nw->mod, "sq_consumer64_all") != hipSuccess) {
+      hipModuleGetFunction(&nw->fn_all, nw->mod, "sq_consumer64_all") != hipSuccess ||
+      hipModuleGetFunction(&nw->fn_x, nw->mod, "sq_consumer64_crossings") != hipSuccess) {
     (void)hipModuleUnload(nw->mod);
     return fail(SQ_ERR_HIP, "consumer kernel not found in the compiled module");
   }
   const size_t lds = (size_t)F64_RING * F64_BLOCK * 8;
-  const hipFunction_t fns[3] = {nw->fn, nw->fn_g, nw->fn_all};
-  for (int k = 0; k < 3; ++k) {
+  const hipFunction_t fns[4] = {nw->fn, nw->fn_g, nw->fn_all, nw->fn_x};
+  for (int k = 0; k < 4; ++k) {
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nw->per_cu[k], fns[k], F64_BLOCK, lds) != hipSuccess) {
       (void)hipModuleUnload(nw->mod);
       return fail(SQ_ERR_HIP, "occupancy query of the consumer kernels failed");
@@ -834,9 +863,11 @@ int sq_raycast64_nearest_consumer(sq_narrow64* nw, sq_bvh64* b, const double* ra
                            nullptr, nullptr);
 }
 
-int sq_raycast64_all_consumer(sq_narrow64* nw, sq_bvh64* b, const double* rays, uint64_t n, double max_toi,
-                              uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total) {
+// all hits (kernel 2) or all crossings (kernel 3): count pass, scan, fill pass
+static int consumer64_lists(sq_narrow64* nw, sq_bvh64* b, int which, const double* rays, uint64_t n, double max_toi,
+                            uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total) {
   if (!nw || !b || !offsets || !total || (n && !rays)) return fail(SQ_ERR_INVALID, "NULL argument");
+  const hipFunction_t fk = which == 3 ? nw->fn_x : nw->fn_all;
   if (int rc = check_pair64(nw, b)) return rc;
   sq_ctx* c = b->ctx;
   *total = 0;
@@ -848,7 +879,7 @@ int sq_raycast64_all_consumer(sq_narrow64* nw, sq_bvh64* b, const double* rays, 
   int rc;
   if ((rc = c->counts.ensure(n * 4))) return rc;
   uint32_t* counts = c->counts.as<uint32_t>();
-  if ((rc = consumer64_launch(nw, b, nw->fn_all, 2, rays, n, max_toi, nullptr, nullptr, counts, nullptr))) return rc;
+  if ((rc = consumer64_launch(nw, b, fk, which, rays, n, max_toi, nullptr, nullptr, counts, nullptr))) return rc;
   std::vector<std::unique_ptr<DevBuf>> temps;
   if ((rc = scan_counts(c, counts, n, offsets, temps))) return rc;
   HIP_TRY(hipMemcpyAsync(total, offsets + n, 8, hipMemcpyDeviceToHost, c->stream));
@@ -858,7 +889,18 @@ int sq_raycast64_all_consumer(sq_narrow64* nw, sq_bvh64* b, const double* rays, 
     return fail(SQ_ERR_CAPACITY, "result needs " + std::to_string(tot) + " entries, capacity is " + std::to_string(capacity));
   if (!tot) return SQ_OK;
   if (!out_leaf || !out_toi) return fail(SQ_ERR_INVALID, "output arrays are NULL");
-  return consumer64_launch(nw, b, nw->fn_all, 2, rays, n, max_toi, out_leaf, out_toi, counts, offsets);
+  return consumer64_launch(nw, b, fk, which, rays, n, max_toi, out_leaf, out_toi, counts, offsets);
+}
+
+int sq_raycast64_all_consumer(sq_narrow64* nw, sq_bvh64* b, const double* rays, uint64_t n, double max_toi,
+                              uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity, uint64_t* total) {
+  return consumer64_lists(nw, b, 2, rays, n, max_toi, offsets, out_leaf, out_toi, capacity, total);
+}
+
+int sq_raycast64_crossings_consumer(sq_narrow64* nw, sq_bvh64* b, const double* rays, uint64_t n, double max_toi,
+                                    uint64_t* offsets, int32_t* out_leaf, double* out_toi, uint64_t capacity,
+                                    uint64_t* total) {
+  return consumer64_lists(nw, b, 3, rays, n, max_toi, offsets, out_leaf, out_toi, capacity, total);
 }
 
 }  // extern "C"

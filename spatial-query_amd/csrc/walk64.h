@@ -145,6 +145,7 @@ __device__ __forceinline__ void slab_pair64(const RayD<D, double>& r, const doub
 // their world box (SQ_HITS_GUARDED).
 template <int D, int KIND, bool CAST, bool GUARD> struct Pol64;
 template <int D, int KIND, bool GUARD> struct Pol64<D, KIND, false, GUARD> {
+  static constexpr bool CROSS = false;   // one hit per primitive (test_ray)
   static constexpr int QS = 2 * D;   // doubles per query
   RayD<D, double> r;
   __device__ void setup(const double* src) { ray_setup<D>(r, src); }
@@ -179,6 +180,7 @@ template <int D, int KIND, bool GUARD> struct Pol64<D, KIND, false, GUARD> {
   }
 };
 template <int D, int KIND, bool GUARD> struct Pol64<D, KIND, true, GUARD> {
+  static constexpr bool CROSS = false;
   static constexpr int QS = 4 * D;   // origin, direction, probe box min, max
   // (instance kinds: the cast's slab and probe; the instance test below)
   CastPolicy<D, is_inst64<KIND>() ? KIND_AABB : KIND, double, GUARD> c;
@@ -273,6 +275,14 @@ __device__ __forceinline__ void pool64_walk(const View64& v, const double* __res
     for (uint32_t k = first; k < first + count; ++k) {
       const uint32_t id = v.ids[k];
       if (!accepts64(v, f, id)) continue;
+      if constexpr (ALL && Q::CROSS) {
+        // every crossing the policy reports (test_ray_crossings)
+        q.crossings(v.prims + (size_t)k * PS, best, [&](double t) {
+          if (offsets) { out_leaf[at] = (int32_t)id; out_toi[at] = t; ++at; }
+          ++cnt;
+        });
+        continue;
+      }
       double t;
       if (!q.narrow(v.prims + (size_t)k * PS, best, t)) continue;
       if constexpr (ALL) {

@@ -965,6 +965,15 @@ class ConsumerBvh64:
                                                    C.byref(total))
         return self._csr(fn, rays.shape[0], rays.device, True, capacity)
 
+    def raycast_crossings(self, rays, max_toi, capacity=0):
+        """Bvh::raycast_crossings: CSR (offsets, leaf, toi) of every crossing
+        the source's sq_narrow64_crossings reports, sorted by (toi, leaf)."""
+        def fn(off, leaf, toi, cap, total):
+            return lib().sq_raycast64_crossings_consumer(self.narrow._h, self._h, _dp(rays, "float64", 2 * self.dim),
+                                                         rays.shape[0], float(max_toi), _dp(off, "int64"), _dp(leaf, "int32"),
+                                                         _dp(toi, "float64"), cap, C.byref(total))
+        return self._csr(fn, rays.shape[0], rays.device, True, capacity)
+
     def close(self):
         if getattr(self, "_h", None):
             lib().sq_bvh64_destroy(self._h)

@@ -138,6 +138,7 @@ SIGNATURES = {
     "sq_bvh64_build_consumer": [P, P, P, U64, PP],
     "sq_raycast64_nearest_consumer": [P, P, P, U64, F64, P, P],
     "sq_raycast64_all_consumer": [P, P, P, U64, F64, P, P, P, U64, C.POINTER(C.c_uint64)],
+    "sq_raycast64_crossings_consumer": [P, P, P, U64, F64, P, P, P, U64, C.POINTER(C.c_uint64)],
     "sq_tlas64_create": [P, I32, U64, P, P, P, PP],
     "sq_tlas64_create_kind": [P, I32, I32, U64, P, P, P, PP],
     "sq_tlas64_raycast_all": [P, P, U64, F64, P, P, P, U64, P],

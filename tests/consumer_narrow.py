@@ -284,7 +284,12 @@ def compile_cpu64(source: str, dim: int, pd: int, workdir: str):
     oracle.Consumer64."""
     code = (f"#define __device__\n#define SQ_RTC_D {dim}\n#define SQ_RTC_PF {pd}\n" + source +
             '\nextern "C" int sq_narrow64_c(const double* p, const double* o, const double* d, double m, double* t) '
-            "{ return sq_narrow64(p, o, d, m, t) ? 1 : 0; }\n")
+            "{ return sq_narrow64(p, o, d, m, t) ? 1 : 0; }\n"
+            "#ifdef SQ_NARROW64_CROSSINGS\n"
+            "struct SqEmit64 { void (*f)(void*, double); void* c; void operator()(double t) const { f(c, t); } };\n"
+            'extern "C" void sq_crossings64_c(const double* p, const double* o, const double* d, double m, '
+            "void (*f)(void*, double), void* c) { SqEmit64 e{f, c}; sq_narrow64_crossings(p, o, d, m, e); }\n"
+            "#endif\n")
     tag = hashlib.sha1(code.encode()).hexdigest()[:12]
     src = os.path.join(workdir, f"narrow64_{tag}.cpp")
     so = os.path.join(workdir, f"narrow64_{tag}.so")
@@ -293,3 +298,44 @@ def compile_cpu64(source: str, dim: int, pd: int, workdir: str):
     subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-o", so, src], check=True)
     lib = ctypes.CDLL(so)
     return lib, ctypes.cast(lib.sq_narrow64_c, ctypes.c_void_p).value
+
+
+# The shells of crossings.rs:38-111 in double: both roots of the unbounded ray
+# are crossings; the nearest hit is the first root at or after the origin.
+# Records: centre [D], radius.
+SHELL64 = r"""
+#define SQ_NARROW64_CROSSINGS 1
+__device__ bool shell64_roots(const double* p, const double* o, const double* d, double* t0, double* t1) {
+  double oc[SQ_RTC_D];
+  for (int k = 0; k < SQ_RTC_D; ++k) oc[k] = o[k] - p[k];
+  double b = 0.0, l = 0.0;
+  for (int k = 0; k < SQ_RTC_D; ++k) b = b + oc[k] * d[k];
+  for (int k = 0; k < SQ_RTC_D; ++k) l = l + oc[k] * oc[k];
+  const double disc = b * b - (l - p[SQ_RTC_D] * p[SQ_RTC_D]);
+  if (disc < 0.0) return false;
+  const double sq = __builtin_sqrt(disc);
+  *t0 = -b - sq;
+  *t1 = -b + sq;
+  return true;
+}
+__device__ bool sq_narrow64(const double* p, const double* o, const double* d, double max_toi, double* toi) {
+  double t0, t1;
+  if (!shell64_roots(p, o, d, &t0, &t1)) return false;
+  const double t = t0 >= 0.0 ? t0 : t1;
+  if (t < 0.0 || t > max_toi) return false;
+  *toi = t;
+  return true;
+}
+template <class Out>
+__device__ void sq_narrow64_crossings(const double* p, const double* o, const double* d, double max_toi, Out& out) {
+  double t0, t1;
+  if (!shell64_roots(p, o, d, &t0, &t1)) return;
+  out(t0);
+  out(t1);
+}
+"""
+
+
+def crossings64_addr(lib):
+    """Address of the crossings entry point of a compile_cpu64 library (0: none)."""
+    return ctypes.cast(lib.sq_crossings64_c, ctypes.c_void_p).value if hasattr(lib, "sq_crossings64_c") else 0

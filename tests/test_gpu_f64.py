@@ -688,3 +688,34 @@ def test_f64_tlas_refit_and_needs_rebuild(ctx, kind):
     with pytest.raises(ValueError):
         tl.refit(rot0[:-1], t0[:-1])
     tl.close()
+
+
+def test_f64_consumer_shell_crossings(ctx, tmp_path):
+    """sq_raycast64_crossings_consumer (Bvh::raycast_crossings in double): a
+    consumer whose sq_narrow64_crossings reports both roots of a shell gives
+    the oracle's crossing lists (the same source built by g++) bit for bit, and
+    its all-hits lists stay one hit per shell; a source without crossings
+    answers crossings with its sq_narrow64 hits (the trait's default)."""
+    from consumer_narrow import BALL64, SHELL64, compile_cpu64, crossings64_addr
+    print("case: f64 consumer shell crossings", flush=True)
+    rng = np.random.default_rng(0x5E12)
+    n = 20000
+    ctr = rng.uniform(-80.0, 80.0, (n, 3)); rad = rng.uniform(0.5, 3.0, (n, 1))
+    prims = np.concatenate([ctr, rad], 1)
+    boxes = np.concatenate([ctr - rad, ctr + rad], 1)
+    o = rng.uniform(-90.0, 90.0, (20000, 3)); d = rng.normal(size=(20000, 3))
+    rays = np.concatenate([o, d / np.linalg.norm(d, axis=1, keepdims=True)], 1)
+    lib, addr = compile_cpu64(SHELL64, 3, 4, str(tmp_path))
+    ref = oracle.Consumer64(3, boxes, prims, addr)
+    nw = sq_amd.Narrow64(ctx, 3, SHELL64, 4)
+    cb = nw.build(boxes, prims)
+    want = ref.raycast_crossings(rays, 250.0, crossings64_addr(lib))
+    assert want[1].size > 20000
+    _csr_equal64(cb.raycast_crossings(dev(rays), 250.0), want)
+    _csr_equal64(cb.raycast_all(dev(rays), 250.0), ref.raycast_all(rays, 250.0))
+    cb.close(); nw.close()
+    nb = sq_amd.Narrow64(ctx, 3, BALL64, 4)
+    bb = nb.build(boxes, prims)
+    _, baddr = compile_cpu64(BALL64, 3, 4, str(tmp_path))
+    _csr_equal64(bb.raycast_crossings(dev(rays), 250.0), oracle.Consumer64(3, boxes, prims, baddr).raycast_all(rays, 250.0))
+    bb.close(); nb.close()

@@ -661,3 +661,46 @@ def test_tlas_refit_pins(dim, kind):
     x, y = g.raycast_nearest(rays, 300.0), gn.raycast_nearest(rays, 300.0)
     assert np.array_equal(x["leaf"], y["leaf"]) and np.array_equal(x["t"], y["t"])
     assert moved.quality() > q0 and moved.needs_rebuild(1.0, q0)
+
+
+def test_consumer64_shell_crossings(tmp_path):
+    """Bvh::raycast_crossings (bvh.rs:264-328) over a consumer in double whose
+    test_ray_crossings reports both roots (the shells of crossings.rs:38-111):
+    closed forms (a shell of radius 1 at x = 5 crossed at 4 and 6 from the
+    origin, at 1 only from its centre, not at all by a ray that passes 2 above
+    it, at 4 only when max_toi is 5); the BVH walk's lists equal the naive
+    scan's; every ray's first crossing is its nearest hit, and the all-hits
+    list (one test_ray hit per shell) is a subset of the crossings."""
+    from consumer_narrow import SHELL64, compile_cpu64, crossings64_addr
+    lib, addr = compile_cpu64(SHELL64, 3, 4, str(tmp_path))
+    xa = crossings64_addr(lib)
+    assert xa
+    one = np.array([[5.0, 0.0, 0.0, 1.0]])
+    box = np.array([[4.0, -1.0, -1.0, 6.0, 1.0, 1.0]])
+    c = oracle.Consumer64(3, box, one, addr)
+    rays = np.array([[0.0, 0, 0, 1, 0, 0], [5.0, 0, 0, 1, 0, 0], [0.0, 2, 0, 1, 0, 0]])
+    off, leaf, t = c.raycast_crossings(rays, 100.0, xa)
+    assert off.tolist() == [0, 2, 3, 3] and leaf.tolist() == [0, 0, 0] and t.tolist() == [4.0, 6.0, 1.0]
+    off, leaf, t = c.raycast_crossings(rays[:1], 5.0, xa)
+    assert off.tolist() == [0, 1] and t.tolist() == [4.0]
+    rng = np.random.default_rng(0x5E11)
+    n = 2000
+    ctr = rng.uniform(-40.0, 40.0, (n, 3)); rad = rng.uniform(0.5, 3.0, (n, 1))
+    prims = np.concatenate([ctr, rad], 1)
+    boxes = np.concatenate([ctr - rad, ctr + rad], 1)
+    o = rng.uniform(-45.0, 45.0, (3000, 3)); d = rng.normal(size=(3000, 3))
+    q = np.concatenate([o, d / np.linalg.norm(d, axis=1, keepdims=True)], 1)
+    bvh = oracle.Consumer64(3, boxes, prims, addr)
+    naive = oracle.Consumer64(3, boxes, prims, addr, naive=True)
+    ob, lb, tb = bvh.raycast_crossings(q, 150.0, xa)
+    on, ln, tn = naive.raycast_crossings(q, 150.0, xa)
+    assert np.array_equal(ob, on) and np.array_equal(lb, ln) and np.array_equal(tb, tn) and lb.size > 3000
+    near = naive.raycast_nearest(q, 150.0)
+    oa, la, ta = naive.raycast_all(q, 150.0)
+    for i in range(len(q)):
+        xs = list(zip(tb[ob[i]:ob[i + 1]].tolist(), lb[ob[i]:ob[i + 1]].tolist()))
+        if near["leaf"][i] >= 0:
+            assert xs[0] == (near["t"][i], near["leaf"][i])
+        else:
+            assert not xs
+        assert set(zip(ta[oa[i]:oa[i + 1]].tolist(), la[oa[i]:oa[i + 1]].tolist())) <= set(xs)
